@@ -1,0 +1,190 @@
+/*
+ * ne_gpu.h -- C ABI of the MI355X (gfx950) rigid-body step.
+ *
+ * This is the drop-in seam for NativeEngine's per-step hot path.  The reference
+ * exposes that path as the C++ value type `ne::World` (src/world.hpp:73-161)
+ * whose `step()` (src/world.cpp:133-146) runs integrate -> collide ->
+ * wrap/walls on `std::vector<Body>`.  This header replaces it with a context
+ * that keeps the bodies resident in HBM as structure-of-arrays FP64 and steps
+ * them with hand-written HIP kernels.  `include/ne_world.hpp` rebuilds the
+ * reference's `ne::World` class on top of these entry points.
+ *
+ * Conventions
+ *   - Every function returns an int status: NE_OK (0) or a negative NE_ERR_*.
+ *     Nothing throws across this boundary.
+ *   - A context is single-threaded: one host thread, one HIP stream.
+ *   - All pointers are host pointers unless the name says `device`.
+ *   - Vectors are packed xyz triples (n*3 doubles); quaternions are packed
+ *     (w, x, y, z) quadruples (n*4 doubles), the reference's `Q` order
+ *     (src/vmath.hpp:47-48).
+ *   - Body index == identity, as in the reference (warm-start keys, pair order).
+ */
+#ifndef NE_GPU_H
+#define NE_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define NE_OK                 0
+#define NE_ERR_INVALID      (-1)   /* bad argument / unsupported configuration */
+#define NE_ERR_HIP          (-2)   /* a HIP runtime call failed               */
+#define NE_ERR_CAPACITY     (-3)   /* a pair/contact buffer overflowed; state is
+                                      undefined, re-upload and raise capacity */
+#define NE_ERR_NONFINITE    (-4)   /* a non-finite position/velocity was found */
+#define NE_ERR_UNSUPPORTED  (-5)   /* shape / feature not on the GPU path yet  */
+#define NE_ERR_NO_DEVICE    (-6)   /* no HIP device visible                    */
+
+/* ---- shapes: the reference's `enum class Shape` order (src/body.hpp:24) -- */
+#define NE_SHAPE_SPHERE    0
+#define NE_SHAPE_CYLINDER  1
+#define NE_SHAPE_BOX       2
+#define NE_SHAPE_CAPSULE   3
+#define NE_SHAPE_CONVEX    4
+#define NE_SHAPE_PLANE     5
+#define NE_SHAPE_MESH      6
+#define NE_SHAPE_COMPOUND  7
+
+/* ---- body flag bits (the reference's bool members, src/body.hpp:106-127) - */
+#define NE_FLAG_KINEMATIC  1u
+#define NE_FLAG_SENSOR     2u
+#define NE_FLAG_DYNAMIC    4u
+#define NE_FLAG_SLEEPING   8u
+#define NE_FLAG_CCD       16u
+
+/* ---- material combine modes (src/world.hpp:85) --------------------------- */
+#define NE_COMBINE_AVERAGE        0
+#define NE_COMBINE_MIN            1
+#define NE_COMBINE_MAX            2
+#define NE_COMBINE_MULTIPLY       3
+#define NE_COMBINE_GEOMETRIC_MEAN 4
+
+/*
+ * World parameters: the public scalar fields of `ne::World`
+ * (src/world.hpp:75-126).  Extensions beyond the reference:
+ *   half[3]            per-axis half extents (the reference's Box is a cube,
+ *                      src/box.hpp:16-18; pass three equal values for parity)
+ *   solver_iterations  the reference hard-codes 8 (src/world.cpp:572)
+ */
+typedef struct ne_world_params {
+    double  half[3];
+    int32_t periodic;
+    int32_t open_boundary;
+    double  dt;
+    double  gravity[3];
+    double  restitution;
+    double  friction;
+    double  contact_beta;
+    int32_t restitution_combine;
+    int32_t friction_combine;
+    int32_t warm_start;
+    int32_t force_brute_force;   /* accepted for parity; results never depend on it */
+    int32_t solver_iterations;
+    int32_t sleep_enabled;
+    double  sleep_lin_vel, sleep_ang_vel, sleep_time, wake_vel;
+} ne_world_params;
+
+/*
+ * Bodies as flat host arrays (n entries each).  Mirrors `ne::Body`
+ * (src/body.hpp:41-129).  Nullable arrays take the reference's defaults:
+ * force/torque 0, friction/restitution -1 (inherit), damping 0, gravity_scale 1,
+ * layer 0, mask 0xFFFFFFFF, flags NE_FLAG_DYNAMIC, no hull, no exclusions.
+ */
+typedef struct ne_bodies {
+    uint32_t        n;
+    const double   *x, *v, *q, *w;            /* n*3, n*3, n*4, n*3            */
+    const double   *force, *torque;           /* n*3 (nullable)                */
+    const double   *inv_mass;                 /* n                             */
+    const double   *inv_inertia;              /* n*3 body-frame diagonal       */
+    const int32_t  *shape;                    /* n, NE_SHAPE_*                 */
+    const double   *radius, *half_height;     /* n                             */
+    const double   *half_extents;             /* n*3 (plane: world normal)     */
+    const double   *friction, *restitution;   /* n (nullable)                  */
+    const double   *linear_damping, *angular_damping, *gravity_scale; /* n (nullable) */
+    const uint32_t *layer, *mask;             /* n (nullable)                  */
+    const uint32_t *flags;                    /* n (nullable)                  */
+    const double   *sleep_timer;              /* n (nullable)                  */
+    const double   *inv_mass_store;           /* n (nullable)                  */
+    const double   *inv_inertia_store;        /* n*3 (nullable)                */
+    const uint32_t *hull_offsets;             /* n+1 (nullable): CSR into hull_verts */
+    const double   *hull_verts;               /* 3 doubles per vertex          */
+    const uint32_t *nocollide_offsets;        /* n+1 (nullable): CSR           */
+    const uint32_t *nocollide;                /* partner body indices          */
+} ne_bodies;
+
+/* Context creation options. Zero fields take defaults. */
+typedef struct ne_gpu_config {
+    int32_t  device;            /* HIP device ordinal                         */
+    uint32_t pair_capacity;     /* candidate pairs per step (default 8*n+4096) */
+    uint32_t contact_capacity;  /* constraints per step (default 4*n+4096)    */
+    double   cell_scale;        /* grid cell = cell_scale * 2*maxR (>=1, default autotuned) */
+} ne_gpu_config;
+
+typedef struct ne_gpu_ctx ne_gpu_ctx;
+
+/* Replaces constructing an `ne::World` (src/world.hpp:73). */
+int ne_gpu_create(const ne_gpu_config *cfg, ne_gpu_ctx **out);
+int ne_gpu_destroy(ne_gpu_ctx *ctx);
+
+/* Sets the World's public scalars (src/world.hpp:75-126). */
+int ne_gpu_set_params(ne_gpu_ctx *ctx, const ne_world_params *p);
+
+/* Replaces assigning `World::bodies` (src/world.hpp:128): uploads all bodies
+ * (shape, mass, material and state).  Keeps the warm-start cache (the
+ * reference's cache survives body edits); call ne_gpu_reset_warm_cache to drop
+ * it, as `World::resetWarmCache` (src/world.hpp:140). */
+int ne_gpu_set_bodies(ne_gpu_ctx *ctx, const ne_bodies *b);
+
+/* Uploads only the dynamic state (x, v, q, w; any may be NULL) of the current
+ * bodies. */
+int ne_gpu_set_state(ne_gpu_ctx *ctx, const double *x, const double *v,
+                     const double *q, const double *w);
+
+/* Replaces `World::step()` (src/world.cpp:133-146), nsteps times, resident. */
+int ne_gpu_step(ne_gpu_ctx *ctx, int32_t nsteps);
+
+/* Downloads x (n*3), v (n*3), q (n*4), w (n*3); any may be NULL. */
+int ne_gpu_get_state(ne_gpu_ctx *ctx, double *x, double *v, double *q, double *w);
+
+/* Downloads per-body sleep state: flags (n), sleep_timer (n), inv_mass (n),
+ * inv_inertia (n*3); any may be NULL. */
+int ne_gpu_get_body_aux(ne_gpu_ctx *ctx, uint32_t *flags, double *sleep_timer,
+                        double *inv_mass, double *inv_inertia);
+
+/* `World::resetWarmCache()` (src/world.hpp:140). */
+int ne_gpu_reset_warm_cache(ne_gpu_ctx *ctx);
+
+/* Parity taps for the last completed step.  The contact list is the
+ * reference's constraint list (src/world.cpp:535-553) in its order: body pair
+ * (i<j), normal b->a, contact point, depth, and the converged normal impulse.
+ * Pass NULL arrays to query the count only; `cap` bounds the copy. */
+int ne_gpu_get_contacts(ne_gpu_ctx *ctx, uint32_t cap, uint32_t *count,
+                        uint32_t *i, uint32_t *j, double *normal, double *point,
+                        double *depth, double *normal_impulse);
+
+/* Distinct hit pairs (pairs with >=1 contact) of the last step, in (i,j) order. */
+int ne_gpu_get_pairs(ne_gpu_ctx *ctx, uint32_t cap, uint32_t *count,
+                     uint32_t *i, uint32_t *j);
+
+/* Diagnostics of `World` (src/world.cpp:704-729): sums over invMass>0 bodies
+ * in index order.  p[3], L[3] and ke may be NULL. */
+int ne_gpu_diagnostics(ne_gpu_ctx *ctx, double *p, double *L, double *ke);
+
+/* Per-step counters of the last step: [candidate pairs, hit pairs,
+ * constraints, islands, largest island constraints, solver levels]. */
+int ne_gpu_get_counters(ne_gpu_ctx *ctx, uint32_t out[8]);
+
+/* The HIP stream the context launches on (as a void*), for event timing. */
+int ne_gpu_get_stream(ne_gpu_ctx *ctx, void **stream);
+
+/* Last error message of this context (static storage, never NULL). */
+const char *ne_gpu_last_error(ne_gpu_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NE_GPU_H */

@@ -1,0 +1,108 @@
+"""ctypes mirror of include/ne_gpu.h (parameter and body layouts, constants).
+
+The same flat-array layout is used by the HIP library and by the test-side
+engines, so one scene can be handed to each of them unchanged.
+"""
+import ctypes as C
+
+import numpy as np
+
+NE_OK = 0
+NE_ERR_INVALID = -1
+NE_ERR_HIP = -2
+NE_ERR_CAPACITY = -3
+NE_ERR_NONFINITE = -4
+NE_ERR_UNSUPPORTED = -5
+NE_ERR_NO_DEVICE = -6
+
+SHAPE_SPHERE, SHAPE_CYLINDER, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CONVEX, SHAPE_PLANE, SHAPE_MESH, SHAPE_COMPOUND = range(8)
+
+FLAG_KINEMATIC = 1
+FLAG_SENSOR = 2
+FLAG_DYNAMIC = 4
+FLAG_SLEEPING = 8
+FLAG_CCD = 16
+
+COMBINE_AVERAGE, COMBINE_MIN, COMBINE_MAX, COMBINE_MULTIPLY, COMBINE_GEOMETRIC_MEAN = range(5)
+
+_dp = C.POINTER(C.c_double)
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+
+
+class WorldParams(C.Structure):
+    _fields_ = [
+        ("half", C.c_double * 3),
+        ("periodic", C.c_int32),
+        ("open_boundary", C.c_int32),
+        ("dt", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("restitution", C.c_double),
+        ("friction", C.c_double),
+        ("contact_beta", C.c_double),
+        ("restitution_combine", C.c_int32),
+        ("friction_combine", C.c_int32),
+        ("warm_start", C.c_int32),
+        ("force_brute_force", C.c_int32),
+        ("solver_iterations", C.c_int32),
+        ("sleep_enabled", C.c_int32),
+        ("sleep_lin_vel", C.c_double),
+        ("sleep_ang_vel", C.c_double),
+        ("sleep_time", C.c_double),
+        ("wake_vel", C.c_double),
+    ]
+
+
+class Bodies(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32),
+        ("x", _dp), ("v", _dp), ("q", _dp), ("w", _dp),
+        ("force", _dp), ("torque", _dp),
+        ("inv_mass", _dp),
+        ("inv_inertia", _dp),
+        ("shape", _i32p),
+        ("radius", _dp), ("half_height", _dp),
+        ("half_extents", _dp),
+        ("friction", _dp), ("restitution", _dp),
+        ("linear_damping", _dp), ("angular_damping", _dp), ("gravity_scale", _dp),
+        ("layer", _u32p), ("mask", _u32p),
+        ("flags", _u32p),
+        ("sleep_timer", _dp),
+        ("inv_mass_store", _dp),
+        ("inv_inertia_store", _dp),
+        ("hull_offsets", _u32p),
+        ("hull_verts", _dp),
+        ("nocollide_offsets", _u32p),
+        ("nocollide", _u32p),
+    ]
+
+
+class GpuConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("pair_capacity", C.c_uint32),
+        ("contact_capacity", C.c_uint32),
+        ("cell_scale", C.c_double),
+    ]
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 array (or NULL for None)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def uptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_u32p)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_i32p)

@@ -1,0 +1,1655 @@
+// ne_gpu.hip -- the MI355X (gfx950) rigid-body step behind include/ne_gpu.h.
+//
+// One context = one world resident in HBM as structure-of-arrays FP64, stepped
+// on one HIP stream.  A step is the reference's `World::step()`
+// (src/world.cpp:133-146) re-designed as a pipeline of data-parallel kernels:
+//
+//   k_integrate        world.cpp:373-411   per body
+//   grid build         world.cpp:424-501   counting sort of bodies into cells
+//   candidate rows     world.cpp:472-499   27-cell scan, rows (i, j>i) in order
+//   k_narrow           detect.hpp:33-77, world.cpp:88-130   per candidate pair
+//   k_cons_build       world.cpp:508-555   per contact, ordered by (i, j, slot)
+//   islands            --                  union-find over dynamic bodies
+//   k_solve_islands    world.cpp:557-616   warm start, I x GS sweep, correction
+//   k_wrap / k_walls   world.cpp:635-702   per body
+//
+// Exactness: the reference solves constraints strictly in sorted (i, j, slot)
+// order (Gauss-Seidel).  Two constraints that share no body whose velocity can
+// change commute exactly, so partitioning constraints into connected islands
+// and sweeping each island in that same order reproduces the sequential
+// result bit for bit, with no float atomics.  All kernels are compiled with
+// -ffp-contract=off and follow the reference's operation order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ne_gpu.h"
+#include "ne_dmath.hpp"
+#include "ne_narrow.hpp"
+#include "ne_prims.hpp"
+
+using namespace nea;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Device views
+// ---------------------------------------------------------------------------
+
+struct BodiesDev {
+    double *x0, *x1, *x2, *v0, *v1, *v2, *q0, *q1, *q2, *q3, *w0, *w1, *w2;
+    double *f0, *f1, *f2, *t0, *t1, *t2;
+    double *im, *ii0, *ii1, *ii2;
+    double *rad, *hh, *he0, *he1, *he2, *bR;
+    double *fric, *rest, *ldamp, *adamp, *gs;
+    int32_t* shape;
+    uint32_t *layer, *mask, *flags;
+    uint32_t *hull_off, *nc_off, *nc_idx;
+    double* hull;
+    uint32_t n;
+};
+
+struct SimParams {
+    Box3 box;
+    double dt;
+    double g[3];
+    double restitution, friction, beta;
+    int rcomb, fcomb;
+    int warm;
+    int iters;
+    int open_boundary;
+};
+
+// Grid description, computed on the device every step.
+struct GridDev {
+    double origin[3];
+    double cw[3];
+    int32_t nc[3];
+    uint32_t ncells;
+    int32_t periodic;
+};
+
+enum CounterSlot : int {
+    CNT_P = 0,        // candidate pairs (clamped)
+    CNT_HIT = 1,      // hit pairs
+    CNT_C = 2,        // constraints (clamped)
+    CNT_ISL = 3,      // islands
+    CNT_MAXISL = 4,   // largest island (constraints)
+    CNT_NGRID = 5,    // gridded bodies
+    CNT_CELLS = 6,    // cells in use
+    CNT_FAULT = 7,    // NE_FAULT_* bits
+    CNT_PRAW = 8,     // candidate pairs before clamping
+    CNT_CRAW = 9,     // constraints before clamping
+    CNT_N = 16
+};
+
+constexpr int TPB = 256;
+
+__device__ __forceinline__ V3 ld3(const double* a, const double* b, const double* c, uint32_t i) {
+    return V3{a[i], b[i], c[i]};
+}
+__device__ __forceinline__ void st3(double* a, double* b, double* c, uint32_t i, V3 v) {
+    a[i] = v.x; b[i] = v.y; c[i] = v.z;
+}
+__device__ __forceinline__ Qt ldq(const BodiesDev& B, uint32_t i) { return Qt{B.q0[i], B.q1[i], B.q2[i], B.q3[i]}; }
+
+__device__ __forceinline__ Geom load_geom(const BodiesDev& B, uint32_t i) {
+    Geom g;
+    g.x = ld3(B.x0, B.x1, B.x2, i);
+    g.q = ldq(B, i);
+    g.he = ld3(B.he0, B.he1, B.he2, i);
+    g.radius = B.rad[i];
+    g.hh = B.hh[i];
+    g.shape = B.shape[i];
+    uint32_t h0 = B.hull_off[i], h1 = B.hull_off[i + 1];
+    g.hull = B.hull + 3u * h0;
+    g.nhull = h1 - h0;
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// Integrate: src/world.cpp:373-411
+// ---------------------------------------------------------------------------
+
+template <bool FORCES>
+__global__ void __launch_bounds__(TPB) k_integrate(BodiesDev B, SimParams P) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    const double dt = P.dt;
+    const uint32_t fl = B.flags[i];
+    V3 f = FORCES ? ld3(B.f0, B.f1, B.f2, i) : V3{0.0, 0.0, 0.0};
+    V3 tq = FORCES ? ld3(B.t0, B.t1, B.t2, i) : V3{0.0, 0.0, 0.0};
+    if (fl & NE_FLAG_KINEMATIC) {
+        V3 x = ld3(B.x0, B.x1, B.x2, i), v = ld3(B.v0, B.v1, B.v2, i), w = ld3(B.w0, B.w1, B.w2, i);
+        st3(B.x0, B.x1, B.x2, i, x + v * dt);
+        Qt q = integrate_orientation(ldq(B, i), w, dt);
+        B.q0[i] = q.w; B.q1[i] = q.x; B.q2[i] = q.y; B.q3[i] = q.z;
+    } else {
+        const double im = B.im[i];
+        if (!(im <= 0.0)) {
+            V3 x = ld3(B.x0, B.x1, B.x2, i), v = ld3(B.v0, B.v1, B.v2, i), w = ld3(B.w0, B.w1, B.w2, i);
+            Qt q = ldq(B, i);
+            V3 invI = ld3(B.ii0, B.ii1, B.ii2, i);
+            const V3 g{P.g[0], P.g[1], P.g[2]};
+            v = v + (g * B.gs[i] + f * im) * dt;
+            x = x + v * dt;
+            V3 L = angular_momentum(q, invI, w);
+            L = L + tq * dt;
+            q = integrate_orientation(q, w, dt);
+            w = apply_inv_inertia(q, invI, L);
+            const double ld = B.ldamp[i], ad = B.adamp[i];
+            if (ld > 0.0) v = v * (1.0 / (1.0 + ld * dt));
+            if (ad > 0.0) w = w * (1.0 / (1.0 + ad * dt));
+            st3(B.x0, B.x1, B.x2, i, x);
+            st3(B.v0, B.v1, B.v2, i, v);
+            st3(B.w0, B.w1, B.w2, i, w);
+            B.q0[i] = q.w; B.q1[i] = q.x; B.q2[i] = q.y; B.q3[i] = q.z;
+        }
+    }
+    if (FORCES) {
+        st3(B.f0, B.f1, B.f2, i, V3{0.0, 0.0, 0.0});
+        st3(B.t0, B.t1, B.t2, i, V3{0.0, 0.0, 0.0});
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Uniform grid (counting sort).  Cells are >= 2*maxR of the gridded bodies
+// wide, so every pair whose bounding spheres overlap lies in the same or an
+// adjacent cell (periodic: modulo the cell count).  Bodies far larger than
+// the rest (planes, a ground box) are kept out of the grid and paired with
+// everything, so they cannot inflate the cell size the way they do in the
+// reference (src/world.cpp:429-433).
+// ---------------------------------------------------------------------------
+
+constexpr int AABB_BLOCKS = 256;
+
+__global__ void __launch_bounds__(TPB) k_aabb_partial(BodiesDev B, const uint32_t* __restrict__ gridded,
+                                                      uint32_t ngrid, double* __restrict__ part) {
+    __shared__ double lo[3][TPB], hi[3][TPB];
+    double l0 = 1e300, l1 = 1e300, l2 = 1e300, h0 = -1e300, h1 = -1e300, h2 = -1e300;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ngrid; t += gridDim.x * blockDim.x) {
+        uint32_t i = gridded[t];
+        double a = B.x0[i], b = B.x1[i], c = B.x2[i];
+        l0 = fmin(l0, a); l1 = fmin(l1, b); l2 = fmin(l2, c);
+        h0 = fmax(h0, a); h1 = fmax(h1, b); h2 = fmax(h2, c);
+    }
+    lo[0][threadIdx.x] = l0; lo[1][threadIdx.x] = l1; lo[2][threadIdx.x] = l2;
+    hi[0][threadIdx.x] = h0; hi[1][threadIdx.x] = h1; hi[2][threadIdx.x] = h2;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int a = 0; a < 3; ++a) {
+                lo[a][threadIdx.x] = fmin(lo[a][threadIdx.x], lo[a][threadIdx.x + s]);
+                hi[a][threadIdx.x] = fmax(hi[a][threadIdx.x], hi[a][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 3; ++a) { part[6 * blockIdx.x + a] = lo[a][0]; part[6 * blockIdx.x + 3 + a] = hi[a][0]; }
+}
+
+__global__ void k_grid_setup(SimParams P, double cell, uint32_t cell_cap, const double* __restrict__ part,
+                             int nparts, GridDev* G, uint32_t* counters) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    GridDev g;
+    g.periodic = P.box.periodic;
+    if (P.box.periodic) {
+        for (int a = 0; a < 3; ++a) {
+            const double L = P.box.edge[a];
+            double k = floor(L / cell);
+            if (k < 1.0) k = 1.0;
+            if (k > 1048576.0) k = 1048576.0;
+            g.nc[a] = (int32_t)k;
+            g.cw[a] = L / k;
+            g.origin[a] = -P.box.half[a];
+        }
+        // keep the cell table within its capacity by coarsening (still >= 2 maxR)
+        while ((double)g.nc[0] * g.nc[1] * g.nc[2] > (double)cell_cap) {
+            for (int a = 0; a < 3; ++a) {
+                int32_t k = g.nc[a] * 4 / 5;
+                if (k < 1) k = 1;
+                g.nc[a] = k;
+                g.cw[a] = P.box.edge[a] / (double)k;
+            }
+        }
+    } else {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int p = 0; p < nparts; ++p)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fmin(lo[a], part[6 * p + a]);
+                hi[a] = fmax(hi[a], part[6 * p + 3 + a]);
+            }
+        double c = cell;
+        for (int tries = 0; tries < 64; ++tries) {
+            double tot = 1.0;
+            for (int a = 0; a < 3; ++a) {
+                double span = hi[a] - lo[a];
+                if (!(span >= 0.0)) span = 0.0;
+                double k = floor(span / c) + 1.0;
+                tot *= k;
+            }
+            if (tot <= (double)cell_cap) break;
+            c *= 1.25;
+        }
+        for (int a = 0; a < 3; ++a) {
+            double span = hi[a] - lo[a];
+            if (!(span >= 0.0)) span = 0.0;
+            double k = floor(span / c) + 1.0;
+            if (k > (double)cell_cap) k = (double)cell_cap;
+            g.nc[a] = (int32_t)k;
+            g.cw[a] = c;
+            g.origin[a] = lo[a] <= hi[a] ? lo[a] : 0.0;
+        }
+    }
+    if ((double)g.nc[0] * (double)g.nc[1] * (double)g.nc[2] > (double)cell_cap) {
+        // degenerate extents (e.g. non-finite positions): one cell holds all
+        for (int a = 0; a < 3; ++a) { g.nc[a] = 1; g.cw[a] = 1.0; g.origin[a] = 0.0; }
+    }
+    g.ncells = (uint32_t)g.nc[0] * (uint32_t)g.nc[1] * (uint32_t)g.nc[2];
+    *G = g;
+    counters[CNT_CELLS] = g.ncells;
+}
+
+// Cell coordinate along axis a, always in [0, nc-1] (NaN maps to 0).
+__device__ __forceinline__ int32_t cell_coord(const GridDev& g, int a, double c) {
+    double k = floor((c - g.origin[a]) / g.cw[a]);
+    const double m = (double)g.nc[a];
+    if (g.periodic) k = k - m * floor(k / m);
+    if (!(k >= 0.0)) k = 0.0;
+    if (k > m - 1.0) k = m - 1.0;
+    return (int32_t)k;
+}
+
+__global__ void __launch_bounds__(TPB) k_cell_count(BodiesDev B, const uint32_t* __restrict__ gridded, uint32_t ngrid,
+                                                    const GridDev* __restrict__ Gp, uint32_t* __restrict__ cell_of,
+                                                    uint32_t* __restrict__ cell_count) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ngrid) return;
+    const GridDev g = *Gp;
+    const uint32_t i = gridded[t];
+    int32_t cx = cell_coord(g, 0, B.x0[i]), cy = cell_coord(g, 1, B.x1[i]), cz = cell_coord(g, 2, B.x2[i]);
+    uint32_t c = ((uint32_t)cz * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[0] + (uint32_t)cx;
+    cell_of[t] = c;
+    atomicAdd(&cell_count[c], 1u);
+}
+
+struct SortedBody {
+    double x, y, z, r;
+};
+
+// cell_start holds exclusive offsets; the atomic bump leaves cell_start[c] at
+// the END of cell c, so begin(c) = cell_start[c-1] afterwards.
+__global__ void __launch_bounds__(TPB) k_cell_fill(BodiesDev B, const uint32_t* __restrict__ gridded, uint32_t ngrid,
+                                                   const uint32_t* __restrict__ cell_of, uint32_t* __restrict__ cell_start,
+                                                   uint32_t* __restrict__ sorted_idx, SortedBody* __restrict__ sorted) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ngrid) return;
+    const uint32_t i = gridded[t];
+    uint32_t slot = atomicAdd(&cell_start[cell_of[t]], 1u);
+    sorted_idx[slot] = i;
+    sorted[slot] = SortedBody{B.x0[i], B.x1[i], B.x2[i], B.bR[i]};
+}
+
+// Pair filters of src/world.cpp:513-516 (world.hpp:94-103).
+__device__ __forceinline__ bool pair_passes(const BodiesDev& B, uint32_t i, uint32_t j) {
+    if (B.im[i] + B.im[j] <= 0.0) return false;
+    const uint32_t li = B.layer[i], lj = B.layer[j];
+    if (!((B.mask[i] & (1u << (lj & 31u))) && (B.mask[j] & (1u << (li & 31u))))) return false;
+    for (uint32_t k = B.nc_off[i]; k < B.nc_off[i + 1]; ++k)
+        if (B.nc_idx[k] == j) return false;
+    if ((B.flags[i] | B.flags[j]) & NE_FLAG_SENSOR) return false;
+    return true;
+}
+
+// Conservative bounding-sphere overlap under the minimum image.
+__device__ __forceinline__ bool bounds_touch(const Box3& box, V3 a, double ra, V3 b, double rb) {
+    V3 d = min_image(box, a - b);
+    double s = (ra + rb) * (1.0 + 1e-9) + 1e-9;
+    return norm2(d) <= s * s;
+}
+
+// Distinct neighbour offsets along one axis (fewer than 3 cells => dedupe).
+__device__ __forceinline__ int axis_neighbours(const GridDev& g, int a, int32_t c, int32_t out[3]) {
+    const int32_t n = g.nc[a];
+    int k = 0;
+    for (int d = -1; d <= 1; ++d) {
+        int32_t v = c + d;
+        if (g.periodic) {
+            v = ((v % n) + n) % n;
+        } else if (v < 0 || v >= n) {
+            continue;
+        }
+        bool dup = false;
+        for (int m = 0; m < k; ++m) dup |= (out[m] == v);
+        if (!dup) out[k++] = v;
+    }
+    return k;
+}
+
+// One thread per gridded body (in cell-sorted order): walks the 27-cell
+// neighbourhood and, in WRITE mode, emits its row (i, j>i) then sorts it.
+template <bool WRITE>
+__global__ void __launch_bounds__(TPB) k_cand_rows(BodiesDev B, SimParams P, const GridDev* __restrict__ Gp,
+                                                   uint32_t ngrid, const uint32_t* __restrict__ sorted_idx,
+                                                   const SortedBody* __restrict__ sorted,
+                                                   const uint32_t* __restrict__ cell_end,
+                                                   const uint32_t* __restrict__ excl, uint32_t nexcl,
+                                                   uint32_t* __restrict__ row_count, const uint32_t* __restrict__ row_start,
+                                                   const uint32_t* __restrict__ pcap_dev, uint32_t pcap,
+                                                   uint32_t* __restrict__ cand_i, uint32_t* __restrict__ cand_j) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ngrid) return;
+    const GridDev g = *Gp;
+    const uint32_t i = sorted_idx[s];
+    const SortedBody me = sorted[s];
+    const V3 xi{me.x, me.y, me.z};
+    int32_t nx[3], ny[3], nz[3];
+    const int kx = axis_neighbours(g, 0, cell_coord(g, 0, me.x), nx);
+    const int ky = axis_neighbours(g, 1, cell_coord(g, 1, me.y), ny);
+    const int kz = axis_neighbours(g, 2, cell_coord(g, 2, me.z), nz);
+    uint32_t cnt = 0;
+    uint32_t base = 0, lim = 0;
+    if (WRITE) {
+        base = row_start[i];
+        lim = pcap;
+    }
+    for (int c = 0; c < kz; ++c)
+        for (int b = 0; b < ky; ++b)
+            for (int a = 0; a < kx; ++a) {
+                const uint32_t cell = ((uint32_t)nz[c] * (uint32_t)g.nc[1] + (uint32_t)ny[b]) * (uint32_t)g.nc[0] + (uint32_t)nx[a];
+                const uint32_t beg = cell ? cell_end[cell - 1] : 0u, end = cell_end[cell];
+                for (uint32_t t = beg; t < end; ++t) {
+                    const uint32_t j = sorted_idx[t];
+                    if (j <= i) continue;
+                    const SortedBody o = sorted[t];
+                    if (!bounds_touch(P.box, xi, me.r, V3{o.x, o.y, o.z}, o.r)) continue;
+                    if (!pair_passes(B, i, j)) continue;
+                    if (WRITE) {
+                        if (base + cnt < lim) { cand_i[base + cnt] = i; cand_j[base + cnt] = j; }
+                    }
+                    ++cnt;
+                }
+            }
+    for (uint32_t e = 0; e < nexcl; ++e) {
+        const uint32_t j = excl[e];
+        if (j <= i) continue;
+        if (!bounds_touch(P.box, xi, me.r, ld3(B.x0, B.x1, B.x2, j), B.bR[j])) continue;
+        if (!pair_passes(B, i, j)) continue;
+        if (WRITE) {
+            if (base + cnt < lim) { cand_i[base + cnt] = i; cand_j[base + cnt] = j; }
+        }
+        ++cnt;
+    }
+    if (!WRITE) {
+        row_count[i] = cnt;
+    } else {
+        // insertion sort of the row (rows are short: a 27-cell neighbourhood)
+        uint32_t m = cnt;
+        if (base + m > lim) m = base < lim ? lim - base : 0u;
+        for (uint32_t a = 1; a < m; ++a) {
+            uint32_t key = cand_j[base + a];
+            uint32_t b = a;
+            while (b > 0 && cand_j[base + b - 1] > key) { cand_j[base + b] = cand_j[base + b - 1]; --b; }
+            cand_j[base + b] = key;
+        }
+    }
+}
+
+// Rows of grid-excluded bodies X: (X, j) for every j > X, in j order.
+// flags[e*n + j] = 1 when the pair survives the bound test and the filters.
+__global__ void __launch_bounds__(TPB) k_excl_flags(BodiesDev B, SimParams P, const uint32_t* __restrict__ excl,
+                                                    uint32_t* __restrict__ flags) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t e = blockIdx.y;
+    if (j >= B.n) return;
+    const uint32_t X = excl[e];
+    uint32_t f = 0;
+    if (j > X && bounds_touch(P.box, ld3(B.x0, B.x1, B.x2, X), B.bR[X], ld3(B.x0, B.x1, B.x2, j), B.bR[j]) &&
+        pair_passes(B, X, j))
+        f = 1;
+    flags[(size_t)e * B.n + j] = f;
+}
+
+__global__ void k_excl_rowcount(const uint32_t* __restrict__ excl, uint32_t nexcl, uint32_t n,
+                                const uint32_t* __restrict__ ex_off, uint32_t* __restrict__ row_count) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nexcl) return;
+    row_count[excl[e]] = ex_off[(size_t)(e + 1) * n] - ex_off[(size_t)e * n];
+}
+
+__global__ void __launch_bounds__(TPB) k_excl_write(uint32_t n, const uint32_t* __restrict__ excl,
+                                                    const uint32_t* __restrict__ flags, const uint32_t* __restrict__ ex_off,
+                                                    const uint32_t* __restrict__ row_start, uint32_t pcap,
+                                                    uint32_t* __restrict__ cand_i, uint32_t* __restrict__ cand_j) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t e = blockIdx.y;
+    if (j >= n) return;
+    const size_t f = (size_t)e * n + j;
+    if (!flags[f]) return;
+    const uint32_t X = excl[e];
+    const uint32_t pos = row_start[X] + (ex_off[f] - ex_off[(size_t)e * n]);
+    if (pos < pcap) { cand_i[pos] = X; cand_j[pos] = j; }
+}
+
+__global__ void k_clamp_count(const uint32_t* __restrict__ raw, uint32_t cap, uint32_t* counters, int slot_eff,
+                              int slot_raw, uint32_t fault_bit) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t r = *raw;
+    counters[slot_raw] = r;
+    if (r > cap) { counters[CNT_FAULT] |= fault_bit; r = cap; }
+    counters[slot_eff] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Narrow phase: one thread per candidate pair, gatherContacts of
+// src/world.cpp:122-129 (detectContact + box-box manifold).
+// ---------------------------------------------------------------------------
+
+template <bool SPHERES_ONLY>
+__global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters_in,
+                                                const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                uint32_t* __restrict__ ncont, double* __restrict__ ct,
+                                                uint32_t* __restrict__ fault) {
+    const uint32_t np = counters_in[CNT_P];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        const uint32_t i = cand_i[p], j = cand_j[p];
+        uint32_t nc = 0;
+        double* out = ct + (size_t)p * 28;   // 4 contacts x (n3, pt3, depth)
+        const int si = SPHERES_ONLY ? (int)SH_SPHERE : B.shape[i];
+        const int sj = SPHERES_ONLY ? (int)SH_SPHERE : B.shape[j];
+        if (SPHERES_ONLY || (si == SH_SPHERE && sj == SH_SPHERE)) {
+            Contact c = sphere_sphere(P.box, ld3(B.x0, B.x1, B.x2, i), B.rad[i], ld3(B.x0, B.x1, B.x2, j), B.rad[j]);
+            if (c.hit) {
+                out[0] = c.normal.x; out[1] = c.normal.y; out[2] = c.normal.z;
+                out[3] = c.point.x; out[4] = c.point.y; out[5] = c.point.z; out[6] = c.overlap;
+                nc = 1;
+            }
+        } else if (!SPHERES_ONLY) {
+            const Geom a = load_geom(B, i), b = load_geom(B, j);
+            Contact c = detect_contact(P.box, a, b, fault);
+            if (c.hit) {
+                if (si == SH_BOX && sj == SH_BOX) {
+                    V3 pts[4];
+                    double dep[4];
+                    int m = box_box_manifold(P.box, a, b, c.normal, pts, dep, fault);
+                    for (int k = 0; k < m; ++k) {
+                        double* o = out + 7 * k;
+                        o[0] = c.normal.x; o[1] = c.normal.y; o[2] = c.normal.z;
+                        o[3] = pts[k].x; o[4] = pts[k].y; o[5] = pts[k].z; o[6] = dep[k];
+                    }
+                    nc = (uint32_t)m;
+                }
+                if (nc == 0) {
+                    out[0] = c.normal.x; out[1] = c.normal.y; out[2] = c.normal.z;
+                    out[3] = c.point.x; out[4] = c.point.y; out[5] = c.point.z; out[6] = c.overlap;
+                    nc = 1;
+                }
+            }
+        }
+        ncont[p] = nc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Constraint build: src/world.cpp:522-553 (+ effMass once, contact.hpp:28-33)
+// ---------------------------------------------------------------------------
+
+struct ConsDev {
+    uint32_t *ci, *cj, *slot;
+    double *n0, *n1, *n2, *p0, *p1, *p2, *ra0, *ra1, *ra2, *rb0, *rb1, *rb2;
+    double *ov, *rbias, *mu, *nimp, *timp, *kn;
+    uint64_t* key;
+};
+
+__device__ __forceinline__ double combine_mat(int m, double a, double b) {   // world.cpp:42-51
+    switch (m) {
+        case NE_COMBINE_AVERAGE: return 0.5 * (a + b);
+        case NE_COMBINE_MIN: return a < b ? a : b;
+        case NE_COMBINE_MAX: return a > b ? a : b;
+        case NE_COMBINE_MULTIPLY: return a * b;
+        case NE_COMBINE_GEOMETRIC_MEAN: return sqrt((a > 0 ? a : 0) * (b > 0 ? b : 0));
+    }
+    return 0.5 * (a + b);
+}
+
+__device__ __forceinline__ double eff_mass(double ima, Qt qa, V3 iia, double imb, Qt qb, V3 iib, V3 ra, V3 rb, V3 u) {
+    V3 angA = cross(apply_inv_inertia(qa, iia, cross(ra, u)), ra);
+    V3 angB = cross(apply_inv_inertia(qb, iib, cross(rb, u)), rb);
+    return ima + imb + dot(angA + angB, u);
+}
+
+__global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters_in,
+                                                    const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                    const uint32_t* __restrict__ ncont, const uint32_t* __restrict__ cons_off,
+                                                    const double* __restrict__ ct, ConsDev C, uint32_t ccap,
+                                                    const uint64_t* __restrict__ prev_key, const double* __restrict__ prev_imp,
+                                                    const uint32_t* __restrict__ prev_n) {
+    const uint32_t np = counters_in[CNT_P];
+    const uint32_t pn = *prev_n;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        const uint32_t nc = ncont[p];
+        if (!nc) continue;
+        const uint32_t i = cand_i[p], j = cand_j[p];
+        const double ima = B.im[i], imb = B.im[j];
+        const double ra = B.rest[i] < 0 ? P.restitution : B.rest[i];
+        const double rb = B.rest[j] < 0 ? P.restitution : B.rest[j];
+        const double fa = B.fric[i] < 0 ? P.friction : B.fric[i];
+        const double fb = B.fric[j] < 0 ? P.friction : B.fric[j];
+        const double e = combine_mat(P.rcomb, ra, rb);
+        const double mu = combine_mat(P.fcomb, fa, fb);
+        const V3 xa = ld3(B.x0, B.x1, B.x2, i), xb = ld3(B.x0, B.x1, B.x2, j);
+        const V3 va = ld3(B.v0, B.v1, B.v2, i), vb = ld3(B.v0, B.v1, B.v2, j);
+        const V3 wa = ld3(B.w0, B.w1, B.w2, i), wb = ld3(B.w0, B.w1, B.w2, j);
+        const Qt qa = ldq(B, i), qb = ldq(B, j);
+        const V3 iia = ld3(B.ii0, B.ii1, B.ii2, i), iib = ld3(B.ii0, B.ii1, B.ii2, j);
+        const uint64_t key = ((uint64_t)i << 32) | (uint64_t)j;   // world.cpp:34-36
+        // warm cache: previous step's constraints are sorted by (key, slot)
+        uint32_t lo = 0;
+        if (P.warm) {
+            uint32_t hi = pn;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (prev_key[mid] < key) lo = mid + 1; else hi = mid;
+            }
+        }
+        const uint32_t k0 = cons_off[p];
+        for (uint32_t s = 0; s < nc; ++s) {
+            const uint32_t k = k0 + s;
+            if (k >= ccap) break;
+            const double* c = ct + (size_t)p * 28 + 7 * s;
+            const V3 n{c[0], c[1], c[2]}, pt{c[3], c[4], c[5]};
+            const V3 rA = min_image(P.box, pt - xa);
+            const V3 rB = min_image(P.box, pt - xb);
+            const V3 vc = (va + cross(wa, rA)) - (vb + cross(wb, rB));
+            const double vn = dot(vc, n);
+            double nimp = 0.0;
+            if (P.warm && lo + s < pn && prev_key[lo + s] == key) nimp = prev_imp[lo + s];
+            C.ci[k] = i; C.cj[k] = j; C.slot[k] = s;
+            C.n0[k] = n.x; C.n1[k] = n.y; C.n2[k] = n.z;
+            C.p0[k] = pt.x; C.p1[k] = pt.y; C.p2[k] = pt.z;
+            C.ra0[k] = rA.x; C.ra1[k] = rA.y; C.ra2[k] = rA.z;
+            C.rb0[k] = rB.x; C.rb1[k] = rB.y; C.rb2[k] = rB.z;
+            C.ov[k] = c[6];
+            C.rbias[k] = (vn < -0.5) ? -e * vn : 0.0;                     // world.cpp:40, 544
+            C.mu[k] = mu;
+            C.nimp[k] = nimp;
+            C.timp[k] = 0.0;
+            C.kn[k] = eff_mass(ima, qa, iia, imb, qb, iib, rA, rB, n);
+            C.key[k] = key;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Islands: union-find over bodies whose velocity can change.  Static and
+// kinematic bodies (invMass == 0 and zero inverse inertia) only receive +-0
+// updates in the reference, so they never join islands and are never written.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool mut_v(const BodiesDev& B, uint32_t b) { return B.im[b] != 0.0; }
+__device__ __forceinline__ bool mut_w(const BodiesDev& B, uint32_t b) {
+    return B.ii0[b] != 0.0 || B.ii1[b] != 0.0 || B.ii2[b] != 0.0;
+}
+__device__ __forceinline__ bool mutable_body(const BodiesDev& B, uint32_t b) { return mut_v(B, b) || mut_w(B, b); }
+
+__device__ __forceinline__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
+    uint32_t p = __atomic_load_n(&parent[x], __ATOMIC_RELAXED);
+    while (p != x) {
+        uint32_t gp = __atomic_load_n(&parent[p], __ATOMIC_RELAXED);
+        if (gp != p) __atomic_store_n(&parent[x], gp, __ATOMIC_RELAXED);   // path halving (benign)
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__global__ void __launch_bounds__(TPB) k_uf_link(BodiesDev B, const uint32_t* __restrict__ counters_in,
+                                                 const ConsDev C, uint32_t* __restrict__ parent) {
+    const uint32_t nc = counters_in[CNT_C];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        const uint32_t i = C.ci[k], j = C.cj[k];
+        if (!mutable_body(B, i) || !mutable_body(B, j)) continue;
+        uint32_t a = i, b = j;
+        for (int guard = 0; guard < 1 << 20; ++guard) {
+            a = uf_find(parent, a);
+            b = uf_find(parent, b);
+            if (a == b) break;
+            if (a < b) { uint32_t t = a; a = b; b = t; }   // hang the larger root under the smaller
+            if (atomicCAS(&parent[a], a, b) == a) break;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_uf_root(BodiesDev B, const uint32_t* __restrict__ counters_in, const ConsDev C,
+                                                 uint32_t* __restrict__ parent, uint32_t* __restrict__ cons_root,
+                                                 uint32_t* __restrict__ used) {
+    const uint32_t nc = counters_in[CNT_C];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        const uint32_t i = C.ci[k];
+        const uint32_t body = mutable_body(B, i) ? i : C.cj[k];
+        const uint32_t r = uf_find(parent, body);
+        cons_root[k] = r;
+        used[r] = 1u;
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_isl_key(const uint32_t* __restrict__ counters_in, const uint32_t* __restrict__ cons_root,
+                                                 const uint32_t* __restrict__ root_id, uint32_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals) {
+    const uint32_t nc = counters_in[CNT_C];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        keys[k] = root_id[cons_root[k]];
+        vals[k] = k;
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_isl_bounds(const uint32_t* __restrict__ counters_in, const uint32_t* __restrict__ skeys,
+                                                    const uint32_t* __restrict__ root_total, uint32_t* __restrict__ isl_start,
+                                                    uint32_t* counters) {
+    const uint32_t nc = counters_in[CNT_C];
+    const uint32_t nisl = *root_total;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += gridDim.x * blockDim.x)
+        if (t == 0 || skeys[t] != skeys[t - 1]) isl_start[skeys[t]] = t;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        isl_start[nisl] = nc;
+        counters[CNT_ISL] = nisl;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Solve: one thread per island sweeps the island's constraints in global
+// (i, j, slot) order: warm start (world.cpp:559-568), I iterations of normal +
+// Coulomb friction (world.cpp:572-612), positional correction (615-616).
+// ---------------------------------------------------------------------------
+
+struct BodyDyn {
+    V3 v, w;
+};
+
+__device__ __forceinline__ void apply_pair_impulse(BodiesDev& B, uint32_t i, uint32_t j, V3 rA, V3 rB, V3 J) {
+    const double ima = B.im[i], imb = B.im[j];
+    if (ima != 0.0) {
+        V3 v = ld3(B.v0, B.v1, B.v2, i);
+        st3(B.v0, B.v1, B.v2, i, v + J * ima);
+    }
+    if (B.ii0[i] != 0.0 || B.ii1[i] != 0.0 || B.ii2[i] != 0.0) {
+        V3 w = ld3(B.w0, B.w1, B.w2, i);
+        st3(B.w0, B.w1, B.w2, i, w + apply_inv_inertia(ldq(B, i), ld3(B.ii0, B.ii1, B.ii2, i), cross(rA, J)));
+    }
+    if (imb != 0.0) {
+        V3 v = ld3(B.v0, B.v1, B.v2, j);
+        st3(B.v0, B.v1, B.v2, j, v - J * imb);
+    }
+    if (B.ii0[j] != 0.0 || B.ii1[j] != 0.0 || B.ii2[j] != 0.0) {
+        V3 w = ld3(B.w0, B.w1, B.w2, j);
+        st3(B.w0, B.w1, B.w2, j, w - apply_inv_inertia(ldq(B, j), ld3(B.ii0, B.ii1, B.ii2, j), cross(rB, J)));
+    }
+}
+
+__device__ __forceinline__ V3 contact_vel(const BodiesDev& B, uint32_t i, uint32_t j, V3 rA, V3 rB) {
+    return (ld3(B.v0, B.v1, B.v2, i) + cross(ld3(B.w0, B.w1, B.w2, i), rA)) -
+           (ld3(B.v0, B.v1, B.v2, j) + cross(ld3(B.w0, B.w1, B.w2, j), rB));
+}
+
+__global__ void __launch_bounds__(TPB) k_solve_islands(BodiesDev B, SimParams P, uint32_t* counters_out,
+                                                       const uint32_t* __restrict__ isl_start,
+                                                       const uint32_t* __restrict__ isl_list, ConsDev C) {
+    const uint32_t nisl = counters_out[CNT_ISL];
+    for (uint32_t isl = blockIdx.x * blockDim.x + threadIdx.x; isl < nisl; isl += gridDim.x * blockDim.x) {
+        const uint32_t s = isl_start[isl], e = isl_start[isl + 1];
+        atomicMax(&counters_out[CNT_MAXISL], e - s);
+        if (P.warm) {
+            for (uint32_t t = s; t < e; ++t) {
+                const uint32_t k = isl_list[t];
+                const double imp = C.nimp[k];
+                if (imp == 0.0) continue;
+                const V3 n{C.n0[k], C.n1[k], C.n2[k]};
+                apply_pair_impulse(B, C.ci[k], C.cj[k], V3{C.ra0[k], C.ra1[k], C.ra2[k]},
+                                   V3{C.rb0[k], C.rb1[k], C.rb2[k]}, n * imp);
+            }
+        }
+        for (int it = 0; it < P.iters; ++it) {
+            for (uint32_t t = s; t < e; ++t) {
+                const uint32_t k = isl_list[t];
+                const uint32_t i = C.ci[k], j = C.cj[k];
+                const V3 n{C.n0[k], C.n1[k], C.n2[k]};
+                const V3 rA{C.ra0[k], C.ra1[k], C.ra2[k]}, rB{C.rb0[k], C.rb1[k], C.rb2[k]};
+                const V3 vc = contact_vel(B, i, j, rA, rB);
+                const double vn = dot(vc, n);
+                const double Kn = C.kn[k];
+                double nimp = C.nimp[k];
+                if (Kn > 1e-18) {
+                    double dj = (C.rbias[k] - vn) / Kn;
+                    const double old = nimp;
+                    nimp = (old + dj > 0.0) ? old + dj : 0.0;
+                    dj = nimp - old;
+                    apply_pair_impulse(B, i, j, rA, rB, n * dj);
+                    C.nimp[k] = nimp;
+                }
+                const double mu = C.mu[k];
+                if (mu > 0.0 && nimp > 0.0) {
+                    const V3 vc2 = contact_vel(B, i, j, rA, rB);
+                    const V3 vt = vc2 - n * dot(vc2, n);
+                    const double vtl = norm(vt);
+                    if (vtl > 1e-12) {
+                        const V3 tdir = vt / vtl;
+                        const double Kt = eff_mass(B.im[i], ldq(B, i), ld3(B.ii0, B.ii1, B.ii2, i), B.im[j], ldq(B, j),
+                                                   ld3(B.ii0, B.ii1, B.ii2, j), rA, rB, tdir);
+                        if (Kt > 1e-18) {
+                            double djt = -vtl / Kt;
+                            const double maxF = mu * nimp;
+                            const double old = C.timp[k];
+                            const double sum = old + djt;
+                            const double tn = (sum < -maxF) ? -maxF : (sum > maxF ? maxF : sum);
+                            djt = tn - old;
+                            C.timp[k] = tn;
+                            apply_pair_impulse(B, i, j, rA, rB, tdir * djt);
+                        }
+                    }
+                }
+            }
+        }
+        for (uint32_t t = s; t < e; ++t) {          // contact.hpp:78-85
+            const uint32_t k = isl_list[t];
+            const uint32_t i = C.ci[k], j = C.cj[k];
+            const double ima = B.im[i], imb = B.im[j];
+            const double invSum = ima + imb;
+            const double ov = C.ov[k];
+            if (invSum <= 0.0 || ov <= 0.0) continue;
+            const V3 n{C.n0[k], C.n1[k], C.n2[k]};
+            const V3 corr = n * (P.beta * ov / invSum);
+            if (ima != 0.0) st3(B.x0, B.x1, B.x2, i, ld3(B.x0, B.x1, B.x2, i) + corr * ima);
+            if (imb != 0.0) st3(B.x0, B.x1, B.x2, j, ld3(B.x0, B.x1, B.x2, j) - corr * imb);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundaries: src/world.cpp:698-702 (wrap) and 635-657 (reflective walls)
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(TPB) k_wrap(BodiesDev B, SimParams P) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    if (!(B.im[i] > 0.0)) return;
+    st3(B.x0, B.x1, B.x2, i, wrap_pos(P.box, ld3(B.x0, B.x1, B.x2, i)));
+}
+
+__global__ void __launch_bounds__(TPB) k_walls(BodiesDev B, SimParams P) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    if (B.im[i] <= 0.0) return;
+    Geom g = load_geom(B, i);
+    V3 x = g.x;
+    V3 v = ld3(B.v0, B.v1, B.v2, i);
+    for (int axis = 0; axis < 3; ++axis) {
+        const double h = P.box.half[axis];
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+            V3 outward{axis == 0 ? (double)sgn : 0.0, axis == 1 ? (double)sgn : 0.0, axis == 2 ? (double)sgn : 0.0};
+            g.x = x;
+            V3 sp = support_world(g, outward);
+            double coord = comp(sp, axis) * sgn;
+            double pen = coord - h;
+            if (pen > 0.0) {
+                double* px = axis == 0 ? &x.x : (axis == 1 ? &x.y : &x.z);
+                double* pv = axis == 0 ? &v.x : (axis == 1 ? &v.y : &v.z);
+                *px -= (double)sgn * pen;
+                if (*pv * sgn > 0.0) *pv = -P.restitution * *pv;
+            }
+        }
+    }
+    st3(B.x0, B.x1, B.x2, i, x);
+    st3(B.v0, B.v1, B.v2, i, v);
+}
+
+// Constraint count = cons_off[P] (P read on the device), clamped to capacity.
+__global__ void k_cons_count(const uint32_t* __restrict__ cons_off, uint32_t ccap, uint32_t* counters) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t r = cons_off[counters[CNT_P]];
+    counters[CNT_CRAW] = r;
+    if (r > ccap) { counters[CNT_FAULT] |= NE_FAULT_CONTACTS; r = ccap; }
+    counters[CNT_C] = r;
+}
+
+// This step's (key, slot)-ordered normal impulses become the next step's warm
+// cache (src/world.cpp:619-626).
+__global__ void __launch_bounds__(TPB) k_copy_cache(const uint32_t* __restrict__ counters_in, const uint64_t* __restrict__ key,
+                                                    const double* __restrict__ nimp, uint64_t* __restrict__ prev_key,
+                                                    double* __restrict__ prev_imp, uint32_t* __restrict__ prev_n) {
+    const uint32_t nc = counters_in[CNT_C];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        prev_key[k] = key[k];
+        prev_imp[k] = nimp[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *prev_n = nc;
+}
+
+// FP probe: the device's sqrt / divide / floor on host-given operands, for the
+// bit-equality checks the whole parity story rests on.
+__global__ void k_fp_probe(int op, uint32_t n, const double* a, const double* b, double* out) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double r;
+    if (op == 0) r = sqrt(a[t]);
+    else if (op == 1) r = a[t] / b[t];
+    else if (op == 2) r = floor(a[t]);
+    else r = a[t] * b[t] + a[t];
+    out[t] = r;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+struct ne_gpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    ne_world_params params{};
+    bool have_params = false;
+    // capacities
+    uint32_t n = 0, ncap = 0;
+    uint32_t pcap = 0, ccap = 0, cellcap = 0;
+    uint32_t user_pcap = 0, user_ccap = 0;
+    double cell_scale = 1.0;
+    // body storage
+    double* dfield = nullptr;     // 38 double fields x ncap
+    uint32_t* ufield = nullptr;   // shape, layer, mask, flags x ncap
+    uint32_t* hull_off = nullptr;
+    double* hull = nullptr;
+    uint32_t* nc_off = nullptr;
+    uint32_t* nc_idx = nullptr;
+    BodiesDev B{};
+    bool has_forces = false;
+    bool spheres_only = false;
+    // broadphase
+    uint32_t* gridded = nullptr;
+    uint32_t ngrid = 0;
+    uint32_t* excl = nullptr;
+    uint32_t nexcl = 0;
+    double cell = 1.0;
+    double* aabb_part = nullptr;
+    GridDev* grid = nullptr;
+    uint32_t *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr;
+    uint32_t* sorted_idx = nullptr;
+    SortedBody* sorted = nullptr;
+    uint32_t *row_count = nullptr, *row_start = nullptr;
+    uint32_t *ex_flags = nullptr, *ex_off = nullptr;
+    uint32_t *cand_i = nullptr, *cand_j = nullptr;
+    uint32_t *ncont = nullptr, *cons_off = nullptr;
+    double* ct = nullptr;
+    // constraints (current) + warm cache (previous)
+    ConsDev C{};
+    void* cons_mem = nullptr;
+    uint64_t* prev_key = nullptr;
+    double* prev_imp = nullptr;
+    uint32_t* prev_n = nullptr;
+    uint32_t prev_cap = 0;
+    // islands
+    uint32_t *parent = nullptr, *cons_root = nullptr, *used = nullptr, *root_id = nullptr;
+    uint32_t *skey0 = nullptr, *sval0 = nullptr, *skey1 = nullptr, *sval1 = nullptr;
+    uint32_t* isl_start = nullptr;
+    uint32_t *rs_hist = nullptr, *scan_part = nullptr;
+    // counters
+    uint32_t* counters = nullptr;
+    uint32_t* h_counters = nullptr;
+    int uf_bits = 1;
+};
+
+namespace {
+
+int set_err(ne_gpu_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return set_err(ctx, NE_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+template <class T>
+hipError_t dalloc(T*& p, size_t count) {
+    dfree(p);
+    if (count == 0) count = 1;
+    return hipMalloc((void**)&p, sizeof(T) * count);
+}
+
+enum DF : int {
+    F_X0, F_X1, F_X2, F_V0, F_V1, F_V2, F_Q0, F_Q1, F_Q2, F_Q3, F_W0, F_W1, F_W2,
+    F_F0, F_F1, F_F2, F_T0, F_T1, F_T2, F_IM, F_II0, F_II1, F_II2,
+    F_RAD, F_HH, F_HE0, F_HE1, F_HE2, F_BR, F_FRIC, F_REST, F_LD, F_AD, F_GS, F_COUNT
+};
+enum UF : int { U_SHAPE, U_LAYER, U_MASK, U_FLAGS, U_COUNT };
+
+double host_bounding_radius(int shape, double r, double hh, const double* he, const double* hull, uint32_t h0,
+                            uint32_t h1) {
+    auto nrm = [](double a, double b, double c) { return std::sqrt(a * a + b * b + c * c); };
+    switch (shape) {
+        case NE_SHAPE_SPHERE: return r;
+        case NE_SHAPE_BOX: return nrm(he[0], he[1], he[2]);
+        case NE_SHAPE_CAPSULE: return hh + r;
+        case NE_SHAPE_PLANE: return 1e30;
+        case NE_SHAPE_CONVEX: {
+            double m = 0.0;
+            for (uint32_t k = h0; k < h1; ++k) m = std::max(m, nrm(hull[3 * k], hull[3 * k + 1], hull[3 * k + 2]));
+            return m;
+        }
+        default: return std::sqrt(hh * hh + r * r);   // cylinder
+    }
+}
+
+SimParams make_sim_params(const ne_world_params& p) {
+    SimParams s{};
+    for (int a = 0; a < 3; ++a) {
+        s.box.half[a] = p.half[a];
+        s.box.edge[a] = 2.0 * p.half[a];
+    }
+    s.box.periodic = p.periodic ? 1 : 0;
+    s.dt = p.dt;
+    for (int a = 0; a < 3; ++a) s.g[a] = p.gravity[a];
+    s.restitution = p.restitution;
+    s.friction = p.friction;
+    s.beta = p.contact_beta;
+    s.rcomb = p.restitution_combine;
+    s.fcomb = p.friction_combine;
+    s.warm = p.warm_start ? 1 : 0;
+    s.iters = p.solver_iterations;
+    s.open_boundary = p.open_boundary ? 1 : 0;
+    return s;
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t tpb = TPB) {
+    uint64_t b = (n + tpb - 1) / tpb;
+    if (b < 1) b = 1;
+    return (uint32_t)b;
+}
+uint32_t gs_blocks(uint64_t n) {   // grid-stride launches over device-side counts
+    uint64_t b = (n + TPB - 1) / TPB;
+    if (b < 1) b = 1;
+    if (b > 4096) b = 4096;
+    return (uint32_t)b;
+}
+
+int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
+    c->pcap = pcap;
+    HIP_TRY(c, dalloc(c->cand_i, pcap));
+    HIP_TRY(c, dalloc(c->cand_j, pcap));
+    HIP_TRY(c, dalloc(c->ncont, (size_t)pcap + 1));
+    HIP_TRY(c, dalloc(c->cons_off, (size_t)pcap + 1));
+    HIP_TRY(c, dalloc(c->ct, (size_t)pcap * 28));
+    ScanPlan sp = scan_plan(std::max(pcap + 1, std::max(c->ncap * 4u, c->cellcap + 1)));
+    (void)sp;
+    return NE_OK;
+}
+
+int alloc_cons(ne_gpu_ctx* c, uint32_t ccap) {
+    c->ccap = ccap;
+    ConsDev& C = c->C;
+    HIP_TRY(c, dalloc(C.ci, ccap));
+    HIP_TRY(c, dalloc(C.cj, ccap));
+    HIP_TRY(c, dalloc(C.slot, ccap));
+    double** dp[] = {&C.n0, &C.n1, &C.n2, &C.p0, &C.p1, &C.p2, &C.ra0, &C.ra1, &C.ra2, &C.rb0, &C.rb1, &C.rb2,
+                     &C.ov, &C.rbias, &C.mu, &C.nimp, &C.timp, &C.kn};
+    for (double** p : dp) HIP_TRY(c, dalloc(*p, ccap));
+    HIP_TRY(c, dalloc(C.key, ccap));
+    HIP_TRY(c, dalloc(c->cons_root, ccap));
+    HIP_TRY(c, dalloc(c->skey0, ccap));
+    HIP_TRY(c, dalloc(c->sval0, ccap));
+    HIP_TRY(c, dalloc(c->skey1, ccap));
+    HIP_TRY(c, dalloc(c->sval1, ccap));
+    return NE_OK;
+}
+
+int ensure_prev(ne_gpu_ctx* c, uint32_t cap) {
+    if (c->prev_cap >= cap && c->prev_key) return NE_OK;
+    uint64_t* nk = nullptr;
+    double* ni = nullptr;
+    HIP_TRY(c, hipMalloc((void**)&nk, sizeof(uint64_t) * std::max(cap, 1u)));
+    HIP_TRY(c, hipMalloc((void**)&ni, sizeof(double) * std::max(cap, 1u)));
+    if (c->prev_key && c->prev_cap) {
+        HIP_TRY(c, hipMemcpyAsync(nk, c->prev_key, sizeof(uint64_t) * c->prev_cap, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(ni, c->prev_imp, sizeof(double) * c->prev_cap, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    dfree(c->prev_key);
+    dfree(c->prev_imp);
+    c->prev_key = nk;
+    c->prev_imp = ni;
+    c->prev_cap = cap;
+    return NE_OK;
+}
+
+// Everything but the warm cache and the counters (kept across re-uploads).
+void free_work(ne_gpu_ctx* c) {
+    dfree(c->dfield); dfree(c->ufield); dfree(c->hull_off); dfree(c->hull); dfree(c->nc_off); dfree(c->nc_idx);
+    dfree(c->gridded); dfree(c->excl); dfree(c->aabb_part); dfree(c->grid);
+    dfree(c->cell_of); dfree(c->cell_count); dfree(c->cell_start); dfree(c->sorted_idx); dfree(c->sorted);
+    dfree(c->row_count); dfree(c->row_start); dfree(c->ex_flags); dfree(c->ex_off);
+    dfree(c->cand_i); dfree(c->cand_j); dfree(c->ncont); dfree(c->cons_off); dfree(c->ct);
+    ConsDev& C = c->C;
+    dfree(C.ci); dfree(C.cj); dfree(C.slot);
+    double** dp[] = {&C.n0, &C.n1, &C.n2, &C.p0, &C.p1, &C.p2, &C.ra0, &C.ra1, &C.ra2, &C.rb0, &C.rb1, &C.rb2,
+                     &C.ov, &C.rbias, &C.mu, &C.nimp, &C.timp, &C.kn};
+    for (double** p : dp) dfree(*p);
+    dfree(C.key);
+    dfree(c->parent); dfree(c->cons_root); dfree(c->used); dfree(c->root_id);
+    dfree(c->skey0); dfree(c->sval0); dfree(c->skey1); dfree(c->sval1); dfree(c->isl_start);
+    dfree(c->rs_hist); dfree(c->scan_part);
+}
+
+void free_all(ne_gpu_ctx* c) {
+    free_work(c);
+    dfree(c->prev_key); dfree(c->prev_imp); dfree(c->prev_n); dfree(c->counters);
+    c->prev_cap = 0;
+    if (c->h_counters) { (void)hipHostFree(c->h_counters); c->h_counters = nullptr; }
+}
+
+// One reference step (src/world.cpp:133-146) as a kernel pipeline.
+int enqueue_step(ne_gpu_ctx* c) {
+    hipStream_t s = c->stream;
+    const SimParams P = make_sim_params(c->params);
+    BodiesDev B = c->B;
+    const uint32_t n = c->n;
+    if (n == 0) return NE_OK;
+    if (c->has_forces) {
+        k_integrate<true><<<blocks_for(n), TPB, 0, s>>>(B, P);
+        c->has_forces = false;
+    } else {
+        k_integrate<false><<<blocks_for(n), TPB, 0, s>>>(B, P);
+    }
+    // --- grid
+    if (c->ngrid > 0) {
+        if (!P.box.periodic)
+            k_aabb_partial<<<AABB_BLOCKS, TPB, 0, s>>>(B, c->gridded, c->ngrid, c->aabb_part);
+        k_grid_setup<<<1, 1, 0, s>>>(P, c->cell, c->cellcap, c->aabb_part, AABB_BLOCKS, c->grid, c->counters);
+        k_fill_u32<<<gs_blocks(c->cellcap), TPB, 0, s>>>(c->cell_count, 0u, c->counters + CNT_CELLS, c->cellcap);
+        k_cell_count<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, c->gridded, c->ngrid, c->grid, c->cell_of, c->cell_count);
+        scan_u32(c->cell_count, c->cell_start, c->counters + CNT_CELLS, c->cellcap, c->scan_part, s);
+        k_cell_fill<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, c->gridded, c->ngrid, c->cell_of, c->cell_start,
+                                                         c->sorted_idx, c->sorted);
+    }
+    // --- candidate rows
+    k_fill_u32<<<gs_blocks(n), TPB, 0, s>>>(c->row_count, 0u, nullptr, n);
+    if (c->ngrid > 0)
+        k_cand_rows<false><<<blocks_for(c->ngrid), TPB, 0, s>>>(B, P, c->grid, c->ngrid, c->sorted_idx, c->sorted,
+                                                                c->cell_start, c->excl, c->nexcl, c->row_count, nullptr,
+                                                                nullptr, 0, nullptr, nullptr);
+    if (c->nexcl > 0) {
+        dim3 g(blocks_for(n), c->nexcl);
+        k_excl_flags<<<g, TPB, 0, s>>>(B, P, c->excl, c->ex_flags);
+        scan_u32(c->ex_flags, c->ex_off, nullptr, c->nexcl * n, c->scan_part, s);
+        k_excl_rowcount<<<blocks_for(c->nexcl), TPB, 0, s>>>(c->excl, c->nexcl, n, c->ex_off, c->row_count);
+    }
+    scan_u32(c->row_count, c->row_start, nullptr, n, c->scan_part, s);
+    k_clamp_count<<<1, 1, 0, s>>>(c->row_start + n, c->pcap, c->counters, CNT_P, CNT_PRAW, NE_FAULT_PAIRS);
+    if (c->ngrid > 0)
+        k_cand_rows<true><<<blocks_for(c->ngrid), TPB, 0, s>>>(B, P, c->grid, c->ngrid, c->sorted_idx, c->sorted,
+                                                               c->cell_start, c->excl, c->nexcl, nullptr, c->row_start,
+                                                               nullptr, c->pcap, c->cand_i, c->cand_j);
+    if (c->nexcl > 0) {
+        dim3 g(blocks_for(n), c->nexcl);
+        k_excl_write<<<g, TPB, 0, s>>>(n, c->excl, c->ex_flags, c->ex_off, c->row_start, c->pcap, c->cand_i, c->cand_j);
+    }
+    // --- narrow phase + constraints
+    if (c->spheres_only)
+        k_narrow<true><<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
+                                                          c->counters + CNT_FAULT);
+    else
+        k_narrow<false><<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
+                                                           c->counters + CNT_FAULT);
+    scan_u32(c->ncont, c->cons_off, c->counters + CNT_P, c->pcap, c->scan_part, s);
+    k_cons_count<<<1, 1, 0, s>>>(c->cons_off, c->ccap, c->counters);
+    k_cons_build<<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->cons_off,
+                                                    c->ct, c->C, c->ccap, c->prev_key, c->prev_imp, c->prev_n);
+    // --- islands
+    k_iota_u32<<<gs_blocks(n), TPB, 0, s>>>(c->parent, nullptr, n);
+    k_fill_u32<<<gs_blocks(n), TPB, 0, s>>>(c->used, 0u, nullptr, n);
+    k_uf_link<<<gs_blocks(c->ccap), TPB, 0, s>>>(B, c->counters, c->C, c->parent);
+    k_uf_root<<<gs_blocks(c->ccap), TPB, 0, s>>>(B, c->counters, c->C, c->parent, c->cons_root, c->used);
+    scan_u32(c->used, c->root_id, nullptr, n, c->scan_part, s);
+    k_isl_key<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, c->cons_root, c->root_id, c->skey0, c->sval0);
+    const bool in1 = radix_sort_u32(c->skey0, c->sval0, c->skey1, c->sval1, c->counters + CNT_C, c->ccap, c->uf_bits,
+                                    c->rs_hist, c->scan_part, s);
+    const uint32_t* skeys = in1 ? c->skey1 : c->skey0;
+    const uint32_t* svals = in1 ? c->sval1 : c->sval0;
+    k_isl_bounds<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, skeys, c->root_id + n, c->isl_start, c->counters);
+    // --- solve
+    k_solve_islands<<<gs_blocks(n), TPB, 0, s>>>(B, P, c->counters, c->isl_start, svals, c->C);
+    // --- boundary
+    if (P.box.periodic)
+        k_wrap<<<blocks_for(n), TPB, 0, s>>>(B, P);
+    else if (!P.open_boundary)
+        k_walls<<<blocks_for(n), TPB, 0, s>>>(B, P);
+    // --- warm cache for the next step (world.cpp:619-626): this step's list
+    if (P.warm) {
+        k_copy_cache<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, c->C.key, c->C.nimp, c->prev_key, c->prev_imp,
+                                                        c->prev_n);
+    }
+    return NE_OK;
+}
+
+int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b);
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" {
+
+int ne_gpu_create(const ne_gpu_config* cfg, ne_gpu_ctx** out) {
+    if (!out) return NE_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return NE_ERR_NO_DEVICE;
+    ne_gpu_ctx* c = new ne_gpu_ctx();
+    c->device = cfg ? cfg->device : 0;
+    if (c->device < 0 || c->device >= ndev) { delete c; return NE_ERR_INVALID; }
+    c->user_pcap = cfg ? cfg->pair_capacity : 0;
+    c->user_ccap = cfg ? cfg->contact_capacity : 0;
+    c->cell_scale = (cfg && cfg->cell_scale >= 1.0) ? cfg->cell_scale : 1.0;
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return NE_ERR_HIP;
+    }
+    // World defaults (src/world.hpp:75-126, src/box.hpp:17-18)
+    ne_world_params& p = c->params;
+    p.half[0] = p.half[1] = p.half[2] = 12.0;
+    p.periodic = 0;
+    p.dt = 1.0 / 120.0;
+    p.restitution = 1.0;
+    p.friction = 0.0;
+    p.contact_beta = 0.2;
+    p.restitution_combine = NE_COMBINE_MAX;
+    p.friction_combine = NE_COMBINE_GEOMETRIC_MEAN;
+    p.warm_start = 1;
+    p.solver_iterations = 8;
+    p.sleep_lin_vel = 0.05; p.sleep_ang_vel = 0.05; p.sleep_time = 0.5; p.wake_vel = 0.5;
+    if (hipMalloc((void**)&c->counters, sizeof(uint32_t) * CNT_N) != hipSuccess ||
+        hipHostMalloc((void**)&c->h_counters, sizeof(uint32_t) * CNT_N, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&c->prev_n, sizeof(uint32_t)) != hipSuccess ||
+        hipMemsetAsync(c->prev_n, 0, sizeof(uint32_t), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->counters, 0, sizeof(uint32_t) * CNT_N, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        free_all(c);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return NE_ERR_HIP;
+    }
+    memset(c->h_counters, 0, sizeof(uint32_t) * CNT_N);
+    *out = c;
+    return NE_OK;
+}
+
+int ne_gpu_destroy(ne_gpu_ctx* c) {
+    if (!c) return NE_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NE_OK;
+}
+
+int ne_gpu_set_params(ne_gpu_ctx* c, const ne_world_params* p) {
+    if (!c || !p) return NE_ERR_INVALID;
+    if (!(p->dt > 0.0) || p->solver_iterations < 0)
+        return set_err(c, NE_ERR_INVALID, "dt must be > 0 and solver_iterations >= 0");
+    for (int a = 0; a < 3; ++a)
+        if (!(p->half[a] > 0.0)) return set_err(c, NE_ERR_INVALID, "box half extents must be > 0");
+    if (p->restitution_combine < 0 || p->restitution_combine > 4 || p->friction_combine < 0 || p->friction_combine > 4)
+        return set_err(c, NE_ERR_INVALID, "unknown combine mode");
+    if (p->sleep_enabled) return set_err(c, NE_ERR_UNSUPPORTED, "island sleeping is not on the GPU path yet");
+    const bool periodic_changed = c->have_params && ((p->periodic != 0) != (c->params.periodic != 0));
+    c->params = *p;
+    c->have_params = true;
+    (void)periodic_changed;
+    return NE_OK;
+}
+
+int ne_gpu_set_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
+    if (!c || !b) return NE_ERR_INVALID;
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    return upload_bodies(c, b);
+}
+
+int ne_gpu_set_state(ne_gpu_ctx* c, const double* x, const double* v, const double* q, const double* w) {
+    if (!c) return NE_ERR_INVALID;
+    const uint32_t n = c->n;
+    if (n == 0) return NE_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::vector<double> tmp((size_t)n);
+    auto put = [&](const double* src, int comps, int stride, const int* fields) -> int {
+        for (int k = 0; k < comps; ++k) {
+            for (uint32_t i = 0; i < n; ++i) tmp[i] = src[(size_t)stride * i + k];
+            HIP_TRY(c, hipMemcpy(c->dfield + (size_t)fields[k] * c->ncap, tmp.data(), sizeof(double) * n,
+                                 hipMemcpyHostToDevice));
+        }
+        return NE_OK;
+    };
+    int r;
+    const int fx[3] = {F_X0, F_X1, F_X2}, fv[3] = {F_V0, F_V1, F_V2}, fq[4] = {F_Q0, F_Q1, F_Q2, F_Q3},
+              fw[3] = {F_W0, F_W1, F_W2};
+    if (x && (r = put(x, 3, 3, fx)) != NE_OK) return r;
+    if (v && (r = put(v, 3, 3, fv)) != NE_OK) return r;
+    if (q && (r = put(q, 4, 4, fq)) != NE_OK) return r;
+    if (w && (r = put(w, 3, 3, fw)) != NE_OK) return r;
+    return NE_OK;
+}
+
+int ne_gpu_step(ne_gpu_ctx* c, int32_t nsteps) {
+    if (!c) return NE_ERR_INVALID;
+    if (nsteps < 0) return set_err(c, NE_ERR_INVALID, "nsteps < 0");
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    if (c->n == 0) return NE_OK;
+    HIP_TRY(c, hipMemsetAsync(c->counters, 0, sizeof(uint32_t) * CNT_N, c->stream));
+    for (int32_t t = 0; t < nsteps; ++t) {
+        int r = enqueue_step(c);
+        if (r != NE_OK) return r;
+    }
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(uint32_t) * CNT_N, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const uint32_t fault = c->h_counters[CNT_FAULT];
+    if (fault) {
+        // grow what overflowed so a re-upload + retry succeeds; state is undefined
+        if (fault & NE_FAULT_PAIRS) {
+            uint32_t want = std::max(c->pcap * 2u, c->h_counters[CNT_PRAW] + c->h_counters[CNT_PRAW] / 2u + 1024u);
+            int r = alloc_pairs(c, want);
+            if (r != NE_OK) return r;
+        }
+        if (fault & NE_FAULT_CONTACTS) {
+            uint32_t want = std::max(c->ccap * 2u, c->h_counters[CNT_CRAW] + c->h_counters[CNT_CRAW] / 2u + 1024u);
+            int r = alloc_cons(c, want);
+            if (r != NE_OK) return r;
+            r = ensure_prev(c, want);
+            if (r != NE_OK) return r;
+        }
+        char buf[160];
+        snprintf(buf, sizeof buf, "step overflowed (fault bits 0x%x: pairs %u/%u, contacts %u/%u); capacity raised, re-upload",
+                 fault, c->h_counters[CNT_PRAW], c->pcap, c->h_counters[CNT_CRAW], c->ccap);
+        return set_err(c, (fault & NE_FAULT_EPA) ? NE_ERR_INVALID : NE_ERR_CAPACITY, buf);
+    }
+    return NE_OK;
+}
+
+int ne_gpu_get_state(ne_gpu_ctx* c, double* x, double* v, double* q, double* w) {
+    if (!c) return NE_ERR_INVALID;
+    const uint32_t n = c->n;
+    if (n == 0) return NE_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::vector<double> tmp((size_t)n);
+    auto get = [&](double* dst, int comps, int stride, const int* fields) -> int {
+        for (int k = 0; k < comps; ++k) {
+            HIP_TRY(c, hipMemcpy(tmp.data(), c->dfield + (size_t)fields[k] * c->ncap, sizeof(double) * n,
+                                 hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < n; ++i) dst[(size_t)stride * i + k] = tmp[i];
+        }
+        return NE_OK;
+    };
+    int r;
+    const int fx[3] = {F_X0, F_X1, F_X2}, fv[3] = {F_V0, F_V1, F_V2}, fq[4] = {F_Q0, F_Q1, F_Q2, F_Q3},
+              fw[3] = {F_W0, F_W1, F_W2};
+    if (x && (r = get(x, 3, 3, fx)) != NE_OK) return r;
+    if (v && (r = get(v, 3, 3, fv)) != NE_OK) return r;
+    if (q && (r = get(q, 4, 4, fq)) != NE_OK) return r;
+    if (w && (r = get(w, 3, 3, fw)) != NE_OK) return r;
+    return NE_OK;
+}
+
+int ne_gpu_get_body_aux(ne_gpu_ctx* c, uint32_t* flags, double* sleep_timer, double* inv_mass, double* inv_inertia) {
+    if (!c) return NE_ERR_INVALID;
+    const uint32_t n = c->n;
+    if (n == 0) return NE_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (flags) HIP_TRY(c, hipMemcpy(flags, c->ufield + (size_t)U_FLAGS * c->ncap, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (sleep_timer) for (uint32_t i = 0; i < n; ++i) sleep_timer[i] = 0.0;
+    if (inv_mass) HIP_TRY(c, hipMemcpy(inv_mass, c->dfield + (size_t)F_IM * c->ncap, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (inv_inertia) {
+        std::vector<double> tmp(n);
+        const int f[3] = {F_II0, F_II1, F_II2};
+        for (int k = 0; k < 3; ++k) {
+            HIP_TRY(c, hipMemcpy(tmp.data(), c->dfield + (size_t)f[k] * c->ncap, sizeof(double) * n, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < n; ++i) inv_inertia[3 * i + k] = tmp[i];
+        }
+    }
+    return NE_OK;
+}
+
+int ne_gpu_reset_warm_cache(ne_gpu_ctx* c) {
+    if (!c) return NE_ERR_INVALID;
+    HIP_TRY(c, hipMemsetAsync(c->prev_n, 0, sizeof(uint32_t), c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return NE_OK;
+}
+
+int ne_gpu_get_contacts(ne_gpu_ctx* c, uint32_t cap, uint32_t* count, uint32_t* ci, uint32_t* cj, double* normal,
+                        double* point, double* depth, double* normal_impulse) {
+    if (!c || !count) return NE_ERR_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    uint32_t nc = 0;
+    if (c->n) HIP_TRY(c, hipMemcpy(&nc, c->counters + CNT_C, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    *count = nc;
+    const uint32_t m = std::min(nc, cap);
+    if (m == 0) return NE_OK;
+    if (ci) HIP_TRY(c, hipMemcpy(ci, c->C.ci, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+    if (cj) HIP_TRY(c, hipMemcpy(cj, c->C.cj, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+    std::vector<double> tmp(m);
+    auto get3 = [&](double* dst, double* a, double* b, double* d) -> int {
+        double* srcs[3] = {a, b, d};
+        for (int k = 0; k < 3; ++k) {
+            HIP_TRY(c, hipMemcpy(tmp.data(), srcs[k], sizeof(double) * m, hipMemcpyDeviceToHost));
+            for (uint32_t t = 0; t < m; ++t) dst[3 * t + k] = tmp[t];
+        }
+        return NE_OK;
+    };
+    int r;
+    if (normal && (r = get3(normal, c->C.n0, c->C.n1, c->C.n2)) != NE_OK) return r;
+    if (point && (r = get3(point, c->C.p0, c->C.p1, c->C.p2)) != NE_OK) return r;
+    if (depth) HIP_TRY(c, hipMemcpy(depth, c->C.ov, sizeof(double) * m, hipMemcpyDeviceToHost));
+    if (normal_impulse) HIP_TRY(c, hipMemcpy(normal_impulse, c->C.nimp, sizeof(double) * m, hipMemcpyDeviceToHost));
+    return NE_OK;
+}
+
+int ne_gpu_get_pairs(ne_gpu_ctx* c, uint32_t cap, uint32_t* count, uint32_t* pi, uint32_t* pj) {
+    if (!c || !count) return NE_ERR_INVALID;
+    uint32_t nc = 0;
+    int r = ne_gpu_get_contacts(c, 0, &nc, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (r != NE_OK) return r;
+    std::vector<uint32_t> a(nc), b(nc);
+    if (nc) {
+        r = ne_gpu_get_contacts(c, nc, &nc, a.data(), b.data(), nullptr, nullptr, nullptr, nullptr);
+        if (r != NE_OK) return r;
+    }
+    uint32_t m = 0;
+    for (uint32_t k = 0; k < nc; ++k) {
+        if (k > 0 && a[k] == a[k - 1] && b[k] == b[k - 1]) continue;
+        if (m < cap) { if (pi) pi[m] = a[k]; if (pj) pj[m] = b[k]; }
+        ++m;
+    }
+    *count = m;
+    return NE_OK;
+}
+
+// Index-order sums of src/world.cpp:704-729 over the downloaded state.
+int ne_gpu_diagnostics(ne_gpu_ctx* c, double* p, double* L, double* ke) {
+    if (!c) return NE_ERR_INVALID;
+    const uint32_t n = c->n;
+    std::vector<double> x(3 * (size_t)n), v(3 * (size_t)n), q(4 * (size_t)n), w(3 * (size_t)n), im(n), ii(3 * (size_t)n);
+    int r = ne_gpu_get_state(c, x.data(), v.data(), q.data(), w.data());
+    if (r != NE_OK) return r;
+    r = ne_gpu_get_body_aux(c, nullptr, nullptr, im.data(), ii.data());
+    if (r != NE_OK) return r;
+    V3 P{0.0, 0.0, 0.0}, LL{0.0, 0.0, 0.0};
+    double e = 0.0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (im[i] > 0.0) P = P + V3{v[3 * i], v[3 * i + 1], v[3 * i + 2]} * (1.0 / im[i]);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (im[i] <= 0.0) continue;
+        const double m = 1.0 / im[i];
+        const V3 xi{x[3 * i], x[3 * i + 1], x[3 * i + 2]}, vi{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+        const V3 wi{w[3 * i], w[3 * i + 1], w[3 * i + 2]}, iii{ii[3 * i], ii[3 * i + 1], ii[3 * i + 2]};
+        const Qt qi{q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]};
+        LL = LL + cross(xi, vi * m);
+        LL = LL + angular_momentum(qi, iii, wi);
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        if (im[i] <= 0.0) continue;
+        const V3 vi{v[3 * i], v[3 * i + 1], v[3 * i + 2]}, wi{w[3 * i], w[3 * i + 1], w[3 * i + 2]};
+        const V3 iii{ii[3 * i], ii[3 * i + 1], ii[3 * i + 2]};
+        const Qt qi{q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]};
+        double lin = 0.5 * (im[i] > 0 ? 1.0 / im[i] : 0.0) * norm2(vi);
+        V3 Ib = inertia_of(iii);
+        V3 wb = qinvrot(qi, wi);
+        V3 Lb{wb.x * Ib.x, wb.y * Ib.y, wb.z * Ib.z};
+        double rot = 0.5 * dot(wb, Lb);
+        e += lin + rot;
+    }
+    if (p) { p[0] = P.x; p[1] = P.y; p[2] = P.z; }
+    if (L) { L[0] = LL.x; L[1] = LL.y; L[2] = LL.z; }
+    if (ke) *ke = e;
+    return NE_OK;
+}
+
+int ne_gpu_get_counters(ne_gpu_ctx* c, uint32_t out[8]) {
+    if (!c || !out) return NE_ERR_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    uint32_t h[CNT_N];
+    HIP_TRY(c, hipMemcpy(h, c->counters, sizeof(h), hipMemcpyDeviceToHost));
+    out[0] = h[CNT_PRAW];
+    out[1] = 0;
+    out[2] = h[CNT_CRAW];
+    out[3] = h[CNT_ISL];
+    out[4] = h[CNT_MAXISL];
+    out[5] = c->ngrid;
+    out[6] = h[CNT_CELLS];
+    out[7] = h[CNT_FAULT];
+    return NE_OK;
+}
+
+int ne_gpu_get_stream(ne_gpu_ctx* c, void** stream) {
+    if (!c || !stream) return NE_ERR_INVALID;
+    *stream = (void*)c->stream;
+    return NE_OK;
+}
+
+const char* ne_gpu_last_error(ne_gpu_ctx* c) {
+    if (!c) return "null context";
+    return c->err.c_str();
+}
+
+int ne_gpu_fp_probe(ne_gpu_ctx* c, int32_t op, uint32_t n, const double* a, const double* b, double* out) {
+    if (!c || !a || !out || (op == 1 && !b)) return NE_ERR_INVALID;
+    if (n == 0) return NE_OK;
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    HIP_TRY(c, hipMalloc((void**)&da, sizeof(double) * n));
+    HIP_TRY(c, hipMalloc((void**)&db, sizeof(double) * n));
+    HIP_TRY(c, hipMalloc((void**)&dout, sizeof(double) * n));
+    HIP_TRY(c, hipMemcpy(da, a, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(db, b ? b : a, sizeof(double) * n, hipMemcpyHostToDevice));
+    k_fp_probe<<<blocks_for(n), TPB, 0, c->stream>>>(op, n, da, db, dout);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost));
+    (void)hipFree(da); (void)hipFree(db); (void)hipFree(dout);
+    return NE_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Upload: AoS-ish host arrays -> device SoA, plus the static broadphase plan
+// (bounding radii, grid-excluded bodies, cell size).
+// ---------------------------------------------------------------------------
+namespace {
+
+int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
+    const uint32_t n = b->n;
+    if (n > 0 && (!b->x || !b->v || !b->q || !b->w || !b->inv_mass || !b->inv_inertia || !b->shape || !b->radius ||
+                  !b->half_height || !b->half_extents))
+        return set_err(c, NE_ERR_INVALID, "required body array missing");
+    if (n >= (1u << 30)) return set_err(c, NE_ERR_INVALID, "too many bodies");
+    for (uint32_t i = 0; i < n; ++i) {
+        const int s = b->shape[i];
+        if (s < 0 || s > NE_SHAPE_PLANE)
+            return set_err(c, NE_ERR_UNSUPPORTED, "mesh/compound shapes are not on the GPU path yet");
+        const uint32_t fl = b->flags ? b->flags[i] : NE_FLAG_DYNAMIC;
+        if (fl & NE_FLAG_CCD) return set_err(c, NE_ERR_UNSUPPORTED, "CCD is not on the GPU path");
+        if (fl & NE_FLAG_SLEEPING) return set_err(c, NE_ERR_UNSUPPORTED, "sleeping bodies are not on the GPU path yet");
+        if (s == NE_SHAPE_CONVEX && (!b->hull_offsets || b->hull_offsets[i + 1] < b->hull_offsets[i]))
+            return set_err(c, NE_ERR_INVALID, "convex body without hull vertices");
+    }
+    // (re)allocate per-body storage
+    if (n > c->ncap || !c->dfield) {
+        free_work(c);
+        uint32_t ncap = std::max<uint32_t>(std::max<uint32_t>(n, 64u), c->ncap + c->ncap / 2u);
+        c->ncap = ncap;
+        HIP_TRY(c, dalloc(c->dfield, (size_t)F_COUNT * ncap));
+        HIP_TRY(c, dalloc(c->ufield, (size_t)U_COUNT * ncap));
+        HIP_TRY(c, dalloc(c->hull_off, (size_t)ncap + 1));
+        HIP_TRY(c, dalloc(c->nc_off, (size_t)ncap + 1));
+        HIP_TRY(c, dalloc(c->gridded, ncap));
+        HIP_TRY(c, dalloc(c->aabb_part, 6 * AABB_BLOCKS));
+        HIP_TRY(c, dalloc(c->grid, 1));
+        HIP_TRY(c, dalloc(c->cell_of, ncap));
+        HIP_TRY(c, dalloc(c->sorted_idx, ncap));
+        HIP_TRY(c, dalloc(c->sorted, ncap));
+        HIP_TRY(c, dalloc(c->row_count, (size_t)ncap + 1));
+        HIP_TRY(c, dalloc(c->row_start, (size_t)ncap + 1));
+        HIP_TRY(c, dalloc(c->parent, ncap));
+        HIP_TRY(c, dalloc(c->used, (size_t)ncap + 1));
+        HIP_TRY(c, dalloc(c->root_id, (size_t)ncap + 1));
+        HIP_TRY(c, dalloc(c->isl_start, (size_t)ncap + 1));
+        c->cellcap = std::max<uint32_t>(2u * ncap, 4096u);
+        HIP_TRY(c, dalloc(c->cell_count, (size_t)c->cellcap + 1));
+        HIP_TRY(c, dalloc(c->cell_start, (size_t)c->cellcap + 1));
+        uint32_t pcap = c->user_pcap ? c->user_pcap : 8u * ncap + 4096u;
+        uint32_t ccap = c->user_ccap ? c->user_ccap : 4u * ncap + 4096u;
+        int r = alloc_pairs(c, pcap);
+        if (r != NE_OK) return r;
+        r = alloc_cons(c, ccap);
+        if (r != NE_OK) return r;
+        r = ensure_prev(c, ccap);
+        if (r != NE_OK) return r;
+        // scan scratch: partial sums (<=1024) and radix histograms
+        HIP_TRY(c, dalloc(c->scan_part, 1024));
+        HIP_TRY(c, dalloc(c->rs_hist, (size_t)RS_BINS * 1024 + 1));
+    }
+    c->n = n;
+    const uint32_t ncap = c->ncap;
+    // --- doubles
+    std::vector<double> hd((size_t)F_COUNT * ncap, 0.0);
+    auto D = [&](int f, uint32_t i) -> double& { return hd[(size_t)f * ncap + i]; };
+    bool forces = false;
+    std::vector<uint32_t> hoff(n + 1, 0), noff(n + 1, 0);
+    uint32_t nh = 0, nn = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        hoff[i] = nh;
+        if (b->hull_offsets) nh += b->hull_offsets[i + 1] - b->hull_offsets[i];
+        noff[i] = nn;
+        if (b->nocollide_offsets) nn += b->nocollide_offsets[i + 1] - b->nocollide_offsets[i];
+    }
+    hoff[n] = nh;
+    noff[n] = nn;
+    std::vector<double> hhull(3 * (size_t)std::max(nh, 1u), 0.0);
+    std::vector<uint32_t> hnc(std::max(nn, 1u), 0u);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (b->hull_offsets)
+            for (uint32_t k = b->hull_offsets[i], t = hoff[i]; k < b->hull_offsets[i + 1]; ++k, ++t)
+                for (int a = 0; a < 3; ++a) hhull[3 * (size_t)t + a] = b->hull_verts[3 * (size_t)k + a];
+        if (b->nocollide_offsets)
+            for (uint32_t k = b->nocollide_offsets[i], t = noff[i]; k < b->nocollide_offsets[i + 1]; ++k, ++t)
+                hnc[t] = b->nocollide[k];
+    }
+    double maxDyn = 0.0;
+    std::vector<double> R(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            D(F_X0 + a, i) = b->x[3 * i + a];
+            D(F_V0 + a, i) = b->v[3 * i + a];
+            D(F_W0 + a, i) = b->w[3 * i + a];
+            D(F_F0 + a, i) = b->force ? b->force[3 * i + a] : 0.0;
+            D(F_T0 + a, i) = b->torque ? b->torque[3 * i + a] : 0.0;
+            D(F_II0 + a, i) = b->inv_inertia[3 * i + a];
+            D(F_HE0 + a, i) = b->half_extents[3 * i + a];
+            forces = forces || D(F_F0 + a, i) != 0.0 || D(F_T0 + a, i) != 0.0 || std::signbit(D(F_F0 + a, i)) ||
+                     std::signbit(D(F_T0 + a, i));
+        }
+        for (int a = 0; a < 4; ++a) D(F_Q0 + a, i) = b->q[4 * i + a];
+        D(F_IM, i) = b->inv_mass[i];
+        D(F_RAD, i) = b->radius[i];
+        D(F_HH, i) = b->half_height[i];
+        D(F_FRIC, i) = b->friction ? b->friction[i] : -1.0;
+        D(F_REST, i) = b->restitution ? b->restitution[i] : -1.0;
+        D(F_LD, i) = b->linear_damping ? b->linear_damping[i] : 0.0;
+        D(F_AD, i) = b->angular_damping ? b->angular_damping[i] : 0.0;
+        D(F_GS, i) = b->gravity_scale ? b->gravity_scale[i] : 1.0;
+        R[i] = host_bounding_radius(b->shape[i], b->radius[i], b->half_height[i], b->half_extents + 3 * i,
+                                    hhull.data(), hoff[i], hoff[i + 1]);
+        D(F_BR, i) = R[i];
+        if (b->shape[i] != NE_SHAPE_PLANE && b->inv_mass[i] > 0.0) maxDyn = std::max(maxDyn, R[i]);
+    }
+    // grid-excluded: planes, and static bodies much larger than any dynamic one
+    std::vector<uint32_t> gridded, excl;
+    double maxR = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        bool ex = b->shape[i] == NE_SHAPE_PLANE || R[i] >= 1e29 || (maxDyn > 0.0 && R[i] > 2.0 * maxDyn);
+        if (ex) excl.push_back(i);
+        else { gridded.push_back(i); maxR = std::max(maxR, R[i]); }
+    }
+    if ((uint64_t)excl.size() * (uint64_t)n >= (1ull << 31))
+        return set_err(c, NE_ERR_UNSUPPORTED, "too many grid-excluded bodies for this body count");
+    c->ngrid = (uint32_t)gridded.size();
+    c->nexcl = (uint32_t)excl.size();
+    c->cell = (2.0 * maxR * (1.0 + 1e-6) + 1e-9) * c->cell_scale;
+    c->has_forces = forces;
+    c->spheres_only = true;
+    for (uint32_t i = 0; i < n; ++i) c->spheres_only = c->spheres_only && b->shape[i] == NE_SHAPE_SPHERE;
+    // island-key bits
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) <= (uint64_t)n) ++bits;
+    c->uf_bits = bits;
+    std::vector<uint32_t> hu((size_t)U_COUNT * ncap, 0u);
+    for (uint32_t i = 0; i < n; ++i) {
+        hu[(size_t)U_SHAPE * ncap + i] = (uint32_t)b->shape[i];
+        hu[(size_t)U_LAYER * ncap + i] = b->layer ? b->layer[i] : 0u;
+        hu[(size_t)U_MASK * ncap + i] = b->mask ? b->mask[i] : 0xFFFFFFFFu;
+        hu[(size_t)U_FLAGS * ncap + i] = b->flags ? b->flags[i] : NE_FLAG_DYNAMIC;
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(c->dfield, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->ufield, hu.data(), sizeof(uint32_t) * hu.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, dalloc(c->hull, 3 * (size_t)std::max(nh, 1u)));
+    HIP_TRY(c, dalloc(c->nc_idx, std::max(nn, 1u)));
+    HIP_TRY(c, hipMemcpy(c->hull, hhull.data(), sizeof(double) * hhull.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->nc_idx, hnc.data(), sizeof(uint32_t) * hnc.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->hull_off, hoff.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->nc_off, noff.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
+    if (!gridded.empty())
+        HIP_TRY(c, hipMemcpy(c->gridded, gridded.data(), sizeof(uint32_t) * gridded.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, dalloc(c->excl, std::max<size_t>(excl.size(), 1)));
+    if (!excl.empty()) {
+        HIP_TRY(c, hipMemcpy(c->excl, excl.data(), sizeof(uint32_t) * excl.size(), hipMemcpyHostToDevice));
+        HIP_TRY(c, dalloc(c->ex_flags, (size_t)excl.size() * n));
+        HIP_TRY(c, dalloc(c->ex_off, (size_t)excl.size() * n + 1));
+    }
+    // device view
+    BodiesDev& B = c->B;
+    double* d = c->dfield;
+    auto F = [&](int f) { return d + (size_t)f * ncap; };
+    B.x0 = F(F_X0); B.x1 = F(F_X1); B.x2 = F(F_X2);
+    B.v0 = F(F_V0); B.v1 = F(F_V1); B.v2 = F(F_V2);
+    B.q0 = F(F_Q0); B.q1 = F(F_Q1); B.q2 = F(F_Q2); B.q3 = F(F_Q3);
+    B.w0 = F(F_W0); B.w1 = F(F_W1); B.w2 = F(F_W2);
+    B.f0 = F(F_F0); B.f1 = F(F_F1); B.f2 = F(F_F2);
+    B.t0 = F(F_T0); B.t1 = F(F_T1); B.t2 = F(F_T2);
+    B.im = F(F_IM); B.ii0 = F(F_II0); B.ii1 = F(F_II1); B.ii2 = F(F_II2);
+    B.rad = F(F_RAD); B.hh = F(F_HH); B.he0 = F(F_HE0); B.he1 = F(F_HE1); B.he2 = F(F_HE2); B.bR = F(F_BR);
+    B.fric = F(F_FRIC); B.rest = F(F_REST); B.ldamp = F(F_LD); B.adamp = F(F_AD); B.gs = F(F_GS);
+    B.shape = (int32_t*)(c->ufield + (size_t)U_SHAPE * ncap);
+    B.layer = c->ufield + (size_t)U_LAYER * ncap;
+    B.mask = c->ufield + (size_t)U_MASK * ncap;
+    B.flags = c->ufield + (size_t)U_FLAGS * ncap;
+    B.hull_off = c->hull_off; B.hull = c->hull; B.nc_off = c->nc_off; B.nc_idx = c->nc_idx;
+    B.n = n;
+    return NE_OK;
+}
+
+}  // namespace

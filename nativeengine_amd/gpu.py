@@ -1,0 +1,184 @@
+"""Python binding of the HIP step (include/ne_gpu.h) over ctypes.
+
+`GpuWorld` is a resident world: upload once, step many times, download when
+needed.  There is no CPU fallback: if the HIP library is missing or no device
+is visible, constructing a GpuWorld raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from .scene import Params, Scene
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnegpu.so")
+
+_lib = None
+
+# every entry point declared in include/ne_gpu.h
+EXPORTS = [
+    "ne_gpu_create", "ne_gpu_destroy", "ne_gpu_set_params", "ne_gpu_set_bodies", "ne_gpu_set_state",
+    "ne_gpu_step", "ne_gpu_get_state", "ne_gpu_get_body_aux", "ne_gpu_reset_warm_cache", "ne_gpu_get_contacts",
+    "ne_gpu_get_pairs", "ne_gpu_diagnostics", "ne_gpu_get_counters", "ne_gpu_get_stream", "ne_gpu_last_error",
+    "ne_gpu_fp_probe",
+]
+
+
+class NeGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ne_gpu error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libnegpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    u32p = C.POINTER(C.c_uint32)
+    dp = C.POINTER(C.c_double)
+    lib.ne_gpu_create.argtypes = [C.POINTER(_abi.GpuConfig), C.POINTER(vp)]
+    lib.ne_gpu_destroy.argtypes = [vp]
+    lib.ne_gpu_set_params.argtypes = [vp, C.POINTER(_abi.WorldParams)]
+    lib.ne_gpu_set_bodies.argtypes = [vp, C.POINTER(_abi.Bodies)]
+    lib.ne_gpu_set_state.argtypes = [vp, dp, dp, dp, dp]
+    lib.ne_gpu_step.argtypes = [vp, C.c_int32]
+    lib.ne_gpu_get_state.argtypes = [vp, dp, dp, dp, dp]
+    lib.ne_gpu_get_body_aux.argtypes = [vp, u32p, dp, dp, dp]
+    lib.ne_gpu_reset_warm_cache.argtypes = [vp]
+    lib.ne_gpu_get_contacts.argtypes = [vp, C.c_uint32, u32p, u32p, u32p, dp, dp, dp, dp]
+    lib.ne_gpu_get_pairs.argtypes = [vp, C.c_uint32, u32p, u32p, u32p]
+    lib.ne_gpu_diagnostics.argtypes = [vp, dp, dp, dp]
+    lib.ne_gpu_get_counters.argtypes = [vp, u32p]
+    lib.ne_gpu_get_stream.argtypes = [vp, C.POINTER(vp)]
+    lib.ne_gpu_last_error.argtypes = [vp]
+    lib.ne_gpu_last_error.restype = C.c_char_p
+    lib.ne_gpu_fp_probe.argtypes = [vp, C.c_int32, C.c_uint32, dp, dp, dp]
+    for name in EXPORTS:
+        if name != "ne_gpu_last_error":
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+class GpuWorld:
+    """A world resident on one MI355X."""
+
+    def __init__(self, device: int = 0, pair_capacity: int = 0, contact_capacity: int = 0, cell_scale: float = 0.0):
+        self.lib = load_library()
+        cfg = _abi.GpuConfig(device, pair_capacity, contact_capacity, cell_scale)
+        h = C.c_void_p()
+        rc = self.lib.ne_gpu_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise NeGpuError(rc, "ne_gpu_create failed (no HIP device?)")
+        self.h = h
+        self.n = 0
+        self.params = Params()
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.ne_gpu_last_error(self.h).decode(errors="replace")
+            raise NeGpuError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ne_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- upload -----------------------------------------------------------
+    def set_params(self, p: Params):
+        self.params = p
+        cp = p.to_c()
+        self._check(self.lib.ne_gpu_set_params(self.h, C.byref(cp)))
+
+    def set_scene(self, s: Scene):
+        self.set_params(s.params)
+        b, keep = s.to_c()
+        self._check(self.lib.ne_gpu_set_bodies(self.h, C.byref(b)))
+        self.n = s.n
+        del keep
+
+    def set_state(self, x=None, v=None, q=None, w=None):
+        f = lambda a: None if a is None else _abi.dptr(np.ascontiguousarray(a, np.float64))
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (x, v, q, w)]
+        self._check(self.lib.ne_gpu_set_state(self.h, *[None if a is None else _abi.dptr(a) for a in arrs]))
+
+    # -- step -------------------------------------------------------------
+    def step(self, nsteps: int = 1):
+        self._check(self.lib.ne_gpu_step(self.h, int(nsteps)))
+
+    def reset_warm_cache(self):
+        self._check(self.lib.ne_gpu_reset_warm_cache(self.h))
+
+    # -- download ---------------------------------------------------------
+    def state(self):
+        n = self.n
+        x, v, w = (np.zeros((n, 3)) for _ in range(3))
+        q = np.zeros((n, 4))
+        self._check(self.lib.ne_gpu_get_state(self.h, _abi.dptr(x), _abi.dptr(v), _abi.dptr(q), _abi.dptr(w)))
+        return x, v, q, w
+
+    def contacts(self):
+        """Last step's constraint list: dict of i, j, normal, point, depth, impulse."""
+        cnt = C.c_uint32(0)
+        self._check(self.lib.ne_gpu_get_contacts(self.h, 0, C.byref(cnt), None, None, None, None, None, None))
+        m = cnt.value
+        i = np.zeros(m, np.uint32)
+        j = np.zeros(m, np.uint32)
+        nrm = np.zeros((m, 3))
+        pt = np.zeros((m, 3))
+        dep = np.zeros(m)
+        imp = np.zeros(m)
+        if m:
+            self._check(self.lib.ne_gpu_get_contacts(self.h, m, C.byref(cnt), _abi.uptr(i), _abi.uptr(j), _abi.dptr(nrm),
+                                                     _abi.dptr(pt), _abi.dptr(dep), _abi.dptr(imp)))
+        return dict(i=i, j=j, normal=nrm, point=pt, depth=dep, impulse=imp)
+
+    def pairs(self):
+        cnt = C.c_uint32(0)
+        self._check(self.lib.ne_gpu_get_pairs(self.h, 0, C.byref(cnt), None, None))
+        m = cnt.value
+        i = np.zeros(m, np.uint32)
+        j = np.zeros(m, np.uint32)
+        if m:
+            self._check(self.lib.ne_gpu_get_pairs(self.h, m, C.byref(cnt), _abi.uptr(i), _abi.uptr(j)))
+        return i, j
+
+    def diagnostics(self):
+        p = np.zeros(3)
+        L = np.zeros(3)
+        ke = C.c_double(0.0)
+        self._check(self.lib.ne_gpu_diagnostics(self.h, _abi.dptr(p), _abi.dptr(L), C.byref(ke)))
+        return p, L, ke.value
+
+    def counters(self):
+        out = np.zeros(8, np.uint32)
+        self._check(self.lib.ne_gpu_get_counters(self.h, _abi.uptr(out)))
+        keys = ["candidate_pairs", "hit_pairs", "constraints", "islands", "max_island", "gridded", "cells", "fault"]
+        return dict(zip(keys, map(int, out)))
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        self._check(self.lib.ne_gpu_get_stream(self.h, C.byref(s)))
+        return s.value or 0
+
+    def fp_probe(self, op: int, a, b=None):
+        a = np.ascontiguousarray(a, np.float64)
+        bb = None if b is None else np.ascontiguousarray(b, np.float64)
+        out = np.zeros_like(a)
+        self._check(self.lib.ne_gpu_fp_probe(self.h, int(op), len(a), _abi.dptr(a), _abi.dptr(bb), _abi.dptr(out)))
+        return out

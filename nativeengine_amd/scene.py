@@ -1,0 +1,405 @@
+"""Scenes: bodies as structure-of-arrays numpy buffers plus world parameters.
+
+Mirrors the reference's `ne::World` public state (src/world.hpp:75-136) and
+the `ne::Body` fields the step reads (src/body.hpp:41-129).  Factories restate
+the reference's mass/inertia formulas (src/body.hpp:206-315) in the same
+operation order, so a scene built here has the same bits as one built with the
+reference's factories.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+
+PI = math.pi  # == ne::PI (src/vmath.hpp:16) rounded to double
+
+
+@dataclass
+class Params:
+    """`ne::World` scalar fields (src/world.hpp:75-126), with the two extensions
+    (per-axis half extents, solver_iterations)."""
+    half: tuple = (12.0, 12.0, 12.0)
+    periodic: bool = False
+    open_boundary: bool = False
+    dt: float = 1.0 / 120.0
+    gravity: tuple = (0.0, 0.0, 0.0)
+    restitution: float = 1.0
+    friction: float = 0.0
+    contact_beta: float = 0.2
+    restitution_combine: int = _abi.COMBINE_MAX
+    friction_combine: int = _abi.COMBINE_GEOMETRIC_MEAN
+    warm_start: bool = True
+    force_brute_force: bool = False
+    solver_iterations: int = 8
+    sleep_enabled: bool = False
+    sleep_lin_vel: float = 0.05
+    sleep_ang_vel: float = 0.05
+    sleep_time: float = 0.5
+    wake_vel: float = 0.5
+
+    def to_c(self) -> _abi.WorldParams:
+        p = _abi.WorldParams()
+        h = self.half
+        if np.isscalar(h):
+            h = (h, h, h)
+        for a in range(3):
+            p.half[a] = float(h[a])
+            p.gravity[a] = float(self.gravity[a])
+        p.periodic = int(bool(self.periodic))
+        p.open_boundary = int(bool(self.open_boundary))
+        p.dt = float(self.dt)
+        p.restitution = float(self.restitution)
+        p.friction = float(self.friction)
+        p.contact_beta = float(self.contact_beta)
+        p.restitution_combine = int(self.restitution_combine)
+        p.friction_combine = int(self.friction_combine)
+        p.warm_start = int(bool(self.warm_start))
+        p.force_brute_force = int(bool(self.force_brute_force))
+        p.solver_iterations = int(self.solver_iterations)
+        p.sleep_enabled = int(bool(self.sleep_enabled))
+        p.sleep_lin_vel = float(self.sleep_lin_vel)
+        p.sleep_ang_vel = float(self.sleep_ang_vel)
+        p.sleep_time = float(self.sleep_time)
+        p.wake_vel = float(self.wake_vel)
+        return p
+
+
+class Scene:
+    """Bodies in SoA form (x: n x 3, q: n x 4 as (w,x,y,z), ...)."""
+
+    def __init__(self, n: int = 0):
+        self.n = n
+        z3 = lambda: np.zeros((n, 3), np.float64)
+        self.x, self.v, self.w = z3(), z3(), z3()
+        self.q = np.zeros((n, 4), np.float64)
+        self.q[:, 0] = 1.0
+        self.force, self.torque = z3(), z3()
+        self.inv_mass = np.zeros(n, np.float64)
+        self.inv_inertia = z3()
+        self.shape = np.zeros(n, np.int32)
+        self.radius = np.ones(n, np.float64)
+        self.half_height = np.zeros(n, np.float64)
+        self.half_extents = z3()
+        self.friction = np.full(n, -1.0)
+        self.restitution = np.full(n, -1.0)
+        self.linear_damping = np.zeros(n)
+        self.angular_damping = np.zeros(n)
+        self.gravity_scale = np.ones(n)
+        self.layer = np.zeros(n, np.uint32)
+        self.mask = np.full(n, 0xFFFFFFFF, np.uint32)
+        self.flags = np.full(n, _abi.FLAG_DYNAMIC, np.uint32)
+        self.hull_offsets = np.zeros(n + 1, np.uint32)
+        self.hull_verts = np.zeros((0, 3), np.float64)
+        self.nocollide_offsets = np.zeros(n + 1, np.uint32)
+        self.nocollide = np.zeros(0, np.uint32)
+        self.params = Params()
+
+    # -- construction -----------------------------------------------------
+    @staticmethod
+    def from_bodies(bodies, params: Params | None = None) -> "Scene":
+        s = Scene(len(bodies))
+        hulls, ncs = [], []
+        for i, b in enumerate(bodies):
+            s.x[i], s.v[i], s.q[i], s.w[i] = b.x, b.v, b.q, b.w
+            s.force[i], s.torque[i] = b.force, b.torque
+            s.inv_mass[i] = b.inv_mass
+            s.inv_inertia[i] = b.inv_inertia
+            s.shape[i] = b.shape
+            s.radius[i] = b.radius
+            s.half_height[i] = b.half_height
+            s.half_extents[i] = b.half_extents
+            s.friction[i], s.restitution[i] = b.friction, b.restitution
+            s.linear_damping[i], s.angular_damping[i] = b.linear_damping, b.angular_damping
+            s.gravity_scale[i] = b.gravity_scale
+            s.layer[i], s.mask[i] = b.layer, b.mask
+            s.flags[i] = b.flags
+            hulls.append(np.asarray(b.vertices, np.float64).reshape(-1, 3))
+            ncs.append(np.asarray(b.no_collide, np.uint32).reshape(-1))
+        s.hull_offsets = np.concatenate([[0], np.cumsum([len(h) for h in hulls])]).astype(np.uint32)
+        s.hull_verts = np.concatenate(hulls, axis=0) if hulls else np.zeros((0, 3))
+        s.nocollide_offsets = np.concatenate([[0], np.cumsum([len(c) for c in ncs])]).astype(np.uint32)
+        s.nocollide = np.concatenate(ncs) if ncs else np.zeros(0, np.uint32)
+        if params is not None:
+            s.params = params
+        return s
+
+    def copy(self) -> "Scene":
+        s = Scene(0)
+        for k, v in self.__dict__.items():
+            setattr(s, k, v.copy() if isinstance(v, np.ndarray) else v)
+        s.params = Params(**self.params.__dict__)
+        return s
+
+    # -- ctypes view ------------------------------------------------------
+    def to_c(self):
+        """Returns (ne_bodies struct, keepalive list)."""
+        keep = []
+
+        def f64(a):
+            a = np.ascontiguousarray(a, np.float64)
+            keep.append(a)
+            return _abi.dptr(a)
+
+        def u32(a):
+            a = np.ascontiguousarray(a, np.uint32)
+            keep.append(a)
+            return _abi.uptr(a)
+
+        b = _abi.Bodies()
+        b.n = self.n
+        b.x, b.v, b.q, b.w = f64(self.x), f64(self.v), f64(self.q), f64(self.w)
+        b.force, b.torque = f64(self.force), f64(self.torque)
+        b.inv_mass = f64(self.inv_mass)
+        b.inv_inertia = f64(self.inv_inertia)
+        sh = np.ascontiguousarray(self.shape, np.int32)
+        keep.append(sh)
+        b.shape = _abi.iptr(sh)
+        b.radius, b.half_height = f64(self.radius), f64(self.half_height)
+        b.half_extents = f64(self.half_extents)
+        b.friction, b.restitution = f64(self.friction), f64(self.restitution)
+        b.linear_damping, b.angular_damping = f64(self.linear_damping), f64(self.angular_damping)
+        b.gravity_scale = f64(self.gravity_scale)
+        b.layer, b.mask, b.flags = u32(self.layer), u32(self.mask), u32(self.flags)
+        b.sleep_timer = None
+        b.inv_mass_store = None
+        b.inv_inertia_store = None
+        b.hull_offsets = u32(self.hull_offsets)
+        hv = np.ascontiguousarray(self.hull_verts.reshape(-1, 3) if len(self.hull_verts) else np.zeros((1, 3)), np.float64)
+        keep.append(hv)
+        b.hull_verts = _abi.dptr(hv)
+        b.nocollide_offsets = u32(self.nocollide_offsets)
+        nc = np.ascontiguousarray(self.nocollide if len(self.nocollide) else np.zeros(1, np.uint32), np.uint32)
+        keep.append(nc)
+        b.nocollide = _abi.uptr(nc)
+        return b, keep
+
+
+# ---------------------------------------------------------------------------
+# Body + factories (src/body.hpp:41-315)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Body:
+    x: tuple = (0.0, 0.0, 0.0)
+    v: tuple = (0.0, 0.0, 0.0)
+    q: tuple = (1.0, 0.0, 0.0, 0.0)
+    w: tuple = (0.0, 0.0, 0.0)
+    force: tuple = (0.0, 0.0, 0.0)
+    torque: tuple = (0.0, 0.0, 0.0)
+    inv_mass: float = 0.0
+    inv_inertia: tuple = (0.0, 0.0, 0.0)
+    shape: int = _abi.SHAPE_SPHERE
+    radius: float = 1.0
+    half_height: float = 0.0
+    half_extents: tuple = (0.0, 0.0, 0.0)
+    vertices: list = field(default_factory=list)
+    friction: float = -1.0
+    restitution: float = -1.0
+    linear_damping: float = 0.0
+    angular_damping: float = 0.0
+    gravity_scale: float = 1.0
+    layer: int = 0
+    mask: int = 0xFFFFFFFF
+    no_collide: list = field(default_factory=list)
+    flags: int = _abi.FLAG_DYNAMIC
+    id: int = -1
+
+    def make_static(self):
+        """PhysicsWorld::makeStatic (src/physics_world.hpp:112-121)."""
+        self.inv_mass = 0.0
+        self.inv_inertia = (0.0, 0.0, 0.0)
+        self.v = (0.0, 0.0, 0.0)
+        self.w = (0.0, 0.0, 0.0)
+        self.flags &= ~(_abi.FLAG_DYNAMIC | _abi.FLAG_KINEMATIC)
+        return self
+
+
+def make_sphere(id, x, radius, density) -> Body:            # body.hpp:206-219
+    b = Body(shape=_abi.SHAPE_SPHERE, id=id, x=tuple(map(float, x)), radius=float(radius), half_height=0.0)
+    m = density * (4.0 / 3.0) * PI * radius * radius * radius
+    b.inv_mass = 1.0 / m if m > 0 else 0.0
+    I = (2.0 / 5.0) * m * radius * radius
+    invI = 1.0 / I if I > 0 else 0.0
+    b.inv_inertia = (invI, invI, invI)
+    return b
+
+
+def make_cylinder(id, x, q, radius, height, density) -> Body:    # body.hpp:222-239
+    b = Body(shape=_abi.SHAPE_CYLINDER, id=id, x=tuple(map(float, x)), q=tuple(map(float, q)),
+             radius=float(radius), half_height=0.5 * height)
+    m = density * PI * radius * radius * height
+    b.inv_mass = 1.0 / m if m > 0 else 0.0
+    Iyy = 0.5 * m * radius * radius
+    Ixx = (1.0 / 12.0) * m * (3.0 * radius * radius + height * height)
+    b.inv_inertia = (1.0 / Ixx if Ixx > 0 else 0.0, 1.0 / Iyy if Iyy > 0 else 0.0, 1.0 / Ixx if Ixx > 0 else 0.0)
+    return b
+
+
+def make_box(id, x, q, half_extents, density) -> Body:          # body.hpp:243-261
+    hx, hy, hz = map(float, half_extents)
+    b = Body(shape=_abi.SHAPE_BOX, id=id, x=tuple(map(float, x)), q=tuple(map(float, q)), half_extents=(hx, hy, hz))
+    m = density * 8.0 * hx * hy * hz
+    b.inv_mass = 1.0 / m if m > 0 else 0.0
+    Ix = (1.0 / 3.0) * m * (hy * hy + hz * hz)
+    Iy = (1.0 / 3.0) * m * (hx * hx + hz * hz)
+    Iz = (1.0 / 3.0) * m * (hx * hx + hy * hy)
+    b.inv_inertia = (1.0 / Ix if Ix > 0 else 0.0, 1.0 / Iy if Iy > 0 else 0.0, 1.0 / Iz if Iz > 0 else 0.0)
+    return b
+
+
+def make_capsule(id, x, q, radius, half_height, density) -> Body:   # body.hpp:266-288
+    b = Body(shape=_abi.SHAPE_CAPSULE, id=id, x=tuple(map(float, x)), q=tuple(map(float, q)),
+             radius=float(radius), half_height=float(half_height))
+    r, L = radius, 2.0 * half_height
+    mc = density * PI * r * r * L
+    mh = density * (4.0 / 3.0) * PI * r * r * r
+    m = mc + mh
+    b.inv_mass = 1.0 / m if m > 0 else 0.0
+    Iyy = 0.5 * mc * r * r + 0.4 * mh * r * r
+    Ixx = mc * (0.25 * r * r + L * L / 12.0) + mh * (0.4 * r * r + 0.25 * L * L)
+    b.inv_inertia = (1.0 / Ixx if Ixx > 0 else 0.0, 1.0 / Iyy if Iyy > 0 else 0.0, 1.0 / Ixx if Ixx > 0 else 0.0)
+    return b
+
+
+def make_convex(id, x, q, verts, density) -> Body:              # body.hpp:293-315
+    verts = [tuple(map(float, v)) for v in verts]
+    b = Body(shape=_abi.SHAPE_CONVEX, id=id, x=tuple(map(float, x)), q=tuple(map(float, q)), vertices=verts)
+    lo = [1e30, 1e30, 1e30]
+    hi = [-1e30, -1e30, -1e30]
+    for v in verts:
+        lo = [min(lo[a], v[a]) for a in range(3)]
+        hi = [max(hi[a], v[a]) for a in range(3)]
+    he = [(hi[a] - lo[a]) * 0.5 for a in range(3)]
+    hx, hy, hz = max(he[0], 1e-4), max(he[1], 1e-4), max(he[2], 1e-4)
+    m = density * 8.0 * hx * hy * hz
+    b.inv_mass = 1.0 / m if m > 0 else 0.0
+    Ix = (1.0 / 3.0) * m * (hy * hy + hz * hz)
+    Iy = (1.0 / 3.0) * m * (hx * hx + hz * hz)
+    Iz = (1.0 / 3.0) * m * (hx * hx + hy * hy)
+    b.inv_inertia = (1.0 / Ix if Ix > 0 else 0.0, 1.0 / Iy if Iy > 0 else 0.0, 1.0 / Iz if Iz > 0 else 0.0)
+    return b
+
+
+def make_plane(id, point, normal) -> Body:                      # body.hpp:335-345
+    n = np.asarray(normal, np.float64)
+    nn = math.sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2])
+    he = (n[0] / nn, n[1] / nn, n[2] / nn) if nn > 1e-300 else (0.0, 0.0, 0.0)
+    b = Body(shape=_abi.SHAPE_PLANE, id=id, x=tuple(map(float, point)), half_extents=tuple(map(float, he)))
+    b.inv_mass = 0.0
+    b.inv_inertia = (0.0, 0.0, 0.0)
+    b.flags = 0
+    return b
+
+
+# ---------------------------------------------------------------------------
+# Synthetic workloads
+# ---------------------------------------------------------------------------
+
+def _reject_overlaps(pos, radius, L, rng, periodic=True, max_rounds=200):
+    """Resample later bodies of overlapping pairs until no two centres are
+    closer than 2*radius (minimum image when periodic)."""
+    from scipy.spatial import cKDTree
+    n = len(pos)
+    for _ in range(max_rounds):
+        if periodic:
+            tree = cKDTree(np.mod(pos + L / 2, L), boxsize=L)
+        else:
+            tree = cKDTree(pos)
+        pairs = tree.query_pairs(2.0 * radius, output_type="ndarray")
+        if len(pairs) == 0:
+            return pos
+        bad = np.unique(pairs.max(axis=1))
+        pos[bad] = rng.uniform(-L / 2, L / 2, size=(len(bad), 3))
+    raise RuntimeError("could not place non-overlapping bodies")
+
+
+def sphere_gas(n: int, L: float | None = None, radius: float = 0.5, seed: int = 0, mass: float = 1.0,
+               sigma: float = 1.0) -> Scene:
+    """Periodic elastic sphere gas: the plain-method workload.
+
+    n spheres of `radius` and `mass` uniform in a periodic cube of side L
+    (default: the density of 4096 spheres at L=40), rejection-sampled to no
+    overlap; velocities N(0, sigma) per axis with the net momentum removed; no
+    gravity, restitution 1, friction 0, dt 1/60, 8 iterations.
+    """
+    if L is None:
+        L = 40.0 * (n / 4096.0) ** (1.0 / 3.0)
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-L / 2, L / 2, size=(n, 3))
+    pos = _reject_overlaps(pos, radius, L, rng)
+    vel = rng.normal(0.0, sigma, size=(n, 3))
+    vel -= vel.mean(axis=0)
+    s = Scene(n)
+    s.x[:] = pos
+    s.v[:] = vel
+    density = mass / ((4.0 / 3.0) * PI * radius * radius * radius)
+    proto = make_sphere(0, (0, 0, 0), radius, density)
+    s.inv_mass[:] = proto.inv_mass
+    s.inv_inertia[:] = proto.inv_inertia
+    s.shape[:] = _abi.SHAPE_SPHERE
+    s.radius[:] = radius
+    s.params = Params(half=(L / 2, L / 2, L / 2), periodic=True, dt=1.0 / 60.0, restitution=1.0, friction=0.0,
+                      gravity=(0.0, 0.0, 0.0), solver_iterations=8)
+    return s
+
+
+def random_unit_quaternions(rng, n):
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return q
+
+
+def hull12(rng, r):
+    """12 points on a jittered sphere of radius r (a random convex hull)."""
+    d = rng.normal(size=(12, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return d * r * rng.uniform(0.85, 1.0, size=(12, 1))
+
+
+def mixed_pile(n: int, half=(8.0, 8.0, 8.0), seed: int = 0, ground: bool = True, spacing: float = 1.3,
+               periodic: bool = False, iterations: int = 8) -> Scene:
+    """Mixed-shape pile under gravity on a static ground box: 40% boxes
+    (half extents U[0.2,0.6]^3 scaled to the lattice), 30% spheres, 20% capsules
+    (r 0.2, hh 0.4), 10% 12-vertex hulls; jittered lattice, random
+    orientations; mu 0.5, e 0.2, g -9.81, dt 1/60."""
+    rng = np.random.default_rng(seed)
+    bodies = []
+    hx, hy, hz = half
+    m = int(math.ceil(n ** (1.0 / 3.0)))
+    kinds = rng.permutation(np.array([0] * (4 * n // 10) + [1] * (3 * n // 10) + [2] * (2 * n // 10) +
+                                     [3] * (n - 4 * n // 10 - 3 * n // 10 - 2 * n // 10)))
+    qs = random_unit_quaternions(rng, n)
+    floor_y = -hy + 1.0
+    idx = 0
+    for a in range(m):
+        for b in range(m):
+            for c in range(m):
+                if idx >= n:
+                    break
+                x = (-0.5 * (m - 1) + a) * spacing + rng.uniform(-0.1, 0.1)
+                z = (-0.5 * (m - 1) + c) * spacing + rng.uniform(-0.1, 0.1)
+                y = floor_y + 1.0 + b * spacing + rng.uniform(-0.1, 0.1)
+                q = tuple(qs[idx])
+                k = kinds[idx]
+                if k == 0:
+                    he = tuple(rng.uniform(0.2, 0.6, size=3) * (spacing / 2.0) / 0.6 * 0.55)
+                    bodies.append(make_box(idx, (x, y, z), q, he, 1.0))
+                elif k == 1:
+                    bodies.append(make_sphere(idx, (x, y, z), rng.uniform(0.2, 0.6) * (spacing / 2.0) / 0.6 * 0.9, 1.0))
+                elif k == 2:
+                    bodies.append(make_capsule(idx, (x, y, z), q, 0.2, 0.4 * spacing / 1.3 * 0.9, 1.0))
+                else:
+                    bodies.append(make_convex(idx, (x, y, z), q, hull12(rng, rng.uniform(0.2, 0.6) * spacing / 1.3), 1.0))
+                idx += 1
+    if ground:
+        g = make_box(idx, (0.0, -hy + 0.5, 0.0), (1.0, 0.0, 0.0, 0.0), (hx - 0.5, 0.5, hz - 0.5), 1.0).make_static()
+        bodies.append(g)
+    s = Scene.from_bodies(bodies)
+    s.params = Params(half=tuple(half), periodic=periodic, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0),
+                      restitution=0.2, friction=0.5, warm_start=True, solver_iterations=iterations)
+    return s

@@ -1,0 +1,268 @@
+// ref_capi.cpp -- test-infrastructure harness around the UNMODIFIED reference.
+//
+// ORACLE / TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile into
+// oracle/_ref/libne_ref.so from the reference's own sources (src/world.cpp plus
+// its headers) where they lie; nothing from the reference is copied.  Only the
+// tests, the smoke check and bench.py's cpu_baseline leg load it.
+//
+// It drives the reference through its public API only:
+//   - `ne::World` public fields and `step()`            (src/world.hpp:73-161)
+//   - `captureContacts` / `debugContacts` for the per-step constraint list
+//                                                       (src/world.hpp:135-136,
+//                                                        src/world.cpp:552)
+//   - the factories' Body fields                        (src/body.hpp:41-129)
+//   - `detectContact` / `boxBoxManifold`                (src/detect.hpp:33,
+//                                                        src/manifold.hpp:90)
+//   - `totalLinearMomentum/AngularMomentum/Kinetic`     (src/world.cpp:704-729)
+// and exposes them through the same flat-array layout as include/ne_gpu.h so a
+// test can feed one scene to the GPU, the C restatement and the reference.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "world.hpp"
+#include "detect.hpp"
+#include "manifold.hpp"
+
+#include "../include/ne_gpu.h"
+
+namespace {
+
+struct RefWorld {
+    ne::World w;
+};
+
+ne::V3 v3(const double* p, uint32_t i) { return ne::V3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+
+int build_bodies(const ne_bodies* b, std::vector<ne::Body>& out) {
+    out.clear();
+    out.resize(b->n);
+    for (uint32_t i = 0; i < b->n; ++i) {
+        ne::Body& B = out[i];
+        B.x = v3(b->x, i);
+        B.v = v3(b->v, i);
+        B.q = ne::Q{b->q[4 * i], b->q[4 * i + 1], b->q[4 * i + 2], b->q[4 * i + 3]};
+        B.w = v3(b->w, i);
+        if (b->force) B.force = v3(b->force, i);
+        if (b->torque) B.torque = v3(b->torque, i);
+        B.invMass = b->inv_mass[i];
+        B.invInertiaBody = v3(b->inv_inertia, i);
+        int s = b->shape[i];
+        if (s < 0 || s > NE_SHAPE_PLANE) return NE_ERR_UNSUPPORTED;
+        B.shape = static_cast<ne::Shape>(s);
+        B.radius = b->radius[i];
+        B.halfHeight = b->half_height[i];
+        B.halfExtents = v3(b->half_extents, i);
+        if (b->hull_offsets) {
+            for (uint32_t k = b->hull_offsets[i]; k < b->hull_offsets[i + 1]; ++k)
+                B.vertices.push_back(v3(b->hull_verts, k));
+        }
+        if (b->friction) B.friction = b->friction[i];
+        if (b->restitution) B.restitution = b->restitution[i];
+        if (b->linear_damping) B.linearDamping = b->linear_damping[i];
+        if (b->angular_damping) B.angularDamping = b->angular_damping[i];
+        if (b->gravity_scale) B.gravityScale = b->gravity_scale[i];
+        if (b->layer) B.layer = b->layer[i];
+        if (b->mask) B.mask = b->mask[i];
+        uint32_t fl = b->flags ? b->flags[i] : NE_FLAG_DYNAMIC;
+        B.kinematic = (fl & NE_FLAG_KINEMATIC) != 0;
+        B.sensor = (fl & NE_FLAG_SENSOR) != 0;
+        B.dynamic = (fl & NE_FLAG_DYNAMIC) != 0;
+        B.sleeping = (fl & NE_FLAG_SLEEPING) != 0;
+        B.ccd = (fl & NE_FLAG_CCD) != 0;
+        if (b->sleep_timer) B.sleepTimer = b->sleep_timer[i];
+        if (b->inv_mass_store) B.invMassStore = b->inv_mass_store[i];
+        if (b->inv_inertia_store) B.invInertiaStore = v3(b->inv_inertia_store, i);
+        if (b->nocollide_offsets) {
+            for (uint32_t k = b->nocollide_offsets[i]; k < b->nocollide_offsets[i + 1]; ++k)
+                B.noCollide.push_back(b->nocollide[k]);
+        }
+        B.id = (int)i;
+    }
+    return NE_OK;
+}
+
+ne::World::Combine combine_of(int32_t m) { return static_cast<ne::World::Combine>(m); }
+
+}  // namespace
+
+extern "C" {
+
+int ne_ref_create(void** out) {
+    if (!out) return NE_ERR_INVALID;
+    *out = new RefWorld();
+    return NE_OK;
+}
+
+int ne_ref_destroy(void* h) {
+    delete static_cast<RefWorld*>(h);
+    return NE_OK;
+}
+
+// The reference can only express a cubic box and 8 solver iterations
+// (src/box.hpp:16-18, src/world.cpp:572); anything else is refused.
+int ne_ref_set_params(void* h, const ne_world_params* p) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || !p) return NE_ERR_INVALID;
+    if (p->half[0] != p->half[1] || p->half[0] != p->half[2]) return NE_ERR_UNSUPPORTED;
+    if (p->solver_iterations != 8) return NE_ERR_UNSUPPORTED;
+    ne::World& w = r->w;
+    w.box.half = p->half[0];
+    w.box.periodic = p->periodic != 0;
+    w.openBoundary = p->open_boundary != 0;
+    w.dt = p->dt;
+    w.gravity = ne::V3{p->gravity[0], p->gravity[1], p->gravity[2]};
+    w.restitution = p->restitution;
+    w.friction = p->friction;
+    w.contactBeta = p->contact_beta;
+    w.restitutionCombine = combine_of(p->restitution_combine);
+    w.frictionCombine = combine_of(p->friction_combine);
+    w.warmStart = p->warm_start != 0;
+    w.forceBruteForce = p->force_brute_force != 0;
+    w.sleepEnabled = p->sleep_enabled != 0;
+    w.sleepLinVel = p->sleep_lin_vel;
+    w.sleepAngVel = p->sleep_ang_vel;
+    w.sleepTime = p->sleep_time;
+    w.wakeVel = p->wake_vel;
+    return NE_OK;
+}
+
+int ne_ref_set_bodies(void* h, const ne_bodies* b) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || !b) return NE_ERR_INVALID;
+    return build_bodies(b, r->w.bodies);
+}
+
+int ne_ref_set_capture(void* h, int on) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r) return NE_ERR_INVALID;
+    r->w.captureContacts = on != 0;
+    return NE_OK;
+}
+
+int ne_ref_step(void* h, int32_t nsteps) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r) return NE_ERR_INVALID;
+    for (int32_t s = 0; s < nsteps; ++s) r->w.step();
+    return NE_OK;
+}
+
+int ne_ref_reset_warm_cache(void* h) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r) return NE_ERR_INVALID;
+    r->w.resetWarmCache();
+    return NE_OK;
+}
+
+int ne_ref_get_state(void* h, double* x, double* v, double* q, double* w) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r) return NE_ERR_INVALID;
+    const auto& B = r->w.bodies;
+    for (size_t i = 0; i < B.size(); ++i) {
+        if (x) { x[3 * i] = B[i].x.x; x[3 * i + 1] = B[i].x.y; x[3 * i + 2] = B[i].x.z; }
+        if (v) { v[3 * i] = B[i].v.x; v[3 * i + 1] = B[i].v.y; v[3 * i + 2] = B[i].v.z; }
+        if (q) { q[4 * i] = B[i].q.w; q[4 * i + 1] = B[i].q.x; q[4 * i + 2] = B[i].q.y; q[4 * i + 3] = B[i].q.z; }
+        if (w) { w[3 * i] = B[i].w.x; w[3 * i + 1] = B[i].w.y; w[3 * i + 2] = B[i].w.z; }
+    }
+    return NE_OK;
+}
+
+int ne_ref_get_body_aux(void* h, uint32_t* flags, double* sleep_timer, double* inv_mass,
+                        double* inv_inertia) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r) return NE_ERR_INVALID;
+    const auto& B = r->w.bodies;
+    for (size_t i = 0; i < B.size(); ++i) {
+        if (flags)
+            flags[i] = (B[i].kinematic ? NE_FLAG_KINEMATIC : 0u) | (B[i].sensor ? NE_FLAG_SENSOR : 0u) |
+                       (B[i].dynamic ? NE_FLAG_DYNAMIC : 0u) | (B[i].sleeping ? NE_FLAG_SLEEPING : 0u) |
+                       (B[i].ccd ? NE_FLAG_CCD : 0u);
+        if (sleep_timer) sleep_timer[i] = B[i].sleepTimer;
+        if (inv_mass) inv_mass[i] = B[i].invMass;
+        if (inv_inertia) {
+            inv_inertia[3 * i] = B[i].invInertiaBody.x;
+            inv_inertia[3 * i + 1] = B[i].invInertiaBody.y;
+            inv_inertia[3 * i + 2] = B[i].invInertiaBody.z;
+        }
+    }
+    return NE_OK;
+}
+
+// The last step's constraint list as captured by the reference itself
+// (src/world.cpp:552): i, j, normal (b->a), point, depth, in constraint order.
+int ne_ref_get_contacts(void* h, uint32_t cap, uint32_t* count, uint32_t* ci, uint32_t* cj,
+                        double* normal, double* point, double* depth) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || !count) return NE_ERR_INVALID;
+    const auto& D = r->w.debugContacts;
+    *count = (uint32_t)D.size();
+    uint32_t m = (uint32_t)D.size() < cap ? (uint32_t)D.size() : cap;
+    for (uint32_t k = 0; k < m; ++k) {
+        if (ci) ci[k] = (uint32_t)D[k].i;
+        if (cj) cj[k] = (uint32_t)D[k].j;
+        if (normal) { normal[3 * k] = D[k].normal.x; normal[3 * k + 1] = D[k].normal.y; normal[3 * k + 2] = D[k].normal.z; }
+        if (point) { point[3 * k] = D[k].point.x; point[3 * k + 1] = D[k].point.y; point[3 * k + 2] = D[k].point.z; }
+        if (depth) depth[k] = D[k].depth;
+    }
+    return NE_OK;
+}
+
+// Narrow phase of one pair (a, b) of the current bodies, the way the
+// reference's collide() gathers it for non-mesh shapes (src/world.cpp:122-129):
+// detectContact, then the box-box clipping manifold, falling back to the single
+// detected point when the manifold is empty.  Returns the contact count.
+int ne_ref_pair_contacts(void* h, uint32_t a, uint32_t b, uint32_t cap, uint32_t* count,
+                         double* normal, double* point, double* depth) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || !count) return NE_ERR_INVALID;
+    const auto& B = r->w.bodies;
+    if (a >= B.size() || b >= B.size()) return NE_ERR_INVALID;
+    std::vector<ne::Contact> out;
+    ne::Contact c = ne::detectContact(B[a], B[b], r->w.box);
+    if (c.hit) {
+        bool done = false;
+        if (B[a].shape == ne::Shape::Box && B[b].shape == ne::Shape::Box) {
+            ne::Manifold mf = ne::boxBoxManifold(B[a], B[b], c.normal, r->w.box);
+            for (int p = 0; p < mf.count; ++p) out.push_back({true, c.normal, mf.point[p], mf.depth[p]});
+            done = mf.count > 0;
+        }
+        if (!done) out.push_back(c);
+    }
+    *count = (uint32_t)out.size();
+    uint32_t m = (uint32_t)out.size() < cap ? (uint32_t)out.size() : cap;
+    for (uint32_t k = 0; k < m; ++k) {
+        if (normal) { normal[3 * k] = out[k].normal.x; normal[3 * k + 1] = out[k].normal.y; normal[3 * k + 2] = out[k].normal.z; }
+        if (point) { point[3 * k] = out[k].point.x; point[3 * k + 1] = out[k].point.y; point[3 * k + 2] = out[k].point.z; }
+        if (depth) depth[k] = out[k].overlap;
+    }
+    return NE_OK;
+}
+
+int ne_ref_diagnostics(void* h, double* p, double* L, double* ke) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r) return NE_ERR_INVALID;
+    if (p) { ne::V3 t = r->w.totalLinearMomentum(); p[0] = t.x; p[1] = t.y; p[2] = t.z; }
+    if (L) { ne::V3 t = r->w.totalAngularMomentum(); L[0] = t.x; L[1] = t.y; L[2] = t.z; }
+    if (ke) *ke = r->w.totalKinetic();
+    return NE_OK;
+}
+
+// Box::minImage / Box::wrap of the current world's box (src/box.hpp:24-41).
+int ne_ref_min_image(void* h, const double* d, double* out) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r) return NE_ERR_INVALID;
+    ne::V3 m = r->w.box.minImage(ne::V3{d[0], d[1], d[2]});
+    out[0] = m.x; out[1] = m.y; out[2] = m.z;
+    return NE_OK;
+}
+
+int ne_ref_wrap(void* h, const double* p, double* out) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r) return NE_ERR_INVALID;
+    ne::V3 m = r->w.box.wrap(ne::V3{p[0], p[1], p[2]});
+    out[0] = m.x; out[1] = m.y; out[2] = m.z;
+    return NE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+"""Test-side engines with the GpuWorld interface, for parity checks.
+
+  OracleWorld  the C restatement (oracle/libne_oracle.so)
+  RefWorld     the unmodified reference behind a public-API harness
+               (oracle/_ref/libne_ref.so), present where it was built
+
+Both are checkers only: the product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from nativeengine_amd import _abi
+from nativeengine_amd.scene import Scene
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "libne_oracle.so")
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libne_ref.so")
+
+_dp = C.POINTER(C.c_double)
+_u32p = C.POINTER(C.c_uint32)
+_vp = C.c_void_p
+
+
+def _bind(lib, prefix):
+    g = lambda n: getattr(lib, prefix + n)
+    g("create").argtypes = [C.POINTER(_vp)]
+    g("destroy").argtypes = [_vp]
+    g("set_params").argtypes = [_vp, C.POINTER(_abi.WorldParams)]
+    g("set_bodies").argtypes = [_vp, C.POINTER(_abi.Bodies)]
+    g("step").argtypes = [_vp, C.c_int32]
+    g("get_state").argtypes = [_vp, _dp, _dp, _dp, _dp]
+    g("get_body_aux").argtypes = [_vp, _u32p, _dp, _dp, _dp]
+    g("reset_warm_cache").argtypes = [_vp]
+    g("pair_contacts").argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, _dp, _dp, _dp]
+    g("diagnostics").argtypes = [_vp, _dp, _dp, _dp]
+    g("min_image").argtypes = [_vp, _dp, _dp]
+    g("wrap").argtypes = [_vp, _dp, _dp]
+    return lib
+
+
+_libs = {}
+
+
+def oracle_available():
+    return os.path.exists(ORACLE_LIB)
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def _load(kind):
+    if kind in _libs:
+        return _libs[kind]
+    if kind == "oracle":
+        lib = _bind(C.CDLL(ORACLE_LIB), "ne_oracle_")
+        lib.ne_oracle_get_contacts.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp, _dp]
+        lib.ne_oracle_get_pairs.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _u32p]
+        lib.ne_oracle_set_state.argtypes = [_vp, _dp, _dp, _dp, _dp]
+        lib.ne_oracle_candidate_count.argtypes = [_vp, _u32p]
+    else:
+        lib = _bind(C.CDLL(REF_LIB), "ne_ref_")
+        lib.ne_ref_get_contacts.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
+        lib.ne_ref_set_capture.argtypes = [_vp, C.c_int]
+    _libs[kind] = lib
+    return lib
+
+
+class _CpuWorld:
+    prefix = ""
+    kind = ""
+
+    def __init__(self):
+        self.lib = _load(self.kind)
+        h = _vp()
+        assert self._f("create")(C.byref(h)) == 0
+        self.h = h
+        self.n = 0
+
+    def _f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{self.kind} {what} failed: {rc}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._f("destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        cp = p.to_c()
+        self._check(self._f("set_params")(self.h, C.byref(cp)), "set_params")
+
+    def set_scene(self, s: Scene):
+        self.set_params(s.params)
+        b, keep = s.to_c()
+        self._check(self._f("set_bodies")(self.h, C.byref(b)), "set_bodies")
+        self.n = s.n
+        del keep
+
+    def step(self, nsteps=1):
+        self._check(self._f("step")(self.h, int(nsteps)), "step")
+
+    def reset_warm_cache(self):
+        self._check(self._f("reset_warm_cache")(self.h), "reset_warm_cache")
+
+    def state(self):
+        n = self.n
+        x, v, w = (np.zeros((n, 3)) for _ in range(3))
+        q = np.zeros((n, 4))
+        self._check(self._f("get_state")(self.h, _abi.dptr(x), _abi.dptr(v), _abi.dptr(q), _abi.dptr(w)), "get_state")
+        return x, v, q, w
+
+    def pair_contacts(self, a, b):
+        cnt = C.c_uint32(0)
+        nrm = np.zeros((4, 3))
+        pt = np.zeros((4, 3))
+        dep = np.zeros(4)
+        self._check(self._f("pair_contacts")(self.h, a, b, 4, C.byref(cnt), _abi.dptr(nrm), _abi.dptr(pt), _abi.dptr(dep)),
+                    "pair_contacts")
+        m = cnt.value
+        return nrm[:m].copy(), pt[:m].copy(), dep[:m].copy()
+
+    def diagnostics(self):
+        p = np.zeros(3)
+        L = np.zeros(3)
+        ke = C.c_double(0.0)
+        self._check(self._f("diagnostics")(self.h, _abi.dptr(p), _abi.dptr(L), C.byref(ke)), "diagnostics")
+        return p, L, ke.value
+
+    def min_image(self, d):
+        d = np.ascontiguousarray(d, np.float64)
+        out = np.zeros(3)
+        self._f("min_image")(self.h, _abi.dptr(d), _abi.dptr(out))
+        return out
+
+    def wrap(self, p):
+        p = np.ascontiguousarray(p, np.float64)
+        out = np.zeros(3)
+        self._f("wrap")(self.h, _abi.dptr(p), _abi.dptr(out))
+        return out
+
+    def pairs(self):
+        c = self.contacts()
+        if len(c["i"]) == 0:
+            return c["i"], c["j"]
+        keep = np.ones(len(c["i"]), bool)
+        keep[1:] = (c["i"][1:] != c["i"][:-1]) | (c["j"][1:] != c["j"][:-1])
+        return c["i"][keep], c["j"][keep]
+
+
+class OracleWorld(_CpuWorld):
+    prefix = "ne_oracle_"
+    kind = "oracle"
+
+    def contacts(self):
+        cnt = C.c_uint32(0)
+        self._check(self.lib.ne_oracle_get_contacts(self.h, 0, C.byref(cnt), None, None, None, None, None, None), "contacts")
+        m = cnt.value
+        i = np.zeros(m, np.uint32)
+        j = np.zeros(m, np.uint32)
+        nrm, pt = np.zeros((m, 3)), np.zeros((m, 3))
+        dep, imp = np.zeros(m), np.zeros(m)
+        if m:
+            self._check(self.lib.ne_oracle_get_contacts(self.h, m, C.byref(cnt), _abi.uptr(i), _abi.uptr(j), _abi.dptr(nrm),
+                                                        _abi.dptr(pt), _abi.dptr(dep), _abi.dptr(imp)), "contacts")
+        return dict(i=i, j=j, normal=nrm, point=pt, depth=dep, impulse=imp)
+
+    def candidate_count(self):
+        c = C.c_uint32(0)
+        self.lib.ne_oracle_candidate_count(self.h, C.byref(c))
+        return c.value
+
+    def set_state(self, x=None, v=None, q=None, w=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (x, v, q, w)]
+        self._check(self.lib.ne_oracle_set_state(self.h, *[None if a is None else _abi.dptr(a) for a in arrs]), "set_state")
+
+
+class RefWorld(_CpuWorld):
+    prefix = "ne_ref_"
+    kind = "ref"
+
+    def __init__(self, capture=True):
+        super().__init__()
+        self.lib.ne_ref_set_capture(self.h, 1 if capture else 0)
+
+    def contacts(self):
+        cnt = C.c_uint32(0)
+        self._check(self.lib.ne_ref_get_contacts(self.h, 0, C.byref(cnt), None, None, None, None, None), "contacts")
+        m = cnt.value
+        i = np.zeros(m, np.uint32)
+        j = np.zeros(m, np.uint32)
+        nrm, pt = np.zeros((m, 3)), np.zeros((m, 3))
+        dep = np.zeros(m)
+        if m:
+            self._check(self.lib.ne_ref_get_contacts(self.h, m, C.byref(cnt), _abi.uptr(i), _abi.uptr(j), _abi.dptr(nrm),
+                                                     _abi.dptr(pt), _abi.dptr(dep)), "contacts")
+        return dict(i=i, j=j, normal=nrm, point=pt, depth=dep)

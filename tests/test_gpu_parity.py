@@ -1,0 +1,159 @@
+"""GPU parity: the HIP step (through the C ABI) against the C restatement
+(oracle/libne_oracle.so) and, where it was built, the unmodified reference
+(oracle/_ref/libne_ref.so), on identical seeded inputs.
+
+Bar: bit-identical state (x, v, q, w), constraint list (pair, normal, point,
+depth) and converged normal impulses.  This is stricter than the stated
+tolerances (contacts 1e-9 relative, velocities 1e-6 relative): the kernels
+keep the reference's operation order and are compiled without FMA
+contraction.  Comparisons use IEEE equality, so a static body's +0/-0
+velocity component compares equal (the reference's own `!=` test does the
+same, src/selftest.cpp:760-762).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import engines
+from nativeengine_amd import scene
+from nativeengine_amd import _abi
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu():
+    from nativeengine_amd.gpu import GpuWorld
+    return GpuWorld()
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _assert_same_state(a, b, what):
+    names = ["x", "v", "q", "w"]
+    for name, u, v in zip(names, a, b):
+        if not np.array_equal(u, v):
+            bad = np.argwhere(~((u == v) | (np.isnan(u) & np.isnan(v))))
+            i = bad[0][0]
+            raise AssertionError(f"{what}: {name} differs at body {i}: {u[i]} vs {v[i]} "
+                                 f"(max abs {np.nanmax(np.abs(u - v)):.3e}, {len(np.unique(bad[:, 0]))} bodies)")
+
+
+def _assert_same_contacts(cg, co, what, impulses=True):
+    assert len(cg["i"]) == len(co["i"]), f"{what}: constraint count {len(cg['i'])} vs {len(co['i'])}"
+    assert np.array_equal(cg["i"], co["i"]) and np.array_equal(cg["j"], co["j"]), f"{what}: pair list differs"
+    for k in ["normal", "point", "depth"]:
+        assert np.array_equal(cg[k], co[k]), f"{what}: contact {k} differs (max {np.max(np.abs(cg[k] - co[k])):.3e})"
+    if impulses and "impulse" in co:
+        assert np.array_equal(cg["impulse"], co["impulse"]), \
+            f"{what}: impulses differ (max {np.max(np.abs(cg['impulse'] - co['impulse'])):.3e})"
+
+
+def _run_parity(s, steps, what, check_every=1, use_ref=True):
+    g = _gpu()
+    o = engines.OracleWorld()
+    r = engines.RefWorld() if (use_ref and engines.ref_available()) else None
+    g.set_scene(s)
+    o.set_scene(s)
+    ref_ok = r is not None
+    if ref_ok:
+        try:
+            r.set_scene(s)
+        except RuntimeError:
+            ref_ok = False   # non-cubic box / iterations != 8: not expressible by the reference
+    total_contacts = 0
+    for t in range(steps):
+        g.step(1)
+        o.step(1)
+        if ref_ok:
+            r.step(1)
+        if (t + 1) % check_every and t != steps - 1:
+            continue
+        sg, so = g.state(), o.state()
+        _assert_same_state(sg, so, f"{what} step {t} gpu vs oracle")
+        cg, co = g.contacts(), o.contacts()
+        _assert_same_contacts(cg, co, f"{what} step {t} gpu vs oracle")
+        total_contacts += len(cg["i"])
+        if ref_ok:
+            _assert_same_state(sg, r.state(), f"{what} step {t} gpu vs reference")
+            _assert_same_contacts(cg, r.contacts(), f"{what} step {t} gpu vs reference", impulses=False)
+    return total_contacts
+
+
+def test_fp_ground_truth(gpu_available):
+    """sqrt, divide and floor on the device are bit-equal to the host's IEEE
+    results: the premise of every bitwise claim below."""
+    _say("case: fp probe")
+    g = _gpu()
+    rng = np.random.default_rng(7)
+    a = np.concatenate([rng.uniform(0, 1, 20000), np.exp(rng.uniform(-700, 700, 20000)),
+                        np.array([0.0, 1.0, 2.0, 4.0, 1e-300, 5e-324, 1e308, 0.25, 3.0])])
+    b = np.concatenate([rng.uniform(-1e3, 1e3, 20000), np.exp(rng.uniform(-300, 300, 20000)),
+                        np.array([1.0, 3.0, 7.0, -3.0, 1e-10, 1.0, 0.5, 1e300, -0.0001])])
+    assert np.array_equal(g.fp_probe(0, a).view(np.uint64), np.sqrt(a).view(np.uint64)), "device sqrt not IEEE-exact"
+    assert np.array_equal(g.fp_probe(1, a, b).view(np.uint64), (a / b).view(np.uint64)), "device divide not IEEE-exact"
+    c = rng.uniform(-1e6, 1e6, 40000)
+    assert np.array_equal(g.fp_probe(2, c).view(np.uint64), np.floor(c).view(np.uint64)), "device floor differs"
+    # no contraction: a*b + a must round twice
+    d = rng.uniform(-1, 1, 40000)
+    e = rng.uniform(-1, 1, 40000)
+    assert np.array_equal(g.fp_probe(3, d, e).view(np.uint64), (d * e + d).view(np.uint64)), "FMA contraction on device"
+
+
+def test_sphere_gas_parity(gpu_available, oracle_lib):
+    _say("case: sphere gas 512, 60 steps")
+    s = scene.sphere_gas(512, seed=11, sigma=2.0)
+    n = _run_parity(s, 60, "gas512")
+    assert n > 0
+
+
+def test_sphere_gas_dense_parity(gpu_available, oracle_lib):
+    """Denser gas: multi-contact islands and warm starting across steps."""
+    _say("case: dense gas 2000, 40 steps")
+    s = scene.sphere_gas(2000, L=18.0, seed=5, sigma=3.0)
+    n = _run_parity(s, 40, "densegas", check_every=5)
+    assert n > 100
+
+
+def test_mixed_pile_parity(gpu_available, oracle_lib):
+    """Boxes (clipping manifolds), capsules, hulls, spheres on a static ground
+    box with friction, restitution, gravity, warm start, reflective walls."""
+    _say("case: mixed pile 200, 90 steps")
+    s = scene.mixed_pile(200, half=(6.0, 6.0, 6.0), seed=1)
+    n = _run_parity(s, 90, "pile200", check_every=3)
+    assert n > 100
+
+
+def test_mixed_pile_periodic_parity(gpu_available, oracle_lib):
+    _say("case: periodic mixed pile 150, 60 steps")
+    s = scene.mixed_pile(150, half=(5.0, 5.0, 5.0), seed=4, ground=False, periodic=True)
+    s.params.gravity = (0.0, 0.0, 0.0)
+    rng = np.random.default_rng(3)
+    s.v[:] = rng.normal(0, 2.0, size=s.v.shape)
+    s.w[:] = rng.normal(0, 1.0, size=s.w.shape)
+    _run_parity(s, 60, "ppile150", check_every=3)
+
+
+def test_extension_noncubic_iterations(gpu_available, oracle_lib):
+    """Per-axis box extents and 10 iterations: beyond the reference, pinned by
+    the restatement."""
+    _say("case: non-cubic box, 10 iterations")
+    s = scene.mixed_pile(120, half=(4.0, 8.0, 5.0), seed=9, iterations=10)
+    _run_parity(s, 60, "noncubic", check_every=3, use_ref=False)
+
+
+def test_determinism_bitwise(gpu_available):
+    """Two GPU runs of the same scene are bit-identical (no float atomics)."""
+    _say("case: determinism")
+    s = scene.mixed_pile(300, half=(7.0, 7.0, 7.0), seed=2)
+    out = []
+    for _ in range(2):
+        g = _gpu()
+        g.set_scene(s)
+        g.step(50)
+        out.append(g.state())
+    for u, v in zip(*out):
+        assert np.array_equal(u.view(np.uint64), v.view(np.uint64))
