@@ -180,6 +180,32 @@ int ne_gpu_get_counters(ne_gpu_ctx *ctx, uint32_t out[8]);
 /* The HIP stream the context launches on (as a void*), for event timing. */
 int ne_gpu_get_stream(ne_gpu_ctx *ctx, void **stream);
 
+/* Kernel ids for ne_gpu_time_kernel (one per step stage). */
+#define NE_KERNEL_NONE        0
+#define NE_KERNEL_INTEGRATE   1
+#define NE_KERNEL_CELL_COUNT  2
+#define NE_KERNEL_CELL_FILL   3
+#define NE_KERNEL_CAND_COUNT  4
+#define NE_KERNEL_CAND_WRITE  5
+#define NE_KERNEL_NARROW      6
+#define NE_KERNEL_CONS_BUILD  7
+#define NE_KERNEL_SOLVE       8
+#define NE_KERNEL_BOUNDARY    9
+#define NE_KERNEL_STEP       10   /* the whole step, first to last launch */
+
+/* Brackets every launch of one kernel with HIP events on the context's
+ * stream during the following ne_gpu_step calls (NE_KERNEL_NONE disables),
+ * and resets the accumulated time.  For live roofline measurement. */
+int ne_gpu_time_kernel(ne_gpu_ctx *ctx, int32_t kernel_id);
+
+/* Total milliseconds and launch count accumulated for the timed kernel. */
+int ne_gpu_kernel_time(ne_gpu_ctx *ctx, double *total_ms, uint32_t *launches);
+
+/* Device arithmetic probe: out[t] = sqrt(a) (op 0), a/b (1), floor(a) (2),
+ * a*b+a (3), computed by a kernel -- checks the IEEE ground truth. */
+int ne_gpu_fp_probe(ne_gpu_ctx *ctx, int32_t op, uint32_t n, const double *a,
+                    const double *b, double *out);
+
 /* Last error message of this context (static storage, never NULL). */
 const char *ne_gpu_last_error(ne_gpu_ctx *ctx);
 

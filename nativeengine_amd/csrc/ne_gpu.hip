@@ -901,6 +901,12 @@ struct ne_gpu_ctx {
     uint32_t* counters = nullptr;
     uint32_t* h_counters = nullptr;
     int uf_bits = 1;
+    // live kernel timing (HIP events on this stream)
+    int timed_kernel = 0;
+    std::vector<hipEvent_t> ev_start, ev_stop;
+    uint32_t ev_used = 0;
+    double timed_ms = 0.0;
+    uint32_t timed_launches = 0;
 };
 
 namespace {
@@ -916,6 +922,39 @@ int set_err(ne_gpu_ctx* c, int code, const std::string& msg) {
         if (e_ != hipSuccess)                                                                \
             return set_err(ctx, NE_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
     } while (0)
+
+// Event bracket around one kernel launch when that kernel is being timed.
+struct KTimer {
+    ne_gpu_ctx* c;
+    bool on;
+    KTimer(ne_gpu_ctx* ctx, int kid) : c(ctx), on(ctx->timed_kernel == kid) {
+        if (!on) return;
+        if (c->ev_used == c->ev_start.size()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { on = false; return; }
+            c->ev_start.push_back(a);
+            c->ev_stop.push_back(b);
+        }
+        (void)hipEventRecord(c->ev_start[c->ev_used], c->stream);
+    }
+    ~KTimer() {
+        if (!on) return;
+        (void)hipEventRecord(c->ev_stop[c->ev_used], c->stream);
+        ++c->ev_used;
+    }
+};
+
+// Reads back the events recorded since the last harvest (after a sync).
+void harvest_timing(ne_gpu_ctx* c) {
+    for (uint32_t k = 0; k < c->ev_used; ++k) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, c->ev_start[k], c->ev_stop[k]) == hipSuccess) {
+            c->timed_ms += ms;
+            c->timed_launches += 1;
+        }
+    }
+    c->ev_used = 0;
+}
 
 template <class T>
 void dfree(T*& p) {
@@ -1067,11 +1106,15 @@ int enqueue_step(ne_gpu_ctx* c) {
     BodiesDev B = c->B;
     const uint32_t n = c->n;
     if (n == 0) return NE_OK;
-    if (c->has_forces) {
-        k_integrate<true><<<blocks_for(n), TPB, 0, s>>>(B, P);
-        c->has_forces = false;
-    } else {
-        k_integrate<false><<<blocks_for(n), TPB, 0, s>>>(B, P);
+    KTimer whole(c, NE_KERNEL_STEP);
+    {
+        KTimer kt(c, NE_KERNEL_INTEGRATE);
+        if (c->has_forces) {
+            k_integrate<true><<<blocks_for(n), TPB, 0, s>>>(B, P);
+            c->has_forces = false;
+        } else {
+            k_integrate<false><<<blocks_for(n), TPB, 0, s>>>(B, P);
+        }
     }
     // --- grid
     if (c->ngrid > 0) {
@@ -1079,17 +1122,25 @@ int enqueue_step(ne_gpu_ctx* c) {
             k_aabb_partial<<<AABB_BLOCKS, TPB, 0, s>>>(B, c->gridded, c->ngrid, c->aabb_part);
         k_grid_setup<<<1, 1, 0, s>>>(P, c->cell, c->cellcap, c->aabb_part, AABB_BLOCKS, c->grid, c->counters);
         k_fill_u32<<<gs_blocks(c->cellcap), TPB, 0, s>>>(c->cell_count, 0u, c->counters + CNT_CELLS, c->cellcap);
-        k_cell_count<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, c->gridded, c->ngrid, c->grid, c->cell_of, c->cell_count);
+        {
+            KTimer kt(c, NE_KERNEL_CELL_COUNT);
+            k_cell_count<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, c->gridded, c->ngrid, c->grid, c->cell_of, c->cell_count);
+        }
         scan_u32(c->cell_count, c->cell_start, c->counters + CNT_CELLS, c->cellcap, c->scan_part, s);
-        k_cell_fill<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, c->gridded, c->ngrid, c->cell_of, c->cell_start,
-                                                         c->sorted_idx, c->sorted);
+        {
+            KTimer kt(c, NE_KERNEL_CELL_FILL);
+            k_cell_fill<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, c->gridded, c->ngrid, c->cell_of, c->cell_start,
+                                                             c->sorted_idx, c->sorted);
+        }
     }
     // --- candidate rows
     k_fill_u32<<<gs_blocks(n), TPB, 0, s>>>(c->row_count, 0u, nullptr, n);
-    if (c->ngrid > 0)
+    if (c->ngrid > 0) {
+        KTimer kt(c, NE_KERNEL_CAND_COUNT);
         k_cand_rows<false><<<blocks_for(c->ngrid), TPB, 0, s>>>(B, P, c->grid, c->ngrid, c->sorted_idx, c->sorted,
                                                                 c->cell_start, c->excl, c->nexcl, c->row_count, nullptr,
                                                                 nullptr, 0, nullptr, nullptr);
+    }
     if (c->nexcl > 0) {
         dim3 g(blocks_for(n), c->nexcl);
         k_excl_flags<<<g, TPB, 0, s>>>(B, P, c->excl, c->ex_flags);
@@ -1098,25 +1149,33 @@ int enqueue_step(ne_gpu_ctx* c) {
     }
     scan_u32(c->row_count, c->row_start, nullptr, n, c->scan_part, s);
     k_clamp_count<<<1, 1, 0, s>>>(c->row_start + n, c->pcap, c->counters, CNT_P, CNT_PRAW, NE_FAULT_PAIRS);
-    if (c->ngrid > 0)
+    if (c->ngrid > 0) {
+        KTimer kt(c, NE_KERNEL_CAND_WRITE);
         k_cand_rows<true><<<blocks_for(c->ngrid), TPB, 0, s>>>(B, P, c->grid, c->ngrid, c->sorted_idx, c->sorted,
                                                                c->cell_start, c->excl, c->nexcl, nullptr, c->row_start,
                                                                nullptr, c->pcap, c->cand_i, c->cand_j);
+    }
     if (c->nexcl > 0) {
         dim3 g(blocks_for(n), c->nexcl);
         k_excl_write<<<g, TPB, 0, s>>>(n, c->excl, c->ex_flags, c->ex_off, c->row_start, c->pcap, c->cand_i, c->cand_j);
     }
     // --- narrow phase + constraints
+    {
+    KTimer kt(c, NE_KERNEL_NARROW);
     if (c->spheres_only)
         k_narrow<true><<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
                                                           c->counters + CNT_FAULT);
     else
         k_narrow<false><<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
                                                            c->counters + CNT_FAULT);
+    }
     scan_u32(c->ncont, c->cons_off, c->counters + CNT_P, c->pcap, c->scan_part, s);
     k_cons_count<<<1, 1, 0, s>>>(c->cons_off, c->ccap, c->counters);
-    k_cons_build<<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->cons_off,
-                                                    c->ct, c->C, c->ccap, c->prev_key, c->prev_imp, c->prev_n);
+    {
+        KTimer kt(c, NE_KERNEL_CONS_BUILD);
+        k_cons_build<<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->cons_off,
+                                                        c->ct, c->C, c->ccap, c->prev_key, c->prev_imp, c->prev_n);
+    }
     // --- islands
     k_iota_u32<<<gs_blocks(n), TPB, 0, s>>>(c->parent, nullptr, n);
     k_fill_u32<<<gs_blocks(n), TPB, 0, s>>>(c->used, 0u, nullptr, n);
@@ -1130,12 +1189,18 @@ int enqueue_step(ne_gpu_ctx* c) {
     const uint32_t* svals = in1 ? c->sval1 : c->sval0;
     k_isl_bounds<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, skeys, c->root_id + n, c->isl_start, c->counters);
     // --- solve
-    k_solve_islands<<<gs_blocks(n), TPB, 0, s>>>(B, P, c->counters, c->isl_start, svals, c->C);
+    {
+        KTimer kt(c, NE_KERNEL_SOLVE);
+        k_solve_islands<<<gs_blocks(n), TPB, 0, s>>>(B, P, c->counters, c->isl_start, svals, c->C);
+    }
     // --- boundary
-    if (P.box.periodic)
-        k_wrap<<<blocks_for(n), TPB, 0, s>>>(B, P);
-    else if (!P.open_boundary)
-        k_walls<<<blocks_for(n), TPB, 0, s>>>(B, P);
+    {
+        KTimer kt(c, NE_KERNEL_BOUNDARY);
+        if (P.box.periodic)
+            k_wrap<<<blocks_for(n), TPB, 0, s>>>(B, P);
+        else if (!P.open_boundary)
+            k_walls<<<blocks_for(n), TPB, 0, s>>>(B, P);
+    }
     // --- warm cache for the next step (world.cpp:619-626): this step's list
     if (P.warm) {
         k_copy_cache<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, c->C.key, c->C.nimp, c->prev_key, c->prev_imp,
@@ -1202,6 +1267,8 @@ int ne_gpu_destroy(ne_gpu_ctx* c) {
     if (!c) return NE_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->ev_start) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->ev_stop) (void)hipEventDestroy(e);
     free_all(c);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1267,6 +1334,7 @@ int ne_gpu_step(ne_gpu_ctx* c, int32_t nsteps) {
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(uint32_t) * CNT_N, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    harvest_timing(c);
     const uint32_t fault = c->h_counters[CNT_FAULT];
     if (fault) {
         // grow what overflowed so a re-upload + retry succeeds; state is undefined
@@ -1453,6 +1521,25 @@ int ne_gpu_get_stream(ne_gpu_ctx* c, void** stream) {
 const char* ne_gpu_last_error(ne_gpu_ctx* c) {
     if (!c) return "null context";
     return c->err.c_str();
+}
+
+int ne_gpu_time_kernel(ne_gpu_ctx* c, int32_t kernel_id) {
+    if (!c || kernel_id < 0 || kernel_id > NE_KERNEL_STEP) return NE_ERR_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->ev_used = 0;
+    c->timed_kernel = kernel_id;
+    c->timed_ms = 0.0;
+    c->timed_launches = 0;
+    return NE_OK;
+}
+
+int ne_gpu_kernel_time(ne_gpu_ctx* c, double* total_ms, uint32_t* launches) {
+    if (!c) return NE_ERR_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    harvest_timing(c);
+    if (total_ms) *total_ms = c->timed_ms;
+    if (launches) *launches = c->timed_launches;
+    return NE_OK;
 }
 
 int ne_gpu_fp_probe(ne_gpu_ctx* c, int32_t op, uint32_t n, const double* a, const double* b, double* out) {

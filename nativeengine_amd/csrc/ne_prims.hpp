@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(PRIM_THREADS) k_rs_scatter(const uint32_t* __r
     const uint32_t n = resolve_n(n_dev, n_max);
     const uint32_t beg = blockIdx.x * chunk;
     const uint32_t end = min(beg + chunk, n);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int wid = threadIdx.x >> 6;
     __syncthreads();
     for (uint32_t tb = beg; tb < end; tb += PRIM_THREADS) {
         for (int d = threadIdx.x; d < RS_BINS * PRIM_WAVES; d += PRIM_THREADS) (&wcnt[0][0])[d] = 0;

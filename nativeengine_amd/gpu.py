@@ -24,8 +24,11 @@ EXPORTS = [
     "ne_gpu_create", "ne_gpu_destroy", "ne_gpu_set_params", "ne_gpu_set_bodies", "ne_gpu_set_state",
     "ne_gpu_step", "ne_gpu_get_state", "ne_gpu_get_body_aux", "ne_gpu_reset_warm_cache", "ne_gpu_get_contacts",
     "ne_gpu_get_pairs", "ne_gpu_diagnostics", "ne_gpu_get_counters", "ne_gpu_get_stream", "ne_gpu_last_error",
-    "ne_gpu_fp_probe",
+    "ne_gpu_fp_probe", "ne_gpu_time_kernel", "ne_gpu_kernel_time",
 ]
+
+KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
+           "cons_build": 7, "solve": 8, "boundary": 9, "step": 10}
 
 
 class NeGpuError(RuntimeError):
@@ -62,6 +65,8 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_last_error.argtypes = [vp]
     lib.ne_gpu_last_error.restype = C.c_char_p
     lib.ne_gpu_fp_probe.argtypes = [vp, C.c_int32, C.c_uint32, dp, dp, dp]
+    lib.ne_gpu_time_kernel.argtypes = [vp, C.c_int32]
+    lib.ne_gpu_kernel_time.argtypes = [vp, C.POINTER(C.c_double), u32p]
     for name in EXPORTS:
         if name != "ne_gpu_last_error":
             getattr(lib, name).restype = C.c_int
@@ -175,6 +180,17 @@ class GpuWorld:
         s = C.c_void_p()
         self._check(self.lib.ne_gpu_get_stream(self.h, C.byref(s)))
         return s.value or 0
+
+    def time_kernel(self, name: str | None):
+        """Bracket every launch of one step kernel with HIP events (None: off)."""
+        self._check(self.lib.ne_gpu_time_kernel(self.h, KERNELS[name] if name else 0))
+
+    def kernel_time(self):
+        """(total ms, launches) of the timed kernel since time_kernel()."""
+        ms = C.c_double(0.0)
+        k = C.c_uint32(0)
+        self._check(self.lib.ne_gpu_kernel_time(self.h, C.byref(ms), C.byref(k)))
+        return ms.value, k.value
 
     def fp_probe(self, op: int, a, b=None):
         a = np.ascontiguousarray(a, np.float64)
