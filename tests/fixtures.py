@@ -1,0 +1,40 @@
+"""Loads the golden fixtures written by tests/golden/make_fixtures.py."""
+import glob
+import json
+import os
+
+import numpy as np
+
+from nativeengine_amd import scene as S
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def scene_names():
+    return sorted(os.path.basename(p)[len("scene_"):-len(".npz")] for p in glob.glob(os.path.join(GOLDEN, "scene_*.npz")))
+
+
+def _scene_from(z):
+    n = len(z["in_inv_mass"])
+    s = S.Scene(n)
+    for k in z.files:
+        if k.startswith("in_"):
+            setattr(s, k[3:], z[k].copy())
+    p = json.loads(bytes(z["params_json"]).decode())
+    p["half"] = tuple(p["half"])
+    p["gravity"] = tuple(p["gravity"])
+    s.params = S.Params(**p)
+    return s
+
+
+def load_scene(name):
+    z = np.load(os.path.join(GOLDEN, f"scene_{name}.npz"), allow_pickle=False)
+    s = _scene_from(z)
+    out = {k[4:]: z[k] for k in z.files if k.startswith("out_")}
+    return s, int(z["steps"][0]), out
+
+
+def load_narrow():
+    z = np.load(os.path.join(GOLDEN, "narrow_pairs.npz"), allow_pickle=False)
+    s = _scene_from(z)
+    return s, z["pairs"], z["count"], z["offset"], z["normal"], z["point"], z["depth"]

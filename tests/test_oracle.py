@@ -1,0 +1,293 @@
+"""Pins the C restatement (oracle/libne_oracle.so) before it is trusted as the
+GPU's checker:
+
+  1. against the golden fixtures made from the unmodified reference
+     (tests/golden/*.npz) -- bitwise state, constraint list, counts;
+  2. against the reference itself (oracle/_ref), live, where it was built;
+  3. against closed forms and invariants the mathematics fixes, restating the
+     reference's own known-answer tests (src/selftest.cpp) -- these would
+     catch a dropped term, a wrong sign or a transposed operand even if the
+     reference were absent;
+  4. broadphase: the grid's constraint list equals a brute-force O(N^2) scan
+     over every pair (the reference's own equivalence, selftest.cpp:744-774).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import engines
+import fixtures
+import scenes
+from nativeengine_amd import _abi
+from nativeengine_amd import scene as S
+
+
+def _oracle(s):
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    return o
+
+
+# ---------------------------------------------------------------- fixtures
+
+@pytest.mark.parametrize("name", fixtures.scene_names())
+def test_oracle_matches_golden_scene(oracle_lib, name):
+    s, steps, out = fixtures.load_scene(name)
+    o = _oracle(s)
+    counts = []
+    for _ in range(steps):
+        o.step(1)
+        counts.append(len(o.contacts()["i"]))
+    x, v, q, w = o.state()
+    for k, a in zip("xvqw", (x, v, q, w)):
+        assert np.array_equal(a, out[k]), f"{name}: {k} differs from the reference fixture"
+    c = o.contacts()
+    assert np.array_equal(np.array(counts), out["counts"]), f"{name}: per-step constraint counts differ"
+    assert np.array_equal(c["i"], out["ci"]) and np.array_equal(c["j"], out["cj"])
+    for k in ["normal", "point", "depth"]:
+        assert np.array_equal(c[k], out[k]), f"{name}: contact {k} differs"
+
+
+def test_oracle_matches_golden_narrowphase(oracle_lib):
+    s, pairs, count, off, nrm, pt, dep = fixtures.load_narrow()
+    o = _oracle(s)
+    for t, (a, b) in enumerate(pairs):
+        n2, p2, d2 = o.pair_contacts(int(a), int(b))
+        sl = slice(off[t], off[t + 1])
+        assert len(d2) == count[t], f"pair {t}: contact count {len(d2)} vs {count[t]}"
+        assert np.array_equal(n2, nrm[sl]) and np.array_equal(p2, pt[sl]) and np.array_equal(d2, dep[sl]), f"pair {t}"
+
+
+# ---------------------------------------------------------------- live reference
+
+@pytest.mark.skipif(not engines.ref_available(), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("seed", [21, 22])
+def test_oracle_matches_reference_live(oracle_lib, seed):
+    cases = [S.mixed_pile(90, half=(4.5, 4.5, 4.5), seed=seed), scenes.features_scene(seed)]
+    for s in cases:
+        o, r = _oracle(s), engines.RefWorld()
+        r.set_scene(s)
+        for t in range(25):
+            o.step(1)
+            r.step(1)
+            for k, a, b in zip("xvqw", o.state(), r.state()):
+                assert np.array_equal(a, b), f"step {t}: {k}"
+            co, cr = o.contacts(), r.contacts()
+            for k in ["i", "j", "normal", "point", "depth"]:
+                assert np.array_equal(co[k], cr[k]), f"step {t}: contact {k}"
+
+
+# ---------------------------------------------------------------- closed forms
+
+def test_min_image_and_wrap_kat(oracle_lib):
+    """selftest.cpp:33-47: edge 10, 9 -> -1; wrap 6 -> -4, -6 -> 4."""
+    s = S.Scene(0)
+    s.params = S.Params(half=(5.0, 5.0, 5.0), periodic=True)
+    o = _oracle(s)
+    assert np.array_equal(o.min_image([9.0, 0.0, 0.0]), [-1.0, 0.0, 0.0])
+    assert np.array_equal(o.wrap([6.0, -6.0, 0.0]), [-4.0, 4.0, 0.0])
+    # exact half-edge: floor(0.5 + 0.5) = 1 -> maps +5 to -5 (wrapped into (-half, +half])
+    assert np.array_equal(o.min_image([5.0, -5.0, 15.0]), [-5.0, -5.0, -5.0])
+    s.params.periodic = False
+    o = _oracle(s)
+    assert np.array_equal(o.min_image([9.0, 0.0, 0.0]), [9.0, 0.0, 0.0])
+
+
+def _pair_scene(a, b, periodic=False, half=100.0):
+    s = S.Scene.from_bodies([a, b])
+    s.params = S.Params(half=(half,) * 3, periodic=periodic)
+    return _oracle(s)
+
+
+def test_sphere_cylinder_geometry_kat(oracle_lib):
+    """selftest.cpp:314-341: cap, side, rim, separated."""
+    cyl = S.make_cylinder(0, (0, 0, 0), (1, 0, 0, 0), 1.0, 4.0, 1.0)
+    n, p, d = _pair_scene(S.make_sphere(1, (0, 2.5, 0), 1.0, 1.0), cyl).pair_contacts(0, 1)
+    assert len(d) == 1 and np.allclose(n[0], [0, 1, 0], atol=1e-9) and abs(d[0] - 0.5) < 1e-9
+    n, p, d = _pair_scene(S.make_sphere(1, (1.5, 0, 0), 1.0, 1.0), cyl).pair_contacts(0, 1)
+    assert len(d) == 1 and np.allclose(n[0], [1, 0, 0], atol=1e-9) and abs(d[0] - 0.5) < 1e-9
+    n, p, d = _pair_scene(S.make_sphere(1, (1.5, 2.5, 0), 1.0, 1.0), cyl).pair_contacts(0, 1)
+    assert len(d) == 1 and np.allclose(p[0], [1, 2, 0], atol=1e-9) and abs(d[0] - (1.0 - math.sqrt(0.5))) < 1e-9
+    n, p, d = _pair_scene(S.make_sphere(1, (0, 4.0, 0), 1.0, 1.0), cyl).pair_contacts(0, 1)
+    assert len(d) == 0
+    # reversed order flips the normal (detect.hpp:49-51)
+    n2, _, d2 = _pair_scene(cyl, S.make_sphere(1, (1.5, 0, 0), 1.0, 1.0)).pair_contacts(0, 1)
+    assert np.allclose(n2[0], [-1, 0, 0], atol=1e-9) and abs(d2[0] - 0.5) < 1e-9
+
+
+def test_epa_against_analytic(oracle_lib):
+    """selftest.cpp:416-447 and 501-518: GJK/EPA normal and depth match the
+    analytic answers to 1e-3 (box-box through the manifold gives 4 points)."""
+    # sphere vs box face: normal +x, depth 0.4 (sphere = A, so normal B->A = +x)
+    n, p, d = _pair_scene(S.make_sphere(0, (1.6, 0, 0), 1.0, 1.0),
+                          S.make_box(1, (0, 0, 0), (1, 0, 0, 0), (1, 1, 1), 1.0)).pair_contacts(0, 1)
+    assert len(d) == 1 and np.linalg.norm(n[0] - [1, 0, 0]) < 1e-2 and abs(d[0] - 0.4) < 1e-2
+    # two unit boxes overlapping 0.5 along x: 4-point face manifold, depth 0.5, normal -x
+    n, p, d = _pair_scene(S.make_box(0, (0, 0, 0), (1, 0, 0, 0), (1, 1, 1), 1.0),
+                          S.make_box(1, (1.5, 0, 0), (1, 0, 0, 0), (1, 1, 1), 1.0)).pair_contacts(0, 1)
+    assert len(d) == 4
+    assert np.allclose(np.abs(n[:, 0]), 1.0, atol=1e-3) and np.allclose(d, 0.5, atol=1e-3)
+    assert np.allclose(np.sort(np.abs(p[:, 1])), 1.0, atol=1e-3) and np.allclose(np.abs(p[:, 2]), 1.0, atol=1e-3)
+    # parallel cylinders overlapping radially by 0.5
+    n, p, d = _pair_scene(S.make_cylinder(0, (0, 0, 0), (1, 0, 0, 0), 1.0, 4.0, 1.0),
+                          S.make_cylinder(1, (1.5, 0, 0), (1, 0, 0, 0), 1.0, 4.0, 1.0)).pair_contacts(0, 1)
+    assert len(d) == 1 and abs(abs(n[0][0]) - 1.0) < 1e-3 and abs(d[0] - 0.5) < 1e-3 and abs(p[0][0] - 0.75) < 0.15
+    # capsule-capsule crossing at right angles: closest points on the axes
+    cA = S.make_capsule(0, (0, 0, 0), (1, 0, 0, 0), 0.5, 1.0, 1.0)
+    ang = math.pi / 4
+    cB = S.make_capsule(1, (0, 0, 0.8), (math.cos(ang), 0, 0, math.sin(ang)), 0.5, 1.0, 1.0)   # axis along -x
+    n, p, d = _pair_scene(cA, cB).pair_contacts(0, 1)
+    assert len(d) == 1 and np.allclose(n[0], [0, 0, -1], atol=1e-12) and abs(d[0] - 0.2) < 1e-12
+
+
+def _world(bodies, **kw):
+    s = S.Scene.from_bodies(bodies)
+    s.params = S.Params(**kw)
+    return _oracle(s)
+
+
+def test_elastic_headon_conservation(oracle_lib):
+    """selftest.cpp:52-78: equal masses exchange velocity; p to 1e-9, E to 1e-6."""
+    a = S.make_sphere(0, (-2, 0, 0), 1.0, 1.0)
+    b = S.make_sphere(1, (2, 0, 0), 1.0, 1.0)
+    a.v, b.v = (1.0, 0, 0), (-1.0, 0, 0)
+    o = _world([a, b], half=(100.0,) * 3, periodic=False, restitution=1.0, dt=1e-3)
+    p0, _, e0 = o.diagnostics()
+    o.step(20000)
+    p1, _, e1 = o.diagnostics()
+    x, v, q, w = o.state()
+    assert np.linalg.norm(p1 - p0) < 1e-9 and abs(e1 - e0) < 1e-6
+    assert v[0, 0] < 0 < v[1, 0]
+
+
+def test_free_spin_invariants(oracle_lib):
+    """selftest.cpp:113-130: |q| stays 1, omega and L constant for a sphere."""
+    a = S.make_sphere(0, (0, 0, 0), 1.0, 1.0)
+    a.w = (0.3, -0.7, 1.1)
+    o = _world([a], half=(100.0,) * 3, dt=1e-3)
+    _, L0, _ = o.diagnostics()
+    o.step(5000)
+    x, v, q, w = o.state()
+    _, L1, _ = o.diagnostics()
+    assert abs(np.linalg.norm(q[0]) - 1.0) < 1e-9
+    assert np.allclose(w[0], [0.3, -0.7, 1.1], atol=1e-9) and np.allclose(L0, L1, atol=1e-9)
+
+
+def test_gravity_closed_form(oracle_lib):
+    """selftest.cpp:581-591: v.y = -g t; symplectic Euler x.y = -g dt^2 n(n+1)/2."""
+    a = S.make_sphere(0, (0, 0, 0), 1.0, 1.0)
+    o = _world([a], half=(1e6,) * 3, gravity=(0.0, -10.0, 0.0), dt=1e-3)
+    o.step(1000)
+    x, v, _, _ = o.state()
+    assert abs(v[0, 1] + 10.0) < 1e-9
+    assert abs(x[0, 1] + 10.0 * 1e-6 * 1000 * 1001 / 2) < 1e-9
+
+
+def test_coulomb_cone_in_step(oracle_lib):
+    """A ball sliding into a static floor: the tangential impulse never
+    exceeds mu times the normal impulse (selftest.cpp:545-576 restated for the
+    step's iterative solver); with mu = 0 the tangential velocity is untouched."""
+    for mu in (0.0, 0.5):
+        ball = S.make_sphere(0, (0, 0.95, 0), 1.0, 1.0)
+        ball.v = (2.0, -1.0, 0.0)
+        floor = S.make_box(1, (0, -1.0, 0), (1, 0, 0, 0), (10, 1, 10), 1.0).make_static()
+        o = _world([ball, floor], half=(100.0,) * 3, restitution=0.0, friction=mu, dt=1e-3, warm_start=False)
+        o.step(1)
+        x, v, q, w = o.state()
+        c = o.contacts()
+        assert len(c["i"]) == 1
+        m = 1.0 / ball.inv_mass
+        dvn = v[0, 1] - (-1.0)
+        dvt = 2.0 - v[0, 0]
+        if mu == 0.0:
+            # the EPA normal is only within its 1e-7 tolerance of +y, and its
+            # witness midpoint sits ~1e-3 off the centre line (a small spin)
+            assert abs(v[0, 0] - 2.0) < 1e-6 and np.all(np.abs(w[0]) < 1e-3)
+        else:
+            assert dvt > 1e-6 and m * dvt <= mu * m * dvn + 1e-9
+            assert abs(c["impulse"][0] - m * dvn) < 1e-9
+
+
+def test_reflective_walls_keep_bodies_inside(oracle_lib):
+    """selftest.cpp:135-161: a wall gas stays in the box."""
+    rng = np.random.default_rng(12345)
+    bodies = []
+    for i in range(60):
+        b = S.make_sphere(i, tuple(rng.uniform(-8, 8, 3)), 1.0, 1.0)
+        b.v = tuple(rng.uniform(-3, 3, 3))
+        bodies.append(b)
+    o = _world(bodies, half=(10.0,) * 3, periodic=False, restitution=1.0, dt=5e-3)
+    _, _, e0 = o.diagnostics()
+    for _ in range(20):
+        o.step(100)
+        x, _, _, _ = o.state()
+        assert np.all(np.abs(x) <= 10.0 + 1e-6)
+    _, _, e1 = o.diagnostics()
+    assert abs(e1 - e0) / e0 < 0.01
+
+
+# ---------------------------------------------------------------- broadphase
+
+def _fixed_point_quaternions(q):
+    """Normalise until q/|q| (vmath.hpp:63-67 order) is a bitwise fixed point,
+    so integrating with zero angular velocity leaves q unchanged."""
+    q = q.copy()
+    for _ in range(8):
+        n = np.sqrt(q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1] + q[:, 2] * q[:, 2] + q[:, 3] * q[:, 3])
+        q2 = q / n[:, None]
+        if np.array_equal(q2, q):
+            return q
+        q = q2
+    n = np.sqrt(q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1] + q[:, 2] * q[:, 2] + q[:, 3] * q[:, 3])
+    cyc = np.any(q / n[:, None] != q, axis=1)   # rows stuck in a 2-cycle: use the identity
+    q[cyc] = [1.0, 0.0, 0.0, 0.0]
+    return q
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_grid_equals_brute_force(oracle_lib, seed):
+    """With zero velocities and no gravity the integrator leaves the poses
+    untouched, so a step's constraint list must equal the narrow phase run on
+    every (i<j) pair under the reference's filters (world.cpp:513-516) -- the
+    grid may only prune pairs that cannot touch (selftest.cpp:744-774)."""
+    gas = S.sphere_gas(300, L=12.0, seed=seed)
+    gas.x[:] = np.random.default_rng(seed).uniform(-6.0, 6.0, size=gas.x.shape)   # overlaps allowed
+    cases = [S.mixed_pile(150, half=(5.0, 5.0, 5.0), seed=seed, spacing=0.5), gas, scenes.features_scene(seed)]
+    for k, s in enumerate(cases):
+        s = s.copy()
+        s.v[:] = 0.0
+        s.w[:] = 0.0
+        s.force[:] = 0.0
+        s.torque[:] = 0.0
+        s.q = _fixed_point_quaternions(s.q)
+        s.params.gravity = (0.0, 0.0, 0.0)
+        s.flags &= ~np.uint32(_abi.FLAG_KINEMATIC)
+        if s.params.periodic:
+            # keep every centre inside the primary cell so wrapping is a no-op
+            L = 2 * np.array(s.params.half)
+            s.x = s.x - L * np.floor((s.x + L / 2) / L)
+        o = _oracle(s)
+        x0 = s.x.copy()
+        o.step(1)
+        got = o.contacts()
+        probe = _oracle(s)   # fresh world at the initial poses
+        brute = []
+        n = s.n
+        for i in range(n):
+            for j in range(i + 1, n):
+                if s.inv_mass[i] + s.inv_mass[j] <= 0.0:
+                    continue
+                li, lj = int(s.layer[i]), int(s.layer[j])
+                if not ((int(s.mask[i]) >> (lj & 31)) & 1 and (int(s.mask[j]) >> (li & 31)) & 1):
+                    continue
+                nc = set(int(t) for t in s.nocollide[s.nocollide_offsets[i]:s.nocollide_offsets[i + 1]])
+                if j in nc:
+                    continue
+                if (int(s.flags[i]) | int(s.flags[j])) & _abi.FLAG_SENSOR:
+                    continue
+                nn, pp, dd = probe.pair_contacts(i, j)
+                brute.extend((i, j, tuple(p)) for p in pp)
+        got_list = [(int(a), int(b), tuple(p)) for a, b, p in zip(got["i"], got["j"], got["point"])]
+        assert len(brute) > 0, f"case {k}: scene has no contacts to compare"
+        assert got_list == brute, f"case {k} seed {seed}: grid list ({len(got_list)}) != brute force ({len(brute)})"
