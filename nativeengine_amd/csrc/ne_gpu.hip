@@ -85,6 +85,7 @@ enum CounterSlot : int {
     CNT_FAULT = 7,    // NE_FAULT_* bits
     CNT_PRAW = 8,     // candidate pairs before clamping
     CNT_CRAW = 9,     // constraints before clamping
+    CNT_NBIG = 10,    // islands handed to the workgroup solver
     CNT_N = 16
 };
 
@@ -116,44 +117,66 @@ __device__ __forceinline__ Geom load_geom(const BodiesDev& B, uint32_t i) {
 // Integrate: src/world.cpp:373-411
 // ---------------------------------------------------------------------------
 
-template <bool FORCES>
-__global__ void __launch_bounds__(TPB) k_integrate(BodiesDev B, SimParams P) {
+constexpr uint32_t FLAG_GRID_EXCLUDED = 1u << 30;   // internal: body kept out of the grid
+
+struct StaticGrid {   // periodic worlds: the grid never changes between steps
+    double origin[3], cw[3];
+    int32_t nc[3];
+};
+
+__device__ __forceinline__ uint32_t static_cell(const StaticGrid& g, V3 x);
+
+// Integrate (src/world.cpp:373-411), optionally preceded by the previous
+// step's periodic wrap (world.cpp:698-702, same per-body arithmetic, moved to
+// the start of the next step) and followed by binning the body into the
+// static periodic grid.
+template <bool FORCES, bool WRAP, bool BIN>
+__global__ void __launch_bounds__(TPB) k_integrate(BodiesDev B, SimParams P, StaticGrid G, uint32_t* __restrict__ cell_of,
+                                                   uint32_t* __restrict__ cell_count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.n) return;
     const double dt = P.dt;
     const uint32_t fl = B.flags[i];
+    const double im = B.im[i];
+    V3 x = ld3(B.x0, B.x1, B.x2, i);
+    if (WRAP && im > 0.0) x = wrap_pos(P.box, x);
     V3 f = FORCES ? ld3(B.f0, B.f1, B.f2, i) : V3{0.0, 0.0, 0.0};
     V3 tq = FORCES ? ld3(B.t0, B.t1, B.t2, i) : V3{0.0, 0.0, 0.0};
     if (fl & NE_FLAG_KINEMATIC) {
-        V3 x = ld3(B.x0, B.x1, B.x2, i), v = ld3(B.v0, B.v1, B.v2, i), w = ld3(B.w0, B.w1, B.w2, i);
-        st3(B.x0, B.x1, B.x2, i, x + v * dt);
+        V3 v = ld3(B.v0, B.v1, B.v2, i), w = ld3(B.w0, B.w1, B.w2, i);
+        x = x + v * dt;
         Qt q = integrate_orientation(ldq(B, i), w, dt);
         B.q0[i] = q.w; B.q1[i] = q.x; B.q2[i] = q.y; B.q3[i] = q.z;
-    } else {
-        const double im = B.im[i];
-        if (!(im <= 0.0)) {
-            V3 x = ld3(B.x0, B.x1, B.x2, i), v = ld3(B.v0, B.v1, B.v2, i), w = ld3(B.w0, B.w1, B.w2, i);
-            Qt q = ldq(B, i);
-            V3 invI = ld3(B.ii0, B.ii1, B.ii2, i);
-            const V3 g{P.g[0], P.g[1], P.g[2]};
-            v = v + (g * B.gs[i] + f * im) * dt;
-            x = x + v * dt;
-            V3 L = angular_momentum(q, invI, w);
-            L = L + tq * dt;
-            q = integrate_orientation(q, w, dt);
-            w = apply_inv_inertia(q, invI, L);
-            const double ld = B.ldamp[i], ad = B.adamp[i];
-            if (ld > 0.0) v = v * (1.0 / (1.0 + ld * dt));
-            if (ad > 0.0) w = w * (1.0 / (1.0 + ad * dt));
-            st3(B.x0, B.x1, B.x2, i, x);
-            st3(B.v0, B.v1, B.v2, i, v);
-            st3(B.w0, B.w1, B.w2, i, w);
-            B.q0[i] = q.w; B.q1[i] = q.x; B.q2[i] = q.y; B.q3[i] = q.z;
-        }
+        st3(B.x0, B.x1, B.x2, i, x);
+    } else if (!(im <= 0.0)) {
+        V3 v = ld3(B.v0, B.v1, B.v2, i), w = ld3(B.w0, B.w1, B.w2, i);
+        Qt q = ldq(B, i);
+        V3 invI = ld3(B.ii0, B.ii1, B.ii2, i);
+        const V3 g{P.g[0], P.g[1], P.g[2]};
+        v = v + (g * B.gs[i] + f * im) * dt;
+        x = x + v * dt;
+        V3 L = angular_momentum(q, invI, w);
+        L = L + tq * dt;
+        q = integrate_orientation(q, w, dt);
+        w = apply_inv_inertia(q, invI, L);
+        const double ld = B.ldamp[i], ad = B.adamp[i];
+        if (ld > 0.0) v = v * (1.0 / (1.0 + ld * dt));
+        if (ad > 0.0) w = w * (1.0 / (1.0 + ad * dt));
+        st3(B.x0, B.x1, B.x2, i, x);
+        st3(B.v0, B.v1, B.v2, i, v);
+        st3(B.w0, B.w1, B.w2, i, w);
+        B.q0[i] = q.w; B.q1[i] = q.x; B.q2[i] = q.y; B.q3[i] = q.z;
+    } else if (WRAP) {
+        // static body: unchanged (wrap only applies to invMass > 0)
     }
     if (FORCES) {
         st3(B.f0, B.f1, B.f2, i, V3{0.0, 0.0, 0.0});
         st3(B.t0, B.t1, B.t2, i, V3{0.0, 0.0, 0.0});
+    }
+    if (BIN && !(fl & FLAG_GRID_EXCLUDED)) {
+        const uint32_t c = static_cell(G, x);
+        cell_of[i] = c;
+        atomicAdd(&cell_count[c], 1u);
     }
 }
 
@@ -265,6 +288,14 @@ __device__ __forceinline__ int32_t cell_coord(const GridDev& g, int a, double c)
     return (int32_t)k;
 }
 
+__device__ __forceinline__ uint32_t static_cell(const StaticGrid& sg, V3 x) {
+    GridDev g;
+    for (int a = 0; a < 3; ++a) { g.origin[a] = sg.origin[a]; g.cw[a] = sg.cw[a]; g.nc[a] = sg.nc[a]; }
+    g.periodic = 1;
+    const int32_t cx = cell_coord(g, 0, x.x), cy = cell_coord(g, 1, x.y), cz = cell_coord(g, 2, x.z);
+    return ((uint32_t)cz * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[0] + (uint32_t)cx;
+}
+
 __global__ void __launch_bounds__(TPB) k_cell_count(BodiesDev B, const uint32_t* __restrict__ gridded, uint32_t ngrid,
                                                     const GridDev* __restrict__ Gp, uint32_t* __restrict__ cell_of,
                                                     uint32_t* __restrict__ cell_count) {
@@ -274,7 +305,7 @@ __global__ void __launch_bounds__(TPB) k_cell_count(BodiesDev B, const uint32_t*
     const uint32_t i = gridded[t];
     int32_t cx = cell_coord(g, 0, B.x0[i]), cy = cell_coord(g, 1, B.x1[i]), cz = cell_coord(g, 2, B.x2[i]);
     uint32_t c = ((uint32_t)cz * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[0] + (uint32_t)cx;
-    cell_of[t] = c;
+    cell_of[i] = c;
     atomicAdd(&cell_count[c], 1u);
 }
 
@@ -286,12 +317,14 @@ struct SortedBody {
 // the END of cell c, so begin(c) = cell_start[c-1] afterwards.
 __global__ void __launch_bounds__(TPB) k_cell_fill(BodiesDev B, const uint32_t* __restrict__ gridded, uint32_t ngrid,
                                                    const uint32_t* __restrict__ cell_of, uint32_t* __restrict__ cell_start,
-                                                   uint32_t* __restrict__ sorted_idx, SortedBody* __restrict__ sorted) {
+                                                   uint32_t* __restrict__ sorted_idx, SortedBody* __restrict__ sorted,
+                                                   uint32_t* __restrict__ slot_of) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ngrid) return;
     const uint32_t i = gridded[t];
-    uint32_t slot = atomicAdd(&cell_start[cell_of[t]], 1u);
+    uint32_t slot = atomicAdd(&cell_start[cell_of[i]], 1u);
     sorted_idx[slot] = i;
+    slot_of[i] = slot;
     sorted[slot] = SortedBody{B.x0[i], B.x1[i], B.x2[i], B.bR[i]};
 }
 
@@ -331,33 +364,21 @@ __device__ __forceinline__ int axis_neighbours(const GridDev& g, int a, int32_t 
     return k;
 }
 
-// One thread per gridded body (in cell-sorted order): walks the 27-cell
-// neighbourhood and, in WRITE mode, emits its row (i, j>i) then sorts it.
-template <bool WRITE>
-__global__ void __launch_bounds__(TPB) k_cand_rows(BodiesDev B, SimParams P, const GridDev* __restrict__ Gp,
-                                                   uint32_t ngrid, const uint32_t* __restrict__ sorted_idx,
+// Visits every candidate partner j > i of gridded body i (sorted slot s):
+// bodies in the 27-cell neighbourhood and the grid-excluded bodies, keeping
+// pairs whose bounding spheres overlap and that pass the reference's filters.
+template <class F>
+__device__ __forceinline__ void for_each_candidate(const BodiesDev& B, const SimParams& P, const GridDev& g,
+                                                   uint32_t i, const SortedBody& me,
+                                                   const uint32_t* __restrict__ sorted_idx,
                                                    const SortedBody* __restrict__ sorted,
                                                    const uint32_t* __restrict__ cell_end,
-                                                   const uint32_t* __restrict__ excl, uint32_t nexcl,
-                                                   uint32_t* __restrict__ row_count, const uint32_t* __restrict__ row_start,
-                                                   const uint32_t* __restrict__ pcap_dev, uint32_t pcap,
-                                                   uint32_t* __restrict__ cand_i, uint32_t* __restrict__ cand_j) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ngrid) return;
-    const GridDev g = *Gp;
-    const uint32_t i = sorted_idx[s];
-    const SortedBody me = sorted[s];
+                                                   const uint32_t* __restrict__ excl, uint32_t nexcl, F&& emit) {
     const V3 xi{me.x, me.y, me.z};
     int32_t nx[3], ny[3], nz[3];
     const int kx = axis_neighbours(g, 0, cell_coord(g, 0, me.x), nx);
     const int ky = axis_neighbours(g, 1, cell_coord(g, 1, me.y), ny);
     const int kz = axis_neighbours(g, 2, cell_coord(g, 2, me.z), nz);
-    uint32_t cnt = 0;
-    uint32_t base = 0, lim = 0;
-    if (WRITE) {
-        base = row_start[i];
-        lim = pcap;
-    }
     for (int c = 0; c < kz; ++c)
         for (int b = 0; b < ky; ++b)
             for (int a = 0; a < kx; ++a) {
@@ -369,10 +390,7 @@ __global__ void __launch_bounds__(TPB) k_cand_rows(BodiesDev B, SimParams P, con
                     const SortedBody o = sorted[t];
                     if (!bounds_touch(P.box, xi, me.r, V3{o.x, o.y, o.z}, o.r)) continue;
                     if (!pair_passes(B, i, j)) continue;
-                    if (WRITE) {
-                        if (base + cnt < lim) { cand_i[base + cnt] = i; cand_j[base + cnt] = j; }
-                    }
-                    ++cnt;
+                    emit(j);
                 }
             }
     for (uint32_t e = 0; e < nexcl; ++e) {
@@ -380,23 +398,70 @@ __global__ void __launch_bounds__(TPB) k_cand_rows(BodiesDev B, SimParams P, con
         if (j <= i) continue;
         if (!bounds_touch(P.box, xi, me.r, ld3(B.x0, B.x1, B.x2, j), B.bR[j])) continue;
         if (!pair_passes(B, i, j)) continue;
-        if (WRITE) {
-            if (base + cnt < lim) { cand_i[base + cnt] = i; cand_j[base + cnt] = j; }
-        }
-        ++cnt;
+        emit(j);
     }
-    if (!WRITE) {
-        row_count[i] = cnt;
+}
+
+constexpr uint32_t ROW_SLOTS = 16;   // candidates kept per body by the single walk
+
+// One walk per gridded body (threads in cell-sorted order for locality):
+// counts its row (i, j>i) and keeps the first ROW_SLOTS partners in scratch.
+__global__ void __launch_bounds__(TPB) k_cand_walk(BodiesDev B, SimParams P, const GridDev* __restrict__ Gp,
+                                                   uint32_t ngrid, const uint32_t* __restrict__ sorted_idx,
+                                                   const SortedBody* __restrict__ sorted,
+                                                   const uint32_t* __restrict__ cell_end,
+                                                   const uint32_t* __restrict__ excl, uint32_t nexcl,
+                                                   uint32_t* __restrict__ row_count, uint32_t* __restrict__ row_scratch) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ngrid) return;
+    const GridDev g = *Gp;
+    const uint32_t i = sorted_idx[s];
+    const SortedBody me = sorted[s];
+    uint32_t cnt = 0;
+    uint32_t* row = row_scratch + (size_t)i * ROW_SLOTS;
+    for_each_candidate(B, P, g, i, me, sorted_idx, sorted, cell_end, excl, nexcl, [&](uint32_t j) {
+        if (cnt < ROW_SLOTS) row[cnt] = j;
+        ++cnt;
+    });
+    row_count[i] = cnt;
+}
+
+// Writes each gridded body's row at its offset in (i, j) order: from the
+// scratch when it fit, else by walking the neighbourhood again; then sorts
+// the row by j (rows are short).
+__global__ void __launch_bounds__(TPB) k_cand_emit(BodiesDev B, SimParams P, const GridDev* __restrict__ Gp,
+                                                   const uint32_t* __restrict__ slot_of,
+                                                   const uint32_t* __restrict__ sorted_idx,
+                                                   const SortedBody* __restrict__ sorted,
+                                                   const uint32_t* __restrict__ cell_end,
+                                                   const uint32_t* __restrict__ excl, uint32_t nexcl,
+                                                   const uint32_t* __restrict__ row_count,
+                                                   const uint32_t* __restrict__ row_scratch,
+                                                   const uint32_t* __restrict__ row_start, uint32_t pcap,
+                                                   uint32_t* __restrict__ cand_i, uint32_t* __restrict__ cand_j) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    const uint32_t cnt = row_count[i];
+    const uint32_t s = slot_of[i];
+    if (cnt == 0 || s == 0xFFFFFFFFu) return;   // empty, or a grid-excluded body's row
+    const uint32_t base = row_start[i];
+    uint32_t m = 0;
+    if (cnt <= ROW_SLOTS) {
+        const uint32_t* row = row_scratch + (size_t)i * ROW_SLOTS;
+        for (; m < cnt && base + m < pcap; ++m) { cand_i[base + m] = i; cand_j[base + m] = row[m]; }
     } else {
-        // insertion sort of the row (rows are short: a 27-cell neighbourhood)
-        uint32_t m = cnt;
-        if (base + m > lim) m = base < lim ? lim - base : 0u;
-        for (uint32_t a = 1; a < m; ++a) {
-            uint32_t key = cand_j[base + a];
-            uint32_t b = a;
-            while (b > 0 && cand_j[base + b - 1] > key) { cand_j[base + b] = cand_j[base + b - 1]; --b; }
-            cand_j[base + b] = key;
-        }
+        const GridDev g = *Gp;
+        for_each_candidate(B, P, g, i, sorted[s], sorted_idx, sorted, cell_end, excl, nexcl, [&](uint32_t j) {
+            if (base + m < pcap) { cand_i[base + m] = i; cand_j[base + m] = j; }
+            ++m;
+        });
+        if (base + m > pcap) m = base < pcap ? pcap - base : 0u;
+    }
+    for (uint32_t a = 1; a < m; ++a) {
+        uint32_t key = cand_j[base + a];
+        uint32_t b = a;
+        while (b > 0 && cand_j[base + b - 1] > key) { cand_j[base + b] = cand_j[base + b - 1]; --b; }
+        cand_j[base + b] = key;
     }
 }
 
@@ -528,13 +593,16 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
                                                     const uint32_t* __restrict__ ncont, const uint32_t* __restrict__ cons_off,
                                                     const double* __restrict__ ct, ConsDev C, uint32_t ccap,
                                                     const uint64_t* __restrict__ prev_key, const double* __restrict__ prev_imp,
-                                                    const uint32_t* __restrict__ prev_n) {
+                                                    const uint32_t* __restrict__ prev_n, uint32_t* __restrict__ parent,
+                                                    uint32_t* __restrict__ claimed) {
     const uint32_t np = counters_in[CNT_P];
     const uint32_t pn = *prev_n;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
         const uint32_t nc = ncont[p];
         if (!nc) continue;
         const uint32_t i = cand_i[p], j = cand_j[p];
+        parent[i] = i; parent[j] = j;       // union-find state of this step's bodies
+        claimed[i] = 0u; claimed[j] = 0u;
         const double ima = B.im[i], imb = B.im[j];
         const double ra = B.rest[i] < 0 ? P.restitution : B.rest[i];
         const double rb = B.rest[j] < 0 ? P.restitution : B.rest[j];
@@ -625,147 +693,291 @@ __global__ void __launch_bounds__(TPB) k_uf_link(BodiesDev B, const uint32_t* __
     }
 }
 
-__global__ void __launch_bounds__(TPB) k_uf_root(BodiesDev B, const uint32_t* __restrict__ counters_in, const ConsDev C,
-                                                 uint32_t* __restrict__ parent, uint32_t* __restrict__ cons_root,
-                                                 uint32_t* __restrict__ used) {
-    const uint32_t nc = counters_in[CNT_C];
+// Island ids: the first constraint to reach a root claims it and takes the
+// next id (ids are a scheduling detail; the results do not depend on them).
+__global__ void __launch_bounds__(TPB) k_isl_claim(BodiesDev B, uint32_t* counters, const ConsDev C,
+                                                   uint32_t* __restrict__ parent, uint32_t* __restrict__ cons_root,
+                                                   uint32_t* __restrict__ claimed, uint32_t* __restrict__ root_id,
+                                                   uint32_t* __restrict__ isl_count, uint32_t* __restrict__ isl_fill) {
+    const uint32_t nc = counters[CNT_C];
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
         const uint32_t i = C.ci[k];
         const uint32_t body = mutable_body(B, i) ? i : C.cj[k];
         const uint32_t r = uf_find(parent, body);
         cons_root[k] = r;
-        used[r] = 1u;
+        if (atomicExch(&claimed[r], 1u) == 0u) {
+            const uint32_t id = atomicAdd(&counters[CNT_ISL], 1u);
+            root_id[r] = id;
+            isl_count[id] = 0u;
+            isl_fill[id] = 0u;
+        }
     }
 }
 
-__global__ void __launch_bounds__(TPB) k_isl_key(const uint32_t* __restrict__ counters_in, const uint32_t* __restrict__ cons_root,
-                                                 const uint32_t* __restrict__ root_id, uint32_t* __restrict__ keys,
-                                                 uint32_t* __restrict__ vals) {
-    const uint32_t nc = counters_in[CNT_C];
+__global__ void __launch_bounds__(TPB) k_isl_count(const uint32_t* __restrict__ counters, const uint32_t* __restrict__ cons_root,
+                                                   const uint32_t* __restrict__ root_id, uint32_t* __restrict__ cons_isl,
+                                                   uint32_t* __restrict__ isl_count) {
+    const uint32_t nc = counters[CNT_C];
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
-        keys[k] = root_id[cons_root[k]];
-        vals[k] = k;
+        const uint32_t id = root_id[cons_root[k]];
+        cons_isl[k] = id;
+        atomicAdd(&isl_count[id], 1u);
     }
 }
 
-__global__ void __launch_bounds__(TPB) k_isl_bounds(const uint32_t* __restrict__ counters_in, const uint32_t* __restrict__ skeys,
-                                                    const uint32_t* __restrict__ root_total, uint32_t* __restrict__ isl_start,
-                                                    uint32_t* counters) {
-    const uint32_t nc = counters_in[CNT_C];
-    const uint32_t nisl = *root_total;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += gridDim.x * blockDim.x)
-        if (t == 0 || skeys[t] != skeys[t - 1]) isl_start[skeys[t]] = t;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        isl_start[nisl] = nc;
-        counters[CNT_ISL] = nisl;
+// Unordered fill of each island's segment; the solvers sort their segment.
+__global__ void __launch_bounds__(TPB) k_isl_fill(const uint32_t* __restrict__ counters, const uint32_t* __restrict__ cons_isl,
+                                                  const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_fill,
+                                                  uint32_t* __restrict__ isl_list) {
+    const uint32_t nc = counters[CNT_C];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        const uint32_t id = cons_isl[k];
+        isl_list[isl_start[id] + atomicAdd(&isl_fill[id], 1u)] = k;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Solve: one thread per island sweeps the island's constraints in global
-// (i, j, slot) order: warm start (world.cpp:559-568), I iterations of normal +
-// Coulomb friction (world.cpp:572-612), positional correction (615-616).
+// Solve: each island sweeps its constraints in global (i, j, slot) order:
+// warm start (world.cpp:559-568), I iterations of normal + Coulomb friction
+// (world.cpp:572-612), positional correction (615-616, contact.hpp:78-85).
 // ---------------------------------------------------------------------------
 
-struct BodyDyn {
+// One body as the solver sees it.  Writes of v (w) are skipped for bodies
+// whose inverse mass (inverse inertia) is zero: the reference adds +-0 there.
+struct BodyView {
     V3 v, w;
+    Qt q;
+    V3 ii;
+    double im;
+    bool mv, mw;
 };
 
-__device__ __forceinline__ void apply_pair_impulse(BodiesDev& B, uint32_t i, uint32_t j, V3 rA, V3 rB, V3 J) {
-    const double ima = B.im[i], imb = B.im[j];
-    if (ima != 0.0) {
-        V3 v = ld3(B.v0, B.v1, B.v2, i);
-        st3(B.v0, B.v1, B.v2, i, v + J * ima);
+__device__ __forceinline__ BodyView load_view(const BodiesDev& B, uint32_t b) {
+    BodyView o;
+    o.v = ld3(B.v0, B.v1, B.v2, b);
+    o.w = ld3(B.w0, B.w1, B.w2, b);
+    o.q = ldq(B, b);
+    o.ii = ld3(B.ii0, B.ii1, B.ii2, b);
+    o.im = B.im[b];
+    o.mv = o.im != 0.0;
+    o.mw = o.ii.x != 0.0 || o.ii.y != 0.0 || o.ii.z != 0.0;
+    return o;
+}
+
+__device__ __forceinline__ void store_view(BodiesDev& B, uint32_t b, const BodyView& o) {
+    if (o.mv) st3(B.v0, B.v1, B.v2, b, o.v);
+    if (o.mw) st3(B.w0, B.w1, B.w2, b, o.w);
+}
+
+// a.v += J m_a^-1, a.w += I_a^-1 (rA x J); b gets the opposite (world.cpp:565-566)
+__device__ __forceinline__ void apply_impulse(BodyView& a, BodyView& b, V3 rA, V3 rB, V3 J) {
+    if (a.mv) a.v = a.v + J * a.im;
+    if (a.mw) a.w = a.w + apply_inv_inertia(a.q, a.ii, cross(rA, J));
+    if (b.mv) b.v = b.v - J * b.im;
+    if (b.mw) b.w = b.w - apply_inv_inertia(b.q, b.ii, cross(rB, J));
+}
+
+struct ConsGeom {
+    V3 n, rA, rB;
+};
+
+__device__ __forceinline__ ConsGeom load_geom(const ConsDev& C, uint32_t k) {
+    return ConsGeom{V3{C.n0[k], C.n1[k], C.n2[k]}, V3{C.ra0[k], C.ra1[k], C.ra2[k]}, V3{C.rb0[k], C.rb1[k], C.rb2[k]}};
+}
+
+__device__ __forceinline__ void warm_apply(BodyView& a, BodyView& b, const ConsGeom& g, double nimp) {
+    if (nimp == 0.0) return;
+    apply_impulse(a, b, g.rA, g.rB, g.n * nimp);
+}
+
+// One Gauss-Seidel visit of constraint k (world.cpp:574-611).
+__device__ __forceinline__ void gs_visit(BodyView& a, BodyView& b, const ConsGeom& g, double rbias, double kn, double mu,
+                                         double& nimp, double& timp) {
+    const V3 vc = (a.v + cross(a.w, g.rA)) - (b.v + cross(b.w, g.rB));
+    const double vn = dot(vc, g.n);
+    if (kn > 1e-18) {
+        double dj = (rbias - vn) / kn;
+        const double old = nimp;
+        nimp = (old + dj > 0.0) ? old + dj : 0.0;
+        dj = nimp - old;
+        apply_impulse(a, b, g.rA, g.rB, g.n * dj);
     }
-    if (B.ii0[i] != 0.0 || B.ii1[i] != 0.0 || B.ii2[i] != 0.0) {
-        V3 w = ld3(B.w0, B.w1, B.w2, i);
-        st3(B.w0, B.w1, B.w2, i, w + apply_inv_inertia(ldq(B, i), ld3(B.ii0, B.ii1, B.ii2, i), cross(rA, J)));
-    }
-    if (imb != 0.0) {
-        V3 v = ld3(B.v0, B.v1, B.v2, j);
-        st3(B.v0, B.v1, B.v2, j, v - J * imb);
-    }
-    if (B.ii0[j] != 0.0 || B.ii1[j] != 0.0 || B.ii2[j] != 0.0) {
-        V3 w = ld3(B.w0, B.w1, B.w2, j);
-        st3(B.w0, B.w1, B.w2, j, w - apply_inv_inertia(ldq(B, j), ld3(B.ii0, B.ii1, B.ii2, j), cross(rB, J)));
+    if (mu > 0.0 && nimp > 0.0) {
+        const V3 vc2 = (a.v + cross(a.w, g.rA)) - (b.v + cross(b.w, g.rB));
+        const V3 vt = vc2 - g.n * dot(vc2, g.n);
+        const double vtl = norm(vt);
+        if (vtl > 1e-12) {
+            const V3 t = vt / vtl;
+            const double Kt = eff_mass(a.im, a.q, a.ii, b.im, b.q, b.ii, g.rA, g.rB, t);
+            if (Kt > 1e-18) {
+                double djt = -vtl / Kt;
+                const double maxF = mu * nimp;
+                const double old = timp;
+                const double sum = old + djt;
+                timp = (sum < -maxF) ? -maxF : (sum > maxF ? maxF : sum);
+                djt = timp - old;
+                apply_impulse(a, b, g.rA, g.rB, t * djt);
+            }
+        }
     }
 }
 
-__device__ __forceinline__ V3 contact_vel(const BodiesDev& B, uint32_t i, uint32_t j, V3 rA, V3 rB) {
-    return (ld3(B.v0, B.v1, B.v2, i) + cross(ld3(B.w0, B.w1, B.w2, i), rA)) -
-           (ld3(B.v0, B.v1, B.v2, j) + cross(ld3(B.w0, B.w1, B.w2, j), rB));
+// x_a += n (beta overlap / (m_a^-1 + m_b^-1)) m_a^-1, b opposite (contact.hpp:78-85)
+__device__ __forceinline__ void correct(BodiesDev& B, const ConsDev& C, uint32_t k, double beta, V3& xa, V3& xb,
+                                        double ima, double imb) {
+    const double invSum = ima + imb;
+    const double ov = C.ov[k];
+    if (invSum <= 0.0 || ov <= 0.0) return;
+    const V3 n{C.n0[k], C.n1[k], C.n2[k]};
+    const V3 corr = n * (beta * ov / invSum);
+    if (ima != 0.0) xa = xa + corr * ima;
+    if (imb != 0.0) xb = xb - corr * imb;
 }
 
-__global__ void __launch_bounds__(TPB) k_solve_islands(BodiesDev B, SimParams P, uint32_t* counters_out,
-                                                       const uint32_t* __restrict__ isl_start,
-                                                       const uint32_t* __restrict__ isl_list, ConsDev C) {
-    const uint32_t nisl = counters_out[CNT_ISL];
+constexpr uint32_t SMALL_ISLAND = 64;   // larger islands go to the workgroup solver
+
+__device__ __forceinline__ void sort_segment(uint32_t* a, uint32_t m) {
+    for (uint32_t t = 1; t < m; ++t) {
+        uint32_t key = a[t];
+        uint32_t u = t;
+        while (u > 0 && a[u - 1] > key) { a[u] = a[u - 1]; --u; }
+        a[u] = key;
+    }
+}
+
+// Generic sequential sweep of one island (global-memory body state).
+__device__ void solve_island_seq(BodiesDev& B, const SimParams& P, const ConsDev& C, const uint32_t* list, uint32_t m) {
+    if (P.warm) {
+        for (uint32_t t = 0; t < m; ++t) {
+            const uint32_t k = list[t];
+            const double imp = C.nimp[k];
+            if (imp == 0.0) continue;
+            const uint32_t i = C.ci[k], j = C.cj[k];
+            BodyView a = load_view(B, i), b = load_view(B, j);
+            warm_apply(a, b, load_geom(C, k), imp);
+            store_view(B, i, a);
+            store_view(B, j, b);
+        }
+    }
+    for (int it = 0; it < P.iters; ++it) {
+        for (uint32_t t = 0; t < m; ++t) {
+            const uint32_t k = list[t];
+            const uint32_t i = C.ci[k], j = C.cj[k];
+            BodyView a = load_view(B, i), b = load_view(B, j);
+            double nimp = C.nimp[k], timp = C.timp[k];
+            gs_visit(a, b, load_geom(C, k), C.rbias[k], C.kn[k], C.mu[k], nimp, timp);
+            C.nimp[k] = nimp;
+            C.timp[k] = timp;
+            store_view(B, i, a);
+            store_view(B, j, b);
+        }
+    }
+    for (uint32_t t = 0; t < m; ++t) {
+        const uint32_t k = list[t];
+        const uint32_t i = C.ci[k], j = C.cj[k];
+        V3 xa = ld3(B.x0, B.x1, B.x2, i), xb = ld3(B.x0, B.x1, B.x2, j);
+        const double ima = B.im[i], imb = B.im[j];
+        correct(B, C, k, P.beta, xa, xb, ima, imb);
+        if (ima != 0.0) st3(B.x0, B.x1, B.x2, i, xa);
+        if (imb != 0.0) st3(B.x0, B.x1, B.x2, j, xb);
+    }
+}
+
+// An island whose constraints all join the same two bodies (a sphere pair,
+// a box-box manifold: <= 4 contacts): both bodies and the impulses stay in
+// registers for the whole solve.
+__device__ void solve_pair_island(BodiesDev& B, const SimParams& P, const ConsDev& C, const uint32_t* list, uint32_t m) {
+    const uint32_t k0 = list[0];
+    const uint32_t i = C.ci[k0], j = C.cj[k0];
+    BodyView a = load_view(B, i), b = load_view(B, j);
+    double nimp[4], timp[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        nimp[t] = t < (int)m ? C.nimp[list[t]] : 0.0;
+        timp[t] = 0.0;
+    }
+    if (P.warm) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (t < (int)m) warm_apply(a, b, load_geom(C, list[t]), nimp[t]);
+    }
+    for (int it = 0; it < P.iters; ++it) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (t < (int)m) {
+                const uint32_t k = list[t];
+                gs_visit(a, b, load_geom(C, k), C.rbias[k], C.kn[k], C.mu[k], nimp[t], timp[t]);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        if (t < (int)m) { C.nimp[list[t]] = nimp[t]; C.timp[list[t]] = timp[t]; }
+    store_view(B, i, a);
+    store_view(B, j, b);
+    V3 xa = ld3(B.x0, B.x1, B.x2, i), xb = ld3(B.x0, B.x1, B.x2, j);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        if (t < (int)m) correct(B, C, list[t], P.beta, xa, xb, a.im, b.im);
+    if (a.im != 0.0) st3(B.x0, B.x1, B.x2, i, xa);
+    if (b.im != 0.0) st3(B.x0, B.x1, B.x2, j, xb);
+}
+
+// One thread per small island.
+__global__ void __launch_bounds__(TPB) k_solve_small(BodiesDev B, SimParams P, uint32_t* counters,
+                                                     const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
+                                                     ConsDev C, uint32_t* __restrict__ big_list) {
+    const uint32_t nisl = counters[CNT_ISL];
     for (uint32_t isl = blockIdx.x * blockDim.x + threadIdx.x; isl < nisl; isl += gridDim.x * blockDim.x) {
         const uint32_t s = isl_start[isl], e = isl_start[isl + 1];
-        atomicMax(&counters_out[CNT_MAXISL], e - s);
-        if (P.warm) {
-            for (uint32_t t = s; t < e; ++t) {
-                const uint32_t k = isl_list[t];
-                const double imp = C.nimp[k];
-                if (imp == 0.0) continue;
-                const V3 n{C.n0[k], C.n1[k], C.n2[k]};
-                apply_pair_impulse(B, C.ci[k], C.cj[k], V3{C.ra0[k], C.ra1[k], C.ra2[k]},
-                                   V3{C.rb0[k], C.rb1[k], C.rb2[k]}, n * imp);
-            }
+        const uint32_t m = e - s;
+        if (m > SMALL_ISLAND) {
+            big_list[atomicAdd(&counters[CNT_NBIG], 1u)] = isl;
+            continue;
         }
-        for (int it = 0; it < P.iters; ++it) {
-            for (uint32_t t = s; t < e; ++t) {
-                const uint32_t k = isl_list[t];
-                const uint32_t i = C.ci[k], j = C.cj[k];
-                const V3 n{C.n0[k], C.n1[k], C.n2[k]};
-                const V3 rA{C.ra0[k], C.ra1[k], C.ra2[k]}, rB{C.rb0[k], C.rb1[k], C.rb2[k]};
-                const V3 vc = contact_vel(B, i, j, rA, rB);
-                const double vn = dot(vc, n);
-                const double Kn = C.kn[k];
-                double nimp = C.nimp[k];
-                if (Kn > 1e-18) {
-                    double dj = (C.rbias[k] - vn) / Kn;
-                    const double old = nimp;
-                    nimp = (old + dj > 0.0) ? old + dj : 0.0;
-                    dj = nimp - old;
-                    apply_pair_impulse(B, i, j, rA, rB, n * dj);
-                    C.nimp[k] = nimp;
-                }
-                const double mu = C.mu[k];
-                if (mu > 0.0 && nimp > 0.0) {
-                    const V3 vc2 = contact_vel(B, i, j, rA, rB);
-                    const V3 vt = vc2 - n * dot(vc2, n);
-                    const double vtl = norm(vt);
-                    if (vtl > 1e-12) {
-                        const V3 tdir = vt / vtl;
-                        const double Kt = eff_mass(B.im[i], ldq(B, i), ld3(B.ii0, B.ii1, B.ii2, i), B.im[j], ldq(B, j),
-                                                   ld3(B.ii0, B.ii1, B.ii2, j), rA, rB, tdir);
-                        if (Kt > 1e-18) {
-                            double djt = -vtl / Kt;
-                            const double maxF = mu * nimp;
-                            const double old = C.timp[k];
-                            const double sum = old + djt;
-                            const double tn = (sum < -maxF) ? -maxF : (sum > maxF ? maxF : sum);
-                            djt = tn - old;
-                            C.timp[k] = tn;
-                            apply_pair_impulse(B, i, j, rA, rB, tdir * djt);
-                        }
+        uint32_t* list = isl_list + s;
+        sort_segment(list, m);
+        bool pair = m <= 4;
+        if (pair) {
+            const uint32_t i0 = C.ci[list[0]], j0 = C.cj[list[0]];
+            for (uint32_t t = 1; t < m; ++t) pair = pair && C.ci[list[t]] == i0 && C.cj[list[t]] == j0;
+        }
+        if (pair) solve_pair_island(B, P, C, list, m);
+        else solve_island_seq(B, P, C, list, m);
+    }
+}
+
+// One workgroup per large island: sort the segment, then sweep it.
+__global__ void __launch_bounds__(1024) k_solve_large(BodiesDev B, SimParams P, uint32_t* counters,
+                                                      const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
+                                                      ConsDev C, const uint32_t* __restrict__ big_list) {
+    const uint32_t nbig = counters[CNT_NBIG];
+    for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const uint32_t isl = big_list[b];
+        const uint32_t s = isl_start[isl], e = isl_start[isl + 1];
+        const uint32_t m = e - s;
+        if (threadIdx.x == 0) atomicMax(&counters[CNT_MAXISL], m);
+        uint32_t* list = isl_list + s;
+        // bitonic network with every comparator ascending (a "flip" stage then
+        // half-cleaners), so the virtual +inf padding past m never moves
+        uint32_t p2 = 1;
+        while (p2 < m) p2 <<= 1;
+        for (uint32_t k = 2; k <= p2; k <<= 1) {
+            for (uint32_t jj = k; jj > 1; jj >>= 1) {
+                const uint32_t mask = (jj == k) ? (k - 1) : (jj >> 1);
+                if (jj != k && mask == 0) break;
+                for (uint32_t t = threadIdx.x; t < p2; t += blockDim.x) {
+                    const uint32_t u = t ^ mask;
+                    if (u > t && u < m) {
+                        const uint32_t x = list[t], y = list[u];
+                        if (x > y) { list[t] = y; list[u] = x; }
                     }
                 }
+                __syncthreads();
             }
         }
-        for (uint32_t t = s; t < e; ++t) {          // contact.hpp:78-85
-            const uint32_t k = isl_list[t];
-            const uint32_t i = C.ci[k], j = C.cj[k];
-            const double ima = B.im[i], imb = B.im[j];
-            const double invSum = ima + imb;
-            const double ov = C.ov[k];
-            if (invSum <= 0.0 || ov <= 0.0) continue;
-            const V3 n{C.n0[k], C.n1[k], C.n2[k]};
-            const V3 corr = n * (P.beta * ov / invSum);
-            if (ima != 0.0) st3(B.x0, B.x1, B.x2, i, ld3(B.x0, B.x1, B.x2, i) + corr * ima);
-            if (imb != 0.0) st3(B.x0, B.x1, B.x2, j, ld3(B.x0, B.x1, B.x2, j) - corr * imb);
-        }
+        if (threadIdx.x == 0) solve_island_seq(B, P, C, list, m);
+        __syncthreads();
     }
 }
 
@@ -814,6 +1026,8 @@ __global__ void k_cons_count(const uint32_t* __restrict__ cons_off, uint32_t cca
     counters[CNT_CRAW] = r;
     if (r > ccap) { counters[CNT_FAULT] |= NE_FAULT_CONTACTS; r = ccap; }
     counters[CNT_C] = r;
+    counters[CNT_ISL] = 0u;
+    counters[CNT_NBIG] = 0u;
 }
 
 // This step's (key, slot)-ordered normal impulses become the next step's warm
@@ -897,6 +1111,11 @@ struct ne_gpu_ctx {
     uint32_t *skey0 = nullptr, *sval0 = nullptr, *skey1 = nullptr, *sval1 = nullptr;
     uint32_t* isl_start = nullptr;
     uint32_t *rs_hist = nullptr, *scan_part = nullptr;
+    uint32_t *slot_of = nullptr, *row_scratch = nullptr, *claimed = nullptr, *cons_isl = nullptr;
+    uint32_t *isl_count = nullptr, *isl_fill = nullptr, *isl_list = nullptr, *big_list = nullptr;
+    StaticGrid sgrid{};
+    GridDev grid_host{};
+    uint32_t sgrid_cells = 0;
     // counters
     uint32_t* counters = nullptr;
     uint32_t* h_counters = nullptr;
@@ -1048,10 +1267,12 @@ int alloc_cons(ne_gpu_ctx* c, uint32_t ccap) {
     for (double** p : dp) HIP_TRY(c, dalloc(*p, ccap));
     HIP_TRY(c, dalloc(C.key, ccap));
     HIP_TRY(c, dalloc(c->cons_root, ccap));
-    HIP_TRY(c, dalloc(c->skey0, ccap));
-    HIP_TRY(c, dalloc(c->sval0, ccap));
-    HIP_TRY(c, dalloc(c->skey1, ccap));
-    HIP_TRY(c, dalloc(c->sval1, ccap));
+    HIP_TRY(c, dalloc(c->cons_isl, ccap));
+    HIP_TRY(c, dalloc(c->isl_list, ccap));
+    HIP_TRY(c, dalloc(c->big_list, ccap));
+    HIP_TRY(c, dalloc(c->isl_count, (size_t)ccap + 1));
+    HIP_TRY(c, dalloc(c->isl_fill, (size_t)ccap + 1));
+    HIP_TRY(c, dalloc(c->isl_start, (size_t)ccap + 1));
     return NE_OK;
 }
 
@@ -1089,6 +1310,8 @@ void free_work(ne_gpu_ctx* c) {
     dfree(C.key);
     dfree(c->parent); dfree(c->cons_root); dfree(c->used); dfree(c->root_id);
     dfree(c->skey0); dfree(c->sval0); dfree(c->skey1); dfree(c->sval1); dfree(c->isl_start);
+    dfree(c->slot_of); dfree(c->row_scratch); dfree(c->claimed); dfree(c->cons_isl);
+    dfree(c->isl_count); dfree(c->isl_fill); dfree(c->isl_list); dfree(c->big_list);
     dfree(c->rs_hist); dfree(c->scan_part);
 }
 
@@ -1100,46 +1323,56 @@ void free_all(ne_gpu_ctx* c) {
 }
 
 // One reference step (src/world.cpp:133-146) as a kernel pipeline.
-int enqueue_step(ne_gpu_ctx* c) {
+template <bool F, bool W>
+void launch_integrate(ne_gpu_ctx* c, const SimParams& P, bool bin) {
+    if (bin)
+        k_integrate<F, W, true><<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P, c->sgrid, c->cell_of, c->cell_count);
+    else
+        k_integrate<F, W, false><<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P, c->sgrid, c->cell_of, c->cell_count);
+}
+
+// One reference step (src/world.cpp:133-146) as a kernel pipeline.  When
+// `wrap_first`, the previous step's periodic wrap runs at the top of this
+// step's integrate kernel; the caller wraps after the last step.
+int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     hipStream_t s = c->stream;
     const SimParams P = make_sim_params(c->params);
     BodiesDev B = c->B;
     const uint32_t n = c->n;
     if (n == 0) return NE_OK;
     KTimer whole(c, NE_KERNEL_STEP);
+    const bool static_grid = P.box.periodic && c->ngrid > 0;
     {
         KTimer kt(c, NE_KERNEL_INTEGRATE);
+        const bool w = wrap_first && P.box.periodic;
         if (c->has_forces) {
-            k_integrate<true><<<blocks_for(n), TPB, 0, s>>>(B, P);
+            if (w) launch_integrate<true, true>(c, P, static_grid);
+            else launch_integrate<true, false>(c, P, static_grid);
             c->has_forces = false;
         } else {
-            k_integrate<false><<<blocks_for(n), TPB, 0, s>>>(B, P);
+            if (w) launch_integrate<false, true>(c, P, static_grid);
+            else launch_integrate<false, false>(c, P, static_grid);
         }
     }
     // --- grid
     if (c->ngrid > 0) {
-        if (!P.box.periodic)
+        if (!static_grid) {
             k_aabb_partial<<<AABB_BLOCKS, TPB, 0, s>>>(B, c->gridded, c->ngrid, c->aabb_part);
-        k_grid_setup<<<1, 1, 0, s>>>(P, c->cell, c->cellcap, c->aabb_part, AABB_BLOCKS, c->grid, c->counters);
-        k_fill_u32<<<gs_blocks(c->cellcap), TPB, 0, s>>>(c->cell_count, 0u, c->counters + CNT_CELLS, c->cellcap);
-        {
+            k_grid_setup<<<1, 1, 0, s>>>(P, c->cell, c->cellcap, c->aabb_part, AABB_BLOCKS, c->grid, c->counters);
             KTimer kt(c, NE_KERNEL_CELL_COUNT);
             k_cell_count<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, c->gridded, c->ngrid, c->grid, c->cell_of, c->cell_count);
         }
-        scan_u32(c->cell_count, c->cell_start, c->counters + CNT_CELLS, c->cellcap, c->scan_part, s);
+        scan_u32(c->cell_count, c->cell_start, static_grid ? nullptr : c->counters + CNT_CELLS,
+                 static_grid ? c->sgrid_cells : c->cellcap, c->scan_part, s, true);
         {
             KTimer kt(c, NE_KERNEL_CELL_FILL);
             k_cell_fill<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, c->gridded, c->ngrid, c->cell_of, c->cell_start,
-                                                             c->sorted_idx, c->sorted);
+                                                             c->sorted_idx, c->sorted, c->slot_of);
         }
-    }
-    // --- candidate rows
-    k_fill_u32<<<gs_blocks(n), TPB, 0, s>>>(c->row_count, 0u, nullptr, n);
-    if (c->ngrid > 0) {
+        // --- candidate rows: one neighbourhood walk per body
         KTimer kt(c, NE_KERNEL_CAND_COUNT);
-        k_cand_rows<false><<<blocks_for(c->ngrid), TPB, 0, s>>>(B, P, c->grid, c->ngrid, c->sorted_idx, c->sorted,
-                                                                c->cell_start, c->excl, c->nexcl, c->row_count, nullptr,
-                                                                nullptr, 0, nullptr, nullptr);
+        k_cand_walk<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, P, c->grid, c->ngrid, c->sorted_idx, c->sorted,
+                                                         c->cell_start, c->excl, c->nexcl, c->row_count, c->row_scratch);
     }
     if (c->nexcl > 0) {
         dim3 g(blocks_for(n), c->nexcl);
@@ -1151,9 +1384,9 @@ int enqueue_step(ne_gpu_ctx* c) {
     k_clamp_count<<<1, 1, 0, s>>>(c->row_start + n, c->pcap, c->counters, CNT_P, CNT_PRAW, NE_FAULT_PAIRS);
     if (c->ngrid > 0) {
         KTimer kt(c, NE_KERNEL_CAND_WRITE);
-        k_cand_rows<true><<<blocks_for(c->ngrid), TPB, 0, s>>>(B, P, c->grid, c->ngrid, c->sorted_idx, c->sorted,
-                                                               c->cell_start, c->excl, c->nexcl, nullptr, c->row_start,
-                                                               nullptr, c->pcap, c->cand_i, c->cand_j);
+        k_cand_emit<<<blocks_for(n), TPB, 0, s>>>(B, P, c->grid, c->slot_of, c->sorted_idx, c->sorted, c->cell_start,
+                                                  c->excl, c->nexcl, c->row_count, c->row_scratch, c->row_start, c->pcap,
+                                                  c->cand_i, c->cand_j);
     }
     if (c->nexcl > 0) {
         dim3 g(blocks_for(n), c->nexcl);
@@ -1161,45 +1394,40 @@ int enqueue_step(ne_gpu_ctx* c) {
     }
     // --- narrow phase + constraints
     {
-    KTimer kt(c, NE_KERNEL_NARROW);
-    if (c->spheres_only)
-        k_narrow<true><<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                          c->counters + CNT_FAULT);
-    else
-        k_narrow<false><<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                           c->counters + CNT_FAULT);
+        KTimer kt(c, NE_KERNEL_NARROW);
+        if (c->spheres_only)
+            k_narrow<true><<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
+                                                              c->counters + CNT_FAULT);
+        else
+            k_narrow<false><<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
+                                                               c->counters + CNT_FAULT);
     }
     scan_u32(c->ncont, c->cons_off, c->counters + CNT_P, c->pcap, c->scan_part, s);
     k_cons_count<<<1, 1, 0, s>>>(c->cons_off, c->ccap, c->counters);
     {
         KTimer kt(c, NE_KERNEL_CONS_BUILD);
         k_cons_build<<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->cons_off,
-                                                        c->ct, c->C, c->ccap, c->prev_key, c->prev_imp, c->prev_n);
+                                                        c->ct, c->C, c->ccap, c->prev_key, c->prev_imp, c->prev_n,
+                                                        c->parent, c->claimed);
     }
     // --- islands
-    k_iota_u32<<<gs_blocks(n), TPB, 0, s>>>(c->parent, nullptr, n);
-    k_fill_u32<<<gs_blocks(n), TPB, 0, s>>>(c->used, 0u, nullptr, n);
     k_uf_link<<<gs_blocks(c->ccap), TPB, 0, s>>>(B, c->counters, c->C, c->parent);
-    k_uf_root<<<gs_blocks(c->ccap), TPB, 0, s>>>(B, c->counters, c->C, c->parent, c->cons_root, c->used);
-    scan_u32(c->used, c->root_id, nullptr, n, c->scan_part, s);
-    k_isl_key<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, c->cons_root, c->root_id, c->skey0, c->sval0);
-    const bool in1 = radix_sort_u32(c->skey0, c->sval0, c->skey1, c->sval1, c->counters + CNT_C, c->ccap, c->uf_bits,
-                                    c->rs_hist, c->scan_part, s);
-    const uint32_t* skeys = in1 ? c->skey1 : c->skey0;
-    const uint32_t* svals = in1 ? c->sval1 : c->sval0;
-    k_isl_bounds<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, skeys, c->root_id + n, c->isl_start, c->counters);
+    k_isl_claim<<<gs_blocks(c->ccap), TPB, 0, s>>>(B, c->counters, c->C, c->parent, c->cons_root, c->claimed,
+                                                   c->root_id, c->isl_count, c->isl_fill);
+    k_isl_count<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, c->cons_root, c->root_id, c->cons_isl, c->isl_count);
+    scan_u32(c->isl_count, c->isl_start, c->counters + CNT_ISL, c->ccap, c->scan_part, s);
+    k_isl_fill<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, c->cons_isl, c->isl_start, c->isl_fill, c->isl_list);
     // --- solve
     {
         KTimer kt(c, NE_KERNEL_SOLVE);
-        k_solve_islands<<<gs_blocks(n), TPB, 0, s>>>(B, P, c->counters, c->isl_start, svals, c->C);
+        k_solve_small<<<gs_blocks(c->ccap), TPB, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C,
+                                                         c->big_list);
+        k_solve_large<<<256, 1024, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list);
     }
-    // --- boundary
-    {
+    // --- boundary (the periodic wrap is deferred to the next integrate)
+    if (!P.box.periodic && !P.open_boundary) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
-        if (P.box.periodic)
-            k_wrap<<<blocks_for(n), TPB, 0, s>>>(B, P);
-        else if (!P.open_boundary)
-            k_walls<<<blocks_for(n), TPB, 0, s>>>(B, P);
+        k_walls<<<blocks_for(n), TPB, 0, s>>>(B, P);
     }
     // --- warm cache for the next step (world.cpp:619-626): this step's list
     if (P.warm) {
@@ -1210,6 +1438,36 @@ int enqueue_step(ne_gpu_ctx* c) {
 }
 
 int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b);
+
+// Periodic worlds: cells of width L/nc >= the gridded bodies' 2*maxR, with the
+// cell table kept within its capacity; the same rule k_grid_setup applies.
+int prepare_static_grid(ne_gpu_ctx* c) {
+    c->sgrid_cells = 0;
+    if (!c->params.periodic || c->ngrid == 0) return NE_OK;
+    StaticGrid g{};
+    GridDev gd{};
+    for (int a = 0; a < 3; ++a) {
+        const double L = 2.0 * c->params.half[a];
+        double k = std::floor(L / c->cell);
+        if (k < 1.0) k = 1.0;
+        if (k > 1048576.0) k = 1048576.0;
+        g.nc[a] = (int32_t)k;
+    }
+    while ((double)g.nc[0] * g.nc[1] * g.nc[2] > (double)c->cellcap)
+        for (int a = 0; a < 3; ++a) g.nc[a] = std::max(1, g.nc[a] * 4 / 5);
+    for (int a = 0; a < 3; ++a) {
+        g.cw[a] = 2.0 * c->params.half[a] / (double)g.nc[a];
+        g.origin[a] = -c->params.half[a];
+        gd.nc[a] = g.nc[a]; gd.cw[a] = g.cw[a]; gd.origin[a] = g.origin[a];
+    }
+    gd.periodic = 1;
+    gd.ncells = (uint32_t)g.nc[0] * (uint32_t)g.nc[1] * (uint32_t)g.nc[2];
+    c->sgrid = g;
+    c->sgrid_cells = gd.ncells;
+    c->grid_host = gd;
+    HIP_TRY(c, hipMemcpy(c->grid, &c->grid_host, sizeof(GridDev), hipMemcpyHostToDevice));
+    return NE_OK;
+}
 
 }  // namespace
 
@@ -1326,10 +1584,18 @@ int ne_gpu_step(ne_gpu_ctx* c, int32_t nsteps) {
     if (nsteps < 0) return set_err(c, NE_ERR_INVALID, "nsteps < 0");
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
     if (c->n == 0) return NE_OK;
+    {
+        int r = prepare_static_grid(c);
+        if (r != NE_OK) return r;
+    }
     HIP_TRY(c, hipMemsetAsync(c->counters, 0, sizeof(uint32_t) * CNT_N, c->stream));
     for (int32_t t = 0; t < nsteps; ++t) {
-        int r = enqueue_step(c);
+        int r = enqueue_step(c, t > 0);
         if (r != NE_OK) return r;
+    }
+    if (c->params.periodic && nsteps > 0) {
+        KTimer kt(c, NE_KERNEL_BOUNDARY);
+        k_wrap<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, make_sim_params(c->params));
     }
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(uint32_t) * CNT_N, hipMemcpyDeviceToHost, c->stream));
@@ -1387,7 +1653,10 @@ int ne_gpu_get_body_aux(ne_gpu_ctx* c, uint32_t* flags, double* sleep_timer, dou
     const uint32_t n = c->n;
     if (n == 0) return NE_OK;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (flags) HIP_TRY(c, hipMemcpy(flags, c->ufield + (size_t)U_FLAGS * c->ncap, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (flags) {
+        HIP_TRY(c, hipMemcpy(flags, c->ufield + (size_t)U_FLAGS * c->ncap, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) flags[i] &= ~FLAG_GRID_EXCLUDED;
+    }
     if (sleep_timer) for (uint32_t i = 0; i < n; ++i) sleep_timer[i] = 0.0;
     if (inv_mass) HIP_TRY(c, hipMemcpy(inv_mass, c->dfield + (size_t)F_IM * c->ncap, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (inv_inertia) {
@@ -1507,7 +1776,7 @@ int ne_gpu_get_counters(ne_gpu_ctx* c, uint32_t out[8]) {
     out[3] = h[CNT_ISL];
     out[4] = h[CNT_MAXISL];
     out[5] = c->ngrid;
-    out[6] = h[CNT_CELLS];
+    out[6] = c->sgrid_cells ? c->sgrid_cells : h[CNT_CELLS];
     out[7] = h[CNT_FAULT];
     return NE_OK;
 }
@@ -1603,10 +1872,13 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
         HIP_TRY(c, dalloc(c->parent, ncap));
         HIP_TRY(c, dalloc(c->used, (size_t)ncap + 1));
         HIP_TRY(c, dalloc(c->root_id, (size_t)ncap + 1));
-        HIP_TRY(c, dalloc(c->isl_start, (size_t)ncap + 1));
+        HIP_TRY(c, dalloc(c->slot_of, ncap));
+        HIP_TRY(c, dalloc(c->claimed, ncap));
+        HIP_TRY(c, dalloc(c->row_scratch, (size_t)ncap * ROW_SLOTS));
         c->cellcap = std::max<uint32_t>(2u * ncap, 4096u);
         HIP_TRY(c, dalloc(c->cell_count, (size_t)c->cellcap + 1));
         HIP_TRY(c, dalloc(c->cell_start, (size_t)c->cellcap + 1));
+        HIP_TRY(c, hipMemset(c->cell_count, 0, sizeof(uint32_t) * ((size_t)c->cellcap + 1)));
         uint32_t pcap = c->user_pcap ? c->user_pcap : 8u * ncap + 4096u;
         uint32_t ccap = c->user_ccap ? c->user_ccap : 4u * ncap + 4096u;
         int r = alloc_pairs(c, pcap);
@@ -1698,8 +1970,9 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
         hu[(size_t)U_SHAPE * ncap + i] = (uint32_t)b->shape[i];
         hu[(size_t)U_LAYER * ncap + i] = b->layer ? b->layer[i] : 0u;
         hu[(size_t)U_MASK * ncap + i] = b->mask ? b->mask[i] : 0xFFFFFFFFu;
-        hu[(size_t)U_FLAGS * ncap + i] = b->flags ? b->flags[i] : NE_FLAG_DYNAMIC;
+        hu[(size_t)U_FLAGS * ncap + i] = (b->flags ? b->flags[i] : NE_FLAG_DYNAMIC) & ~FLAG_GRID_EXCLUDED;
     }
+    for (uint32_t e : excl) hu[(size_t)U_FLAGS * ncap + e] |= FLAG_GRID_EXCLUDED;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipMemcpy(c->dfield, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->ufield, hu.data(), sizeof(uint32_t) * hu.size(), hipMemcpyHostToDevice));
@@ -1711,6 +1984,7 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     HIP_TRY(c, hipMemcpy(c->nc_off, noff.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
     if (!gridded.empty())
         HIP_TRY(c, hipMemcpy(c->gridded, gridded.data(), sizeof(uint32_t) * gridded.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(c->slot_of, 0xFF, sizeof(uint32_t) * ncap));
     HIP_TRY(c, dalloc(c->excl, std::max<size_t>(excl.size(), 1)));
     if (!excl.empty()) {
         HIP_TRY(c, hipMemcpy(c->excl, excl.data(), sizeof(uint32_t) * excl.size(), hipMemcpyHostToDevice));

@@ -87,9 +87,10 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ partia
 }
 
 // Phase 3: rescan each chunk from its block offset.
-__global__ void __launch_bounds__(PRIM_THREADS) k_scan_final(const uint32_t* __restrict__ in, const uint32_t* n_dev,
+__global__ void __launch_bounds__(PRIM_THREADS) k_scan_final(uint32_t* in, const uint32_t* n_dev,
                                                             uint32_t n_max, uint32_t chunk,
-                                                            const uint32_t* __restrict__ partial, uint32_t* __restrict__ out) {
+                                                            const uint32_t* __restrict__ partial, uint32_t* out,
+                                                            int clear_in) {
     __shared__ uint32_t lds[PRIM_WAVES + 1];
     const uint32_t n = resolve_n(n_dev, n_max);
     const uint32_t beg = blockIdx.x * chunk;
@@ -100,7 +101,10 @@ __global__ void __launch_bounds__(PRIM_THREADS) k_scan_final(const uint32_t* __r
         uint32_t v = t < end ? in[t] : 0;
         uint32_t tot;
         uint32_t ex = block_exclusive_scan(v, &tot, lds);
-        if (t < end) out[t] = run + ex;
+        if (t < end) {
+            if (clear_in) in[t] = 0u;   // leaves a zeroed histogram for the next use
+            out[t] = run + ex;
+        }
         run += tot;
     }
 }
@@ -201,12 +205,13 @@ inline ScanPlan scan_plan(uint32_t n_max) {
 }
 
 // out has n+1 entries: out[t] = sum(in[0..t)), out[n] = total.
-inline void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_max, uint32_t* partial,
-                     hipStream_t s) {
+// clear_in: zero the input as it is consumed (it must not alias out).
+inline void scan_u32(uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_max, uint32_t* partial,
+                     hipStream_t s, bool clear_in = false) {
     ScanPlan p = scan_plan(n_max);
     k_scan_partial<<<p.nblocks, PRIM_THREADS, 0, s>>>(in, n_dev, n_max, p.chunk, partial);
     k_scan_top<<<1, 1024, 0, s>>>(partial, p.nblocks, n_dev, n_max, out);
-    k_scan_final<<<p.nblocks, PRIM_THREADS, 0, s>>>(in, n_dev, n_max, p.chunk, partial, out);
+    k_scan_final<<<p.nblocks, PRIM_THREADS, 0, s>>>(in, n_dev, n_max, p.chunk, partial, out, clear_in ? 1 : 0);
 }
 
 // Sorts (keys, vals) by the low `bits` bits of keys, stably.  Ping-pongs
