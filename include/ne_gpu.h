@@ -157,6 +157,17 @@ int ne_gpu_get_body_aux(ne_gpu_ctx *ctx, uint32_t *flags, double *sleep_timer,
 /* `World::resetWarmCache()` (src/world.hpp:140). */
 int ne_gpu_reset_warm_cache(ne_gpu_ctx *ctx);
 
+/* The warm-start cache (`World::warmCache_`, src/world.hpp:157): the last
+ * warm-started step's normal impulses, one entry per contact, ordered by
+ * pair key (i << 32 | j, src/world.cpp:34-36) then contact slot.  Export with
+ * NULL arrays to get the count; import replaces the cache (keys must be in
+ * that order).  Lets a copied World keep its cache, as the reference's value
+ * semantics do. */
+int ne_gpu_export_warm_cache(ne_gpu_ctx *ctx, uint32_t cap, uint32_t *count,
+                             uint64_t *keys, double *normal_impulse);
+int ne_gpu_import_warm_cache(ne_gpu_ctx *ctx, uint32_t count, const uint64_t *keys,
+                             const double *normal_impulse);
+
 /* Parity taps for the last completed step.  The contact list is the
  * reference's constraint list (src/world.cpp:535-553) in its order: body pair
  * (i<j), normal b->a, contact point, depth, and the converged normal impulse.

@@ -1677,6 +1677,33 @@ int ne_gpu_reset_warm_cache(ne_gpu_ctx* c) {
     return NE_OK;
 }
 
+int ne_gpu_export_warm_cache(ne_gpu_ctx* c, uint32_t cap, uint32_t* count, uint64_t* keys, double* imp) {
+    if (!c || !count) return NE_ERR_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    uint32_t pn = 0;
+    HIP_TRY(c, hipMemcpy(&pn, c->prev_n, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    *count = pn;
+    const uint32_t m = std::min(pn, cap);
+    if (m && keys) HIP_TRY(c, hipMemcpy(keys, c->prev_key, sizeof(uint64_t) * m, hipMemcpyDeviceToHost));
+    if (m && imp) HIP_TRY(c, hipMemcpy(imp, c->prev_imp, sizeof(double) * m, hipMemcpyDeviceToHost));
+    return NE_OK;
+}
+
+int ne_gpu_import_warm_cache(ne_gpu_ctx* c, uint32_t count, const uint64_t* keys, const double* imp) {
+    if (!c || (count && (!keys || !imp))) return NE_ERR_INVALID;
+    for (uint32_t k = 1; k < count; ++k)
+        if (keys[k] < keys[k - 1]) return set_err(c, NE_ERR_INVALID, "warm cache keys must be sorted");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int r = ensure_prev(c, std::max(count, c->ccap));
+    if (r != NE_OK) return r;
+    if (count) {
+        HIP_TRY(c, hipMemcpy(c->prev_key, keys, sizeof(uint64_t) * count, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->prev_imp, imp, sizeof(double) * count, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(c, hipMemcpy(c->prev_n, &count, sizeof(uint32_t), hipMemcpyHostToDevice));
+    return NE_OK;
+}
+
 int ne_gpu_get_contacts(ne_gpu_ctx* c, uint32_t cap, uint32_t* count, uint32_t* ci, uint32_t* cj, double* normal,
                         double* point, double* depth, double* normal_impulse) {
     if (!c || !count) return NE_ERR_INVALID;

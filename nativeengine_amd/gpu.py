@@ -24,7 +24,8 @@ EXPORTS = [
     "ne_gpu_create", "ne_gpu_destroy", "ne_gpu_set_params", "ne_gpu_set_bodies", "ne_gpu_set_state",
     "ne_gpu_step", "ne_gpu_get_state", "ne_gpu_get_body_aux", "ne_gpu_reset_warm_cache", "ne_gpu_get_contacts",
     "ne_gpu_get_pairs", "ne_gpu_diagnostics", "ne_gpu_get_counters", "ne_gpu_get_stream", "ne_gpu_last_error",
-    "ne_gpu_fp_probe", "ne_gpu_time_kernel", "ne_gpu_kernel_time",
+    "ne_gpu_fp_probe", "ne_gpu_time_kernel", "ne_gpu_kernel_time", "ne_gpu_export_warm_cache",
+    "ne_gpu_import_warm_cache",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
@@ -66,6 +67,9 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_last_error.restype = C.c_char_p
     lib.ne_gpu_fp_probe.argtypes = [vp, C.c_int32, C.c_uint32, dp, dp, dp]
     lib.ne_gpu_time_kernel.argtypes = [vp, C.c_int32]
+    u64p = C.POINTER(C.c_uint64)
+    lib.ne_gpu_export_warm_cache.argtypes = [vp, C.c_uint32, u32p, u64p, dp]
+    lib.ne_gpu_import_warm_cache.argtypes = [vp, C.c_uint32, u64p, dp]
     lib.ne_gpu_kernel_time.argtypes = [vp, C.POINTER(C.c_double), u32p]
     for name in EXPORTS:
         if name != "ne_gpu_last_error":
@@ -128,6 +132,22 @@ class GpuWorld:
 
     def reset_warm_cache(self):
         self._check(self.lib.ne_gpu_reset_warm_cache(self.h))
+
+    def export_warm_cache(self):
+        cnt = C.c_uint32(0)
+        self._check(self.lib.ne_gpu_export_warm_cache(self.h, 0, C.byref(cnt), None, None))
+        keys = np.zeros(cnt.value, np.uint64)
+        imp = np.zeros(cnt.value)
+        if cnt.value:
+            self._check(self.lib.ne_gpu_export_warm_cache(self.h, cnt.value, C.byref(cnt),
+                                                          keys.ctypes.data_as(C.POINTER(C.c_uint64)), _abi.dptr(imp)))
+        return keys, imp
+
+    def import_warm_cache(self, keys, imp):
+        keys = np.ascontiguousarray(keys, np.uint64)
+        imp = np.ascontiguousarray(imp, np.float64)
+        self._check(self.lib.ne_gpu_import_warm_cache(self.h, len(keys), keys.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                      _abi.dptr(imp)))
 
     # -- download ---------------------------------------------------------
     def state(self):

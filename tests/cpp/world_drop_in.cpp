@@ -1,0 +1,201 @@
+// world_drop_in.cpp -- the C++ drop-in `ne::World` (include/ne_world.hpp) on the
+// GPU, exercised the way the reference's own selftest drives its World
+// (src/selftest.cpp).  Built and run by tests/test_cpp_drop_in.py.
+//
+//   world_drop_in checks           restated selftest scenarios, exit code = failures
+//   world_drop_in scene IN OUT K   steps a flat-array scene K times, writes x v q w
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <vector>
+
+#include "ne_world.hpp"
+
+using namespace ne;
+
+static int g_fail = 0;
+static void check(bool ok, const char* name) {
+    std::printf("  [%s] %s\n", ok ? "PASS" : "FAIL", name);
+    if (!ok) ++g_fail;
+}
+static bool closeV(const V3& a, const V3& b, double tol) { return (a - b).norm() <= tol; }
+
+// splitmix64: a portable generator for the scenes below
+struct Rng {
+    std::uint64_t s;
+    explicit Rng(std::uint64_t seed) : s(seed) {}
+    std::uint64_t next() {
+        std::uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uni(double a, double b) { return a + (b - a) * ((next() >> 11) * (1.0 / 9007199254740992.0)); }
+};
+
+static void headon() {        // selftest.cpp:52-78
+    World w;
+    w.box.half = 100.0; w.box.periodic = false; w.restitution = 1.0; w.dt = 1e-3;
+    Body a = makeSphere(0, V3{-2, 0, 0}, 1.0, 1.0), b = makeSphere(1, V3{2, 0, 0}, 1.0, 1.0);
+    a.v = V3{1, 0, 0}; b.v = V3{-1, 0, 0};
+    w.bodies = {a, b};
+    V3 p0 = w.totalLinearMomentum(); double e0 = w.totalKinetic();
+    w.stepMany(20000);
+    check(closeV(p0, w.totalLinearMomentum(), 1e-9), "head-on: linear momentum conserved");
+    check(std::fabs(e0 - w.totalKinetic()) <= 1e-6, "head-on: kinetic energy conserved (e=1)");
+    check(w.bodies[0].v.x < 0 && w.bodies[1].v.x > 0, "head-on: equal masses exchange velocity sign");
+}
+
+static void transit() {       // selftest.cpp:84-107
+    World w;
+    w.box.half = 5.0; w.box.periodic = true; w.dt = 1e-2;
+    Body a = makeSphere(0, V3{4.9, 0, 0}, 0.5, 1.0);
+    a.v = V3{1, 0, 0};
+    w.bodies = {a};
+    V3 prev = w.bodies[0].x, unwrapped = prev;
+    for (int s = 0; s < 300; ++s) {
+        w.step();
+        V3 cur = w.bodies[0].x;
+        unwrapped += w.box.minImage(cur - prev);
+        prev = cur;
+    }
+    check(closeV(w.bodies[0].v, V3{1, 0, 0}, 1e-12), "periodic: velocity unchanged crossing face");
+    check(std::fabs(unwrapped.x - 7.9) < 1e-9, "periodic: unwrapped trajectory continuous");
+    check(std::fabs(w.bodies[0].x.x) <= 5.0 + 1e-9, "periodic: stays in primary cell");
+}
+
+static World gas(bool periodic, bool brute) {   // selftest.cpp:166-181 shape, portable RNG
+    World w;
+    w.box.half = 8.0; w.box.periodic = periodic; w.restitution = 1.0; w.dt = 5e-3; w.forceBruteForce = brute;
+    Rng r(999);
+    for (int i = 0; i < 120; ++i) {
+        Body b = makeSphere(i, V3{r.uni(-6, 6), r.uni(-6, 6), r.uni(-6, 6)}, 0.7, 1.0);
+        b.v = V3{r.uni(-2, 2), r.uni(-2, 2), r.uni(-2, 2)};
+        b.w = V3{r.uni(-2, 2), r.uni(-2, 2), r.uni(-2, 2)};
+        w.bodies.push_back(b);
+    }
+    return w;
+}
+
+static bool same(const World& a, const World& b) {
+    for (size_t i = 0; i < a.bodies.size(); ++i) {
+        const Body &x = a.bodies[i], &y = b.bodies[i];
+        if (x.x.x != y.x.x || x.x.y != y.x.y || x.x.z != y.x.z || x.v.x != y.v.x || x.v.y != y.v.y ||
+            x.v.z != y.v.z || x.q.w != y.q.w || x.w.x != y.w.x)
+            return false;
+    }
+    return true;
+}
+
+static void determinism_and_copies() {   // selftest.cpp:183-197, 744-774, 223-240
+    World a = gas(true, false), b = gas(true, false);
+    for (int s = 0; s < 200; ++s) { a.step(); b.step(); }
+    check(same(a, b), "determinism: identical runs are bit-for-bit equal");
+    World g = gas(true, false), br = gas(true, true);
+    g.stepMany(300);
+    br.stepMany(300);
+    check(same(g, br), "broadphase: grid result is bit-identical to brute force");
+    // a copy carries the warm cache and then evolves like the original
+    World c = a;
+    for (int s = 0; s < 100; ++s) { a.step(); c.step(); }
+    check(same(a, c), "copy: a copied World evolves bit-identically to its source");
+    World e = gas(true, false);
+    V3 p0 = e.totalLinearMomentum(); double e0 = e.totalKinetic();
+    e.stepMany(2000);
+    check(closeV(e.totalLinearMomentum(), p0, 1e-6), "periodic gas: linear momentum conserved");
+    check(std::fabs(e.totalKinetic() - e0) / e0 < 0.02, "periodic gas: kinetic energy conserved to 2%");
+}
+
+static void cross_boundary() {   // selftest.cpp:248-267
+    auto run = [](bool periodic) {
+        World w;
+        w.dt = 1.0 / 240.0; w.restitution = 1.0; w.box.periodic = periodic; w.box.half = periodic ? 5.0 : 50.0;
+        Body a = makeSphere(0, periodic ? V3{4.4, 0, 0} : V3{-0.6, 0, 0}, 0.5, 1.0);
+        Body b = makeSphere(1, periodic ? V3{-4.4, 0, 0} : V3{0.6, 0, 0}, 0.5, 1.0);
+        a.v = V3{1, 0, 0}; b.v = V3{-1, 0, 0};
+        w.bodies = {a, b};
+        w.stepMany(400);
+        return w;
+    };
+    World fp = run(false), pr = run(true);
+    check(std::fabs(pr.bodies[0].v.x - fp.bodies[0].v.x) < 1e-9 && std::fabs(pr.bodies[1].v.x - fp.bodies[1].v.x) < 1e-9,
+          "periodic: cross-boundary collision equals free space");
+    check(std::fabs(pr.bodies[0].v.x + 1.0) < 1e-6 && std::fabs(pr.bodies[1].v.x - 1.0) < 1e-6,
+          "periodic: cross-boundary head-on exchanges velocity");
+}
+
+static void stacking() {   // selftest.cpp:663-697 on the World directly
+    World w;
+    w.box.half = 1e6; w.box.periodic = false; w.dt = 1.0 / 240.0; w.gravity = V3{0, -10, 0};
+    w.restitution = 0.0; w.friction = 0.6;
+    Body floor = makeBox(0, V3{0, -1, 0}, Q{1, 0, 0, 0}, V3{20, 1, 20}, 1.0);
+    floor.invMass = 0.0; floor.invInertiaBody = V3{0, 0, 0}; floor.dynamic = false;
+    w.bodies = {floor, makeBox(1, V3{0, 1.05, 0}, Q{1, 0, 0, 0}, V3{1, 1, 1}, 1.0),
+                makeBox(2, V3{0, 3.10, 0}, Q{1, 0, 0, 0}, V3{1, 1, 1}, 1.0)};
+    w.stepMany(4000);
+    double yl = w.bodies[1].x.y, yh = w.bodies[2].x.y;
+    check(yl > 0.85 && yl < 1.2, "stacking: lower box rests on floor (~y=1)");
+    check(yh > 2.7 && yh < 3.3, "stacking: upper box rests on lower (~y=3)");
+    check(w.bodies[2].v.norm() < 0.3, "stacking: stack is at rest");
+}
+
+// scene IN OUT K: IN holds n then per body 36 doubles (x3 v3 q4 w3 invMass
+// invI3 shape radius halfHeight he3 friction restitution layer mask flags
+// pad...), followed by the world parameters; OUT receives x v q w.
+static int run_scene(const char* in, const char* out, int k) {
+    std::ifstream f(in, std::ios::binary);
+    if (!f) return 2;
+    double hdr[24];
+    f.read(reinterpret_cast<char*>(hdr), sizeof hdr);
+    const size_t n = (size_t)hdr[0];
+    World w;
+    w.box.half = hdr[1]; w.box.halfXYZ = V3{hdr[1], hdr[2], hdr[3]}; w.box.periodic = hdr[4] != 0;
+    w.openBoundary = hdr[5] != 0; w.dt = hdr[6]; w.gravity = V3{hdr[7], hdr[8], hdr[9]};
+    w.restitution = hdr[10]; w.friction = hdr[11]; w.contactBeta = hdr[12];
+    w.restitutionCombine = static_cast<World::Combine>((int)hdr[13]);
+    w.frictionCombine = static_cast<World::Combine>((int)hdr[14]);
+    w.warmStart = hdr[15] != 0; w.solverIterations = (int)hdr[16];
+    std::vector<double> row(26);
+    for (size_t i = 0; i < n; ++i) {
+        f.read(reinterpret_cast<char*>(row.data()), sizeof(double) * row.size());
+        Body b;
+        b.x = V3{row[0], row[1], row[2]}; b.v = V3{row[3], row[4], row[5]};
+        b.q = Q{row[6], row[7], row[8], row[9]}; b.w = V3{row[10], row[11], row[12]};
+        b.invMass = row[13]; b.invInertiaBody = V3{row[14], row[15], row[16]};
+        b.shape = static_cast<Shape>((int)row[17]); b.radius = row[18]; b.halfHeight = row[19];
+        b.halfExtents = V3{row[20], row[21], row[22]}; b.friction = row[23]; b.restitution = row[24];
+        int nv = (int)row[25];
+        for (int t = 0; t < nv; ++t) {
+            double v[3];
+            f.read(reinterpret_cast<char*>(v), sizeof v);
+            b.vertices.push_back(V3{v[0], v[1], v[2]});
+        }
+        w.bodies.push_back(b);
+    }
+    for (int s = 0; s < k; ++s) w.step();   // compat mode: upload, step, download each time
+    std::ofstream o(out, std::ios::binary);
+    for (const Body& b : w.bodies) {
+        double r[13] = {b.x.x, b.x.y, b.x.z, b.v.x, b.v.y, b.v.z, b.q.w, b.q.x, b.q.y, b.q.z, b.w.x, b.w.y, b.w.z};
+        o.write(reinterpret_cast<const char*>(r), sizeof r);
+    }
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    try {
+        if (argc >= 5 && std::strcmp(argv[1], "scene") == 0) return run_scene(argv[2], argv[3], std::atoi(argv[4]));
+        std::printf("=== ne::World drop-in on the GPU ===\n");
+        headon();
+        transit();
+        determinism_and_copies();
+        cross_boundary();
+        stacking();
+        std::printf("failures: %d\n", g_fail);
+        return g_fail == 0 ? 0 : 1;
+    } catch (const std::exception& e) {
+        std::printf("exception: %s\n", e.what());
+        return 3;
+    }
+}
