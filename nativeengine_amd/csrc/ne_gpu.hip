@@ -426,6 +426,174 @@ __global__ void __launch_bounds__(TPB) k_cand_walk(BodiesDev B, SimParams P, con
     row_count[i] = cnt;
 }
 
+// LDS-staged neighbourhood walk.  One workgroup per brick of BX x BY x BZ
+// cells stages the bodies of the brick and its one-cell halo into LDS (one
+// coalesced pass over the cell-sorted array) and walks every interior body's
+// 27-cell neighbourhood from LDS.  Emits exactly what k_cand_walk emits.  Used
+// when every axis has >= brick + 2 cells (so halo cells never alias);
+// a brick whose halo holds more bodies than the LDS pool walks from global
+// memory instead.
+constexpr int BRICK_X = 8, BRICK_Y = 8, BRICK_Z = 4;
+constexpr int HALO_X = BRICK_X + 2, HALO_Y = BRICK_Y + 2, HALO_Z = BRICK_Z + 2;
+constexpr int HALO_CELLS = HALO_X * HALO_Y * HALO_Z;   // 600
+constexpr int BRICK_POOL = 1024;                       // bodies staged per brick
+constexpr int BRICK_THREADS = 256;
+
+__global__ void __launch_bounds__(BRICK_THREADS) k_cand_brick(BodiesDev B, SimParams P, const GridDev* __restrict__ Gp,
+                                                              const uint32_t* __restrict__ sorted_idx,
+                                                              const SortedBody* __restrict__ sorted,
+                                                              const uint32_t* __restrict__ cell_end,
+                                                              const uint32_t* __restrict__ excl, uint32_t nexcl,
+                                                              uint32_t* __restrict__ row_count,
+                                                              uint32_t* __restrict__ row_scratch) {
+    __shared__ uint32_t cbeg[HALO_CELLS];
+    __shared__ uint32_t coff[HALO_CELLS + 1];
+    __shared__ SortedBody pool[BRICK_POOL];
+    __shared__ uint32_t pidx[BRICK_POOL];
+    __shared__ uint16_t inner[BRICK_POOL];
+    __shared__ uint8_t cint[HALO_CELLS];
+    __shared__ uint32_t scan_tmp[PRIM_WAVES + 1];
+    __shared__ uint32_t n_inner;
+    const GridDev g = *Gp;
+    const int nbx = (g.nc[0] + BRICK_X - 1) / BRICK_X, nby = (g.nc[1] + BRICK_Y - 1) / BRICK_Y,
+              nbz = (g.nc[2] + BRICK_Z - 1) / BRICK_Z;
+    const uint32_t nbricks = (uint32_t)nbx * (uint32_t)nby * (uint32_t)nbz;
+    // XCD-aware order: blocks are dealt round-robin to the 8 XCDs; give each
+    // XCD a contiguous run of bricks so neighbouring halos share its L2.
+    const uint32_t per_xcd = (nbricks + 7u) / 8u;
+    for (uint32_t blk = blockIdx.x; blk < per_xcd * 8u; blk += gridDim.x) {
+        const uint32_t brick = (blk & 7u) * per_xcd + (blk >> 3);
+        if (brick >= nbricks) continue;   // uniform per block
+        const int bx = (int)(brick % (uint32_t)nbx), by = (int)((brick / (uint32_t)nbx) % (uint32_t)nby),
+                  bz = (int)(brick / ((uint32_t)nbx * (uint32_t)nby));
+        // 1. halo cell ranges.  Periodic: every halo position wraps to a
+        // distinct cell (nc >= brick + 2); positions of a partial brick past
+        // the grid edge are the wrapped neighbours, not interior cells.
+        for (int hc = threadIdx.x; hc < HALO_CELLS; hc += BRICK_THREADS) {
+            const int hx = hc % HALO_X, hy = (hc / HALO_X) % HALO_Y, hz = hc / (HALO_X * HALO_Y);
+            const int ux = bx * BRICK_X + hx - 1, uy = by * BRICK_Y + hy - 1, uz = bz * BRICK_Z + hz - 1;
+            int cx = ux, cy = uy, cz = uz;
+            bool ok = true;
+            if (g.periodic) {
+                cx = (cx + g.nc[0]) % g.nc[0];
+                cy = (cy + g.nc[1]) % g.nc[1];
+                cz = (cz + g.nc[2]) % g.nc[2];
+            } else {
+                ok = cx >= 0 && cx < g.nc[0] && cy >= 0 && cy < g.nc[1] && cz >= 0 && cz < g.nc[2];
+            }
+            cint[hc] = (hx >= 1 && hx <= BRICK_X && hy >= 1 && hy <= BRICK_Y && hz >= 1 && hz <= BRICK_Z &&
+                        ux < g.nc[0] && uy < g.nc[1] && uz < g.nc[2]) ? 1 : 0;
+            uint32_t b0 = 0, b1 = 0;
+            if (ok) {
+                const uint32_t cell = ((uint32_t)cz * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[0] + (uint32_t)cx;
+                b0 = cell ? cell_end[cell - 1] : 0u;
+                b1 = cell_end[cell];
+            }
+            cbeg[hc] = b0;
+            coff[hc] = b1 - b0;   // count, scanned below
+        }
+        if (threadIdx.x == 0) n_inner = 0;
+        __syncthreads();
+        // 2. exclusive scan of the 600 counts (3 per thread, in order)
+        {
+            constexpr int PER = (HALO_CELLS + BRICK_THREADS - 1) / BRICK_THREADS;   // 3
+            uint32_t loc[PER];
+            uint32_t sum = 0;
+            for (int k = 0; k < PER; ++k) {
+                const int hc = threadIdx.x * PER + k;
+                loc[k] = hc < HALO_CELLS ? coff[hc] : 0u;
+                sum += loc[k];
+            }
+            uint32_t tot;
+            uint32_t ex = block_exclusive_scan(sum, &tot, scan_tmp);
+            for (int k = 0; k < PER; ++k) {
+                const int hc = threadIdx.x * PER + k;
+                if (hc < HALO_CELLS) coff[hc] = ex;
+                ex += loc[k];
+            }
+            if (threadIdx.x == 0) coff[HALO_CELLS] = tot;
+            __syncthreads();
+        }
+        const uint32_t H = coff[HALO_CELLS];
+        const bool staged = H <= (uint32_t)BRICK_POOL;
+        // 3. stage halo bodies (consecutive slots of a cell are consecutive in memory)
+        if (staged) {
+            for (uint32_t h = threadIdx.x; h < H; h += BRICK_THREADS) {
+                int lo = 0, hi = HALO_CELLS - 1;   // last cell with coff <= h
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (coff[mid] <= h) lo = mid; else hi = mid - 1;
+                }
+                const uint32_t src = cbeg[lo] + (h - coff[lo]);
+                pool[h] = sorted[src];
+                pidx[h] = sorted_idx[src];
+                if (cint[lo]) inner[atomicAdd(&n_inner, 1u)] = (uint16_t)h;
+            }
+        }
+        __syncthreads();
+        if (staged) {
+            // 4. walk every interior body's neighbourhood inside LDS
+            const uint32_t ni = n_inner;
+            for (uint32_t t = threadIdx.x; t < ni; t += BRICK_THREADS) {
+                const uint32_t h = inner[t];
+                int lo = 0, hi = HALO_CELLS - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (coff[mid] <= h) lo = mid; else hi = mid - 1;
+                }
+                const int hx = lo % HALO_X, hy = (lo / HALO_X) % HALO_Y, hz = lo / (HALO_X * HALO_Y);
+                const uint32_t i = pidx[h];
+                const SortedBody me = pool[h];
+                const V3 xi{me.x, me.y, me.z};
+                uint32_t cnt = 0;
+                uint32_t* row = row_scratch + (size_t)i * ROW_SLOTS;
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int c0 = ((hz + dz) * HALO_Y + (hy + dy)) * HALO_X + (hx - 1);
+                        const uint32_t u0 = coff[c0], u1 = coff[c0 + 3];   // 3 consecutive x-cells
+                        for (uint32_t u = u0; u < u1; ++u) {
+                            const uint32_t j = pidx[u];
+                            if (j <= i) continue;
+                            const SortedBody o = pool[u];
+                            if (!bounds_touch(P.box, xi, me.r, V3{o.x, o.y, o.z}, o.r)) continue;
+                            if (!pair_passes(B, i, j)) continue;
+                            if (cnt < ROW_SLOTS) row[cnt] = j;
+                            ++cnt;
+                        }
+                    }
+                for (uint32_t e = 0; e < nexcl; ++e) {
+                    const uint32_t j = excl[e];
+                    if (j <= i) continue;
+                    if (!bounds_touch(P.box, xi, me.r, ld3(B.x0, B.x1, B.x2, j), B.bR[j])) continue;
+                    if (!pair_passes(B, i, j)) continue;
+                    if (cnt < ROW_SLOTS) row[cnt] = j;
+                    ++cnt;
+                }
+                row_count[i] = cnt;
+            }
+        } else {
+            // crowded brick: walk its interior bodies from global memory
+            for (int hc = 0; hc < HALO_CELLS; ++hc) {
+                if (!cint[hc]) continue;
+                const uint32_t b0 = cbeg[hc], nb = coff[hc + 1] - coff[hc];
+                for (uint32_t t = threadIdx.x; t < nb; t += BRICK_THREADS) {
+                    const uint32_t sidx = b0 + t;
+                    const uint32_t i = sorted_idx[sidx];
+                    uint32_t cnt = 0;
+                    uint32_t* row = row_scratch + (size_t)i * ROW_SLOTS;
+                    for_each_candidate(B, P, g, i, sorted[sidx], sorted_idx, sorted, cell_end, excl, nexcl,
+                                       [&](uint32_t j) {
+                                           if (cnt < ROW_SLOTS) row[cnt] = j;
+                                           ++cnt;
+                                       });
+                    row_count[i] = cnt;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Writes each gridded body's row at its offset in (i, j) order: from the
 // scratch when it fit, else by walking the neighbourhood again; then sorts
 // the row by j (rows are short).
@@ -1114,6 +1282,7 @@ struct ne_gpu_ctx {
     uint32_t *slot_of = nullptr, *row_scratch = nullptr, *claimed = nullptr, *cons_isl = nullptr;
     uint32_t *isl_count = nullptr, *isl_fill = nullptr, *isl_list = nullptr, *big_list = nullptr;
     StaticGrid sgrid{};
+    bool use_bricks = true;
     GridDev grid_host{};
     uint32_t sgrid_cells = 0;
     // counters
@@ -1371,8 +1540,19 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
         }
         // --- candidate rows: one neighbourhood walk per body
         KTimer kt(c, NE_KERNEL_CAND_COUNT);
-        k_cand_walk<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, P, c->grid, c->ngrid, c->sorted_idx, c->sorted,
-                                                         c->cell_start, c->excl, c->nexcl, c->row_count, c->row_scratch);
+        if (static_grid && c->use_bricks && c->sgrid.nc[0] >= BRICK_X + 2 && c->sgrid.nc[1] >= BRICK_Y + 2 &&
+            c->sgrid.nc[2] >= BRICK_Z + 2) {
+            const uint32_t nb = (uint32_t)((c->sgrid.nc[0] + BRICK_X - 1) / BRICK_X) *
+                                (uint32_t)((c->sgrid.nc[1] + BRICK_Y - 1) / BRICK_Y) *
+                                (uint32_t)((c->sgrid.nc[2] + BRICK_Z - 1) / BRICK_Z);
+            const uint32_t blocks = ((nb + 7u) / 8u) * 8u;
+            k_cand_brick<<<blocks, BRICK_THREADS, 0, s>>>(B, P, c->grid, c->sorted_idx, c->sorted, c->cell_start,
+                                                          c->excl, c->nexcl, c->row_count, c->row_scratch);
+        } else {
+            k_cand_walk<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, P, c->grid, c->ngrid, c->sorted_idx, c->sorted,
+                                                             c->cell_start, c->excl, c->nexcl, c->row_count,
+                                                             c->row_scratch);
+        }
     }
     if (c->nexcl > 0) {
         dim3 g(blocks_for(n), c->nexcl);
