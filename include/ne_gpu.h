@@ -120,8 +120,16 @@ typedef struct ne_gpu_config {
     int32_t  device;            /* HIP device ordinal                         */
     uint32_t pair_capacity;     /* candidate pairs per step (default 8*n+4096) */
     uint32_t contact_capacity;  /* constraints per step (default 4*n+4096)    */
-    double   cell_scale;        /* grid cell = cell_scale * 2*maxR (>=1, default autotuned) */
+    double   cell_scale;        /* grid cell >= cell_scale * 2*maxR (>= 1; 0 = 1) */
+    int32_t  broadphase;        /* NE_BROADPHASE_*: the neighbourhood walk variant */
+    uint32_t cells_per_body;    /* cell-table capacity per body (0 = 8)       */
 } ne_gpu_config;
+
+/* Broadphase walk variants (same candidate rows; a tuning knob). */
+#define NE_BROADPHASE_AUTO   0
+#define NE_BROADPHASE_WALK   1   /* one thread per body walks 27 cells in global memory */
+#define NE_BROADPHASE_BRICK  2   /* one workgroup per 8x8x4-cell brick, halo staged in LDS */
+#define NE_BROADPHASE_CELLS  3   /* one thread per cell (coalesced neighbour rows); AUTO picks this */
 
 typedef struct ne_gpu_ctx ne_gpu_ctx;
 

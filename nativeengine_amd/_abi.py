@@ -83,7 +83,11 @@ class GpuConfig(C.Structure):
         ("pair_capacity", C.c_uint32),
         ("contact_capacity", C.c_uint32),
         ("cell_scale", C.c_double),
+        ("broadphase", C.c_int32),
+        ("cells_per_body", C.c_uint32),
     ]
+
+BROADPHASE_AUTO, BROADPHASE_WALK, BROADPHASE_BRICK, BROADPHASE_CELLS = 0, 1, 2, 3
 
 
 def dptr(a):

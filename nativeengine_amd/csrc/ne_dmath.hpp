@@ -69,7 +69,8 @@ NE_HD Qt integrate_orientation(Qt q, V3 om, double dt) {
 // Domain box with per-axis half extents (a cube when all three are equal).
 struct Box3 {
     double half[3];
-    double edge[3];   // 2*half
+    double edge[3];       // 2*half
+    double inv_edge[3];   // 1/edge: conservative candidate tests only, never the exact path
     int periodic;
 };
 

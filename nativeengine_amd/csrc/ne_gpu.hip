@@ -69,6 +69,7 @@ struct SimParams {
 struct GridDev {
     double origin[3];
     double cw[3];
+    double inv_cw[3];   // cell coordinates use x * (1/cw): binning and lookup share one rule
     int32_t nc[3];
     uint32_t ncells;
     int32_t periodic;
@@ -119,8 +120,16 @@ __device__ __forceinline__ Geom load_geom(const BodiesDev& B, uint32_t i) {
 
 constexpr uint32_t FLAG_GRID_EXCLUDED = 1u << 30;   // internal: body kept out of the grid
 
+__device__ __forceinline__ bool same_bits(double a, double b) {
+    return __double_as_longlong(a) == __double_as_longlong(b);
+}
+__device__ __forceinline__ bool same_bits(V3 a, V3 b) { return same_bits(a.x, b.x) && same_bits(a.y, b.y) && same_bits(a.z, b.z); }
+__device__ __forceinline__ bool same_bits(Qt a, Qt b) {
+    return same_bits(a.w, b.w) && same_bits(a.x, b.x) && same_bits(a.y, b.y) && same_bits(a.z, b.z);
+}
+
 struct StaticGrid {   // periodic worlds: the grid never changes between steps
-    double origin[3], cw[3];
+    double origin[3], cw[3], inv_cw[3];
     int32_t nc[3];
 };
 
@@ -130,11 +139,18 @@ __device__ __forceinline__ uint32_t static_cell(const StaticGrid& g, V3 x);
 // step's periodic wrap (world.cpp:698-702, same per-body arithmetic, moved to
 // the start of the next step) and followed by binning the body into the
 // static periodic grid.
+struct SortedBody {
+    double x, y, z, r;
+};
+
 template <bool FORCES, bool WRAP, bool BIN>
-__global__ void __launch_bounds__(TPB) k_integrate(BodiesDev B, SimParams P, StaticGrid G, uint32_t* __restrict__ cell_of,
-                                                   uint32_t* __restrict__ cell_count) {
+__global__ void __launch_bounds__(TPB) k_integrate(BodiesDev B, SimParams P, StaticGrid G, SortedBody* __restrict__ xp,
+                                                   uint32_t* __restrict__ cell_count, uint32_t* __restrict__ parent,
+                                                   uint32_t* __restrict__ claimed) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.n) return;
+    parent[i] = i;      // this step's union-find forest starts as singletons
+    claimed[i] = 0u;
     const double dt = P.dt;
     const uint32_t fl = B.flags[i];
     const double im = B.im[i];
@@ -151,6 +167,8 @@ __global__ void __launch_bounds__(TPB) k_integrate(BodiesDev B, SimParams P, Sta
     } else if (!(im <= 0.0)) {
         V3 v = ld3(B.v0, B.v1, B.v2, i), w = ld3(B.w0, B.w1, B.w2, i);
         Qt q = ldq(B, i);
+        const V3 w0 = w;
+        const Qt q0 = q;
         V3 invI = ld3(B.ii0, B.ii1, B.ii2, i);
         const V3 g{P.g[0], P.g[1], P.g[2]};
         v = v + (g * B.gs[i] + f * im) * dt;
@@ -164,8 +182,9 @@ __global__ void __launch_bounds__(TPB) k_integrate(BodiesDev B, SimParams P, Sta
         if (ad > 0.0) w = w * (1.0 / (1.0 + ad * dt));
         st3(B.x0, B.x1, B.x2, i, x);
         st3(B.v0, B.v1, B.v2, i, v);
-        st3(B.w0, B.w1, B.w2, i, w);
-        B.q0[i] = q.w; B.q1[i] = q.x; B.q2[i] = q.y; B.q3[i] = q.z;
+        // a body that does not turn keeps q and w bit for bit: skip those stores
+        if (!same_bits(w, w0)) st3(B.w0, B.w1, B.w2, i, w);
+        if (!same_bits(q, q0)) { B.q0[i] = q.w; B.q1[i] = q.x; B.q2[i] = q.y; B.q3[i] = q.z; }
     } else if (WRAP) {
         // static body: unchanged (wrap only applies to invMass > 0)
     }
@@ -173,11 +192,8 @@ __global__ void __launch_bounds__(TPB) k_integrate(BodiesDev B, SimParams P, Sta
         st3(B.f0, B.f1, B.f2, i, V3{0.0, 0.0, 0.0});
         st3(B.t0, B.t1, B.t2, i, V3{0.0, 0.0, 0.0});
     }
-    if (BIN && !(fl & FLAG_GRID_EXCLUDED)) {
-        const uint32_t c = static_cell(G, x);
-        cell_of[i] = c;
-        atomicAdd(&cell_count[c], 1u);
-    }
+    xp[i] = SortedBody{x.x, x.y, x.z, B.bR[i]};   // packed pose for the grid fill
+    if (BIN && !(fl & FLAG_GRID_EXCLUDED)) atomicAdd(&cell_count[static_cell(G, x)], 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -274,15 +290,23 @@ __global__ void k_grid_setup(SimParams P, double cell, uint32_t cell_cap, const 
         for (int a = 0; a < 3; ++a) { g.nc[a] = 1; g.cw[a] = 1.0; g.origin[a] = 0.0; }
     }
     g.ncells = (uint32_t)g.nc[0] * (uint32_t)g.nc[1] * (uint32_t)g.nc[2];
+    for (int a = 0; a < 3; ++a) g.inv_cw[a] = 1.0 / g.cw[a];
     *G = g;
     counters[CNT_CELLS] = g.ncells;
 }
 
-// Cell coordinate along axis a, always in [0, nc-1] (NaN maps to 0).
+// Cell coordinate along axis a, always in [0, nc-1] (NaN maps to 0).  Every
+// binning and every lookup go through this one function, so a body's cell is
+// consistent; cells are wider than 2*maxR*(1+1e-6), which absorbs the
+// rounding of the reciprocal (a touching pair is always <= 1 cell apart).
 __device__ __forceinline__ int32_t cell_coord(const GridDev& g, int a, double c) {
-    double k = floor((c - g.origin[a]) / g.cw[a]);
+    double k = floor((c - g.origin[a]) * g.inv_cw[a]);
     const double m = (double)g.nc[a];
-    if (g.periodic) k = k - m * floor(k / m);
+    if (g.periodic && (k < 0.0 || k >= m)) {   // bodies within one box length: one add/subtract
+        if (k < 0.0 && k >= -m) k += m;
+        else if (k >= m && k < 2.0 * m) k -= m;
+        else k = k - m * floor(k / m);
+    }
     if (!(k >= 0.0)) k = 0.0;
     if (k > m - 1.0) k = m - 1.0;
     return (int32_t)k;
@@ -290,42 +314,45 @@ __device__ __forceinline__ int32_t cell_coord(const GridDev& g, int a, double c)
 
 __device__ __forceinline__ uint32_t static_cell(const StaticGrid& sg, V3 x) {
     GridDev g;
-    for (int a = 0; a < 3; ++a) { g.origin[a] = sg.origin[a]; g.cw[a] = sg.cw[a]; g.nc[a] = sg.nc[a]; }
+    for (int a = 0; a < 3; ++a) {
+        g.origin[a] = sg.origin[a]; g.cw[a] = sg.cw[a]; g.inv_cw[a] = sg.inv_cw[a]; g.nc[a] = sg.nc[a];
+    }
     g.periodic = 1;
     const int32_t cx = cell_coord(g, 0, x.x), cy = cell_coord(g, 1, x.y), cz = cell_coord(g, 2, x.z);
     return ((uint32_t)cz * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[0] + (uint32_t)cx;
 }
 
-__global__ void __launch_bounds__(TPB) k_cell_count(BodiesDev B, const uint32_t* __restrict__ gridded, uint32_t ngrid,
-                                                    const GridDev* __restrict__ Gp, uint32_t* __restrict__ cell_of,
+__global__ void __launch_bounds__(TPB) k_cell_count(const uint32_t* __restrict__ gridded, uint32_t ngrid,
+                                                    const GridDev* __restrict__ Gp, const SortedBody* __restrict__ xp,
                                                     uint32_t* __restrict__ cell_count) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ngrid) return;
     const GridDev g = *Gp;
-    const uint32_t i = gridded[t];
-    int32_t cx = cell_coord(g, 0, B.x0[i]), cy = cell_coord(g, 1, B.x1[i]), cz = cell_coord(g, 2, B.x2[i]);
-    uint32_t c = ((uint32_t)cz * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[0] + (uint32_t)cx;
-    cell_of[i] = c;
+    const SortedBody p = xp[gridded[t]];
+    const uint32_t c = ((uint32_t)cell_coord(g, 2, p.z) * (uint32_t)g.nc[1] + (uint32_t)cell_coord(g, 1, p.y)) *
+                           (uint32_t)g.nc[0] + (uint32_t)cell_coord(g, 0, p.x);
     atomicAdd(&cell_count[c], 1u);
 }
 
-struct SortedBody {
-    double x, y, z, r;
-};
-
 // cell_start holds exclusive offsets; the atomic bump leaves cell_start[c] at
 // the END of cell c, so begin(c) = cell_start[c-1] afterwards.
-__global__ void __launch_bounds__(TPB) k_cell_fill(BodiesDev B, const uint32_t* __restrict__ gridded, uint32_t ngrid,
-                                                   const uint32_t* __restrict__ cell_of, uint32_t* __restrict__ cell_start,
-                                                   uint32_t* __restrict__ sorted_idx, SortedBody* __restrict__ sorted,
-                                                   uint32_t* __restrict__ slot_of) {
+// Bodies are visited in the previous step's cell order (bodies move little
+// per step), so the slot bumps and the sorted-array writes of a wave land
+// close together instead of scattering across the whole table.
+__global__ void __launch_bounds__(TPB) k_cell_fill(const uint32_t* __restrict__ order, uint32_t ngrid,
+                                                   const GridDev* __restrict__ Gp, const SortedBody* __restrict__ xp,
+                                                   uint32_t* __restrict__ cell_start, uint32_t* __restrict__ sorted_idx,
+                                                   SortedBody* __restrict__ sorted) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ngrid) return;
-    const uint32_t i = gridded[t];
-    uint32_t slot = atomicAdd(&cell_start[cell_of[i]], 1u);
+    const GridDev g = *Gp;
+    const uint32_t i = order[t];
+    const SortedBody p = xp[i];
+    const uint32_t c = ((uint32_t)cell_coord(g, 2, p.z) * (uint32_t)g.nc[1] + (uint32_t)cell_coord(g, 1, p.y)) *
+                           (uint32_t)g.nc[0] + (uint32_t)cell_coord(g, 0, p.x);
+    const uint32_t slot = atomicAdd(&cell_start[c], 1u);
     sorted_idx[slot] = i;
-    slot_of[i] = slot;
-    sorted[slot] = SortedBody{B.x0[i], B.x1[i], B.x2[i], B.bR[i]};
+    sorted[slot] = p;
 }
 
 // Pair filters of src/world.cpp:513-516 (world.hpp:94-103).
@@ -339,9 +366,17 @@ __device__ __forceinline__ bool pair_passes(const BodiesDev& B, uint32_t i, uint
     return true;
 }
 
-// Conservative bounding-sphere overlap under the minimum image.
+// Conservative bounding-sphere overlap under the minimum image.  Only decides
+// which pairs reach the narrow phase (which recomputes the exact minimum
+// image), so it may use 1/L: a rounding-chosen image differs from the exact
+// one only for |d| ~ L/2, far from touching.
 __device__ __forceinline__ bool bounds_touch(const Box3& box, V3 a, double ra, V3 b, double rb) {
-    V3 d = min_image(box, a - b);
+    V3 d = a - b;
+    if (box.periodic) {
+        d.x -= box.edge[0] * floor(d.x * box.inv_edge[0] + 0.5);
+        d.y -= box.edge[1] * floor(d.y * box.inv_edge[1] + 0.5);
+        d.z -= box.edge[2] * floor(d.z * box.inv_edge[2] + 0.5);
+    }
     double s = (ra + rb) * (1.0 + 1e-9) + 1e-9;
     return norm2(d) <= s * s;
 }
@@ -364,9 +399,32 @@ __device__ __forceinline__ int axis_neighbours(const GridDev& g, int a, int32_t 
     return k;
 }
 
-// Visits every candidate partner j > i of gridded body i (sorted slot s):
-// bodies in the 27-cell neighbourhood and the grid-excluded bodies, keeping
-// pairs whose bounding spheres overlap and that pass the reference's filters.
+// Neighbour coordinates along one axis: v[d] for offset d-1, ok[d] false
+// when outside a non-periodic grid or a repeat of an earlier offset (fewer
+// than 3 cells along a periodic axis).
+__device__ __forceinline__ void axis_offsets(const GridDev& g, int a, int32_t c, int32_t v[3], bool ok[3]) {
+    const int32_t n = g.nc[a];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int32_t x = c + d - 1;   // in [-1, n]
+        bool good = true;
+        if (g.periodic) x = x < 0 ? x + n : (x >= n ? x - n : x);
+        else good = x >= 0 && x < n;
+        v[d] = x;
+        ok[d] = good;
+    }
+    if (g.periodic) {
+        ok[1] = ok[1] && v[1] != v[0];
+        ok[2] = ok[2] && v[2] != v[0] && v[2] != v[1];
+    }
+}
+
+// Visits every candidate partner j > i of gridded body i: bodies in the
+// 27-cell neighbourhood and the grid-excluded bodies, keeping pairs whose
+// bounding spheres overlap and that pass the reference's filters.  All
+// neighbour-range bounds are loaded before any range is walked (the walk is
+// latency-bound); when the three x-neighbours are consecutive cells a row is
+// one range.
 template <class F>
 __device__ __forceinline__ void for_each_candidate(const BodiesDev& B, const SimParams& P, const GridDev& g,
                                                    uint32_t i, const SortedBody& me,
@@ -375,24 +433,64 @@ __device__ __forceinline__ void for_each_candidate(const BodiesDev& B, const Sim
                                                    const uint32_t* __restrict__ cell_end,
                                                    const uint32_t* __restrict__ excl, uint32_t nexcl, F&& emit) {
     const V3 xi{me.x, me.y, me.z};
-    int32_t nx[3], ny[3], nz[3];
-    const int kx = axis_neighbours(g, 0, cell_coord(g, 0, me.x), nx);
-    const int ky = axis_neighbours(g, 1, cell_coord(g, 1, me.y), ny);
-    const int kz = axis_neighbours(g, 2, cell_coord(g, 2, me.z), nz);
-    for (int c = 0; c < kz; ++c)
-        for (int b = 0; b < ky; ++b)
-            for (int a = 0; a < kx; ++a) {
-                const uint32_t cell = ((uint32_t)nz[c] * (uint32_t)g.nc[1] + (uint32_t)ny[b]) * (uint32_t)g.nc[0] + (uint32_t)nx[a];
-                const uint32_t beg = cell ? cell_end[cell - 1] : 0u, end = cell_end[cell];
-                for (uint32_t t = beg; t < end; ++t) {
-                    const uint32_t j = sorted_idx[t];
-                    if (j <= i) continue;
-                    const SortedBody o = sorted[t];
-                    if (!bounds_touch(P.box, xi, me.r, V3{o.x, o.y, o.z}, o.r)) continue;
-                    if (!pair_passes(B, i, j)) continue;
-                    emit(j);
-                }
+    int32_t vx[3], vy[3], vz[3];
+    bool ox[3], oy[3], oz[3];
+    const int32_t cx = cell_coord(g, 0, me.x);
+    axis_offsets(g, 0, cx, vx, ox);
+    axis_offsets(g, 1, cell_coord(g, 1, me.y), vy, oy);
+    axis_offsets(g, 2, cell_coord(g, 2, me.z), vz, oz);
+    const uint32_t nc0 = (uint32_t)g.nc[0], nc1 = (uint32_t)g.nc[1];
+    auto walk = [&](uint32_t beg, uint32_t end) {
+        for (uint32_t t = beg; t < end; ++t) {
+            const uint32_t j = sorted_idx[t];
+            const SortedBody o = sorted[t];
+            if (j <= i) continue;
+            if (!bounds_touch(P.box, xi, me.r, V3{o.x, o.y, o.z}, o.r)) continue;
+            if (!pair_passes(B, i, j)) continue;
+            emit(j);
+        }
+    };
+    if (cx >= 1 && cx + 1 < g.nc[0]) {
+        // x-interior cell: each (y, z) row of three cells is one contiguous
+        // range; the nine ranges are walked as one flattened sequence so a
+        // wave waits for one load per candidate, not one per row.
+        uint32_t rb[9], rl[9];
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            const int b = r % 3, c = r / 3;
+            const bool ok = oy[b] && oz[c];
+            const uint32_t c0 = ((uint32_t)vz[c] * nc1 + (uint32_t)vy[b]) * nc0 + (uint32_t)(cx - 1);
+            const uint32_t e = ok ? cell_end[c0 + 2] : 0u;
+            const uint32_t bgn = (ok && c0 > 0) ? cell_end[c0 - 1] : 0u;
+            rb[r] = bgn;
+            rl[r] = ok ? e - bgn : 0u;
+        }
+        uint32_t total = 0;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) total += rl[r];
+        for (uint32_t u = 0; u < total; ++u) {
+            uint32_t off = u, t = 0;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {        // select the row holding element u
+                const bool here = off < rl[r] && t == 0;
+                t = here ? rb[r] + off + 1 : t;  // +1 keeps 0 as "not found"
+                off = (t == 0) ? off - rl[r] : off;
             }
+            walk(t - 1, t);
+        }
+    } else {
+        uint32_t cb[27], ce[27];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+            const int a = k % 3, b = (k / 3) % 3, c = k / 9;
+            const bool ok = ox[a] && oy[b] && oz[c];
+            const uint32_t cell = ((uint32_t)vz[c] * nc1 + (uint32_t)vy[b]) * nc0 + (uint32_t)vx[a];
+            cb[k] = (ok && cell) ? cell_end[cell - 1] : 0u;
+            ce[k] = ok ? cell_end[cell] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 27; ++k) walk(cb[k], ce[k]);
+    }
     for (uint32_t e = 0; e < nexcl; ++e) {
         const uint32_t j = excl[e];
         if (j <= i) continue;
@@ -411,7 +509,8 @@ __global__ void __launch_bounds__(TPB) k_cand_walk(BodiesDev B, SimParams P, con
                                                    const SortedBody* __restrict__ sorted,
                                                    const uint32_t* __restrict__ cell_end,
                                                    const uint32_t* __restrict__ excl, uint32_t nexcl,
-                                                   uint32_t* __restrict__ row_count, uint32_t* __restrict__ row_scratch) {
+                                                   uint32_t* __restrict__ row_count, uint32_t* __restrict__ row_scratch,
+                                                   uint32_t* __restrict__ slot_of) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ngrid) return;
     const GridDev g = *Gp;
@@ -424,6 +523,7 @@ __global__ void __launch_bounds__(TPB) k_cand_walk(BodiesDev B, SimParams P, con
         ++cnt;
     });
     row_count[i] = cnt;
+    if (cnt > ROW_SLOTS) slot_of[i] = s;   // the emit kernel walks this row again
 }
 
 // LDS-staged neighbourhood walk.  One workgroup per brick of BX x BY x BZ
@@ -445,7 +545,8 @@ __global__ void __launch_bounds__(BRICK_THREADS) k_cand_brick(BodiesDev B, SimPa
                                                               const uint32_t* __restrict__ cell_end,
                                                               const uint32_t* __restrict__ excl, uint32_t nexcl,
                                                               uint32_t* __restrict__ row_count,
-                                                              uint32_t* __restrict__ row_scratch) {
+                                                              uint32_t* __restrict__ row_scratch,
+                                                              uint32_t* __restrict__ slot_of) {
     __shared__ uint32_t cbeg[HALO_CELLS];
     __shared__ uint32_t coff[HALO_CELLS + 1];
     __shared__ SortedBody pool[BRICK_POOL];
@@ -570,6 +671,14 @@ __global__ void __launch_bounds__(BRICK_THREADS) k_cand_brick(BodiesDev B, SimPa
                     ++cnt;
                 }
                 row_count[i] = cnt;
+                if (cnt > ROW_SLOTS) {   // the emit kernel walks this row again from its sorted slot
+                    int lo2 = 0, hi2 = HALO_CELLS - 1;
+                    while (lo2 < hi2) {
+                        const int mid = (lo2 + hi2 + 1) >> 1;
+                        if (coff[mid] <= h) lo2 = mid; else hi2 = mid - 1;
+                    }
+                    slot_of[i] = cbeg[lo2] + (h - coff[lo2]);
+                }
             }
         } else {
             // crowded brick: walk its interior bodies from global memory
@@ -587,10 +696,40 @@ __global__ void __launch_bounds__(BRICK_THREADS) k_cand_brick(BodiesDev B, SimPa
                                            ++cnt;
                                        });
                     row_count[i] = cnt;
+                    if (cnt > ROW_SLOTS) slot_of[i] = sidx;
                 }
             }
         }
         __syncthreads();
+    }
+}
+
+// Cell-major walk: one thread per grid cell (consecutive lanes = consecutive
+// cells along x), so the neighbour-row bounds a wave loads are contiguous and
+// the loads coalesce.  Each thread walks the neighbourhood for every body in
+// its cell; the rows it produces are exactly k_cand_walk's.
+__global__ void __launch_bounds__(TPB) k_cand_cells(BodiesDev B, SimParams P, const GridDev* __restrict__ Gp,
+                                                    const uint32_t* __restrict__ sorted_idx,
+                                                    const SortedBody* __restrict__ sorted,
+                                                    const uint32_t* __restrict__ cell_end,
+                                                    const uint32_t* __restrict__ excl, uint32_t nexcl,
+                                                    uint32_t* __restrict__ row_count, uint32_t* __restrict__ row_scratch,
+                                                    uint32_t* __restrict__ slot_of) {
+    const GridDev g = *Gp;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < g.ncells; c += gridDim.x * blockDim.x) {
+        const uint32_t cb = c ? cell_end[c - 1] : 0u, ce = cell_end[c];
+        if (cb == ce) continue;
+        for (uint32_t s = cb; s < ce; ++s) {
+            const uint32_t i = sorted_idx[s];
+            uint32_t cnt = 0;
+            uint32_t* row = row_scratch + (size_t)i * ROW_SLOTS;
+            for_each_candidate(B, P, g, i, sorted[s], sorted_idx, sorted, cell_end, excl, nexcl, [&](uint32_t j) {
+                if (cnt < ROW_SLOTS) row[cnt] = j;
+                ++cnt;
+            });
+            row_count[i] = cnt;
+            if (cnt > ROW_SLOTS) slot_of[i] = s;
+        }
     }
 }
 
@@ -610,8 +749,7 @@ __global__ void __launch_bounds__(TPB) k_cand_emit(BodiesDev B, SimParams P, con
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.n) return;
     const uint32_t cnt = row_count[i];
-    const uint32_t s = slot_of[i];
-    if (cnt == 0 || s == 0xFFFFFFFFu) return;   // empty, or a grid-excluded body's row
+    if (cnt == 0 || (B.flags[i] & FLAG_GRID_EXCLUDED)) return;   // empty, or a grid-excluded body's row
     const uint32_t base = row_start[i];
     uint32_t m = 0;
     if (cnt <= ROW_SLOTS) {
@@ -619,6 +757,7 @@ __global__ void __launch_bounds__(TPB) k_cand_emit(BodiesDev B, SimParams P, con
         for (; m < cnt && base + m < pcap; ++m) { cand_i[base + m] = i; cand_j[base + m] = row[m]; }
     } else {
         const GridDev g = *Gp;
+        const uint32_t s = slot_of[i];
         for_each_candidate(B, P, g, i, sorted[s], sorted_idx, sorted, cell_end, excl, nexcl, [&](uint32_t j) {
             if (base + m < pcap) { cand_i[base + m] = i; cand_j[base + m] = j; }
             ++m;
@@ -750,6 +889,8 @@ __device__ __forceinline__ double combine_mat(int m, double a, double b) {   // 
     return 0.5 * (a + b);
 }
 
+__device__ __forceinline__ void uf_union(const BodiesDev& B, uint32_t* parent, uint32_t i, uint32_t j);
+
 __device__ __forceinline__ double eff_mass(double ima, Qt qa, V3 iia, double imb, Qt qb, V3 iib, V3 ra, V3 rb, V3 u) {
     V3 angA = cross(apply_inv_inertia(qa, iia, cross(ra, u)), ra);
     V3 angB = cross(apply_inv_inertia(qb, iib, cross(rb, u)), rb);
@@ -769,8 +910,6 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
         const uint32_t nc = ncont[p];
         if (!nc) continue;
         const uint32_t i = cand_i[p], j = cand_j[p];
-        parent[i] = i; parent[j] = j;       // union-find state of this step's bodies
-        claimed[i] = 0u; claimed[j] = 0u;
         const double ima = B.im[i], imb = B.im[j];
         const double ra = B.rest[i] < 0 ? P.restitution : B.rest[i];
         const double rb = B.rest[j] < 0 ? P.restitution : B.rest[j];
@@ -783,6 +922,7 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
         const V3 wa = ld3(B.w0, B.w1, B.w2, i), wb = ld3(B.w0, B.w1, B.w2, j);
         const Qt qa = ldq(B, i), qb = ldq(B, j);
         const V3 iia = ld3(B.ii0, B.ii1, B.ii2, i), iib = ld3(B.ii0, B.ii1, B.ii2, j);
+        uf_union(B, parent, i, j);   // islands: join the two bodies when both can move
         const uint64_t key = ((uint64_t)i << 32) | (uint64_t)j;   // world.cpp:34-36
         // warm cache: previous step's constraints are sorted by (key, slot)
         uint32_t lo = 0;
@@ -844,20 +984,16 @@ __device__ __forceinline__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
     return x;
 }
 
-__global__ void __launch_bounds__(TPB) k_uf_link(BodiesDev B, const uint32_t* __restrict__ counters_in,
-                                                 const ConsDev C, uint32_t* __restrict__ parent) {
-    const uint32_t nc = counters_in[CNT_C];
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
-        const uint32_t i = C.ci[k], j = C.cj[k];
-        if (!mutable_body(B, i) || !mutable_body(B, j)) continue;
-        uint32_t a = i, b = j;
-        for (int guard = 0; guard < 1 << 20; ++guard) {
-            a = uf_find(parent, a);
-            b = uf_find(parent, b);
-            if (a == b) break;
-            if (a < b) { uint32_t t = a; a = b; b = t; }   // hang the larger root under the smaller
-            if (atomicCAS(&parent[a], a, b) == a) break;
-        }
+// Lock-free union of the sets of i and j (both must be able to move).
+__device__ __forceinline__ void uf_union(const BodiesDev& B, uint32_t* parent, uint32_t i, uint32_t j) {
+    if (!mutable_body(B, i) || !mutable_body(B, j)) return;
+    uint32_t a = i, b = j;
+    for (int guard = 0; guard < 1 << 20; ++guard) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) break;
+        if (a < b) { uint32_t t = a; a = b; b = t; }   // hang the larger root under the smaller
+        if (atomicCAS(&parent[a], a, b) == a) break;
     }
 }
 
@@ -1013,43 +1149,99 @@ __device__ __forceinline__ void sort_segment(uint32_t* a, uint32_t m) {
     }
 }
 
-// Generic sequential sweep of one island (global-memory body state).
-__device__ void solve_island_seq(BodiesDev& B, const SimParams& P, const ConsDev& C, const uint32_t* list, uint32_t m) {
+// Body-state stores for the sequential island sweep: global memory, or the
+// island's few bodies cached in this thread's slice of LDS.
+struct GlobalStore {
+    BodiesDev& B;
+    __device__ BodyView load(uint32_t b) const { return load_view(B, b); }
+    __device__ void store(uint32_t b, const BodyView& o) const { store_view(B, b, o); }
+    __device__ V3 loadx(uint32_t b) const { return ld3(B.x0, B.x1, B.x2, b); }
+    __device__ void storex(uint32_t b, V3 x) const { st3(B.x0, B.x1, B.x2, b, x); }
+};
+
+constexpr int CACHE_BODIES = 8;
+
+struct LdsStore {
+    BodiesDev& B;
+    const uint32_t* ids;   // cached body indices
+    int nb;
+    V3* v;
+    V3* w;
+    V3* x;
+    __device__ int slot(uint32_t b) const {
+        int s = 0;
+        for (int k = 0; k < nb; ++k) s = (ids[k] == b) ? k : s;
+        return s;
+    }
+    __device__ BodyView load(uint32_t b) const {
+        BodyView o = load_view_static(b);
+        const int s = slot(b);
+        o.v = v[s];
+        o.w = w[s];
+        return o;
+    }
+    __device__ BodyView load_view_static(uint32_t b) const {
+        BodyView o;
+        o.q = ldq(B, b);
+        o.ii = ld3(B.ii0, B.ii1, B.ii2, b);
+        o.im = B.im[b];
+        o.mv = o.im != 0.0;
+        o.mw = o.ii.x != 0.0 || o.ii.y != 0.0 || o.ii.z != 0.0;
+        return o;
+    }
+    __device__ void store(uint32_t b, const BodyView& o) const {
+        const int s = slot(b);
+        v[s] = o.v;
+        w[s] = o.w;
+    }
+    __device__ V3 loadx(uint32_t b) const { return x[slot(b)]; }
+    __device__ void storex(uint32_t b, V3 xx) const { x[slot(b)] = xx; }
+};
+
+template <class Store>
+__device__ __forceinline__ double C_im(const Store& S, uint32_t b);
+
+// Sequential sweep of one island in constraint order (world.cpp:559-616).
+template <class Store>
+__device__ void solve_island_seq(const Store& S, const SimParams& P, const ConsDev& C, const uint32_t* list, uint32_t m) {
     if (P.warm) {
         for (uint32_t t = 0; t < m; ++t) {
             const uint32_t k = list[t];
             const double imp = C.nimp[k];
             if (imp == 0.0) continue;
             const uint32_t i = C.ci[k], j = C.cj[k];
-            BodyView a = load_view(B, i), b = load_view(B, j);
+            BodyView a = S.load(i), b = S.load(j);
             warm_apply(a, b, load_geom(C, k), imp);
-            store_view(B, i, a);
-            store_view(B, j, b);
+            S.store(i, a);
+            S.store(j, b);
         }
     }
     for (int it = 0; it < P.iters; ++it) {
         for (uint32_t t = 0; t < m; ++t) {
             const uint32_t k = list[t];
             const uint32_t i = C.ci[k], j = C.cj[k];
-            BodyView a = load_view(B, i), b = load_view(B, j);
+            BodyView a = S.load(i), b = S.load(j);
             double nimp = C.nimp[k], timp = C.timp[k];
             gs_visit(a, b, load_geom(C, k), C.rbias[k], C.kn[k], C.mu[k], nimp, timp);
             C.nimp[k] = nimp;
             C.timp[k] = timp;
-            store_view(B, i, a);
-            store_view(B, j, b);
+            S.store(i, a);
+            S.store(j, b);
         }
     }
     for (uint32_t t = 0; t < m; ++t) {
         const uint32_t k = list[t];
         const uint32_t i = C.ci[k], j = C.cj[k];
-        V3 xa = ld3(B.x0, B.x1, B.x2, i), xb = ld3(B.x0, B.x1, B.x2, j);
-        const double ima = B.im[i], imb = B.im[j];
-        correct(B, C, k, P.beta, xa, xb, ima, imb);
-        if (ima != 0.0) st3(B.x0, B.x1, B.x2, i, xa);
-        if (imb != 0.0) st3(B.x0, B.x1, B.x2, j, xb);
+        V3 xa = S.loadx(i), xb = S.loadx(j);
+        const double ima = C_im(S, i), imb = C_im(S, j);
+        correct(S.B, C, k, P.beta, xa, xb, ima, imb);
+        if (ima != 0.0) S.storex(i, xa);
+        if (imb != 0.0) S.storex(j, xb);
     }
 }
+
+template <class Store>
+__device__ __forceinline__ double C_im(const Store& S, uint32_t b) { return S.B.im[b]; }
 
 // An island whose constraints all join the same two bodies (a sphere pair,
 // a box-box manifold: <= 4 contacts): both bodies and the impulses stay in
@@ -1092,9 +1284,28 @@ __device__ void solve_pair_island(BodiesDev& B, const SimParams& P, const ConsDe
 }
 
 // One thread per small island.
-__global__ void __launch_bounds__(TPB) k_solve_small(BodiesDev B, SimParams P, uint32_t* counters,
+// This step's (key, slot)-ordered normal impulses become the next step's warm
+// cache (src/world.cpp:619-626): each constraint keeps its global position.
+__device__ __forceinline__ void publish_cache(const ConsDev& C, const uint32_t* list, uint32_t m,
+                                              uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
+    for (uint32_t t = 0; t < m; ++t) {
+        const uint32_t k = list[t];
+        prev_key[k] = C.key[k];
+        prev_imp[k] = C.nimp[k];
+    }
+}
+
+constexpr int SOLVE_THREADS = 64;
+
+__global__ void __launch_bounds__(SOLVE_THREADS) k_solve_small(BodiesDev B, SimParams P, uint32_t* counters,
                                                      const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
-                                                     ConsDev C, uint32_t* __restrict__ big_list) {
+                                                     ConsDev C, uint32_t* __restrict__ big_list,
+                                                     uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp,
+                                                     uint32_t* __restrict__ prev_n) {
+    __shared__ uint32_t lds_ids[SOLVE_THREADS * CACHE_BODIES];
+    __shared__ V3 lds_v[SOLVE_THREADS * CACHE_BODIES];
+    __shared__ V3 lds_w[SOLVE_THREADS * CACHE_BODIES];
+    __shared__ V3 lds_x[SOLVE_THREADS * CACHE_BODIES];
     const uint32_t nisl = counters[CNT_ISL];
     for (uint32_t isl = blockIdx.x * blockDim.x + threadIdx.x; isl < nisl; isl += gridDim.x * blockDim.x) {
         const uint32_t s = isl_start[isl], e = isl_start[isl + 1];
@@ -1110,15 +1321,56 @@ __global__ void __launch_bounds__(TPB) k_solve_small(BodiesDev B, SimParams P, u
             const uint32_t i0 = C.ci[list[0]], j0 = C.cj[list[0]];
             for (uint32_t t = 1; t < m; ++t) pair = pair && C.ci[list[t]] == i0 && C.cj[list[t]] == j0;
         }
-        if (pair) solve_pair_island(B, P, C, list, m);
-        else solve_island_seq(B, P, C, list, m);
+        if (pair) {
+            solve_pair_island(B, P, C, list, m);
+        } else {
+            // gather the island's distinct bodies; cache them in LDS when few
+            uint32_t* ids = lds_ids + threadIdx.x * CACHE_BODIES;
+            int nb = 0;
+            bool fits = true;
+            for (uint32_t t = 0; t < m && fits; ++t) {
+                const uint32_t k = list[t];
+                const uint32_t e2[2] = {C.ci[k], C.cj[k]};
+                for (int q = 0; q < 2; ++q) {
+                    bool seen = false;
+                    for (int u = 0; u < nb; ++u) seen |= ids[u] == e2[q];
+                    if (!seen) {
+                        if (nb == CACHE_BODIES) { fits = false; break; }
+                        ids[nb++] = e2[q];
+                    }
+                }
+            }
+            if (fits) {
+                V3* vv = lds_v + threadIdx.x * CACHE_BODIES;
+                V3* ww = lds_w + threadIdx.x * CACHE_BODIES;
+                V3* xx = lds_x + threadIdx.x * CACHE_BODIES;
+                for (int u = 0; u < nb; ++u) {
+                    vv[u] = ld3(B.v0, B.v1, B.v2, ids[u]);
+                    ww[u] = ld3(B.w0, B.w1, B.w2, ids[u]);
+                    xx[u] = ld3(B.x0, B.x1, B.x2, ids[u]);
+                }
+                LdsStore S{B, ids, nb, vv, ww, xx};
+                solve_island_seq(S, P, C, list, m);
+                for (int u = 0; u < nb; ++u) {       // write back what may change (store_view's rule)
+                    const uint32_t b = ids[u];
+                    const double im = B.im[b];
+                    if (im != 0.0) { st3(B.v0, B.v1, B.v2, b, vv[u]); st3(B.x0, B.x1, B.x2, b, xx[u]); }
+                    if (B.ii0[b] != 0.0 || B.ii1[b] != 0.0 || B.ii2[b] != 0.0) st3(B.w0, B.w1, B.w2, b, ww[u]);
+                }
+            } else {
+                solve_island_seq(GlobalStore{B}, P, C, list, m);
+            }
+        }
+        if (P.warm) publish_cache(C, list, m, prev_key, prev_imp);
     }
+    if (P.warm && blockIdx.x == 0 && threadIdx.x == 0) *prev_n = counters[CNT_C];
 }
 
 // One workgroup per large island: sort the segment, then sweep it.
 __global__ void __launch_bounds__(1024) k_solve_large(BodiesDev B, SimParams P, uint32_t* counters,
                                                       const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
-                                                      ConsDev C, const uint32_t* __restrict__ big_list) {
+                                                      ConsDev C, const uint32_t* __restrict__ big_list,
+                                                      uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
     const uint32_t nbig = counters[CNT_NBIG];
     for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
         const uint32_t isl = big_list[b];
@@ -1144,7 +1396,10 @@ __global__ void __launch_bounds__(1024) k_solve_large(BodiesDev B, SimParams P, 
                 __syncthreads();
             }
         }
-        if (threadIdx.x == 0) solve_island_seq(B, P, C, list, m);
+        if (threadIdx.x == 0) {
+            solve_island_seq(GlobalStore{B}, P, C, list, m);
+            if (P.warm) publish_cache(C, list, m, prev_key, prev_imp);
+        }
         __syncthreads();
     }
 }
@@ -1261,7 +1516,9 @@ struct ne_gpu_ctx {
     GridDev* grid = nullptr;
     uint32_t *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr;
     uint32_t* sorted_idx = nullptr;
+    uint32_t* order_prev = nullptr;   // previous step's cell order (a permutation of `gridded`)
     SortedBody* sorted = nullptr;
+    SortedBody* xp = nullptr;         // packed post-integrate positions + bounding radius
     uint32_t *row_count = nullptr, *row_start = nullptr;
     uint32_t *ex_flags = nullptr, *ex_off = nullptr;
     uint32_t *cand_i = nullptr, *cand_j = nullptr;
@@ -1282,7 +1539,9 @@ struct ne_gpu_ctx {
     uint32_t *slot_of = nullptr, *row_scratch = nullptr, *claimed = nullptr, *cons_isl = nullptr;
     uint32_t *isl_count = nullptr, *isl_fill = nullptr, *isl_list = nullptr, *big_list = nullptr;
     StaticGrid sgrid{};
-    bool use_bricks = true;
+    int broadphase = NE_BROADPHASE_AUTO;
+    uint32_t hint_p = 4096, hint_c = 4096;   // last call's pair/constraint counts (launch sizing)
+    uint32_t cells_per_body = 2;
     GridDev grid_host{};
     uint32_t sgrid_cells = 0;
     // counters
@@ -1386,6 +1645,7 @@ SimParams make_sim_params(const ne_world_params& p) {
     for (int a = 0; a < 3; ++a) {
         s.box.half[a] = p.half[a];
         s.box.edge[a] = 2.0 * p.half[a];
+        s.box.inv_edge[a] = 1.0 / (2.0 * p.half[a]);
     }
     s.box.periodic = p.periodic ? 1 : 0;
     s.dt = p.dt;
@@ -1406,6 +1666,16 @@ uint32_t blocks_for(uint64_t n, uint32_t tpb = TPB) {
     if (b < 1) b = 1;
     return (uint32_t)b;
 }
+// Grid for a grid-stride kernel over a device-side count: sized from the
+// count the previous call saw (any count is correct; this only sets the
+// parallelism), between 64 and 4096 blocks.
+uint32_t hint_blocks(uint32_t hint) {
+    uint64_t b = ((uint64_t)hint * 2 + TPB - 1) / TPB;
+    if (b < 64) b = 64;
+    if (b > 4096) b = 4096;
+    return (uint32_t)b;
+}
+
 uint32_t gs_blocks(uint64_t n) {   // grid-stride launches over device-side counts
     uint64_t b = (n + TPB - 1) / TPB;
     if (b < 1) b = 1;
@@ -1469,6 +1739,7 @@ void free_work(ne_gpu_ctx* c) {
     dfree(c->dfield); dfree(c->ufield); dfree(c->hull_off); dfree(c->hull); dfree(c->nc_off); dfree(c->nc_idx);
     dfree(c->gridded); dfree(c->excl); dfree(c->aabb_part); dfree(c->grid);
     dfree(c->cell_of); dfree(c->cell_count); dfree(c->cell_start); dfree(c->sorted_idx); dfree(c->sorted);
+    dfree(c->order_prev); dfree(c->xp);
     dfree(c->row_count); dfree(c->row_start); dfree(c->ex_flags); dfree(c->ex_off);
     dfree(c->cand_i); dfree(c->cand_j); dfree(c->ncont); dfree(c->cons_off); dfree(c->ct);
     ConsDev& C = c->C;
@@ -1495,9 +1766,11 @@ void free_all(ne_gpu_ctx* c) {
 template <bool F, bool W>
 void launch_integrate(ne_gpu_ctx* c, const SimParams& P, bool bin) {
     if (bin)
-        k_integrate<F, W, true><<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P, c->sgrid, c->cell_of, c->cell_count);
+        k_integrate<F, W, true><<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P, c->sgrid, c->xp, c->cell_count,
+                                                                          c->parent, c->claimed);
     else
-        k_integrate<F, W, false><<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P, c->sgrid, c->cell_of, c->cell_count);
+        k_integrate<F, W, false><<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P, c->sgrid, c->xp, c->cell_count,
+                                                                           c->parent, c->claimed);
 }
 
 // One reference step (src/world.cpp:133-146) as a kernel pipeline.  When
@@ -1529,29 +1802,34 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
             k_aabb_partial<<<AABB_BLOCKS, TPB, 0, s>>>(B, c->gridded, c->ngrid, c->aabb_part);
             k_grid_setup<<<1, 1, 0, s>>>(P, c->cell, c->cellcap, c->aabb_part, AABB_BLOCKS, c->grid, c->counters);
             KTimer kt(c, NE_KERNEL_CELL_COUNT);
-            k_cell_count<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, c->gridded, c->ngrid, c->grid, c->cell_of, c->cell_count);
+            k_cell_count<<<blocks_for(c->ngrid), TPB, 0, s>>>(c->gridded, c->ngrid, c->grid, c->xp, c->cell_count);
         }
         scan_u32(c->cell_count, c->cell_start, static_grid ? nullptr : c->counters + CNT_CELLS,
                  static_grid ? c->sgrid_cells : c->cellcap, c->scan_part, s, true);
         {
             KTimer kt(c, NE_KERNEL_CELL_FILL);
-            k_cell_fill<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, c->gridded, c->ngrid, c->cell_of, c->cell_start,
-                                                             c->sorted_idx, c->sorted, c->slot_of);
+            std::swap(c->sorted_idx, c->order_prev);   // last step's cell order drives this fill
+            k_cell_fill<<<blocks_for(c->ngrid), TPB, 0, s>>>(c->order_prev, c->ngrid, c->grid, c->xp, c->cell_start,
+                                                             c->sorted_idx, c->sorted);
         }
         // --- candidate rows: one neighbourhood walk per body
         KTimer kt(c, NE_KERNEL_CAND_COUNT);
-        if (static_grid && c->use_bricks && c->sgrid.nc[0] >= BRICK_X + 2 && c->sgrid.nc[1] >= BRICK_Y + 2 &&
+        if (static_grid && c->broadphase == NE_BROADPHASE_BRICK && c->sgrid.nc[0] >= BRICK_X + 2 && c->sgrid.nc[1] >= BRICK_Y + 2 &&
             c->sgrid.nc[2] >= BRICK_Z + 2) {
             const uint32_t nb = (uint32_t)((c->sgrid.nc[0] + BRICK_X - 1) / BRICK_X) *
                                 (uint32_t)((c->sgrid.nc[1] + BRICK_Y - 1) / BRICK_Y) *
                                 (uint32_t)((c->sgrid.nc[2] + BRICK_Z - 1) / BRICK_Z);
             const uint32_t blocks = ((nb + 7u) / 8u) * 8u;
             k_cand_brick<<<blocks, BRICK_THREADS, 0, s>>>(B, P, c->grid, c->sorted_idx, c->sorted, c->cell_start,
-                                                          c->excl, c->nexcl, c->row_count, c->row_scratch);
-        } else {
+                                                          c->excl, c->nexcl, c->row_count, c->row_scratch, c->slot_of);
+        } else if (c->broadphase == NE_BROADPHASE_WALK) {
             k_cand_walk<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, P, c->grid, c->ngrid, c->sorted_idx, c->sorted,
                                                              c->cell_start, c->excl, c->nexcl, c->row_count,
-                                                             c->row_scratch);
+                                                             c->row_scratch, c->slot_of);
+        } else {
+            const uint32_t cells = static_grid ? c->sgrid_cells : c->cellcap;
+            k_cand_cells<<<gs_blocks(cells), TPB, 0, s>>>(B, P, c->grid, c->sorted_idx, c->sorted, c->cell_start,
+                                                          c->excl, c->nexcl, c->row_count, c->row_scratch, c->slot_of);
         }
     }
     if (c->nexcl > 0) {
@@ -1560,8 +1838,8 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
         scan_u32(c->ex_flags, c->ex_off, nullptr, c->nexcl * n, c->scan_part, s);
         k_excl_rowcount<<<blocks_for(c->nexcl), TPB, 0, s>>>(c->excl, c->nexcl, n, c->ex_off, c->row_count);
     }
-    scan_u32(c->row_count, c->row_start, nullptr, n, c->scan_part, s);
-    k_clamp_count<<<1, 1, 0, s>>>(c->row_start + n, c->pcap, c->counters, CNT_P, CNT_PRAW, NE_FAULT_PAIRS);
+    scan_u32(c->row_count, c->row_start, nullptr, n, c->scan_part, s, false,
+             ScanTotal{c->counters, CNT_PRAW, CNT_P, CNT_FAULT, c->pcap, NE_FAULT_PAIRS, -1, -1});
     if (c->ngrid > 0) {
         KTimer kt(c, NE_KERNEL_CAND_WRITE);
         k_cand_emit<<<blocks_for(n), TPB, 0, s>>>(B, P, c->grid, c->slot_of, c->sorted_idx, c->sorted, c->cell_start,
@@ -1576,43 +1854,40 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     {
         KTimer kt(c, NE_KERNEL_NARROW);
         if (c->spheres_only)
-            k_narrow<true><<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
+            k_narrow<true><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
                                                               c->counters + CNT_FAULT);
         else
-            k_narrow<false><<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
+            k_narrow<false><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
                                                                c->counters + CNT_FAULT);
     }
-    scan_u32(c->ncont, c->cons_off, c->counters + CNT_P, c->pcap, c->scan_part, s);
-    k_cons_count<<<1, 1, 0, s>>>(c->cons_off, c->ccap, c->counters);
+    scan_u32(c->ncont, c->cons_off, c->counters + CNT_P, c->pcap, c->scan_part, s, false,
+             ScanTotal{c->counters, CNT_CRAW, CNT_C, CNT_FAULT, c->ccap, NE_FAULT_CONTACTS, CNT_ISL, CNT_NBIG});
     {
         KTimer kt(c, NE_KERNEL_CONS_BUILD);
-        k_cons_build<<<gs_blocks(c->pcap), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->cons_off,
+        k_cons_build<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->cons_off,
                                                         c->ct, c->C, c->ccap, c->prev_key, c->prev_imp, c->prev_n,
                                                         c->parent, c->claimed);
     }
-    // --- islands
-    k_uf_link<<<gs_blocks(c->ccap), TPB, 0, s>>>(B, c->counters, c->C, c->parent);
-    k_isl_claim<<<gs_blocks(c->ccap), TPB, 0, s>>>(B, c->counters, c->C, c->parent, c->cons_root, c->claimed,
+    // --- islands (linked inside k_cons_build)
+    k_isl_claim<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, c->parent, c->cons_root, c->claimed,
                                                    c->root_id, c->isl_count, c->isl_fill);
-    k_isl_count<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, c->cons_root, c->root_id, c->cons_isl, c->isl_count);
+    k_isl_count<<<hint_blocks(c->hint_c), TPB, 0, s>>>(c->counters, c->cons_root, c->root_id, c->cons_isl, c->isl_count);
     scan_u32(c->isl_count, c->isl_start, c->counters + CNT_ISL, c->ccap, c->scan_part, s);
-    k_isl_fill<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, c->cons_isl, c->isl_start, c->isl_fill, c->isl_list);
+    k_isl_fill<<<hint_blocks(c->hint_c), TPB, 0, s>>>(c->counters, c->cons_isl, c->isl_start, c->isl_fill, c->isl_list);
     // --- solve
     {
         KTimer kt(c, NE_KERNEL_SOLVE);
-        k_solve_small<<<gs_blocks(c->ccap), TPB, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C,
-                                                         c->big_list);
-        k_solve_large<<<256, 1024, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list);
+        k_solve_small<<<std::min<uint32_t>(4096u, std::max<uint32_t>(64u, (c->hint_c * 2 + SOLVE_THREADS - 1) / SOLVE_THREADS)),
+                        SOLVE_THREADS, 0, s>>>(
+            B, P, c->counters, c->isl_start, c->isl_list, c->C,
+                                                         c->big_list, c->prev_key, c->prev_imp, c->prev_n);
+        k_solve_large<<<256, 1024, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list,
+                                           c->prev_key, c->prev_imp);
     }
     // --- boundary (the periodic wrap is deferred to the next integrate)
     if (!P.box.periodic && !P.open_boundary) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_walls<<<blocks_for(n), TPB, 0, s>>>(B, P);
-    }
-    // --- warm cache for the next step (world.cpp:619-626): this step's list
-    if (P.warm) {
-        k_copy_cache<<<gs_blocks(c->ccap), TPB, 0, s>>>(c->counters, c->C.key, c->C.nimp, c->prev_key, c->prev_imp,
-                                                        c->prev_n);
     }
     return NE_OK;
 }
@@ -1637,8 +1912,10 @@ int prepare_static_grid(ne_gpu_ctx* c) {
         for (int a = 0; a < 3; ++a) g.nc[a] = std::max(1, g.nc[a] * 4 / 5);
     for (int a = 0; a < 3; ++a) {
         g.cw[a] = 2.0 * c->params.half[a] / (double)g.nc[a];
+        g.inv_cw[a] = 1.0 / g.cw[a];
         g.origin[a] = -c->params.half[a];
         gd.nc[a] = g.nc[a]; gd.cw[a] = g.cw[a]; gd.origin[a] = g.origin[a];
+        gd.inv_cw[a] = 1.0 / g.cw[a];
     }
     gd.periodic = 1;
     gd.ncells = (uint32_t)g.nc[0] * (uint32_t)g.nc[1] * (uint32_t)g.nc[2];
@@ -1668,6 +1945,9 @@ int ne_gpu_create(const ne_gpu_config* cfg, ne_gpu_ctx** out) {
     c->user_pcap = cfg ? cfg->pair_capacity : 0;
     c->user_ccap = cfg ? cfg->contact_capacity : 0;
     c->cell_scale = (cfg && cfg->cell_scale >= 1.0) ? cfg->cell_scale : 1.0;
+    c->broadphase = cfg ? cfg->broadphase : NE_BROADPHASE_AUTO;
+    c->cells_per_body = (cfg && cfg->cells_per_body > 0) ? cfg->cells_per_body : 8u;
+    if (c->broadphase < NE_BROADPHASE_AUTO || c->broadphase > NE_BROADPHASE_CELLS) { delete c; return NE_ERR_INVALID; }
     if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return NE_ERR_HIP;
@@ -1781,6 +2061,8 @@ int ne_gpu_step(ne_gpu_ctx* c, int32_t nsteps) {
     HIP_TRY(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(uint32_t) * CNT_N, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     harvest_timing(c);
+    c->hint_p = c->h_counters[CNT_PRAW];
+    c->hint_c = c->h_counters[CNT_CRAW];
     const uint32_t fault = c->h_counters[CNT_FAULT];
     if (fault) {
         // grow what overflowed so a re-upload + retry succeeds; state is undefined
@@ -2072,6 +2354,8 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
         HIP_TRY(c, dalloc(c->aabb_part, 6 * AABB_BLOCKS));
         HIP_TRY(c, dalloc(c->grid, 1));
         HIP_TRY(c, dalloc(c->cell_of, ncap));
+        HIP_TRY(c, dalloc(c->order_prev, ncap));
+        HIP_TRY(c, dalloc(c->xp, ncap));
         HIP_TRY(c, dalloc(c->sorted_idx, ncap));
         HIP_TRY(c, dalloc(c->sorted, ncap));
         HIP_TRY(c, dalloc(c->row_count, (size_t)ncap + 1));
@@ -2082,7 +2366,7 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
         HIP_TRY(c, dalloc(c->slot_of, ncap));
         HIP_TRY(c, dalloc(c->claimed, ncap));
         HIP_TRY(c, dalloc(c->row_scratch, (size_t)ncap * ROW_SLOTS));
-        c->cellcap = std::max<uint32_t>(2u * ncap, 4096u);
+        c->cellcap = (uint32_t)std::min<uint64_t>((uint64_t)c->cells_per_body * ncap + 4096u, 0x7FFFFFFFull);
         HIP_TRY(c, dalloc(c->cell_count, (size_t)c->cellcap + 1));
         HIP_TRY(c, dalloc(c->cell_start, (size_t)c->cellcap + 1));
         HIP_TRY(c, hipMemset(c->cell_count, 0, sizeof(uint32_t) * ((size_t)c->cellcap + 1)));
@@ -2189,8 +2473,12 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     HIP_TRY(c, hipMemcpy(c->nc_idx, hnc.data(), sizeof(uint32_t) * hnc.size(), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->hull_off, hoff.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
     HIP_TRY(c, hipMemcpy(c->nc_off, noff.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
-    if (!gridded.empty())
+    if (!gridded.empty()) {
         HIP_TRY(c, hipMemcpy(c->gridded, gridded.data(), sizeof(uint32_t) * gridded.size(), hipMemcpyHostToDevice));
+        // until a step has sorted them, the cell order is the index order
+        HIP_TRY(c, hipMemcpy(c->sorted_idx, gridded.data(), sizeof(uint32_t) * gridded.size(), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->order_prev, gridded.data(), sizeof(uint32_t) * gridded.size(), hipMemcpyHostToDevice));
+    }
     HIP_TRY(c, hipMemset(c->slot_of, 0xFF, sizeof(uint32_t) * ncap));
     HIP_TRY(c, dalloc(c->excl, std::max<size_t>(excl.size(), 1)));
     if (!excl.empty()) {

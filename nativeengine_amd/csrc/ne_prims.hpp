@@ -59,9 +59,20 @@ __global__ void __launch_bounds__(PRIM_THREADS) k_scan_partial(const uint32_t* _
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
+// Optional bookkeeping done by the scan's single-block phase on the grand
+// total: counters[raw] = total, counters[eff] = min(total, cap), fault bit
+// when it overflowed, and a list of counters to zero for the next stage.
+struct ScanTotal {
+    uint32_t* counters;
+    int raw_slot, eff_slot, fault_slot;
+    uint32_t cap, fault_bit;
+    int zero_slot0, zero_slot1;
+};
+
 // Phase 2: one block scans the partials; writes the grand total to out[n].
 __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ partial, uint32_t nblocks,
-                                                  const uint32_t* n_dev, uint32_t n_max, uint32_t* __restrict__ out) {
+                                                  const uint32_t* n_dev, uint32_t n_max, uint32_t* __restrict__ out,
+                                                  ScanTotal tot) {
     __shared__ uint32_t buf[1024];
     __shared__ uint32_t carry;
     if (threadIdx.x == 0) carry = 0;
@@ -83,7 +94,17 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ partia
         if (threadIdx.x == 1023) carry += incl;
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[resolve_n(n_dev, n_max)] = carry;
+    if (threadIdx.x == 0) {
+        out[resolve_n(n_dev, n_max)] = carry;
+        if (tot.counters) {
+            uint32_t r = carry;
+            tot.counters[tot.raw_slot] = r;
+            if (r > tot.cap) { tot.counters[tot.fault_slot] |= tot.fault_bit; r = tot.cap; }
+            tot.counters[tot.eff_slot] = r;
+            if (tot.zero_slot0 >= 0) tot.counters[tot.zero_slot0] = 0u;
+            if (tot.zero_slot1 >= 0) tot.counters[tot.zero_slot1] = 0u;
+        }
+    }
 }
 
 // Phase 3: rescan each chunk from its block offset.
@@ -207,10 +228,10 @@ inline ScanPlan scan_plan(uint32_t n_max) {
 // out has n+1 entries: out[t] = sum(in[0..t)), out[n] = total.
 // clear_in: zero the input as it is consumed (it must not alias out).
 inline void scan_u32(uint32_t* in, uint32_t* out, const uint32_t* n_dev, uint32_t n_max, uint32_t* partial,
-                     hipStream_t s, bool clear_in = false) {
+                     hipStream_t s, bool clear_in = false, ScanTotal tot = ScanTotal{nullptr, 0, 0, 0, 0, 0, -1, -1}) {
     ScanPlan p = scan_plan(n_max);
     k_scan_partial<<<p.nblocks, PRIM_THREADS, 0, s>>>(in, n_dev, n_max, p.chunk, partial);
-    k_scan_top<<<1, 1024, 0, s>>>(partial, p.nblocks, n_dev, n_max, out);
+    k_scan_top<<<1, 1024, 0, s>>>(partial, p.nblocks, n_dev, n_max, out, tot);
     k_scan_final<<<p.nblocks, PRIM_THREADS, 0, s>>>(in, n_dev, n_max, p.chunk, partial, out, clear_in ? 1 : 0);
 }
 

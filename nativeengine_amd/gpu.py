@@ -81,9 +81,10 @@ def load_library(path: str = LIB_PATH):
 class GpuWorld:
     """A world resident on one MI355X."""
 
-    def __init__(self, device: int = 0, pair_capacity: int = 0, contact_capacity: int = 0, cell_scale: float = 0.0):
+    def __init__(self, device: int = 0, pair_capacity: int = 0, contact_capacity: int = 0, cell_scale: float = 0.0,
+                 broadphase: int = 0, cells_per_body: int = 0):
         self.lib = load_library()
-        cfg = _abi.GpuConfig(device, pair_capacity, contact_capacity, cell_scale)
+        cfg = _abi.GpuConfig(device, pair_capacity, contact_capacity, cell_scale, broadphase, cells_per_body)
         h = C.c_void_p()
         rc = self.lib.ne_gpu_create(C.byref(cfg), C.byref(h))
         if rc != 0:
