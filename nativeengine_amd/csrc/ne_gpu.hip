@@ -1295,21 +1295,134 @@ __device__ __forceinline__ void publish_cache(const ConsDev& C, const uint32_t* 
     }
 }
 
-constexpr int SOLVE_THREADS = 64;
+// Islands of exactly one constraint (isolated contacts: most of a gas): the
+// two bodies, the constraint and its impulses stay in registers.
+__global__ void __launch_bounds__(TPB) k_solve_single(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters,
+                                                      const uint32_t* __restrict__ isl_start,
+                                                      const uint32_t* __restrict__ isl_list, ConsDev C,
+                                                      uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
+    const uint32_t nisl = counters[CNT_ISL];
+    for (uint32_t isl = blockIdx.x * blockDim.x + threadIdx.x; isl < nisl; isl += gridDim.x * blockDim.x) {
+        const uint32_t s = isl_start[isl];
+        if (isl_start[isl + 1] - s != 1) continue;
+        const uint32_t k = isl_list[s];
+        const uint32_t i = C.ci[k], j = C.cj[k];
+        BodyView a = load_view(B, i), b = load_view(B, j);
+        const ConsGeom g = load_geom(C, k);
+        const double rbias = C.rbias[k], kn = C.kn[k], mu = C.mu[k];
+        double nimp = C.nimp[k], timp = 0.0;
+        if (P.warm) warm_apply(a, b, g, nimp);
+        for (int it = 0; it < P.iters; ++it) gs_visit(a, b, g, rbias, kn, mu, nimp, timp);
+        C.nimp[k] = nimp;
+        C.timp[k] = timp;
+        store_view(B, i, a);
+        store_view(B, j, b);
+        V3 xa = ld3(B.x0, B.x1, B.x2, i), xb = ld3(B.x0, B.x1, B.x2, j);
+        correct(B, C, k, P.beta, xa, xb, a.im, b.im);
+        if (a.im != 0.0) st3(B.x0, B.x1, B.x2, i, xa);
+        if (b.im != 0.0) st3(B.x0, B.x1, B.x2, j, xb);
+        if (P.warm) { prev_key[k] = C.key[k]; prev_imp[k] = nimp; }
+    }
+}
+
+// Small islands (2..STAGE_CONS constraints over <= STAGE_BODIES bodies): one
+// thread stages the whole island -- body state, constraint geometry and
+// impulses -- in its slice of LDS and sweeps it there, so the sequential
+// Gauss-Seidel chain never waits on global memory.  Larger small islands take
+// the global-memory sweep.
+constexpr int SOLVE_THREADS = 32;
+constexpr int STAGE_BODIES = 8;
+constexpr int STAGE_CONS = 16;
+
+struct StagedBody {
+    V3 v, w, x, ii;
+    Qt q;
+    double im;
+    int mv, mw;
+};
+struct StagedCons {
+    V3 n, rA, rB;
+    double rbias, kn, mu, ov, nimp, timp;
+    int a, b;   // staged body slots
+};
+
+__device__ __forceinline__ BodyView view_of(const StagedBody& s) {
+    BodyView o;
+    o.v = s.v; o.w = s.w; o.q = s.q; o.ii = s.ii; o.im = s.im; o.mv = s.mv != 0; o.mw = s.mw != 0;
+    return o;
+}
+
+__device__ void solve_island_staged(BodiesDev& B, const SimParams& P, ConsDev& C, const uint32_t* list, uint32_t m,
+                                    const uint32_t* ids, int nb, StagedBody* sb, StagedCons* sc) {
+    for (int u = 0; u < nb; ++u) {
+        const uint32_t b = ids[u];
+        BodyView o = load_view(B, b);
+        sb[u].v = o.v; sb[u].w = o.w; sb[u].q = o.q; sb[u].ii = o.ii; sb[u].im = o.im;
+        sb[u].mv = o.mv; sb[u].mw = o.mw;
+        sb[u].x = ld3(B.x0, B.x1, B.x2, b);
+    }
+    for (uint32_t t = 0; t < m; ++t) {
+        const uint32_t k = list[t];
+        const ConsGeom g = load_geom(C, k);
+        StagedCons& c = sc[t];
+        c.n = g.n; c.rA = g.rA; c.rB = g.rB;
+        c.rbias = C.rbias[k]; c.kn = C.kn[k]; c.mu = C.mu[k]; c.ov = C.ov[k];
+        c.nimp = C.nimp[k]; c.timp = 0.0;
+        const uint32_t i = C.ci[k], j = C.cj[k];
+        int sa = 0, sbb = 0;
+        for (int u = 0; u < nb; ++u) { sa = ids[u] == i ? u : sa; sbb = ids[u] == j ? u : sbb; }
+        c.a = sa; c.b = sbb;
+    }
+    if (P.warm) {
+        for (uint32_t t = 0; t < m; ++t) {
+            StagedCons& c = sc[t];
+            if (c.nimp == 0.0) continue;
+            BodyView a = view_of(sb[c.a]), b = view_of(sb[c.b]);
+            warm_apply(a, b, ConsGeom{c.n, c.rA, c.rB}, c.nimp);
+            sb[c.a].v = a.v; sb[c.a].w = a.w; sb[c.b].v = b.v; sb[c.b].w = b.w;
+        }
+    }
+    for (int it = 0; it < P.iters; ++it) {
+        for (uint32_t t = 0; t < m; ++t) {
+            StagedCons& c = sc[t];
+            BodyView a = view_of(sb[c.a]), b = view_of(sb[c.b]);
+            double nimp = c.nimp, timp = c.timp;
+            gs_visit(a, b, ConsGeom{c.n, c.rA, c.rB}, c.rbias, c.kn, c.mu, nimp, timp);
+            c.nimp = nimp; c.timp = timp;
+            sb[c.a].v = a.v; sb[c.a].w = a.w; sb[c.b].v = b.v; sb[c.b].w = b.w;
+        }
+    }
+    for (uint32_t t = 0; t < m; ++t) {           // contact.hpp:78-85
+        StagedCons& c = sc[t];
+        const double ima = sb[c.a].im, imb = sb[c.b].im;
+        const double invSum = ima + imb;
+        if (invSum <= 0.0 || c.ov <= 0.0) continue;
+        const V3 corr = c.n * (P.beta * c.ov / invSum);
+        if (ima != 0.0) sb[c.a].x = sb[c.a].x + corr * ima;
+        if (imb != 0.0) sb[c.b].x = sb[c.b].x - corr * imb;
+    }
+    for (uint32_t t = 0; t < m; ++t) { C.nimp[list[t]] = sc[t].nimp; C.timp[list[t]] = sc[t].timp; }
+    for (int u = 0; u < nb; ++u) {               // write back what may change (store_view's rule)
+        const uint32_t b = ids[u];
+        if (sb[u].mv) { st3(B.v0, B.v1, B.v2, b, sb[u].v); st3(B.x0, B.x1, B.x2, b, sb[u].x); }
+        if (sb[u].mw) st3(B.w0, B.w1, B.w2, b, sb[u].w);
+    }
+}
+
 
 __global__ void __launch_bounds__(SOLVE_THREADS) k_solve_small(BodiesDev B, SimParams P, uint32_t* counters,
                                                      const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
                                                      ConsDev C, uint32_t* __restrict__ big_list,
                                                      uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp,
                                                      uint32_t* __restrict__ prev_n) {
-    __shared__ uint32_t lds_ids[SOLVE_THREADS * CACHE_BODIES];
-    __shared__ V3 lds_v[SOLVE_THREADS * CACHE_BODIES];
-    __shared__ V3 lds_w[SOLVE_THREADS * CACHE_BODIES];
-    __shared__ V3 lds_x[SOLVE_THREADS * CACHE_BODIES];
+    __shared__ uint32_t lds_ids[SOLVE_THREADS * STAGE_BODIES];
+    __shared__ StagedBody lds_b[SOLVE_THREADS * STAGE_BODIES];
+    __shared__ StagedCons lds_c[SOLVE_THREADS * STAGE_CONS];
     const uint32_t nisl = counters[CNT_ISL];
     for (uint32_t isl = blockIdx.x * blockDim.x + threadIdx.x; isl < nisl; isl += gridDim.x * blockDim.x) {
         const uint32_t s = isl_start[isl], e = isl_start[isl + 1];
         const uint32_t m = e - s;
+        if (m == 1) continue;   // k_solve_single
         if (m > SMALL_ISLAND) {
             big_list[atomicAdd(&counters[CNT_NBIG], 1u)] = isl;
             continue;
@@ -1324,10 +1437,9 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve_small(BodiesDev B, SimP
         if (pair) {
             solve_pair_island(B, P, C, list, m);
         } else {
-            // gather the island's distinct bodies; cache them in LDS when few
-            uint32_t* ids = lds_ids + threadIdx.x * CACHE_BODIES;
+            uint32_t* ids = lds_ids + threadIdx.x * STAGE_BODIES;
             int nb = 0;
-            bool fits = true;
+            bool fits = m <= (uint32_t)STAGE_CONS;
             for (uint32_t t = 0; t < m && fits; ++t) {
                 const uint32_t k = list[t];
                 const uint32_t e2[2] = {C.ci[k], C.cj[k]};
@@ -1335,31 +1447,16 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve_small(BodiesDev B, SimP
                     bool seen = false;
                     for (int u = 0; u < nb; ++u) seen |= ids[u] == e2[q];
                     if (!seen) {
-                        if (nb == CACHE_BODIES) { fits = false; break; }
+                        if (nb == STAGE_BODIES) { fits = false; break; }
                         ids[nb++] = e2[q];
                     }
                 }
             }
-            if (fits) {
-                V3* vv = lds_v + threadIdx.x * CACHE_BODIES;
-                V3* ww = lds_w + threadIdx.x * CACHE_BODIES;
-                V3* xx = lds_x + threadIdx.x * CACHE_BODIES;
-                for (int u = 0; u < nb; ++u) {
-                    vv[u] = ld3(B.v0, B.v1, B.v2, ids[u]);
-                    ww[u] = ld3(B.w0, B.w1, B.w2, ids[u]);
-                    xx[u] = ld3(B.x0, B.x1, B.x2, ids[u]);
-                }
-                LdsStore S{B, ids, nb, vv, ww, xx};
-                solve_island_seq(S, P, C, list, m);
-                for (int u = 0; u < nb; ++u) {       // write back what may change (store_view's rule)
-                    const uint32_t b = ids[u];
-                    const double im = B.im[b];
-                    if (im != 0.0) { st3(B.v0, B.v1, B.v2, b, vv[u]); st3(B.x0, B.x1, B.x2, b, xx[u]); }
-                    if (B.ii0[b] != 0.0 || B.ii1[b] != 0.0 || B.ii2[b] != 0.0) st3(B.w0, B.w1, B.w2, b, ww[u]);
-                }
-            } else {
+            if (fits)
+                solve_island_staged(B, P, C, list, m, ids, nb, lds_b + threadIdx.x * STAGE_BODIES,
+                                    lds_c + threadIdx.x * STAGE_CONS);
+            else
                 solve_island_seq(GlobalStore{B}, P, C, list, m);
-            }
         }
         if (P.warm) publish_cache(C, list, m, prev_key, prev_imp);
     }
@@ -1822,7 +1919,7 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
             const uint32_t blocks = ((nb + 7u) / 8u) * 8u;
             k_cand_brick<<<blocks, BRICK_THREADS, 0, s>>>(B, P, c->grid, c->sorted_idx, c->sorted, c->cell_start,
                                                           c->excl, c->nexcl, c->row_count, c->row_scratch, c->slot_of);
-        } else if (c->broadphase == NE_BROADPHASE_WALK) {
+        } else if (c->broadphase == NE_BROADPHASE_WALK || c->broadphase == NE_BROADPHASE_AUTO) {
             k_cand_walk<<<blocks_for(c->ngrid), TPB, 0, s>>>(B, P, c->grid, c->ngrid, c->sorted_idx, c->sorted,
                                                              c->cell_start, c->excl, c->nexcl, c->row_count,
                                                              c->row_scratch, c->slot_of);
@@ -1877,6 +1974,8 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     // --- solve
     {
         KTimer kt(c, NE_KERNEL_SOLVE);
+        k_solve_single<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C,
+                                                              c->prev_key, c->prev_imp);
         k_solve_small<<<std::min<uint32_t>(4096u, std::max<uint32_t>(64u, (c->hint_c * 2 + SOLVE_THREADS - 1) / SOLVE_THREADS)),
                         SOLVE_THREADS, 0, s>>>(
             B, P, c->counters, c->isl_start, c->isl_list, c->C,
