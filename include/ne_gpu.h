@@ -32,8 +32,9 @@ extern "C" {
 #define NE_OK                 0
 #define NE_ERR_INVALID      (-1)   /* bad argument / unsupported configuration */
 #define NE_ERR_HIP          (-2)   /* a HIP runtime call failed               */
-#define NE_ERR_CAPACITY     (-3)   /* a pair/contact buffer overflowed; state is
-                                      undefined, re-upload and raise capacity */
+#define NE_ERR_CAPACITY     (-3)   /* a work buffer still overflowed after the
+                                      step's own reruns; state is undefined,
+                                      re-upload and retry */
 #define NE_ERR_NONFINITE    (-4)   /* a non-finite position/velocity was found */
 #define NE_ERR_UNSUPPORTED  (-5)   /* shape / feature not on the GPU path yet  */
 #define NE_ERR_NO_DEVICE    (-6)   /* no HIP device visible                    */
@@ -129,7 +130,8 @@ typedef struct ne_gpu_config {
 #define NE_BROADPHASE_AUTO   0
 #define NE_BROADPHASE_WALK   1   /* one thread per body walks 27 cells in global memory */
 #define NE_BROADPHASE_BRICK  2   /* one workgroup per 8x8x4-cell brick, halo staged in LDS */
-#define NE_BROADPHASE_CELLS  3   /* one thread per cell (coalesced neighbour rows); AUTO picks this */
+#define NE_BROADPHASE_CELLS  3   /* one thread per cell (coalesced neighbour rows) */
+/* NE_BROADPHASE_AUTO is the per-body walk; ne_gpu_autotune measures the others. */
 
 typedef struct ne_gpu_ctx ne_gpu_ctx;
 
@@ -151,7 +153,11 @@ int ne_gpu_set_bodies(ne_gpu_ctx *ctx, const ne_bodies *b);
 int ne_gpu_set_state(ne_gpu_ctx *ctx, const double *x, const double *v,
                      const double *q, const double *w);
 
-/* Replaces `World::step()` (src/world.cpp:133-146), nsteps times, resident. */
+/* Replaces `World::step()` (src/world.cpp:133-146), nsteps times, resident.
+ * Work buffers (candidate pairs, constraints, the large EPA workspace) are
+ * sized from the bodies; when one overflows, the call enlarges it, restores
+ * the state it started from (a device-side copy taken at the start of every
+ * call) and runs again, up to 4 times, so results never depend on capacity. */
 int ne_gpu_step(ne_gpu_ctx *ctx, int32_t nsteps);
 
 /* Downloads x (n*3), v (n*3), q (n*4), w (n*3); any may be NULL. */
@@ -224,6 +230,50 @@ int ne_gpu_kernel_time(ne_gpu_ctx *ctx, double *total_ms, uint32_t *launches);
  * a*b+a (3), computed by a kernel -- checks the IEEE ground truth. */
 int ne_gpu_fp_probe(ne_gpu_ctx *ctx, int32_t op, uint32_t n, const double *a,
                     const double *b, double *out);
+
+/* Batched worlds.  No counterpart in the reference, which steps one World at
+ * a time; this steps many independent Worlds in one set of launches.  Splits
+ * the uploaded bodies into `nworlds` periodic worlds: world w holds bodies
+ * [offsets[w], offsets[w+1]) (offsets[0] = 0, offsets[nworlds] = n), has box
+ * half extents half[3w..3w+2], and default restitution / friction
+ * restitution[w] / friction[w] (the World::restitution and World::friction of
+ * src/world.hpp:91-92; NULL keeps ne_world_params').  dt, gravity, combine
+ * modes, contact beta, iterations and warm start are shared.  Each world's
+ * results equal stepping that world alone; pair, contact and warm-cache
+ * indices are global body indices.  Requires periodic params.
+ * ne_gpu_set_bodies returns to one world, as does nworlds = 0. */
+int ne_gpu_set_worlds(ne_gpu_ctx *ctx, uint32_t nworlds, const uint32_t *offsets,
+                      const double *half, const double *restitution,
+                      const double *friction);
+
+/* EPA workspace use during the last ne_gpu_step call.  Most EPA runs fit a
+ * small per-thread workspace; the rest (near-degenerate contacts, where the
+ * reference's vectors grow to thousands of faces) are rerun with a
+ * global-memory workspace that grows up to 131072 faces -- `reruns` counts
+ * them.  A run that would need more stops at the closest face found so far,
+ * as at the reference's 64-iteration limit, where the reference would keep
+ * growing its vectors -- `truncated` counts those (the only results that can
+ * differ from the reference's). */
+int ne_gpu_epa_stats(ne_gpu_ctx *ctx, uint32_t *reruns, uint32_t *truncated);
+
+/* Replays each step from a captured HIP graph (enable = 1) instead of
+ * launching its kernels one by one.  Results are identical.  Steps that apply
+ * uploaded forces, and steps with kernel timing on, run eagerly. */
+int ne_gpu_use_graph(ne_gpu_ctx *ctx, int32_t enable);
+
+/* Autotuner: from the current state, times `steps` steps for every
+ * combination of cell scale (1, 1.25, 1.5), neighbourhood walk
+ * (NE_BROADPHASE_WALK, NE_BROADPHASE_CELLS) and graph replay (off, on),
+ * restoring bodies and warm cache before each trial and at the end, then
+ * keeps the fastest.  Results do not depend on the choice. */
+typedef struct ne_tune_result {
+    double   cell_scale;
+    int32_t  broadphase;
+    int32_t  graph;
+    double   ms_per_step;
+    uint32_t trials;
+} ne_tune_result;
+int ne_gpu_autotune(ne_gpu_ctx *ctx, int32_t steps, ne_tune_result *out);
 
 /* Last error message of this context (static storage, never NULL). */
 const char *ne_gpu_last_error(ne_gpu_ctx *ctx);

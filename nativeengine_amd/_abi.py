@@ -90,6 +90,16 @@ class GpuConfig(C.Structure):
 BROADPHASE_AUTO, BROADPHASE_WALK, BROADPHASE_BRICK, BROADPHASE_CELLS = 0, 1, 2, 3
 
 
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("cell_scale", C.c_double),
+        ("broadphase", C.c_int32),
+        ("graph", C.c_int32),
+        ("ms_per_step", C.c_double),
+        ("trials", C.c_uint32),
+    ]
+
+
 def dptr(a):
     """Pointer to a C-contiguous float64 array (or NULL for None)."""
     if a is None:

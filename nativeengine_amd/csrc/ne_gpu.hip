@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -52,6 +53,10 @@ struct BodiesDev {
     uint32_t *hull_off, *nc_off, *nc_idx;
     double* hull;
     uint32_t n;
+    // Batched worlds: world id per body and the world table (both null for
+    // one world).  Bodies of a world are contiguous.
+    const uint32_t* wid;
+    const struct WorldDev* W;
 };
 
 struct SimParams {
@@ -75,6 +80,23 @@ struct GridDev {
     int32_t periodic;
 };
 
+// One of several independent periodic worlds stepped together: its box, its
+// default material (src/world.hpp:91-92), its slice of the shared cell table
+// and its range of bodies and of grid-excluded bodies.
+struct WorldDev {
+    Box3 box;
+    GridDev grid;
+    double restitution, friction;
+    uint32_t cell_base;
+    uint32_t body_begin, body_end;
+    uint32_t excl_begin, excl_end;   // excl[excl_begin..excl_end) lie in this world
+    uint32_t flag_base;              // first flag of this world's excluded-body rows
+};
+
+__device__ __forceinline__ const Box3& box_of(const BodiesDev& B, const SimParams& P, uint32_t i) {
+    return B.W ? B.W[B.wid[i]].box : P.box;
+}
+
 enum CounterSlot : int {
     CNT_P = 0,        // candidate pairs (clamped)
     CNT_HIT = 1,      // hit pairs
@@ -87,6 +109,9 @@ enum CounterSlot : int {
     CNT_PRAW = 8,     // candidate pairs before clamping
     CNT_CRAW = 9,     // constraints before clamping
     CNT_NBIG = 10,    // islands handed to the workgroup solver
+    CNT_EPABIG = 11,  // pairs queued for the large-workspace EPA rerun (this step)
+    CNT_EPATRUNC = 12,   // EPA runs stopped at the largest workspace (this call)
+    CNT_EPARERUN = 13,   // large-workspace EPA reruns (this call)
     CNT_N = 16
 };
 
@@ -134,6 +159,7 @@ struct StaticGrid {   // periodic worlds: the grid never changes between steps
 };
 
 __device__ __forceinline__ uint32_t static_cell(const StaticGrid& g, V3 x);
+__device__ __forceinline__ uint32_t grid_cell(const GridDev& g, V3 x);
 
 // Integrate (src/world.cpp:373-411), optionally preceded by the previous
 // step's periodic wrap (world.cpp:698-702, same per-body arithmetic, moved to
@@ -155,7 +181,7 @@ __global__ void __launch_bounds__(TPB) k_integrate(BodiesDev B, SimParams P, Sta
     const uint32_t fl = B.flags[i];
     const double im = B.im[i];
     V3 x = ld3(B.x0, B.x1, B.x2, i);
-    if (WRAP && im > 0.0) x = wrap_pos(P.box, x);
+    if (WRAP && im > 0.0) x = wrap_pos(box_of(B, P, i), x);
     V3 f = FORCES ? ld3(B.f0, B.f1, B.f2, i) : V3{0.0, 0.0, 0.0};
     V3 tq = FORCES ? ld3(B.t0, B.t1, B.t2, i) : V3{0.0, 0.0, 0.0};
     if (fl & NE_FLAG_KINEMATIC) {
@@ -193,7 +219,16 @@ __global__ void __launch_bounds__(TPB) k_integrate(BodiesDev B, SimParams P, Sta
         st3(B.t0, B.t1, B.t2, i, V3{0.0, 0.0, 0.0});
     }
     xp[i] = SortedBody{x.x, x.y, x.z, B.bR[i]};   // packed pose for the grid fill
-    if (BIN && !(fl & FLAG_GRID_EXCLUDED)) atomicAdd(&cell_count[static_cell(G, x)], 1u);
+    if (BIN && !(fl & FLAG_GRID_EXCLUDED)) {
+        uint32_t cell;
+        if (B.W) {
+            const WorldDev& w = B.W[B.wid[i]];
+            cell = w.cell_base + grid_cell(w.grid, x);
+        } else {
+            cell = static_cell(G, x);
+        }
+        atomicAdd(&cell_count[cell], 1u);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -312,6 +347,11 @@ __device__ __forceinline__ int32_t cell_coord(const GridDev& g, int a, double c)
     return (int32_t)k;
 }
 
+__device__ __forceinline__ uint32_t grid_cell(const GridDev& g, V3 x) {
+    const int32_t cx = cell_coord(g, 0, x.x), cy = cell_coord(g, 1, x.y), cz = cell_coord(g, 2, x.z);
+    return ((uint32_t)cz * (uint32_t)g.nc[1] + (uint32_t)cy) * (uint32_t)g.nc[0] + (uint32_t)cx;
+}
+
 __device__ __forceinline__ uint32_t static_cell(const StaticGrid& sg, V3 x) {
     GridDev g;
     for (int a = 0; a < 3; ++a) {
@@ -340,16 +380,21 @@ __global__ void __launch_bounds__(TPB) k_cell_count(const uint32_t* __restrict__
 // per step), so the slot bumps and the sorted-array writes of a wave land
 // close together instead of scattering across the whole table.
 __global__ void __launch_bounds__(TPB) k_cell_fill(const uint32_t* __restrict__ order, uint32_t ngrid,
-                                                   const GridDev* __restrict__ Gp, const SortedBody* __restrict__ xp,
+                                                   const GridDev* __restrict__ Gp, const uint32_t* __restrict__ wid,
+                                                   const WorldDev* __restrict__ W, const SortedBody* __restrict__ xp,
                                                    uint32_t* __restrict__ cell_start, uint32_t* __restrict__ sorted_idx,
                                                    SortedBody* __restrict__ sorted) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ngrid) return;
-    const GridDev g = *Gp;
     const uint32_t i = order[t];
     const SortedBody p = xp[i];
-    const uint32_t c = ((uint32_t)cell_coord(g, 2, p.z) * (uint32_t)g.nc[1] + (uint32_t)cell_coord(g, 1, p.y)) *
-                           (uint32_t)g.nc[0] + (uint32_t)cell_coord(g, 0, p.x);
+    uint32_t c;
+    if (W) {
+        const WorldDev& w = W[wid[i]];
+        c = w.cell_base + grid_cell(w.grid, V3{p.x, p.y, p.z});
+    } else {
+        c = grid_cell(*Gp, V3{p.x, p.y, p.z});
+    }
     const uint32_t slot = atomicAdd(&cell_start[c], 1u);
     sorted_idx[slot] = i;
     sorted[slot] = p;
@@ -425,13 +470,35 @@ __device__ __forceinline__ void axis_offsets(const GridDev& g, int a, int32_t c,
 // neighbour-range bounds are loaded before any range is walked (the walk is
 // latency-bound); when the three x-neighbours are consecutive cells a row is
 // one range.
+// The grid, cell offset, box and grid-excluded range that body i sees.
+struct CellView {
+    GridDev g;
+    uint32_t base;
+    Box3 box;
+    uint32_t e0, e1;
+};
+
+__device__ __forceinline__ CellView cell_view(const BodiesDev& B, const SimParams& P, const GridDev* Gp, uint32_t i,
+                                              uint32_t nexcl) {
+    CellView v;
+    if (B.W) {
+        const WorldDev& w = B.W[B.wid[i]];
+        v.g = w.grid; v.base = w.cell_base; v.box = w.box; v.e0 = w.excl_begin; v.e1 = w.excl_end;
+    } else {
+        v.g = *Gp; v.base = 0; v.box = P.box; v.e0 = 0; v.e1 = nexcl;
+    }
+    return v;
+}
+
 template <class F>
-__device__ __forceinline__ void for_each_candidate(const BodiesDev& B, const SimParams& P, const GridDev& g,
+__device__ __forceinline__ void for_each_candidate(const BodiesDev& B, const CellView& cv,
                                                    uint32_t i, const SortedBody& me,
                                                    const uint32_t* __restrict__ sorted_idx,
                                                    const SortedBody* __restrict__ sorted,
                                                    const uint32_t* __restrict__ cell_end,
-                                                   const uint32_t* __restrict__ excl, uint32_t nexcl, F&& emit) {
+                                                   const uint32_t* __restrict__ excl, F&& emit) {
+    const GridDev& g = cv.g;
+    const uint32_t base = cv.base;
     const V3 xi{me.x, me.y, me.z};
     int32_t vx[3], vy[3], vz[3];
     bool ox[3], oy[3], oz[3];
@@ -445,7 +512,7 @@ __device__ __forceinline__ void for_each_candidate(const BodiesDev& B, const Sim
             const uint32_t j = sorted_idx[t];
             const SortedBody o = sorted[t];
             if (j <= i) continue;
-            if (!bounds_touch(P.box, xi, me.r, V3{o.x, o.y, o.z}, o.r)) continue;
+            if (!bounds_touch(cv.box, xi, me.r, V3{o.x, o.y, o.z}, o.r)) continue;
             if (!pair_passes(B, i, j)) continue;
             emit(j);
         }
@@ -459,7 +526,7 @@ __device__ __forceinline__ void for_each_candidate(const BodiesDev& B, const Sim
         for (int r = 0; r < 9; ++r) {
             const int b = r % 3, c = r / 3;
             const bool ok = oy[b] && oz[c];
-            const uint32_t c0 = ((uint32_t)vz[c] * nc1 + (uint32_t)vy[b]) * nc0 + (uint32_t)(cx - 1);
+            const uint32_t c0 = base + ((uint32_t)vz[c] * nc1 + (uint32_t)vy[b]) * nc0 + (uint32_t)(cx - 1);
             const uint32_t e = ok ? cell_end[c0 + 2] : 0u;
             const uint32_t bgn = (ok && c0 > 0) ? cell_end[c0 - 1] : 0u;
             rb[r] = bgn;
@@ -484,17 +551,17 @@ __device__ __forceinline__ void for_each_candidate(const BodiesDev& B, const Sim
         for (int k = 0; k < 27; ++k) {
             const int a = k % 3, b = (k / 3) % 3, c = k / 9;
             const bool ok = ox[a] && oy[b] && oz[c];
-            const uint32_t cell = ((uint32_t)vz[c] * nc1 + (uint32_t)vy[b]) * nc0 + (uint32_t)vx[a];
+            const uint32_t cell = base + ((uint32_t)vz[c] * nc1 + (uint32_t)vy[b]) * nc0 + (uint32_t)vx[a];
             cb[k] = (ok && cell) ? cell_end[cell - 1] : 0u;
             ce[k] = ok ? cell_end[cell] : 0u;
         }
 #pragma unroll
         for (int k = 0; k < 27; ++k) walk(cb[k], ce[k]);
     }
-    for (uint32_t e = 0; e < nexcl; ++e) {
+    for (uint32_t e = cv.e0; e < cv.e1; ++e) {
         const uint32_t j = excl[e];
         if (j <= i) continue;
-        if (!bounds_touch(P.box, xi, me.r, ld3(B.x0, B.x1, B.x2, j), B.bR[j])) continue;
+        if (!bounds_touch(cv.box, xi, me.r, ld3(B.x0, B.x1, B.x2, j), B.bR[j])) continue;
         if (!pair_passes(B, i, j)) continue;
         emit(j);
     }
@@ -513,12 +580,12 @@ __global__ void __launch_bounds__(TPB) k_cand_walk(BodiesDev B, SimParams P, con
                                                    uint32_t* __restrict__ slot_of) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ngrid) return;
-    const GridDev g = *Gp;
     const uint32_t i = sorted_idx[s];
     const SortedBody me = sorted[s];
+    const CellView cv = cell_view(B, P, Gp, i, nexcl);
     uint32_t cnt = 0;
     uint32_t* row = row_scratch + (size_t)i * ROW_SLOTS;
-    for_each_candidate(B, P, g, i, me, sorted_idx, sorted, cell_end, excl, nexcl, [&](uint32_t j) {
+    for_each_candidate(B, cv, i, me, sorted_idx, sorted, cell_end, excl, [&](uint32_t j) {
         if (cnt < ROW_SLOTS) row[cnt] = j;
         ++cnt;
     });
@@ -690,7 +757,8 @@ __global__ void __launch_bounds__(BRICK_THREADS) k_cand_brick(BodiesDev B, SimPa
                     const uint32_t i = sorted_idx[sidx];
                     uint32_t cnt = 0;
                     uint32_t* row = row_scratch + (size_t)i * ROW_SLOTS;
-                    for_each_candidate(B, P, g, i, sorted[sidx], sorted_idx, sorted, cell_end, excl, nexcl,
+                    const CellView cv = cell_view(B, P, Gp, i, nexcl);
+                    for_each_candidate(B, cv, i, sorted[sidx], sorted_idx, sorted, cell_end, excl,
                                        [&](uint32_t j) {
                                            if (cnt < ROW_SLOTS) row[cnt] = j;
                                            ++cnt;
@@ -723,7 +791,8 @@ __global__ void __launch_bounds__(TPB) k_cand_cells(BodiesDev B, SimParams P, co
             const uint32_t i = sorted_idx[s];
             uint32_t cnt = 0;
             uint32_t* row = row_scratch + (size_t)i * ROW_SLOTS;
-            for_each_candidate(B, P, g, i, sorted[s], sorted_idx, sorted, cell_end, excl, nexcl, [&](uint32_t j) {
+            const CellView cv = cell_view(B, P, Gp, i, nexcl);
+            for_each_candidate(B, cv, i, sorted[s], sorted_idx, sorted, cell_end, excl, [&](uint32_t j) {
                 if (cnt < ROW_SLOTS) row[cnt] = j;
                 ++cnt;
             });
@@ -756,9 +825,9 @@ __global__ void __launch_bounds__(TPB) k_cand_emit(BodiesDev B, SimParams P, con
         const uint32_t* row = row_scratch + (size_t)i * ROW_SLOTS;
         for (; m < cnt && base + m < pcap; ++m) { cand_i[base + m] = i; cand_j[base + m] = row[m]; }
     } else {
-        const GridDev g = *Gp;
+        const CellView cv = cell_view(B, P, Gp, i, nexcl);
         const uint32_t s = slot_of[i];
-        for_each_candidate(B, P, g, i, sorted[s], sorted_idx, sorted, cell_end, excl, nexcl, [&](uint32_t j) {
+        for_each_candidate(B, cv, i, sorted[s], sorted_idx, sorted, cell_end, excl, [&](uint32_t j) {
             if (base + m < pcap) { cand_i[base + m] = i; cand_j[base + m] = j; }
             ++m;
         });
@@ -772,39 +841,60 @@ __global__ void __launch_bounds__(TPB) k_cand_emit(BodiesDev B, SimParams P, con
     }
 }
 
-// Rows of grid-excluded bodies X: (X, j) for every j > X, in j order.
-// flags[e*n + j] = 1 when the pair survives the bound test and the filters.
-__global__ void __launch_bounds__(TPB) k_excl_flags(BodiesDev B, SimParams P, const uint32_t* __restrict__ excl,
-                                                    uint32_t* __restrict__ flags) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t e = blockIdx.y;
-    if (j >= B.n) return;
-    const uint32_t X = excl[e];
-    uint32_t f = 0;
-    if (j > X && bounds_touch(P.box, ld3(B.x0, B.x1, B.x2, X), B.bR[X], ld3(B.x0, B.x1, B.x2, j), B.bR[j]) &&
-        pair_passes(B, X, j))
-        f = 1;
-    flags[(size_t)e * B.n + j] = f;
+// Rows of grid-excluded bodies X: (X, j) for every j > X of X's world, in j
+// order.  Excluded body e owns one flag per body of its world (every body
+// with one world), set when the pair survives the bound test and the filters.
+struct ExclRow {
+    uint32_t begin, end;   // bodies of X's world
+    size_t base;           // first flag of the row
+};
+
+__device__ __forceinline__ ExclRow excl_row(const BodiesDev& B, uint32_t e, uint32_t X) {
+    if (B.W) {
+        const WorldDev& w = B.W[B.wid[X]];
+        const uint32_t m = w.body_end - w.body_begin;
+        return ExclRow{w.body_begin, w.body_end, (size_t)w.flag_base + (size_t)(e - w.excl_begin) * m};
+    }
+    return ExclRow{0u, B.n, (size_t)e * B.n};
 }
 
-__global__ void k_excl_rowcount(const uint32_t* __restrict__ excl, uint32_t nexcl, uint32_t n,
+__global__ void __launch_bounds__(TPB) k_excl_flags(BodiesDev B, SimParams P, const uint32_t* __restrict__ excl,
+                                                    uint32_t* __restrict__ flags) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t e = blockIdx.y;
+    const uint32_t X = excl[e];
+    const ExclRow r = excl_row(B, e, X);
+    const uint32_t j = r.begin + u;
+    if (j >= r.end) return;
+    uint32_t f = 0;
+    if (j > X && bounds_touch(box_of(B, P, X), ld3(B.x0, B.x1, B.x2, X), B.bR[X], ld3(B.x0, B.x1, B.x2, j), B.bR[j]) &&
+        pair_passes(B, X, j))
+        f = 1;
+    flags[r.base + u] = f;
+}
+
+__global__ void k_excl_rowcount(BodiesDev B, const uint32_t* __restrict__ excl, uint32_t nexcl,
                                 const uint32_t* __restrict__ ex_off, uint32_t* __restrict__ row_count) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nexcl) return;
-    row_count[excl[e]] = ex_off[(size_t)(e + 1) * n] - ex_off[(size_t)e * n];
+    const uint32_t X = excl[e];
+    const ExclRow r = excl_row(B, e, X);
+    row_count[X] = ex_off[r.base + (r.end - r.begin)] - ex_off[r.base];
 }
 
-__global__ void __launch_bounds__(TPB) k_excl_write(uint32_t n, const uint32_t* __restrict__ excl,
+__global__ void __launch_bounds__(TPB) k_excl_write(BodiesDev B, const uint32_t* __restrict__ excl,
                                                     const uint32_t* __restrict__ flags, const uint32_t* __restrict__ ex_off,
                                                     const uint32_t* __restrict__ row_start, uint32_t pcap,
                                                     uint32_t* __restrict__ cand_i, uint32_t* __restrict__ cand_j) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t e = blockIdx.y;
-    if (j >= n) return;
-    const size_t f = (size_t)e * n + j;
-    if (!flags[f]) return;
     const uint32_t X = excl[e];
-    const uint32_t pos = row_start[X] + (ex_off[f] - ex_off[(size_t)e * n]);
+    const ExclRow r = excl_row(B, e, X);
+    const uint32_t j = r.begin + u;
+    if (j >= r.end) return;
+    const size_t f = r.base + u;
+    if (!flags[f]) return;
+    const uint32_t pos = row_start[X] + (ex_off[f] - ex_off[r.base]);
     if (pos < pcap) { cand_i[pos] = X; cand_j[pos] = j; }
 }
 
@@ -822,45 +912,120 @@ __global__ void k_clamp_count(const uint32_t* __restrict__ raw, uint32_t cap, ui
 // src/world.cpp:122-129 (detectContact + box-box manifold).
 // ---------------------------------------------------------------------------
 
+// Contacts of one candidate pair into out[0..27] (<= 4 x (n3, pt3, depth));
+// returns how many.  ws.overflow is set when EPA outgrew its workspace.
 template <bool SPHERES_ONLY>
-__global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters_in,
-                                                const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
-                                                uint32_t* __restrict__ ncont, double* __restrict__ ct,
-                                                uint32_t* __restrict__ fault) {
-    const uint32_t np = counters_in[CNT_P];
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
-        const uint32_t i = cand_i[p], j = cand_j[p];
-        uint32_t nc = 0;
-        double* out = ct + (size_t)p * 28;   // 4 contacts x (n3, pt3, depth)
-        const int si = SPHERES_ONLY ? (int)SH_SPHERE : B.shape[i];
-        const int sj = SPHERES_ONLY ? (int)SH_SPHERE : B.shape[j];
-        if (SPHERES_ONLY || (si == SH_SPHERE && sj == SH_SPHERE)) {
-            Contact c = sphere_sphere(P.box, ld3(B.x0, B.x1, B.x2, i), B.rad[i], ld3(B.x0, B.x1, B.x2, j), B.rad[j]);
-            if (c.hit) {
+__device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimParams& P, uint32_t i, uint32_t j,
+                                                double* out, EpaWS& ws, uint32_t* fault) {
+    const Box3& box = box_of(B, P, i);   // i and j share a world
+    uint32_t nc = 0;
+    const int si = SPHERES_ONLY ? (int)SH_SPHERE : B.shape[i];
+    const int sj = SPHERES_ONLY ? (int)SH_SPHERE : B.shape[j];
+    if (SPHERES_ONLY || (si == SH_SPHERE && sj == SH_SPHERE)) {
+        Contact c = sphere_sphere(box, ld3(B.x0, B.x1, B.x2, i), B.rad[i], ld3(B.x0, B.x1, B.x2, j), B.rad[j]);
+        if (c.hit) {
+            out[0] = c.normal.x; out[1] = c.normal.y; out[2] = c.normal.z;
+            out[3] = c.point.x; out[4] = c.point.y; out[5] = c.point.z; out[6] = c.overlap;
+            nc = 1;
+        }
+    } else if (!SPHERES_ONLY) {
+        const Geom a = load_geom(B, i), b = load_geom(B, j);
+        Contact c = detect_contact(box, a, b, ws);
+        if (ws.overflow && !ws.truncate) return 0;
+        if (c.hit) {
+            if (si == SH_BOX && sj == SH_BOX) {
+                V3 pts[4];
+                double dep[4];
+                int m = box_box_manifold(box, a, b, c.normal, pts, dep, fault);
+                for (int k = 0; k < m; ++k) {
+                    double* o = out + 7 * k;
+                    o[0] = c.normal.x; o[1] = c.normal.y; o[2] = c.normal.z;
+                    o[3] = pts[k].x; o[4] = pts[k].y; o[5] = pts[k].z; o[6] = dep[k];
+                }
+                nc = (uint32_t)m;
+            }
+            if (nc == 0) {
                 out[0] = c.normal.x; out[1] = c.normal.y; out[2] = c.normal.z;
                 out[3] = c.point.x; out[4] = c.point.y; out[5] = c.point.z; out[6] = c.overlap;
                 nc = 1;
             }
-        } else if (!SPHERES_ONLY) {
-            const Geom a = load_geom(B, i), b = load_geom(B, j);
-            Contact c = detect_contact(P.box, a, b, fault);
-            if (c.hit) {
-                if (si == SH_BOX && sj == SH_BOX) {
-                    V3 pts[4];
-                    double dep[4];
-                    int m = box_box_manifold(P.box, a, b, c.normal, pts, dep, fault);
-                    for (int k = 0; k < m; ++k) {
-                        double* o = out + 7 * k;
-                        o[0] = c.normal.x; o[1] = c.normal.y; o[2] = c.normal.z;
-                        o[3] = pts[k].x; o[4] = pts[k].y; o[5] = pts[k].z; o[6] = dep[k];
-                    }
-                    nc = (uint32_t)m;
-                }
-                if (nc == 0) {
-                    out[0] = c.normal.x; out[1] = c.normal.y; out[2] = c.normal.z;
-                    out[3] = c.point.x; out[4] = c.point.y; out[5] = c.point.z; out[6] = c.overlap;
-                    nc = 1;
-                }
+        }
+    }
+    return nc;
+}
+
+// Pairs whose EPA polytope outgrows the per-thread workspace (rare
+// near-degenerate contacts, where the reference's vectors grow to thousands
+// of faces) are queued for k_narrow_big, which reruns them with a workspace
+// in global memory and the table-driven horizon.  Both capacities grow when
+// exceeded (the step is then run again).
+constexpr uint32_t EPA_BIG_PAIRS = 64;    // initial queue capacity
+constexpr int EPA_BIG_FACES = 4096;       // initial faces (and horizon entries) per queued pair
+constexpr int EPA_BIG_FACES_MAX = 1 << 17;   // beyond this a run stops at its closest face
+constexpr int EPA_TAB = EPA_MAXV * EPA_MAXV;
+
+struct EpaBig {
+    uint32_t* list;   // queued pair indices
+    uint32_t cap;     // queue slots
+    int faces_per;    // faces == horizon entries per slot
+    Face* faces;
+    int (*hor)[2];
+    int* next;
+    uint8_t* live;
+    int* tab;         // 3 * EPA_TAB per slot: head, tail, stamp
+};
+
+template <bool SPHERES_ONLY>
+__global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
+                                                const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                uint32_t* __restrict__ ncont, double* __restrict__ ct, EpaBig big) {
+    const uint32_t np = counters[CNT_P];
+    uint32_t* fault = counters + CNT_FAULT;
+    SupV verts[SPHERES_ONLY ? 1 : EPA_MAXV];
+    Face faces[SPHERES_ONLY ? 1 : EPA_MAXF];
+    int hor[SPHERES_ONLY ? 1 : EPA_MAXH][2];
+    EpaWS ws{};
+    ws.verts = verts; ws.faces = faces; ws.maxf = EPA_MAXF; ws.big = false;
+    ws.list.hor = hor; ws.list.cap = EPA_MAXH;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        ws.overflow = false;
+        uint32_t nc = narrow_pair<SPHERES_ONLY>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
+        if (!SPHERES_ONLY && ws.overflow) {
+            const uint32_t slot = atomicAdd(&counters[CNT_EPABIG], 1u);
+            if (slot < big.cap) big.list[slot] = p;
+            else atomicOr(fault, NE_FAULT_EPA_QUEUE);
+            nc = 0;
+        }
+        ncont[p] = nc;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_narrow_big(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
+                                                   const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                   uint32_t* __restrict__ ncont, double* __restrict__ ct, EpaBig big) {
+    const uint32_t nbig = min(counters[CNT_EPABIG], big.cap);
+    uint32_t* fault = counters + CNT_FAULT;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && nbig) atomicAdd(&counters[CNT_EPARERUN], nbig);
+    SupV verts[EPA_MAXV];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nbig; t += gridDim.x * blockDim.x) {
+        const uint32_t p = big.list[t];
+        const size_t fo = (size_t)t * (size_t)big.faces_per;
+        int* tab = big.tab + (size_t)t * 3 * EPA_TAB;
+        for (int k = 0; k < EPA_TAB; ++k) tab[2 * EPA_TAB + k] = 0;   // stamps: no queue written yet
+        EpaWS ws{};
+        ws.verts = verts; ws.faces = big.faces + fo; ws.maxf = big.faces_per; ws.big = true;
+        ws.table.hor = big.hor + fo; ws.table.next = big.next + fo; ws.table.live = big.live + fo;
+        ws.table.head = tab; ws.table.tail = tab + EPA_TAB; ws.table.stamp = tab + 2 * EPA_TAB;
+        ws.table.cap = big.faces_per; ws.table.it = 0;
+        ws.overflow = false;
+        ws.truncate = big.faces_per >= EPA_BIG_FACES_MAX;
+        uint32_t nc = narrow_pair<false>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
+        if (ws.overflow) {
+            if (ws.truncate) {
+                atomicAdd(&counters[CNT_EPATRUNC], 1u);
+            } else {
+                atomicOr(fault, NE_FAULT_EPA_BIG);
+                nc = 0;
             }
         }
         ncont[p] = nc;
@@ -911,10 +1076,14 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
         if (!nc) continue;
         const uint32_t i = cand_i[p], j = cand_j[p];
         const double ima = B.im[i], imb = B.im[j];
-        const double ra = B.rest[i] < 0 ? P.restitution : B.rest[i];
-        const double rb = B.rest[j] < 0 ? P.restitution : B.rest[j];
-        const double fa = B.fric[i] < 0 ? P.friction : B.fric[i];
-        const double fb = B.fric[j] < 0 ? P.friction : B.fric[j];
+        const WorldDev* wd = B.W ? &B.W[B.wid[i]] : nullptr;
+        const Box3& box = wd ? wd->box : P.box;
+        const double wrest = wd ? wd->restitution : P.restitution;
+        const double wfric = wd ? wd->friction : P.friction;
+        const double ra = B.rest[i] < 0 ? wrest : B.rest[i];
+        const double rb = B.rest[j] < 0 ? wrest : B.rest[j];
+        const double fa = B.fric[i] < 0 ? wfric : B.fric[i];
+        const double fb = B.fric[j] < 0 ? wfric : B.fric[j];
         const double e = combine_mat(P.rcomb, ra, rb);
         const double mu = combine_mat(P.fcomb, fa, fb);
         const V3 xa = ld3(B.x0, B.x1, B.x2, i), xb = ld3(B.x0, B.x1, B.x2, j);
@@ -939,8 +1108,8 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
             if (k >= ccap) break;
             const double* c = ct + (size_t)p * 28 + 7 * s;
             const V3 n{c[0], c[1], c[2]}, pt{c[3], c[4], c[5]};
-            const V3 rA = min_image(P.box, pt - xa);
-            const V3 rB = min_image(P.box, pt - xb);
+            const V3 rA = min_image(box, pt - xa);
+            const V3 rB = min_image(box, pt - xb);
             const V3 vc = (va + cross(wa, rA)) - (vb + cross(wb, rB));
             const double vn = dot(vc, n);
             double nimp = 0.0;
@@ -1149,53 +1318,13 @@ __device__ __forceinline__ void sort_segment(uint32_t* a, uint32_t m) {
     }
 }
 
-// Body-state stores for the sequential island sweep: global memory, or the
-// island's few bodies cached in this thread's slice of LDS.
+// Body-state store for the sequential island sweep in global memory.
 struct GlobalStore {
     BodiesDev& B;
     __device__ BodyView load(uint32_t b) const { return load_view(B, b); }
     __device__ void store(uint32_t b, const BodyView& o) const { store_view(B, b, o); }
     __device__ V3 loadx(uint32_t b) const { return ld3(B.x0, B.x1, B.x2, b); }
     __device__ void storex(uint32_t b, V3 x) const { st3(B.x0, B.x1, B.x2, b, x); }
-};
-
-constexpr int CACHE_BODIES = 8;
-
-struct LdsStore {
-    BodiesDev& B;
-    const uint32_t* ids;   // cached body indices
-    int nb;
-    V3* v;
-    V3* w;
-    V3* x;
-    __device__ int slot(uint32_t b) const {
-        int s = 0;
-        for (int k = 0; k < nb; ++k) s = (ids[k] == b) ? k : s;
-        return s;
-    }
-    __device__ BodyView load(uint32_t b) const {
-        BodyView o = load_view_static(b);
-        const int s = slot(b);
-        o.v = v[s];
-        o.w = w[s];
-        return o;
-    }
-    __device__ BodyView load_view_static(uint32_t b) const {
-        BodyView o;
-        o.q = ldq(B, b);
-        o.ii = ld3(B.ii0, B.ii1, B.ii2, b);
-        o.im = B.im[b];
-        o.mv = o.im != 0.0;
-        o.mw = o.ii.x != 0.0 || o.ii.y != 0.0 || o.ii.z != 0.0;
-        return o;
-    }
-    __device__ void store(uint32_t b, const BodyView& o) const {
-        const int s = slot(b);
-        v[s] = o.v;
-        w[s] = o.w;
-    }
-    __device__ V3 loadx(uint32_t b) const { return x[slot(b)]; }
-    __device__ void storex(uint32_t b, V3 xx) const { x[slot(b)] = xx; }
 };
 
 template <class Store>
@@ -1509,7 +1638,7 @@ __global__ void __launch_bounds__(TPB) k_wrap(BodiesDev B, SimParams P) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.n) return;
     if (!(B.im[i] > 0.0)) return;
-    st3(B.x0, B.x1, B.x2, i, wrap_pos(P.box, ld3(B.x0, B.x1, B.x2, i)));
+    st3(B.x0, B.x1, B.x2, i, wrap_pos(box_of(B, P, i), ld3(B.x0, B.x1, B.x2, i)));
 }
 
 __global__ void __launch_bounds__(TPB) k_walls(BodiesDev B, SimParams P) {
@@ -1609,6 +1738,7 @@ struct ne_gpu_ctx {
     uint32_t* excl = nullptr;
     uint32_t nexcl = 0;
     double cell = 1.0;
+    double grid_maxR = 0.0;
     double* aabb_part = nullptr;
     GridDev* grid = nullptr;
     uint32_t *cell_of = nullptr, *cell_count = nullptr, *cell_start = nullptr;
@@ -1632,6 +1762,14 @@ struct ne_gpu_ctx {
     uint32_t *parent = nullptr, *cons_root = nullptr, *used = nullptr, *root_id = nullptr;
     uint32_t *skey0 = nullptr, *sval0 = nullptr, *skey1 = nullptr, *sval1 = nullptr;
     uint32_t* isl_start = nullptr;
+    EpaBig epa_big{};                    // pairs rerun with the large EPA workspace
+    // start-of-call snapshot (rerun after a capacity overflow)
+    double* snap_body = nullptr;
+    size_t snap_nd = 0;
+    uint64_t* snap_key = nullptr;
+    double* snap_imp = nullptr;
+    uint32_t* snap_n = nullptr;
+    uint32_t snap_pcap = 0, snap_prev = 0;
     uint32_t *rs_hist = nullptr, *scan_part = nullptr;
     uint32_t *slot_of = nullptr, *row_scratch = nullptr, *claimed = nullptr, *cons_isl = nullptr;
     uint32_t *isl_count = nullptr, *isl_fill = nullptr, *isl_list = nullptr, *big_list = nullptr;
@@ -1641,6 +1779,25 @@ struct ne_gpu_ctx {
     uint32_t cells_per_body = 2;
     GridDev grid_host{};
     uint32_t sgrid_cells = 0;
+    // host copies from the last upload (grid plans are rebuilt from them)
+    std::vector<double> h_R;          // bounding radii
+    std::vector<uint32_t> h_excl;     // grid-excluded bodies, ascending
+    std::vector<uint8_t> h_isexcl;
+    bool grid_dirty = true;           // the static grid / world table must be rebuilt
+    size_t ex_cap = 0;                // capacity of ex_flags / ex_off
+    // batched worlds (nworlds == 0: one world)
+    uint32_t nworlds = 0;
+    std::vector<uint32_t> w_off;
+    std::vector<double> w_half, w_rest, w_fric;
+    uint32_t w_maxsize = 0;
+    size_t ex_total = 0;
+    uint32_t* d_wid = nullptr;
+    WorldDev* d_worlds = nullptr;
+    // HIP graphs of one step: [wrap_first][cell-order buffer parity]
+    bool use_graph = false;
+    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    uint32_t graph_hint_p = 0, graph_hint_c = 0;
+    int order_parity = 0;
     // counters
     uint32_t* counters = nullptr;
     uint32_t* h_counters = nullptr;
@@ -1713,6 +1870,14 @@ hipError_t dalloc(T*& p, size_t count) {
     return hipMalloc((void**)&p, sizeof(T) * count);
 }
 
+void drop_graphs(ne_gpu_ctx* c) {
+    for (auto& row : c->gexec)
+        for (hipGraphExec_t& g : row) {
+            if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
 enum DF : int {
     F_X0, F_X1, F_X2, F_V0, F_V1, F_V2, F_Q0, F_Q1, F_Q2, F_Q3, F_W0, F_W1, F_W2,
     F_F0, F_F1, F_F2, F_T0, F_T1, F_T2, F_IM, F_II0, F_II1, F_II2,
@@ -1781,6 +1946,7 @@ uint32_t gs_blocks(uint64_t n) {   // grid-stride launches over device-side coun
 }
 
 int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
+    drop_graphs(c);
     c->pcap = pcap;
     HIP_TRY(c, dalloc(c->cand_i, pcap));
     HIP_TRY(c, dalloc(c->cand_j, pcap));
@@ -1793,6 +1959,7 @@ int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
 }
 
 int alloc_cons(ne_gpu_ctx* c, uint32_t ccap) {
+    drop_graphs(c);
     c->ccap = ccap;
     ConsDev& C = c->C;
     HIP_TRY(c, dalloc(C.ci, ccap));
@@ -1812,8 +1979,24 @@ int alloc_cons(ne_gpu_ctx* c, uint32_t ccap) {
     return NE_OK;
 }
 
+int alloc_epa_big(ne_gpu_ctx* c, uint32_t cap, int faces_per) {
+    drop_graphs(c);
+    EpaBig& g = c->epa_big;
+    g.cap = cap;
+    g.faces_per = faces_per;
+    const size_t nf = (size_t)cap * (size_t)faces_per;
+    HIP_TRY(c, dalloc(g.list, cap));
+    HIP_TRY(c, dalloc(g.faces, nf));
+    HIP_TRY(c, dalloc(g.hor, nf));
+    HIP_TRY(c, dalloc(g.next, nf));
+    HIP_TRY(c, dalloc(g.live, nf));
+    HIP_TRY(c, dalloc(g.tab, (size_t)cap * 3 * EPA_TAB));
+    return NE_OK;
+}
+
 int ensure_prev(ne_gpu_ctx* c, uint32_t cap) {
     if (c->prev_cap >= cap && c->prev_key) return NE_OK;
+    drop_graphs(c);
     uint64_t* nk = nullptr;
     double* ni = nullptr;
     HIP_TRY(c, hipMalloc((void**)&nk, sizeof(uint64_t) * std::max(cap, 1u)));
@@ -1850,11 +2033,20 @@ void free_work(ne_gpu_ctx* c) {
     dfree(c->slot_of); dfree(c->row_scratch); dfree(c->claimed); dfree(c->cons_isl);
     dfree(c->isl_count); dfree(c->isl_fill); dfree(c->isl_list); dfree(c->big_list);
     dfree(c->rs_hist); dfree(c->scan_part);
+    dfree(c->d_wid); dfree(c->d_worlds);
+    dfree(c->epa_big.list); dfree(c->epa_big.faces); dfree(c->epa_big.hor); dfree(c->epa_big.next);
+    dfree(c->epa_big.live); dfree(c->epa_big.tab);
+    c->epa_big.cap = 0;
+    c->ex_cap = 0;
+    drop_graphs(c);
 }
 
 void free_all(ne_gpu_ctx* c) {
     free_work(c);
     dfree(c->prev_key); dfree(c->prev_imp); dfree(c->prev_n); dfree(c->counters);
+    dfree(c->snap_body); dfree(c->snap_key); dfree(c->snap_imp); dfree(c->snap_n);
+    c->snap_nd = 0;
+    c->snap_pcap = 0;
     c->prev_cap = 0;
     if (c->h_counters) { (void)hipHostFree(c->h_counters); c->h_counters = nullptr; }
 }
@@ -1906,12 +2098,13 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
         {
             KTimer kt(c, NE_KERNEL_CELL_FILL);
             std::swap(c->sorted_idx, c->order_prev);   // last step's cell order drives this fill
-            k_cell_fill<<<blocks_for(c->ngrid), TPB, 0, s>>>(c->order_prev, c->ngrid, c->grid, c->xp, c->cell_start,
-                                                             c->sorted_idx, c->sorted);
+            c->order_parity ^= 1;
+            k_cell_fill<<<blocks_for(c->ngrid), TPB, 0, s>>>(c->order_prev, c->ngrid, c->grid, B.wid, B.W, c->xp,
+                                                             c->cell_start, c->sorted_idx, c->sorted);
         }
         // --- candidate rows: one neighbourhood walk per body
         KTimer kt(c, NE_KERNEL_CAND_COUNT);
-        if (static_grid && c->broadphase == NE_BROADPHASE_BRICK && c->sgrid.nc[0] >= BRICK_X + 2 && c->sgrid.nc[1] >= BRICK_Y + 2 &&
+        if (static_grid && c->nworlds == 0 && c->broadphase == NE_BROADPHASE_BRICK && c->sgrid.nc[0] >= BRICK_X + 2 && c->sgrid.nc[1] >= BRICK_Y + 2 &&
             c->sgrid.nc[2] >= BRICK_Z + 2) {
             const uint32_t nb = (uint32_t)((c->sgrid.nc[0] + BRICK_X - 1) / BRICK_X) *
                                 (uint32_t)((c->sgrid.nc[1] + BRICK_Y - 1) / BRICK_Y) *
@@ -1929,14 +2122,15 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
                                                           c->excl, c->nexcl, c->row_count, c->row_scratch, c->slot_of);
         }
     }
+    const uint32_t ex_width = c->nworlds ? c->w_maxsize : n;   // flags per excluded row (at most)
     if (c->nexcl > 0) {
-        dim3 g(blocks_for(n), c->nexcl);
+        dim3 g(blocks_for(ex_width), c->nexcl);
         k_excl_flags<<<g, TPB, 0, s>>>(B, P, c->excl, c->ex_flags);
-        scan_u32(c->ex_flags, c->ex_off, nullptr, c->nexcl * n, c->scan_part, s);
-        k_excl_rowcount<<<blocks_for(c->nexcl), TPB, 0, s>>>(c->excl, c->nexcl, n, c->ex_off, c->row_count);
+        scan_u32(c->ex_flags, c->ex_off, nullptr, (uint32_t)c->ex_total, c->scan_part, s);
+        k_excl_rowcount<<<blocks_for(c->nexcl), TPB, 0, s>>>(B, c->excl, c->nexcl, c->ex_off, c->row_count);
     }
     scan_u32(c->row_count, c->row_start, nullptr, n, c->scan_part, s, false,
-             ScanTotal{c->counters, CNT_PRAW, CNT_P, CNT_FAULT, c->pcap, NE_FAULT_PAIRS, -1, -1});
+             ScanTotal{c->counters, CNT_PRAW, CNT_P, CNT_FAULT, c->pcap, NE_FAULT_PAIRS, CNT_EPABIG, -1});
     if (c->ngrid > 0) {
         KTimer kt(c, NE_KERNEL_CAND_WRITE);
         k_cand_emit<<<blocks_for(n), TPB, 0, s>>>(B, P, c->grid, c->slot_of, c->sorted_idx, c->sorted, c->cell_start,
@@ -1944,18 +2138,20 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
                                                   c->cand_i, c->cand_j);
     }
     if (c->nexcl > 0) {
-        dim3 g(blocks_for(n), c->nexcl);
-        k_excl_write<<<g, TPB, 0, s>>>(n, c->excl, c->ex_flags, c->ex_off, c->row_start, c->pcap, c->cand_i, c->cand_j);
+        dim3 g(blocks_for(ex_width), c->nexcl);
+        k_excl_write<<<g, TPB, 0, s>>>(B, c->excl, c->ex_flags, c->ex_off, c->row_start, c->pcap, c->cand_i, c->cand_j);
     }
     // --- narrow phase + constraints
     {
         KTimer kt(c, NE_KERNEL_NARROW);
-        if (c->spheres_only)
+        if (c->spheres_only) {
             k_narrow<true><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                              c->counters + CNT_FAULT);
-        else
-            k_narrow<false><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                               c->counters + CNT_FAULT);
+                                                              c->epa_big);
+        } else {
+            k_narrow<false><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont,
+                                                               c->ct, c->epa_big);
+            k_narrow_big<<<16, 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_big);
+        }
     }
     scan_u32(c->ncont, c->cons_off, c->counters + CNT_P, c->pcap, c->scan_part, s, false,
              ScanTotal{c->counters, CNT_CRAW, CNT_C, CNT_FAULT, c->ccap, NE_FAULT_CONTACTS, CNT_ISL, CNT_NBIG});
@@ -1993,35 +2189,211 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
 
 int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b);
 
+// Rows of grid-excluded bodies: one flag per (excluded body, body of its
+// world); the flag and offset arrays are sized for that.
+int ensure_excl_flags(ne_gpu_ctx* c) {
+    size_t total = 0;
+    if (c->nworlds) {
+        total = c->ex_total;
+    } else {
+        total = (size_t)c->nexcl * c->n;
+    }
+    if (total >= (1ull << 31))
+        return set_err(c, NE_ERR_UNSUPPORTED, "too many grid-excluded bodies for this body count");
+    c->ex_total = total;
+    if (total == 0 || total <= c->ex_cap) return NE_OK;
+    drop_graphs(c);
+    HIP_TRY(c, dalloc(c->ex_flags, total));
+    HIP_TRY(c, dalloc(c->ex_off, total + 1));
+    c->ex_cap = total;
+    return NE_OK;
+}
+
+// Cell count along one periodic axis of length L for cells >= `cell` wide.
+int32_t axis_cells(double L, double cell) {
+    double k = std::floor(L / cell);
+    if (!(k >= 1.0)) k = 1.0;
+    if (k > 1048576.0) k = 1048576.0;
+    return (int32_t)k;
+}
+
+// Batched worlds: each world's periodic grid (cells >= 2*maxR of its gridded
+// bodies, times cell_scale) gets a slice of the shared cell table; all worlds
+// are coarsened together until the table fits its capacity.
+int prepare_worlds(ne_gpu_ctx* c) {
+    const uint32_t nw = c->nworlds;
+    std::vector<double> maxR(nw, 0.0);
+    std::vector<uint32_t> e0(nw, 0), e1(nw, 0);
+    for (uint32_t w = 0; w < nw; ++w) {
+        for (uint32_t i = c->w_off[w]; i < c->w_off[w + 1]; ++i)
+            if (!c->h_isexcl[i]) maxR[w] = std::max(maxR[w], c->h_R[i]);
+        e0[w] = (uint32_t)(std::lower_bound(c->h_excl.begin(), c->h_excl.end(), c->w_off[w]) - c->h_excl.begin());
+        e1[w] = (uint32_t)(std::lower_bound(c->h_excl.begin(), c->h_excl.end(), c->w_off[w + 1]) - c->h_excl.begin());
+    }
+    std::vector<WorldDev> W(nw);
+    double grow = 1.0;
+    for (int attempt = 0;; ++attempt) {
+        uint64_t total = 0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            const double cell = (2.0 * maxR[w] * (1.0 + 1e-6) + 1e-9) * c->cell_scale * grow;
+            GridDev g{};
+            for (int a = 0; a < 3; ++a) {
+                const double L = 2.0 * c->w_half[3 * w + a];
+                g.nc[a] = maxR[w] > 0.0 ? axis_cells(L, cell) : 1;
+                g.cw[a] = L / (double)g.nc[a];
+                g.inv_cw[a] = 1.0 / g.cw[a];
+                g.origin[a] = -c->w_half[3 * w + a];
+            }
+            g.periodic = 1;
+            g.ncells = (uint32_t)g.nc[0] * (uint32_t)g.nc[1] * (uint32_t)g.nc[2];
+            W[w].grid = g;
+            W[w].cell_base = (uint32_t)std::min<uint64_t>(total, 0xFFFFFFFFull);
+            total += g.ncells;
+        }
+        if (total <= c->cellcap) { c->sgrid_cells = (uint32_t)total; break; }
+        if (attempt > 200) return set_err(c, NE_ERR_CAPACITY, "cell table cannot hold the worlds' grids");
+        grow *= 1.25;
+    }
+    size_t flag_base = 0;
+    for (uint32_t w = 0; w < nw; ++w) {
+        WorldDev& d = W[w];
+        for (int a = 0; a < 3; ++a) {
+            d.box.half[a] = c->w_half[3 * w + a];
+            d.box.edge[a] = 2.0 * c->w_half[3 * w + a];
+            d.box.inv_edge[a] = 1.0 / d.box.edge[a];
+        }
+        d.box.periodic = 1;
+        d.restitution = c->w_rest[w];
+        d.friction = c->w_fric[w];
+        d.body_begin = c->w_off[w];
+        d.body_end = c->w_off[w + 1];
+        d.excl_begin = e0[w];
+        d.excl_end = e1[w];
+        d.flag_base = (uint32_t)std::min<size_t>(flag_base, 0xFFFFFFFFu);
+        flag_base += (size_t)(e1[w] - e0[w]) * (d.body_end - d.body_begin);
+    }
+    c->ex_total = flag_base;
+    HIP_TRY(c, dalloc(c->d_worlds, std::max<uint32_t>(nw, 1u)));
+    HIP_TRY(c, hipMemcpy(c->d_worlds, W.data(), sizeof(WorldDev) * nw, hipMemcpyHostToDevice));
+    GridDev gd{};
+    gd.periodic = 1;
+    gd.ncells = c->sgrid_cells;   // the cell-major walk visits every world's cells
+    c->grid_host = gd;
+    HIP_TRY(c, hipMemcpy(c->grid, &c->grid_host, sizeof(GridDev), hipMemcpyHostToDevice));
+    c->B.W = c->d_worlds;
+    c->B.wid = c->d_wid;
+    return NE_OK;
+}
+
 // Periodic worlds: cells of width L/nc >= the gridded bodies' 2*maxR, with the
 // cell table kept within its capacity; the same rule k_grid_setup applies.
 int prepare_static_grid(ne_gpu_ctx* c) {
+    if (!c->grid_dirty) return NE_OK;
+    drop_graphs(c);
     c->sgrid_cells = 0;
-    if (!c->params.periodic || c->ngrid == 0) return NE_OK;
-    StaticGrid g{};
-    GridDev gd{};
-    for (int a = 0; a < 3; ++a) {
-        const double L = 2.0 * c->params.half[a];
-        double k = std::floor(L / c->cell);
-        if (k < 1.0) k = 1.0;
-        if (k > 1048576.0) k = 1048576.0;
-        g.nc[a] = (int32_t)k;
+    c->B.W = nullptr;
+    c->B.wid = nullptr;
+    if (c->nworlds) {
+        int r = prepare_worlds(c);
+        if (r != NE_OK) return r;
+    } else if (c->params.periodic && c->ngrid > 0) {
+        StaticGrid g{};
+        GridDev gd{};
+        for (int a = 0; a < 3; ++a) g.nc[a] = axis_cells(2.0 * c->params.half[a], c->cell);
+        while ((double)g.nc[0] * g.nc[1] * g.nc[2] > (double)c->cellcap)
+            for (int a = 0; a < 3; ++a) g.nc[a] = std::max(1, g.nc[a] * 4 / 5);
+        for (int a = 0; a < 3; ++a) {
+            g.cw[a] = 2.0 * c->params.half[a] / (double)g.nc[a];
+            g.inv_cw[a] = 1.0 / g.cw[a];
+            g.origin[a] = -c->params.half[a];
+            gd.nc[a] = g.nc[a]; gd.cw[a] = g.cw[a]; gd.origin[a] = g.origin[a];
+            gd.inv_cw[a] = 1.0 / g.cw[a];
+        }
+        gd.periodic = 1;
+        gd.ncells = (uint32_t)g.nc[0] * (uint32_t)g.nc[1] * (uint32_t)g.nc[2];
+        c->sgrid = g;
+        c->sgrid_cells = gd.ncells;
+        c->grid_host = gd;
+        HIP_TRY(c, hipMemcpy(c->grid, &c->grid_host, sizeof(GridDev), hipMemcpyHostToDevice));
     }
-    while ((double)g.nc[0] * g.nc[1] * g.nc[2] > (double)c->cellcap)
-        for (int a = 0; a < 3; ++a) g.nc[a] = std::max(1, g.nc[a] * 4 / 5);
-    for (int a = 0; a < 3; ++a) {
-        g.cw[a] = 2.0 * c->params.half[a] / (double)g.nc[a];
-        g.inv_cw[a] = 1.0 / g.cw[a];
-        g.origin[a] = -c->params.half[a];
-        gd.nc[a] = g.nc[a]; gd.cw[a] = g.cw[a]; gd.origin[a] = g.origin[a];
-        gd.inv_cw[a] = 1.0 / g.cw[a];
+    int r = ensure_excl_flags(c);
+    if (r != NE_OK) return r;
+    c->grid_dirty = false;
+    return NE_OK;
+}
+
+// One step, eagerly or as a replay of its captured graph.  A graph bakes the
+// cell-order buffers in (the fill reads the previous order and writes the new
+// one), so one graph is kept per buffer parity and per wrap-first flag.
+int run_step(ne_gpu_ctx* c, bool wrap_first) {
+    const bool graphable = c->use_graph && !c->has_forces && c->timed_kernel == NE_KERNEL_NONE;
+    if (!graphable) return enqueue_step(c, wrap_first);
+    // launch sizes follow the counts seen when captured: recapture when they drift
+    auto drift = [](uint32_t now, uint32_t then) { return now > 2u * then + 1024u || 2u * now + 1024u < then; };
+    if (drift(c->hint_p, c->graph_hint_p) || drift(c->hint_c, c->graph_hint_c)) drop_graphs(c);
+    hipGraphExec_t& ex = c->gexec[wrap_first ? 1 : 0][c->order_parity];
+    if (!ex) {
+        if (!c->gexec[0][0] && !c->gexec[0][1] && !c->gexec[1][0] && !c->gexec[1][1]) {
+            c->graph_hint_p = c->hint_p;
+            c->graph_hint_c = c->hint_c;
+        }
+        uint32_t* si = c->sorted_idx;
+        uint32_t* op = c->order_prev;
+        const int par = c->order_parity;
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int r = enqueue_step(c, wrap_first);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        c->sorted_idx = si;   // nothing ran yet: undo the host-side bookkeeping
+        c->order_prev = op;
+        c->order_parity = par;
+        if (r != NE_OK) { if (graph) (void)hipGraphDestroy(graph); return r; }
+        if (e != hipSuccess) return set_err(c, NE_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        const hipError_t e2 = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e2 != hipSuccess) {
+            ex = nullptr;
+            return set_err(c, NE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+        }
     }
-    gd.periodic = 1;
-    gd.ncells = (uint32_t)g.nc[0] * (uint32_t)g.nc[1] * (uint32_t)g.nc[2];
-    c->sgrid = g;
-    c->sgrid_cells = gd.ncells;
-    c->grid_host = gd;
-    HIP_TRY(c, hipMemcpy(c->grid, &c->grid_host, sizeof(GridDev), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipGraphLaunch(ex, c->stream));
+    std::swap(c->sorted_idx, c->order_prev);   // what the replayed fill did
+    c->order_parity ^= 1;
+    return NE_OK;
+}
+
+int step_call(ne_gpu_ctx* c, int32_t nsteps);
+
+// Start-of-call copy of what a step changes (pose, velocity, force and torque
+// fields, which lead the field table, and the warm cache), taken on the
+// stream, so a call that overflows a buffer can be run again from its start.
+int snapshot_take(ne_gpu_ctx* c) {
+    const size_t nd = (size_t)(F_T2 + 1) * c->ncap;
+    if (c->snap_nd < nd) {
+        HIP_TRY(c, dalloc(c->snap_body, nd));
+        c->snap_nd = nd;
+    }
+    if (c->snap_pcap < c->prev_cap) {
+        HIP_TRY(c, dalloc(c->snap_key, c->prev_cap));
+        HIP_TRY(c, dalloc(c->snap_imp, c->prev_cap));
+        c->snap_pcap = c->prev_cap;
+    }
+    if (!c->snap_n) HIP_TRY(c, dalloc(c->snap_n, 1));
+    c->snap_prev = c->prev_cap;
+    HIP_TRY(c, hipMemcpyAsync(c->snap_body, c->dfield, sizeof(double) * nd, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->snap_key, c->prev_key, sizeof(uint64_t) * c->prev_cap, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->snap_imp, c->prev_imp, sizeof(double) * c->prev_cap, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->snap_n, c->prev_n, sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    return NE_OK;
+}
+
+int snapshot_restore(ne_gpu_ctx* c) {
+    const size_t nd = (size_t)(F_T2 + 1) * c->ncap;
+    const uint32_t m = std::min(c->snap_prev, c->prev_cap);
+    HIP_TRY(c, hipMemcpyAsync(c->dfield, c->snap_body, sizeof(double) * nd, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->prev_key, c->snap_key, sizeof(uint64_t) * m, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->prev_imp, c->snap_imp, sizeof(double) * m, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->prev_n, c->snap_n, sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
     return NE_OK;
 }
 
@@ -2101,10 +2473,11 @@ int ne_gpu_set_params(ne_gpu_ctx* c, const ne_world_params* p) {
     if (p->restitution_combine < 0 || p->restitution_combine > 4 || p->friction_combine < 0 || p->friction_combine > 4)
         return set_err(c, NE_ERR_INVALID, "unknown combine mode");
     if (p->sleep_enabled) return set_err(c, NE_ERR_UNSUPPORTED, "island sleeping is not on the GPU path yet");
-    const bool periodic_changed = c->have_params && ((p->periodic != 0) != (c->params.periodic != 0));
+    if (c->nworlds && !p->periodic) return set_err(c, NE_ERR_UNSUPPORTED, "batched worlds must be periodic");
     c->params = *p;
     c->have_params = true;
-    (void)periodic_changed;
+    c->grid_dirty = true;
+    drop_graphs(c);
     return NE_OK;
 }
 
@@ -2142,14 +2515,30 @@ int ne_gpu_step(ne_gpu_ctx* c, int32_t nsteps) {
     if (!c) return NE_ERR_INVALID;
     if (nsteps < 0) return set_err(c, NE_ERR_INVALID, "nsteps < 0");
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
-    if (c->n == 0) return NE_OK;
-    {
-        int r = prepare_static_grid(c);
+    if (c->n == 0 || nsteps == 0) return NE_OK;
+    int r = snapshot_take(c);
+    if (r != NE_OK) return r;
+    const bool forces = c->has_forces;
+    for (int attempt = 0;; ++attempt) {
+        r = prepare_static_grid(c);
         if (r != NE_OK) return r;
+        r = step_call(c, nsteps);
+        if (r != NE_ERR_CAPACITY || attempt == 4) return r;
+        // a buffer overflowed and has been enlarged: rerun the call from its start
+        r = snapshot_restore(c);
+        if (r != NE_OK) return r;
+        c->has_forces = forces;
     }
+}
+
+}  // extern "C"
+
+namespace {
+
+int step_call(ne_gpu_ctx* c, int32_t nsteps) {
     HIP_TRY(c, hipMemsetAsync(c->counters, 0, sizeof(uint32_t) * CNT_N, c->stream));
     for (int32_t t = 0; t < nsteps; ++t) {
-        int r = enqueue_step(c, t > 0);
+        int r = run_step(c, t > 0);
         if (r != NE_OK) return r;
     }
     if (c->params.periodic && nsteps > 0) {
@@ -2170,6 +2559,15 @@ int ne_gpu_step(ne_gpu_ctx* c, int32_t nsteps) {
             int r = alloc_pairs(c, want);
             if (r != NE_OK) return r;
         }
+        if (fault & (NE_FAULT_EPA_QUEUE | NE_FAULT_EPA_BIG)) {
+            uint32_t slots = c->epa_big.cap;
+            int per = c->epa_big.faces_per;
+            if (fault & NE_FAULT_EPA_QUEUE)
+                slots = std::max(slots * 2u, c->h_counters[CNT_EPABIG] + c->h_counters[CNT_EPABIG] / 2u + 16u);
+            if (fault & NE_FAULT_EPA_BIG) per = std::min(per * 4, EPA_BIG_FACES_MAX);
+            int r = alloc_epa_big(c, slots, per);
+            if (r != NE_OK) return r;
+        }
         if (fault & NE_FAULT_CONTACTS) {
             uint32_t want = std::max(c->ccap * 2u, c->h_counters[CNT_CRAW] + c->h_counters[CNT_CRAW] / 2u + 1024u);
             int r = alloc_cons(c, want);
@@ -2184,6 +2582,10 @@ int ne_gpu_step(ne_gpu_ctx* c, int32_t nsteps) {
     }
     return NE_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int ne_gpu_get_state(ne_gpu_ctx* c, double* x, double* v, double* q, double* w) {
     if (!c) return NE_ERR_INVALID;
@@ -2399,6 +2801,149 @@ int ne_gpu_kernel_time(ne_gpu_ctx* c, double* total_ms, uint32_t* launches) {
     return NE_OK;
 }
 
+
+int ne_gpu_set_worlds(ne_gpu_ctx* c, uint32_t nworlds, const uint32_t* offsets, const double* half,
+                      const double* restitution, const double* friction) {
+    if (!c) return NE_ERR_INVALID;
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    if (nworlds == 0) {
+        c->nworlds = 0;
+        c->grid_dirty = true;
+        return NE_OK;
+    }
+    if (!offsets || !half) return set_err(c, NE_ERR_INVALID, "offsets and half are required");
+    if (!c->params.periodic) return set_err(c, NE_ERR_UNSUPPORTED, "batched worlds must be periodic");
+    if (offsets[0] != 0 || offsets[nworlds] != c->n)
+        return set_err(c, NE_ERR_INVALID, "world offsets must run from 0 to the body count");
+    uint32_t maxsize = 0;
+    for (uint32_t w = 0; w < nworlds; ++w) {
+        if (offsets[w + 1] < offsets[w]) return set_err(c, NE_ERR_INVALID, "world offsets must not decrease");
+        maxsize = std::max(maxsize, offsets[w + 1] - offsets[w]);
+        for (int a = 0; a < 3; ++a)
+            if (!(half[3 * w + a] > 0.0)) return set_err(c, NE_ERR_INVALID, "world half extents must be > 0");
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->w_off.assign(offsets, offsets + nworlds + 1);
+    c->w_half.assign(half, half + 3 * (size_t)nworlds);
+    c->w_rest.assign(nworlds, c->params.restitution);
+    c->w_fric.assign(nworlds, c->params.friction);
+    for (uint32_t w = 0; w < nworlds; ++w) {
+        if (restitution) c->w_rest[w] = restitution[w];
+        if (friction) c->w_fric[w] = friction[w];
+    }
+    c->w_maxsize = maxsize;
+    std::vector<uint32_t> wid(c->n);
+    for (uint32_t w = 0; w < nworlds; ++w)
+        for (uint32_t i = offsets[w]; i < offsets[w + 1]; ++i) wid[i] = w;
+    HIP_TRY(c, dalloc(c->d_wid, std::max<uint32_t>(c->n, 1u)));
+    if (c->n) HIP_TRY(c, hipMemcpy(c->d_wid, wid.data(), sizeof(uint32_t) * c->n, hipMemcpyHostToDevice));
+    c->nworlds = nworlds;
+    c->grid_dirty = true;
+    drop_graphs(c);
+    return prepare_static_grid(c);
+}
+
+int ne_gpu_epa_stats(ne_gpu_ctx* c, uint32_t* reruns, uint32_t* truncated) {
+    if (!c) return NE_ERR_INVALID;
+    if (reruns) *reruns = c->h_counters[CNT_EPARERUN];
+    if (truncated) *truncated = c->h_counters[CNT_EPATRUNC];
+    return NE_OK;
+}
+
+int ne_gpu_use_graph(ne_gpu_ctx* c, int32_t enable) {
+    if (!c) return NE_ERR_INVALID;
+    c->use_graph = enable != 0;
+    if (!c->use_graph) drop_graphs(c);
+    return NE_OK;
+}
+
+int ne_gpu_autotune(ne_gpu_ctx* c, int32_t steps, ne_tune_result* out) {
+    if (!c || steps <= 0) return NE_ERR_INVALID;
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    if (c->n == 0) return set_err(c, NE_ERR_INVALID, "no bodies to tune on");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // snapshot of everything a step changes: body fields, warm cache, flags
+    const size_t nd = (size_t)F_COUNT * c->ncap;
+    double* snap = nullptr;
+    uint64_t* skey = nullptr;
+    double* simp = nullptr;
+    uint32_t scap = c->prev_cap, spn = 0;
+    HIP_TRY(c, hipMalloc((void**)&snap, sizeof(double) * nd));
+    if (hipMalloc((void**)&skey, sizeof(uint64_t) * std::max(scap, 1u)) != hipSuccess ||
+        hipMalloc((void**)&simp, sizeof(double) * std::max(scap, 1u)) != hipSuccess) {
+        (void)hipFree(snap); if (skey) (void)hipFree(skey);
+        return set_err(c, NE_ERR_HIP, "autotune snapshot allocation failed");
+    }
+    auto cleanup = [&]() { (void)hipFree(snap); (void)hipFree(skey); (void)hipFree(simp); };
+    HIP_TRY(c, hipMemcpy(snap, c->dfield, sizeof(double) * nd, hipMemcpyDeviceToDevice));
+    HIP_TRY(c, hipMemcpy(skey, c->prev_key, sizeof(uint64_t) * scap, hipMemcpyDeviceToDevice));
+    HIP_TRY(c, hipMemcpy(simp, c->prev_imp, sizeof(double) * scap, hipMemcpyDeviceToDevice));
+    HIP_TRY(c, hipMemcpy(&spn, c->prev_n, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    const bool forces = c->has_forces;
+    auto restore = [&]() -> int {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipMemcpy(c->dfield, snap, sizeof(double) * nd, hipMemcpyDeviceToDevice));
+        const uint32_t m = std::min(scap, c->prev_cap);
+        HIP_TRY(c, hipMemcpy(c->prev_key, skey, sizeof(uint64_t) * m, hipMemcpyDeviceToDevice));
+        HIP_TRY(c, hipMemcpy(c->prev_imp, simp, sizeof(double) * m, hipMemcpyDeviceToDevice));
+        HIP_TRY(c, hipMemcpy(c->prev_n, &spn, sizeof(uint32_t), hipMemcpyHostToDevice));
+        c->has_forces = forces;
+        return NE_OK;
+    };
+    const double scales[3] = {1.0, 1.25, 1.5};
+    const int32_t walks[2] = {NE_BROADPHASE_WALK, NE_BROADPHASE_CELLS};
+    double best = 1e300;
+    ne_tune_result res{};
+    uint32_t trials = 0;
+    for (double sc : scales)
+        for (int32_t bp : walks)
+            for (int g = 0; g < 2; ++g) {
+                c->cell_scale = sc;
+                c->cell = (2.0 * c->grid_maxR * (1.0 + 1e-6) + 1e-9) * sc;
+                c->broadphase = bp;
+                c->use_graph = g != 0;
+                c->grid_dirty = true;
+                drop_graphs(c);
+                double ms = 1e300;
+                for (int attempt = 0; attempt < 3; ++attempt) {
+                    int r = restore();
+                    if (r != NE_OK) { cleanup(); return r; }
+                    r = ne_gpu_step(c, 1);   // warm-up: builds graphs, sizes launches
+                    if (r == NE_ERR_CAPACITY) continue;
+                    if (r != NE_OK) { cleanup(); return r; }
+                    r = restore();
+                    if (r != NE_OK) { cleanup(); return r; }
+                    const auto t0 = std::chrono::steady_clock::now();
+                    r = ne_gpu_step(c, steps);
+                    const auto t1 = std::chrono::steady_clock::now();
+                    if (r == NE_ERR_CAPACITY) continue;
+                    if (r != NE_OK) { cleanup(); return r; }
+                    ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
+                    break;
+                }
+                ++trials;
+                if (ms < best) {
+                    best = ms;
+                    res.cell_scale = sc;
+                    res.broadphase = bp;
+                    res.graph = g;
+                    res.ms_per_step = ms;
+                }
+            }
+    res.trials = trials;
+    c->cell_scale = res.cell_scale;
+    c->cell = (2.0 * c->grid_maxR * (1.0 + 1e-6) + 1e-9) * res.cell_scale;
+    c->broadphase = res.broadphase;
+    c->use_graph = res.graph != 0;
+    c->grid_dirty = true;
+    drop_graphs(c);
+    int r = restore();
+    cleanup();
+    if (r != NE_OK) return r;
+    if (out) *out = res;
+    return best < 1e300 ? NE_OK : set_err(c, NE_ERR_CAPACITY, "every autotune trial overflowed");
+}
+
 int ne_gpu_fp_probe(ne_gpu_ctx* c, int32_t op, uint32_t n, const double* a, const double* b, double* out) {
     if (!c || !a || !out || (op == 1 && !b)) return NE_ERR_INVALID;
     if (n == 0) return NE_OK;
@@ -2477,6 +3022,8 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
         if (r != NE_OK) return r;
         r = ensure_prev(c, ccap);
         if (r != NE_OK) return r;
+        r = alloc_epa_big(c, EPA_BIG_PAIRS, EPA_BIG_FACES);
+        if (r != NE_OK) return r;
         // scan scratch: partial sums (<=1024) and radix histograms
         HIP_TRY(c, dalloc(c->scan_part, 1024));
         HIP_TRY(c, dalloc(c->rs_hist, (size_t)RS_BINS * 1024 + 1));
@@ -2543,11 +3090,17 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
         if (ex) excl.push_back(i);
         else { gridded.push_back(i); maxR = std::max(maxR, R[i]); }
     }
-    if ((uint64_t)excl.size() * (uint64_t)n >= (1ull << 31))
-        return set_err(c, NE_ERR_UNSUPPORTED, "too many grid-excluded bodies for this body count");
     c->ngrid = (uint32_t)gridded.size();
     c->nexcl = (uint32_t)excl.size();
+    c->grid_maxR = maxR;
     c->cell = (2.0 * maxR * (1.0 + 1e-6) + 1e-9) * c->cell_scale;
+    c->h_R = R;
+    c->h_excl = excl;
+    c->h_isexcl.assign(n, 0);
+    for (uint32_t e : excl) c->h_isexcl[e] = 1;
+    c->nworlds = 0;   // a new body set starts as one world
+    c->grid_dirty = true;
+    drop_graphs(c);
     c->has_forces = forces;
     c->spheres_only = true;
     for (uint32_t i = 0; i < n; ++i) c->spheres_only = c->spheres_only && b->shape[i] == NE_SHAPE_SPHERE;
@@ -2580,11 +3133,8 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     }
     HIP_TRY(c, hipMemset(c->slot_of, 0xFF, sizeof(uint32_t) * ncap));
     HIP_TRY(c, dalloc(c->excl, std::max<size_t>(excl.size(), 1)));
-    if (!excl.empty()) {
+    if (!excl.empty())
         HIP_TRY(c, hipMemcpy(c->excl, excl.data(), sizeof(uint32_t) * excl.size(), hipMemcpyHostToDevice));
-        HIP_TRY(c, dalloc(c->ex_flags, (size_t)excl.size() * n));
-        HIP_TRY(c, dalloc(c->ex_off, (size_t)excl.size() * n + 1));
-    }
     // device view
     BodiesDev& B = c->B;
     double* d = c->dfield;
@@ -2604,6 +3154,8 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     B.flags = c->ufield + (size_t)U_FLAGS * ncap;
     B.hull_off = c->hull_off; B.hull = c->hull; B.nc_off = c->nc_off; B.nc_idx = c->nc_idx;
     B.n = n;
+    B.wid = nullptr;
+    B.W = nullptr;
     return NE_OK;
 }
 

@@ -9,9 +9,12 @@
 // Conventions: normal points from body b toward body a; the point is on a's
 // side; depth (overlap) is positive when penetrating.
 //
-// GJK/EPA keep their polytope in a per-thread workspace whose bounds follow
-// from the iteration caps (<= 4 + 64 vertices); a polytope that would outgrow
-// the face/horizon arrays raises NE_FAULT_EPA instead of writing past them.
+// GJK/EPA keep their polytope in a caller-supplied workspace.  Vertices are
+// bounded by the iteration caps (<= 4 + 64); faces of a well-conditioned
+// polytope by 2V-4, but near-degenerate overlaps (deeply interpenetrating
+// boxes) can leave many more, as the reference's growable vectors allow.  A
+// polytope that would outgrow the workspace stops with `overflow` set, never
+// writing past it, and the caller reruns the pair with a larger workspace.
 #pragma once
 
 #include "ne_dmath.hpp"
@@ -28,6 +31,8 @@ enum : uint32_t {
     NE_FAULT_CONTACTS = 2u,   // constraint buffer overflow
     NE_FAULT_EPA = 4u,        // EPA polytope exceeded its workspace
     NE_FAULT_CELLS = 8u,      // grid cell capacity exceeded
+    NE_FAULT_EPA_QUEUE = 16u, // more large-workspace EPA reruns than queue slots
+    NE_FAULT_EPA_BIG = 32u,   // an EPA rerun outgrew the large workspace
 };
 
 // Geometry of one body as the narrow phase sees it.
@@ -153,6 +158,76 @@ struct Face {
     int a, b, c;
 };
 
+// Horizon of one EPA iteration: the edges of the faces that see the new
+// point, where an edge whose reverse is already listed removes the first such
+// entry instead of being added (the reference's find + erase on a vector).
+// The surviving edges, in insertion order, become the new faces.
+//
+// HorizonList keeps the list compact and searches it linearly (small
+// polytopes, per-thread arrays).
+struct HorizonList {
+    int (*hor)[2];
+    int cap;
+    int n;
+    __device__ void begin() { n = 0; }
+    __device__ bool add(int u, int w) {
+        for (int e = 0; e < n; ++e) {
+            if (hor[e][0] == w && hor[e][1] == u) {
+                for (int m = e; m + 1 < n; ++m) { hor[m][0] = hor[m + 1][0]; hor[m][1] = hor[m + 1][1]; }
+                --n;
+                return true;
+            }
+        }
+        if (n >= cap) return false;
+        hor[n][0] = u; hor[n][1] = w; ++n;
+        return true;
+    }
+    __device__ int alive() const { return n; }
+    __device__ int entries() const { return n; }
+    __device__ bool get(int e, int& u, int& w) const { u = hor[e][0]; w = hor[e][1]; return true; }
+};
+
+// HorizonTable gives the same list for very large polytopes in time linear
+// in the edges: entries are never moved, a removed one is marked dead, and
+// the live entries of each directed edge (u, w) form a queue in insertion
+// order whose head is the first match.  Vertex ids are < EPA_MAXV, so the
+// queues are indexed directly by u * EPA_MAXV + w; `stamp` marks the queues
+// written in this iteration, so nothing is cleared between iterations.
+struct HorizonTable {
+    int (*hor)[2];
+    int* next;
+    uint8_t* live;
+    int* head;    // EPA_MAXV * EPA_MAXV
+    int* tail;
+    int* stamp;
+    int cap;
+    int n, nlive, it;
+    __device__ void begin() { n = 0; nlive = 0; ++it; }
+    __device__ bool add(int u, int w) {
+        const int rk = w * EPA_MAXV + u;
+        if (stamp[rk] == it && head[rk] >= 0) {
+            const int e = head[rk];
+            live[e] = 0;
+            head[rk] = next[e];
+            if (head[rk] < 0) tail[rk] = -1;
+            --nlive;
+            return true;
+        }
+        if (n >= cap) return false;
+        const int k = u * EPA_MAXV + w;
+        if (stamp[k] != it) { stamp[k] = it; head[k] = -1; tail[k] = -1; }
+        const int e = n++;
+        hor[e][0] = u; hor[e][1] = w; live[e] = 1; next[e] = -1;
+        if (tail[k] < 0) head[k] = e; else next[tail[k]] = e;
+        tail[k] = e;
+        ++nlive;
+        return true;
+    }
+    __device__ int alive() const { return nlive; }
+    __device__ int entries() const { return n; }
+    __device__ bool get(int e, int& u, int& w) const { u = hor[e][0]; w = hor[e][1]; return live[e] != 0; }
+};
+
 __device__ __forceinline__ Face make_face(const SupV* verts, int i, int j, int k) {
     Face f;
     V3 n = cross(verts[j].v - verts[i].v, verts[k].v - verts[i].v);
@@ -166,12 +241,12 @@ __device__ __forceinline__ Face make_face(const SupV* verts, int i, int j, int k
 
 // src/epa.hpp:71-150.  Faces are compacted in place (kept faces in their
 // original order, then the new horizon faces), exactly the order the
-// reference's `keep` + push_back produces; horizon edges use erase semantics.
-__device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, const SupV tet[4],
-                                      uint32_t* fault) {
-    SupV verts[EPA_MAXV];
-    Face faces[EPA_MAXF];
-    int hor[EPA_MAXH][2];
+// reference's `keep` + push_back produces.  `verts` holds EPA_MAXV entries,
+// `faces` maxf; exceeding maxf or the horizon's capacity sets `overflow` and
+// stops (the result is then not used).
+template <class Hz>
+__device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, const SupV tet[4], SupV* verts,
+                                      Face* faces, int maxf, Hz& hz, bool& overflow) {
     int nv = 4;
     for (int k = 0; k < 4; ++k) verts[k] = tet[k];
     int nf = 4;
@@ -190,39 +265,27 @@ __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, co
         if (d - best.dist < 1e-7) break;
         int pidx = nv;
         verts[nv++] = p;   // nv <= 4 + 64 == EPA_MAXV
-        int nh = 0, nk = 0;
+        hz.begin();
+        int nk = 0;
         bool over = false;
-        for (int fi = 0; fi < nf; ++fi) {
+        for (int fi = 0; fi < nf && !over; ++fi) {
             Face f = faces[fi];
             if (dot(f.n, p.v - verts[f.a].v) > 1e-12) {
-                int eu[3] = {f.a, f.b, f.c}, ew[3] = {f.b, f.c, f.a};
-                for (int q = 0; q < 3; ++q) {
-                    int u = eu[q], w = ew[q];
-                    bool found = false;
-                    for (int e = 0; e < nh; ++e) {
-                        if (hor[e][0] == w && hor[e][1] == u) {
-                            for (int m = e; m + 1 < nh; ++m) { hor[m][0] = hor[m + 1][0]; hor[m][1] = hor[m + 1][1]; }
-                            --nh;
-                            found = true;
-                            break;
-                        }
-                    }
-                    if (!found) {
-                        if (nh >= EPA_MAXH) { over = true; break; }
-                        hor[nh][0] = u; hor[nh][1] = w; ++nh;
-                    }
-                }
+                over = !hz.add(f.a, f.b) || !hz.add(f.b, f.c) || !hz.add(f.c, f.a);
             } else {
                 faces[nk++] = f;   // nk <= fi: in-place compaction keeps order
             }
-            if (over) break;
         }
-        if (over || nk + nh > EPA_MAXF) {
-            atomicOr(fault, NE_FAULT_EPA);
+        if (over || nk + hz.alive() > maxf) {
+            overflow = true;
             break;
         }
         nf = nk;
-        for (int e = 0; e < nh; ++e) faces[nf++] = make_face(verts, hor[e][0], hor[e][1], pidx);
+        const int ne = hz.entries();
+        for (int e = 0; e < ne; ++e) {
+            int u, w;
+            if (hz.get(e, u, w)) faces[nf++] = make_face(verts, u, w, pidx);
+        }
         if (nf == 0) break;
     }
     const SupV A0 = verts[best.a], B0 = verts[best.b], C0 = verts[best.c];
@@ -247,11 +310,25 @@ __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, co
     return c;
 }
 
-__device__ inline Contact convex_contact(const Box3& box, const Geom& A, const Geom& B, uint32_t* fault) {
+// EPA workspace handed down the detect path: per-thread arrays with a linear
+// horizon, or (big) a global-memory workspace with the edge table.
+struct EpaWS {
+    SupV* verts;
+    Face* faces;
+    int maxf;
+    bool big;
+    HorizonList list;
+    HorizonTable table;
+    bool overflow;
+    bool truncate;   // on overflow, keep the closest face found so far (largest workspace only)
+};
+
+__device__ inline Contact convex_contact(const Box3& box, const Geom& A, const Geom& B, EpaWS& ws) {
     V3 shift = min_image_shift(box, A.x, B.x);
     SupV tet[4];
     if (!gjk_tetra(A, B, shift, tet)) return no_contact();
-    return epa_contact(A, B, shift, tet, fault);
+    if (ws.big) return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.table, ws.overflow);
+    return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.list, ws.overflow);
 }
 
 // src/narrowphase.hpp:34-81
@@ -396,7 +473,7 @@ __device__ __forceinline__ Contact sphere_sphere(const Box3& box, V3 ax, double 
 }
 
 // src/detect.hpp:33-77
-__device__ inline Contact detect_contact(const Box3& box, const Geom& a, const Geom& b, uint32_t* fault) {
+__device__ inline Contact detect_contact(const Box3& box, const Geom& a, const Geom& b, EpaWS& ws) {
     if (a.shape == SH_PLANE || b.shape == SH_PLANE) {
         if (a.shape == SH_PLANE && b.shape == SH_PLANE) return no_contact();
         const Geom& plane = (a.shape == SH_PLANE) ? a : b;
@@ -421,7 +498,7 @@ __device__ inline Contact detect_contact(const Box3& box, const Geom& a, const G
         if (t.hit) t.normal = -t.normal;
         return t;
     }
-    return convex_contact(box, a, b, fault);
+    return convex_contact(box, a, b, ws);
 }
 
 // ---- box-box manifold: src/manifold.hpp:16-126 ----------------------------

@@ -25,7 +25,7 @@ EXPORTS = [
     "ne_gpu_step", "ne_gpu_get_state", "ne_gpu_get_body_aux", "ne_gpu_reset_warm_cache", "ne_gpu_get_contacts",
     "ne_gpu_get_pairs", "ne_gpu_diagnostics", "ne_gpu_get_counters", "ne_gpu_get_stream", "ne_gpu_last_error",
     "ne_gpu_fp_probe", "ne_gpu_time_kernel", "ne_gpu_kernel_time", "ne_gpu_export_warm_cache",
-    "ne_gpu_import_warm_cache",
+    "ne_gpu_import_warm_cache", "ne_gpu_set_worlds", "ne_gpu_use_graph", "ne_gpu_autotune", "ne_gpu_epa_stats",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
@@ -71,6 +71,10 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_export_warm_cache.argtypes = [vp, C.c_uint32, u32p, u64p, dp]
     lib.ne_gpu_import_warm_cache.argtypes = [vp, C.c_uint32, u64p, dp]
     lib.ne_gpu_kernel_time.argtypes = [vp, C.POINTER(C.c_double), u32p]
+    lib.ne_gpu_set_worlds.argtypes = [vp, C.c_uint32, u32p, dp, dp, dp]
+    lib.ne_gpu_use_graph.argtypes = [vp, C.c_int32]
+    lib.ne_gpu_epa_stats.argtypes = [vp, u32p, u32p]
+    lib.ne_gpu_autotune.argtypes = [vp, C.c_int32, C.POINTER(_abi.TuneResult)]
     for name in EXPORTS:
         if name != "ne_gpu_last_error":
             getattr(lib, name).restype = C.c_int
@@ -122,8 +126,34 @@ class GpuWorld:
         self.n = s.n
         del keep
 
+    def set_worlds(self, offsets, half, restitution=None, friction=None):
+        """Batched worlds: bodies [offsets[w], offsets[w+1]) form periodic world w
+        with box half extents half[w] (3) and default restitution / friction."""
+        off = np.ascontiguousarray(offsets, np.uint32)
+        nw = len(off) - 1
+        hh = np.ascontiguousarray(np.broadcast_to(np.asarray(half, np.float64).reshape(-1, 3) if np.ndim(half) > 1
+                                                  else np.asarray(half, np.float64).reshape(-1, 1), (nw, 3)))
+        e = None if restitution is None else np.ascontiguousarray(restitution, np.float64)
+        m = None if friction is None else np.ascontiguousarray(friction, np.float64)
+        self._check(self.lib.ne_gpu_set_worlds(self.h, nw, _abi.uptr(off), _abi.dptr(hh), _abi.dptr(e), _abi.dptr(m)))
+
+    def epa_stats(self):
+        """(large-workspace EPA reruns, runs stopped at the largest workspace)
+        during the last step call."""
+        a, b = C.c_uint32(), C.c_uint32()
+        self._check(self.lib.ne_gpu_epa_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def use_graph(self, enable: bool = True):
+        self._check(self.lib.ne_gpu_use_graph(self.h, 1 if enable else 0))
+
+    def autotune(self, steps: int = 3):
+        r = _abi.TuneResult()
+        self._check(self.lib.ne_gpu_autotune(self.h, int(steps), C.byref(r)))
+        return dict(cell_scale=r.cell_scale, broadphase=r.broadphase, graph=r.graph, ms_per_step=r.ms_per_step,
+                    trials=r.trials)
+
     def set_state(self, x=None, v=None, q=None, w=None):
-        f = lambda a: None if a is None else _abi.dptr(np.ascontiguousarray(a, np.float64))
         arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (x, v, q, w)]
         self._check(self.lib.ne_gpu_set_state(self.h, *[None if a is None else _abi.dptr(a) for a in arrs]))
 

@@ -403,3 +403,195 @@ def mixed_pile(n: int, half=(8.0, 8.0, 8.0), seed: int = 0, ground: bool = True,
     s.params = Params(half=tuple(half), periodic=periodic, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0),
                       restitution=0.2, friction=0.5, warm_start=True, solver_iterations=iterations)
     return s
+
+
+# ---------------------------------------------------------------------------
+# Batched worlds: many independent periodic worlds in one body array
+# ---------------------------------------------------------------------------
+
+def concat_scenes(scenes) -> Scene:
+    """One body array holding the scenes one after another (hull and
+    no-collide indices shifted); params are the first scene's."""
+    n = sum(s.n for s in scenes)
+    out = Scene(n)
+    per_body = ["x", "v", "q", "w", "force", "torque", "inv_mass", "inv_inertia", "shape", "radius", "half_height",
+                "half_extents", "friction", "restitution", "linear_damping", "angular_damping", "gravity_scale",
+                "layer", "mask", "flags"]
+    for k in per_body:
+        setattr(out, k, np.concatenate([getattr(s, k) for s in scenes], axis=0))
+    hv, ho, nc, no = [], [0], [], [0]
+    base = 0
+    for s in scenes:
+        hv.append(s.hull_verts.reshape(-1, 3))
+        ho.extend((s.hull_offsets[1:] + ho[-1]).tolist())
+        nc.append(s.nocollide.astype(np.uint32) + np.uint32(base))
+        no.extend((s.nocollide_offsets[1:] + no[-1]).tolist())
+        base += s.n
+    out.hull_verts = np.concatenate(hv, axis=0) if hv else np.zeros((0, 3))
+    out.hull_offsets = np.asarray(ho, np.uint32)
+    out.nocollide = np.concatenate(nc) if nc else np.zeros(0, np.uint32)
+    out.nocollide_offsets = np.asarray(no, np.uint32)
+    out.params = Params(**scenes[0].params.__dict__) if scenes else Params()
+    return out
+
+
+def slice_scene(s: Scene, begin: int, end: int) -> Scene:
+    """Bodies [begin, end) of a concatenated scene as a scene of their own."""
+    out = Scene(end - begin)
+    for k in ["x", "v", "q", "w", "force", "torque", "inv_mass", "inv_inertia", "shape", "radius", "half_height",
+              "half_extents", "friction", "restitution", "linear_damping", "angular_damping", "gravity_scale",
+              "layer", "mask", "flags"]:
+        setattr(out, k, getattr(s, k)[begin:end].copy())
+    h0, h1 = int(s.hull_offsets[begin]), int(s.hull_offsets[end])
+    out.hull_verts = s.hull_verts.reshape(-1, 3)[h0:h1].copy()
+    out.hull_offsets = (s.hull_offsets[begin:end + 1] - s.hull_offsets[begin]).astype(np.uint32)
+    c0, c1 = int(s.nocollide_offsets[begin]), int(s.nocollide_offsets[end])
+    out.nocollide = (s.nocollide[c0:c1] - begin).astype(np.uint32)
+    out.nocollide_offsets = (s.nocollide_offsets[begin:end + 1] - s.nocollide_offsets[begin]).astype(np.uint32)
+    out.params = Params(**s.params.__dict__)
+    return out
+
+
+@dataclass
+class Batch:
+    """Concatenated worlds: world w is bodies [offsets[w], offsets[w+1]) with
+    box half extents half[w] and default restitution / friction."""
+    scene: Scene
+    offsets: np.ndarray
+    half: np.ndarray
+    restitution: np.ndarray
+    friction: np.ndarray
+
+    @property
+    def nworlds(self) -> int:
+        return len(self.offsets) - 1
+
+    def world(self, w: int) -> Scene:
+        """World w alone, with its own box and material defaults."""
+        s = slice_scene(self.scene, int(self.offsets[w]), int(self.offsets[w + 1]))
+        s.params.half = tuple(float(h) for h in self.half[w])
+        s.params.restitution = float(self.restitution[w])
+        s.params.friction = float(self.friction[w])
+        return s
+
+
+def _lattice(rng, n, L, jitter=0.1):
+    m = int(math.ceil(n ** (1.0 / 3.0)))
+    sp = L / m
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+    g = g[rng.permutation(len(g))[:n]]
+    return (-L / 2 + (g + 0.5) * sp + rng.uniform(-jitter, jitter, size=(n, 3)) * sp).astype(np.float64)
+
+
+def _place_periodic(rng, rho, L, max_rounds=500):
+    """Uniform positions in a periodic cube of side L, resampled until no two
+    bodies are closer than rho_i + rho_j (minimum image)."""
+    from scipy.spatial import cKDTree
+    n = len(rho)
+    pos = rng.uniform(-L / 2, L / 2, size=(n, 3))
+    reach = 2.0 * float(rho.max())
+    for _ in range(max_rounds):
+        tree = cKDTree(np.mod(pos + L / 2, L), boxsize=L)
+        pairs = tree.query_pairs(reach, output_type="ndarray")
+        if len(pairs):
+            d = pos[pairs[:, 0]] - pos[pairs[:, 1]]
+            d -= L * np.floor(d / L + 0.5)
+            close = np.einsum("ij,ij->i", d, d) < (rho[pairs[:, 0]] + rho[pairs[:, 1]]) ** 2
+            pairs = pairs[close]
+        if len(pairs) == 0:
+            return pos
+        bad = np.unique(pairs.max(axis=1))
+        pos[bad] = rng.uniform(-L / 2, L / 2, size=(len(bad), 3))
+    raise RuntimeError("could not place the bodies")
+
+
+def batched_worlds(nworlds: int = 1024, seed: int = 1, nmin: int = 64, nmax: int = 4096, phi: float = 0.2,
+                   iterations: int = 8) -> Batch:
+    """Independent periodic worlds of uneven size: N_w log-uniform in
+    [nmin, nmax]; half spheres (r U[0.2,0.6]) and half boxes (half extents
+    U[0.2,0.6]^3), density 1, random orientations, in a cube of side
+    (sum of volumes / phi)^(1/3); placed uniformly with no sphere overlap and
+    no two box centres closer than half their bounding radii (box corners
+    may touch at start); velocities N(0,1); per-world friction U[0,0.8] and
+    restitution U[0,1]; no gravity, dt 1/60."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(math.log(nmin), math.log(nmax + 1), size=nworlds))).astype(int)
+    sizes = np.clip(sizes, nmin, nmax)
+    scenes, halves = [], []
+    mu = rng.uniform(0.0, 0.8, size=nworlds)
+    e = rng.uniform(0.0, 1.0, size=nworlds)
+    for w in range(nworlds):
+        n = int(sizes[w])
+        ns = n // 2
+        r = rng.uniform(0.2, 0.6, size=ns)
+        he = rng.uniform(0.2, 0.6, size=(n - ns, 3))
+        vol = float(np.sum(4.0 / 3.0 * PI * r ** 3) + np.sum(8.0 * np.prod(he, axis=1)))
+        L = (vol / phi) ** (1.0 / 3.0)
+        kinds = rng.permutation(n) < ns          # True: sphere
+        rho = np.empty(n)
+        rho[kinds] = r
+        rho[~kinds] = 0.5 * np.linalg.norm(he, axis=1)
+        pos = _place_periodic(rng, rho, L)
+        qs = random_unit_quaternions(rng, n)
+        vel = rng.normal(0.0, 1.0, size=(n, 3))
+        s = Scene(n)
+        si = bi = 0
+        for i in range(n):
+            if kinds[i]:
+                b = make_sphere(i, tuple(pos[i]), float(r[si]), 1.0)
+                si += 1
+            else:
+                b = make_box(i, tuple(pos[i]), tuple(qs[i]), tuple(he[bi]), 1.0)
+                bi += 1
+            s.x[i], s.q[i] = b.x, b.q
+            s.inv_mass[i], s.inv_inertia[i] = b.inv_mass, b.inv_inertia
+            s.shape[i], s.radius[i], s.half_extents[i] = b.shape, b.radius, b.half_extents
+        s.v[:] = vel
+        s.params = Params(half=(L / 2, L / 2, L / 2), periodic=True, dt=1.0 / 60.0, gravity=(0.0, 0.0, 0.0),
+                          restitution=float(e[w]), friction=float(mu[w]), solver_iterations=iterations)
+        scenes.append(s)
+        halves.append((L / 2, L / 2, L / 2))
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+    return Batch(concat_scenes(scenes), off, np.asarray(halves, np.float64), e, mu)
+
+
+def adjoint_worlds(nworlds: int = 256, n_spheres: int = 256, n_boxes: int = 256, side: float = 16.0, seed: int = 2,
+                   friction: float = 0.4, restitution: float = 0.3, ground: bool = True) -> Batch:
+    """Differentiable-rollout worlds: each a periodic cube of `side` holding a
+    static ground box (first body, spanning the cube in x and z) and
+    `n_spheres` spheres r=0.4 plus `n_boxes` boxes of half extent 0.3 on a
+    jittered lattice above it; velocities N(0,1); gravity -9.81, dt 1/60,
+    8 iterations."""
+    rng = np.random.default_rng(seed)
+    h = side / 2.0
+    scenes = []
+    for w in range(nworlds):
+        bodies = []
+        if ground:
+            bodies.append(make_box(0, (0.0, -h + 0.5, 0.0), (1.0, 0.0, 0.0, 0.0), (h, 0.5, h), 1.0).make_static())
+        n = n_spheres + n_boxes
+        m = int(math.ceil((n / 2.0) ** (1.0 / 3.0)))
+        sp = side / max(m, 1)
+        cells = np.stack(np.meshgrid(np.arange(m), np.arange(2 * m), np.arange(m), indexing="ij"), -1).reshape(-1, 3)
+        cells = cells[rng.permutation(len(cells))[:n]]
+        qs = random_unit_quaternions(rng, n)
+        kinds = rng.permutation(n) < n_spheres
+        for t in range(n):
+            c = cells[t]
+            x = -h + (c[0] + 0.5) * sp + rng.uniform(-0.1, 0.1)
+            y = -h + 1.5 + (c[1] + 0.5) * (side - 2.0) / (2 * m) + rng.uniform(-0.05, 0.05)
+            z = -h + (c[2] + 0.5) * sp + rng.uniform(-0.1, 0.1)
+            k = len(bodies)
+            if kinds[t]:
+                b = make_sphere(k, (x, y, z), 0.4, 1.0)
+            else:
+                b = make_box(k, (x, y, z), tuple(qs[t]), (0.3, 0.3, 0.3), 1.0)
+            b.v = tuple(rng.normal(0.0, 1.0, size=3))
+            bodies.append(b)
+        s = Scene.from_bodies(bodies)
+        s.params = Params(half=(h, h, h), periodic=True, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0),
+                          restitution=restitution, friction=friction, solver_iterations=8)
+        scenes.append(s)
+    off = np.concatenate([[0], np.cumsum([s.n for s in scenes])]).astype(np.uint32)
+    return Batch(concat_scenes(scenes), off, np.full((nworlds, 3), h), np.full(nworlds, restitution),
+                 np.full(nworlds, friction))

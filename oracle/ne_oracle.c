@@ -281,6 +281,30 @@ static oface make_face(const supv* verts, int i, int j, int k) {                
     return f;
 }
 
+/* Largest EPA working set seen by this process (faces after an iteration,
+ * horizon edges): sizes the fixed workspace of a device implementation. */
+static int g_epa_max_faces = 0, g_epa_max_horizon = 0;
+static long g_epa_calls = 0, g_epa_big = 0, g_epa_big_faces = 160, g_epa_big_horizon = 96;
+
+void ne_oracle_epa_stats(int32_t* max_faces, int32_t* max_horizon) {
+    if (max_faces) *max_faces = g_epa_max_faces;
+    if (max_horizon) *max_horizon = g_epa_max_horizon;
+}
+
+/* Resets the statistics; later EPA calls whose face list exceeds `faces` or
+ * whose horizon exceeds `horizon` are counted as big. */
+void ne_oracle_epa_stats_reset(int32_t faces, int32_t horizon) {
+    g_epa_max_faces = g_epa_max_horizon = 0;
+    g_epa_calls = g_epa_big = 0;
+    g_epa_big_faces = faces;
+    g_epa_big_horizon = horizon;
+}
+
+void ne_oracle_epa_counts(int64_t* calls, int64_t* big) {
+    if (calls) *calls = g_epa_calls;
+    if (big) *big = g_epa_big;
+}
+
 static ocontact epa_contact(const ne_oracle* o, const obody* A, const obody* B, v3 shift, const supv tet[4]) {
     int nv = 4, capv = 80;
     supv* verts = (supv*)malloc(sizeof(supv) * capv);
@@ -296,6 +320,8 @@ static ocontact epa_contact(const ne_oracle* o, const obody* A, const obody* B, 
     faces[3] = make_face(verts, 1, 3, 2);
     oface best;
     memset(&best, 0, sizeof(best));
+    int was_big = 0;
+    ++g_epa_calls;
     for (int iter = 0; iter < 64; ++iter) {
         int ci = 0;
         for (int i = 1; i < nf; ++i)
@@ -330,6 +356,9 @@ static ocontact epa_contact(const ne_oracle* o, const obody* A, const obody* B, 
             }
         }
         int need = nk + nh;
+        if (need > g_epa_max_faces) g_epa_max_faces = need;
+        if (nh > g_epa_max_horizon) g_epa_max_horizon = nh;
+        if (!was_big && (need > g_epa_big_faces || nh > g_epa_big_horizon)) { was_big = 1; ++g_epa_big; }
         if (need > capf) {
             while (capf < need) capf *= 2;
             faces = (oface*)realloc(faces, sizeof(oface) * capf);

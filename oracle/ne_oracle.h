@@ -44,6 +44,10 @@ int ne_oracle_candidate_count(ne_oracle *o, uint32_t *count);
 int ne_oracle_diagnostics(ne_oracle *o, double *p, double *L, double *ke);
 int ne_oracle_min_image(ne_oracle *o, const double *d, double *out);
 int ne_oracle_wrap(ne_oracle *o, const double *p, double *out);
+/* Largest EPA face list / horizon seen in this process (workspace sizing). */
+void ne_oracle_epa_stats(int32_t *max_faces, int32_t *max_horizon);
+void ne_oracle_epa_stats_reset(int32_t faces, int32_t horizon);
+void ne_oracle_epa_counts(int64_t *calls, int64_t *big);
 
 #ifdef __cplusplus
 }

@@ -52,12 +52,14 @@ int main(void) {
   printf("ne_world_params %zu\n", sizeof(ne_world_params));
   printf("ne_bodies %zu\n", sizeof(ne_bodies));
   printf("ne_gpu_config %zu\n", sizeof(ne_gpu_config));
+  printf("ne_tune_result %zu\n", sizeof(ne_tune_result));
   O(ne_world_params, half); O(ne_world_params, periodic); O(ne_world_params, dt); O(ne_world_params, gravity);
   O(ne_world_params, contact_beta); O(ne_world_params, warm_start); O(ne_world_params, solver_iterations);
   O(ne_world_params, sleep_lin_vel); O(ne_world_params, wake_vel);
   O(ne_bodies, n); O(ne_bodies, x); O(ne_bodies, inv_mass); O(ne_bodies, shape); O(ne_bodies, flags);
   O(ne_bodies, hull_offsets); O(ne_bodies, nocollide);
   O(ne_gpu_config, cell_scale);
+  O(ne_tune_result, broadphase); O(ne_tune_result, ms_per_step); O(ne_tune_result, trials);
   return 0;
 }
 '''
@@ -75,11 +77,13 @@ def test_ctypes_mirror_matches_c_layout():
     assert lay["ne_world_params"] == C.sizeof(_abi.WorldParams)
     assert lay["ne_bodies"] == C.sizeof(_abi.Bodies)
     assert lay["ne_gpu_config"] == C.sizeof(_abi.GpuConfig)
+    assert lay["ne_tune_result"] == C.sizeof(_abi.TuneResult)
     for key, off in lay.items():
         if "." not in key:
             continue
         t, f = key.split(".")
-        cls = {"ne_world_params": _abi.WorldParams, "ne_bodies": _abi.Bodies, "ne_gpu_config": _abi.GpuConfig}[t]
+        cls = {"ne_world_params": _abi.WorldParams, "ne_bodies": _abi.Bodies, "ne_gpu_config": _abi.GpuConfig,
+               "ne_tune_result": _abi.TuneResult}[t]
         assert getattr(cls, f).offset == off, key
 
 
