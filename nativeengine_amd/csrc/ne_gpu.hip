@@ -112,6 +112,7 @@ enum CounterSlot : int {
     CNT_EPABIG = 11,  // pairs queued for the large-workspace EPA rerun (this step)
     CNT_EPATRUNC = 12,   // EPA runs stopped at the largest workspace (this call)
     CNT_EPARERUN = 13,   // large-workspace EPA reruns (this call)
+    CNT_EPAQ = 14,       // convex pairs queued for EPA (this step)
     CNT_N = 16
 };
 
@@ -913,7 +914,9 @@ __global__ void k_clamp_count(const uint32_t* __restrict__ raw, uint32_t cap, ui
 // ---------------------------------------------------------------------------
 
 // Contacts of one candidate pair into out[0..27] (<= 4 x (n3, pt3, depth));
-// returns how many.  ws.overflow is set when EPA outgrew its workspace.
+// returns how many.  ws.overflow is set when EPA outgrew its workspace; with
+// ws.defer, a convex pair that GJK finds overlapping returns NARROW_DEFERRED.
+constexpr uint32_t NARROW_DEFERRED = 0xFFFFFFFFu;
 template <bool SPHERES_ONLY>
 __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimParams& P, uint32_t i, uint32_t j,
                                                 double* out, EpaWS& ws, uint32_t* fault) {
@@ -931,6 +934,7 @@ __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimPar
     } else if (!SPHERES_ONLY) {
         const Geom a = load_geom(B, i), b = load_geom(B, j);
         Contact c = detect_contact(box, a, b, ws);
+        if (c.hit == 2) return NARROW_DEFERRED;   // GJK overlap: EPA runs in k_epa
         if (ws.overflow && !ws.truncate) return 0;
         if (c.hit) {
             if (si == SH_BOX && sj == SH_BOX) {
@@ -975,22 +979,49 @@ struct EpaBig {
     int* tab;         // 3 * EPA_TAB per slot: head, tail, stamp
 };
 
+// Narrow phase, sorted by pair type: k_narrow computes the analytic pairs
+// and runs GJK on the convex ones; convex pairs that overlap are queued
+// (their EPA keeps a large per-thread workspace and runs for tens of
+// iterations) and k_epa runs them together, so waves doing EPA hold only EPA
+// pairs.  Their contacts are the same as computed inline.
 template <bool SPHERES_ONLY>
 __global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
                                                 const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
-                                                uint32_t* __restrict__ ncont, double* __restrict__ ct, EpaBig big) {
+                                                uint32_t* __restrict__ ncont, double* __restrict__ ct,
+                                                uint32_t* __restrict__ epa_q, uint32_t epa_cap) {
     const uint32_t np = counters[CNT_P];
     uint32_t* fault = counters + CNT_FAULT;
-    SupV verts[SPHERES_ONLY ? 1 : EPA_MAXV];
-    Face faces[SPHERES_ONLY ? 1 : EPA_MAXF];
-    int hor[SPHERES_ONLY ? 1 : EPA_MAXH][2];
+    EpaWS ws{};
+    ws.defer = true;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        uint32_t nc = narrow_pair<SPHERES_ONLY>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
+        if (!SPHERES_ONLY && nc == NARROW_DEFERRED) {
+            const uint32_t slot = atomicAdd(&counters[CNT_EPAQ], 1u);
+            if (slot < epa_cap) epa_q[slot] = p;
+            else atomicOr(fault, NE_FAULT_PAIRS);
+            nc = 0;
+        }
+        ncont[p] = nc;
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_epa(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
+                                             const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                             uint32_t* __restrict__ ncont, double* __restrict__ ct,
+                                             const uint32_t* __restrict__ epa_q, uint32_t epa_cap, EpaBig big) {
+    const uint32_t nq = min(counters[CNT_EPAQ], epa_cap);
+    uint32_t* fault = counters + CNT_FAULT;
+    SupV verts[EPA_MAXV];
+    Face faces[EPA_MAXF];
+    int hor[EPA_MAXH][2];
     EpaWS ws{};
     ws.verts = verts; ws.faces = faces; ws.maxf = EPA_MAXF; ws.big = false;
     ws.list.hor = hor; ws.list.cap = EPA_MAXH;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += gridDim.x * blockDim.x) {
+        const uint32_t p = epa_q[t];
         ws.overflow = false;
-        uint32_t nc = narrow_pair<SPHERES_ONLY>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
-        if (!SPHERES_ONLY && ws.overflow) {
+        uint32_t nc = narrow_pair<false>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
+        if (ws.overflow) {
             const uint32_t slot = atomicAdd(&counters[CNT_EPABIG], 1u);
             if (slot < big.cap) big.list[slot] = p;
             else atomicOr(fault, NE_FAULT_EPA_QUEUE);
@@ -1000,35 +1031,165 @@ __global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, uint32
     }
 }
 
+// EPA of one queued pair by one wave (src/epa.hpp:71-150, the same steps as
+// epa_contact).  The face list lives in global memory; each iteration the 64
+// lanes find the first closest face (a min/first-index reduction), test the
+// faces for visibility and compact the kept ones in rounds of 64 (order is
+// kept by ballot ranks), stage each round's visible faces in LDS for lane 0,
+// which runs the horizon table (LDS queues, global entries), and finally
+// build the new faces from the live horizon entries in parallel.  All lanes
+// hold identical copies of the uniform values (best face, support point).
+__device__ Contact epa_wave(const Geom& A, const Geom& B, V3 shift, const SupV tet[4], SupV* verts, Face* faces,
+                            int maxf, HorizonTable& hz, int* stage, int* shared, bool truncate, bool& overflow) {
+    const int lane = threadIdx.x;
+    if (lane == 0)
+        for (int k = 0; k < 4; ++k) verts[k] = tet[k];
+    __syncthreads();
+    int nv = 4, nf = 4;
+    if (lane < 4) {
+        const int fi[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 3, 2}};
+        faces[lane] = make_face(verts, fi[lane][0], fi[lane][1], fi[lane][2]);
+    }
+    __syncthreads();
+    Face best = faces[0];
+    for (int iter = 0; iter < 64; ++iter) {
+        // first face of minimal distance
+        double bd = 0.0;
+        int bi = -1;
+        for (int i = lane; i < nf; i += 64) {
+            const double d = faces[i].dist;
+            if (bi < 0 || d < bd) { bd = d; bi = i; }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double od = __shfl_xor(bd, off);
+            const int oi = __shfl_xor(bi, off);
+            const bool take = oi >= 0 && (bi < 0 || od < bd || (od == bd && oi < bi));
+            bd = take ? od : bd;
+            bi = take ? oi : bi;
+        }
+        best = faces[bi];
+        const SupV p = cso_support(A, B, shift, best.n);
+        const double d = dot(p.v, best.n);
+        if (d - best.dist < 1e-7) break;
+        const int pidx = nv;
+        if (lane == 0) verts[nv] = p;
+        ++nv;
+        if (lane == 0) { hz.begin(); shared[0] = 0; }
+        __syncthreads();
+        // visibility rounds: compact kept faces, hand visible ones to lane 0
+        int nk = 0;
+        for (int base = 0; base < nf; base += 64) {
+            const int i = base + lane;
+            const bool in = i < nf;
+            Face f{};
+            bool vis = false;
+            if (in) {
+                f = faces[i];
+                vis = dot(f.n, p.v - verts[f.a].v) > 1e-12;
+            }
+            const uint64_t km = __ballot(in && !vis), vm = __ballot(vis);
+            const uint64_t below = lanemask_lt();
+            if (in && !vis) faces[nk + (int)__popcll(km & below)] = f;
+            if (vis) {
+                const int r = (int)__popcll(vm & below);
+                stage[3 * r] = f.a; stage[3 * r + 1] = f.b; stage[3 * r + 2] = f.c;
+            }
+            nk += (int)__popcll(km);
+            __syncthreads();
+            if (lane == 0 && !shared[0]) {
+                const int nvis = (int)__popcll(vm);
+                for (int k = 0; k < nvis; ++k) {
+                    const int a = stage[3 * k], b = stage[3 * k + 1], c = stage[3 * k + 2];
+                    if (!hz.add(a, b) || !hz.add(b, c) || !hz.add(c, a)) { shared[0] = 1; break; }
+                }
+            }
+            __syncthreads();
+            if (shared[0]) break;
+        }
+        if (lane == 0) { shared[1] = hz.entries(); shared[2] = hz.alive(); }
+        __syncthreads();
+        const int ne = shared[1], nalive = shared[2];
+        if (shared[0] || nk + nalive > maxf) {
+            overflow = true;
+            break;
+        }
+        __threadfence_block();
+        // new faces from the live horizon entries, in insertion order
+        int w = nk;
+        for (int base = 0; base < ne; base += 64) {
+            const int e = base + lane;
+            int u = 0, v = 0;
+            const bool live = e < ne && hz.get(e, u, v);
+            const uint64_t lm = __ballot(live);
+            if (live) faces[w + (int)__popcll(lm & lanemask_lt())] = make_face(verts, u, v, pidx);
+            w += (int)__popcll(lm);
+        }
+        nf = w;
+        __syncthreads();
+        if (nf == 0) break;
+    }
+    (void)truncate;
+    return epa_result(verts, best);
+}
+
 __global__ void __launch_bounds__(64) k_narrow_big(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
                                                    const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
                                                    uint32_t* __restrict__ ncont, double* __restrict__ ct, EpaBig big) {
+    __shared__ SupV verts[EPA_MAXV];
+    __shared__ int head[EPA_TAB], tail[EPA_TAB], stamp[EPA_TAB];
+    __shared__ int stage[3 * 64];
+    __shared__ int shared[4];
     const uint32_t nbig = min(counters[CNT_EPABIG], big.cap);
     uint32_t* fault = counters + CNT_FAULT;
     if (blockIdx.x == 0 && threadIdx.x == 0 && nbig) atomicAdd(&counters[CNT_EPARERUN], nbig);
-    SupV verts[EPA_MAXV];
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nbig; t += gridDim.x * blockDim.x) {
+    for (uint32_t t = blockIdx.x; t < nbig; t += gridDim.x) {
         const uint32_t p = big.list[t];
         const size_t fo = (size_t)t * (size_t)big.faces_per;
-        int* tab = big.tab + (size_t)t * 3 * EPA_TAB;
-        for (int k = 0; k < EPA_TAB; ++k) tab[2 * EPA_TAB + k] = 0;   // stamps: no queue written yet
-        EpaWS ws{};
-        ws.verts = verts; ws.faces = big.faces + fo; ws.maxf = big.faces_per; ws.big = true;
-        ws.table.hor = big.hor + fo; ws.table.next = big.next + fo; ws.table.live = big.live + fo;
-        ws.table.head = tab; ws.table.tail = tab + EPA_TAB; ws.table.stamp = tab + 2 * EPA_TAB;
-        ws.table.cap = big.faces_per; ws.table.it = 0;
-        ws.overflow = false;
-        ws.truncate = big.faces_per >= EPA_BIG_FACES_MAX;
-        uint32_t nc = narrow_pair<false>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
-        if (ws.overflow) {
-            if (ws.truncate) {
-                atomicAdd(&counters[CNT_EPATRUNC], 1u);
-            } else {
-                atomicOr(fault, NE_FAULT_EPA_BIG);
-                nc = 0;
+        for (int k = threadIdx.x; k < EPA_TAB; k += blockDim.x) stamp[k] = 0;
+        __syncthreads();
+        HorizonTable hz;
+        hz.hor = big.hor + fo; hz.next = big.next + fo; hz.live = big.live + fo;
+        hz.head = head; hz.tail = tail; hz.stamp = stamp;
+        hz.cap = big.faces_per; hz.n = 0; hz.nlive = 0; hz.it = 0;
+        const bool truncate = big.faces_per >= EPA_BIG_FACES_MAX;
+        const uint32_t i = cand_i[p], j = cand_j[p];
+        const Box3& box = box_of(B, P, i);
+        const Geom a = load_geom(B, i), b = load_geom(B, j);
+        // only pairs that reached EPA are queued: convexContact (src/epa.hpp:153-158)
+        const V3 shift = min_image_shift(box, a.x, b.x);
+        SupV tet[4];
+        bool overflow = false;
+        Contact c = no_contact();
+        if (gjk_tetra(a, b, shift, tet))
+            c = epa_wave(a, b, shift, tet, verts, big.faces + fo, big.faces_per, hz, stage, shared, truncate, overflow);
+        uint32_t nc = 0;
+        double* out = ct + (size_t)p * 28;
+        if (overflow && !truncate) {
+            if (threadIdx.x == 0) atomicOr(fault, NE_FAULT_EPA_BIG);
+        } else if (c.hit) {
+            if (overflow && threadIdx.x == 0) atomicAdd(&counters[CNT_EPATRUNC], 1u);
+            if (a.shape == SH_BOX && b.shape == SH_BOX) {
+                V3 pts[4];
+                double dep[4];
+                const int m = box_box_manifold(box, a, b, c.normal, pts, dep, fault);
+                if (threadIdx.x == 0)
+                    for (int k = 0; k < m; ++k) {
+                        double* o = out + 7 * k;
+                        o[0] = c.normal.x; o[1] = c.normal.y; o[2] = c.normal.z;
+                        o[3] = pts[k].x; o[4] = pts[k].y; o[5] = pts[k].z; o[6] = dep[k];
+                    }
+                nc = (uint32_t)m;
+            }
+            if (nc == 0) {
+                if (threadIdx.x == 0) {
+                    out[0] = c.normal.x; out[1] = c.normal.y; out[2] = c.normal.z;
+                    out[3] = c.point.x; out[4] = c.point.y; out[5] = c.point.z; out[6] = c.overlap;
+                }
+                nc = 1;
             }
         }
-        ncont[p] = nc;
+        if (threadIdx.x == 0) ncont[p] = nc;
+        __syncthreads();
     }
 }
 
@@ -1749,6 +1910,7 @@ struct ne_gpu_ctx {
     uint32_t *row_count = nullptr, *row_start = nullptr;
     uint32_t *ex_flags = nullptr, *ex_off = nullptr;
     uint32_t *cand_i = nullptr, *cand_j = nullptr;
+    uint32_t* epa_q = nullptr;        // convex pairs queued for EPA (pcap entries)
     uint32_t *ncont = nullptr, *cons_off = nullptr;
     double* ct = nullptr;
     // constraints (current) + warm cache (previous)
@@ -1950,6 +2112,7 @@ int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
     c->pcap = pcap;
     HIP_TRY(c, dalloc(c->cand_i, pcap));
     HIP_TRY(c, dalloc(c->cand_j, pcap));
+    HIP_TRY(c, dalloc(c->epa_q, pcap));
     HIP_TRY(c, dalloc(c->ncont, (size_t)pcap + 1));
     HIP_TRY(c, dalloc(c->cons_off, (size_t)pcap + 1));
     HIP_TRY(c, dalloc(c->ct, (size_t)pcap * 28));
@@ -2021,7 +2184,7 @@ void free_work(ne_gpu_ctx* c) {
     dfree(c->cell_of); dfree(c->cell_count); dfree(c->cell_start); dfree(c->sorted_idx); dfree(c->sorted);
     dfree(c->order_prev); dfree(c->xp);
     dfree(c->row_count); dfree(c->row_start); dfree(c->ex_flags); dfree(c->ex_off);
-    dfree(c->cand_i); dfree(c->cand_j); dfree(c->ncont); dfree(c->cons_off); dfree(c->ct);
+    dfree(c->cand_i); dfree(c->cand_j); dfree(c->epa_q); dfree(c->ncont); dfree(c->cons_off); dfree(c->ct);
     ConsDev& C = c->C;
     dfree(C.ci); dfree(C.cj); dfree(C.slot);
     double** dp[] = {&C.n0, &C.n1, &C.n2, &C.p0, &C.p1, &C.p2, &C.ra0, &C.ra1, &C.ra2, &C.rb0, &C.rb1, &C.rb2,
@@ -2130,7 +2293,7 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
         k_excl_rowcount<<<blocks_for(c->nexcl), TPB, 0, s>>>(B, c->excl, c->nexcl, c->ex_off, c->row_count);
     }
     scan_u32(c->row_count, c->row_start, nullptr, n, c->scan_part, s, false,
-             ScanTotal{c->counters, CNT_PRAW, CNT_P, CNT_FAULT, c->pcap, NE_FAULT_PAIRS, CNT_EPABIG, -1});
+             ScanTotal{c->counters, CNT_PRAW, CNT_P, CNT_FAULT, c->pcap, NE_FAULT_PAIRS, CNT_EPABIG, CNT_EPAQ});
     if (c->ngrid > 0) {
         KTimer kt(c, NE_KERNEL_CAND_WRITE);
         k_cand_emit<<<blocks_for(n), TPB, 0, s>>>(B, P, c->grid, c->slot_of, c->sorted_idx, c->sorted, c->cell_start,
@@ -2146,10 +2309,12 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
         KTimer kt(c, NE_KERNEL_NARROW);
         if (c->spheres_only) {
             k_narrow<true><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                              c->epa_big);
+                                                              c->epa_q, c->pcap);
         } else {
             k_narrow<false><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont,
-                                                               c->ct, c->epa_big);
+                                                               c->ct, c->epa_q, c->pcap);
+            k_epa<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
+                                                         c->epa_q, c->pcap, c->epa_big);
             k_narrow_big<<<16, 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_big);
         }
     }

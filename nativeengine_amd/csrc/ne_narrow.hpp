@@ -239,6 +239,30 @@ __device__ __forceinline__ Face make_face(const SupV* verts, int i, int j, int k
     return f;
 }
 
+// Barycentric witness points of the closest face (src/epa.hpp:123-149).
+__device__ inline Contact epa_result(const SupV* verts, const Face& best) {
+    const SupV A0 = verts[best.a], B0 = verts[best.b], C0 = verts[best.c];
+    V3 proj = best.n * best.dist;
+    V3 v0 = B0.v - A0.v, v1 = C0.v - A0.v, v2 = proj - A0.v;
+    double d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
+    double d20 = dot(v2, v0), d21 = dot(v2, v1);
+    double den = d00 * d11 - d01 * d01;
+    double bv = 0, bw = 0;
+    if (fabs(den) > 1e-18) {
+        bv = (d11 * d20 - d01 * d21) / den;
+        bw = (d00 * d21 - d01 * d20) / den;
+    }
+    double bu = 1.0 - bv - bw;
+    V3 onA = (A0.a * bu + B0.a * bv) + C0.a * bw;
+    V3 onB = (A0.b * bu + B0.b * bv) + C0.b * bw;
+    Contact c;
+    c.hit = 1;
+    c.normal = -best.n;
+    c.overlap = best.dist;
+    c.point = (onA + onB) * 0.5;
+    return c;
+}
+
 // src/epa.hpp:71-150.  Faces are compacted in place (kept faces in their
 // original order, then the new horizon faces), exactly the order the
 // reference's `keep` + push_back produces.  `verts` holds EPA_MAXV entries,
@@ -288,26 +312,7 @@ __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, co
         }
         if (nf == 0) break;
     }
-    const SupV A0 = verts[best.a], B0 = verts[best.b], C0 = verts[best.c];
-    V3 proj = best.n * best.dist;
-    V3 v0 = B0.v - A0.v, v1 = C0.v - A0.v, v2 = proj - A0.v;
-    double d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
-    double d20 = dot(v2, v0), d21 = dot(v2, v1);
-    double den = d00 * d11 - d01 * d01;
-    double bv = 0, bw = 0;
-    if (fabs(den) > 1e-18) {
-        bv = (d11 * d20 - d01 * d21) / den;
-        bw = (d00 * d21 - d01 * d20) / den;
-    }
-    double bu = 1.0 - bv - bw;
-    V3 onA = (A0.a * bu + B0.a * bv) + C0.a * bw;
-    V3 onB = (A0.b * bu + B0.b * bv) + C0.b * bw;
-    Contact c;
-    c.hit = 1;
-    c.normal = -best.n;
-    c.overlap = best.dist;
-    c.point = (onA + onB) * 0.5;
-    return c;
+    return epa_result(verts, best);
 }
 
 // EPA workspace handed down the detect path: per-thread arrays with a linear
@@ -321,12 +326,18 @@ struct EpaWS {
     HorizonTable table;
     bool overflow;
     bool truncate;   // on overflow, keep the closest face found so far (largest workspace only)
+    bool defer;      // stop after GJK: an overlapping convex pair reports hit == 2 (EPA runs later)
 };
 
 __device__ inline Contact convex_contact(const Box3& box, const Geom& A, const Geom& B, EpaWS& ws) {
     V3 shift = min_image_shift(box, A.x, B.x);
     SupV tet[4];
     if (!gjk_tetra(A, B, shift, tet)) return no_contact();
+    if (ws.defer) {
+        Contact c = no_contact();
+        c.hit = 2;
+        return c;
+    }
     if (ws.big) return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.table, ws.overflow);
     return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.list, ws.overflow);
 }

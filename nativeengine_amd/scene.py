@@ -483,26 +483,59 @@ def _lattice(rng, n, L, jitter=0.1):
     return (-L / 2 + (g + 0.5) * sp + rng.uniform(-jitter, jitter, size=(n, 3)) * sp).astype(np.float64)
 
 
-def _place_periodic(rng, rho, L, max_rounds=500):
+def _place_periodic(rng, rho, L, max_rounds=200, tries=8, k=32):
     """Uniform positions in a periodic cube of side L, resampled until no two
-    bodies are closer than rho_i + rho_j (minimum image)."""
+    bodies are closer than rho_i + rho_j (minimum image).  Each round moves one
+    body of every conflicting pair to the first of `tries` random spots that
+    is free of the other bodies (checked against the k nearest; a spot with k
+    bodies in reach counts as taken)."""
     from scipy.spatial import cKDTree
     n = len(rho)
     pos = rng.uniform(-L / 2, L / 2, size=(n, 3))
     reach = 2.0 * float(rho.max())
+    k = min(k, n)
     for _ in range(max_rounds):
         tree = cKDTree(np.mod(pos + L / 2, L), boxsize=L)
         pairs = tree.query_pairs(reach, output_type="ndarray")
         if len(pairs):
             d = pos[pairs[:, 0]] - pos[pairs[:, 1]]
             d -= L * np.floor(d / L + 0.5)
-            close = np.einsum("ij,ij->i", d, d) < (rho[pairs[:, 0]] + rho[pairs[:, 1]]) ** 2
-            pairs = pairs[close]
+            pairs = pairs[np.einsum("ij,ij->i", d, d) < (rho[pairs[:, 0]] + rho[pairs[:, 1]]) ** 2]
         if len(pairs) == 0:
             return pos
         bad = np.unique(pairs.max(axis=1))
-        pos[bad] = rng.uniform(-L / 2, L / 2, size=(len(bad), 3))
+        cand = rng.uniform(-L / 2, L / 2, size=(len(bad), tries, 3))
+        dist, idx = tree.query(np.mod(cand.reshape(-1, 3) + L / 2, L), k=k, distance_upper_bound=reach)
+        dist = dist.reshape(len(bad), tries, k)
+        idx = idx.reshape(len(bad), tries, k)
+        present = idx < n
+        other = np.where(present & (idx != bad[:, None, None]), idx, n)
+        rr = np.concatenate([rho, [0.0]])[other]
+        hit = (other < n) & (dist < rho[bad][:, None, None] + rr)
+        full = present[:, :, -1]          # k bodies in reach: cannot tell, call it taken
+        free = ~(hit.any(axis=2) | full)
+        pick = np.argmax(free, axis=1)
+        pos[bad] = cand[np.arange(len(bad)), pick]
     raise RuntimeError("could not place the bodies")
+
+
+def _sphere_mass(radius, density):
+    """make_sphere's inv_mass and inverse inertia (body.hpp:206-219), vectorised
+    in the same operation order."""
+    m = density * (4.0 / 3.0) * PI * radius * radius * radius
+    I = (2.0 / 5.0) * m * radius * radius
+    return np.where(m > 0, 1.0 / m, 0.0), np.where(I > 0, 1.0 / I, 0.0)
+
+
+def _box_mass(he, density):
+    """make_box's inv_mass and inverse inertia (body.hpp:243-261), vectorised."""
+    hx, hy, hz = he[:, 0], he[:, 1], he[:, 2]
+    m = density * 8.0 * hx * hy * hz
+    Ix = (1.0 / 3.0) * m * (hy * hy + hz * hz)
+    Iy = (1.0 / 3.0) * m * (hx * hx + hz * hz)
+    Iz = (1.0 / 3.0) * m * (hx * hx + hy * hy)
+    inv = lambda a: np.where(a > 0, 1.0 / a, 0.0)
+    return inv(m), np.stack([inv(Ix), inv(Iy), inv(Iz)], axis=1)
 
 
 def batched_worlds(nworlds: int = 1024, seed: int = 1, nmin: int = 64, nmax: int = 4096, phi: float = 0.2,
@@ -535,18 +568,17 @@ def batched_worlds(nworlds: int = 1024, seed: int = 1, nmin: int = 64, nmax: int
         qs = random_unit_quaternions(rng, n)
         vel = rng.normal(0.0, 1.0, size=(n, 3))
         s = Scene(n)
-        si = bi = 0
-        for i in range(n):
-            if kinds[i]:
-                b = make_sphere(i, tuple(pos[i]), float(r[si]), 1.0)
-                si += 1
-            else:
-                b = make_box(i, tuple(pos[i]), tuple(qs[i]), tuple(he[bi]), 1.0)
-                bi += 1
-            s.x[i], s.q[i] = b.x, b.q
-            s.inv_mass[i], s.inv_inertia[i] = b.inv_mass, b.inv_inertia
-            s.shape[i], s.radius[i], s.half_extents[i] = b.shape, b.radius, b.half_extents
+        s.x[:] = pos
         s.v[:] = vel
+        sph, box = np.flatnonzero(kinds), np.flatnonzero(~kinds)
+        s.shape[sph] = _abi.SHAPE_SPHERE
+        s.radius[sph] = r
+        s.inv_mass[sph], ii = _sphere_mass(r, 1.0)
+        s.inv_inertia[sph] = ii[:, None]
+        s.shape[box] = _abi.SHAPE_BOX
+        s.half_extents[box] = he
+        s.q[box] = qs[box]
+        s.inv_mass[box], s.inv_inertia[box] = _box_mass(he, 1.0)
         s.params = Params(half=(L / 2, L / 2, L / 2), periodic=True, dt=1.0 / 60.0, gravity=(0.0, 0.0, 0.0),
                           restitution=float(e[w]), friction=float(mu[w]), solver_iterations=iterations)
         scenes.append(s)

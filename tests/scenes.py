@@ -93,3 +93,27 @@ def fixture_scenes():
     f2.params.warm_start = False
     out["features_open_cold"] = (f2, 30)
     return out
+
+
+def epa_blowup_pair():
+    """A sphere resting on a box face (depth ~0.015) whose EPA in the
+    reference never converges: repeated support points leave zero-area faces
+    closest to the origin, so all 64 iterations run and the face list grows
+    to 99,198 faces (horizon 30,066).  Poses are exact (hex) and the bodies
+    are at rest, so the first step's integrate leaves the positions as given."""
+    h = float.fromhex
+    s = S.Scene(2)
+    s.shape[:] = [0, 2]
+    s.radius[:] = [h("0x1.ef4dd7f0435fbp-2"), 1.0]
+    s.half_extents[1] = [h("0x1.276c43f7bd600p-1"), h("0x1.e7c9c1ac97badp-2"), h("0x1.992e5138addc7p-2")]
+    s.x[0] = [h("0x1.c394008719fd0p+1"), h("-0x1.1c5034f14a186p+2"), h("0x1.a7c4b1f09b5b7p-1")]
+    s.x[1] = [h("0x1.1e97196826b1bp+2"), h("-0x1.18f2cbc959722p+2"), h("0x1.202bb87e0f37bp+0")]
+    s.q[1] = [h("0x1.2805d03e0f562p-1"), h("-0x1.d44633bcb72e1p-5"), h("0x1.f5c9a1d01dbd3p-2"), h("0x1.4cbb46f1f0bb6p-1")]
+    s.inv_mass[:] = [h("0x1.0e06a5864bc52p+1"), h("0x1.235c03dc48146p+0")]
+    s.inv_inertia[0] = [h("0x1.68abde1878be3p+4")] * 3
+    s.inv_inertia[1] = [h("0x1.1aa058a7dba55p+3"), h("0x1.bb9ae8cb9b100p+2"), h("0x1.8653004bf7bddp+2")]
+    half = h("0x1.2e9a56266c037p+2")
+    s.params = S.Params(half=(half, half, half), periodic=True, dt=1.0 / 60.0, gravity=(0.0, 0.0, 0.0),
+                        restitution=h("0x1.869dcfc14a600p-10"), friction=h("0x1.a750454fa41b2p-2"),
+                        solver_iterations=8)
+    return s

@@ -69,17 +69,34 @@ def test_batched_gas_worlds_match_each_world_alone(gpu_available, oracle_lib, gr
     _say(f"batched gas: {batch.nworlds} worlds, {batch.scene.n} bodies, graph={graph}")
     want = _oracle_worlds(batch, steps)
     g = _gpu_batch(batch, graph)
-    reruns = 0
     for t in range(steps):
         g.step(1)
-        r, trunc = g.epa_stats()
-        reruns += r
-        assert trunc == 0
+        assert g.epa_stats()[1] == 0
     _check(batch, g, want, f"batched gas graph={graph}")
     assert len(g.contacts()["i"]) > 0
-    # this batch holds a sphere-box contact whose EPA face list grows to ~99k
-    # faces in the reference: it went through the large-workspace rerun
-    assert reruns > 0
+    g.close()
+
+
+def test_epa_blowup_pair_matches_through_the_large_workspace(gpu_available, oracle_lib):
+    """A sphere-box contact whose EPA grows to 99,198 faces in the reference
+    (tests/scenes.py): the GPU reruns it in the wave-cooperative large
+    workspace and gives the oracle's contact and state bit for bit."""
+    import scenes
+    from nativeengine_amd.gpu import GpuWorld
+    s = scenes.epa_blowup_pair()
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    o.step(1)
+    g = GpuWorld()
+    g.set_scene(s)
+    g.step(1)
+    reruns, trunc = g.epa_stats()
+    assert reruns >= 1 and trunc == 0
+    cg, co = g.contacts(), o.contacts()
+    for f in ("i", "j", "normal", "point", "depth", "impulse"):
+        assert np.array_equal(cg[f], co[f]), f
+    for u, v in zip(g.state(), o.state()):
+        assert np.array_equal(u, v)
     g.close()
 
 

@@ -291,3 +291,21 @@ def test_grid_equals_brute_force(oracle_lib, seed):
         got_list = [(int(a), int(b), tuple(p)) for a, b, p in zip(got["i"], got["j"], got["point"])]
         assert len(brute) > 0, f"case {k}: scene has no contacts to compare"
         assert got_list == brute, f"case {k} seed {seed}: grid list ({len(got_list)}) != brute force ({len(brute)})"
+
+
+def test_epa_blowup_pair_is_pathological_in_the_oracle(oracle_lib):
+    """Pins the EPA regression case: the restated EPA, like the reference's,
+    grows this sphere-box polytope to ~99k faces (the GPU must reproduce it
+    with its large workspace)."""
+    import ctypes as C
+    import scenes
+    lib = C.CDLL(engines.ORACLE_LIB)
+    lib.ne_oracle_epa_stats_reset(2048, 1024)
+    o = engines.OracleWorld()
+    o.set_scene(scenes.epa_blowup_pair())
+    o.step(1)
+    f, h = C.c_int32(), C.c_int32()
+    lib.ne_oracle_epa_stats(C.byref(f), C.byref(h))
+    assert f.value == 99198 and h.value == 30066
+    c = o.contacts()
+    assert len(c["i"]) == 1 and abs(c["depth"][0] - 0.0148141) < 1e-6
