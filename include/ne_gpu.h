@@ -275,6 +275,26 @@ typedef struct ne_tune_result {
 } ne_tune_result;
 int ne_gpu_autotune(ne_gpu_ctx *ctx, int32_t steps, ne_tune_result *out);
 
+/* Differentiable rollouts.  No counterpart in the reference.  A rollout runs
+ * `nsteps` steps from the current state (after replacing the velocities by
+ * v0 and the worlds' default friction / restitution by mu / e, when given),
+ * starting with an empty warm cache, keeps a checkpoint every
+ * `checkpoint_every` steps and writes the final positions to x_out.
+ * ne_gpu_rollout_backward then takes d loss / d x_final and returns
+ * d loss / d v0 and d loss / d (mu, e) of each world, by a hand-written
+ * adjoint of integrate, the constraint build, the warm start and the
+ * fixed-iteration impulse solve, with each step's contact set (pairs,
+ * normals, points, depths) held fixed; it recomputes each checkpointed
+ * segment forward and leaves the world at the rollout's start.
+ * v0, x_out, grad_x and grad_v0 are n x 3 doubles in host or device memory
+ * (copied with hipMemcpyDefault); mu, e, grad_mu and grad_e are host arrays
+ * with one entry per world (1 without ne_gpu_set_worlds).  Periodic worlds
+ * without applied forces only. */
+int ne_gpu_rollout(ne_gpu_ctx *ctx, const double *v0, const double *mu, const double *e,
+                   int32_t nsteps, int32_t checkpoint_every, double *x_out);
+int ne_gpu_rollout_backward(ne_gpu_ctx *ctx, const double *grad_x, double *grad_v0,
+                            double *grad_mu, double *grad_e);
+
 /* Last error message of this context (static storage, never NULL). */
 const char *ne_gpu_last_error(ne_gpu_ctx *ctx);
 

@@ -32,6 +32,8 @@
 #include "../../include/ne_gpu.h"
 #include "ne_dmath.hpp"
 #include "ne_narrow.hpp"
+#include "ne_solve.hpp"
+#include "ne_adjoint.hpp"
 #include "ne_prims.hpp"
 
 using namespace nea;
@@ -1202,26 +1204,12 @@ struct ConsDev {
     double *n0, *n1, *n2, *p0, *p1, *p2, *ra0, *ra1, *ra2, *rb0, *rb1, *rb2;
     double *ov, *rbias, *mu, *nimp, *timp, *kn;
     uint64_t* key;
+    uint32_t* src;   // index of the previous step's constraint that warm-started this one, or ~0u
 };
 
-__device__ __forceinline__ double combine_mat(int m, double a, double b) {   // world.cpp:42-51
-    switch (m) {
-        case NE_COMBINE_AVERAGE: return 0.5 * (a + b);
-        case NE_COMBINE_MIN: return a < b ? a : b;
-        case NE_COMBINE_MAX: return a > b ? a : b;
-        case NE_COMBINE_MULTIPLY: return a * b;
-        case NE_COMBINE_GEOMETRIC_MEAN: return sqrt((a > 0 ? a : 0) * (b > 0 ? b : 0));
-    }
-    return 0.5 * (a + b);
-}
 
 __device__ __forceinline__ void uf_union(const BodiesDev& B, uint32_t* parent, uint32_t i, uint32_t j);
 
-__device__ __forceinline__ double eff_mass(double ima, Qt qa, V3 iia, double imb, Qt qb, V3 iib, V3 ra, V3 rb, V3 u) {
-    V3 angA = cross(apply_inv_inertia(qa, iia, cross(ra, u)), ra);
-    V3 angB = cross(apply_inv_inertia(qb, iib, cross(rb, u)), rb);
-    return ima + imb + dot(angA + angB, u);
-}
 
 __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters_in,
                                                     const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
@@ -1287,6 +1275,7 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
             C.timp[k] = 0.0;
             C.kn[k] = eff_mass(ima, qa, iia, imb, qb, iib, rA, rB, n);
             C.key[k] = key;
+            C.src[k] = (P.warm && lo + s < pn && prev_key[lo + s] == key) ? lo + s : 0xFFFFFFFFu;
         }
     }
 }
@@ -1376,15 +1365,6 @@ __global__ void __launch_bounds__(TPB) k_isl_fill(const uint32_t* __restrict__ c
 // (world.cpp:572-612), positional correction (615-616, contact.hpp:78-85).
 // ---------------------------------------------------------------------------
 
-// One body as the solver sees it.  Writes of v (w) are skipped for bodies
-// whose inverse mass (inverse inertia) is zero: the reference adds +-0 there.
-struct BodyView {
-    V3 v, w;
-    Qt q;
-    V3 ii;
-    double im;
-    bool mv, mw;
-};
 
 __device__ __forceinline__ BodyView load_view(const BodiesDev& B, uint32_t b) {
     BodyView o;
@@ -1403,58 +1383,13 @@ __device__ __forceinline__ void store_view(BodiesDev& B, uint32_t b, const BodyV
     if (o.mw) st3(B.w0, B.w1, B.w2, b, o.w);
 }
 
-// a.v += J m_a^-1, a.w += I_a^-1 (rA x J); b gets the opposite (world.cpp:565-566)
-__device__ __forceinline__ void apply_impulse(BodyView& a, BodyView& b, V3 rA, V3 rB, V3 J) {
-    if (a.mv) a.v = a.v + J * a.im;
-    if (a.mw) a.w = a.w + apply_inv_inertia(a.q, a.ii, cross(rA, J));
-    if (b.mv) b.v = b.v - J * b.im;
-    if (b.mw) b.w = b.w - apply_inv_inertia(b.q, b.ii, cross(rB, J));
-}
 
-struct ConsGeom {
-    V3 n, rA, rB;
-};
 
 __device__ __forceinline__ ConsGeom load_geom(const ConsDev& C, uint32_t k) {
     return ConsGeom{V3{C.n0[k], C.n1[k], C.n2[k]}, V3{C.ra0[k], C.ra1[k], C.ra2[k]}, V3{C.rb0[k], C.rb1[k], C.rb2[k]}};
 }
 
-__device__ __forceinline__ void warm_apply(BodyView& a, BodyView& b, const ConsGeom& g, double nimp) {
-    if (nimp == 0.0) return;
-    apply_impulse(a, b, g.rA, g.rB, g.n * nimp);
-}
 
-// One Gauss-Seidel visit of constraint k (world.cpp:574-611).
-__device__ __forceinline__ void gs_visit(BodyView& a, BodyView& b, const ConsGeom& g, double rbias, double kn, double mu,
-                                         double& nimp, double& timp) {
-    const V3 vc = (a.v + cross(a.w, g.rA)) - (b.v + cross(b.w, g.rB));
-    const double vn = dot(vc, g.n);
-    if (kn > 1e-18) {
-        double dj = (rbias - vn) / kn;
-        const double old = nimp;
-        nimp = (old + dj > 0.0) ? old + dj : 0.0;
-        dj = nimp - old;
-        apply_impulse(a, b, g.rA, g.rB, g.n * dj);
-    }
-    if (mu > 0.0 && nimp > 0.0) {
-        const V3 vc2 = (a.v + cross(a.w, g.rA)) - (b.v + cross(b.w, g.rB));
-        const V3 vt = vc2 - g.n * dot(vc2, g.n);
-        const double vtl = norm(vt);
-        if (vtl > 1e-12) {
-            const V3 t = vt / vtl;
-            const double Kt = eff_mass(a.im, a.q, a.ii, b.im, b.q, b.ii, g.rA, g.rB, t);
-            if (Kt > 1e-18) {
-                double djt = -vtl / Kt;
-                const double maxF = mu * nimp;
-                const double old = timp;
-                const double sum = old + djt;
-                timp = (sum < -maxF) ? -maxF : (sum > maxF ? maxF : sum);
-                djt = timp - old;
-                apply_impulse(a, b, g.rA, g.rB, t * djt);
-            }
-        }
-    }
-}
 
 // x_a += n (beta overlap / (m_a^-1 + m_b^-1)) m_a^-1, b opposite (contact.hpp:78-85)
 __device__ __forceinline__ void correct(BodiesDev& B, const ConsDev& C, uint32_t k, double beta, V3& xa, V3& xb,
@@ -1869,6 +1804,304 @@ __global__ void k_fp_probe(int op, uint32_t n, const double* a, const double* b,
 }  // namespace
 
 // ===========================================================================
+// Adjoint of the step with the contact set of each step held fixed.
+//
+// The function differentiated is one step as the kernels above run it, except
+// that the contacts gatherContacts returns (pairs, normals, points, depths)
+// are constants of the step.  Everything downstream is differentiated:
+// integrate (src/world.cpp:373-411), the constraint build (lever arms
+// rA = minImage(point - x_a), restitution bias, effective mass, combined
+// friction and restitution; world.cpp:535-553), the warm start carried from
+// the previous step (559-568), the Gauss-Seidel iterations (572-612).  The
+// positional correction (contact.hpp:78-85) and the periodic wrap add terms
+// that do not depend on the inputs.
+//
+// Body adjoints use the field order of the body table (x0..2, v0..2, q0..3,
+// w0..2), SoA over ncap.  Per-constraint accumulators (SoA over ccap):
+constexpr int CA_NINIT = 0, CA_NIMP = 1, CA_TIMP = 2, CA_RA = 3, CA_RB = 6, CA_RBIAS = 9, CA_KN = 10, CA_MU = 11,
+              CA_COUNT = 12;
+constexpr int TAPE_DOUBLES = 14;   // pre-visit va, wa, vb, wb, nimp, timp
+
+struct AdjBodies {   // adjoints of one body table
+    double* f;       // 13 fields x ncap
+    uint32_t ncap;
+    __device__ V3 get3(int f0, uint32_t b) const {
+        return V3{f[(size_t)f0 * ncap + b], f[(size_t)(f0 + 1) * ncap + b], f[(size_t)(f0 + 2) * ncap + b]};
+    }
+    __device__ void put3(int f0, uint32_t b, V3 v) const {
+        f[(size_t)f0 * ncap + b] = v.x; f[(size_t)(f0 + 1) * ncap + b] = v.y; f[(size_t)(f0 + 2) * ncap + b] = v.z;
+    }
+    __device__ Qt getq(uint32_t b) const {
+        return Qt{f[(size_t)6 * ncap + b], f[(size_t)7 * ncap + b], f[(size_t)8 * ncap + b], f[(size_t)9 * ncap + b]};
+    }
+    __device__ void putq(uint32_t b, Qt q) const {
+        f[(size_t)6 * ncap + b] = q.w; f[(size_t)7 * ncap + b] = q.x; f[(size_t)8 * ncap + b] = q.y;
+        f[(size_t)9 * ncap + b] = q.z;
+    }
+};
+constexpr int AF_X = 0, AF_V = 3, AF_W = 10;
+
+__device__ __forceinline__ Qt qadd(Qt a, Qt b) { return Qt{a.w + b.w, a.x + b.x, a.y + b.y, a.z + b.z}; }
+
+// One thread per island: replays the island's solve (warm start, iterations)
+// recording each visit's inputs, then runs it backwards, then reverses the
+// constraint build.  Only bodies that can move carry adjoints (a body that
+// cannot move belongs to no island and its velocity never changes).
+__global__ void __launch_bounds__(64) k_adj_island(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters,
+                                                   const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
+                                                   ConsDev C, uint32_t ccap, double* __restrict__ tape,
+                                                   const double* __restrict__ pre, uint32_t ncap, AdjBodies A,
+                                                   double* __restrict__ ca, const double* __restrict__ nfin,
+                                                   double* __restrict__ nprev, double* __restrict__ cw) {
+    const uint32_t nisl = counters[CNT_ISL];
+    auto CA = [&](int f, uint32_t k) -> double& { return ca[(size_t)f * ccap + k]; };
+    for (uint32_t isl = blockIdx.x * blockDim.x + threadIdx.x; isl < nisl; isl += gridDim.x * blockDim.x) {
+        const uint32_t s = isl_start[isl], e = isl_start[isl + 1];
+        const uint32_t m = e - s;
+        uint32_t* list = isl_list + s;
+        sort_segment(list, m);
+        // --- forward replay with tape (solve_island_seq on global memory)
+        for (uint32_t t = 0; t < m; ++t) {
+            const uint32_t k = list[t];
+            CA(CA_NINIT, k) = C.nimp[k];
+            for (int f = 1; f < CA_COUNT; ++f) CA(f, k) = 0.0;
+            CA(CA_NIMP, k) = nfin[k];
+        }
+        if (P.warm)
+            for (uint32_t t = 0; t < m; ++t) {
+                const uint32_t k = list[t];
+                if (C.nimp[k] == 0.0) continue;
+                BodyView a = load_view(B, C.ci[k]), b = load_view(B, C.cj[k]);
+                warm_apply(a, b, load_geom(C, k), C.nimp[k]);
+                store_view(B, C.ci[k], a);
+                store_view(B, C.cj[k], b);
+            }
+        for (int it = 0; it < P.iters; ++it)
+            for (uint32_t t = 0; t < m; ++t) {
+                const uint32_t k = list[t];
+                const uint32_t i = C.ci[k], j = C.cj[k];
+                BodyView a = load_view(B, i), b = load_view(B, j);
+                double* tp = tape + ((size_t)it * ccap + k) * TAPE_DOUBLES;
+                tp[0] = a.v.x; tp[1] = a.v.y; tp[2] = a.v.z; tp[3] = a.w.x; tp[4] = a.w.y; tp[5] = a.w.z;
+                tp[6] = b.v.x; tp[7] = b.v.y; tp[8] = b.v.z; tp[9] = b.w.x; tp[10] = b.w.y; tp[11] = b.w.z;
+                tp[12] = C.nimp[k]; tp[13] = C.timp[k];
+                double nimp = C.nimp[k], timp = C.timp[k];
+                gs_visit(a, b, load_geom(C, k), C.rbias[k], C.kn[k], C.mu[k], nimp, timp);
+                C.nimp[k] = nimp;
+                C.timp[k] = timp;
+                store_view(B, i, a);
+                store_view(B, j, b);
+            }
+        // --- reverse of the iterations
+        for (int it = P.iters - 1; it >= 0; --it)
+            for (int t = (int)m - 1; t >= 0; --t) {
+                const uint32_t k = list[t];
+                const uint32_t i = C.ci[k], j = C.cj[k];
+                BodyView a = load_view(B, i), b = load_view(B, j);
+                const double* tp = tape + ((size_t)it * ccap + k) * TAPE_DOUBLES;
+                a.v = V3{tp[0], tp[1], tp[2]}; a.w = V3{tp[3], tp[4], tp[5]};
+                b.v = V3{tp[6], tp[7], tp[8]}; b.w = V3{tp[9], tp[10], tp[11]};
+                const bool ma = a.mv || a.mw, mb = b.mv || b.mw;
+                V3 avb = ma ? A.get3(AF_V, i) : V3{0, 0, 0}, awb = ma ? A.get3(AF_W, i) : V3{0, 0, 0};
+                V3 bvb = mb ? A.get3(AF_V, j) : V3{0, 0, 0}, bwb = mb ? A.get3(AF_W, j) : V3{0, 0, 0};
+                Qt qab{0, 0, 0, 0}, qbb{0, 0, 0, 0};
+                ConsAdj cj{V3{CA(CA_RA, k), CA(CA_RA + 1, k), CA(CA_RA + 2, k)},
+                           V3{CA(CA_RB, k), CA(CA_RB + 1, k), CA(CA_RB + 2, k)}, CA(CA_RBIAS, k), CA(CA_KN, k),
+                           CA(CA_MU, k)};
+                double nb = CA(CA_NIMP, k), tb = CA(CA_TIMP, k);
+                gs_visit_rev(a, b, load_geom(C, k), C.rbias[k], C.kn[k], C.mu[k], tp[12], tp[13], avb, awb, bvb, bwb,
+                             nb, tb, cj, qab, qbb);
+                CA(CA_NIMP, k) = nb; CA(CA_TIMP, k) = tb;
+                CA(CA_RA, k) = cj.rA.x; CA(CA_RA + 1, k) = cj.rA.y; CA(CA_RA + 2, k) = cj.rA.z;
+                CA(CA_RB, k) = cj.rB.x; CA(CA_RB + 1, k) = cj.rB.y; CA(CA_RB + 2, k) = cj.rB.z;
+                CA(CA_RBIAS, k) = cj.rbias; CA(CA_KN, k) = cj.kn; CA(CA_MU, k) = cj.mu;
+                if (ma) { A.put3(AF_V, i, avb); A.put3(AF_W, i, awb); A.putq(i, qadd(A.getq(i), qab)); }
+                if (mb) { A.put3(AF_V, j, bvb); A.put3(AF_W, j, bwb); A.putq(j, qadd(A.getq(j), qbb)); }
+            }
+        // --- reverse of the warm start: every warm impulse sees the adjoint of
+        // the velocities after the whole warm start, so this pass ends before
+        // the constraint build adds its terms to the body adjoints
+        if (P.warm)
+            for (int t = (int)m - 1; t >= 0; --t) {
+                const uint32_t k = list[t];
+                const uint32_t i = C.ci[k], j = C.cj[k];
+                const BodyView a = load_view(B, i), b = load_view(B, j);
+                const bool ma = a.mv || a.mw, mb = b.mv || b.mw;
+                const ConsGeom g = load_geom(C, k);
+                V3 rAb{CA(CA_RA, k), CA(CA_RA + 1, k), CA(CA_RA + 2, k)};
+                V3 rBb{CA(CA_RB, k), CA(CA_RB + 1, k), CA(CA_RB + 2, k)};
+                Qt qab{0, 0, 0, 0}, qbb{0, 0, 0, 0};
+                const V3 avb = ma ? A.get3(AF_V, i) : V3{0, 0, 0}, awb = ma ? A.get3(AF_W, i) : V3{0, 0, 0};
+                const V3 bvb = mb ? A.get3(AF_V, j) : V3{0, 0, 0}, bwb = mb ? A.get3(AF_W, j) : V3{0, 0, 0};
+                V3 Jb{0, 0, 0};
+                apply_impulse_rev(a, b, g.rA, g.rB, g.n * CA(CA_NINIT, k), avb, awb, bvb, bwb, Jb, qab, qbb, rAb, rBb);
+                const double ninitb = CA(CA_NIMP, k) + dot(Jb, g.n);
+                const uint32_t src = C.src[k];
+                if (src != 0xFFFFFFFFu) nprev[src] = ninitb;
+                CA(CA_RA, k) = rAb.x; CA(CA_RA + 1, k) = rAb.y; CA(CA_RA + 2, k) = rAb.z;
+                CA(CA_RB, k) = rBb.x; CA(CA_RB + 1, k) = rBb.y; CA(CA_RB + 2, k) = rBb.z;
+                if (ma) A.putq(i, qadd(A.getq(i), qab));
+                if (mb) A.putq(j, qadd(A.getq(j), qbb));
+            }
+        // --- reverse of the constraint build
+        for (int t = (int)m - 1; t >= 0; --t) {
+            const uint32_t k = list[t];
+            const uint32_t i = C.ci[k], j = C.cj[k];
+            const BodyView a = load_view(B, i), b = load_view(B, j);
+            const bool ma = a.mv || a.mw, mb = b.mv || b.mw;
+            const ConsGeom g = load_geom(C, k);
+            V3 rAb{CA(CA_RA, k), CA(CA_RA + 1, k), CA(CA_RA + 2, k)}, rBb{CA(CA_RB, k), CA(CA_RB + 1, k), CA(CA_RB + 2, k)};
+            Qt qab{0, 0, 0, 0}, qbb{0, 0, 0, 0};
+            V3 avb = ma ? A.get3(AF_V, i) : V3{0, 0, 0}, awb = ma ? A.get3(AF_W, i) : V3{0, 0, 0};
+            V3 bvb = mb ? A.get3(AF_V, j) : V3{0, 0, 0}, bwb = mb ? A.get3(AF_W, j) : V3{0, 0, 0};
+            // restitution bias from the pre-solve velocities (world.cpp:40, 544)
+            const V3 va{pre[(size_t)0 * ncap + i], pre[(size_t)1 * ncap + i], pre[(size_t)2 * ncap + i]};
+            const V3 wa{pre[(size_t)3 * ncap + i], pre[(size_t)4 * ncap + i], pre[(size_t)5 * ncap + i]};
+            const V3 vb{pre[(size_t)0 * ncap + j], pre[(size_t)1 * ncap + j], pre[(size_t)2 * ncap + j]};
+            const V3 wb{pre[(size_t)3 * ncap + j], pre[(size_t)4 * ncap + j], pre[(size_t)5 * ncap + j]};
+            const V3 vc = (va + cross(wa, g.rA)) - (vb + cross(wb, g.rB));
+            const double vn = dot(vc, g.n);
+            const WorldDev* wd = B.W ? &B.W[B.wid[i]] : nullptr;
+            const double wrest = wd ? wd->restitution : P.restitution;
+            const double wfric = wd ? wd->friction : P.friction;
+            const double ra = B.rest[i] < 0 ? wrest : B.rest[i], rb = B.rest[j] < 0 ? wrest : B.rest[j];
+            const double fa = B.fric[i] < 0 ? wfric : B.fric[i], fb = B.fric[j] < 0 ? wfric : B.fric[j];
+            const double ecomb = combine_mat(P.rcomb, ra, rb);
+            double eb = 0.0;
+            if (vn < -0.5) {
+                const double rbb = CA(CA_RBIAS, k);
+                eb = -vn * rbb;
+                const V3 vcb = g.n * (-ecomb * rbb);
+                avb = avb + vcb;
+                bvb = bvb - vcb;
+                cross_rev(wa, g.rA, vcb, awb, rAb);
+                cross_rev(wb, g.rB, -vcb, bwb, rBb);
+            }
+            V3 nb{0, 0, 0};
+            eff_mass_rev(a.q, a.ii, b.q, b.ii, g.rA, g.rB, g.n, CA(CA_KN, k), qab, qbb, rAb, rBb, nb);
+            cw[k] = CA(CA_MU, k) * combine_dw(P.fcomb, fa, fb, B.fric[i] < 0, B.fric[j] < 0);
+            cw[(size_t)ccap + k] = eb * combine_dw(P.rcomb, ra, rb, B.rest[i] < 0, B.rest[j] < 0);
+            // rA = minImage(point - x_a): the minimum image is a constant shift
+            if (ma) {
+                A.put3(AF_V, i, avb); A.put3(AF_W, i, awb); A.putq(i, qadd(A.getq(i), qab));
+                A.put3(AF_X, i, A.get3(AF_X, i) - rAb);
+            }
+            if (mb) {
+                A.put3(AF_V, j, bvb); A.put3(AF_W, j, bwb); A.putq(j, qadd(A.getq(j), qbb));
+                A.put3(AF_X, j, A.get3(AF_X, j) - rBb);
+            }
+        }
+    }
+}
+
+// Reverse of integrate (world.cpp:373-411) per body: adjoints at the end of
+// integrate -> adjoints at the start of the step.  `st` is the state at the
+// start of the step (same field order).
+__global__ void __launch_bounds__(TPB) k_adj_integrate(BodiesDev B, SimParams P, const double* __restrict__ st,
+                                                       uint32_t ncap, AdjBodies A) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    auto S3 = [&](int f0) { return V3{st[(size_t)f0 * ncap + i], st[(size_t)(f0 + 1) * ncap + i], st[(size_t)(f0 + 2) * ncap + i]}; };
+    const Qt q0{st[(size_t)6 * ncap + i], st[(size_t)7 * ncap + i], st[(size_t)8 * ncap + i], st[(size_t)9 * ncap + i]};
+    const V3 w0 = S3(10);
+    const double dt = P.dt;
+    V3 xb = A.get3(AF_X, i), vb = A.get3(AF_V, i), wb = A.get3(AF_W, i);
+    Qt qb = A.getq(i);
+    const uint32_t fl = B.flags[i];
+    const double im = B.im[i];
+    if (fl & NE_FLAG_KINEMATIC) {
+        // x1 = x0 + v dt, q1 = integrate_orientation(q0, w, dt); v and w unchanged
+        vb = vb + xb * dt;
+        Qt q0b{0, 0, 0, 0};
+        integrate_orientation_rev(q0, w0, dt, qb, q0b, wb);
+        qb = q0b;
+    } else if (!(im <= 0.0)) {
+        const V3 invI = ld3(B.ii0, B.ii1, B.ii2, i);
+        const double ld = B.ldamp[i], ad = B.adamp[i];
+        if (ld > 0.0) vb = vb * (1.0 / (1.0 + ld * dt));
+        if (ad > 0.0) wb = wb * (1.0 / (1.0 + ad * dt));
+        const Qt q1 = integrate_orientation(q0, w0, dt);
+        V3 L1 = angular_momentum(q0, invI, w0);   // + torque * dt: a constant
+        const V3 tq = ld3(B.t0, B.t1, B.t2, i);
+        L1 = L1 + tq * dt;
+        // w1 = apply_inv_inertia(q1, invI, L1)
+        V3 L1b{0, 0, 0};
+        Qt q1b = qb;
+        sandwich_rev(q1, invI, L1, wb, q1b, L1b);
+        // q1 = integrate_orientation(q0, w0, dt);  L = angular_momentum(q0, invI, w0)
+        Qt q0b{0, 0, 0, 0};
+        V3 w0b{0, 0, 0};
+        integrate_orientation_rev(q0, w0, dt, q1b, q0b, w0b);
+        sandwich_rev(q0, inertia_of(invI), w0, L1b, q0b, w0b);
+        // x1 = x0 + v1 dt, v1 = v0 + (g gs + f im) dt
+        vb = vb + xb * dt;
+        wb = w0b;
+        qb = q0b;
+    }
+    A.put3(AF_V, i, vb);
+    A.put3(AF_W, i, wb);
+    A.putq(i, qb);
+    (void)xb;
+}
+
+// Per-world sums of the constraints' d loss / d mu_world and d loss /
+// d e_world (one block per world, a fixed-order reduction), added to gw.
+__global__ void __launch_bounds__(256) k_adj_world_sum(BodiesDev B, const uint32_t* __restrict__ counters, ConsDev C,
+                                                       const double* __restrict__ cw, uint32_t ccap,
+                                                       double* __restrict__ gw, uint32_t nw) {
+    __shared__ double sm[2][256];
+    const uint32_t w = blockIdx.x;
+    if (w >= nw) return;
+    const uint32_t nc = counters[CNT_C];
+    uint32_t k0 = 0, k1 = nc;
+    if (B.W) {   // constraints are in (i, j) order and a world's bodies are contiguous
+        const uint32_t b0 = B.W[w].body_begin, b1 = B.W[w].body_end;
+        uint32_t lo = 0, hi = nc;
+        while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (C.ci[mid] < b0) lo = mid + 1; else hi = mid; }
+        k0 = lo;
+        hi = nc;
+        while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (C.ci[mid] < b1) lo = mid + 1; else hi = mid; }
+        k1 = lo;
+    }
+    double a = 0.0, b = 0.0;
+    for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        a += cw[k];
+        b += cw[(size_t)ccap + k];
+    }
+    sm[0][threadIdx.x] = a;
+    sm[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + st];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        gw[w] += sm[0][0];
+        gw[nw + w] += sm[1][0];
+    }
+}
+
+// n x 3 interleaved <-> three SoA fields
+__global__ void __launch_bounds__(TPB) k_scatter3(const double* __restrict__ src, uint32_t n, double* __restrict__ f0,
+                                                  double* __restrict__ f1, double* __restrict__ f2) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f0[i] = src[3 * (size_t)i]; f1[i] = src[3 * (size_t)i + 1]; f2[i] = src[3 * (size_t)i + 2];
+}
+__global__ void __launch_bounds__(TPB) k_gather3(const double* __restrict__ f0, const double* __restrict__ f1,
+                                                 const double* __restrict__ f2, uint32_t n, double* __restrict__ dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[3 * (size_t)i] = f0[i]; dst[3 * (size_t)i + 1] = f1[i]; dst[3 * (size_t)i + 2] = f2[i];
+}
+__global__ void __launch_bounds__(TPB) k_fill_f64(double* __restrict__ p, size_t n, double v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ===========================================================================
 // Host side
 // ===========================================================================
 
@@ -1960,6 +2193,28 @@ struct ne_gpu_ctx {
     hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     uint32_t graph_hint_p = 0, graph_hint_c = 0;
     int order_parity = 0;
+    // rollout checkpoints and adjoint work space
+    struct Saved {
+        double* state = nullptr;   // x, v, q, w fields (13 x ncap)
+        uint64_t* key = nullptr;   // warm cache
+        double* imp = nullptr;
+        uint32_t* n = nullptr;
+        uint32_t cap = 0;
+    };
+    std::vector<Saved> ckpt, seg;
+    int32_t roll_steps = 0, roll_every = 0;
+    double* adj = nullptr;        // 13 x ncap body adjoints
+    double* adj_pre = nullptr;    // 6 x ncap pre-solve v, w
+    double* adj_tape = nullptr;   // iterations x ccap x TAPE_DOUBLES
+    double* adj_ca = nullptr;     // CA_COUNT x ccap
+    double* adj_nfin = nullptr;   // ccap
+    double* adj_nprev = nullptr;  // ccap
+    double* adj_cw = nullptr;     // 2 x ccap
+    double* adj_gw = nullptr;     // 2 x worlds
+    double* stage = nullptr;      // n x 3 staging for caller buffers (host or device)
+    uint32_t stage_n = 0;
+    size_t adj_tape_n = 0;
+    uint32_t adj_ccap = 0, adj_ncap = 0, adj_nw = 0;
     // counters
     uint32_t* counters = nullptr;
     uint32_t* h_counters = nullptr;
@@ -2132,6 +2387,7 @@ int alloc_cons(ne_gpu_ctx* c, uint32_t ccap) {
                      &C.ov, &C.rbias, &C.mu, &C.nimp, &C.timp, &C.kn};
     for (double** p : dp) HIP_TRY(c, dalloc(*p, ccap));
     HIP_TRY(c, dalloc(C.key, ccap));
+    HIP_TRY(c, dalloc(C.src, ccap));
     HIP_TRY(c, dalloc(c->cons_root, ccap));
     HIP_TRY(c, dalloc(c->cons_isl, ccap));
     HIP_TRY(c, dalloc(c->isl_list, ccap));
@@ -2191,6 +2447,7 @@ void free_work(ne_gpu_ctx* c) {
                      &C.ov, &C.rbias, &C.mu, &C.nimp, &C.timp, &C.kn};
     for (double** p : dp) dfree(*p);
     dfree(C.key);
+    dfree(C.src);
     dfree(c->parent); dfree(c->cons_root); dfree(c->used); dfree(c->root_id);
     dfree(c->skey0); dfree(c->sval0); dfree(c->skey1); dfree(c->sval1); dfree(c->isl_start);
     dfree(c->slot_of); dfree(c->row_scratch); dfree(c->claimed); dfree(c->cons_isl);
@@ -2204,7 +2461,24 @@ void free_work(ne_gpu_ctx* c) {
     drop_graphs(c);
 }
 
+void free_saved(std::vector<ne_gpu_ctx::Saved>& v) {
+    for (auto& x : v) { dfree(x.state); dfree(x.key); dfree(x.imp); dfree(x.n); }
+    v.clear();
+}
+
+void free_adjoint(ne_gpu_ctx* c) {
+    free_saved(c->ckpt);
+    free_saved(c->seg);
+    dfree(c->adj); dfree(c->adj_pre); dfree(c->adj_tape); dfree(c->adj_ca); dfree(c->adj_nfin);
+    dfree(c->adj_nprev); dfree(c->adj_cw); dfree(c->adj_gw); dfree(c->stage);
+    c->stage_n = 0;
+    c->adj_tape_n = 0;
+    c->adj_ccap = c->adj_ncap = c->adj_nw = 0;
+    c->roll_steps = 0;
+}
+
 void free_all(ne_gpu_ctx* c) {
+    free_adjoint(c);
     free_work(c);
     dfree(c->prev_key); dfree(c->prev_imp); dfree(c->prev_n); dfree(c->counters);
     dfree(c->snap_body); dfree(c->snap_key); dfree(c->snap_imp); dfree(c->snap_n);
@@ -2228,13 +2502,13 @@ void launch_integrate(ne_gpu_ctx* c, const SimParams& P, bool bin) {
 // One reference step (src/world.cpp:133-146) as a kernel pipeline.  When
 // `wrap_first`, the previous step's periodic wrap runs at the top of this
 // step's integrate kernel; the caller wraps after the last step.
-int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
+// The step up to the solve: integrate, grid, candidate rows, narrow phase,
+// constraint build and islands.
+void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
     hipStream_t s = c->stream;
     const SimParams P = make_sim_params(c->params);
     BodiesDev B = c->B;
     const uint32_t n = c->n;
-    if (n == 0) return NE_OK;
-    KTimer whole(c, NE_KERNEL_STEP);
     const bool static_grid = P.box.periodic && c->ngrid > 0;
     {
         KTimer kt(c, NE_KERNEL_INTEGRATE);
@@ -2332,7 +2606,12 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     k_isl_count<<<hint_blocks(c->hint_c), TPB, 0, s>>>(c->counters, c->cons_root, c->root_id, c->cons_isl, c->isl_count);
     scan_u32(c->isl_count, c->isl_start, c->counters + CNT_ISL, c->ccap, c->scan_part, s);
     k_isl_fill<<<hint_blocks(c->hint_c), TPB, 0, s>>>(c->counters, c->cons_isl, c->isl_start, c->isl_fill, c->isl_list);
-    // --- solve
+}
+
+void enqueue_solve(ne_gpu_ctx* c) {
+    hipStream_t s = c->stream;
+    const SimParams P = make_sim_params(c->params);
+    BodiesDev B = c->B;
     {
         KTimer kt(c, NE_KERNEL_SOLVE);
         k_solve_single<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C,
@@ -2344,10 +2623,18 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
         k_solve_large<<<256, 1024, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list,
                                            c->prev_key, c->prev_imp);
     }
+}
+
+int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
+    if (c->n == 0) return NE_OK;
+    KTimer whole(c, NE_KERNEL_STEP);
+    enqueue_front(c, wrap_first);
+    enqueue_solve(c);
     // --- boundary (the periodic wrap is deferred to the next integrate)
+    const SimParams P = make_sim_params(c->params);
     if (!P.box.periodic && !P.open_boundary) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
-        k_walls<<<blocks_for(n), TPB, 0, s>>>(B, P);
+        k_walls<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P);
     }
     return NE_OK;
 }
@@ -2526,6 +2813,223 @@ int run_step(ne_gpu_ctx* c, bool wrap_first) {
     c->order_parity ^= 1;
     return NE_OK;
 }
+
+
+// ---------------------------------------------------------------------------
+// Rollouts and their adjoint
+// ---------------------------------------------------------------------------
+
+int save_state(ne_gpu_ctx* c, ne_gpu_ctx::Saved& d) {
+    const size_t nd = (size_t)13 * c->ncap;
+    if (!d.state) HIP_TRY(c, dalloc(d.state, nd));
+    if (!d.n) HIP_TRY(c, dalloc(d.n, 1));
+    if (d.cap < c->prev_cap) {
+        HIP_TRY(c, dalloc(d.key, c->prev_cap));
+        HIP_TRY(c, dalloc(d.imp, c->prev_cap));
+        d.cap = c->prev_cap;
+    }
+    HIP_TRY(c, hipMemcpyAsync(d.state, c->dfield, sizeof(double) * nd, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(d.key, c->prev_key, sizeof(uint64_t) * c->prev_cap, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(d.imp, c->prev_imp, sizeof(double) * c->prev_cap, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(d.n, c->prev_n, sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    return NE_OK;
+}
+
+int load_state(ne_gpu_ctx* c, const ne_gpu_ctx::Saved& d) {
+    const size_t nd = (size_t)13 * c->ncap;
+    const uint32_t m = std::min(d.cap, c->prev_cap);
+    HIP_TRY(c, hipMemcpyAsync(c->dfield, d.state, sizeof(double) * nd, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->prev_key, d.key, sizeof(uint64_t) * m, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->prev_imp, d.imp, sizeof(double) * m, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->prev_n, d.n, sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    return NE_OK;
+}
+
+int ensure_saved(ne_gpu_ctx* c, std::vector<ne_gpu_ctx::Saved>& v, size_t count) {
+    if (v.size() > count) {
+        for (size_t k = count; k < v.size(); ++k) { dfree(v[k].state); dfree(v[k].key); dfree(v[k].imp); dfree(v[k].n); }
+        v.resize(count);
+    }
+    v.resize(count);
+    return NE_OK;
+}
+
+// Caller n x 3 arrays may be host or device memory: they go through a device
+// staging buffer with hipMemcpyDefault.
+int stage_in(ne_gpu_ctx* c, const double* src, double* f0, double* f1, double* f2) {
+    if (c->stage_n < c->n) {
+        HIP_TRY(c, dalloc(c->stage, (size_t)3 * c->ncap));
+        c->stage_n = c->ncap;
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->stage, src, sizeof(double) * 3 * c->n, hipMemcpyDefault, c->stream));
+    k_scatter3<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->stage, c->n, f0, f1, f2);
+    return NE_OK;
+}
+
+int stage_out(ne_gpu_ctx* c, const double* f0, const double* f1, const double* f2, double* dst) {
+    if (c->stage_n < c->n) {
+        HIP_TRY(c, dalloc(c->stage, (size_t)3 * c->ncap));
+        c->stage_n = c->ncap;
+    }
+    k_gather3<<<blocks_for(c->n), TPB, 0, c->stream>>>(f0, f1, f2, c->n, c->stage);
+    HIP_TRY(c, hipMemcpyAsync(dst, c->stage, sizeof(double) * 3 * c->n, hipMemcpyDefault, c->stream));
+    return NE_OK;
+}
+
+void fill(ne_gpu_ctx* c, double* p, size_t n, double v) {
+    if (n) k_fill_f64<<<gs_blocks(n), TPB, 0, c->stream>>>(p, n, v);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ne_gpu_rollout(ne_gpu_ctx* c, const double* v0, const double* mu, const double* e, int32_t nsteps, int32_t every,
+                   double* x_out) {
+    if (!c || nsteps <= 0 || every <= 0) return NE_ERR_INVALID;
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    if (c->n == 0) return set_err(c, NE_ERR_INVALID, "no bodies");
+    if (!c->params.periodic) return set_err(c, NE_ERR_UNSUPPORTED, "rollouts are differentiated for periodic worlds");
+    if (c->has_forces) return set_err(c, NE_ERR_UNSUPPORTED, "rollouts do not take applied forces");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const uint32_t nw = c->nworlds ? c->nworlds : 1u;
+    if (mu || e) {
+        if (c->nworlds) {
+            for (uint32_t w = 0; w < nw; ++w) {
+                if (mu) c->w_fric[w] = mu[w];
+                if (e) c->w_rest[w] = e[w];
+            }
+        } else {
+            if (mu) c->params.friction = mu[0];
+            if (e) c->params.restitution = e[0];
+        }
+        c->grid_dirty = true;
+    }
+    int r = prepare_static_grid(c);
+    if (r != NE_OK) return r;
+    if (v0) {
+        r = stage_in(c, v0, c->B.v0, c->B.v1, c->B.v2);
+        if (r != NE_OK) return r;
+    }
+    HIP_TRY(c, hipMemsetAsync(c->prev_n, 0, sizeof(uint32_t), c->stream));   // a rollout starts with no warm cache
+    const int32_t nck = (nsteps + every - 1) / every;
+    ensure_saved(c, c->ckpt, (size_t)nck);
+    for (int32_t k = 0; k < nck; ++k) {
+        r = save_state(c, c->ckpt[k]);
+        if (r != NE_OK) return r;
+        const int32_t m = std::min(every, nsteps - k * every);
+        r = ne_gpu_step(c, m);
+        if (r != NE_OK) return r;
+    }
+    if (x_out) {
+        r = stage_out(c, c->B.x0, c->B.x1, c->B.x2, x_out);
+        if (r != NE_OK) return r;
+    }
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->roll_steps = nsteps;
+    c->roll_every = every;
+    c->adj_nw = nw;
+    return NE_OK;
+}
+
+int ne_gpu_rollout_backward(ne_gpu_ctx* c, const double* gx, double* gv0, double* gmu, double* ge) {
+    if (!c || !gx) return NE_ERR_INVALID;
+    if (c->roll_steps <= 0) return set_err(c, NE_ERR_INVALID, "no rollout to differentiate");
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    const uint32_t n = c->n, ncap = c->ncap, ccap = c->ccap, nw = c->adj_nw;
+    const int iters = std::max(c->params.solver_iterations, 1);
+    // work space
+    if (c->adj_ncap != ncap) {
+        HIP_TRY(c, dalloc(c->adj, (size_t)13 * ncap));
+        HIP_TRY(c, dalloc(c->adj_pre, (size_t)6 * ncap));
+        c->adj_ncap = ncap;
+    }
+    if (c->adj_ccap != ccap) {
+        HIP_TRY(c, dalloc(c->adj_ca, (size_t)CA_COUNT * ccap));
+        HIP_TRY(c, dalloc(c->adj_nfin, ccap));
+        HIP_TRY(c, dalloc(c->adj_nprev, ccap));
+        HIP_TRY(c, dalloc(c->adj_cw, (size_t)2 * ccap));
+        c->adj_ccap = ccap;
+        c->adj_tape_n = 0;
+    }
+    const size_t tape_n = (size_t)iters * ccap * TAPE_DOUBLES;
+    if (c->adj_tape_n < tape_n) {
+        HIP_TRY(c, dalloc(c->adj_tape, tape_n));
+        c->adj_tape_n = tape_n;
+    }
+    HIP_TRY(c, dalloc(c->adj_gw, (size_t)2 * nw));
+    hipStream_t s = c->stream;
+    fill(c, c->adj, (size_t)13 * ncap, 0.0);
+    fill(c, c->adj_nfin, ccap, 0.0);
+    fill(c, c->adj_gw, (size_t)2 * nw, 0.0);
+    {
+        int r = stage_in(c, gx, c->adj, c->adj + ncap, c->adj + 2 * (size_t)ncap);
+        if (r != NE_OK) return r;
+    }
+    const int32_t every = c->roll_every, nsteps = c->roll_steps;
+    const int32_t nck = (nsteps + every - 1) / every;
+    ensure_saved(c, c->seg, (size_t)every);
+    AdjBodies A{c->adj, ncap};
+    for (int32_t k = nck - 1; k >= 0; --k) {
+        const int32_t t0 = k * every, t1 = std::min(nsteps, t0 + every);
+        // states at the start of every step of the segment
+        int r = load_state(c, c->ckpt[k]);
+        if (r != NE_OK) return r;
+        for (int32_t t = t0; t < t1; ++t) {
+            r = save_state(c, c->seg[t - t0]);
+            if (r != NE_OK) return r;
+            if (t + 1 < t1) {
+                r = ne_gpu_step(c, 1);
+                if (r != NE_OK) return r;
+            }
+        }
+        for (int32_t t = t1 - 1; t >= t0; --t) {
+            r = load_state(c, c->seg[t - t0]);
+            if (r != NE_OK) return r;
+            r = prepare_static_grid(c);
+            if (r != NE_OK) return r;
+            HIP_TRY(c, hipMemsetAsync(c->counters, 0, sizeof(uint32_t) * CNT_N, s));
+            enqueue_front(c, false);
+            const SimParams P = make_sim_params(c->params);
+            BodiesDev B = c->B;
+            HIP_TRY(c, hipMemcpyAsync(c->adj_pre, B.v0, sizeof(double) * 3 * ncap, hipMemcpyDeviceToDevice, s));
+            HIP_TRY(c, hipMemcpyAsync(c->adj_pre + 3 * (size_t)ncap, B.w0, sizeof(double) * 3 * ncap,
+                                      hipMemcpyDeviceToDevice, s));
+            fill(c, c->adj_nprev, ccap, 0.0);
+            k_adj_island<<<hint_blocks(c->hint_c), 64, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, ccap,
+                                                               c->adj_tape, c->adj_pre, ncap, A, c->adj_ca, c->adj_nfin,
+                                                               c->adj_nprev, c->adj_cw);
+            k_adj_integrate<<<blocks_for(n), TPB, 0, s>>>(B, P, c->seg[t - t0].state, ncap, A);
+            k_adj_world_sum<<<nw, 256, 0, s>>>(B, c->counters, c->C, c->adj_cw, ccap, c->adj_gw, nw);
+            std::swap(c->adj_nfin, c->adj_nprev);
+            HIP_TRY(c, hipGetLastError());
+            HIP_TRY(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(uint32_t) * CNT_N, hipMemcpyDeviceToHost, s));
+            HIP_TRY(c, hipStreamSynchronize(s));
+            if (c->h_counters[CNT_FAULT])
+                return set_err(c, NE_ERR_CAPACITY, "a step overflowed while recomputing the rollout");
+        }
+    }
+    if (gv0) {
+        int r = stage_out(c, c->adj + 3 * (size_t)ncap, c->adj + 4 * (size_t)ncap, c->adj + 5 * (size_t)ncap, gv0);
+        if (r != NE_OK) return r;
+    }
+    std::vector<double> gw(2 * (size_t)nw);
+    HIP_TRY(c, hipMemcpyAsync(gw.data(), c->adj_gw, sizeof(double) * 2 * nw, hipMemcpyDeviceToHost, s));
+    // leave the world at the rollout's start
+    int r = load_state(c, c->ckpt[0]);
+    if (r != NE_OK) return r;
+    HIP_TRY(c, hipStreamSynchronize(s));
+    for (uint32_t w = 0; w < nw; ++w) {
+        if (gmu) gmu[w] = gw[w];
+        if (ge) ge[w] = gw[nw + w];
+    }
+    return NE_OK;
+}
+
+}  // extern "C"
+
+namespace {
 
 int step_call(ne_gpu_ctx* c, int32_t nsteps);
 

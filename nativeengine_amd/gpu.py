@@ -26,6 +26,7 @@ EXPORTS = [
     "ne_gpu_get_pairs", "ne_gpu_diagnostics", "ne_gpu_get_counters", "ne_gpu_get_stream", "ne_gpu_last_error",
     "ne_gpu_fp_probe", "ne_gpu_time_kernel", "ne_gpu_kernel_time", "ne_gpu_export_warm_cache",
     "ne_gpu_import_warm_cache", "ne_gpu_set_worlds", "ne_gpu_use_graph", "ne_gpu_autotune", "ne_gpu_epa_stats",
+    "ne_gpu_rollout", "ne_gpu_rollout_backward",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
@@ -74,6 +75,8 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_set_worlds.argtypes = [vp, C.c_uint32, u32p, dp, dp, dp]
     lib.ne_gpu_use_graph.argtypes = [vp, C.c_int32]
     lib.ne_gpu_epa_stats.argtypes = [vp, u32p, u32p]
+    lib.ne_gpu_rollout.argtypes = [vp, vp, dp, dp, C.c_int32, C.c_int32, vp]
+    lib.ne_gpu_rollout_backward.argtypes = [vp, vp, vp, dp, dp]
     lib.ne_gpu_autotune.argtypes = [vp, C.c_int32, C.POINTER(_abi.TuneResult)]
     for name in EXPORTS:
         if name != "ne_gpu_last_error":
@@ -143,6 +146,29 @@ class GpuWorld:
         a, b = C.c_uint32(), C.c_uint32()
         self._check(self.lib.ne_gpu_epa_stats(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def rollout(self, v0=None, mu=None, e=None, nsteps: int = 120, every: int = 16):
+        """Differentiable rollout from the current state (velocities v0, n x 3,
+        when given; mu, e: one value per world or None).  Returns the final
+        positions (n x 3)."""
+        v = None if v0 is None else np.ascontiguousarray(v0, np.float64)
+        m = None if mu is None else np.ascontiguousarray(mu, np.float64)
+        r = None if e is None else np.ascontiguousarray(e, np.float64)
+        x = np.empty((self.n, 3))
+        self._check(self.lib.ne_gpu_rollout(self.h, None if v is None else v.ctypes.data, _abi.dptr(m), _abi.dptr(r),
+                                            int(nsteps), int(every), x.ctypes.data))
+        return x
+
+    def rollout_backward(self, gx, nworlds: int):
+        """Adjoint of the last rollout for d loss / d x_final = gx (n x 3):
+        returns (d loss / d v0, d loss / d mu per world, d loss / d e per world)."""
+        g = np.ascontiguousarray(gx, np.float64)
+        gv = np.empty((self.n, 3))
+        gmu = np.zeros(nworlds)
+        ge = np.zeros(nworlds)
+        self._check(self.lib.ne_gpu_rollout_backward(self.h, g.ctypes.data, gv.ctypes.data, _abi.dptr(gmu),
+                                                     _abi.dptr(ge)))
+        return gv, gmu, ge
 
     def use_graph(self, enable: bool = True):
         self._check(self.lib.ne_gpu_use_graph(self.h, 1 if enable else 0))

@@ -130,6 +130,22 @@ struct ne_oracle {
     uint64_t* cache_key;
     double* cache_imp;
     uint32_t ncand;
+    /* contact-set tape: mode 1 records each step's filtered contact lists,
+     * mode 2 replays them in place of broadphase + narrow phase */
+    int tape_mode;
+    uint32_t tape_steps, tape_cap, tape_cursor;
+    struct otape_step* tape;
+};
+
+struct otape_pair {
+    uint32_t i, j;
+    int nc;
+    ocontact cs[4];
+};
+
+struct otape_step {
+    size_t n, cap;
+    struct otape_pair* p;
 };
 
 static v3 ld3(const double* p, uint32_t i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
@@ -882,23 +898,56 @@ static void wake(obody* b) {                                                    
     b->sleepTimer = 0.0;
 }
 
+static void tape_push(struct otape_step* st, uint32_t i, uint32_t j, int nc, const ocontact* cs) {
+    if (st->n == st->cap) {
+        st->cap = st->cap ? 2 * st->cap : 256;
+        st->p = (struct otape_pair*)realloc(st->p, sizeof(struct otape_pair) * st->cap);
+    }
+    struct otape_pair* r = &st->p[st->n++];
+    r->i = i; r->j = j; r->nc = nc;
+    memcpy(r->cs, cs, sizeof(ocontact) * (size_t)nc);
+}
+
 static void collide(ne_oracle* o) {                                                  /* world.cpp:503-627 */
     pairvec pv = {0, 0, 0};
-    broadphase(o, &pv);
-    o->ncand = (uint32_t)pv.n;
+    const struct otape_step* replay = NULL;
+    struct otape_step* record = NULL;
+    if (o->tape_mode == 2) {
+        replay = (o->tape_cursor < o->tape_steps) ? &o->tape[o->tape_cursor] : NULL;
+        ++o->tape_cursor;
+    } else {
+        broadphase(o, &pv);
+        if (o->tape_mode == 1) {
+            if (o->tape_steps == o->tape_cap) {
+                o->tape_cap = o->tape_cap ? 2 * o->tape_cap : 64;
+                o->tape = (struct otape_step*)realloc(o->tape, sizeof(struct otape_step) * o->tape_cap);
+            }
+            record = &o->tape[o->tape_steps++];
+            memset(record, 0, sizeof(*record));
+        }
+    }
+    const size_t npairs = (o->tape_mode == 2) ? (replay ? replay->n : 0) : pv.n;
+    o->ncand = (uint32_t)npairs;
     o->ncons = 0;
     const ne_world_params* P = &o->P;
-    for (size_t pi = 0; pi < pv.n; ++pi) {
-        const uint32_t i = pv.p[pi].i, j = pv.p[pi].j;
+    for (size_t pi = 0; pi < npairs; ++pi) {
+        const uint32_t i = replay ? replay->p[pi].i : pv.p[pi].i, j = replay ? replay->p[pi].j : pv.p[pi].j;
         obody* a = &o->B[i];
         obody* b = &o->B[j];
-        if (a->invMass + b->invMass <= 0.0) continue;
-        if (!layers_collide(a, b)) continue;
-        if (pair_excluded(o, a, j)) continue;
-        if ((a->flags & NE_FLAG_SENSOR) || (b->flags & NE_FLAG_SENSOR)) continue;
         ocontact cs[4];
-        int nc = gather_contacts(o, a, b, cs);
-        if (nc == 0) continue;
+        int nc;
+        if (replay) {                 /* the recorded contacts of this pair: fixed contact set */
+            nc = replay->p[pi].nc;
+            memcpy(cs, replay->p[pi].cs, sizeof(ocontact) * (size_t)nc);
+        } else {
+            if (a->invMass + b->invMass <= 0.0) continue;
+            if (!layers_collide(a, b)) continue;
+            if (pair_excluded(o, a, j)) continue;
+            if ((a->flags & NE_FLAG_SENSOR) || (b->flags & NE_FLAG_SENSOR)) continue;
+            nc = gather_contacts(o, a, b, cs);
+            if (nc == 0) continue;
+            if (record) tape_push(record, i, j, nc, cs);
+        }
         const double e = combine_mat(P->restitution_combine, eff_restitution(o, a), eff_restitution(o, b));
         const double mu = combine_mat(P->friction_combine, eff_friction(o, a), eff_friction(o, b));
         if (P->sleep_enabled) {                                                      /* world.cpp:527-531 */
@@ -1083,6 +1132,8 @@ int ne_oracle_create(ne_oracle** out) {
 int ne_oracle_destroy(ne_oracle* o) {
     if (!o) return NE_OK;
     free(o->B); free(o->hull); free(o->nocol); free(o->cons); free(o->cache_key); free(o->cache_imp);
+    for (uint32_t t = 0; t < o->tape_steps; ++t) free(o->tape[t].p);
+    free(o->tape);
     free(o);
     return NE_OK;
 }
@@ -1160,6 +1211,24 @@ int ne_oracle_set_state(ne_oracle* o, const double* x, const double* v, const do
 int ne_oracle_step(ne_oracle* o, int32_t nsteps) {
     if (!o) return NE_ERR_INVALID;
     for (int32_t s = 0; s < nsteps; ++s) world_step(o);
+    return NE_OK;
+}
+
+/* Contact-set tape (test infrastructure for the adjoint): mode 1 clears the
+ * tape and records every following step's contact lists; mode 2 replays the
+ * tape from its first step, using the recorded contacts (pairs, normals,
+ * points, depths) instead of broadphase and narrow phase, with the rest of
+ * the step unchanged; mode 0 steps normally.  Replaying at the recorded
+ * inputs reproduces the recorded run exactly; at perturbed inputs it is the
+ * same step with the contact set held fixed. */
+int ne_oracle_tape(ne_oracle* o, int32_t mode) {
+    if (!o || mode < 0 || mode > 2) return NE_ERR_INVALID;
+    if (mode == 1) {
+        for (uint32_t t = 0; t < o->tape_steps; ++t) free(o->tape[t].p);
+        o->tape_steps = 0;
+    }
+    o->tape_cursor = 0;
+    o->tape_mode = mode;
     return NE_OK;
 }
 

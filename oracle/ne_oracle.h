@@ -46,6 +46,7 @@ int ne_oracle_min_image(ne_oracle *o, const double *d, double *out);
 int ne_oracle_wrap(ne_oracle *o, const double *p, double *out);
 /* Largest EPA face list / horizon seen in this process (workspace sizing). */
 void ne_oracle_epa_stats(int32_t *max_faces, int32_t *max_horizon);
+int ne_oracle_tape(ne_oracle *o, int32_t mode);
 void ne_oracle_epa_stats_reset(int32_t faces, int32_t horizon);
 void ne_oracle_epa_counts(int64_t *calls, int64_t *big);
 

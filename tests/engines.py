@@ -62,6 +62,7 @@ def _load(kind):
         lib.ne_oracle_get_pairs.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _u32p]
         lib.ne_oracle_set_state.argtypes = [_vp, _dp, _dp, _dp, _dp]
         lib.ne_oracle_candidate_count.argtypes = [_vp, _u32p]
+        lib.ne_oracle_tape.argtypes = [_vp, C.c_int32]
     else:
         lib = _bind(C.CDLL(REF_LIB), "ne_ref_")
         lib.ne_ref_get_contacts.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
@@ -186,6 +187,11 @@ class OracleWorld(_CpuWorld):
     def set_state(self, x=None, v=None, q=None, w=None):
         arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (x, v, q, w)]
         self._check(self.lib.ne_oracle_set_state(self.h, *[None if a is None else _abi.dptr(a) for a in arrs]), "set_state")
+
+    def tape(self, mode):
+        """Contact-set tape: 1 records each following step's contact lists, 2
+        replays them from the first recorded step (fixed contact set), 0 off."""
+        self._check(self.lib.ne_oracle_tape(self.h, int(mode)), "tape")
 
 
 class RefWorld(_CpuWorld):

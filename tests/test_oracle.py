@@ -309,3 +309,28 @@ def test_epa_blowup_pair_is_pathological_in_the_oracle(oracle_lib):
     assert f.value == 99198 and h.value == 30066
     c = o.contacts()
     assert len(c["i"]) == 1 and abs(c["depth"][0] - 0.0148141) < 1e-6
+
+
+def test_contact_tape_replay_reproduces_the_recorded_run(oracle_lib):
+    """The adjoint's fixed-contact-set function: replaying the recorded
+    contact lists at the recorded inputs gives the recorded run bit for bit,
+    and at perturbed velocities gives a nearby run (same constraint count)."""
+    import scenes
+    s = S.adjoint_worlds(1, n_spheres=24, n_boxes=24, side=6.0, seed=3).world(0)
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    o.tape(1)
+    o.step(20)
+    want = o.state()
+    o.tape(2)
+    o.set_state(s.x, s.v, s.q, s.w)
+    o.reset_warm_cache()
+    o.step(20)
+    for u, v in zip(o.state(), want):
+        assert np.array_equal(u, v)
+    o.tape(2)
+    o.set_state(s.x, s.v * (1.0 + 1e-7), s.q, s.w)
+    o.reset_warm_cache()
+    o.step(20)
+    x = o.state()[0]
+    assert 0.0 < np.max(np.abs(x - want[0])) < 1e-4
