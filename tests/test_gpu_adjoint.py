@@ -112,3 +112,69 @@ def test_torch_autograd_function(gpu_available):
     assert np.allclose(v0.grad.numpy(), gv, rtol=1e-12, atol=0)
     assert np.allclose(mu.grad.numpy(), gmu, rtol=1e-12, atol=0) and np.allclose(e.grad.numpy(), ge, rtol=1e-12, atol=0)
     dw.close()
+
+
+def test_adjoint_at_full_rollout_size_against_fd(gpu_available, oracle_lib):
+    """The differentiable-rollout workload at full size: 256 worlds of 256
+    spheres (r 0.4) and 256 boxes (half extent 0.3) on a static ground box,
+    periodic side 16, 120 steps, checkpoint every 16, MSE loss against
+    targets made at mu 0.4 / e 0.3, evaluated at mu 0.5 / e 0.2.  World 0's
+    gradients (d/d mu, d/d e, 4 random directional derivatives in v0) are
+    checked against central differences of the oracle's fixed-contact-set
+    replay.  Over 120 steps the sensitivities grow ~4000-fold and clamps
+    switch within +-1e-6 of the nominal inputs, so a step of 1e-6 measures
+    kinks rather than the derivative; the difference quotients settle at
+    h = 1e-8 .. 1e-9 (where rounding limits them to ~1e-6).  Bar: the better
+    of the two within 1e-4 relative."""
+    from nativeengine_amd.gpu import GpuWorld
+    b = S.adjoint_worlds(256, n_spheres=256, n_boxes=256, side=16.0, seed=2)
+    nw, steps, every = b.nworlds, 120, 16
+    g = GpuWorld()
+    g.set_scene(b.scene)
+    g.set_worlds(b.offsets, b.half, b.restitution, b.friction)
+    target = g.rollout(b.scene.v, np.full(nw, 0.4), np.full(nw, 0.3), steps, every)
+    mu, e = np.full(nw, 0.5), np.full(nw, 0.2)
+    g.set_state(b.scene.x, b.scene.v, b.scene.q, b.scene.w)
+    x = g.rollout(b.scene.v, mu, e, steps, every)
+    gx = 2.0 * (x - target) / x.size
+    gv, gmu, ge = g.rollout_backward(gx, nw)
+    g.close()
+    w = 0
+    a, c = int(b.offsets[w]), int(b.offsets[w + 1])
+    world = b.world(w)
+    o = engines.OracleWorld()
+    s0 = world.copy()
+    s0.params.friction, s0.params.restitution = mu[w], e[w]
+    o.set_scene(s0)
+    o.tape(1)
+    o.step(steps)
+    assert np.array_equal(o.state()[0], x[a:c])
+    tw = target[a:c]
+
+    def f(v, m, r):
+        s = world.copy()
+        s.params.friction, s.params.restitution = m, r
+        o.set_params(s.params)
+        o.set_state(s.x, v, s.q, s.w)
+        o.reset_warm_cache()
+        o.tape(2)
+        o.step(steps)
+        return float(np.sum((o.state()[0] - tw) ** 2)) / x.size
+
+    rng = np.random.default_rng(11)
+    dirs = []
+    for k in range(4):
+        d = rng.normal(size=world.v.shape)
+        d[world.inv_mass == 0] = 0.0
+        dirs.append(d)
+    worst = 0.0
+    quantities = [("mu", gmu[w], lambda h: (f(world.v, mu[w] + h, e[w]) - f(world.v, mu[w] - h, e[w])) / (2 * h)),
+                  ("e", ge[w], lambda h: (f(world.v, mu[w], e[w] + h) - f(world.v, mu[w], e[w] - h)) / (2 * h))]
+    for k, d in enumerate(dirs):
+        quantities.append((f"v0 dir {k}", float(np.sum(gv[a:c] * d)),
+                           lambda h, d=d: (f(world.v + h * d, mu[w], e[w]) - f(world.v - h * d, mu[w], e[w])) / (2 * h)))
+    for name, ad, fdf in quantities:
+        best = min((abs(ad - fd) / max(abs(ad), abs(fd), 1e-12), h, fd) for h in (1e-8, 1e-9) for fd in [fdf(h)])
+        worst = max(worst, best[0])
+        _say(f"full size world {w} {name}: adjoint {ad:.10e} fd {best[2]:.10e} (h {best[1]:.0e}) rel {best[0]:.2e}")
+    assert worst <= 1e-4, worst
