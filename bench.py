@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""bench.py -- body-steps/s of the MI355X step at 1M bodies.
+"""bench.py -- body-steps/s of the MI355X step at 1M bodies; batched
+world-steps/s; backward/forward time ratio of differentiable rollouts.
 
 Workload: the plain-method point of the headline metric, a periodic elastic
 sphere gas of 1,048,576 spheres (r = 0.5, m = 1) at the 4096-sphere/L=40
@@ -10,7 +11,21 @@ HBM before the timed region; a step is one full `World::step()`
 (integrate, broadphase, narrow phase, warm-started sequential impulses,
 positional correction, periodic wrap).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--cpu-steps S]
+Batched worlds (reported under "batched"): 1024 independent periodic worlds
+of log-uniform size in [64, 4096] (seed 1; ~1M bodies), half spheres and half
+boxes (sizes U[0.2,0.6]) at 20% volume fraction, per-world friction U[0,0.8]
+and restitution U[0,1], 8 iterations, stepped together in one set of launches
+(HIP-graph replay, configuration chosen by the autotuner) against looping the
+plain single-world step over a sample of the same worlds, from the same state.
+
+Differentiable rollouts (reported under "adjoint"): 256 worlds of 256 spheres
+(r 0.4) and 256 boxes (half extent 0.3) on a static ground box, periodic side
+16, gravity, 120 steps at dt 1/60, checkpoint every 16; loss = MSE of final
+positions against targets made at friction 0.4 / restitution 0.3, evaluated
+at 0.5 / 0.2; gradients with respect to the initial velocities and each
+world's friction and restitution.  Reports backward time / forward time.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--cpu-steps S] [--no-batched] [--no-adjoint]
 
 Prints one JSON line.  `roofline` is measured live with HIP events on the
 library's own stream around every launch of the dominant kernel inside the
@@ -88,22 +103,129 @@ def cpu_baseline(scene, steps: int):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--breakdown", action="store_true", help="also time every stage (extra untimed steps)")
-    args = ap.parse_args()
+def batched_bench(worlds: int, steps: int, warmup: int, loop_worlds: int, loop_steps: int):
+    """Batched world-steps/s against looping the plain step over `loop_worlds`
+    of the same worlds (their own contexts, same state and warm cache)."""
+    import numpy as np
+    from nativeengine_amd import scene as scn
+    from nativeengine_amd.gpu import GpuWorld
+    t0 = time.perf_counter()
+    b = scn.batched_worlds(worlds, seed=1)
+    log(f"batched: {b.nworlds} worlds, {b.scene.n} bodies, built in {time.perf_counter() - t0:.1f}s")
+    g = GpuWorld()
+    g.set_scene(b.scene)
+    g.set_worlds(b.offsets, b.half, b.restitution, b.friction)
+    g.use_graph(True)
+    t0 = time.perf_counter()
+    for _ in range(warmup):
+        g.step(1)
+    log(f"batched warm-up {warmup} steps: {time.perf_counter() - t0:.1f}s")
+    tune = g.autotune(steps=3)
+    log(f"autotuner: {tune}")
+    t0 = time.perf_counter()
+    g.step(steps)
+    t1 = time.perf_counter()
+    reruns, trunc = g.epa_stats()
+    batched_ms = (t1 - t0) * 1e3 / steps
+    # the same state and warm cache, each sampled world in its own context
+    x, v, q, w = g.state()
+    keys, imp = g.export_warm_cache()
+    ki, kj = (keys >> np.uint64(32)).astype(np.int64), (keys & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    rng = np.random.default_rng(7)
+    sample = np.sort(rng.choice(b.nworlds, size=min(loop_worlds, b.nworlds), replace=False))
+    ctxs = []
+    for wi in sample:
+        a, e = int(b.offsets[wi]), int(b.offsets[wi + 1])
+        ws = b.world(int(wi))
+        ws.x, ws.v, ws.q, ws.w = x[a:e].copy(), v[a:e].copy(), q[a:e].copy(), w[a:e].copy()
+        c = GpuWorld()
+        c.set_scene(ws)
+        sel = (ki >= a) & (ki < e)
+        c.import_warm_cache(((ki[sel] - a).astype(np.uint64) << np.uint64(32)) | (kj[sel] - a).astype(np.uint64),
+                            imp[sel])
+        ctxs.append(c)
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        for c in ctxs:
+            c.step(1)
+    t1 = time.perf_counter()
+    loop_ms_per_world_step = (t1 - t0) * 1e3 / (loop_steps * len(ctxs))
+    for c in ctxs:
+        c.close()
+    g.close()
+    batched_wps = b.nworlds / (batched_ms * 1e-3)
+    loop_wps = 1e3 / loop_ms_per_world_step
+    return {
+        "metric": "batched world-steps/s",
+        "value": batched_wps,
+        "unit": "world-steps/s",
+        "ms_per_step": batched_ms,
+        "steps": steps,
+        "warmup": warmup,
+        "worlds": b.nworlds,
+        "bodies": int(b.scene.n),
+        "body_steps_per_s": b.scene.n / (batched_ms * 1e-3),
+        "loop_world_steps_per_s": loop_wps,
+        "loop_sample": f"{len(ctxs)} of the worlds, each in its own context, {loop_steps} steps each, same state",
+        "vs_loop": batched_wps / loop_wps,
+        "autotune": tune,
+        "epa_large_workspace_reruns": reruns,
+        "epa_truncated": trunc,
+        "config": "1024 periodic worlds, N_w log-uniform [64,4096] seed 1, half spheres r U[0.2,0.6] / half boxes "
+                  "he U[0.2,0.6]^3, 20% volume fraction, mu_w U[0,0.8], e_w U[0,1], g=0, dt=1/60, 8 iterations",
+    }
 
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
-                                   "(DESIGN.md), so --gpus must be 1", "n_gpus": args.gpus}))
-        sys.exit(2)
 
+def adjoint_bench(worlds: int, steps: int, every: int):
+    """Forward rollout and adjoint times on the differentiable-rollout workload."""
+    import numpy as np
+    from nativeengine_amd import scene as scn
+    from nativeengine_amd.gpu import GpuWorld
+    b = scn.adjoint_worlds(worlds, n_spheres=256, n_boxes=256, side=16.0, seed=2)
+    nw = b.nworlds
+    g = GpuWorld()
+    g.set_scene(b.scene)
+    g.set_worlds(b.offsets, b.half, b.restitution, b.friction)
+    target = g.rollout(b.scene.v, np.full(nw, 0.4), np.full(nw, 0.3), steps, every)
+    mu, e = np.full(nw, 0.5), np.full(nw, 0.2)
+    fwd, bwd = [], []
+    for _ in range(2):
+        g.set_state(b.scene.x, b.scene.v, b.scene.q, b.scene.w)
+        t0 = time.perf_counter()
+        x = g.rollout(b.scene.v, mu, e, steps, every)
+        t1 = time.perf_counter()
+        gv, gmu, ge = g.rollout_backward(2.0 * (x - target) / x.size, nw)
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t2 - t1)
+    g.close()
+    f, bk = min(fwd), min(bwd)
+    return {
+        "metric": "backward/forward time ratio",
+        "value": bk / f,
+        "unit": "ratio",
+        "higher_is_better": False,
+        "forward_s": f,
+        "backward_s": bk,
+        "worlds": nw,
+        "bodies": int(b.scene.n),
+        "steps": steps,
+        "checkpoint_every": every,
+        "body_steps_per_s_forward": b.scene.n * steps / f,
+        "loss": float(np.mean((x - target) ** 2)),
+        "grad_norms": {"v0": float(np.linalg.norm(gv)), "mu": float(np.linalg.norm(gmu)), "e": float(np.linalg.norm(ge))},
+        "gradient": "exact adjoint of the step with each step's contact set fixed (integrate, constraint build, "
+                    "warm start, 8 Gauss-Seidel iterations); checked against finite differences of the oracle's "
+                    "fixed-contact replay in tests/test_gpu_adjoint.py",
+        "config": "256 worlds x (256 spheres r=0.4 + 256 boxes he=0.3 + static ground box), periodic side 16, "
+                  "g=-9.81, dt=1/60, 8 iterations, 120 steps, loss MSE vs targets at mu 0.4 / e 0.3, "
+                  "evaluated at mu 0.5 / e 0.2",
+    }
+
+
+def gas_bench(args):
+    """The 1M-sphere gas: the headline body-steps/s line, with roofline and
+    CPU baseline."""
     from nativeengine_amd import scene as scn
     from nativeengine_amd.gpu import GpuWorld, KERNELS
 
@@ -160,6 +282,7 @@ def main():
         "units_per_launch": units,
         "avg_launch_us": round(avg_launch_ms * 1e3, 3),
     }
+    g.close()
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"cpu baseline: {args.cpu_steps} steps ...")
@@ -167,8 +290,8 @@ def main():
         if cpu:
             log(f"cpu baseline {cpu['kind']}: {cpu['value']:.4g} body-steps/s ({cpu['seconds']:.1f}s)")
 
-    out = {
-        "metric": "body-steps/s at 1M bodies",
+    return {
+        "metric": "body-steps/s at 1M bodies; batched world-steps/s; backward/forward time ratio",
         "value": value,
         "unit": "body-steps/s",
         "n_gpus": 1,
@@ -188,13 +311,59 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "gpu_step_span_ms": round(step_gpu_ms, 4),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "per_step": {"candidate_pairs": counters["candidate_pairs"], "constraints": counters["constraints"],
                      "islands": counters["islands"], "cells": counters["cells"]},
     }
-    if cpu:
-        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--breakdown", action="store_true", help="also time every stage (extra untimed steps)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--worlds", type=int, default=1024)
+    ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--batched-warmup", type=int, default=30)
+    ap.add_argument("--loop-worlds", type=int, default=128)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-gas", action="store_true", help="skip the 1M gas (profiling the other workloads alone)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
+                                   "(DESIGN.md), so --gpus must be 1", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    out = {}
+    if not args.no_gas:
+        out = gas_bench(args)
+    batched = None
+    if not args.no_batched:
+        batched = batched_bench(args.worlds, args.batched_steps, args.batched_warmup, args.loop_worlds, 3)
+        log(f"batched: {batched['value']:.4g} world-steps/s, {batched['ms_per_step']:.3f} ms/step, "
+            f"{batched['vs_loop']:.1f}x the loop")
+
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_bench(256, 120, 16)
+        log(f"adjoint: forward {adjoint['forward_s']:.3f}s backward {adjoint['backward_s']:.3f}s "
+            f"ratio {adjoint['value']:.2f}")
+
+    if not out:
+        out = {"metric": "body-steps/s at 1M bodies; batched world-steps/s; backward/forward time ratio",
+               "value": None, "unit": "body-steps/s", "n_gpus": 1, "note": "1M gas skipped (--no-gas)"}
+    if batched:
+        out["batched"] = batched
+    if adjoint:
+        out["adjoint"] = adjoint
     print(json.dumps(out))
 
 
