@@ -116,6 +116,17 @@ typedef struct ne_bodies {
     const uint32_t *nocollide;                /* partner body indices          */
 } ne_bodies;
 
+/* A distance joint (src/world.hpp:20-26): bodies a and b joined at body-frame
+ * anchors local_a / local_b.  stiffness == 0: a rigid link (bilateral
+ * velocity constraint with Baumgarte length correction, solved after the
+ * contacts); stiffness > 0: a spring-damper applied as a force before
+ * integration. */
+typedef struct ne_distance_joint {
+    uint32_t a, b;
+    double   local_a[3], local_b[3];
+    double   rest, stiffness, damping;
+} ne_distance_joint;
+
 /* Context creation options. Zero fields take defaults. */
 typedef struct ne_gpu_config {
     int32_t  device;            /* HIP device ordinal                         */
@@ -147,6 +158,11 @@ int ne_gpu_set_params(ne_gpu_ctx *ctx, const ne_world_params *p);
  * reference's cache survives body edits); call ne_gpu_reset_warm_cache to drop
  * it, as `World::resetWarmCache` (src/world.hpp:140). */
 int ne_gpu_set_bodies(ne_gpu_ctx *ctx, const ne_bodies *b);
+
+/* Replaces assigning `World::distanceJoints` (src/world.hpp:129): the joints
+ * act in the order given, as the reference's vector does.  Bodies must be
+ * uploaded first; ne_gpu_set_bodies clears the joints. */
+int ne_gpu_set_joints(ne_gpu_ctx *ctx, uint32_t n, const ne_distance_joint *joints);
 
 /* Uploads only the dynamic state (x, v, q, w; any may be NULL) of the current
  * bodies. */

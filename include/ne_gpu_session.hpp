@@ -67,6 +67,7 @@ private:
     std::vector<double> x_, v_, q_, w_, f_, t_, im_, ii_, r_, hh_, he_, fr_, re_, ld_, ad_, gs_, hull_;
     std::vector<std::int32_t> shape_;
     std::vector<std::uint32_t> layer_, mask_, flags_, hoff_, noff_, nc_;
+    std::vector<ne_distance_joint> joints_;
 
     [[noreturn]] void fail(int rc) {
         throw std::runtime_error("ne_gpu step failed (" + std::to_string(rc) + "): " + ne_gpu_last_error(ctx_));
@@ -138,7 +139,26 @@ private:
         B.hull_offsets = hoff_.data(); B.hull_verts = hull_.data();
         B.nocollide_offsets = noff_.data(); B.nocollide = nc_.data();
         check(ne_gpu_set_bodies(ctx_, &B));
+        uploadJoints(w, 0);
     }
+
+    // World::distanceJoints (src/world.hpp:20-26, 129), when the type has them
+    template <class W>
+    auto uploadJoints(W& w, int) -> decltype(w.distanceJoints.size(), void()) {
+        joints_.resize(w.distanceJoints.size());
+        for (size_t k = 0; k < joints_.size(); ++k) {
+            const auto& d = w.distanceJoints[k];
+            ne_distance_joint& j = joints_[k];
+            j.a = static_cast<std::uint32_t>(d.a);
+            j.b = static_cast<std::uint32_t>(d.b);
+            j.local_a[0] = d.localA.x; j.local_a[1] = d.localA.y; j.local_a[2] = d.localA.z;
+            j.local_b[0] = d.localB.x; j.local_b[1] = d.localB.y; j.local_b[2] = d.localB.z;
+            j.rest = d.rest; j.stiffness = d.stiffness; j.damping = d.damping;
+        }
+        check(ne_gpu_set_joints(ctx_, static_cast<std::uint32_t>(joints_.size()), joints_.data()));
+    }
+    template <class W>
+    void uploadJoints(W&, long) {}
 
     template <class W>
     void download(W& w) {

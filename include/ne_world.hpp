@@ -218,6 +218,16 @@ inline Body makePlane(int id, const V3& point, const V3& normal) {
 
 
 
+// A distance joint (src/world.hpp:20-26): stiffness 0 = a rigid link,
+// stiffness > 0 = a spring-damper.
+struct DistanceJoint {
+    std::size_t a = 0, b = 0;
+    V3 localA, localB;
+    double rest = 0.0;
+    double stiffness = 0.0;
+    double damping = 0.0;
+};
+
 // The reference's World (src/world.hpp:73-161), stepping on the GPU.
 class World {
 public:
@@ -246,6 +256,7 @@ public:
     bool forceBruteForce = false;
     int solverIterations = 8;          // extension: the reference fixes 8
     std::vector<Body> bodies;
+    std::vector<DistanceJoint> distanceJoints;
 
     World() = default;
     // Value semantics like the reference's World: a copy carries the bodies
@@ -258,6 +269,7 @@ public:
         sleepEnabled = o.sleepEnabled; sleepLinVel = o.sleepLinVel; sleepAngVel = o.sleepAngVel;
         sleepTime = o.sleepTime; wakeVel = o.wakeVel; openBoundary = o.openBoundary; warmStart = o.warmStart;
         forceBruteForce = o.forceBruteForce; solverIterations = o.solverIterations; bodies = o.bodies;
+        distanceJoints = o.distanceJoints;
         session_.reset();
         cacheKeys_ = o.cacheKeys_;
         cacheImp_ = o.cacheImp_;

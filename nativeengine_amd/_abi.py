@@ -89,6 +89,10 @@ class GpuConfig(C.Structure):
 
 BROADPHASE_AUTO, BROADPHASE_WALK, BROADPHASE_BRICK, BROADPHASE_CELLS = 0, 1, 2, 3
 
+# ne_distance_joint as a numpy record (80 bytes, the C layout)
+JOINT_DTYPE = np.dtype([("a", "<u4"), ("b", "<u4"), ("local_a", "<f8", (3,)), ("local_b", "<f8", (3,)),
+                        ("rest", "<f8"), ("stiffness", "<f8"), ("damping", "<f8")])
+
 
 class TuneResult(C.Structure):
     _fields_ = [

@@ -1803,6 +1803,97 @@ __global__ void k_fp_probe(int op, uint32_t n, const double* a, const double* b,
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Distance joints (src/world.cpp:150-197).  Joints are grouped on the host
+// into components: joints that share a body that can move are in one
+// component, swept by one thread in the joints' own order, so the
+// reference's sequential order is kept where it matters and components run
+// in parallel.
+// ---------------------------------------------------------------------------
+
+struct JointDev {
+    const uint32_t *a, *b;
+    const double *la0, *la1, *la2, *lb0, *lb1, *lb2, *rest, *stiff, *damp;
+    const uint32_t *comp_start, *comp_list;
+    uint32_t ncomp;
+};
+
+struct JointGeom {
+    V3 nrm, rA, rB;
+    double dist;
+};
+
+// Anchor geometry of joint k; false when the anchors coincide (dist < 1e-9).
+__device__ __forceinline__ bool joint_geom(const BodiesDev& B, const SimParams& P, const JointDev& J, uint32_t k,
+                                           uint32_t a, uint32_t b, JointGeom& g) {
+    const Box3& box = box_of(B, P, a);
+    const V3 xa = ld3(B.x0, B.x1, B.x2, a), xb = ld3(B.x0, B.x1, B.x2, b);
+    const V3 wa = xa + qrot(ldq(B, a), V3{J.la0[k], J.la1[k], J.la2[k]});
+    const V3 wb = xb + qrot(ldq(B, b), V3{J.lb0[k], J.lb1[k], J.lb2[k]});
+    const V3 d = min_image(box, wa - wb);
+    g.dist = norm(d);
+    if (g.dist < 1e-9) return false;
+    g.nrm = d / g.dist;
+    g.rA = wa - xa;
+    g.rB = min_image(box, wb - xb);
+    return true;
+}
+
+// Spring-damper joints: force and torque added before integrate
+// (world.cpp:150-170), in joint order within a component.
+__global__ void __launch_bounds__(64) k_joint_springs(BodiesDev B, SimParams P, JointDev J) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < J.ncomp; c += gridDim.x * blockDim.x) {
+        for (uint32_t t = J.comp_start[c]; t < J.comp_start[c + 1]; ++t) {
+            const uint32_t k = J.comp_list[t];
+            if (J.stiff[k] <= 0.0) continue;
+            const uint32_t a = J.a[k], b = J.b[k];
+            JointGeom g;
+            if (!joint_geom(B, P, J, k, a, b, g)) continue;
+            const V3 va = ld3(B.v0, B.v1, B.v2, a), wa = ld3(B.w0, B.w1, B.w2, a);
+            const V3 vb = ld3(B.v0, B.v1, B.v2, b), wb = ld3(B.w0, B.w1, B.w2, b);
+            const V3 vrel = (va + cross(wa, g.rA)) - (vb + cross(wb, g.rB));
+            const double vn = dot(vrel, g.nrm);
+            const double f = -J.stiff[k] * (g.dist - J.rest[k]) - J.damp[k] * vn;
+            const V3 F = g.nrm * f;
+            // a body that cannot move ignores force and torque (integrate skips it)
+            if (mutable_body(B, a)) {
+                st3(B.f0, B.f1, B.f2, a, ld3(B.f0, B.f1, B.f2, a) + F);
+                st3(B.t0, B.t1, B.t2, a, ld3(B.t0, B.t1, B.t2, a) + cross(g.rA, F));
+            }
+            if (mutable_body(B, b)) {
+                st3(B.f0, B.f1, B.f2, b, ld3(B.f0, B.f1, B.f2, b) - F);
+                st3(B.t0, B.t1, B.t2, b, ld3(B.t0, B.t1, B.t2, b) - cross(g.rB, F));
+            }
+        }
+    }
+}
+
+// Rigid joints: 8 sweeps of bilateral impulses with a Baumgarte length term
+// after the contact solve (world.cpp:173-197).
+__global__ void __launch_bounds__(64) k_joint_rigid(BodiesDev B, SimParams P, JointDev J) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < J.ncomp; c += gridDim.x * blockDim.x) {
+        const uint32_t t0 = J.comp_start[c], t1 = J.comp_start[c + 1];
+        for (int it = 0; it < 8; ++it)
+            for (uint32_t t = t0; t < t1; ++t) {
+                const uint32_t k = J.comp_list[t];
+                if (J.stiff[k] > 0.0) continue;
+                const uint32_t a = J.a[k], b = J.b[k];
+                JointGeom g;
+                if (!joint_geom(B, P, J, k, a, b, g)) continue;
+                BodyView A = load_view(B, a), Bv = load_view(B, b);
+                const double K = eff_mass(A.im, A.q, A.ii, Bv.im, Bv.q, Bv.ii, g.rA, g.rB, g.nrm);
+                if (K <= 1e-18) continue;
+                const V3 vrel = (A.v + cross(A.w, g.rA)) - (Bv.v + cross(Bv.w, g.rB));
+                const double vn = dot(vrel, g.nrm);
+                const double bias = 0.2 * (g.dist - J.rest[k]) / P.dt;
+                const double dj = -(vn + bias) / K;
+                apply_impulse(A, Bv, g.rA, g.rB, g.nrm * dj);
+                store_view(B, a, A);
+                store_view(B, b, Bv);
+            }
+    }
+}
+
 // ===========================================================================
 // Adjoint of the step with the contact set of each step held fixed.
 //
@@ -2193,6 +2284,11 @@ struct ne_gpu_ctx {
     hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     uint32_t graph_hint_p = 0, graph_hint_c = 0;
     int order_parity = 0;
+    // distance joints
+    uint32_t njoints = 0, nspring = 0, nrigid = 0, ncomp = 0;
+    uint32_t *j_ab = nullptr, *j_comp = nullptr;   // a[n], b[n]; comp_start[ncomp+1], comp_list[n]
+    double* j_f = nullptr;                         // la0..2, lb0..2, rest, stiff, damp (9 x n)
+    std::vector<uint8_t> h_mut;                    // bodies that can move (im != 0 or ii != 0)
     // rollout checkpoints and adjoint work space
     struct Saved {
         double* state = nullptr;   // x, v, q, w fields (13 x ncap)
@@ -2454,6 +2550,8 @@ void free_work(ne_gpu_ctx* c) {
     dfree(c->isl_count); dfree(c->isl_fill); dfree(c->isl_list); dfree(c->big_list);
     dfree(c->rs_hist); dfree(c->scan_part);
     dfree(c->d_wid); dfree(c->d_worlds);
+    dfree(c->j_ab); dfree(c->j_comp); dfree(c->j_f);
+    c->njoints = c->nspring = c->nrigid = c->ncomp = 0;
     dfree(c->epa_big.list); dfree(c->epa_big.faces); dfree(c->epa_big.hor); dfree(c->epa_big.next);
     dfree(c->epa_big.live); dfree(c->epa_big.tab);
     c->epa_big.cap = 0;
@@ -2502,6 +2600,19 @@ void launch_integrate(ne_gpu_ctx* c, const SimParams& P, bool bin) {
 // One reference step (src/world.cpp:133-146) as a kernel pipeline.  When
 // `wrap_first`, the previous step's periodic wrap runs at the top of this
 // step's integrate kernel; the caller wraps after the last step.
+JointDev joint_dev(const ne_gpu_ctx* c) {
+    const uint32_t n = c->njoints;
+    const double* f = c->j_f;
+    JointDev J;
+    J.a = c->j_ab; J.b = c->j_ab + n;
+    J.la0 = f; J.la1 = f + n; J.la2 = f + 2 * (size_t)n;
+    J.lb0 = f + 3 * (size_t)n; J.lb1 = f + 4 * (size_t)n; J.lb2 = f + 5 * (size_t)n;
+    J.rest = f + 6 * (size_t)n; J.stiff = f + 7 * (size_t)n; J.damp = f + 8 * (size_t)n;
+    J.comp_start = c->j_comp; J.comp_list = c->j_comp + c->ncomp + 1;
+    J.ncomp = c->ncomp;
+    return J;
+}
+
 // The step up to the solve: integrate, grid, candidate rows, narrow phase,
 // constraint build and islands.
 void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
@@ -2510,10 +2621,12 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
     BodiesDev B = c->B;
     const uint32_t n = c->n;
     const bool static_grid = P.box.periodic && c->ngrid > 0;
+    if (c->nspring)   // spring joints add forces before integration (world.cpp:134)
+        k_joint_springs<<<blocks_for(c->ncomp, 64), 64, 0, s>>>(B, P, joint_dev(c));
     {
         KTimer kt(c, NE_KERNEL_INTEGRATE);
-        const bool w = wrap_first && P.box.periodic;
-        if (c->has_forces) {
+        const bool w = wrap_first && P.box.periodic && c->njoints == 0;
+        if (c->has_forces || c->nspring) {
             if (w) launch_integrate<true, true>(c, P, static_grid);
             else launch_integrate<true, false>(c, P, static_grid);
             c->has_forces = false;
@@ -2630,11 +2743,18 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     KTimer whole(c, NE_KERNEL_STEP);
     enqueue_front(c, wrap_first);
     enqueue_solve(c);
-    // --- boundary (the periodic wrap is deferred to the next integrate)
     const SimParams P = make_sim_params(c->params);
+    if (c->nrigid)   // rigid joints after the contacts (world.cpp:139)
+        k_joint_rigid<<<blocks_for(c->ncomp, 64), 64, 0, c->stream>>>(c->B, P, joint_dev(c));
+    // --- boundary.  The periodic wrap is deferred to the next integrate,
+    // except with joints: spring forces at the next step's start read the
+    // wrapped positions.
     if (!P.box.periodic && !P.open_boundary) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_walls<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P);
+    } else if (P.box.periodic && c->njoints) {
+        KTimer kt(c, NE_KERNEL_BOUNDARY);
+        k_wrap<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P);
     }
     return NE_OK;
 }
@@ -2890,7 +3010,8 @@ int ne_gpu_rollout(ne_gpu_ctx* c, const double* v0, const double* mu, const doub
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
     if (c->n == 0) return set_err(c, NE_ERR_INVALID, "no bodies");
     if (!c->params.periodic) return set_err(c, NE_ERR_UNSUPPORTED, "rollouts are differentiated for periodic worlds");
-    if (c->has_forces) return set_err(c, NE_ERR_UNSUPPORTED, "rollouts do not take applied forces");
+    if (c->has_forces || c->njoints)
+        return set_err(c, NE_ERR_UNSUPPORTED, "rollouts do not take applied forces or joints");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const uint32_t nw = c->nworlds ? c->nworlds : 1u;
     if (mu || e) {
@@ -3156,6 +3277,71 @@ int ne_gpu_set_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     return upload_bodies(c, b);
 }
 
+int ne_gpu_set_joints(ne_gpu_ctx* c, uint32_t n, const ne_distance_joint* j) {
+    if (!c || (n && !j)) return NE_ERR_INVALID;
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    for (uint32_t k = 0; k < n; ++k)
+        if (j[k].a >= c->n || j[k].b >= c->n) return set_err(c, NE_ERR_INVALID, "joint body index out of range");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    c->njoints = c->nspring = c->nrigid = c->ncomp = 0;
+    if (n == 0) return NE_OK;
+    // components: union-find over the bodies that can move
+    std::vector<uint32_t> parent(c->n);
+    for (uint32_t i = 0; i < c->n; ++i) parent[i] = i;
+    auto find = [&](uint32_t x) { while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; } return x; };
+    for (uint32_t k = 0; k < n; ++k)
+        if (c->h_mut[j[k].a] && c->h_mut[j[k].b]) {
+            const uint32_t ra = find(j[k].a), rb = find(j[k].b);
+            if (ra != rb) parent[std::max(ra, rb)] = std::min(ra, rb);
+        }
+    std::vector<uint32_t> root(n), comp_of_root(c->n, 0xFFFFFFFFu), count;
+    uint32_t nspring = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t a = j[k].a, b = j[k].b;
+        root[k] = c->h_mut[a] ? find(a) : (c->h_mut[b] ? find(b) : c->n + k);
+        nspring += j[k].stiffness > 0.0;
+    }
+    std::vector<uint32_t> comp(n);
+    uint32_t ncomp = 0;   // a joint with no movable body gets a component of its own
+    for (uint32_t k = 0; k < n; ++k) {
+        if (root[k] >= c->n) { comp[k] = ncomp++; count.push_back(1); continue; }
+        uint32_t& id = comp_of_root[root[k]];
+        if (id == 0xFFFFFFFFu) { id = ncomp++; count.push_back(0); }
+        comp[k] = id;
+        ++count[id];
+    }
+    std::vector<uint32_t> start(ncomp + 1, 0), list(n), fill(ncomp, 0);
+    for (uint32_t q = 0; q < ncomp; ++q) start[q + 1] = start[q] + count[q];
+    for (uint32_t k = 0; k < n; ++k) list[start[comp[k]] + fill[comp[k]]++] = k;   // ascending joint order
+    std::vector<uint32_t> ab(2 * (size_t)n);
+    std::vector<double> f(9 * (size_t)n);
+    for (uint32_t k = 0; k < n; ++k) {
+        ab[k] = j[k].a;
+        ab[n + k] = j[k].b;
+        for (int a = 0; a < 3; ++a) {
+            f[(size_t)a * n + k] = j[k].local_a[a];
+            f[(size_t)(3 + a) * n + k] = j[k].local_b[a];
+        }
+        f[6 * (size_t)n + k] = j[k].rest;
+        f[7 * (size_t)n + k] = j[k].stiffness;
+        f[8 * (size_t)n + k] = j[k].damping;
+    }
+    std::vector<uint32_t> cl(start.begin(), start.end());
+    cl.insert(cl.end(), list.begin(), list.end());
+    HIP_TRY(c, dalloc(c->j_ab, ab.size()));
+    HIP_TRY(c, dalloc(c->j_f, f.size()));
+    HIP_TRY(c, dalloc(c->j_comp, cl.size()));
+    HIP_TRY(c, hipMemcpy(c->j_ab, ab.data(), sizeof(uint32_t) * ab.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->j_f, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->j_comp, cl.data(), sizeof(uint32_t) * cl.size(), hipMemcpyHostToDevice));
+    c->njoints = n;
+    c->nspring = nspring;
+    c->nrigid = n - nspring;
+    c->ncomp = ncomp;
+    return NE_OK;
+}
+
 int ne_gpu_set_state(ne_gpu_ctx* c, const double* x, const double* v, const double* q, const double* w) {
     if (!c) return NE_ERR_INVALID;
     const uint32_t n = c->n;
@@ -3210,7 +3396,7 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps) {
         int r = run_step(c, t > 0);
         if (r != NE_OK) return r;
     }
-    if (c->params.periodic && nsteps > 0) {
+    if (c->params.periodic && nsteps > 0 && c->njoints == 0) {   // with joints every step wrapped itself
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_wrap<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, make_sim_params(c->params));
     }
@@ -3768,6 +3954,11 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     c->h_isexcl.assign(n, 0);
     for (uint32_t e : excl) c->h_isexcl[e] = 1;
     c->nworlds = 0;   // a new body set starts as one world
+    c->njoints = c->nspring = c->nrigid = c->ncomp = 0;   // and without joints
+    c->h_mut.assign(n, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        c->h_mut[i] = b->inv_mass[i] != 0.0 || b->inv_inertia[3 * i] != 0.0 || b->inv_inertia[3 * i + 1] != 0.0 ||
+                      b->inv_inertia[3 * i + 2] != 0.0;
     c->grid_dirty = true;
     drop_graphs(c);
     c->has_forces = forces;

@@ -26,7 +26,7 @@ EXPORTS = [
     "ne_gpu_get_pairs", "ne_gpu_diagnostics", "ne_gpu_get_counters", "ne_gpu_get_stream", "ne_gpu_last_error",
     "ne_gpu_fp_probe", "ne_gpu_time_kernel", "ne_gpu_kernel_time", "ne_gpu_export_warm_cache",
     "ne_gpu_import_warm_cache", "ne_gpu_set_worlds", "ne_gpu_use_graph", "ne_gpu_autotune", "ne_gpu_epa_stats",
-    "ne_gpu_rollout", "ne_gpu_rollout_backward",
+    "ne_gpu_rollout", "ne_gpu_rollout_backward", "ne_gpu_set_joints",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
@@ -75,6 +75,7 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_set_worlds.argtypes = [vp, C.c_uint32, u32p, dp, dp, dp]
     lib.ne_gpu_use_graph.argtypes = [vp, C.c_int32]
     lib.ne_gpu_epa_stats.argtypes = [vp, u32p, u32p]
+    lib.ne_gpu_set_joints.argtypes = [vp, C.c_uint32, vp]
     lib.ne_gpu_rollout.argtypes = [vp, vp, dp, dp, C.c_int32, C.c_int32, vp]
     lib.ne_gpu_rollout_backward.argtypes = [vp, vp, vp, dp, dp]
     lib.ne_gpu_autotune.argtypes = [vp, C.c_int32, C.POINTER(_abi.TuneResult)]
@@ -128,6 +129,13 @@ class GpuWorld:
         self._check(self.lib.ne_gpu_set_bodies(self.h, C.byref(b)))
         self.n = s.n
         del keep
+        if len(s.joints):
+            self.set_joints(s.joints)
+
+    def set_joints(self, joints):
+        """World::distanceJoints: a JOINT_DTYPE record array, in order."""
+        j = np.ascontiguousarray(joints, _abi.JOINT_DTYPE)
+        self._check(self.lib.ne_gpu_set_joints(self.h, len(j), j.ctypes.data if len(j) else None))
 
     def set_worlds(self, offsets, half, restitution=None, friction=None):
         """Batched worlds: bodies [offsets[w], offsets[w+1]) form periodic world w

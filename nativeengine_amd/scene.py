@@ -97,6 +97,7 @@ class Scene:
         self.hull_verts = np.zeros((0, 3), np.float64)
         self.nocollide_offsets = np.zeros(n + 1, np.uint32)
         self.nocollide = np.zeros(0, np.uint32)
+        self.joints = np.zeros(0, _abi.JOINT_DTYPE)   # World::distanceJoints, in order
         self.params = Params()
 
     # -- construction -----------------------------------------------------
@@ -431,6 +432,14 @@ def concat_scenes(scenes) -> Scene:
     out.hull_offsets = np.asarray(ho, np.uint32)
     out.nocollide = np.concatenate(nc) if nc else np.zeros(0, np.uint32)
     out.nocollide_offsets = np.asarray(no, np.uint32)
+    js, base = [], 0
+    for s in scenes:
+        j = s.joints.copy()
+        j["a"] += np.uint32(base)
+        j["b"] += np.uint32(base)
+        js.append(j)
+        base += s.n
+    out.joints = np.concatenate(js) if js else np.zeros(0, _abi.JOINT_DTYPE)
     out.params = Params(**scenes[0].params.__dict__) if scenes else Params()
     return out
 
@@ -448,6 +457,10 @@ def slice_scene(s: Scene, begin: int, end: int) -> Scene:
     c0, c1 = int(s.nocollide_offsets[begin]), int(s.nocollide_offsets[end])
     out.nocollide = (s.nocollide[c0:c1] - begin).astype(np.uint32)
     out.nocollide_offsets = (s.nocollide_offsets[begin:end + 1] - s.nocollide_offsets[begin]).astype(np.uint32)
+    sel = (s.joints["a"] >= begin) & (s.joints["a"] < end) & (s.joints["b"] >= begin) & (s.joints["b"] < end)
+    out.joints = s.joints[sel].copy()
+    out.joints["a"] -= np.uint32(begin)
+    out.joints["b"] -= np.uint32(begin)
     out.params = Params(**s.params.__dict__)
     return out
 
@@ -627,3 +640,45 @@ def adjoint_worlds(nworlds: int = 256, n_spheres: int = 256, n_boxes: int = 256,
     off = np.concatenate([[0], np.cumsum([s.n for s in scenes])]).astype(np.uint32)
     return Batch(concat_scenes(scenes), off, np.full((nworlds, 3), h), np.full(nworlds, restitution),
                  np.full(nworlds, friction))
+
+
+def rope_scene(nropes: int = 8, links: int = 16, radius: float = 0.2, half=(6.0, 6.0, 6.0), seed: int = 0,
+               periodic: bool = True, anchored: bool = True, springs: int = 0, spacing: float = 1.5) -> Scene:
+    """Ropes of `links` spheres joined surface to surface by rigid distance
+    joints (rest = the gap between surfaces, so the joints act at off-centre
+    anchors and turn the links), each optionally hung from a static anchor
+    sphere; `springs` extra spring-damper joints tie random pairs of links.
+    A few loose boxes fall among the ropes.  Gravity -9.81, friction 0.4,
+    restitution 0.1, dt 1/60."""
+    rng = np.random.default_rng(seed)
+    bodies, joints = [], []
+    top = half[1] - 1.0
+    for r in range(nropes):
+        x0 = -half[0] + 1.0 + (r % 4) * spacing + rng.uniform(-0.1, 0.1)
+        z0 = -half[2] + 1.0 + (r // 4) * spacing + rng.uniform(-0.1, 0.1)
+        prev = None
+        if anchored:
+            bodies.append(make_sphere(len(bodies), (x0, top, z0), 0.1, 1.0).make_static())
+            prev = len(bodies) - 1
+        for k in range(links):
+            # links step sideways as well as down so the rope starts bent and swings
+            x = (x0 + 0.6 * (k + 1) * 2 * radius, top - (k + 1) * 2.2 * radius, z0)
+            bodies.append(make_sphere(len(bodies), x, radius, 1.0))
+            cur = len(bodies) - 1
+            if prev is not None:
+                ra = 0.1 if (anchored and k == 0) else radius
+                joints.append((prev, cur, (0.0, -ra, 0.0), (0.0, radius, 0.0), 0.2 * radius, 0.0, 0.0))
+            prev = cur
+    for b in range(max(2, nropes // 2)):
+        q = tuple(random_unit_quaternions(rng, 1)[0])
+        bodies.append(make_box(len(bodies), (rng.uniform(-2, 2), top - 0.5, rng.uniform(-2, 2)), q, (0.3, 0.2, 0.25), 1.0))
+    dyn = [i for i, bd in enumerate(bodies) if bd.inv_mass > 0 and bd.shape == _abi.SHAPE_SPHERE]
+    for _ in range(springs):
+        a, b = rng.choice(dyn, size=2, replace=False)
+        joints.append((int(min(a, b)), int(max(a, b)), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), float(rng.uniform(0.5, 2.0)),
+                       float(rng.uniform(20.0, 200.0)), float(rng.uniform(0.5, 5.0))))
+    s = Scene.from_bodies(bodies)
+    s.joints = np.array([(a, b, la, lb, rest, k, d) for a, b, la, lb, rest, k, d in joints], _abi.JOINT_DTYPE)
+    s.params = Params(half=tuple(half), periodic=periodic, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0),
+                      restitution=0.1, friction=0.4, solver_iterations=8)
+    return s

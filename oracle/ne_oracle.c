@@ -135,6 +135,9 @@ struct ne_oracle {
     int tape_mode;
     uint32_t tape_steps, tape_cap, tape_cursor;
     struct otape_step* tape;
+    /* distance joints, in order (world.hpp:129) */
+    uint32_t njoints;
+    ne_distance_joint* joints;
 };
 
 struct otape_pair {
@@ -1099,9 +1102,64 @@ static void wrap_positions(ne_oracle* o) {                                      
         if (o->B[i].invMass > 0.0) o->B[i].x = wrap_pos(&o->box, o->B[i].x);
 }
 
+/* Spring-damper joints add force and torque before integration
+ * (world.cpp:150-170). */
+static void apply_joint_forces(ne_oracle* o) {
+    for (uint32_t k = 0; k < o->njoints; ++k) {
+        const ne_distance_joint* j = &o->joints[k];
+        if (j->stiffness <= 0.0) continue;
+        obody* a = &o->B[j->a];
+        obody* b = &o->B[j->b];
+        const v3 wa = add(a->x, qrot(a->q, ld3(j->local_a, 0)));
+        const v3 wb = add(b->x, qrot(b->q, ld3(j->local_b, 0)));
+        const v3 d = min_image(&o->box, sub(wa, wb));
+        const double dist = norm(d);
+        if (dist < 1e-9) continue;
+        const v3 nrm = dvs(d, dist);
+        const v3 rA = sub(wa, a->x), rB = min_image(&o->box, sub(wb, b->x));
+        const v3 vrel = contact_vel(a, b, rA, rB);
+        const double vn = dot(vrel, nrm);
+        const double f = -j->stiffness * (dist - j->rest) - j->damping * vn;
+        const v3 F = scl(nrm, f);
+        a->force = add(a->force, F);
+        a->torque = add(a->torque, cross(rA, F));
+        b->force = sub(b->force, F);
+        b->torque = sub(b->torque, cross(rB, F));
+    }
+}
+
+/* Rigid joints: bilateral velocity constraints with a Baumgarte length term,
+ * 8 sequential sweeps after the contact solve (world.cpp:173-197). */
+static void solve_rigid_joints(ne_oracle* o, int iterations) {
+    for (int it = 0; it < iterations; ++it) {
+        for (uint32_t k = 0; k < o->njoints; ++k) {
+            const ne_distance_joint* j = &o->joints[k];
+            if (j->stiffness > 0.0) continue;
+            obody* a = &o->B[j->a];
+            obody* b = &o->B[j->b];
+            const v3 wa = add(a->x, qrot(a->q, ld3(j->local_a, 0)));
+            const v3 wb = add(b->x, qrot(b->q, ld3(j->local_b, 0)));
+            const v3 d = min_image(&o->box, sub(wa, wb));
+            const double dist = norm(d);
+            if (dist < 1e-9) continue;
+            const v3 nrm = dvs(d, dist);
+            const v3 rA = sub(wa, a->x), rB = min_image(&o->box, sub(wb, b->x));
+            const double K = eff_mass(a, b, rA, rB, nrm);
+            if (K <= 1e-18) continue;
+            const v3 vrel = contact_vel(a, b, rA, rB);
+            const double vn = dot(vrel, nrm);
+            const double bias = 0.2 * (dist - j->rest) / o->P.dt;
+            const double dj = -(vn + bias) / K;
+            apply_impulse(a, b, rA, rB, scl(nrm, dj));
+        }
+    }
+}
+
 static void world_step(ne_oracle* o) {                                               /* world.cpp:133-146 */
+    apply_joint_forces(o);
     integrate(o);
     collide(o);
+    solve_rigid_joints(o, 8);
     if (o->box.periodic) wrap_positions(o);
     else if (!o->P.open_boundary) apply_walls(o);
 }
@@ -1134,6 +1192,7 @@ int ne_oracle_destroy(ne_oracle* o) {
     free(o->B); free(o->hull); free(o->nocol); free(o->cons); free(o->cache_key); free(o->cache_imp);
     for (uint32_t t = 0; t < o->tape_steps; ++t) free(o->tape[t].p);
     free(o->tape);
+    free(o->joints);
     free(o);
     return NE_OK;
 }
@@ -1229,6 +1288,17 @@ int ne_oracle_tape(ne_oracle* o, int32_t mode) {
     }
     o->tape_cursor = 0;
     o->tape_mode = mode;
+    return NE_OK;
+}
+
+int ne_oracle_set_joints(ne_oracle* o, uint32_t n, const ne_distance_joint* j) {
+    if (!o || (n && !j)) return NE_ERR_INVALID;
+    for (uint32_t k = 0; k < n; ++k)
+        if (j[k].a >= o->n || j[k].b >= o->n) return NE_ERR_INVALID;
+    free(o->joints);
+    o->joints = (ne_distance_joint*)malloc(sizeof(ne_distance_joint) * (n ? n : 1));
+    if (n) memcpy(o->joints, j, sizeof(ne_distance_joint) * n);
+    o->njoints = n;
     return NE_OK;
 }
 

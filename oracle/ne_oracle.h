@@ -47,6 +47,7 @@ int ne_oracle_wrap(ne_oracle *o, const double *p, double *out);
 /* Largest EPA face list / horizon seen in this process (workspace sizing). */
 void ne_oracle_epa_stats(int32_t *max_faces, int32_t *max_horizon);
 int ne_oracle_tape(ne_oracle *o, int32_t mode);
+int ne_oracle_set_joints(ne_oracle *o, uint32_t n, const ne_distance_joint *joints);
 void ne_oracle_epa_stats_reset(int32_t faces, int32_t horizon);
 void ne_oracle_epa_counts(int64_t *calls, int64_t *big);
 

@@ -7,6 +7,7 @@
 //
 // It drives the reference through its public API only:
 //   - `ne::World` public fields and `step()`            (src/world.hpp:73-161)
+//   - `World::distanceJoints`                           (src/world.hpp:20-26, 129)
 //   - `captureContacts` / `debugContacts` for the per-step constraint list
 //                                                       (src/world.hpp:135-136,
 //                                                        src/world.cpp:552)
@@ -132,6 +133,24 @@ int ne_ref_set_bodies(void* h, const ne_bodies* b) {
     RefWorld* r = static_cast<RefWorld*>(h);
     if (!r || !b) return NE_ERR_INVALID;
     return build_bodies(b, r->w.bodies);
+}
+
+int ne_ref_set_joints(void* h, uint32_t n, const ne_distance_joint* j) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || (n && !j)) return NE_ERR_INVALID;
+    r->w.distanceJoints.clear();
+    for (uint32_t k = 0; k < n; ++k) {
+        ne::DistanceJoint d;
+        d.a = j[k].a;
+        d.b = j[k].b;
+        d.localA = v3(j[k].local_a, 0);
+        d.localB = v3(j[k].local_b, 0);
+        d.rest = j[k].rest;
+        d.stiffness = j[k].stiffness;
+        d.damping = j[k].damping;
+        r->w.distanceJoints.push_back(d);
+    }
+    return NE_OK;
 }
 
 int ne_ref_set_capture(void* h, int on) {

@@ -4,6 +4,7 @@
 //
 //   world_drop_in checks           restated selftest scenarios, exit code = failures
 //   world_drop_in scene IN OUT K   steps a flat-array scene K times, writes x v q w
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -141,6 +142,40 @@ static void stacking() {   // selftest.cpp:663-697 on the World directly
     check(w.bodies[2].v.norm() < 0.3, "stacking: stack is at rest");
 }
 
+static void joints() {   // selftest.cpp:779-823
+    {   // rigid rod of length 3 from a static anchor; the bob starts level and swings
+        World w;
+        w.box.half = 1e6; w.box.periodic = false; w.dt = 1.0 / 240.0; w.gravity = V3{0, -10, 0};
+        Body anchor = makeSphere(0, V3{0, 5, 0}, 0.1, 1.0);
+        anchor.invMass = 0.0; anchor.invInertiaBody = V3{0, 0, 0}; anchor.dynamic = false;
+        w.bodies = {anchor, makeSphere(1, V3{3, 5, 0}, 1.0, 1.0)};
+        DistanceJoint j; j.a = 0; j.b = 1; j.rest = 3.0;
+        w.distanceJoints = {j};
+        double maxErr = 0.0, minY = 5.0;
+        for (int s = 0; s < 2000; ++s) {
+            w.step();
+            maxErr = std::max(maxErr, std::fabs((w.bodies[1].x - w.bodies[0].x).norm() - 3.0));
+            minY = std::min(minY, w.bodies[1].x.y);
+        }
+        check(maxErr < 0.15, "joint: rigid rod keeps pendulum length ~constant");
+        check(minY < 3.0, "joint: pendulum actually swings down under gravity");
+    }
+    {   // spring k = 200, c = 20, rest 2: settles where k (dist - rest) = m g
+        World w;
+        w.box.half = 1e6; w.box.periodic = false; w.dt = 1.0 / 240.0; w.gravity = V3{0, -10, 0};
+        Body anchor = makeSphere(0, V3{0, 5, 0}, 0.1, 1.0);
+        anchor.invMass = 0.0; anchor.invInertiaBody = V3{0, 0, 0}; anchor.dynamic = false;
+        w.bodies = {anchor, makeSphere(1, V3{0, 3, 0}, 1.0, 1.0)};
+        DistanceJoint j; j.a = 0; j.b = 1; j.rest = 2.0; j.stiffness = 200.0; j.damping = 20.0;
+        w.distanceJoints = {j};
+        w.stepMany(6000);
+        const double mass = 4.18879;   // density 1, r = 1
+        const double len = (w.bodies[1].x - w.bodies[0].x).norm();
+        check(std::fabs((len - 2.0) - mass * 10.0 / 200.0) < 0.05, "joint: spring settles at the gravity-balanced stretch");
+        check(w.bodies[1].v.norm() < 0.1, "joint: damped spring comes to rest");
+    }
+}
+
 // scene IN OUT K: IN holds n then per body 36 doubles (x3 v3 q4 w3 invMass
 // invI3 shape radius halfHeight he3 friction restitution layer mask flags
 // pad...), followed by the world parameters; OUT receives x v q w.
@@ -192,6 +227,7 @@ int main(int argc, char** argv) {
         determinism_and_copies();
         cross_boundary();
         stacking();
+        joints();
         std::printf("failures: %d\n", g_fail);
         return g_fail == 0 ? 0 : 1;
     } catch (const std::exception& e) {

@@ -39,6 +39,7 @@ def _bind(lib, prefix):
     g("diagnostics").argtypes = [_vp, _dp, _dp, _dp]
     g("min_image").argtypes = [_vp, _dp, _dp]
     g("wrap").argtypes = [_vp, _dp, _dp]
+    g("set_joints").argtypes = [_vp, C.c_uint32, _vp]
     return lib
 
 
@@ -110,6 +111,10 @@ class _CpuWorld:
         self._check(self._f("set_bodies")(self.h, C.byref(b)), "set_bodies")
         self.n = s.n
         del keep
+        j = np.ascontiguousarray(s.joints, _abi.JOINT_DTYPE)
+        if len(j) or getattr(self, "_had_joints", False):
+            self._check(self._f("set_joints")(self.h, len(j), j.ctypes.data if len(j) else None), "set_joints")
+            self._had_joints = len(j) > 0
 
     def step(self, nsteps=1):
         self._check(self._f("step")(self.h, int(nsteps)), "step")

@@ -157,3 +157,23 @@ def test_determinism_bitwise(gpu_available):
         out.append(g.state())
     for u, v in zip(*out):
         assert np.array_equal(u.view(np.uint64), v.view(np.uint64))
+
+
+@pytest.mark.parametrize("periodic,anchored,springs,graph", [(True, True, 0, False), (False, True, 6, False),
+                                                            (True, False, 4, True)])
+def test_rope_joints_parity(gpu_available, oracle_lib, periodic, anchored, springs, graph):
+    """Rigid rope links (world.cpp:173-197) and spring-damper joints
+    (150-170) among colliding bodies: GPU = oracle bit for bit, multi-step
+    calls and graph replay included."""
+    s = scene.rope_scene(nropes=12, links=16, periodic=periodic, anchored=anchored, springs=springs, seed=5)
+    _say(f"ropes: {s.n} bodies, {len(s.joints)} joints, periodic={periodic} springs={springs} graph={graph}")
+    g = _gpu()
+    g.set_scene(s)
+    g.use_graph(graph)
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    for chunk in (1, 7, 30, 22):
+        g.step(chunk)
+        o.step(chunk)
+        _assert_same_state(g.state(), o.state(), f"ropes after +{chunk}")
+    _assert_same_contacts(g.contacts(), o.contacts(), "ropes last step")

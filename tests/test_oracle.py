@@ -334,3 +334,58 @@ def test_contact_tape_replay_reproduces_the_recorded_run(oracle_lib):
     o.step(20)
     x = o.state()[0]
     assert 0.0 < np.max(np.abs(x - want[0])) < 1e-4
+
+
+def _live_compare(s, steps):
+    o, r = engines.OracleWorld(), engines.RefWorld()
+    o.set_scene(s)
+    r.set_scene(s)
+    for t in range(steps):
+        o.step(1)
+        r.step(1)
+        for name, u, v in zip("xvqw", o.state(), r.state()):
+            assert np.array_equal(u, v), f"step {t}: {name} differs"
+
+
+@pytest.mark.parametrize("periodic,anchored,springs", [(True, True, 0), (False, True, 6), (True, False, 4)])
+def test_oracle_joints_match_reference_live(periodic, anchored, springs):
+    """Rigid rope links and spring-damper joints (src/world.cpp:150-197)
+    against the unmodified reference, bitwise."""
+    if not engines.ref_available():
+        pytest.skip("reference not built")
+    s = S.rope_scene(nropes=6, links=10, periodic=periodic, anchored=anchored, springs=springs, seed=3)
+    _live_compare(s, 90)
+
+
+def _pendulum(rigid):
+    """src/selftest.cpp:779-823 restated: a bob hung from a static anchor by a
+    rigid rod of length 3 (started level), or by a spring (rest 2, k 200, c 20)."""
+    anchor = S.make_sphere(0, (0.0, 5.0, 0.0), 0.1, 1.0).make_static()
+    bob = S.make_sphere(1, (3.0, 5.0, 0.0) if rigid else (0.0, 3.0, 0.0), 1.0, 1.0)
+    s = S.Scene.from_bodies([anchor, bob])
+    s.joints = np.array([(0, 1, (0, 0, 0), (0, 0, 0), 3.0 if rigid else 2.0, 0.0 if rigid else 200.0,
+                          0.0 if rigid else 20.0)], _abi.JOINT_DTYPE)
+    s.params = S.Params(half=(1e6, 1e6, 1e6), periodic=False, dt=1.0 / 240.0, gravity=(0.0, -10.0, 0.0))
+    return s
+
+
+def test_oracle_rigid_pendulum_keeps_its_length():
+    o = engines.OracleWorld()
+    o.set_scene(_pendulum(True))
+    max_err, min_y = 0.0, 5.0
+    for _ in range(2000):
+        o.step(1)
+        x = o.state()[0]
+        max_err = max(max_err, abs(np.linalg.norm(x[1] - x[0]) - 3.0))
+        min_y = min(min_y, x[1, 1])
+    assert max_err < 0.15 and min_y < 3.0
+
+
+def test_oracle_spring_settles_at_gravity_balanced_stretch():
+    o = engines.OracleWorld()
+    o.set_scene(_pendulum(False))
+    o.step(6000)
+    x, v = o.state()[:2]
+    mass = (4.0 / 3.0) * math.pi          # density 1, r = 1
+    assert abs((np.linalg.norm(x[1] - x[0]) - 2.0) - mass * 10.0 / 200.0) < 0.05
+    assert np.linalg.norm(v[1]) < 0.1
