@@ -54,6 +54,8 @@ struct BodiesDev {
     uint32_t *layer, *mask, *flags;
     uint32_t *hull_off, *nc_off, *nc_idx;
     double* hull;
+    double *st, *ims, *iis0, *iis1, *iis2;   // sleep timer, stored inverse mass / inertia (world.cpp:317-371)
+    int sleep_on;                            // island sleeping enabled: candidates keep sleeping bodies
     uint32_t n;
     // Batched worlds: world id per body and the world table (both null for
     // one world).  Bodies of a world are contiguous.
@@ -70,6 +72,8 @@ struct SimParams {
     int warm;
     int iters;
     int open_boundary;
+    int sleep;
+    double sleep_lin, sleep_ang, sleep_time, wake_vel;
 };
 
 // Grid description, computed on the device every step.
@@ -179,7 +183,7 @@ __global__ void __launch_bounds__(TPB) k_integrate(BodiesDev B, SimParams P, Sta
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.n) return;
     parent[i] = i;      // this step's union-find forest starts as singletons
-    claimed[i] = 0u;
+    claimed[i] = 0u;    // (also: no island claim, and with sleeping no wake yet -- see k_sleep_wake_mark)
     const double dt = P.dt;
     const uint32_t fl = B.flags[i];
     const double im = B.im[i];
@@ -403,9 +407,17 @@ __global__ void __launch_bounds__(TPB) k_cell_fill(const uint32_t* __restrict__ 
     sorted[slot] = p;
 }
 
-// Pair filters of src/world.cpp:513-516 (world.hpp:94-103).
+// Pair filters of src/world.cpp:513-516 (world.hpp:94-103).  With island
+// sleeping, a sleeping body counts with its stored mass here: it may be woken
+// earlier in the same step's pair order, after which the reference keeps its
+// pairs (the exact decision is taken after the narrow phase, k_sleep_fate).
 __device__ __forceinline__ bool pair_passes(const BodiesDev& B, uint32_t i, uint32_t j) {
-    if (B.im[i] + B.im[j] <= 0.0) return false;
+    double mi = B.im[i], mj = B.im[j];
+    if (B.sleep_on) {
+        if (B.flags[i] & NE_FLAG_SLEEPING) mi = fmax(mi, B.ims[i]);
+        if (B.flags[j] & NE_FLAG_SLEEPING) mj = fmax(mj, B.ims[j]);
+    }
+    if (mi + mj <= 0.0) return false;
     const uint32_t li = B.layer[i], lj = B.layer[j];
     if (!((B.mask[i] & (1u << (lj & 31u))) && (B.mask[j] & (1u << (li & 31u))))) return false;
     for (uint32_t k = B.nc_off[i]; k < B.nc_off[i + 1]; ++k)
@@ -1804,6 +1816,175 @@ __global__ void k_fp_probe(int op, uint32_t n, const double* a, const double* b,
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Island sleeping (src/world.cpp:317-371, wake in collide 527-531).
+//
+// The reference walks the pairs in (i, j) order; a sleeping body is woken by
+// the first pair with contacts whose other body is awake at that point,
+// dynamic or kinematic, and faster than wakeVel (after the pair's mass filter).
+// Each sleeper's waking pair is the smallest such index, found by rounds of
+// atomic minima: every round only lowers an estimate that stays at or above
+// the true index, and a round that finds nothing new is the fixed point.  A
+// woken body keeps the velocity it slept with -- zero when the step put it to
+// sleep -- so a second round is needed only when a body set asleep with
+// velocity above wakeVel is woken, and it may then wake others further on.
+// Each pair then sees its bodies as the reference does at its position --
+// asleep before the body's waking pair, awake from it on -- which decides
+// whether the pair is skipped and whether it joins the sleep islands.
+// ---------------------------------------------------------------------------
+
+constexpr int WAKE_ROUNDS = 8;   // rounds after the first; a longer wake chain is reported
+
+__device__ __forceinline__ bool sleeping_now(const BodiesDev& B, uint32_t i) { return B.flags[i] & NE_FLAG_SLEEPING; }
+
+// Inverse mass of a body at a point of the pair walk: its current value while
+// asleep, the stored one once woken during this step.
+__device__ __forceinline__ double walk_mass(const BodiesDev& B, uint32_t k, bool slept, bool asleep) {
+    return (slept && !asleep) ? B.ims[k] : B.im[k];
+}
+
+__device__ __forceinline__ bool wakes_others(const BodiesDev& B, const SimParams& P, uint32_t k) {
+    return (B.flags[k] & (NE_FLAG_DYNAMIC | NE_FLAG_KINEMATIC)) && norm(ld3(B.v0, B.v1, B.v2, k)) > P.wake_vel;
+}
+
+__global__ void __launch_bounds__(TPB) k_sleep_reset(BodiesDev B, uint32_t* __restrict__ wake_at, uint32_t* __restrict__ again) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < B.n) wake_at[i] = 0xFFFFFFFFu;
+    if (i <= (uint32_t)WAKE_ROUNDS) again[i] = 0u;
+}
+
+__global__ void __launch_bounds__(TPB) k_sleep_wake_mark(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters,
+                                                         const uint32_t* __restrict__ cand_i,
+                                                         const uint32_t* __restrict__ cand_j,
+                                                         const uint32_t* __restrict__ ncont, uint32_t* wake_at,
+                                                         uint32_t* again, int round) {
+    if (round > 0 && !again[round - 1]) return;
+    const uint32_t np = counters[CNT_P];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        if (!ncont[p]) continue;
+        const uint32_t i = cand_i[p], j = cand_j[p];
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t sl = side ? j : i, o = side ? i : j;   // sleeper, other
+            if (!sleeping_now(B, sl) || wake_at[sl] <= p) continue;
+            const bool o_slept = sleeping_now(B, o);
+            const bool o_asleep = o_slept && !(wake_at[o] < p);
+            if (o_asleep || !wakes_others(B, P, o)) continue;
+            if (B.im[sl] + walk_mass(B, o, o_slept, false) <= 0.0) continue;   // the pair's mass filter
+            const uint32_t old = atomicMin(&wake_at[sl], p);
+            if (old > p && wakes_others(B, P, sl)) again[round] = 1u;
+        }
+    }
+}
+
+// Decides each pair as the reference's walk does at its position; drops the
+// skipped ones (ncont = 0) and flags dynamic-dynamic pairs for the islands.
+__global__ void __launch_bounds__(TPB) k_sleep_fate(BodiesDev B, uint32_t* __restrict__ counters,
+                                                    const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                    uint32_t* __restrict__ ncont, const uint32_t* __restrict__ wake_at,
+                                                    const uint32_t* __restrict__ again, uint8_t* __restrict__ isp) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && again[WAKE_ROUNDS]) atomicOr(&counters[CNT_FAULT], NE_FAULT_WAKE);
+    const uint32_t np = counters[CNT_P];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        isp[p] = 0;
+        if (!ncont[p]) continue;
+        const uint32_t i = cand_i[p], j = cand_j[p];
+        const bool si = sleeping_now(B, i), sj = sleeping_now(B, j);
+        const bool si_pre = si && !(wake_at[i] < p), sj_pre = sj && !(wake_at[j] < p);
+        if (walk_mass(B, i, si, si_pre) + walk_mass(B, j, sj, sj_pre) <= 0.0) { ncont[p] = 0; continue; }
+        const bool si_post = si_pre && wake_at[i] != p, sj_post = sj_pre && wake_at[j] != p;
+        if (si_post && sj_post) { ncont[p] = 0; continue; }
+        isp[p] = (walk_mass(B, i, si, si_post) > 0.0 && walk_mass(B, j, sj, sj_post) > 0.0) ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_sleep_apply_wake(BodiesDev B, const uint32_t* __restrict__ wake_at) {   // world.cpp:317-323
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n || wake_at[i] == 0xFFFFFFFFu) return;
+    B.im[i] = B.ims[i];
+    B.ii0[i] = B.iis0[i]; B.ii1[i] = B.iis1[i]; B.ii2[i] = B.iis2[i];
+    B.flags[i] &= ~NE_FLAG_SLEEPING;
+    B.st[i] = 0.0;
+}
+
+__device__ __forceinline__ bool awake_dyn(const BodiesDev& B, uint32_t i) {
+    return (B.flags[i] & NE_FLAG_DYNAMIC) && !(B.flags[i] & NE_FLAG_SLEEPING) && B.im[i] > 0.0;
+}
+
+__device__ __forceinline__ void sleep_union(const BodiesDev& B, uint32_t* parent, uint32_t i, uint32_t j) {
+    if (!awake_dyn(B, i) || !awake_dyn(B, j)) return;
+    uint32_t a = i, b = j;
+    for (int guard = 0; guard < 1 << 20; ++guard) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) break;
+        if (a < b) { uint32_t t = a; a = b; b = t; }
+        if (atomicCAS(&parent[a], a, b) == a) break;
+    }
+}
+
+// Doubles as unsigned keys that order as the values (negative ones reversed),
+// so an island's smallest timer is an atomic minimum.
+__device__ __forceinline__ unsigned long long timer_key(double t) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(t);
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+__device__ __forceinline__ double timer_of_key(unsigned long long k) {
+    return __longlong_as_double((long long)((k >> 63) ? (k & ~(1ull << 63)) : ~k));
+}
+
+__global__ void __launch_bounds__(TPB) k_sleep_init(BodiesDev B, uint32_t* __restrict__ parent,
+                                                    unsigned long long* __restrict__ mint) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    parent[i] = i;
+    mint[i] = ~0ull;
+}
+
+__global__ void __launch_bounds__(TPB) k_sleep_join_pairs(BodiesDev B, const uint32_t* __restrict__ counters,
+                                                          const uint32_t* __restrict__ cand_i,
+                                                          const uint32_t* __restrict__ cand_j,
+                                                          const uint8_t* __restrict__ isp, uint32_t* __restrict__ parent) {
+    const uint32_t np = counters[CNT_P];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x)
+        if (isp[p]) sleep_union(B, parent, cand_i[p], cand_j[p]);
+}
+
+__global__ void __launch_bounds__(TPB) k_sleep_join_joints(BodiesDev B, const uint32_t* __restrict__ ja,
+                                                           const uint32_t* __restrict__ jb, uint32_t nj,
+                                                           uint32_t* __restrict__ parent) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nj) sleep_union(B, parent, ja[k], jb[k]);
+}
+
+// Slow-timers (world.cpp:346-351) and each island's smallest timer.
+__global__ void __launch_bounds__(TPB) k_sleep_timers(BodiesDev B, SimParams P, uint32_t* __restrict__ parent,
+                                                      unsigned long long* __restrict__ mint) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n || !awake_dyn(B, i)) return;
+    double t = B.st[i];
+    if (norm(ld3(B.v0, B.v1, B.v2, i)) < P.sleep_lin && norm(ld3(B.w0, B.w1, B.w2, i)) < P.sleep_ang) t = t + P.dt;
+    else t = 0.0;
+    B.st[i] = t;
+    atomicMin(&mint[uf_find(parent, i)], timer_key(t));
+}
+
+// Islands whose slowest member has been slow for sleepTime go to sleep
+// (world.cpp:360-370).
+__global__ void __launch_bounds__(TPB) k_sleep_apply(BodiesDev B, SimParams P, uint32_t* __restrict__ parent,
+                                                     const unsigned long long* __restrict__ mint) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n || !awake_dyn(B, i)) return;
+    const double m = timer_of_key(mint[uf_find(parent, i)]);
+    if (m < P.sleep_time) return;
+    B.ims[i] = B.im[i];
+    B.iis0[i] = B.ii0[i]; B.iis1[i] = B.ii1[i]; B.iis2[i] = B.ii2[i];
+    B.im[i] = 0.0;
+    B.ii0[i] = 0.0; B.ii1[i] = 0.0; B.ii2[i] = 0.0;
+    st3(B.v0, B.v1, B.v2, i, V3{0.0, 0.0, 0.0});
+    st3(B.w0, B.w1, B.w2, i, V3{0.0, 0.0, 0.0});
+    B.flags[i] |= NE_FLAG_SLEEPING;
+}
+
+// ---------------------------------------------------------------------------
 // Distance joints (src/world.cpp:150-197).  Joints are grouped on the host
 // into components: joints that share a body that can move are in one
 // component, swept by one thread in the joints' own order, so the
@@ -2249,8 +2430,14 @@ struct ne_gpu_ctx {
     uint32_t *skey0 = nullptr, *sval0 = nullptr, *skey1 = nullptr, *sval1 = nullptr;
     uint32_t* isl_start = nullptr;
     EpaBig epa_big{};                    // pairs rerun with the large EPA workspace
+    uint32_t* wake_at = nullptr;         // island sleeping: first waking pair per body
+    uint32_t* wake_again = nullptr;      // island sleeping: per-round "a fast body woke" flags
+    uint8_t* isp = nullptr;              // island sleeping: dynamic-dynamic pair flags (pcap)
+    unsigned long long* mint = nullptr;  // island sleeping: smallest timer per island root
     // start-of-call snapshot (rerun after a capacity overflow)
     double* snap_body = nullptr;
+    uint32_t* snap_flags = nullptr;
+    uint32_t snap_fcap = 0;
     size_t snap_nd = 0;
     uint64_t* snap_key = nullptr;
     double* snap_imp = nullptr;
@@ -2394,7 +2581,8 @@ void drop_graphs(ne_gpu_ctx* c) {
 enum DF : int {
     F_X0, F_X1, F_X2, F_V0, F_V1, F_V2, F_Q0, F_Q1, F_Q2, F_Q3, F_W0, F_W1, F_W2,
     F_F0, F_F1, F_F2, F_T0, F_T1, F_T2, F_IM, F_II0, F_II1, F_II2,
-    F_RAD, F_HH, F_HE0, F_HE1, F_HE2, F_BR, F_FRIC, F_REST, F_LD, F_AD, F_GS, F_COUNT
+    F_RAD, F_HH, F_HE0, F_HE1, F_HE2, F_BR, F_FRIC, F_REST, F_LD, F_AD, F_GS,
+    F_ST, F_IMS, F_IIS0, F_IIS1, F_IIS2, F_COUNT
 };
 enum UF : int { U_SHAPE, U_LAYER, U_MASK, U_FLAGS, U_COUNT };
 
@@ -2433,6 +2621,11 @@ SimParams make_sim_params(const ne_world_params& p) {
     s.warm = p.warm_start ? 1 : 0;
     s.iters = p.solver_iterations;
     s.open_boundary = p.open_boundary ? 1 : 0;
+    s.sleep = p.sleep_enabled ? 1 : 0;
+    s.sleep_lin = p.sleep_lin_vel;
+    s.sleep_ang = p.sleep_ang_vel;
+    s.sleep_time = p.sleep_time;
+    s.wake_vel = p.wake_vel;
     return s;
 }
 
@@ -2464,6 +2657,7 @@ int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
     HIP_TRY(c, dalloc(c->cand_i, pcap));
     HIP_TRY(c, dalloc(c->cand_j, pcap));
     HIP_TRY(c, dalloc(c->epa_q, pcap));
+    HIP_TRY(c, dalloc(c->isp, pcap));
     HIP_TRY(c, dalloc(c->ncont, (size_t)pcap + 1));
     HIP_TRY(c, dalloc(c->cons_off, (size_t)pcap + 1));
     HIP_TRY(c, dalloc(c->ct, (size_t)pcap * 28));
@@ -2536,7 +2730,8 @@ void free_work(ne_gpu_ctx* c) {
     dfree(c->cell_of); dfree(c->cell_count); dfree(c->cell_start); dfree(c->sorted_idx); dfree(c->sorted);
     dfree(c->order_prev); dfree(c->xp);
     dfree(c->row_count); dfree(c->row_start); dfree(c->ex_flags); dfree(c->ex_off);
-    dfree(c->cand_i); dfree(c->cand_j); dfree(c->epa_q); dfree(c->ncont); dfree(c->cons_off); dfree(c->ct);
+    dfree(c->cand_i); dfree(c->cand_j); dfree(c->epa_q); dfree(c->isp);
+    dfree(c->wake_at); dfree(c->mint); dfree(c->wake_again); dfree(c->ncont); dfree(c->cons_off); dfree(c->ct);
     ConsDev& C = c->C;
     dfree(C.ci); dfree(C.cj); dfree(C.slot);
     double** dp[] = {&C.n0, &C.n1, &C.n2, &C.p0, &C.p1, &C.p2, &C.ra0, &C.ra1, &C.ra2, &C.rb0, &C.rb1, &C.rb2,
@@ -2579,8 +2774,9 @@ void free_all(ne_gpu_ctx* c) {
     free_adjoint(c);
     free_work(c);
     dfree(c->prev_key); dfree(c->prev_imp); dfree(c->prev_n); dfree(c->counters);
-    dfree(c->snap_body); dfree(c->snap_key); dfree(c->snap_imp); dfree(c->snap_n);
+    dfree(c->snap_body); dfree(c->snap_key); dfree(c->snap_imp); dfree(c->snap_n); dfree(c->snap_flags);
     c->snap_nd = 0;
+    c->snap_fcap = 0;
     c->snap_pcap = 0;
     c->prev_cap = 0;
     if (c->h_counters) { (void)hipHostFree(c->h_counters); c->h_counters = nullptr; }
@@ -2625,7 +2821,7 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
         k_joint_springs<<<blocks_for(c->ncomp, 64), 64, 0, s>>>(B, P, joint_dev(c));
     {
         KTimer kt(c, NE_KERNEL_INTEGRATE);
-        const bool w = wrap_first && P.box.periodic && c->njoints == 0;
+        const bool w = wrap_first && P.box.periodic && c->njoints == 0 && !P.sleep;
         if (c->has_forces || c->nspring) {
             if (w) launch_integrate<true, true>(c, P, static_grid);
             else launch_integrate<true, false>(c, P, static_grid);
@@ -2705,6 +2901,15 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
             k_narrow_big<<<16, 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_big);
         }
     }
+    if (P.sleep) {   // wakes in pair order (world.cpp:527-531), then the woken bodies' masses
+        k_sleep_reset<<<blocks_for(n), TPB, 0, s>>>(B, c->wake_at, c->wake_again);
+        for (int r = 0; r <= WAKE_ROUNDS; ++r)
+            k_sleep_wake_mark<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont,
+                                                                     c->wake_at, c->wake_again, r);
+        k_sleep_fate<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, c->counters, c->cand_i, c->cand_j, c->ncont,
+                                                            c->wake_at, c->wake_again, c->isp);
+        k_sleep_apply_wake<<<blocks_for(n), TPB, 0, s>>>(B, c->wake_at);
+    }
     scan_u32(c->ncont, c->cons_off, c->counters + CNT_P, c->pcap, c->scan_part, s, false,
              ScanTotal{c->counters, CNT_CRAW, CNT_C, CNT_FAULT, c->ccap, NE_FAULT_CONTACTS, CNT_ISL, CNT_NBIG});
     {
@@ -2752,9 +2957,20 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     if (!P.box.periodic && !P.open_boundary) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_walls<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P);
-    } else if (P.box.periodic && c->njoints) {
+    } else if (P.box.periodic && (c->njoints || P.sleep)) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_wrap<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P);
+    }
+    if (P.sleep) {   // world.cpp:328-371
+        hipStream_t s = c->stream;
+        k_sleep_init<<<blocks_for(c->n), TPB, 0, s>>>(c->B, c->parent, c->mint);
+        k_sleep_join_pairs<<<hint_blocks(c->hint_p), TPB, 0, s>>>(c->B, c->counters, c->cand_i, c->cand_j, c->isp,
+                                                                  c->parent);
+        if (c->njoints)
+            k_sleep_join_joints<<<blocks_for(c->njoints), TPB, 0, s>>>(c->B, c->j_ab, c->j_ab + c->njoints, c->njoints,
+                                                                        c->parent);
+        k_sleep_timers<<<blocks_for(c->n), TPB, 0, s>>>(c->B, P, c->parent, c->mint);
+        k_sleep_apply<<<blocks_for(c->n), TPB, 0, s>>>(c->B, P, c->parent, c->mint);
     }
     return NE_OK;
 }
@@ -3010,8 +3226,8 @@ int ne_gpu_rollout(ne_gpu_ctx* c, const double* v0, const double* mu, const doub
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
     if (c->n == 0) return set_err(c, NE_ERR_INVALID, "no bodies");
     if (!c->params.periodic) return set_err(c, NE_ERR_UNSUPPORTED, "rollouts are differentiated for periodic worlds");
-    if (c->has_forces || c->njoints)
-        return set_err(c, NE_ERR_UNSUPPORTED, "rollouts do not take applied forces or joints");
+    if (c->has_forces || c->njoints || c->params.sleep_enabled)
+        return set_err(c, NE_ERR_UNSUPPORTED, "rollouts do not take applied forces, joints or sleeping");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const uint32_t nw = c->nworlds ? c->nworlds : 1u;
     if (mu || e) {
@@ -3157,8 +3373,12 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps);
 // Start-of-call copy of what a step changes (pose, velocity, force and torque
 // fields, which lead the field table, and the warm cache), taken on the
 // stream, so a call that overflows a buffer can be run again from its start.
+// With island sleeping, masses, timers and flags change too: then the whole
+// body table is kept.
+size_t snapshot_fields(const ne_gpu_ctx* c) { return c->params.sleep_enabled ? (size_t)F_COUNT : (size_t)(F_T2 + 1); }
+
 int snapshot_take(ne_gpu_ctx* c) {
-    const size_t nd = (size_t)(F_T2 + 1) * c->ncap;
+    const size_t nd = snapshot_fields(c) * c->ncap;
     if (c->snap_nd < nd) {
         HIP_TRY(c, dalloc(c->snap_body, nd));
         c->snap_nd = nd;
@@ -3171,6 +3391,14 @@ int snapshot_take(ne_gpu_ctx* c) {
     if (!c->snap_n) HIP_TRY(c, dalloc(c->snap_n, 1));
     c->snap_prev = c->prev_cap;
     HIP_TRY(c, hipMemcpyAsync(c->snap_body, c->dfield, sizeof(double) * nd, hipMemcpyDeviceToDevice, c->stream));
+    if (c->params.sleep_enabled) {
+        if (c->snap_fcap < c->ncap) {
+            HIP_TRY(c, dalloc(c->snap_flags, c->ncap));
+            c->snap_fcap = c->ncap;
+        }
+        HIP_TRY(c, hipMemcpyAsync(c->snap_flags, c->ufield + (size_t)U_FLAGS * c->ncap, sizeof(uint32_t) * c->ncap,
+                                  hipMemcpyDeviceToDevice, c->stream));
+    }
     HIP_TRY(c, hipMemcpyAsync(c->snap_key, c->prev_key, sizeof(uint64_t) * c->prev_cap, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->snap_imp, c->prev_imp, sizeof(double) * c->prev_cap, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->snap_n, c->prev_n, sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
@@ -3178,9 +3406,12 @@ int snapshot_take(ne_gpu_ctx* c) {
 }
 
 int snapshot_restore(ne_gpu_ctx* c) {
-    const size_t nd = (size_t)(F_T2 + 1) * c->ncap;
+    const size_t nd = snapshot_fields(c) * c->ncap;
     const uint32_t m = std::min(c->snap_prev, c->prev_cap);
     HIP_TRY(c, hipMemcpyAsync(c->dfield, c->snap_body, sizeof(double) * nd, hipMemcpyDeviceToDevice, c->stream));
+    if (c->params.sleep_enabled && c->snap_flags)
+        HIP_TRY(c, hipMemcpyAsync(c->ufield + (size_t)U_FLAGS * c->ncap, c->snap_flags, sizeof(uint32_t) * c->ncap,
+                                  hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->prev_key, c->snap_key, sizeof(uint64_t) * m, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->prev_imp, c->snap_imp, sizeof(double) * m, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->prev_n, c->snap_n, sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
@@ -3262,11 +3493,12 @@ int ne_gpu_set_params(ne_gpu_ctx* c, const ne_world_params* p) {
         if (!(p->half[a] > 0.0)) return set_err(c, NE_ERR_INVALID, "box half extents must be > 0");
     if (p->restitution_combine < 0 || p->restitution_combine > 4 || p->friction_combine < 0 || p->friction_combine > 4)
         return set_err(c, NE_ERR_INVALID, "unknown combine mode");
-    if (p->sleep_enabled) return set_err(c, NE_ERR_UNSUPPORTED, "island sleeping is not on the GPU path yet");
+    if (p->sleep_enabled && !(p->wake_vel >= 0.0)) return set_err(c, NE_ERR_INVALID, "wake_vel must be >= 0");
     if (c->nworlds && !p->periodic) return set_err(c, NE_ERR_UNSUPPORTED, "batched worlds must be periodic");
     c->params = *p;
     c->have_params = true;
     c->grid_dirty = true;
+    c->B.sleep_on = p->sleep_enabled ? 1 : 0;
     drop_graphs(c);
     return NE_OK;
 }
@@ -3396,7 +3628,7 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps) {
         int r = run_step(c, t > 0);
         if (r != NE_OK) return r;
     }
-    if (c->params.periodic && nsteps > 0 && c->njoints == 0) {   // with joints every step wrapped itself
+    if (c->params.periodic && nsteps > 0 && c->njoints == 0 && !c->params.sleep_enabled) {   // else each step wrapped itself
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_wrap<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, make_sim_params(c->params));
     }
@@ -3433,6 +3665,8 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps) {
         char buf[160];
         snprintf(buf, sizeof buf, "step overflowed (fault bits 0x%x: pairs %u/%u, contacts %u/%u); capacity raised, re-upload",
                  fault, c->h_counters[CNT_PRAW], c->pcap, c->h_counters[CNT_CRAW], c->ccap);
+        if (fault & NE_FAULT_WAKE)
+            return set_err(c, NE_ERR_UNSUPPORTED, "a chain of woken bodies longer than the wake rounds; state is undefined");
         return set_err(c, (fault & NE_FAULT_EPA) ? NE_ERR_INVALID : NE_ERR_CAPACITY, buf);
     }
     return NE_OK;
@@ -3475,7 +3709,8 @@ int ne_gpu_get_body_aux(ne_gpu_ctx* c, uint32_t* flags, double* sleep_timer, dou
         HIP_TRY(c, hipMemcpy(flags, c->ufield + (size_t)U_FLAGS * c->ncap, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < n; ++i) flags[i] &= ~FLAG_GRID_EXCLUDED;
     }
-    if (sleep_timer) for (uint32_t i = 0; i < n; ++i) sleep_timer[i] = 0.0;
+    if (sleep_timer) HIP_TRY(c, hipMemcpy(sleep_timer, c->dfield + (size_t)F_ST * c->ncap, sizeof(double) * n,
+                                          hipMemcpyDeviceToHost));
     if (inv_mass) HIP_TRY(c, hipMemcpy(inv_mass, c->dfield + (size_t)F_IM * c->ncap, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (inv_inertia) {
         std::vector<double> tmp(n);
@@ -3836,7 +4071,6 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
             return set_err(c, NE_ERR_UNSUPPORTED, "mesh/compound shapes are not on the GPU path yet");
         const uint32_t fl = b->flags ? b->flags[i] : NE_FLAG_DYNAMIC;
         if (fl & NE_FLAG_CCD) return set_err(c, NE_ERR_UNSUPPORTED, "CCD is not on the GPU path");
-        if (fl & NE_FLAG_SLEEPING) return set_err(c, NE_ERR_UNSUPPORTED, "sleeping bodies are not on the GPU path yet");
         if (s == NE_SHAPE_CONVEX && (!b->hull_offsets || b->hull_offsets[i + 1] < b->hull_offsets[i]))
             return set_err(c, NE_ERR_INVALID, "convex body without hull vertices");
     }
@@ -3864,6 +4098,9 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
         HIP_TRY(c, dalloc(c->root_id, (size_t)ncap + 1));
         HIP_TRY(c, dalloc(c->slot_of, ncap));
         HIP_TRY(c, dalloc(c->claimed, ncap));
+        HIP_TRY(c, dalloc(c->wake_at, ncap));
+        if (!c->wake_again) HIP_TRY(c, dalloc(c->wake_again, WAKE_ROUNDS + 1));
+        HIP_TRY(c, dalloc(c->mint, ncap));
         HIP_TRY(c, dalloc(c->row_scratch, (size_t)ncap * ROW_SLOTS));
         c->cellcap = (uint32_t)std::min<uint64_t>((uint64_t)c->cells_per_body * ncap + 4096u, 0x7FFFFFFFull);
         HIP_TRY(c, dalloc(c->cell_count, (size_t)c->cellcap + 1));
@@ -3932,6 +4169,9 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
         D(F_LD, i) = b->linear_damping ? b->linear_damping[i] : 0.0;
         D(F_AD, i) = b->angular_damping ? b->angular_damping[i] : 0.0;
         D(F_GS, i) = b->gravity_scale ? b->gravity_scale[i] : 1.0;
+        D(F_ST, i) = b->sleep_timer ? b->sleep_timer[i] : 0.0;
+        D(F_IMS, i) = b->inv_mass_store ? b->inv_mass_store[i] : 0.0;
+        for (int a = 0; a < 3; ++a) D(F_IIS0 + a, i) = b->inv_inertia_store ? b->inv_inertia_store[3 * i + a] : 0.0;
         R[i] = host_bounding_radius(b->shape[i], b->radius[i], b->half_height[i], b->half_extents + 3 * i,
                                     hhull.data(), hoff[i], hoff[i + 1]);
         D(F_BR, i) = R[i];
@@ -3958,7 +4198,8 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     c->h_mut.assign(n, 0);
     for (uint32_t i = 0; i < n; ++i)
         c->h_mut[i] = b->inv_mass[i] != 0.0 || b->inv_inertia[3 * i] != 0.0 || b->inv_inertia[3 * i + 1] != 0.0 ||
-                      b->inv_inertia[3 * i + 2] != 0.0;
+                      b->inv_inertia[3 * i + 2] != 0.0 ||
+                      ((b->flags ? b->flags[i] : NE_FLAG_DYNAMIC) & NE_FLAG_SLEEPING);   // may wake
     c->grid_dirty = true;
     drop_graphs(c);
     c->has_forces = forces;
@@ -4008,6 +4249,8 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     B.im = F(F_IM); B.ii0 = F(F_II0); B.ii1 = F(F_II1); B.ii2 = F(F_II2);
     B.rad = F(F_RAD); B.hh = F(F_HH); B.he0 = F(F_HE0); B.he1 = F(F_HE1); B.he2 = F(F_HE2); B.bR = F(F_BR);
     B.fric = F(F_FRIC); B.rest = F(F_REST); B.ldamp = F(F_LD); B.adamp = F(F_AD); B.gs = F(F_GS);
+    B.st = F(F_ST); B.ims = F(F_IMS); B.iis0 = F(F_IIS0); B.iis1 = F(F_IIS1); B.iis2 = F(F_IIS2);
+    B.sleep_on = c->params.sleep_enabled ? 1 : 0;
     B.shape = (int32_t*)(c->ufield + (size_t)U_SHAPE * ncap);
     B.layer = c->ufield + (size_t)U_LAYER * ncap;
     B.mask = c->ufield + (size_t)U_MASK * ncap;

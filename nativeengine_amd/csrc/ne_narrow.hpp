@@ -33,6 +33,7 @@ enum : uint32_t {
     NE_FAULT_CELLS = 8u,      // grid cell capacity exceeded
     NE_FAULT_EPA_QUEUE = 16u, // more large-workspace EPA reruns than queue slots
     NE_FAULT_EPA_BIG = 32u,   // an EPA rerun outgrew the large workspace
+    NE_FAULT_WAKE = 64u,      // a chain of sleeping bodies woken one by another outran the wake rounds
 };
 
 // Geometry of one body as the narrow phase sees it.

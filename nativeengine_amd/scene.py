@@ -98,6 +98,10 @@ class Scene:
         self.nocollide_offsets = np.zeros(n + 1, np.uint32)
         self.nocollide = np.zeros(0, np.uint32)
         self.joints = np.zeros(0, _abi.JOINT_DTYPE)   # World::distanceJoints, in order
+        # Body::sleepTimer / invMassStore / invIStore (body.hpp); None uploads zeros
+        self.sleep_timer = None
+        self.inv_mass_store = None
+        self.inv_inertia_store = None
         self.params = Params()
 
     # -- construction -----------------------------------------------------
@@ -166,9 +170,9 @@ class Scene:
         b.linear_damping, b.angular_damping = f64(self.linear_damping), f64(self.angular_damping)
         b.gravity_scale = f64(self.gravity_scale)
         b.layer, b.mask, b.flags = u32(self.layer), u32(self.mask), u32(self.flags)
-        b.sleep_timer = None
-        b.inv_mass_store = None
-        b.inv_inertia_store = None
+        b.sleep_timer = None if self.sleep_timer is None else f64(self.sleep_timer)
+        b.inv_mass_store = None if self.inv_mass_store is None else f64(self.inv_mass_store)
+        b.inv_inertia_store = None if self.inv_inertia_store is None else f64(self.inv_inertia_store)
         b.hull_offsets = u32(self.hull_offsets)
         hv = np.ascontiguousarray(self.hull_verts.reshape(-1, 3) if len(self.hull_verts) else np.zeros((1, 3)), np.float64)
         keep.append(hv)
@@ -681,4 +685,67 @@ def rope_scene(nropes: int = 8, links: int = 16, radius: float = 0.2, half=(6.0,
     s.joints = np.array([(a, b, la, lb, rest, k, d) for a, b, la, lb, rest, k, d in joints], _abi.JOINT_DTYPE)
     s.params = Params(half=tuple(half), periodic=periodic, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0),
                       restitution=0.1, friction=0.4, solver_iterations=8)
+    return s
+
+
+def sleeping_pile(n: int = 120, seed: int = 0, half=(6.0, 6.0, 6.0), periodic: bool = False,
+                  projectile_delay: float = 2.5) -> Scene:
+    """A pile of boxes and spheres settling on a static ground box with island
+    sleeping on, and a fast sphere that reaches the pile after about
+    `projectile_delay` seconds and wakes part of it."""
+    s = mixed_pile(n, half=half, seed=seed, periodic=periodic, spacing=1.2)
+    bodies = []
+    for i in range(s.n):
+        b = Body(shape=int(s.shape[i]), id=i, x=tuple(s.x[i]), q=tuple(s.q[i]), radius=float(s.radius[i]),
+                 half_height=float(s.half_height[i]), half_extents=tuple(s.half_extents[i]))
+        b.inv_mass = float(s.inv_mass[i])
+        b.inv_inertia = tuple(s.inv_inertia[i])
+        b.flags = int(s.flags[i])
+        b.vertices = s.hull_verts[s.hull_offsets[i]:s.hull_offsets[i + 1]].tolist()
+        bodies.append(b)
+    speed = 8.0
+    start = (-half[0] + 0.6, -half[1] + 2.0, 0.0)
+    # flies along +x from behind a wall of empty space: distance = speed * delay (clamped to the box)
+    x0 = max(-half[0] + 0.6, -speed * projectile_delay * 0.1)
+    proj = make_sphere(len(bodies), (x0, start[1], start[2]), 0.35, 4.0)
+    proj.v = (speed, 0.0, 0.0)
+    bodies.append(proj)
+    out = Scene.from_bodies(bodies)
+    out.params = s.params
+    out.params.sleep_enabled = True
+    return out
+
+
+def wake_chain(chain: int = 6, rows: int = 3, seed: int = 0, drift: float = 1.0) -> Scene:
+    """Rows of touching spheres set asleep with a velocity above wakeVel, each
+    row led by an awake sphere moving into it.  The reference wakes along a
+    row within one step, since each woken sphere keeps its velocity and the
+    pairs come in index order (world.cpp:527-531).  Every sleeper carries its
+    stored inverse mass and inertia and a nonzero timer."""
+    rng = np.random.default_rng(seed)
+    bodies = []
+    r = 0.5
+    for row in range(rows):
+        y = -4.0 + 2.5 * row
+        for k in range(chain + 1):
+            b = make_sphere(len(bodies), (-5.0 + 0.98 * k, y, rng.uniform(-0.01, 0.01)), r, 1.0)
+            b.v = (drift + 0.5 * k, 0.0, 0.0) if k else (3.0, 0.0, 0.0)
+            bodies.append(b)
+    s = Scene.from_bodies(bodies)
+    n = s.n
+    s.sleep_timer = np.zeros(n)
+    s.inv_mass_store = np.zeros(n)
+    s.inv_inertia_store = np.zeros((n, 3))
+    for i in range(n):
+        if i % (chain + 1) == 0:
+            continue
+        s.inv_mass_store[i] = s.inv_mass[i]
+        s.inv_inertia_store[i] = s.inv_inertia[i]
+        s.inv_mass[i] = 0.0
+        s.inv_inertia[i] = 0.0
+        s.flags[i] |= _abi.FLAG_SLEEPING
+        s.sleep_timer[i] = 0.75
+    s.params.gravity = (0.0, 0.0, 0.0)
+    s.params.half = (8.0, 8.0, 8.0)
+    s.params.sleep_enabled = True
     return s

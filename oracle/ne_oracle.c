@@ -138,6 +138,9 @@ struct ne_oracle {
     /* distance joints, in order (world.hpp:129) */
     uint32_t njoints;
     ne_distance_joint* joints;
+    /* this step's dynamic-dynamic contact pairs, for island sleeping (world.hpp:158) */
+    uint32_t nisp, capisp;
+    uint32_t* isp;
 };
 
 struct otape_pair {
@@ -932,6 +935,7 @@ static void collide(ne_oracle* o) {                                             
     const size_t npairs = (o->tape_mode == 2) ? (replay ? replay->n : 0) : pv.n;
     o->ncand = (uint32_t)npairs;
     o->ncons = 0;
+    o->nisp = 0;
     const ne_world_params* P = &o->P;
     for (size_t pi = 0; pi < npairs; ++pi) {
         const uint32_t i = replay ? replay->p[pi].i : pv.p[pi].i, j = replay ? replay->p[pi].j : pv.p[pi].j;
@@ -960,6 +964,15 @@ static void collide(ne_oracle* o) {                                             
             if (bS && !aS && (a->flags & (NE_FLAG_DYNAMIC | NE_FLAG_KINEMATIC)) && norm(a->v) > P->wake_vel) wake(b);
             bS = (b->flags & NE_FLAG_SLEEPING) != 0;
             if (aS && bS) continue;
+        }
+        if (a->invMass > 0.0 && b->invMass > 0.0) {                                 /* world.cpp:533 */
+            if (o->nisp == o->capisp) {
+                o->capisp = o->capisp ? 2 * o->capisp : 256;
+                o->isp = (uint32_t*)realloc(o->isp, sizeof(uint32_t) * 2 * o->capisp);
+            }
+            o->isp[2 * o->nisp] = i;
+            o->isp[2 * o->nisp + 1] = j;
+            ++o->nisp;
         }
         const uint64_t key = pair_key_of(i, j);
         int slot = 0;
@@ -1155,6 +1168,63 @@ static void solve_rigid_joints(ne_oracle* o, int iterations) {
     }
 }
 
+/* Island sleeping (world.cpp:328-371): awake dynamic bodies joined by this
+ * step's contact pairs and by distance joints; an island sleeps when every
+ * member has been slow for sleep_time. */
+static int awake_dyn(const obody* b) {
+    return (b->flags & NE_FLAG_DYNAMIC) && !(b->flags & NE_FLAG_SLEEPING) && b->invMass > 0.0;
+}
+
+static uint32_t uf_find(uint32_t* parent, uint32_t x) {
+    while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; }
+    return x;
+}
+
+static void update_sleep(ne_oracle* o) {
+    const uint32_t n = o->n;
+    uint32_t* parent = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for (uint32_t i = 0; i < n; ++i) parent[i] = i;
+#define JOIN(a_, b_)                                                                  \
+    do {                                                                              \
+        const uint32_t ja_ = (a_), jb_ = (b_);                                        \
+        if (awake_dyn(&o->B[ja_]) && awake_dyn(&o->B[jb_])) {                         \
+            const uint32_t ra_ = uf_find(parent, ja_);                                \
+            parent[ra_] = uf_find(parent, jb_);                                       \
+        }                                                                             \
+    } while (0)
+    for (uint32_t k = 0; k < o->nisp; ++k) JOIN(o->isp[2 * k], o->isp[2 * k + 1]);
+    for (uint32_t k = 0; k < o->njoints; ++k) JOIN(o->joints[k].a, o->joints[k].b);
+#undef JOIN
+    for (uint32_t i = 0; i < n; ++i) {
+        obody* b = &o->B[i];
+        if (!awake_dyn(b)) continue;
+        if (norm(b->v) < o->P.sleep_lin_vel && norm(b->w) < o->P.sleep_ang_vel) b->sleepTimer += o->P.dt;
+        else b->sleepTimer = 0.0;
+    }
+    double* mint = (double*)malloc(sizeof(double) * (n ? n : 1));
+    unsigned char* seen = (unsigned char*)calloc(n ? n : 1, 1);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!awake_dyn(&o->B[i])) continue;
+        const uint32_t r = uf_find(parent, i);
+        if (!seen[r] || o->B[i].sleepTimer < mint[r]) { mint[r] = o->B[i].sleepTimer; seen[r] = 1; }
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        obody* b = &o->B[i];
+        if (!awake_dyn(b)) continue;
+        if (mint[uf_find(parent, i)] < o->P.sleep_time) continue;
+        b->invMassStore = b->invMass;
+        b->invIStore = b->invI;
+        b->invMass = 0.0;
+        b->invI = mk(0.0, 0.0, 0.0);
+        b->v = mk(0.0, 0.0, 0.0);
+        b->w = mk(0.0, 0.0, 0.0);
+        b->flags |= NE_FLAG_SLEEPING;
+    }
+    free(parent);
+    free(mint);
+    free(seen);
+}
+
 static void world_step(ne_oracle* o) {                                               /* world.cpp:133-146 */
     apply_joint_forces(o);
     integrate(o);
@@ -1162,6 +1232,7 @@ static void world_step(ne_oracle* o) {                                          
     solve_rigid_joints(o, 8);
     if (o->box.periodic) wrap_positions(o);
     else if (!o->P.open_boundary) apply_walls(o);
+    if (o->P.sleep_enabled) update_sleep(o);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1193,6 +1264,7 @@ int ne_oracle_destroy(ne_oracle* o) {
     for (uint32_t t = 0; t < o->tape_steps; ++t) free(o->tape[t].p);
     free(o->tape);
     free(o->joints);
+    free(o->isp);
     free(o);
     return NE_OK;
 }
@@ -1200,7 +1272,6 @@ int ne_oracle_destroy(ne_oracle* o) {
 int ne_oracle_set_params(ne_oracle* o, const ne_world_params* p) {
     if (!o || !p) return NE_ERR_INVALID;
     if (p->solver_iterations < 0) return NE_ERR_INVALID;
-    if (p->sleep_enabled) return NE_ERR_UNSUPPORTED;
     o->P = *p;
     for (int a = 0; a < 3; ++a) o->box.half[a] = p->half[a];
     o->box.periodic = p->periodic != 0;

@@ -389,3 +389,73 @@ def test_oracle_spring_settles_at_gravity_balanced_stretch():
     mass = (4.0 / 3.0) * math.pi          # density 1, r = 1
     assert abs((np.linalg.norm(x[1] - x[0]) - 2.0) - mass * 10.0 / 200.0) < 0.05
     assert np.linalg.norm(v[1]) < 0.1
+
+
+def _aux(w, n):
+    import ctypes as C
+    fl, tm, im, ii = np.zeros(n, np.uint32), np.zeros(n), np.zeros(n), np.zeros((n, 3))
+    w._f("get_body_aux")(w.h, fl.ctypes.data_as(C.POINTER(C.c_uint32)), tm.ctypes.data_as(C.POINTER(C.c_double)),
+                         im.ctypes.data_as(C.POINTER(C.c_double)), ii.ctypes.data_as(C.POINTER(C.c_double)))
+    return fl & np.uint32(_abi.FLAG_SLEEPING), tm, im, ii
+
+
+@pytest.mark.parametrize("periodic,seed", [(False, 1), (True, 2)])
+def test_oracle_island_sleeping_matches_reference_live(periodic, seed):
+    """Island sleeping (world.cpp:328-371) and waking in collide (527-531)
+    against the unmodified reference: state, sleep flags, timers and the
+    zeroed / restored masses, every step, through settling and a wake-up."""
+    if not engines.ref_available():
+        pytest.skip("reference not built")
+    s = S.sleeping_pile(80, seed=seed, periodic=periodic)
+    o, r = engines.OracleWorld(), engines.RefWorld()
+    o.set_scene(s)
+    r.set_scene(s)
+    slept = woke = 0
+    prev = np.zeros(s.n, np.uint32)
+    for t in range(300):
+        o.step(1)
+        r.step(1)
+        for name, u, v in zip("xvqw", o.state(), r.state()):
+            assert np.array_equal(u, v), f"step {t}: {name} differs"
+        ao, ar = _aux(o, s.n), _aux(r, s.n)
+        for name, u, v in zip(["sleeping", "timer", "inv_mass", "inv_inertia"], ao, ar):
+            assert np.array_equal(u, v), f"step {t}: {name} differs"
+        slept += int(np.sum((ao[0] != 0) & (prev == 0)))
+        woke += int(np.sum((ao[0] == 0) & (prev != 0)))
+        prev = ao[0]
+    assert slept > 5 and woke > 0, (slept, woke)
+
+
+def test_oracle_wake_chain_matches_reference_live():
+    """Sleepers set asleep with a velocity above wakeVel wake one another along
+    a row within a single step (pairs in index order, world.cpp:527-531); the
+    stored masses, timers and flags follow the reference through the chain."""
+    if not engines.ref_available():
+        pytest.skip("reference not built")
+    s = S.wake_chain(chain=6, rows=3)
+    o, r = engines.OracleWorld(), engines.RefWorld()
+    o.set_scene(s)
+    r.set_scene(s)
+    for t in range(40):
+        o.step(1)
+        r.step(1)
+        for name, u, v in zip("xvqw", o.state(), r.state()):
+            assert np.array_equal(u, v), f"step {t}: {name} differs"
+        ao, ar = _aux(o, s.n), _aux(r, s.n)
+        for name, u, v in zip(["sleeping", "timer", "inv_mass", "inv_inertia"], ao, ar):
+            assert np.array_equal(u, v), f"step {t}: {name} differs"
+        if t == 0:
+            assert not np.any(ao[0]), "the whole of every row wakes in the first step"
+
+
+def test_oracle_wake_chain_first_step_closed_form():
+    """One step of a row by hand: every sleeper wakes with its stored masses,
+    a zero timer and its sleeping velocity before the contacts act."""
+    s = S.wake_chain(chain=4, rows=1)
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    o.step(1)
+    fl, tm, im, ii = _aux(o, s.n)
+    assert not np.any(fl)
+    assert np.array_equal(im, s.inv_mass_store + s.inv_mass)   # leader keeps its own, sleepers get the store
+    assert np.all(tm == 0.0)   # woken timers restart at zero, and every sphere is still fast
