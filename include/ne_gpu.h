@@ -127,6 +127,19 @@ typedef struct ne_distance_joint {
     double   rest, stiffness, damping;
 } ne_distance_joint;
 
+/* A static triangle mesh attached to a body of shape NE_SHAPE_MESH
+ * (`Body::mesh`, src/body.hpp:62; `MeshData`, src/mesh.hpp:14-93): vertices
+ * in the body frame, triangles as vertex-index triples.  A heightfield
+ * (`PhysicsWorld::addHeightfield`, src/physics_world.hpp:96-109) is such a mesh
+ * with the grid's two triangles per cell. */
+typedef struct ne_mesh {
+    uint32_t       body;      /* index of the NE_SHAPE_MESH body            */
+    uint32_t       nverts;
+    const double  *verts;     /* nverts*3                                   */
+    uint32_t       ntris;
+    const int32_t *tris;      /* ntris*3, each in [0, nverts)               */
+} ne_mesh;
+
 /* Context creation options. Zero fields take defaults. */
 typedef struct ne_gpu_config {
     int32_t  device;            /* HIP device ordinal                         */
@@ -163,6 +176,23 @@ int ne_gpu_set_bodies(ne_gpu_ctx *ctx, const ne_bodies *b);
  * act in the order given, as the reference's vector does.  Bodies must be
  * uploaded first; ne_gpu_set_bodies clears the joints. */
 int ne_gpu_set_joints(ne_gpu_ctx *ctx, uint32_t n, const ne_distance_joint *joints);
+
+/* Replaces building `MeshData` and assigning `Body::mesh` (src/mesh.hpp:38-43,
+ * src/physics_world.hpp:83-93): each mesh's bounding-volume tree is built
+ * here, once, by the reference's median split (leaves of at most 4
+ * triangles), and the meshes stay resident.  Mesh bodies must be static
+ * (zero inverse mass and inertia).  A mesh body with no mesh has no contacts,
+ * as in the reference.  Bodies must be uploaded first; ne_gpu_set_bodies
+ * clears the meshes. */
+int ne_gpu_set_meshes(ne_gpu_ctx *ctx, uint32_t n, const ne_mesh *meshes);
+
+/* The tree built for mesh `m` (the index in the last ne_gpu_set_meshes call):
+ * `order` (ntris triangle indices, tree order), then per node, in the
+ * reference's creation order, `bounds` (lo xyz, hi xyz) and `links` (left,
+ * right, start, count; -1 children for a leaf).  Call with NULL arrays to
+ * get the node count.  For checking against `MeshData::order` / `nodes`. */
+int ne_gpu_mesh_bvh(ne_gpu_ctx *ctx, uint32_t m, int32_t *order, uint32_t node_cap,
+                    uint32_t *nnodes, double *bounds, int32_t *links);
 
 /* Uploads only the dynamic state (x, v, q, w; any may be NULL) of the current
  * bodies. */

@@ -94,6 +94,17 @@ JOINT_DTYPE = np.dtype([("a", "<u4"), ("b", "<u4"), ("local_a", "<f8", (3,)), ("
                         ("rest", "<f8"), ("stiffness", "<f8"), ("damping", "<f8")])
 
 
+class Mesh(C.Structure):
+    """ne_mesh: a static triangle mesh attached to a NE_SHAPE_MESH body."""
+    _fields_ = [
+        ("body", C.c_uint32),
+        ("nverts", C.c_uint32),
+        ("verts", _dp),
+        ("ntris", C.c_uint32),
+        ("tris", _i32p),
+    ]
+
+
 class TuneResult(C.Structure):
     _fields_ = [
         ("cell_scale", C.c_double),

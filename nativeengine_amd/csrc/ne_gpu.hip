@@ -119,7 +119,9 @@ enum CounterSlot : int {
     CNT_EPATRUNC = 12,   // EPA runs stopped at the largest workspace (this call)
     CNT_EPARERUN = 13,   // large-workspace EPA reruns (this call)
     CNT_EPAQ = 14,       // convex pairs queued for EPA (this step)
-    CNT_N = 16
+    CNT_MQRAW = 16,      // mesh triangle candidates before clamping (this step)
+    CNT_MQ = 17,         // mesh triangle candidates (clamped)
+    CNT_N = 20
 };
 
 constexpr int TPB = 256;
@@ -938,6 +940,7 @@ __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimPar
     uint32_t nc = 0;
     const int si = SPHERES_ONLY ? (int)SH_SPHERE : B.shape[i];
     const int sj = SPHERES_ONLY ? (int)SH_SPHERE : B.shape[j];
+    if (!SPHERES_ONLY && (si == SH_MESH || sj == SH_MESH)) return 0;   // the mesh kernels take these pairs
     if (SPHERES_ONLY || (si == SH_SPHERE && sj == SH_SPHERE)) {
         Contact c = sphere_sphere(box, ld3(B.x0, B.x1, B.x2, i), B.rad[i], ld3(B.x0, B.x1, B.x2, j), B.rad[j]);
         if (c.hit) {
@@ -982,8 +985,10 @@ constexpr int EPA_BIG_FACES = 4096;       // initial faces (and horizon entries)
 constexpr int EPA_BIG_FACES_MAX = 1 << 17;   // beyond this a run stops at its closest face
 constexpr int EPA_TAB = EPA_MAXV * EPA_MAXV;
 
+constexpr uint32_t MESH_ENTRY = 0x80000000u;   // queue entry is a mesh triangle candidate, not a pair
+
 struct EpaBig {
-    uint32_t* list;   // queued pair indices
+    uint32_t* list;   // queued pair indices (or MESH_ENTRY | mesh candidate index)
     uint32_t cap;     // queue slots
     int faces_per;    // faces == horizon entries per slot
     Face* faces;
@@ -1158,6 +1163,7 @@ __global__ void __launch_bounds__(64) k_narrow_big(BodiesDev B, SimParams P, uin
     if (blockIdx.x == 0 && threadIdx.x == 0 && nbig) atomicAdd(&counters[CNT_EPARERUN], nbig);
     for (uint32_t t = blockIdx.x; t < nbig; t += gridDim.x) {
         const uint32_t p = big.list[t];
+        if (p & MESH_ENTRY) continue;   // a mesh triangle: k_mesh_big
         const size_t fo = (size_t)t * (size_t)big.faces_per;
         for (int k = threadIdx.x; k < EPA_TAB; k += blockDim.x) stamp[k] = 0;
         __syncthreads();
@@ -1207,6 +1213,266 @@ __global__ void __launch_bounds__(64) k_narrow_big(BodiesDev B, SimParams P, uin
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Static triangle meshes: src/mesh.hpp:14-93 (tree, query) and
+// meshContacts, src/world.cpp:56-83.
+//
+// All meshes share one set of arrays; indices inside are absolute.  Per step
+// every candidate pair with a mesh body queries the tree twice (count, then
+// write the triangles in the query's order), each reported triangle becomes
+// a candidate of its own for the narrow phase, and a pass per pair keeps the
+// hits in that order: the reference's per-pair contact vector.
+// ---------------------------------------------------------------------------
+
+struct MeshNode {
+    double lo[3], hi[3];
+    int32_t left, right, start, count;   // children (absolute) or a leaf's [start, start+count) into order
+};
+
+struct MeshDev {
+    const double* verts;      // 3 per vertex, body frame
+    const int32_t* tris;      // 3 per triangle, absolute vertex indices
+    const int32_t* order;     // absolute triangle indices in tree order
+    const MeshNode* nodes;
+    const int32_t* root;      // per mesh: root node, or -1 for an empty mesh
+    const int32_t* of_body;   // per body: mesh index, or -1
+};
+
+// The pair's mesh body and the other body, as gatherContacts picks them
+// (world.cpp:89-94): b's mesh when b is a mesh, else a's with normals flipped.
+__device__ __forceinline__ bool mesh_pair(const BodiesDev& B, uint32_t i, uint32_t j, uint32_t& o, uint32_t& m,
+                                          bool& flip) {
+    if (B.shape[j] == SH_MESH) { m = j; o = i; flip = false; return true; }
+    if (B.shape[i] == SH_MESH) { m = i; o = j; flip = true; return true; }
+    return false;
+}
+
+// MeshData::query (mesh.hpp:46-59) over the cube of half-side
+// boundingRadius + 0.05 about the other body's centre in the mesh frame:
+// depth first, right child popped first, children dropped once 62 entries
+// are held.
+template <class Emit>
+__device__ __forceinline__ uint32_t mesh_query(const BodiesDev& B, const SimParams& P, const MeshDev& M,
+                                               uint32_t o, uint32_t m, Emit emit) {
+    const int mi = M.of_body[m];
+    if (mi < 0) return 0;
+    const int root = M.root[mi];
+    if (root < 0) return 0;
+    const Box3& box = box_of(B, P, o);
+    const V3 rel = min_image(box, ld3(B.x0, B.x1, B.x2, o) - ld3(B.x0, B.x1, B.x2, m));
+    const V3 cl = qinvrot(ldq(B, m), rel);
+    const double rr = B.bR[o] + 0.05;
+    const V3 lo = cl - V3{rr, rr, rr}, hi = cl + V3{rr, rr, rr};
+    int stack[64];
+    int sp = 0;
+    stack[sp++] = root;
+    uint32_t n = 0;
+    while (sp > 0) {
+        const MeshNode& nd = M.nodes[stack[--sp]];
+        if (lo.x > nd.hi[0] || hi.x < nd.lo[0] || lo.y > nd.hi[1] || hi.y < nd.lo[1] || lo.z > nd.hi[2] ||
+            hi.z < nd.lo[2])
+            continue;
+        if (nd.left < 0) {
+            for (int k = 0; k < nd.count; ++k) emit(n++, M.order[nd.start + k]);
+        } else if (sp < 62) {
+            stack[sp++] = nd.left;
+            stack[sp++] = nd.right;
+        }
+    }
+    return n;
+}
+
+__global__ void __launch_bounds__(TPB) k_mesh_count(BodiesDev B, SimParams P, MeshDev M,
+                                                    const uint32_t* __restrict__ counters,
+                                                    const uint32_t* __restrict__ cand_i,
+                                                    const uint32_t* __restrict__ cand_j, uint32_t* __restrict__ mq_cnt) {
+    const uint32_t np = counters[CNT_P];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        uint32_t o, m;
+        bool flip;
+        uint32_t n = 0;
+        if (mesh_pair(B, cand_i[p], cand_j[p], o, m, flip)) n = mesh_query(B, P, M, o, m, [](uint32_t, int32_t) {});
+        mq_cnt[p] = n;
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_mesh_emit(BodiesDev B, SimParams P, MeshDev M,
+                                                   const uint32_t* __restrict__ counters,
+                                                   const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                   const uint32_t* __restrict__ mq_off, uint32_t mqcap,
+                                                   uint32_t* __restrict__ mq_pair, int32_t* __restrict__ mq_tri) {
+    const uint32_t np = counters[CNT_P];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        uint32_t o, m;
+        bool flip;
+        const uint32_t base = mq_off[p];
+        if (base == mq_off[p + 1] || !mesh_pair(B, cand_i[p], cand_j[p], o, m, flip)) continue;
+        mesh_query(B, P, M, o, m, [&](uint32_t k, int32_t t) {
+            if (base + k < mqcap) { mq_pair[base + k] = p; mq_tri[base + k] = t; }
+        });
+    }
+}
+
+// The triangle as a thin convex prism (world.cpp:66-79): world vertices,
+// unit normal, the vertices pushed 1e-3 along and against it; a temporary
+// body at the origin with identity orientation.  False for a degenerate
+// triangle (|n| < 1e-12), which the reference skips.
+__device__ __forceinline__ bool mesh_prism(const BodiesDev& B, const MeshDev& M, uint32_t m, int32_t t, double* pv,
+                                           Geom& g) {
+    const V3 xm = ld3(B.x0, B.x1, B.x2, m);
+    const Qt qm = ldq(B, m);
+    const int32_t* tv = M.tris + 3 * (size_t)t;
+    const double* a = M.verts + 3 * (size_t)tv[0];
+    const double* b = M.verts + 3 * (size_t)tv[1];
+    const double* c = M.verts + 3 * (size_t)tv[2];
+    const V3 w0 = xm + qrot(qm, V3{a[0], a[1], a[2]});
+    const V3 w1 = xm + qrot(qm, V3{b[0], b[1], b[2]});
+    const V3 w2 = xm + qrot(qm, V3{c[0], c[1], c[2]});
+    V3 n = cross(w1 - w0, w2 - w0);
+    const double nl = norm(n);
+    if (nl < 1e-12) return false;
+    n = n / nl;
+    const double thick = 1e-3;
+    const V3 up = n * thick;
+    const V3 v[6] = {w0 + up, w1 + up, w2 + up, w0 - up, w1 - up, w2 - up};
+    for (int k = 0; k < 6; ++k) { pv[3 * k] = v[k].x; pv[3 * k + 1] = v[k].y; pv[3 * k + 2] = v[k].z; }
+    g.x = V3{0.0, 0.0, 0.0};
+    g.q = Qt{1.0, 0.0, 0.0, 0.0};
+    g.he = V3{0.0, 0.0, 0.0};
+    g.radius = 1.0;
+    g.hh = 0.0;
+    g.hull = pv;
+    g.nhull = 6;
+    g.shape = SH_CONVEX;
+    return true;
+}
+
+__device__ __forceinline__ void store_contact(double* out, const Contact& c, bool flip) {
+    out[0] = flip ? -c.normal.x : c.normal.x;
+    out[1] = flip ? -c.normal.y : c.normal.y;
+    out[2] = flip ? -c.normal.z : c.normal.z;
+    out[3] = c.point.x; out[4] = c.point.y; out[5] = c.point.z; out[6] = c.overlap;
+}
+
+// detectContact(other, prism) per triangle candidate (GJK + EPA with the
+// per-thread workspace); EPA runs that outgrow it go to the shared
+// large-workspace queue (k_mesh_big).
+__global__ void __launch_bounds__(TPB) k_mesh_narrow(BodiesDev B, SimParams P, MeshDev M, uint32_t* __restrict__ counters,
+                                                     const uint32_t* __restrict__ cand_i,
+                                                     const uint32_t* __restrict__ cand_j,
+                                                     const uint32_t* __restrict__ mq_pair,
+                                                     const int32_t* __restrict__ mq_tri, uint8_t* __restrict__ mhit,
+                                                     double* __restrict__ mct, EpaBig big) {
+    const uint32_t nq = counters[CNT_MQ];
+    uint32_t* fault = counters + CNT_FAULT;
+    SupV verts[EPA_MAXV];
+    Face faces[EPA_MAXF];
+    int hor[EPA_MAXH][2];
+    EpaWS ws{};
+    ws.verts = verts; ws.faces = faces; ws.maxf = EPA_MAXF; ws.big = false;
+    ws.list.hor = hor; ws.list.cap = EPA_MAXH;
+    double pv[18];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        const uint32_t p = mq_pair[q];
+        uint32_t o, m;
+        bool flip;
+        mesh_pair(B, cand_i[p], cand_j[p], o, m, flip);
+        Geom tri;
+        uint8_t hit = 0;
+        if (mesh_prism(B, M, m, mq_tri[q], pv, tri)) {
+            ws.overflow = false;
+            const Geom other = load_geom(B, o);
+            const Contact c = detect_contact(box_of(B, P, o), other, tri, ws);
+            if (ws.overflow) {
+                const uint32_t slot = atomicAdd(&counters[CNT_EPABIG], 1u);
+                if (slot < big.cap) big.list[slot] = MESH_ENTRY | q;
+                else atomicOr(fault, NE_FAULT_EPA_QUEUE);
+            } else if (c.hit) {
+                store_contact(mct + 7 * (size_t)q, c, flip);
+                hit = 1;
+            }
+        }
+        mhit[q] = hit;
+    }
+}
+
+// The queued triangle candidates with the wave-cooperative EPA (as
+// k_narrow_big does for pairs).
+__global__ void __launch_bounds__(64) k_mesh_big(BodiesDev B, SimParams P, MeshDev M, uint32_t* __restrict__ counters,
+                                                 const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                 const uint32_t* __restrict__ mq_pair, const int32_t* __restrict__ mq_tri,
+                                                 uint8_t* __restrict__ mhit, double* __restrict__ mct, EpaBig big) {
+    __shared__ SupV verts[EPA_MAXV];
+    __shared__ int head[EPA_TAB], tail[EPA_TAB], stamp[EPA_TAB];
+    __shared__ int stage[3 * 64];
+    __shared__ int shared[4];
+    const uint32_t nbig = min(counters[CNT_EPABIG], big.cap);
+    uint32_t* fault = counters + CNT_FAULT;
+    double pv[18];
+    for (uint32_t t = blockIdx.x; t < nbig; t += gridDim.x) {
+        const uint32_t e = big.list[t];
+        if (!(e & MESH_ENTRY)) continue;
+        const uint32_t q = e & ~MESH_ENTRY;
+        if (threadIdx.x == 0) atomicAdd(&counters[CNT_EPARERUN], 1u);
+        const size_t fo = (size_t)t * (size_t)big.faces_per;
+        for (int k = threadIdx.x; k < EPA_TAB; k += blockDim.x) stamp[k] = 0;
+        __syncthreads();
+        HorizonTable hz;
+        hz.hor = big.hor + fo; hz.next = big.next + fo; hz.live = big.live + fo;
+        hz.head = head; hz.tail = tail; hz.stamp = stamp;
+        hz.cap = big.faces_per; hz.n = 0; hz.nlive = 0; hz.it = 0;
+        const bool truncate = big.faces_per >= EPA_BIG_FACES_MAX;
+        const uint32_t p = mq_pair[q];
+        uint32_t o, m;
+        bool flip;
+        mesh_pair(B, cand_i[p], cand_j[p], o, m, flip);
+        Geom tri;
+        mesh_prism(B, M, m, mq_tri[q], pv, tri);   // non-degenerate: it was queued
+        const Geom a = load_geom(B, o);
+        const Box3& box = box_of(B, P, o);
+        const V3 shift = min_image_shift(box, a.x, tri.x);
+        SupV tet[4];
+        bool overflow = false;
+        Contact c = no_contact();
+        if (gjk_tetra(a, tri, shift, tet))
+            c = epa_wave(a, tri, shift, tet, verts, big.faces + fo, big.faces_per, hz, stage, shared, truncate, overflow);
+        uint8_t hit = 0;
+        if (overflow && !truncate) {
+            if (threadIdx.x == 0) atomicOr(fault, NE_FAULT_EPA_BIG);
+        } else if (c.hit) {
+            if (overflow && threadIdx.x == 0) atomicAdd(&counters[CNT_EPATRUNC], 1u);
+            if (threadIdx.x == 0) store_contact(mct + 7 * (size_t)q, c, flip);
+            hit = 1;
+        }
+        if (threadIdx.x == 0) mhit[q] = hit;
+        __syncthreads();
+    }
+}
+
+// Per mesh pair: its hits, in query order, become the pair's contacts.
+__global__ void __launch_bounds__(TPB) k_mesh_gather(BodiesDev B, const uint32_t* __restrict__ counters,
+                                                     const uint32_t* __restrict__ cand_i,
+                                                     const uint32_t* __restrict__ cand_j,
+                                                     const uint32_t* __restrict__ mq_off, uint32_t mqcap,
+                                                     const uint8_t* __restrict__ mhit, double* __restrict__ mct,
+                                                     uint32_t* __restrict__ ncont) {
+    const uint32_t np = counters[CNT_P];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        const uint32_t b0 = mq_off[p], b1 = min(mq_off[p + 1], mqcap);
+        if (b0 >= b1) continue;
+        uint32_t h = 0;
+        for (uint32_t q = b0; q < b1; ++q) {
+            if (!mhit[q]) continue;
+            const uint32_t d = b0 + h;
+            if (d != q)
+                for (int k = 0; k < 7; ++k) mct[7 * (size_t)d + k] = mct[7 * (size_t)q + k];
+            ++h;
+        }
+        ncont[p] = h;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Constraint build: src/world.cpp:522-553 (+ effMass once, contact.hpp:28-33)
 // ---------------------------------------------------------------------------
@@ -1229,7 +1495,8 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
                                                     const double* __restrict__ ct, ConsDev C, uint32_t ccap,
                                                     const uint64_t* __restrict__ prev_key, const double* __restrict__ prev_imp,
                                                     const uint32_t* __restrict__ prev_n, uint32_t* __restrict__ parent,
-                                                    uint32_t* __restrict__ claimed) {
+                                                    uint32_t* __restrict__ claimed, const uint32_t* __restrict__ mq_off,
+                                                    const double* __restrict__ mct) {
     const uint32_t np = counters_in[CNT_P];
     const uint32_t pn = *prev_n;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
@@ -1264,10 +1531,11 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
             }
         }
         const uint32_t k0 = cons_off[p];
+        const bool meshp = mct && (B.shape[i] == SH_MESH || B.shape[j] == SH_MESH);
         for (uint32_t s = 0; s < nc; ++s) {
             const uint32_t k = k0 + s;
             if (k >= ccap) break;
-            const double* c = ct + (size_t)p * 28 + 7 * s;
+            const double* c = meshp ? mct + 7 * ((size_t)mq_off[p] + s) : ct + (size_t)p * 28 + 7 * s;
             const V3 n{c[0], c[1], c[2]}, pt{c[3], c[4], c[5]};
             const V3 rA = min_image(box, pt - xa);
             const V3 rB = min_image(box, pt - xb);
@@ -2377,6 +2645,73 @@ __global__ void __launch_bounds__(TPB) k_fill_f64(double* __restrict__ p, size_t
 // Host side
 // ===========================================================================
 
+// A mesh's bounding-volume tree as MeshData::build makes it (src/mesh.hpp:
+// 38-43, buildNode 62-92): nodes in creation order (parent, then the left
+// subtree, then the right); a node bounds its triangles' boxes; at most 4
+// triangles make a leaf; otherwise the triangles are split at the middle
+// index along the widest axis of the node box (x when strictly widest, else
+// y when wider than z, else z) by std::nth_element on the sum of each
+// triangle box's low and high coordinate on that axis -- the same library
+// routine and comparison as the reference, so the halves, the leaves and the
+// query's triangle order come out the same.
+struct MeshTree {
+    const double* v = nullptr;
+    const int32_t* t = nullptr;
+    std::vector<int32_t> order;
+    std::vector<double> bounds;   // 6 per node: lo xyz, hi xyz
+    std::vector<int32_t> links;   // 4 per node: left, right, start, count (relative)
+
+    void tri_box(int32_t tri, double lo[3], double hi[3]) const {
+        const double* a = v + 3 * (size_t)t[3 * (size_t)tri];
+        const double* b = v + 3 * (size_t)t[3 * (size_t)tri + 1];
+        const double* c = v + 3 * (size_t)t[3 * (size_t)tri + 2];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min({a[k], b[k], c[k]});
+            hi[k] = std::max({a[k], b[k], c[k]});
+        }
+    }
+
+    int32_t build(int32_t start, int32_t count) {
+        const int32_t idx = (int32_t)(links.size() / 4);
+        bounds.resize(bounds.size() + 6);
+        links.resize(links.size() + 4);
+        double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
+        for (int32_t i = 0; i < count; ++i) {
+            double tl[3], th[3];
+            tri_box(order[(size_t)start + i], tl, th);
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], tl[k]);
+                hi[k] = std::max(hi[k], th[k]);
+            }
+        }
+        int32_t left = -1, right = -1, leaf_start = 0, leaf_count = 0;
+        if (count <= 4) {
+            leaf_start = start;
+            leaf_count = count;
+        } else {
+            const double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+            const int axis = (ex > ey && ex > ez) ? 0 : (ey > ez ? 1 : 2);
+            const int32_t mid = start + count / 2;
+            std::nth_element(order.begin() + start, order.begin() + mid, order.begin() + start + count,
+                             [&](int32_t A, int32_t Bt) {
+                                 double al[3], ah[3], bl[3], bh[3];
+                                 tri_box(A, al, ah);
+                                 tri_box(Bt, bl, bh);
+                                 return al[axis] + ah[axis] < bl[axis] + bh[axis];
+                             });
+            left = build(start, mid - start);
+            right = build(mid, start + count - mid);
+        }
+        for (int k = 0; k < 3; ++k) {
+            bounds[6 * (size_t)idx + k] = lo[k];
+            bounds[6 * (size_t)idx + 3 + k] = hi[k];
+        }
+        int32_t* l = &links[4 * (size_t)idx];
+        l[0] = left; l[1] = right; l[2] = leaf_start; l[3] = leaf_count;
+        return idx;
+    }
+};
+
 struct ne_gpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -2476,6 +2811,20 @@ struct ne_gpu_ctx {
     uint32_t *j_ab = nullptr, *j_comp = nullptr;   // a[n], b[n]; comp_start[ncomp+1], comp_list[n]
     double* j_f = nullptr;                         // la0..2, lb0..2, rest, stiff, damp (9 x n)
     std::vector<uint8_t> h_mut;                    // bodies that can move (im != 0 or ii != 0)
+    // static triangle meshes (all meshes in shared arrays, absolute indices)
+    uint32_t nmeshes = 0;
+    double* m_verts = nullptr;
+    int32_t *m_tris = nullptr, *m_order = nullptr, *m_root = nullptr, *m_of_body = nullptr;
+    MeshNode* m_nodes = nullptr;
+    std::vector<MeshTree> h_trees;                 // per mesh, relative indices (for ne_gpu_mesh_bvh)
+    std::vector<int32_t> h_shape;                  // body shapes as uploaded
+    std::vector<uint32_t> h_tris_n;
+    // per step: triangle candidates of the mesh pairs
+    uint32_t *mq_cnt = nullptr, *mq_off = nullptr, *mq_pair = nullptr;
+    int32_t* mq_tri = nullptr;
+    uint8_t* mhit = nullptr;
+    double* mct = nullptr;                         // 7 doubles per candidate: normal, point, depth
+    uint32_t mqcap = 0, mq_pcap = 0, hint_mq = 4096;
     // rollout checkpoints and adjoint work space
     struct Saved {
         double* state = nullptr;   // x, v, q, w fields (13 x ncap)
@@ -2594,6 +2943,7 @@ double host_bounding_radius(int shape, double r, double hh, const double* he, co
         case NE_SHAPE_BOX: return nrm(he[0], he[1], he[2]);
         case NE_SHAPE_CAPSULE: return hh + r;
         case NE_SHAPE_PLANE: return 1e30;
+        case NE_SHAPE_MESH: return 1e30;
         case NE_SHAPE_CONVEX: {
             double m = 0.0;
             for (uint32_t k = h0; k < h1; ++k) m = std::max(m, nrm(hull[3 * k], hull[3 * k + 1], hull[3 * k + 2]));
@@ -2651,6 +3001,8 @@ uint32_t gs_blocks(uint64_t n) {   // grid-stride launches over device-side coun
     return (uint32_t)b;
 }
 
+int alloc_mesh_pairs(ne_gpu_ctx* c);
+
 int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
     drop_graphs(c);
     c->pcap = pcap;
@@ -2663,6 +3015,7 @@ int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
     HIP_TRY(c, dalloc(c->ct, (size_t)pcap * 28));
     ScanPlan sp = scan_plan(std::max(pcap + 1, std::max(c->ncap * 4u, c->cellcap + 1)));
     (void)sp;
+    if (c->nmeshes) return alloc_mesh_pairs(c);
     return NE_OK;
 }
 
@@ -2686,6 +3039,48 @@ int alloc_cons(ne_gpu_ctx* c, uint32_t ccap) {
     HIP_TRY(c, dalloc(c->isl_fill, (size_t)ccap + 1));
     HIP_TRY(c, dalloc(c->isl_start, (size_t)ccap + 1));
     return NE_OK;
+}
+
+int alloc_mesh_pairs(ne_gpu_ctx* c) {
+    if (c->mq_pcap >= c->pcap && c->mq_cnt) return NE_OK;
+    drop_graphs(c);
+    HIP_TRY(c, dalloc(c->mq_cnt, (size_t)c->pcap + 1));
+    HIP_TRY(c, dalloc(c->mq_off, (size_t)c->pcap + 1));
+    c->mq_pcap = c->pcap;
+    return NE_OK;
+}
+
+int alloc_mesh_cands(ne_gpu_ctx* c, uint32_t cap) {
+    drop_graphs(c);
+    c->mqcap = cap;
+    HIP_TRY(c, dalloc(c->mq_pair, cap));
+    HIP_TRY(c, dalloc(c->mq_tri, cap));
+    HIP_TRY(c, dalloc(c->mhit, cap));
+    HIP_TRY(c, dalloc(c->mct, (size_t)cap * 7));
+    return NE_OK;
+}
+
+void free_meshes(ne_gpu_ctx* c) {
+    dfree(c->m_verts); dfree(c->m_tris); dfree(c->m_order); dfree(c->m_root); dfree(c->m_of_body); dfree(c->m_nodes);
+    c->nmeshes = 0;
+    c->h_trees.clear();
+    c->h_tris_n.clear();
+}
+
+void free_mesh_work(ne_gpu_ctx* c) {
+    dfree(c->mq_cnt); dfree(c->mq_off); dfree(c->mq_pair); dfree(c->mq_tri); dfree(c->mhit); dfree(c->mct);
+    c->mqcap = c->mq_pcap = 0;
+}
+
+MeshDev mesh_dev(const ne_gpu_ctx* c) {
+    MeshDev M;
+    M.verts = c->m_verts;
+    M.tris = c->m_tris;
+    M.order = c->m_order;
+    M.nodes = c->m_nodes;
+    M.root = c->m_root;
+    M.of_body = c->m_of_body;
+    return M;
 }
 
 int alloc_epa_big(ne_gpu_ctx* c, uint32_t cap, int faces_per) {
@@ -2773,6 +3168,8 @@ void free_adjoint(ne_gpu_ctx* c) {
 void free_all(ne_gpu_ctx* c) {
     free_adjoint(c);
     free_work(c);
+    free_meshes(c);
+    free_mesh_work(c);
     dfree(c->prev_key); dfree(c->prev_imp); dfree(c->prev_n); dfree(c->counters);
     dfree(c->snap_body); dfree(c->snap_key); dfree(c->snap_imp); dfree(c->snap_n); dfree(c->snap_flags);
     c->snap_nd = 0;
@@ -2900,6 +3297,20 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
                                                          c->epa_q, c->pcap, c->epa_big);
             k_narrow_big<<<16, 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_big);
         }
+        if (c->nmeshes) {   // meshContacts (world.cpp:56-83)
+            const MeshDev M = mesh_dev(c);
+            k_mesh_count<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_cnt);
+            scan_u32(c->mq_cnt, c->mq_off, c->counters + CNT_P, c->pcap, c->scan_part, s, false,
+                     ScanTotal{c->counters, CNT_MQRAW, CNT_MQ, CNT_FAULT, c->mqcap, NE_FAULT_MESH, -1, -1});
+            k_mesh_emit<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_off,
+                                                               c->mqcap, c->mq_pair, c->mq_tri);
+            k_mesh_narrow<<<hint_blocks(c->hint_mq), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
+                                                                  c->mq_tri, c->mhit, c->mct, c->epa_big);
+            k_mesh_big<<<16, 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri, c->mhit,
+                                         c->mct, c->epa_big);
+            k_mesh_gather<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, c->counters, c->cand_i, c->cand_j, c->mq_off,
+                                                                 c->mqcap, c->mhit, c->mct, c->ncont);
+        }
     }
     if (P.sleep) {   // wakes in pair order (world.cpp:527-531), then the woken bodies' masses
         k_sleep_reset<<<blocks_for(n), TPB, 0, s>>>(B, c->wake_at, c->wake_again);
@@ -2916,7 +3327,8 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
         KTimer kt(c, NE_KERNEL_CONS_BUILD);
         k_cons_build<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->cons_off,
                                                         c->ct, c->C, c->ccap, c->prev_key, c->prev_imp, c->prev_n,
-                                                        c->parent, c->claimed);
+                                                        c->parent, c->claimed, c->nmeshes ? c->mq_off : nullptr,
+                                                        c->nmeshes ? c->mct : nullptr);
     }
     // --- islands (linked inside k_cons_build)
     k_isl_claim<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, c->parent, c->cons_root, c->claimed,
@@ -3226,8 +3638,8 @@ int ne_gpu_rollout(ne_gpu_ctx* c, const double* v0, const double* mu, const doub
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
     if (c->n == 0) return set_err(c, NE_ERR_INVALID, "no bodies");
     if (!c->params.periodic) return set_err(c, NE_ERR_UNSUPPORTED, "rollouts are differentiated for periodic worlds");
-    if (c->has_forces || c->njoints || c->params.sleep_enabled)
-        return set_err(c, NE_ERR_UNSUPPORTED, "rollouts do not take applied forces, joints or sleeping");
+    if (c->has_forces || c->njoints || c->params.sleep_enabled || c->nmeshes)
+        return set_err(c, NE_ERR_UNSUPPORTED, "rollouts do not take applied forces, joints, sleeping or meshes");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const uint32_t nw = c->nworlds ? c->nworlds : 1u;
     if (mu || e) {
@@ -3638,6 +4050,7 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps) {
     harvest_timing(c);
     c->hint_p = c->h_counters[CNT_PRAW];
     c->hint_c = c->h_counters[CNT_CRAW];
+    if (c->nmeshes) c->hint_mq = c->h_counters[CNT_MQRAW];
     const uint32_t fault = c->h_counters[CNT_FAULT];
     if (fault) {
         // grow what overflowed so a re-upload + retry succeeds; state is undefined
@@ -3653,6 +4066,11 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps) {
                 slots = std::max(slots * 2u, c->h_counters[CNT_EPABIG] + c->h_counters[CNT_EPABIG] / 2u + 16u);
             if (fault & NE_FAULT_EPA_BIG) per = std::min(per * 4, EPA_BIG_FACES_MAX);
             int r = alloc_epa_big(c, slots, per);
+            if (r != NE_OK) return r;
+        }
+        if (fault & NE_FAULT_MESH) {
+            uint32_t want = std::max(c->mqcap * 2u, c->h_counters[CNT_MQRAW] + c->h_counters[CNT_MQRAW] / 2u + 1024u);
+            int r = alloc_mesh_cands(c, want);
             if (r != NE_OK) return r;
         }
         if (fault & NE_FAULT_CONTACTS) {
@@ -3719,6 +4137,100 @@ int ne_gpu_get_body_aux(ne_gpu_ctx* c, uint32_t* flags, double* sleep_timer, dou
             HIP_TRY(c, hipMemcpy(tmp.data(), c->dfield + (size_t)f[k] * c->ncap, sizeof(double) * n, hipMemcpyDeviceToHost));
             for (uint32_t i = 0; i < n; ++i) inv_inertia[3 * i + k] = tmp[i];
         }
+    }
+    return NE_OK;
+}
+
+int ne_gpu_set_meshes(ne_gpu_ctx* c, uint32_t n, const ne_mesh* ms) {
+    if (!c || (n && !ms)) return NE_ERR_INVALID;
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    uint64_t tv = 0, tt = 0;
+    for (uint32_t m = 0; m < n; ++m) {
+        const ne_mesh& e = ms[m];
+        if (e.body >= c->n || c->h_shape[e.body] != NE_SHAPE_MESH)
+            return set_err(c, NE_ERR_INVALID, "mesh attached to a body that is not a mesh body");
+        if ((e.nverts && !e.verts) || (e.ntris && !e.tris)) return set_err(c, NE_ERR_INVALID, "mesh array missing");
+        for (uint64_t k = 0; k < 3ull * e.ntris; ++k)
+            if (e.tris[k] < 0 || (uint32_t)e.tris[k] >= e.nverts)
+                return set_err(c, NE_ERR_INVALID, "mesh triangle vertex index out of range");
+        tv += e.nverts;
+        tt += e.ntris;
+    }
+    if (tv >= (1ull << 30) || tt >= (1ull << 29)) return set_err(c, NE_ERR_INVALID, "meshes too large");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    free_meshes(c);
+    if (n == 0) return NE_OK;
+    std::vector<double> hv;
+    std::vector<int32_t> ht, ho, hroot(n, -1), hof(c->n, -1);
+    std::vector<MeshNode> hn;
+    hv.reserve(3 * tv);
+    ht.reserve(3 * tt);
+    ho.reserve(tt);
+    c->h_trees.resize(n);
+    c->h_tris_n.resize(n);
+    for (uint32_t m = 0; m < n; ++m) {
+        const ne_mesh& e = ms[m];
+        MeshTree& T = c->h_trees[m];
+        T.v = e.verts;
+        T.t = e.tris;
+        T.order.resize(e.ntris);
+        for (uint32_t k = 0; k < e.ntris; ++k) T.order[k] = (int32_t)k;
+        if (e.ntris) T.build(0, (int32_t)e.ntris);
+        T.v = nullptr;
+        T.t = nullptr;
+        c->h_tris_n[m] = e.ntris;
+        const int32_t vbase = (int32_t)(hv.size() / 3), tbase = (int32_t)(ht.size() / 3), nbase = (int32_t)hn.size();
+        hv.insert(hv.end(), e.verts, e.verts + 3 * (size_t)e.nverts);
+        for (uint64_t k = 0; k < 3ull * e.ntris; ++k) ht.push_back(e.tris[k] + vbase);
+        for (int32_t t : T.order) ho.push_back(t + tbase);
+        const size_t nn = T.links.size() / 4;
+        for (size_t k = 0; k < nn; ++k) {
+            MeshNode d;
+            for (int a = 0; a < 3; ++a) { d.lo[a] = T.bounds[6 * k + a]; d.hi[a] = T.bounds[6 * k + 3 + a]; }
+            const int32_t* l = &T.links[4 * k];
+            d.left = l[0] < 0 ? -1 : l[0] + nbase;
+            d.right = l[1] < 0 ? -1 : l[1] + nbase;
+            d.start = l[0] < 0 ? l[2] + tbase : 0;
+            d.count = l[3];
+            hn.push_back(d);
+        }
+        hroot[m] = nn ? nbase : -1;
+        hof[e.body] = (int32_t)m;
+    }
+    HIP_TRY(c, dalloc(c->m_verts, std::max<size_t>(hv.size(), 1)));
+    HIP_TRY(c, dalloc(c->m_tris, std::max<size_t>(ht.size(), 1)));
+    HIP_TRY(c, dalloc(c->m_order, std::max<size_t>(ho.size(), 1)));
+    HIP_TRY(c, dalloc(c->m_nodes, std::max<size_t>(hn.size(), 1)));
+    HIP_TRY(c, dalloc(c->m_root, n));
+    HIP_TRY(c, dalloc(c->m_of_body, std::max<uint32_t>(c->n, 1u)));
+    if (!hv.empty()) HIP_TRY(c, hipMemcpy(c->m_verts, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice));
+    if (!ht.empty()) HIP_TRY(c, hipMemcpy(c->m_tris, ht.data(), sizeof(int32_t) * ht.size(), hipMemcpyHostToDevice));
+    if (!ho.empty()) HIP_TRY(c, hipMemcpy(c->m_order, ho.data(), sizeof(int32_t) * ho.size(), hipMemcpyHostToDevice));
+    if (!hn.empty()) HIP_TRY(c, hipMemcpy(c->m_nodes, hn.data(), sizeof(MeshNode) * hn.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->m_root, hroot.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    if (c->n) HIP_TRY(c, hipMemcpy(c->m_of_body, hof.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice));
+    c->nmeshes = n;
+    int r = alloc_mesh_pairs(c);
+    if (r != NE_OK) return r;
+    if (c->mqcap == 0) {
+        r = alloc_mesh_cands(c, 8u * c->n + 4096u);
+        if (r != NE_OK) return r;
+    }
+    return NE_OK;
+}
+
+int ne_gpu_mesh_bvh(ne_gpu_ctx* c, uint32_t m, int32_t* order, uint32_t node_cap, uint32_t* nnodes, double* bounds,
+                    int32_t* links) {
+    if (!c) return NE_ERR_INVALID;
+    if (m >= c->nmeshes) return set_err(c, NE_ERR_INVALID, "no such mesh");
+    const MeshTree& T = c->h_trees[m];
+    const uint32_t nn = (uint32_t)(T.links.size() / 4);
+    if (nnodes) *nnodes = nn;
+    if (order) std::copy(T.order.begin(), T.order.end(), order);
+    if (bounds && links && node_cap >= nn) {
+        std::copy(T.bounds.begin(), T.bounds.end(), bounds);
+        std::copy(T.links.begin(), T.links.end(), links);
     }
     return NE_OK;
 }
@@ -4067,8 +4579,11 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     if (n >= (1u << 30)) return set_err(c, NE_ERR_INVALID, "too many bodies");
     for (uint32_t i = 0; i < n; ++i) {
         const int s = b->shape[i];
-        if (s < 0 || s > NE_SHAPE_PLANE)
-            return set_err(c, NE_ERR_UNSUPPORTED, "mesh/compound shapes are not on the GPU path yet");
+        if (s < 0 || s > NE_SHAPE_MESH)
+            return set_err(c, NE_ERR_UNSUPPORTED, "compound shapes are not on the GPU path");
+        if (s == NE_SHAPE_MESH && (b->inv_mass[i] != 0.0 || b->inv_inertia[3 * i] != 0.0 ||
+                                   b->inv_inertia[3 * i + 1] != 0.0 || b->inv_inertia[3 * i + 2] != 0.0))
+            return set_err(c, NE_ERR_UNSUPPORTED, "mesh bodies must be static (zero inverse mass and inertia)");
         const uint32_t fl = b->flags ? b->flags[i] : NE_FLAG_DYNAMIC;
         if (fl & NE_FLAG_CCD) return set_err(c, NE_ERR_UNSUPPORTED, "CCD is not on the GPU path");
         if (s == NE_SHAPE_CONVEX && (!b->hull_offsets || b->hull_offsets[i + 1] < b->hull_offsets[i]))
@@ -4194,6 +4709,8 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     c->h_isexcl.assign(n, 0);
     for (uint32_t e : excl) c->h_isexcl[e] = 1;
     c->nworlds = 0;   // a new body set starts as one world
+    free_meshes(c);   // and without meshes
+    c->h_shape.assign(b->shape, b->shape + n);
     c->njoints = c->nspring = c->nrigid = c->ncomp = 0;   // and without joints
     c->h_mut.assign(n, 0);
     for (uint32_t i = 0; i < n; ++i)

@@ -34,6 +34,7 @@ enum : uint32_t {
     NE_FAULT_EPA_QUEUE = 16u, // more large-workspace EPA reruns than queue slots
     NE_FAULT_EPA_BIG = 32u,   // an EPA rerun outgrew the large workspace
     NE_FAULT_WAKE = 64u,      // a chain of sleeping bodies woken one by another outran the wake rounds
+    NE_FAULT_MESH = 128u,     // mesh triangle candidate buffer overflow
 };
 
 // Geometry of one body as the narrow phase sees it.

@@ -26,7 +26,7 @@ EXPORTS = [
     "ne_gpu_get_pairs", "ne_gpu_diagnostics", "ne_gpu_get_counters", "ne_gpu_get_stream", "ne_gpu_last_error",
     "ne_gpu_fp_probe", "ne_gpu_time_kernel", "ne_gpu_kernel_time", "ne_gpu_export_warm_cache",
     "ne_gpu_import_warm_cache", "ne_gpu_set_worlds", "ne_gpu_use_graph", "ne_gpu_autotune", "ne_gpu_epa_stats",
-    "ne_gpu_rollout", "ne_gpu_rollout_backward", "ne_gpu_set_joints",
+    "ne_gpu_rollout", "ne_gpu_rollout_backward", "ne_gpu_set_joints", "ne_gpu_set_meshes", "ne_gpu_mesh_bvh",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
@@ -58,6 +58,8 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_step.argtypes = [vp, C.c_int32]
     lib.ne_gpu_get_state.argtypes = [vp, dp, dp, dp, dp]
     lib.ne_gpu_get_body_aux.argtypes = [vp, u32p, dp, dp, dp]
+    lib.ne_gpu_set_meshes.argtypes = [vp, C.c_uint32, C.POINTER(_abi.Mesh)]
+    lib.ne_gpu_mesh_bvh.argtypes = [vp, C.c_uint32, C.POINTER(C.c_int32), C.c_uint32, u32p, dp, C.POINTER(C.c_int32)]
     lib.ne_gpu_reset_warm_cache.argtypes = [vp]
     lib.ne_gpu_get_contacts.argtypes = [vp, C.c_uint32, u32p, u32p, u32p, dp, dp, dp, dp]
     lib.ne_gpu_get_pairs.argtypes = [vp, C.c_uint32, u32p, u32p, u32p]
@@ -131,6 +133,22 @@ class GpuWorld:
         del keep
         if len(s.joints):
             self.set_joints(s.joints)
+        if s.meshes:
+            arr, keep = s.meshes_c()
+            self._check(self.lib.ne_gpu_set_meshes(self.h, len(s.meshes), arr))
+            del keep
+
+    def mesh_bvh(self, m, ntris):
+        """(order, bounds (nn, 6), links (nn, 4)) of mesh m's tree, as built here."""
+        nn = C.c_uint32(0)
+        self._check(self.lib.ne_gpu_mesh_bvh(self.h, m, None, 0, C.byref(nn), None, None))
+        order = np.zeros(ntris, np.int32)
+        bounds = np.zeros((nn.value, 6))
+        links = np.zeros((nn.value, 4), np.int32)
+        ip = C.POINTER(C.c_int32)
+        self._check(self.lib.ne_gpu_mesh_bvh(self.h, m, order.ctypes.data_as(ip), nn.value, C.byref(nn),
+                                             _abi.dptr(bounds), links.ctypes.data_as(ip)))
+        return order, bounds, links
 
     def set_joints(self, joints):
         """World::distanceJoints: a JOINT_DTYPE record array, in order."""

@@ -98,6 +98,8 @@ class Scene:
         self.nocollide_offsets = np.zeros(n + 1, np.uint32)
         self.nocollide = np.zeros(0, np.uint32)
         self.joints = np.zeros(0, _abi.JOINT_DTYPE)   # World::distanceJoints, in order
+        # static triangle meshes: dicts {body, verts (nv, 3) f64, tris (nt, 3) i32} (Body::mesh)
+        self.meshes = []
         # Body::sleepTimer / invMassStore / invIStore (body.hpp); None uploads zeros
         self.sleep_timer = None
         self.inv_mass_store = None
@@ -137,6 +139,7 @@ class Scene:
         s = Scene(0)
         for k, v in self.__dict__.items():
             setattr(s, k, v.copy() if isinstance(v, np.ndarray) else v)
+        s.meshes = [dict(m) for m in self.meshes]
         s.params = Params(**self.params.__dict__)
         return s
 
@@ -182,6 +185,21 @@ class Scene:
         keep.append(nc)
         b.nocollide = _abi.uptr(nc)
         return b, keep
+
+    def meshes_c(self):
+        """(ne_mesh array, keepalive list) for the scene's meshes."""
+        keep = []
+        arr = (_abi.Mesh * max(1, len(self.meshes)))()
+        for k, m in enumerate(self.meshes):
+            v = np.ascontiguousarray(m["verts"], np.float64).reshape(-1, 3)
+            t = np.ascontiguousarray(m["tris"], np.int32).reshape(-1, 3)
+            keep += [v, t]
+            arr[k].body = int(m["body"])
+            arr[k].nverts = len(v)
+            arr[k].verts = _abi.dptr(v)
+            arr[k].ntris = len(t)
+            arr[k].tris = _abi.iptr(t)
+        return arr, keep
 
 
 # ---------------------------------------------------------------------------
@@ -748,4 +766,150 @@ def wake_chain(chain: int = 6, rows: int = 3, seed: int = 0, drift: float = 1.0)
     s.params.gravity = (0.0, 0.0, 0.0)
     s.params.half = (8.0, 8.0, 8.0)
     s.params.sleep_enabled = True
+    return s
+
+
+# ---------------------------------------------------------------------------
+# Heightfields (physics_world.hpp:96-109) and scenes on them
+# ---------------------------------------------------------------------------
+
+def value_noise(rows: int, cols: int, seed: int = 0, octaves: int = 4, amplitude: float = 1.0,
+                base_cells: int = 4) -> np.ndarray:
+    """Seeded value noise on a rows x cols grid: `octaves` layers of smoothly
+    interpolated random lattice values, each of twice the frequency and half
+    the amplitude of the last; the sum is scaled to [-amplitude, amplitude]."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((rows, cols))
+    amp, cells, total = 1.0, base_cells, 0.0
+    for _ in range(octaves):
+        lat = rng.uniform(-1.0, 1.0, (cells + 2, cells + 2))
+        u = np.linspace(0.0, cells, rows)
+        v = np.linspace(0.0, cells, cols)
+        iu, iv = np.minimum(u.astype(int), cells - 1), np.minimum(v.astype(int), cells - 1)
+        fu, fv = u - iu, v - iv
+        su, sv = fu * fu * (3 - 2 * fu), fv * fv * (3 - 2 * fv)
+        a = lat[iu][:, iv]
+        b = lat[iu + 1][:, iv]
+        c = lat[iu][:, iv + 1]
+        d = lat[iu + 1][:, iv + 1]
+        top = a + (b - a) * su[:, None]
+        bot = c + (d - c) * su[:, None]
+        out += amp * (top + (bot - top) * sv[None, :])
+        total += amp
+        amp *= 0.5
+        cells *= 2
+    return out * (amplitude / total)
+
+
+def heightfield_mesh(origin, spacing: float, heights: np.ndarray):
+    """Vertices and triangles of PhysicsWorld::addHeightfield
+    (physics_world.hpp:96-109): vertex (r, c) at origin + (c*spacing,
+    heights[r, c], r*spacing), two triangles per cell, (i00, i10, i11) and
+    (i00, i11, i01)."""
+    rows, cols = heights.shape
+    o = np.asarray(origin, np.float64)
+    r, c = np.meshgrid(np.arange(rows), np.arange(cols), indexing="ij")
+    verts = np.empty((rows * cols, 3))
+    verts[:, 0] = o[0] + (c.ravel() * spacing)
+    verts[:, 1] = o[1] + heights.ravel()
+    verts[:, 2] = o[2] + (r.ravel() * spacing)
+    rr, cc = np.meshgrid(np.arange(rows - 1), np.arange(cols - 1), indexing="ij")
+    i00 = (rr * cols + cc).ravel()
+    i01, i10 = i00 + 1, i00 + cols
+    i11 = i10 + 1
+    tris = np.stack([i00, i10, i11, i00, i11, i01], axis=1).reshape(-1, 3).astype(np.int32)
+    return verts, tris
+
+
+def mesh_body(index: int) -> Body:
+    """makeMesh (body.hpp:319-330): a static mesh body at the origin."""
+    b = Body(shape=_abi.SHAPE_MESH, id=index, x=(0.0, 0.0, 0.0))
+    b.inv_mass = 0.0
+    b.inv_inertia = (0.0, 0.0, 0.0)
+    b.flags = 0
+    return b
+
+
+def terrain_pile(n: int = 120, seed: int = 0, rows: int = 24, cols: int = 24, spacing: float = 0.5,
+                 amplitude: float = 1.0, half: float = 8.0, periodic: bool = False, shapes: str = "mixed",
+                 drop: float = 1.2, spread: float | None = None) -> Scene:
+    """Bodies dropped onto a value-noise heightfield (the mesh is body 0, so
+    every pair with it is (0, j)): spheres, boxes, capsules and hulls, or one
+    shape kind, placed at random over the terrain within `spread` above it
+    (default 4 + 0.05 n).  The terrain is centred under the bodies and lies
+    within the box."""
+    rng = np.random.default_rng(seed)
+    h = value_noise(rows, cols, seed=seed + 101, amplitude=amplitude)
+    origin = (-(cols - 1) * spacing / 2.0, -half + amplitude + 1.0, -(rows - 1) * spacing / 2.0)
+    verts, tris = heightfield_mesh(origin, spacing, h)
+    bodies = [mesh_body(0)]
+    xs = (cols - 1) * spacing / 2.0 - 0.6
+    zs = (rows - 1) * spacing / 2.0 - 0.6
+    top = origin[1] + amplitude + drop
+    kinds = {"mixed": ["sphere", "box", "capsule", "hull"], "sphere": ["sphere"], "box": ["box"],
+             "capsule": ["capsule"], "hull": ["hull"]}[shapes]
+    qs = random_unit_quaternions(rng, n)
+    placed = []
+    tries = 0
+    while len(bodies) < n + 1 and tries < 200 * n:
+        tries += 1
+        p = np.array([rng.uniform(-xs, xs), top + rng.uniform(0.0, 4.0 + 0.05 * n if spread is None else spread),
+                      rng.uniform(-zs, zs)])
+        if any(np.linalg.norm(p - q) < 0.95 for q in placed):
+            continue
+        placed.append(p)
+        i = len(bodies)
+        k = kinds[i % len(kinds)]
+        q = tuple(qs[i - 1])
+        if k == "sphere":
+            b = make_sphere(i, tuple(p), rng.uniform(0.2, 0.4), 1.0)
+        elif k == "box":
+            b = make_box(i, tuple(p), q, tuple(rng.uniform(0.15, 0.35, 3)), 1.0)
+        elif k == "capsule":
+            b = make_capsule(i, tuple(p), q, 0.15, 0.2, 1.0)
+        else:
+            b = make_convex(i, tuple(p), q, hull12(rng, 0.3), 1.0)
+        b.v = tuple(rng.uniform(-0.5, 0.5, 3))
+        bodies.append(b)
+    s = Scene.from_bodies(bodies)
+    s.meshes = [dict(body=0, verts=verts, tris=tris)]
+    s.params.half = (half, half, half)
+    s.params.periodic = periodic
+    s.params.gravity = (0.0, -9.81, 0.0)
+    s.params.friction = 0.6
+    s.params.restitution = 0.1
+    return s
+
+
+def triangle_soup(ntris: int = 300, seed: int = 0, lattice: bool = True, degenerate: int = 5):
+    """Vertices and triangles for tree checks: on a coarse integer lattice
+    (many equal split keys) or continuous, plus a few degenerate
+    (collinear) triangles."""
+    rng = np.random.default_rng(seed)
+    nv = max(3, ntris // 2)
+    if lattice:
+        verts = rng.integers(-4, 5, (nv, 3)).astype(np.float64)
+    else:
+        verts = rng.uniform(-4.0, 4.0, (nv, 3))
+    tris = np.stack([rng.choice(nv, 3, replace=False) for _ in range(ntris)]).astype(np.int32)
+    if degenerate:
+        extra = []
+        for _ in range(degenerate):
+            a, b = rng.uniform(-3, 3, 3), rng.uniform(-3, 3, 3)
+            base = len(verts) + 3 * len(extra)
+            extra.append([a, b, 0.5 * (a + b)])
+        verts = np.concatenate([verts, np.asarray(extra).reshape(-1, 3)])
+        dt = np.arange(len(extra) * 3, dtype=np.int32).reshape(-1, 3) + nv
+        tris = np.concatenate([tris, dt])
+    return verts, tris
+
+
+def tilted_mesh_pile(n: int = 40, seed: int = 0, half: float = 8.0, periodic: bool = False) -> Scene:
+    """Bodies on a heightfield mesh whose body is moved off the origin and
+    tilted (PhysicsWorld::addMesh with a pose, physics_world.hpp:83-93), so the
+    queries run in the mesh frame and the triangles are transformed."""
+    s = terrain_pile(n, seed=seed, rows=16, cols=16, spacing=0.6, amplitude=0.6, half=half, periodic=periodic)
+    ang = 0.15
+    s.x[0] = (0.3, 0.4, -0.2)
+    s.q[0] = (math.cos(ang / 2), 0.0, 0.0, math.sin(ang / 2))
     return s

@@ -109,7 +109,24 @@ typedef struct {
     uint32_t layer, mask, flags;
     uint32_t hull0, hull1;  /* [hull0, hull1) into world hull vertex array */
     uint32_t nc0, nc1;      /* [nc0, nc1) into world no-collide array      */
+    int32_t mesh;           /* mesh body: index into the world's meshes, or -1 */
+    const v3* hvp;          /* when set, the hull vertices (a temporary prism) */
 } obody;
+
+/* MeshData (src/mesh.hpp:14-93): vertices, triangles, the tree's triangle
+ * order and its nodes (lo, hi; children or a leaf's [start, start+count)). */
+typedef struct { v3 lo, hi; int32_t left, right, start, count; } onode;
+typedef struct {
+    uint32_t nv, nt, nn;
+    v3* verts;
+    int32_t* tris;
+    int32_t* order;
+    onode* nodes;
+} omesh;
+
+/* the tree build (ne_oracle_bvh.cpp, through std::nth_element) */
+int32_t ne_oracle_bvh_build(const double* verts, const int32_t* tris, int32_t ntris, int32_t* order,
+                            int32_t node_cap, double* bounds, int32_t* links);
 
 typedef struct { int hit; v3 normal, point; double overlap; } ocontact;   /* narrowphase.hpp:16-21 */
 
@@ -141,6 +158,12 @@ struct ne_oracle {
     /* this step's dynamic-dynamic contact pairs, for island sleeping (world.hpp:158) */
     uint32_t nisp, capisp;
     uint32_t* isp;
+    /* static triangle meshes (Body::mesh) */
+    uint32_t nmeshes;
+    omesh* meshes;
+    /* one pair's contacts (gatherContacts' output vector) */
+    uint32_t capcs;
+    ocontact* cs;
 };
 
 struct otape_pair {
@@ -222,9 +245,10 @@ static v3 support_world(const ne_oracle* o, const obody* b, v3 d) {             
     } else if (b->shape == NE_SHAPE_CONVEX) {
         double best = -1e300;
         sb = mk(0, 0, 0);
+        const v3* H = b->hvp ? b->hvp : o->hull;
         for (uint32_t k = b->hull0; k < b->hull1; ++k) {
-            double dp = dot(o->hull[k], db);
-            if (dp > best) { best = dp; sb = o->hull[k]; }
+            double dp = dot(H[k], db);
+            if (dp > best) { best = dp; sb = H[k]; }
         }
     } else {
         double sy = (db.y >= 0.0) ? b->hh : -b->hh;
@@ -671,7 +695,98 @@ static int box_box_manifold(const ne_oracle* o, const obody* a, const obody* b, 
 }
 
 /* gatherContacts for non-mesh, non-compound shapes: src/world.cpp:122-129 */
-static int gather_contacts(const ne_oracle* o, const obody* a, const obody* b, ocontact out[4]) {
+static void cs_reserve(ne_oracle* o, uint32_t need) {
+    if (need <= o->capcs) return;
+    uint32_t cap = o->capcs ? o->capcs : 16;
+    while (cap < need) cap *= 2;
+    o->cs = (ocontact*)realloc(o->cs, sizeof(ocontact) * cap);
+    o->capcs = cap;
+}
+
+/* MeshData::query (mesh.hpp:46-59): depth-first with an explicit stack of 64,
+ * right child popped first; children are dropped once 62 entries are held. */
+static uint32_t mesh_query(const omesh* m, v3 lo, v3 hi, int32_t** out, uint32_t* cap) {
+    uint32_t n = 0;
+    if (m->nn == 0) return 0;
+    int32_t stack[64];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const onode* nd = &m->nodes[stack[--sp]];
+        if (lo.x > nd->hi.x || hi.x < nd->lo.x || lo.y > nd->hi.y || hi.y < nd->lo.y || lo.z > nd->hi.z ||
+            hi.z < nd->lo.z)
+            continue;
+        if (nd->left < 0) {
+            for (int32_t i = 0; i < nd->count; ++i) {
+                if (n == *cap) {
+                    *cap = *cap ? 2 * *cap : 64;
+                    *out = (int32_t*)realloc(*out, sizeof(int32_t) * *cap);
+                }
+                (*out)[n++] = m->order[nd->start + i];
+            }
+        } else if (sp < 62) {
+            stack[sp++] = nd->left;
+            stack[sp++] = nd->right;
+        }
+    }
+    return n;
+}
+
+/* meshContacts (world.cpp:56-83): every triangle whose box meets the body's
+ * cube of half-side boundingRadius + 0.05 (in the mesh frame) becomes a prism
+ * of thickness 2e-3 about its plane, a temporary convex at the origin, and
+ * goes through detectContact(other, prism).  Normals point mesh -> other. */
+static uint32_t mesh_contacts(ne_oracle* o, const obody* other, const obody* mb, uint32_t at) {
+    if (mb->mesh < 0) return 0;
+    const omesh* m = &o->meshes[mb->mesh];
+    v3 rel = min_image(&o->box, sub(other->x, mb->x));
+    v3 cl = qinvrot(mb->q, rel);
+    double rr = bounding_radius(o, other) + 0.05;
+    int32_t* tris = NULL;
+    uint32_t cap = 0;
+    uint32_t nt = mesh_query(m, sub(cl, mk(rr, rr, rr)), add(cl, mk(rr, rr, rr)), &tris, &cap);
+    const double thick = 1e-3;
+    obody tri;
+    memset(&tri, 0, sizeof(tri));
+    tri.q = mq(1.0, 0.0, 0.0, 0.0);
+    tri.shape = NE_SHAPE_CONVEX;
+    tri.mesh = -1;
+    tri.hull0 = 0;
+    tri.hull1 = 6;
+    v3 pv[6];
+    tri.hvp = pv;
+    uint32_t nc = 0;
+    for (uint32_t k = 0; k < nt; ++k) {
+        const int32_t* t = m->tris + 3 * (size_t)tris[k];
+        v3 w0 = add(mb->x, qrot(mb->q, m->verts[t[0]]));
+        v3 w1 = add(mb->x, qrot(mb->q, m->verts[t[1]]));
+        v3 w2 = add(mb->x, qrot(mb->q, m->verts[t[2]]));
+        v3 n = cross(sub(w1, w0), sub(w2, w0));
+        double nl = norm(n);
+        if (nl < 1e-12) continue;
+        n = dvs(n, nl);
+        pv[0] = add(w0, scl(n, thick)); pv[1] = add(w1, scl(n, thick)); pv[2] = add(w2, scl(n, thick));
+        pv[3] = sub(w0, scl(n, thick)); pv[4] = sub(w1, scl(n, thick)); pv[5] = sub(w2, scl(n, thick));
+        ocontact c = detect_contact(o, other, &tri);
+        if (c.hit) {
+            cs_reserve(o, at + nc + 1);
+            o->cs[at + nc++] = c;
+        }
+    }
+    free(tris);
+    return nc;
+}
+
+/* gatherContacts (world.cpp:88-127) into o->cs; returns the count. */
+static int gather_contacts(ne_oracle* o, const obody* a, const obody* b) {
+    if (b->shape == NE_SHAPE_MESH) return (int)mesh_contacts(o, a, b, 0);
+    if (a->shape == NE_SHAPE_MESH) {
+        int m = (int)mesh_contacts(o, b, a, 0);
+        for (int k = 0; k < m; ++k) o->cs[k].normal = neg(o->cs[k].normal);
+        return m;
+    }
+    cs_reserve(o, 4);
+    ocontact* out = o->cs;
     ocontact c = detect_contact(o, a, b);
     if (!c.hit) return 0;
     if (a->shape == NE_SHAPE_BOX && b->shape == NE_SHAPE_BOX) {
@@ -941,20 +1056,21 @@ static void collide(ne_oracle* o) {                                             
         const uint32_t i = replay ? replay->p[pi].i : pv.p[pi].i, j = replay ? replay->p[pi].j : pv.p[pi].j;
         obody* a = &o->B[i];
         obody* b = &o->B[j];
-        ocontact cs[4];
         int nc;
         if (replay) {                 /* the recorded contacts of this pair: fixed contact set */
             nc = replay->p[pi].nc;
-            memcpy(cs, replay->p[pi].cs, sizeof(ocontact) * (size_t)nc);
+            cs_reserve(o, 4);
+            memcpy(o->cs, replay->p[pi].cs, sizeof(ocontact) * (size_t)nc);
         } else {
             if (a->invMass + b->invMass <= 0.0) continue;
             if (!layers_collide(a, b)) continue;
             if (pair_excluded(o, a, j)) continue;
             if ((a->flags & NE_FLAG_SENSOR) || (b->flags & NE_FLAG_SENSOR)) continue;
-            nc = gather_contacts(o, a, b, cs);
+            nc = gather_contacts(o, a, b);
             if (nc == 0) continue;
-            if (record) tape_push(record, i, j, nc, cs);
+            if (record) tape_push(record, i, j, nc > 4 ? 4 : nc, o->cs);   /* tapes hold no mesh pairs */
         }
+        const ocontact* cs = o->cs;
         const double e = combine_mat(P->restitution_combine, eff_restitution(o, a), eff_restitution(o, b));
         const double mu = combine_mat(P->friction_combine, eff_friction(o, a), eff_friction(o, b));
         if (P->sleep_enabled) {                                                      /* world.cpp:527-531 */
@@ -1258,8 +1374,20 @@ int ne_oracle_create(ne_oracle** out) {
     return NE_OK;
 }
 
+static void free_meshes(ne_oracle* o) {
+    for (uint32_t m = 0; m < o->nmeshes; ++m) {
+        free(o->meshes[m].verts); free(o->meshes[m].tris); free(o->meshes[m].order); free(o->meshes[m].nodes);
+    }
+    free(o->meshes);
+    o->meshes = NULL;
+    o->nmeshes = 0;
+    for (uint32_t i = 0; i < o->n; ++i) o->B[i].mesh = -1;
+}
+
 int ne_oracle_destroy(ne_oracle* o) {
     if (!o) return NE_OK;
+    free_meshes(o);
+    free(o->cs);
     free(o->B); free(o->hull); free(o->nocol); free(o->cons); free(o->cache_key); free(o->cache_imp);
     for (uint32_t t = 0; t < o->tape_steps; ++t) free(o->tape[t].p);
     free(o->tape);
@@ -1280,6 +1408,7 @@ int ne_oracle_set_params(ne_oracle* o, const ne_world_params* p) {
 
 int ne_oracle_set_bodies(ne_oracle* o, const ne_bodies* b) {
     if (!o || !b) return NE_ERR_INVALID;
+    free_meshes(o);
     free(o->B); free(o->hull); free(o->nocol);
     o->B = NULL; o->hull = NULL; o->nocol = NULL; o->nhull = 0; o->nnocol = 0;
     o->n = b->n;
@@ -1303,7 +1432,8 @@ int ne_oracle_set_bodies(ne_oracle* o, const ne_bodies* b) {
         B->invMass = b->inv_mass[i];
         B->invI = ld3(b->inv_inertia, i);
         B->shape = b->shape[i];
-        if (B->shape < 0 || B->shape > NE_SHAPE_PLANE) return NE_ERR_UNSUPPORTED;
+        if (B->shape < 0 || B->shape > NE_SHAPE_MESH) return NE_ERR_UNSUPPORTED;
+        B->mesh = -1;
         B->radius = b->radius[i];
         B->hh = b->half_height[i];
         B->he = ld3(b->half_extents, i);
@@ -1373,6 +1503,66 @@ int ne_oracle_set_joints(ne_oracle* o, uint32_t n, const ne_distance_joint* j) {
     return NE_OK;
 }
 
+/* Builds each mesh's tree (MeshData::build, mesh.hpp:38-43) and attaches it
+ * to its body (Body::mesh). */
+int ne_oracle_set_meshes(ne_oracle* o, uint32_t n, const ne_mesh* ms) {
+    if (!o || (n && !ms)) return NE_ERR_INVALID;
+    for (uint32_t m = 0; m < n; ++m) {
+        if (ms[m].body >= o->n || o->B[ms[m].body].shape != NE_SHAPE_MESH) return NE_ERR_INVALID;
+        if (ms[m].ntris && (!ms[m].tris || !ms[m].verts)) return NE_ERR_INVALID;
+        for (uint64_t k = 0; k < 3ull * ms[m].ntris; ++k)
+            if (ms[m].tris[k] < 0 || (uint32_t)ms[m].tris[k] >= ms[m].nverts) return NE_ERR_INVALID;
+    }
+    free_meshes(o);
+    o->meshes = (omesh*)calloc(n ? n : 1, sizeof(omesh));
+    o->nmeshes = n;
+    for (uint32_t m = 0; m < n; ++m) {
+        omesh* M = &o->meshes[m];
+        const ne_mesh* src = &ms[m];
+        M->nv = src->nverts;
+        M->nt = src->ntris;
+        M->verts = (v3*)malloc(sizeof(v3) * (M->nv ? M->nv : 1));
+        for (uint32_t k = 0; k < M->nv; ++k) M->verts[k] = ld3(src->verts, k);
+        M->tris = (int32_t*)malloc(sizeof(int32_t) * 3 * (M->nt ? M->nt : 1));
+        if (M->nt) memcpy(M->tris, src->tris, sizeof(int32_t) * 3 * M->nt);
+        M->order = (int32_t*)malloc(sizeof(int32_t) * (M->nt ? M->nt : 1));
+        const int32_t cap = 2 * (int32_t)M->nt + 1;
+        double* bounds = (double*)malloc(sizeof(double) * 6 * (size_t)cap);
+        int32_t* links = (int32_t*)malloc(sizeof(int32_t) * 4 * (size_t)cap);
+        int32_t nn = ne_oracle_bvh_build(src->verts, src->tris, (int32_t)M->nt, M->order, cap, bounds, links);
+        if (nn < 0 || nn > cap) { free(bounds); free(links); return NE_ERR_INVALID; }
+        M->nn = (uint32_t)nn;
+        M->nodes = (onode*)malloc(sizeof(onode) * (nn ? (size_t)nn : 1));
+        for (int32_t k = 0; k < nn; ++k) {
+            onode* d = &M->nodes[k];
+            d->lo = mk(bounds[6 * k], bounds[6 * k + 1], bounds[6 * k + 2]);
+            d->hi = mk(bounds[6 * k + 3], bounds[6 * k + 4], bounds[6 * k + 5]);
+            d->left = links[4 * k]; d->right = links[4 * k + 1];
+            d->start = links[4 * k + 2]; d->count = links[4 * k + 3];
+        }
+        free(bounds);
+        free(links);
+        o->B[src->body].mesh = (int32_t)m;
+    }
+    return NE_OK;
+}
+
+int ne_oracle_mesh_bvh(ne_oracle* o, uint32_t m, int32_t* order, uint32_t node_cap, uint32_t* nnodes, double* bounds,
+                       int32_t* links) {
+    if (!o || m >= o->nmeshes) return NE_ERR_INVALID;
+    const omesh* M = &o->meshes[m];
+    if (nnodes) *nnodes = M->nn;
+    if (order) memcpy(order, M->order, sizeof(int32_t) * M->nt);
+    if (bounds && links && node_cap >= M->nn)
+        for (uint32_t k = 0; k < M->nn; ++k) {
+            const onode* d = &M->nodes[k];
+            bounds[6 * k] = d->lo.x; bounds[6 * k + 1] = d->lo.y; bounds[6 * k + 2] = d->lo.z;
+            bounds[6 * k + 3] = d->hi.x; bounds[6 * k + 4] = d->hi.y; bounds[6 * k + 5] = d->hi.z;
+            links[4 * k] = d->left; links[4 * k + 1] = d->right; links[4 * k + 2] = d->start; links[4 * k + 3] = d->count;
+        }
+    return NE_OK;
+}
+
 int ne_oracle_reset_warm_cache(ne_oracle* o) {
     if (!o) return NE_ERR_INVALID;
     o->ncache = 0;
@@ -1435,8 +1625,8 @@ int ne_oracle_get_pairs(ne_oracle* o, uint32_t cap, uint32_t* count, uint32_t* p
 int ne_oracle_pair_contacts(ne_oracle* o, uint32_t a, uint32_t b, uint32_t cap, uint32_t* count,
                             double* normal, double* point, double* depth) {
     if (!o || !count || a >= o->n || b >= o->n) return NE_ERR_INVALID;
-    ocontact cs[4];
-    int nc = gather_contacts(o, &o->B[a], &o->B[b], cs);
+    int nc = gather_contacts(o, &o->B[a], &o->B[b]);
+    const ocontact* cs = o->cs;
     *count = (uint32_t)nc;
     uint32_t m = (uint32_t)nc < cap ? (uint32_t)nc : cap;
     for (uint32_t k = 0; k < m; ++k) {

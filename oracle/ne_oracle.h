@@ -48,6 +48,9 @@ int ne_oracle_wrap(ne_oracle *o, const double *p, double *out);
 void ne_oracle_epa_stats(int32_t *max_faces, int32_t *max_horizon);
 int ne_oracle_tape(ne_oracle *o, int32_t mode);
 int ne_oracle_set_joints(ne_oracle *o, uint32_t n, const ne_distance_joint *joints);
+int ne_oracle_set_meshes(ne_oracle *o, uint32_t n, const ne_mesh *meshes);
+int ne_oracle_mesh_bvh(ne_oracle *o, uint32_t m, int32_t *order, uint32_t node_cap, uint32_t *nnodes,
+                       double *bounds, int32_t *links);
 void ne_oracle_epa_stats_reset(int32_t faces, int32_t horizon);
 void ne_oracle_epa_counts(int64_t *calls, int64_t *big);
 

@@ -8,6 +8,9 @@
 // It drives the reference through its public API only:
 //   - `ne::World` public fields and `step()`            (src/world.hpp:73-161)
 //   - `World::distanceJoints`                           (src/world.hpp:20-26, 129)
+//   - `MeshData` built by its own `build()` and attached as `Body::mesh`
+//                                                       (src/mesh.hpp:14-93,
+//                                                        src/body.hpp:62)
 //   - `captureContacts` / `debugContacts` for the per-step constraint list
 //                                                       (src/world.hpp:135-136,
 //                                                        src/world.cpp:552)
@@ -20,6 +23,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "world.hpp"
@@ -32,6 +36,7 @@ namespace {
 
 struct RefWorld {
     ne::World w;
+    std::vector<std::shared_ptr<const ne::MeshData>> meshes;   // in ne_ref_set_meshes order
 };
 
 ne::V3 v3(const double* p, uint32_t i) { return ne::V3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
@@ -50,7 +55,7 @@ int build_bodies(const ne_bodies* b, std::vector<ne::Body>& out) {
         B.invMass = b->inv_mass[i];
         B.invInertiaBody = v3(b->inv_inertia, i);
         int s = b->shape[i];
-        if (s < 0 || s > NE_SHAPE_PLANE) return NE_ERR_UNSUPPORTED;
+        if (s < 0 || s > NE_SHAPE_MESH) return NE_ERR_UNSUPPORTED;
         B.shape = static_cast<ne::Shape>(s);
         B.radius = b->radius[i];
         B.halfHeight = b->half_height[i];
@@ -150,6 +155,40 @@ int ne_ref_set_joints(void* h, uint32_t n, const ne_distance_joint* j) {
         d.damping = j[k].damping;
         r->w.distanceJoints.push_back(d);
     }
+    return NE_OK;
+}
+
+int ne_ref_set_meshes(void* h, uint32_t n, const ne_mesh* ms) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || (n && !ms)) return NE_ERR_INVALID;
+    r->meshes.clear();
+    for (uint32_t m = 0; m < n; ++m) {
+        if (ms[m].body >= r->w.bodies.size() || r->w.bodies[ms[m].body].shape != ne::Shape::Mesh) return NE_ERR_INVALID;
+        auto md = std::make_shared<ne::MeshData>();
+        for (uint32_t k = 0; k < ms[m].nverts; ++k) md->verts.push_back(v3(ms[m].verts, k));
+        for (uint32_t t = 0; t < ms[m].ntris; ++t)
+            md->tris.push_back({ms[m].tris[3 * t], ms[m].tris[3 * t + 1], ms[m].tris[3 * t + 2]});
+        md->build();
+        r->w.bodies[ms[m].body].mesh = md;
+        r->meshes.push_back(md);
+    }
+    return NE_OK;
+}
+
+int ne_ref_mesh_bvh(void* h, uint32_t m, int32_t* order, uint32_t node_cap, uint32_t* nnodes, double* bounds,
+                    int32_t* links) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || m >= r->meshes.size()) return NE_ERR_INVALID;
+    const ne::MeshData& md = *r->meshes[m];
+    if (nnodes) *nnodes = (uint32_t)md.nodes.size();
+    if (order) for (size_t k = 0; k < md.order.size(); ++k) order[k] = md.order[k];
+    if (bounds && links && node_cap >= md.nodes.size())
+        for (size_t k = 0; k < md.nodes.size(); ++k) {
+            const auto& d = md.nodes[k];
+            bounds[6 * k] = d.lo.x; bounds[6 * k + 1] = d.lo.y; bounds[6 * k + 2] = d.lo.z;
+            bounds[6 * k + 3] = d.hi.x; bounds[6 * k + 4] = d.hi.y; bounds[6 * k + 5] = d.hi.z;
+            links[4 * k] = d.left; links[4 * k + 1] = d.right; links[4 * k + 2] = d.start; links[4 * k + 3] = d.count;
+        }
     return NE_OK;
 }
 

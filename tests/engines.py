@@ -40,10 +40,27 @@ def _bind(lib, prefix):
     g("min_image").argtypes = [_vp, _dp, _dp]
     g("wrap").argtypes = [_vp, _dp, _dp]
     g("set_joints").argtypes = [_vp, C.c_uint32, _vp]
+    g("set_meshes").argtypes = [_vp, C.c_uint32, C.POINTER(_abi.Mesh)]
+    g("mesh_bvh").argtypes = [_vp, C.c_uint32, C.POINTER(C.c_int32), C.c_uint32, _u32p, _dp, C.POINTER(C.c_int32)]
     return lib
 
 
 _libs = {}
+
+
+def mesh_bvh_of(fn, h, m, ntris):
+    nn = C.c_uint32(0)
+    rc = fn(h, m, None, 0, C.byref(nn), None, None)
+    if rc != 0:
+        raise RuntimeError(f"mesh_bvh failed: {rc}")
+    order = np.zeros(ntris, np.int32)
+    bounds = np.zeros((nn.value, 6))
+    links = np.zeros((nn.value, 4), np.int32)
+    ip = C.POINTER(C.c_int32)
+    rc = fn(h, m, order.ctypes.data_as(ip), nn.value, C.byref(nn), _abi.dptr(bounds), links.ctypes.data_as(ip))
+    if rc != 0:
+        raise RuntimeError(f"mesh_bvh failed: {rc}")
+    return order, bounds, links
 
 
 def oracle_available():
@@ -115,6 +132,14 @@ class _CpuWorld:
         if len(j) or getattr(self, "_had_joints", False):
             self._check(self._f("set_joints")(self.h, len(j), j.ctypes.data if len(j) else None), "set_joints")
             self._had_joints = len(j) > 0
+        if s.meshes:
+            arr, keep = s.meshes_c()
+            self._check(self._f("set_meshes")(self.h, len(s.meshes), arr), "set_meshes")
+            del keep
+
+    def mesh_bvh(self, m, ntris):
+        """(order, bounds (nn, 6), links (nn, 4)) of mesh m's tree."""
+        return mesh_bvh_of(self._f("mesh_bvh"), self.h, m, ntris)
 
     def step(self, nsteps=1):
         self._check(self._f("step")(self.h, int(nsteps)), "step")

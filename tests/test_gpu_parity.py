@@ -248,3 +248,78 @@ def test_wake_chain_parity(gpu_available, oracle_lib):
         _assert_same_state(g.state(), o.state(), f"wake chain step {t}")
         _assert_same_sleep(g, o, s.n, f"wake chain step {t}")
     _assert_same_contacts(g.contacts(), o.contacts(), "wake chain last step")
+
+
+@pytest.mark.parametrize("kind", ["heightfield", "lattice"])
+def test_mesh_tree_matches_oracle(gpu_available, oracle_lib, kind):
+    """The library's tree (built once at upload) equals the oracle's, node for
+    node and in triangle order."""
+    if kind == "heightfield":
+        verts, tris = scene.heightfield_mesh((-6.0, -1.0, -6.0), 0.4, scene.value_noise(31, 30, seed=2, amplitude=1.5))
+    else:
+        verts, tris = scene.triangle_soup(900, seed=5, lattice=True)
+    s = scene.Scene.from_bodies([scene.mesh_body(0), scene.make_sphere(1, (0.0, 6.0, 0.0), 0.3, 1.0)])
+    s.meshes = [dict(body=0, verts=verts, tris=tris)]
+    _say(f"mesh tree: {kind}, {len(tris)} triangles")
+    g = _gpu()
+    g.set_scene(s)
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    for name, a, b in zip(["order", "bounds", "links"], g.mesh_bvh(0, len(tris)), o.mesh_bvh(0, len(tris))):
+        assert np.array_equal(a, b), f"tree {name} differs"
+
+
+@pytest.mark.parametrize("shapes,periodic,graph", [("sphere", False, False), ("box", False, False),
+                                                   ("mixed", True, False), ("mixed", False, True),
+                                                   ("hull", False, False)])
+def test_terrain_parity(gpu_available, oracle_lib, shapes, periodic, graph):
+    """Bodies settling on a heightfield: the tree query, the per-triangle prism
+    contacts (GJK/EPA) and their order within each pair equal the oracle's;
+    state after every step, the constraint list every 10 steps."""
+    s = scene.terrain_pile(40, seed=3, shapes=shapes, periodic=periodic)
+    _say(f"terrain: {s.n} bodies, {len(s.meshes[0]['tris'])} triangles, {shapes} periodic={periodic} graph={graph}")
+    g = _gpu()
+    g.set_scene(s)
+    g.use_graph(graph)
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    mesh_contacts = 0
+    for t in range(200):
+        g.step(1)
+        o.step(1)
+        _assert_same_state(g.state(), o.state(), f"terrain step {t}")
+        if t % 10 == 9:
+            cg, co = g.contacts(), o.contacts()
+            _assert_same_contacts(cg, co, f"terrain step {t}")
+            mesh_contacts += int(np.sum(co["i"] == 0))
+    assert mesh_contacts > 50
+    g.step(30)
+    o.step(30)
+    _assert_same_state(g.state(), o.state(), "terrain after +30")
+
+
+def test_tilted_mesh_parity(gpu_available, oracle_lib):
+    """A mesh body off the origin and rotated (queries in the mesh frame)."""
+    s = scene.tilted_mesh_pile(40, seed=7)
+    _say(f"tilted mesh: {s.n} bodies")
+    _run_parity(s, 150, "tilted mesh", check_every=10, use_ref=False)
+
+
+def test_terrain_many_bodies_grows_candidate_buffer(gpu_available, oracle_lib):
+    """A larger heightfield under 1500 bodies: more triangle candidates than
+    the initial buffer holds, so a step grows it and reruns; the result still
+    equals the oracle's."""
+    s = scene.terrain_pile(1500, seed=9, rows=96, cols=96, spacing=0.4, amplitude=1.5, half=24.0, shapes="mixed",
+                           spread=4.0)
+    _say(f"terrain large: {s.n} bodies, {len(s.meshes[0]['tris'])} triangles")
+    g = _gpu()
+    g.set_scene(s)
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    for chunk in (150, 50):
+        g.step(chunk)
+        o.step(chunk)
+        _assert_same_state(g.state(), o.state(), f"terrain large after +{chunk}")
+    co = o.contacts()
+    _assert_same_contacts(g.contacts(), co, "terrain large last step")
+    assert int(np.sum(co["i"] == 0)) > 1000, "the bodies have landed on the terrain"
