@@ -76,18 +76,25 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(scene, steps: int):
+def cpu_baseline(scene, steps: int, what: str):
     """Single-threaded CPU step of the same scene: the reference itself
-    (oracle/_ref) when built, else the C restatement."""
+    (oracle/_ref) when built and able to express the scene (a cubic box,
+    8 iterations), else the C restatement."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import engines
+    w = kind = None
     if engines.ref_available():
         w, kind = engines.RefWorld(capture=False), "reference"
-    elif engines.oracle_available():
+        try:
+            w.set_scene(scene)
+        except RuntimeError:        # not expressible by the reference
+            w.close()
+            w = None
+    if w is None:
+        if not engines.oracle_available():
+            return None
         w, kind = engines.OracleWorld(), "port"
-    else:
-        return None
-    w.set_scene(scene)
+        w.set_scene(scene)
     t0 = time.perf_counter()
     w.step(steps)
     dt = time.perf_counter() - t0
@@ -97,10 +104,164 @@ def cpu_baseline(scene, steps: int):
         "unit": "body-steps/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"{steps} steps of the same {scene.n}-sphere gas, single thread, {cpu_model()}, "
-                  f"nproc={os.cpu_count()}",
+        "sample": f"{steps} steps of {what}, single thread, {cpu_model()}, nproc={os.cpu_count()}",
         "seconds": dt,
     }
+
+
+def stage_bytes(name: str, iterations: int):
+    """Algorithmic bytes per unit of a step stage: {unit kind: bytes}."""
+    table = {
+        "integrate": {"body": 240.0},
+        "cell_count": {"body": 48.0},
+        "cell_fill": {"body": 48.0},
+        "cand_count": {"body": 48.0},
+        "cand_write": {"body": 48.0},
+        "boundary": {"body": 48.0},
+        "narrow": {"pair": 200.0, "mesh_triangle": 72.0},
+        "cons_build": {"contact": 352.0},
+        "solve": {"contact": 440.0 * (iterations + 1)},
+    }
+    return table.get(name)
+
+
+def scene_bench(s, steps: int, warmup: int, cpu_steps: int, cpu_what: str, label: str, max_seconds: float = 150.0):
+    """Body-steps/s of one resident scene over `steps` single-step calls after
+    `warmup` untimed steps, with the dominant stage's HIP-event time and the
+    algorithmic bytes it moved (units counted from every timed step)."""
+    from nativeengine_amd.gpu import GpuWorld, KERNELS
+    import numpy as np
+    cpu = None
+    if cpu_steps > 0:
+        log(f"{label}: cpu baseline, {cpu_steps} steps ...")
+        cpu = cpu_baseline(s, cpu_steps, cpu_what)
+        if cpu:
+            log(f"{label}: cpu {cpu['kind']} {cpu['value']:.4g} body-steps/s ({cpu['seconds']:.1f}s)")
+    iters = int(s.params.solver_iterations)
+    t0 = time.perf_counter()
+    g = GpuWorld()
+    g.set_scene(s)
+    log(f"{label}: upload {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    for _ in range(warmup):
+        g.step(1)
+    log(f"{label}: warm-up {warmup} steps {time.perf_counter() - t0:.2f}s")
+    stage_ms = {}
+    for name in KERNELS:
+        if name == "step":
+            continue
+        g.time_kernel(name)
+        g.step(2)
+        ms, k = g.kernel_time()
+        stage_ms[name] = ms / max(k, 1)
+    g.time_kernel(None)
+    ranked = sorted(((v, k) for k, v in stage_ms.items() if stage_bytes(k, iters)), reverse=True)
+    dom = ranked[0][1]
+    log(f"{label}: stage ms/launch " + ", ".join(f"{k}={v:.4f}" for k, v in sorted(stage_ms.items(), key=lambda t: -t[1])))
+    g.time_kernel(dom)
+    units = {"body": 0, "pair": 0, "mesh_triangle": 0, "contact": 0}
+    per_step_ms = []
+    t0 = time.perf_counter()
+    last_log = t0
+    done = 0
+    for _ in range(steps):
+        ts = time.perf_counter()
+        g.step(1)
+        c = g.counters()
+        now = time.perf_counter()
+        per_step_ms.append((now - ts) * 1e3)
+        done += 1
+        units["body"] += s.n
+        units["pair"] += c["candidate_pairs"]
+        units["mesh_triangle"] += c["mesh_triangles"]
+        units["contact"] += c["constraints"]
+        if now - last_log > 10.0:
+            log(f"{label}: {done} steps, {(now - t0) * 1e3 / done:.2f} ms/step, last {per_step_ms[-1]:.2f} ms, "
+                f"{c['constraints']} constraints, largest island {c['max_island']}")
+            last_log = now
+        if now - t0 > max_seconds:
+            log(f"{label}: stopping after {done} steps ({max_seconds:.0f} s cap)")
+            break
+    t1 = time.perf_counter()
+    steps = done
+    dom_ms, dom_launches = g.kernel_time()
+    g.time_kernel(None)
+    last = g.counters()
+    reruns, trunc = g.epa_stats()
+    fl = g.body_aux()[0] if s.params.sleep_enabled else None
+    g.close()
+    value = s.n * steps / (t1 - t0)
+    per = stage_bytes(dom, iters)
+    bytes_total = sum(per[k] * units[k] for k in per)
+    avg_launch_ms = dom_ms / max(dom_launches, 1)
+    achieved = bytes_total / max(dom_launches, 1) / (avg_launch_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "bytes_per_unit": per,
+        "units_per_launch": {k: round(units[k] / max(dom_launches, 1), 1) for k in per},
+        "avg_launch_us": round(avg_launch_ms * 1e3, 3),
+    }
+    pct = np.percentile(per_step_ms, [50, 90, 100])
+    out = {
+        "value": value,
+        "unit": "body-steps/s",
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": (t1 - t0) * 1e3 / steps,
+        "step_ms_p50_p90_max": [round(float(x), 3) for x in pct],
+        "bodies": s.n,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "per_step_mean": {"candidate_pairs": units["pair"] / steps, "mesh_triangles": units["mesh_triangle"] / steps,
+                          "constraints": units["contact"] / steps},
+        "last_step": {"candidate_pairs": last["candidate_pairs"], "constraints": last["constraints"],
+                      "islands": last["islands"], "largest_island": last["max_island"]},
+        "epa_large_workspace_reruns": reruns,
+        "epa_truncated": trunc,
+    }
+    if fl is not None:
+        out["sleeping_bodies_at_end"] = int(np.sum((fl & 8) != 0))
+    return out
+
+
+def large_bench(args):
+    """The large terrain scene at 1,048,576 bodies: the headline line."""
+    from nativeengine_amd import scene as scn
+    t0 = time.perf_counter()
+    s = scn.large_scene(args.large_n, seed=5)
+    h = s.params.half
+    log(f"large scene: {s.n} bodies, {len(s.joints)} joints, {len(s.meshes[0]['tris'])} triangles, "
+        f"built in {time.perf_counter() - t0:.1f}s")
+    r = scene_bench(s, args.large_steps, args.large_warmup, 0 if args.no_cpu else args.cpu_steps,
+                    f"the same scene from its initial state (bodies falling, no contacts yet: the CPU's "
+                    f"fastest phase of the run)", "large")
+    r["workload"] = (f"large terrain scene: {s.n} bodies (mesh + {int((s.shape == 2).sum())} boxes in columns of 8 + "
+                     f"{int((s.shape == 0).sum())} spheres, {len(s.joints)} rope joints in 16-link ropes), periodic "
+                     f"{2 * h[0]:.0f}x{2 * h[1]:.0f}x{2 * h[2]:.0f}, {len(s.meshes[0]['verts'])}-vertex value-noise "
+                     f"heightfield amplitude 8, mu 0.6, e 0.1, sleeping on, dt 1/60, "
+                     f"{s.params.solver_iterations} iterations, seed 5")
+    return r
+
+
+def pile_bench(args):
+    """The mixed-shape pile under gravity (65,536 bodies, reflective walls)."""
+    from nativeengine_amd import scene as scn
+    t0 = time.perf_counter()
+    s = scn.mixed_pile(args.pile_n, half=(50.0, 100.0, 50.0), seed=0, iterations=10)
+    log(f"pile: {s.n} bodies, built in {time.perf_counter() - t0:.1f}s")
+    r = scene_bench(s, args.pile_steps, args.pile_warmup, 0, "", "pile")
+    r["workload"] = (f"mixed-shape pile: {s.n - 1} dynamic bodies (40% boxes, 30% spheres, 20% capsules, 10% "
+                     f"12-vertex hulls) on a static ground box, reflective 100x200x100 box, g -9.81, mu 0.5, e 0.2, "
+                     f"dt 1/60, 10 iterations, warm start, seed 0")
+    return r
 
 
 def batched_bench(worlds: int, steps: int, warmup: int, loop_worlds: int, loop_steps: int):
@@ -235,7 +396,7 @@ def gas_bench(args):
 
     g = GpuWorld()
     g.set_scene(s)
-    g.step(max(args.warmup, 1))
+    g.step(max(args.gas_warmup, 1))
 
     # pick the dominant stage on a few untimed steps
     stage_ms = {}
@@ -258,14 +419,14 @@ def gas_bench(args):
     # timed region: K steps, inputs resident, events on the dominant kernel
     g.time_kernel(dom)
     t0 = time.perf_counter()
-    g.step(args.steps)                       # returns after the stream is synchronised
+    g.step(args.gas_steps)                   # returns after the stream is synchronised
     t1 = time.perf_counter()
     dom_ms, dom_launches = g.kernel_time()
     g.time_kernel(None)
     counters = g.counters()
 
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = s.n * args.steps / (t1 - t0)
+    ms_per_step = (t1 - t0) * 1e3 / args.gas_steps
+    value = s.n * args.gas_steps / (t1 - t0)
     unit_kind, per_unit = KERNEL_BYTES[dom]
     units = s.n
     avg_launch_ms = dom_ms / max(dom_launches, 1)
@@ -286,7 +447,7 @@ def gas_bench(args):
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"cpu baseline: {args.cpu_steps} steps ...")
-        cpu = cpu_baseline(s, args.cpu_steps)
+        cpu = cpu_baseline(s, args.cpu_steps, f"the same {s.n}-sphere gas")
         if cpu:
             log(f"cpu baseline {cpu['kind']}: {cpu['value']:.4g} body-steps/s ({cpu['seconds']:.1f}s)")
 
@@ -295,8 +456,8 @@ def gas_bench(args):
         "value": value,
         "unit": "body-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": args.gas_steps,
+        "warmup": args.gas_warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -322,47 +483,65 @@ def gas_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--steps", type=int, default=300, help="timed steps of the large terrain scene")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed steps of the large terrain scene first")
+    ap.add_argument("--large-n", type=int, default=1 << 20)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--breakdown", action="store_true", help="also time every stage (extra untimed steps)")
+    ap.add_argument("--no-large", action="store_true", help="skip the large terrain scene")
+    ap.add_argument("--no-gas", action="store_true", help="skip the 1M sphere gas")
+    ap.add_argument("--n", type=int, default=1 << 20, help="spheres in the gas")
+    ap.add_argument("--gas-steps", type=int, default=30)
+    ap.add_argument("--gas-warmup", type=int, default=5)
+    ap.add_argument("--no-pile", action="store_true")
+    ap.add_argument("--pile-n", type=int, default=65536)
+    ap.add_argument("--pile-steps", type=int, default=600)
+    ap.add_argument("--pile-warmup", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--worlds", type=int, default=1024)
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--batched-warmup", type=int, default=30)
     ap.add_argument("--loop-worlds", type=int, default=128)
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-gas", action="store_true", help="skip the 1M gas (profiling the other workloads alone)")
     args = ap.parse_args()
+    args.large_steps, args.large_warmup = args.steps, args.warmup
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
                                    "(DESIGN.md), so --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    out = {}
+    metric = "body-steps/s at 1M bodies; batched world-steps/s; backward/forward time ratio"
+    out = {"metric": metric, "value": None, "unit": "body-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    if not args.no_large:
+        large = large_bench(args)
+        log(f"large: {large['value']:.4g} body-steps/s, {large['ms_per_step']:.3f} ms/step")
+        out.update({"value": large["value"], "ms_per_step": large["ms_per_step"], "steps": large["steps"],
+                    "config": {"workload": large["workload"], "bodies": large["bodies"], "parallelism": "single GPU"},
+                    "roofline": large["roofline"], "cpu_baseline": large["cpu_baseline"],
+                    "speedup_vs_cpu": large["speedup_vs_cpu"],
+                    "large": {k: v for k, v in large.items() if k not in ("roofline", "cpu_baseline", "workload")}})
+    else:
+        out["note"] = "large terrain scene skipped (--no-large)"
     if not args.no_gas:
-        out = gas_bench(args)
-    batched = None
+        gas = gas_bench(args)
+        log(f"gas: {gas['value']:.4g} body-steps/s, {gas['ms_per_step']:.3f} ms/step")
+        out["gas"] = gas
+    if not args.no_pile:
+        pile = pile_bench(args)
+        log(f"pile: {pile['value']:.4g} body-steps/s, {pile['ms_per_step']:.3f} ms/step")
+        out["pile"] = pile
     if not args.no_batched:
         batched = batched_bench(args.worlds, args.batched_steps, args.batched_warmup, args.loop_worlds, 3)
         log(f"batched: {batched['value']:.4g} world-steps/s, {batched['ms_per_step']:.3f} ms/step, "
             f"{batched['vs_loop']:.1f}x the loop")
-
-    adjoint = None
+        out["batched"] = batched
     if not args.no_adjoint:
         adjoint = adjoint_bench(256, 120, 16)
         log(f"adjoint: forward {adjoint['forward_s']:.3f}s backward {adjoint['backward_s']:.3f}s "
             f"ratio {adjoint['value']:.2f}")
-
-    if not out:
-        out = {"metric": "body-steps/s at 1M bodies; batched world-steps/s; backward/forward time ratio",
-               "value": None, "unit": "body-steps/s", "n_gpus": 1, "note": "1M gas skipped (--no-gas)"}
-    if batched:
-        out["batched"] = batched
-    if adjoint:
         out["adjoint"] = adjoint
     print(json.dumps(out))
 

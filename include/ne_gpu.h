@@ -214,6 +214,11 @@ int ne_gpu_get_state(ne_gpu_ctx *ctx, double *x, double *v, double *q, double *w
 int ne_gpu_get_body_aux(ne_gpu_ctx *ctx, uint32_t *flags, double *sleep_timer,
                         double *inv_mass, double *inv_inertia);
 
+/* Downloads `Body::invMassStore` (n) and `Body::invInertiaStore` (n*3)
+ * (src/body.hpp:107-110): the masses a sleeping body gets back when it wakes
+ * (World::wake, src/world.cpp:317-323).  Either may be NULL. */
+int ne_gpu_get_sleep_store(ne_gpu_ctx *ctx, double *inv_mass_store, double *inv_inertia_store);
+
 /* `World::resetWarmCache()` (src/world.hpp:140). */
 int ne_gpu_reset_warm_cache(ne_gpu_ctx *ctx);
 
@@ -244,8 +249,9 @@ int ne_gpu_get_pairs(ne_gpu_ctx *ctx, uint32_t cap, uint32_t *count,
  * in index order.  p[3], L[3] and ke may be NULL. */
 int ne_gpu_diagnostics(ne_gpu_ctx *ctx, double *p, double *L, double *ke);
 
-/* Per-step counters of the last step: [candidate pairs, hit pairs,
- * constraints, islands, largest island constraints, solver levels]. */
+/* Counters of the last step: [candidate pairs, mesh triangle candidates,
+ * constraints, islands, constraints in the largest island, gridded bodies,
+ * grid cells, fault bits]. */
 int ne_gpu_get_counters(ne_gpu_ctx *ctx, uint32_t out[8]);
 
 /* The HIP stream the context launches on (as a void*), for event timing. */

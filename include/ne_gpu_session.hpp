@@ -65,6 +65,11 @@ public:
 private:
     ne_gpu_ctx* ctx_ = nullptr;
     std::vector<double> x_, v_, q_, w_, f_, t_, im_, ii_, r_, hh_, he_, fr_, re_, ld_, ad_, gs_, hull_;
+    std::vector<double> st_, ims_, iis_;                  // sleep timer, stored inverse mass / inertia
+    std::vector<ne_mesh> meshes_;
+    std::vector<std::vector<double>> meshVerts_;
+    std::vector<std::vector<std::int32_t>> meshTris_;
+    std::vector<std::uint32_t> fl_;
     std::vector<std::int32_t> shape_;
     std::vector<std::uint32_t> layer_, mask_, flags_, hoff_, noff_, nc_;
     std::vector<ne_distance_joint> joints_;
@@ -102,7 +107,8 @@ private:
         const size_t n = w.bodies.size();
         for (auto* a : {&x_, &v_, &w_, &f_, &t_, &ii_, &he_}) a->assign(3 * n, 0.0);
         q_.assign(4 * n, 0.0);
-        for (auto* a : {&im_, &r_, &hh_, &fr_, &re_, &ld_, &ad_, &gs_}) a->assign(n, 0.0);
+        for (auto* a : {&im_, &r_, &hh_, &fr_, &re_, &ld_, &ad_, &gs_, &st_, &ims_}) a->assign(n, 0.0);
+        iis_.assign(3 * n, 0.0);
         shape_.assign(n, 0);
         layer_.assign(n, 0); mask_.assign(n, 0); flags_.assign(n, 0);
         hoff_.assign(n + 1, 0); noff_.assign(n + 1, 0);
@@ -124,6 +130,7 @@ private:
             hoff_[i + 1] = static_cast<std::uint32_t>(hull_.size() / 3);
             for (auto k : b.noCollide) nc_.push_back(k);
             noff_[i + 1] = static_cast<std::uint32_t>(nc_.size());
+            putSleep(b, i, 0);
         }
         if (hull_.empty()) hull_.assign(3, 0.0);
         if (nc_.empty()) nc_.assign(1, 0u);
@@ -138,9 +145,60 @@ private:
         B.layer = layer_.data(); B.mask = mask_.data(); B.flags = flags_.data();
         B.hull_offsets = hoff_.data(); B.hull_verts = hull_.data();
         B.nocollide_offsets = noff_.data(); B.nocollide = nc_.data();
+        B.sleep_timer = st_.data(); B.inv_mass_store = ims_.data(); B.inv_inertia_store = iis_.data();
         check(ne_gpu_set_bodies(ctx_, &B));
         uploadJoints(w, 0);
+        uploadMeshes(w, 0);
     }
+
+    // Body::sleepTimer / invMassStore / invInertiaStore (src/body.hpp:107-110),
+    // when the body type has them
+    template <class B>
+    auto putSleep(const B& b, size_t i, int) -> decltype(b.sleepTimer, b.invMassStore, b.invInertiaStore.x, void()) {
+        st_[i] = b.sleepTimer; ims_[i] = b.invMassStore; put3(iis_, i, b.invInertiaStore);
+    }
+    template <class B>
+    void putSleep(const B&, size_t, long) {}
+    template <class B>
+    auto getSleep(B& b, size_t i, int) -> decltype(b.sleepTimer, b.invMassStore, b.invInertiaStore.x, void()) {
+        b.sleeping = (fl_[i] & NE_FLAG_SLEEPING) != 0;
+        b.sleepTimer = st_[i];
+        b.invMass = im_[i];
+        b.invInertiaBody.x = ii_[3 * i]; b.invInertiaBody.y = ii_[3 * i + 1]; b.invInertiaBody.z = ii_[3 * i + 2];
+        b.invMassStore = ims_[i];
+        b.invInertiaStore.x = iis_[3 * i]; b.invInertiaStore.y = iis_[3 * i + 1]; b.invInertiaStore.z = iis_[3 * i + 2];
+    }
+    template <class B>
+    void getSleep(B&, size_t, long) {}
+
+    // Body::mesh (src/body.hpp:62) of the Shape::Mesh bodies, when the body
+    // type has it.  The bodies upload drops the meshes, so they go up again.
+    template <class W>
+    auto uploadMeshes(W& w, int) -> decltype(w.bodies[0].mesh->verts[0].x, w.bodies[0].mesh->tris[0].a, void()) {
+        meshes_.clear(); meshVerts_.clear(); meshTris_.clear();
+        for (size_t i = 0; i < w.bodies.size(); ++i) {
+            const auto& b = w.bodies[i];
+            if (static_cast<int>(b.shape) != NE_SHAPE_MESH || !b.mesh) continue;
+            std::vector<double> vv;
+            std::vector<std::int32_t> tt;
+            for (const auto& p : b.mesh->verts) { vv.push_back(p.x); vv.push_back(p.y); vv.push_back(p.z); }
+            for (const auto& t : b.mesh->tris) { tt.push_back(t.a); tt.push_back(t.b); tt.push_back(t.c); }
+            meshVerts_.push_back(std::move(vv));
+            meshTris_.push_back(std::move(tt));
+            ne_mesh m{};
+            m.body = static_cast<std::uint32_t>(i);
+            m.nverts = static_cast<std::uint32_t>(b.mesh->verts.size());
+            m.ntris = static_cast<std::uint32_t>(b.mesh->tris.size());
+            meshes_.push_back(m);
+        }
+        for (size_t k = 0; k < meshes_.size(); ++k) {
+            meshes_[k].verts = meshVerts_[k].data();
+            meshes_[k].tris = meshTris_[k].data();
+        }
+        if (!meshes_.empty()) check(ne_gpu_set_meshes(ctx_, static_cast<std::uint32_t>(meshes_.size()), meshes_.data()));
+    }
+    template <class W>
+    void uploadMeshes(W&, long) {}
 
     // World::distanceJoints (src/world.hpp:20-26, 129), when the type has them
     template <class W>
@@ -164,6 +222,9 @@ private:
     void download(W& w) {
         const size_t n = w.bodies.size();
         check(ne_gpu_get_state(ctx_, x_.data(), v_.data(), q_.data(), w_.data()));
+        fl_.assign(n, 0u);
+        check(ne_gpu_get_body_aux(ctx_, fl_.data(), st_.data(), im_.data(), ii_.data()));
+        check(ne_gpu_get_sleep_store(ctx_, ims_.data(), iis_.data()));
         for (size_t i = 0; i < n; ++i) {
             auto& b = w.bodies[i];
             b.prevX = b.x;            // world.cpp:135 snapshot (pre-step pose)
@@ -174,6 +235,7 @@ private:
             b.q.w = q_[4 * i]; b.q.x = q_[4 * i + 1]; b.q.y = q_[4 * i + 2]; b.q.z = q_[4 * i + 3];
             b.force.x = b.force.y = b.force.z = 0.0;   // integrate clears the accumulators
             b.torque.x = b.torque.y = b.torque.z = 0.0;
+            getSleep(b, i, 0);
         }
     }
 

@@ -113,6 +113,17 @@ struct Box {
 
 enum class Shape { Sphere, Cylinder, Box, Capsule, Convex, Plane, Mesh, Compound };
 
+// A static triangle mesh (src/mesh.hpp:14-25): vertices and index triples in
+// the mesh body's frame.  The GPU library builds the bounding-volume tree
+// itself when the mesh is uploaded (by the reference's median split), so
+// build() has nothing to do here and the tree is not kept on the host.
+struct MeshData {
+    struct Tri { int a, b, c; };
+    std::vector<V3> verts;
+    std::vector<Tri> tris;
+    void build() {}
+};
+
 struct Body {
     V3 x, v;
     Q q{1, 0, 0, 0};
@@ -132,6 +143,7 @@ struct Body {
     bool dynamic = true, sleeping = false;
     double sleepTimer = 0.0, invMassStore = 0.0;
     V3 invInertiaStore;
+    std::shared_ptr<const MeshData> mesh;   // Shape::Mesh only
     bool kinematic = false, sensor = false, ccd = false;
     V3 prevX;
     Q prevQ{1, 0, 0, 0};
@@ -212,6 +224,12 @@ inline Body makeConvex(int id, const V3& x, const Q& q, std::vector<V3> verts, d
 }
 inline Body makePlane(int id, const V3& point, const V3& normal) {
     Body b; b.shape = Shape::Plane; b.id = id; b.x = point; b.halfExtents = normal.normalized();
+    b.invMass = 0.0; b.invInertiaBody = {0, 0, 0}; b.dynamic = false;
+    return b;
+}
+// A static mesh placed by (point, q) (src/body.hpp:319-330).
+inline Body makeMesh(int id, const V3& point, const Q& q, std::shared_ptr<const MeshData> data) {
+    Body b; b.shape = Shape::Mesh; b.id = id; b.x = point; b.q = q; b.mesh = std::move(data);
     b.invMass = 0.0; b.invInertiaBody = {0, 0, 0}; b.dynamic = false;
     return b;
 }

@@ -121,6 +121,7 @@ enum CounterSlot : int {
     CNT_EPAQ = 14,       // convex pairs queued for EPA (this step)
     CNT_MQRAW = 16,      // mesh triangle candidates before clamping (this step)
     CNT_MQ = 17,         // mesh triangle candidates (clamped)
+    CNT_WFT = 18,        // tasks (contact pairs) of the large islands (this step)
     CNT_N = 20
 };
 
@@ -1930,6 +1931,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve_small(BodiesDev B, SimP
         if (m == 1) continue;   // k_solve_single
         if (m > SMALL_ISLAND) {
             big_list[atomicAdd(&counters[CNT_NBIG], 1u)] = isl;
+            atomicMax(&counters[CNT_MAXISL], m);
             continue;
         }
         uint32_t* list = isl_list + s;
@@ -1978,7 +1980,6 @@ __global__ void __launch_bounds__(1024) k_solve_large(BodiesDev B, SimParams P, 
         const uint32_t isl = big_list[b];
         const uint32_t s = isl_start[isl], e = isl_start[isl + 1];
         const uint32_t m = e - s;
-        if (threadIdx.x == 0) atomicMax(&counters[CNT_MAXISL], m);
         uint32_t* list = isl_list + s;
         // bitonic network with every comparator ascending (a "flip" stage then
         // half-cleaners), so the virtual +inf padding past m never moves
@@ -2003,6 +2004,176 @@ __global__ void __launch_bounds__(1024) k_solve_large(BodiesDev B, SimParams P, 
             if (P.warm) publish_cache(C, list, m, prev_key, prev_imp);
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Islands of more than SMALL_ISLAND constraints: the sweep as a dataflow of
+// (pass, pair) visits, one round of independent visits per launch.
+//
+// A task is one pair's run of constraints (consecutive in (i, j, slot)
+// order); a visit is one pass over one task: pass 0 the warm start
+// (world.cpp:559-568), passes 1..I the iterations (572-612), pass I+1 the
+// positional correction (615-616).  On each movable body the tasks of a pass
+// are chained in constraint order, and the body's first task of pass p+1
+// follows its last task of pass p.  A visit runs in the round after both of
+// its predecessors ran, so the visits of one round touch disjoint movable
+// bodies and every body receives exactly the sequential sweep's updates in
+// the sequential order: the result is the reference's bit for bit.  Static
+// bodies (nothing writes them) chain nothing.
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t WF_WRAP = 0x80000000u;   // successor is the body's first task of the next pass
+constexpr uint32_t WF_NONE = 0xFFFFFFFFu;   // no successor on this side (a static body)
+constexpr uint32_t WF_MAX_ROUNDS = 1u << 16;
+constexpr int WF_THREADS = 128;
+constexpr uint32_t WF_PASS_MAX = 66;        // warm start + up to 64 iterations + correction
+
+struct WfDev {
+    uint32_t* flag;        // per constraint: 1 if it starts a task of a large island
+    uint32_t* tid;         // per constraint: exclusive scan of flag (its task index)
+    uint32_t* task_k;      // per task: first constraint
+    uint32_t* next;        // per task and side (0 = body i, 1 = body j): successor task, WF_WRAP | first, WF_NONE
+    uint32_t* pend0;       // per task: predecessors in pass 0
+    uint32_t* pend;        // per visit (pass * T + task): predecessors not yet run
+    uint32_t* fr;          // visits in the order their rounds ran them
+    uint32_t* rcnt;        // per round: visits queued for it
+    uint32_t* roff;        // per round: index in fr of its first visit
+    uint32_t* inc_cnt;     // per body: tasks touching it (zero between steps)
+    uint32_t* inc_start;   // per body: start of its slice of inc_list
+    uint32_t* inc_list;    // per movable body: its tasks
+};
+
+__global__ void __launch_bounds__(TPB) k_wf_mark(BodiesDev B, const uint32_t* __restrict__ counters, ConsDev C,
+                                                 const uint32_t* __restrict__ cons_isl,
+                                                 const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ flag) {
+    const uint32_t nc = counters[CNT_C];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        const bool start = k == 0 || C.ci[k] != C.ci[k - 1] || C.cj[k] != C.cj[k - 1];
+        const uint32_t id = cons_isl[k];
+        flag[k] = (start && isl_start[id + 1] - isl_start[id] > SMALL_ISLAND) ? 1u : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_wf_tasks(BodiesDev B, const uint32_t* __restrict__ counters, ConsDev C,
+                                                  WfDev W) {
+    const uint32_t nc = counters[CNT_C];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        if (!W.flag[k]) continue;
+        W.task_k[W.tid[k]] = k;
+        const uint32_t i = C.ci[k], j = C.cj[k];
+        if (mutable_body(B, i)) atomicAdd(&W.inc_cnt[i], 1u);
+        if (mutable_body(B, j)) atomicAdd(&W.inc_cnt[j], 1u);
+    }
+}
+
+// inc_cnt was zeroed by the scan that made inc_start: it counts the fill here.
+__global__ void __launch_bounds__(TPB) k_wf_fill(BodiesDev B, uint32_t T, ConsDev C, WfDev W) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+        const uint32_t k = W.task_k[t];
+        const uint32_t i = C.ci[k], j = C.cj[k];
+        if (mutable_body(B, i)) W.inc_list[W.inc_start[i] + atomicAdd(&W.inc_cnt[i], 1u)] = t;
+        if (mutable_body(B, j)) W.inc_list[W.inc_start[j] + atomicAdd(&W.inc_cnt[j], 1u)] = t;
+    }
+}
+
+// Per movable body: its tasks in order, chained; the last links to the first
+// of the next pass.  Leaves inc_cnt zero for the next step.
+__global__ void __launch_bounds__(TPB) k_wf_link(uint32_t n, ConsDev C, WfDev W) {
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += gridDim.x * blockDim.x) {
+        const uint32_t m = W.inc_cnt[x];
+        if (m == 0) continue;
+        W.inc_cnt[x] = 0u;
+        uint32_t* a = W.inc_list + W.inc_start[x];
+        sort_segment(a, m);
+        for (uint32_t u = 0; u < m; ++u) {
+            const uint32_t t = a[u];
+            const uint32_t side = C.ci[W.task_k[t]] == x ? 0u : 1u;
+            W.next[2 * t + side] = (u + 1 < m) ? a[u + 1] : (WF_WRAP | a[0]);
+            if (u > 0) atomicAdd(&W.pend0[t], 1u);
+        }
+    }
+}
+
+// Pending counts of every visit; the pass-0 visits with no predecessor are
+// round 0.
+__global__ void __launch_bounds__(TPB) k_wf_init(BodiesDev B, uint32_t T, uint32_t npass, ConsDev C, WfDev W) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) W.roff[0] = 0u;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+        const uint32_t k = W.task_k[t];
+        const uint32_t sides = (mutable_body(B, C.ci[k]) ? 1u : 0u) + (mutable_body(B, C.cj[k]) ? 1u : 0u);
+        const uint32_t p0 = W.pend0[t];
+        W.pend[t] = p0;
+        for (uint32_t p = 1; p < npass; ++p) W.pend[(size_t)p * T + t] = sides;
+        if (p0 == 0u) W.fr[atomicAdd(&W.rcnt[0], 1u)] = t;
+    }
+}
+
+// One pass over one task's constraints [k0, k1): bodies and impulses stay in
+// registers across the run, as in the pair solver.
+__device__ __forceinline__ void wf_visit(BodiesDev& B, const SimParams& P, const ConsDev& C, uint32_t pass, uint32_t k0,
+                                         uint32_t k1, uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
+    const uint32_t i = C.ci[k0], j = C.cj[k0];
+    if (pass == 0) {
+        if (!P.warm) return;
+        BodyView a = load_view(B, i), b = load_view(B, j);
+        for (uint32_t k = k0; k < k1; ++k) {
+            const double imp = C.nimp[k];
+            if (imp == 0.0) continue;
+            warm_apply(a, b, load_geom(C, k), imp);
+        }
+        store_view(B, i, a);
+        store_view(B, j, b);
+    } else if (pass <= (uint32_t)P.iters) {
+        BodyView a = load_view(B, i), b = load_view(B, j);
+        for (uint32_t k = k0; k < k1; ++k) {
+            double nimp = C.nimp[k], timp = C.timp[k];
+            gs_visit(a, b, load_geom(C, k), C.rbias[k], C.kn[k], C.mu[k], nimp, timp);
+            C.nimp[k] = nimp;
+            C.timp[k] = timp;
+        }
+        store_view(B, i, a);
+        store_view(B, j, b);
+    } else {
+        V3 xa = ld3(B.x0, B.x1, B.x2, i), xb = ld3(B.x0, B.x1, B.x2, j);
+        const double ima = B.im[i], imb = B.im[j];
+        for (uint32_t k = k0; k < k1; ++k) correct(B, C, k, P.beta, xa, xb, ima, imb);
+        if (ima != 0.0) st3(B.x0, B.x1, B.x2, i, xa);
+        if (imb != 0.0) st3(B.x0, B.x1, B.x2, j, xb);
+        if (P.warm)
+            for (uint32_t k = k0; k < k1; ++k) { prev_key[k] = C.key[k]; prev_imp[k] = C.nimp[k]; }
+    }
+}
+
+// Round r: run its visits, then queue each successor whose last predecessor
+// this was for round r + 1.
+__global__ void __launch_bounds__(WF_THREADS) k_wf_round(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters,
+                                                        ConsDev C, WfDev W, uint32_t T, uint32_t npass, uint32_t r,
+                                                        uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
+    const uint32_t base = W.roff[r], cnt = W.rcnt[r];
+    const uint32_t nbase = base + cnt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) W.roff[r + 1] = nbase;
+    const uint32_t nc = counters[CNT_C];
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < cnt; u += gridDim.x * blockDim.x) {
+        const uint32_t v = W.fr[base + u];
+        const uint32_t pass = v / T, t = v - pass * T;
+        const uint32_t k0 = W.task_k[t];
+        const uint32_t i = C.ci[k0], j = C.cj[k0];
+        uint32_t k1 = k0 + 1;
+        while (k1 < nc && C.ci[k1] == i && C.cj[k1] == j) ++k1;
+        wf_visit(B, P, C, pass, k0, k1, prev_key, prev_imp);
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t nx = W.next[2 * t + side];
+            if (nx == WF_NONE) continue;
+            uint32_t tv;
+            if (nx & WF_WRAP) {
+                if (pass + 1 >= npass) continue;
+                tv = (pass + 1) * T + (nx & ~WF_WRAP);
+            } else {
+                tv = pass * T + nx;
+            }
+            if (atomicSub(&W.pend[tv], 1u) == 1u) W.fr[nbase + atomicAdd(&W.rcnt[r + 1], 1u)] = tv;
+        }
     }
 }
 
@@ -2847,6 +3018,13 @@ struct ne_gpu_ctx {
     uint32_t stage_n = 0;
     size_t adj_tape_n = 0;
     uint32_t adj_ccap = 0, adj_ncap = 0, adj_nw = 0;
+    // large islands: the (pass, task) dataflow solve (eager steps; graphs keep k_solve_large)
+    WfDev wf{};
+    uint32_t wf_tcap = 0, wf_ccap = 0, wf_ncap = 0;   // capacities: tasks, constraints, bodies
+    uint32_t wf_vcap_pass = 0;                        // passes the visit arrays hold per task
+    uint32_t wf_rounds_hint = 0;                      // rounds the last solve took
+    uint32_t* h_wf = nullptr;                         // pinned: values read back mid-step
+    bool capturing = false;                           // a step is being captured into a graph
     // counters
     uint32_t* counters = nullptr;
     uint32_t* h_counters = nullptr;
@@ -3165,7 +3343,16 @@ void free_adjoint(ne_gpu_ctx* c) {
     c->roll_steps = 0;
 }
 
+void free_wf(ne_gpu_ctx* c) {
+    WfDev& W = c->wf;
+    dfree(W.flag); dfree(W.tid); dfree(W.task_k); dfree(W.next); dfree(W.pend0); dfree(W.pend); dfree(W.fr);
+    dfree(W.rcnt); dfree(W.roff); dfree(W.inc_cnt); dfree(W.inc_start); dfree(W.inc_list);
+    c->wf_tcap = c->wf_ccap = c->wf_ncap = c->wf_vcap_pass = 0;
+}
+
 void free_all(ne_gpu_ctx* c) {
+    free_wf(c);
+    if (c->h_wf) { (void)hipHostFree(c->h_wf); c->h_wf = nullptr; }
     free_adjoint(c);
     free_work(c);
     free_meshes(c);
@@ -3338,28 +3525,130 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
     k_isl_fill<<<hint_blocks(c->hint_c), TPB, 0, s>>>(c->counters, c->cons_isl, c->isl_start, c->isl_fill, c->isl_list);
 }
 
-void enqueue_solve(ne_gpu_ctx* c) {
+// Work space of the large-island solve, grown to the constraint capacity, the
+// body capacity and T tasks.
+int ensure_wf(ne_gpu_ctx* c, uint32_t T, uint32_t npass) {
+    WfDev& W = c->wf;
+    if (!c->h_wf) HIP_TRY(c, hipHostMalloc((void**)&c->h_wf, sizeof(uint32_t) * 4, hipHostMallocDefault));
+    if (!W.rcnt) {
+        HIP_TRY(c, dalloc(W.rcnt, (size_t)WF_MAX_ROUNDS + 2));
+        HIP_TRY(c, dalloc(W.roff, (size_t)WF_MAX_ROUNDS + 2));
+    }
+    if (c->wf_ccap < c->ccap) {
+        HIP_TRY(c, dalloc(W.flag, c->ccap));
+        HIP_TRY(c, dalloc(W.tid, (size_t)c->ccap + 1));
+        c->wf_ccap = c->ccap;
+    }
+    if (c->wf_ncap < c->ncap) {
+        HIP_TRY(c, dalloc(W.inc_cnt, c->ncap));
+        HIP_TRY(c, dalloc(W.inc_start, (size_t)c->ncap + 1));
+        c->wf_ncap = c->ncap;
+    }
+    if (T == 0) return NE_OK;
+    if ((uint64_t)T * npass >= (1ull << 31)) return set_err(c, NE_ERR_UNSUPPORTED, "too many contact pairs in large islands");
+    if (c->wf_tcap < T) {
+        const uint32_t cap = T + T / 2 + 1024;
+        HIP_TRY(c, dalloc(W.task_k, cap));
+        HIP_TRY(c, dalloc(W.next, 2 * (size_t)cap));
+        HIP_TRY(c, dalloc(W.pend0, cap));
+        HIP_TRY(c, dalloc(W.inc_list, 2 * (size_t)cap));
+        c->wf_tcap = cap;
+        c->wf_vcap_pass = 0;
+    }
+    if (c->wf_vcap_pass < npass) {
+        HIP_TRY(c, dalloc(W.pend, (size_t)c->wf_tcap * npass));
+        HIP_TRY(c, dalloc(W.fr, (size_t)c->wf_tcap * npass));
+        c->wf_vcap_pass = npass;
+    }
+    return NE_OK;
+}
+
+// Islands of more than SMALL_ISLAND constraints as a dataflow of (pass,
+// task) visits: one launch per round, the host reading back the number of
+// visits run after a batch of rounds (the schedule's depth is not known in
+// advance).  Two synchronisations per step.
+int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
+    hipStream_t s = c->stream;
+    BodiesDev B = c->B;
+    WfDev& W = c->wf;
+    int r0 = ensure_wf(c, 0, 1);
+    if (r0 != NE_OK) return r0;
+    k_fill_u32<<<gs_blocks(c->n), TPB, 0, s>>>(W.inc_cnt, 0u, nullptr, c->n);
+    k_wf_mark<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, c->cons_isl, c->isl_start, W.flag);
+    scan_u32(W.flag, W.tid, c->counters + CNT_C, c->ccap, c->scan_part, s, false,
+             ScanTotal{c->counters, CNT_WFT, CNT_WFT, CNT_FAULT, c->ccap, 0u, -1, -1});
+    HIP_TRY(c, hipMemcpyAsync(c->h_wf, c->counters + CNT_WFT, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    const uint32_t T = c->h_wf[0];
+    if (T == 0) return NE_OK;
+    const uint32_t npass = (uint32_t)P.iters + 2u;
+    if (npass > WF_PASS_MAX) return set_err(c, NE_ERR_UNSUPPORTED, "too many solver iterations for the large-island solve");
+    r0 = ensure_wf(c, T, npass);
+    if (r0 != NE_OK) return r0;
+    k_fill_u32<<<gs_blocks(2 * (size_t)T), TPB, 0, s>>>(W.next, WF_NONE, nullptr, 2 * T);
+    k_fill_u32<<<gs_blocks(T), TPB, 0, s>>>(W.pend0, 0u, nullptr, T);
+    uint32_t cleared = std::min<uint32_t>(WF_MAX_ROUNDS + 2, 2 * c->wf_rounds_hint + 256);
+    k_fill_u32<<<gs_blocks(cleared), TPB, 0, s>>>(W.rcnt, 0u, nullptr, cleared);
+    k_wf_tasks<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, W);
+    scan_u32(W.inc_cnt, W.inc_start, nullptr, c->n, c->scan_part, s, true);
+    k_wf_fill<<<gs_blocks(T), TPB, 0, s>>>(B, T, c->C, W);
+    k_wf_link<<<gs_blocks(c->n), TPB, 0, s>>>(c->n, c->C, W);
+    k_wf_init<<<gs_blocks(T), TPB, 0, s>>>(B, T, npass, c->C, W);
+    const uint64_t total = (uint64_t)T * npass;
+    const uint32_t hint = c->wf_rounds_hint ? c->wf_rounds_hint : 64u;
+    uint64_t per_round = total / hint + 1;
+    uint32_t grid = (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(8, (per_round * 3 / 2 + WF_THREADS - 1) / WF_THREADS));
+    uint32_t r = 0, batch = hint + 16;
+    for (;;) {
+        if (r + batch + 2 >= WF_MAX_ROUNDS) return set_err(c, NE_ERR_UNSUPPORTED, "large-island solve needs too many rounds");
+        if (r + batch + 2 > cleared) {   // round r fills the count of round r + 1
+            k_fill_u32<<<gs_blocks(r + batch + 2 - cleared), TPB, 0, s>>>(W.rcnt + cleared, 0u, nullptr,
+                                                                          r + batch + 2 - cleared);
+            cleared = r + batch + 2;
+        }
+        for (uint32_t q = 0; q < batch; ++q, ++r)
+            k_wf_round<<<grid, WF_THREADS, 0, s>>>(B, P, c->counters, c->C, W, T, npass, r, c->prev_key, c->prev_imp);
+        HIP_TRY(c, hipMemcpyAsync(c->h_wf, W.roff + r, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipMemcpyAsync(c->h_wf + 1, W.rcnt + r, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        if (c->h_wf[0] >= total) break;
+        if (c->h_wf[1] == 0)
+            return set_err(c, NE_ERR_INVALID, "large-island solve stalled (a visit never became ready)");
+        batch = 64;
+    }
+    // the depth: rounds up to the last one that ran a visit
+    std::vector<uint32_t> cnt(r);
+    HIP_TRY(c, hipMemcpy(cnt.data(), W.rcnt, sizeof(uint32_t) * r, hipMemcpyDeviceToHost));
+    uint32_t depth = r;
+    while (depth > 0 && cnt[depth - 1] == 0) --depth;
+    c->wf_rounds_hint = depth;
+    return NE_OK;
+}
+
+int enqueue_solve(ne_gpu_ctx* c) {
     hipStream_t s = c->stream;
     const SimParams P = make_sim_params(c->params);
     BodiesDev B = c->B;
-    {
-        KTimer kt(c, NE_KERNEL_SOLVE);
-        k_solve_single<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C,
-                                                              c->prev_key, c->prev_imp);
-        k_solve_small<<<std::min<uint32_t>(4096u, std::max<uint32_t>(64u, (c->hint_c * 2 + SOLVE_THREADS - 1) / SOLVE_THREADS)),
-                        SOLVE_THREADS, 0, s>>>(
-            B, P, c->counters, c->isl_start, c->isl_list, c->C,
-                                                         c->big_list, c->prev_key, c->prev_imp, c->prev_n);
+    KTimer kt(c, NE_KERNEL_SOLVE);
+    k_solve_single<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C,
+                                                          c->prev_key, c->prev_imp);
+    k_solve_small<<<std::min<uint32_t>(4096u, std::max<uint32_t>(64u, (c->hint_c * 2 + SOLVE_THREADS - 1) / SOLVE_THREADS)),
+                    SOLVE_THREADS, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list, c->prev_key,
+                                           c->prev_imp, c->prev_n);
+    if (c->capturing) {   // a graph cannot read back mid-step: one workgroup sweeps each large island
         k_solve_large<<<256, 1024, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list,
                                            c->prev_key, c->prev_imp);
+        return NE_OK;
     }
+    return solve_large_islands(c, P);
 }
 
 int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     if (c->n == 0) return NE_OK;
     KTimer whole(c, NE_KERNEL_STEP);
     enqueue_front(c, wrap_first);
-    enqueue_solve(c);
+    const int rs = enqueue_solve(c);
+    if (rs != NE_OK) return rs;
     const SimParams P = make_sim_params(c->params);
     if (c->nrigid)   // rigid joints after the contacts (world.cpp:139)
         k_joint_rigid<<<blocks_for(c->ncomp, 64), 64, 0, c->stream>>>(c->B, P, joint_dev(c));
@@ -3541,7 +3830,9 @@ int run_step(ne_gpu_ctx* c, bool wrap_first) {
         uint32_t* op = c->order_prev;
         const int par = c->order_parity;
         HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
         const int r = enqueue_step(c, wrap_first);
+        c->capturing = false;
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(c->stream, &graph);
         c->sorted_idx = si;   // nothing ran yet: undo the host-side bookkeeping
@@ -4141,6 +4432,25 @@ int ne_gpu_get_body_aux(ne_gpu_ctx* c, uint32_t* flags, double* sleep_timer, dou
     return NE_OK;
 }
 
+int ne_gpu_get_sleep_store(ne_gpu_ctx* c, double* inv_mass_store, double* inv_inertia_store) {
+    if (!c) return NE_ERR_INVALID;
+    const uint32_t n = c->n;
+    if (n == 0) return NE_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (inv_mass_store)
+        HIP_TRY(c, hipMemcpy(inv_mass_store, c->dfield + (size_t)F_IMS * c->ncap, sizeof(double) * n,
+                             hipMemcpyDeviceToHost));
+    if (inv_inertia_store) {
+        std::vector<double> tmp(n);
+        for (int k = 0; k < 3; ++k) {
+            HIP_TRY(c, hipMemcpy(tmp.data(), c->dfield + (size_t)(F_IIS0 + k) * c->ncap, sizeof(double) * n,
+                                 hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < n; ++i) inv_inertia_store[3 * i + k] = tmp[i];
+        }
+    }
+    return NE_OK;
+}
+
 int ne_gpu_set_meshes(ne_gpu_ctx* c, uint32_t n, const ne_mesh* ms) {
     if (!c || (n && !ms)) return NE_ERR_INVALID;
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
@@ -4363,7 +4673,7 @@ int ne_gpu_get_counters(ne_gpu_ctx* c, uint32_t out[8]) {
     uint32_t h[CNT_N];
     HIP_TRY(c, hipMemcpy(h, c->counters, sizeof(h), hipMemcpyDeviceToHost));
     out[0] = h[CNT_PRAW];
-    out[1] = 0;
+    out[1] = c->nmeshes ? h[CNT_MQRAW] : 0;
     out[2] = h[CNT_CRAW];
     out[3] = h[CNT_ISL];
     out[4] = h[CNT_MAXISL];

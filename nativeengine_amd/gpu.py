@@ -27,6 +27,7 @@ EXPORTS = [
     "ne_gpu_fp_probe", "ne_gpu_time_kernel", "ne_gpu_kernel_time", "ne_gpu_export_warm_cache",
     "ne_gpu_import_warm_cache", "ne_gpu_set_worlds", "ne_gpu_use_graph", "ne_gpu_autotune", "ne_gpu_epa_stats",
     "ne_gpu_rollout", "ne_gpu_rollout_backward", "ne_gpu_set_joints", "ne_gpu_set_meshes", "ne_gpu_mesh_bvh",
+    "ne_gpu_get_sleep_store",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
@@ -58,6 +59,7 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_step.argtypes = [vp, C.c_int32]
     lib.ne_gpu_get_state.argtypes = [vp, dp, dp, dp, dp]
     lib.ne_gpu_get_body_aux.argtypes = [vp, u32p, dp, dp, dp]
+    lib.ne_gpu_get_sleep_store.argtypes = [vp, dp, dp]
     lib.ne_gpu_set_meshes.argtypes = [vp, C.c_uint32, C.POINTER(_abi.Mesh)]
     lib.ne_gpu_mesh_bvh.argtypes = [vp, C.c_uint32, C.POINTER(C.c_int32), C.c_uint32, u32p, dp, C.POINTER(C.c_int32)]
     lib.ne_gpu_reset_warm_cache.argtypes = [vp]
@@ -240,6 +242,20 @@ class GpuWorld:
         self._check(self.lib.ne_gpu_get_state(self.h, _abi.dptr(x), _abi.dptr(v), _abi.dptr(q), _abi.dptr(w)))
         return x, v, q, w
 
+    def body_aux(self):
+        """(flags, sleep timer, stored inverse mass, stored inverse inertia (n, 3))
+        of every body: Body::flags, sleepTimer, invMassStore, invIStore."""
+        n = self.n
+        fl, tm, im, ii = np.zeros(n, np.uint32), np.zeros(n), np.zeros(n), np.zeros((n, 3))
+        self._check(self.lib.ne_gpu_get_body_aux(self.h, _abi.uptr(fl), _abi.dptr(tm), _abi.dptr(im), _abi.dptr(ii)))
+        return fl, tm, im, ii
+
+    def sleep_store(self):
+        """(Body::invMassStore (n), Body::invInertiaStore (n, 3)) of every body."""
+        im, ii = np.zeros(self.n), np.zeros((self.n, 3))
+        self._check(self.lib.ne_gpu_get_sleep_store(self.h, _abi.dptr(im), _abi.dptr(ii)))
+        return im, ii
+
     def contacts(self):
         """Last step's constraint list: dict of i, j, normal, point, depth, impulse."""
         cnt = C.c_uint32(0)
@@ -276,7 +292,7 @@ class GpuWorld:
     def counters(self):
         out = np.zeros(8, np.uint32)
         self._check(self.lib.ne_gpu_get_counters(self.h, _abi.uptr(out)))
-        keys = ["candidate_pairs", "hit_pairs", "constraints", "islands", "max_island", "gridded", "cells", "fault"]
+        keys = ["candidate_pairs", "mesh_triangles", "constraints", "islands", "max_island", "gridded", "cells", "fault"]
         return dict(zip(keys, map(int, out)))
 
     def stream(self) -> int:

@@ -913,3 +913,122 @@ def tilted_mesh_pile(n: int = 40, seed: int = 0, half: float = 8.0, periodic: bo
     s.x[0] = (0.3, 0.4, -0.2)
     s.q[0] = (math.cos(ang / 2), 0.0, 0.0, math.sin(ang / 2))
     return s
+
+
+def large_scene(n: int = 1 << 20, seed: int = 5, height: float = 100.0, amplitude: float = 8.0,
+                iterations: int = 10, sleep: bool = True) -> Scene:
+    """The large terrain scene: `n` bodies (the heightfield mesh is body 0) in
+    a periodic side x height x side domain over a static heightfield.
+
+    At n = 2^20 the domain is 400 x 100 x 400 and the heightfield 1024 x 1024
+    vertices; smaller n shrink side and vertex count with sqrt(n) so the
+    density stays the same.  Height is 4 octaves of seeded value noise of
+    amplitude `amplitude`.  Of the n - 1 dynamic bodies, 70% are boxes in
+    stacked columns of 8 (half extents U[0.25,0.5] across, U[0.15,0.35]
+    tall, each column on the terrain), 5% are spheres of radius 0.2 in
+    16-link ropes joined by rigid distance joints (surface anchors, rest
+    0.1), and the rest are spheres of radius U[0.2,0.5] on a lattice above
+    the columns with velocities N(0, 0.5).  Density 1, friction 0.6,
+    restitution 0.1, gravity -9.81, dt 1/60, island sleeping on,
+    `iterations` solver iterations."""
+    from scipy.ndimage import maximum_filter
+    rng = np.random.default_rng(seed)
+    scale = math.sqrt(n / float(1 << 20))
+    side = 400.0 * scale
+    nv = max(8, int(round(1024 * scale)))
+    ndyn = n - 1
+    n_rope = (int(0.05 * ndyn) // 16) * 16
+    n_box = (int(0.70 * ndyn) // 8) * 8
+    n_sph = ndyn - n_rope - n_box
+    hy = height / 2.0
+
+    # terrain: vertex (r, c) at (-side/2 + c*sp, base + h, -side/2 + r*sp)
+    sp_t = side / (nv - 1)
+    h = value_noise(nv, nv, seed=seed + 1000, amplitude=amplitude)
+    base = -hy + amplitude + 2.0
+    verts, tris = heightfield_mesh((-side / 2.0, base, -side / 2.0), sp_t, h)
+    reach = int(math.ceil(0.8 / sp_t))
+    hmax = maximum_filter(h, size=2 * reach + 1, mode="nearest")
+
+    s = Scene(n)
+    s.shape[0] = _abi.SHAPE_MESH
+    s.inv_mass[0] = 0.0
+    s.flags[0] = 0
+    s.radius[0] = 1.0
+
+    # box columns on a lattice of sites, each sitting on the terrain
+    ncol = n_box // 8
+    m = int(math.ceil(math.sqrt(ncol)))
+    sp_c = side / m
+    sites = rng.permutation(m * m)[:ncol]
+    cx = -side / 2.0 + (sites % m + 0.5) * sp_c + rng.uniform(-0.1, 0.1, ncol)
+    cz = -side / 2.0 + (sites // m + 0.5) * sp_c + rng.uniform(-0.1, 0.1, ncol)
+    ci = np.clip(np.rint((cz + side / 2.0) / sp_t).astype(int), 0, nv - 1)
+    cj = np.clip(np.rint((cx + side / 2.0) / sp_t).astype(int), 0, nv - 1)
+    ground = base + hmax[ci, cj]
+    he = np.empty((ncol, 8, 3))
+    he[:, :, 0] = rng.uniform(0.25, 0.5, (ncol, 8))
+    he[:, :, 1] = rng.uniform(0.15, 0.35, (ncol, 8))
+    he[:, :, 2] = rng.uniform(0.25, 0.5, (ncol, 8))
+    tops = np.cumsum(2.0 * he[:, :, 1] + 0.02, axis=1)
+    cy = ground[:, None] + 0.05 + tops - he[:, :, 1] - 0.02
+    b0, b1 = 1, 1 + n_box
+    s.shape[b0:b1] = _abi.SHAPE_BOX
+    s.x[b0:b1, 0] = np.repeat(cx, 8)
+    s.x[b0:b1, 1] = cy.ravel()
+    s.x[b0:b1, 2] = np.repeat(cz, 8)
+    s.half_extents[b0:b1] = he.reshape(-1, 3)
+    s.inv_mass[b0:b1], s.inv_inertia[b0:b1] = _box_mass(he.reshape(-1, 3), 1.0)
+
+    # loose spheres on a lattice above the columns
+    col_top = float(ground.max() + tops[:, -1].max()) + 1.5
+    ms = int(side / 1.2)
+    per_layer = ms * ms
+    layers = -(-n_sph // per_layer)
+    slots = rng.permutation(per_layer * layers)[:n_sph]
+    lay, rem = slots // per_layer, slots % per_layer
+    s0, s1 = b1, b1 + n_sph
+    r = rng.uniform(0.2, 0.5, n_sph)
+    s.shape[s0:s1] = _abi.SHAPE_SPHERE
+    s.radius[s0:s1] = r
+    s.x[s0:s1, 0] = -side / 2.0 + (rem % ms + 0.5) * (side / ms) + rng.uniform(-0.05, 0.05, n_sph)
+    s.x[s0:s1, 1] = col_top + 1.2 * lay + rng.uniform(-0.05, 0.05, n_sph)
+    s.x[s0:s1, 2] = -side / 2.0 + (rem // ms + 0.5) * (side / ms) + rng.uniform(-0.05, 0.05, n_sph)
+    s.v[s0:s1] = rng.normal(0.0, 0.5, (n_sph, 3))
+    im, ii = _sphere_mass(r, 1.0)
+    s.inv_mass[s0:s1] = im
+    s.inv_inertia[s0:s1] = ii[:, None]
+
+    # ropes: 16 spheres of radius 0.2 along x, 0.5 apart, above the spheres
+    nr = n_rope // 16
+    rx = max(1, int(side // 9.0))
+    rz = max(1, int(side / 1.2))
+    rope_y = col_top + 1.2 * layers + 1.5
+    rlayers = -(-nr // (rx * rz))
+    rslots = rng.permutation(rx * rz * rlayers)[:nr]
+    rl, rrem = rslots // (rx * rz), rslots % (rx * rz)
+    x0 = -side / 2.0 + (rrem % rx) * (side / rx) + 0.5
+    z0 = -side / 2.0 + (rrem // rx + 0.5) * (side / rz)
+    r0 = 1 + n_box + n_sph
+    rr = 0.2
+    k = np.arange(16)
+    s.shape[r0:] = _abi.SHAPE_SPHERE
+    s.radius[r0:] = rr
+    s.x[r0:, 0] = (x0[:, None] + 0.5 * k[None, :]).ravel()
+    s.x[r0:, 1] = np.repeat(rope_y + 1.2 * rl, 16)
+    s.x[r0:, 2] = np.repeat(z0, 16)
+    im, ii = _sphere_mass(np.full(n_rope, rr), 1.0)
+    s.inv_mass[r0:] = im
+    s.inv_inertia[r0:] = ii[:, None]
+    a = (r0 + np.arange(nr)[:, None] * 16 + np.arange(15)[None, :]).ravel()
+    j = np.zeros(len(a), _abi.JOINT_DTYPE)
+    j["a"], j["b"] = a, a + 1
+    j["local_a"] = (rr, 0.0, 0.0)
+    j["local_b"] = (-rr, 0.0, 0.0)
+    j["rest"] = 0.1
+    s.joints = j
+
+    s.meshes = [dict(body=0, verts=verts, tris=tris)]
+    s.params = Params(half=(side / 2.0, hy, side / 2.0), periodic=True, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0),
+                      restitution=0.1, friction=0.6, solver_iterations=iterations, sleep_enabled=sleep)
+    return s

@@ -176,9 +176,81 @@ static void joints() {   // selftest.cpp:779-823
     }
 }
 
-// scene IN OUT K: IN holds n then per body 36 doubles (x3 v3 q4 w3 invMass
-// invI3 shape radius halfHeight he3 friction restitution layer mask flags
-// pad...), followed by the world parameters; OUT receives x v q w.
+static World floorWorld() {
+    World w;
+    w.box.half = 1e6; w.box.periodic = false; w.dt = 1.0 / 240.0; w.gravity = V3{0, -10, 0};
+    w.restitution = 0.0;
+    return w;
+}
+
+static Body staticBox(int id, const V3& x, const V3& he) {
+    Body b = makeBox(id, x, Q{1, 0, 0, 0}, he, 1.0);
+    b.invMass = 0.0; b.invInertiaBody = V3{0, 0, 0}; b.dynamic = false;
+    return b;
+}
+
+static void sleeping() {   // selftest.cpp:703-738, compat-mode step() each step
+    World w = floorWorld();
+    w.sleepEnabled = true;
+    w.bodies = {staticBox(0, V3{0, -1, 0}, V3{20, 1, 20}), makeSphere(1, V3{0, 3.0, 0}, 1.0, 1.0)};
+    for (int s = 0; s < 2000; ++s) w.step();
+    check(w.bodies[1].sleeping, "sleeping: a settled body falls asleep");
+    check(w.bodies[1].invMass == 0.0 && w.bodies[1].invMassStore > 0.0, "sleeping: its mass is stored while asleep");
+    const V3 rest = w.bodies[1].x;
+    for (int s = 0; s < 2000; ++s) w.step();
+    check((w.bodies[1].x - rest).norm() < 1e-6, "sleeping: an asleep body does not move");
+    World::wake(w.bodies[1]);                       // PhysicsWorld::applyImpulse
+    w.bodies[1].v += V3{5, 0, 0} * w.bodies[1].invMass;
+    check(!w.bodies[1].sleeping, "sleeping: the impulse wakes the body");
+    w.step();
+    check(w.bodies[1].v.x > 0.1, "sleeping: the woken body responds to the impulse");
+
+    World w2 = floorWorld();
+    w2.sleepEnabled = true;
+    w2.bodies = {staticBox(0, V3{0, -1, 0}, V3{20, 1, 20}), makeBox(1, V3{0, 1.0, 0}, Q{1, 0, 0, 0}, V3{1, 1, 1}, 1.0)};
+    w2.stepMany(2000);
+    check(w2.bodies[1].sleeping, "sleeping: the lower box sleeps");
+    w2.bodies.push_back(makeBox(2, V3{0, 6.0, 0}, Q{1, 0, 0, 0}, V3{1, 1, 1}, 1.0));
+    bool woke = false;
+    for (int s = 0; s < 2000 && !woke; ++s) { w2.step(); woke = !w2.bodies[1].sleeping; }
+    check(woke, "sleeping: a falling box wakes the sleeper it lands on");
+}
+
+static void meshes() {   // selftest.cpp:1300-1326, compat-mode step() each step
+    World w = floorWorld();
+    w.box.half = 1000.0; w.friction = 0.5;
+    auto quad = std::make_shared<MeshData>();
+    quad->verts = {{-10, 0, -10}, {10, 0, -10}, {10, 0, 10}, {-10, 0, 10}};
+    quad->tris = {{0, 1, 2}, {0, 2, 3}};
+    quad->build();
+    w.bodies = {makeMesh(0, V3{0, 0, 0}, Q{1, 0, 0, 0}, quad), makeSphere(1, V3{1, 5, 1}, 0.5, 1.0),
+                makeBox(2, V3{-3, 5, -2}, Q{1, 0, 0, 0}, V3{0.5, 0.5, 0.5}, 1.0)};
+    for (int s = 0; s < 1500; ++s) w.step();
+    check(std::fabs(w.bodies[1].x.y - 0.5) <= 0.05, "mesh: sphere rests on the mesh floor");
+    check(w.bodies[1].v.norm() < 0.15, "mesh: sphere comes to rest");
+    check(std::fabs(w.bodies[2].x.y - 0.5) <= 0.08, "mesh: box rests on the mesh floor");
+
+    // a flat 5 x 5 heightfield (PhysicsWorld::addHeightfield, physics_world.hpp:96-109)
+    World h = floorWorld();
+    h.box.half = 1000.0; h.friction = 0.5;
+    auto hf = std::make_shared<MeshData>();
+    for (int r = 0; r < 5; ++r)
+        for (int c = 0; c < 5; ++c) hf->verts.push_back(V3{-4.0 + c * 2.0, 0.0, -4.0 + r * 2.0});
+    for (int r = 0; r + 1 < 5; ++r)
+        for (int c = 0; c + 1 < 5; ++c) {
+            const int i00 = r * 5 + c, i01 = i00 + 1, i10 = i00 + 5, i11 = i10 + 1;
+            hf->tris.push_back({i00, i10, i11});
+            hf->tris.push_back({i00, i11, i01});
+        }
+    h.bodies = {makeMesh(0, V3{0, 0, 0}, Q{1, 0, 0, 0}, hf), makeSphere(1, V3{0, 5, 0}, 0.5, 1.0)};
+    h.stepMany(1500);
+    check(std::fabs(h.bodies[1].x.y - 0.5) <= 0.06, "heightfield: sphere rests on the terrain");
+}
+
+// scene IN OUT K: IN holds 24 doubles of world parameters (n first), then
+// per body 26 doubles (x3 v3 q4 w3 invMass invI3 shape radius halfHeight he3
+// friction restitution hull-vertex count) and its hull vertices, then the
+// meshes; OUT receives x v q w, the sleeping flag, sleep timer and invMass.
 static int run_scene(const char* in, const char* out, int k) {
     std::ifstream f(in, std::ios::binary);
     if (!f) return 2;
@@ -192,6 +264,8 @@ static int run_scene(const char* in, const char* out, int k) {
     w.restitutionCombine = static_cast<World::Combine>((int)hdr[13]);
     w.frictionCombine = static_cast<World::Combine>((int)hdr[14]);
     w.warmStart = hdr[15] != 0; w.solverIterations = (int)hdr[16];
+    w.sleepEnabled = hdr[17] != 0;
+    const int nmesh = (int)hdr[18];
     std::vector<double> row(26);
     for (size_t i = 0; i < n; ++i) {
         f.read(reinterpret_cast<char*>(row.data()), sizeof(double) * row.size());
@@ -207,12 +281,30 @@ static int run_scene(const char* in, const char* out, int k) {
             f.read(reinterpret_cast<char*>(v), sizeof v);
             b.vertices.push_back(V3{v[0], v[1], v[2]});
         }
+        if (b.shape == Shape::Mesh || b.invMass == 0.0) b.dynamic = false;
         w.bodies.push_back(b);
+    }
+    for (int m = 0; m < nmesh; ++m) {   // body, nverts, ntris, then verts and tris as doubles
+        double mh[3];
+        f.read(reinterpret_cast<char*>(mh), sizeof mh);
+        auto md = std::make_shared<MeshData>();
+        for (int t = 0; t < (int)mh[1]; ++t) {
+            double v[3];
+            f.read(reinterpret_cast<char*>(v), sizeof v);
+            md->verts.push_back(V3{v[0], v[1], v[2]});
+        }
+        for (int t = 0; t < (int)mh[2]; ++t) {
+            double v[3];
+            f.read(reinterpret_cast<char*>(v), sizeof v);
+            md->tris.push_back({(int)v[0], (int)v[1], (int)v[2]});
+        }
+        w.bodies[(size_t)mh[0]].mesh = md;
     }
     for (int s = 0; s < k; ++s) w.step();   // compat mode: upload, step, download each time
     std::ofstream o(out, std::ios::binary);
     for (const Body& b : w.bodies) {
-        double r[13] = {b.x.x, b.x.y, b.x.z, b.v.x, b.v.y, b.v.z, b.q.w, b.q.x, b.q.y, b.q.z, b.w.x, b.w.y, b.w.z};
+        double r[16] = {b.x.x, b.x.y, b.x.z, b.v.x, b.v.y, b.v.z, b.q.w, b.q.x, b.q.y, b.q.z, b.w.x, b.w.y, b.w.z,
+                        b.sleeping ? 1.0 : 0.0, b.sleepTimer, b.invMass};
         o.write(reinterpret_cast<const char*>(r), sizeof r);
     }
     return 0;
@@ -228,6 +320,8 @@ int main(int argc, char** argv) {
         cross_boundary();
         stacking();
         joints();
+        sleeping();
+        meshes();
         std::printf("failures: %d\n", g_fail);
         return g_fail == 0 ? 0 : 1;
     } catch (const std::exception& e) {

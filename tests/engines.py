@@ -89,6 +89,14 @@ def _load(kind):
     return lib
 
 
+def body_aux(w, n):
+    """(flags, sleep timer, inverse mass, inverse inertia (n, 3)) of an oracle
+    or reference engine's bodies."""
+    fl, tm, im, ii = np.zeros(n, np.uint32), np.zeros(n), np.zeros(n), np.zeros((n, 3))
+    w._f("get_body_aux")(w.h, fl.ctypes.data_as(C.POINTER(C.c_uint32)), _abi.dptr(tm), _abi.dptr(im), _abi.dptr(ii))
+    return fl, tm, im, ii
+
+
 class _CpuWorld:
     prefix = ""
     kind = ""

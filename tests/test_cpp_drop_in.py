@@ -60,7 +60,7 @@ def _write_scene(path, s):
     h = p.half
     hdr = [s.n, h[0], h[1], h[2], float(p.periodic), float(p.open_boundary), p.dt, *p.gravity, p.restitution,
            p.friction, p.contact_beta, float(p.restitution_combine), float(p.friction_combine), float(p.warm_start),
-           float(p.solver_iterations)]
+           float(p.solver_iterations), float(p.sleep_enabled), float(len(s.meshes))]
     hdr += [0.0] * (24 - len(hdr))
     with open(path, "wb") as f:
         f.write(struct.pack("24d", *hdr))
@@ -71,6 +71,11 @@ def _write_scene(path, s):
             f.write(struct.pack("26d", *row))
             for t in range(s.hull_offsets[i], s.hull_offsets[i + 1]):
                 f.write(struct.pack("3d", *s.hull_verts[t]))
+        for m in s.meshes:
+            v, t = np.asarray(m["verts"], np.float64), np.asarray(m["tris"], np.float64)
+            f.write(struct.pack("3d", float(m["body"]), float(len(v)), float(len(t))))
+            f.write(np.ascontiguousarray(v).tobytes())
+            f.write(np.ascontiguousarray(t).tobytes())
 
 
 @pytest.mark.gpu
@@ -87,10 +92,37 @@ def test_drop_in_compat_mode_matches_oracle(tmp_path, gpu_available, oracle_lib)
     _write_scene(fin, s)
     r = subprocess.run([exe, "scene", fin, fout, str(k)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    got = np.fromfile(fout, np.float64).reshape(s.n, 13)
+    got = np.fromfile(fout, np.float64).reshape(s.n, 16)[:, :13]
     o = engines.OracleWorld()
     o.set_scene(s)
     o.step(k)
     x, v, q, w = o.state()
     want = np.concatenate([x, v, q, w], axis=1)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_drop_in_compat_mode_sleep_and_terrain(tmp_path, gpu_available, oracle_lib):
+    """Compat mode over a terrain mesh with island sleeping on: the mesh goes
+    up with the bodies, and the sleep state (flag, timer, zeroed and stored
+    masses) comes back to the bodies and up again every step, so k single
+    steps equal k resident steps of the oracle, sleep state included."""
+    exe = _build(tmp_path)
+    s = S.terrain_pile(30, seed=4, shapes="mixed")
+    s.params.sleep_enabled = True
+    k = 400                       # the oracle has 8 bodies asleep by then
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_scene(fin, s)
+    r = subprocess.run([exe, "scene", fin, fout, str(k)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(fout, np.float64).reshape(s.n, 16)
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    o.step(k)
+    x, v, q, w = o.state()
+    assert np.array_equal(got[:, :13], np.concatenate([x, v, q, w], axis=1))
+    fl, tm, im, _ = engines.body_aux(o, s.n)
+    assert np.array_equal(got[:, 13], ((fl & 8) != 0).astype(np.float64))
+    assert np.array_equal(got[:, 14], tm)
+    assert np.array_equal(got[:, 15], im)
+    assert np.any(got[:, 13] == 1.0), "some bodies fell asleep on the terrain"

@@ -323,3 +323,27 @@ def test_terrain_many_bodies_grows_candidate_buffer(gpu_available, oracle_lib):
     co = o.contacts()
     _assert_same_contacts(g.contacts(), co, "terrain large last step")
     assert int(np.sum(co["i"] == 0)) > 1000, "the bodies have landed on the terrain"
+
+
+def test_large_scene_parity(gpu_available, oracle_lib):
+    """The large terrain scene at 4096 bodies (periodic 25 x 100 x 25, a
+    64 x 64 heightfield, box columns of 8, loose spheres, 16-link ropes,
+    island sleeping, 10 iterations): state and sleep state every 10 steps,
+    the constraint list every 50, over the first 100 steps while the
+    columns land on the terrain.  (Past step ~110 this seed reaches one of
+    the reference's non-converging EPA runs, which takes the oracle minutes.)"""
+    s = scene.large_scene(4096)
+    _say(f"large scene: {s.n} bodies, {len(s.joints)} joints, {len(s.meshes[0]['tris'])} triangles")
+    g = _gpu()
+    g.set_scene(s)
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    for t in range(10, 101, 10):
+        g.step(10)
+        o.step(10)
+        _assert_same_state(g.state(), o.state(), f"large scene step {t}")
+        _assert_same_sleep(g, o, s.n, f"large scene step {t}")
+        if t % 50 == 0:
+            co = o.contacts()
+            _assert_same_contacts(g.contacts(), co, f"large scene step {t}")
+            assert int(np.sum(co["i"] == 0)) > 100, "the columns stand on the terrain"
