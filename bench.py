@@ -188,6 +188,17 @@ def scene_bench(s, steps: int, warmup: int, cpu_steps: int, cpu_what: str, label
     g.time_kernel(None)
     last = g.counters()
     reruns, trunc = g.epa_stats()
+    stage_ms_end = {}   # the same stages at the end state (untimed steps after the timed region)
+    for name in KERNELS:
+        if name == "step":
+            continue
+        g.time_kernel(name)
+        g.step(2)
+        ms, k = g.kernel_time()
+        stage_ms_end[name] = ms / max(k, 1)
+    g.time_kernel(None)
+    log(f"{label}: end-state stage ms/launch " +
+        ", ".join(f"{k}={v:.3f}" for k, v in sorted(stage_ms_end.items(), key=lambda t: -t[1])))
     fl = g.body_aux()[0] if s.params.sleep_enabled else None
     g.close()
     value = s.n * steps / (t1 - t0)
@@ -220,6 +231,7 @@ def scene_bench(s, steps: int, warmup: int, cpu_steps: int, cpu_what: str, label
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "stage_ms_end": {k: round(v, 4) for k, v in stage_ms_end.items()},
         "per_step_mean": {"candidate_pairs": units["pair"] / steps, "mesh_triangles": units["mesh_triangle"] / steps,
                           "constraints": units["contact"] / steps},
         "last_step": {"candidate_pairs": last["candidate_pairs"], "constraints": last["constraints"],
@@ -232,6 +244,90 @@ def scene_bench(s, steps: int, warmup: int, cpu_steps: int, cpu_what: str, label
     return out
 
 
+def _save_state(g, s, path):
+    """The world's bodies as the GPU holds them now (pose, velocity, sleep
+    state) with the scene's shapes, materials, joints and meshes."""
+    import numpy as np
+    x, v, q, w = g.state()
+    fl, tm, im, ii = g.body_aux()
+    ims, iis = g.sleep_store()
+    np.savez(path, x=x, v=v, q=q, w=w, flags=fl, timer=tm, im=im, ii=ii, ims=ims, iis=iis)
+
+
+def _cpu_one_step(scene_args, state_path, timeout):
+    """Seconds the C restatement takes for one step from a saved state, in a
+    child process (a single step can hit the reference's non-converging EPA,
+    which runs for minutes): None if it does not finish within `timeout`."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-sample", state_path, "--cpu-scene", scene_args]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return None
+    if r.returncode != 0:
+        log(r.stderr[-2000:])
+        return None
+    return float(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_sample_main(state_path, scene_args):
+    """Child process: rebuild the scene, load the saved state, time one step."""
+    import numpy as np
+    from nativeengine_amd import scene as scn
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import engines
+    n, seed = (int(t) for t in scene_args.split(","))
+    s = scn.large_scene(n, seed=seed)
+    d = np.load(state_path)
+    s.x, s.v, s.q, s.w = d["x"], d["v"], d["q"], d["w"]
+    s.flags = d["flags"].astype(np.uint32)
+    s.sleep_timer, s.inv_mass, s.inv_inertia = d["timer"], d["im"], d["ii"]
+    s.inv_mass_store, s.inv_inertia_store = d["ims"], d["iis"]
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    t0 = time.perf_counter()
+    o.step(1)
+    print(time.perf_counter() - t0)
+
+
+def contact_phase_cpu(n: int, marks, timeout: float):
+    """The C restatement (single thread) on the n-body instance of the large
+    scene at several points of its run, from the GPU's state there (the GPU's
+    states are the restatement's, bit for bit): one step each."""
+    import tempfile
+    from nativeengine_amd import scene as scn
+    from nativeengine_amd.gpu import GpuWorld
+    s = scn.large_scene(n, seed=5)
+    g = GpuWorld()
+    g.set_scene(s)
+    done, times = 0, []
+    with tempfile.TemporaryDirectory() as tmp:
+        for m in marks:
+            g.step(m - done)
+            done = m
+            path = os.path.join(tmp, f"state_{m}.npz")
+            _save_state(g, s, path)
+            sec = _cpu_one_step(f"{n},5", path, timeout)
+            log(f"contact-phase cpu: step {m}: " + (f"{sec:.2f}s" if sec is not None else f"> {timeout:.0f}s (stopped)"))
+            times.append(sec)
+    g.close()
+    finished = [t for t in times if t is not None]
+    # a step that did not finish counts as the timeout (the CPU figure is then an upper bound)
+    total = sum(t if t is not None else timeout for t in times)
+    return {
+        "value": n * len(times) / total,
+        "unit": "body-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"one step at each of steps {list(marks)} of the {n}-body instance of the same scene (same density, "
+                  f"side {400.0 * (n / float(1 << 20)) ** 0.5:.0f}), each from the GPU's state at that step, single "
+                  f"thread, {cpu_model()}, nproc={os.cpu_count()}",
+        "seconds": total,
+        "steps_over_timeout": len(times) - len(finished),
+        "upper_bound": len(finished) < len(times),
+    }
+
+
 def large_bench(args):
     """The large terrain scene at 1,048,576 bodies: the headline line."""
     from nativeengine_amd import scene as scn
@@ -241,8 +337,15 @@ def large_bench(args):
     log(f"large scene: {s.n} bodies, {len(s.joints)} joints, {len(s.meshes[0]['tris'])} triangles, "
         f"built in {time.perf_counter() - t0:.1f}s")
     r = scene_bench(s, args.large_steps, args.large_warmup, 0 if args.no_cpu else args.cpu_steps,
-                    f"the same scene from its initial state (bodies falling, no contacts yet: the CPU's "
+                    f"the same 1M-body scene from its initial state (bodies falling, no contacts yet: the CPU's "
                     f"fastest phase of the run)", "large")
+    if not args.no_cpu:
+        r["cpu_start_phase"] = r["cpu_baseline"]
+        cp = contact_phase_cpu(args.cpu_n, [10, 100, 200, 300], args.cpu_timeout)
+        r["cpu_contact_phases"] = cp
+        r["cpu_baseline"] = cp
+        r["speedup_vs_cpu"] = round(r["value"] / cp["value"], 1)
+        log(f"large: cpu over the run {cp['value']:.4g} body-steps/s, GPU/CPU {r['speedup_vs_cpu']}")
     r["workload"] = (f"large terrain scene: {s.n} bodies (mesh + {int((s.shape == 2).sum())} boxes in columns of 8 + "
                      f"{int((s.shape == 0).sum())} spheres, {len(s.joints)} rope joints in 16-link ropes), periodic "
                      f"{2 * h[0]:.0f}x{2 * h[1]:.0f}x{2 * h[2]:.0f}, {len(s.meshes[0]['verts'])}-vertex value-noise "
@@ -503,7 +606,14 @@ def main():
     ap.add_argument("--batched-warmup", type=int, default=30)
     ap.add_argument("--loop-worlds", type=int, default=128)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=65536, help="bodies in the CPU's contact-phase sample")
+    ap.add_argument("--cpu-timeout", type=float, default=90.0)
+    ap.add_argument("--cpu-sample", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-scene", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_sample:
+        cpu_sample_main(args.cpu_sample, args.cpu_scene)
+        return
     args.large_steps, args.large_warmup = args.steps, args.warmup
 
     if args.gpus != 1:

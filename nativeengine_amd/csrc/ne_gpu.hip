@@ -127,6 +127,48 @@ enum CounterSlot : int {
 
 constexpr int TPB = 256;
 
+// Wave-aggregated atomics.  When one island spans the scene, every lane of a
+// wave targets the same counter; lanes with equal keys combine first and one
+// lane per key issues the atomic.  All 64 lanes must call these together
+// (`active` selects the lanes that take part).
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
+
+// atomicAdd(&ctr[key], 1) for each active lane; returns the lane's old value
+// (distinct per lane, in lane order within the wave).
+__device__ __forceinline__ uint32_t wave_add1(uint32_t* ctr, uint32_t key, bool active) {
+    uint64_t pending = __ballot(active);
+    uint32_t out = 0;
+    const uint64_t below = lane_id() ? (~0ull >> (64 - lane_id())) : 0ull;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lkey = __shfl(key, leader);
+        const uint64_t same = __ballot(active && key == lkey) & pending;
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(&ctr[lkey], (uint32_t)__popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane_id()) & 1ull) out = base + (uint32_t)__popcll(same & below);
+        pending &= ~same;
+    }
+    return out;
+}
+
+// atomicMin(&ctr[key], val) for each active lane.
+__device__ __forceinline__ void wave_min_u64(unsigned long long* ctr, uint32_t key, unsigned long long val, bool active) {
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lkey = __shfl(key, leader);
+        const uint64_t same = __ballot(active && key == lkey) & pending;
+        unsigned long long m = ((same >> lane_id()) & 1ull) ? val : ~0ull;
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(m, off);
+            m = o < m ? o : m;
+        }
+        if (lane_id() == leader) atomicMin(&ctr[lkey], m);
+        pending &= ~same;
+    }
+}
+
 __device__ __forceinline__ V3 ld3(const double* a, const double* b, const double* c, uint32_t i) {
     return V3{a[i], b[i], c[i]};
 }
@@ -1013,15 +1055,20 @@ __global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, uint32
     uint32_t* fault = counters + CNT_FAULT;
     EpaWS ws{};
     ws.defer = true;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
-        uint32_t nc = narrow_pair<SPHERES_ONLY>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
-        if (!SPHERES_ONLY && nc == NARROW_DEFERRED) {
-            const uint32_t slot = atomicAdd(&counters[CNT_EPAQ], 1u);
-            if (slot < epa_cap) epa_q[slot] = p;
-            else atomicOr(fault, NE_FAULT_PAIRS);
-            nc = 0;
+    for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < np; p0 += gridDim.x * blockDim.x) {
+        const uint32_t p = p0 + threadIdx.x;
+        uint32_t nc = 0;
+        if (p < np) nc = narrow_pair<SPHERES_ONLY>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
+        if (!SPHERES_ONLY) {
+            const bool defer = p < np && nc == NARROW_DEFERRED;
+            const uint32_t slot = wave_add1(counters, CNT_EPAQ, defer);
+            if (defer) {
+                if (slot < epa_cap) epa_q[slot] = p;
+                else atomicOr(fault, NE_FAULT_PAIRS);
+                nc = 0;
+            }
         }
-        ncont[p] = nc;
+        if (p < np) ncont[p] = nc;
     }
 }
 
@@ -1609,6 +1656,7 @@ __global__ void __launch_bounds__(TPB) k_isl_claim(BodiesDev B, uint32_t* counte
         const uint32_t body = mutable_body(B, i) ? i : C.cj[k];
         const uint32_t r = uf_find(parent, body);
         cons_root[k] = r;
+        if (__atomic_load_n(&claimed[r], __ATOMIC_RELAXED) != 0u) continue;   // most lanes of a large island
         if (atomicExch(&claimed[r], 1u) == 0u) {
             const uint32_t id = atomicAdd(&counters[CNT_ISL], 1u);
             root_id[r] = id;
@@ -1622,10 +1670,12 @@ __global__ void __launch_bounds__(TPB) k_isl_count(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ root_id, uint32_t* __restrict__ cons_isl,
                                                    uint32_t* __restrict__ isl_count) {
     const uint32_t nc = counters[CNT_C];
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
-        const uint32_t id = root_id[cons_root[k]];
-        cons_isl[k] = id;
-        atomicAdd(&isl_count[id], 1u);
+    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < nc; k0 += gridDim.x * blockDim.x) {
+        const uint32_t k = k0 + threadIdx.x;
+        const bool on = k < nc;
+        const uint32_t id = on ? root_id[cons_root[k]] : 0u;
+        if (on) cons_isl[k] = id;
+        wave_add1(isl_count, id, on);
     }
 }
 
@@ -1634,9 +1684,12 @@ __global__ void __launch_bounds__(TPB) k_isl_fill(const uint32_t* __restrict__ c
                                                   const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_fill,
                                                   uint32_t* __restrict__ isl_list) {
     const uint32_t nc = counters[CNT_C];
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
-        const uint32_t id = cons_isl[k];
-        isl_list[isl_start[id] + atomicAdd(&isl_fill[id], 1u)] = k;
+    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < nc; k0 += gridDim.x * blockDim.x) {
+        const uint32_t k = k0 + threadIdx.x;
+        const bool on = k < nc;
+        const uint32_t id = on ? cons_isl[k] : 0u;
+        const uint32_t pos = wave_add1(isl_fill, id, on);
+        if (on) isl_list[isl_start[id] + pos] = k;
     }
 }
 
@@ -1920,7 +1973,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve_small(BodiesDev B, SimP
                                                      const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
                                                      ConsDev C, uint32_t* __restrict__ big_list,
                                                      uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp,
-                                                     uint32_t* __restrict__ prev_n) {
+                                                     uint32_t* __restrict__ prev_n, uint32_t big_above) {
     __shared__ uint32_t lds_ids[SOLVE_THREADS * STAGE_BODIES];
     __shared__ StagedBody lds_b[SOLVE_THREADS * STAGE_BODIES];
     __shared__ StagedCons lds_c[SOLVE_THREADS * STAGE_CONS];
@@ -1929,7 +1982,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve_small(BodiesDev B, SimP
         const uint32_t s = isl_start[isl], e = isl_start[isl + 1];
         const uint32_t m = e - s;
         if (m == 1) continue;   // k_solve_single
-        if (m > SMALL_ISLAND) {
+        if (m > big_above) {
             big_list[atomicAdd(&counters[CNT_NBIG], 1u)] = isl;
             atomicMax(&counters[CNT_MAXISL], m);
             continue;
@@ -2028,6 +2081,7 @@ constexpr uint32_t WF_NONE = 0xFFFFFFFFu;   // no successor on this side (a stat
 constexpr uint32_t WF_MAX_ROUNDS = 1u << 16;
 constexpr int WF_THREADS = 128;
 constexpr uint32_t WF_PASS_MAX = 66;        // warm start + up to 64 iterations + correction
+constexpr uint32_t WF_ISLAND = STAGE_CONS;  // eager steps: islands above this many constraints take the dataflow solve
 
 struct WfDev {
     uint32_t* flag;        // per constraint: 1 if it starts a task of a large island
@@ -2046,12 +2100,13 @@ struct WfDev {
 
 __global__ void __launch_bounds__(TPB) k_wf_mark(BodiesDev B, const uint32_t* __restrict__ counters, ConsDev C,
                                                  const uint32_t* __restrict__ cons_isl,
-                                                 const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ flag) {
+                                                 const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ flag,
+                                                 uint32_t big_above) {
     const uint32_t nc = counters[CNT_C];
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
         const bool start = k == 0 || C.ci[k] != C.ci[k - 1] || C.cj[k] != C.cj[k - 1];
         const uint32_t id = cons_isl[k];
-        flag[k] = (start && isl_start[id + 1] - isl_start[id] > SMALL_ISLAND) ? 1u : 0u;
+        flag[k] = (start && isl_start[id + 1] - isl_start[id] > big_above) ? 1u : 0u;
     }
 }
 
@@ -2099,13 +2154,19 @@ __global__ void __launch_bounds__(TPB) k_wf_link(uint32_t n, ConsDev C, WfDev W)
 // round 0.
 __global__ void __launch_bounds__(TPB) k_wf_init(BodiesDev B, uint32_t T, uint32_t npass, ConsDev C, WfDev W) {
     if (blockIdx.x == 0 && threadIdx.x == 0) W.roff[0] = 0u;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
-        const uint32_t k = W.task_k[t];
-        const uint32_t sides = (mutable_body(B, C.ci[k]) ? 1u : 0u) + (mutable_body(B, C.cj[k]) ? 1u : 0u);
-        const uint32_t p0 = W.pend0[t];
-        W.pend[t] = p0;
-        for (uint32_t p = 1; p < npass; ++p) W.pend[(size_t)p * T + t] = sides;
-        if (p0 == 0u) W.fr[atomicAdd(&W.rcnt[0], 1u)] = t;
+    for (uint32_t t0 = blockIdx.x * blockDim.x; t0 < T; t0 += gridDim.x * blockDim.x) {
+        const uint32_t t = t0 + threadIdx.x;
+        bool root = false;
+        if (t < T) {
+            const uint32_t k = W.task_k[t];
+            const uint32_t sides = (mutable_body(B, C.ci[k]) ? 1u : 0u) + (mutable_body(B, C.cj[k]) ? 1u : 0u);
+            const uint32_t p0 = W.pend0[t];
+            W.pend[t] = p0;
+            for (uint32_t p = 1; p < npass; ++p) W.pend[(size_t)p * T + t] = sides;
+            root = p0 == 0u;
+        }
+        const uint32_t pos = wave_add1(W.rcnt, 0u, root);
+        if (root) W.fr[pos] = t;
     }
 }
 
@@ -2154,25 +2215,33 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(BodiesDev B, SimParams 
     const uint32_t nbase = base + cnt;
     if (blockIdx.x == 0 && threadIdx.x == 0) W.roff[r + 1] = nbase;
     const uint32_t nc = counters[CNT_C];
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < cnt; u += gridDim.x * blockDim.x) {
-        const uint32_t v = W.fr[base + u];
-        const uint32_t pass = v / T, t = v - pass * T;
-        const uint32_t k0 = W.task_k[t];
-        const uint32_t i = C.ci[k0], j = C.cj[k0];
-        uint32_t k1 = k0 + 1;
-        while (k1 < nc && C.ci[k1] == i && C.cj[k1] == j) ++k1;
-        wf_visit(B, P, C, pass, k0, k1, prev_key, prev_imp);
-        for (int side = 0; side < 2; ++side) {
-            const uint32_t nx = W.next[2 * t + side];
-            if (nx == WF_NONE) continue;
-            uint32_t tv;
-            if (nx & WF_WRAP) {
-                if (pass + 1 >= npass) continue;
-                tv = (pass + 1) * T + (nx & ~WF_WRAP);
-            } else {
-                tv = pass * T + nx;
+    for (uint32_t u0 = blockIdx.x * blockDim.x; u0 < cnt; u0 += gridDim.x * blockDim.x) {
+        const uint32_t u = u0 + threadIdx.x;
+        uint32_t tv[2] = {WF_NONE, WF_NONE};
+        bool ready[2] = {false, false};
+        if (u < cnt) {
+            const uint32_t v = W.fr[base + u];
+            const uint32_t pass = v / T, t = v - pass * T;
+            const uint32_t k0 = W.task_k[t];
+            const uint32_t i = C.ci[k0], j = C.cj[k0];
+            uint32_t k1 = k0 + 1;
+            while (k1 < nc && C.ci[k1] == i && C.cj[k1] == j) ++k1;
+            wf_visit(B, P, C, pass, k0, k1, prev_key, prev_imp);
+            for (int side = 0; side < 2; ++side) {
+                const uint32_t nx = W.next[2 * t + side];
+                if (nx == WF_NONE) continue;
+                if (nx & WF_WRAP) {
+                    if (pass + 1 >= npass) continue;
+                    tv[side] = (pass + 1) * T + (nx & ~WF_WRAP);
+                } else {
+                    tv[side] = pass * T + nx;
+                }
+                ready[side] = atomicSub(&W.pend[tv[side]], 1u) == 1u;
             }
-            if (atomicSub(&W.pend[tv], 1u) == 1u) W.fr[nbase + atomicAdd(&W.rcnt[r + 1], 1u)] = tv;
+        }
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t pos = wave_add1(W.rcnt, r + 1, ready[side]);
+            if (ready[side]) W.fr[nbase + pos] = tv[side];
         }
     }
 }
@@ -2398,12 +2467,18 @@ __global__ void __launch_bounds__(TPB) k_sleep_join_joints(BodiesDev B, const ui
 __global__ void __launch_bounds__(TPB) k_sleep_timers(BodiesDev B, SimParams P, uint32_t* __restrict__ parent,
                                                       unsigned long long* __restrict__ mint) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= B.n || !awake_dyn(B, i)) return;
-    double t = B.st[i];
-    if (norm(ld3(B.v0, B.v1, B.v2, i)) < P.sleep_lin && norm(ld3(B.w0, B.w1, B.w2, i)) < P.sleep_ang) t = t + P.dt;
-    else t = 0.0;
-    B.st[i] = t;
-    atomicMin(&mint[uf_find(parent, i)], timer_key(t));
+    const bool on = i < B.n && awake_dyn(B, i);
+    uint32_t root = 0;
+    unsigned long long key = 0;
+    if (on) {
+        double t = B.st[i];
+        if (norm(ld3(B.v0, B.v1, B.v2, i)) < P.sleep_lin && norm(ld3(B.w0, B.w1, B.w2, i)) < P.sleep_ang) t = t + P.dt;
+        else t = 0.0;
+        B.st[i] = t;
+        root = uf_find(parent, i);
+        key = timer_key(t);
+    }
+    wave_min_u64(mint, root, key, on);
 }
 
 // Islands whose slowest member has been slow for sleepTime go to sleep
@@ -3181,6 +3256,10 @@ uint32_t gs_blocks(uint64_t n) {   // grid-stride launches over device-side coun
 
 int alloc_mesh_pairs(ne_gpu_ctx* c);
 
+// Workgroups of the large-workspace EPA reruns: one wave per queued pair,
+// up to the queue's capacity.
+uint32_t big_blocks(const ne_gpu_ctx* c);
+
 int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
     drop_graphs(c);
     c->pcap = pcap;
@@ -3482,7 +3561,7 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
                                                                c->ct, c->epa_q, c->pcap);
             k_epa<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
                                                          c->epa_q, c->pcap, c->epa_big);
-            k_narrow_big<<<16, 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_big);
+            k_narrow_big<<<big_blocks(c), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_big);
         }
         if (c->nmeshes) {   // meshContacts (world.cpp:56-83)
             const MeshDev M = mesh_dev(c);
@@ -3493,7 +3572,7 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
                                                                c->mqcap, c->mq_pair, c->mq_tri);
             k_mesh_narrow<<<hint_blocks(c->hint_mq), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
                                                                   c->mq_tri, c->mhit, c->mct, c->epa_big);
-            k_mesh_big<<<16, 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri, c->mhit,
+            k_mesh_big<<<big_blocks(c), 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri, c->mhit,
                                          c->mct, c->epa_big);
             k_mesh_gather<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, c->counters, c->cand_i, c->cand_j, c->mq_off,
                                                                  c->mqcap, c->mhit, c->mct, c->ncont);
@@ -3574,7 +3653,8 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     int r0 = ensure_wf(c, 0, 1);
     if (r0 != NE_OK) return r0;
     k_fill_u32<<<gs_blocks(c->n), TPB, 0, s>>>(W.inc_cnt, 0u, nullptr, c->n);
-    k_wf_mark<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, c->cons_isl, c->isl_start, W.flag);
+    k_wf_mark<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, c->cons_isl, c->isl_start, W.flag,
+                                                     WF_ISLAND);
     scan_u32(W.flag, W.tid, c->counters + CNT_C, c->ccap, c->scan_part, s, false,
              ScanTotal{c->counters, CNT_WFT, CNT_WFT, CNT_FAULT, c->ccap, 0u, -1, -1});
     HIP_TRY(c, hipMemcpyAsync(c->h_wf, c->counters + CNT_WFT, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -3625,6 +3705,8 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     return NE_OK;
 }
 
+uint32_t big_blocks(const ne_gpu_ctx* c) { return std::min<uint32_t>(std::max<uint32_t>(c->epa_big.cap, 16u), 1024u); }
+
 int enqueue_solve(ne_gpu_ctx* c) {
     hipStream_t s = c->stream;
     const SimParams P = make_sim_params(c->params);
@@ -3634,7 +3716,7 @@ int enqueue_solve(ne_gpu_ctx* c) {
                                                           c->prev_key, c->prev_imp);
     k_solve_small<<<std::min<uint32_t>(4096u, std::max<uint32_t>(64u, (c->hint_c * 2 + SOLVE_THREADS - 1) / SOLVE_THREADS)),
                     SOLVE_THREADS, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list, c->prev_key,
-                                           c->prev_imp, c->prev_n);
+                                           c->prev_imp, c->prev_n, c->capturing ? SMALL_ISLAND : WF_ISLAND);
     if (c->capturing) {   // a graph cannot read back mid-step: one workgroup sweeps each large island
         k_solve_large<<<256, 1024, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list,
                                            c->prev_key, c->prev_imp);
