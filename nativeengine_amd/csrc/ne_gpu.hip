@@ -2096,7 +2096,54 @@ struct WfDev {
     uint32_t* inc_cnt;     // per body: tasks touching it (zero between steps)
     uint32_t* inc_start;   // per body: start of its slice of inc_list
     uint32_t* inc_list;    // per movable body: its tasks
+    uint32_t* task_n;      // per task: constraints in it
+    double* rec;           // per constraint of a task: n, rA, rB, restBias, Kn, mu, nImp, tImp, overlap (16 doubles)
+    double* bp;            // per body: v, w, q, invI, invMass (16 doubles), the sweep's working copy
 };
+
+constexpr int WF_REC = 16;   // doubles per constraint record and per body record (one 128-byte line)
+
+__device__ __forceinline__ BodyView load_packed(const double* __restrict__ bp, uint32_t b) {
+    const double2* r = reinterpret_cast<const double2*>(bp + (size_t)b * WF_REC);
+    const double2 a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3], a4 = r[4], a5 = r[5], a6 = r[6];
+    BodyView o;
+    o.v = V3{a0.x, a0.y, a1.x};
+    o.w = V3{a1.y, a2.x, a2.y};
+    o.q = Qt{a3.x, a3.y, a4.x, a4.y};
+    o.ii = V3{a5.x, a5.y, a6.x};
+    o.im = a6.y;
+    o.mv = o.im != 0.0;
+    o.mw = o.ii.x != 0.0 || o.ii.y != 0.0 || o.ii.z != 0.0;
+    return o;
+}
+
+__device__ __forceinline__ void store_packed(double* __restrict__ bp, uint32_t b, const BodyView& o) {
+    double* r = bp + (size_t)b * WF_REC;
+    if (o.mv) { r[0] = o.v.x; r[1] = o.v.y; r[2] = o.v.z; }
+    if (o.mw) { r[3] = o.w.x; r[4] = o.w.y; r[5] = o.w.z; }
+}
+
+__global__ void __launch_bounds__(TPB) k_wf_pack_bodies(BodiesDev B, double* __restrict__ bp) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B.n) return;
+    double2* r = reinterpret_cast<double2*>(bp + (size_t)b * WF_REC);
+    r[0] = double2{B.v0[b], B.v1[b]};
+    r[1] = double2{B.v2[b], B.w0[b]};
+    r[2] = double2{B.w1[b], B.w2[b]};
+    r[3] = double2{B.q0[b], B.q1[b]};
+    r[4] = double2{B.q2[b], B.q3[b]};
+    r[5] = double2{B.ii0[b], B.ii1[b]};
+    r[6] = double2{B.ii2[b], B.im[b]};
+}
+
+// Back to the body fields: what the sweep may have changed (store_view's rule).
+__global__ void __launch_bounds__(TPB) k_wf_unpack_bodies(BodiesDev B, const double* __restrict__ bp) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B.n) return;
+    const double* r = bp + (size_t)b * WF_REC;
+    if (B.im[b] != 0.0) { B.v0[b] = r[0]; B.v1[b] = r[1]; B.v2[b] = r[2]; }
+    if (B.ii0[b] != 0.0 || B.ii1[b] != 0.0 || B.ii2[b] != 0.0) { B.w0[b] = r[3]; B.w1[b] = r[4]; B.w2[b] = r[5]; }
+}
 
 __global__ void __launch_bounds__(TPB) k_wf_mark(BodiesDev B, const uint32_t* __restrict__ counters, ConsDev C,
                                                  const uint32_t* __restrict__ cons_isl,
@@ -2115,10 +2162,25 @@ __global__ void __launch_bounds__(TPB) k_wf_tasks(BodiesDev B, const uint32_t* _
     const uint32_t nc = counters[CNT_C];
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
         if (!W.flag[k]) continue;
-        W.task_k[W.tid[k]] = k;
+        const uint32_t t = W.tid[k];
+        W.task_k[t] = k;
         const uint32_t i = C.ci[k], j = C.cj[k];
         if (mutable_body(B, i)) atomicAdd(&W.inc_cnt[i], 1u);
         if (mutable_body(B, j)) atomicAdd(&W.inc_cnt[j], 1u);
+        uint32_t k1 = k + 1;
+        while (k1 < nc && C.ci[k1] == i && C.cj[k1] == j) ++k1;
+        W.task_n[t] = k1 - k;
+        for (uint32_t q = k; q < k1; ++q) {   // the sweep's record of constraint q
+            double2* r = reinterpret_cast<double2*>(W.rec + (size_t)q * WF_REC);
+            r[0] = double2{C.n0[q], C.n1[q]};
+            r[1] = double2{C.n2[q], C.ra0[q]};
+            r[2] = double2{C.ra1[q], C.ra2[q]};
+            r[3] = double2{C.rb0[q], C.rb1[q]};
+            r[4] = double2{C.rb2[q], C.rbias[q]};
+            r[5] = double2{C.kn[q], C.mu[q]};
+            r[6] = double2{C.nimp[q], C.timp[q]};
+            r[7] = double2{C.ov[q], 0.0};
+        }
     }
 }
 
@@ -2170,39 +2232,55 @@ __global__ void __launch_bounds__(TPB) k_wf_init(BodiesDev B, uint32_t T, uint32
     }
 }
 
-// One pass over one task's constraints [k0, k1): bodies and impulses stay in
-// registers across the run, as in the pair solver.
-__device__ __forceinline__ void wf_visit(BodiesDev& B, const SimParams& P, const ConsDev& C, uint32_t pass, uint32_t k0,
-                                         uint32_t k1, uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
-    const uint32_t i = C.ci[k0], j = C.cj[k0];
+// One pass over one task's constraints [k0, k1) on the packed records: the
+// two bodies stay in registers across the run, as in the pair solver.
+__device__ __forceinline__ void wf_visit(BodiesDev& B, const SimParams& P, const ConsDev& C, const WfDev& W,
+                                         uint32_t pass, uint32_t i, uint32_t j, uint32_t k0, uint32_t k1,
+                                         uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
     if (pass == 0) {
         if (!P.warm) return;
-        BodyView a = load_view(B, i), b = load_view(B, j);
+        BodyView a = load_packed(W.bp, i), b = load_packed(W.bp, j);
         for (uint32_t k = k0; k < k1; ++k) {
-            const double imp = C.nimp[k];
-            if (imp == 0.0) continue;
-            warm_apply(a, b, load_geom(C, k), imp);
+            const double2* r = reinterpret_cast<const double2*>(W.rec + (size_t)k * WF_REC);
+            const double2 r6 = r[6];
+            if (r6.x == 0.0) continue;
+            const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4];
+            const ConsGeom g{V3{r0.x, r0.y, r1.x}, V3{r1.y, r2.x, r2.y}, V3{r3.x, r3.y, r4.x}};
+            warm_apply(a, b, g, r6.x);
         }
-        store_view(B, i, a);
-        store_view(B, j, b);
+        store_packed(W.bp, i, a);
+        store_packed(W.bp, j, b);
     } else if (pass <= (uint32_t)P.iters) {
-        BodyView a = load_view(B, i), b = load_view(B, j);
+        BodyView a = load_packed(W.bp, i), b = load_packed(W.bp, j);
         for (uint32_t k = k0; k < k1; ++k) {
-            double nimp = C.nimp[k], timp = C.timp[k];
-            gs_visit(a, b, load_geom(C, k), C.rbias[k], C.kn[k], C.mu[k], nimp, timp);
-            C.nimp[k] = nimp;
-            C.timp[k] = timp;
+            double2* r = reinterpret_cast<double2*>(W.rec + (size_t)k * WF_REC);
+            const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6];
+            const ConsGeom g{V3{r0.x, r0.y, r1.x}, V3{r1.y, r2.x, r2.y}, V3{r3.x, r3.y, r4.x}};
+            double nimp = r6.x, timp = r6.y;
+            gs_visit(a, b, g, r4.y, r5.x, r5.y, nimp, timp);
+            r[6] = double2{nimp, timp};
         }
-        store_view(B, i, a);
-        store_view(B, j, b);
+        store_packed(W.bp, i, a);
+        store_packed(W.bp, j, b);
     } else {
         V3 xa = ld3(B.x0, B.x1, B.x2, i), xb = ld3(B.x0, B.x1, B.x2, j);
         const double ima = B.im[i], imb = B.im[j];
-        for (uint32_t k = k0; k < k1; ++k) correct(B, C, k, P.beta, xa, xb, ima, imb);
+        const double invSum = ima + imb;
+        for (uint32_t k = k0; k < k1; ++k) {   // contact.hpp:78-85, as correct()
+            const double2* r = reinterpret_cast<const double2*>(W.rec + (size_t)k * WF_REC);
+            const double2 r0 = r[0], r1 = r[1], r6 = r[6], r7 = r[7];
+            C.nimp[k] = r6.x;
+            C.timp[k] = r6.y;
+            if (P.warm) { prev_key[k] = C.key[k]; prev_imp[k] = r6.x; }
+            const double ov = r7.x;
+            if (invSum <= 0.0 || ov <= 0.0) continue;
+            const V3 n{r0.x, r0.y, r1.x};
+            const V3 corr = n * (P.beta * ov / invSum);
+            if (ima != 0.0) xa = xa + corr * ima;
+            if (imb != 0.0) xb = xb - corr * imb;
+        }
         if (ima != 0.0) st3(B.x0, B.x1, B.x2, i, xa);
         if (imb != 0.0) st3(B.x0, B.x1, B.x2, j, xb);
-        if (P.warm)
-            for (uint32_t k = k0; k < k1; ++k) { prev_key[k] = C.key[k]; prev_imp[k] = C.nimp[k]; }
     }
 }
 
@@ -2214,7 +2292,6 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(BodiesDev B, SimParams 
     const uint32_t base = W.roff[r], cnt = W.rcnt[r];
     const uint32_t nbase = base + cnt;
     if (blockIdx.x == 0 && threadIdx.x == 0) W.roff[r + 1] = nbase;
-    const uint32_t nc = counters[CNT_C];
     for (uint32_t u0 = blockIdx.x * blockDim.x; u0 < cnt; u0 += gridDim.x * blockDim.x) {
         const uint32_t u = u0 + threadIdx.x;
         uint32_t tv[2] = {WF_NONE, WF_NONE};
@@ -2224,9 +2301,7 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(BodiesDev B, SimParams 
             const uint32_t pass = v / T, t = v - pass * T;
             const uint32_t k0 = W.task_k[t];
             const uint32_t i = C.ci[k0], j = C.cj[k0];
-            uint32_t k1 = k0 + 1;
-            while (k1 < nc && C.ci[k1] == i && C.cj[k1] == j) ++k1;
-            wf_visit(B, P, C, pass, k0, k1, prev_key, prev_imp);
+            wf_visit(B, P, C, W, pass, i, j, k0, k0 + W.task_n[t], prev_key, prev_imp);
             for (int side = 0; side < 2; ++side) {
                 const uint32_t nx = W.next[2 * t + side];
                 if (nx == WF_NONE) continue;
@@ -3426,6 +3501,7 @@ void free_wf(ne_gpu_ctx* c) {
     WfDev& W = c->wf;
     dfree(W.flag); dfree(W.tid); dfree(W.task_k); dfree(W.next); dfree(W.pend0); dfree(W.pend); dfree(W.fr);
     dfree(W.rcnt); dfree(W.roff); dfree(W.inc_cnt); dfree(W.inc_start); dfree(W.inc_list);
+    dfree(W.task_n); dfree(W.rec); dfree(W.bp);
     c->wf_tcap = c->wf_ccap = c->wf_ncap = c->wf_vcap_pass = 0;
 }
 
@@ -3616,11 +3692,13 @@ int ensure_wf(ne_gpu_ctx* c, uint32_t T, uint32_t npass) {
     if (c->wf_ccap < c->ccap) {
         HIP_TRY(c, dalloc(W.flag, c->ccap));
         HIP_TRY(c, dalloc(W.tid, (size_t)c->ccap + 1));
+        HIP_TRY(c, dalloc(W.rec, (size_t)c->ccap * WF_REC));
         c->wf_ccap = c->ccap;
     }
     if (c->wf_ncap < c->ncap) {
         HIP_TRY(c, dalloc(W.inc_cnt, c->ncap));
         HIP_TRY(c, dalloc(W.inc_start, (size_t)c->ncap + 1));
+        HIP_TRY(c, dalloc(W.bp, (size_t)c->ncap * WF_REC));
         c->wf_ncap = c->ncap;
     }
     if (T == 0) return NE_OK;
@@ -3628,6 +3706,7 @@ int ensure_wf(ne_gpu_ctx* c, uint32_t T, uint32_t npass) {
     if (c->wf_tcap < T) {
         const uint32_t cap = T + T / 2 + 1024;
         HIP_TRY(c, dalloc(W.task_k, cap));
+        HIP_TRY(c, dalloc(W.task_n, cap));
         HIP_TRY(c, dalloc(W.next, 2 * (size_t)cap));
         HIP_TRY(c, dalloc(W.pend0, cap));
         HIP_TRY(c, dalloc(W.inc_list, 2 * (size_t)cap));
@@ -3674,6 +3753,7 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     k_wf_fill<<<gs_blocks(T), TPB, 0, s>>>(B, T, c->C, W);
     k_wf_link<<<gs_blocks(c->n), TPB, 0, s>>>(c->n, c->C, W);
     k_wf_init<<<gs_blocks(T), TPB, 0, s>>>(B, T, npass, c->C, W);
+    k_wf_pack_bodies<<<blocks_for(c->n), TPB, 0, s>>>(B, W.bp);
     const uint64_t total = (uint64_t)T * npass;
     const uint32_t hint = c->wf_rounds_hint ? c->wf_rounds_hint : 64u;
     uint64_t per_round = total / hint + 1;
@@ -3696,6 +3776,7 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
             return set_err(c, NE_ERR_INVALID, "large-island solve stalled (a visit never became ready)");
         batch = 64;
     }
+    k_wf_unpack_bodies<<<blocks_for(c->n), TPB, 0, s>>>(B, W.bp);
     // the depth: rounds up to the last one that ran a visit
     std::vector<uint32_t> cnt(r);
     HIP_TRY(c, hipMemcpy(cnt.data(), W.rcnt, sizeof(uint32_t) * r, hipMemcpyDeviceToHost));
