@@ -608,11 +608,23 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=65536, help="bodies in the CPU's contact-phase sample")
     ap.add_argument("--cpu-timeout", type=float, default=90.0)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run one elementwise kernel of known traffic (2^25 doubles: reads 512 MiB, writes "
+                         "256 MiB), the scale for rocprofv3 FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--cpu-sample", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-scene", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_sample:
         cpu_sample_main(args.cpu_sample, args.cpu_scene)
+        return
+    if args.traffic_probe:
+        import numpy as np
+        from nativeengine_amd.gpu import GpuWorld
+        rng = np.random.default_rng(0)
+        a, b = rng.uniform(-1, 1, 1 << 25), rng.uniform(-1, 1, 1 << 25)
+        out = GpuWorld().fp_probe(3, a, b)   # k_fp_probe: out = a*b + a
+        assert np.array_equal(out, a * b + a)
+        print(json.dumps({"traffic_probe": "k_fp_probe", "read_bytes": 2 * 8 << 25, "write_bytes": 8 << 25}))
         return
     args.large_steps, args.large_warmup = args.steps, args.warmup
 

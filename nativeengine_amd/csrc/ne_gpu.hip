@@ -1403,6 +1403,8 @@ __device__ __forceinline__ void store_contact(double* out, const Contact& c, boo
     out[3] = c.point.x; out[4] = c.point.y; out[5] = c.point.z; out[6] = c.overlap;
 }
 
+constexpr int PV_STRIDE = 19;   // doubles per thread for a prism's vertices (18, padded against bank conflicts)
+
 // detectContact(other, prism) per triangle candidate (GJK + EPA with the
 // per-thread workspace); EPA runs that outgrow it go to the shared
 // large-workspace queue (k_mesh_big).
@@ -1420,7 +1422,10 @@ __global__ void __launch_bounds__(TPB) k_mesh_narrow(BodiesDev B, SimParams P, M
     EpaWS ws{};
     ws.verts = verts; ws.faces = faces; ws.maxf = EPA_MAXF; ws.big = false;
     ws.list.hor = hor; ws.list.cap = EPA_MAXH;
-    double pv[18];
+    // the prism's six vertices in LDS (every support call reads them; in
+    // per-thread scratch they were most of this kernel's memory traffic)
+    __shared__ double lds_pv[TPB * PV_STRIDE];
+    double* pv = lds_pv + threadIdx.x * PV_STRIDE;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
         const uint32_t p = mq_pair[q];
         uint32_t o, m;

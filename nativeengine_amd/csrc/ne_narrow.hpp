@@ -121,7 +121,11 @@ __device__ inline bool gjk_tetra(const Geom& A, const Geom& B, V3 shift, SupV te
         if (norm2(d) < 1e-24) d = V3{1.0, 0.0, 0.0};
         SupV p = cso_support(A, B, shift, d);
         if (dot(p.v, d) <= 1e-12) return false;
-        for (int k = n; k > 0; --k) W[k] = W[k - 1];
+        // shift the simplex (n <= 3 here) with fixed indices so it stays in
+        // registers; slots past n hold stale points that are never read
+        W[3] = W[2];
+        W[2] = W[1];
+        W[1] = W[0];
         W[0] = p;
         ++n;
         V3 a = W[0].v, ao = -a;
@@ -299,7 +303,8 @@ __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, co
             if (dot(f.n, p.v - verts[f.a].v) > 1e-12) {
                 over = !hz.add(f.a, f.b) || !hz.add(f.b, f.c) || !hz.add(f.c, f.a);
             } else {
-                faces[nk++] = f;   // nk <= fi: in-place compaction keeps order
+                if (nk != fi) faces[nk] = f;   // nk <= fi: in-place compaction keeps order
+                ++nk;
             }
         }
         if (over || nk + hz.alive() > maxf) {
