@@ -2,36 +2,38 @@
 """bench.py -- body-steps/s of the MI355X step at 1M bodies; batched
 world-steps/s; backward/forward time ratio of differentiable rollouts.
 
-Workload: the plain-method point of the headline metric, a periodic elastic
-sphere gas of 1,048,576 spheres (r = 0.5, m = 1) at the 4096-sphere/L=40
-density (L = 253.98), uniform positions rejection-sampled to no overlap
-(seed 0), velocities N(0,1) with zero net momentum, e = 1, mu = 0, no gravity,
-dt = 1/60, 8 solver iterations, FP64.  Inputs are synthetic and resident in
-HBM before the timed region; a step is one full `World::step()`
-(integrate, broadphase, narrow phase, warm-started sequential impulses,
-positional correction, periodic wrap).
+Headline (`value`): the large terrain scene, 1,048,576 bodies in a periodic
+400 x 100 x 400 domain over a 1024 x 1024-vertex value-noise heightfield:
+box columns of 8 (70%), loose spheres (25%), 16-link rope chains of rigid
+distance joints (5%); friction 0.6, restitution 0.1, island sleeping on,
+dt 1/60, 10 iterations, FP64, seed 5.  Inputs are synthetic and resident in
+HBM before the timed region; a step is one full `World::step()`.  The timed
+region is 300 single-step calls after 10 warm-up steps.  `cpu_baseline`
+times the C restatement on one host core over the contact phase (one step at
+steps 10/100/200/300 of the 65,536-body instance of the same scene, from the
+GPU's state there); `cpu_start_phase` on the 1M scene's first two steps.
 
-Batched worlds (reported under "batched"): 1024 independent periodic worlds
-of log-uniform size in [64, 4096] (seed 1; ~1M bodies), half spheres and half
-boxes (sizes U[0.2,0.6]) at 20% volume fraction, per-world friction U[0,0.8]
-and restitution U[0,1], 8 iterations, stepped together in one set of launches
-(HIP-graph replay, configuration chosen by the autotuner) against looping the
-plain single-world step over a sample of the same worlds, from the same state.
+Other lines in the same JSON object:
+  gas      the 1,048,576-sphere periodic elastic gas (r 0.5, L 253.98, e 1,
+           mu 0, no gravity, 8 iterations, seed 0), the plain-method 1M point,
+           with the reference itself (oracle/_ref) timed on the host;
+  pile     the 65,536-body mixed-shape pile (40% boxes, 30% spheres, 20%
+           capsules, 10% hulls) on a ground box inside 100 x 200 x 100 walls,
+           10 iterations, 600 steps;
+  batched  1024 periodic worlds of log-uniform size in [64, 4096] stepped in
+           one set of launches (HIP-graph replay, autotuned) against looping
+           the plain single-world step over a sample of the same worlds;
+  adjoint  256 worlds of 256 spheres + 256 boxes on a ground box, 120 steps,
+           checkpoint every 16: backward time / forward time.
 
-Differentiable rollouts (reported under "adjoint"): 256 worlds of 256 spheres
-(r 0.4) and 256 boxes (half extent 0.3) on a static ground box, periodic side
-16, gravity, 120 steps at dt 1/60, checkpoint every 16; loss = MSE of final
-positions against targets made at friction 0.4 / restitution 0.3, evaluated
-at 0.5 / 0.2; gradients with respect to the initial velocities and each
-world's friction and restitution.  Reports backward time / forward time.
-
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--cpu-steps S] [--no-batched] [--no-adjoint]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-large] [--no-gas] [--no-pile]
+                    [--no-batched] [--no-adjoint] [--traffic-probe]
 
 Prints one JSON line.  `roofline` is measured live with HIP events on the
-library's own stream around every launch of the dominant kernel inside the
-timed region; `cpu_baseline` times the reference (oracle/_ref, single thread)
-or, where it was not built, the C restatement on a bounded sample of the same
-workload on this host.
+library's own stream around every launch of the dominant stage inside the
+timed region, credited with the algorithmic bytes of every timed step;
+`traffic` is the HBM bytes per launch of that stage from the committed
+rocprofv3 PMC summary.
 """
 from __future__ import annotations
 
@@ -109,6 +111,25 @@ def cpu_baseline(scene, steps: int, what: str):
     }
 
 
+def measured_traffic(stage: str, profile: str = "large_terrain_1M_traffic.md"):
+    """HBM bytes per launch of a stage from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a kernel of known traffic), or
+    None.  PMC counters need their own profiler runs, so bench.py does not
+    measure them itself."""
+    path = os.path.join(ROOT, "profiles", profile)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    for item in line.split(":", 1)[1].split():
+                        k, v = item.split("=")
+                        if k == stage:
+                            return float(v)
+    except OSError:
+        pass
+    return None
+
+
 def stage_bytes(name: str, iterations: int):
     """Algorithmic bytes per unit of a step stage: {unit kind: bytes}."""
     table = {
@@ -125,7 +146,8 @@ def stage_bytes(name: str, iterations: int):
     return table.get(name)
 
 
-def scene_bench(s, steps: int, warmup: int, cpu_steps: int, cpu_what: str, label: str, max_seconds: float = 150.0):
+def scene_bench(s, steps: int, warmup: int, cpu_steps: int, cpu_what: str, label: str, max_seconds: float = 150.0,
+                traffic_profile: str | None = None):
     """Body-steps/s of one resident scene over `steps` single-step calls after
     `warmup` untimed steps, with the dominant stage's HIP-event time and the
     algorithmic bytes it moved (units counted from every timed step)."""
@@ -213,7 +235,9 @@ def scene_bench(s, steps: int, warmup: int, cpu_steps: int, cpu_what: str, label
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(dom, traffic_profile) if traffic_profile else None,
+        "traffic_source": f"profiles/{traffic_profile} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a "
+                          f"kernel of known traffic)" if traffic_profile else None,
         "bytes_per_unit": per,
         "units_per_launch": {k: round(units[k] / max(dom_launches, 1), 1) for k in per},
         "avg_launch_us": round(avg_launch_ms * 1e3, 3),
@@ -338,7 +362,7 @@ def large_bench(args):
         f"built in {time.perf_counter() - t0:.1f}s")
     r = scene_bench(s, args.large_steps, args.large_warmup, 0 if args.no_cpu else args.cpu_steps,
                     f"the same 1M-body scene from its initial state (bodies falling, no contacts yet: the CPU's "
-                    f"fastest phase of the run)", "large")
+                    f"fastest phase of the run)", "large", traffic_profile="large_terrain_1M_traffic.md")
     if not args.no_cpu:
         r["cpu_start_phase"] = r["cpu_baseline"]
         cp = contact_phase_cpu(args.cpu_n, [10, 100, 200, 300], args.cpu_timeout)

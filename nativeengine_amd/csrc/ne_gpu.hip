@@ -1078,12 +1078,12 @@ __global__ void __launch_bounds__(TPB) k_epa(BodiesDev B, SimParams P, uint32_t*
                                              const uint32_t* __restrict__ epa_q, uint32_t epa_cap, EpaBig big) {
     const uint32_t nq = min(counters[CNT_EPAQ], epa_cap);
     uint32_t* fault = counters + CNT_FAULT;
-    SupV verts[EPA_MAXV];
-    Face faces[EPA_MAXF];
-    int hor[EPA_MAXH][2];
+    SupV verts[EPA1_MAXV];
+    Face faces[EPA1_MAXF];
+    int hor[EPA1_MAXH][2];
     EpaWS ws{};
-    ws.verts = verts; ws.faces = faces; ws.maxf = EPA_MAXF; ws.big = false;
-    ws.list.hor = hor; ws.list.cap = EPA_MAXH;
+    ws.verts = verts; ws.faces = faces; ws.maxf = EPA1_MAXF; ws.maxv = EPA1_MAXV; ws.big = false;
+    ws.list.hor = hor; ws.list.cap = EPA1_MAXH;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += gridDim.x * blockDim.x) {
         const uint32_t p = epa_q[t];
         ws.overflow = false;
@@ -1416,12 +1416,12 @@ __global__ void __launch_bounds__(TPB) k_mesh_narrow(BodiesDev B, SimParams P, M
                                                      double* __restrict__ mct, EpaBig big) {
     const uint32_t nq = counters[CNT_MQ];
     uint32_t* fault = counters + CNT_FAULT;
-    SupV verts[EPA_MAXV];
-    Face faces[EPA_MAXF];
-    int hor[EPA_MAXH][2];
+    SupV verts[EPA1_MAXV];
+    Face faces[EPA1_MAXF];
+    int hor[EPA1_MAXH][2];
     EpaWS ws{};
-    ws.verts = verts; ws.faces = faces; ws.maxf = EPA_MAXF; ws.big = false;
-    ws.list.hor = hor; ws.list.cap = EPA_MAXH;
+    ws.verts = verts; ws.faces = faces; ws.maxf = EPA1_MAXF; ws.maxv = EPA1_MAXV; ws.big = false;
+    ws.list.hor = hor; ws.list.cap = EPA1_MAXH;
     // the prism's six vertices in LDS (every support call reads them; in
     // per-thread scratch they were most of this kernel's memory traffic)
     __shared__ double lds_pv[TPB * PV_STRIDE];

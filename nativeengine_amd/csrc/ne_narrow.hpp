@@ -154,9 +154,16 @@ __device__ inline bool gjk_tetra(const Geom& A, const Geom& B, V3 shift, SupV te
 
 // EPA workspace bounds: <= 4 + 64 vertices; a convex polytope on V vertices
 // has <= 2V-4 faces.  Faces/horizon carry headroom for near-degenerate hulls.
-constexpr int EPA_MAXV = 68;
+constexpr int EPA_MAXV = 68;    // 4 + 64 iterations: every vertex an EPA run can make
 constexpr int EPA_MAXF = 160;
 constexpr int EPA_MAXH = 96;
+// The per-thread workspace of the first tier.  Nearly every run ends within
+// a few iterations on a few dozen faces; the rare run that needs more stops
+// with `overflow` set and is rerun from the start by the wave-cooperative
+// second tier.  A small workspace keeps more waves resident.
+constexpr int EPA1_MAXV = 36;
+constexpr int EPA1_MAXF = 64;
+constexpr int EPA1_MAXH = 48;
 
 struct Face {
     V3 n;
@@ -276,7 +283,7 @@ __device__ inline Contact epa_result(const SupV* verts, const Face& best) {
 // stops (the result is then not used).
 template <class Hz>
 __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, const SupV tet[4], SupV* verts,
-                                      Face* faces, int maxf, Hz& hz, bool& overflow) {
+                                      Face* faces, int maxf, int maxv, Hz& hz, bool& overflow) {
     int nv = 4;
     for (int k = 0; k < 4; ++k) verts[k] = tet[k];
     int nf = 4;
@@ -293,6 +300,10 @@ __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, co
         SupV p = cso_support(A, B, shift, best.n);
         double d = dot(p.v, best.n);
         if (d - best.dist < 1e-7) break;
+        if (nv == maxv) {   // the workspace holds no more vertices
+            overflow = true;
+            break;
+        }
         int pidx = nv;
         verts[nv++] = p;   // nv <= 4 + 64 == EPA_MAXV
         hz.begin();
@@ -328,6 +339,7 @@ struct EpaWS {
     SupV* verts;
     Face* faces;
     int maxf;
+    int maxv;
     bool big;
     HorizonList list;
     HorizonTable table;
@@ -345,8 +357,8 @@ __device__ inline Contact convex_contact(const Box3& box, const Geom& A, const G
         c.hit = 2;
         return c;
     }
-    if (ws.big) return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.table, ws.overflow);
-    return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.list, ws.overflow);
+    if (ws.big) return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.maxv, ws.table, ws.overflow);
+    return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.maxv, ws.list, ws.overflow);
 }
 
 // src/narrowphase.hpp:34-81
