@@ -122,7 +122,10 @@ enum CounterSlot : int {
     CNT_MQRAW = 16,      // mesh triangle candidates before clamping (this step)
     CNT_MQ = 17,         // mesh triangle candidates (clamped)
     CNT_WFT = 18,        // tasks (contact pairs) of the large islands (this step)
-    CNT_N = 20
+    CNT_EPALONG = 19,    // convex pairs whose EPA ran past the first launch's budget (this step)
+    CNT_MQLONG = 20,     // mesh triangle candidates likewise
+    CNT_EPABIG_PAIRS = 21,   // pair entries of the large-workspace queue (mesh entries follow them)
+    CNT_N = 24
 };
 
 constexpr int TPB = 256;
@@ -1024,6 +1027,7 @@ __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimPar
 // in global memory and the table-driven horizon.  Both capacities grow when
 // exceeded (the step is then run again).
 constexpr uint32_t EPA_BIG_PAIRS = 64;    // initial queue capacity
+constexpr int EPA_FIRST_ITERS = 8;        // EPA iterations of the first tier-1 launch
 constexpr int EPA_BIG_FACES = 4096;       // initial faces (and horizon entries) per queued pair
 constexpr int EPA_BIG_FACES_MAX = 1 << 17;   // beyond this a run stops at its closest face
 constexpr int EPA_TAB = EPA_MAXV * EPA_MAXV;
@@ -1053,6 +1057,7 @@ __global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, uint32
                                                 uint32_t* __restrict__ epa_q, uint32_t epa_cap) {
     const uint32_t np = counters[CNT_P];
     uint32_t* fault = counters + CNT_FAULT;
+    if (!SPHERES_ONLY && blockIdx.x == 0 && threadIdx.x == 0) counters[CNT_EPALONG] = 0u;   // k_epa runs after
     EpaWS ws{};
     ws.defer = true;
     for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < np; p0 += gridDim.x * blockDim.x) {
@@ -1072,11 +1077,17 @@ __global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, uint32
     }
 }
 
+// EPA over a queue of overlapping convex pairs, at most `max_iter`
+// iterations each.  The first launch gives every pair a short budget (most
+// runs end within it), so a wave is not held for long by its slowest pair;
+// pairs that need more go to `long_q` and the second launch reruns them from
+// the start with the reference's full 64.
 __global__ void __launch_bounds__(TPB) k_epa(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
                                              const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
                                              uint32_t* __restrict__ ncont, double* __restrict__ ct,
-                                             const uint32_t* __restrict__ epa_q, uint32_t epa_cap, EpaBig big) {
-    const uint32_t nq = min(counters[CNT_EPAQ], epa_cap);
+                                             const uint32_t* __restrict__ epa_q, int count_slot, uint32_t epa_cap,
+                                             EpaBig big, int max_iter, uint32_t* __restrict__ long_q) {
+    const uint32_t nq = min(counters[count_slot], epa_cap);
     uint32_t* fault = counters + CNT_FAULT;
     SupV verts[EPA1_MAXV];
     Face faces[EPA1_MAXF];
@@ -1084,17 +1095,30 @@ __global__ void __launch_bounds__(TPB) k_epa(BodiesDev B, SimParams P, uint32_t*
     EpaWS ws{};
     ws.verts = verts; ws.faces = faces; ws.maxf = EPA1_MAXF; ws.maxv = EPA1_MAXV; ws.big = false;
     ws.list.hor = hor; ws.list.cap = EPA1_MAXH;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += gridDim.x * blockDim.x) {
-        const uint32_t p = epa_q[t];
-        ws.overflow = false;
-        uint32_t nc = narrow_pair<false>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
-        if (ws.overflow) {
-            const uint32_t slot = atomicAdd(&counters[CNT_EPABIG], 1u);
-            if (slot < big.cap) big.list[slot] = p;
-            else atomicOr(fault, NE_FAULT_EPA_QUEUE);
-            nc = 0;
+    ws.max_iter = max_iter;
+    for (uint32_t t0 = blockIdx.x * blockDim.x; t0 < nq; t0 += gridDim.x * blockDim.x) {
+        const uint32_t t = t0 + threadIdx.x;
+        const uint32_t p = t < nq ? epa_q[t] : 0u;
+        bool defer = false;
+        if (t < nq) {
+            ws.overflow = false;
+            ws.too_long = false;
+            uint32_t nc = narrow_pair<false>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
+            if (ws.overflow) {
+                const uint32_t slot = atomicAdd(&counters[CNT_EPABIG], 1u);
+                if (slot < big.cap) big.list[slot] = p;
+                else atomicOr(fault, NE_FAULT_EPA_QUEUE);
+                nc = 0;
+            } else if (ws.too_long) {
+                defer = true;
+                nc = 0;
+            }
+            ncont[p] = nc;
         }
-        ncont[p] = nc;
+        if (long_q) {
+            const uint32_t slot = wave_add1(counters, CNT_EPALONG, defer);
+            if (defer) long_q[slot] = p;   // at most one entry per queued pair: fits the queue's capacity
+        }
     }
 }
 
@@ -1199,6 +1223,12 @@ __device__ Contact epa_wave(const Geom& A, const Geom& B, V3 shift, const SupV t
     return epa_result(verts, best);
 }
 
+// The queue's pair entries, counted into CNT_EPABIG_PAIRS before the mesh
+// kernels append theirs: this kernel may run beside them on a second stream.
+__global__ void k_snap_counter(uint32_t* counters, int from, int to) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[to] = counters[from];
+}
+
 __global__ void __launch_bounds__(64) k_narrow_big(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
                                                    const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
                                                    uint32_t* __restrict__ ncont, double* __restrict__ ct, EpaBig big) {
@@ -1206,7 +1236,7 @@ __global__ void __launch_bounds__(64) k_narrow_big(BodiesDev B, SimParams P, uin
     __shared__ int head[EPA_TAB], tail[EPA_TAB], stamp[EPA_TAB];
     __shared__ int stage[3 * 64];
     __shared__ int shared[4];
-    const uint32_t nbig = min(counters[CNT_EPABIG], big.cap);
+    const uint32_t nbig = min(counters[CNT_EPABIG_PAIRS], big.cap);
     uint32_t* fault = counters + CNT_FAULT;
     if (blockIdx.x == 0 && threadIdx.x == 0 && nbig) atomicAdd(&counters[CNT_EPARERUN], nbig);
     for (uint32_t t = blockIdx.x; t < nbig; t += gridDim.x) {
@@ -1408,13 +1438,16 @@ constexpr int PV_STRIDE = 19;   // doubles per thread for a prism's vertices (18
 // detectContact(other, prism) per triangle candidate (GJK + EPA with the
 // per-thread workspace); EPA runs that outgrow it go to the shared
 // large-workspace queue (k_mesh_big).
+// As k_epa, two launches: every candidate with a short EPA budget, then the
+// ones that needed more (`list` = the first launch's `long_q`) in full.
 __global__ void __launch_bounds__(TPB) k_mesh_narrow(BodiesDev B, SimParams P, MeshDev M, uint32_t* __restrict__ counters,
                                                      const uint32_t* __restrict__ cand_i,
                                                      const uint32_t* __restrict__ cand_j,
                                                      const uint32_t* __restrict__ mq_pair,
                                                      const int32_t* __restrict__ mq_tri, uint8_t* __restrict__ mhit,
-                                                     double* __restrict__ mct, EpaBig big) {
-    const uint32_t nq = counters[CNT_MQ];
+                                                     double* __restrict__ mct, EpaBig big, const uint32_t* __restrict__ list,
+                                                     int count_slot, int max_iter, uint32_t* __restrict__ long_q) {
+    const uint32_t nq = counters[count_slot];
     uint32_t* fault = counters + CNT_FAULT;
     SupV verts[EPA1_MAXV];
     Face faces[EPA1_MAXF];
@@ -1422,31 +1455,45 @@ __global__ void __launch_bounds__(TPB) k_mesh_narrow(BodiesDev B, SimParams P, M
     EpaWS ws{};
     ws.verts = verts; ws.faces = faces; ws.maxf = EPA1_MAXF; ws.maxv = EPA1_MAXV; ws.big = false;
     ws.list.hor = hor; ws.list.cap = EPA1_MAXH;
+    ws.max_iter = max_iter;
     // the prism's six vertices in LDS (every support call reads them; in
     // per-thread scratch they were most of this kernel's memory traffic)
     __shared__ double lds_pv[TPB * PV_STRIDE];
     double* pv = lds_pv + threadIdx.x * PV_STRIDE;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
-        const uint32_t p = mq_pair[q];
-        uint32_t o, m;
-        bool flip;
-        mesh_pair(B, cand_i[p], cand_j[p], o, m, flip);
-        Geom tri;
-        uint8_t hit = 0;
-        if (mesh_prism(B, M, m, mq_tri[q], pv, tri)) {
-            ws.overflow = false;
-            const Geom other = load_geom(B, o);
-            const Contact c = detect_contact(box_of(B, P, o), other, tri, ws);
-            if (ws.overflow) {
-                const uint32_t slot = atomicAdd(&counters[CNT_EPABIG], 1u);
-                if (slot < big.cap) big.list[slot] = MESH_ENTRY | q;
-                else atomicOr(fault, NE_FAULT_EPA_QUEUE);
-            } else if (c.hit) {
-                store_contact(mct + 7 * (size_t)q, c, flip);
-                hit = 1;
+    for (uint32_t u0 = blockIdx.x * blockDim.x; u0 < nq; u0 += gridDim.x * blockDim.x) {
+        const uint32_t u = u0 + threadIdx.x;
+        bool defer = false;
+        uint32_t q = 0;
+        if (u < nq) {
+            q = list ? list[u] : u;
+            const uint32_t p = mq_pair[q];
+            uint32_t o, m;
+            bool flip;
+            mesh_pair(B, cand_i[p], cand_j[p], o, m, flip);
+            Geom tri;
+            uint8_t hit = 0;
+            if (mesh_prism(B, M, m, mq_tri[q], pv, tri)) {
+                ws.overflow = false;
+                ws.too_long = false;
+                const Geom other = load_geom(B, o);
+                const Contact c = detect_contact(box_of(B, P, o), other, tri, ws);
+                if (ws.overflow) {
+                    const uint32_t slot = atomicAdd(&counters[CNT_EPABIG], 1u);
+                    if (slot < big.cap) big.list[slot] = MESH_ENTRY | q;
+                    else atomicOr(fault, NE_FAULT_EPA_QUEUE);
+                } else if (ws.too_long) {
+                    defer = true;
+                } else if (c.hit) {
+                    store_contact(mct + 7 * (size_t)q, c, flip);
+                    hit = 1;
+                }
             }
+            mhit[q] = hit;
         }
-        mhit[q] = hit;
+        if (long_q) {
+            const uint32_t slot = wave_add1(counters, CNT_MQLONG, defer);
+            if (defer) long_q[slot] = q;   // at most one entry per candidate: fits the candidate capacity
+        }
     }
 }
 
@@ -3077,6 +3124,7 @@ struct ne_gpu_ctx {
     uint32_t *ex_flags = nullptr, *ex_off = nullptr;
     uint32_t *cand_i = nullptr, *cand_j = nullptr;
     uint32_t* epa_q = nullptr;        // convex pairs queued for EPA (pcap entries)
+    uint32_t* epa_long = nullptr;     // of those, the ones the first EPA launch left for the second (pcap entries)
     uint32_t *ncont = nullptr, *cons_off = nullptr;
     double* ct = nullptr;
     // constraints (current) + warm cache (previous)
@@ -3148,6 +3196,7 @@ struct ne_gpu_ctx {
     // per step: triangle candidates of the mesh pairs
     uint32_t *mq_cnt = nullptr, *mq_off = nullptr, *mq_pair = nullptr;
     int32_t* mq_tri = nullptr;
+    uint32_t* mq_long = nullptr;                   // candidates the first mesh launch left for the second
     uint8_t* mhit = nullptr;
     double* mct = nullptr;                         // 7 doubles per candidate: normal, point, depth
     uint32_t mqcap = 0, mq_pcap = 0, hint_mq = 4096;
@@ -3180,6 +3229,8 @@ struct ne_gpu_ctx {
     uint32_t wf_rounds_hint = 0;                      // rounds the last solve took
     uint32_t* h_wf = nullptr;                         // pinned: values read back mid-step
     bool capturing = false;                           // a step is being captured into a graph
+    hipStream_t side = nullptr;                       // second stream (default priority) for work beside the main one
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // counters
     uint32_t* counters = nullptr;
     uint32_t* h_counters = nullptr;
@@ -3346,6 +3397,7 @@ int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
     HIP_TRY(c, dalloc(c->cand_i, pcap));
     HIP_TRY(c, dalloc(c->cand_j, pcap));
     HIP_TRY(c, dalloc(c->epa_q, pcap));
+    HIP_TRY(c, dalloc(c->epa_long, pcap));
     HIP_TRY(c, dalloc(c->isp, pcap));
     HIP_TRY(c, dalloc(c->ncont, (size_t)pcap + 1));
     HIP_TRY(c, dalloc(c->cons_off, (size_t)pcap + 1));
@@ -3392,6 +3444,7 @@ int alloc_mesh_cands(ne_gpu_ctx* c, uint32_t cap) {
     c->mqcap = cap;
     HIP_TRY(c, dalloc(c->mq_pair, cap));
     HIP_TRY(c, dalloc(c->mq_tri, cap));
+    HIP_TRY(c, dalloc(c->mq_long, cap));
     HIP_TRY(c, dalloc(c->mhit, cap));
     HIP_TRY(c, dalloc(c->mct, (size_t)cap * 7));
     return NE_OK;
@@ -3405,7 +3458,8 @@ void free_meshes(ne_gpu_ctx* c) {
 }
 
 void free_mesh_work(ne_gpu_ctx* c) {
-    dfree(c->mq_cnt); dfree(c->mq_off); dfree(c->mq_pair); dfree(c->mq_tri); dfree(c->mhit); dfree(c->mct);
+    dfree(c->mq_cnt); dfree(c->mq_off); dfree(c->mq_pair); dfree(c->mq_tri); dfree(c->mq_long); dfree(c->mhit);
+    dfree(c->mct);
     c->mqcap = c->mq_pcap = 0;
 }
 
@@ -3462,7 +3516,7 @@ void free_work(ne_gpu_ctx* c) {
     dfree(c->cell_of); dfree(c->cell_count); dfree(c->cell_start); dfree(c->sorted_idx); dfree(c->sorted);
     dfree(c->order_prev); dfree(c->xp);
     dfree(c->row_count); dfree(c->row_start); dfree(c->ex_flags); dfree(c->ex_off);
-    dfree(c->cand_i); dfree(c->cand_j); dfree(c->epa_q); dfree(c->isp);
+    dfree(c->cand_i); dfree(c->cand_j); dfree(c->epa_q); dfree(c->epa_long); dfree(c->isp);
     dfree(c->wake_at); dfree(c->mint); dfree(c->wake_again); dfree(c->ncont); dfree(c->cons_off); dfree(c->ct);
     ConsDev& C = c->C;
     dfree(C.ci); dfree(C.cj); dfree(C.slot);
@@ -3553,6 +3607,14 @@ JointDev joint_dev(const ne_gpu_ctx* c) {
     return J;
 }
 
+int ensure_side(ne_gpu_ctx* c) {
+    if (c->side) return NE_OK;
+    HIP_TRY(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIP_TRY(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIP_TRY(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    return NE_OK;
+}
+
 // The step up to the solve: integrate, grid, candidate rows, narrow phase,
 // constraint build and islands.
 void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
@@ -3632,6 +3694,7 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
         k_excl_write<<<g, TPB, 0, s>>>(B, c->excl, c->ex_flags, c->ex_off, c->row_start, c->pcap, c->cand_i, c->cand_j);
     }
     // --- narrow phase + constraints
+    bool fork = false;
     {
         KTimer kt(c, NE_KERNEL_NARROW);
         if (c->spheres_only) {
@@ -3641,8 +3704,22 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
             k_narrow<false><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont,
                                                                c->ct, c->epa_q, c->pcap);
             k_epa<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                         c->epa_q, c->pcap, c->epa_big);
-            k_narrow_big<<<big_blocks(c), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_big);
+                                                         c->epa_q, CNT_EPAQ, c->pcap, c->epa_big, EPA_FIRST_ITERS,
+                                                         c->epa_long);
+            k_epa<<<hint_blocks(c->hint_c / 8), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
+                                                             c->epa_long, CNT_EPALONG, c->pcap, c->epa_big, 64, nullptr);
+            k_snap_counter<<<1, 64, 0, s>>>(c->counters, CNT_EPABIG, CNT_EPABIG_PAIRS);
+            // The large-workspace reruns are few pairs on few waves: in eager
+            // steps they run on the second stream beside the mesh kernels.
+            // (If a stream call fails, the host waits for the second stream instead.)
+            fork = !c->capturing && c->nmeshes && ensure_side(c) == NE_OK &&
+                   hipEventRecord(c->ev_fork, s) == hipSuccess && hipStreamWaitEvent(c->side, c->ev_fork, 0) == hipSuccess;
+            k_narrow_big<<<big_blocks(c), 64, 0, fork ? c->side : s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont,
+                                                                       c->ct, c->epa_big);
+            if (fork && hipEventRecord(c->ev_join, c->side) != hipSuccess) {
+                (void)hipStreamSynchronize(c->side);
+                fork = false;
+            }
         }
         if (c->nmeshes) {   // meshContacts (world.cpp:56-83)
             const MeshDev M = mesh_dev(c);
@@ -3651,13 +3728,19 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
                      ScanTotal{c->counters, CNT_MQRAW, CNT_MQ, CNT_FAULT, c->mqcap, NE_FAULT_MESH, -1, -1});
             k_mesh_emit<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_off,
                                                                c->mqcap, c->mq_pair, c->mq_tri);
+            k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_MQLONG, 0u, nullptr, 1u);
             k_mesh_narrow<<<hint_blocks(c->hint_mq), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
-                                                                  c->mq_tri, c->mhit, c->mct, c->epa_big);
+                                                                  c->mq_tri, c->mhit, c->mct, c->epa_big, nullptr, CNT_MQ,
+                                                                  EPA_FIRST_ITERS, c->mq_long);
+            k_mesh_narrow<<<hint_blocks(c->hint_mq / 8), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j,
+                                                                      c->mq_pair, c->mq_tri, c->mhit, c->mct, c->epa_big,
+                                                                      c->mq_long, CNT_MQLONG, 64, nullptr);
             k_mesh_big<<<big_blocks(c), 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri, c->mhit,
                                          c->mct, c->epa_big);
             k_mesh_gather<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, c->counters, c->cand_i, c->cand_j, c->mq_off,
                                                                  c->mqcap, c->mhit, c->mct, c->ncont);
         }
+        if (fork && hipStreamWaitEvent(s, c->ev_join, 0) != hipSuccess) (void)hipStreamSynchronize(c->side);
     }
     if (P.sleep) {   // wakes in pair order (world.cpp:527-531), then the woken bodies' masses
         k_sleep_reset<<<blocks_for(n), TPB, 0, s>>>(B, c->wake_at, c->wake_again);
@@ -4350,6 +4433,12 @@ int ne_gpu_destroy(ne_gpu_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->ev_start) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->ev_stop) (void)hipEventDestroy(e);
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipEventDestroy(c->ev_fork);
+        (void)hipEventDestroy(c->ev_join);
+        (void)hipStreamDestroy(c->side);
+    }
     free_all(c);
     (void)hipStreamDestroy(c->stream);
     delete c;

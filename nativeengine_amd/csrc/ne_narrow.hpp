@@ -283,7 +283,8 @@ __device__ inline Contact epa_result(const SupV* verts, const Face& best) {
 // stops (the result is then not used).
 template <class Hz>
 __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, const SupV tet[4], SupV* verts,
-                                      Face* faces, int maxf, int maxv, Hz& hz, bool& overflow) {
+                                      Face* faces, int maxf, int maxv, int max_iter, Hz& hz, bool& overflow,
+                                      bool& too_long) {
     int nv = 4;
     for (int k = 0; k < 4; ++k) verts[k] = tet[k];
     int nf = 4;
@@ -293,6 +294,10 @@ __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, co
     faces[3] = make_face(verts, 1, 3, 2);
     Face best = faces[0];
     for (int iter = 0; iter < 64; ++iter) {
+        if (iter == max_iter) {   // not converged within this launch's budget: rerun later from the start
+            too_long = true;
+            break;
+        }
         int ci = 0;
         for (int i = 1; i < nf; ++i)
             if (faces[i].dist < faces[ci].dist) ci = i;
@@ -340,6 +345,8 @@ struct EpaWS {
     Face* faces;
     int maxf;
     int maxv;
+    int max_iter;      // EPA iterations this launch runs; a run that needs more sets too_long
+    bool too_long;
     bool big;
     HorizonList list;
     HorizonTable table;
@@ -357,8 +364,11 @@ __device__ inline Contact convex_contact(const Box3& box, const Geom& A, const G
         c.hit = 2;
         return c;
     }
-    if (ws.big) return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.maxv, ws.table, ws.overflow);
-    return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.maxv, ws.list, ws.overflow);
+    if (ws.big)
+        return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.maxv, ws.max_iter, ws.table, ws.overflow,
+                           ws.too_long);
+    return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.maxv, ws.max_iter, ws.list, ws.overflow,
+                       ws.too_long);
 }
 
 // src/narrowphase.hpp:34-81

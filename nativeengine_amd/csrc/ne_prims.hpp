@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* __restrict__ partia
         if (tot.counters) {
             uint32_t r = carry;
             tot.counters[tot.raw_slot] = r;
-            if (r > tot.cap) { tot.counters[tot.fault_slot] |= tot.fault_bit; r = tot.cap; }
+            if (r > tot.cap) { atomicOr(&tot.counters[tot.fault_slot], tot.fault_bit); r = tot.cap; }
             tot.counters[tot.eff_slot] = r;
             if (tot.zero_slot0 >= 0) tot.counters[tot.zero_slot0] = 0u;
             if (tot.zero_slot1 >= 0) tot.counters[tot.zero_slot1] = 0u;
