@@ -125,6 +125,7 @@ enum CounterSlot : int {
     CNT_EPALONG = 19,    // convex pairs whose EPA ran past the first launch's budget (this step)
     CNT_MQLONG = 20,     // mesh triangle candidates likewise
     CNT_EPABIG_PAIRS = 21,   // pair entries of the large-workspace queue (mesh entries follow them)
+    CNT_EPALONG2 = 22,   // convex pairs whose EPA ran past the second launch's budget (this step)
     CNT_N = 24
 };
 
@@ -1028,6 +1029,7 @@ __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimPar
 // exceeded (the step is then run again).
 constexpr uint32_t EPA_BIG_PAIRS = 64;    // initial queue capacity
 constexpr int EPA_FIRST_ITERS = 8;        // EPA iterations of the first tier-1 launch
+constexpr int EPA_SECOND_ITERS = 24;      // of the second (the third runs the rest)
 constexpr int EPA_BIG_FACES = 4096;       // initial faces (and horizon entries) per queued pair
 constexpr int EPA_BIG_FACES_MAX = 1 << 17;   // beyond this a run stops at its closest face
 constexpr int EPA_TAB = EPA_MAXV * EPA_MAXV;
@@ -1088,6 +1090,7 @@ __global__ void __launch_bounds__(TPB) k_epa(BodiesDev B, SimParams P, uint32_t*
                                              const uint32_t* __restrict__ epa_q, int count_slot, uint32_t epa_cap,
                                              EpaBig big, int max_iter, uint32_t* __restrict__ long_q) {
     const uint32_t nq = min(counters[count_slot], epa_cap);
+    const int long_slot = count_slot == CNT_EPAQ ? CNT_EPALONG : CNT_EPALONG2;
     uint32_t* fault = counters + CNT_FAULT;
     SupV verts[EPA1_MAXV];
     Face faces[EPA1_MAXF];
@@ -1116,7 +1119,7 @@ __global__ void __launch_bounds__(TPB) k_epa(BodiesDev B, SimParams P, uint32_t*
             ncont[p] = nc;
         }
         if (long_q) {
-            const uint32_t slot = wave_add1(counters, CNT_EPALONG, defer);
+            const uint32_t slot = wave_add1(counters, long_slot, defer);
             if (defer) long_q[slot] = p;   // at most one entry per queued pair: fits the queue's capacity
         }
     }
@@ -2149,6 +2152,7 @@ struct WfDev {
     uint32_t* inc_start;   // per body: start of its slice of inc_list
     uint32_t* inc_list;    // per movable body: its tasks
     uint32_t* task_n;      // per task: constraints in it
+    uint4* task4;          // per task: first constraint, constraints, body i, body j (one load per visit)
     double* rec;           // per constraint of a task: n, rA, rB, restBias, Kn, mu, nImp, tImp, overlap (16 doubles)
     double* bp;            // per body: v, w, q, invI, invMass (16 doubles), the sweep's working copy
 };
@@ -2222,6 +2226,7 @@ __global__ void __launch_bounds__(TPB) k_wf_tasks(BodiesDev B, const uint32_t* _
         uint32_t k1 = k + 1;
         while (k1 < nc && C.ci[k1] == i && C.cj[k1] == j) ++k1;
         W.task_n[t] = k1 - k;
+        W.task4[t] = make_uint4(k, k1 - k, i, j);
         for (uint32_t q = k; q < k1; ++q) {   // the sweep's record of constraint q
             double2* r = reinterpret_cast<double2*>(W.rec + (size_t)q * WF_REC);
             r[0] = double2{C.n0[q], C.n1[q]};
@@ -2304,13 +2309,20 @@ __device__ __forceinline__ void wf_visit(BodiesDev& B, const SimParams& P, const
         store_packed(W.bp, j, b);
     } else if (pass <= (uint32_t)P.iters) {
         BodyView a = load_packed(W.bp, i), b = load_packed(W.bp, j);
+        // the next record is loaded while this one is solved (the records
+        // do not depend on the sweep's results; only nImp/tImp are written)
+        const double2* r = reinterpret_cast<const double2*>(W.rec + (size_t)k0 * WF_REC);
+        double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6];
         for (uint32_t k = k0; k < k1; ++k) {
-            double2* r = reinterpret_cast<double2*>(W.rec + (size_t)k * WF_REC);
-            const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6];
             const ConsGeom g{V3{r0.x, r0.y, r1.x}, V3{r1.y, r2.x, r2.y}, V3{r3.x, r3.y, r4.x}};
+            const double rbias = r4.y, kn = r5.x, mu = r5.y;
             double nimp = r6.x, timp = r6.y;
-            gs_visit(a, b, g, r4.y, r5.x, r5.y, nimp, timp);
-            r[6] = double2{nimp, timp};
+            if (k + 1 < k1) {
+                const double2* rn = reinterpret_cast<const double2*>(W.rec + (size_t)(k + 1) * WF_REC);
+                r0 = rn[0]; r1 = rn[1]; r2 = rn[2]; r3 = rn[3]; r4 = rn[4]; r5 = rn[5]; r6 = rn[6];
+            }
+            gs_visit(a, b, g, rbias, kn, mu, nimp, timp);
+            reinterpret_cast<double2*>(W.rec + (size_t)k * WF_REC)[6] = double2{nimp, timp};
         }
         store_packed(W.bp, i, a);
         store_packed(W.bp, j, b);
@@ -2351,11 +2363,11 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(BodiesDev B, SimParams 
         if (u < cnt) {
             const uint32_t v = W.fr[base + u];
             const uint32_t pass = v / T, t = v - pass * T;
-            const uint32_t k0 = W.task_k[t];
-            const uint32_t i = C.ci[k0], j = C.cj[k0];
-            wf_visit(B, P, C, W, pass, i, j, k0, k0 + W.task_n[t], prev_key, prev_imp);
+            const uint4 tk = W.task4[t];
+            const uint2 nxt = reinterpret_cast<const uint2*>(W.next)[t];
+            wf_visit(B, P, C, W, pass, tk.z, tk.w, tk.x, tk.x + tk.y, prev_key, prev_imp);
             for (int side = 0; side < 2; ++side) {
-                const uint32_t nx = W.next[2 * t + side];
+                const uint32_t nx = side ? nxt.y : nxt.x;
                 if (nx == WF_NONE) continue;
                 if (nx & WF_WRAP) {
                     if (pass + 1 >= npass) continue;
@@ -2366,10 +2378,15 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(BodiesDev B, SimParams 
                 ready[side] = atomicSub(&W.pend[tv[side]], 1u) == 1u;
             }
         }
-        for (int side = 0; side < 2; ++side) {
-            const uint32_t pos = wave_add1(W.rcnt, r + 1, ready[side]);
-            if (ready[side]) W.fr[nbase + pos] = tv[side];
-        }
+        // one append per wave for both sides: every lane targets round r + 1
+        const uint64_t b0 = __ballot(ready[0]), b1 = __ballot(ready[1]);
+        const uint32_t tot = (uint32_t)(__popcll(b0) + __popcll(b1));
+        uint32_t slot0 = 0;
+        if (lane_id() == 0 && tot) slot0 = atomicAdd(&W.rcnt[r + 1], tot);
+        slot0 = __shfl(slot0, 0);
+        const uint64_t below = lane_id() ? (~0ull >> (64 - lane_id())) : 0ull;
+        if (ready[0]) W.fr[nbase + slot0 + (uint32_t)__popcll(b0 & below)] = tv[0];
+        if (ready[1]) W.fr[nbase + slot0 + (uint32_t)__popcll(b0) + (uint32_t)__popcll(b1 & below)] = tv[1];
     }
 }
 
@@ -3125,6 +3142,7 @@ struct ne_gpu_ctx {
     uint32_t *cand_i = nullptr, *cand_j = nullptr;
     uint32_t* epa_q = nullptr;        // convex pairs queued for EPA (pcap entries)
     uint32_t* epa_long = nullptr;     // of those, the ones the first EPA launch left for the second (pcap entries)
+    uint32_t* epa_long2 = nullptr;    // and the ones the second left for the third
     uint32_t *ncont = nullptr, *cons_off = nullptr;
     double* ct = nullptr;
     // constraints (current) + warm cache (previous)
@@ -3398,6 +3416,7 @@ int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
     HIP_TRY(c, dalloc(c->cand_j, pcap));
     HIP_TRY(c, dalloc(c->epa_q, pcap));
     HIP_TRY(c, dalloc(c->epa_long, pcap));
+    HIP_TRY(c, dalloc(c->epa_long2, pcap));
     HIP_TRY(c, dalloc(c->isp, pcap));
     HIP_TRY(c, dalloc(c->ncont, (size_t)pcap + 1));
     HIP_TRY(c, dalloc(c->cons_off, (size_t)pcap + 1));
@@ -3516,7 +3535,7 @@ void free_work(ne_gpu_ctx* c) {
     dfree(c->cell_of); dfree(c->cell_count); dfree(c->cell_start); dfree(c->sorted_idx); dfree(c->sorted);
     dfree(c->order_prev); dfree(c->xp);
     dfree(c->row_count); dfree(c->row_start); dfree(c->ex_flags); dfree(c->ex_off);
-    dfree(c->cand_i); dfree(c->cand_j); dfree(c->epa_q); dfree(c->epa_long); dfree(c->isp);
+    dfree(c->cand_i); dfree(c->cand_j); dfree(c->epa_q); dfree(c->epa_long); dfree(c->epa_long2); dfree(c->isp);
     dfree(c->wake_at); dfree(c->mint); dfree(c->wake_again); dfree(c->ncont); dfree(c->cons_off); dfree(c->ct);
     ConsDev& C = c->C;
     dfree(C.ci); dfree(C.cj); dfree(C.slot);
@@ -3560,7 +3579,7 @@ void free_wf(ne_gpu_ctx* c) {
     WfDev& W = c->wf;
     dfree(W.flag); dfree(W.tid); dfree(W.task_k); dfree(W.next); dfree(W.pend0); dfree(W.pend); dfree(W.fr);
     dfree(W.rcnt); dfree(W.roff); dfree(W.inc_cnt); dfree(W.inc_start); dfree(W.inc_list);
-    dfree(W.task_n); dfree(W.rec); dfree(W.bp);
+    dfree(W.task_n); dfree(W.task4); dfree(W.rec); dfree(W.bp);
     c->wf_tcap = c->wf_ccap = c->wf_ncap = c->wf_vcap_pass = 0;
 }
 
@@ -3706,8 +3725,12 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
             k_epa<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
                                                          c->epa_q, CNT_EPAQ, c->pcap, c->epa_big, EPA_FIRST_ITERS,
                                                          c->epa_long);
+            k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_EPALONG2, 0u, nullptr, 1u);
             k_epa<<<hint_blocks(c->hint_c / 8), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                             c->epa_long, CNT_EPALONG, c->pcap, c->epa_big, 64, nullptr);
+                                                             c->epa_long, CNT_EPALONG, c->pcap, c->epa_big,
+                                                             EPA_SECOND_ITERS, c->epa_long2);
+            k_epa<<<hint_blocks(c->hint_c / 64), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
+                                                              c->epa_long2, CNT_EPALONG2, c->pcap, c->epa_big, 64, nullptr);
             k_snap_counter<<<1, 64, 0, s>>>(c->counters, CNT_EPABIG, CNT_EPABIG_PAIRS);
             // The large-workspace reruns are few pairs on few waves: in eager
             // steps they run on the second stream beside the mesh kernels.
@@ -3795,6 +3818,7 @@ int ensure_wf(ne_gpu_ctx* c, uint32_t T, uint32_t npass) {
         const uint32_t cap = T + T / 2 + 1024;
         HIP_TRY(c, dalloc(W.task_k, cap));
         HIP_TRY(c, dalloc(W.task_n, cap));
+        HIP_TRY(c, dalloc(W.task4, cap));
         HIP_TRY(c, dalloc(W.next, 2 * (size_t)cap));
         HIP_TRY(c, dalloc(W.pend0, cap));
         HIP_TRY(c, dalloc(W.inc_list, 2 * (size_t)cap));
