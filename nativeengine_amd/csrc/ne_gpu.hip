@@ -138,12 +138,14 @@ constexpr int TPB = 256;
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 
 // atomicAdd(&ctr[key], 1) for each active lane; returns the lane's old value
-// (distinct per lane, in lane order within the wave).
+// (distinct per lane).  The keys of the first two leaders are combined; the
+// remaining lanes (keys that are not shared much, as in a gas of small
+// islands) add on their own, so the cost stays two rounds.
 __device__ __forceinline__ uint32_t wave_add1(uint32_t* ctr, uint32_t key, bool active) {
     uint64_t pending = __ballot(active);
     uint32_t out = 0;
     const uint64_t below = lane_id() ? (~0ull >> (64 - lane_id())) : 0ull;
-    while (pending) {
+    for (int round = 0; round < 2 && pending; ++round) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lkey = __shfl(key, leader);
         const uint64_t same = __ballot(active && key == lkey) & pending;
@@ -153,13 +155,14 @@ __device__ __forceinline__ uint32_t wave_add1(uint32_t* ctr, uint32_t key, bool 
         if ((same >> lane_id()) & 1ull) out = base + (uint32_t)__popcll(same & below);
         pending &= ~same;
     }
+    if ((pending >> lane_id()) & 1ull) out = atomicAdd(&ctr[key], 1u);
     return out;
 }
 
-// atomicMin(&ctr[key], val) for each active lane.
+// atomicMin(&ctr[key], val) for each active lane (the same two rounds).
 __device__ __forceinline__ void wave_min_u64(unsigned long long* ctr, uint32_t key, unsigned long long val, bool active) {
     uint64_t pending = __ballot(active);
-    while (pending) {
+    for (int round = 0; round < 2 && pending; ++round) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lkey = __shfl(key, leader);
         const uint64_t same = __ballot(active && key == lkey) & pending;
@@ -171,6 +174,7 @@ __device__ __forceinline__ void wave_min_u64(unsigned long long* ctr, uint32_t k
         if (lane_id() == leader) atomicMin(&ctr[lkey], m);
         pending &= ~same;
     }
+    if ((pending >> lane_id()) & 1ull) atomicMin(&ctr[key], val);
 }
 
 __device__ __forceinline__ V3 ld3(const double* a, const double* b, const double* c, uint32_t i) {
@@ -2348,6 +2352,11 @@ __device__ __forceinline__ void wf_visit(BodiesDev& B, const SimParams& P, const
     }
 }
 
+// k_solve_small counted the islands above its limit (CNT_NBIG).
+__global__ void k_wf_expect_none(uint32_t* counters) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && counters[CNT_NBIG] != 0u) atomicOr(&counters[CNT_FAULT], NE_FAULT_WF_PLAN);
+}
+
 // Round r: run its visits, then queue each successor whose last predecessor
 // this was for round r + 1.
 __global__ void __launch_bounds__(WF_THREADS) k_wf_round(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters,
@@ -3245,6 +3254,9 @@ struct ne_gpu_ctx {
     uint32_t wf_tcap = 0, wf_ccap = 0, wf_ncap = 0;   // capacities: tasks, constraints, bodies
     uint32_t wf_vcap_pass = 0;                        // passes the visit arrays hold per task
     uint32_t wf_rounds_hint = 0;                      // rounds the last solve took
+    bool wf_expect = true;                            // expect large islands: read their task count back
+    uint32_t wf_quiet = 0;                            // consecutive steps without large islands (checked ones)
+    bool wf_seen = false;                             // this call had a step with large islands
     uint32_t* h_wf = nullptr;                         // pinned: values read back mid-step
     bool capturing = false;                           // a step is being captured into a graph
     hipStream_t side = nullptr;                       // second stream (default priority) for work beside the main one
@@ -3841,9 +3853,14 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     hipStream_t s = c->stream;
     BodiesDev B = c->B;
     WfDev& W = c->wf;
+    if (!c->wf_expect) {
+        // recent steps had no large islands: do not stop the stream to ask;
+        // if this one has some, the call reruns from its start reading them back
+        k_wf_expect_none<<<1, 64, 0, s>>>(c->counters);
+        return NE_OK;
+    }
     int r0 = ensure_wf(c, 0, 1);
     if (r0 != NE_OK) return r0;
-    k_fill_u32<<<gs_blocks(c->n), TPB, 0, s>>>(W.inc_cnt, 0u, nullptr, c->n);
     k_wf_mark<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, c->cons_isl, c->isl_start, W.flag,
                                                      WF_ISLAND);
     scan_u32(W.flag, W.tid, c->counters + CNT_C, c->ccap, c->scan_part, s, false,
@@ -3851,11 +3868,17 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     HIP_TRY(c, hipMemcpyAsync(c->h_wf, c->counters + CNT_WFT, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
     const uint32_t T = c->h_wf[0];
-    if (T == 0) return NE_OK;
+    if (T == 0) {
+        ++c->wf_quiet;   // step_call stops asking after 8 such steps, between calls
+        return NE_OK;
+    }
+    c->wf_quiet = 0;
+    c->wf_seen = true;
     const uint32_t npass = (uint32_t)P.iters + 2u;
     if (npass > WF_PASS_MAX) return set_err(c, NE_ERR_UNSUPPORTED, "too many solver iterations for the large-island solve");
     r0 = ensure_wf(c, T, npass);
     if (r0 != NE_OK) return r0;
+    k_fill_u32<<<gs_blocks(c->n), TPB, 0, s>>>(W.inc_cnt, 0u, nullptr, c->n);
     k_fill_u32<<<gs_blocks(2 * (size_t)T), TPB, 0, s>>>(W.next, WF_NONE, nullptr, 2 * T);
     k_fill_u32<<<gs_blocks(T), TPB, 0, s>>>(W.pend0, 0u, nullptr, T);
     uint32_t cleared = std::min<uint32_t>(WF_MAX_ROUNDS + 2, 2 * c->wf_rounds_hint + 256);
@@ -4608,6 +4631,7 @@ namespace {
 
 int step_call(ne_gpu_ctx* c, int32_t nsteps) {
     HIP_TRY(c, hipMemsetAsync(c->counters, 0, sizeof(uint32_t) * CNT_N, c->stream));
+    c->wf_seen = false;
     for (int32_t t = 0; t < nsteps; ++t) {
         int r = run_step(c, t > 0);
         if (r != NE_OK) return r;
@@ -4624,6 +4648,16 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps) {
     c->hint_c = c->h_counters[CNT_CRAW];
     if (c->nmeshes) c->hint_mq = c->h_counters[CNT_MQRAW];
     const uint32_t fault = c->h_counters[CNT_FAULT];
+    if (!fault && c->wf_expect && !c->wf_seen && c->wf_quiet >= 8) c->wf_expect = false;   // for the next call
+    if (fault == NE_FAULT_WF_PLAN) {   // large islands after all: rerun the call, reading their tasks back
+        c->wf_expect = true;
+        c->wf_quiet = 0;
+        return set_err(c, NE_ERR_CAPACITY, "large islands appeared; the call reruns");
+    }
+    if (fault & NE_FAULT_WF_PLAN) {
+        c->wf_expect = true;
+        c->wf_quiet = 0;
+    }
     if (fault) {
         // grow what overflowed so a re-upload + retry succeeds; state is undefined
         if (fault & NE_FAULT_PAIRS) {

@@ -35,6 +35,7 @@ enum : uint32_t {
     NE_FAULT_EPA_BIG = 32u,   // an EPA rerun outgrew the large workspace
     NE_FAULT_WAKE = 64u,      // a chain of sleeping bodies woken one by another outran the wake rounds
     NE_FAULT_MESH = 128u,     // mesh triangle candidate buffer overflow
+    NE_FAULT_WF_PLAN = 256u,  // a step had large islands where none were expected (rerun, reading them back)
 };
 
 // Geometry of one body as the narrow phase sees it.
@@ -157,13 +158,13 @@ __device__ inline bool gjk_tetra(const Geom& A, const Geom& B, V3 shift, SupV te
 constexpr int EPA_MAXV = 68;    // 4 + 64 iterations: every vertex an EPA run can make
 constexpr int EPA_MAXF = 160;
 constexpr int EPA_MAXH = 96;
-// The per-thread workspace of the first tier.  Nearly every run ends within
-// a few iterations on a few dozen faces; the rare run that needs more stops
+// The per-thread workspace of the first tier.  A run that needs more stops
 // with `overflow` set and is rerun from the start by the wave-cooperative
-// second tier.  A small workspace keeps more waves resident.
-constexpr int EPA1_MAXV = 36;
-constexpr int EPA1_MAXF = 64;
-constexpr int EPA1_MAXH = 48;
+// second tier.  (A third of this size sent so many sphere-box runs to the
+// second tier that its workspace, slots x faces, outgrew the device.)
+constexpr int EPA1_MAXV = EPA_MAXV;
+constexpr int EPA1_MAXF = EPA_MAXF;
+constexpr int EPA1_MAXH = EPA_MAXH;
 
 struct Face {
     V3 n;
