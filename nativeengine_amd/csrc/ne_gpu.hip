@@ -2359,12 +2359,12 @@ __global__ void k_wf_expect_none(uint32_t* counters) {
 
 // Round r: run its visits, then queue each successor whose last predecessor
 // this was for round r + 1.
-__global__ void __launch_bounds__(WF_THREADS) k_wf_round(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters,
-                                                        ConsDev C, WfDev W, uint32_t T, uint32_t npass, uint32_t r,
-                                                        uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
-    const uint32_t base = W.roff[r], cnt = W.rcnt[r];
+// The visits of one round [base, base + cnt) of the visit list, over the
+// launch's threads; ready successors go to round r + 1.
+__device__ __forceinline__ void wf_run_round(BodiesDev& B, const SimParams& P, const ConsDev& C, const WfDev& W,
+                                             uint32_t T, uint32_t npass, uint32_t r, uint32_t base, uint32_t cnt,
+                                             uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
     const uint32_t nbase = base + cnt;
-    if (blockIdx.x == 0 && threadIdx.x == 0) W.roff[r + 1] = nbase;
     for (uint32_t u0 = blockIdx.x * blockDim.x; u0 < cnt; u0 += gridDim.x * blockDim.x) {
         const uint32_t u = u0 + threadIdx.x;
         uint32_t tv[2] = {WF_NONE, WF_NONE};
@@ -2397,6 +2397,31 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(BodiesDev B, SimParams 
         if (ready[0]) W.fr[nbase + slot0 + (uint32_t)__popcll(b0 & below)] = tv[0];
         if (ready[1]) W.fr[nbase + slot0 + (uint32_t)__popcll(b0) + (uint32_t)__popcll(b1 & below)] = tv[1];
     }
+}
+
+}  // namespace
+
+// The round kernel's arguments, copied to the device once per step: each of
+// the hundreds of launches per step then passes one pointer, not ~1 KB.
+struct WfArgs {
+    BodiesDev B;
+    SimParams P;
+    ConsDev C;
+    WfDev W;
+    uint32_t T, npass;
+    uint64_t* prev_key;
+    double* prev_imp;
+};
+
+namespace {
+
+__global__ void __launch_bounds__(WF_THREADS) k_wf_round(const WfArgs* __restrict__ a, uint32_t r) {
+    BodiesDev B = a->B;
+    const WfDev W = a->W;
+    const uint32_t base = W.roff[r], cnt = W.rcnt[r];
+    if (blockIdx.x == 0 && threadIdx.x == 0) W.roff[r + 1] = base + cnt;
+    if (cnt == 0) return;
+    wf_run_round(B, a->P, a->C, W, a->T, a->npass, r, base, cnt, a->prev_key, a->prev_imp);
 }
 
 // ---------------------------------------------------------------------------
@@ -3257,6 +3282,8 @@ struct ne_gpu_ctx {
     bool wf_expect = true;                            // expect large islands: read their task count back
     uint32_t wf_quiet = 0;                            // consecutive steps without large islands (checked ones)
     bool wf_seen = false;                             // this call had a step with large islands
+    struct WfArgs* wf_args = nullptr;                 // device copy of the round kernel's arguments
+    struct WfArgs* h_wf_args = nullptr;               // its pinned host staging copy
     uint32_t* h_wf = nullptr;                         // pinned: values read back mid-step
     bool capturing = false;                           // a step is being captured into a graph
     hipStream_t side = nullptr;                       // second stream (default priority) for work beside the main one
@@ -3588,6 +3615,8 @@ void free_adjoint(ne_gpu_ctx* c) {
 }
 
 void free_wf(ne_gpu_ctx* c) {
+    dfree(c->wf_args);
+    if (c->h_wf_args) { (void)hipHostFree(c->h_wf_args); c->h_wf_args = nullptr; }
     WfDev& W = c->wf;
     dfree(W.flag); dfree(W.tid); dfree(W.task_k); dfree(W.next); dfree(W.pend0); dfree(W.pend); dfree(W.fr);
     dfree(W.rcnt); dfree(W.roff); dfree(W.inc_cnt); dfree(W.inc_start); dfree(W.inc_list);
@@ -3890,6 +3919,12 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     k_wf_init<<<gs_blocks(T), TPB, 0, s>>>(B, T, npass, c->C, W);
     k_wf_pack_bodies<<<blocks_for(c->n), TPB, 0, s>>>(B, W.bp);
     const uint64_t total = (uint64_t)T * npass;
+    if (!c->wf_args) HIP_TRY(c, hipMalloc((void**)&c->wf_args, sizeof(WfArgs)));
+    if (!c->h_wf_args) HIP_TRY(c, hipHostMalloc((void**)&c->h_wf_args, sizeof(WfArgs), hipHostMallocDefault));
+    // (the last step's copy from the staging buffer finished before this
+    // step's read-back of T above)
+    *c->h_wf_args = WfArgs{B, P, c->C, W, T, npass, c->prev_key, c->prev_imp};
+    HIP_TRY(c, hipMemcpyAsync(c->wf_args, c->h_wf_args, sizeof(WfArgs), hipMemcpyHostToDevice, s));
     const uint32_t hint = c->wf_rounds_hint ? c->wf_rounds_hint : 64u;
     uint64_t per_round = total / hint + 1;
     uint32_t grid = (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(8, (per_round * 3 / 2 + WF_THREADS - 1) / WF_THREADS));
@@ -3901,8 +3936,7 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
                                                                           r + batch + 2 - cleared);
             cleared = r + batch + 2;
         }
-        for (uint32_t q = 0; q < batch; ++q, ++r)
-            k_wf_round<<<grid, WF_THREADS, 0, s>>>(B, P, c->counters, c->C, W, T, npass, r, c->prev_key, c->prev_imp);
+        for (uint32_t q = 0; q < batch; ++q, ++r) k_wf_round<<<grid, WF_THREADS, 0, s>>>(c->wf_args, r);
         HIP_TRY(c, hipMemcpyAsync(c->h_wf, W.roff + r, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(c, hipMemcpyAsync(c->h_wf + 1, W.rcnt + r, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(c, hipStreamSynchronize(s));
