@@ -1137,8 +1137,9 @@ __global__ void __launch_bounds__(TPB) k_epa(BodiesDev B, SimParams P, uint32_t*
 // which runs the horizon table (LDS queues, global entries), and finally
 // build the new faces from the live horizon entries in parallel.  All lanes
 // hold identical copies of the uniform values (best face, support point).
+template <class Hz>
 __device__ Contact epa_wave(const Geom& A, const Geom& B, V3 shift, const SupV tet[4], SupV* verts, Face* faces,
-                            int maxf, HorizonTable& hz, int* stage, int* shared, bool truncate, bool& overflow) {
+                            int maxf, Hz& hz, int* stage, int* shared, bool truncate, bool& overflow) {
     const int lane = threadIdx.x;
     if (lane == 0)
         for (int k = 0; k < 4; ++k) verts[k] = tet[k];
@@ -1298,6 +1299,69 @@ __global__ void __launch_bounds__(64) k_narrow_big(BodiesDev B, SimParams P, uin
     }
 }
 
+
+// The EPA runs that went past the second per-thread budget, one wave per
+// pair with the whole polytope (vertices, faces, horizon list) in LDS: a long
+// run no longer holds 63 other lanes, and its faces stay out of scratch.
+// A run that outgrows the LDS workspace goes to the global-memory queue.
+__global__ void __launch_bounds__(64) k_epa_wave(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
+                                                 const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                 uint32_t* __restrict__ ncont, double* __restrict__ ct,
+                                                 const uint32_t* __restrict__ queue, int count_slot, uint32_t cap,
+                                                 EpaBig big) {
+    __shared__ SupV verts[EPA_MAXV];
+    __shared__ Face faces[EPA_MAXF];
+    __shared__ int hor[EPA_MAXH][2];
+    __shared__ int stage[3 * 64];
+    __shared__ int shared[4];
+    const uint32_t nq = min(counters[count_slot], cap);
+    uint32_t* fault = counters + CNT_FAULT;
+    for (uint32_t t = blockIdx.x; t < nq; t += gridDim.x) {
+        const uint32_t p = queue[t];
+        HorizonList hz;
+        hz.hor = hor; hz.cap = EPA_MAXH; hz.n = 0;
+        const uint32_t i = cand_i[p], j = cand_j[p];
+        const Box3& box = box_of(B, P, i);
+        const Geom a = load_geom(B, i), b = load_geom(B, j);
+        const V3 shift = min_image_shift(box, a.x, b.x);   // convexContact (src/epa.hpp:153-158)
+        SupV tet[4];
+        bool overflow = false;
+        Contact c = no_contact();
+        if (gjk_tetra(a, b, shift, tet))
+            c = epa_wave(a, b, shift, tet, verts, faces, EPA_MAXF, hz, stage, shared, false, overflow);
+        uint32_t nc = 0;
+        double* out = ct + (size_t)p * 28;
+        if (overflow) {   // rerun from the start with the large workspace
+            if (threadIdx.x == 0) {
+                const uint32_t slot = atomicAdd(&counters[CNT_EPABIG], 1u);
+                if (slot < big.cap) big.list[slot] = p;
+                else atomicOr(fault, NE_FAULT_EPA_QUEUE);
+            }
+        } else if (c.hit) {
+            if (a.shape == SH_BOX && b.shape == SH_BOX) {
+                V3 pts[4];
+                double dep[4];
+                const int m = box_box_manifold(box, a, b, c.normal, pts, dep, fault);
+                if (threadIdx.x == 0)
+                    for (int k = 0; k < m; ++k) {
+                        double* o = out + 7 * k;
+                        o[0] = c.normal.x; o[1] = c.normal.y; o[2] = c.normal.z;
+                        o[3] = pts[k].x; o[4] = pts[k].y; o[5] = pts[k].z; o[6] = dep[k];
+                    }
+                nc = (uint32_t)m;
+            }
+            if (nc == 0) {
+                if (threadIdx.x == 0) {
+                    out[0] = c.normal.x; out[1] = c.normal.y; out[2] = c.normal.z;
+                    out[3] = c.point.x; out[4] = c.point.y; out[5] = c.point.z; out[6] = c.overlap;
+                }
+                nc = 1;
+            }
+        }
+        if (threadIdx.x == 0) ncont[p] = nc;
+        __syncthreads();
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Static triangle meshes: src/mesh.hpp:14-93 (tree, query) and
@@ -3770,8 +3834,8 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
             k_epa<<<hint_blocks(c->hint_c / 8), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
                                                              c->epa_long, CNT_EPALONG, c->pcap, c->epa_big,
                                                              EPA_SECOND_ITERS, c->epa_long2);
-            k_epa<<<hint_blocks(c->hint_c / 64), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                              c->epa_long2, CNT_EPALONG2, c->pcap, c->epa_big, 64, nullptr);
+            k_epa_wave<<<2048, 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_long2,
+                                           CNT_EPALONG2, c->pcap, c->epa_big);
             k_snap_counter<<<1, 64, 0, s>>>(c->counters, CNT_EPABIG, CNT_EPABIG_PAIRS);
             // The large-workspace reruns are few pairs on few waves: in eager
             // steps they run on the second stream beside the mesh kernels.
