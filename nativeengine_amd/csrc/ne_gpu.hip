@@ -1621,6 +1621,55 @@ __global__ void __launch_bounds__(64) k_mesh_big(BodiesDev B, SimParams P, MeshD
     }
 }
 
+// The triangle candidates whose EPA went past the first launch's budget:
+// one wave per candidate, polytope and horizon list in LDS (as k_epa_wave).
+__global__ void __launch_bounds__(64) k_mesh_wave(BodiesDev B, SimParams P, MeshDev M, uint32_t* __restrict__ counters,
+                                                  const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                  const uint32_t* __restrict__ mq_pair, const int32_t* __restrict__ mq_tri,
+                                                  uint8_t* __restrict__ mhit, double* __restrict__ mct,
+                                                  const uint32_t* __restrict__ list, int count_slot, EpaBig big) {
+    __shared__ SupV verts[EPA_MAXV];
+    __shared__ Face faces[EPA_MAXF];
+    __shared__ int hor[EPA_MAXH][2];
+    __shared__ int stage[3 * 64];
+    __shared__ int shared[4];
+    const uint32_t nq = counters[count_slot];
+    uint32_t* fault = counters + CNT_FAULT;
+    double pv[18];
+    for (uint32_t t = blockIdx.x; t < nq; t += gridDim.x) {
+        const uint32_t q = list[t];
+        const uint32_t p = mq_pair[q];
+        uint32_t o, m;
+        bool flip;
+        mesh_pair(B, cand_i[p], cand_j[p], o, m, flip);
+        Geom tri;
+        mesh_prism(B, M, m, mq_tri[q], pv, tri);   // non-degenerate: its EPA ran
+        const Geom a = load_geom(B, o);
+        const Box3& box = box_of(B, P, o);
+        const V3 shift = min_image_shift(box, a.x, tri.x);
+        HorizonList hz;
+        hz.hor = hor; hz.cap = EPA_MAXH; hz.n = 0;
+        SupV tet[4];
+        bool overflow = false;
+        Contact c = no_contact();
+        if (gjk_tetra(a, tri, shift, tet))
+            c = epa_wave(a, tri, shift, tet, verts, faces, EPA_MAXF, hz, stage, shared, false, overflow);
+        if (threadIdx.x == 0) {
+            uint8_t hit = 0;
+            if (overflow) {   // rerun from the start with the large workspace
+                const uint32_t slot = atomicAdd(&counters[CNT_EPABIG], 1u);
+                if (slot < big.cap) big.list[slot] = MESH_ENTRY | q;
+                else atomicOr(fault, NE_FAULT_EPA_QUEUE);
+            } else if (c.hit) {
+                store_contact(mct + 7 * (size_t)q, c, flip);
+                hit = 1;
+            }
+            mhit[q] = hit;
+        }
+        __syncthreads();
+    }
+}
+
 // Per mesh pair: its hits, in query order, become the pair's contacts.
 __global__ void __launch_bounds__(TPB) k_mesh_gather(BodiesDev B, const uint32_t* __restrict__ counters,
                                                      const uint32_t* __restrict__ cand_i,
@@ -3860,9 +3909,8 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
             k_mesh_narrow<<<hint_blocks(c->hint_mq), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
                                                                   c->mq_tri, c->mhit, c->mct, c->epa_big, nullptr, CNT_MQ,
                                                                   EPA_FIRST_ITERS, c->mq_long);
-            k_mesh_narrow<<<hint_blocks(c->hint_mq / 8), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j,
-                                                                      c->mq_pair, c->mq_tri, c->mhit, c->mct, c->epa_big,
-                                                                      c->mq_long, CNT_MQLONG, 64, nullptr);
+            k_mesh_wave<<<2048, 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri,
+                                            c->mhit, c->mct, c->mq_long, CNT_MQLONG, c->epa_big);
             k_mesh_big<<<big_blocks(c), 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri, c->mhit,
                                          c->mct, c->epa_big);
             k_mesh_gather<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, c->counters, c->cand_i, c->cand_j, c->mq_off,
