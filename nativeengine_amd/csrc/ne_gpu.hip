@@ -1155,6 +1155,7 @@ __device__ Contact epa_wave(const Geom& A, const Geom& B, V3 shift, const SupV t
         // first face of minimal distance
         double bd = 0.0;
         int bi = -1;
+#pragma unroll 8
         for (int i = lane; i < nf; i += 64) {
             const double d = faces[i].dist;
             if (bi < 0 || d < bd) { bd = d; bi = i; }
@@ -1177,15 +1178,18 @@ __device__ Contact epa_wave(const Geom& A, const Geom& B, V3 shift, const SupV t
         __syncthreads();
         // visibility rounds: compact kept faces, hand visible ones to lane 0
         int nk = 0;
+        // the next chunk is loaded while this one is compacted: compaction
+        // writes only below this chunk's end, so it never overwrites it
+        bool in_next = lane < nf;
+        Face f_next{};
+        if (in_next) f_next = faces[lane];
         for (int base = 0; base < nf; base += 64) {
-            const int i = base + lane;
-            const bool in = i < nf;
-            Face f{};
+            const bool in = in_next;
+            const Face f = f_next;
+            in_next = base + 64 + lane < nf;
+            if (in_next) f_next = faces[base + 64 + lane];
             bool vis = false;
-            if (in) {
-                f = faces[i];
-                vis = dot(f.n, p.v - verts[f.a].v) > 1e-12;
-            }
+            if (in) vis = dot(f.n, p.v - verts[f.a].v) > 1e-12;
             const uint64_t km = __ballot(in && !vis), vm = __ballot(vis);
             const uint64_t below = lanemask_lt();
             if (in && !vis) faces[nk + (int)__popcll(km & below)] = f;
