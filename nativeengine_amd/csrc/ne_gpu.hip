@@ -1003,7 +1003,7 @@ __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimPar
         const Geom a = load_geom(B, i), b = load_geom(B, j);
         Contact c = detect_contact(box, a, b, ws);
         if (c.hit == 2) return NARROW_DEFERRED;   // GJK overlap: EPA runs in k_epa
-        if (ws.overflow && !ws.truncate) return 0;
+        if (ws.overflow) return 0;
         if (c.hit) {
             if (si == SH_BOX && sj == SH_BOX) {
                 V3 pts[4];
@@ -1026,29 +1026,27 @@ __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimPar
     return nc;
 }
 
-// Pairs whose EPA polytope outgrows the per-thread workspace (rare
-// near-degenerate contacts, where the reference's vectors grow to thousands
-// of faces) are queued for k_narrow_big, which reruns them with a workspace
-// in global memory and the table-driven horizon.  Both capacities grow when
-// exceeded (the step is then run again).
+// Pairs whose EPA polytope outgrows the per-thread and per-wave workspaces
+// (near-degenerate contacts, where the reference's vectors grow to tens of
+// thousands of faces) are queued for k_narrow_big, which reruns each with one
+// block, a workspace in global memory and the horizon built in parallel.
+// Both capacities grow when exceeded (the step is then run again).
 constexpr uint32_t EPA_BIG_PAIRS = 64;    // initial queue capacity
 constexpr int EPA_FIRST_ITERS = 8;        // EPA iterations of the first tier-1 launch
 constexpr int EPA_SECOND_ITERS = 24;      // of the second (the third runs the rest)
-constexpr int EPA_BIG_FACES = 4096;       // initial faces (and horizon entries) per queued pair
+constexpr int EPA_BIG_FACES = 4096;       // initial faces per queued pair
 constexpr int EPA_BIG_FACES_MAX = 1 << 17;   // beyond this a run stops at its closest face
-constexpr int EPA_TAB = EPA_MAXV * EPA_MAXV;
+constexpr int EPA_KEYS = EPA_MAXV * EPA_MAXV;   // directed edges u * EPA_MAXV + w of one polytope
+constexpr int BIG_NW = 8;                       // waves per large-workspace run
 
 constexpr uint32_t MESH_ENTRY = 0x80000000u;   // queue entry is a mesh triangle candidate, not a pair
 
 struct EpaBig {
     uint32_t* list;   // queued pair indices (or MESH_ENTRY | mesh candidate index)
     uint32_t cap;     // queue slots
-    int faces_per;    // faces == horizon entries per slot
+    int faces_per;    // faces per slot
     Face* faces;
-    int (*hor)[2];
-    int* next;
-    uint8_t* live;
-    int* tab;         // 3 * EPA_TAB per slot: head, tail, stamp
+    uint16_t* occ;    // 3 per face: directed edges of the visible faces, in order
 };
 
 // Narrow phase, sorted by pair type: k_narrow computes the analytic pairs
@@ -1100,7 +1098,7 @@ __global__ void __launch_bounds__(TPB) k_epa(BodiesDev B, SimParams P, uint32_t*
     Face faces[EPA1_MAXF];
     int hor[EPA1_MAXH][2];
     EpaWS ws{};
-    ws.verts = verts; ws.faces = faces; ws.maxf = EPA1_MAXF; ws.maxv = EPA1_MAXV; ws.big = false;
+    ws.verts = verts; ws.faces = faces; ws.maxf = EPA1_MAXF; ws.maxv = EPA1_MAXV;
     ws.list.hor = hor; ws.list.cap = EPA1_MAXH;
     ws.max_iter = max_iter;
     for (uint32_t t0 = blockIdx.x * blockDim.x; t0 < nq; t0 += gridDim.x * blockDim.x) {
@@ -1129,14 +1127,14 @@ __global__ void __launch_bounds__(TPB) k_epa(BodiesDev B, SimParams P, uint32_t*
     }
 }
 
-// EPA of one queued pair by one wave (src/epa.hpp:71-150, the same steps as
-// epa_contact).  The face list lives in global memory; each iteration the 64
+// EPA of one pair by one wave (src/epa.hpp:71-150, the same steps as
+// epa_contact), polytope and horizon list in LDS.  Each iteration the 64
 // lanes find the first closest face (a min/first-index reduction), test the
 // faces for visibility and compact the kept ones in rounds of 64 (order is
-// kept by ballot ranks), stage each round's visible faces in LDS for lane 0,
-// which runs the horizon table (LDS queues, global entries), and finally
-// build the new faces from the live horizon entries in parallel.  All lanes
-// hold identical copies of the uniform values (best face, support point).
+// kept by ballot ranks), stage each round's visible faces for lane 0, which
+// runs the horizon list, and finally build the new faces from the live
+// horizon entries in parallel.  All lanes hold identical copies of the
+// uniform values (best face, support point).
 template <class Hz>
 __device__ Contact epa_wave(const Geom& A, const Geom& B, V3 shift, const SupV tet[4], SupV* verts, Face* faces,
                             int maxf, Hz& hz, int* stage, int* shared, bool truncate, bool& overflow) {
@@ -1241,13 +1239,197 @@ __global__ void k_snap_counter(uint32_t* counters, int from, int to) {
     if (blockIdx.x == 0 && threadIdx.x == 0) counters[to] = counters[from];
 }
 
-__global__ void __launch_bounds__(64) k_narrow_big(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
-                                                   const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
-                                                   uint32_t* __restrict__ ncont, double* __restrict__ ct, EpaBig big) {
-    __shared__ SupV verts[EPA_MAXV];
-    __shared__ int head[EPA_TAB], tail[EPA_TAB], stamp[EPA_TAB];
-    __shared__ int stage[3 * 64];
-    __shared__ int shared[4];
+// ---- Large-workspace EPA: one block of BIG_NW waves per run --------------
+//
+// The queued runs are the near-degenerate ones whose polytope grows to tens
+// of thousands of faces and whose horizon holds thousands of edges; one wave
+// scanning them, with one lane building the horizon, is latency bound.  Here
+// a block shares the scans, and the horizon list is built in parallel with
+// the same result as the reference's find + erase on a vector
+// (src/epa.hpp:95-112):
+//
+// Edges of different undirected pairs {u, w} never interact, and the list's
+// entries of one pair always point the same way (an arriving reverse edge
+// removes one).  An arriving reverse edge removes the oldest entry, so after
+// all visible faces are processed, with P occurrences of (u, w) and N of
+// (w, u), the survivors are the last P - N occurrences of (u, w) when
+// P > N (none when P <= N), and the list holds the survivors of all pairs in
+// the order they were added.  Each iteration therefore records the three
+// edges of each visible face in order (occurrence s = 3 * visible rank + edge)
+// and counts each directed edge; an occurrence is kept when its edge has
+// P > N and, if N > 0, its rank among the edge's occurrences is >= N.  The
+// ranks come from per-edge counters advanced in order (rounds of the block in
+// turn, waves of a round in turn, lanes of a wave by their position).  A
+// block-wide ordered compaction of the kept occurrences gives the new faces
+// in list order.
+struct EpaBlockLds {
+    SupV verts[EPA_MAXV];
+    int cnt[EPA_KEYS];   // occurrences of each directed edge this iteration
+    int cur[EPA_KEYS];   // occurrences of each edge ranked so far (edges seen both ways)
+    int wsum[2 * BIG_NW];
+    double wd[BIG_NW];
+    int wi[BIG_NW];
+};
+
+// Exclusive block-wide ranks of two predicates in thread order, and totals.
+__device__ __forceinline__ void block_rank2(bool p0, bool p1, int* wsum, int& r0, int& r1, int& t0, int& t1) {
+    const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { wsum[w] = (int)__popcll(m0); wsum[BIG_NW + w] = (int)__popcll(m1); }
+    __syncthreads();
+    int b0 = 0, b1 = 0;
+    t0 = 0; t1 = 0;
+    for (int k = 0; k < BIG_NW; ++k) {
+        const int c0 = wsum[k], c1 = wsum[BIG_NW + k];
+        if (k < w) { b0 += c0; b1 += c1; }
+        t0 += c0; t1 += c1;
+    }
+    __syncthreads();
+    const uint64_t below = lanemask_lt();
+    r0 = b0 + (int)__popcll(m0 & below);
+    r1 = b1 + (int)__popcll(m1 & below);
+}
+
+__device__ Contact epa_block(const Geom& A, const Geom& B, V3 shift, const SupV tet[4], EpaBlockLds& L, Face* faces,
+                             uint16_t* occ, int maxf, bool& overflow) {
+    constexpr int T = 64 * BIG_NW;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    SupV* verts = L.verts;
+    if (tid == 0)
+        for (int k = 0; k < 4; ++k) verts[k] = tet[k];
+    __syncthreads();
+    int nv = 4, nf = 4;
+    if (tid < 4) {
+        const int fi[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 3, 2}};
+        faces[tid] = make_face(verts, fi[tid][0], fi[tid][1], fi[tid][2]);
+    }
+    __syncthreads();
+    Face best = faces[0];
+    for (int iter = 0; iter < 64; ++iter) {
+        // first face of minimal distance
+        double bd = 0.0;
+        int bi = -1;
+#pragma unroll 4
+        for (int i = tid; i < nf; i += T) {
+            const double d = faces[i].dist;
+            if (bi < 0 || d < bd) { bd = d; bi = i; }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double od = __shfl_xor(bd, off);
+            const int oi = __shfl_xor(bi, off);
+            const bool take = oi >= 0 && (bi < 0 || od < bd || (od == bd && oi < bi));
+            bd = take ? od : bd;
+            bi = take ? oi : bi;
+        }
+        if ((tid & 63) == 0) { L.wd[w] = bd; L.wi[w] = bi; }
+        __syncthreads();
+        bd = L.wd[0];
+        bi = L.wi[0];
+        for (int k = 1; k < BIG_NW; ++k) {
+            const double od = L.wd[k];
+            const int oi = L.wi[k];
+            const bool take = oi >= 0 && (bi < 0 || od < bd || (od == bd && oi < bi));
+            bd = take ? od : bd;
+            bi = take ? oi : bi;
+        }
+        best = faces[bi];
+        const SupV p = cso_support(A, B, shift, best.n);
+        const double d = dot(p.v, best.n);
+        if (d - best.dist < 1e-7) break;
+        const int pidx = nv;
+        if (tid == 0) verts[nv] = p;
+        ++nv;
+        for (int k = tid; k < EPA_KEYS; k += T) { L.cnt[k] = 0; L.cur[k] = 0; }
+        __syncthreads();
+        // visibility rounds: compact the kept faces in order, record the
+        // visible faces' edges in order.  A kept face moves to an index at or
+        // below its own, never into the chunk loaded ahead.
+        int nk = 0, nvis = 0;
+        bool in_next = tid < nf;
+        Face f_next{};
+        if (in_next) f_next = faces[tid];
+        for (int base = 0; base < nf; base += T) {
+            const bool in = in_next;
+            const Face f = f_next;
+            in_next = base + T + tid < nf;
+            if (in_next) f_next = faces[base + T + tid];
+            const bool vis = in && dot(f.n, p.v - verts[f.a].v) > 1e-12;
+            int rk, rv, tk, tv;
+            block_rank2(in && !vis, vis, L.wsum, rk, rv, tk, tv);
+            if (in && !vis) faces[nk + rk] = f;
+            if (vis) {
+                const int e0 = f.a * EPA_MAXV + f.b, e1 = f.b * EPA_MAXV + f.c, e2 = f.c * EPA_MAXV + f.a;
+                uint16_t* o = occ + 3 * (size_t)(nvis + rv);
+                o[0] = (uint16_t)e0; o[1] = (uint16_t)e1; o[2] = (uint16_t)e2;
+                atomicAdd(&L.cnt[e0], 1);
+                atomicAdd(&L.cnt[e1], 1);
+                atomicAdd(&L.cnt[e2], 1);
+            }
+            nk += tk;
+            nvis += tv;
+        }
+        __syncthreads();
+        // the surviving occurrences, in order, become the new faces
+        const int nocc = 3 * nvis;
+        int wpos = nk;
+        for (int base = 0; base < nocc; base += T) {
+            const int q = base + tid;
+            int e = -1, u = 0, v = 0, np = 0, nn = 0;
+            if (q < nocc) {
+                e = occ[q];
+                u = e / EPA_MAXV;
+                v = e - u * EPA_MAXV;
+                np = L.cnt[e];
+                nn = L.cnt[v * EPA_MAXV + u];
+            }
+            bool keep = np > nn;
+            const bool mixed = keep && nn > 0;
+            if (__syncthreads_or(mixed)) {
+                // rank among the edge's occurrences: earlier rounds (the
+                // edge's counter), earlier waves of this round (the waves
+                // take turns), earlier lanes of this wave
+                const int key = mixed ? e : -1;
+                int before = 0;
+                bool last = true;
+                for (int l = 0; l < 64; ++l) {
+                    const int o = __shfl(key, l);
+                    if (o == key) {
+                        before += l < lane;
+                        last = last && l <= lane;
+                    }
+                }
+                for (int ph = 0; ph < BIG_NW; ++ph) {
+                    if (w == ph && mixed) {
+                        const int rank = L.cur[e] + before;
+                        keep = rank >= nn;
+                        if (last) L.cur[e] = rank + 1;
+                    }
+                    __syncthreads();
+                }
+            }
+            int rs, r1, ts, t1;
+            block_rank2(keep, false, L.wsum, rs, r1, ts, t1);
+            if (keep && wpos + rs < maxf) faces[wpos + rs] = make_face(verts, u, v, pidx);
+            wpos += ts;
+        }
+        if (wpos > maxf) {
+            overflow = true;
+            break;
+        }
+        nf = wpos;
+        __syncthreads();
+        if (nf == 0) break;
+    }
+    return epa_result(verts, best);
+}
+
+// Large-workspace EPA reruns of the queued pairs, one block per pair.
+__global__ void __launch_bounds__(64 * BIG_NW) k_narrow_big(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
+                                                            const uint32_t* __restrict__ cand_i,
+                                                            const uint32_t* __restrict__ cand_j,
+                                                            uint32_t* __restrict__ ncont, double* __restrict__ ct,
+                                                            EpaBig big) {
+    __shared__ EpaBlockLds L;
     const uint32_t nbig = min(counters[CNT_EPABIG_PAIRS], big.cap);
     uint32_t* fault = counters + CNT_FAULT;
     if (blockIdx.x == 0 && threadIdx.x == 0 && nbig) atomicAdd(&counters[CNT_EPARERUN], nbig);
@@ -1255,12 +1437,6 @@ __global__ void __launch_bounds__(64) k_narrow_big(BodiesDev B, SimParams P, uin
         const uint32_t p = big.list[t];
         if (p & MESH_ENTRY) continue;   // a mesh triangle: k_mesh_big
         const size_t fo = (size_t)t * (size_t)big.faces_per;
-        for (int k = threadIdx.x; k < EPA_TAB; k += blockDim.x) stamp[k] = 0;
-        __syncthreads();
-        HorizonTable hz;
-        hz.hor = big.hor + fo; hz.next = big.next + fo; hz.live = big.live + fo;
-        hz.head = head; hz.tail = tail; hz.stamp = stamp;
-        hz.cap = big.faces_per; hz.n = 0; hz.nlive = 0; hz.it = 0;
         const bool truncate = big.faces_per >= EPA_BIG_FACES_MAX;
         const uint32_t i = cand_i[p], j = cand_j[p];
         const Box3& box = box_of(B, P, i);
@@ -1271,7 +1447,8 @@ __global__ void __launch_bounds__(64) k_narrow_big(BodiesDev B, SimParams P, uin
         bool overflow = false;
         Contact c = no_contact();
         if (gjk_tetra(a, b, shift, tet))
-            c = epa_wave(a, b, shift, tet, verts, big.faces + fo, big.faces_per, hz, stage, shared, truncate, overflow);
+            c = epa_block(a, b, shift, tet, L, big.faces + fo, big.occ + 3 * fo, big.faces_per,
+                          overflow);
         uint32_t nc = 0;
         double* out = ct + (size_t)p * 28;
         if (overflow && !truncate) {
@@ -1302,7 +1479,6 @@ __global__ void __launch_bounds__(64) k_narrow_big(BodiesDev B, SimParams P, uin
         __syncthreads();
     }
 }
-
 
 // The EPA runs that went past the second per-thread budget, one wave per
 // pair with the whole polytope (vertices, faces, horizon list) in LDS: a long
@@ -1528,7 +1704,7 @@ __global__ void __launch_bounds__(TPB) k_mesh_narrow(BodiesDev B, SimParams P, M
     Face faces[EPA1_MAXF];
     int hor[EPA1_MAXH][2];
     EpaWS ws{};
-    ws.verts = verts; ws.faces = faces; ws.maxf = EPA1_MAXF; ws.maxv = EPA1_MAXV; ws.big = false;
+    ws.verts = verts; ws.faces = faces; ws.maxf = EPA1_MAXF; ws.maxv = EPA1_MAXV;
     ws.list.hor = hor; ws.list.cap = EPA1_MAXH;
     ws.max_iter = max_iter;
     // the prism's six vertices in LDS (every support call reads them; in
@@ -1572,16 +1748,16 @@ __global__ void __launch_bounds__(TPB) k_mesh_narrow(BodiesDev B, SimParams P, M
     }
 }
 
-// The queued triangle candidates with the wave-cooperative EPA (as
-// k_narrow_big does for pairs).
-__global__ void __launch_bounds__(64) k_mesh_big(BodiesDev B, SimParams P, MeshDev M, uint32_t* __restrict__ counters,
-                                                 const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
-                                                 const uint32_t* __restrict__ mq_pair, const int32_t* __restrict__ mq_tri,
-                                                 uint8_t* __restrict__ mhit, double* __restrict__ mct, EpaBig big) {
-    __shared__ SupV verts[EPA_MAXV];
-    __shared__ int head[EPA_TAB], tail[EPA_TAB], stamp[EPA_TAB];
-    __shared__ int stage[3 * 64];
-    __shared__ int shared[4];
+// The queued triangle candidates, one block per candidate (as k_narrow_big
+// does for pairs).
+__global__ void __launch_bounds__(64 * BIG_NW) k_mesh_big(BodiesDev B, SimParams P, MeshDev M,
+                                                          uint32_t* __restrict__ counters,
+                                                          const uint32_t* __restrict__ cand_i,
+                                                          const uint32_t* __restrict__ cand_j,
+                                                          const uint32_t* __restrict__ mq_pair,
+                                                          const int32_t* __restrict__ mq_tri, uint8_t* __restrict__ mhit,
+                                                          double* __restrict__ mct, EpaBig big) {
+    __shared__ EpaBlockLds L;
     const uint32_t nbig = min(counters[CNT_EPABIG], big.cap);
     uint32_t* fault = counters + CNT_FAULT;
     double pv[18];
@@ -1591,12 +1767,6 @@ __global__ void __launch_bounds__(64) k_mesh_big(BodiesDev B, SimParams P, MeshD
         const uint32_t q = e & ~MESH_ENTRY;
         if (threadIdx.x == 0) atomicAdd(&counters[CNT_EPARERUN], 1u);
         const size_t fo = (size_t)t * (size_t)big.faces_per;
-        for (int k = threadIdx.x; k < EPA_TAB; k += blockDim.x) stamp[k] = 0;
-        __syncthreads();
-        HorizonTable hz;
-        hz.hor = big.hor + fo; hz.next = big.next + fo; hz.live = big.live + fo;
-        hz.head = head; hz.tail = tail; hz.stamp = stamp;
-        hz.cap = big.faces_per; hz.n = 0; hz.nlive = 0; hz.it = 0;
         const bool truncate = big.faces_per >= EPA_BIG_FACES_MAX;
         const uint32_t p = mq_pair[q];
         uint32_t o, m;
@@ -1611,7 +1781,8 @@ __global__ void __launch_bounds__(64) k_mesh_big(BodiesDev B, SimParams P, MeshD
         bool overflow = false;
         Contact c = no_contact();
         if (gjk_tetra(a, tri, shift, tet))
-            c = epa_wave(a, tri, shift, tet, verts, big.faces + fo, big.faces_per, hz, stage, shared, truncate, overflow);
+            c = epa_block(a, tri, shift, tet, L, big.faces + fo, big.occ + 3 * fo, big.faces_per,
+                          overflow);
         uint8_t hit = 0;
         if (overflow && !truncate) {
             if (threadIdx.x == 0) atomicOr(fault, NE_FAULT_EPA_BIG);
@@ -3657,10 +3828,7 @@ int alloc_epa_big(ne_gpu_ctx* c, uint32_t cap, int faces_per) {
     const size_t nf = (size_t)cap * (size_t)faces_per;
     HIP_TRY(c, dalloc(g.list, cap));
     HIP_TRY(c, dalloc(g.faces, nf));
-    HIP_TRY(c, dalloc(g.hor, nf));
-    HIP_TRY(c, dalloc(g.next, nf));
-    HIP_TRY(c, dalloc(g.live, nf));
-    HIP_TRY(c, dalloc(g.tab, (size_t)cap * 3 * EPA_TAB));
+    HIP_TRY(c, dalloc(g.occ, 3 * nf));
     return NE_OK;
 }
 
@@ -3708,8 +3876,7 @@ void free_work(ne_gpu_ctx* c) {
     dfree(c->d_wid); dfree(c->d_worlds);
     dfree(c->j_ab); dfree(c->j_comp); dfree(c->j_f);
     c->njoints = c->nspring = c->nrigid = c->ncomp = 0;
-    dfree(c->epa_big.list); dfree(c->epa_big.faces); dfree(c->epa_big.hor); dfree(c->epa_big.next);
-    dfree(c->epa_big.live); dfree(c->epa_big.tab);
+    dfree(c->epa_big.list); dfree(c->epa_big.faces); dfree(c->epa_big.occ);
     c->epa_big.cap = 0;
     c->ex_cap = 0;
     drop_graphs(c);
@@ -3895,7 +4062,7 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
             // (If a stream call fails, the host waits for the second stream instead.)
             fork = !c->capturing && c->nmeshes && ensure_side(c) == NE_OK &&
                    hipEventRecord(c->ev_fork, s) == hipSuccess && hipStreamWaitEvent(c->side, c->ev_fork, 0) == hipSuccess;
-            k_narrow_big<<<big_blocks(c), 64, 0, fork ? c->side : s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont,
+            k_narrow_big<<<big_blocks(c), 64 * BIG_NW, 0, fork ? c->side : s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont,
                                                                        c->ct, c->epa_big);
             if (fork && hipEventRecord(c->ev_join, c->side) != hipSuccess) {
                 (void)hipStreamSynchronize(c->side);
@@ -3915,7 +4082,7 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
                                                                   EPA_FIRST_ITERS, c->mq_long);
             k_mesh_wave<<<2048, 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri,
                                             c->mhit, c->mct, c->mq_long, CNT_MQLONG, c->epa_big);
-            k_mesh_big<<<big_blocks(c), 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri, c->mhit,
+            k_mesh_big<<<big_blocks(c), 64 * BIG_NW, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri, c->mhit,
                                          c->mct, c->epa_big);
             k_mesh_gather<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, c->counters, c->cand_i, c->cand_j, c->mq_off,
                                                                  c->mqcap, c->mhit, c->mct, c->ncont);

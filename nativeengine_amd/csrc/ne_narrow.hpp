@@ -178,7 +178,7 @@ struct Face {
 // The surviving edges, in insertion order, become the new faces.
 //
 // HorizonList keeps the list compact and searches it linearly (small
-// polytopes, per-thread arrays).
+// polytopes: per-thread arrays, or one wave with the polytope in LDS).
 struct HorizonList {
     int (*hor)[2];
     int cap;
@@ -199,47 +199,6 @@ struct HorizonList {
     __device__ int alive() const { return n; }
     __device__ int entries() const { return n; }
     __device__ bool get(int e, int& u, int& w) const { u = hor[e][0]; w = hor[e][1]; return true; }
-};
-
-// HorizonTable gives the same list for very large polytopes in time linear
-// in the edges: entries are never moved, a removed one is marked dead, and
-// the live entries of each directed edge (u, w) form a queue in insertion
-// order whose head is the first match.  Vertex ids are < EPA_MAXV, so the
-// queues are indexed directly by u * EPA_MAXV + w; `stamp` marks the queues
-// written in this iteration, so nothing is cleared between iterations.
-struct HorizonTable {
-    int (*hor)[2];
-    int* next;
-    uint8_t* live;
-    int* head;    // EPA_MAXV * EPA_MAXV
-    int* tail;
-    int* stamp;
-    int cap;
-    int n, nlive, it;
-    __device__ void begin() { n = 0; nlive = 0; ++it; }
-    __device__ bool add(int u, int w) {
-        const int rk = w * EPA_MAXV + u;
-        if (stamp[rk] == it && head[rk] >= 0) {
-            const int e = head[rk];
-            live[e] = 0;
-            head[rk] = next[e];
-            if (head[rk] < 0) tail[rk] = -1;
-            --nlive;
-            return true;
-        }
-        if (n >= cap) return false;
-        const int k = u * EPA_MAXV + w;
-        if (stamp[k] != it) { stamp[k] = it; head[k] = -1; tail[k] = -1; }
-        const int e = n++;
-        hor[e][0] = u; hor[e][1] = w; live[e] = 1; next[e] = -1;
-        if (tail[k] < 0) head[k] = e; else next[tail[k]] = e;
-        tail[k] = e;
-        ++nlive;
-        return true;
-    }
-    __device__ int alive() const { return nlive; }
-    __device__ int entries() const { return n; }
-    __device__ bool get(int e, int& u, int& w) const { u = hor[e][0]; w = hor[e][1]; return live[e] != 0; }
 };
 
 __device__ __forceinline__ Face make_face(const SupV* verts, int i, int j, int k) {
@@ -340,7 +299,7 @@ __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, co
 }
 
 // EPA workspace handed down the detect path: per-thread arrays with a linear
-// horizon, or (big) a global-memory workspace with the edge table.
+// horizon.
 struct EpaWS {
     SupV* verts;
     Face* faces;
@@ -348,11 +307,8 @@ struct EpaWS {
     int maxv;
     int max_iter;      // EPA iterations this launch runs; a run that needs more sets too_long
     bool too_long;
-    bool big;
     HorizonList list;
-    HorizonTable table;
     bool overflow;
-    bool truncate;   // on overflow, keep the closest face found so far (largest workspace only)
     bool defer;      // stop after GJK: an overlapping convex pair reports hit == 2 (EPA runs later)
 };
 
@@ -365,9 +321,6 @@ __device__ inline Contact convex_contact(const Box3& box, const Geom& A, const G
         c.hit = 2;
         return c;
     }
-    if (ws.big)
-        return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.maxv, ws.max_iter, ws.table, ws.overflow,
-                           ws.too_long);
     return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.maxv, ws.max_iter, ws.list, ws.overflow,
                        ws.too_long);
 }
