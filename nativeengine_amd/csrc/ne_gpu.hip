@@ -3095,42 +3095,132 @@ constexpr int AF_X = 0, AF_V = 3, AF_W = 10;
 
 __device__ __forceinline__ Qt qadd(Qt a, Qt b) { return Qt{a.w + b.w, a.x + b.x, a.y + b.y, a.z + b.z}; }
 
-// One thread per island: replays the island's solve (warm start, iterations)
+constexpr uint32_t ADJ_NONE = 0xFFFFFFFFu;
+constexpr uint32_t ADJ_SORT = 4096;   // islands up to this size are sorted in LDS
+
+__device__ __forceinline__ uint32_t adj_movable(const BodiesDev& B, uint32_t b) {
+    return (B.im[b] != 0.0 || B.ii0[b] != 0.0 || B.ii1[b] != 0.0 || B.ii2[b] != 0.0) ? b : ADJ_NONE;
+}
+
+// Runs visits 0..count-1 of a sequence on one wave (64 threads per block)
+// with the sequential result: bodies(v, bi, bj) names the movable bodies
+// visit v reads and writes (ADJ_NONE for none), run(v) performs it.  Visits
+// go in windows of 64; a lane runs once the latest earlier lane of its
+// window touching each of its bodies has run, so every body sees its visits
+// in sequence order and nothing else is shared between visits.
+template <class Bodies, class Run>
+__device__ __forceinline__ void wave_in_order(uint32_t count, Bodies bodies, Run run) {
+    const int lane = threadIdx.x & 63;
+    for (uint32_t base = 0; base < count; base += 64) {
+        const uint32_t v = base + (uint32_t)lane;
+        const bool act = v < count;
+        uint32_t bi = ADJ_NONE, bj = ADJ_NONE;
+        if (act) bodies(v, bi, bj);
+        int di = -1, dj = -1;
+        for (int l = 0; l < 63; ++l) {
+            const uint32_t oi = __shfl(bi, l), oj = __shfl(bj, l);
+            if (l < lane) {
+                if (bi != ADJ_NONE && (oi == bi || oj == bi)) di = l;
+                if (bj != ADJ_NONE && (oi == bj || oj == bj)) dj = l;
+            }
+        }
+        bool done = !act;
+        uint64_t dm = __ballot(done);
+        while (dm != ~0ull) {   // each pass runs at least the lowest lane not done
+            const bool ready = !done && (di < 0 || ((dm >> di) & 1ull)) && (dj < 0 || ((dm >> dj) & 1ull));
+            if (ready) {
+                run(v);
+                done = true;
+            }
+            __threadfence_block();   // this pass's stores are seen by the next
+            dm = __ballot(done);
+        }
+    }
+}
+
+// Sorts an island's constraint list ascending (the sequential sweep order),
+// one wave: a bitonic sort in LDS, or lane 0 for a very large island.
+__device__ void adj_sort_list(uint32_t* list, uint32_t m, uint32_t* sk) {
+    const int lane = threadIdx.x & 63;
+    if (m <= ADJ_SORT) {
+        uint32_t p2 = 1;
+        while (p2 < m) p2 <<= 1;
+        for (uint32_t t = lane; t < p2; t += 64) sk[t] = t < m ? list[t] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= p2; k <<= 1)
+            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+                for (uint32_t t = lane; t < p2; t += 64) {
+                    const uint32_t l = t ^ jj;
+                    if (l > t) {
+                        const uint32_t a = sk[t], b = sk[l];
+                        if ((a > b) == ((t & k) == 0)) { sk[t] = b; sk[l] = a; }
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t t = lane; t < m; t += 64) list[t] = sk[t];
+    } else if (lane == 0) {
+        sort_segment(list, m);
+    }
+    __threadfence_block();
+    __syncthreads();
+}
+
+// One wave per island: replays the island's solve (warm start, iterations)
 // recording each visit's inputs, then runs it backwards, then reverses the
-// constraint build.  Only bodies that can move carry adjoints (a body that
-// cannot move belongs to no island and its velocity never changes).
+// constraint build, each phase in sequence order through wave_in_order.
+// Only bodies that can move carry adjoints (a body that cannot move belongs
+// to no island and its velocity never changes).
 __global__ void __launch_bounds__(64) k_adj_island(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters,
                                                    const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
                                                    ConsDev C, uint32_t ccap, double* __restrict__ tape,
                                                    const double* __restrict__ pre, uint32_t ncap, AdjBodies A,
                                                    double* __restrict__ ca, const double* __restrict__ nfin,
                                                    double* __restrict__ nprev, double* __restrict__ cw) {
+    __shared__ uint32_t sk[ADJ_SORT];
     const uint32_t nisl = counters[CNT_ISL];
+    const int lane = threadIdx.x & 63;
     auto CA = [&](int f, uint32_t k) -> double& { return ca[(size_t)f * ccap + k]; };
-    for (uint32_t isl = blockIdx.x * blockDim.x + threadIdx.x; isl < nisl; isl += gridDim.x * blockDim.x) {
+    for (uint32_t isl = blockIdx.x; isl < nisl; isl += gridDim.x) {
         const uint32_t s = isl_start[isl], e = isl_start[isl + 1];
         const uint32_t m = e - s;
         uint32_t* list = isl_list + s;
-        sort_segment(list, m);
+        adj_sort_list(list, m, sk);
+        const uint32_t iters = (uint32_t)P.iters;
+        // the movable bodies of the visit to list[t]; a constraint between two
+        // bodies that cannot move still orders its own visits (its impulses)
+        auto cons_bodies = [&](uint32_t t, uint32_t& bi, uint32_t& bj) {
+            const uint32_t k = list[t];
+            bi = adj_movable(B, C.ci[k]);
+            bj = adj_movable(B, C.cj[k]);
+            if (bi == ADJ_NONE && bj == ADJ_NONE) bi = 0x80000000u | k;
+        };
         // --- forward replay with tape (solve_island_seq on global memory)
-        for (uint32_t t = 0; t < m; ++t) {
+        for (uint32_t t = lane; t < m; t += 64) {
             const uint32_t k = list[t];
             CA(CA_NINIT, k) = C.nimp[k];
             for (int f = 1; f < CA_COUNT; ++f) CA(f, k) = 0.0;
             CA(CA_NIMP, k) = nfin[k];
         }
+        __threadfence_block();
         if (P.warm)
-            for (uint32_t t = 0; t < m; ++t) {
-                const uint32_t k = list[t];
-                if (C.nimp[k] == 0.0) continue;
-                BodyView a = load_view(B, C.ci[k]), b = load_view(B, C.cj[k]);
-                warm_apply(a, b, load_geom(C, k), C.nimp[k]);
-                store_view(B, C.ci[k], a);
-                store_view(B, C.cj[k], b);
-            }
-        for (int it = 0; it < P.iters; ++it)
-            for (uint32_t t = 0; t < m; ++t) {
-                const uint32_t k = list[t];
+            wave_in_order(
+                m,
+                [&](uint32_t t, uint32_t& bi, uint32_t& bj) {
+                    if (C.nimp[list[t]] != 0.0) cons_bodies(t, bi, bj);
+                },
+                [&](uint32_t t) {
+                    const uint32_t k = list[t];
+                    if (C.nimp[k] == 0.0) return;
+                    BodyView a = load_view(B, C.ci[k]), b = load_view(B, C.cj[k]);
+                    warm_apply(a, b, load_geom(C, k), C.nimp[k]);
+                    store_view(B, C.ci[k], a);
+                    store_view(B, C.cj[k], b);
+                });
+        wave_in_order(
+            iters * m, [&](uint32_t v, uint32_t& bi, uint32_t& bj) { cons_bodies(v % m, bi, bj); },
+            [&](uint32_t v) {
+                const uint32_t it = v / m, k = list[v % m];
                 const uint32_t i = C.ci[k], j = C.cj[k];
                 BodyView a = load_view(B, i), b = load_view(B, j);
                 double* tp = tape + ((size_t)it * ccap + k) * TAPE_DOUBLES;
@@ -3143,11 +3233,14 @@ __global__ void __launch_bounds__(64) k_adj_island(BodiesDev B, SimParams P, con
                 C.timp[k] = timp;
                 store_view(B, i, a);
                 store_view(B, j, b);
-            }
+            });
         // --- reverse of the iterations
-        for (int it = P.iters - 1; it >= 0; --it)
-            for (int t = (int)m - 1; t >= 0; --t) {
-                const uint32_t k = list[t];
+        const uint32_t nv = iters * m;
+        wave_in_order(
+            nv, [&](uint32_t v, uint32_t& bi, uint32_t& bj) { cons_bodies((nv - 1 - v) % m, bi, bj); },
+            [&](uint32_t v) {
+                const uint32_t r = nv - 1 - v;
+                const uint32_t it = r / m, k = list[r % m];
                 const uint32_t i = C.ci[k], j = C.cj[k];
                 BodyView a = load_view(B, i), b = load_view(B, j);
                 const double* tp = tape + ((size_t)it * ccap + k) * TAPE_DOUBLES;
@@ -3169,13 +3262,40 @@ __global__ void __launch_bounds__(64) k_adj_island(BodiesDev B, SimParams P, con
                 CA(CA_RBIAS, k) = cj.rbias; CA(CA_KN, k) = cj.kn; CA(CA_MU, k) = cj.mu;
                 if (ma) { A.put3(AF_V, i, avb); A.put3(AF_W, i, awb); A.putq(i, qadd(A.getq(i), qab)); }
                 if (mb) { A.put3(AF_V, j, bvb); A.put3(AF_W, j, bwb); A.putq(j, qadd(A.getq(j), qbb)); }
-            }
+            });
         // --- reverse of the warm start: every warm impulse sees the adjoint of
         // the velocities after the whole warm start, so this pass ends before
         // the constraint build adds its terms to the body adjoints
         if (P.warm)
-            for (int t = (int)m - 1; t >= 0; --t) {
-                const uint32_t k = list[t];
+            wave_in_order(
+                m, [&](uint32_t v, uint32_t& bi, uint32_t& bj) { cons_bodies(m - 1 - v, bi, bj); },
+                [&](uint32_t v) {
+                    const uint32_t k = list[m - 1 - v];
+                    const uint32_t i = C.ci[k], j = C.cj[k];
+                    const BodyView a = load_view(B, i), b = load_view(B, j);
+                    const bool ma = a.mv || a.mw, mb = b.mv || b.mw;
+                    const ConsGeom g = load_geom(C, k);
+                    V3 rAb{CA(CA_RA, k), CA(CA_RA + 1, k), CA(CA_RA + 2, k)};
+                    V3 rBb{CA(CA_RB, k), CA(CA_RB + 1, k), CA(CA_RB + 2, k)};
+                    Qt qab{0, 0, 0, 0}, qbb{0, 0, 0, 0};
+                    const V3 avb = ma ? A.get3(AF_V, i) : V3{0, 0, 0}, awb = ma ? A.get3(AF_W, i) : V3{0, 0, 0};
+                    const V3 bvb = mb ? A.get3(AF_V, j) : V3{0, 0, 0}, bwb = mb ? A.get3(AF_W, j) : V3{0, 0, 0};
+                    V3 Jb{0, 0, 0};
+                    apply_impulse_rev(a, b, g.rA, g.rB, g.n * CA(CA_NINIT, k), avb, awb, bvb, bwb, Jb, qab, qbb, rAb,
+                                      rBb);
+                    const double ninitb = CA(CA_NIMP, k) + dot(Jb, g.n);
+                    const uint32_t src = C.src[k];
+                    if (src != 0xFFFFFFFFu) nprev[src] = ninitb;
+                    CA(CA_RA, k) = rAb.x; CA(CA_RA + 1, k) = rAb.y; CA(CA_RA + 2, k) = rAb.z;
+                    CA(CA_RB, k) = rBb.x; CA(CA_RB + 1, k) = rBb.y; CA(CA_RB + 2, k) = rBb.z;
+                    if (ma) A.putq(i, qadd(A.getq(i), qab));
+                    if (mb) A.putq(j, qadd(A.getq(j), qbb));
+                });
+        // --- reverse of the constraint build
+        wave_in_order(
+            m, [&](uint32_t v, uint32_t& bi, uint32_t& bj) { cons_bodies(m - 1 - v, bi, bj); },
+            [&](uint32_t v) {
+                const uint32_t k = list[m - 1 - v];
                 const uint32_t i = C.ci[k], j = C.cj[k];
                 const BodyView a = load_view(B, i), b = load_view(B, j);
                 const bool ma = a.mv || a.mw, mb = b.mv || b.mw;
@@ -3183,66 +3303,46 @@ __global__ void __launch_bounds__(64) k_adj_island(BodiesDev B, SimParams P, con
                 V3 rAb{CA(CA_RA, k), CA(CA_RA + 1, k), CA(CA_RA + 2, k)};
                 V3 rBb{CA(CA_RB, k), CA(CA_RB + 1, k), CA(CA_RB + 2, k)};
                 Qt qab{0, 0, 0, 0}, qbb{0, 0, 0, 0};
-                const V3 avb = ma ? A.get3(AF_V, i) : V3{0, 0, 0}, awb = ma ? A.get3(AF_W, i) : V3{0, 0, 0};
-                const V3 bvb = mb ? A.get3(AF_V, j) : V3{0, 0, 0}, bwb = mb ? A.get3(AF_W, j) : V3{0, 0, 0};
-                V3 Jb{0, 0, 0};
-                apply_impulse_rev(a, b, g.rA, g.rB, g.n * CA(CA_NINIT, k), avb, awb, bvb, bwb, Jb, qab, qbb, rAb, rBb);
-                const double ninitb = CA(CA_NIMP, k) + dot(Jb, g.n);
-                const uint32_t src = C.src[k];
-                if (src != 0xFFFFFFFFu) nprev[src] = ninitb;
-                CA(CA_RA, k) = rAb.x; CA(CA_RA + 1, k) = rAb.y; CA(CA_RA + 2, k) = rAb.z;
-                CA(CA_RB, k) = rBb.x; CA(CA_RB + 1, k) = rBb.y; CA(CA_RB + 2, k) = rBb.z;
-                if (ma) A.putq(i, qadd(A.getq(i), qab));
-                if (mb) A.putq(j, qadd(A.getq(j), qbb));
-            }
-        // --- reverse of the constraint build
-        for (int t = (int)m - 1; t >= 0; --t) {
-            const uint32_t k = list[t];
-            const uint32_t i = C.ci[k], j = C.cj[k];
-            const BodyView a = load_view(B, i), b = load_view(B, j);
-            const bool ma = a.mv || a.mw, mb = b.mv || b.mw;
-            const ConsGeom g = load_geom(C, k);
-            V3 rAb{CA(CA_RA, k), CA(CA_RA + 1, k), CA(CA_RA + 2, k)}, rBb{CA(CA_RB, k), CA(CA_RB + 1, k), CA(CA_RB + 2, k)};
-            Qt qab{0, 0, 0, 0}, qbb{0, 0, 0, 0};
-            V3 avb = ma ? A.get3(AF_V, i) : V3{0, 0, 0}, awb = ma ? A.get3(AF_W, i) : V3{0, 0, 0};
-            V3 bvb = mb ? A.get3(AF_V, j) : V3{0, 0, 0}, bwb = mb ? A.get3(AF_W, j) : V3{0, 0, 0};
-            // restitution bias from the pre-solve velocities (world.cpp:40, 544)
-            const V3 va{pre[(size_t)0 * ncap + i], pre[(size_t)1 * ncap + i], pre[(size_t)2 * ncap + i]};
-            const V3 wa{pre[(size_t)3 * ncap + i], pre[(size_t)4 * ncap + i], pre[(size_t)5 * ncap + i]};
-            const V3 vb{pre[(size_t)0 * ncap + j], pre[(size_t)1 * ncap + j], pre[(size_t)2 * ncap + j]};
-            const V3 wb{pre[(size_t)3 * ncap + j], pre[(size_t)4 * ncap + j], pre[(size_t)5 * ncap + j]};
-            const V3 vc = (va + cross(wa, g.rA)) - (vb + cross(wb, g.rB));
-            const double vn = dot(vc, g.n);
-            const WorldDev* wd = B.W ? &B.W[B.wid[i]] : nullptr;
-            const double wrest = wd ? wd->restitution : P.restitution;
-            const double wfric = wd ? wd->friction : P.friction;
-            const double ra = B.rest[i] < 0 ? wrest : B.rest[i], rb = B.rest[j] < 0 ? wrest : B.rest[j];
-            const double fa = B.fric[i] < 0 ? wfric : B.fric[i], fb = B.fric[j] < 0 ? wfric : B.fric[j];
-            const double ecomb = combine_mat(P.rcomb, ra, rb);
-            double eb = 0.0;
-            if (vn < -0.5) {
-                const double rbb = CA(CA_RBIAS, k);
-                eb = -vn * rbb;
-                const V3 vcb = g.n * (-ecomb * rbb);
-                avb = avb + vcb;
-                bvb = bvb - vcb;
-                cross_rev(wa, g.rA, vcb, awb, rAb);
-                cross_rev(wb, g.rB, -vcb, bwb, rBb);
-            }
-            V3 nb{0, 0, 0};
-            eff_mass_rev(a.q, a.ii, b.q, b.ii, g.rA, g.rB, g.n, CA(CA_KN, k), qab, qbb, rAb, rBb, nb);
-            cw[k] = CA(CA_MU, k) * combine_dw(P.fcomb, fa, fb, B.fric[i] < 0, B.fric[j] < 0);
-            cw[(size_t)ccap + k] = eb * combine_dw(P.rcomb, ra, rb, B.rest[i] < 0, B.rest[j] < 0);
-            // rA = minImage(point - x_a): the minimum image is a constant shift
-            if (ma) {
-                A.put3(AF_V, i, avb); A.put3(AF_W, i, awb); A.putq(i, qadd(A.getq(i), qab));
-                A.put3(AF_X, i, A.get3(AF_X, i) - rAb);
-            }
-            if (mb) {
-                A.put3(AF_V, j, bvb); A.put3(AF_W, j, bwb); A.putq(j, qadd(A.getq(j), qbb));
-                A.put3(AF_X, j, A.get3(AF_X, j) - rBb);
-            }
-        }
+                V3 avb = ma ? A.get3(AF_V, i) : V3{0, 0, 0}, awb = ma ? A.get3(AF_W, i) : V3{0, 0, 0};
+                V3 bvb = mb ? A.get3(AF_V, j) : V3{0, 0, 0}, bwb = mb ? A.get3(AF_W, j) : V3{0, 0, 0};
+                // restitution bias from the pre-solve velocities (world.cpp:40, 544)
+                const V3 va{pre[(size_t)0 * ncap + i], pre[(size_t)1 * ncap + i], pre[(size_t)2 * ncap + i]};
+                const V3 wa{pre[(size_t)3 * ncap + i], pre[(size_t)4 * ncap + i], pre[(size_t)5 * ncap + i]};
+                const V3 vb{pre[(size_t)0 * ncap + j], pre[(size_t)1 * ncap + j], pre[(size_t)2 * ncap + j]};
+                const V3 wb{pre[(size_t)3 * ncap + j], pre[(size_t)4 * ncap + j], pre[(size_t)5 * ncap + j]};
+                const V3 vc = (va + cross(wa, g.rA)) - (vb + cross(wb, g.rB));
+                const double vn = dot(vc, g.n);
+                const WorldDev* wd = B.W ? &B.W[B.wid[i]] : nullptr;
+                const double wrest = wd ? wd->restitution : P.restitution;
+                const double wfric = wd ? wd->friction : P.friction;
+                const double ra = B.rest[i] < 0 ? wrest : B.rest[i], rb = B.rest[j] < 0 ? wrest : B.rest[j];
+                const double fa = B.fric[i] < 0 ? wfric : B.fric[i], fb = B.fric[j] < 0 ? wfric : B.fric[j];
+                const double ecomb = combine_mat(P.rcomb, ra, rb);
+                double eb = 0.0;
+                if (vn < -0.5) {
+                    const double rbb = CA(CA_RBIAS, k);
+                    eb = -vn * rbb;
+                    const V3 vcb = g.n * (-ecomb * rbb);
+                    avb = avb + vcb;
+                    bvb = bvb - vcb;
+                    cross_rev(wa, g.rA, vcb, awb, rAb);
+                    cross_rev(wb, g.rB, -vcb, bwb, rBb);
+                }
+                V3 nb{0, 0, 0};
+                eff_mass_rev(a.q, a.ii, b.q, b.ii, g.rA, g.rB, g.n, CA(CA_KN, k), qab, qbb, rAb, rBb, nb);
+                cw[k] = CA(CA_MU, k) * combine_dw(P.fcomb, fa, fb, B.fric[i] < 0, B.fric[j] < 0);
+                cw[(size_t)ccap + k] = eb * combine_dw(P.rcomb, ra, rb, B.rest[i] < 0, B.rest[j] < 0);
+                // rA = minImage(point - x_a): the minimum image is a constant shift
+                if (ma) {
+                    A.put3(AF_V, i, avb); A.put3(AF_W, i, awb); A.putq(i, qadd(A.getq(i), qab));
+                    A.put3(AF_X, i, A.get3(AF_X, i) - rAb);
+                }
+                if (mb) {
+                    A.put3(AF_V, j, bvb); A.put3(AF_W, j, bwb); A.putq(j, qadd(A.getq(j), qbb));
+                    A.put3(AF_X, j, A.get3(AF_X, j) - rBb);
+                }
+            });
+        __syncthreads();   // the next island's sort reuses the LDS
     }
 }
 
@@ -4652,7 +4752,7 @@ int ne_gpu_rollout_backward(ne_gpu_ctx* c, const double* gx, double* gv0, double
             HIP_TRY(c, hipMemcpyAsync(c->adj_pre + 3 * (size_t)ncap, B.w0, sizeof(double) * 3 * ncap,
                                       hipMemcpyDeviceToDevice, s));
             fill(c, c->adj_nprev, ccap, 0.0);
-            k_adj_island<<<hint_blocks(c->hint_c), 64, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, ccap,
+            k_adj_island<<<std::min<uint32_t>(std::max<uint32_t>(c->hint_c, 64u), 16384u), 64, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, ccap,
                                                                c->adj_tape, c->adj_pre, ncap, A, c->adj_ca, c->adj_nfin,
                                                                c->adj_nprev, c->adj_cw);
             k_adj_integrate<<<blocks_for(n), TPB, 0, s>>>(B, P, c->seg[t - t0].state, ncap, A);
