@@ -2084,8 +2084,6 @@ __device__ __forceinline__ void correct(BodiesDev& B, const ConsDev& C, uint32_t
     if (imb != 0.0) xb = xb - corr * imb;
 }
 
-constexpr uint32_t SMALL_ISLAND = 64;   // larger islands go to the workgroup solver
-
 __device__ __forceinline__ void sort_segment(uint32_t* a, uint32_t m) {
     for (uint32_t t = 1; t < m; ++t) {
         uint32_t key = a[t];
@@ -2093,6 +2091,77 @@ __device__ __forceinline__ void sort_segment(uint32_t* a, uint32_t m) {
         while (u > 0 && a[u - 1] > key) { a[u] = a[u - 1]; --u; }
         a[u] = key;
     }
+}
+
+constexpr uint32_t NO_BODY = 0xFFFFFFFFu;
+constexpr uint32_t WAVE_SORT = 4096;   // wave_sort_list sorts lists up to this size in LDS
+
+__device__ __forceinline__ uint32_t movable_or_none(const BodiesDev& B, uint32_t b) {
+    return (B.im[b] != 0.0 || B.ii0[b] != 0.0 || B.ii1[b] != 0.0 || B.ii2[b] != 0.0) ? b : NO_BODY;
+}
+
+// Runs visits 0..count-1 of a sequence on one wave (64 threads per block)
+// with the sequential result: bodies(v, bi, bj) names the movable bodies
+// visit v reads and writes (NO_BODY for none), run(v) performs it.  Visits
+// go in windows of 64; a lane runs once the latest earlier lane of its
+// window touching each of its bodies has run, so every body sees its visits
+// in sequence order and nothing else is shared between visits.
+template <class Bodies, class Run>
+__device__ __forceinline__ void wave_in_order(uint32_t count, Bodies bodies, Run run) {
+    const int lane = threadIdx.x & 63;
+    for (uint32_t base = 0; base < count; base += 64) {
+        const uint32_t v = base + (uint32_t)lane;
+        const bool act = v < count;
+        uint32_t bi = NO_BODY, bj = NO_BODY;
+        if (act) bodies(v, bi, bj);
+        int di = -1, dj = -1;
+        for (int l = 0; l < 63; ++l) {
+            const uint32_t oi = __shfl(bi, l), oj = __shfl(bj, l);
+            if (l < lane) {
+                if (bi != NO_BODY && (oi == bi || oj == bi)) di = l;
+                if (bj != NO_BODY && (oi == bj || oj == bj)) dj = l;
+            }
+        }
+        bool done = !act;
+        uint64_t dm = __ballot(done);
+        while (dm != ~0ull) {   // each pass runs at least the lowest lane not done
+            const bool ready = !done && (di < 0 || ((dm >> di) & 1ull)) && (dj < 0 || ((dm >> dj) & 1ull));
+            if (ready) {
+                run(v);
+                done = true;
+            }
+            __threadfence_block();   // this pass's stores are seen by the next
+            dm = __ballot(done);
+        }
+    }
+}
+
+// Sorts an island's constraint list ascending (the sequential sweep order),
+// one wave: a bitonic sort in LDS, or lane 0 for a very large island.
+__device__ void wave_sort_list(uint32_t* list, uint32_t m, uint32_t* sk) {
+    const int lane = threadIdx.x & 63;
+    if (m <= WAVE_SORT) {
+        uint32_t p2 = 1;
+        while (p2 < m) p2 <<= 1;
+        for (uint32_t t = lane; t < p2; t += 64) sk[t] = t < m ? list[t] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= p2; k <<= 1)
+            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+                for (uint32_t t = lane; t < p2; t += 64) {
+                    const uint32_t l = t ^ jj;
+                    if (l > t) {
+                        const uint32_t a = sk[t], b = sk[l];
+                        if ((a > b) == ((t & k) == 0)) { sk[t] = b; sk[l] = a; }
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t t = lane; t < m; t += 64) list[t] = sk[t];
+    } else if (lane == 0) {
+        sort_segment(list, m);
+    }
+    __threadfence_block();
+    __syncthreads();
 }
 
 // Body-state store for the sequential island sweep in global memory.
@@ -2148,6 +2217,59 @@ __device__ void solve_island_seq(const Store& S, const SimParams& P, const ConsD
 
 template <class Store>
 __device__ __forceinline__ double C_im(const Store& S, uint32_t b) { return S.B.im[b]; }
+
+// The same sweep on one wave (64 threads per block), with the sequential
+// result bit for bit: each pass goes through wave_in_order, so every movable
+// body sees its visits in constraint order, and a constraint between two
+// bodies that cannot move still orders its own visits (its impulses).
+__device__ void solve_island_wave(BodiesDev& B, const SimParams& P, const ConsDev& C, const uint32_t* list, uint32_t m) {
+    auto bodies_of = [&](uint32_t t, uint32_t& bi, uint32_t& bj) {
+        const uint32_t k = list[t];
+        bi = movable_or_none(B, C.ci[k]);
+        bj = movable_or_none(B, C.cj[k]);
+        if (bi == NO_BODY && bj == NO_BODY) bi = 0x80000000u | k;
+    };
+    if (P.warm)
+        wave_in_order(
+            m,
+            [&](uint32_t t, uint32_t& bi, uint32_t& bj) {
+                if (C.nimp[list[t]] != 0.0) bodies_of(t, bi, bj);
+            },
+            [&](uint32_t t) {
+                const uint32_t k = list[t];
+                const double imp = C.nimp[k];
+                if (imp == 0.0) return;
+                const uint32_t i = C.ci[k], j = C.cj[k];
+                BodyView a = load_view(B, i), b = load_view(B, j);
+                warm_apply(a, b, load_geom(C, k), imp);
+                store_view(B, i, a);
+                store_view(B, j, b);
+            });
+    wave_in_order(
+        (uint32_t)P.iters * m, [&](uint32_t v, uint32_t& bi, uint32_t& bj) { bodies_of(v % m, bi, bj); },
+        [&](uint32_t v) {
+            const uint32_t k = list[v % m];
+            const uint32_t i = C.ci[k], j = C.cj[k];
+            BodyView a = load_view(B, i), b = load_view(B, j);
+            double nimp = C.nimp[k], timp = C.timp[k];
+            gs_visit(a, b, load_geom(C, k), C.rbias[k], C.kn[k], C.mu[k], nimp, timp);
+            C.nimp[k] = nimp;
+            C.timp[k] = timp;
+            store_view(B, i, a);
+            store_view(B, j, b);
+        });
+    wave_in_order(
+        m, bodies_of,
+        [&](uint32_t t) {
+            const uint32_t k = list[t];
+            const uint32_t i = C.ci[k], j = C.cj[k];
+            V3 xa = ld3(B.x0, B.x1, B.x2, i), xb = ld3(B.x0, B.x1, B.x2, j);
+            const double ima = B.im[i], imb = B.im[j];
+            correct(B, C, k, P.beta, xa, xb, ima, imb);
+            if (ima != 0.0) st3(B.x0, B.x1, B.x2, i, xa);
+            if (imb != 0.0) st3(B.x0, B.x1, B.x2, j, xb);
+        });
+}
 
 // An island whose constraints all join the same two bodies (a sphere pair,
 // a box-box manifold: <= 4 contacts): both bodies and the impulses stay in
@@ -2370,46 +2492,34 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve_small(BodiesDev B, SimP
     if (P.warm && blockIdx.x == 0 && threadIdx.x == 0) *prev_n = counters[CNT_C];
 }
 
-// One workgroup per large island: sort the segment, then sweep it.
-__global__ void __launch_bounds__(1024) k_solve_large(BodiesDev B, SimParams P, uint32_t* counters,
-                                                      const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
-                                                      ConsDev C, const uint32_t* __restrict__ big_list,
-                                                      uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
+// One wave per large island (captured steps, which cannot read back the
+// island sizes for the dataflow solve): sort the segment, then sweep it.
+__global__ void __launch_bounds__(64) k_solve_large(BodiesDev B, SimParams P, uint32_t* counters,
+                                                    const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
+                                                    ConsDev C, const uint32_t* __restrict__ big_list,
+                                                    uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
+    __shared__ uint32_t sk[WAVE_SORT];
     const uint32_t nbig = counters[CNT_NBIG];
     for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
         const uint32_t isl = big_list[b];
         const uint32_t s = isl_start[isl], e = isl_start[isl + 1];
         const uint32_t m = e - s;
         uint32_t* list = isl_list + s;
-        // bitonic network with every comparator ascending (a "flip" stage then
-        // half-cleaners), so the virtual +inf padding past m never moves
-        uint32_t p2 = 1;
-        while (p2 < m) p2 <<= 1;
-        for (uint32_t k = 2; k <= p2; k <<= 1) {
-            for (uint32_t jj = k; jj > 1; jj >>= 1) {
-                const uint32_t mask = (jj == k) ? (k - 1) : (jj >> 1);
-                if (jj != k && mask == 0) break;
-                for (uint32_t t = threadIdx.x; t < p2; t += blockDim.x) {
-                    const uint32_t u = t ^ mask;
-                    if (u > t && u < m) {
-                        const uint32_t x = list[t], y = list[u];
-                        if (x > y) { list[t] = y; list[u] = x; }
-                    }
-                }
-                __syncthreads();
+        wave_sort_list(list, m, sk);
+        solve_island_wave(B, P, C, list, m);
+        if (P.warm)
+            for (uint32_t t = threadIdx.x; t < m; t += 64) {
+                const uint32_t k = list[t];
+                prev_key[k] = C.key[k];
+                prev_imp[k] = C.nimp[k];
             }
-        }
-        if (threadIdx.x == 0) {
-            solve_island_seq(GlobalStore{B}, P, C, list, m);
-            if (P.warm) publish_cache(C, list, m, prev_key, prev_imp);
-        }
-        __syncthreads();
+        __syncthreads();   // the next island's sort reuses the LDS
     }
 }
 
 // ---------------------------------------------------------------------------
-// Islands of more than SMALL_ISLAND constraints: the sweep as a dataflow of
-// (pass, pair) visits, one round of independent visits per launch.
+// Islands of more than WF_ISLAND constraints in eager steps: the sweep as a
+// dataflow of (pass, pair) visits, one round of independent visits per launch.
 //
 // A task is one pair's run of constraints (consecutive in (i, j, slot)
 // order); a visit is one pass over one task: pass 0 the warm start
@@ -2428,7 +2538,7 @@ constexpr uint32_t WF_NONE = 0xFFFFFFFFu;   // no successor on this side (a stat
 constexpr uint32_t WF_MAX_ROUNDS = 1u << 16;
 constexpr int WF_THREADS = 128;
 constexpr uint32_t WF_PASS_MAX = 66;        // warm start + up to 64 iterations + correction
-constexpr uint32_t WF_ISLAND = STAGE_CONS;  // eager steps: islands above this many constraints take the dataflow solve
+constexpr uint32_t WF_ISLAND = STAGE_CONS;  // islands above this many constraints: the dataflow solve (eager) or k_solve_large (captured)
 
 struct WfDev {
     uint32_t* flag;        // per constraint: 1 if it starts a task of a large island
@@ -3095,77 +3205,6 @@ constexpr int AF_X = 0, AF_V = 3, AF_W = 10;
 
 __device__ __forceinline__ Qt qadd(Qt a, Qt b) { return Qt{a.w + b.w, a.x + b.x, a.y + b.y, a.z + b.z}; }
 
-constexpr uint32_t ADJ_NONE = 0xFFFFFFFFu;
-constexpr uint32_t ADJ_SORT = 4096;   // islands up to this size are sorted in LDS
-
-__device__ __forceinline__ uint32_t adj_movable(const BodiesDev& B, uint32_t b) {
-    return (B.im[b] != 0.0 || B.ii0[b] != 0.0 || B.ii1[b] != 0.0 || B.ii2[b] != 0.0) ? b : ADJ_NONE;
-}
-
-// Runs visits 0..count-1 of a sequence on one wave (64 threads per block)
-// with the sequential result: bodies(v, bi, bj) names the movable bodies
-// visit v reads and writes (ADJ_NONE for none), run(v) performs it.  Visits
-// go in windows of 64; a lane runs once the latest earlier lane of its
-// window touching each of its bodies has run, so every body sees its visits
-// in sequence order and nothing else is shared between visits.
-template <class Bodies, class Run>
-__device__ __forceinline__ void wave_in_order(uint32_t count, Bodies bodies, Run run) {
-    const int lane = threadIdx.x & 63;
-    for (uint32_t base = 0; base < count; base += 64) {
-        const uint32_t v = base + (uint32_t)lane;
-        const bool act = v < count;
-        uint32_t bi = ADJ_NONE, bj = ADJ_NONE;
-        if (act) bodies(v, bi, bj);
-        int di = -1, dj = -1;
-        for (int l = 0; l < 63; ++l) {
-            const uint32_t oi = __shfl(bi, l), oj = __shfl(bj, l);
-            if (l < lane) {
-                if (bi != ADJ_NONE && (oi == bi || oj == bi)) di = l;
-                if (bj != ADJ_NONE && (oi == bj || oj == bj)) dj = l;
-            }
-        }
-        bool done = !act;
-        uint64_t dm = __ballot(done);
-        while (dm != ~0ull) {   // each pass runs at least the lowest lane not done
-            const bool ready = !done && (di < 0 || ((dm >> di) & 1ull)) && (dj < 0 || ((dm >> dj) & 1ull));
-            if (ready) {
-                run(v);
-                done = true;
-            }
-            __threadfence_block();   // this pass's stores are seen by the next
-            dm = __ballot(done);
-        }
-    }
-}
-
-// Sorts an island's constraint list ascending (the sequential sweep order),
-// one wave: a bitonic sort in LDS, or lane 0 for a very large island.
-__device__ void adj_sort_list(uint32_t* list, uint32_t m, uint32_t* sk) {
-    const int lane = threadIdx.x & 63;
-    if (m <= ADJ_SORT) {
-        uint32_t p2 = 1;
-        while (p2 < m) p2 <<= 1;
-        for (uint32_t t = lane; t < p2; t += 64) sk[t] = t < m ? list[t] : 0xFFFFFFFFu;
-        __syncthreads();
-        for (uint32_t k = 2; k <= p2; k <<= 1)
-            for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-                for (uint32_t t = lane; t < p2; t += 64) {
-                    const uint32_t l = t ^ jj;
-                    if (l > t) {
-                        const uint32_t a = sk[t], b = sk[l];
-                        if ((a > b) == ((t & k) == 0)) { sk[t] = b; sk[l] = a; }
-                    }
-                }
-                __syncthreads();
-            }
-        for (uint32_t t = lane; t < m; t += 64) list[t] = sk[t];
-    } else if (lane == 0) {
-        sort_segment(list, m);
-    }
-    __threadfence_block();
-    __syncthreads();
-}
-
 // One wave per island: replays the island's solve (warm start, iterations)
 // recording each visit's inputs, then runs it backwards, then reverses the
 // constraint build, each phase in sequence order through wave_in_order.
@@ -3177,7 +3216,7 @@ __global__ void __launch_bounds__(64) k_adj_island(BodiesDev B, SimParams P, con
                                                    const double* __restrict__ pre, uint32_t ncap, AdjBodies A,
                                                    double* __restrict__ ca, const double* __restrict__ nfin,
                                                    double* __restrict__ nprev, double* __restrict__ cw) {
-    __shared__ uint32_t sk[ADJ_SORT];
+    __shared__ uint32_t sk[WAVE_SORT];
     const uint32_t nisl = counters[CNT_ISL];
     const int lane = threadIdx.x & 63;
     auto CA = [&](int f, uint32_t k) -> double& { return ca[(size_t)f * ccap + k]; };
@@ -3185,15 +3224,15 @@ __global__ void __launch_bounds__(64) k_adj_island(BodiesDev B, SimParams P, con
         const uint32_t s = isl_start[isl], e = isl_start[isl + 1];
         const uint32_t m = e - s;
         uint32_t* list = isl_list + s;
-        adj_sort_list(list, m, sk);
+        wave_sort_list(list, m, sk);
         const uint32_t iters = (uint32_t)P.iters;
         // the movable bodies of the visit to list[t]; a constraint between two
         // bodies that cannot move still orders its own visits (its impulses)
         auto cons_bodies = [&](uint32_t t, uint32_t& bi, uint32_t& bj) {
             const uint32_t k = list[t];
-            bi = adj_movable(B, C.ci[k]);
-            bj = adj_movable(B, C.cj[k]);
-            if (bi == ADJ_NONE && bj == ADJ_NONE) bi = 0x80000000u | k;
+            bi = movable_or_none(B, C.ci[k]);
+            bj = movable_or_none(B, C.cj[k]);
+            if (bi == NO_BODY && bj == NO_BODY) bi = 0x80000000u | k;
         };
         // --- forward replay with tape (solve_island_seq on global memory)
         for (uint32_t t = lane; t < m; t += 64) {
@@ -4257,7 +4296,7 @@ int ensure_wf(ne_gpu_ctx* c, uint32_t T, uint32_t npass) {
     return NE_OK;
 }
 
-// Islands of more than SMALL_ISLAND constraints as a dataflow of (pass,
+// Islands of more than WF_ISLAND constraints as a dataflow of (pass,
 // task) visits: one launch per round, the host reading back the number of
 // visits run after a batch of rounds (the schedule's depth is not known in
 // advance).  Two synchronisations per step.
@@ -4349,10 +4388,10 @@ int enqueue_solve(ne_gpu_ctx* c) {
                                                           c->prev_key, c->prev_imp);
     k_solve_small<<<std::min<uint32_t>(4096u, std::max<uint32_t>(64u, (c->hint_c * 2 + SOLVE_THREADS - 1) / SOLVE_THREADS)),
                     SOLVE_THREADS, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list, c->prev_key,
-                                           c->prev_imp, c->prev_n, c->capturing ? SMALL_ISLAND : WF_ISLAND);
-    if (c->capturing) {   // a graph cannot read back mid-step: one workgroup sweeps each large island
-        k_solve_large<<<256, 1024, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list,
-                                           c->prev_key, c->prev_imp);
+                                           c->prev_imp, c->prev_n, WF_ISLAND);
+    if (c->capturing) {   // a graph cannot read back mid-step: one wave sweeps each large island
+        k_solve_large<<<std::min<uint32_t>(std::max<uint32_t>(c->hint_c / WF_ISLAND, 64u), 8192u), 64, 0, s>>>(
+            B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list, c->prev_key, c->prev_imp);
         return NE_OK;
     }
     return solve_large_islands(c, P);
