@@ -222,6 +222,25 @@ int ne_gpu_get_sleep_store(ne_gpu_ctx *ctx, double *inv_mass_store, double *inv_
 /* `World::resetWarmCache()` (src/world.hpp:140). */
 int ne_gpu_reset_warm_cache(ne_gpu_ctx *ctx);
 
+/* `PhysicsWorld::saveState()` (src/physics_world.cpp:25-39): the bodies in
+ * the reference's "NES1" bytes -- magic 0x4E455331, body count (uint32), then
+ * per body x, q (w,x,y,z), v, w, invMass, invInertiaBody, invMassStore,
+ * invInertiaStore, sleepTimer (doubles) and sleeping (one byte), packed, then
+ * the facade's joint count (uint32; 0, since the general joints are not on
+ * this path).  Packed on the device.  *size is always set to the snapshot's
+ * length; the bytes are written when out is non-NULL, and a cap below *size
+ * is NE_ERR_INVALID. */
+int ne_gpu_save_state(ne_gpu_ctx *ctx, void *out, uint64_t cap, uint64_t *size);
+
+/* `PhysicsWorld::loadState()` (src/physics_world.cpp:41-64): restores every
+ * body field of an NES1 snapshot and resets the warm cache.  NE_ERR_INVALID
+ * (nothing changed) for a wrong magic, another body count, facade joints, or
+ * too few bytes -- where the reference returns false, possibly after loading
+ * some bodies.  NE_ERR_UNSUPPORTED (nothing changed) when a static body that
+ * the grid leaves out would become movable or asleep: upload such a world
+ * with ne_gpu_set_bodies. */
+int ne_gpu_load_state(ne_gpu_ctx *ctx, const void *data, uint64_t size);
+
 /* The warm-start cache (`World::warmCache_`, src/world.hpp:157): the last
  * warm-started step's normal impulses, one entry per contact, ordered by
  * pair key (i << 32 | j, src/world.cpp:34-36) then contact slot.  Export with

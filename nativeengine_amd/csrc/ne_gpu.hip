@@ -126,6 +126,7 @@ enum CounterSlot : int {
     CNT_MQLONG = 20,     // mesh triangle candidates likewise
     CNT_EPABIG_PAIRS = 21,   // pair entries of the large-workspace queue (mesh entries follow them)
     CNT_EPALONG2 = 22,   // convex pairs whose EPA ran past the second launch's budget (this step)
+    CNT_HOSTCHK = 23,    // a host call's own check (snapshot loads); no step kernel uses it
     CNT_N = 24
 };
 
@@ -3625,6 +3626,8 @@ struct ne_gpu_ctx {
     // start-of-call snapshot (rerun after a capacity overflow)
     double* snap_body = nullptr;
     uint32_t* snap_flags = nullptr;
+    uint8_t* nes1 = nullptr;   // snapshot records staged on the device
+    uint64_t nes1_cap = 0;
     uint32_t snap_fcap = 0;
     size_t snap_nd = 0;
     uint64_t* snap_key = nullptr;
@@ -3802,6 +3805,64 @@ enum DF : int {
     F_ST, F_IMS, F_IIS0, F_IIS1, F_IIS2, F_COUNT
 };
 enum UF : int { U_SHAPE, U_LAYER, U_MASK, U_FLAGS, U_COUNT };
+
+// ---- Snapshot in the reference's "NES1" bytes (src/physics_world.cpp:22-64)
+//
+// After the magic and the body count, each body is one packed record in the
+// order saveState writes it: x, q (w, x, y, z), v, w, invMass,
+// invInertiaBody, invMassStore, invInertiaStore, sleepTimer (doubles, native
+// little-endian), then sleeping as one byte.  The facade's joint count ends
+// the blob.
+constexpr uint32_t NES1_MAGIC = 0x4E455331u;
+constexpr uint64_t NES1_HEAD = 8;       // magic, body count
+constexpr uint64_t NES1_RECORD = 177;   // 22 doubles + 1 byte
+constexpr int NES1_FIELDS[22] = {F_X0, F_X1,  F_X2,  F_Q0,  F_Q1,  F_Q2,  F_Q3, F_V0, F_V1, F_V2, F_W0,
+                                 F_W1, F_W2,  F_IM,  F_II0, F_II1, F_II2, F_IMS, F_IIS0, F_IIS1, F_IIS2, F_ST};
+
+// One thread per body: its record, at 177-byte stride (unaligned, so the
+// doubles go through byte copies).
+__global__ void k_nes1_pack(const double* __restrict__ F, uint32_t ncap, const uint32_t* __restrict__ flags,
+                            uint32_t n, uint8_t* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint8_t* o = out + NES1_RECORD * i;
+#pragma unroll
+        for (int k = 0; k < 22; ++k) {
+            const double v = F[(size_t)NES1_FIELDS[k] * ncap + i];
+            __builtin_memcpy(o + 8 * k, &v, 8);
+        }
+        o[176] = (flags[i] & NE_FLAG_SLEEPING) ? 1 : 0;
+    }
+}
+
+// Counts the bodies kept out of the grid as static that a snapshot would make
+// movable (the grid's layout assumes they never move).
+__global__ void k_nes1_check(const uint32_t* __restrict__ flags, uint32_t n, const uint8_t* __restrict__ in,
+                             uint32_t* __restrict__ bad) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (!(flags[i] & FLAG_GRID_EXCLUDED)) continue;
+        const uint8_t* r = in + NES1_RECORD * i;
+        double im, ii[3];
+        __builtin_memcpy(&im, r + 8 * 13, 8);
+        __builtin_memcpy(ii, r + 8 * 14, 24);
+        if (im != 0.0 || ii[0] != 0.0 || ii[1] != 0.0 || ii[2] != 0.0 || r[176] != 0) atomicAdd(bad, 1u);
+    }
+}
+
+// loadState's body loop (src/physics_world.cpp:46-53): every field of the
+// record, and the sleeping flag.
+__global__ void k_nes1_unpack(double* __restrict__ F, uint32_t ncap, uint32_t* __restrict__ flags, uint32_t n,
+                              const uint8_t* __restrict__ in) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint8_t* r = in + NES1_RECORD * i;
+#pragma unroll
+        for (int k = 0; k < 22; ++k) {
+            double v;
+            __builtin_memcpy(&v, r + 8 * k, 8);
+            F[(size_t)NES1_FIELDS[k] * ncap + i] = v;
+        }
+        flags[i] = r[176] ? (flags[i] | NE_FLAG_SLEEPING) : (flags[i] & ~NE_FLAG_SLEEPING);
+    }
+}
 
 double host_bounding_radius(int shape, double r, double hh, const double* he, const double* hull, uint32_t h0,
                             uint32_t h1) {
@@ -4056,6 +4117,7 @@ void free_all(ne_gpu_ctx* c) {
     free_mesh_work(c);
     dfree(c->prev_key); dfree(c->prev_imp); dfree(c->prev_n); dfree(c->counters);
     dfree(c->snap_body); dfree(c->snap_key); dfree(c->snap_imp); dfree(c->snap_n); dfree(c->snap_flags);
+    dfree(c->nes1); c->nes1_cap = 0;
     c->snap_nd = 0;
     c->snap_fcap = 0;
     c->snap_pcap = 0;
@@ -5219,6 +5281,79 @@ int ne_gpu_get_sleep_store(ne_gpu_ctx* c, double* inv_mass_store, double* inv_in
             for (uint32_t i = 0; i < n; ++i) inv_inertia_store[3 * i + k] = tmp[i];
         }
     }
+    return NE_OK;
+}
+
+int ensure_nes1(ne_gpu_ctx* c, uint64_t bytes) {
+    if (c->nes1_cap >= bytes && c->nes1) return NE_OK;
+    HIP_TRY(c, dalloc(c->nes1, (size_t)bytes));
+    c->nes1_cap = bytes;
+    return NE_OK;
+}
+
+int ne_gpu_save_state(ne_gpu_ctx* c, void* out, uint64_t cap, uint64_t* size) {
+    if (!c || !size) return NE_ERR_INVALID;
+    const uint32_t n = c->n;
+    const uint64_t recs = NES1_RECORD * (uint64_t)n;
+    const uint64_t need = NES1_HEAD + recs + 4;
+    *size = need;
+    if (!out) return NE_OK;
+    if (cap < need) return set_err(c, NE_ERR_INVALID, "ne_gpu_save_state: the buffer is smaller than the snapshot");
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    uint8_t* o = static_cast<uint8_t*>(out);
+    const uint32_t head[2] = {NES1_MAGIC, n};
+    std::memcpy(o, head, sizeof(head));
+    if (n) {
+        const int r = ensure_nes1(c, recs);
+        if (r != NE_OK) return r;
+        k_nes1_pack<<<blocks_for(n), TPB, 0, c->stream>>>(c->dfield, c->ncap, c->ufield + (size_t)U_FLAGS * c->ncap, n,
+                                                          c->nes1);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(o + NES1_HEAD, c->nes1, (size_t)recs, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    const uint32_t njoints = 0;   // the facade's general joints are not on this path
+    std::memcpy(o + NES1_HEAD + recs, &njoints, 4);
+    return NE_OK;
+}
+
+int ne_gpu_load_state(ne_gpu_ctx* c, const void* data, uint64_t size) {
+    if (!c || (size && !data)) return NE_ERR_INVALID;
+    const uint8_t* d = static_cast<const uint8_t*>(data);
+    const uint32_t n = c->n;
+    const uint64_t recs = NES1_RECORD * (uint64_t)n;
+    uint32_t head[2] = {0, 0};
+    if (size < NES1_HEAD) return set_err(c, NE_ERR_INVALID, "ne_gpu_load_state: shorter than the snapshot header");
+    std::memcpy(head, d, sizeof(head));
+    if (head[0] != NES1_MAGIC) return set_err(c, NE_ERR_INVALID, "ne_gpu_load_state: not an NES1 snapshot");
+    if (head[1] != n) return set_err(c, NE_ERR_INVALID, "ne_gpu_load_state: body count differs from the world's");
+    if (size < NES1_HEAD + recs + 4) return set_err(c, NE_ERR_INVALID, "ne_gpu_load_state: truncated snapshot");
+    uint32_t njoints = 0;
+    std::memcpy(&njoints, d + NES1_HEAD + recs, 4);
+    if (njoints != 0)
+        return set_err(c, NE_ERR_INVALID, "ne_gpu_load_state: the snapshot has facade joints, which this path has none of");
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    if (n == 0) {
+        HIP_TRY(c, hipMemsetAsync(c->prev_n, 0, sizeof(uint32_t), c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return NE_OK;
+    }
+    int r = ensure_nes1(c, recs);
+    if (r != NE_OK) return r;
+    uint32_t* flags = c->ufield + (size_t)U_FLAGS * c->ncap;
+    HIP_TRY(c, hipMemcpyAsync(c->nes1, d + NES1_HEAD, (size_t)recs, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->counters + CNT_HOSTCHK, 0, sizeof(uint32_t), c->stream));
+    k_nes1_check<<<blocks_for(n), TPB, 0, c->stream>>>(flags, n, c->nes1, c->counters + CNT_HOSTCHK);
+    uint32_t bad = 0;
+    HIP_TRY(c, hipMemcpyAsync(&bad, c->counters + CNT_HOSTCHK, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (bad)
+        return set_err(c, NE_ERR_UNSUPPORTED,
+                       "ne_gpu_load_state: a static body kept out of the grid would move; upload with ne_gpu_set_bodies");
+    k_nes1_unpack<<<blocks_for(n), TPB, 0, c->stream>>>(c->dfield, c->ncap, flags, n, c->nes1);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemsetAsync(c->prev_n, 0, sizeof(uint32_t), c->stream));   // resetWarmCache (world.hpp:140)
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     return NE_OK;
 }
 

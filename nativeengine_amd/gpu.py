@@ -27,7 +27,7 @@ EXPORTS = [
     "ne_gpu_fp_probe", "ne_gpu_time_kernel", "ne_gpu_kernel_time", "ne_gpu_export_warm_cache",
     "ne_gpu_import_warm_cache", "ne_gpu_set_worlds", "ne_gpu_use_graph", "ne_gpu_autotune", "ne_gpu_epa_stats",
     "ne_gpu_rollout", "ne_gpu_rollout_backward", "ne_gpu_set_joints", "ne_gpu_set_meshes", "ne_gpu_mesh_bvh",
-    "ne_gpu_get_sleep_store",
+    "ne_gpu_get_sleep_store", "ne_gpu_save_state", "ne_gpu_load_state",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
@@ -63,6 +63,8 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_set_meshes.argtypes = [vp, C.c_uint32, C.POINTER(_abi.Mesh)]
     lib.ne_gpu_mesh_bvh.argtypes = [vp, C.c_uint32, C.POINTER(C.c_int32), C.c_uint32, u32p, dp, C.POINTER(C.c_int32)]
     lib.ne_gpu_reset_warm_cache.argtypes = [vp]
+    lib.ne_gpu_save_state.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
+    lib.ne_gpu_load_state.argtypes = [vp, vp, C.c_uint64]
     lib.ne_gpu_get_contacts.argtypes = [vp, C.c_uint32, u32p, u32p, u32p, dp, dp, dp, dp]
     lib.ne_gpu_get_pairs.argtypes = [vp, C.c_uint32, u32p, u32p, u32p]
     lib.ne_gpu_diagnostics.argtypes = [vp, dp, dp, dp]
@@ -249,6 +251,19 @@ class GpuWorld:
         fl, tm, im, ii = np.zeros(n, np.uint32), np.zeros(n), np.zeros(n), np.zeros((n, 3))
         self._check(self.lib.ne_gpu_get_body_aux(self.h, _abi.uptr(fl), _abi.dptr(tm), _abi.dptr(im), _abi.dptr(ii)))
         return fl, tm, im, ii
+
+    def save_state(self) -> bytes:
+        """The bodies as the reference's NES1 snapshot (PhysicsWorld::saveState)."""
+        size = C.c_uint64(0)
+        self._check(self.lib.ne_gpu_save_state(self.h, None, 0, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        self._check(self.lib.ne_gpu_save_state(self.h, buf, size.value, C.byref(size)))
+        return buf.raw[: size.value]
+
+    def load_state(self, blob: bytes):
+        """Restores an NES1 snapshot (PhysicsWorld::loadState); resets the warm cache."""
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        self._check(self.lib.ne_gpu_load_state(self.h, buf, len(blob)))
 
     def sleep_store(self):
         """(Body::invMassStore (n), Body::invInertiaStore (n, 3)) of every body."""

@@ -18,6 +18,9 @@
 //   - `detectContact` / `boxBoxManifold`                (src/detect.hpp:33,
 //                                                        src/manifold.hpp:90)
 //   - `totalLinearMomentum/AngularMomentum/Kinetic`     (src/world.cpp:704-729)
+//   - `PhysicsWorld::saveState` / `loadState` over a copy of the world, for
+//     the "NES1" snapshot bytes                         (src/physics_world.cpp:22-64,
+//                                                        physics_world.hpp:379)
 // and exposes them through the same flat-array layout as include/ne_gpu.h so a
 // test can feed one scene to the GPU, the C restatement and the reference.
 #include <cmath>
@@ -29,6 +32,7 @@
 #include "world.hpp"
 #include "detect.hpp"
 #include "manifold.hpp"
+#include "physics_world.hpp"
 
 #include "../include/ne_gpu.h"
 
@@ -320,6 +324,33 @@ int ne_ref_wrap(void* h, const double* p, double* out) {
     if (!r) return NE_ERR_INVALID;
     ne::V3 m = r->w.box.wrap(ne::V3{p[0], p[1], p[2]});
     out[0] = m.x; out[1] = m.y; out[2] = m.z;
+    return NE_OK;
+}
+
+// The reference's snapshot of the world's bodies: a PhysicsWorld holding a
+// copy of the world (no facade joints) writes it.  *size is always set; the
+// bytes are copied when cap is enough.
+int ne_ref_save_state(void* h, void* out, uint64_t cap, uint64_t* size) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || !size) return NE_ERR_INVALID;
+    ne::PhysicsWorld pw;
+    pw.world() = r->w;
+    const std::vector<char> b = pw.saveState();
+    *size = b.size();
+    if (out && cap >= b.size()) std::memcpy(out, b.data(), b.size());
+    return NE_OK;
+}
+
+// The reference's loadState on a copy of the world, copied back on success
+// (bodies from the snapshot, warm cache reset).
+int ne_ref_load_state(void* h, const void* data, uint64_t size) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || (size && !data)) return NE_ERR_INVALID;
+    ne::PhysicsWorld pw;
+    pw.world() = r->w;
+    const char* p = static_cast<const char*>(data);
+    if (!pw.loadState(std::vector<char>(p, p + size))) return NE_ERR_INVALID;
+    r->w = pw.world();
     return NE_OK;
 }
 

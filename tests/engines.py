@@ -85,6 +85,8 @@ def _load(kind):
         lib = _bind(C.CDLL(REF_LIB), "ne_ref_")
         lib.ne_ref_get_contacts.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
         lib.ne_ref_set_capture.argtypes = [_vp, C.c_int]
+        lib.ne_ref_save_state.argtypes = [_vp, _vp, C.c_uint64, C.POINTER(C.c_uint64)]
+        lib.ne_ref_load_state.argtypes = [_vp, _vp, C.c_uint64]
     _libs[kind] = lib
     return lib
 
@@ -239,6 +241,18 @@ class RefWorld(_CpuWorld):
     def __init__(self, capture=True):
         super().__init__()
         self.lib.ne_ref_set_capture(self.h, 1 if capture else 0)
+
+    def save_state(self) -> bytes:
+        """The reference's own PhysicsWorld::saveState bytes for this world."""
+        size = C.c_uint64(0)
+        self._check(self.lib.ne_ref_save_state(self.h, None, 0, C.byref(size)), "save_state")
+        buf = C.create_string_buffer(size.value)
+        self._check(self.lib.ne_ref_save_state(self.h, buf, size.value, C.byref(size)), "save_state")
+        return buf.raw[: size.value]
+
+    def load_state(self, blob: bytes):
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        self._check(self.lib.ne_ref_load_state(self.h, buf, len(blob)), "load_state")
 
     def contacts(self):
         cnt = C.c_uint32(0)
