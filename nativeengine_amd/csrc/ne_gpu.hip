@@ -2360,7 +2360,7 @@ __global__ void __launch_bounds__(TPB) k_solve_single(BodiesDev B, SimParams P, 
 // Gauss-Seidel chain never waits on global memory.  Larger small islands take
 // the global-memory sweep.
 constexpr int SOLVE_THREADS = 32;
-constexpr int STAGE_BODIES = 8;
+constexpr int STAGE_BODIES = 17;   // 16 constraints join at most 17 bodies: every island of k_solve_small fits
 constexpr int STAGE_CONS = 16;
 
 struct StagedBody {
