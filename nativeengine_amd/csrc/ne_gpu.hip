@@ -3923,6 +3923,12 @@ uint32_t hint_blocks(uint32_t hint) {
     return (uint32_t)b;
 }
 
+// One-wave blocks for the per-thread EPA kernels: they run one wave per SIMD
+// and their waves end at very different times, so a SIMD whose wave ended
+// takes new work at once instead of waiting for the rest of a larger block.
+uint32_t wave_blocks(uint32_t hint) { return std::min<uint32_t>(4u * hint_blocks(hint), 16384u); }
+
+
 uint32_t gs_blocks(uint64_t n) {   // grid-stride launches over device-side counts
     uint64_t b = (n + TPB - 1) / TPB;
     if (b < 1) b = 1;
@@ -4248,11 +4254,11 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
         } else {
             k_narrow<false><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont,
                                                                c->ct, c->epa_q, c->pcap);
-            k_epa<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
+            k_epa<<<wave_blocks(c->hint_c), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
                                                          c->epa_q, CNT_EPAQ, c->pcap, c->epa_big, EPA_FIRST_ITERS,
                                                          c->epa_long);
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_EPALONG2, 0u, nullptr, 1u);
-            k_epa<<<hint_blocks(c->hint_c / 8), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
+            k_epa<<<wave_blocks(c->hint_c / 8), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
                                                              c->epa_long, CNT_EPALONG, c->pcap, c->epa_big,
                                                              EPA_SECOND_ITERS, c->epa_long2);
             k_epa_wave<<<2048, 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_long2,
@@ -4278,7 +4284,7 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
             k_mesh_emit<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_off,
                                                                c->mqcap, c->mq_pair, c->mq_tri);
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_MQLONG, 0u, nullptr, 1u);
-            k_mesh_narrow<<<hint_blocks(c->hint_mq), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
+            k_mesh_narrow<<<wave_blocks(c->hint_mq), 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
                                                                   c->mq_tri, c->mhit, c->mct, c->epa_big, nullptr, CNT_MQ,
                                                                   EPA_FIRST_ITERS, c->mq_long);
             k_mesh_wave<<<2048, 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri,
