@@ -49,7 +49,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
-# Algorithmic bytes per unit for each step kernel ("Roofline"):
+# Algorithmic bytes per unit for each step kernel:
 # integrate reads 17 and writes 13 doubles per body; the narrow phase and the
 # candidate kernels are credited per candidate pair (bound-sphere-overlapping,
 # filter-passing pair: pair write 8 + 2 x 96 B pose/geometry reads).
@@ -653,8 +653,8 @@ def main():
     args.large_steps, args.large_warmup = args.steps, args.warmup
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
-                                   "(DESIGN.md), so --gpus must be 1", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the step runs on one GPU; sharding one world across GPUs is not "
+                                   "implemented, so --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)
 
     metric = "body-steps/s at 1M bodies; batched world-steps/s; backward/forward time ratio"
