@@ -114,6 +114,26 @@ def test_torch_autograd_function(gpu_available):
     dw.close()
 
 
+def test_adjoint_is_bitwise_repeatable(gpu_available):
+    """Two backward passes over one rollout give the same bits: the island
+    pass runs each body's operations in sequence order whatever the waves'
+    timing, and nothing accumulates with float atomics.  Piles large enough
+    for islands of hundreds of constraints (many ordered windows)."""
+    from nativeengine_amd.gpu import GpuWorld
+    b = S.adjoint_worlds(8, n_spheres=96, n_boxes=96, side=8.0, seed=3)
+    g = GpuWorld()
+    g.set_scene(b.scene)
+    g.set_worlds(b.offsets, b.half, b.restitution, b.friction)
+    x = g.rollout(b.scene.v, b.friction.copy(), b.restitution.copy(), 60, 16)
+    wgt = np.random.default_rng(11).normal(size=x.shape)
+    first = g.rollout_backward(wgt, b.nworlds)
+    second = g.rollout_backward(wgt, b.nworlds)
+    for name, u, t in zip(("v0", "mu", "e"), first, second):
+        assert np.array_equal(np.asarray(u).view(np.uint64), np.asarray(t).view(np.uint64)), name
+    assert np.any(np.asarray(first[0]) != 0.0)
+    g.close()
+
+
 def test_adjoint_at_full_rollout_size_against_fd(gpu_available, oracle_lib):
     """The differentiable-rollout workload at full size: 256 worlds of 256
     spheres (r 0.4) and 256 boxes (half extent 0.3) on a static ground box,
