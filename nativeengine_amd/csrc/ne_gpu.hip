@@ -2537,7 +2537,7 @@ __global__ void __launch_bounds__(64) k_solve_large(BodiesDev B, SimParams P, ui
 constexpr uint32_t WF_WRAP = 0x80000000u;   // successor is the body's first task of the next pass
 constexpr uint32_t WF_NONE = 0xFFFFFFFFu;   // no successor on this side (a static body)
 constexpr uint32_t WF_MAX_ROUNDS = 1u << 16;
-constexpr int WF_THREADS = 128;
+constexpr int WF_THREADS = 64;
 constexpr uint32_t WF_PASS_MAX = 66;        // warm start + up to 64 iterations + correction
 constexpr uint32_t WF_ISLAND = STAGE_CONS;  // islands above this many constraints: the dataflow solve (eager) or k_solve_large (captured)
 
