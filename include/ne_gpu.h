@@ -10,7 +10,9 @@
  * reference's `ne::World` class on top of these entry points.
  *
  * Conventions
- *   - Every function returns an int status: NE_OK (0) or a negative NE_ERR_*.
+ *   - Every function returns an int status: NE_OK (0) or a negative NE_ERR_*;
+ *     ne_gpu_step and ne_gpu_rollout can also return the positive warning
+ *     NE_WARN_EPA_CAPPED.
  *     Nothing throws across this boundary.
  *   - A context is single-threaded: one host thread, one HIP stream.
  *   - All pointers are host pointers unless the name says `device`.
@@ -38,6 +40,12 @@ extern "C" {
 #define NE_ERR_NONFINITE    (-4)   /* a non-finite position/velocity was found */
 #define NE_ERR_UNSUPPORTED  (-5)   /* shape / feature not on the GPU path yet  */
 #define NE_ERR_NO_DEVICE    (-6)   /* no HIP device visible                    */
+/* Positive: the call completed, with a warning. */
+#define NE_WARN_EPA_CAPPED    1    /* ne_gpu_step: at least one EPA run stopped
+                                      at the workspace's face cap
+                                      (ne_gpu_config.epa_face_cap), so its
+                                      contact can differ from the reference's;
+                                      ne_gpu_epa_stats counts them */
 
 /* ---- shapes: the reference's `enum class Shape` order (src/body.hpp:24) -- */
 #define NE_SHAPE_SPHERE    0
@@ -148,6 +156,12 @@ typedef struct ne_gpu_config {
     double   cell_scale;        /* grid cell >= cell_scale * 2*maxR (>= 1; 0 = 1) */
     int32_t  broadphase;        /* NE_BROADPHASE_*: the neighbourhood walk variant */
     uint32_t cells_per_body;    /* cell-table capacity per body (0 = 8)       */
+    uint32_t epa_face_cap;      /* faces per EPA run in the large workspace
+                                   (0 = 131072; at least 64, at most 2^26).
+                                   The reference's EPA grows its face list
+                                   without bound; a run that needs more than
+                                   the cap stops at its closest face and the
+                                   step returns NE_WARN_EPA_CAPPED */
 } ne_gpu_config;
 
 /* Broadphase walk variants (same candidate rows; a tuning knob). */
@@ -203,7 +217,9 @@ int ne_gpu_set_state(ne_gpu_ctx *ctx, const double *x, const double *v,
  * Work buffers (candidate pairs, constraints, the large EPA workspace) are
  * sized from the bodies; when one overflows, the call enlarges it, restores
  * the state it started from (a device-side copy taken at the start of every
- * call) and runs again, up to 4 times, so results never depend on capacity. */
+ * call) and runs again, up to 4 times, so results never depend on capacity.
+ * Returns NE_WARN_EPA_CAPPED (> 0) when the steps completed but an EPA run
+ * reached the face cap of ne_gpu_config.epa_face_cap. */
 int ne_gpu_step(ne_gpu_ctx *ctx, int32_t nsteps);
 
 /* Downloads x (n*3), v (n*3), q (n*4), w (n*3); any may be NULL. */
@@ -320,11 +336,12 @@ int ne_gpu_set_worlds(ne_gpu_ctx *ctx, uint32_t nworlds, const uint32_t *offsets
 /* EPA workspace use during the last ne_gpu_step call.  Most EPA runs fit a
  * small per-thread workspace; the rest (near-degenerate contacts, where the
  * reference's vectors grow to thousands of faces) are rerun with a
- * global-memory workspace that grows up to 131072 faces -- `reruns` counts
- * them.  A run that would need more stops at the closest face found so far,
- * as at the reference's 64-iteration limit, where the reference would keep
- * growing its vectors -- `truncated` counts those (the only results that can
- * differ from the reference's). */
+ * global-memory workspace that grows up to ne_gpu_config.epa_face_cap
+ * faces -- `reruns` counts them.  A run that would need more stops at the
+ * closest face found so far, as at the reference's 64-iteration limit, where
+ * the reference would keep growing its vectors -- `truncated` counts those
+ * (the only results that can differ from the reference's; ne_gpu_step then
+ * returns NE_WARN_EPA_CAPPED). */
 int ne_gpu_epa_stats(ne_gpu_ctx *ctx, uint32_t *reruns, uint32_t *truncated);
 
 /* Replays each step from a captured HIP graph (enable = 1) instead of
@@ -360,7 +377,9 @@ int ne_gpu_autotune(ne_gpu_ctx *ctx, int32_t steps, ne_tune_result *out);
  * v0, x_out, grad_x and grad_v0 are n x 3 doubles in host or device memory
  * (copied with hipMemcpyDefault); mu, e, grad_mu and grad_e are host arrays
  * with one entry per world (1 without ne_gpu_set_worlds).  Periodic worlds
- * without applied forces only. */
+ * without applied forces only.  ne_gpu_rollout returns NE_WARN_EPA_CAPPED
+ * (the rollout and its checkpoints complete) when a step reached the EPA
+ * face cap. */
 int ne_gpu_rollout(ne_gpu_ctx *ctx, const double *v0, const double *mu, const double *e,
                    int32_t nsteps, int32_t checkpoint_every, double *x_out);
 int ne_gpu_rollout_backward(ne_gpu_ctx *ctx, const double *grad_x, double *grad_v0,

@@ -45,11 +45,18 @@ public:
         for (int attempt = 0; attempt < 4; ++attempt) {
             upload(w, iterations);
             int rc = ne_gpu_step(ctx_, nsteps);
-            if (rc == NE_OK) { download(w); return; }
+            if (rc == NE_OK || rc == NE_WARN_EPA_CAPPED) {   // the warning: the steps ran, see epaCapped()
+                epaCapped_ = rc == NE_WARN_EPA_CAPPED;
+                download(w);
+                return;
+            }
             if (rc != NE_ERR_CAPACITY) fail(rc);
         }
         fail(NE_ERR_CAPACITY);
     }
+    // Whether the last step() had an EPA run stop at the workspace's face cap
+    // (its contact can then differ from the reference's; see NE_WARN_EPA_CAPPED).
+    bool epaCapped() const { return epaCapped_; }
     void resetWarmCache() { check(ne_gpu_reset_warm_cache(ctx_)); }
     void exportWarmCache(std::vector<std::uint64_t>& keys, std::vector<double>& imp) {
         std::uint32_t n = 0;
@@ -64,6 +71,7 @@ public:
 
 private:
     ne_gpu_ctx* ctx_ = nullptr;
+    bool epaCapped_ = false;
     std::vector<double> x_, v_, q_, w_, f_, t_, im_, ii_, r_, hh_, he_, fr_, re_, ld_, ad_, gs_, hull_;
     std::vector<double> st_, ims_, iis_;                  // sleep timer, stored inverse mass / inertia
     std::vector<ne_mesh> meshes_;

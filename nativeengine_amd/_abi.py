@@ -14,6 +14,7 @@ NE_ERR_CAPACITY = -3
 NE_ERR_NONFINITE = -4
 NE_ERR_UNSUPPORTED = -5
 NE_ERR_NO_DEVICE = -6
+NE_WARN_EPA_CAPPED = 1
 
 SHAPE_SPHERE, SHAPE_CYLINDER, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CONVEX, SHAPE_PLANE, SHAPE_MESH, SHAPE_COMPOUND = range(8)
 
@@ -85,6 +86,7 @@ class GpuConfig(C.Structure):
         ("cell_scale", C.c_double),
         ("broadphase", C.c_int32),
         ("cells_per_body", C.c_uint32),
+        ("epa_face_cap", C.c_uint32),
     ]
 
 BROADPHASE_AUTO, BROADPHASE_WALK, BROADPHASE_BRICK, BROADPHASE_CELLS = 0, 1, 2, 3

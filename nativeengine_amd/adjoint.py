@@ -19,16 +19,21 @@ from .scene import Batch, Scene
 class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, v0, mu, e, world, nsteps, every):
-        world._reset()
-        x = world.gpu.rollout(v0.detach().double().cpu().numpy(), mu.detach().double().cpu().numpy(),
-                              e.detach().double().cpu().numpy(), nsteps, every)
+        inputs = tuple(t.detach().double().cpu().numpy().copy() for t in (v0, mu, e))
         ctx.world = world
+        ctx.gen = world._run(inputs, nsteps, every)[1]
+        ctx.inputs, ctx.nsteps, ctx.every = inputs, nsteps, every
         ctx.nw = mu.numel()
         ctx.devs = (v0.device, mu.device, e.device)
-        return torch.from_numpy(x).to(v0.device)
+        return torch.from_numpy(world._last_x).to(v0.device)
 
     @staticmethod
     def backward(ctx, gx):
+        # The library holds the checkpoints of the last rollout only: when
+        # another rollout ran on these worlds since this node's forward, this
+        # node's rollout is recomputed (bit-identically) before its adjoint.
+        if ctx.world._generation != ctx.gen:
+            ctx.gen = ctx.world._run(ctx.inputs, ctx.nsteps, ctx.every)[1]
         gv, gmu, ge = ctx.world.gpu.rollout_backward(gx.detach().double().cpu().numpy(), ctx.nw)
         return (torch.from_numpy(gv).to(ctx.devs[0]), torch.from_numpy(gmu).to(ctx.devs[1]),
                 torch.from_numpy(ge).to(ctx.devs[2]), None, None, None)
@@ -59,10 +64,20 @@ class DifferentiableWorlds:
         self.n = self.scene.n
         self.nsteps = nsteps
         self.every = every
+        self._generation = 0   # which rollout the library's checkpoints belong to
+        self._last_x = None
 
     def _reset(self):
         s = self.scene
         self.gpu.set_state(s.x, s.v, s.q, s.w)
+
+    def _run(self, inputs, nsteps, every):
+        """Rollout from the scene's initial poses; returns (final x, generation)."""
+        v0, mu, e = inputs
+        self._reset()
+        self._last_x = self.gpu.rollout(v0, mu, e, nsteps, every)
+        self._generation += 1
+        return self._last_x, self._generation
 
     def rollout(self, v0: torch.Tensor, mu: torch.Tensor, e: torch.Tensor) -> torch.Tensor:
         return _Rollout.apply(v0, mu, e, self, self.nsteps, self.every)

@@ -1036,7 +1036,7 @@ constexpr uint32_t EPA_BIG_PAIRS = 64;    // initial queue capacity
 constexpr int EPA_FIRST_ITERS = 8;        // EPA iterations of the first tier-1 launch
 constexpr int EPA_SECOND_ITERS = 24;      // of the second (the third runs the rest)
 constexpr int EPA_BIG_FACES = 4096;       // initial faces per queued pair
-constexpr int EPA_BIG_FACES_MAX = 1 << 17;   // beyond this a run stops at its closest face
+constexpr int EPA_BIG_FACES_MAX = 1 << 17;   // default cap: beyond it a run stops at its closest face
 constexpr int EPA_KEYS = EPA_MAXV * EPA_MAXV;   // directed edges u * EPA_MAXV + w of one polytope
 constexpr int BIG_NW = 8;                       // waves per large-workspace run
 
@@ -1046,6 +1046,7 @@ struct EpaBig {
     uint32_t* list;   // queued pair indices (or MESH_ENTRY | mesh candidate index)
     uint32_t cap;     // queue slots
     int faces_per;    // faces per slot
+    int faces_max;    // the cap faces_per grows to; a run that needs more stops there
     Face* faces;
     uint16_t* occ;    // 3 per face: directed edges of the visible faces, in order
 };
@@ -1438,7 +1439,7 @@ __global__ void __launch_bounds__(64 * BIG_NW) k_narrow_big(BodiesDev B, SimPara
         const uint32_t p = big.list[t];
         if (p & MESH_ENTRY) continue;   // a mesh triangle: k_mesh_big
         const size_t fo = (size_t)t * (size_t)big.faces_per;
-        const bool truncate = big.faces_per >= EPA_BIG_FACES_MAX;
+        const bool truncate = big.faces_per >= big.faces_max;
         const uint32_t i = cand_i[p], j = cand_j[p];
         const Box3& box = box_of(B, P, i);
         const Geom a = load_geom(B, i), b = load_geom(B, j);
@@ -1768,7 +1769,7 @@ __global__ void __launch_bounds__(64 * BIG_NW) k_mesh_big(BodiesDev B, SimParams
         const uint32_t q = e & ~MESH_ENTRY;
         if (threadIdx.x == 0) atomicAdd(&counters[CNT_EPARERUN], 1u);
         const size_t fo = (size_t)t * (size_t)big.faces_per;
-        const bool truncate = big.faces_per >= EPA_BIG_FACES_MAX;
+        const bool truncate = big.faces_per >= big.faces_max;
         const uint32_t p = mq_pair[q];
         uint32_t o, m;
         bool flip;
@@ -3641,6 +3642,7 @@ struct ne_gpu_ctx {
     int broadphase = NE_BROADPHASE_AUTO;
     uint32_t hint_p = 4096, hint_c = 4096;   // last call's pair/constraint counts (launch sizing)
     uint32_t cells_per_body = 2;
+    int epa_faces_max = EPA_BIG_FACES_MAX;   // the large EPA workspace's face cap per run
     GridDev grid_host{};
     uint32_t sgrid_cells = 0;
     // host copies from the last upload (grid plans are rebuilt from them)
@@ -4030,8 +4032,9 @@ int alloc_epa_big(ne_gpu_ctx* c, uint32_t cap, int faces_per) {
     drop_graphs(c);
     EpaBig& g = c->epa_big;
     g.cap = cap;
-    g.faces_per = faces_per;
-    const size_t nf = (size_t)cap * (size_t)faces_per;
+    g.faces_per = std::min(faces_per, c->epa_faces_max);
+    g.faces_max = c->epa_faces_max;
+    const size_t nf = (size_t)cap * (size_t)g.faces_per;
     HIP_TRY(c, dalloc(g.list, cap));
     HIP_TRY(c, dalloc(g.faces, nf));
     HIP_TRY(c, dalloc(g.occ, 3 * nf));
@@ -4776,11 +4779,13 @@ int ne_gpu_rollout(ne_gpu_ctx* c, const double* v0, const double* mu, const doub
     HIP_TRY(c, hipMemsetAsync(c->prev_n, 0, sizeof(uint32_t), c->stream));   // a rollout starts with no warm cache
     const int32_t nck = (nsteps + every - 1) / every;
     ensure_saved(c, c->ckpt, (size_t)nck);
+    bool capped = false;
     for (int32_t k = 0; k < nck; ++k) {
         r = save_state(c, c->ckpt[k]);
         if (r != NE_OK) return r;
         const int32_t m = std::min(every, nsteps - k * every);
         r = ne_gpu_step(c, m);
+        if (r == NE_WARN_EPA_CAPPED) { capped = true; r = NE_OK; }   // reported once the rollout is complete
         if (r != NE_OK) return r;
     }
     if (x_out) {
@@ -4792,6 +4797,9 @@ int ne_gpu_rollout(ne_gpu_ctx* c, const double* v0, const double* mu, const doub
     c->roll_steps = nsteps;
     c->roll_every = every;
     c->adj_nw = nw;
+    if (capped)
+        return set_err(c, NE_WARN_EPA_CAPPED, "the rollout completed, but an EPA run stopped at the workspace's face "
+                                              "cap in at least one step");
     return NE_OK;
 }
 
@@ -4843,6 +4851,7 @@ int ne_gpu_rollout_backward(ne_gpu_ctx* c, const double* gx, double* gv0, double
             if (r != NE_OK) return r;
             if (t + 1 < t1) {
                 r = ne_gpu_step(c, 1);
+                if (r == NE_WARN_EPA_CAPPED) r = NE_OK;   // the forward rollout reported it
                 if (r != NE_OK) return r;
             }
         }
@@ -4964,6 +4973,8 @@ int ne_gpu_create(const ne_gpu_config* cfg, ne_gpu_ctx** out) {
     c->cell_scale = (cfg && cfg->cell_scale >= 1.0) ? cfg->cell_scale : 1.0;
     c->broadphase = cfg ? cfg->broadphase : NE_BROADPHASE_AUTO;
     c->cells_per_body = (cfg && cfg->cells_per_body > 0) ? cfg->cells_per_body : 8u;
+    c->epa_faces_max = (cfg && cfg->epa_face_cap > 0) ? (int)std::min<uint32_t>(std::max<uint32_t>(cfg->epa_face_cap, 64u), 1u << 26)
+                                                       : EPA_BIG_FACES_MAX;
     if (c->broadphase < NE_BROADPHASE_AUTO || c->broadphase > NE_BROADPHASE_CELLS) { delete c; return NE_ERR_INVALID; }
     if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -5141,6 +5152,13 @@ int ne_gpu_step(ne_gpu_ctx* c, int32_t nsteps) {
         r = prepare_static_grid(c);
         if (r != NE_OK) return r;
         r = step_call(c, nsteps);
+        if (r == NE_OK && c->h_counters[CNT_EPATRUNC]) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "%u EPA run(s) stopped at the %d-face workspace cap; their contacts can differ "
+                     "from the reference's", c->h_counters[CNT_EPATRUNC], c->epa_faces_max);
+            (void)set_err(c, NE_WARN_EPA_CAPPED, buf);
+            return NE_WARN_EPA_CAPPED;
+        }
         if (r != NE_ERR_CAPACITY || attempt == 4) return r;
         // a buffer overflowed and has been enlarged: rerun the call from its start
         r = snapshot_restore(c);
@@ -5194,7 +5212,7 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps) {
             int per = c->epa_big.faces_per;
             if (fault & NE_FAULT_EPA_QUEUE)
                 slots = std::max(slots * 2u, c->h_counters[CNT_EPABIG] + c->h_counters[CNT_EPABIG] / 2u + 16u);
-            if (fault & NE_FAULT_EPA_BIG) per = std::min(per * 4, EPA_BIG_FACES_MAX);
+            if (fault & NE_FAULT_EPA_BIG) per = std::min(per * 4, c->epa_faces_max);
             int r = alloc_epa_big(c, slots, per);
             if (r != NE_OK) return r;
         }
@@ -5733,6 +5751,7 @@ int ne_gpu_autotune(ne_gpu_ctx* c, int32_t steps, ne_tune_result* out) {
                     int r = restore();
                     if (r != NE_OK) { cleanup(); return r; }
                     r = ne_gpu_step(c, 1);   // warm-up: builds graphs, sizes launches
+                    if (r == NE_WARN_EPA_CAPPED) r = NE_OK;   // a trial step: only its time is used
                     if (r == NE_ERR_CAPACITY) continue;
                     if (r != NE_OK) { cleanup(); return r; }
                     r = restore();
@@ -5740,6 +5759,7 @@ int ne_gpu_autotune(ne_gpu_ctx* c, int32_t steps, ne_tune_result* out) {
                     const auto t0 = std::chrono::steady_clock::now();
                     r = ne_gpu_step(c, steps);
                     const auto t1 = std::chrono::steady_clock::now();
+                    if (r == NE_WARN_EPA_CAPPED) r = NE_OK;
                     if (r == NE_ERR_CAPACITY) continue;
                     if (r != NE_OK) { cleanup(); return r; }
                     ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;

@@ -6,8 +6,12 @@ is visible, constructing a GpuWorld raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import warnings
+import weakref
 
 import numpy as np
 
@@ -38,6 +42,27 @@ class NeGpuError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"ne_gpu error {code}: {msg}")
         self.code = code
+
+
+class EpaCappedWarning(RuntimeWarning):
+    """A step completed, but an EPA run stopped at the workspace's face cap
+    (NE_WARN_EPA_CAPPED): its contact can differ from the reference's."""
+
+
+# Contexts still open at interpreter exit are destroyed by an exit handler,
+# while the HIP runtime is still up; finalizers that run later skip the call.
+_live = weakref.WeakSet()
+
+
+def _close_all():
+    for w in list(_live):
+        try:
+            w.close()
+        except Exception:
+            pass
+
+
+atexit.register(_close_all)
 
 
 def load_library(path: str = LIB_PATH):
@@ -96,9 +121,13 @@ class GpuWorld:
     """A world resident on one MI355X."""
 
     def __init__(self, device: int = 0, pair_capacity: int = 0, contact_capacity: int = 0, cell_scale: float = 0.0,
-                 broadphase: int = 0, cells_per_body: int = 0):
+                 broadphase: int = 0, cells_per_body: int = 0, epa_face_cap: int = 0, strict: bool = True):
+        """strict: a step whose EPA run reached the face cap (NE_WARN_EPA_CAPPED)
+        raises NeGpuError; otherwise it warns (EpaCappedWarning) and counts
+        the call in `epa_capped_calls`."""
         self.lib = load_library()
-        cfg = _abi.GpuConfig(device, pair_capacity, contact_capacity, cell_scale, broadphase, cells_per_body)
+        cfg = _abi.GpuConfig(device, pair_capacity, contact_capacity, cell_scale, broadphase, cells_per_body,
+                             epa_face_cap)
         h = C.c_void_p()
         rc = self.lib.ne_gpu_create(C.byref(cfg), C.byref(h))
         if rc != 0:
@@ -106,6 +135,9 @@ class GpuWorld:
         self.h = h
         self.n = 0
         self.params = Params()
+        self.strict = strict
+        self.epa_capped_calls = 0
+        _live.add(self)
 
     def _check(self, rc):
         if rc != 0:
@@ -118,6 +150,8 @@ class GpuWorld:
             self.h = None
 
     def __del__(self):
+        if sys.is_finalizing():   # the exit handler has closed it, or the runtime is going away
+            return
         try:
             self.close()
         except Exception:
@@ -185,8 +219,8 @@ class GpuWorld:
         m = None if mu is None else np.ascontiguousarray(mu, np.float64)
         r = None if e is None else np.ascontiguousarray(e, np.float64)
         x = np.empty((self.n, 3))
-        self._check(self.lib.ne_gpu_rollout(self.h, None if v is None else v.ctypes.data, _abi.dptr(m), _abi.dptr(r),
-                                            int(nsteps), int(every), x.ctypes.data))
+        self._check_step(self.lib.ne_gpu_rollout(self.h, None if v is None else v.ctypes.data, _abi.dptr(m),
+                                                 _abi.dptr(r), int(nsteps), int(every), x.ctypes.data))
         return x
 
     def rollout_backward(self, gx, nworlds: int):
@@ -214,8 +248,20 @@ class GpuWorld:
         self._check(self.lib.ne_gpu_set_state(self.h, *[None if a is None else _abi.dptr(a) for a in arrs]))
 
     # -- step -------------------------------------------------------------
+    def _check_step(self, rc):
+        """A step or rollout status: NE_WARN_EPA_CAPPED raises in strict mode,
+        else warns and is counted."""
+        if rc == _abi.NE_WARN_EPA_CAPPED:
+            self.epa_capped_calls += 1
+            msg = self.lib.ne_gpu_last_error(self.h).decode(errors="replace")
+            if self.strict:
+                raise NeGpuError(rc, msg)
+            warnings.warn(msg, EpaCappedWarning, stacklevel=3)
+            return
+        self._check(rc)
+
     def step(self, nsteps: int = 1):
-        self._check(self.lib.ne_gpu_step(self.h, int(nsteps)))
+        self._check_step(self.lib.ne_gpu_step(self.h, int(nsteps)))
 
     def reset_warm_cache(self):
         self._check(self.lib.ne_gpu_reset_warm_cache(self.h))

@@ -384,12 +384,23 @@ def hull12(rng, r):
     return d * r * rng.uniform(0.85, 1.0, size=(12, 1))
 
 
-def mixed_pile(n: int, half=(8.0, 8.0, 8.0), seed: int = 0, ground: bool = True, spacing: float = 1.3,
-               periodic: bool = False, iterations: int = 8) -> Scene:
-    """Mixed-shape pile under gravity on a static ground box: 40% boxes
-    (half extents U[0.2,0.6]^3 scaled to the lattice), 30% spheres, 20% capsules
-    (r 0.2, hh 0.4), 10% 12-vertex hulls; jittered lattice, random
-    orientations; mu 0.5, e 0.2, g -9.81, dt 1/60."""
+def mixed_pile(n: int, half=(8.0, 8.0, 8.0), seed: int = 0, ground: bool = True, spacing: float | None = None,
+               periodic: bool = False, iterations: int = 8, sizes: str = "lattice") -> Scene:
+    """Mixed-shape pile under gravity on a static ground box: 40% boxes, 30%
+    spheres, 20% capsules, 10% 12-vertex hulls, dropped from a jittered
+    lattice with random orientations; mu 0.5, e 0.2, g -9.81, dt 1/60.
+
+    sizes="config": box half extents U[0.2,0.6]^3, sphere radii U[0.2,0.6],
+    capsules r 0.2 / half-height 0.4, hull radii U[0.2,0.6], on a lattice of
+    spacing 2.3 (the largest box's bounding diameter plus the jitter).
+    sizes="lattice" (the small test piles): the same distributions scaled to
+    a lattice of `spacing` (default 1.3), so that many bodies fit a small box.
+    """
+    if sizes not in ("lattice", "config"):
+        raise ValueError(sizes)
+    if spacing is None:
+        spacing = 2.3 if sizes == "config" else 1.3
+    lat = sizes == "lattice"
     rng = np.random.default_rng(seed)
     bodies = []
     hx, hy, hz = half
@@ -410,14 +421,17 @@ def mixed_pile(n: int, half=(8.0, 8.0, 8.0), seed: int = 0, ground: bool = True,
                 q = tuple(qs[idx])
                 k = kinds[idx]
                 if k == 0:
-                    he = tuple(rng.uniform(0.2, 0.6, size=3) * (spacing / 2.0) / 0.6 * 0.55)
+                    he = rng.uniform(0.2, 0.6, size=3)
+                    he = tuple(he * (spacing / 2.0) / 0.6 * 0.55 if lat else he)
                     bodies.append(make_box(idx, (x, y, z), q, he, 1.0))
                 elif k == 1:
-                    bodies.append(make_sphere(idx, (x, y, z), rng.uniform(0.2, 0.6) * (spacing / 2.0) / 0.6 * 0.9, 1.0))
+                    r = rng.uniform(0.2, 0.6)
+                    bodies.append(make_sphere(idx, (x, y, z), r * (spacing / 2.0) / 0.6 * 0.9 if lat else r, 1.0))
                 elif k == 2:
-                    bodies.append(make_capsule(idx, (x, y, z), q, 0.2, 0.4 * spacing / 1.3 * 0.9, 1.0))
+                    bodies.append(make_capsule(idx, (x, y, z), q, 0.2, 0.4 * spacing / 1.3 * 0.9 if lat else 0.4, 1.0))
                 else:
-                    bodies.append(make_convex(idx, (x, y, z), q, hull12(rng, rng.uniform(0.2, 0.6) * spacing / 1.3), 1.0))
+                    r = rng.uniform(0.2, 0.6)
+                    bodies.append(make_convex(idx, (x, y, z), q, hull12(rng, r * spacing / 1.3 if lat else r), 1.0))
                 idx += 1
     if ground:
         g = make_box(idx, (0.0, -hy + 0.5, 0.0), (1.0, 0.0, 0.0, 0.0), (hx - 0.5, 0.5, hz - 0.5), 1.0).make_static()

@@ -31,7 +31,20 @@ def load_scene(name):
     z = np.load(os.path.join(GOLDEN, f"scene_{name}.npz"), allow_pickle=False)
     s = _scene_from(z)
     out = {k[4:]: z[k] for k in z.files if k.startswith("out_")}
+    if "diag_every" in z.files:
+        out["diag_every"] = int(z["diag_every"][0])
     return s, int(z["steps"][0]), out
+
+
+def load_snapshots():
+    """The NES1 fixture: (scene, saves (k, bytes), save_every, load_blob,
+    load_at, after_load (m, bytes), after_load_steps)."""
+    z = np.load(os.path.join(GOLDEN, "nes1_sleeping_pile.npz"), allow_pickle=False)
+    s = _scene_from(z)
+    saves = [bytes(b) for b in z["saves"]]
+    after = [bytes(b) for b in z["after_load"]]
+    return (s, saves, int(z["save_every"][0]), bytes(z["load_blob"]), int(z["load_at"][0]), after,
+            [int(k) for k in z["after_load_steps"]])
 
 
 def load_narrow():

@@ -74,25 +74,43 @@ def features_scene(seed=0):
 
 
 def fixture_scenes():
-    """name -> (scene, steps) for the golden fixtures."""
+    """name -> (scene, steps, diagnostics every k steps or 0) for the golden
+    fixtures."""
     out = {}
-    out["gas256"] = (S.sphere_gas(256, seed=3, sigma=2.0), 30)
+    out["gas256"] = (S.sphere_gas(256, seed=3, sigma=2.0), 30, 0)
     g = S.sphere_gas(300, L=11.0, seed=8, sigma=2.5)
     g.w[:] = np.random.default_rng(1).normal(0, 1, g.w.shape)
-    out["densegas300"] = (g, 25)
-    out["pile120"] = (S.mixed_pile(120, half=(5.0, 5.0, 5.0), seed=1), 40)
+    out["densegas300"] = (g, 25, 0)
+    out["pile120"] = (S.mixed_pile(120, half=(5.0, 5.0, 5.0), seed=1), 40, 0)
     p = S.mixed_pile(80, half=(4.0, 4.0, 4.0), seed=4, ground=False, periodic=True)
     p.params.gravity = (0.0, 0.0, 0.0)
     rng = np.random.default_rng(3)
     p.v[:] = rng.normal(0, 2.0, p.v.shape)
     p.w[:] = rng.normal(0, 1.0, p.w.shape)
-    out["ppile80"] = (p, 30)
-    out["features"] = (features_scene(0), 40)
+    out["ppile80"] = (p, 30, 0)
+    out["features"] = (features_scene(0), 40, 0)
     f2 = features_scene(1)
     f2.params.open_boundary = True
     f2.params.warm_start = False
-    out["features_open_cold"] = (f2, 30)
+    out["features_open_cold"] = (f2, 30, 0)
+    out["gas4096"] = (elastic_gas_4096(), 1000, 100)
     return out
+
+
+def elastic_gas_4096():
+    """The plain-method workload as specified: 4096 spheres of radius 0.5 and
+    mass 1 uniform in a periodic cube of side 40, rejection-sampled to no
+    overlap (seed 0), velocities N(0,1) per axis with the net momentum
+    removed, no gravity, restitution 1, friction 0, dt 1/60, 8 iterations."""
+    s = S.sphere_gas(4096, L=40.0, seed=0, sigma=1.0)
+    assert s.params.dt == 1.0 / 60.0 and s.params.restitution == 1.0 and s.params.friction == 0.0
+    return s
+
+
+def snapshot_scene():
+    """The settling pile with island sleeping that the NES1 snapshot fixture
+    is written from."""
+    return S.sleeping_pile(80, seed=1, periodic=False)
 
 
 def epa_blowup_pair():

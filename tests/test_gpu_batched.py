@@ -100,6 +100,39 @@ def test_epa_blowup_pair_matches_through_the_large_workspace(gpu_available, orac
     g.close()
 
 
+def test_epa_face_cap_is_reported(gpu_available, oracle_lib):
+    """The same pair with the large workspace capped at 16,384 faces: the run
+    stops at its closest face, so the step reports NE_WARN_EPA_CAPPED (raised
+    in strict mode, a warning otherwise) and counts the truncation; with the
+    default cap the same step returns NE_OK and equals the oracle."""
+    import warnings
+    import scenes
+    from nativeengine_amd import _abi
+    from nativeengine_amd.gpu import EpaCappedWarning, GpuWorld, NeGpuError
+    s = scenes.epa_blowup_pair()
+    g = GpuWorld(epa_face_cap=16384)
+    g.set_scene(s)
+    with pytest.raises(NeGpuError) as err:
+        g.step(1)
+    assert err.value.code == _abi.NE_WARN_EPA_CAPPED
+    assert g.epa_stats()[1] >= 1
+    g.close()
+    g = GpuWorld(epa_face_cap=16384, strict=False)
+    g.set_scene(s)
+    with warnings.catch_warnings(record=True) as caught:
+        warnings.simplefilter("always")
+        g.step(1)
+    assert any(issubclass(w.category, EpaCappedWarning) for w in caught)
+    assert g.epa_capped_calls == 1 and g.epa_stats()[1] >= 1
+    assert all(np.all(np.isfinite(a)) for a in g.state())
+    g.close()
+    g = GpuWorld()
+    g.set_scene(s)
+    g.step(1)
+    assert g.epa_stats()[1] == 0 and g.epa_capped_calls == 0
+    g.close()
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_batched_ground_worlds_with_excluded_ground(gpu_available, oracle_lib, graph):
     """Gravity worlds whose first body is a static ground box spanning the

@@ -1,9 +1,11 @@
 """Snapshots in the reference's "NES1" bytes (PhysicsWorld::saveState /
 loadState, src/physics_world.cpp:22-64) written and read on the device.
 
-The expected bytes come from the reference itself: oracle/_ref builds
-src/physics_world.cpp unmodified, and its harness runs saveState / loadState
-on a copy of the reference world stepped beside the GPU one.
+The expected bytes are the reference's own: tests/golden/nes1_sleeping_pile.npz
+holds its saveState output for a settling pile with island sleeping, every 60
+steps, and the bytes of a world that loaded its snapshot at step 150 and
+stepped on (written by tests/golden/make_fixtures.py; pinned field by field
+against the oracle in tests/test_snapshot_format.py).
 """
 import struct
 import sys
@@ -11,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-import engines
+import fixtures
 from nativeengine_amd import scene
 
 pytestmark = pytest.mark.gpu
@@ -29,75 +31,58 @@ def _gpu():
     return GpuWorld()
 
 
-def _need_ref():
-    if not engines.ref_available():
-        pytest.skip("oracle/_ref/libne_ref.so (the reference build) is not present")
-
-
 def _sleeping(blob, n):
     return sum(blob[8 + RECORD * i + 176] for i in range(n))
 
 
+def _first_difference(a, b):
+    d = next(k for k in range(min(len(a), len(b))) if a[k] != b[k])
+    return f"first differing byte {d} (body {(d - 8) // RECORD}, offset {(d - 8) % RECORD})"
+
+
 def test_save_state_matches_the_reference_bytes(gpu_available):
-    """A settling pile with island sleeping: after every 60 steps the device's
-    snapshot equals the reference's saveState byte for byte, including bodies
-    asleep (zeroed masses, stores, timers, the sleeping byte)."""
-    _need_ref()
-    s = scene.sleeping_pile(80, seed=1, periodic=False)
+    """After every 60 steps of the settling pile, the device's snapshot equals
+    the reference's saveState byte for byte, including bodies asleep (zeroed
+    masses, stores, timers, the sleeping byte)."""
+    s, saves, every, _, _, _, _ = fixtures.load_snapshots()
     _say(f"snapshot: sleeping pile of {s.n} bodies")
     g = _gpu()
     g.set_scene(s)
-    r = engines.RefWorld(capture=False)
-    r.set_scene(s)
     asleep = 0
-    for t in range(4):
-        g.step(60)
-        r.step(60)
-        bg, br = g.save_state(), r.save_state()
-        assert len(br) == 8 + RECORD * s.n + 4
-        assert struct.unpack_from("<II", br, 0) == (0x4E455331, s.n)
-        if bg != br:
-            d = next(k for k in range(len(br)) if bg[k] != br[k])
-            raise AssertionError(f"after {60 * (t + 1)} steps: first differing byte {d} "
-                                 f"(body {(d - 8) // RECORD}, offset {(d - 8) % RECORD})")
-        asleep = max(asleep, _sleeping(bg, s.n))
+    for t, ref in enumerate(saves):
+        g.step(every)
+        got = g.save_state()
+        assert len(got) == len(ref) == 8 + RECORD * s.n + 4
+        assert struct.unpack_from("<II", got, 0) == (0x4E455331, s.n)
+        assert got == ref, f"after {every * (t + 1)} steps: {_first_difference(got, ref)}"
+        asleep = max(asleep, _sleeping(got, s.n))
     assert asleep > 5, asleep
     g.close()
-    r.close()
 
 
 def test_load_state_then_step_matches_the_reference(gpu_available):
-    """loadState restores every field and resets the warm cache: a world loaded
-    from a snapshot and stepped on, on the device and in the reference, stays
-    byte-identical; the reference's own bytes load on the device unchanged."""
-    _need_ref()
-    s = scene.sleeping_pile(80, seed=1, periodic=False)
+    """loadState restores every field and resets the warm cache: the
+    reference's snapshot of step 150, loaded into a device world that had
+    stepped elsewhere, reads back unchanged and steps on byte-identically to
+    the reference's loaded world."""
+    s, _, _, blob, load_at, after, after_steps = fixtures.load_snapshots()
+    assert _sleeping(blob, s.n) > 0
     src = _gpu()
     src.set_scene(s)
-    src.step(150)
-    blob = src.save_state()
-    assert _sleeping(blob, s.n) > 0
+    src.step(load_at)
+    assert src.save_state() == blob, "the device's own run reaches the same snapshot"
     g = _gpu()
     g.set_scene(s)
     g.step(7)                       # a different state and warm cache before the load
     g.load_state(blob)
     assert g.save_state() == blob
-    r = engines.RefWorld(capture=False)
-    r.set_scene(s)
-    r.step(3)
-    r.load_state(blob)
-    assert r.save_state() == blob
-    for t in range(3):
-        g.step(30)
-        r.step(30)
-        assert g.save_state() == r.save_state(), f"{30 * (t + 1)} steps after the load"
-    # the reference's bytes, loaded on the device, read back unchanged
-    g2 = _gpu()
-    g2.set_scene(s)
-    rb = r.save_state()
-    g2.load_state(rb)
-    assert g2.save_state() == rb
-    for w in (src, g, g2, r):
+    done = 0
+    for k, ref in zip(after_steps, after):
+        g.step(k - done)
+        done = k
+        got = g.save_state()
+        assert got == ref, f"{k} steps after the load: {_first_difference(got, ref)}"
+    for w in (src, g):
         w.close()
 
 

@@ -36,9 +36,16 @@ def test_oracle_matches_golden_scene(oracle_lib, name):
     s, steps, out = fixtures.load_scene(name)
     o = _oracle(s)
     counts = []
-    for _ in range(steps):
+    every = out.get("diag_every", 0)
+    diag = [np.concatenate([*o.diagnostics()[:2], [o.diagnostics()[2]]])] if every else []
+    for t in range(steps):
         o.step(1)
         counts.append(len(o.contacts()["i"]))
+        if every and (t + 1) % every == 0:
+            p, L, ke = o.diagnostics()
+            diag.append(np.concatenate([p, L, [ke]]))
+    if every:   # World::totalLinearMomentum / totalAngularMomentum / totalKinetic (world.cpp:704-729)
+        assert np.array_equal(np.array(diag), out["diag"]), f"{name}: diagnostics differ from the reference's"
     x, v, q, w = o.state()
     for k, a in zip("xvqw", (x, v, q, w)):
         assert np.array_equal(a, out[k]), f"{name}: {k} differs from the reference fixture"

@@ -45,3 +45,40 @@ def test_reference_snapshot_layout_and_round_trip(oracle_lib):
     assert r2.save_state() == b
     r.close()
     r2.close()
+
+
+def _nes1_record(blob, i):
+    """Body i of an NES1 snapshot: x, q, v, w, invMass, invInertiaBody,
+    invMassStore, invInertiaStore, sleepTimer, sleeping (physics_world.cpp:25-39)."""
+    o = 8 + 177 * i
+    d = struct.unpack_from("<22d", blob, o)
+    return dict(x=d[0:3], q=d[3:7], v=d[7:10], w=d[10:13], im=d[13], ii=d[14:17], ims=d[17], iis=d[18:21],
+                timer=d[21], sleeping=blob[o + 176])
+
+
+def test_snapshot_fixture_matches_the_oracle_state(oracle_lib):
+    """The committed NES1 fixture (the reference's own saveState bytes) holds,
+    field by field, the oracle's state after the same steps: pose, velocity,
+    live and stored masses, sleep timer and flag -- so the GPU snapshot tests
+    can compare against it with no reference build present."""
+    import fixtures
+    import numpy as np
+    s, saves, every, load_blob, load_at, after, after_steps = fixtures.load_snapshots()
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    asleep = 0
+    for k, blob in enumerate(saves):
+        o.step(every)
+        assert len(blob) == 8 + 177 * s.n + 4
+        assert struct.unpack_from("<II", blob, 0) == (0x4E455331, s.n)
+        assert struct.unpack_from("<I", blob, 8 + 177 * s.n) == (0,)
+        x, v, q, w = o.state()
+        fl, tm, im, ii = engines.body_aux(o, s.n)
+        for i in range(s.n):
+            r = _nes1_record(blob, i)
+            assert r["x"] == tuple(x[i]) and r["q"] == tuple(q[i]) and r["v"] == tuple(v[i]) and r["w"] == tuple(w[i])
+            assert r["im"] == im[i] and r["ii"] == tuple(ii[i]) and r["timer"] == tm[i]
+            assert r["sleeping"] == (1 if fl[i] & _abi.FLAG_SLEEPING else 0)
+        asleep = max(asleep, sum(blob[8 + 177 * i + 176] for i in range(s.n)))
+    assert asleep > 5
+    o.close()
