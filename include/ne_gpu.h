@@ -248,6 +248,15 @@ int ne_gpu_reset_warm_cache(ne_gpu_ctx *ctx);
  * is NE_ERR_INVALID. */
 int ne_gpu_save_state(ne_gpu_ctx *ctx, void *out, uint64_t cap, uint64_t *size);
 
+/* ne_gpu_save_state without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The same
+ * bytes as ne_gpu_save_state, packed into a static 8 MiB device buffer (one
+ * per GPU, held until the process ends) instead of a context allocation, so
+ * at most 47,393 bodies: more is NE_ERR_INVALID, with *size set.  Calls that
+ * overlap on one GPU share that buffer and race. */
+int ne_gpu_save_state_noalloc(ne_gpu_ctx *ctx, void *out, uint64_t cap, uint64_t *size);
+
 /* `PhysicsWorld::loadState()` (src/physics_world.cpp:41-64): restores every
  * body field of an NES1 snapshot and resets the warm cache.  NE_ERR_INVALID
  * (nothing changed) for a wrong magic, another body count, facade joints, or

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3643,6 +3644,7 @@ struct ne_gpu_ctx {
     uint32_t hint_p = 4096, hint_c = 4096;   // last call's pair/constraint counts (launch sizing)
     uint32_t cells_per_body = 2;
     int epa_faces_max = EPA_BIG_FACES_MAX;   // the large EPA workspace's face cap per run
+    uint8_t* scratch_dev = nullptr;          // this device's address of `scratch`
     GridDev grid_host{};
     uint32_t sgrid_cells = 0;
     // host copies from the last upload (grid plans are rebuilt from them)
@@ -3864,6 +3866,36 @@ __global__ void k_nes1_unpack(double* __restrict__ F, uint32_t ncap, uint32_t* _
         }
         flags[i] = r[176] ? (flags[i] | NE_FLAG_SLEEPING) : (flags[i] & ~NE_FLAG_SLEEPING);
     }
+}
+
+// Staging of ne_gpu_save_state_noalloc: a file-scope device array, one copy
+// per GPU, held until the process ends.  The calls on one GPU share it and
+// find the last call's bytes there; every byte a call reads back it has
+// written first, so nothing needs clearing, but calls that overlap on one GPU
+// (two threads, two contexts) race on it.  Host code never passes the array's
+// name to a kernel or a copy (on the host it is a stand-in, not a device
+// address): it uses the address looked up once per device by scratch_device().
+constexpr uint64_t NOALLOC_SCRATCH_BYTES = 8ull << 20;   // 47,393 NES1 records
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device`, which must be current; looked
+// up on the first call for that device.  Returns nullptr if the lookup fails.
+uint8_t* scratch_device(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_dev[device] = static_cast<uint8_t*>(ptr);
+    }
+    return g_scratch_dev[device];
 }
 
 double host_bounding_radius(int shape, double r, double hh, const double* he, const double* hull, uint32_t h0,
@@ -5005,6 +5037,13 @@ int ne_gpu_create(const ne_gpu_config* cfg, ne_gpu_ctx** out) {
         return NE_ERR_HIP;
     }
     memset(c->h_counters, 0, sizeof(uint32_t) * CNT_N);
+    c->scratch_dev = scratch_device(c->device);   // the device is current (hipSetDevice above)
+    if (!c->scratch_dev) {
+        free_all(c);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return NE_ERR_HIP;
+    }
     *out = c;
     return NE_OK;
 }
@@ -5334,6 +5373,38 @@ int ne_gpu_save_state(ne_gpu_ctx* c, void* out, uint64_t cap, uint64_t* size) {
                                                           c->nes1);
         HIP_TRY(c, hipGetLastError());
         HIP_TRY(c, hipMemcpyAsync(o + NES1_HEAD, c->nes1, (size_t)recs, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    const uint32_t njoints = 0;   // the facade's general joints are not on this path
+    std::memcpy(o + NES1_HEAD + recs, &njoints, 4);
+    return NE_OK;
+}
+
+int ne_gpu_save_state_noalloc(ne_gpu_ctx* c, void* out, uint64_t cap, uint64_t* size) {
+    if (!c || !size) return NE_ERR_INVALID;
+    const uint32_t n = c->n;
+    const uint64_t recs = NES1_RECORD * (uint64_t)n;
+    const uint64_t need = NES1_HEAD + recs + 4;
+    *size = need;
+    if (recs > NOALLOC_SCRATCH_BYTES) {
+        char buf[200];
+        snprintf(buf, sizeof buf, "ne_gpu_save_state_noalloc: %u bodies need %llu bytes of staging; its static buffer "
+                 "holds %llu (%llu bodies): use ne_gpu_save_state", n, (unsigned long long)recs,
+                 (unsigned long long)NOALLOC_SCRATCH_BYTES, (unsigned long long)(NOALLOC_SCRATCH_BYTES / NES1_RECORD));
+        return set_err(c, NE_ERR_INVALID, buf);
+    }
+    if (!out) return NE_OK;
+    if (cap < need) return set_err(c, NE_ERR_INVALID, "ne_gpu_save_state_noalloc: the buffer is smaller than the snapshot");
+    if (!c->scratch_dev) return set_err(c, NE_ERR_HIP, "ne_gpu_save_state_noalloc: no device address for the staging");
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    uint8_t* o = static_cast<uint8_t*>(out);
+    const uint32_t head[2] = {NES1_MAGIC, n};
+    std::memcpy(o, head, sizeof(head));
+    if (n) {
+        k_nes1_pack<<<blocks_for(n), TPB, 0, c->stream>>>(c->dfield, c->ncap, c->ufield + (size_t)U_FLAGS * c->ncap, n,
+                                                          c->scratch_dev);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(o + NES1_HEAD, c->scratch_dev, (size_t)recs, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
     }
     const uint32_t njoints = 0;   // the facade's general joints are not on this path

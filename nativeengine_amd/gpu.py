@@ -31,7 +31,7 @@ EXPORTS = [
     "ne_gpu_fp_probe", "ne_gpu_time_kernel", "ne_gpu_kernel_time", "ne_gpu_export_warm_cache",
     "ne_gpu_import_warm_cache", "ne_gpu_set_worlds", "ne_gpu_use_graph", "ne_gpu_autotune", "ne_gpu_epa_stats",
     "ne_gpu_rollout", "ne_gpu_rollout_backward", "ne_gpu_set_joints", "ne_gpu_set_meshes", "ne_gpu_mesh_bvh",
-    "ne_gpu_get_sleep_store", "ne_gpu_save_state", "ne_gpu_load_state",
+    "ne_gpu_get_sleep_store", "ne_gpu_save_state", "ne_gpu_load_state", "ne_gpu_save_state_noalloc",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
@@ -90,6 +90,7 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_reset_warm_cache.argtypes = [vp]
     lib.ne_gpu_save_state.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
     lib.ne_gpu_load_state.argtypes = [vp, vp, C.c_uint64]
+    lib.ne_gpu_save_state_noalloc.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
     lib.ne_gpu_get_contacts.argtypes = [vp, C.c_uint32, u32p, u32p, u32p, dp, dp, dp, dp]
     lib.ne_gpu_get_pairs.argtypes = [vp, C.c_uint32, u32p, u32p, u32p]
     lib.ne_gpu_diagnostics.argtypes = [vp, dp, dp, dp]
@@ -304,6 +305,15 @@ class GpuWorld:
         self._check(self.lib.ne_gpu_save_state(self.h, None, 0, C.byref(size)))
         buf = C.create_string_buffer(size.value)
         self._check(self.lib.ne_gpu_save_state(self.h, buf, size.value, C.byref(size)))
+        return buf.raw[: size.value]
+
+    def save_state_noalloc(self) -> bytes:
+        """save_state through ne_gpu_save_state_noalloc: the same bytes, staged
+        in the library's static device buffer (no device allocation)."""
+        size = C.c_uint64(0)
+        self._check(self.lib.ne_gpu_save_state_noalloc(self.h, None, 0, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        self._check(self.lib.ne_gpu_save_state_noalloc(self.h, buf, size.value, C.byref(size)))
         return buf.raw[: size.value]
 
     def load_state(self, blob: bytes):
