@@ -15,6 +15,8 @@
 //                                                       (src/world.hpp:135-136,
 //                                                        src/world.cpp:552)
 //   - the factories' Body fields                        (src/body.hpp:41-129)
+//   - the factories themselves, for their mass and inertia bits
+//                                                       (src/body.hpp:206-345)
 //   - `detectContact` / `boxBoxManifold`                (src/detect.hpp:33,
 //                                                        src/manifold.hpp:90)
 //   - `totalLinearMomentum/AngularMomentum/Kinetic`     (src/world.cpp:704-729)
@@ -351,6 +353,38 @@ int ne_ref_load_state(void* h, const void* data, uint64_t size) {
     const char* p = static_cast<const char*>(data);
     if (!pw.loadState(std::vector<char>(p, p + size))) return NE_ERR_INVALID;
     r->w = pw.world();
+    return NE_OK;
+}
+
+// The reference's factories (src/body.hpp:206-345) on the given sizes:
+// sphere dims = (r), cylinder (r, height), box (hx, hy, hz), capsule
+// (r, halfHeight), convex the vertices (dims unused), plane dims = the normal.
+// Writes invMass, invInertiaBody (3), halfExtents (3), radius, halfHeight.
+int ne_ref_make_body(int32_t shape, const double* dims, uint32_t nverts, const double* verts, double density,
+                     double* inv_mass, double* inv_inertia, double* half_extents, double* radius,
+                     double* half_height) {
+    const ne::V3 x{0.0, 0.0, 0.0};
+    const ne::Q q{1.0, 0.0, 0.0, 0.0};
+    ne::Body b;
+    switch (shape) {
+        case NE_SHAPE_SPHERE: b = ne::makeSphere(0, x, dims[0], density); break;
+        case NE_SHAPE_CYLINDER: b = ne::makeCylinder(0, x, q, dims[0], dims[1], density); break;
+        case NE_SHAPE_BOX: b = ne::makeBox(0, x, q, ne::V3{dims[0], dims[1], dims[2]}, density); break;
+        case NE_SHAPE_CAPSULE: b = ne::makeCapsule(0, x, q, dims[0], dims[1], density); break;
+        case NE_SHAPE_CONVEX: {
+            std::vector<ne::V3> vs;
+            for (uint32_t k = 0; k < nverts; ++k) vs.push_back(v3(verts, k));
+            b = ne::makeConvex(0, x, q, vs, density);
+            break;
+        }
+        case NE_SHAPE_PLANE: b = ne::makePlane(0, x, ne::V3{dims[0], dims[1], dims[2]}); break;
+        default: return NE_ERR_INVALID;
+    }
+    *inv_mass = b.invMass;
+    inv_inertia[0] = b.invInertiaBody.x; inv_inertia[1] = b.invInertiaBody.y; inv_inertia[2] = b.invInertiaBody.z;
+    half_extents[0] = b.halfExtents.x; half_extents[1] = b.halfExtents.y; half_extents[2] = b.halfExtents.z;
+    *radius = b.radius;
+    *half_height = b.halfHeight;
     return NE_OK;
 }
 
