@@ -7,33 +7,39 @@ Headline (`value`): the large terrain scene, 1,048,576 bodies in a periodic
 box columns of 8 (70%), loose spheres (25%), 16-link rope chains of rigid
 distance joints (5%); friction 0.6, restitution 0.1, island sleeping on,
 dt 1/60, 10 iterations, FP64, seed 5.  Inputs are synthetic and resident in
-HBM before the timed region; a step is one full `World::step()`.  The timed
-region is 300 single-step calls after 10 warm-up steps.  `cpu_baseline`
-times the C restatement on one host core over the contact phase (one step at
-steps 10/100/200/300 of the 65,536-body instance of the same scene, from the
-GPU's state there); `cpu_start_phase` on the 1M scene's first two steps.
+HBM before the first step; a step is one full `World::step()`.  The scene's
+whole 300-step run is stepped, one synchronised call per step, every call
+timed.  `value` is the body-steps/s of --steps K of those steps spread evenly
+over the run after its first --warmup W steps (all 300 by default);
+`whole_run` is the whole run's.  `cpu_baseline` times the C restatement on
+one host core for one step at steps 10/100/200/300 of the 65,536-body
+instance of the same scene, from the GPU's state there, and
+`speedup_vs_cpu` compares the GPU's own steps 10/100/200/300.
 
 Other lines in the same JSON object:
   gas      the 1,048,576-sphere periodic elastic gas (r 0.5, L 253.98, e 1,
-           mu 0, no gravity, 8 iterations, seed 0), the plain-method 1M point,
-           with the reference itself (oracle/_ref) timed on the host;
+           mu 0, no gravity, 8 iterations, seed 0), the plain-method 1M point;
+  gas4096  the 4096-sphere gas of side 40 over 1000 steps, with its momentum
+           and energy drift beside the reference's own (golden fixture);
   pile     the 65,536-body mixed-shape pile (40% boxes, 30% spheres, 20%
-           capsules, 10% hulls) on a ground box inside 100 x 200 x 100 walls,
-           10 iterations, 600 steps;
+           capsules, 10% hulls, sizes as specified) on a ground box inside
+           100 x 200 x 100 walls, 10 iterations, its whole 600-step run, with
+           the C restatement at steps 10/150/300/450/600;
   batched  1024 periodic worlds of log-uniform size in [64, 4096] stepped in
-           one set of launches (HIP-graph replay, autotuned) against looping
-           the plain single-world step over a sample of the same worlds;
+           one set of launches (HIP-graph replay, autotuned) over their whole
+           300-step run, against looping the plain single-world step over a
+           sample of the same worlds at three points of the run;
   adjoint  256 worlds of 256 spheres + 256 boxes on a ground box, 120 steps,
            checkpoint every 16: backward time / forward time.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-large] [--no-gas] [--no-pile]
-                    [--no-batched] [--no-adjoint] [--traffic-probe]
+                    [--no-config1] [--no-batched] [--no-adjoint] [--traffic-probe]
 
-Prints one JSON line.  `roofline` is measured live with HIP events on the
-library's own stream around every launch of the dominant stage inside the
-timed region, credited with the algorithmic bytes of every timed step;
-`traffic` is the HBM bytes per launch of that stage from the committed
-rocprofv3 PMC summary.
+Prints one JSON line.  `roofline` is the dominant stage of the run: every
+stage is timed with HIP events on the library's own stream in every 9th
+step of the run, in rotation, and credited with the algorithmic bytes of
+those steps; `traffic` is the HBM bytes per launch of that stage from the
+committed rocprofv3 PMC summary.
 """
 from __future__ import annotations
 
@@ -146,126 +152,134 @@ def stage_bytes(name: str, iterations: int):
     return table.get(name)
 
 
-def scene_bench(s, steps: int, warmup: int, cpu_steps: int, cpu_what: str, label: str, max_seconds: float = 150.0,
-                traffic_profile: str | None = None):
-    """Body-steps/s of one resident scene over `steps` single-step calls after
-    `warmup` untimed steps, with the dominant stage's HIP-event time and the
-    algorithmic bytes it moved (units counted from every timed step)."""
-    from nativeengine_amd.gpu import GpuWorld, KERNELS
+STAGES = ["integrate", "cell_count", "cell_fill", "cand_count", "cand_write", "narrow", "cons_build", "solve",
+          "boundary"]
+
+
+def timed_steps(run: int, warmup: int, k: int):
+    """The steps (1-based) whose times make `value`: k steps spread evenly
+    over the run after its first `warmup` steps (midpoints of k equal
+    strata), or all of them when k covers the rest."""
+    rest = run - warmup
+    if k >= rest:
+        return list(range(warmup + 1, run + 1))
+    return sorted({warmup + 1 + int((i + 0.5) * rest / k) for i in range(k)})
+
+
+def scene_run(s, run: int, warmup: int, k: int, label: str, marks=()):
+    """One resident scene stepped through its whole run of `run` single-step
+    calls, each bracketed by a stream synchronisation (the call returns after
+    it) and a read of the step's counters.  Every step's wall time is kept;
+    `value` comes from the k timed steps spread over the run.  Stage times
+    come from HIP events on the library's stream, one stage per step in
+    rotation over the whole run, with that step's algorithmic units, so the
+    dominant stage and its roofline are measured over the same run.  At the
+    steps in `marks` the GPU's state is saved for the CPU's phase-matched
+    samples."""
+    import tempfile
     import numpy as np
-    cpu = None
-    if cpu_steps > 0:
-        log(f"{label}: cpu baseline, {cpu_steps} steps ...")
-        cpu = cpu_baseline(s, cpu_steps, cpu_what)
-        if cpu:
-            log(f"{label}: cpu {cpu['kind']} {cpu['value']:.4g} body-steps/s ({cpu['seconds']:.1f}s)")
+    from nativeengine_amd.gpu import GpuWorld
     iters = int(s.params.solver_iterations)
     t0 = time.perf_counter()
-    g = GpuWorld()
+    g = GpuWorld(strict=False)
     g.set_scene(s)
     log(f"{label}: upload {time.perf_counter() - t0:.1f}s")
-    t0 = time.perf_counter()
-    for _ in range(warmup):
-        g.step(1)
-    log(f"{label}: warm-up {warmup} steps {time.perf_counter() - t0:.2f}s")
-    stage_ms = {}
-    for name in KERNELS:
-        if name == "step":
-            continue
+    timed = set(timed_steps(run, warmup, k))
+    step_ms = []
+    stage = {name: {"ms": 0.0, "steps": 0, "units": {"body": 0, "pair": 0, "mesh_triangle": 0, "contact": 0}}
+             for name in STAGES}
+    units_all = {"body": 0, "pair": 0, "mesh_triangle": 0, "contact": 0}
+    capped_steps = 0
+    tmp = tempfile.mkdtemp(prefix="bench_states_")
+    states = {}
+    tstart = last_log = time.perf_counter()
+    for t in range(1, run + 1):
+        name = STAGES[(t - 1) % len(STAGES)]
         g.time_kernel(name)
-        g.step(2)
-        ms, k = g.kernel_time()
-        stage_ms[name] = ms / max(k, 1)
-    g.time_kernel(None)
-    ranked = sorted(((v, k) for k, v in stage_ms.items() if stage_bytes(k, iters)), reverse=True)
-    dom = ranked[0][1]
-    log(f"{label}: stage ms/launch " + ", ".join(f"{k}={v:.4f}" for k, v in sorted(stage_ms.items(), key=lambda t: -t[1])))
-    g.time_kernel(dom)
-    units = {"body": 0, "pair": 0, "mesh_triangle": 0, "contact": 0}
-    per_step_ms = []
-    t0 = time.perf_counter()
-    last_log = t0
-    done = 0
-    for _ in range(steps):
+        calls0 = g.epa_capped_calls
         ts = time.perf_counter()
         g.step(1)
         c = g.counters()
         now = time.perf_counter()
-        per_step_ms.append((now - ts) * 1e3)
-        done += 1
-        units["body"] += s.n
-        units["pair"] += c["candidate_pairs"]
-        units["mesh_triangle"] += c["mesh_triangles"]
-        units["contact"] += c["constraints"]
+        step_ms.append((now - ts) * 1e3)
+        if g.epa_capped_calls != calls0:
+            capped_steps += 1
+        ms, launches = g.kernel_time()
+        u = {"body": s.n, "pair": c["candidate_pairs"], "mesh_triangle": c["mesh_triangles"],
+             "contact": c["constraints"]}
+        for kk in units_all:
+            units_all[kk] += u[kk]
+        if launches:
+            st = stage[name]
+            st["ms"] += ms
+            st["steps"] += 1
+            for kk in st["units"]:
+                st["units"][kk] += u[kk]
+        if t in marks:
+            path = os.path.join(tmp, f"state_{t}.npz")
+            _save_state(g, s, path)
+            states[t] = path
         if now - last_log > 10.0:
-            log(f"{label}: {done} steps, {(now - t0) * 1e3 / done:.2f} ms/step, last {per_step_ms[-1]:.2f} ms, "
+            log(f"{label}: step {t}/{run}, {(now - tstart) * 1e3 / t:.2f} ms/step so far, last {step_ms[-1]:.2f} ms, "
                 f"{c['constraints']} constraints, largest island {c['max_island']}")
             last_log = now
-        if now - t0 > max_seconds:
-            log(f"{label}: stopping after {done} steps ({max_seconds:.0f} s cap)")
-            break
-    t1 = time.perf_counter()
-    steps = done
-    dom_ms, dom_launches = g.kernel_time()
     g.time_kernel(None)
     last = g.counters()
     reruns, trunc = g.epa_stats()
-    stage_ms_end = {}   # the same stages at the end state (untimed steps after the timed region)
-    for name in KERNELS:
-        if name == "step":
-            continue
-        g.time_kernel(name)
-        g.step(2)
-        ms, k = g.kernel_time()
-        stage_ms_end[name] = ms / max(k, 1)
-    g.time_kernel(None)
-    log(f"{label}: end-state stage ms/launch " +
-        ", ".join(f"{k}={v:.3f}" for k, v in sorted(stage_ms_end.items(), key=lambda t: -t[1])))
     fl = g.body_aux()[0] if s.params.sleep_enabled else None
     g.close()
-    value = s.n * steps / (t1 - t0)
+    sel = sorted(timed)
+    timed_ms = [step_ms[t - 1] for t in sel]
+    value = s.n * len(sel) / (sum(timed_ms) * 1e-3)
+    mean_ms = {kk: v["ms"] / max(v["steps"], 1) for kk, v in stage.items()}
+    ranked = sorted(((v, kk) for kk, v in mean_ms.items() if stage_bytes(kk, iters)), reverse=True)
+    dom = ranked[0][1]
+    st = stage[dom]
     per = stage_bytes(dom, iters)
-    bytes_total = sum(per[k] * units[k] for k in per)
-    avg_launch_ms = dom_ms / max(dom_launches, 1)
-    achieved = bytes_total / max(dom_launches, 1) / (avg_launch_ms * 1e-3) / 1e9
+    bytes_total = sum(per[kk] * st["units"][kk] for kk in per)
+    avg_launch_ms = st["ms"] / max(st["steps"], 1)
+    achieved = bytes_total / (st["ms"] * 1e-3) / 1e9 if st["ms"] > 0 else 0.0
     roofline = {
         "bound": "hbm",
         "kernel": dom,
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(dom, traffic_profile) if traffic_profile else None,
-        "traffic_source": f"profiles/{traffic_profile} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a "
-                          f"kernel of known traffic)" if traffic_profile else None,
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": None,
         "bytes_per_unit": per,
-        "units_per_launch": {k: round(units[k] / max(dom_launches, 1), 1) for k in per},
+        "units_per_launch": {kk: round(st["units"][kk] / max(st["steps"], 1), 1) for kk in per},
         "avg_launch_us": round(avg_launch_ms * 1e3, 3),
+        "launches_timed": st["steps"],
+        "window": f"the stage's span on the library's stream in every {len(STAGES)}th step of the {run}-step run "
+                  f"({st['steps']} steps), one launch = the stage's span in one step",
     }
-    pct = np.percentile(per_step_ms, [50, 90, 100])
+    pct = np.percentile(step_ms, [50, 90, 100])
     out = {
         "value": value,
         "unit": "body-steps/s",
-        "steps": steps,
+        "run_steps": run,
+        "steps": len(sel),
         "warmup": warmup,
-        "ms_per_step": (t1 - t0) * 1e3 / steps,
-        "step_ms_p50_p90_max": [round(float(x), 3) for x in pct],
+        "timed_steps": sel if len(sel) < 40 else f"steps {sel[0]}..{sel[-1]} (all)",
+        "ms_per_step": sum(timed_ms) / len(sel),
+        "whole_run": {"value": s.n * run / (sum(step_ms) * 1e-3), "ms_per_step": sum(step_ms) / run,
+                      "step_ms_p50_p90_max": [round(float(x), 3) for x in pct]},
         "bodies": s.n,
         "roofline": roofline,
-        "cpu_baseline": cpu,
-        "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
-        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-        "stage_ms_end": {k: round(v, 4) for k, v in stage_ms_end.items()},
-        "per_step_mean": {"candidate_pairs": units["pair"] / steps, "mesh_triangles": units["mesh_triangle"] / steps,
-                          "constraints": units["contact"] / steps},
+        "stage_ms_mean": {kk: round(v, 4) for kk, v in mean_ms.items()},
+        "per_step_mean": {"candidate_pairs": units_all["pair"] / run, "mesh_triangles": units_all["mesh_triangle"] / run,
+                          "constraints": units_all["contact"] / run},
         "last_step": {"candidate_pairs": last["candidate_pairs"], "constraints": last["constraints"],
                       "islands": last["islands"], "largest_island": last["max_island"]},
-        "epa_large_workspace_reruns": reruns,
-        "epa_truncated": trunc,
+        "epa_large_workspace_reruns_last_step": reruns,
+        "epa_truncated_last_step": trunc,
+        "steps_with_epa_at_face_cap": capped_steps,
+        "step_ms_at": {t: round(step_ms[t - 1], 3) for t in marks},
     }
     if fl is not None:
         out["sleeping_bodies_at_end"] = int(np.sum((fl & 8) != 0))
-    return out
+    return out, states, tmp
 
 
 def _save_state(g, s, path):
@@ -278,30 +292,29 @@ def _save_state(g, s, path):
     np.savez(path, x=x, v=v, q=q, w=w, flags=fl, timer=tm, im=im, ii=ii, ims=ims, iis=iis)
 
 
-def _cpu_one_step(scene_args, state_path, timeout):
-    """Seconds the C restatement takes for one step from a saved state, in a
-    child process (a single step can hit the reference's non-converging EPA,
-    which runs for minutes): None if it does not finish within `timeout`."""
-    import subprocess
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-sample", state_path, "--cpu-scene", scene_args]
-    try:
-        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
-    except subprocess.TimeoutExpired:
-        return None
-    if r.returncode != 0:
-        log(r.stderr[-2000:])
-        return None
-    return float(r.stdout.strip().splitlines()[-1])
-
-
-def cpu_sample_main(state_path, scene_args):
-    """Child process: rebuild the scene, load the saved state, time one step."""
-    import numpy as np
+def _build_scene(spec: str):
+    """A bench scene from its spec string "large,<n>,<seed>" or "pile,<n>,<seed>"."""
     from nativeengine_amd import scene as scn
+    kind, n, seed = spec.split(",")
+    if kind == "large":
+        return scn.large_scene(int(n), seed=int(seed))
+    return pile_scene(int(n), int(seed))
+
+
+def pile_scene(n: int, seed: int = 0):
+    """The mixed-shape pile as specified: n dynamic bodies on a static ground
+    box inside 100 x 200 x 100 reflective walls, 10 iterations."""
+    from nativeengine_amd import scene as scn
+    return scn.mixed_pile(n, half=(50.0, 100.0, 50.0), seed=seed, iterations=10, sizes="config")
+
+
+def cpu_sample_main(state_path, spec):
+    """Child process: rebuild the scene, load the saved state, time one step
+    of the C restatement."""
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import engines
-    n, seed = (int(t) for t in scene_args.split(","))
-    s = scn.large_scene(n, seed=seed)
+    s = _build_scene(spec)
     d = np.load(state_path)
     s.x, s.v, s.q, s.w = d["x"], d["v"], d["q"], d["w"]
     s.flags = d["flags"].astype(np.uint32)
@@ -314,153 +327,256 @@ def cpu_sample_main(state_path, scene_args):
     print(time.perf_counter() - t0)
 
 
-def contact_phase_cpu(n: int, marks, timeout: float):
-    """The C restatement (single thread) on the n-body instance of the large
-    scene at several points of its run, from the GPU's state there (the GPU's
-    states are the restatement's, bit for bit): one step each."""
-    import tempfile
-    from nativeengine_amd import scene as scn
-    from nativeengine_amd.gpu import GpuWorld
-    s = scn.large_scene(n, seed=5)
-    g = GpuWorld()
-    g.set_scene(s)
-    done, times = 0, []
-    with tempfile.TemporaryDirectory() as tmp:
-        for m in marks:
-            g.step(m - done)
-            done = m
-            path = os.path.join(tmp, f"state_{m}.npz")
-            _save_state(g, s, path)
-            sec = _cpu_one_step(f"{n},5", path, timeout)
-            log(f"contact-phase cpu: step {m}: " + (f"{sec:.2f}s" if sec is not None else f"> {timeout:.0f}s (stopped)"))
-            times.append(sec)
-    g.close()
-    finished = [t for t in times if t is not None]
+def _cpu_one_step(spec, state_path, timeout):
+    """Seconds the C restatement takes for one step from a saved state, in a
+    child process (a single step can meet one of the reference's
+    non-converging EPA runs, which take minutes): None past `timeout`."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-sample", state_path, "--cpu-scene", spec]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return None
+    if r.returncode != 0:
+        log(r.stderr[-2000:])
+        return None
+    return float(r.stdout.strip().splitlines()[-1])
+
+
+def phase_cpu(spec: str, n: int, states: dict, timeout: float, what: str):
+    """The C restatement (one thread) for one step from each saved GPU state
+    (the GPU's states are the restatement's, bit for bit)."""
+    times = {}
+    for m in sorted(states):
+        sec = _cpu_one_step(spec, states[m], timeout)
+        log(f"cpu ({what}): step {m}: " + (f"{sec:.2f}s" if sec is not None else f"> {timeout:.0f}s (stopped)"))
+        times[m] = sec
     # a step that did not finish counts as the timeout (the CPU figure is then an upper bound)
-    total = sum(t if t is not None else timeout for t in times)
+    total = sum(t if t is not None else timeout for t in times.values())
     return {
         "value": n * len(times) / total,
         "unit": "body-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"one step at each of steps {list(marks)} of the {n}-body instance of the same scene (same density, "
-                  f"side {400.0 * (n / float(1 << 20)) ** 0.5:.0f}), each from the GPU's state at that step, single "
-                  f"thread, {cpu_model()}, nproc={os.cpu_count()}",
+        "sample": f"one step at each of steps {sorted(times)} of {what}, each from the GPU's state at that step, "
+                  f"single thread, {cpu_model()}, nproc={os.cpu_count()}",
         "seconds": total,
-        "steps_over_timeout": len(times) - len(finished),
-        "upper_bound": len(finished) < len(times),
+        "seconds_at": {m: (round(t, 3) if t is not None else None) for m, t in times.items()},
+        "steps_over_timeout": sum(1 for t in times.values() if t is None),
+        "upper_bound": any(t is None for t in times.values()),
     }
+
+
+def _phase_matched(r, cpu, n_gpu, n_cpu):
+    """GPU over CPU body-steps/s on the same steps of the run."""
+    gpu_ms = sum(r["step_ms_at"][m] for m in cpu["seconds_at"])
+    gpu_rate = n_gpu * len(cpu["seconds_at"]) / (gpu_ms * 1e-3)
+    return {"gpu_value": gpu_rate, "cpu_value": cpu["value"], "ratio": round(gpu_rate / cpu["value"], 1),
+            "steps": sorted(cpu["seconds_at"]), "gpu_bodies": n_gpu, "cpu_bodies": n_cpu}
 
 
 def large_bench(args):
     """The large terrain scene at 1,048,576 bodies: the headline line."""
     from nativeengine_amd import scene as scn
+    from nativeengine_amd.gpu import GpuWorld
     t0 = time.perf_counter()
     s = scn.large_scene(args.large_n, seed=5)
     h = s.params.half
     log(f"large scene: {s.n} bodies, {len(s.joints)} joints, {len(s.meshes[0]['tris'])} triangles, "
         f"built in {time.perf_counter() - t0:.1f}s")
-    r = scene_bench(s, args.large_steps, args.large_warmup, 0 if args.no_cpu else args.cpu_steps,
-                    f"the same 1M-body scene from its initial state (bodies falling, no contacts yet: the CPU's "
-                    f"fastest phase of the run)", "large", traffic_profile="large_terrain_1M_traffic.md")
+    marks = [10, 100, 200, 300]
+    r, _, _ = scene_run(s, 300, args.warmup, args.steps, "large", marks=marks)
+    r["traffic_profile"] = "profiles/large_terrain_1M_traffic.md"
+    r["roofline"]["traffic"] = measured_traffic(r["roofline"]["kernel"], "large_terrain_1M_traffic.md")
     if not args.no_cpu:
-        r["cpu_start_phase"] = r["cpu_baseline"]
-        cp = contact_phase_cpu(args.cpu_n, [10, 100, 200, 300], args.cpu_timeout)
-        r["cpu_contact_phases"] = cp
-        r["cpu_baseline"] = cp
-        r["speedup_vs_cpu"] = round(r["value"] / cp["value"], 1)
-        log(f"large: cpu over the run {cp['value']:.4g} body-steps/s, GPU/CPU {r['speedup_vs_cpu']}")
+        # the CPU's phase samples on the 65,536-body instance of the same scene
+        small = scn.large_scene(args.cpu_n, seed=5)
+        g = GpuWorld(strict=False)
+        g.set_scene(small)
+        states, done = {}, 0
+        import tempfile
+        tmp = tempfile.mkdtemp(prefix="bench_cpu_")
+        for m in marks:
+            g.step(m - done)
+            done = m
+            path = os.path.join(tmp, f"state_{m}.npz")
+            _save_state(g, small, path)
+            states[m] = path
+        g.close()
+        cpu = phase_cpu(f"large,{args.cpu_n},5", args.cpu_n, states, args.cpu_timeout,
+                        f"the {args.cpu_n}-body instance of the same scene (same density, side "
+                        f"{400.0 * (args.cpu_n / float(1 << 20)) ** 0.5:.0f})")
+        r["cpu_baseline"] = cpu
+        r["phase_matched"] = _phase_matched(r, cpu, s.n, args.cpu_n)
+        r["speedup_vs_cpu"] = r["phase_matched"]["ratio"]
+        log(f"large: cpu {cpu['value']:.4g} body-steps/s; GPU/CPU at the same steps {r['speedup_vs_cpu']}")
+    else:
+        r["cpu_baseline"], r["speedup_vs_cpu"] = None, None
     r["workload"] = (f"large terrain scene: {s.n} bodies (mesh + {int((s.shape == 2).sum())} boxes in columns of 8 + "
                      f"{int((s.shape == 0).sum())} spheres, {len(s.joints)} rope joints in 16-link ropes), periodic "
                      f"{2 * h[0]:.0f}x{2 * h[1]:.0f}x{2 * h[2]:.0f}, {len(s.meshes[0]['verts'])}-vertex value-noise "
                      f"heightfield amplitude 8, mu 0.6, e 0.1, sleeping on, dt 1/60, "
-                     f"{s.params.solver_iterations} iterations, seed 5")
+                     f"{s.params.solver_iterations} iterations, 300 steps, seed 5")
     return r
 
 
 def pile_bench(args):
-    """The mixed-shape pile under gravity (65,536 bodies, reflective walls)."""
-    from nativeengine_amd import scene as scn
+    """The mixed-shape pile under gravity (65,536 bodies, reflective walls),
+    its whole 600-step run, with the C restatement at five steps of it."""
     t0 = time.perf_counter()
-    s = scn.mixed_pile(args.pile_n, half=(50.0, 100.0, 50.0), seed=0, iterations=10)
+    s = pile_scene(args.pile_n, 0)
     log(f"pile: {s.n} bodies, built in {time.perf_counter() - t0:.1f}s")
-    r = scene_bench(s, args.pile_steps, args.pile_warmup, 0, "", "pile")
-    r["workload"] = (f"mixed-shape pile: {s.n - 1} dynamic bodies (40% boxes, 30% spheres, 20% capsules, 10% "
-                     f"12-vertex hulls) on a static ground box, reflective 100x200x100 box, g -9.81, mu 0.5, e 0.2, "
-                     f"dt 1/60, 10 iterations, warm start, seed 0")
+    marks = [10, 150, 300, 450, 600]
+    r, states, tmp = scene_run(s, 600, 0, args.pile_steps, "pile", marks=marks if not args.no_cpu else ())
+    if not args.no_cpu:
+        cpu = phase_cpu(f"pile,{args.pile_n},0", s.n, states, args.cpu_timeout, "the same pile")
+        r["cpu_baseline"] = cpu
+        r["phase_matched"] = _phase_matched(r, cpu, s.n, s.n)
+        r["speedup_vs_cpu"] = r["phase_matched"]["ratio"]
+        log(f"pile: cpu {cpu['value']:.4g} body-steps/s; GPU/CPU at the same steps {r['speedup_vs_cpu']}")
+    r["workload"] = (f"mixed-shape pile: {s.n - 1} dynamic bodies (40% boxes half extents U[0.2,0.6]^3, 30% spheres "
+                     f"r U[0.2,0.6], 20% capsules r 0.2 half-height 0.4, 10% 12-vertex hulls r U[0.2,0.6]) on a "
+                     f"jittered lattice of spacing 2.3 with random orientations, on a static ground box, reflective "
+                     f"100x200x100 box, g -9.81, mu 0.5, e 0.2, dt 1/60, 10 iterations, warm start, 600 steps, seed 0")
     return r
 
 
-def batched_bench(worlds: int, steps: int, warmup: int, loop_worlds: int, loop_steps: int):
-    """Batched world-steps/s against looping the plain step over `loop_worlds`
-    of the same worlds (their own contexts, same state and warm cache)."""
+def config1_bench(args):
+    """The elastic gas as specified for the CPU reference: 4096 spheres,
+    side 40, 1000 steps; ms per step and the conservation of momentum and
+    energy, beside the reference's own drift from its golden fixture."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import fixtures
+    from nativeengine_amd.gpu import GpuWorld
+    s, steps, out = fixtures.load_scene("gas4096")
+    g = GpuWorld()
+    g.set_scene(s)
+    p0, L0, e0 = g.diagnostics()
+    g.step(1)
+    g.set_scene(s)          # the first call loaded the kernels; start again from the initial state
+    g.reset_warm_cache()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step(1)
+    t1 = time.perf_counter()
+    p1, L1, e1 = g.diagnostics()
+    x = g.state()[0]
+    g.close()
+    ref = out["diag"]
+    same = bool(np.array_equal(np.concatenate([p1, L1, [e1]]), ref[-1]) and np.array_equal(x, out["x"]))
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(s, 200, "the same gas (first 200 of its 1000 steps)")
+    return {
+        "value": s.n * steps / (t1 - t0),
+        "unit": "body-steps/s",
+        "ms_per_step": (t1 - t0) * 1e3 / steps,
+        "steps": steps,
+        "bodies": s.n,
+        "momentum_drift": float(np.max(np.abs(p1 - p0))),
+        "energy_drift_rel": float((e1 - e0) / e0),
+        "reference_energy_drift_rel": float((ref[-1][6] - ref[0][6]) / ref[0][6]),
+        "reference_momentum_drift": float(np.max(np.abs(ref[-1][:3] - ref[0][:3]))),
+        "bitwise_equal_to_reference": same,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(s.n * steps / (t1 - t0) / cpu["value"], 1) if cpu else None,
+        "note": "one synchronised call per step; 4096 bodies leave the GPU latency-bound, so no roofline is given",
+        "workload": "periodic elastic sphere gas: 4096 spheres r 0.5 m 1, side 40, rejection-sampled (seed 0), "
+                    "v N(0,1) with the net momentum removed, e 1, mu 0, g 0, dt 1/60, 8 iterations, 1000 steps",
+    }
+
+
+def _loop_sample(g, b, sample, loop_steps):
+    """ms per world-step of the plain single-world step looped over the
+    sampled worlds, each in its own context from the batch's current state
+    and warm cache."""
+    import numpy as np
+    from nativeengine_amd.gpu import GpuWorld
+    x, v, q, w = g.state()
+    keys, imp = g.export_warm_cache()
+    ki, kj = (keys >> np.uint64(32)).astype(np.int64), (keys & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    ctxs = []
+    for wi in sample:
+        a, e = int(b.offsets[wi]), int(b.offsets[wi + 1])
+        ws = b.world(int(wi))
+        ws.x, ws.v, ws.q, ws.w = x[a:e].copy(), v[a:e].copy(), q[a:e].copy(), w[a:e].copy()
+        c = GpuWorld(strict=False)
+        c.set_scene(ws)
+        sel = (ki >= a) & (ki < e)
+        c.import_warm_cache(((ki[sel] - a).astype(np.uint64) << np.uint64(32)) | (kj[sel] - a).astype(np.uint64),
+                            imp[sel])
+        ctxs.append(c)
+    for c in ctxs:       # first call of each context (launch sizing) outside the timing
+        c.step(1)
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        for c in ctxs:
+            c.step(1)
+    t1 = time.perf_counter()
+    for c in ctxs:
+        c.close()
+    return (t1 - t0) * 1e3 / (loop_steps * len(ctxs))
+
+
+def batched_bench(worlds: int, loop_worlds: int, loop_steps: int):
+    """Batched world-steps/s over the whole 300-step run of the batch (HIP
+    graph replay; the autotuner runs after step 30, from a snapshot it
+    restores, outside the timing), against looping the plain single-world
+    step over a sample of the same worlds at steps 30, 150 and 300."""
     import numpy as np
     from nativeengine_amd import scene as scn
     from nativeengine_amd.gpu import GpuWorld
     t0 = time.perf_counter()
     b = scn.batched_worlds(worlds, seed=1)
     log(f"batched: {b.nworlds} worlds, {b.scene.n} bodies, built in {time.perf_counter() - t0:.1f}s")
-    g = GpuWorld()
+    g = GpuWorld(strict=False)
     g.set_scene(b.scene)
     g.set_worlds(b.offsets, b.half, b.restitution, b.friction)
     g.use_graph(True)
-    t0 = time.perf_counter()
-    for _ in range(warmup):
-        g.step(1)
-    log(f"batched warm-up {warmup} steps: {time.perf_counter() - t0:.1f}s")
-    tune = g.autotune(steps=3)
-    log(f"autotuner: {tune}")
-    t0 = time.perf_counter()
-    g.step(steps)
-    t1 = time.perf_counter()
-    reruns, trunc = g.epa_stats()
-    batched_ms = (t1 - t0) * 1e3 / steps
-    # the same state and warm cache, each sampled world in its own context
-    x, v, q, w = g.state()
-    keys, imp = g.export_warm_cache()
-    ki, kj = (keys >> np.uint64(32)).astype(np.int64), (keys & np.uint64(0xFFFFFFFF)).astype(np.int64)
     rng = np.random.default_rng(7)
     sample = np.sort(rng.choice(b.nworlds, size=min(loop_worlds, b.nworlds), replace=False))
-    ctxs = []
-    for wi in sample:
-        a, e = int(b.offsets[wi]), int(b.offsets[wi + 1])
-        ws = b.world(int(wi))
-        ws.x, ws.v, ws.q, ws.w = x[a:e].copy(), v[a:e].copy(), q[a:e].copy(), w[a:e].copy()
-        c = GpuWorld()
-        c.set_scene(ws)
-        sel = (ki >= a) & (ki < e)
-        c.import_warm_cache(((ki[sel] - a).astype(np.uint64) << np.uint64(32)) | (kj[sel] - a).astype(np.uint64),
-                            imp[sel])
-        ctxs.append(c)
-    t0 = time.perf_counter()
-    for _ in range(loop_steps):
-        for c in ctxs:
-            c.step(1)
-    t1 = time.perf_counter()
-    loop_ms_per_world_step = (t1 - t0) * 1e3 / (loop_steps * len(ctxs))
-    for c in ctxs:
-        c.close()
+    timed_s, loop_ms, capped, tune = 0.0, {}, 0, None
+    done = 0
+    for chunk_end in (30, 150, 300):
+        calls0 = g.epa_capped_calls
+        t0 = time.perf_counter()
+        g.step(chunk_end - done)
+        timed_s += time.perf_counter() - t0
+        capped += g.epa_capped_calls - calls0
+        done = chunk_end
+        if tune is None:
+            tune = g.autotune(steps=3)
+            log(f"autotuner: {tune}")
+        loop_ms[chunk_end] = _loop_sample(g, b, sample, loop_steps)
+        log(f"batched: step {done}, {timed_s * 1e3 / done:.2f} ms/step so far; loop {loop_ms[chunk_end]:.3f} ms "
+            f"per world-step")
+    reruns, trunc = g.epa_stats()
     g.close()
+    batched_ms = timed_s * 1e3 / 300
     batched_wps = b.nworlds / (batched_ms * 1e-3)
-    loop_wps = 1e3 / loop_ms_per_world_step
+    loop_wps = 1e3 / (sum(loop_ms.values()) / len(loop_ms))
     return {
         "metric": "batched world-steps/s",
         "value": batched_wps,
         "unit": "world-steps/s",
         "ms_per_step": batched_ms,
-        "steps": steps,
-        "warmup": warmup,
+        "steps": 300,
         "worlds": b.nworlds,
         "bodies": int(b.scene.n),
         "body_steps_per_s": b.scene.n / (batched_ms * 1e-3),
         "loop_world_steps_per_s": loop_wps,
-        "loop_sample": f"{len(ctxs)} of the worlds, each in its own context, {loop_steps} steps each, same state",
+        "loop_ms_per_world_step_at": {k: round(v, 4) for k, v in loop_ms.items()},
+        "loop_sample": f"{len(sample)} of the worlds, each in its own context from the batch's state and warm cache "
+                       f"at steps 30, 150 and 300, {loop_steps} steps each",
         "vs_loop": batched_wps / loop_wps,
         "autotune": tune,
-        "epa_large_workspace_reruns": reruns,
-        "epa_truncated": trunc,
+        "calls_with_epa_at_face_cap": capped,
+        "epa_large_workspace_reruns_last_call": reruns,
+        "epa_truncated_last_call": trunc,
         "config": "1024 periodic worlds, N_w log-uniform [64,4096] seed 1, half spheres r U[0.2,0.6] / half boxes "
-                  "he U[0.2,0.6]^3, 20% volume fraction, mu_w U[0,0.8], e_w U[0,1], g=0, dt=1/60, 8 iterations",
+                  "he U[0.2,0.6]^3, 20% volume fraction, mu_w U[0,0.8], e_w U[0,1], g=0, dt=1/60, 8 iterations, "
+                  "300 steps",
     }
 
 
@@ -471,7 +587,7 @@ def adjoint_bench(worlds: int, steps: int, every: int):
     from nativeengine_amd.gpu import GpuWorld
     b = scn.adjoint_worlds(worlds, n_spheres=256, n_boxes=256, side=16.0, seed=2)
     nw = b.nworlds
-    g = GpuWorld()
+    g = GpuWorld(strict=False)
     g.set_scene(b.scene)
     g.set_worlds(b.offsets, b.half, b.restitution, b.friction)
     target = g.rollout(b.scene.v, np.full(nw, 0.4), np.full(nw, 0.3), steps, every)
@@ -610,8 +726,9 @@ def gas_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300, help="timed steps of the large terrain scene")
-    ap.add_argument("--warmup", type=int, default=10, help="untimed steps of the large terrain scene first")
+    ap.add_argument("--steps", type=int, default=300,
+                    help="timed steps of the large terrain scene's 300-step run, spread evenly over it")
+    ap.add_argument("--warmup", type=int, default=0, help="the run's first steps, left out of the timed set")
     ap.add_argument("--large-n", type=int, default=1 << 20)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
@@ -622,13 +739,12 @@ def main():
     ap.add_argument("--gas-warmup", type=int, default=5)
     ap.add_argument("--no-pile", action="store_true")
     ap.add_argument("--pile-n", type=int, default=65536)
-    ap.add_argument("--pile-steps", type=int, default=600)
-    ap.add_argument("--pile-warmup", type=int, default=0)
+    ap.add_argument("--pile-steps", type=int, default=600, help="timed steps of the pile's 600-step run")
+    ap.add_argument("--no-config1", action="store_true", help="skip the 4096-sphere elastic gas")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--worlds", type=int, default=1024)
-    ap.add_argument("--batched-steps", type=int, default=20)
-    ap.add_argument("--batched-warmup", type=int, default=30)
     ap.add_argument("--loop-worlds", type=int, default=128)
+    ap.add_argument("--loop-steps", type=int, default=5)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=65536, help="bodies in the CPU's contact-phase sample")
     ap.add_argument("--cpu-timeout", type=float, default=90.0)
@@ -650,7 +766,6 @@ def main():
         assert np.array_equal(out, a * b + a)
         print(json.dumps({"traffic_probe": "k_fp_probe", "read_bytes": 2 * 8 << 25, "write_bytes": 8 << 25}))
         return
-    args.large_steps, args.large_warmup = args.steps, args.warmup
 
     if args.gpus != 1:
         print(json.dumps({"error": "the step runs on one GPU; sharding one world across GPUs is not "
@@ -665,6 +780,7 @@ def main():
         large = large_bench(args)
         log(f"large: {large['value']:.4g} body-steps/s, {large['ms_per_step']:.3f} ms/step")
         out.update({"value": large["value"], "ms_per_step": large["ms_per_step"], "steps": large["steps"],
+                    "warmup": large["warmup"],
                     "config": {"workload": large["workload"], "bodies": large["bodies"], "parallelism": "single GPU"},
                     "roofline": large["roofline"], "cpu_baseline": large["cpu_baseline"],
                     "speedup_vs_cpu": large["speedup_vs_cpu"],
@@ -679,8 +795,13 @@ def main():
         pile = pile_bench(args)
         log(f"pile: {pile['value']:.4g} body-steps/s, {pile['ms_per_step']:.3f} ms/step")
         out["pile"] = pile
+    if not args.no_config1:
+        c1 = config1_bench(args)
+        log(f"gas4096: {c1['value']:.4g} body-steps/s, {c1['ms_per_step']:.3f} ms/step, dE/E {c1['energy_drift_rel']:.3e} "
+            f"(reference {c1['reference_energy_drift_rel']:.3e}), bitwise {c1['bitwise_equal_to_reference']}")
+        out["gas4096"] = c1
     if not args.no_batched:
-        batched = batched_bench(args.worlds, args.batched_steps, args.batched_warmup, args.loop_worlds, 3)
+        batched = batched_bench(args.worlds, args.loop_worlds, args.loop_steps)
         log(f"batched: {batched['value']:.4g} world-steps/s, {batched['ms_per_step']:.3f} ms/step, "
             f"{batched['vs_loop']:.1f}x the loop")
         out["batched"] = batched

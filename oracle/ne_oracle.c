@@ -172,9 +172,26 @@ struct otape_pair {
     ocontact cs[4];
 };
 
+/* Branch bits of one Gauss-Seidel visit (world.cpp:572-612), recorded in
+ * mode 1 and replayed in mode 3. */
+enum {
+    TB_KN = 1,        /* Kn > 1e-18: the normal impulse is applied            */
+    TB_UNCLAMPED = 2, /* old + dj > 0: the accumulated normal impulse is kept  */
+    TB_FRIC = 4,      /* mu > 0 and nImp > 0: friction is visited             */
+    TB_VT = 8,        /* |vt| > 1e-12                                          */
+    TB_KT = 16,       /* Kt > 1e-18                                            */
+    TB_LO = 32,       /* the friction impulse clamps at -mu nImp               */
+    TB_HI = 64        /* ... at +mu nImp                                       */
+};
+
 struct otape_step {
     size_t n, cap;
     struct otape_pair* p;
+    /* per constraint: the restitution branch (vn < -0.5, world.cpp:544);
+     * per constraint and iteration: TB_* bits */
+    uint32_t ncons, iters;
+    uint8_t* rest;
+    uint8_t* visit;
 };
 
 static v3 ld3(const double* p, uint32_t i) { return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
@@ -947,6 +964,7 @@ struct ocons {                                                                  
     double overlap, restBias, mu, nImp, tImp;
     uint64_t pairKey;
     int slot;
+    int bounce;   /* vn < -0.5 at the build: the restitution bias is on (world.cpp:544) */
 };
 
 static uint64_t pair_key_of(uint32_t i, uint32_t j) { return ((uint64_t)i << 32) | (uint32_t)j; } /* world.cpp:34-36 */
@@ -1033,7 +1051,7 @@ static void collide(ne_oracle* o) {                                             
     pairvec pv = {0, 0, 0};
     const struct otape_step* replay = NULL;
     struct otape_step* record = NULL;
-    if (o->tape_mode == 2) {
+    if (o->tape_mode >= 2) {
         replay = (o->tape_cursor < o->tape_steps) ? &o->tape[o->tape_cursor] : NULL;
         ++o->tape_cursor;
     } else {
@@ -1047,7 +1065,8 @@ static void collide(ne_oracle* o) {                                             
             memset(record, 0, sizeof(*record));
         }
     }
-    const size_t npairs = (o->tape_mode == 2) ? (replay ? replay->n : 0) : pv.n;
+    const size_t npairs = (o->tape_mode >= 2) ? (replay ? replay->n : 0) : pv.n;
+    const struct otape_step* frozen = (o->tape_mode == 3) ? replay : NULL;
     o->ncand = (uint32_t)npairs;
     o->ncons = 0;
     o->nisp = 0;
@@ -1101,7 +1120,10 @@ static void collide(ne_oracle* o) {                                             
             k.rB = min_image(&o->box, sub(cs[c].point, b->x));
             v3 vc = contact_vel(a, b, k.rA, k.rB);
             double vn = dot(vc, k.n);
-            k.restBias = (vn < -0.5) ? -e * vn : 0.0;                               /* world.cpp:40, 544 */
+            int bounce = vn < -0.5;
+            if (frozen) bounce = (o->ncons < frozen->ncons) ? frozen->rest[o->ncons] : bounce;
+            k.restBias = bounce ? -e * vn : 0.0;                                     /* world.cpp:40, 544 */
+            k.bounce = bounce;
             k.pairKey = key; k.slot = slot++;
             k.nImp = 0.0; k.tImp = 0.0;
             if (P->warm_start) {
@@ -1115,6 +1137,15 @@ static void collide(ne_oracle* o) {                                             
 
     struct ocons* C = o->cons;
     const uint32_t nC = o->ncons;
+    const int iters = P->solver_iterations;                                          /* world.cpp:572 */
+    if (frozen && (frozen->ncons != nC || frozen->iters != (uint32_t)iters)) frozen = NULL;   /* not the recorded set */
+    if (record) {
+        record->ncons = nC;
+        record->iters = (uint32_t)iters;
+        record->rest = (uint8_t*)malloc(nC ? nC : 1);
+        record->visit = (uint8_t*)malloc((size_t)nC * (size_t)iters + 1);
+        for (uint32_t q = 0; q < nC; ++q) record->rest[q] = (uint8_t)C[q].bounce;
+    }
     if (P->warm_start) {                                                             /* world.cpp:559-568 */
         for (uint32_t q = 0; q < nC; ++q) {
             if (C[q].nImp == 0.0) continue;
@@ -1122,40 +1153,56 @@ static void collide(ne_oracle* o) {                                             
             apply_impulse(&o->B[C[q].i], &o->B[C[q].j], C[q].rA, C[q].rB, J);
         }
     }
-    const int iters = P->solver_iterations;                                          /* world.cpp:572 */
+    /* In mode 3 every branch below takes the recorded run's side (frozen
+     * active set), so the step is one smooth function of its inputs near the
+     * recorded ones; otherwise the branches are evaluated, as in the
+     * reference, and mode 1 records them. */
     for (int it = 0; it < iters; ++it) {
         for (uint32_t q = 0; q < nC; ++q) {
             struct ocons* k = &C[q];
             obody* a = &o->B[k->i];
             obody* b = &o->B[k->j];
+            const uint8_t fz = frozen ? frozen->visit[(size_t)it * nC + q] : 0;
+            uint8_t bits = 0;
             v3 vc = contact_vel(a, b, k->rA, k->rB);
             double vn = dot(vc, k->n);
             double Kn = eff_mass(a, b, k->rA, k->rB, k->n);
-            if (Kn > 1e-18) {
+            if (frozen ? (fz & TB_KN) : Kn > 1e-18) {
+                bits |= TB_KN;
                 double dj = (k->restBias - vn) / Kn;
                 double old = k->nImp;
-                k->nImp = (old + dj > 0.0) ? old + dj : 0.0;
+                const int keep = frozen ? (fz & TB_UNCLAMPED) != 0 : old + dj > 0.0;
+                if (keep) bits |= TB_UNCLAMPED;
+                k->nImp = keep ? old + dj : 0.0;
                 dj = k->nImp - old;
                 apply_impulse(a, b, k->rA, k->rB, scl(k->n, dj));
             }
-            if (k->mu > 0.0 && k->nImp > 0.0) {
+            if (frozen ? (fz & TB_FRIC) : (k->mu > 0.0 && k->nImp > 0.0)) {
+                bits |= TB_FRIC;
                 v3 vc2 = contact_vel(a, b, k->rA, k->rB);
                 v3 vt = sub(vc2, scl(k->n, dot(vc2, k->n)));
                 double vtl = norm(vt);
-                if (vtl > 1e-12) {
+                if (frozen ? (fz & TB_VT) : vtl > 1e-12) {
+                    bits |= TB_VT;
                     v3 t = dvs(vt, vtl);
                     double Kt = eff_mass(a, b, k->rA, k->rB, t);
-                    if (Kt > 1e-18) {
+                    if (frozen ? (fz & TB_KT) : Kt > 1e-18) {
+                        bits |= TB_KT;
                         double djt = -vtl / Kt;
                         double maxF = k->mu * k->nImp;
                         double old = k->tImp;
                         double sum = old + djt;
-                        k->tImp = (sum < -maxF) ? -maxF : (sum > maxF ? maxF : sum);
+                        const int lo = frozen ? (fz & TB_LO) != 0 : sum < -maxF;
+                        const int hi = frozen ? (fz & TB_HI) != 0 : (!lo && sum > maxF);
+                        if (lo) bits |= TB_LO;
+                        if (hi) bits |= TB_HI;
+                        k->tImp = lo ? -maxF : (hi ? maxF : sum);
                         djt = k->tImp - old;
                         apply_impulse(a, b, k->rA, k->rB, scl(t, djt));
                     }
                 }
             }
+            if (record) record->visit[(size_t)it * nC + q] = bits;
         }
     }
     for (uint32_t q = 0; q < nC; ++q) {                                              /* world.cpp:615-616, contact.hpp:78-85 */
@@ -1389,7 +1436,7 @@ int ne_oracle_destroy(ne_oracle* o) {
     free_meshes(o);
     free(o->cs);
     free(o->B); free(o->hull); free(o->nocol); free(o->cons); free(o->cache_key); free(o->cache_imp);
-    for (uint32_t t = 0; t < o->tape_steps; ++t) free(o->tape[t].p);
+    for (uint32_t t = 0; t < o->tape_steps; ++t) { free(o->tape[t].p); free(o->tape[t].rest); free(o->tape[t].visit); }
     free(o->tape);
     free(o->joints);
     free(o->isp);
@@ -1480,11 +1527,17 @@ int ne_oracle_step(ne_oracle* o, int32_t nsteps) {
  * points, depths) instead of broadphase and narrow phase, with the rest of
  * the step unchanged; mode 0 steps normally.  Replaying at the recorded
  * inputs reproduces the recorded run exactly; at perturbed inputs it is the
- * same step with the contact set held fixed. */
+ * same step with the contact set held fixed.  Mode 3 replays as mode 2 and
+ * also holds the solver's branches fixed at the recorded run's sides: the
+ * restitution bias (vn < -0.5), each visit's normal clamp, whether friction
+ * is visited, |vt| > 1e-12, and the friction cone's clamp side
+ * (world.cpp:544, 572-612).  With the active set frozen, the rollout is one
+ * smooth function near the recorded inputs, whose derivative is the one an
+ * adjoint through the recorded branches computes. */
 int ne_oracle_tape(ne_oracle* o, int32_t mode) {
-    if (!o || mode < 0 || mode > 2) return NE_ERR_INVALID;
+    if (!o || mode < 0 || mode > 3) return NE_ERR_INVALID;
     if (mode == 1) {
-        for (uint32_t t = 0; t < o->tape_steps; ++t) free(o->tape[t].p);
+        for (uint32_t t = 0; t < o->tape_steps; ++t) { free(o->tape[t].p); free(o->tape[t].rest); free(o->tape[t].visit); }
         o->tape_steps = 0;
     }
     o->tape_cursor = 0;

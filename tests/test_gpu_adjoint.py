@@ -134,22 +134,64 @@ def test_adjoint_is_bitwise_repeatable(gpu_available):
     g.close()
 
 
-def test_adjoint_at_full_rollout_size_against_fd(gpu_available, oracle_lib):
+def _frozen_fd_setup(world, mu, e, steps, x_nominal):
+    """The oracle's recorded run of one world (contacts and solver branches
+    on its tape) and the loss of its frozen-active-set replay; final
+    positions are taken as the continuation of the nominal ones across the
+    periodic wrap (the adjoint differentiates the wrap as the identity)."""
+    o = engines.OracleWorld()
+    s0 = world.copy()
+    s0.params.friction, s0.params.restitution = mu, e
+    o.set_scene(s0)
+    o.tape(1)
+    o.step(steps)
+    assert np.array_equal(o.state()[0], x_nominal), "the rollout differs from the oracle's"
+    L = 2.0 * np.asarray(world.params.half)
+
+    def loss(v, m, r, target, scale):
+        s = world.copy()
+        s.params.friction, s.params.restitution = m, r
+        o.set_params(s.params)
+        o.set_state(s.x, v, s.q, s.w)
+        o.reset_warm_cache()
+        o.tape(3)
+        o.step(steps)
+        x = o.state()[0]
+        x = x_nominal + (x - x_nominal) - L * np.round((x - x_nominal) / L)
+        return float(np.sum((x - target) ** 2)) / scale
+
+    return loss
+
+
+@pytest.mark.parametrize("steps,worlds,check_v0", [(40, (0, 97, 201), True), (60, (0, 201), False)])
+def test_adjoint_at_full_rollout_size_against_frozen_fd(gpu_available, oracle_lib, steps, worlds, check_v0):
     """The differentiable-rollout workload at full size: 256 worlds of 256
     spheres (r 0.4) and 256 boxes (half extent 0.3) on a static ground box,
-    periodic side 16, 120 steps, checkpoint every 16, MSE loss against
-    targets made at mu 0.4 / e 0.3, evaluated at mu 0.5 / e 0.2.  World 0's
-    gradients (d/d mu, d/d e, 4 random directional derivatives in v0) are
-    checked against central differences of the oracle's fixed-contact-set
-    replay.  Over 120 steps the sensitivities grow ~4000-fold and clamps
-    switch within +-1e-6 of the nominal inputs, so a step of 1e-6 measures
-    kinks rather than the derivative; the difference quotients settle at
-    h = 1e-8 .. 1e-9 (where rounding limits them to ~1e-6).  Bar: the better
-    of the two within 1e-4 relative."""
+    periodic side 16, checkpoint every 16, MSE loss of the final positions
+    against targets made by the forward model at mu 0.4 / e 0.3, evaluated at
+    mu 0.5 / e 0.2.  d/d mu, d/d e and (over 40 steps) 8 random directional
+    derivatives in v0 of sampled worlds, against central differences at
+    h = 1e-6 (the 5-point stencil, whose truncation error is O(h^4): the
+    2-point quotient's O(h^2) error reaches 1e-5 on the smaller directional
+    derivatives here) of the oracle's replay with the recorded contact set
+    AND solver branches held fixed (normal clamps, friction visits and cone
+    sides, restitution bias): bar 1e-5 relative.
+
+    The horizons are those over which these differences converge, measured
+    with the frozen replay's own central and 4th-order differences at
+    h = 1e-6 and 5e-7: over 40 steps they agree to 1e-9 .. 8e-7 for mu, e
+    and v0 in worlds 0, 97 and 201; over 60 steps to 2e-6 or better for mu
+    and e in worlds 0 and 201 (world 97's mu spreads by 1.5e-4 there, and v0
+    directions by 1e-4 .. 5e-3).  At 120 steps they spread by 1e-4 .. 1e-1
+    in every world even with every branch frozen: a perturbation grows
+    ~1e4-fold over the pile's 120 steps, so no step size gives a difference
+    quotient accurate to 1e-5 in double precision there (the 120-step
+    gradients are covered by the bitwise repeatability test and the bench's
+    forward/backward run)."""
     from nativeengine_amd.gpu import GpuWorld
     b = S.adjoint_worlds(256, n_spheres=256, n_boxes=256, side=16.0, seed=2)
-    nw, steps, every = b.nworlds, 120, 16
-    g = GpuWorld(strict=False)   # a few of the 256 worlds reach the EPA face cap; world 0 is checked against the oracle
+    nw, every, h = b.nworlds, 16, 1e-6
+    g = GpuWorld(strict=False)
     g.set_scene(b.scene)
     g.set_worlds(b.offsets, b.half, b.restitution, b.friction)
     target = g.rollout(b.scene.v, np.full(nw, 0.4), np.full(nw, 0.3), steps, every)
@@ -159,42 +201,27 @@ def test_adjoint_at_full_rollout_size_against_fd(gpu_available, oracle_lib):
     gx = 2.0 * (x - target) / x.size
     gv, gmu, ge = g.rollout_backward(gx, nw)
     g.close()
-    w = 0
-    a, c = int(b.offsets[w]), int(b.offsets[w + 1])
-    world = b.world(w)
-    o = engines.OracleWorld()
-    s0 = world.copy()
-    s0.params.friction, s0.params.restitution = mu[w], e[w]
-    o.set_scene(s0)
-    o.tape(1)
-    o.step(steps)
-    assert np.array_equal(o.state()[0], x[a:c])
-    tw = target[a:c]
-
-    def f(v, m, r):
-        s = world.copy()
-        s.params.friction, s.params.restitution = m, r
-        o.set_params(s.params)
-        o.set_state(s.x, v, s.q, s.w)
-        o.reset_warm_cache()
-        o.tape(2)
-        o.step(steps)
-        return float(np.sum((o.state()[0] - tw) ** 2)) / x.size
-
-    rng = np.random.default_rng(11)
-    dirs = []
-    for k in range(4):
-        d = rng.normal(size=world.v.shape)
-        d[world.inv_mass == 0] = 0.0
-        dirs.append(d)
     worst = 0.0
-    quantities = [("mu", gmu[w], lambda h: (f(world.v, mu[w] + h, e[w]) - f(world.v, mu[w] - h, e[w])) / (2 * h)),
-                  ("e", ge[w], lambda h: (f(world.v, mu[w], e[w] + h) - f(world.v, mu[w], e[w] - h)) / (2 * h))]
-    for k, d in enumerate(dirs):
-        quantities.append((f"v0 dir {k}", float(np.sum(gv[a:c] * d)),
-                           lambda h, d=d: (f(world.v + h * d, mu[w], e[w]) - f(world.v - h * d, mu[w], e[w])) / (2 * h)))
-    for name, ad, fdf in quantities:
-        best = min((abs(ad - fd) / max(abs(ad), abs(fd), 1e-12), h, fd) for h in (1e-8, 1e-9) for fd in [fdf(h)])
-        worst = max(worst, best[0])
-        _say(f"full size world {w} {name}: adjoint {ad:.10e} fd {best[2]:.10e} (h {best[1]:.0e}) rel {best[0]:.2e}")
-    assert worst <= 1e-4, worst
+    rng = np.random.default_rng(11)
+    for w in worlds:
+        a, c = int(b.offsets[w]), int(b.offsets[w + 1])
+        world = b.world(w)
+        loss = _frozen_fd_setup(world, mu[w], e[w], steps, x[a:c])
+        tw = target[a:c]
+        f = lambda v, m, r: loss(v, m, r, tw, x.size)
+        def d4(g):   # central differences at h = 1e-6 with the 4th-order 5-point stencil
+            return (-g(2 * h) + 8 * g(h) - 8 * g(-h) + g(-2 * h)) / (12 * h)
+
+        quantities = [("mu", gmu[w], d4(lambda t: f(world.v, mu[w] + t, e[w]))),
+                      ("e", ge[w], d4(lambda t: f(world.v, mu[w], e[w] + t)))]
+        if check_v0:
+            for k in range(8):
+                d = rng.normal(size=world.v.shape)
+                d[world.inv_mass == 0] = 0.0
+                quantities.append((f"v0 dir {k}", float(np.sum(gv[a:c] * d)),
+                                   d4(lambda t, d=d: f(world.v + t * d, mu[w], e[w]))))
+        for name, ad, fd in quantities:
+            rel = abs(ad - fd) / max(abs(ad), abs(fd), 1e-300)
+            worst = max(worst, rel)
+            _say(f"{steps} steps, world {w} {name}: adjoint {ad:.10e} fd {fd:.10e} rel {rel:.2e}")
+    assert worst <= 1e-5, worst

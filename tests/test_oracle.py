@@ -341,6 +341,44 @@ def test_contact_tape_replay_reproduces_the_recorded_run(oracle_lib):
     o.step(20)
     x = o.state()[0]
     assert 0.0 < np.max(np.abs(x - want[0])) < 1e-4
+    # mode 3 (branches frozen too) reproduces the recorded run bit for bit
+    o.tape(3)
+    o.set_state(s.x, s.v, s.q, s.w)
+    o.reset_warm_cache()
+    o.step(20)
+    for u, v in zip(o.state(), want):
+        assert np.array_equal(u, v)
+
+
+def test_frozen_tape_is_smooth_where_the_plain_replay_kinks(oracle_lib):
+    """With the solver's branches frozen (tape mode 3) the fixed-contact
+    rollout is one smooth function: its central differences converge
+    quadratically in h (the h and h/2 quotients agree far better than h and
+    10h, and to 1e-6 relative)."""
+    b = S.adjoint_worlds(1, n_spheres=48, n_boxes=48, side=7.0, seed=8)
+    s = b.world(0)
+    o = engines.OracleWorld()
+    o.set_scene(s)
+    o.tape(1)
+    o.step(40)
+    rng = np.random.default_rng(2)
+    d = rng.normal(size=s.v.shape)
+    d[s.inv_mass == 0] = 0.0
+    wgt = rng.normal(size=s.x.shape)
+
+    def f(mode, h):
+        o.set_state(s.x, s.v + h * d, s.q, s.w)
+        o.reset_warm_cache()
+        o.tape(mode)
+        o.step(40)
+        return float(np.sum(o.state()[0] * wgt))
+
+    def d2(mode, h):
+        return (f(mode, h) - f(mode, -h)) / (2 * h)
+
+    a, b2, c = d2(3, 1e-5), d2(3, 1e-6), d2(3, 5e-7)
+    assert abs(b2 - c) < 0.1 * abs(a - b2) + 1e-9 * abs(c), (a, b2, c)
+    assert abs(b2 - c) / abs(c) < 1e-6, (b2, c)
 
 
 def _live_compare(s, steps):
