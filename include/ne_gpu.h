@@ -64,6 +64,16 @@ extern "C" {
 #define NE_FLAG_SLEEPING   8u
 #define NE_FLAG_CCD       16u
 
+/* ---- facade contact events (PhysicsWorld::processContacts,
+ *      src/physics_world.cpp:253-302): the callback each list feeds ------- */
+#define NE_EVENT_CONTACT        0   /* onContact: every solid contact of the step */
+#define NE_EVENT_BEGIN          1   /* onBegin: solid pairs new this step          */
+#define NE_EVENT_STAY           2   /* onStay: solid pairs also in the last step   */
+#define NE_EVENT_END            3   /* onEnd: last step's solid pairs gone now (last step's contact) */
+#define NE_EVENT_TRIGGER_ENTER  4   /* onTrigEnter: pairs with a sensor, new       */
+#define NE_EVENT_TRIGGER_STAY   5   /* onTrigStay                                  */
+#define NE_EVENT_TRIGGER_EXIT   6   /* onTrigExit (last step's contact)            */
+
 /* ---- material combine modes (src/world.hpp:85) --------------------------- */
 #define NE_COMBINE_AVERAGE        0
 #define NE_COMBINE_MIN            1
@@ -288,6 +298,29 @@ int ne_gpu_get_contacts(ne_gpu_ctx *ctx, uint32_t cap, uint32_t *count,
 /* Distinct hit pairs (pairs with >=1 contact) of the last step, in (i,j) order. */
 int ne_gpu_get_pairs(ne_gpu_ctx *ctx, uint32_t cap, uint32_t *count,
                      uint32_t *i, uint32_t *j);
+
+/* Facade contact events: `PhysicsWorld::processContacts`
+ * (src/physics_world.cpp:253-302), run after every step while enabled, on the
+ * device.  The pairs i < j of the post-step state that pass the facade's
+ * filters (a pair with a sensor always, else not two immovable bodies;
+ * layers; no-collide lists) and overlap under detectContact (one contact,
+ * no box-box manifold) are found with the step's own grid and narrow-phase
+ * kernels and diffed against the last step's.  Enabling (or uploading
+ * bodies) empties the last-step lists, as the facade's maps start empty.
+ * While enabled, ne_gpu_step takes one step per call (NE_ERR_INVALID
+ * otherwise) and HIP-graph replay is off.  Worlds with mesh bodies are
+ * NE_ERR_UNSUPPORTED (the reference's processContacts runs detectContact on
+ * a mesh body as a cylinder of its radius and half height), as are batched
+ * worlds. */
+int ne_gpu_set_events(ne_gpu_ctx *ctx, int32_t enable);
+
+/* The last step's events of one kind (NE_EVENT_*), in (i, j) order -- the
+ * facade's handle order when handles follow body indices; the facade maps
+ * indices to handles.  Contacts: normal b -> a, point, depth; END and
+ * TRIGGER_EXIT carry the previous step's contact.  NULL arrays query the
+ * count; `cap` bounds the copy. */
+int ne_gpu_get_events(ne_gpu_ctx *ctx, int32_t kind, uint32_t cap, uint32_t *count, uint32_t *i, uint32_t *j,
+                      double *point, double *normal, double *depth);
 
 /* Diagnostics of `World` (src/world.cpp:704-729): sums over invMass>0 bodies
  * in index order.  p[3], L[3] and ke may be NULL. */

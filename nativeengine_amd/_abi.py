@@ -24,6 +24,10 @@ FLAG_DYNAMIC = 4
 FLAG_SLEEPING = 8
 FLAG_CCD = 16
 
+# facade event kinds (PhysicsWorld::processContacts callbacks)
+(NE_EVENT_CONTACT, NE_EVENT_BEGIN, NE_EVENT_STAY, NE_EVENT_END, NE_EVENT_TRIGGER_ENTER, NE_EVENT_TRIGGER_STAY,
+ NE_EVENT_TRIGGER_EXIT) = range(7)
+
 COMBINE_AVERAGE, COMBINE_MIN, COMBINE_MAX, COMBINE_MULTIPLY, COMBINE_GEOMETRIC_MEAN = range(5)
 
 _dp = C.POINTER(C.c_double)

@@ -57,6 +57,7 @@ struct BodiesDev {
     double* hull;
     double *st, *ims, *iis0, *iis1, *iis2;   // sleep timer, stored inverse mass / inertia (world.cpp:317-371)
     int sleep_on;                            // island sleeping enabled: candidates keep sleeping bodies
+    int ev_filter;                           // the facade's event filter instead of the step's (events pass)
     uint32_t n;
     // Batched worlds: world id per body and the world table (both null for
     // one world).  Bodies of a world are contiguous.
@@ -75,6 +76,7 @@ struct SimParams {
     int open_boundary;
     int sleep;
     double sleep_lin, sleep_ang, sleep_time, wake_vel;
+    int no_manifold;   // detectContact's one contact per pair, no box-box manifold (events pass)
 };
 
 // Grid description, computed on the device every step.
@@ -128,7 +130,8 @@ enum CounterSlot : int {
     CNT_EPABIG_PAIRS = 21,   // pair entries of the large-workspace queue (mesh entries follow them)
     CNT_EPALONG2 = 22,   // convex pairs whose EPA ran past the second launch's budget (this step)
     CNT_HOSTCHK = 23,    // a host call's own check (snapshot loads); no step kernel uses it
-    CNT_N = 24
+    CNT_EV = 24,         // facade events pass: overlapping pairs of the post-step state
+    CNT_N = 25
 };
 
 constexpr int TPB = 256;
@@ -467,6 +470,15 @@ __global__ void __launch_bounds__(TPB) k_cell_fill(const uint32_t* __restrict__ 
 // earlier in the same step's pair order, after which the reference keeps its
 // pairs (the exact decision is taken after the narrow phase, k_sleep_fate).
 __device__ __forceinline__ bool pair_passes(const BodiesDev& B, uint32_t i, uint32_t j) {
+    if (B.ev_filter) {   // PhysicsWorld::processContacts (src/physics_world.cpp:261-266)
+        const bool any_sensor = ((B.flags[i] | B.flags[j]) & NE_FLAG_SENSOR) != 0;
+        if (!any_sensor && B.im[i] + B.im[j] <= 0.0) return false;
+        const uint32_t li = B.layer[i], lj = B.layer[j];
+        if (!((B.mask[i] & (1u << (lj & 31u))) && (B.mask[j] & (1u << (li & 31u))))) return false;
+        for (uint32_t k = B.nc_off[i]; k < B.nc_off[i + 1]; ++k)
+            if (B.nc_idx[k] == j) return false;
+        return true;
+    }
     double mi = B.im[i], mj = B.im[j];
     if (B.sleep_on) {
         if (B.flags[i] & NE_FLAG_SLEEPING) mi = fmax(mi, B.ims[i]);
@@ -1007,7 +1019,7 @@ __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimPar
         if (c.hit == 2) return NARROW_DEFERRED;   // GJK overlap: EPA runs in k_epa
         if (ws.overflow) return 0;
         if (c.hit) {
-            if (si == SH_BOX && sj == SH_BOX) {
+            if (si == SH_BOX && sj == SH_BOX && !P.no_manifold) {
                 V3 pts[4];
                 double dep[4];
                 int m = box_box_manifold(box, a, b, c.normal, pts, dep, fault);
@@ -1458,7 +1470,7 @@ __global__ void __launch_bounds__(64 * BIG_NW) k_narrow_big(BodiesDev B, SimPara
             if (threadIdx.x == 0) atomicOr(fault, NE_FAULT_EPA_BIG);
         } else if (c.hit) {
             if (overflow && threadIdx.x == 0) atomicAdd(&counters[CNT_EPATRUNC], 1u);
-            if (a.shape == SH_BOX && b.shape == SH_BOX) {
+            if (a.shape == SH_BOX && b.shape == SH_BOX && !P.no_manifold) {
                 V3 pts[4];
                 double dep[4];
                 const int m = box_box_manifold(box, a, b, c.normal, pts, dep, fault);
@@ -1521,7 +1533,7 @@ __global__ void __launch_bounds__(64) k_epa_wave(BodiesDev B, SimParams P, uint3
                 else atomicOr(fault, NE_FAULT_EPA_QUEUE);
             }
         } else if (c.hit) {
-            if (a.shape == SH_BOX && b.shape == SH_BOX) {
+            if (a.shape == SH_BOX && b.shape == SH_BOX && !P.no_manifold) {
                 V3 pts[4];
                 double dep[4];
                 const int m = box_box_manifold(box, a, b, c.normal, pts, dep, fault);
@@ -3645,6 +3657,19 @@ struct ne_gpu_ctx {
     uint32_t cells_per_body = 2;
     int epa_faces_max = EPA_BIG_FACES_MAX;   // the large EPA workspace's face cap per run
     uint8_t* scratch_dev = nullptr;          // this device's address of `scratch`
+    // facade events (PhysicsWorld::processContacts): two lists, this step's
+    // and the last step's (ev_cur is the last completed step's), each of
+    // pair keys (i << 32 | j), trigger flags, contacts (n, point, depth) and
+    // the diff's marks; counts on the device (ev_n) and the host (h_ev_n)
+    bool events_on = false;
+    int ev_cur = 0;
+    uint32_t h_ev_n[2] = {0, 0};
+    uint32_t ev_cap = 0;
+    uint64_t* ev_key[2] = {nullptr, nullptr};
+    uint32_t* ev_trig[2] = {nullptr, nullptr};
+    double* ev_dat[2] = {nullptr, nullptr};
+    uint32_t* ev_state[2] = {nullptr, nullptr};
+    uint32_t *ev_off = nullptr, *ev_n = nullptr, *ev_saved = nullptr;
     GridDev grid_host{};
     uint32_t sgrid_cells = 0;
     // host copies from the last upload (grid plans are rebuilt from them)
@@ -3866,6 +3891,93 @@ __global__ void k_nes1_unpack(double* __restrict__ F, uint32_t ncap, uint32_t* _
         }
         flags[i] = r[176] ? (flags[i] | NE_FLAG_SLEEPING) : (flags[i] & ~NE_FLAG_SLEEPING);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Facade contact events (PhysicsWorld::processContacts,
+// src/physics_world.cpp:253-302).  After a step, the overlapping pairs of the
+// post-step state are found again with the step's own grid and narrow-phase
+// kernels, run with the facade's filter (a sensor pair always, else not two
+// immovable bodies; layers; no-collide lists) and detectContact's single
+// contact per pair (no box-box manifold).  They are gathered in (i, j) order
+// with their contact and a trigger flag (a sensor in the pair), and each
+// list entry is marked by a binary search in the other step's list: a
+// current pair also present last step (same kind) stays, else begins or
+// enters; a last-step pair absent now ends or exits.
+// ---------------------------------------------------------------------------
+
+// The packed pose of every body for the grid fill, binned into the static
+// periodic grid when BIN (the step bins inside integrate).
+template <bool BIN>
+__global__ void __launch_bounds__(TPB) k_ev_bin(BodiesDev B, StaticGrid G, SortedBody* __restrict__ xp,
+                                                uint32_t* __restrict__ cell_count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    const V3 x = ld3(B.x0, B.x1, B.x2, i);
+    xp[i] = SortedBody{x.x, x.y, x.z, B.bR[i]};
+    if (BIN && !(B.flags[i] & FLAG_GRID_EXCLUDED)) {
+        uint32_t cell;
+        if (B.W) {
+            const WorldDev& w = B.W[B.wid[i]];
+            cell = w.cell_base + grid_cell(w.grid, x);
+        } else {
+            cell = static_cell(G, x);
+        }
+        atomicAdd(&cell_count[cell], 1u);
+    }
+}
+
+// The overlapping pairs, in candidate order (i, j), into list slots ev_off[p].
+__global__ void __launch_bounds__(TPB) k_ev_gather(BodiesDev B, const uint32_t* __restrict__ counters,
+                                                   const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                   const uint32_t* __restrict__ ncont, const uint32_t* __restrict__ ev_off,
+                                                   const double* __restrict__ ct, uint64_t* __restrict__ key,
+                                                   uint32_t* __restrict__ trig, double* __restrict__ dat,
+                                                   uint32_t* __restrict__ ev_n) {
+    const uint32_t np = counters[CNT_P];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ev_n = counters[CNT_EV];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        if (!ncont[p]) continue;
+        const uint32_t k = ev_off[p], i = cand_i[p], j = cand_j[p];
+        key[k] = ((uint64_t)i << 32) | (uint64_t)j;
+        trig[k] = ((B.flags[i] | B.flags[j]) & NE_FLAG_SENSOR) ? 1u : 0u;
+        const double* c = ct + (size_t)p * 28;
+#pragma unroll
+        for (int f = 0; f < 7; ++f) dat[7 * (size_t)k + f] = c[f];
+    }
+}
+
+// Whether the (ascending, unique) key list holds `key` with trigger flag t.
+__device__ __forceinline__ uint32_t ev_find(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ trig,
+                                            uint32_t n, uint64_t key, uint32_t t) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < n && keys[lo] == key && trig[lo] == t) ? 1u : 0u;
+}
+
+// state_cur[k] = 1: the current pair was in the last step's list (stay);
+// state_prev[k] = 1: the last step's pair is still there (no end).
+__global__ void __launch_bounds__(TPB) k_ev_diff(const uint32_t* __restrict__ ev_n, int cur, int prev,
+                                                 const uint64_t* __restrict__ key_c, const uint32_t* __restrict__ trig_c,
+                                                 const uint64_t* __restrict__ key_p, const uint32_t* __restrict__ trig_p,
+                                                 uint32_t* __restrict__ state_c, uint32_t* __restrict__ state_p) {
+    const uint32_t nc = ev_n[cur], np = ev_n[prev];
+    const uint32_t m = nc > np ? nc : np;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        if (k < nc) state_c[k] = ev_find(key_p, trig_p, np, key_c[k], trig_c[k]);
+        if (k < np) state_p[k] = ev_find(key_c, trig_c, nc, key_p[k], trig_p[k]);
+    }
+}
+
+// The step's counters back after the events pass, but for what the pass
+// adds to: faults, EPA truncations and reruns, and its own pair count.
+__global__ void k_ev_restore(uint32_t* __restrict__ counters, const uint32_t* __restrict__ saved) {
+    const int s = (int)threadIdx.x;
+    if (s < CNT_N && s != CNT_FAULT && s != CNT_EPATRUNC && s != CNT_EPARERUN && s != CNT_EV) counters[s] = saved[s];
 }
 
 // Staging of ne_gpu_save_state_noalloc: a file-scope device array, one copy
@@ -4159,6 +4271,9 @@ void free_all(ne_gpu_ctx* c) {
     dfree(c->prev_key); dfree(c->prev_imp); dfree(c->prev_n); dfree(c->counters);
     dfree(c->snap_body); dfree(c->snap_key); dfree(c->snap_imp); dfree(c->snap_n); dfree(c->snap_flags);
     dfree(c->nes1); c->nes1_cap = 0;
+    for (int b = 0; b < 2; ++b) { dfree(c->ev_key[b]); dfree(c->ev_trig[b]); dfree(c->ev_dat[b]); dfree(c->ev_state[b]); }
+    dfree(c->ev_off); dfree(c->ev_n); dfree(c->ev_saved);
+    c->ev_cap = 0;
     c->snap_nd = 0;
     c->snap_fcap = 0;
     c->snap_pcap = 0;
@@ -4203,26 +4318,13 @@ int ensure_side(ne_gpu_ctx* c) {
 
 // The step up to the solve: integrate, grid, candidate rows, narrow phase,
 // constraint build and islands.
-void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
+// The grid, the candidate rows and the narrow phase of one step (or of the
+// facade events pass, with its own filter and detectContact's one contact
+// per pair), from the packed poses in c->xp and, with a static grid, the
+// cell counts binned beside them.
+void enqueue_detect(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool static_grid) {
     hipStream_t s = c->stream;
-    const SimParams P = make_sim_params(c->params);
-    BodiesDev B = c->B;
     const uint32_t n = c->n;
-    const bool static_grid = P.box.periodic && c->ngrid > 0;
-    if (c->nspring)   // spring joints add forces before integration (world.cpp:134)
-        k_joint_springs<<<blocks_for(c->ncomp, 64), 64, 0, s>>>(B, P, joint_dev(c));
-    {
-        KTimer kt(c, NE_KERNEL_INTEGRATE);
-        const bool w = wrap_first && P.box.periodic && c->njoints == 0 && !P.sleep;
-        if (c->has_forces || c->nspring) {
-            if (w) launch_integrate<true, true>(c, P, static_grid);
-            else launch_integrate<true, false>(c, P, static_grid);
-            c->has_forces = false;
-        } else {
-            if (w) launch_integrate<false, true>(c, P, static_grid);
-            else launch_integrate<false, false>(c, P, static_grid);
-        }
-    }
     // --- grid
     if (c->ngrid > 0) {
         if (!static_grid) {
@@ -4331,6 +4433,29 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
         }
         if (fork && hipStreamWaitEvent(s, c->ev_join, 0) != hipSuccess) (void)hipStreamSynchronize(c->side);
     }
+}
+
+void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
+    hipStream_t s = c->stream;
+    const SimParams P = make_sim_params(c->params);
+    BodiesDev B = c->B;
+    const uint32_t n = c->n;
+    const bool static_grid = P.box.periodic && c->ngrid > 0;
+    if (c->nspring)   // spring joints add forces before integration (world.cpp:134)
+        k_joint_springs<<<blocks_for(c->ncomp, 64), 64, 0, s>>>(B, P, joint_dev(c));
+    {
+        KTimer kt(c, NE_KERNEL_INTEGRATE);
+        const bool w = wrap_first && P.box.periodic && c->njoints == 0 && !P.sleep && !c->events_on;
+        if (c->has_forces || c->nspring) {
+            if (w) launch_integrate<true, true>(c, P, static_grid);
+            else launch_integrate<true, false>(c, P, static_grid);
+            c->has_forces = false;
+        } else {
+            if (w) launch_integrate<false, true>(c, P, static_grid);
+            else launch_integrate<false, false>(c, P, static_grid);
+        }
+    }
+    enqueue_detect(c, B, P, static_grid);
     if (P.sleep) {   // wakes in pair order (world.cpp:527-531), then the woken bodies' masses
         k_sleep_reset<<<blocks_for(n), TPB, 0, s>>>(B, c->wake_at, c->wake_again);
         for (int r = 0; r <= WAKE_ROUNDS; ++r)
@@ -4500,6 +4625,79 @@ int enqueue_solve(ne_gpu_ctx* c) {
     return solve_large_islands(c, P);
 }
 
+// Event lists for `cap` pairs.  Growing keeps the last step's list (the
+// current one is rewritten by the pass).
+int ensure_events(ne_gpu_ctx* c, uint32_t cap) {
+    if (!c->ev_n) {
+        HIP_TRY(c, dalloc(c->ev_n, 2));
+        HIP_TRY(c, dalloc(c->ev_saved, CNT_N));
+        HIP_TRY(c, hipMemsetAsync(c->ev_n, 0, sizeof(uint32_t) * 2, c->stream));
+    }
+    if (cap <= c->ev_cap) return NE_OK;
+    const int prev = c->ev_cur;
+    const uint32_t keep = c->h_ev_n[prev];
+    for (int b = 0; b < 2; ++b) {
+        uint64_t* k = nullptr;
+        uint32_t *t = nullptr, *st = nullptr;
+        double* d = nullptr;
+        HIP_TRY(c, dalloc(k, cap));
+        HIP_TRY(c, dalloc(t, cap));
+        HIP_TRY(c, dalloc(st, cap));
+        HIP_TRY(c, dalloc(d, (size_t)7 * cap));
+        if (b == prev && keep) {
+            HIP_TRY(c, hipMemcpyAsync(k, c->ev_key[b], sizeof(uint64_t) * keep, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(t, c->ev_trig[b], sizeof(uint32_t) * keep, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(st, c->ev_state[b], sizeof(uint32_t) * keep, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(c, hipMemcpyAsync(d, c->ev_dat[b], sizeof(double) * 7 * keep, hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+        }
+        dfree(c->ev_key[b]); dfree(c->ev_trig[b]); dfree(c->ev_state[b]); dfree(c->ev_dat[b]);
+        c->ev_key[b] = k; c->ev_trig[b] = t; c->ev_state[b] = st; c->ev_dat[b] = d;
+    }
+    HIP_TRY(c, dalloc(c->ev_off, (size_t)cap + 1));
+    c->ev_cap = cap;
+    return NE_OK;
+}
+
+// The facade events pass after a step (PhysicsWorld::processContacts,
+// src/physics_world.cpp:253-302): the step's grid and narrow-phase stages on
+// the post-step state with the facade's filter and one contact per pair,
+// then the gather into the other list and the diff against the last one.
+// The step's counters are kept (the pass adds only faults, EPA counts and
+// its pair count), so a fault here reruns the call like one in the step.
+int enqueue_events(ne_gpu_ctx* c) {
+    if (c->nmeshes)
+        return set_err(c, NE_ERR_UNSUPPORTED, "facade events with mesh bodies: the reference's processContacts runs "
+                                              "detectContact on them as cylinders of their radius and half height");
+    int r = ensure_events(c, c->pcap);
+    if (r != NE_OK) return r;
+    hipStream_t s = c->stream;
+    SimParams P = make_sim_params(c->params);
+    P.no_manifold = 1;
+    BodiesDev B = c->B;
+    B.ev_filter = 1;
+    const int timed = c->timed_kernel;   // the step's stage timers stay off in this pass
+    c->timed_kernel = NE_KERNEL_NONE;
+    HIP_TRY(c, hipMemcpyAsync(c->ev_saved, c->counters, sizeof(uint32_t) * CNT_N, hipMemcpyDeviceToDevice, s));
+    const bool static_grid = P.box.periodic && c->ngrid > 0;
+    if (static_grid) k_ev_bin<true><<<blocks_for(c->n), TPB, 0, s>>>(B, c->sgrid, c->xp, c->cell_count);
+    else k_ev_bin<false><<<blocks_for(c->n), TPB, 0, s>>>(B, c->sgrid, c->xp, c->cell_count);
+    enqueue_detect(c, B, P, static_grid);
+    scan_u32(c->ncont, c->ev_off, c->counters + CNT_P, c->pcap, c->scan_part, s, false,
+             ScanTotal{c->counters, CNT_EV, CNT_EV, CNT_FAULT, c->pcap, NE_FAULT_PAIRS, -1, -1});
+    const int cur = c->ev_cur ^ 1, prev = c->ev_cur;
+    k_ev_gather<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, c->counters, c->cand_i, c->cand_j, c->ncont, c->ev_off,
+                                                        c->ct, c->ev_key[cur], c->ev_trig[cur], c->ev_dat[cur],
+                                                        c->ev_n + cur);
+    k_ev_diff<<<hint_blocks(c->hint_p), TPB, 0, s>>>(c->ev_n, cur, prev, c->ev_key[cur], c->ev_trig[cur],
+                                                      c->ev_key[prev], c->ev_trig[prev], c->ev_state[cur],
+                                                      c->ev_state[prev]);
+    k_ev_restore<<<1, 64, 0, s>>>(c->counters, c->ev_saved);
+    c->timed_kernel = timed;
+    HIP_TRY(c, hipGetLastError());
+    return NE_OK;
+}
+
 int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     if (c->n == 0) return NE_OK;
     KTimer whole(c, NE_KERNEL_STEP);
@@ -4515,7 +4713,7 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     if (!P.box.periodic && !P.open_boundary) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_walls<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P);
-    } else if (P.box.periodic && (c->njoints || P.sleep)) {
+    } else if (P.box.periodic && (c->njoints || P.sleep || c->events_on)) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_wrap<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P);
     }
@@ -4672,7 +4870,7 @@ int prepare_static_grid(ne_gpu_ctx* c) {
 // cell-order buffers in (the fill reads the previous order and writes the new
 // one), so one graph is kept per buffer parity and per wrap-first flag.
 int run_step(ne_gpu_ctx* c, bool wrap_first) {
-    const bool graphable = c->use_graph && !c->has_forces && c->timed_kernel == NE_KERNEL_NONE;
+    const bool graphable = c->use_graph && !c->has_forces && c->timed_kernel == NE_KERNEL_NONE && !c->events_on;
     if (!graphable) return enqueue_step(c, wrap_first);
     // launch sizes follow the counts seen when captured: recapture when they drift
     auto drift = [](uint32_t now, uint32_t then) { return now > 2u * then + 1024u || 2u * now + 1024u < then; };
@@ -5084,10 +5282,75 @@ int ne_gpu_set_params(ne_gpu_ctx* c, const ne_world_params* p) {
     return NE_OK;
 }
 
+int reset_events(ne_gpu_ctx* c) {
+    c->h_ev_n[0] = c->h_ev_n[1] = 0;
+    if (c->ev_n) {
+        HIP_TRY(c, hipMemsetAsync(c->ev_n, 0, sizeof(uint32_t) * 2, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    return NE_OK;
+}
+
 int ne_gpu_set_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     if (!c || !b) return NE_ERR_INVALID;
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
-    return upload_bodies(c, b);
+    const int r = upload_bodies(c, b);
+    if (r != NE_OK) return r;
+    return reset_events(c);   // a new world: no last-step pairs
+}
+
+int ne_gpu_set_events(ne_gpu_ctx* c, int32_t enable) {
+    if (!c) return NE_ERR_INVALID;
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    if (enable && c->nmeshes)
+        return set_err(c, NE_ERR_UNSUPPORTED, "facade events with mesh bodies: the reference's processContacts runs "
+                                              "detectContact on them as cylinders of their radius and half height");
+    if (enable && c->nworlds)
+        return set_err(c, NE_ERR_UNSUPPORTED, "facade events are for one world (PhysicsWorld), not batched worlds");
+    c->events_on = enable != 0;
+    drop_graphs(c);
+    return reset_events(c);   // the facade's maps start empty
+}
+
+int ne_gpu_get_events(ne_gpu_ctx* c, int32_t kind, uint32_t cap, uint32_t* count, uint32_t* ei, uint32_t* ej,
+                      double* point, double* normal, double* depth) {
+    if (!c || !count || kind < NE_EVENT_CONTACT || kind > NE_EVENT_TRIGGER_EXIT) return NE_ERR_INVALID;
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    const bool from_prev = kind == NE_EVENT_END || kind == NE_EVENT_TRIGGER_EXIT;
+    const uint32_t trig = kind >= NE_EVENT_TRIGGER_ENTER ? 1u : 0u;
+    const int b = from_prev ? (c->ev_cur ^ 1) : c->ev_cur;
+    const uint32_t n = c->h_ev_n[b];
+    *count = 0;
+    if (!c->events_on || n == 0 || !c->ev_key[b]) return NE_OK;
+    std::vector<uint64_t> key(n);
+    std::vector<uint32_t> tr(n), st(n);
+    std::vector<double> dat(7 * (size_t)n);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(key.data(), c->ev_key[b], sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(tr.data(), c->ev_trig[b], sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(st.data(), c->ev_state[b], sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(dat.data(), c->ev_dat[b], sizeof(double) * 7 * n, hipMemcpyDeviceToHost));
+    uint32_t m = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (tr[k] != trig) continue;
+        bool take;
+        if (from_prev) take = st[k] == 0;                                    // gone this step
+        else if (kind == NE_EVENT_CONTACT) take = true;                      // every solid pair
+        else if (kind == NE_EVENT_BEGIN || kind == NE_EVENT_TRIGGER_ENTER) take = st[k] == 0;
+        else take = st[k] != 0;                                              // stay
+        if (!take) continue;
+        if (m < cap) {
+            if (ei) ei[m] = (uint32_t)(key[k] >> 32);
+            if (ej) ej[m] = (uint32_t)(key[k] & 0xFFFFFFFFu);
+            const double* d = dat.data() + 7 * (size_t)k;
+            if (normal) { normal[3 * m] = d[0]; normal[3 * m + 1] = d[1]; normal[3 * m + 2] = d[2]; }
+            if (point) { point[3 * m] = d[3]; point[3 * m + 1] = d[4]; point[3 * m + 2] = d[5]; }
+            if (depth) depth[m] = d[6];
+        }
+        ++m;
+    }
+    *count = m;
+    return NE_OK;
 }
 
 int ne_gpu_set_joints(ne_gpu_ctx* c, uint32_t n, const ne_distance_joint* j) {
@@ -5184,6 +5447,8 @@ int ne_gpu_step(ne_gpu_ctx* c, int32_t nsteps) {
     if (nsteps < 0) return set_err(c, NE_ERR_INVALID, "nsteps < 0");
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
     if (c->n == 0 || nsteps == 0) return NE_OK;
+    if (c->events_on && nsteps != 1)
+        return set_err(c, NE_ERR_INVALID, "with facade events on, ne_gpu_step takes one step per call");
     int r = snapshot_take(c);
     if (r != NE_OK) return r;
     const bool forces = c->has_forces;
@@ -5191,6 +5456,10 @@ int ne_gpu_step(ne_gpu_ctx* c, int32_t nsteps) {
         r = prepare_static_grid(c);
         if (r != NE_OK) return r;
         r = step_call(c, nsteps);
+        if ((r == NE_OK || r == NE_WARN_EPA_CAPPED) && c->events_on) {   // the step's list becomes the last one
+            c->ev_cur ^= 1;
+            c->h_ev_n[c->ev_cur] = c->h_counters[CNT_EV];
+        }
         if (r == NE_OK && c->h_counters[CNT_EPATRUNC]) {
             char buf[160];
             snprintf(buf, sizeof buf, "%u EPA run(s) stopped at the %d-face workspace cap; their contacts can differ "
@@ -5216,8 +5485,12 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps) {
     for (int32_t t = 0; t < nsteps; ++t) {
         int r = run_step(c, t > 0);
         if (r != NE_OK) return r;
+        if (c->events_on) {
+            r = enqueue_events(c);
+            if (r != NE_OK) return r;
+        }
     }
-    if (c->params.periodic && nsteps > 0 && c->njoints == 0 && !c->params.sleep_enabled) {   // else each step wrapped itself
+    if (c->params.periodic && nsteps > 0 && c->njoints == 0 && !c->params.sleep_enabled && !c->events_on) {   // else each step wrapped itself
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_wrap<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, make_sim_params(c->params));
     }

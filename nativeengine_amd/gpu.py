@@ -32,6 +32,7 @@ EXPORTS = [
     "ne_gpu_import_warm_cache", "ne_gpu_set_worlds", "ne_gpu_use_graph", "ne_gpu_autotune", "ne_gpu_epa_stats",
     "ne_gpu_rollout", "ne_gpu_rollout_backward", "ne_gpu_set_joints", "ne_gpu_set_meshes", "ne_gpu_mesh_bvh",
     "ne_gpu_get_sleep_store", "ne_gpu_save_state", "ne_gpu_load_state", "ne_gpu_save_state_noalloc",
+    "ne_gpu_set_events", "ne_gpu_get_events",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
@@ -91,6 +92,8 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_save_state.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
     lib.ne_gpu_load_state.argtypes = [vp, vp, C.c_uint64]
     lib.ne_gpu_save_state_noalloc.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_uint64)]
+    lib.ne_gpu_set_events.argtypes = [vp, C.c_int32]
+    lib.ne_gpu_get_events.argtypes = [vp, C.c_int32, C.c_uint32, u32p, u32p, u32p, dp, dp, dp]
     lib.ne_gpu_get_contacts.argtypes = [vp, C.c_uint32, u32p, u32p, u32p, dp, dp, dp, dp]
     lib.ne_gpu_get_pairs.argtypes = [vp, C.c_uint32, u32p, u32p, u32p]
     lib.ne_gpu_diagnostics.argtypes = [vp, dp, dp, dp]
@@ -320,6 +323,24 @@ class GpuWorld:
         """Restores an NES1 snapshot (PhysicsWorld::loadState); resets the warm cache."""
         buf = C.create_string_buffer(bytes(blob), len(blob))
         self._check(self.lib.ne_gpu_load_state(self.h, buf, len(blob)))
+
+    def set_events(self, enable: bool = True):
+        """Facade contact / trigger events after every step
+        (PhysicsWorld::processContacts); one step per call while on."""
+        self._check(self.lib.ne_gpu_set_events(self.h, 1 if enable else 0))
+
+    def events(self, kind: int):
+        """The last step's events of one kind (_abi.NE_EVENT_*): dict of i, j,
+        point, normal, depth, in (i, j) order."""
+        cnt = C.c_uint32(0)
+        self._check(self.lib.ne_gpu_get_events(self.h, int(kind), 0, C.byref(cnt), None, None, None, None, None))
+        m = cnt.value
+        i, j = np.zeros(m, np.uint32), np.zeros(m, np.uint32)
+        pt, nrm, dep = np.zeros((m, 3)), np.zeros((m, 3)), np.zeros(m)
+        if m:
+            self._check(self.lib.ne_gpu_get_events(self.h, int(kind), m, C.byref(cnt), _abi.uptr(i), _abi.uptr(j),
+                                                   _abi.dptr(pt), _abi.dptr(nrm), _abi.dptr(dep)))
+        return dict(i=i, j=j, point=pt, normal=nrm, depth=dep)
 
     def sleep_store(self):
         """(Body::invMassStore (n), Body::invInertiaStore (n, 3)) of every body."""

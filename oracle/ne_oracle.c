@@ -164,6 +164,18 @@ struct ne_oracle {
     /* one pair's contacts (gatherContacts' output vector) */
     uint32_t capcs;
     ocontact* cs;
+    /* facade contact events: this step's and the last step's overlapping
+     * pairs (physics_world.cpp:253-302), each in (i, j) order */
+    int events_on;
+    uint32_t nev[2], capev[2];
+    struct oevent* ev[2];
+    int ev_cur;
+};
+
+struct oevent {
+    uint32_t i, j;
+    int trigger;   /* a sensor is in the pair: curTriggers, else curContacts */
+    ocontact c;
 };
 
 struct otape_pair {
@@ -1398,6 +1410,48 @@ static void world_step(ne_oracle* o) {                                          
     if (o->P.sleep_enabled) update_sleep(o);
 }
 
+/* PhysicsWorld::processContacts (physics_world.cpp:253-302), step 1: every
+ * pair i < j of the post-step state that passes the facade's filters and
+ * overlaps under detectContact (one contact, no box-box manifold), split by
+ * whether a sensor is in it.  The diff against the last step is done when
+ * the lists are read. */
+static void process_contacts(ne_oracle* o) {
+    const int cur = o->ev_cur ^ 1;
+    o->nev[cur] = 0;
+    for (uint32_t i = 0; i < o->n; ++i) {
+        for (uint32_t j = i + 1; j < o->n; ++j) {
+            const obody* a = &o->B[i];
+            const obody* b = &o->B[j];
+            const int any_sensor = ((a->flags | b->flags) & NE_FLAG_SENSOR) != 0;
+            if (!any_sensor && a->invMass + b->invMass <= 0.0) continue;
+            if (!layers_collide(a, b)) continue;
+            if (pair_excluded(o, a, j)) continue;
+            const ocontact c = detect_contact(o, a, b);
+            if (!c.hit) continue;
+            if (o->nev[cur] == o->capev[cur]) {
+                o->capev[cur] = o->capev[cur] ? 2 * o->capev[cur] : 256;
+                o->ev[cur] = (struct oevent*)realloc(o->ev[cur], sizeof(struct oevent) * o->capev[cur]);
+            }
+            struct oevent* e = &o->ev[cur][o->nev[cur]++];
+            e->i = i; e->j = j; e->trigger = any_sensor; e->c = c;
+        }
+    }
+    o->ev_cur = cur;
+}
+
+/* Whether list l holds pair (i, j) with the same trigger flag (the lists are
+ * in (i, j) order: binary search). */
+static int ev_has(const ne_oracle* o, int l, uint32_t i, uint32_t j, int trigger) {
+    uint32_t lo = 0, hi = o->nev[l];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        const struct oevent* e = &o->ev[l][mid];
+        if (e->i < i || (e->i == i && e->j < j)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < o->nev[l] && o->ev[l][lo].i == i && o->ev[l][lo].j == j && o->ev[l][lo].trigger == trigger;
+}
+
 /* ------------------------------------------------------------------------ */
 /* C API                                                                     */
 /* ------------------------------------------------------------------------ */
@@ -1440,6 +1494,7 @@ int ne_oracle_destroy(ne_oracle* o) {
     free(o->tape);
     free(o->joints);
     free(o->isp);
+    free(o->ev[0]); free(o->ev[1]);
     free(o);
     return NE_OK;
 }
@@ -1455,6 +1510,7 @@ int ne_oracle_set_params(ne_oracle* o, const ne_world_params* p) {
 
 int ne_oracle_set_bodies(ne_oracle* o, const ne_bodies* b) {
     if (!o || !b) return NE_ERR_INVALID;
+    o->nev[0] = o->nev[1] = 0;   /* a new world: no last-step pairs */
     free_meshes(o);
     free(o->B); free(o->hull); free(o->nocol);
     o->B = NULL; o->hull = NULL; o->nocol = NULL; o->nhull = 0; o->nnocol = 0;
@@ -1517,7 +1573,54 @@ int ne_oracle_set_state(ne_oracle* o, const double* x, const double* v, const do
 
 int ne_oracle_step(ne_oracle* o, int32_t nsteps) {
     if (!o) return NE_ERR_INVALID;
-    for (int32_t s = 0; s < nsteps; ++s) world_step(o);
+    for (int32_t s = 0; s < nsteps; ++s) {
+        world_step(o);
+        if (o->events_on) process_contacts(o);
+    }
+    return NE_OK;
+}
+
+int ne_oracle_set_events(ne_oracle* o, int32_t enable) {
+    if (!o) return NE_ERR_INVALID;
+    o->events_on = enable != 0;
+    o->nev[0] = o->nev[1] = 0;   /* the facade's maps start empty */
+    return NE_OK;
+}
+
+/* The step's event lists (physics_world.cpp:277-299): contact = every solid
+ * pair; begin / stay = solid pairs absent from / present in the last step's
+ * solid list; end = the last step's solid pairs absent now, with their last
+ * contact; the trigger kinds likewise over the sensor pairs.  In (i, j)
+ * order, which is the reference's map order when handles follow indices. */
+int ne_oracle_get_events(ne_oracle* o, int32_t kind, uint32_t cap, uint32_t* count, uint32_t* ci, uint32_t* cj,
+                         double* point, double* normal, double* depth) {
+    if (!o || !count || kind < NE_EVENT_CONTACT || kind > NE_EVENT_TRIGGER_EXIT) return NE_ERR_INVALID;
+    const int cur = o->ev_cur, prev = cur ^ 1;
+    const int trig = kind >= NE_EVENT_TRIGGER_ENTER;
+    const int from_prev = (kind == NE_EVENT_END || kind == NE_EVENT_TRIGGER_EXIT);
+    const int l = from_prev ? prev : cur;
+    uint32_t m = 0;
+    for (uint32_t k = 0; k < o->nev[l]; ++k) {
+        const struct oevent* e = &o->ev[l][k];
+        if (e->trigger != trig) continue;
+        int take;
+        if (from_prev) take = !ev_has(o, cur, e->i, e->j, trig);
+        else if (kind == NE_EVENT_CONTACT) take = 1;
+        else {
+            const int was = ev_has(o, prev, e->i, e->j, trig);
+            take = (kind == NE_EVENT_BEGIN || kind == NE_EVENT_TRIGGER_ENTER) ? !was : was;
+        }
+        if (!take) continue;
+        if (m < cap) {
+            if (ci) ci[m] = e->i;
+            if (cj) cj[m] = e->j;
+            if (point) { point[3 * m] = e->c.point.x; point[3 * m + 1] = e->c.point.y; point[3 * m + 2] = e->c.point.z; }
+            if (normal) { normal[3 * m] = e->c.normal.x; normal[3 * m + 1] = e->c.normal.y; normal[3 * m + 2] = e->c.normal.z; }
+            if (depth) depth[m] = e->c.overlap;
+        }
+        ++m;
+    }
+    *count = m;
     return NE_OK;
 }
 

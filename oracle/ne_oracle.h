@@ -52,6 +52,11 @@ int ne_oracle_set_meshes(ne_oracle *o, uint32_t n, const ne_mesh *meshes);
 int ne_oracle_mesh_bvh(ne_oracle *o, uint32_t m, int32_t *order, uint32_t node_cap, uint32_t *nnodes,
                        double *bounds, int32_t *links);
 void ne_oracle_epa_stats_reset(int32_t faces, int32_t horizon);
+/* Facade contact events after every step (PhysicsWorld::processContacts,
+ * src/physics_world.cpp:253-302); kinds NE_EVENT_*, pairs in (i, j) order. */
+int ne_oracle_set_events(ne_oracle *o, int32_t enable);
+int ne_oracle_get_events(ne_oracle *o, int32_t kind, uint32_t cap, uint32_t *count, uint32_t *i, uint32_t *j,
+                         double *point, double *normal, double *depth);
 void ne_oracle_epa_counts(int64_t *calls, int64_t *big);
 
 #ifdef __cplusplus

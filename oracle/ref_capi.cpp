@@ -23,6 +23,9 @@
 //   - `PhysicsWorld::saveState` / `loadState` over a copy of the world, for
 //     the "NES1" snapshot bytes                         (src/physics_world.cpp:22-64,
 //                                                        physics_world.hpp:379)
+//   - `PhysicsWorld::step` with its contact callbacks set, for the facade's
+//     contact / trigger events                          (physics_world.hpp:218-246,
+//                                                        physics_world.cpp:253-302)
 // and exposes them through the same flat-array layout as include/ne_gpu.h so a
 // test can feed one scene to the GPU, the C restatement and the reference.
 #include <cmath>
@@ -40,9 +43,16 @@
 
 namespace {
 
+struct RefEvent {
+    int kind;   // NE_EVENT_*
+    ne::ContactInfo c;
+};
+
 struct RefWorld {
     ne::World w;
     std::vector<std::shared_ptr<const ne::MeshData>> meshes;   // in ne_ref_set_meshes order
+    std::unique_ptr<ne::PhysicsWorld> pw;                       // the facade, once events are on
+    std::vector<RefEvent> events;                                // the last facade step's callbacks
 };
 
 ne::V3 v3(const double* p, uint32_t i) { return ne::V3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
@@ -385,6 +395,61 @@ int ne_ref_make_body(int32_t shape, const double* dims, uint32_t nverts, const d
     half_extents[0] = b.halfExtents.x; half_extents[1] = b.halfExtents.y; half_extents[2] = b.halfExtents.z;
     *radius = b.radius;
     *half_height = b.halfHeight;
+    return NE_OK;
+}
+
+// Facade events: moves the world into a PhysicsWorld (one handle per body,
+// handle = index + 1, as add() hands them out in order) with every contact
+// and trigger callback recording into the event list.  The world must have
+// no distance joints (PhysicsWorld::step rebuilds them from its own).
+int ne_ref_facade_begin(void* h) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || !r->w.distanceJoints.empty()) return NE_ERR_INVALID;
+    r->pw = std::make_unique<ne::PhysicsWorld>();
+    for (size_t k = 0; k < r->w.bodies.size(); ++k) r->pw->addSphere(ne::V3{0.0, 0.0, 0.0}, 0.5, 1.0);
+    r->pw->world() = r->w;
+    auto rec = [r](int kind) { return [r, kind](const ne::ContactInfo& c) { r->events.push_back({kind, c}); }; };
+    r->pw->setContactCallback(rec(NE_EVENT_CONTACT));
+    r->pw->setContactBeginCallback(rec(NE_EVENT_BEGIN));
+    r->pw->setContactStayCallback(rec(NE_EVENT_STAY));
+    r->pw->setContactEndCallback(rec(NE_EVENT_END));
+    r->pw->setTriggerEnterCallback(rec(NE_EVENT_TRIGGER_ENTER));
+    r->pw->setTriggerStayCallback(rec(NE_EVENT_TRIGGER_STAY));
+    r->pw->setTriggerExitCallback(rec(NE_EVENT_TRIGGER_EXIT));
+    return NE_OK;
+}
+
+// PhysicsWorld::step, nsteps times; the event list keeps the last step's.
+int ne_ref_facade_step(void* h, int32_t nsteps) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || !r->pw) return NE_ERR_INVALID;
+    for (int32_t t = 0; t < nsteps; ++t) {
+        r->events.clear();
+        r->pw->step();
+    }
+    r->w = r->pw->world();
+    return NE_OK;
+}
+
+// The last facade step's events of one kind, in callback order, as body
+// indices (handle - 1).
+int ne_ref_facade_events(void* h, int32_t kind, uint32_t cap, uint32_t* count, uint32_t* ci, uint32_t* cj,
+                         double* point, double* normal, double* depth) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || !count) return NE_ERR_INVALID;
+    uint32_t m = 0;
+    for (const RefEvent& e : r->events) {
+        if (e.kind != kind) continue;
+        if (m < cap) {
+            if (ci) ci[m] = e.c.a - 1;
+            if (cj) cj[m] = e.c.b - 1;
+            if (point) { point[3 * m] = e.c.point.x; point[3 * m + 1] = e.c.point.y; point[3 * m + 2] = e.c.point.z; }
+            if (normal) { normal[3 * m] = e.c.normal.x; normal[3 * m + 1] = e.c.normal.y; normal[3 * m + 2] = e.c.normal.z; }
+            if (depth) depth[m] = e.c.depth;
+        }
+        ++m;
+    }
+    *count = m;
     return NE_OK;
 }
 

@@ -81,14 +81,38 @@ def _load(kind):
         lib.ne_oracle_set_state.argtypes = [_vp, _dp, _dp, _dp, _dp]
         lib.ne_oracle_candidate_count.argtypes = [_vp, _u32p]
         lib.ne_oracle_tape.argtypes = [_vp, C.c_int32]
+        lib.ne_oracle_set_events.argtypes = [_vp, C.c_int32]
+        lib.ne_oracle_get_events.argtypes = [_vp, C.c_int32, C.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
     else:
         lib = _bind(C.CDLL(REF_LIB), "ne_ref_")
         lib.ne_ref_get_contacts.argtypes = [_vp, C.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
         lib.ne_ref_set_capture.argtypes = [_vp, C.c_int]
         lib.ne_ref_save_state.argtypes = [_vp, _vp, C.c_uint64, C.POINTER(C.c_uint64)]
         lib.ne_ref_load_state.argtypes = [_vp, _vp, C.c_uint64]
+        lib.ne_ref_facade_begin.argtypes = [_vp]
+        lib.ne_ref_facade_step.argtypes = [_vp, C.c_int32]
+        lib.ne_ref_facade_events.argtypes = [_vp, C.c_int32, C.c_uint32, _u32p, _u32p, _u32p, _dp, _dp, _dp]
     _libs[kind] = lib
     return lib
+
+
+EVENT_KINDS = ["contact", "begin", "stay", "end", "trigger_enter", "trigger_stay", "trigger_exit"]
+
+
+def events_of(fn, h, kind):
+    """One kind of facade event as a dict of i, j, point, normal, depth."""
+    cnt = C.c_uint32(0)
+    rc = fn(h, kind, 0, C.byref(cnt), None, None, None, None, None)
+    if rc != 0:
+        raise RuntimeError(f"events failed: {rc}")
+    m = cnt.value
+    i, j = np.zeros(m, np.uint32), np.zeros(m, np.uint32)
+    pt, nrm, dep = np.zeros((m, 3)), np.zeros((m, 3)), np.zeros(m)
+    if m:
+        rc = fn(h, kind, m, C.byref(cnt), _abi.uptr(i), _abi.uptr(j), _abi.dptr(pt), _abi.dptr(nrm), _abi.dptr(dep))
+        if rc != 0:
+            raise RuntimeError(f"events failed: {rc}")
+    return dict(i=i, j=j, point=pt, normal=nrm, depth=dep)
 
 
 def body_aux(w, n):
@@ -228,6 +252,13 @@ class OracleWorld(_CpuWorld):
         arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (x, v, q, w)]
         self._check(self.lib.ne_oracle_set_state(self.h, *[None if a is None else _abi.dptr(a) for a in arrs]), "set_state")
 
+    def set_events(self, on=True):
+        self._check(self.lib.ne_oracle_set_events(self.h, 1 if on else 0), "set_events")
+
+    def events(self, kind):
+        """The last step's facade events of one kind (NE_EVENT_*)."""
+        return events_of(self.lib.ne_oracle_get_events, self.h, kind)
+
     def tape(self, mode):
         """Contact-set tape: 1 records each following step's contact lists, 2
         replays them from the first recorded step (fixed contact set), 0 off."""
@@ -253,6 +284,18 @@ class RefWorld(_CpuWorld):
     def load_state(self, blob: bytes):
         buf = C.create_string_buffer(bytes(blob), len(blob))
         self._check(self.lib.ne_ref_load_state(self.h, buf, len(blob)), "load_state")
+
+    def facade_begin(self):
+        """Step through PhysicsWorld from now on, recording its contact and
+        trigger callbacks."""
+        self._check(self.lib.ne_ref_facade_begin(self.h), "facade_begin")
+
+    def facade_step(self, nsteps=1):
+        self._check(self.lib.ne_ref_facade_step(self.h, int(nsteps)), "facade_step")
+
+    def events(self, kind):
+        """The last facade step's callbacks of one kind, as body indices."""
+        return events_of(self.lib.ne_ref_facade_events, self.h, kind)
 
     def contacts(self):
         cnt = C.c_uint32(0)

@@ -51,3 +51,22 @@ def load_narrow():
     z = np.load(os.path.join(GOLDEN, "narrow_pairs.npz"), allow_pickle=False)
     s = _scene_from(z)
     return s, z["pairs"], z["count"], z["offset"], z["normal"], z["point"], z["depth"]
+
+
+def load_events(name):
+    """A facade-event fixture: (scene, steps, events) with events[t][kind] a
+    dict of i, j, point, normal, depth for step t (0-based)."""
+    z = np.load(os.path.join(GOLDEN, f"events_{name}.npz"), allow_pickle=False)
+    s = _scene_from(z)
+    steps = int(z["steps"][0])
+    off = z["ev_offset"]
+    nk = (len(off) - 1) // steps
+    ev = []
+    for t in range(steps):
+        row = []
+        for k in range(nk):
+            a, b = int(off[t * nk + k]), int(off[t * nk + k + 1])
+            row.append({f: z[f"ev_{f}"][a:b] for f in ("i", "j", "point", "normal", "depth")})
+        ev.append(row)
+    out = {k[4:]: z[k] for k in z.files if k.startswith("out_")}
+    return s, steps, ev, out

@@ -7,8 +7,9 @@ world parameters, and what the reference produced -- final body state, the
 last step's constraint list and the per-step constraint counts; plus
 narrow-phase vectors (random poses per shape pair -> contacts); the
 elastic gas of 4096 spheres over 1000 steps with the reference's momentum,
-angular momentum and kinetic energy every 100 steps; and "NES1" snapshot
-bytes from the reference's own PhysicsWorld::saveState / loadState.
+angular momentum and kinetic energy every 100 steps; "NES1" snapshot bytes
+from the reference's own PhysicsWorld::saveState / loadState; and the
+facade's contact / trigger events from PhysicsWorld::step with callbacks.
 
     python tests/golden/make_fixtures.py [--only NAME ...]
 """
@@ -195,6 +196,34 @@ def nes1_vectors(every=60, saves=4, load_at=150, after_load=(30, 60, 90)):
     return len(blob)
 
 
+def event_vectors(name, s, steps):
+    """The reference's facade events (PhysicsWorld::step with every contact
+    and trigger callback set, physics_world.cpp:253-302) for `steps` steps:
+    per step and kind (NE_EVENT_*), the pairs as body indices and their
+    contact, concatenated with offsets."""
+    r = engines.RefWorld()
+    r.set_scene(s)
+    r.facade_begin()
+    arrays = scene_arrays(s)
+    cols = {f: [] for f in ("i", "j", "point", "normal", "depth")}
+    off = [0]
+    for _ in range(steps):
+        r.facade_step(1)
+        for k in range(len(engines.EVENT_KINDS)):
+            e = r.events(k)
+            for f in cols:
+                cols[f].append(e[f])
+            off.append(off[-1] + len(e["i"]))
+    for f in cols:
+        arrays[f"ev_{f}"] = np.concatenate(cols[f])
+    arrays["ev_offset"] = np.array(off, np.uint32)
+    arrays["steps"] = np.array([steps], np.int32)
+    x, v, q, w = r.state()
+    arrays.update(out_x=x, out_v=v, out_q=q, out_w=w)
+    np.savez_compressed(os.path.join(HERE, f"events_{name}.npz"), **arrays)
+    return int(off[-1])
+
+
 def main():
     if not engines.ref_available():
         sys.exit("oracle/_ref/libne_ref.so is not built (make -C oracle ref)")
@@ -210,6 +239,10 @@ def main():
     if not only or "nes1" in only:
         size = nes1_vectors()
         print(f"nes1_sleeping_pile: {size} bytes per snapshot")
+    if not only or "events" in only:
+        for name, (s, steps) in scenes.event_scenes().items():
+            total = event_vectors(name, s, steps)
+            print(f"events_{name}: {s.n} bodies, {steps} steps, {total} events")
 
 
 if __name__ == "__main__":

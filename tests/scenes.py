@@ -135,3 +135,17 @@ def epa_blowup_pair():
                         restitution=h("0x1.869dcfc14a600p-10"), friction=h("0x1.a750454fa41b2p-2"),
                         solver_iterations=8)
     return s
+
+
+def event_scenes():
+    """name -> (scene, steps) for the facade-event fixtures: the feature scene
+    (sensors give trigger events; layers, no-collide lists, a plane, a
+    kinematic body, cylinders) and a periodic pile with island sleeping and
+    a sensor box in it (sleeping bodies, whose stored masses the facade's
+    filter does not see)."""
+    out = {"features": (features_scene(2), 60)}
+    p = S.sleeping_pile(60, seed=3, periodic=True)
+    k = int(np.argmax(p.shape == _abi.SHAPE_BOX))
+    p.flags[k] |= np.uint32(_abi.FLAG_SENSOR)
+    out["sleeping_sensor"] = (p, 240)
+    return out
