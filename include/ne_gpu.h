@@ -322,6 +322,64 @@ int ne_gpu_set_events(ne_gpu_ctx *ctx, int32_t enable);
 int ne_gpu_get_events(ne_gpu_ctx *ctx, int32_t kind, uint32_t cap, uint32_t *count, uint32_t *i, uint32_t *j,
                       double *point, double *normal, double *depth);
 
+/* ---- scene queries: PhysicsWorld (src/physics_world.cpp:146-251) ---------
+ * Batched over many rays or query shapes, against the current state on the
+ * device (after the last ne_gpu_step).  Bodies are found by index; the facade
+ * maps index -> handle and userData.  A mask passes body b when
+ * mask & (1 << (b.layer & 31)) (layerAllowed, physics_world.cpp:67); a NULL
+ * mask array passes every layer.  Each periodic body is tested at its
+ * minimum image nearest the ray's origin (or the query shape).  One world
+ * only: batched worlds are NE_ERR_UNSUPPORTED. */
+
+/* Nearest hit of each of n rays (PhysicsWorld::raycast, :146-164), or of n
+ * swept spheres when cast_radius is not NULL (PhysicsWorld::sphereCast,
+ * :229-251: every body inflated by the ray's radius, the point moved back
+ * onto the real surface).  origin and dir are n*3 (dir is normalised here),
+ * max_dist n.  A body wins when its distance is strictly below the best so
+ * far, in body order, exactly as the reference's loop.  Only spheres, boxes,
+ * cylinders and capsules are hit (rayVsBody, src/queries.hpp:61-122).
+ * Outputs per ray: body (-1: no hit, and distance, point and normal 0),
+ * distance, point n*3, outward normal n*3. */
+int ne_gpu_raycast(ne_gpu_ctx *ctx, uint32_t n, const double *origin, const double *dir,
+                   const double *max_dist, const uint32_t *mask, const double *cast_radius,
+                   int32_t *body, double *distance, double *point, double *normal);
+
+/* Every hit of each of n rays within max_dist (PhysicsWorld::raycastAll,
+ * :166-188), ray k's hits at [offsets[k], offsets[k+1]) sorted by distance
+ * (stable: equal distances keep body order -- the reference's std::sort order
+ * whenever a ray's distances differ or it has at most 16 hits).  offsets has
+ * n+1 entries and is always written; when offsets[n] > cap nothing else is
+ * and the call returns NE_ERR_CAPACITY (call again with cap >= offsets[n]). */
+int ne_gpu_raycast_all(ne_gpu_ctx *ctx, uint32_t n, const double *origin, const double *dir,
+                       const double *max_dist, const uint32_t *mask, uint32_t cap, uint32_t *offsets,
+                       int32_t *body, double *distance, double *point, double *normal);
+
+/* A query body for ne_gpu_overlap: a shape at a pose, as a `Body` the
+ * reference would build (makeSphere / makeBox for overlapSphere /
+ * overlapBox, any Body for overlapContacts).  Fields as in ne_bodies;
+ * verts: the hull's nverts body-frame vertices for NE_SHAPE_CONVEX. */
+typedef struct ne_query_shape {
+    int32_t       shape;              /* NE_SHAPE_SPHERE .. NE_SHAPE_PLANE     */
+    uint32_t      nverts;
+    double        x[3];
+    double        q[4];               /* w, x, y, z                            */
+    double        half_extents[3];    /* box; plane: normal                    */
+    double        radius, half_height;
+    const double *verts;              /* nverts*3 (convex only)                */
+} ne_query_shape;
+
+/* Bodies overlapping each of n query shapes under detectContact(query, body)
+ * (PhysicsWorld::overlapSphere / overlapBox, :190-212, for sphere and box
+ * shapes; contacts != 0: overlapContacts, :214-227, which also skips sensors
+ * and returns each overlap's normal (body -> query) and depth).  Query k's
+ * bodies, in body order, at [offsets[k], offsets[k+1]); normal (3 per hit)
+ * and depth may be NULL.  Capacity as ne_gpu_raycast_all.  Worlds with mesh
+ * bodies are NE_ERR_UNSUPPORTED (detectContact would take a mesh body for a
+ * cylinder).  May return NE_WARN_EPA_CAPPED (contacts only). */
+int ne_gpu_overlap(ne_gpu_ctx *ctx, uint32_t n, const ne_query_shape *shapes, const uint32_t *mask,
+                   int32_t contacts, uint32_t cap, uint32_t *offsets, int32_t *body, double *normal,
+                   double *depth);
+
 /* Diagnostics of `World` (src/world.cpp:704-729): sums over invMass>0 bodies
  * in index order.  p[3], L[3] and ke may be NULL. */
 int ne_gpu_diagnostics(ne_gpu_ctx *ctx, double *p, double *L, double *ke);
@@ -346,9 +404,11 @@ int ne_gpu_get_stream(ne_gpu_ctx *ctx, void **stream);
 #define NE_KERNEL_SOLVE       8
 #define NE_KERNEL_BOUNDARY    9
 #define NE_KERNEL_STEP       10   /* the whole step, first to last launch */
+#define NE_KERNEL_RAY        11   /* the ray kernel of ne_gpu_raycast */
 
 /* Brackets every launch of one kernel with HIP events on the context's
- * stream during the following ne_gpu_step calls (NE_KERNEL_NONE disables),
+ * stream during the following ne_gpu_step (or, for NE_KERNEL_RAY,
+ * ne_gpu_raycast) calls (NE_KERNEL_NONE disables),
  * and resets the accumulated time.  For live roofline measurement. */
 int ne_gpu_time_kernel(ne_gpu_ctx *ctx, int32_t kernel_id);
 

@@ -121,6 +121,20 @@ class TuneResult(C.Structure):
     ]
 
 
+class QueryShape(C.Structure):
+    """ne_query_shape: a query body for ne_gpu_overlap."""
+    _fields_ = [
+        ("shape", C.c_int32),
+        ("nverts", C.c_uint32),
+        ("x", C.c_double * 3),
+        ("q", C.c_double * 4),
+        ("half_extents", C.c_double * 3),
+        ("radius", C.c_double),
+        ("half_height", C.c_double),
+        ("verts", C.POINTER(C.c_double)),
+    ]
+
+
 def dptr(a):
     """Pointer to a C-contiguous float64 array (or NULL for None)."""
     if a is None:

@@ -3670,6 +3670,11 @@ struct ne_gpu_ctx {
     double* ev_dat[2] = {nullptr, nullptr};
     uint32_t* ev_state[2] = {nullptr, nullptr};
     uint32_t *ev_off = nullptr, *ev_n = nullptr, *ev_saved = nullptr;
+    // scene queries: device arenas, grown on demand and carved per call (a: the
+    // call's inputs and counts, b: its outputs
+    // and w: EPA workspaces of overlapContacts)
+    uint8_t *q_mem_a = nullptr, *q_mem_b = nullptr, *q_mem_w = nullptr;
+    size_t q_cap_a = 0, q_cap_b = 0, q_cap_w = 0;
     GridDev grid_host{};
     uint32_t sgrid_cells = 0;
     // host copies from the last upload (grid plans are rebuilt from them)
@@ -3758,6 +3763,10 @@ struct ne_gpu_ctx {
     double timed_ms = 0.0;
     uint32_t timed_launches = 0;
 };
+
+// Scene queries (rays, swept spheres, overlaps): kernels over the body
+// layout above.
+#include "ne_query.hpp"
 
 namespace {
 
@@ -4273,6 +4282,8 @@ void free_all(ne_gpu_ctx* c) {
     dfree(c->nes1); c->nes1_cap = 0;
     for (int b = 0; b < 2; ++b) { dfree(c->ev_key[b]); dfree(c->ev_trig[b]); dfree(c->ev_dat[b]); dfree(c->ev_state[b]); }
     dfree(c->ev_off); dfree(c->ev_n); dfree(c->ev_saved);
+    dfree(c->q_mem_a); dfree(c->q_mem_b); dfree(c->q_mem_w);
+    c->q_cap_a = c->q_cap_b = c->q_cap_w = 0;
     c->ev_cap = 0;
     c->snap_nd = 0;
     c->snap_fcap = 0;
@@ -5353,6 +5364,384 @@ int ne_gpu_get_events(ne_gpu_ctx* c, int32_t kind, uint32_t cap, uint32_t* count
     return NE_OK;
 }
 
+}  // extern "C"
+
+// ---- scene queries (PhysicsWorld, src/physics_world.cpp:146-251) --------
+
+namespace {
+
+
+// Carves 256-byte aligned pieces from an arena; with base == nullptr it only
+// measures.
+struct Carve {
+    uint8_t* base = nullptr;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += sizeof(T) * (count ? count : 1);
+        return p;
+    }
+};
+
+hipError_t ensure_arena(uint8_t*& mem, size_t& cap, size_t need) {
+    if (mem && cap >= need) return hipSuccess;
+    dfree(mem);
+    cap = 0;
+    const hipError_t e = hipMalloc((void**)&mem, need);
+    if (e == hipSuccess) cap = need;
+    return e;
+}
+
+int query_world_check(ne_gpu_ctx* c) {
+    if (c->nworlds) return set_err(c, NE_ERR_UNSUPPORTED, "scene queries are for one world (PhysicsWorld), not batched worlds");
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    return NE_OK;
+}
+
+// The rays' inputs on the device, as structure of arrays.
+struct RayUpload {
+    RayIn in{};
+    double* tiles = nullptr;
+    bool cull = false;
+};
+
+int upload_rays(ne_gpu_ctx* c, Carve& cv, uint32_t n, const double* origin, const double* dir, const double* max_dist,
+                const uint32_t* mask, const double* rad, RayUpload& u) {
+    const uint32_t ntiles = (c->n + RAY_THREADS - 1) / RAY_THREADS;
+    double* f = cv.take<double>((size_t)8 * n);
+    uint32_t* m = mask ? cv.take<uint32_t>(n) : nullptr;
+    u.tiles = cv.take<double>((size_t)6 * (ntiles ? ntiles : 1));
+    if (!cv.base) return NE_OK;
+    std::vector<double> h((size_t)8 * n);
+    for (uint32_t k = 0; k < n; ++k) {
+        for (int a = 0; a < 3; ++a) {
+            h[(size_t)a * n + k] = origin[3 * (size_t)k + a];
+            h[(size_t)(3 + a) * n + k] = dir[3 * (size_t)k + a];
+        }
+        h[(size_t)6 * n + k] = max_dist[k];
+        h[(size_t)7 * n + k] = rad ? rad[k] : 0.0;
+    }
+    HIP_TRY(c, hipMemcpyAsync(f, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+    if (m) HIP_TRY(c, hipMemcpyAsync(m, mask, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream));
+    u.in = RayIn{f, f + n, f + 2 * (size_t)n, f + 3 * (size_t)n, f + 4 * (size_t)n, f + 5 * (size_t)n,
+                 f + 6 * (size_t)n, f + 7 * (size_t)n, m};
+    // Tiles are skipped by their bounding boxes in a closed box; a periodic
+    // world tests every body at its image nearest each ray's origin.
+    u.cull = !make_sim_params(c->params).box.periodic;
+    if (u.cull && ntiles) k_ray_tiles<<<ntiles, RAY_THREADS, 0, c->stream>>>(c->B, u.tiles);
+    return NE_OK;
+}
+
+RayOut carve_ray_out(Carve& cv, size_t m) {
+    RayOut o{};
+    o.body = cv.take<int32_t>(m);
+    double* f = cv.take<double>(7 * m);
+    o.t = f;
+    o.px = f + m; o.py = f + 2 * m; o.pz = f + 3 * m;
+    o.nx = f + 4 * m; o.ny = f + 5 * m; o.nz = f + 6 * m;
+    return o;
+}
+
+// Device results (m hits) back into the caller's packed arrays at `at`.
+int download_hits(ne_gpu_ctx* c, const RayOut& o, size_t m, size_t at, int32_t* body, double* distance, double* point,
+                  double* normal) {
+    if (m == 0) return NE_OK;
+    std::vector<double> h(7 * m);
+    HIP_TRY(c, hipMemcpyAsync(body + at, o.body, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(h.data(), o.t, sizeof(double) * 7 * m, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < m; ++k) {
+        distance[at + k] = h[k];
+        for (int a = 0; a < 3; ++a) {
+            point[3 * (at + k) + a] = h[(1 + a) * m + k];
+            normal[3 * (at + k) + a] = h[(4 + a) * m + k];
+        }
+    }
+    return NE_OK;
+}
+
+template <int MODE, bool CAST>
+void launch_ray(ne_gpu_ctx* c, uint32_t n, const RayUpload& u, const RayOut& o) {
+    const Box3 box = make_sim_params(c->params).box;
+    const uint32_t blocks = (n + RAY_THREADS - 1) / RAY_THREADS;
+    if (u.cull)
+        k_ray<MODE, CAST, true><<<blocks, RAY_THREADS, 0, c->stream>>>(c->B, box, n, u.in, u.tiles, o);
+    else
+        k_ray<MODE, CAST, false><<<blocks, RAY_THREADS, 0, c->stream>>>(c->B, box, n, u.in, u.tiles, o);
+}
+
+// EPA of the queued overlapContacts pairs: one thread per pair with 256
+// faces and 128 horizon edges each (chunks of OVL_CHUNK pairs sharing the
+// workspace), then the runs that did not fit, one block per pair with the
+// context's face cap (OVL_BIG_BLOCKS blocks at a time).  *capped_out: runs
+// stopped at the cap.
+constexpr int OVL_FACES = 256;
+constexpr uint32_t OVL_CHUNK = 8192, OVL_BIG_BLOCKS = 32;
+
+size_t ovl_epa_bytes(const ne_gpu_ctx* c) {
+    const size_t small = OVL_CHUNK * (sizeof(SupV) * EPA_MAXV + sizeof(Face) * OVL_FACES + sizeof(int) * OVL_FACES);
+    const size_t big = OVL_BIG_BLOCKS * (size_t)c->epa_faces_max * (sizeof(Face) + 3 * sizeof(uint16_t));
+    return std::max(small, big) + 3 * 256;
+}
+
+int ovl_epa(ne_gpu_ctx* c, const QueryDev* Q, uint32_t* ent, uint32_t nent, uint32_t* ent2, uint32_t* ctr,
+            double* nrm, double* dep, uint8_t* ws, uint32_t* capped_out) {
+    const Box3 box = make_sim_params(c->params).box;
+    *capped_out = 0;
+    HIP_TRY(c, hipMemsetAsync(ctr, 0, 2 * sizeof(uint32_t), c->stream));
+    {
+        const uint32_t ch = std::min(OVL_CHUNK, nent);
+        Carve w{ws, 0};
+        SupV* wv = w.take<SupV>((size_t)ch * EPA_MAXV);
+        Face* wf = w.take<Face>((size_t)ch * OVL_FACES);
+        int(*wh)[2] = reinterpret_cast<int(*)[2]>(w.take<int>((size_t)ch * OVL_FACES));
+        for (uint32_t b = 0; b < nent; b += ch) {
+            const uint32_t m = std::min(ch, nent - b);
+            k_ovl_epa<<<(m + 63) / 64, 64, 0, c->stream>>>(c->B, box, Q, ent + 3 * (size_t)b, m, wv, wf, wh, OVL_FACES,
+                                                          OVL_FACES / 2, nrm, dep, ent2, ctr);
+        }
+    }
+    HIP_TRY(c, hipGetLastError());
+    uint32_t nbig = 0;
+    HIP_TRY(c, hipMemcpyAsync(&nbig, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (nbig) {
+        Carve w{ws, 0};
+        const int fp = c->epa_faces_max;
+        Face* faces = w.take<Face>((size_t)OVL_BIG_BLOCKS * fp);
+        uint16_t* occ = w.take<uint16_t>((size_t)OVL_BIG_BLOCKS * 3 * fp);
+        k_ovl_big<<<std::min(nbig, OVL_BIG_BLOCKS), 64 * BIG_NW, 0, c->stream>>>(c->B, box, Q, ent2, nbig, faces, occ,
+                                                                                fp, nrm, dep, ctr + 1);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(capped_out, ctr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    return NE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ne_gpu_raycast(ne_gpu_ctx* c, uint32_t n, const double* origin, const double* dir, const double* max_dist,
+                   const uint32_t* mask, const double* cast_radius, int32_t* body, double* distance, double* point,
+                   double* normal) {
+    if (!c || (n && (!origin || !dir || !max_dist || !body || !distance || !point || !normal))) return NE_ERR_INVALID;
+    int r = query_world_check(c);
+    if (r != NE_OK || n == 0) return r;
+    RayUpload u;
+    Carve cv;
+    upload_rays(c, cv, n, origin, dir, max_dist, mask, cast_radius, u);
+    carve_ray_out(cv, n);
+    HIP_TRY(c, ensure_arena(c->q_mem_a, c->q_cap_a, cv.off));
+    cv = Carve{c->q_mem_a, 0};
+    if ((r = upload_rays(c, cv, n, origin, dir, max_dist, mask, cast_radius, u)) != NE_OK) return r;
+    const RayOut o = carve_ray_out(cv, n);
+    {
+        KTimer kt(c, NE_KERNEL_RAY);
+        if (cast_radius) launch_ray<0, true>(c, n, u, o);
+        else launch_ray<0, false>(c, n, u, o);
+    }
+    HIP_TRY(c, hipGetLastError());
+    if ((r = download_hits(c, o, n, 0, body, distance, point, normal)) != NE_OK) return r;
+    harvest_timing(c);
+    return NE_OK;
+}
+
+int ne_gpu_raycast_all(ne_gpu_ctx* c, uint32_t n, const double* origin, const double* dir, const double* max_dist,
+                       const uint32_t* mask, uint32_t cap, uint32_t* offsets, int32_t* body, double* distance,
+                       double* point, double* normal) {
+    if (!c || !offsets || (n && (!origin || !dir || !max_dist))) return NE_ERR_INVALID;
+    offsets[0] = 0;
+    int r = query_world_check(c);
+    if (r != NE_OK || n == 0) return r;
+    RayUpload u;
+    Carve cv;
+    upload_rays(c, cv, n, origin, dir, max_dist, mask, nullptr, u);
+    cv.take<uint32_t>(n);
+    cv.take<uint32_t>((size_t)n + 1);
+    cv.take<uint32_t>(1025);
+    HIP_TRY(c, ensure_arena(c->q_mem_a, c->q_cap_a, cv.off));
+    cv = Carve{c->q_mem_a, 0};
+    if ((r = upload_rays(c, cv, n, origin, dir, max_dist, mask, nullptr, u)) != NE_OK) return r;
+    uint32_t* cnt = cv.take<uint32_t>(n);
+    uint32_t* off = cv.take<uint32_t>((size_t)n + 1);
+    uint32_t* part = cv.take<uint32_t>(1025);
+    RayOut o{};
+    o.count = cnt;
+    launch_ray<1, false>(c, n, u, o);
+    scan_u32(cnt, off, nullptr, n, part, c->stream);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipMemcpyAsync(offsets, off, sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const size_t total = offsets[n];
+    if (total > cap) return set_err(c, NE_ERR_CAPACITY, "raycast_all: more hits than cap (offsets[n] holds the count)");
+    if (total == 0) return NE_OK;
+    if (!body || !distance || !point || !normal) return NE_ERR_INVALID;
+    Carve cb;
+    carve_ray_out(cb, total);
+    HIP_TRY(c, ensure_arena(c->q_mem_b, c->q_cap_b, cb.off));
+    cb = Carve{c->q_mem_b, 0};
+    RayOut e = carve_ray_out(cb, total);
+    e.off = off;
+    launch_ray<2, false>(c, n, u, e);
+    HIP_TRY(c, hipGetLastError());
+    return download_hits(c, e, total, 0, body, distance, point, normal);
+}
+
+int ne_gpu_overlap(ne_gpu_ctx* c, uint32_t n, const ne_query_shape* shapes, const uint32_t* mask, int32_t contacts,
+                   uint32_t cap, uint32_t* offsets, int32_t* body, double* normal, double* depth) {
+    if (!c || !offsets || (n && !shapes)) return NE_ERR_INVALID;
+    offsets[0] = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const ne_query_shape& s = shapes[k];
+        if (s.shape < NE_SHAPE_SPHERE || s.shape > NE_SHAPE_PLANE) return set_err(c, NE_ERR_INVALID, "query shape kind");
+        if (s.shape == NE_SHAPE_CONVEX && (!s.verts || s.nverts == 0))
+            return set_err(c, NE_ERR_INVALID, "convex query shape without vertices");
+    }
+    int r = query_world_check(c);
+    if (r != NE_OK) return r;
+    if (c->nmeshes)
+        return set_err(c, NE_ERR_UNSUPPORTED, "overlap queries with mesh bodies: detectContact would take a mesh body "
+                                              "for a cylinder of its radius and half height");
+    if (n == 0) return NE_OK;
+    if (c->n == 0) {
+        for (uint32_t k = 0; k < n; ++k) offsets[k + 1] = 0;
+        return NE_OK;
+    }
+    const uint32_t nchunks = (c->n + OVL_THREADS - 1) / OVL_THREADS;
+    const uint32_t qb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>({(uint64_t)n, 65535u, (16u << 20) / nchunks}));
+    size_t nverts = 0;
+    for (uint32_t k = 0; k < n; ++k)
+        if (shapes[k].shape == NE_SHAPE_CONVEX) nverts += shapes[k].nverts;
+    auto carve_a = [&](Carve& cv, QueryDev*& Q, double*& hull, uint32_t*& cnt, uint32_t*& off, uint32_t*& part,
+                       uint32_t*& qoff, uint32_t*& ctr) {
+        Q = cv.take<QueryDev>(n);
+        hull = cv.take<double>(3 * nverts);
+        cnt = cv.take<uint32_t>((size_t)qb * nchunks);
+        off = cv.take<uint32_t>((size_t)qb * nchunks + 1);
+        part = cv.take<uint32_t>(1025);
+        qoff = cv.take<uint32_t>((size_t)qb + 1);
+        ctr = cv.take<uint32_t>(4);
+    };
+    QueryDev* Q;
+    double* hull;
+    uint32_t *cnt, *off, *part, *qoff, *ctr;
+    Carve cv;
+    carve_a(cv, Q, hull, cnt, off, part, qoff, ctr);
+    HIP_TRY(c, ensure_arena(c->q_mem_a, c->q_cap_a, cv.off));
+    cv = Carve{c->q_mem_a, 0};
+    carve_a(cv, Q, hull, cnt, off, part, qoff, ctr);
+    // query geometry: the Body the reference builds (fields as given), its
+    // hull in device memory, and a bounding radius padded for the pair test
+    std::vector<QueryDev> hq(n);
+    std::vector<double> hv(3 * nverts);
+    size_t vat = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const ne_query_shape& s = shapes[k];
+        QueryDev& d = hq[k];
+        d.g.x = V3{s.x[0], s.x[1], s.x[2]};
+        d.g.q = Qt{s.q[0], s.q[1], s.q[2], s.q[3]};
+        d.g.he = V3{s.half_extents[0], s.half_extents[1], s.half_extents[2]};
+        d.g.radius = s.radius;
+        d.g.hh = s.half_height;
+        d.g.shape = s.shape;
+        d.g.hull = nullptr;
+        d.g.nhull = 0;
+        if (s.shape == NE_SHAPE_CONVEX) {
+            std::memcpy(hv.data() + 3 * vat, s.verts, sizeof(double) * 3 * s.nverts);
+            d.g.hull = hull + 3 * vat;
+            d.g.nhull = s.nverts;
+            vat += s.nverts;
+        }
+        const double q2 = s.q[0] * s.q[0] + s.q[1] * s.q[1] + s.q[2] * s.q[2] + s.q[3] * s.q[3];
+        d.bR = std::fabs(q2 - 1.0) > 1e-6 ? 1e30
+                                          : host_bounding_radius(s.shape, s.radius, s.half_height, s.half_extents,
+                                                                 s.verts, 0, s.shape == NE_SHAPE_CONVEX ? s.nverts : 0);
+        d.mask = mask ? mask[k] : 0xFFFFFFFFu;
+    }
+    HIP_TRY(c, hipMemcpyAsync(Q, hq.data(), sizeof(QueryDev) * n, hipMemcpyHostToDevice, c->stream));
+    if (nverts) HIP_TRY(c, hipMemcpyAsync(hull, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, c->stream));
+    const Box3 box = make_sim_params(c->params).box;
+    // pass 1: hits per query (every batch), into offsets
+    std::vector<uint32_t> h_qoff((size_t)qb + 1);
+    auto count_batch = [&](uint32_t q0, uint32_t m) -> int {
+        k_ovl<false><<<dim3(nchunks, m), OVL_THREADS, 0, c->stream>>>(c->B, box, Q, q0, contacts, cnt, nullptr, nullptr,
+                                                                       nullptr, nullptr, nullptr, nullptr);
+        scan_u32(cnt, off, nullptr, m * nchunks, part, c->stream);
+        k_ovl_qoff<<<(m + 1 + 255) / 256, 256, 0, c->stream>>>(off, nchunks, m, qoff);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(h_qoff.data(), qoff, sizeof(uint32_t) * ((size_t)m + 1), hipMemcpyDeviceToHost,
+                                  c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return NE_OK;
+    };
+    uint64_t total = 0, max_batch = 0;
+    for (uint32_t q0 = 0; q0 < n; q0 += qb) {
+        const uint32_t m = std::min(qb, n - q0);
+        if ((r = count_batch(q0, m)) != NE_OK) return r;
+        for (uint32_t t = 1; t <= m; ++t) {
+            const uint64_t v = total + h_qoff[t];
+            offsets[q0 + t] = (uint32_t)std::min<uint64_t>(v, 0xFFFFFFFFu);
+        }
+        total += h_qoff[m];
+        max_batch = std::max<uint64_t>(max_batch, h_qoff[m]);
+    }
+    if (total > cap) return set_err(c, NE_ERR_CAPACITY, "overlap: more hits than cap (offsets[n] holds the count)");
+    if (total == 0) return NE_OK;
+    if (!body) return NE_ERR_INVALID;
+    // pass 2: bodies in order, contacts, EPA of the convex pairs
+    auto carve_b = [&](Carve& w, int32_t*& ob, double*& on, double*& od, uint32_t*& e1, uint32_t*& e2) {
+        ob = w.take<int32_t>(max_batch);
+        on = w.take<double>(3 * max_batch);
+        od = w.take<double>(max_batch);
+        e1 = w.take<uint32_t>(3 * max_batch);
+        e2 = w.take<uint32_t>(3 * max_batch);
+    };
+    int32_t* ob;
+    double *on, *od;
+    uint32_t *e1, *e2;
+    Carve w;
+    carve_b(w, ob, on, od, e1, e2);
+    HIP_TRY(c, ensure_arena(c->q_mem_b, c->q_cap_b, w.off));
+    w = Carve{c->q_mem_b, 0};
+    carve_b(w, ob, on, od, e1, e2);
+    if (contacts) HIP_TRY(c, ensure_arena(c->q_mem_w, c->q_cap_w, ovl_epa_bytes(c)));
+    uint32_t capped_all = 0;
+    uint64_t done = 0;
+    for (uint32_t q0 = 0; q0 < n; q0 += qb) {
+        const uint32_t m = std::min(qb, n - q0);
+        if (n > qb && (r = count_batch(q0, m)) != NE_OK) return r;   // one batch: pass 1's offsets still hold
+        const uint32_t bt = h_qoff[m];
+        if (bt) {
+            if (contacts) HIP_TRY(c, hipMemsetAsync(ctr + 2, 0, sizeof(uint32_t), c->stream));
+            k_ovl<true><<<dim3(nchunks, m), OVL_THREADS, 0, c->stream>>>(c->B, box, Q, q0, contacts, nullptr, off, ob,
+                                                                          on, od, e1, ctr + 2);
+            HIP_TRY(c, hipGetLastError());
+            if (contacts) {
+                uint32_t ne = 0;
+                HIP_TRY(c, hipMemcpyAsync(&ne, ctr + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+                HIP_TRY(c, hipStreamSynchronize(c->stream));
+                if (ne) {
+                    uint32_t capped = 0;
+                    if ((r = ovl_epa(c, Q, e1, ne, e2, ctr, on, od, c->q_mem_w, &capped)) != NE_OK) return r;
+                    capped_all += capped;
+                }
+            }
+            HIP_TRY(c, hipMemcpyAsync(body + done, ob, sizeof(int32_t) * bt, hipMemcpyDeviceToHost, c->stream));
+            if (contacts && normal)
+                HIP_TRY(c, hipMemcpyAsync(normal + 3 * done, on, sizeof(double) * 3 * bt, hipMemcpyDeviceToHost, c->stream));
+            if (contacts && depth)
+                HIP_TRY(c, hipMemcpyAsync(depth + done, od, sizeof(double) * bt, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+        }
+        done += bt;
+    }
+    return capped_all ? NE_WARN_EPA_CAPPED : NE_OK;
+}
+
 int ne_gpu_set_joints(ne_gpu_ctx* c, uint32_t n, const ne_distance_joint* j) {
     if (!c || (n && !j)) return NE_ERR_INVALID;
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
@@ -5969,7 +6358,7 @@ const char* ne_gpu_last_error(ne_gpu_ctx* c) {
 }
 
 int ne_gpu_time_kernel(ne_gpu_ctx* c, int32_t kernel_id) {
-    if (!c || kernel_id < 0 || kernel_id > NE_KERNEL_STEP) return NE_ERR_INVALID;
+    if (!c || kernel_id < 0 || kernel_id > NE_KERNEL_RAY) return NE_ERR_INVALID;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->ev_used = 0;
     c->timed_kernel = kernel_id;

@@ -16,6 +16,7 @@ import weakref
 import numpy as np
 
 from . import _abi
+from . import queries
 from .scene import Params, Scene
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -32,11 +33,11 @@ EXPORTS = [
     "ne_gpu_import_warm_cache", "ne_gpu_set_worlds", "ne_gpu_use_graph", "ne_gpu_autotune", "ne_gpu_epa_stats",
     "ne_gpu_rollout", "ne_gpu_rollout_backward", "ne_gpu_set_joints", "ne_gpu_set_meshes", "ne_gpu_mesh_bvh",
     "ne_gpu_get_sleep_store", "ne_gpu_save_state", "ne_gpu_load_state", "ne_gpu_save_state_noalloc",
-    "ne_gpu_set_events", "ne_gpu_get_events",
+    "ne_gpu_set_events", "ne_gpu_get_events", "ne_gpu_raycast", "ne_gpu_raycast_all", "ne_gpu_overlap",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
-           "cons_build": 7, "solve": 8, "boundary": 9, "step": 10}
+           "cons_build": 7, "solve": 8, "boundary": 9, "step": 10, "ray": 11}
 
 
 class NeGpuError(RuntimeError):
@@ -114,6 +115,7 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_rollout.argtypes = [vp, vp, dp, dp, C.c_int32, C.c_int32, vp]
     lib.ne_gpu_rollout_backward.argtypes = [vp, vp, vp, dp, dp]
     lib.ne_gpu_autotune.argtypes = [vp, C.c_int32, C.POINTER(_abi.TuneResult)]
+    queries.bind(lib, "ne_gpu")
     for name in EXPORTS:
         if name != "ne_gpu_last_error":
             getattr(lib, name).restype = C.c_int
@@ -341,6 +343,44 @@ class GpuWorld:
             self._check(self.lib.ne_gpu_get_events(self.h, int(kind), m, C.byref(cnt), _abi.uptr(i), _abi.uptr(j),
                                                    _abi.dptr(pt), _abi.dptr(nrm), _abi.dptr(dep)))
         return dict(i=i, j=j, point=pt, normal=nrm, depth=dep)
+
+    # -- scene queries (PhysicsWorld, src/physics_world.cpp:146-251) --------
+    def raycast(self, origin, direction, max_dist, mask=None) -> dict:
+        """Nearest hit per ray (PhysicsWorld::raycast): dict of body (-1: no
+        hit), distance, point, normal; origin / direction (n, 3), max_dist
+        and mask per ray or scalar."""
+        return queries.raycast(self.lib.ne_gpu_raycast, self.h, self._check, origin, direction, max_dist, mask)
+
+    def sphere_cast(self, origin, radius, direction, max_dist, mask=None) -> dict:
+        """Nearest hit per swept sphere (PhysicsWorld::sphereCast)."""
+        return queries.raycast(self.lib.ne_gpu_raycast, self.h, self._check, origin, direction, max_dist, mask,
+                               radius=radius)
+
+    def raycast_all(self, origin, direction, max_dist, mask=None) -> dict:
+        """Every hit per ray, sorted by distance (PhysicsWorld::raycastAll):
+        CSR dict (offsets, body, distance, point, normal)."""
+        return queries.raycast_all(self.lib.ne_gpu_raycast_all, self.h, self._check, origin, direction, max_dist,
+                                   mask)
+
+    def overlap(self, shapes, mask=None, contacts=False) -> dict:
+        """Bodies overlapping each query shape (queries.Shape), in body order:
+        CSR dict (offsets, body, normal, depth).  Sphere / box shapes without
+        contacts are overlapSphere / overlapBox; with contacts,
+        overlapContacts (sensors skipped, normal body -> query, depth)."""
+        return queries.overlap(self.lib.ne_gpu_overlap, self.h, self._check_step, shapes, mask, contacts)
+
+    def overlap_sphere(self, centers, radii, mask=None) -> list:
+        """PhysicsWorld::overlapSphere per (center, radius): one index array each."""
+        c = np.atleast_2d(np.asarray(centers, np.float64))
+        r = np.broadcast_to(np.asarray(radii, np.float64), (len(c),))
+        return queries.split(self.overlap([queries.sphere(c[k], r[k]) for k in range(len(c))], mask))
+
+    def overlap_box(self, centers, rots, halves, mask=None) -> list:
+        """PhysicsWorld::overlapBox per (center, rot, half): one index array each."""
+        c = np.atleast_2d(np.asarray(centers, np.float64))
+        q = np.broadcast_to(np.asarray(rots, np.float64), (len(c), 4))
+        hv = np.broadcast_to(np.asarray(halves, np.float64), (len(c), 3))
+        return queries.split(self.overlap([queries.box(c[k], q[k], hv[k]) for k in range(len(c))], mask))
 
     def sleep_store(self):
         """(Body::invMassStore (n), Body::invInertiaStore (n, 3)) of every body."""

@@ -1835,3 +1835,298 @@ int ne_oracle_wrap(ne_oracle* o, const double* p, double* out) {
     out[0] = m.x; out[1] = m.y; out[2] = m.z;
     return NE_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Scene queries: PhysicsWorld::raycast / raycastAll / overlapSphere /      */
+/* overlapBox / overlapContacts / sphereCast (physics_world.cpp:146-251)    */
+/* over rayVsBody (queries.hpp:18-122).                                     */
+/* ------------------------------------------------------------------------ */
+
+static int layer_allowed(uint32_t mask, const obody* b) {                         /* physics_world.cpp:67 */
+    return (mask & (1u << (b->layer & 31u))) != 0;
+}
+
+/* queries.hpp:18-31 */
+static int ray_vs_sphere_at(v3 o, v3 d, v3 c, double r, double maxT, double* tOut, v3* nOut) {
+    v3 m = sub(o, c);
+    double b = dot(m, d), cc = dot(m, m) - r * r;
+    if (cc > 0.0 && b > 0.0) return 0;
+    double disc = b * b - cc;
+    if (disc < 0.0) return 0;
+    double t = -b - sqrt(disc);
+    if (t < 0.0) t = 0.0;
+    if (t > maxT) return 0;
+    *tOut = t;
+    *nOut = normalized(sub(add(o, scl(d, t)), c));
+    return 1;
+}
+
+/* queries.hpp:35-59 */
+static int ray_vs_box_local(v3 ol, v3 dl, v3 he, double maxT, double* tOut, v3* nLocal) {
+    double tmin = 0.0, tmax = maxT;
+    int axis = -1;
+    double sign = 1.0;
+    const double o[3] = {ol.x, ol.y, ol.z}, dd[3] = {dl.x, dl.y, dl.z}, h[3] = {he.x, he.y, he.z};
+    for (int a = 0; a < 3; ++a) {
+        if (fabs(dd[a]) < 1e-12) {
+            if (o[a] < -h[a] || o[a] > h[a]) return 0;
+        } else {
+            double inv = 1.0 / dd[a];
+            double t1 = (-h[a] - o[a]) * inv, t2 = (h[a] - o[a]) * inv;
+            double s = -1.0;
+            if (t1 > t2) { double tmp = t1; t1 = t2; t2 = tmp; s = 1.0; }
+            if (t1 > tmin) { tmin = t1; axis = a; sign = s; }
+            if (t2 < tmax) tmax = t2;
+            if (tmin > tmax) return 0;
+        }
+    }
+    if (axis < 0) return 0;
+    *tOut = tmin;
+    *nLocal = mk(axis == 0 ? sign : 0.0, axis == 1 ? sign : 0.0, axis == 2 ? sign : 0.0);
+    return 1;
+}
+
+/* queries.hpp:61-122: a body of shape, orientation q, half extents he,
+ * radius r and half height h, centred at c. */
+static int ray_vs_body(int shape, qt q, v3 he, double r, double h, v3 c, v3 o, v3 d, double maxT,
+                       double* tOut, v3* pOut, v3* nOut) {
+    if (shape == NE_SHAPE_CONVEX || shape == NE_SHAPE_PLANE || shape == NE_SHAPE_MESH) return 0;
+    if (shape == NE_SHAPE_SPHERE) {
+        if (!ray_vs_sphere_at(o, d, c, r, maxT, tOut, nOut)) return 0;
+        *pOut = add(o, scl(d, *tOut));
+        return 1;
+    }
+    v3 ol = qinvrot(q, sub(o, c));
+    v3 dl = qinvrot(q, d);
+    if (shape == NE_SHAPE_BOX) {
+        v3 nl;
+        if (!ray_vs_box_local(ol, dl, he, maxT, tOut, &nl)) return 0;
+        *nOut = qrot(q, nl);
+        *pOut = add(o, scl(d, *tOut));
+        return 1;
+    }
+    double best = maxT + 1.0;
+    v3 nl = mk(0.0, 0.0, 0.0);
+    double a2 = dl.x * dl.x + dl.z * dl.z;
+    double b2 = 2.0 * (ol.x * dl.x + ol.z * dl.z);
+    double c2 = ol.x * ol.x + ol.z * ol.z - r * r;
+    if (a2 > 1e-18) {
+        double disc = b2 * b2 - 4.0 * a2 * c2;
+        if (disc >= 0.0) {
+            double t = (-b2 - sqrt(disc)) / (2.0 * a2);
+            if (t < 0.0) t = (-b2 + sqrt(disc)) / (2.0 * a2);
+            if (t >= 0.0 && t <= maxT) {
+                double y = ol.y + t * dl.y;
+                if (y >= -h && y <= h && t < best) {
+                    best = t;
+                    nl = mk((ol.x + t * dl.x) / r, 0.0, (ol.z + t * dl.z) / r);
+                }
+            }
+        }
+    }
+    if (shape == NE_SHAPE_CYLINDER) {
+        for (int s = -1; s <= 1; s += 2) {
+            double yc = s * h;
+            if (fabs(dl.y) < 1e-12) continue;
+            double t = (yc - ol.y) / dl.y;
+            if (t < 0.0 || t > maxT || t >= best) continue;
+            double px = ol.x + t * dl.x, pz = ol.z + t * dl.z;
+            if (px * px + pz * pz <= r * r) { best = t; nl = mk(0.0, (double)s, 0.0); }
+        }
+    } else if (best > maxT) {   /* capsule: the cap spheres only when the side missed */
+        for (int s = -1; s <= 1; s += 2) {
+            double tS;
+            v3 nS;
+            if (ray_vs_sphere_at(ol, dl, mk(0.0, (double)s * h, 0.0), r, maxT, &tS, &nS) && tS < best) {
+                best = tS;
+                nl = nS;
+            }
+        }
+    }
+    if (best > maxT) return 0;
+    *tOut = best;
+    *nOut = normalized(qrot(q, nl));
+    *pOut = add(o, scl(d, *tOut));
+    return 1;
+}
+
+/* Body b as rayVsBody sees it, inflated by R for sphereCast
+ * (physics_world.cpp:237-240). */
+static int ray_vs_world_body(const ne_oracle* o, const obody* b, double R, int cast, v3 origin, v3 d, double maxT,
+                             double* t, v3* p, v3* n) {
+    double r = b->radius, h = b->hh;
+    v3 he = b->he;
+    if (cast) {
+        r = b->radius + R;
+        if (b->shape == NE_SHAPE_BOX) he = add(b->he, mk(R, R, R));
+        else if (b->shape == NE_SHAPE_CYLINDER) h = b->hh + R;
+    }
+    v3 cImg = add(origin, min_image(&o->box, sub(b->x, origin)));
+    return ray_vs_body(b->shape, b->q, he, r, h, cImg, origin, d, maxT, t, p, n);
+}
+
+int ne_oracle_raycast(ne_oracle* o, uint32_t n, const double* origin, const double* dir, const double* max_dist,
+                      const uint32_t* mask, const double* cast_radius, int32_t* body, double* distance,
+                      double* point, double* normal) {
+    if (!o || (n && (!origin || !dir || !max_dist || !body || !distance || !point || !normal))) return NE_ERR_INVALID;
+    for (uint32_t k = 0; k < n; ++k) {
+        const v3 org = ld3(origin, k);
+        const v3 d = normalized(ld3(dir, k));
+        const uint32_t m = mask ? mask[k] : 0xFFFFFFFFu;
+        const int cast = cast_radius != NULL;
+        const double R = cast ? cast_radius[k] : 0.0;
+        double best = max_dist[k];
+        int32_t hb = -1;
+        v3 hp = mk(0, 0, 0), hn = mk(0, 0, 0);
+        double hd = 0.0;
+        for (uint32_t i = 0; i < o->n; ++i) {
+            const obody* b = &o->B[i];
+            if (!layer_allowed(m, b)) continue;
+            double t;
+            v3 p, nn;
+            if (ray_vs_world_body(o, b, R, cast, org, d, best, &t, &p, &nn) && t < best) {
+                best = t;
+                hb = (int32_t)i;
+                hd = t;
+                hn = nn;
+                hp = cast ? sub(p, scl(nn, R)) : p;   /* physics_world.cpp:245 */
+            }
+        }
+        body[k] = hb;
+        distance[k] = hd;
+        point[3 * k] = hp.x; point[3 * k + 1] = hp.y; point[3 * k + 2] = hp.z;
+        normal[3 * k] = hn.x; normal[3 * k + 1] = hn.y; normal[3 * k + 2] = hn.z;
+    }
+    return NE_OK;
+}
+
+int ne_oracle_raycast_all(ne_oracle* o, uint32_t n, const double* origin, const double* dir, const double* max_dist,
+                          const uint32_t* mask, uint32_t cap, uint32_t* offsets, int32_t* body, double* distance,
+                          double* point, double* normal) {
+    if (!o || !offsets || (n && (!origin || !dir || !max_dist))) return NE_ERR_INVALID;
+    uint32_t total = 0;
+    offsets[0] = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1 && (total > cap || (total && (!body || !distance || !point || !normal))))
+            return total > cap ? NE_ERR_CAPACITY : NE_ERR_INVALID;
+        for (uint32_t k = 0; k < n; ++k) {
+            const v3 org = ld3(origin, k);
+            const v3 d = normalized(ld3(dir, k));
+            const uint32_t m = mask ? mask[k] : 0xFFFFFFFFu;
+            uint32_t at = pass ? offsets[k] : 0, cnt = 0;
+            for (uint32_t i = 0; i < o->n; ++i) {
+                const obody* b = &o->B[i];
+                if (!layer_allowed(m, b)) continue;
+                double t;
+                v3 p, nn;
+                if (!ray_vs_world_body(o, b, 0.0, 0, org, d, max_dist[k], &t, &p, &nn)) continue;
+                if (pass) {
+                    /* stable insertion by distance (equal distances keep body order) */
+                    uint32_t s = at + cnt;
+                    while (s > at && distance[s - 1] > t) {
+                        body[s] = body[s - 1];
+                        distance[s] = distance[s - 1];
+                        for (int c = 0; c < 3; ++c) {
+                            point[3 * s + c] = point[3 * (s - 1) + c];
+                            normal[3 * s + c] = normal[3 * (s - 1) + c];
+                        }
+                        --s;
+                    }
+                    body[s] = (int32_t)i;
+                    distance[s] = t;
+                    point[3 * s] = p.x; point[3 * s + 1] = p.y; point[3 * s + 2] = p.z;
+                    normal[3 * s] = nn.x; normal[3 * s + 1] = nn.y; normal[3 * s + 2] = nn.z;
+                }
+                ++cnt;
+            }
+            if (!pass) { total += cnt; offsets[k + 1] = total; }
+        }
+    }
+    return NE_OK;
+}
+
+/* The query Body (physics_world.cpp:190-227): makeSphere / makeBox at the
+ * query pose for overlapSphere / overlapBox, or the caller's Body. */
+static int query_body(const ne_query_shape* s, obody* q, v3* hv) {
+    memset(q, 0, sizeof(*q));
+    if (s->shape < NE_SHAPE_SPHERE || s->shape > NE_SHAPE_PLANE) return 0;
+    q->shape = s->shape;
+    q->x = mk(s->x[0], s->x[1], s->x[2]);
+    q->q = mq(s->q[0], s->q[1], s->q[2], s->q[3]);
+    q->he = mk(s->half_extents[0], s->half_extents[1], s->half_extents[2]);
+    q->radius = s->radius;
+    q->hh = s->half_height;
+    q->mesh = -1;
+    if (s->shape == NE_SHAPE_CONVEX) {
+        if (!s->verts || s->nverts == 0) return 0;
+        for (uint32_t k = 0; k < s->nverts; ++k) hv[k] = ld3(s->verts, k);
+        q->hvp = hv;
+        q->hull0 = 0;
+        q->hull1 = s->nverts;
+    }
+    return 1;
+}
+
+/* detectContact(a, b).hit (detect.hpp:33-77): the same dispatch, with the
+ * GJK/EPA fallback reduced to GJK -- epaContact reports a hit whenever GJK
+ * enclosed the origin (epa.hpp:134-158), so the flag needs no EPA run. */
+static int overlap_hit(const ne_oracle* o, const obody* a, const obody* b) {
+    const int plane = a->shape == NE_SHAPE_PLANE || b->shape == NE_SHAPE_PLANE;
+    const int sph = is_sphere(a) || is_sphere(b);
+    const int cyl = a->shape == NE_SHAPE_CYLINDER || b->shape == NE_SHAPE_CYLINDER;
+    const int cap = a->shape == NE_SHAPE_CAPSULE || b->shape == NE_SHAPE_CAPSULE;
+    const int analytic = plane || (is_sphere(a) && is_sphere(b)) || (sph && cyl) ||
+                         (a->shape == NE_SHAPE_CAPSULE && b->shape == NE_SHAPE_CAPSULE) || (sph && cap);
+    if (analytic) return detect_contact(o, a, b).hit;
+    supv tet[4];
+    return gjk_tetra(o, a, b, min_image_shift(&o->box, a, b), tet);   /* convexContact, epa.hpp:153-158 */
+}
+
+int ne_oracle_overlap(ne_oracle* o, uint32_t n, const ne_query_shape* shapes, const uint32_t* mask, int32_t contacts,
+                      uint32_t cap, uint32_t* offsets, int32_t* body, double* normal, double* depth) {
+    if (!o || !offsets || (n && !shapes)) return NE_ERR_INVALID;
+    for (uint32_t i = 0; i < o->n; ++i)
+        if (o->B[i].shape == NE_SHAPE_MESH) return NE_ERR_UNSUPPORTED;
+    uint32_t total = 0, maxv = 0;
+    for (uint32_t k = 0; k < n; ++k)
+        if (shapes[k].shape == NE_SHAPE_CONVEX && shapes[k].nverts > maxv) maxv = shapes[k].nverts;
+    v3* hv = (v3*)malloc(sizeof(v3) * (maxv ? maxv : 1));
+    offsets[0] = 0;
+    int rc = NE_OK;
+    for (int pass = 0; pass < 2 && rc == NE_OK; ++pass) {
+        if (pass == 1 && total > cap) { rc = NE_ERR_CAPACITY; break; }
+        if (pass == 1 && total && !body) { rc = NE_ERR_INVALID; break; }
+        for (uint32_t k = 0; k < n; ++k) {
+            obody q;
+            if (!query_body(&shapes[k], &q, hv)) { rc = NE_ERR_INVALID; break; }
+            const uint32_t m = mask ? mask[k] : 0xFFFFFFFFu;
+            uint32_t at = pass ? offsets[k] : 0, cnt = 0;
+            for (uint32_t i = 0; i < o->n; ++i) {
+                const obody* b = &o->B[i];
+                if (contacts && (b->flags & NE_FLAG_SENSOR)) continue;   /* triggers don't block */
+                if (!layer_allowed(m, b)) continue;
+                ocontact c;
+                if (contacts) {
+                    c = detect_contact(o, &q, b);
+                    if (!c.hit) continue;
+                } else {
+                    if (!overlap_hit(o, &q, b)) continue;
+                    memset(&c, 0, sizeof(c));
+                }
+                if (pass) {
+                    body[at + cnt] = (int32_t)i;
+                    if (normal) {
+                        normal[3 * (at + cnt)] = c.normal.x;
+                        normal[3 * (at + cnt) + 1] = c.normal.y;
+                        normal[3 * (at + cnt) + 2] = c.normal.z;
+                    }
+                    if (depth) depth[at + cnt] = c.overlap;
+                }
+                ++cnt;
+            }
+            if (!pass) { total += cnt; offsets[k + 1] = total; }
+        }
+    }
+    free(hv);
+    return rc;
+}

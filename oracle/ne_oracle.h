@@ -58,6 +58,16 @@ int ne_oracle_set_events(ne_oracle *o, int32_t enable);
 int ne_oracle_get_events(ne_oracle *o, int32_t kind, uint32_t cap, uint32_t *count, uint32_t *i, uint32_t *j,
                          double *point, double *normal, double *depth);
 void ne_oracle_epa_counts(int64_t *calls, int64_t *big);
+/* Scene queries (src/physics_world.cpp:146-251), the signatures of
+ * ne_gpu_raycast / ne_gpu_raycast_all / ne_gpu_overlap. */
+int ne_oracle_raycast(ne_oracle *o, uint32_t n, const double *origin, const double *dir, const double *max_dist,
+                      const uint32_t *mask, const double *cast_radius, int32_t *body, double *distance,
+                      double *point, double *normal);
+int ne_oracle_raycast_all(ne_oracle *o, uint32_t n, const double *origin, const double *dir, const double *max_dist,
+                          const uint32_t *mask, uint32_t cap, uint32_t *offsets, int32_t *body, double *distance,
+                          double *point, double *normal);
+int ne_oracle_overlap(ne_oracle *o, uint32_t n, const ne_query_shape *shapes, const uint32_t *mask, int32_t contacts,
+                      uint32_t cap, uint32_t *offsets, int32_t *body, double *normal, double *depth);
 
 #ifdef __cplusplus
 }

@@ -454,3 +454,110 @@ int ne_ref_facade_events(void* h, int32_t kind, uint32_t cap, uint32_t* count, u
 }
 
 }  // extern "C"
+
+// ---- scene queries (PhysicsWorld, src/physics_world.cpp:146-251) --------
+// The queries run on a PhysicsWorld over the current world: the stepping
+// facade when events are on, else a fresh facade holding a copy (handles
+// 1..n in body order, so index = handle - 1).
+namespace {
+ne::PhysicsWorld& query_facade(RefWorld* r, std::unique_ptr<ne::PhysicsWorld>& tmp) {
+    if (r->pw) return *r->pw;
+    tmp = std::make_unique<ne::PhysicsWorld>();
+    for (size_t k = 0; k < r->w.bodies.size(); ++k) tmp->addSphere(ne::V3{0.0, 0.0, 0.0}, 0.5, 1.0);
+    tmp->world() = r->w;
+    return *tmp;
+}
+
+void put_hit(const ne::RayHit& hit, uint32_t k, int32_t* body, double* distance, double* point, double* normal) {
+    body[k] = hit.hit ? (int32_t)hit.body - 1 : -1;
+    distance[k] = hit.distance;
+    point[3 * k] = hit.point.x; point[3 * k + 1] = hit.point.y; point[3 * k + 2] = hit.point.z;
+    normal[3 * k] = hit.normal.x; normal[3 * k + 1] = hit.normal.y; normal[3 * k + 2] = hit.normal.z;
+}
+}  // namespace
+
+extern "C" {
+int ne_ref_raycast(void* h, uint32_t n, const double* origin, const double* dir, const double* max_dist,
+                   const uint32_t* mask, const double* cast_radius, int32_t* body, double* distance, double* point,
+                   double* normal) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r) return NE_ERR_INVALID;
+    std::unique_ptr<ne::PhysicsWorld> tmp;
+    ne::PhysicsWorld& pw = query_facade(r, tmp);
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t m = mask ? mask[k] : 0xFFFFFFFFu;
+        ne::RayHit hit = cast_radius ? pw.sphereCast(v3(origin, k), cast_radius[k], v3(dir, k), max_dist[k], m)
+                                     : pw.raycast(v3(origin, k), v3(dir, k), max_dist[k], m);
+        put_hit(hit, k, body, distance, point, normal);
+    }
+    return NE_OK;
+}
+
+int ne_ref_raycast_all(void* h, uint32_t n, const double* origin, const double* dir, const double* max_dist,
+                       const uint32_t* mask, uint32_t cap, uint32_t* offsets, int32_t* body, double* distance,
+                       double* point, double* normal) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r) return NE_ERR_INVALID;
+    std::unique_ptr<ne::PhysicsWorld> tmp;
+    ne::PhysicsWorld& pw = query_facade(r, tmp);
+    std::vector<std::vector<ne::RayHit>> all(n);
+    offsets[0] = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        all[k] = pw.raycastAll(v3(origin, k), v3(dir, k), max_dist[k], mask ? mask[k] : 0xFFFFFFFFu);
+        offsets[k + 1] = offsets[k] + (uint32_t)all[k].size();
+    }
+    if (offsets[n] > cap) return NE_ERR_CAPACITY;
+    for (uint32_t k = 0; k < n; ++k)
+        for (size_t e = 0; e < all[k].size(); ++e) put_hit(all[k][e], offsets[k] + (uint32_t)e, body, distance, point, normal);
+    return NE_OK;
+}
+
+// contacts == 0: overlapSphere (sphere shapes) / overlapBox (box shapes);
+// contacts != 0: overlapContacts with the shape as the query Body.
+int ne_ref_overlap(void* h, uint32_t n, const ne_query_shape* s, const uint32_t* mask, int32_t contacts, uint32_t cap,
+                   uint32_t* offsets, int32_t* body, double* normal, double* depth) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r) return NE_ERR_INVALID;
+    std::unique_ptr<ne::PhysicsWorld> tmp;
+    ne::PhysicsWorld& pw = query_facade(r, tmp);
+    std::vector<std::vector<ne::PhysicsWorld::OverlapHit>> all(n);
+    offsets[0] = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t m = mask ? mask[k] : 0xFFFFFFFFu;
+        const ne::V3 x{s[k].x[0], s[k].x[1], s[k].x[2]};
+        const ne::Q q{s[k].q[0], s[k].q[1], s[k].q[2], s[k].q[3]};
+        const ne::V3 he{s[k].half_extents[0], s[k].half_extents[1], s[k].half_extents[2]};
+        if (!contacts) {
+            std::vector<ne::BodyId> ids;
+            if (s[k].shape == NE_SHAPE_SPHERE) ids = pw.overlapSphere(x, s[k].radius, m);
+            else if (s[k].shape == NE_SHAPE_BOX) ids = pw.overlapBox(x, q, he, m);
+            else return NE_ERR_INVALID;
+            for (ne::BodyId id : ids) {
+                ne::PhysicsWorld::OverlapHit oh;
+                oh.body = id;
+                all[k].push_back(oh);
+            }
+        } else {
+            ne::Body b;
+            b.shape = static_cast<ne::Shape>(s[k].shape);
+            b.x = x;
+            b.q = q;
+            b.halfExtents = he;
+            b.radius = s[k].radius;
+            b.halfHeight = s[k].half_height;
+            for (uint32_t v = 0; v < s[k].nverts; ++v) b.vertices.push_back(v3(s[k].verts, v));
+            all[k] = pw.overlapContacts(b, m);
+        }
+        offsets[k + 1] = offsets[k] + (uint32_t)all[k].size();
+    }
+    if (offsets[n] > cap) return NE_ERR_CAPACITY;
+    for (uint32_t k = 0; k < n; ++k)
+        for (size_t e = 0; e < all[k].size(); ++e) {
+            const uint32_t at = offsets[k] + (uint32_t)e;
+            body[at] = (int32_t)all[k][e].body - 1;
+            if (normal) { normal[3 * at] = all[k][e].normal.x; normal[3 * at + 1] = all[k][e].normal.y; normal[3 * at + 2] = all[k][e].normal.z; }
+            if (depth) depth[at] = all[k][e].depth;
+        }
+    return NE_OK;
+}
+}  // extern "C"

@@ -14,6 +14,7 @@ import os
 import numpy as np
 
 from nativeengine_amd import _abi
+from nativeengine_amd import queries
 from nativeengine_amd.scene import Scene
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -42,6 +43,7 @@ def _bind(lib, prefix):
     g("set_joints").argtypes = [_vp, C.c_uint32, _vp]
     g("set_meshes").argtypes = [_vp, C.c_uint32, C.POINTER(_abi.Mesh)]
     g("mesh_bvh").argtypes = [_vp, C.c_uint32, C.POINTER(C.c_int32), C.c_uint32, _u32p, _dp, C.POINTER(C.c_int32)]
+    queries.bind(lib, prefix.rstrip("_"))
     return lib
 
 
@@ -151,6 +153,23 @@ class _CpuWorld:
             self.close()
         except Exception:
             pass
+
+    # scene queries, through the same packing as GpuWorld's
+    def _qcheck(self, rc):
+        self._check(rc, "query")
+
+    def raycast(self, origin, direction, max_dist, mask=None):
+        return queries.raycast(self._f("raycast"), self.h, self._qcheck, origin, direction, max_dist, mask)
+
+    def sphere_cast(self, origin, radius, direction, max_dist, mask=None):
+        return queries.raycast(self._f("raycast"), self.h, self._qcheck, origin, direction, max_dist, mask,
+                               radius=radius)
+
+    def raycast_all(self, origin, direction, max_dist, mask=None):
+        return queries.raycast_all(self._f("raycast_all"), self.h, self._qcheck, origin, direction, max_dist, mask)
+
+    def overlap(self, shapes, mask=None, contacts=False):
+        return queries.overlap(self._f("overlap"), self.h, self._qcheck, shapes, mask, contacts)
 
     def set_params(self, p):
         cp = p.to_c()

@@ -70,3 +70,73 @@ def load_events(name):
         ev.append(row)
     out = {k[4:]: z[k] for k in z.files if k.startswith("out_")}
     return s, steps, ev, out
+
+
+QUERY_NAMES = ["features", "pile_periodic", "special"]
+
+
+def _shapes_from(z, prefix):
+    from nativeengine_amd import queries as Q
+    out = []
+    voff = z[f"{prefix}_voff"]
+    for k in range(len(z[f"{prefix}_shape"])):
+        v = z[f"{prefix}_verts"][voff[k]:voff[k + 1]]
+        out.append(Q.Shape(int(z[f"{prefix}_shape"][k]), tuple(z[f"{prefix}_x"][k]), q=tuple(z[f"{prefix}_q"][k]),
+                           half_extents=tuple(z[f"{prefix}_he"][k]), radius=float(z[f"{prefix}_r"][k]),
+                           half_height=float(z[f"{prefix}_hh"][k]), verts=v.copy() if len(v) else None))
+    return out
+
+
+def load_queries(name):
+    """A scene-query fixture: (scene, state (x, v, q, w) the queries run on,
+    inputs, expected).  inputs: org, dir, max_dist, mask, rad (rays; rad for
+    the swept spheres), shapes0 / mask0 (overlap spheres and boxes), shapes1
+    (overlapContacts bodies).  expected: ray / cast (nearest hits), all
+    (raycastAll, CSR), ovl0 (overlaps, CSR), ovl1 (contacts, CSR)."""
+    z = np.load(os.path.join(GOLDEN, f"queries_{name}.npz"), allow_pickle=False)
+    s = _scene_from(z)
+    state = tuple(z[f"state_{k}"] for k in "xvqw")
+    inputs = dict(org=z["ray_org"], dir=z["ray_dir"], max_dist=z["ray_max"], mask=z["ray_mask"], rad=z["ray_rad"],
+                  shapes0=_shapes_from(z, "ovl0"), mask0=z["ovl0_mask"], shapes1=_shapes_from(z, "ovl1"))
+    hit = lambda p: dict(body=z[f"{p}_body"], distance=z[f"{p}_t"], point=z[f"{p}_p"], normal=z[f"{p}_n"])
+    exp = dict(ray=hit("ray"), cast=hit("cast"),
+               all=dict(offsets=z["all_off"], **hit("all")),
+               ovl0=dict(offsets=z["ovl0_off"], body=z["ovl0_body"]),
+               ovl1=dict(offsets=z["ovl1_off"], body=z["ovl1_body"], normal=z["ovl1_n"], depth=z["ovl1_d"]))
+    return s, state, inputs, exp
+
+
+QUERY_KINDS = ("ray", "cast", "all", "ovl0", "ovl1")
+
+
+def run_queries(world, inputs, kinds=QUERY_KINDS):
+    """The query kinds of a fixture's inputs on an engine with the query
+    methods (GpuWorld, OracleWorld, RefWorld): results shaped as load_queries'
+    expected."""
+    i = inputs
+    out = {}
+    if "ray" in kinds:
+        out["ray"] = world.raycast(i["org"], i["dir"], i["max_dist"], i["mask"])
+    if "cast" in kinds:
+        out["cast"] = world.sphere_cast(i["org"], i["rad"], i["dir"], i["max_dist"], i["mask"])
+    if "all" in kinds:
+        out["all"] = world.raycast_all(i["org"], i["dir"], i["max_dist"], i["mask"])
+    if "ovl0" in kinds:
+        o0 = world.overlap(i["shapes0"], i["mask0"])
+        out["ovl0"] = dict(offsets=o0["offsets"], body=o0["body"])
+    if "ovl1" in kinds:
+        out["ovl1"] = world.overlap(i["shapes1"], None, contacts=True)
+    return out
+
+
+def same_queries(got, exp, what=""):
+    """Bitwise equality of query results (every array of every kind)."""
+    for kind, g in got.items():
+        e = exp[kind]
+        for f, a in e.items():
+            b = g[f]
+            b = np.ascontiguousarray(b).astype(a.dtype, copy=False)
+            assert a.shape == b.shape, f"{what} {kind}.{f}: shape {b.shape} vs {a.shape}"
+            if a.tobytes() != b.tobytes():
+                rows = np.nonzero(np.any((a.view(np.uint8) != b.view(np.uint8)).reshape(len(a), -1), axis=1))[0]
+                raise AssertionError(f"{what} {kind}.{f} differs in rows {rows[:8]} ({len(rows)} rows)")

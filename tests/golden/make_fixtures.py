@@ -9,9 +9,11 @@ narrow-phase vectors (random poses per shape pair -> contacts); the
 elastic gas of 4096 spheres over 1000 steps with the reference's momentum,
 angular momentum and kinetic energy every 100 steps; "NES1" snapshot bytes
 from the reference's own PhysicsWorld::saveState / loadState; and the
-facade's contact / trigger events from PhysicsWorld::step with callbacks.
+facade's contact / trigger events from PhysicsWorld::step with callbacks;
+and scene-query vectors from the facade's raycast / sphereCast / raycastAll /
+overlapSphere / overlapBox / overlapContacts.
 
-    python tests/golden/make_fixtures.py [--only NAME ...]
+    python tests/golden/make_fixtures.py [--only NAME ...]   (scene names, narrow, nes1, events, queries)
 """
 import json
 import os
@@ -224,6 +226,60 @@ def event_vectors(name, s, steps):
     return int(off[-1])
 
 
+def shape_arrays(prefix, shapes):
+    """A list of query shapes as flat arrays (vertices as CSR)."""
+    verts = [np.zeros((0, 3)) if q.verts is None else np.asarray(q.verts, float).reshape(-1, 3) for q in shapes]
+    return {
+        f"{prefix}_shape": np.array([q.shape for q in shapes], np.int32),
+        f"{prefix}_x": np.array([q.x for q in shapes], float),
+        f"{prefix}_q": np.array([q.q for q in shapes], float),
+        f"{prefix}_he": np.array([q.half_extents for q in shapes], float),
+        f"{prefix}_r": np.array([q.radius for q in shapes], float),
+        f"{prefix}_hh": np.array([q.half_height for q in shapes], float),
+        f"{prefix}_voff": np.concatenate([[0], np.cumsum([len(v) for v in verts])]).astype(np.uint32),
+        f"{prefix}_verts": np.concatenate(verts) if verts else np.zeros((0, 3)),
+    }
+
+
+def query_vectors(name, s, steps, seed):
+    """The reference facade's scene queries (physics_world.cpp:146-251) on
+    the state after `steps` steps: random rays (nearest hit, swept spheres,
+    every hit), overlap spheres and boxes, and overlapContacts with bodies of
+    the scene's shapes; the special-case scene adds its designed rays."""
+    r = engines.RefWorld()
+    r.set_scene(s)
+    if steps:
+        r.step(steps)
+    x, v, q, w = r.state()
+    arrays = scene_arrays(s)
+    arrays.update(state_x=x, state_v=v, state_q=q, state_w=w)
+    qi = scenes.query_inputs(s, seed)
+    org, d, md, mask, rad = qi["org"], qi["dir"], qi["max_dist"], qi["mask"], qi["rad"]
+    if name == "special":
+        so, sd, sm, sk = scenes.special_rays()
+        org, d, md, mask = np.concatenate([so, org]), np.concatenate([sd, d]), np.concatenate([sm, md]), \
+            np.concatenate([sk, mask])
+        rad = np.concatenate([np.full(len(so), 0.25), rad])
+    arrays.update(ray_org=org, ray_dir=d, ray_max=md, ray_mask=mask, ray_rad=rad)
+    a = r.raycast(org, d, md, mask)
+    arrays.update(ray_body=a["body"], ray_t=a["distance"], ray_p=a["point"], ray_n=a["normal"])
+    a = r.sphere_cast(org, rad, d, md, mask)
+    arrays.update(cast_body=a["body"], cast_t=a["distance"], cast_p=a["point"], cast_n=a["normal"])
+    a = r.raycast_all(org, d, md, mask)
+    arrays.update(all_off=a["offsets"], all_body=a["body"], all_t=a["distance"], all_p=a["point"], all_n=a["normal"])
+    arrays.update(shape_arrays("ovl0", qi["shapes0"]))
+    arrays.update(shape_arrays("ovl1", qi["shapes1"]))
+    arrays["ovl0_mask"] = qi["mask0"]
+    a = r.overlap(qi["shapes0"], qi["mask0"])
+    arrays.update(ovl0_off=a["offsets"], ovl0_body=a["body"])
+    a = r.overlap(qi["shapes1"], None, contacts=True)
+    arrays.update(ovl1_off=a["offsets"], ovl1_body=a["body"], ovl1_n=a["normal"], ovl1_d=a["depth"])
+    arrays["steps"] = np.array([steps], np.int32)
+    np.savez_compressed(os.path.join(HERE, f"queries_{name}.npz"), **arrays)
+    return int((arrays["ray_body"] >= 0).sum()), len(org), len(arrays["all_body"]), len(arrays["ovl0_body"]), \
+        len(arrays["ovl1_body"])
+
+
 def main():
     if not engines.ref_available():
         sys.exit("oracle/_ref/libne_ref.so is not built (make -C oracle ref)")
@@ -243,6 +299,11 @@ def main():
         for name, (s, steps) in scenes.event_scenes().items():
             total = event_vectors(name, s, steps)
             print(f"events_{name}: {s.n} bodies, {steps} steps, {total} events")
+    if not only or "queries" in only:
+        for k, (name, (s, steps)) in enumerate(scenes.query_scenes().items()):
+            hits, nr, nall, n0, n1 = query_vectors(name, s, steps, seed=100 + k)
+            print(f"queries_{name}: {s.n} bodies, {hits}/{nr} rays hit, {nall} hits in all, "
+                  f"{n0} overlap and {n1} contact hits")
 
 
 if __name__ == "__main__":

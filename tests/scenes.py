@@ -149,3 +149,113 @@ def event_scenes():
     p.flags[k] |= np.uint32(_abi.FLAG_SENSOR)
     out["sleeping_sensor"] = (p, 240)
     return out
+
+
+def special_query_scene():
+    """Static bodies placed for the ray tests' special cases: a capsule whose
+    side exit and cap entry disagree (the reference tries the caps only when
+    the side misses within the current best distance), with a sphere of
+    lower index between the two on the same ray; a cylinder met along its
+    axis; axis-aligned boxes met along a face plane and from inside; nested
+    spheres around an origin (distance 0); a rotated box; a hull and a plane
+    (never hit by rays, found by overlaps); a sensor and a masked layer."""
+    b = []
+    b.append(S.make_sphere(0, (0.26, 0.94, 0.0), 0.12, 1.0))                             # 0: sphere on the capsule ray
+    b.append(S.make_capsule(1, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), 0.5, 1.0, 1.0))    # 1: capsule, axis y
+    b.append(S.make_cylinder(2, (4.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), 0.6, 1.6, 1.0))   # 2: cylinder, axis y
+    b.append(S.make_box(3, (-4.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), (0.5, 0.7, 0.9), 1.0))  # 3: axis-aligned box
+    b.append(S.make_sphere(4, (0.0, 0.0, 4.0), 0.8, 1.0))                                # 4, 5: nested spheres
+    b.append(S.make_sphere(5, (0.1, 0.0, 4.0), 0.5, 1.0))
+    r = 2.0 ** -0.5
+    b.append(S.make_box(6, (0.0, 0.0, -4.0), (r, 0.0, r, 0.0), (0.4, 0.4, 0.8), 1.0))      # 6: box turned 90 deg about y
+    b.append(S.make_convex(7, (4.0, 0.0, 4.0), (1.0, 0.0, 0.0, 0.0), S.hull12(np.random.default_rng(7), 0.6), 1.0))
+    b.append(S.make_plane(8, (0.0, -3.0, 0.0), (0.0, 1.0, 0.0)))
+    sen = S.make_sphere(9, (-4.0, 0.0, 4.0), 0.7, 1.0)
+    sen.flags |= _abi.FLAG_SENSOR
+    b.append(sen)
+    lay = S.make_capsule(10, (-4.0, 0.0, -4.0), (r, r, 0.0, 0.0), 0.4, 0.6, 1.0)          # 10: capsule along z, layer 5
+    lay.layer = 5
+    b.append(lay)
+    b[0].layer = 4
+    for x in b:
+        x.make_static()
+    s = S.Scene.from_bodies(b)
+    s.params = S.Params(half=(8.0, 8.0, 8.0), periodic=False, dt=1.0 / 60.0, gravity=(0.0, 0.0, 0.0))
+    return s
+
+
+def special_rays():
+    """(origin, direction, max_dist, mask) rows for special_query_scene."""
+    rows = []
+    d = np.array([0.3, -1.0, 0.0])
+    for md in (10.0, 1.2, 0.5, 0.2):   # capsule: side exit vs cap entry, and the sphere between them
+        rows.append(((0.0, 1.8, 0.0), tuple(d), md, 0xFFFFFFFF))
+    for md in (10.0, 1.0):               # the sphere's layer masked out: the side exit, then the cap
+        rows.append(((0.0, 1.8, 0.0), tuple(d), md, 0xFFFFFFFF & ~(1 << 4)))
+    rows.append(((4.1, 5.0, 0.0), (0.0, -1.0, 0.0), 20.0, 0xFFFFFFFF))     # cylinder along its axis: the cap
+    rows.append(((4.1, 0.0, 0.0), (0.0, 1.0, 0.0), 20.0, 0xFFFFFFFF))      # from inside the cylinder
+    rows.append(((4.0, 0.5, 5.0), (0.0, 0.0, -1.0), 20.0, 0xFFFFFFFF))     # cylinder side, head on
+    rows.append(((-6.0, 0.7, 0.0), (1.0, 0.0, 0.0), 20.0, 0xFFFFFFFF))     # box: along its top face plane
+    rows.append(((-6.0, 0.2, 0.3), (1.0, 0.0, 0.0), 20.0, 0xFFFFFFFF))     # box: head on
+    rows.append(((-4.0, 0.1, 0.2), (0.0, 0.0, 1.0), 20.0, 0xFFFFFFFF))     # box: from inside (no hit)
+    rows.append(((0.05, 0.0, 4.0), (1.0, 0.0, 0.0), 20.0, 0xFFFFFFFF))     # inside both spheres: distance 0
+    rows.append(((0.0, 0.0, 2.0), (0.0, 0.0, 1.0), 20.0, 0xFFFFFFFF))      # nested spheres from outside
+    rows.append(((0.0, 3.0, -4.0), (0.0, -1.0, 0.0), 20.0, 0xFFFFFFFF))    # turned box from above
+    rows.append(((2.0, 0.0, -4.0), (-1.0, 0.0, 0.0), 20.0, 0xFFFFFFFF))    # turned box side
+    rows.append(((4.0, 3.0, 4.0), (0.0, -1.0, 0.0), 20.0, 0xFFFFFFFF))     # hull, then the plane: no ray hit
+    rows.append(((-4.0, 3.0, 4.0), (0.0, -1.0, 0.0), 20.0, 0xFFFFFFFF))    # sensor sphere: hit
+    rows.append(((-4.0, 3.0, -4.0), (0.0, -1.0, 0.0), 20.0, 0xFFFFFFFF))   # layer-5 capsule
+    rows.append(((-4.0, 3.0, -4.0), (0.0, -1.0, 0.0), 20.0, 0xFFFFFFFF & ~(1 << 5)))
+    rows.append(((-4.0, 0.0, -7.0), (0.0, 0.0, 1.0), 20.0, 0xFFFFFFFF))    # that capsule along its axis
+    rows.append(((0.0, 6.0, 0.0), (0.0, -1.0, 0.0), 20.0, 0xFFFFFFFF))     # capsule along its axis
+    rows.append(((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0, 0xFFFFFFFF))       # zero search distance
+    o, dd, m, k = zip(*rows)
+    return np.array(o, float), np.array(dd, float), np.array(m, float), np.array(k, np.uint32)
+
+
+def query_scenes():
+    """name -> (scene, steps before the queries) for the scene-query
+    fixtures: the feature scene (planes, cylinders, capsules, hulls,
+    sensors, layers) after 30 steps, a periodic pile (minimum images) after
+    20, and the special-case scene."""
+    p = S.mixed_pile(400, half=(6.0, 6.0, 6.0), seed=5, periodic=True, ground=False)
+    p.params.gravity = (0.0, 0.0, 0.0)
+    rng = np.random.default_rng(5)
+    p.v[:] = rng.normal(0, 2.0, p.v.shape)
+    p.w[:] = rng.normal(0, 1.0, p.w.shape)
+    sel = np.arange(p.n)[::17]
+    p.flags[sel] |= np.uint32(_abi.FLAG_SENSOR)
+    p.layer[np.arange(p.n)[3::11]] = 2
+    return {"features": (features_scene(3), 30), "pile_periodic": (p, 20), "special": (special_query_scene(), 0)}
+
+
+def query_inputs(s, seed, nrays=1500, nshapes=200):
+    """Random rays and query shapes over a scene's box: origins anywhere in
+    it (some inside bodies), a tenth of the directions along an axis, search
+    distances up to the box's width, a fifth of the masks random; casts of
+    radius up to 0.5; overlap spheres and boxes (random orientation), and
+    bodies of the scene's own shapes at random places for overlapContacts."""
+    from nativeengine_amd import queries as Q
+    rng = np.random.default_rng(seed)
+    h = np.asarray(s.params.half, float)
+    org = rng.uniform(-h, h, (nrays, 3))
+    d = rng.normal(size=(nrays, 3))
+    ax = rng.integers(0, 3, nrays // 10)
+    d[: nrays // 10] = 0.0
+    d[np.arange(nrays // 10), ax] = rng.choice([-1.0, 1.0], nrays // 10)
+    md = rng.uniform(0.2, 2.0 * h.max(), nrays)
+    mask = np.full(nrays, 0xFFFFFFFF, np.uint32)
+    pick = rng.random(nrays) < 0.2
+    mask[pick] = rng.integers(0, 2 ** 32, int(pick.sum()), dtype=np.uint64).astype(np.uint32)
+    rad = rng.uniform(0.0, 0.5, nrays)
+    shapes0 = []
+    for _ in range(nshapes):
+        c = rng.uniform(-h, h)
+        if rng.random() < 0.5:
+            shapes0.append(Q.sphere(c, rng.uniform(0.05, 1.5)))
+        else:
+            shapes0.append(Q.box(c, S.random_unit_quaternions(rng, 1)[0], rng.uniform(0.05, 1.0, 3)))
+    shapes1 = [Q.body_shape(s, int(i), x=rng.uniform(-h, h), q=S.random_unit_quaternions(rng, 1)[0])
+               for i in rng.integers(0, s.n, nshapes)]
+    mask0 = mask[:nshapes].copy()
+    return dict(org=org, dir=d, max_dist=md, mask=mask, rad=rad, shapes0=shapes0, shapes1=shapes1, mask0=mask0)
