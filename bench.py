@@ -24,7 +24,10 @@ Other lines in the same JSON object:
   pile     the 65,536-body mixed-shape pile (40% boxes, 30% spheres, 20%
            capsules, 10% hulls, sizes as specified) on a ground box inside
            100 x 200 x 100 walls, 10 iterations, its whole 600-step run, with
-           the C restatement at steps 10/150/300/450/600;
+           the C restatement at steps 10/150/300/450/600; at the end of its
+           run, scene queries on the settled pile ("queries": 1M rays cast
+           down over its floor, the same as swept spheres, raycastAll,
+           65,536 overlapSphere queries, the C restatement on a sample);
   batched  1024 periodic worlds of log-uniform size in [64, 4096] stepped in
            one set of launches (HIP-graph replay, autotuned) over their whole
            300-step run, against looping the plain single-world step over a
@@ -33,7 +36,7 @@ Other lines in the same JSON object:
            checkpoint every 16: backward time / forward time.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-large] [--no-gas] [--no-pile]
-                    [--no-config1] [--no-batched] [--no-adjoint] [--traffic-probe]
+                    [--no-queries] [--no-config1] [--no-batched] [--no-adjoint] [--traffic-probe]
 
 Prints one JSON line.  `roofline` is the dominant stage of the run: every
 stage is timed with HIP events on the library's own stream in every 9th
@@ -166,7 +169,7 @@ def timed_steps(run: int, warmup: int, k: int):
     return sorted({warmup + 1 + int((i + 0.5) * rest / k) for i in range(k)})
 
 
-def scene_run(s, run: int, warmup: int, k: int, label: str, marks=()):
+def scene_run(s, run: int, warmup: int, k: int, label: str, marks=(), after=None):
     """One resident scene stepped through its whole run of `run` single-step
     calls, each bracketed by a stream synchronisation (the call returns after
     it) and a read of the step's counters.  Every step's wall time is kept;
@@ -175,7 +178,8 @@ def scene_run(s, run: int, warmup: int, k: int, label: str, marks=()):
     rotation over the whole run, with that step's algorithmic units, so the
     dominant stage and its roofline are measured over the same run.  At the
     steps in `marks` the GPU's state is saved for the CPU's phase-matched
-    samples."""
+    samples.  `after(g)`, if given, runs on the world at the end of the run
+    and its result is kept under "queries"."""
     import tempfile
     import numpy as np
     from nativeengine_amd.gpu import GpuWorld
@@ -227,6 +231,7 @@ def scene_run(s, run: int, warmup: int, k: int, label: str, marks=()):
     last = g.counters()
     reruns, trunc = g.epa_stats()
     fl = g.body_aux()[0] if s.params.sleep_enabled else None
+    after_out = after(g) if after else None
     g.close()
     sel = sorted(timed)
     timed_ms = [step_ms[t - 1] for t in sel]
@@ -279,7 +284,94 @@ def scene_run(s, run: int, warmup: int, k: int, label: str, marks=()):
     }
     if fl is not None:
         out["sleeping_bodies_at_end"] = int(np.sum((fl & 8) != 0))
+    if after_out is not None:
+        out["queries"] = after_out
     return out, states, tmp
+
+
+RAY_BYTES = 128   # per ray: origin, direction, search distance, cast radius, mask in; body, distance, point, normal out
+
+
+def query_bench(g, s, no_cpu: bool):
+    """Scene queries on the settled pile: a 1024 x 1024 grid of rays cast
+    down over the box's whole floor (a depth camera; nearest hit), the same
+    grid as swept spheres of radius 0.25, raycastAll on its 262,144 middle
+    rays, and 65,536 overlapSphere queries in the occupied layer; the ray
+    kernel's time from HIP events on the library's stream.  The CPU baseline
+    is the C restatement on 512 of the rays and 256 of the spheres at the
+    same state."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    h = np.asarray(s.params.half, float)
+    k = 1024
+    gx, gz = np.meshgrid(np.linspace(-h[0] + 0.5, h[0] - 0.5, k), np.linspace(-h[2] + 0.5, h[2] - 0.5, k))
+    org = np.column_stack([gx.ravel(), np.full(k * k, h[1] - 0.5), gz.ravel()])
+    rng = np.random.default_rng(0)
+    d = np.column_stack([rng.normal(0, 0.05, k * k), -np.ones(k * k), rng.normal(0, 0.05, k * k)])
+    md = 2.0 * h[1]
+    out = {"rays": k * k}
+    for label, fn in (("raycast", lambda: g.raycast(org, d, md)), ("sphere_cast", lambda: g.sphere_cast(org, 0.25, d, md))):
+        fn()
+        g.time_kernel("ray")
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            res = fn()
+        wall = (time.perf_counter() - t0) / reps
+        ms, launches = g.kernel_time()
+        kms = ms / max(launches, 1)
+        out[label] = {"kernel_ms": round(kms, 3), "call_ms": round(wall * 1e3, 3),
+                      "rays_per_s": k * k / (kms * 1e-3), "rays_per_s_call": k * k / wall,
+                      "hit_fraction": round(float((res["body"] >= 0).mean()), 4)}
+    g.time_kernel(None)
+    kms = out["raycast"]["kernel_ms"]
+    achieved = RAY_BYTES * k * k / (kms * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "kernel": "k_ray", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                       "bytes_per_ray": RAY_BYTES,
+                       "note": "the fold is bound by its FP64 ray-body tests over LDS-staged tiles, not by HBM: "
+                               "the fraction only prices the rays' own bytes"}
+    mid = (np.abs(gx.ravel()) < h[0] / 2) & (np.abs(gz.ravel()) < h[2] / 2)
+    t0 = time.perf_counter()
+    allh = g.raycast_all(org[mid], d[mid], md)
+    out["raycast_all"] = {"rays": int(mid.sum()), "call_ms": round((time.perf_counter() - t0) * 1e3, 3),
+                          "hits": int(len(allh["body"]))}
+    x = g.state()[0]
+    dyn = s.inv_mass > 0
+    lo, hi = np.percentile(x[dyn], 1, axis=0), np.percentile(x[dyn], 99, axis=0)
+    nq = 65536
+    cs = rng.uniform(lo, hi, (nq, 3))
+    rs = rng.uniform(0.3, 1.5, nq)
+    from nativeengine_amd import queries as Q
+    shapes = Q.spheres(cs, rs)
+    g.overlap(shapes[:1024])
+    t0 = time.perf_counter()
+    ov = g.overlap(shapes)
+    wall = time.perf_counter() - t0
+    out["overlap_sphere"] = {"queries": nq, "call_ms": round(wall * 1e3, 3), "queries_per_s": nq / wall,
+                             "hits": int(len(ov["body"]))}
+    if not no_cpu:
+        import engines
+        o = engines.OracleWorld()
+        o.set_scene(s)
+        xs, vs, qs, ws = g.state()
+        o.set_state(xs, vs, qs, ws)
+        sel = rng.choice(k * k, 512, replace=False)
+        t0 = time.perf_counter()
+        cr = o.raycast(org[sel], d[sel], md)
+        tr = time.perf_counter() - t0
+        same = bool(all(np.array_equal(cr[f], g.raycast(org[sel], d[sel], md)[f]) for f in cr))
+        t0 = time.perf_counter()
+        o.overlap(shapes[:256])
+        to = time.perf_counter() - t0
+        out["cpu_baseline"] = {"raycast_rays_per_s": 512 / tr, "overlap_queries_per_s": 256 / to, "cores": 1,
+                               "kind": "port", "sample": "512 of the rays and 256 of the spheres, same state",
+                               "sample_equal_to_gpu": same}
+        out["speedup_vs_cpu"] = {"raycast": round(out["raycast"]["rays_per_s_call"] / (512 / tr), 1),
+                                 "overlap_sphere": round(out["overlap_sphere"]["queries_per_s"] / (256 / to), 1)}
+    log(f"queries: raycast {out['raycast']['kernel_ms']} ms kernel for {k * k} rays, "
+        f"overlap {out['overlap_sphere']['call_ms']} ms for {nq} spheres")
+    return out
 
 
 def _save_state(g, s, path):
@@ -427,7 +519,8 @@ def pile_bench(args):
     s = pile_scene(args.pile_n, 0)
     log(f"pile: {s.n} bodies, built in {time.perf_counter() - t0:.1f}s")
     marks = [10, 150, 300, 450, 600]
-    r, states, tmp = scene_run(s, 600, 0, args.pile_steps, "pile", marks=marks if not args.no_cpu else ())
+    after = None if args.no_queries else (lambda g: query_bench(g, s, args.no_cpu))
+    r, states, tmp = scene_run(s, 600, 0, args.pile_steps, "pile", marks=marks if not args.no_cpu else (), after=after)
     if not args.no_cpu:
         cpu = phase_cpu(f"pile,{args.pile_n},0", s.n, states, args.cpu_timeout, "the same pile")
         r["cpu_baseline"] = cpu
@@ -740,6 +833,7 @@ def main():
     ap.add_argument("--no-pile", action="store_true")
     ap.add_argument("--pile-n", type=int, default=65536)
     ap.add_argument("--pile-steps", type=int, default=600, help="timed steps of the pile's 600-step run")
+    ap.add_argument("--no-queries", action="store_true", help="skip the scene queries on the settled pile")
     ap.add_argument("--no-config1", action="store_true", help="skip the 4096-sphere elastic gas")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--worlds", type=int, default=1024)

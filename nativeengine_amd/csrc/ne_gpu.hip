@@ -5430,7 +5430,7 @@ int upload_rays(ne_gpu_ctx* c, Carve& cv, uint32_t n, const double* origin, cons
     // Tiles are skipped by their bounding boxes in a closed box; a periodic
     // world tests every body at its image nearest each ray's origin.
     u.cull = !make_sim_params(c->params).box.periodic;
-    if (u.cull && ntiles) k_ray_tiles<<<ntiles, RAY_THREADS, 0, c->stream>>>(c->B, u.tiles);
+    if (u.cull && ntiles) k_query_tiles<true><<<ntiles, RAY_THREADS, 0, c->stream>>>(c->B, u.tiles);
     return NE_OK;
 }
 
@@ -5617,7 +5617,7 @@ int ne_gpu_overlap(ne_gpu_ctx* c, uint32_t n, const ne_query_shape* shapes, cons
     for (uint32_t k = 0; k < n; ++k)
         if (shapes[k].shape == NE_SHAPE_CONVEX) nverts += shapes[k].nverts;
     auto carve_a = [&](Carve& cv, QueryDev*& Q, double*& hull, uint32_t*& cnt, uint32_t*& off, uint32_t*& part,
-                       uint32_t*& qoff, uint32_t*& ctr) {
+                       uint32_t*& qoff, uint32_t*& ctr, double*& tiles) {
         Q = cv.take<QueryDev>(n);
         hull = cv.take<double>(3 * nverts);
         cnt = cv.take<uint32_t>((size_t)qb * nchunks);
@@ -5625,15 +5625,16 @@ int ne_gpu_overlap(ne_gpu_ctx* c, uint32_t n, const ne_query_shape* shapes, cons
         part = cv.take<uint32_t>(1025);
         qoff = cv.take<uint32_t>((size_t)qb + 1);
         ctr = cv.take<uint32_t>(4);
+        tiles = cv.take<double>(6 * (size_t)nchunks);
     };
     QueryDev* Q;
-    double* hull;
+    double *hull, *tiles;
     uint32_t *cnt, *off, *part, *qoff, *ctr;
     Carve cv;
-    carve_a(cv, Q, hull, cnt, off, part, qoff, ctr);
+    carve_a(cv, Q, hull, cnt, off, part, qoff, ctr, tiles);
     HIP_TRY(c, ensure_arena(c->q_mem_a, c->q_cap_a, cv.off));
     cv = Carve{c->q_mem_a, 0};
-    carve_a(cv, Q, hull, cnt, off, part, qoff, ctr);
+    carve_a(cv, Q, hull, cnt, off, part, qoff, ctr, tiles);
     // query geometry: the Body the reference builds (fields as given), its
     // hull in device memory, and a bounding radius padded for the pair test
     std::vector<QueryDev> hq(n);
@@ -5665,11 +5666,18 @@ int ne_gpu_overlap(ne_gpu_ctx* c, uint32_t n, const ne_query_shape* shapes, cons
     HIP_TRY(c, hipMemcpyAsync(Q, hq.data(), sizeof(QueryDev) * n, hipMemcpyHostToDevice, c->stream));
     if (nverts) HIP_TRY(c, hipMemcpyAsync(hull, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice, c->stream));
     const Box3 box = make_sim_params(c->params).box;
+    // tiles of bodies skipped whole by their bounds in a closed box (a
+    // periodic world tests every pair at its minimum image)
+    const double* tl = nullptr;
+    if (!box.periodic) {
+        k_query_tiles<false><<<nchunks, RAY_THREADS, 0, c->stream>>>(c->B, tiles);
+        tl = tiles;
+    }
     // pass 1: hits per query (every batch), into offsets
     std::vector<uint32_t> h_qoff((size_t)qb + 1);
     auto count_batch = [&](uint32_t q0, uint32_t m) -> int {
-        k_ovl<false><<<dim3(nchunks, m), OVL_THREADS, 0, c->stream>>>(c->B, box, Q, q0, contacts, cnt, nullptr, nullptr,
-                                                                       nullptr, nullptr, nullptr, nullptr);
+        k_ovl<false><<<dim3(nchunks, m), OVL_THREADS, 0, c->stream>>>(c->B, box, Q, q0, tl, contacts, cnt, nullptr,
+                                                                       nullptr, nullptr, nullptr, nullptr, nullptr);
         scan_u32(cnt, off, nullptr, m * nchunks, part, c->stream);
         k_ovl_qoff<<<(m + 1 + 255) / 256, 256, 0, c->stream>>>(off, nchunks, m, qoff);
         HIP_TRY(c, hipGetLastError());
@@ -5717,8 +5725,8 @@ int ne_gpu_overlap(ne_gpu_ctx* c, uint32_t n, const ne_query_shape* shapes, cons
         const uint32_t bt = h_qoff[m];
         if (bt) {
             if (contacts) HIP_TRY(c, hipMemsetAsync(ctr + 2, 0, sizeof(uint32_t), c->stream));
-            k_ovl<true><<<dim3(nchunks, m), OVL_THREADS, 0, c->stream>>>(c->B, box, Q, q0, contacts, nullptr, off, ob,
-                                                                          on, od, e1, ctr + 2);
+            k_ovl<true><<<dim3(nchunks, m), OVL_THREADS, 0, c->stream>>>(c->B, box, Q, q0, tl, contacts, nullptr, off,
+                                                                          ob, on, od, e1, ctr + 2);
             HIP_TRY(c, hipGetLastError());
             if (contacts) {
                 uint32_t ne = 0;

@@ -159,26 +159,33 @@ struct RayOut {
 };
 
 constexpr int RAY_THREADS = 256;
-constexpr int RAY_F = 12;   // staged fields per body: x(3) q(4) he(3) radius, half height
+constexpr int RAY_F = 13;   // staged fields per body: x(3) q(4) he(3) radius, half height, bounding radius
+
+// The bounding radius of body i for the skip tests, padded 1e-5 relative;
+// infinite (never skipped) for a quaternion far from unit length, whose
+// rotation may stretch the body, and for planes and meshes.
+__device__ __forceinline__ double skip_radius(const BodiesDev& B, uint32_t i) {
+    const double q2 = B.q0[i] * B.q0[i] + B.q1[i] * B.q1[i] + B.q2[i] * B.q2[i] + B.q3[i] * B.q3[i];
+    const double r = B.bR[i];
+    return (fabs(q2 - 1.0) > 1e-6 || r >= 1e29) ? INFINITY : r * (1.0 + 1e-5);
+}
 
 // Per tile of RAY_THREADS bodies in index order: the bounding box of the
-// centres of its ray-hittable bodies, grown by each one's bounding radius
-// (scaled for a quaternion off unit length).  A tile holding a body whose
-// quaternion is far from unit length is never skipped (lo = -inf, hi = +inf);
-// a tile with no ray-hittable body is always skipped (lo = +inf, hi = -inf).
-__global__ void __launch_bounds__(RAY_THREADS) k_ray_tiles(BodiesDev B, double* __restrict__ tiles) {
+// centres of its bodies (RAYS: only the ray-hittable ones) grown by each
+// one's skip radius.  A tile with a body of infinite skip radius is never
+// skipped (lo = -inf, hi = +inf); a tile with no body to test is always
+// skipped (lo = +inf, hi = -inf).
+template <bool RAYS>
+__global__ void __launch_bounds__(RAY_THREADS) k_query_tiles(BodiesDev B, double* __restrict__ tiles) {
     __shared__ double red[6][RAY_THREADS];
     const uint32_t i = blockIdx.x * RAY_THREADS + threadIdx.x;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (i < B.n && ray_shape(B.shape[i])) {
-        const Qt q = Qt{B.q0[i], B.q1[i], B.q2[i], B.q3[i]};
-        const double q2 = q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z;
-        if (fabs(q2 - 1.0) > 1e-6) {
-            for (int a = 0; a < 3; ++a) { lo[a] = -INFINITY; hi[a] = INFINITY; }
-        } else {
-            const double R = B.bR[i] * (1.0 + 1e-5);
-            const double x[3] = {B.x0[i], B.x1[i], B.x2[i]};
-            for (int a = 0; a < 3; ++a) { lo[a] = x[a] - R; hi[a] = x[a] + R; }
+    if (i < B.n && (!RAYS || ray_shape(B.shape[i]))) {
+        const double R = skip_radius(B, i);
+        const double x[3] = {B.x0[i], B.x1[i], B.x2[i]};
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = R == INFINITY ? -INFINITY : x[a] - R;
+            hi[a] = R == INFINITY ? INFINITY : x[a] + R;
         }
     }
     for (int a = 0; a < 3; ++a) { red[a][threadIdx.x] = lo[a]; red[3 + a][threadIdx.x] = hi[a]; }
@@ -265,6 +272,7 @@ __global__ void __launch_bounds__(RAY_THREADS) k_ray(BodiesDev B, Box3 box, uint
             sf[3][t] = B.q0[i]; sf[4][t] = B.q1[i]; sf[5][t] = B.q2[i]; sf[6][t] = B.q3[i];
             sf[7][t] = B.he0[i]; sf[8][t] = B.he1[i]; sf[9][t] = B.he2[i];
             sf[10][t] = B.rad[i]; sf[11][t] = B.hh[i];
+            sf[12][t] = skip_radius(B, i);
             sshape[t] = B.shape[i];
             slayer[t] = B.layer[i];
         }
@@ -285,6 +293,16 @@ __global__ void __launch_bounds__(RAY_THREADS) k_ray(BodiesDev B, Box3 box, uint
                     else if (sh == SH_CYLINDER) hh = hh + rc;
                 }
                 const V3 cimg = org + min_image(box, x - org);
+                {   // skip a body whose padded bounding sphere (grown by a cast's
+                    // inflation) the segment cannot reach: any hit lies within it
+                    const double reach = sf[12][j] + 2.0 * rc;
+                    const double tl = MODE == 0 ? best : maxd;
+                    const V3 mo = cimg - org;
+                    const double tc = fmin(fmax(dot(mo, d), 0.0), tl);
+                    const V3 cp = mo - d * tc;
+                    const double pad = 1e-6 * (1.0 + fabs(org.x) + fabs(org.y) + fabs(org.z) + tl + reach);
+                    if (norm2(cp) > (reach + pad) * (reach + pad)) continue;
+                }
                 double t;
                 V3 p, n;
                 if (MODE == 0) {
@@ -336,8 +354,9 @@ struct QueryDev {
 };
 
 constexpr int OVL_THREADS = 256;
+static_assert(OVL_THREADS == RAY_THREADS, "an overlap block covers one body tile of k_query_tiles");
 
-// Overlap of query q (blockIdx.y + q0) with body i: a padded bounding-sphere
+// Overlap of query q with body i: a padded bounding-sphere
 // test at the minimum image first (bodies with a quaternion far from unit
 // length, planes and queries with bR 1e30 are always tested), then
 // detectContact with EPA deferred: hit 1 (analytic contact) or 2 (GJK
@@ -361,12 +380,32 @@ __device__ __forceinline__ Contact ovl_test(const BodiesDev& B, const Box3& box,
     return detect_contact(box, Q.g, load_geom(B, i), ws);
 }
 
-// EMIT false: hits per (query, block of bodies) into cnt; EMIT true: the
-// hits' body indices at off[...] + rank (body order), contacts of analytic
-// pairs, and convex pairs queued (slot, body, query) for EPA.
+// Query q0 + blockIdx.y against the bodies of tile blockIdx.x (skipped
+// whole when tiles is set and the query cannot reach the tile's box).
+// EMIT false: hits per (query, tile) into cnt; EMIT true: the hits' body
+// indices at off[...] + rank (body order), contacts of analytic pairs, and
+// convex pairs queued (slot, body, query) for EPA.
+// Whether query Q can reach a body of a tile (k_query_tiles without RAYS):
+// the distance from its centre to the tile's box against its padded
+// bounding radius.
+__device__ __forceinline__ bool query_meets_tile(const QueryDev& Q, const double* __restrict__ tb) {
+    if (!(tb[0] <= tb[3])) return false;
+    if (Q.bR >= 1e29) return true;
+    const double xq[3] = {Q.g.x.x, Q.g.x.y, Q.g.x.z};
+    double d2 = 0.0, span = fabs(xq[0]) + fabs(xq[1]) + fabs(xq[2]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double e = fmax(fmax(tb[a] - xq[a], 0.0), xq[a] - tb[3 + a]);
+        d2 += e * e;
+        span += fmax(fabs(tb[a]), fabs(tb[3 + a]));
+    }
+    const double reach = Q.bR * (1.0 + 1e-5) + 1e-6 * (1.0 + span + Q.bR);
+    return d2 <= reach * reach;
+}
+
 template <bool EMIT>
 __global__ void __launch_bounds__(OVL_THREADS) k_ovl(BodiesDev B, Box3 box, const QueryDev* __restrict__ Q, uint32_t q0,
-                                                     int contacts, uint32_t* __restrict__ cnt,
+                                                     const double* __restrict__ tiles, int contacts, uint32_t* __restrict__ cnt,
                                                      const uint32_t* __restrict__ off, int32_t* __restrict__ obody,
                                                      double* __restrict__ onrm, double* __restrict__ odep,
                                                      uint32_t* __restrict__ epa_ent, uint32_t* __restrict__ epa_n) {
@@ -374,7 +413,8 @@ __global__ void __launch_bounds__(OVL_THREADS) k_ovl(BodiesDev B, Box3 box, cons
     const uint32_t q = q0 + blockIdx.y;
     const uint32_t i = blockIdx.x * OVL_THREADS + threadIdx.x;
     Contact c = no_contact();
-    if (i < B.n) c = ovl_test(B, box, Q[q], i, contacts);
+    const bool reach = !tiles || query_meets_tile(Q[q], tiles + 6 * (size_t)blockIdx.x);   // the same for the block
+    if (i < B.n && reach) c = ovl_test(B, box, Q[q], i, contacts);
     uint32_t tot;
     const uint32_t ex = block_exclusive_scan(c.hit ? 1u : 0u, &tot, lds);
     const size_t cell = (size_t)blockIdx.y * gridDim.x + blockIdx.x;

@@ -371,16 +371,11 @@ class GpuWorld:
 
     def overlap_sphere(self, centers, radii, mask=None) -> list:
         """PhysicsWorld::overlapSphere per (center, radius): one index array each."""
-        c = np.atleast_2d(np.asarray(centers, np.float64))
-        r = np.broadcast_to(np.asarray(radii, np.float64), (len(c),))
-        return queries.split(self.overlap([queries.sphere(c[k], r[k]) for k in range(len(c))], mask))
+        return queries.split(self.overlap(queries.spheres(centers, radii), mask))
 
     def overlap_box(self, centers, rots, halves, mask=None) -> list:
         """PhysicsWorld::overlapBox per (center, rot, half): one index array each."""
-        c = np.atleast_2d(np.asarray(centers, np.float64))
-        q = np.broadcast_to(np.asarray(rots, np.float64), (len(c), 4))
-        hv = np.broadcast_to(np.asarray(halves, np.float64), (len(c), 3))
-        return queries.split(self.overlap([queries.box(c[k], q[k], hv[k]) for k in range(len(c))], mask))
+        return queries.split(self.overlap(queries.boxes(centers, rots, halves), mask))
 
     def sleep_store(self):
         """(Body::invMassStore (n), Body::invInertiaStore (n, 3)) of every body."""

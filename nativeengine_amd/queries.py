@@ -99,7 +99,42 @@ def raycast_all(fn, h, check, origin, direction, max_dist, mask=None, cap_guess=
         return dict(offsets=off, body=body[:t], distance=dist[:t], point=pt[:t], normal=nrm[:t])
 
 
+SHAPE_DTYPE = np.dtype({"names": ["shape", "nverts", "x", "q", "half_extents", "radius", "half_height", "verts"],
+                        "formats": ["<i4", "<u4", ("<f8", (3,)), ("<f8", (4,)), ("<f8", (3,)), "<f8", "<f8", "<u8"],
+                        "offsets": [_abi.QueryShape.shape.offset, _abi.QueryShape.nverts.offset,
+                                    _abi.QueryShape.x.offset, _abi.QueryShape.q.offset,
+                                    _abi.QueryShape.half_extents.offset, _abi.QueryShape.radius.offset,
+                                    _abi.QueryShape.half_height.offset, _abi.QueryShape.verts.offset],
+                        "itemsize": C.sizeof(_abi.QueryShape)})
+
+
+def spheres(centers, radii) -> np.ndarray:
+    """Many overlapSphere queries at once, packed as ne_query_shape records."""
+    c = np.atleast_2d(np.asarray(centers, np.float64))
+    a = np.zeros(len(c), SHAPE_DTYPE)
+    a["shape"] = _abi.SHAPE_SPHERE
+    a["x"] = c
+    a["q"] = (1.0, 0.0, 0.0, 0.0)
+    a["radius"] = np.broadcast_to(np.asarray(radii, np.float64), (len(c),))
+    return a
+
+
+def boxes(centers, rots, halves) -> np.ndarray:
+    """Many overlapBox queries at once, packed as ne_query_shape records."""
+    c = np.atleast_2d(np.asarray(centers, np.float64))
+    a = np.zeros(len(c), SHAPE_DTYPE)
+    a["shape"] = _abi.SHAPE_BOX
+    a["x"] = c
+    a["q"] = np.broadcast_to(np.asarray(rots, np.float64), (len(c), 4))
+    a["half_extents"] = np.broadcast_to(np.asarray(halves, np.float64), (len(c), 3))
+    return a
+
+
 def _shape_array(shapes):
+    if isinstance(shapes, np.ndarray):   # packed records (spheres / boxes): no vertices to keep alive
+        assert shapes.dtype == SHAPE_DTYPE and not np.any(shapes["nverts"])
+        a = np.ascontiguousarray(shapes)
+        return a.ctypes.data_as(C.POINTER(_abi.QueryShape)), [a]
     arr = (_abi.QueryShape * max(1, len(shapes)))()
     keep = []
     for k, s in enumerate(shapes):
@@ -119,11 +154,13 @@ def _shape_array(shapes):
 
 
 def overlap(fn, h, check, shapes, mask=None, contacts=False, cap_guess=None) -> dict:
-    """Bodies overlapping each query shape, query k's at
+    """Bodies overlapping each query shape (a list of Shape, or records from
+    spheres() / boxes()), query k's at
     offsets[k]:offsets[k+1] in body order (overlapSphere / overlapBox; with
     contacts, overlapContacts: sensors skipped, normal body -> query and
     depth): dict of offsets, body, normal, depth."""
-    shapes = list(shapes)
+    if not isinstance(shapes, np.ndarray):
+        shapes = list(shapes)
     n = len(shapes)
     arr, keep = _shape_array(shapes)
     mk = None if mask is None else np.ascontiguousarray(np.broadcast_to(np.asarray(mask, np.uint32), (n,)))
