@@ -223,7 +223,10 @@ int ne_gpu_mesh_bvh(ne_gpu_ctx *ctx, uint32_t m, int32_t *order, uint32_t node_c
 int ne_gpu_set_state(ne_gpu_ctx *ctx, const double *x, const double *v,
                      const double *q, const double *w);
 
-/* Replaces `World::step()` (src/world.cpp:133-146), nsteps times, resident.
+/* Replaces `World::step()` (src/world.cpp:133-146), nsteps times, resident:
+ * spring joints, integrate, the CCD pass of bodies flagged NE_FLAG_CCD
+ * (World::ccdPass, :658-696), collide, rigid joints, wrap or walls,
+ * sleeping.  General joints (World::solveJoints) are not on this path.
  * Work buffers (candidate pairs, constraints, the large EPA workspace) are
  * sized from the bodies; when one overflows, the call enlarges it, restores
  * the state it started from (a device-side copy taken at the start of every

@@ -3670,6 +3670,12 @@ struct ne_gpu_ctx {
     double* ev_dat[2] = {nullptr, nullptr};
     uint32_t* ev_state[2] = {nullptr, nullptr};
     uint32_t *ev_off = nullptr, *ev_n = nullptr, *ev_saved = nullptr;
+    // continuous collision (World::ccdPass): the CCD bodies in index order,
+    // their step-start positions, and the candidate pass's counts, offsets,
+    // bodies and tile boxes
+    uint32_t nccd = 0, ccd_cap = 0;
+    uint32_t *ccd_idx = nullptr, *ccd_cnt = nullptr, *ccd_off = nullptr, *ccd_cand = nullptr;
+    double *ccd_prev = nullptr, *ccd_tiles = nullptr;
     // scene queries: device arenas, grown on demand and carved per call (a: the
     // call's inputs and counts, b: its outputs
     // and w: EPA workspaces of overlapContacts)
@@ -3764,9 +3770,10 @@ struct ne_gpu_ctx {
     uint32_t timed_launches = 0;
 };
 
-// Scene queries (rays, swept spheres, overlaps): kernels over the body
-// layout above.
+// Scene queries (rays, swept spheres, overlaps) and continuous collision:
+// kernels over the body layout above.
 #include "ne_query.hpp"
+#include "ne_ccd.hpp"
 
 namespace {
 
@@ -4283,6 +4290,8 @@ void free_all(ne_gpu_ctx* c) {
     for (int b = 0; b < 2; ++b) { dfree(c->ev_key[b]); dfree(c->ev_trig[b]); dfree(c->ev_dat[b]); dfree(c->ev_state[b]); }
     dfree(c->ev_off); dfree(c->ev_n); dfree(c->ev_saved);
     dfree(c->q_mem_a); dfree(c->q_mem_b); dfree(c->q_mem_w);
+    dfree(c->ccd_idx); dfree(c->ccd_cnt); dfree(c->ccd_off); dfree(c->ccd_cand); dfree(c->ccd_prev); dfree(c->ccd_tiles);
+    c->nccd = c->ccd_cap = 0;
     c->q_cap_a = c->q_cap_b = c->q_cap_w = 0;
     c->ev_cap = 0;
     c->snap_nd = 0;
@@ -4446,6 +4455,33 @@ void enqueue_detect(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool 
     }
 }
 
+// World::ccdPass (world.cpp:658-696) after integrate: the candidate pass
+// over (CCD body, tile), its scan, and the one-thread fold that stops the
+// bodies and moves their packed poses and static-grid cells.
+void enqueue_ccd(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool static_grid) {
+    hipStream_t s = c->stream;
+    const uint32_t ntiles = (c->n + RAY_THREADS - 1) / RAY_THREADS;
+    const dim3 grid(ntiles, std::min<uint32_t>(c->nccd, 65535u));
+    const bool cull = !P.box.periodic;
+    if (cull) {
+        k_query_tiles<true><<<ntiles, RAY_THREADS, 0, s>>>(B, c->ccd_tiles);
+        k_ccd_cand<false, true><<<grid, RAY_THREADS, 0, s>>>(B, P.box, c->ccd_idx, c->nccd, c->ccd_prev, c->ccd_tiles,
+                                                            c->ccd_cnt, nullptr, nullptr, 0);
+    } else {
+        k_ccd_cand<false, false><<<grid, RAY_THREADS, 0, s>>>(B, P.box, c->ccd_idx, c->nccd, c->ccd_prev, nullptr,
+                                                             c->ccd_cnt, nullptr, nullptr, 0);
+    }
+    scan_u32(c->ccd_cnt, c->ccd_off, nullptr, c->nccd * ntiles, c->scan_part, s);
+    if (cull)
+        k_ccd_cand<true, true><<<grid, RAY_THREADS, 0, s>>>(B, P.box, c->ccd_idx, c->nccd, c->ccd_prev, c->ccd_tiles,
+                                                           nullptr, c->ccd_off, c->ccd_cand, c->ccd_cap);
+    else
+        k_ccd_cand<true, false><<<grid, RAY_THREADS, 0, s>>>(B, P.box, c->ccd_idx, c->nccd, c->ccd_prev, nullptr,
+                                                            nullptr, c->ccd_off, c->ccd_cand, c->ccd_cap);
+    k_ccd_fold<<<1, 64, 0, s>>>(B, P.box, c->ccd_idx, c->nccd, c->ccd_prev, ntiles, c->ccd_off, c->ccd_cand,
+                                c->ccd_cap, c->xp, static_grid ? 1 : 0, c->sgrid, c->cell_count);
+}
+
 void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
     hipStream_t s = c->stream;
     const SimParams P = make_sim_params(c->params);
@@ -4454,9 +4490,11 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
     const bool static_grid = P.box.periodic && c->ngrid > 0;
     if (c->nspring)   // spring joints add forces before integration (world.cpp:134)
         k_joint_springs<<<blocks_for(c->ncomp, 64), 64, 0, s>>>(B, P, joint_dev(c));
+    if (c->nccd)   // step-start positions of the CCD bodies (world.cpp:135)
+        k_ccd_save<<<blocks_for(c->nccd), TPB, 0, s>>>(B, c->ccd_idx, c->nccd, c->ccd_prev);
     {
         KTimer kt(c, NE_KERNEL_INTEGRATE);
-        const bool w = wrap_first && P.box.periodic && c->njoints == 0 && !P.sleep && !c->events_on;
+        const bool w = wrap_first && P.box.periodic && c->njoints == 0 && !P.sleep && !c->events_on && !c->nccd;
         if (c->has_forces || c->nspring) {
             if (w) launch_integrate<true, true>(c, P, static_grid);
             else launch_integrate<true, false>(c, P, static_grid);
@@ -4466,6 +4504,7 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
             else launch_integrate<false, false>(c, P, static_grid);
         }
     }
+    if (c->nccd) enqueue_ccd(c, B, P, static_grid);
     enqueue_detect(c, B, P, static_grid);
     if (P.sleep) {   // wakes in pair order (world.cpp:527-531), then the woken bodies' masses
         k_sleep_reset<<<blocks_for(n), TPB, 0, s>>>(B, c->wake_at, c->wake_again);
@@ -4724,7 +4763,7 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     if (!P.box.periodic && !P.open_boundary) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_walls<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P);
-    } else if (P.box.periodic && (c->njoints || P.sleep || c->events_on)) {
+    } else if (P.box.periodic && (c->njoints || P.sleep || c->events_on || c->nccd)) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_wrap<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P);
     }
@@ -4995,8 +5034,8 @@ int ne_gpu_rollout(ne_gpu_ctx* c, const double* v0, const double* mu, const doub
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
     if (c->n == 0) return set_err(c, NE_ERR_INVALID, "no bodies");
     if (!c->params.periodic) return set_err(c, NE_ERR_UNSUPPORTED, "rollouts are differentiated for periodic worlds");
-    if (c->has_forces || c->njoints || c->params.sleep_enabled || c->nmeshes)
-        return set_err(c, NE_ERR_UNSUPPORTED, "rollouts do not take applied forces, joints, sleeping or meshes");
+    if (c->has_forces || c->njoints || c->params.sleep_enabled || c->nmeshes || c->nccd)
+        return set_err(c, NE_ERR_UNSUPPORTED, "rollouts do not take applied forces, joints, sleeping, meshes or CCD");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const uint32_t nw = c->nworlds ? c->nworlds : 1u;
     if (mu || e) {
@@ -6396,6 +6435,7 @@ int ne_gpu_set_worlds(ne_gpu_ctx* c, uint32_t nworlds, const uint32_t* offsets, 
     }
     if (!offsets || !half) return set_err(c, NE_ERR_INVALID, "offsets and half are required");
     if (!c->params.periodic) return set_err(c, NE_ERR_UNSUPPORTED, "batched worlds must be periodic");
+    if (c->nccd) return set_err(c, NE_ERR_UNSUPPORTED, "CCD bodies in batched worlds");
     if (offsets[0] != 0 || offsets[nworlds] != c->n)
         return set_err(c, NE_ERR_INVALID, "world offsets must run from 0 to the body count");
     uint32_t maxsize = 0;
@@ -6568,7 +6608,7 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
                                    b->inv_inertia[3 * i + 1] != 0.0 || b->inv_inertia[3 * i + 2] != 0.0))
             return set_err(c, NE_ERR_UNSUPPORTED, "mesh bodies must be static (zero inverse mass and inertia)");
         const uint32_t fl = b->flags ? b->flags[i] : NE_FLAG_DYNAMIC;
-        if (fl & NE_FLAG_CCD) return set_err(c, NE_ERR_UNSUPPORTED, "CCD is not on the GPU path");
+        (void)fl;
         if (s == NE_SHAPE_CONVEX && (!b->hull_offsets || b->hull_offsets[i + 1] < b->hull_offsets[i]))
             return set_err(c, NE_ERR_INVALID, "convex body without hull vertices");
     }
@@ -6694,6 +6734,25 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
     c->nworlds = 0;   // a new body set starts as one world
     free_meshes(c);   // and without meshes
     c->h_shape.assign(b->shape, b->shape + n);
+    {   // CCD bodies (World::ccdPass), in index order
+        std::vector<uint32_t> ccd;
+        for (uint32_t i = 0; i < n; ++i)
+            if ((b->flags ? b->flags[i] : NE_FLAG_DYNAMIC) & NE_FLAG_CCD) ccd.push_back(i);
+        const uint64_t ntiles = (n + RAY_THREADS - 1) / RAY_THREADS;
+        if ((uint64_t)ccd.size() * ntiles >= (1ull << 31))
+            return set_err(c, NE_ERR_UNSUPPORTED, "too many CCD bodies for this body count");
+        c->nccd = (uint32_t)ccd.size();
+        if (c->nccd) {
+            c->ccd_cap = std::max<uint32_t>(4096u, 256u * c->nccd);
+            HIP_TRY(c, dalloc(c->ccd_idx, c->nccd));
+            HIP_TRY(c, dalloc(c->ccd_prev, 3 * (size_t)c->nccd));
+            HIP_TRY(c, dalloc(c->ccd_cnt, (size_t)c->nccd * ntiles));
+            HIP_TRY(c, dalloc(c->ccd_off, (size_t)c->nccd * ntiles + 1));
+            HIP_TRY(c, dalloc(c->ccd_cand, c->ccd_cap));
+            HIP_TRY(c, dalloc(c->ccd_tiles, 6 * ntiles));
+            HIP_TRY(c, hipMemcpy(c->ccd_idx, ccd.data(), sizeof(uint32_t) * ccd.size(), hipMemcpyHostToDevice));
+        }
+    }
     c->njoints = c->nspring = c->nrigid = c->ncomp = 0;   // and without joints
     c->h_mut.assign(n, 0);
     for (uint32_t i = 0; i < n; ++i)

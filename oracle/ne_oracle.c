@@ -111,6 +111,7 @@ typedef struct {
     uint32_t nc0, nc1;      /* [nc0, nc1) into world no-collide array      */
     int32_t mesh;           /* mesh body: index into the world's meshes, or -1 */
     const v3* hvp;          /* when set, the hull vertices (a temporary prism) */
+    v3 prevX;               /* the pose at the step's start (world.cpp:135), for CCD */
 } obody;
 
 /* MeshData (src/mesh.hpp:14-93): vertices, triangles, the tree's triangle
@@ -167,6 +168,7 @@ struct ne_oracle {
     /* facade contact events: this step's and the last step's overlapping
      * pairs (physics_world.cpp:253-302), each in (i, j) order */
     int events_on;
+    uint32_t nccd;   /* bodies with the CCD flag */
     uint32_t nev[2], capev[2];
     struct oevent* ev[2];
     int ev_cur;
@@ -1400,9 +1402,58 @@ static void update_sleep(ne_oracle* o) {
     free(seen);
 }
 
+static int ray_vs_body(int shape, qt q, v3 he, double r, double h, v3 c, v3 o, v3 d, double maxT,
+                       double* tOut, v3* pOut, v3* nOut);
+
+/* World::ccdPass (world.cpp:658-696): each fast CCD body in index order
+ * sweeps its bounding sphere from its step-start position along this step's
+ * displacement against every other non-sensor body it collides with
+ * (inflated by the sweep radius, at the minimum image), stops at the first
+ * impact and loses its velocity into the surface.  Later bodies see the
+ * earlier ones' stopped positions. */
+static void ccd_pass(ne_oracle* o) {
+    for (uint32_t i = 0; i < o->n; ++i) {
+        obody* bi = &o->B[i];
+        if (!(bi->flags & NE_FLAG_CCD) || bi->invMass <= 0.0 || (bi->flags & NE_FLAG_SLEEPING)) continue;
+        v3 disp = sub(bi->x, bi->prevX);
+        double dist = norm(disp);
+        double R = bounding_radius(o, bi);
+        if (dist < R * 0.5 || dist < 1e-9) continue;
+        v3 dir = dvs(disp, dist);
+        double bestT = dist;
+        v3 bestN = mk(0.0, 0.0, 0.0);
+        int hit = 0;
+        for (uint32_t j = 0; j < o->n; ++j) {
+            if (j == i) continue;
+            const obody* bj = &o->B[j];
+            if (bj->flags & NE_FLAG_SENSOR) continue;
+            if (!layers_collide(bi, bj)) continue;
+            double r = bj->radius + R, h = bj->hh;
+            v3 he = bj->he;
+            if (bj->shape == NE_SHAPE_BOX) he = add(bj->he, mk(R, R, R));
+            else if (bj->shape == NE_SHAPE_CYLINDER) h = bj->hh + R;
+            v3 cImg = add(bi->prevX, min_image(&o->box, sub(bj->x, bi->prevX)));
+            double t;
+            v3 p, nn;
+            if (ray_vs_body(bj->shape, bj->q, he, r, h, cImg, bi->prevX, dir, bestT, &t, &p, &nn) && t < bestT) {
+                bestT = t;
+                bestN = nn;
+                hit = 1;
+            }
+        }
+        if (hit) {
+            bi->x = add(bi->prevX, scl(dir, bestT));
+            double vn = dot(bi->v, bestN);
+            if (vn < 0.0) bi->v = sub(bi->v, scl(bestN, vn));
+        }
+    }
+}
+
 static void world_step(ne_oracle* o) {                                               /* world.cpp:133-146 */
     apply_joint_forces(o);
+    for (uint32_t i = 0; i < o->n; ++i) o->B[i].prevX = o->B[i].x;
     integrate(o);
+    if (o->nccd) ccd_pass(o);
     collide(o);
     solve_rigid_joints(o, 8);
     if (o->box.periodic) wrap_positions(o);
@@ -1511,6 +1562,7 @@ int ne_oracle_set_params(ne_oracle* o, const ne_world_params* p) {
 int ne_oracle_set_bodies(ne_oracle* o, const ne_bodies* b) {
     if (!o || !b) return NE_ERR_INVALID;
     o->nev[0] = o->nev[1] = 0;   /* a new world: no last-step pairs */
+    o->nccd = 0;
     free_meshes(o);
     free(o->B); free(o->hull); free(o->nocol);
     o->B = NULL; o->hull = NULL; o->nocol = NULL; o->nhull = 0; o->nnocol = 0;
@@ -1548,7 +1600,7 @@ int ne_oracle_set_bodies(ne_oracle* o, const ne_bodies* b) {
         B->layer = b->layer ? b->layer[i] : 0u;
         B->mask = b->mask ? b->mask[i] : 0xFFFFFFFFu;
         B->flags = b->flags ? b->flags[i] : NE_FLAG_DYNAMIC;
-        if (B->flags & NE_FLAG_CCD) return NE_ERR_UNSUPPORTED;
+        if (B->flags & NE_FLAG_CCD) ++o->nccd;
         B->sleepTimer = b->sleep_timer ? b->sleep_timer[i] : 0.0;
         B->invMassStore = b->inv_mass_store ? b->inv_mass_store[i] : 0.0;
         B->invIStore = b->inv_inertia_store ? ld3(b->inv_inertia_store, i) : mk(0, 0, 0);

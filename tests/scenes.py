@@ -94,7 +94,84 @@ def fixture_scenes():
     f2.params.warm_start = False
     out["features_open_cold"] = (f2, 30, 0)
     out["gas4096"] = (elastic_gas_4096(), 1000, 100)
+    out["ccd"] = (ccd_scene(0, periodic=False), 60, 0)
+    out["ccd_periodic"] = (ccd_scene(1, periodic=True), 60, 0)
     return out
+
+
+def ccd_selftest_scene(ccd: bool):
+    """selftest.cpp:1241-1257: a sphere of radius 0.2 fired at 600 units/s
+    (10 units a step) at a static wall 0.1 thick."""
+    wall = S.make_box(0, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), (0.05, 5.0, 5.0), 1.0).make_static()
+    ball = S.make_sphere(1, (-5.0, 0.0, 0.0), 0.2, 1.0)
+    ball.v = (600.0, 0.0, 0.0)
+    if ccd:
+        ball.flags |= _abi.FLAG_CCD
+    s = S.Scene.from_bodies([wall, ball])
+    s.params = S.Params(half=(5000.0, 5000.0, 5000.0), periodic=False, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0),
+                        restitution=0.0)
+    return s
+
+
+def ccd_scene(seed=0, periodic=False):
+    """Fast CCD bodies (spheres, boxes, capsules, cylinders, a hull) fired at
+    thin static plates, a static cylinder and capsule, and each other (so
+    later CCD bodies meet earlier ones' stopped positions), among slow CCD
+    bodies that do not sweep, a sleeping CCD body, a sensor plate and a
+    plate on a masked layer (both passed through), ordinary dynamic bodies,
+    a ground box; periodic: obstacles across the boundary (minimum images)."""
+    rng = np.random.default_rng(seed)
+    b = []
+    h = 8.0
+    plates = [((3.0, 0.0, 0.0), (0.04, 2.5, 2.5)), ((-3.0, 0.0, 0.0), (0.04, 2.5, 2.5)),
+              ((0.0, 0.0, 3.0), (2.5, 2.5, 0.04)), ((0.0, 3.5, 0.0), (2.5, 0.04, 2.5))]
+    if periodic:
+        plates += [((h - 0.05, 0.0, 0.0), (0.04, 2.0, 2.0))]
+    for x, he in plates:
+        b.append(S.make_box(len(b), x, (1.0, 0.0, 0.0, 0.0), he, 1.0).make_static())
+    b.append(S.make_cylinder(len(b), (0.0, 0.0, -3.0), (1.0, 0.0, 0.0, 0.0), 0.6, 3.0, 1.0).make_static())
+    r = 2.0 ** -0.5
+    b.append(S.make_capsule(len(b), (-3.0, 0.0, -3.0), (r, r, 0.0, 0.0), 0.3, 1.0, 1.0).make_static())
+    sens = S.make_box(len(b), (0.0, 0.0, -6.0), (1.0, 0.0, 0.0, 0.0), (2.5, 2.5, 0.04), 1.0).make_static()
+    sens.flags |= _abi.FLAG_SENSOR
+    b.append(sens)
+    lay = S.make_box(len(b), (6.0, 0.0, 0.0), (1.0, 0.0, 0.0, 0.0), (0.04, 2.5, 2.5), 1.0).make_static()
+    lay.layer = 6
+    b.append(lay)
+    if not periodic:
+        b.append(S.make_box(len(b), (0.0, -7.5, 0.0), (1.0, 0.0, 0.0, 0.0), (7.5, 0.3, 7.5), 1.0).make_static())
+    for k in range(24):
+        x = tuple(rng.uniform(-1.5, 1.5, 3))
+        kind = k % 5
+        q = tuple(S.random_unit_quaternions(rng, 1)[0])
+        if kind == 0:
+            bd = S.make_sphere(len(b), x, float(rng.uniform(0.1, 0.25)), 1.0)
+        elif kind == 1:
+            bd = S.make_box(len(b), x, q, tuple(rng.uniform(0.08, 0.2, 3)), 1.0)
+        elif kind == 2:
+            bd = S.make_capsule(len(b), x, q, 0.1, 0.15, 1.0)
+        elif kind == 3:
+            bd = S.make_cylinder(len(b), x, q, 0.12, 0.3, 1.0)
+        else:
+            bd = S.make_convex(len(b), x, q, S.hull12(rng, 0.18), 1.0)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        speed = 700.0 if k % 6 else 2.0   # every sixth is too slow to sweep
+        bd.v = tuple(d * speed)
+        bd.w = tuple(rng.normal(0, 3.0, 3))
+        bd.flags |= _abi.FLAG_CCD
+        if k == 7:
+            bd.mask = 0xFFFFFFFF & ~(1 << 6)   # passes the layer-6 plate
+        if k == 11:
+            bd.flags |= _abi.FLAG_SLEEPING
+            bd.v, bd.w = (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)
+        b.append(bd)
+    for k in range(10):   # ordinary bodies in the way
+        b.append(S.make_sphere(len(b), tuple(rng.uniform(-2.5, 2.5, 3)), 0.3, 1.0))
+    s = S.Scene.from_bodies(b)
+    s.params = S.Params(half=(h, h, h), periodic=periodic, dt=1.0 / 60.0,
+                        gravity=(0.0, 0.0, 0.0) if periodic else (0.0, -9.81, 0.0), restitution=0.2, friction=0.4)
+    return s
 
 
 def elastic_gas_4096():
