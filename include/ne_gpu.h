@@ -145,6 +145,30 @@ typedef struct ne_distance_joint {
     double   rest, stiffness, damping;
 } ne_distance_joint;
 
+/* A general joint (`ne::Joint`, src/world.hpp:31-54): ball, hinge, fixed or
+ * slider between bodies a and b, with anchors and axes in each body's frame
+ * and ref_rel = qA * conj(qB) at creation (PhysicsWorld::addGeneralJoint,
+ * src/physics_world.hpp:408-423, computes them from world anchors).  Hinges
+ * may drive a motor (clamped impulse per step) and hold an angle limit;
+ * breakable joints break when a step's accumulated impulse exceeds
+ * break_impulse; broken is the persistent state (a broken joint is skipped). */
+#define NE_JOINT_BALL    0
+#define NE_JOINT_HINGE   1
+#define NE_JOINT_FIXED   2
+#define NE_JOINT_SLIDER  3
+
+typedef struct ne_joint {
+    int32_t  type;                 /* NE_JOINT_*                            */
+    uint32_t a, b;
+    int32_t  use_motor, use_limit, breakable, broken;
+    double   local_a[3], local_b[3];
+    double   axis_a[3], axis_b[3];
+    double   ref_rel[4];           /* w, x, y, z                            */
+    double   motor_speed, max_motor_impulse;
+    double   lower, upper;         /* hinge angle limits (rad)              */
+    double   break_impulse;
+} ne_joint;
+
 /* A static triangle mesh attached to a body of shape NE_SHAPE_MESH
  * (`Body::mesh`, src/body.hpp:62; `MeshData`, src/mesh.hpp:14-93): vertices
  * in the body frame, triangles as vertex-index triples.  A heightfield
@@ -201,6 +225,24 @@ int ne_gpu_set_bodies(ne_gpu_ctx *ctx, const ne_bodies *b);
  * uploaded first; ne_gpu_set_bodies clears the joints. */
 int ne_gpu_set_joints(ne_gpu_ctx *ctx, uint32_t n, const ne_distance_joint *joints);
 
+/* Replaces assigning `World::joints` (src/world.hpp:130): the general joints,
+ * solved after the rigid distance joints by `World::solveJoints`
+ * (src/world.cpp:206-315), 8 sequential-impulse iterations over the joints
+ * in the order given.  Bodies must be uploaded first; ne_gpu_set_bodies
+ * clears them.  The hinge limit's angle is 2 atan2(.) of the relative
+ * rotation (world.cpp:300); atan2 here is rounded to nearest, which the host
+ * C library's is not always (glibc 2.35: off by one unit in the last place
+ * for about 4 in 10,000 arguments), so a step whose active hinge limit meets
+ * such an argument differs from the reference's in the last bits; every
+ * other joint term is the reference's arithmetic. */
+int ne_gpu_set_general_joints(ne_gpu_ctx *ctx, uint32_t n, const ne_joint *joints);
+
+/* The general joints' runtime state after the last step, n entries each (any
+ * may be NULL): broken, the impulse accumulated in that step's solve (the
+ * break test's appliedImpulse) and the motor accumulator. */
+int ne_gpu_get_general_joints(ne_gpu_ctx *ctx, uint32_t n, int32_t *broken, double *applied_impulse,
+                              double *motor_acc);
+
 /* Replaces building `MeshData` and assigning `Body::mesh` (src/mesh.hpp:38-43,
  * src/physics_world.hpp:83-93): each mesh's bounding-volume tree is built
  * here, once, by the reference's median split (leaves of at most 4
@@ -225,8 +267,8 @@ int ne_gpu_set_state(ne_gpu_ctx *ctx, const double *x, const double *v,
 
 /* Replaces `World::step()` (src/world.cpp:133-146), nsteps times, resident:
  * spring joints, integrate, the CCD pass of bodies flagged NE_FLAG_CCD
- * (World::ccdPass, :658-696), collide, rigid joints, wrap or walls,
- * sleeping.  General joints (World::solveJoints) are not on this path.
+ * (World::ccdPass, :658-696), collide, rigid distance joints, general joints
+ * (World::solveJoints, :206-315), wrap or walls, sleeping.
  * Work buffers (candidate pairs, constraints, the large EPA workspace) are
  * sized from the bodies; when one overflows, the call enlarges it, restores
  * the state it started from (a device-side copy taken at the start of every
@@ -419,7 +461,9 @@ int ne_gpu_time_kernel(ne_gpu_ctx *ctx, int32_t kernel_id);
 int ne_gpu_kernel_time(ne_gpu_ctx *ctx, double *total_ms, uint32_t *launches);
 
 /* Device arithmetic probe: out[t] = sqrt(a) (op 0), a/b (1), floor(a) (2),
- * a*b+a (3), computed by a kernel -- checks the IEEE ground truth. */
+ * a*b+a (3), atan2(a, b) rounded to nearest as the hinge limit takes it (4),
+ * the device library's atan2(a, b) (5), computed by a kernel -- checks the
+ * IEEE ground truth and the two atan2s against the host's. */
 int ne_gpu_fp_probe(ne_gpu_ctx *ctx, int32_t op, uint32_t n, const double *a,
                     const double *b, double *out);
 

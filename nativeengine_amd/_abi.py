@@ -100,6 +100,29 @@ JOINT_DTYPE = np.dtype([("a", "<u4"), ("b", "<u4"), ("local_a", "<f8", (3,)), ("
                         ("rest", "<f8"), ("stiffness", "<f8"), ("damping", "<f8")])
 
 
+class Joint(C.Structure):
+    """ne_joint: a general joint (World::joints)."""
+    _fields_ = [
+        ("type", C.c_int32), ("a", C.c_uint32), ("b", C.c_uint32),
+        ("use_motor", C.c_int32), ("use_limit", C.c_int32), ("breakable", C.c_int32), ("broken", C.c_int32),
+        ("local_a", C.c_double * 3), ("local_b", C.c_double * 3), ("axis_a", C.c_double * 3),
+        ("axis_b", C.c_double * 3), ("ref_rel", C.c_double * 4),
+        ("motor_speed", C.c_double), ("max_motor_impulse", C.c_double), ("lower", C.c_double), ("upper", C.c_double),
+        ("break_impulse", C.c_double),
+    ]
+
+
+JOINT_BALL, JOINT_HINGE, JOINT_FIXED, JOINT_SLIDER = range(4)
+
+# ne_joint as a numpy record (the C layout, offsets from the ctypes mirror)
+_GJ = [("type", "<i4"), ("a", "<u4"), ("b", "<u4"), ("use_motor", "<i4"), ("use_limit", "<i4"), ("breakable", "<i4"),
+       ("broken", "<i4"), ("local_a", ("<f8", (3,))), ("local_b", ("<f8", (3,))), ("axis_a", ("<f8", (3,))),
+       ("axis_b", ("<f8", (3,))), ("ref_rel", ("<f8", (4,))), ("motor_speed", "<f8"), ("max_motor_impulse", "<f8"),
+       ("lower", "<f8"), ("upper", "<f8"), ("break_impulse", "<f8")]
+GJOINT_DTYPE = np.dtype({"names": [n for n, _ in _GJ], "formats": [f for _, f in _GJ],
+                         "offsets": [getattr(Joint, n).offset for n, _ in _GJ], "itemsize": C.sizeof(Joint)})
+
+
 class Mesh(C.Structure):
     """ne_mesh: a static triangle mesh attached to a NE_SHAPE_MESH body."""
     _fields_ = [

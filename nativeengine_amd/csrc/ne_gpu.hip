@@ -36,6 +36,7 @@
 #include "ne_solve.hpp"
 #include "ne_adjoint.hpp"
 #include "ne_prims.hpp"
+#include "ne_atan2.hpp"
 
 using namespace nea;
 
@@ -2908,6 +2909,8 @@ __global__ void k_fp_probe(int op, uint32_t n, const double* a, const double* b,
     if (op == 0) r = sqrt(a[t]);
     else if (op == 1) r = a[t] / b[t];
     else if (op == 2) r = floor(a[t]);
+    else if (op == 4) r = atan2_rn(a[t], b[t]);   // the hinge limit's, rounded to nearest
+    else if (op == 5) r = atan2(a[t], b[t]);      // the device library's
     else r = a[t] * b[t] + a[t];
     out[t] = r;
 }
@@ -3670,6 +3673,12 @@ struct ne_gpu_ctx {
     double* ev_dat[2] = {nullptr, nullptr};
     uint32_t* ev_state[2] = {nullptr, nullptr};
     uint32_t *ev_off = nullptr, *ev_n = nullptr, *ev_saved = nullptr;
+    // general joints (World::joints): the records on the device, the step's
+    // accumulators, and the components (ncomp + 1 starts, then joint lists)
+    uint32_t ngj = 0, ngj_comp = 0;
+    ne_joint* gj = nullptr;
+    double *gj_applied = nullptr, *gj_motor = nullptr;
+    uint32_t* gj_comp = nullptr;
     // continuous collision (World::ccdPass): the CCD bodies in index order,
     // their step-start positions, and the candidate pass's counts, offsets,
     // bodies and tile boxes
@@ -3770,10 +3779,11 @@ struct ne_gpu_ctx {
     uint32_t timed_launches = 0;
 };
 
-// Scene queries (rays, swept spheres, overlaps) and continuous collision:
-// kernels over the body layout above.
+// Scene queries (rays, swept spheres, overlaps), continuous collision and
+// general joints: kernels over the body layout above.
 #include "ne_query.hpp"
 #include "ne_ccd.hpp"
+#include "ne_joint.hpp"
 
 namespace {
 
@@ -4291,6 +4301,8 @@ void free_all(ne_gpu_ctx* c) {
     dfree(c->ev_off); dfree(c->ev_n); dfree(c->ev_saved);
     dfree(c->q_mem_a); dfree(c->q_mem_b); dfree(c->q_mem_w);
     dfree(c->ccd_idx); dfree(c->ccd_cnt); dfree(c->ccd_off); dfree(c->ccd_cand); dfree(c->ccd_prev); dfree(c->ccd_tiles);
+    dfree(c->gj); dfree(c->gj_applied); dfree(c->gj_motor); dfree(c->gj_comp);
+    c->ngj = c->ngj_comp = 0;
     c->nccd = c->ccd_cap = 0;
     c->q_cap_a = c->q_cap_b = c->q_cap_w = 0;
     c->ev_cap = 0;
@@ -4494,7 +4506,8 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
         k_ccd_save<<<blocks_for(c->nccd), TPB, 0, s>>>(B, c->ccd_idx, c->nccd, c->ccd_prev);
     {
         KTimer kt(c, NE_KERNEL_INTEGRATE);
-        const bool w = wrap_first && P.box.periodic && c->njoints == 0 && !P.sleep && !c->events_on && !c->nccd;
+        const bool w = wrap_first && P.box.periodic && c->njoints == 0 && c->ngj == 0 && !P.sleep && !c->events_on &&
+                       !c->nccd;
         if (c->has_forces || c->nspring) {
             if (w) launch_integrate<true, true>(c, P, static_grid);
             else launch_integrate<true, false>(c, P, static_grid);
@@ -4757,13 +4770,17 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     const SimParams P = make_sim_params(c->params);
     if (c->nrigid)   // rigid joints after the contacts (world.cpp:139)
         k_joint_rigid<<<blocks_for(c->ncomp, 64), 64, 0, c->stream>>>(c->B, P, joint_dev(c));
+    if (c->ngj) {    // then the general joints (world.cpp:140)
+        const GJointDev G{c->gj, c->gj_applied, c->gj_motor, c->gj_comp, c->gj_comp + c->ngj_comp + 1, c->ngj_comp};
+        k_joint_general<<<blocks_for(c->ngj_comp, 64), 64, 0, c->stream>>>(c->B, P, G);
+    }
     // --- boundary.  The periodic wrap is deferred to the next integrate,
     // except with joints: spring forces at the next step's start read the
     // wrapped positions.
     if (!P.box.periodic && !P.open_boundary) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_walls<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P);
-    } else if (P.box.periodic && (c->njoints || P.sleep || c->events_on || c->nccd)) {
+    } else if (P.box.periodic && (c->njoints || c->ngj || P.sleep || c->events_on || c->nccd)) {
         KTimer kt(c, NE_KERNEL_BOUNDARY);
         k_wrap<<<blocks_for(c->n), TPB, 0, c->stream>>>(c->B, P);
     }
@@ -4775,6 +4792,8 @@ int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
         if (c->njoints)
             k_sleep_join_joints<<<blocks_for(c->njoints), TPB, 0, s>>>(c->B, c->j_ab, c->j_ab + c->njoints, c->njoints,
                                                                         c->parent);
+        if (c->ngj)
+            k_sleep_join_gjoints<<<blocks_for(c->ngj), TPB, 0, s>>>(c->B, c->gj, c->ngj, c->parent);
         k_sleep_timers<<<blocks_for(c->n), TPB, 0, s>>>(c->B, P, c->parent, c->mint);
         k_sleep_apply<<<blocks_for(c->n), TPB, 0, s>>>(c->B, P, c->parent, c->mint);
     }
@@ -5034,7 +5053,7 @@ int ne_gpu_rollout(ne_gpu_ctx* c, const double* v0, const double* mu, const doub
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
     if (c->n == 0) return set_err(c, NE_ERR_INVALID, "no bodies");
     if (!c->params.periodic) return set_err(c, NE_ERR_UNSUPPORTED, "rollouts are differentiated for periodic worlds");
-    if (c->has_forces || c->njoints || c->params.sleep_enabled || c->nmeshes || c->nccd)
+    if (c->has_forces || c->njoints || c->ngj || c->params.sleep_enabled || c->nmeshes || c->nccd)
         return set_err(c, NE_ERR_UNSUPPORTED, "rollouts do not take applied forces, joints, sleeping, meshes or CCD");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const uint32_t nw = c->nworlds ? c->nworlds : 1u;
@@ -5789,6 +5808,70 @@ int ne_gpu_overlap(ne_gpu_ctx* c, uint32_t n, const ne_query_shape* shapes, cons
     return capped_all ? NE_WARN_EPA_CAPPED : NE_OK;
 }
 
+int ne_gpu_set_general_joints(ne_gpu_ctx* c, uint32_t n, const ne_joint* j) {
+    if (!c || (n && !j)) return NE_ERR_INVALID;
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    if (n && c->nworlds) return set_err(c, NE_ERR_UNSUPPORTED, "general joints in batched worlds");
+    for (uint32_t k = 0; k < n; ++k) {
+        if (j[k].a >= c->n || j[k].b >= c->n) return set_err(c, NE_ERR_INVALID, "joint body index out of range");
+        if (j[k].a == j[k].b) return set_err(c, NE_ERR_INVALID, "a joint needs two different bodies");
+        if (j[k].type < NE_JOINT_BALL || j[k].type > NE_JOINT_SLIDER) return set_err(c, NE_ERR_INVALID, "joint type");
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    c->ngj = c->ngj_comp = 0;
+    if (n == 0) return NE_OK;
+    // components: union-find over the bodies that can move; joints between
+    // immovable bodies alone form components of their own
+    std::vector<uint32_t> parent(c->n);
+    for (uint32_t i = 0; i < c->n; ++i) parent[i] = i;
+    auto find = [&](uint32_t x) { while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; } return x; };
+    for (uint32_t k = 0; k < n; ++k)
+        if (c->h_mut[j[k].a] && c->h_mut[j[k].b]) {
+            const uint32_t ra = find(j[k].a), rb = find(j[k].b);
+            if (ra != rb) parent[std::max(ra, rb)] = std::min(ra, rb);
+        }
+    std::vector<uint32_t> comp(n), count, comp_of_root(c->n, 0xFFFFFFFFu);
+    uint32_t ncomp = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t a = j[k].a, b = j[k].b;
+        if (!c->h_mut[a] && !c->h_mut[b]) { comp[k] = ncomp++; count.push_back(1); continue; }
+        const uint32_t root = c->h_mut[a] ? find(a) : find(b);
+        uint32_t& id = comp_of_root[root];
+        if (id == 0xFFFFFFFFu) { id = ncomp++; count.push_back(0); }
+        comp[k] = id;
+        ++count[id];
+    }
+    std::vector<uint32_t> cl(ncomp + 1 + n, 0), fill(ncomp, 0);
+    for (uint32_t q = 0; q < ncomp; ++q) cl[q + 1] = cl[q] + count[q];
+    for (uint32_t k = 0; k < n; ++k) cl[ncomp + 1 + cl[comp[k]] + fill[comp[k]]++] = k;   // ascending joint order
+    HIP_TRY(c, dalloc(c->gj, n));
+    HIP_TRY(c, dalloc(c->gj_applied, n));
+    HIP_TRY(c, dalloc(c->gj_motor, n));
+    HIP_TRY(c, dalloc(c->gj_comp, cl.size()));
+    HIP_TRY(c, hipMemcpy(c->gj, j, sizeof(ne_joint) * n, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->gj_comp, cl.data(), sizeof(uint32_t) * cl.size(), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemset(c->gj_applied, 0, sizeof(double) * n));
+    HIP_TRY(c, hipMemset(c->gj_motor, 0, sizeof(double) * n));
+    c->ngj = n;
+    c->ngj_comp = ncomp;
+    return NE_OK;
+}
+
+int ne_gpu_get_general_joints(ne_gpu_ctx* c, uint32_t n, int32_t* broken, double* applied, double* motor) {
+    if (!c || n > c->ngj) return NE_ERR_INVALID;
+    if (n == 0) return NE_OK;
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    std::vector<ne_joint> h(n);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(h.data(), c->gj, sizeof(ne_joint) * n, hipMemcpyDeviceToHost));
+    if (applied) HIP_TRY(c, hipMemcpy(applied, c->gj_applied, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (motor) HIP_TRY(c, hipMemcpy(motor, c->gj_motor, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (broken)
+        for (uint32_t k = 0; k < n; ++k) broken[k] = h[k].broken;
+    return NE_OK;
+}
+
 int ne_gpu_set_joints(ne_gpu_ctx* c, uint32_t n, const ne_distance_joint* j) {
     if (!c || (n && !j)) return NE_ERR_INVALID;
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
@@ -6056,6 +6139,17 @@ int ne_gpu_get_sleep_store(ne_gpu_ctx* c, double* inv_mass_store, double* inv_in
     return NE_OK;
 }
 
+// The general joints' broken flags as the snapshot's bytes, and back
+// (physics_world.cpp:36-37, 56-61).
+__global__ void k_nes1_joints_pack(const ne_joint* __restrict__ gj, uint32_t n, uint8_t* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = gj[k].broken ? 1u : 0u;
+}
+__global__ void k_nes1_joints_unpack(ne_joint* __restrict__ gj, uint32_t n, const uint8_t* __restrict__ in) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) gj[k].broken = in[k] != 0 ? 1 : 0;
+}
+
 int ensure_nes1(ne_gpu_ctx* c, uint64_t bytes) {
     if (c->nes1_cap >= bytes && c->nes1) return NE_OK;
     HIP_TRY(c, dalloc(c->nes1, (size_t)bytes));
@@ -6067,7 +6161,8 @@ int ne_gpu_save_state(ne_gpu_ctx* c, void* out, uint64_t cap, uint64_t* size) {
     if (!c || !size) return NE_ERR_INVALID;
     const uint32_t n = c->n;
     const uint64_t recs = NES1_RECORD * (uint64_t)n;
-    const uint64_t need = NES1_HEAD + recs + 4;
+    const uint32_t ngj = c->ngj;
+    const uint64_t need = NES1_HEAD + recs + 4 + ngj;
     *size = need;
     if (!out) return NE_OK;
     if (cap < need) return set_err(c, NE_ERR_INVALID, "ne_gpu_save_state: the buffer is smaller than the snapshot");
@@ -6075,17 +6170,24 @@ int ne_gpu_save_state(ne_gpu_ctx* c, void* out, uint64_t cap, uint64_t* size) {
     uint8_t* o = static_cast<uint8_t*>(out);
     const uint32_t head[2] = {NES1_MAGIC, n};
     std::memcpy(o, head, sizeof(head));
-    if (n) {
-        const int r = ensure_nes1(c, recs);
+    if (n || ngj) {
+        const int r = ensure_nes1(c, recs + ngj);
         if (r != NE_OK) return r;
+    }
+    if (n) {
         k_nes1_pack<<<blocks_for(n), TPB, 0, c->stream>>>(c->dfield, c->ncap, c->ufield + (size_t)U_FLAGS * c->ncap, n,
                                                           c->nes1);
         HIP_TRY(c, hipGetLastError());
         HIP_TRY(c, hipMemcpyAsync(o + NES1_HEAD, c->nes1, (size_t)recs, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
     }
-    const uint32_t njoints = 0;   // the facade's general joints are not on this path
-    std::memcpy(o + NES1_HEAD + recs, &njoints, 4);
+    std::memcpy(o + NES1_HEAD + recs, &ngj, 4);   // the general joints' count and broken flags
+    if (ngj) {
+        k_nes1_joints_pack<<<blocks_for(ngj), TPB, 0, c->stream>>>(c->gj, ngj, c->nes1 + recs);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(o + NES1_HEAD + recs + 4, c->nes1 + recs, ngj, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
     return NE_OK;
 }
 
@@ -6093,12 +6195,13 @@ int ne_gpu_save_state_noalloc(ne_gpu_ctx* c, void* out, uint64_t cap, uint64_t* 
     if (!c || !size) return NE_ERR_INVALID;
     const uint32_t n = c->n;
     const uint64_t recs = NES1_RECORD * (uint64_t)n;
-    const uint64_t need = NES1_HEAD + recs + 4;
+    const uint32_t ngj = c->ngj;
+    const uint64_t need = NES1_HEAD + recs + 4 + ngj;
     *size = need;
-    if (recs > NOALLOC_SCRATCH_BYTES) {
+    if (recs + ngj > NOALLOC_SCRATCH_BYTES) {
         char buf[200];
         snprintf(buf, sizeof buf, "ne_gpu_save_state_noalloc: %u bodies need %llu bytes of staging; its static buffer "
-                 "holds %llu (%llu bodies): use ne_gpu_save_state", n, (unsigned long long)recs,
+                 "holds %llu (%llu bodies): use ne_gpu_save_state", n, (unsigned long long)(recs + ngj),
                  (unsigned long long)NOALLOC_SCRATCH_BYTES, (unsigned long long)(NOALLOC_SCRATCH_BYTES / NES1_RECORD));
         return set_err(c, NE_ERR_INVALID, buf);
     }
@@ -6116,8 +6219,14 @@ int ne_gpu_save_state_noalloc(ne_gpu_ctx* c, void* out, uint64_t cap, uint64_t* 
         HIP_TRY(c, hipMemcpyAsync(o + NES1_HEAD, c->scratch_dev, (size_t)recs, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
     }
-    const uint32_t njoints = 0;   // the facade's general joints are not on this path
-    std::memcpy(o + NES1_HEAD + recs, &njoints, 4);
+    std::memcpy(o + NES1_HEAD + recs, &ngj, 4);   // the general joints' count and broken flags
+    if (ngj) {
+        k_nes1_joints_pack<<<blocks_for(ngj), TPB, 0, c->stream>>>(c->gj, ngj, c->scratch_dev + recs);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(o + NES1_HEAD + recs + 4, c->scratch_dev + recs, ngj, hipMemcpyDeviceToHost,
+                                  c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
     return NE_OK;
 }
 
@@ -6134,16 +6243,25 @@ int ne_gpu_load_state(ne_gpu_ctx* c, const void* data, uint64_t size) {
     if (size < NES1_HEAD + recs + 4) return set_err(c, NE_ERR_INVALID, "ne_gpu_load_state: truncated snapshot");
     uint32_t njoints = 0;
     std::memcpy(&njoints, d + NES1_HEAD + recs, 4);
-    if (njoints != 0)
-        return set_err(c, NE_ERR_INVALID, "ne_gpu_load_state: the snapshot has facade joints, which this path has none of");
+    if (njoints != c->ngj)
+        return set_err(c, NE_ERR_INVALID, "ne_gpu_load_state: the snapshot's general-joint count differs from the world's");
+    if (size < NES1_HEAD + recs + 4 + njoints) return set_err(c, NE_ERR_INVALID, "ne_gpu_load_state: truncated snapshot");
     if (hipSetDevice(c->device) != hipSuccess) return set_err(c, NE_ERR_HIP, "hipSetDevice failed");
+    int r = ensure_nes1(c, recs + njoints);
+    if (r != NE_OK) return r;
+    auto unpack_joints = [&]() -> int {   // the general joints' broken flags
+        if (!njoints) return NE_OK;
+        HIP_TRY(c, hipMemcpyAsync(c->nes1 + recs, d + NES1_HEAD + recs + 4, njoints, hipMemcpyHostToDevice, c->stream));
+        k_nes1_joints_unpack<<<blocks_for(njoints), TPB, 0, c->stream>>>(c->gj, njoints, c->nes1 + recs);
+        HIP_TRY(c, hipGetLastError());
+        return NE_OK;
+    };
     if (n == 0) {
+        if ((r = unpack_joints()) != NE_OK) return r;
         HIP_TRY(c, hipMemsetAsync(c->prev_n, 0, sizeof(uint32_t), c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         return NE_OK;
     }
-    int r = ensure_nes1(c, recs);
-    if (r != NE_OK) return r;
     uint32_t* flags = c->ufield + (size_t)U_FLAGS * c->ncap;
     HIP_TRY(c, hipMemcpyAsync(c->nes1, d + NES1_HEAD, (size_t)recs, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->counters + CNT_HOSTCHK, 0, sizeof(uint32_t), c->stream));
@@ -6156,6 +6274,7 @@ int ne_gpu_load_state(ne_gpu_ctx* c, const void* data, uint64_t size) {
                        "ne_gpu_load_state: a static body kept out of the grid would move; upload with ne_gpu_set_bodies");
     k_nes1_unpack<<<blocks_for(n), TPB, 0, c->stream>>>(c->dfield, c->ncap, flags, n, c->nes1);
     HIP_TRY(c, hipGetLastError());
+    if ((r = unpack_joints()) != NE_OK) return r;
     HIP_TRY(c, hipMemsetAsync(c->prev_n, 0, sizeof(uint32_t), c->stream));   // resetWarmCache (world.hpp:140)
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return NE_OK;
@@ -6436,6 +6555,7 @@ int ne_gpu_set_worlds(ne_gpu_ctx* c, uint32_t nworlds, const uint32_t* offsets, 
     if (!offsets || !half) return set_err(c, NE_ERR_INVALID, "offsets and half are required");
     if (!c->params.periodic) return set_err(c, NE_ERR_UNSUPPORTED, "batched worlds must be periodic");
     if (c->nccd) return set_err(c, NE_ERR_UNSUPPORTED, "CCD bodies in batched worlds");
+    if (c->ngj) return set_err(c, NE_ERR_UNSUPPORTED, "general joints in batched worlds");
     if (offsets[0] != 0 || offsets[nworlds] != c->n)
         return set_err(c, NE_ERR_INVALID, "world offsets must run from 0 to the body count");
     uint32_t maxsize = 0;
@@ -6754,6 +6874,7 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
         }
     }
     c->njoints = c->nspring = c->nrigid = c->ncomp = 0;   // and without joints
+    c->ngj = c->ngj_comp = 0;
     c->h_mut.assign(n, 0);
     for (uint32_t i = 0; i < n; ++i)
         c->h_mut[i] = b->inv_mass[i] != 0.0 || b->inv_inertia[3 * i] != 0.0 || b->inv_inertia[3 * i + 1] != 0.0 ||

@@ -34,6 +34,7 @@ EXPORTS = [
     "ne_gpu_rollout", "ne_gpu_rollout_backward", "ne_gpu_set_joints", "ne_gpu_set_meshes", "ne_gpu_mesh_bvh",
     "ne_gpu_get_sleep_store", "ne_gpu_save_state", "ne_gpu_load_state", "ne_gpu_save_state_noalloc",
     "ne_gpu_set_events", "ne_gpu_get_events", "ne_gpu_raycast", "ne_gpu_raycast_all", "ne_gpu_overlap",
+    "ne_gpu_set_general_joints", "ne_gpu_get_general_joints",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
@@ -116,6 +117,8 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_rollout_backward.argtypes = [vp, vp, vp, dp, dp]
     lib.ne_gpu_autotune.argtypes = [vp, C.c_int32, C.POINTER(_abi.TuneResult)]
     queries.bind(lib, "ne_gpu")
+    lib.ne_gpu_set_general_joints.argtypes = [vp, C.c_uint32, vp]
+    lib.ne_gpu_get_general_joints.argtypes = [vp, C.c_uint32, C.POINTER(C.c_int32), dp, dp]
     for name in EXPORTS:
         if name != "ne_gpu_last_error":
             getattr(lib, name).restype = C.c_int
@@ -174,9 +177,12 @@ class GpuWorld:
         b, keep = s.to_c()
         self._check(self.lib.ne_gpu_set_bodies(self.h, C.byref(b)))
         self.n = s.n
+        self._ngj = 0
         del keep
         if len(s.joints):
             self.set_joints(s.joints)
+        if len(getattr(s, "gjoints", ())):
+            self.set_general_joints(s.gjoints)
         if s.meshes:
             arr, keep = s.meshes_c()
             self._check(self.lib.ne_gpu_set_meshes(self.h, len(s.meshes), arr))
@@ -193,6 +199,22 @@ class GpuWorld:
         self._check(self.lib.ne_gpu_mesh_bvh(self.h, m, order.ctypes.data_as(ip), nn.value, C.byref(nn),
                                              _abi.dptr(bounds), links.ctypes.data_as(ip)))
         return order, bounds, links
+
+    def set_general_joints(self, joints):
+        """World::joints: ball / hinge / fixed / slider joints (records of
+        _abi.GJOINT_DTYPE, e.g. from scene.general_joint), solved in order."""
+        j = np.ascontiguousarray(joints, _abi.GJOINT_DTYPE)
+        self._check(self.lib.ne_gpu_set_general_joints(self.h, len(j), j.ctypes.data if len(j) else None))
+        self._ngj = len(j)
+
+    def general_joints(self):
+        """(broken, applied impulse, motor accumulator) of the general joints
+        after the last step."""
+        n = getattr(self, "_ngj", 0)
+        br, ap, mo = np.zeros(n, np.int32), np.zeros(n), np.zeros(n)
+        self._check(self.lib.ne_gpu_get_general_joints(self.h, n, br.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                       _abi.dptr(ap), _abi.dptr(mo)))
+        return br, ap, mo
 
     def set_joints(self, joints):
         """World::distanceJoints: a JOINT_DTYPE record array, in order."""

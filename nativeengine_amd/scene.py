@@ -98,6 +98,7 @@ class Scene:
         self.nocollide_offsets = np.zeros(n + 1, np.uint32)
         self.nocollide = np.zeros(0, np.uint32)
         self.joints = np.zeros(0, _abi.JOINT_DTYPE)   # World::distanceJoints, in order
+        self.gjoints = np.zeros(0, _abi.GJOINT_DTYPE)  # World::joints (general joints), in order
         # static triangle meshes: dicts {body, verts (nv, 3) f64, tris (nt, 3) i32} (Body::mesh)
         self.meshes = []
         # Body::sleepTimer / invMassStore / invIStore (body.hpp); None uploads zeros
@@ -1046,3 +1047,65 @@ def large_scene(n: int = 1 << 20, seed: int = 5, height: float = 100.0, amplitud
     s.params = Params(half=(side / 2.0, hy, side / 2.0), periodic=True, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0),
                       restitution=0.1, friction=0.6, solver_iterations=iterations, sleep_enabled=sleep)
     return s
+
+
+# ---- general joints as PhysicsWorld::addGeneralJoint builds them
+# (src/physics_world.hpp:191-203, 408-423), with the reference's V3 / Q
+# arithmetic (src/vmath.hpp:24-77) in its operation order.
+
+def _v_cross(a, b):
+    return (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
+
+
+def _v_normalized(a):
+    n = math.sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])
+    return (a[0] / n, a[1] / n, a[2] / n) if n > 1e-300 else (0.0, 0.0, 0.0)
+
+
+def _q_rotate(q, v):
+    w, qv = q[0], (q[1], q[2], q[3])
+    c = _v_cross(qv, v)
+    t = (c[0] * 2.0, c[1] * 2.0, c[2] * 2.0)
+    c2 = _v_cross(qv, t)
+    return ((v[0] + t[0] * w) + c2[0], (v[1] + t[1] * w) + c2[1], (v[2] + t[2] * w) + c2[2])
+
+
+def _q_conj(q):
+    return (q[0], -q[1], -q[2], -q[3])
+
+
+def _q_mul(a, o):
+    return (a[0] * o[0] - a[1] * o[1] - a[2] * o[2] - a[3] * o[3],
+            a[0] * o[1] + a[1] * o[0] + a[2] * o[3] - a[3] * o[2],
+            a[0] * o[2] - a[1] * o[3] + a[2] * o[0] + a[3] * o[1],
+            a[0] * o[3] + a[1] * o[2] - a[2] * o[1] + a[3] * o[0])
+
+
+def general_joint(s: Scene, kind: int, a: int, b: int, anchor=None, axis=(0.0, 1.0, 0.0), motor=None, limit=None,
+                  break_impulse=None):
+    """One ne_joint record between bodies a and b of scene s at their
+    current poses: a world anchor (a fixed joint's defaults to the midpoint
+    of the centres, addFixedJoint) and axis, converted to each body's frame,
+    with ref_rel = qA * conj(qB); motor (speed, max impulse), limit
+    (lower, upper), break_impulse as setJointMotor / setJointLimit /
+    setJointBreakable."""
+    xa, xb = tuple(float(v) for v in s.x[a]), tuple(float(v) for v in s.x[b])
+    qa, qb = tuple(float(v) for v in s.q[a]), tuple(float(v) for v in s.q[b])
+    if anchor is None:
+        anchor = ((xa[0] + xb[0]) * 0.5, (xa[1] + xb[1]) * 0.5, (xa[2] + xb[2]) * 0.5)
+    anchor = tuple(float(v) for v in anchor)
+    j = np.zeros((), _abi.GJOINT_DTYPE)
+    j["type"], j["a"], j["b"] = kind, a, b
+    j["local_a"] = _q_rotate(_q_conj(qa), (anchor[0] - xa[0], anchor[1] - xa[1], anchor[2] - xa[2]))
+    j["local_b"] = _q_rotate(_q_conj(qb), (anchor[0] - xb[0], anchor[1] - xb[1], anchor[2] - xb[2]))
+    ax = _v_normalized(tuple(float(v) for v in axis))
+    j["axis_a"] = _q_rotate(_q_conj(qa), ax)
+    j["axis_b"] = _q_rotate(_q_conj(qb), ax)
+    j["ref_rel"] = _q_mul(qa, _q_conj(qb))
+    if motor is not None:
+        j["use_motor"], j["motor_speed"], j["max_motor_impulse"] = 1, float(motor[0]), float(motor[1])
+    if limit is not None:
+        j["use_limit"], j["lower"], j["upper"] = 1, float(limit[0]), float(limit[1])
+    if break_impulse is not None:
+        j["breakable"], j["break_impulse"] = 1, float(break_impulse)
+    return j

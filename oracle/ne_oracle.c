@@ -169,6 +169,11 @@ struct ne_oracle {
      * pairs (physics_world.cpp:253-302), each in (i, j) order */
     int events_on;
     uint32_t nccd;   /* bodies with the CCD flag */
+    /* general joints (world.hpp:130), with their runtime accumulators */
+    uint32_t ngj;
+    ne_joint* gj;
+    double* gj_applied;
+    double* gj_motor;
     uint32_t nev[2], capev[2];
     struct oevent* ev[2];
     int ev_cur;
@@ -1371,6 +1376,8 @@ static void update_sleep(ne_oracle* o) {
     } while (0)
     for (uint32_t k = 0; k < o->nisp; ++k) JOIN(o->isp[2 * k], o->isp[2 * k + 1]);
     for (uint32_t k = 0; k < o->njoints; ++k) JOIN(o->joints[k].a, o->joints[k].b);
+    for (uint32_t k = 0; k < o->ngj; ++k)
+        if (!o->gj[k].broken) JOIN(o->gj[k].a, o->gj[k].b);
 #undef JOIN
     for (uint32_t i = 0; i < n; ++i) {
         obody* b = &o->B[i];
@@ -1400,6 +1407,180 @@ static void update_sleep(ne_oracle* o) {
     free(parent);
     free(mint);
     free(seen);
+}
+
+/* ---- general joints: World::solveJoints (world.cpp:206-315) over Mat3
+ * (vmath.hpp:81-123), m[row*3 + col] ---------------------------------------- */
+typedef struct { double m[9]; } m3;
+
+static m3 m3_zero(void) { m3 r; for (int i = 0; i < 9; ++i) r.m[i] = 0.0; return r; }   /* vmath.hpp:84 */
+static m3 m3_scaled(double s) { m3 r = m3_zero(); r.m[0] = r.m[4] = r.m[8] = s; return r; } /* vmath.hpp:86 */
+static m3 m3_make(double a, double b, double c, double d, double e, double f, double g, double h, double i) {
+    m3 r;
+    r.m[0] = a; r.m[1] = b; r.m[2] = c; r.m[3] = d; r.m[4] = e; r.m[5] = f; r.m[6] = g; r.m[7] = h; r.m[8] = i;
+    return r;
+}
+static m3 m3_skew(v3 v) { return m3_make(0, -v.z, v.y, v.z, 0, -v.x, -v.y, v.x, 0); }     /* vmath.hpp:89-91 */
+static v3 m3_mulv(const m3* a, v3 v) {                                                  /* vmath.hpp:96-100 */
+    return mk(a->m[0] * v.x + a->m[1] * v.y + a->m[2] * v.z, a->m[3] * v.x + a->m[4] * v.y + a->m[5] * v.z,
+              a->m[6] * v.x + a->m[7] * v.y + a->m[8] * v.z);
+}
+static m3 m3_mul(const m3* a, const m3* o) {                                            /* vmath.hpp:101-107 */
+    m3 r = m3_zero();
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) r.m[i * 3 + j] += a->m[i * 3 + k] * o->m[k * 3 + j];
+    return r;
+}
+static m3 m3_add(const m3* a, const m3* o) { m3 r; for (int i = 0; i < 9; ++i) r.m[i] = a->m[i] + o->m[i]; return r; }
+static m3 m3_sub(const m3* a, const m3* o) { m3 r; for (int i = 0; i < 9; ++i) r.m[i] = a->m[i] - o->m[i]; return r; }
+static m3 m3_transposed(const m3* a) {                                                  /* vmath.hpp:110 */
+    return m3_make(a->m[0], a->m[3], a->m[6], a->m[1], a->m[4], a->m[7], a->m[2], a->m[5], a->m[8]);
+}
+static m3 m3_inverse(const m3* M) {                                                     /* vmath.hpp:112-121 */
+    const double a = M->m[0], b = M->m[1], c = M->m[2], d = M->m[3], e = M->m[4], f = M->m[5], g = M->m[6],
+                 h = M->m[7], i = M->m[8];
+    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    double det = a * A + b * B + c * C;
+    if (fabs(det) < 1e-18) return m3_zero();
+    double inv = 1.0 / det;
+    return m3_make(A * inv, (c * h - b * i) * inv, (b * f - c * e) * inv, B * inv, (a * i - c * g) * inv,
+                   (c * d - a * f) * inv, C * inv, (b * g - a * h) * inv, (a * e - b * d) * inv);
+}
+static v3 m3_solve(const m3* M, v3 rhs) { m3 inv = m3_inverse(M); return m3_mulv(&inv, rhs); } /* vmath.hpp:123 */
+
+static m3 inv_inertia_world(const obody* b) {                                           /* body.hpp:184-192 */
+    v3 cx = qrot(b->q, mk(1, 0, 0)), cy = qrot(b->q, mk(0, 1, 0)), cz = qrot(b->q, mk(0, 0, 1));
+    m3 R = m3_make(cx.x, cy.x, cz.x, cx.y, cy.y, cz.y, cx.z, cy.z, cz.z);
+    m3 D = m3_zero();
+    D.m[0] = b->invI.x; D.m[4] = b->invI.y; D.m[8] = b->invI.z;
+    m3 RD = m3_mul(&R, &D), Rt = m3_transposed(&R);
+    return m3_mul(&RD, &Rt);
+}
+
+static v3 perpendicular(v3 v) {                                                        /* world.cpp:200-203 */
+    v3 a = fabs(v.x) < 0.9 ? mk(1, 0, 0) : mk(0, 1, 0);
+    return normalized(cross(v, a));
+}
+static double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); } /* world.cpp:204 */
+
+/* K = (mA + mB) I - [rA]x IA [rA]x - [rB]x IB [rB]x (world.cpp:232-234, 248-250) */
+static m3 point_mass_matrix(const obody* A, const obody* B, const m3* IA, const m3* IB, v3 rA, v3 rB) {
+    m3 s = m3_scaled(A->invMass + B->invMass);
+    m3 sa = m3_skew(rA), sb = m3_skew(rB);
+    m3 t = m3_mul(&sa, IA);
+    m3 ta = m3_mul(&t, &sa);
+    t = m3_mul(&sb, IB);
+    m3 tb = m3_mul(&t, &sb);
+    m3 k1 = m3_sub(&s, &ta);
+    return m3_sub(&k1, &tb);
+}
+
+static void solve_joints(ne_oracle* o, int iterations) {                                /* world.cpp:206-315 */
+    const double beta = 0.2, dt = o->P.dt;
+    for (uint32_t k = 0; k < o->ngj; ++k) { o->gj_applied[k] = 0.0; o->gj_motor[k] = 0.0; }
+    for (int it = 0; it < iterations; ++it) {
+        for (uint32_t k = 0; k < o->ngj; ++k) {
+            ne_joint* j = &o->gj[k];
+            if (j->broken) continue;
+            obody* A = &o->B[j->a];
+            obody* B = &o->B[j->b];
+            const m3 IA = inv_inertia_world(A), IB = inv_inertia_world(B);
+            v3 rA = qrot(A->q, ld3(j->local_a, 0)), rB = qrot(B->q, ld3(j->local_b, 0));
+            v3 posErr = min_image(&o->box, sub(add(A->x, rA), add(B->x, rB)));
+            v3 vRel = sub(add(A->v, cross(A->w, rA)), add(B->v, cross(B->w, rB)));
+            if (j->type == NE_JOINT_SLIDER) {
+                v3 ax = normalized(qrot(A->q, ld3(j->axis_a, 0)));
+                v3 t1 = perpendicular(ax), t2 = normalized(cross(ax, t1));
+                m3 K = point_mass_matrix(A, B, &IA, &IB, rA, rB);
+                const v3 ts[2] = {t1, t2};
+                for (int q = 0; q < 2; ++q) {
+                    const v3 t = ts[q];
+                    double jv = dot(vRel, t) + dot(posErr, t) * (beta / dt);
+                    double kk = dot(t, m3_mulv(&K, t));
+                    if (kk > 1e-18) {
+                        double l = -jv / kk;
+                        v3 P = scl(t, l);
+                        A->v = add(A->v, scl(P, A->invMass)); A->w = add(A->w, m3_mulv(&IA, cross(rA, P)));
+                        B->v = sub(B->v, scl(P, B->invMass)); B->w = sub(B->w, m3_mulv(&IB, cross(rB, P)));
+                        o->gj_applied[k] += fabs(l);
+                        vRel = sub(add(A->v, cross(A->w, rA)), add(B->v, cross(B->w, rB)));
+                    }
+                }
+            } else {
+                m3 K = point_mass_matrix(A, B, &IA, &IB, rA, rB);
+                v3 P = m3_solve(&K, neg(add(vRel, scl(posErr, beta / dt))));
+                A->v = add(A->v, scl(P, A->invMass)); A->w = add(A->w, m3_mulv(&IA, cross(rA, P)));
+                B->v = sub(B->v, scl(P, B->invMass)); B->w = sub(B->w, m3_mulv(&IB, cross(rB, P)));
+                o->gj_applied[k] += norm(P);
+            }
+            if (j->type == NE_JOINT_FIXED || j->type == NE_JOINT_SLIDER) {
+                v3 wRel = sub(A->w, B->w);
+                qt dq = qmul(qmul(A->q, qconj(B->q)), qconj(mq(j->ref_rel[0], j->ref_rel[1], j->ref_rel[2], j->ref_rel[3])));
+                v3 angErr = mk(dq.x, dq.y, dq.z);
+                if (dq.w < 0.0) angErr = neg(angErr);
+                angErr = scl(angErr, 2.0);
+                m3 Kang = m3_add(&IA, &IB);
+                v3 L = m3_solve(&Kang, neg(add(wRel, scl(angErr, beta / dt))));
+                A->w = add(A->w, m3_mulv(&IA, L));
+                B->w = sub(B->w, m3_mulv(&IB, L));
+            } else if (j->type == NE_JOINT_HINGE) {
+                v3 aA = normalized(qrot(A->q, ld3(j->axis_a, 0)));
+                v3 aB = normalized(qrot(B->q, ld3(j->axis_b, 0)));
+                v3 t1 = perpendicular(aA), t2 = normalized(cross(aA, t1));
+                v3 alignErr = cross(aA, aB);
+                m3 Kang = m3_add(&IA, &IB);
+                const v3 ts[2] = {t1, t2};
+                for (int q = 0; q < 2; ++q) {
+                    const v3 t = ts[q];
+                    v3 wRel = sub(A->w, B->w);
+                    double jv = dot(wRel, t) - dot(alignErr, t) * (beta / dt);
+                    double kk = dot(t, m3_mulv(&Kang, t));
+                    if (kk > 1e-18) {
+                        v3 L = scl(t, -jv / kk);
+                        A->w = add(A->w, m3_mulv(&IA, L));
+                        B->w = sub(B->w, m3_mulv(&IB, L));
+                    }
+                }
+                if (j->use_motor) {
+                    double kk = dot(aA, m3_mulv(&Kang, aA));
+                    if (kk > 1e-18) {
+                        double jv = dot(sub(A->w, B->w), aA) - j->motor_speed;
+                        double l = -jv / kk;
+                        double old = o->gj_motor[k];
+                        double nn = clampd(old + l, -j->max_motor_impulse, j->max_motor_impulse);
+                        l = nn - old;
+                        o->gj_motor[k] = nn;
+                        v3 L = scl(aA, l);
+                        A->w = add(A->w, m3_mulv(&IA, L));
+                        B->w = sub(B->w, m3_mulv(&IB, L));
+                    }
+                }
+                if (j->use_limit) {
+                    qt dq = qmul(qmul(A->q, qconj(B->q)), qconj(mq(j->ref_rel[0], j->ref_rel[1], j->ref_rel[2], j->ref_rel[3])));
+                    v3 dv = mk(dq.x, dq.y, dq.z);
+                    double angle = 2.0 * atan2(dot(dv, aA), dq.w);
+                    double c = 0.0;
+                    int active = 0;
+                    if (angle <= j->lower) { c = angle - j->lower; active = 1; }
+                    else if (angle >= j->upper) { c = angle - j->upper; active = 1; }
+                    if (active) {
+                        double kk = dot(aA, m3_mulv(&Kang, aA));
+                        if (kk > 1e-18) {
+                            double jv = dot(sub(A->w, B->w), aA) + c * (beta / dt);
+                            v3 L = scl(aA, -jv / kk);
+                            A->w = add(A->w, m3_mulv(&IA, L));
+                            B->w = sub(B->w, m3_mulv(&IB, L));
+                        }
+                    }
+                }
+            }
+        }
+    }
+    for (uint32_t k = 0; k < o->ngj; ++k) {
+        ne_joint* j = &o->gj[k];
+        if (j->breakable && !j->broken && o->gj_applied[k] > j->break_impulse) j->broken = 1;
+    }
 }
 
 static int ray_vs_body(int shape, qt q, v3 he, double r, double h, v3 c, v3 o, v3 d, double maxT,
@@ -1456,6 +1637,7 @@ static void world_step(ne_oracle* o) {                                          
     if (o->nccd) ccd_pass(o);
     collide(o);
     solve_rigid_joints(o, 8);
+    if (o->ngj) solve_joints(o, 8);
     if (o->box.periodic) wrap_positions(o);
     else if (!o->P.open_boundary) apply_walls(o);
     if (o->P.sleep_enabled) update_sleep(o);
@@ -1546,6 +1728,7 @@ int ne_oracle_destroy(ne_oracle* o) {
     free(o->joints);
     free(o->isp);
     free(o->ev[0]); free(o->ev[1]);
+    free(o->gj); free(o->gj_applied); free(o->gj_motor);
     free(o);
     return NE_OK;
 }
@@ -1563,6 +1746,7 @@ int ne_oracle_set_bodies(ne_oracle* o, const ne_bodies* b) {
     if (!o || !b) return NE_ERR_INVALID;
     o->nev[0] = o->nev[1] = 0;   /* a new world: no last-step pairs */
     o->nccd = 0;
+    o->ngj = 0;   /* and no general joints */
     free_meshes(o);
     free(o->B); free(o->hull); free(o->nocol);
     o->B = NULL; o->hull = NULL; o->nocol = NULL; o->nhull = 0; o->nnocol = 0;
@@ -2181,4 +2365,29 @@ int ne_oracle_overlap(ne_oracle* o, uint32_t n, const ne_query_shape* shapes, co
     }
     free(hv);
     return rc;
+}
+
+/* World::joints (world.hpp:130): the general joints, in order. */
+int ne_oracle_set_general_joints(ne_oracle* o, uint32_t n, const ne_joint* joints) {
+    if (!o || (n && !joints)) return NE_ERR_INVALID;
+    for (uint32_t k = 0; k < n; ++k)
+        if (joints[k].a >= o->n || joints[k].b >= o->n || joints[k].type < NE_JOINT_BALL || joints[k].type > NE_JOINT_SLIDER)
+            return NE_ERR_INVALID;
+    free(o->gj); free(o->gj_applied); free(o->gj_motor);
+    o->ngj = n;
+    o->gj = (ne_joint*)malloc(sizeof(ne_joint) * (n ? n : 1));
+    o->gj_applied = (double*)calloc(n ? n : 1, sizeof(double));
+    o->gj_motor = (double*)calloc(n ? n : 1, sizeof(double));
+    if (n) memcpy(o->gj, joints, sizeof(ne_joint) * n);
+    return NE_OK;
+}
+
+int ne_oracle_get_general_joints(ne_oracle* o, uint32_t n, int32_t* broken, double* applied, double* motor) {
+    if (!o || n > o->ngj) return NE_ERR_INVALID;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (broken) broken[k] = o->gj[k].broken;
+        if (applied) applied[k] = o->gj_applied[k];
+        if (motor) motor[k] = o->gj_motor[k];
+    }
+    return NE_OK;
 }

@@ -58,6 +58,10 @@ int ne_oracle_set_events(ne_oracle *o, int32_t enable);
 int ne_oracle_get_events(ne_oracle *o, int32_t kind, uint32_t cap, uint32_t *count, uint32_t *i, uint32_t *j,
                          double *point, double *normal, double *depth);
 void ne_oracle_epa_counts(int64_t *calls, int64_t *big);
+/* General joints (World::joints / solveJoints, src/world.hpp:130,
+ * src/world.cpp:206-315), as ne_gpu_set_general_joints / _get_general_joints. */
+int ne_oracle_set_general_joints(ne_oracle *o, uint32_t n, const ne_joint *joints);
+int ne_oracle_get_general_joints(ne_oracle *o, uint32_t n, int32_t *broken, double *applied, double *motor);
 /* Scene queries (src/physics_world.cpp:146-251), the signatures of
  * ne_gpu_raycast / ne_gpu_raycast_all / ne_gpu_overlap. */
 int ne_oracle_raycast(ne_oracle *o, uint32_t n, const double *origin, const double *dir, const double *max_dist,

@@ -561,3 +561,68 @@ int ne_ref_overlap(void* h, uint32_t n, const ne_query_shape* s, const uint32_t*
     return NE_OK;
 }
 }  // extern "C"
+
+// ---- general joints (World::joints, src/world.hpp:31-54, 130) -----------
+extern "C" {
+int ne_ref_set_general_joints(void* h, uint32_t n, const ne_joint* js) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || (n && !js)) return NE_ERR_INVALID;
+    r->w.joints.clear();
+    for (uint32_t k = 0; k < n; ++k) {
+        const ne_joint& s = js[k];
+        if (s.a >= r->w.bodies.size() || s.b >= r->w.bodies.size() || s.type < 0 || s.type > NE_JOINT_SLIDER)
+            return NE_ERR_INVALID;
+        ne::Joint j;
+        j.type = static_cast<ne::JointType>(s.type);
+        j.a = s.a;
+        j.b = s.b;
+        j.localA = v3(s.local_a, 0);
+        j.localB = v3(s.local_b, 0);
+        j.axisA = v3(s.axis_a, 0);
+        j.axisB = v3(s.axis_b, 0);
+        j.refRel = ne::Q{s.ref_rel[0], s.ref_rel[1], s.ref_rel[2], s.ref_rel[3]};
+        j.useMotor = s.use_motor != 0;
+        j.motorSpeed = s.motor_speed;
+        j.maxMotorImpulse = s.max_motor_impulse;
+        j.useLimit = s.use_limit != 0;
+        j.lower = s.lower;
+        j.upper = s.upper;
+        j.breakable = s.breakable != 0;
+        j.breakImpulse = s.break_impulse;
+        j.broken = s.broken != 0;
+        r->w.joints.push_back(j);
+    }
+    return NE_OK;
+}
+
+int ne_ref_get_general_joints(void* h, uint32_t n, int32_t* broken, double* applied, double* motor) {
+    RefWorld* r = static_cast<RefWorld*>(h);
+    if (!r || n > r->w.joints.size()) return NE_ERR_INVALID;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (broken) broken[k] = r->w.joints[k].broken ? 1 : 0;
+        if (applied) applied[k] = r->w.joints[k].appliedImpulse;
+        if (motor) motor[k] = r->w.joints[k].motorAcc;
+    }
+    return NE_OK;
+}
+
+// The frames PhysicsWorld::addGeneralJoint computes (src/physics_world.hpp:
+// 408-423) for bodies at poses (xa, qa), (xb, qb), in the reference's own
+// V3 / Q arithmetic; anchor NULL: addFixedJoint's midpoint of the centres.
+int ne_ref_joint_frames(const double* xa, const double* qa, const double* xb, const double* qb, const double* anchor,
+                        const double* axis, ne_joint* out) {
+    const ne::V3 XA = v3(xa, 0), XB = v3(xb, 0);
+    const ne::Q QA{qa[0], qa[1], qa[2], qa[3]}, QB{qb[0], qb[1], qb[2], qb[3]};
+    const ne::V3 A = anchor ? v3(anchor, 0) : (XA + XB) * 0.5;
+    const ne::V3 la = QA.inverseRotate(A - XA), lb = QB.inverseRotate(A - XB);
+    const ne::V3 ax = v3(axis, 0).normalized();
+    const ne::V3 aa = QA.inverseRotate(ax), ab = QB.inverseRotate(ax);
+    const ne::Q rr = QA * QB.conjugate();
+    const double* src[4] = {&la.x, &lb.x, &aa.x, &ab.x};
+    double* dst[4] = {out->local_a, out->local_b, out->axis_a, out->axis_b};
+    for (int k = 0; k < 4; ++k)
+        for (int c = 0; c < 3; ++c) dst[k][c] = src[k][c];
+    out->ref_rel[0] = rr.w; out->ref_rel[1] = rr.x; out->ref_rel[2] = rr.y; out->ref_rel[3] = rr.z;
+    return NE_OK;
+}
+}  // extern "C"

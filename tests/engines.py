@@ -44,6 +44,8 @@ def _bind(lib, prefix):
     g("set_meshes").argtypes = [_vp, C.c_uint32, C.POINTER(_abi.Mesh)]
     g("mesh_bvh").argtypes = [_vp, C.c_uint32, C.POINTER(C.c_int32), C.c_uint32, _u32p, _dp, C.POINTER(C.c_int32)]
     queries.bind(lib, prefix.rstrip("_"))
+    g("set_general_joints").argtypes = [_vp, C.c_uint32, _vp]
+    g("get_general_joints").argtypes = [_vp, C.c_uint32, C.POINTER(C.c_int32), _dp, _dp]
     return lib
 
 
@@ -180,6 +182,7 @@ class _CpuWorld:
         b, keep = s.to_c()
         self._check(self._f("set_bodies")(self.h, C.byref(b)), "set_bodies")
         self.n = s.n
+        self._ngj = 0
         del keep
         j = np.ascontiguousarray(s.joints, _abi.JOINT_DTYPE)
         if len(j) or getattr(self, "_had_joints", False):
@@ -189,6 +192,23 @@ class _CpuWorld:
             arr, keep = s.meshes_c()
             self._check(self._f("set_meshes")(self.h, len(s.meshes), arr), "set_meshes")
             del keep
+        gj = np.ascontiguousarray(getattr(s, "gjoints", np.zeros(0, _abi.GJOINT_DTYPE)), _abi.GJOINT_DTYPE)
+        if len(gj):
+            self.set_general_joints(gj)
+
+    def set_general_joints(self, gj):
+        gj = np.ascontiguousarray(gj, _abi.GJOINT_DTYPE)
+        self._check(self._f("set_general_joints")(self.h, len(gj), gj.ctypes.data if len(gj) else None),
+                    "set_general_joints")
+        self._ngj = len(gj)
+
+    def general_joints(self):
+        """(broken, applied impulse, motor accumulator) of the general joints."""
+        n = getattr(self, "_ngj", 0)
+        br, ap, mo = np.zeros(n, np.int32), np.zeros(n), np.zeros(n)
+        self._check(self._f("get_general_joints")(self.h, n, br.ctypes.data_as(C.POINTER(C.c_int32)), _abi.dptr(ap),
+                                                  _abi.dptr(mo)), "get_general_joints")
+        return br, ap, mo
 
     def mesh_bvh(self, m, ntris):
         """(order, bounds (nn, 6), links (nn, 4)) of mesh m's tree."""

@@ -20,6 +20,9 @@ def _scene_from(z):
     for k in z.files:
         if k.startswith("in_"):
             setattr(s, k[3:], z[k].copy())
+    if "gjoints_raw" in z.files:
+        from nativeengine_amd import _abi
+        s.gjoints = z["gjoints_raw"].copy().view(_abi.GJOINT_DTYPE)
     p = json.loads(bytes(z["params_json"]).decode())
     p["half"] = tuple(p["half"])
     p["gravity"] = tuple(p["gravity"])

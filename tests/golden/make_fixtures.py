@@ -40,6 +40,8 @@ def scene_arrays(s):
     """The scene's input arrays and world parameters, as fixture entries."""
     arrays = {f"in_{k}": getattr(s, k) for k in SCENE_KEYS}
     arrays["params_json"] = np.frombuffer(json.dumps(dict(s.params.__dict__)).encode(), np.uint8)
+    if len(s.gjoints):   # general joints: their C records as raw bytes
+        arrays["gjoints_raw"] = np.ascontiguousarray(s.gjoints, _abi.GJOINT_DTYPE).view(np.uint8).copy()
     return arrays
 
 
@@ -67,6 +69,9 @@ def save_scene_fixture(name, s, steps, diag_every=0):
     arrays = scene_arrays(s)
     arrays.update(out_x=x, out_v=v, out_q=q, out_w=w, out_ci=c["i"], out_cj=c["j"], out_normal=c["normal"],
                   out_point=c["point"], out_depth=c["depth"], out_counts=np.array(counts, np.uint32))
+    if len(s.gjoints):
+        br, ap, mo = r.general_joints()
+        arrays.update(out_gj_broken=br, out_gj_applied=ap, out_gj_motor=mo)
     if diag_every:
         arrays["out_diag"] = np.array(diag)
         arrays["diag_every"] = np.array([diag_every], np.int32)

@@ -96,7 +96,18 @@ def fixture_scenes():
     out["gas4096"] = (elastic_gas_4096(), 1000, 100)
     out["ccd"] = (ccd_scene(0, periodic=False), 60, 0)
     out["ccd_periodic"] = (ccd_scene(1, periodic=True), 60, 0)
+    for name, (js, steps) in joint_selftest_scenes().items():
+        out[f"joints_{name}"] = (js, steps, 0)
+    out["joints_mix"] = (joint_mix_scene(0), 300, 0)
+    out["joints_mix_periodic"] = (joint_mix_scene(1, periodic=True), 300, 0)
+    out["joints_limits"] = (joint_mix_scene(2, limits=True), 120, 0)
     return out
+
+
+# Fixture scenes whose hinge limits go through atan2: the device library's
+# atan2 can differ from the host's in the last bit, so the GPU matches these
+# to a tolerance instead of bit for bit.
+ATAN2_SCENES = {"joints_limits"}
 
 
 def ccd_selftest_scene(ccd: bool):
@@ -336,3 +347,124 @@ def query_inputs(s, seed, nrays=1500, nshapes=200):
                for i in rng.integers(0, s.n, nshapes)]
     mask0 = mask[:nshapes].copy()
     return dict(org=org, dir=d, max_dist=md, mask=mask, rad=rad, shapes0=shapes0, shapes1=shapes1, mask0=mask0)
+
+
+def _static(b):
+    return b.make_static()
+
+
+def joint_selftest_scenes():
+    """name -> (scene, steps) for the reference's general-joint checks
+    (selftest.cpp:1088-1179), built as the facade builds them: a fixed joint
+    holding a box beside a static anchor, a ball-joint pendulum, a hinge
+    driven by a motor, a breakable ball joint under a heavy box, a slider
+    pushed every way, and a bridge of 8 hinged planks."""
+    out = {}
+
+    def world(bodies, gravity, dt=1.0 / 240.0):
+        s = S.Scene.from_bodies(bodies)
+        s.params = S.Params(half=(1000.0, 1000.0, 1000.0), periodic=False, dt=dt, gravity=gravity)
+        return s
+
+    q0 = (1.0, 0.0, 0.0, 0.0)
+    s = world([_static(S.make_box(0, (0, 5, 0), q0, (0.5, 0.5, 0.5), 1.0)), S.make_box(1, (1, 5, 0), q0, (0.5, 0.5, 0.5), 1.0)],
+              (0.0, -10.0, 0.0))
+    s.gjoints = np.array([S.general_joint(s, _abi.JOINT_FIXED, 0, 1)], _abi.GJOINT_DTYPE)
+    out["fixed"] = (s, 600)
+    s = world([_static(S.make_sphere(0, (0, 5, 0), 0.2, 1.0)), S.make_sphere(1, (0, 3, 0), 0.5, 1.0)], (0.0, -10.0, 0.0))
+    s.gjoints = np.array([S.general_joint(s, _abi.JOINT_BALL, 0, 1, anchor=(0, 5, 0))], _abi.GJOINT_DTYPE)
+    out["ball"] = (s, 800)
+    s = world([_static(S.make_box(0, (0, 0, 0), q0, (0.2, 0.2, 0.2), 1.0)), S.make_box(1, (1, 0, 0), q0, (1, 0.1, 0.1), 1.0)],
+              (0.0, 0.0, 0.0))
+    s.gjoints = np.array([S.general_joint(s, _abi.JOINT_HINGE, 0, 1, anchor=(0, 0, 0), axis=(0, 1, 0), motor=(2.0, 200.0))], _abi.GJOINT_DTYPE)
+    out["hinge_motor"] = (s, 240)
+    s = world([_static(S.make_sphere(0, (0, 5, 0), 0.2, 1.0)), S.make_box(1, (0, 3, 0), q0, (1, 1, 1), 50.0)],
+              (0.0, -30.0, 0.0))
+    s.gjoints = np.array([S.general_joint(s, _abi.JOINT_BALL, 0, 1, anchor=(0, 5, 0), break_impulse=5.0)], _abi.GJOINT_DTYPE)
+    out["breakable"] = (s, 300)
+    s = world([_static(S.make_box(0, (0, 0, 0), q0, (0.2, 0.2, 0.2), 1.0)), S.make_box(1, (0, 0, 0), q0, (0.5, 0.5, 0.5), 1.0)],
+              (0.0, 0.0, 0.0))
+    s.v[1] = (2.0, 3.0, 1.0)
+    s.gjoints = np.array([S.general_joint(s, _abi.JOINT_SLIDER, 0, 1, anchor=(0, 0, 0), axis=(1, 0, 0))], _abi.GJOINT_DTYPE)
+    out["slider"] = (s, 240)
+    bodies = [_static(S.make_box(0, (-5, 6, 0), q0, (1, 1, 2), 1.0))]
+    for i in range(8):
+        bodies.append(S.make_box(len(bodies), (-4.5 + i * 1.0, 6, 0), q0, (0.5, 0.12, 2), 1.0))
+    bodies.append(_static(S.make_box(len(bodies), (3, 6, 0), q0, (1, 1, 2), 1.0)))
+    s = world(bodies, (0.0, -10.0, 0.0))
+    js = [S.general_joint(s, _abi.JOINT_HINGE, i, i + 1, anchor=(-4.5 + i * 1.0 - 0.5, 6, 0), axis=(0, 0, 1))
+          for i in range(8)]
+    js.append(S.general_joint(s, _abi.JOINT_HINGE, 8, 9, anchor=(2.5, 6, 0), axis=(0, 0, 1)))
+    s.gjoints = np.array(js, _abi.GJOINT_DTYPE)
+    out["bridge"] = (s, 1200)
+    return out
+
+
+def joint_mix_scene(seed=0, periodic=False, limits=False):
+    """Chains of boxes, spheres and capsules joined by every joint type at
+    random anchors and axes (hinges with motors and angle limits, fixed
+    joints, sliders, breakable ball joints), hung from static anchors under
+    gravity (or free in a periodic box, chains crossing its boundary), with
+    ground contacts and island sleeping on.  Hinge angle limits only with
+    `limits` (their angle goes through atan2)."""
+    rng = np.random.default_rng(seed)
+    bodies = []
+    h = 6.0
+    anchors = []
+    for c in range(5):
+        x0 = (-4.0 + 2.0 * c, 4.5, rng.uniform(-1, 1))
+        if periodic:
+            x0 = (h - 0.6 + 0.3 * c, rng.uniform(-3, 3), rng.uniform(-3, 3))
+        a = _static(S.make_box(len(bodies), x0, (1.0, 0.0, 0.0, 0.0), (0.2, 0.2, 0.2), 1.0))
+        bodies.append(a)
+        anchors.append(len(bodies) - 1)
+    chains = []
+    for c, ai in enumerate(anchors):
+        prev = ai
+        links = []
+        for k in range(4):
+            x = np.array(bodies[ai].x) + np.array([0.0, -0.9 * (k + 1), 0.0]) if not periodic else \
+                np.array(bodies[ai].x) + np.array([0.8 * (k + 1), 0.0, 0.0])
+            q = tuple(S.random_unit_quaternions(rng, 1)[0]) if k % 2 else (1.0, 0.0, 0.0, 0.0)
+            kind = (c + k) % 3
+            if kind == 0:
+                bd = S.make_box(len(bodies), tuple(x), q, tuple(rng.uniform(0.15, 0.3, 3)), 1.0)
+            elif kind == 1:
+                bd = S.make_sphere(len(bodies), tuple(x), float(rng.uniform(0.15, 0.3)), 1.0)
+            else:
+                bd = S.make_capsule(len(bodies), tuple(x), q, 0.12, 0.2, 1.0)
+            bd.v = tuple(rng.normal(0, 1.0, 3))
+            bodies.append(bd)
+            links.append((prev, len(bodies) - 1))
+            prev = len(bodies) - 1
+        chains.append(links)
+    if not periodic:
+        bodies.append(_static(S.make_box(len(bodies), (0.0, -5.5, 0.0), (1.0, 0.0, 0.0, 0.0), (5.5, 0.3, 5.5), 1.0)))
+        for k in range(6):
+            bodies.append(S.make_box(len(bodies), (rng.uniform(-4, 4), -4.5, rng.uniform(-4, 4)),
+                                     tuple(S.random_unit_quaternions(rng, 1)[0]), (0.3, 0.3, 0.3), 1.0))
+    s = S.Scene.from_bodies(bodies)
+    s.params = S.Params(half=(h, h, h), periodic=periodic, dt=1.0 / 120.0,
+                        gravity=(0.0, 0.0, 0.0) if periodic else (0.0, -9.81, 0.0), sleep_enabled=not periodic)
+    js = []
+    for c, links in enumerate(chains):
+        for k, (a, b) in enumerate(links):
+            mid = (np.array(s.x[a]) + np.array(s.x[b])) * 0.5
+            axis = rng.normal(size=3)
+            t = (c * 4 + k) % 6
+            if t == 0:
+                js.append(S.general_joint(s, _abi.JOINT_BALL, a, b, anchor=mid, break_impulse=float(rng.uniform(2, 8))))
+            elif t == 1:
+                js.append(S.general_joint(s, _abi.JOINT_HINGE, a, b, anchor=mid, axis=axis,
+                                          motor=(float(rng.uniform(-3, 3)), float(rng.uniform(0.5, 5)))))
+            elif t == 2:
+                js.append(S.general_joint(s, _abi.JOINT_HINGE, a, b, anchor=mid, axis=axis,
+                                          limit=(-0.3, 0.4) if limits else None))
+            elif t == 3:
+                js.append(S.general_joint(s, _abi.JOINT_FIXED, a, b))
+            elif t == 4:
+                js.append(S.general_joint(s, _abi.JOINT_SLIDER, a, b, anchor=mid, axis=axis))
+            else:
+                js.append(S.general_joint(s, _abi.JOINT_BALL, a, b, anchor=mid))
+    s.gjoints = np.array(js, _abi.GJOINT_DTYPE)
+    return s

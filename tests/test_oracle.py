@@ -54,6 +54,9 @@ def test_oracle_matches_golden_scene(oracle_lib, name):
     assert np.array_equal(c["i"], out["ci"]) and np.array_equal(c["j"], out["cj"])
     for k in ["normal", "point", "depth"]:
         assert np.array_equal(c[k], out[k]), f"{name}: contact {k} differs"
+    if "gj_broken" in out:   # general joints: broken flags and the last step's accumulators
+        for k, a in zip(("gj_broken", "gj_applied", "gj_motor"), o.general_joints()):
+            assert np.array_equal(a, out[k]), f"{name}: {k} differs"
 
 
 def test_oracle_matches_golden_narrowphase(oracle_lib):
