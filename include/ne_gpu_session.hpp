@@ -81,6 +81,9 @@ private:
     std::vector<std::int32_t> shape_;
     std::vector<std::uint32_t> layer_, mask_, flags_, hoff_, noff_, nc_;
     std::vector<ne_distance_joint> joints_;
+    std::vector<ne_joint> gjoints_;
+    std::vector<std::int32_t> gjBroken_;
+    std::vector<double> gjApplied_, gjMotor_;
 
     [[noreturn]] void fail(int rc) {
         throw std::runtime_error("ne_gpu step failed (" + std::to_string(rc) + "): " + ne_gpu_last_error(ctx_));
@@ -156,6 +159,7 @@ private:
         B.sleep_timer = st_.data(); B.inv_mass_store = ims_.data(); B.inv_inertia_store = iis_.data();
         check(ne_gpu_set_bodies(ctx_, &B));
         uploadJoints(w, 0);
+        uploadGeneralJoints(w, 0);
         uploadMeshes(w, 0);
     }
 
@@ -226,6 +230,56 @@ private:
     template <class W>
     void uploadJoints(W&, long) {}
 
+    // World::joints (src/world.hpp:28-53, 130), when the type has them: the
+    // general joints with their broken flags; the step's accumulators start
+    // from zero on the device as in World::solveJoints.
+    template <class W>
+    auto uploadGeneralJoints(W& w, int) -> decltype(w.joints.size(), void()) {
+        gjoints_.assign(w.joints.size(), ne_joint{});
+        for (size_t k = 0; k < gjoints_.size(); ++k) {
+            const auto& s = w.joints[k];
+            ne_joint& j = gjoints_[k];
+            j.type = static_cast<std::int32_t>(s.type);
+            j.a = static_cast<std::uint32_t>(s.a);
+            j.b = static_cast<std::uint32_t>(s.b);
+            j.use_motor = s.useMotor ? 1 : 0;
+            j.use_limit = s.useLimit ? 1 : 0;
+            j.breakable = s.breakable ? 1 : 0;
+            j.broken = s.broken ? 1 : 0;
+            j.local_a[0] = s.localA.x; j.local_a[1] = s.localA.y; j.local_a[2] = s.localA.z;
+            j.local_b[0] = s.localB.x; j.local_b[1] = s.localB.y; j.local_b[2] = s.localB.z;
+            j.axis_a[0] = s.axisA.x; j.axis_a[1] = s.axisA.y; j.axis_a[2] = s.axisA.z;
+            j.axis_b[0] = s.axisB.x; j.axis_b[1] = s.axisB.y; j.axis_b[2] = s.axisB.z;
+            j.ref_rel[0] = s.refRel.w; j.ref_rel[1] = s.refRel.x; j.ref_rel[2] = s.refRel.y; j.ref_rel[3] = s.refRel.z;
+            j.motor_speed = s.motorSpeed;
+            j.max_motor_impulse = s.maxMotorImpulse;
+            j.lower = s.lower;
+            j.upper = s.upper;
+            j.break_impulse = s.breakImpulse;
+        }
+        check(ne_gpu_set_general_joints(ctx_, static_cast<std::uint32_t>(gjoints_.size()), gjoints_.data()));
+    }
+    template <class W>
+    void uploadGeneralJoints(W&, long) {}
+
+    template <class W>
+    auto downloadGeneralJoints(W& w, int) -> decltype(w.joints.size(), void()) {
+        const size_t n = w.joints.size();
+        if (n == 0) return;
+        gjBroken_.assign(n, 0);
+        gjApplied_.assign(n, 0.0);
+        gjMotor_.assign(n, 0.0);
+        check(ne_gpu_get_general_joints(ctx_, static_cast<std::uint32_t>(n), gjBroken_.data(), gjApplied_.data(),
+                                        gjMotor_.data()));
+        for (size_t k = 0; k < n; ++k) {
+            w.joints[k].broken = gjBroken_[k] != 0;
+            w.joints[k].appliedImpulse = gjApplied_[k];
+            w.joints[k].motorAcc = gjMotor_[k];
+        }
+    }
+    template <class W>
+    void downloadGeneralJoints(W&, long) {}
+
     template <class W>
     void download(W& w) {
         const size_t n = w.bodies.size();
@@ -245,6 +299,7 @@ private:
             b.torque.x = b.torque.y = b.torque.z = 0.0;
             getSleep(b, i, 0);
         }
+        downloadGeneralJoints(w, 0);
     }
 
     struct H3 { double x, y, z; };

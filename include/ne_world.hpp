@@ -246,6 +246,30 @@ struct DistanceJoint {
     double damping = 0.0;
 };
 
+// A general joint (src/world.hpp:28-53), solved by sequential impulse after
+// the rigid distance joints: Ball pins the anchors together, Hinge and Fixed
+// add angular rows, Slider frees one translation axis.  The runtime fields
+// (broken, appliedImpulse, motorAcc) come back after every step.
+enum class JointType { Ball, Hinge, Fixed, Slider };
+
+struct Joint {
+    JointType type = JointType::Ball;
+    std::size_t a = 0, b = 0;
+    V3 localA, localB;
+    V3 axisA{0, 1, 0}, axisB{0, 1, 0};
+    Q refRel{1, 0, 0, 0};
+    bool useMotor = false;
+    double motorSpeed = 0.0;
+    double maxMotorImpulse = 0.0;
+    bool useLimit = false;
+    double lower = 0.0, upper = 0.0;
+    bool breakable = false;
+    double breakImpulse = 0.0;
+    bool broken = false;
+    double appliedImpulse = 0.0;
+    double motorAcc = 0.0;
+};
+
 // The reference's World (src/world.hpp:73-161), stepping on the GPU.
 class World {
 public:
@@ -275,6 +299,7 @@ public:
     int solverIterations = 8;          // extension: the reference fixes 8
     std::vector<Body> bodies;
     std::vector<DistanceJoint> distanceJoints;
+    std::vector<Joint> joints;
 
     World() = default;
     // Value semantics like the reference's World: a copy carries the bodies
@@ -288,6 +313,7 @@ public:
         sleepTime = o.sleepTime; wakeVel = o.wakeVel; openBoundary = o.openBoundary; warmStart = o.warmStart;
         forceBruteForce = o.forceBruteForce; solverIterations = o.solverIterations; bodies = o.bodies;
         distanceJoints = o.distanceJoints;
+        joints = o.joints;
         session_.reset();
         cacheKeys_ = o.cacheKeys_;
         cacheImp_ = o.cacheImp_;

@@ -176,6 +176,112 @@ static void joints() {   // selftest.cpp:779-823
     }
 }
 
+// PhysicsWorld::addGeneralJoint (src/physics_world.hpp:408-423) on indices:
+// anchors and axis in each body's frame, the current relative orientation.
+static Joint makeJoint(const World& w, JointType type, std::size_t a, std::size_t b, const V3& anchor, const V3& axis) {
+    const Body& ba = w.bodies[a];
+    const Body& bb = w.bodies[b];
+    Joint j;
+    j.type = type;
+    j.a = a; j.b = b;
+    j.localA = ba.q.inverseRotate(anchor - ba.x);
+    j.localB = bb.q.inverseRotate(anchor - bb.x);
+    const V3 ax = axis.normalized();
+    j.axisA = ba.q.inverseRotate(ax);
+    j.axisB = bb.q.inverseRotate(ax);
+    j.refRel = ba.q * bb.q.conjugate();
+    return j;
+}
+
+static Body makeStaticBody(Body b) {   // PhysicsWorld::makeStatic (src/physics_world.hpp:112-120)
+    b.invMass = 0.0; b.invInertiaBody = V3{0, 0, 0};
+    b.v = V3{}; b.w = V3{};
+    b.dynamic = false; b.kinematic = false;
+    return b;
+}
+
+static World jointWorld(double gy) {
+    World w;
+    w.box.half = 1000.0; w.box.periodic = false; w.dt = 1.0 / 240.0; w.gravity = V3{0, gy, 0};
+    return w;
+}
+
+static void general_joints() {   // selftest.cpp:1088-1175, World::joints on the GPU
+    {   // (a) fixed joint beside a static anchor
+        World w = jointWorld(-10.0);
+        w.bodies = {makeStaticBody(makeBox(0, V3{0, 5, 0}, Q{1, 0, 0, 0}, V3{0.5, 0.5, 0.5}, 1.0)),
+                    makeBox(1, V3{1, 5, 0}, Q{1, 0, 0, 0}, V3{0.5, 0.5, 0.5}, 1.0)};
+        w.joints = {makeJoint(w, JointType::Fixed, 0, 1, (w.bodies[0].x + w.bodies[1].x) * 0.5, V3{0, 1, 0})};
+        w.stepMany(600);
+        const V3 p = w.bodies[1].x;
+        check(std::fabs(p.x - 1.0) <= 0.08 && std::fabs(p.y - 5.0) <= 0.15 && std::fabs(p.z) <= 0.08,
+              "fixed joint: holds the relative pose under gravity");
+    }
+    {   // (b) ball joint: a bob hangs below a static anchor
+        World w = jointWorld(-10.0);
+        w.bodies = {makeStaticBody(makeSphere(0, V3{0, 5, 0}, 0.2, 1.0)), makeSphere(1, V3{0, 3, 0}, 0.5, 1.0)};
+        w.joints = {makeJoint(w, JointType::Ball, 0, 1, V3{0, 5, 0}, V3{0, 1, 0})};
+        for (int s = 0; s < 800; ++s) w.step();   // compat mode: joints go up and come back every step
+        const V3 p = w.bodies[1].x;
+        check(std::fabs((p - V3{0, 5, 0}).norm() - 2.0) <= 0.05, "ball joint: anchor distance preserved");
+        check(p.y < 5.0, "ball joint: hangs below the anchor");
+    }
+    {   // (c) hinge with a motor drives rotation about its axis only
+        World w = jointWorld(0.0);
+        w.bodies = {makeStaticBody(makeBox(0, V3{0, 0, 0}, Q{1, 0, 0, 0}, V3{0.2, 0.2, 0.2}, 1.0)),
+                    makeBox(1, V3{1, 0, 0}, Q{1, 0, 0, 0}, V3{1, 0.1, 0.1}, 1.0)};
+        Joint h = makeJoint(w, JointType::Hinge, 0, 1, V3{0, 0, 0}, V3{0, 1, 0});
+        h.useMotor = true; h.motorSpeed = 2.0; h.maxMotorImpulse = 200.0;
+        w.joints = {h};
+        w.stepMany(240);
+        check(std::fabs(w.bodies[1].w.y) > 1.0, "hinge motor: drives rotation about the axis");
+        check(std::fabs(w.bodies[1].x.z) > 0.2, "hinge motor: arm swept around the axis");
+        check(w.joints[0].motorAcc != 0.0, "hinge motor: the step's motor impulse comes back");
+    }
+    {   // (d) breakable: enough load severs the joint and the body falls
+        World w = jointWorld(-30.0);
+        w.bodies = {makeStaticBody(makeSphere(0, V3{0, 5, 0}, 0.2, 1.0)),
+                    makeBox(1, V3{0, 3, 0}, Q{1, 0, 0, 0}, V3{1, 1, 1}, 50.0)};
+        Joint j = makeJoint(w, JointType::Ball, 0, 1, V3{0, 5, 0}, V3{0, 1, 0});
+        j.breakable = true; j.breakImpulse = 5.0;
+        w.joints = {j};
+        w.stepMany(300);
+        check(w.joints[0].broken, "breakable joint: breaks under heavy load");
+        check(w.bodies[1].x.y < 2.0, "breakable joint: body falls after breaking");
+    }
+    {   // (e) slider: free along its axis, locked across it
+        World w = jointWorld(0.0);
+        w.bodies = {makeStaticBody(makeBox(0, V3{0, 0, 0}, Q{1, 0, 0, 0}, V3{0.2, 0.2, 0.2}, 1.0)),
+                    makeBox(1, V3{0, 0, 0}, Q{1, 0, 0, 0}, V3{0.5, 0.5, 0.5}, 1.0)};
+        w.joints = {makeJoint(w, JointType::Slider, 0, 1, V3{0, 0, 0}, V3{1, 0, 0})};
+        w.bodies[1].v = V3{2, 3, 1};
+        w.stepMany(240);
+        const V3 p = w.bodies[1].x;
+        check(std::fabs(p.x) > 0.5, "slider: translates along the axis");
+        check(std::fabs(p.y) < 0.05 && std::fabs(p.z) < 0.05, "slider: locked perpendicular to the axis");
+    }
+    {   // (f) a bridge of hinges stays planar
+        World w = jointWorld(-10.0);
+        w.bodies.push_back(makeStaticBody(makeBox(0, V3{-5, 6, 0}, Q{1, 0, 0, 0}, V3{1, 1, 2}, 1.0)));
+        std::vector<Joint> js;
+        for (int i = 0; i < 8; ++i) {
+            const double x = -4.5 + i * 1.0;
+            w.bodies.push_back(makeBox(i + 1, V3{x, 6, 0}, Q{1, 0, 0, 0}, V3{0.5, 0.12, 2}, 1.0));
+            js.push_back(makeJoint(w, JointType::Hinge, i, i + 1, V3{x - 0.5, 6, 0}, V3{0, 0, 1}));
+        }
+        w.bodies.push_back(makeStaticBody(makeBox(9, V3{3, 6, 0}, Q{1, 0, 0, 0}, V3{1, 1, 2}, 1.0)));
+        js.push_back(makeJoint(w, JointType::Hinge, 8, 9, V3{2.5, 6, 0}, V3{0, 0, 1}));
+        w.joints = js;
+        w.stepMany(1200);
+        double maxTilt = 0.0;
+        for (int i = 1; i <= 8; ++i) {
+            const V3 lz = w.bodies[i].q.rotate(V3{0, 0, 1});
+            maxTilt = std::max(maxTilt, std::acos(std::min(1.0, std::fabs(lz.z))) * 180.0 / 3.14159265358979323846);
+        }
+        check(maxTilt < 5.0, "hinge chain: a bridge of hinges stays planar (no twisting)");
+    }
+}
+
 static World floorWorld() {
     World w;
     w.box.half = 1e6; w.box.periodic = false; w.dt = 1.0 / 240.0; w.gravity = V3{0, -10, 0};
@@ -320,6 +426,7 @@ int main(int argc, char** argv) {
         cross_boundary();
         stacking();
         joints();
+        general_joints();
         sleeping();
         meshes();
         std::printf("failures: %d\n", g_fail);
