@@ -132,7 +132,8 @@ enum CounterSlot : int {
     CNT_EPALONG2 = 22,   // convex pairs whose EPA ran past the second launch's budget (this step)
     CNT_HOSTCHK = 23,    // a host call's own check (snapshot loads); no step kernel uses it
     CNT_EV = 24,         // facade events pass: overlapping pairs of the post-step state
-    CNT_N = 25
+    CNT_EPALONG3 = 25,   // convex pairs the second LDS EPA tier left for the per-thread scratch tier
+    CNT_N = 26
 };
 
 constexpr int TPB = 256;
@@ -1017,7 +1018,7 @@ __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimPar
     } else if (!SPHERES_ONLY) {
         const Geom a = load_geom(B, i), b = load_geom(B, j);
         Contact c = detect_contact(box, a, b, ws);
-        if (c.hit == 2) return NARROW_DEFERRED;   // GJK overlap: EPA runs in k_epa
+        if (c.hit == 2) return NARROW_DEFERRED;   // GJK overlap: EPA runs in the EPA tiers
         if (ws.overflow) return 0;
         if (c.hit) {
             if (si == SH_BOX && sj == SH_BOX && !P.no_manifold) {
@@ -1047,8 +1048,6 @@ __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimPar
 // block, a workspace in global memory and the horizon built in parallel.
 // Both capacities grow when exceeded (the step is then run again).
 constexpr uint32_t EPA_BIG_PAIRS = 64;    // initial queue capacity
-constexpr int EPA_FIRST_ITERS = 8;        // EPA iterations of the first tier-1 launch
-constexpr int EPA_SECOND_ITERS = 24;      // of the second (the third runs the rest)
 constexpr int EPA_BIG_FACES = 4096;       // initial faces per queued pair
 constexpr int EPA_BIG_FACES_MAX = 1 << 17;   // default cap: beyond it a run stops at its closest face
 constexpr int EPA_KEYS = EPA_MAXV * EPA_MAXV;   // directed edges u * EPA_MAXV + w of one polytope
@@ -1066,10 +1065,11 @@ struct EpaBig {
 };
 
 // Narrow phase, sorted by pair type: k_narrow computes the analytic pairs
-// and runs GJK on the convex ones; convex pairs that overlap are queued
-// (their EPA keeps a large per-thread workspace and runs for tens of
-// iterations) and k_epa runs them together, so waves doing EPA hold only EPA
-// pairs.  Their contacts are the same as computed inline.
+// and runs GJK on the convex ones; convex pairs that overlap are queued for
+// the EPA tiers (ne_epa_lds.hpp: per-lane polytopes in LDS, then groups of
+// lanes per run, then a wave, then a block per run), so waves doing EPA hold
+// only EPA runs of like length.  Their contacts are the same as computed
+// inline.
 template <bool SPHERES_ONLY>
 __global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
                                                 const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
@@ -1077,7 +1077,7 @@ __global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, uint32
                                                 uint32_t* __restrict__ epa_q, uint32_t epa_cap) {
     const uint32_t np = counters[CNT_P];
     uint32_t* fault = counters + CNT_FAULT;
-    if (!SPHERES_ONLY && blockIdx.x == 0 && threadIdx.x == 0) counters[CNT_EPALONG] = 0u;   // k_epa runs after
+    if (!SPHERES_ONLY && blockIdx.x == 0 && threadIdx.x == 0) counters[CNT_EPALONG] = 0u;   // the EPA tiers run after
     EpaWS ws{};
     ws.defer = true;
     for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < np; p0 += gridDim.x * blockDim.x) {
@@ -1097,51 +1097,7 @@ __global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, uint32
     }
 }
 
-// EPA over a queue of overlapping convex pairs, at most `max_iter`
-// iterations each.  The first launch gives every pair a short budget (most
-// runs end within it), so a wave is not held for long by its slowest pair;
-// pairs that need more go to `long_q` and the second launch reruns them from
-// the start with the reference's full 64.
-__global__ void __launch_bounds__(TPB) k_epa(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
-                                             const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
-                                             uint32_t* __restrict__ ncont, double* __restrict__ ct,
-                                             const uint32_t* __restrict__ epa_q, int count_slot, uint32_t epa_cap,
-                                             EpaBig big, int max_iter, uint32_t* __restrict__ long_q) {
-    const uint32_t nq = min(counters[count_slot], epa_cap);
-    const int long_slot = count_slot == CNT_EPAQ ? CNT_EPALONG : CNT_EPALONG2;
-    uint32_t* fault = counters + CNT_FAULT;
-    SupV verts[EPA1_MAXV];
-    Face faces[EPA1_MAXF];
-    int hor[EPA1_MAXH][2];
-    EpaWS ws{};
-    ws.verts = verts; ws.faces = faces; ws.maxf = EPA1_MAXF; ws.maxv = EPA1_MAXV;
-    ws.list.hor = hor; ws.list.cap = EPA1_MAXH;
-    ws.max_iter = max_iter;
-    for (uint32_t t0 = blockIdx.x * blockDim.x; t0 < nq; t0 += gridDim.x * blockDim.x) {
-        const uint32_t t = t0 + threadIdx.x;
-        const uint32_t p = t < nq ? epa_q[t] : 0u;
-        bool defer = false;
-        if (t < nq) {
-            ws.overflow = false;
-            ws.too_long = false;
-            uint32_t nc = narrow_pair<false>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
-            if (ws.overflow) {
-                const uint32_t slot = atomicAdd(&counters[CNT_EPABIG], 1u);
-                if (slot < big.cap) big.list[slot] = p;
-                else atomicOr(fault, NE_FAULT_EPA_QUEUE);
-                nc = 0;
-            } else if (ws.too_long) {
-                defer = true;
-                nc = 0;
-            }
-            ncont[p] = nc;
-        }
-        if (long_q) {
-            const uint32_t slot = wave_add1(counters, long_slot, defer);
-            if (defer) long_q[slot] = p;   // at most one entry per queued pair: fits the queue's capacity
-        }
-    }
-}
+#include "ne_epa_lds.hpp"
 
 // EPA of one pair by one wave (src/epa.hpp:71-150, the same steps as
 // epa_contact), polytope and horizon list in LDS.  Each iteration the 64
@@ -1702,37 +1658,25 @@ __device__ __forceinline__ void store_contact(double* out, const Contact& c, boo
 
 constexpr int PV_STRIDE = 19;   // doubles per thread for a prism's vertices (18, padded against bank conflicts)
 
-// detectContact(other, prism) per triangle candidate (GJK + EPA with the
-// per-thread workspace); EPA runs that outgrow it go to the shared
-// large-workspace queue (k_mesh_big).
-// As k_epa, two launches: every candidate with a short EPA budget, then the
-// ones that needed more (`list` = the first launch's `long_q`) in full.
-__global__ void __launch_bounds__(TPB) k_mesh_narrow(BodiesDev B, SimParams P, MeshDev M, uint32_t* __restrict__ counters,
-                                                     const uint32_t* __restrict__ cand_i,
-                                                     const uint32_t* __restrict__ cand_j,
-                                                     const uint32_t* __restrict__ mq_pair,
-                                                     const int32_t* __restrict__ mq_tri, uint8_t* __restrict__ mhit,
-                                                     double* __restrict__ mct, EpaBig big, const uint32_t* __restrict__ list,
-                                                     int count_slot, int max_iter, uint32_t* __restrict__ long_q) {
-    const uint32_t nq = counters[count_slot];
-    uint32_t* fault = counters + CNT_FAULT;
-    SupV verts[EPA1_MAXV];
-    Face faces[EPA1_MAXF];
-    int hor[EPA1_MAXH][2];
-    EpaWS ws{};
-    ws.verts = verts; ws.faces = faces; ws.maxf = EPA1_MAXF; ws.maxv = EPA1_MAXV;
-    ws.list.hor = hor; ws.list.cap = EPA1_MAXH;
-    ws.max_iter = max_iter;
-    // the prism's six vertices in LDS (every support call reads them; in
-    // per-thread scratch they were most of this kernel's memory traffic)
-    __shared__ double lds_pv[TPB * PV_STRIDE];
+// detectContact(other, prism) per triangle candidate: the plane's analytic
+// path, else GJK and the LDS EPA tier A (k_epa_lds's workspace, beside the
+// prism's six vertices: three waves a CU).  Runs that outgrow the workspace
+// go to `long_q` for k_mesh_wave, which reruns them from the start.
+__global__ void __launch_bounds__(64) k_mesh_lds(BodiesDev B, SimParams P, MeshDev M, uint32_t* __restrict__ counters,
+                                                 const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                 const uint32_t* __restrict__ mq_pair, const int32_t* __restrict__ mq_tri,
+                                                 uint8_t* __restrict__ mhit, double* __restrict__ mct,
+                                                 uint32_t* __restrict__ long_q) {
+    __shared__ EpaLaneLds<8, 12, 8> lds;
+    __shared__ double lds_pv[64 * PV_STRIDE];
+    EpaLane<8, 12, 8> E{lds, (int)threadIdx.x};
     double* pv = lds_pv + threadIdx.x * PV_STRIDE;
-    for (uint32_t u0 = blockIdx.x * blockDim.x; u0 < nq; u0 += gridDim.x * blockDim.x) {
-        const uint32_t u = u0 + threadIdx.x;
+    const uint32_t nq = counters[CNT_MQ];
+    V3 dirs[8];
+    for (uint32_t u0 = blockIdx.x * 64u; u0 < nq; u0 += gridDim.x * 64u) {
+        const uint32_t q = u0 + threadIdx.x;
         bool defer = false;
-        uint32_t q = 0;
-        if (u < nq) {
-            q = list ? list[u] : u;
+        if (q < nq) {
             const uint32_t p = mq_pair[q];
             uint32_t o, m;
             bool flip;
@@ -1740,27 +1684,27 @@ __global__ void __launch_bounds__(TPB) k_mesh_narrow(BodiesDev B, SimParams P, M
             Geom tri;
             uint8_t hit = 0;
             if (mesh_prism(B, M, m, mq_tri[q], pv, tri)) {
-                ws.overflow = false;
-                ws.too_long = false;
                 const Geom other = load_geom(B, o);
-                const Contact c = detect_contact(box_of(B, P, o), other, tri, ws);
-                if (ws.overflow) {
-                    const uint32_t slot = atomicAdd(&counters[CNT_EPABIG], 1u);
-                    if (slot < big.cap) big.list[slot] = MESH_ENTRY | q;
-                    else atomicOr(fault, NE_FAULT_EPA_QUEUE);
-                } else if (ws.too_long) {
-                    defer = true;
-                } else if (c.hit) {
+                const Box3& box = box_of(B, P, o);
+                Contact c = no_contact();
+                if (other.shape == SH_PLANE) {
+                    EpaWS ws{};   // the plane's path runs no EPA (and would stop after GJK)
+                    ws.defer = true;
+                    c = detect_contact(box, other, tri, ws);
+                } else {
+                    const V3 shift = min_image_shift(box, other.x, tri.x);   // convex_contact (epa.hpp:153-158)
+                    SupD tet[4];
+                    if (gjk_tetra(other, tri, shift, tet) && !epa_lane(E, other, tri, shift, tet, dirs, c)) defer = true;
+                }
+                if (!defer && c.hit) {
                     store_contact(mct + 7 * (size_t)q, c, flip);
                     hit = 1;
                 }
             }
             mhit[q] = hit;
         }
-        if (long_q) {
-            const uint32_t slot = wave_add1(counters, CNT_MQLONG, defer);
-            if (defer) long_q[slot] = q;   // at most one entry per candidate: fits the candidate capacity
-        }
+        const uint32_t slot = wave_add1(counters, CNT_MQLONG, defer);
+        if (defer) long_q[slot] = q;   // at most one entry per candidate: fits the candidate capacity
     }
 }
 
@@ -3621,7 +3565,8 @@ struct ne_gpu_ctx {
     uint32_t *cand_i = nullptr, *cand_j = nullptr;
     uint32_t* epa_q = nullptr;        // convex pairs queued for EPA (pcap entries)
     uint32_t* epa_long = nullptr;     // of those, the ones the first EPA launch left for the second (pcap entries)
-    uint32_t* epa_long2 = nullptr;    // and the ones the second left for the third
+    uint32_t* epa_long2 = nullptr;    // the ones the per-thread scratch tier left for the wave tier
+    uint32_t* epa_long3 = nullptr;    // the ones the second LDS tier left for the scratch tier
     uint32_t *ncont = nullptr, *cons_off = nullptr;
     double* ct = nullptr;
     // constraints (current) + warm cache (previous)
@@ -4122,6 +4067,7 @@ int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
     HIP_TRY(c, dalloc(c->epa_q, pcap));
     HIP_TRY(c, dalloc(c->epa_long, pcap));
     HIP_TRY(c, dalloc(c->epa_long2, pcap));
+    HIP_TRY(c, dalloc(c->epa_long3, pcap));
     HIP_TRY(c, dalloc(c->isp, pcap));
     HIP_TRY(c, dalloc(c->ncont, (size_t)pcap + 1));
     HIP_TRY(c, dalloc(c->cons_off, (size_t)pcap + 1));
@@ -4238,7 +4184,7 @@ void free_work(ne_gpu_ctx* c) {
     dfree(c->cell_of); dfree(c->cell_count); dfree(c->cell_start); dfree(c->sorted_idx); dfree(c->sorted);
     dfree(c->order_prev); dfree(c->xp);
     dfree(c->row_count); dfree(c->row_start); dfree(c->ex_flags); dfree(c->ex_off);
-    dfree(c->cand_i); dfree(c->cand_j); dfree(c->epa_q); dfree(c->epa_long); dfree(c->epa_long2); dfree(c->isp);
+    dfree(c->cand_i); dfree(c->cand_j); dfree(c->epa_q); dfree(c->epa_long); dfree(c->epa_long2); dfree(c->epa_long3); dfree(c->isp);
     dfree(c->wake_at); dfree(c->mint); dfree(c->wake_again); dfree(c->ncont); dfree(c->cons_off); dfree(c->ct);
     ConsDev& C = c->C;
     dfree(C.ci); dfree(C.cj); dfree(C.slot);
@@ -4423,13 +4369,17 @@ void enqueue_detect(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool 
         } else {
             k_narrow<false><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont,
                                                                c->ct, c->epa_q, c->pcap);
-            k_epa<<<wave_blocks(c->hint_c), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                         c->epa_q, CNT_EPAQ, c->pcap, c->epa_big, EPA_FIRST_ITERS,
-                                                         c->epa_long);
+            k_epa_lds<8, 12, 8><<<wave_blocks(c->hint_c), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j,
+                                                                      c->ncont, c->ct, c->epa_q, CNT_EPAQ, c->pcap,
+                                                                      c->epa_long, CNT_EPALONG);
+            k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_EPALONG3, 0u, nullptr, 1u);
+            k_epa_lds<16, 24, 16><<<wave_blocks(c->hint_c / 4), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j,
+                                                                          c->ncont, c->ct, c->epa_long, CNT_EPALONG,
+                                                                          c->pcap, c->epa_long3, CNT_EPALONG3);
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_EPALONG2, 0u, nullptr, 1u);
-            k_epa<<<wave_blocks(c->hint_c / 8), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                             c->epa_long, CNT_EPALONG, c->pcap, c->epa_big,
-                                                             EPA_SECOND_ITERS, c->epa_long2);
+            k_epa_group<8, 36, 64><<<wave_blocks(c->hint_c / 8), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j,
+                                                                           c->ncont, c->ct, c->epa_long3, CNT_EPALONG3,
+                                                                           c->pcap, c->epa_long2, CNT_EPALONG2);
             k_epa_wave<<<2048, 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_long2,
                                            CNT_EPALONG2, c->pcap, c->epa_big);
             k_snap_counter<<<1, 64, 0, s>>>(c->counters, CNT_EPABIG, CNT_EPABIG_PAIRS);
@@ -4453,9 +4403,8 @@ void enqueue_detect(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool 
             k_mesh_emit<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_off,
                                                                c->mqcap, c->mq_pair, c->mq_tri);
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_MQLONG, 0u, nullptr, 1u);
-            k_mesh_narrow<<<wave_blocks(c->hint_mq), 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
-                                                                  c->mq_tri, c->mhit, c->mct, c->epa_big, nullptr, CNT_MQ,
-                                                                  EPA_FIRST_ITERS, c->mq_long);
+            k_mesh_lds<<<wave_blocks(c->hint_mq), 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
+                                                               c->mq_tri, c->mhit, c->mct, c->mq_long);
             k_mesh_wave<<<2048, 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri,
                                             c->mhit, c->mct, c->mq_long, CNT_MQLONG, c->epa_big);
             k_mesh_big<<<big_blocks(c), 64 * BIG_NW, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri, c->mhit,

@@ -108,19 +108,45 @@ __device__ __forceinline__ SupV cso_support(const Geom& A, const Geom& B, V3 shi
     return SupV{pa - pb, pa, pb};
 }
 
+// A Minkowski point with the direction it was found along instead of its two
+// witness points: the witnesses are the same support calls again, so a
+// polytope of these needs them only for its final face.
+struct SupD {
+    V3 v, d;
+};
+
+__device__ __forceinline__ void make_sup(const Geom& A, const Geom& B, V3 shift, V3 d, SupV& out) {
+    out = cso_support(A, B, shift, d);
+}
+__device__ __forceinline__ void make_sup(const Geom& A, const Geom& B, V3 shift, V3 d, SupD& out) {
+    out.v = support_world(A, d) - (support_world(B, -d) + shift);
+    out.d = d;
+}
+// The witness points of a SupD: exactly cso_support's for its direction.
+__device__ __forceinline__ SupV witness(const Geom& A, const Geom& B, V3 shift, V3 v, V3 d) {
+    SupV s;
+    s.a = support_world(A, d);
+    s.b = support_world(B, -d) + shift;
+    s.v = v;
+    return s;
+}
+
 // src/epa.hpp:28-64 -- GJK evolving simplex; on overlap returns an
-// origin-enclosing tetrahedron.
-__device__ inline bool gjk_tetra(const Geom& A, const Geom& B, V3 shift, SupV tet[4]) {
+// origin-enclosing tetrahedron (S = SupV with witness points, or SupD).
+template <class S>
+__device__ inline bool gjk_tetra(const Geom& A, const Geom& B, V3 shift, S tet[4]) {
     V3 d{1.0, 0.0, 0.0};
-    SupV s = cso_support(A, B, shift, d);
-    if (norm2(s.v) < 1e-24) { d = V3{0.0, 1.0, 0.0}; s = cso_support(A, B, shift, d); }
-    SupV W[4];
+    S s;
+    make_sup(A, B, shift, d, s);
+    if (norm2(s.v) < 1e-24) { d = V3{0.0, 1.0, 0.0}; make_sup(A, B, shift, d, s); }
+    S W[4];
     int n = 1;
     W[0] = s;
     d = -s.v;
     for (int iter = 0; iter < 64; ++iter) {
         if (norm2(d) < 1e-24) d = V3{1.0, 0.0, 0.0};
-        SupV p = cso_support(A, B, shift, d);
+        S p;
+        make_sup(A, B, shift, d, p);
         if (dot(p.v, d) <= 1e-12) return false;
         // shift the simplex (n <= 3 here) with fixed indices so it stays in
         // registers; slots past n hold stale points that are never read
@@ -138,15 +164,15 @@ __device__ inline bool gjk_tetra(const Geom& A, const Geom& B, V3 shift, SupV te
             V3 ab = W[1].v - a, ac = W[2].v - a;
             V3 abc = cross(ab, ac);
             if (dot(abc, ao) > 0) d = abc;
-            else { SupV t = W[1]; W[1] = W[2]; W[2] = t; d = -abc; }
+            else { S t = W[1]; W[1] = W[2]; W[2] = t; d = -abc; }
             if (norm2(d) < 1e-24) return false;
         } else {
             V3 b = W[1].v, c = W[2].v, dd = W[3].v;
             V3 ab = b - a, ac = c - a, ad = dd - a;
             V3 abc = cross(ab, ac), acd = cross(ac, ad), adb = cross(ad, ab);
             if (dot(abc, ao) > 0) { n = 3; d = abc; }
-            else if (dot(acd, ao) > 0) { SupV w2 = W[2], w3 = W[3]; W[1] = w2; W[2] = w3; n = 3; d = acd; }
-            else if (dot(adb, ao) > 0) { SupV w1 = W[1], w3 = W[3]; W[1] = w3; W[2] = w1; n = 3; d = adb; }
+            else if (dot(acd, ao) > 0) { S w2 = W[2], w3 = W[3]; W[1] = w2; W[2] = w3; n = 3; d = acd; }
+            else if (dot(adb, ao) > 0) { S w1 = W[1], w3 = W[3]; W[1] = w3; W[2] = w1; n = 3; d = adb; }
             else { tet[0] = W[0]; tet[1] = W[1]; tet[2] = W[2]; tet[3] = W[3]; return true; }
         }
     }
@@ -158,14 +184,6 @@ __device__ inline bool gjk_tetra(const Geom& A, const Geom& B, V3 shift, SupV te
 constexpr int EPA_MAXV = 68;    // 4 + 64 iterations: every vertex an EPA run can make
 constexpr int EPA_MAXF = 160;
 constexpr int EPA_MAXH = 96;
-// The per-thread workspace of the first tier.  A run that needs more stops
-// with `overflow` set and is rerun from the start by the wave-cooperative
-// second tier.  (A third of this size sent so many sphere-box runs to the
-// second tier that its workspace, slots x faces, outgrew the device.)
-constexpr int EPA1_MAXV = EPA_MAXV;
-constexpr int EPA1_MAXF = EPA_MAXF;
-constexpr int EPA1_MAXH = EPA_MAXH;
-
 struct Face {
     V3 n;
     double dist;
