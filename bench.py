@@ -612,6 +612,9 @@ def _loop_sample(g, b, sample, loop_steps):
     return (t1 - t0) * 1e3 / (loop_steps * len(ctxs))
 
 
+BATCH_CALL_STEPS = 10
+
+
 def batched_bench(worlds: int, loop_worlds: int, loop_steps: int):
     """Batched world-steps/s over the whole 300-step run of the batch (HIP
     graph replay; the autotuner runs after step 30, from a snapshot it
@@ -634,10 +637,14 @@ def batched_bench(worlds: int, loop_worlds: int, loop_steps: int):
     for chunk_end in (30, 150, 300):
         calls0 = g.epa_capped_calls
         t0 = time.perf_counter()
-        g.step(chunk_end - done)
+        # calls of BATCH_CALL_STEPS graph-replayed steps: a call whose step
+        # outgrows a buffer reruns from the call's start with the buffer grown
+        while done < chunk_end:
+            k = min(BATCH_CALL_STEPS, chunk_end - done)
+            g.step(k)
+            done += k
         timed_s += time.perf_counter() - t0
         capped += g.epa_capped_calls - calls0
-        done = chunk_end
         if tune is None:
             tune = g.autotune(steps=3)
             log(f"autotuner: {tune}")

@@ -2510,6 +2510,7 @@ struct WfDev {
     uint32_t* fr;          // visits in the order their rounds ran them
     uint32_t* rcnt;        // per round: visits queued for it
     uint32_t* roff;        // per round: index in fr of its first visit
+    uint32_t* done;        // visits run so far this step (one counter)
     uint32_t* inc_cnt;     // per body: tasks touching it (zero between steps)
     uint32_t* inc_start;   // per body: start of its slice of inc_list
     uint32_t* inc_list;    // per movable body: its tasks
@@ -2715,46 +2716,82 @@ __global__ void k_wf_expect_none(uint32_t* counters) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && counters[CNT_NBIG] != 0u) atomicOr(&counters[CNT_FAULT], NE_FAULT_WF_PLAN);
 }
 
-// Round r: run its visits, then queue each successor whose last predecessor
-// this was for round r + 1.
-// The visits of one round [base, base + cnt) of the visit list, over the
-// launch's threads; ready successors go to round r + 1.
-__device__ __forceinline__ void wf_run_round(BodiesDev& B, const SimParams& P, const ConsDev& C, const WfDev& W,
-                                             uint32_t T, uint32_t npass, uint32_t r, uint32_t base, uint32_t cnt,
-                                             uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
-    const uint32_t nbase = base + cnt;
-    for (uint32_t u0 = blockIdx.x * blockDim.x; u0 < cnt; u0 += gridDim.x * blockDim.x) {
-        const uint32_t u = u0 + threadIdx.x;
-        uint32_t tv[2] = {WF_NONE, WF_NONE};
-        bool ready[2] = {false, false};
-        if (u < cnt) {
-            const uint32_t v = W.fr[base + u];
-            const uint32_t pass = v / T, t = v - pass * T;
-            const uint4 tk = W.task4[t];
-            const uint2 nxt = reinterpret_cast<const uint2*>(W.next)[t];
-            wf_visit(B, P, C, W, pass, tk.z, tk.w, tk.x, tk.x + tk.y, prev_key, prev_imp);
-            for (int side = 0; side < 2; ++side) {
-                const uint32_t nx = side ? nxt.y : nxt.x;
-                if (nx == WF_NONE) continue;
-                if (nx & WF_WRAP) {
-                    if (pass + 1 >= npass) continue;
-                    tv[side] = (pass + 1) * T + (nx & ~WF_WRAP);
-                } else {
-                    tv[side] = pass * T + nx;
-                }
-                ready[side] = atomicSub(&W.pend[tv[side]], 1u) == 1u;
-            }
+// Launch r: runs the visits queued for it, then, in the same block, up to
+// WF_LEVELS - 1 further levels of the successors that became ready there.
+//
+// A visit may run in the block that made it ready only if every predecessor
+// that ran in this launch ran in this block: that block's earlier writes are
+// visible to it after a barrier (one CU), while another block's are not
+// until the launch ends.  So each pending count also records who released
+// the visit: a predecessor adds (tag << 2) - 1, tag = (launch + 1) << 12 |
+// block, to the count (at most 2, in the low two bits); the predecessor that
+// takes it to zero sees the other one's tag in the old value (0 when it ran
+// in no launch yet, or never existed).  A visit released by another block in
+// this launch, or past the last local level or the block's list capacity,
+// is queued for launch r + 1 instead.  Every visit still runs after all of
+// its predecessors, on bodies no other ready visit touches, so the result is
+// the sequential sweep's bit for bit.
+constexpr int WF_LEVELS = 4;     // levels a block runs per launch when rounds are wide
+constexpr int WF_LCAP = 512;     // a block's list of locally ready visits (per level)
+// Measured on one MI355X: local levels help where rounds are wide (the 1M
+// terrain scene, about 78k visits a round: 55.5 -> 52.7 ms a step) and cost
+// where they are narrow (the 65,536-body pile, about 1,900 visits a round:
+// 23.2 -> 25.9 ms), where a visit's chain of dependent memory accesses, not
+// the launch, bounds a round.  The host picks one level below this width.
+constexpr uint64_t WF_WIDE_ROUND = 16384;
+
+// Run visit v, release its successors: out[side] = the successor made ready
+// (WF_NONE: none), local[side] = whether it may run in this block.
+__device__ __forceinline__ void wf_run_visit(BodiesDev& B, const SimParams& P, const ConsDev& C, const WfDev& W,
+                                             uint32_t T, uint32_t npass, uint32_t v, uint32_t tag, uint32_t out[2],
+                                             bool local[2], uint64_t* __restrict__ prev_key,
+                                             double* __restrict__ prev_imp) {
+    const uint32_t pass = v / T, t = v - pass * T;
+    const uint4 tk = W.task4[t];
+    const uint2 nxt = reinterpret_cast<const uint2*>(W.next)[t];
+    wf_visit(B, P, C, W, pass, tk.z, tk.w, tk.x, tk.x + tk.y, prev_key, prev_imp);
+    for (int side = 0; side < 2; ++side) {
+        out[side] = WF_NONE;
+        local[side] = false;
+        const uint32_t nx = side ? nxt.y : nxt.x;
+        if (nx == WF_NONE) continue;
+        uint32_t w;
+        if (nx & WF_WRAP) {
+            if (pass + 1 >= npass) continue;
+            w = (pass + 1) * T + (nx & ~WF_WRAP);
+        } else {
+            w = pass * T + nx;
         }
-        // one append per wave for both sides: every lane targets round r + 1
-        const uint64_t b0 = __ballot(ready[0]), b1 = __ballot(ready[1]);
-        const uint32_t tot = (uint32_t)(__popcll(b0) + __popcll(b1));
-        uint32_t slot0 = 0;
-        if (lane_id() == 0 && tot) slot0 = atomicAdd(&W.rcnt[r + 1], tot);
-        slot0 = __shfl(slot0, 0);
-        const uint64_t below = lane_id() ? (~0ull >> (64 - lane_id())) : 0ull;
-        if (ready[0]) W.fr[nbase + slot0 + (uint32_t)__popcll(b0 & below)] = tv[0];
-        if (ready[1]) W.fr[nbase + slot0 + (uint32_t)__popcll(b0) + (uint32_t)__popcll(b1 & below)] = tv[1];
+        const uint32_t old = atomicAdd(&W.pend[w], (tag << 2) - 1u);
+        if ((old & 3u) != 1u) continue;   // another predecessor is still to run
+        const uint32_t other = old >> 2;
+        out[side] = w;
+        local[side] = (other >> 12) != (tag >> 12) || other == tag;
     }
+}
+
+// Queue the successors this wave's lanes made ready: local ones into the
+// block's list for the next level (when `may_local` and it has room), the
+// rest for launch r + 1 (one append per wave).
+__device__ __forceinline__ void wf_release(const WfDev& W, uint32_t r, uint32_t nbase, const uint32_t out[2],
+                                           const bool local[2], bool may_local, uint32_t* lq, uint32_t* lqn) {
+    bool glob[2];
+    for (int side = 0; side < 2; ++side) {
+        glob[side] = out[side] != WF_NONE;
+        if (glob[side] && local[side] && may_local) {
+            const uint32_t at = atomicAdd(lqn, 1u);
+            if (at < (uint32_t)WF_LCAP) { lq[at] = out[side]; glob[side] = false; }
+        }
+    }
+    const uint64_t b0 = __ballot(glob[0]), b1 = __ballot(glob[1]);
+    const uint32_t tot = (uint32_t)(__popcll(b0) + __popcll(b1));
+    if (tot == 0) return;
+    uint32_t slot0 = 0;
+    if (lane_id() == 0) slot0 = atomicAdd(&W.rcnt[r + 1], tot);
+    slot0 = __shfl(slot0, 0);
+    const uint64_t below = lane_id() ? (~0ull >> (64 - lane_id())) : 0ull;
+    if (glob[0]) W.fr[nbase + slot0 + (uint32_t)__popcll(b0 & below)] = out[0];
+    if (glob[1]) W.fr[nbase + slot0 + (uint32_t)__popcll(b0) + (uint32_t)__popcll(b1 & below)] = out[1];
 }
 
 }  // namespace
@@ -2773,13 +2810,54 @@ struct WfArgs {
 
 namespace {
 
-__global__ void __launch_bounds__(WF_THREADS) k_wf_round(const WfArgs* __restrict__ a, uint32_t r) {
+__global__ void __launch_bounds__(WF_THREADS) k_wf_round(const WfArgs* __restrict__ a, uint32_t r, int levels) {
+    __shared__ uint32_t lq[2][WF_LCAP];
+    __shared__ uint32_t lqn[2];
     BodiesDev B = a->B;
     const WfDev W = a->W;
+    const uint32_t T = a->T, npass = a->npass;
     const uint32_t base = W.roff[r], cnt = W.rcnt[r];
     if (blockIdx.x == 0 && threadIdx.x == 0) W.roff[r + 1] = base + cnt;
     if (cnt == 0) return;
-    wf_run_round(B, a->P, a->C, W, a->T, a->npass, r, base, cnt, a->prev_key, a->prev_imp);
+    const uint32_t nbase = base + cnt;
+    const uint32_t tag = ((r + 1u) << 12) | blockIdx.x;
+    if (threadIdx.x == 0) { lqn[0] = 0u; lqn[1] = 0u; }
+    __syncthreads();
+    uint32_t ran = 0;
+    // level 0: this block's share of the visits queued for launch r
+    for (uint32_t u0 = blockIdx.x * blockDim.x; u0 < cnt; u0 += gridDim.x * blockDim.x) {
+        const uint32_t u = u0 + threadIdx.x;
+        uint32_t out[2] = {WF_NONE, WF_NONE};
+        bool local[2] = {false, false};
+        if (u < cnt) {
+            wf_run_visit(B, a->P, a->C, W, T, npass, W.fr[base + u], tag, out, local, a->prev_key, a->prev_imp);
+            ++ran;
+        }
+        wf_release(W, r, nbase, out, local, levels > 1, lq[0], &lqn[0]);
+    }
+    // levels 1 ..: the visits this block made ready, from its own list
+    for (int lev = 1; lev < levels; ++lev) {
+        const int cur = (lev - 1) & 1, nxl = lev & 1;
+        __syncthreads();   // level lev - 1's writes and list are complete
+        const uint32_t n = min(lqn[cur], (uint32_t)WF_LCAP);
+        if (n == 0) break;
+        if (threadIdx.x == 0) lqn[nxl] = 0u;   // its last reader was level lev - 1
+        __syncthreads();
+        for (uint32_t u0 = 0; u0 < n; u0 += blockDim.x) {
+            const uint32_t u = u0 + threadIdx.x;
+            uint32_t out[2] = {WF_NONE, WF_NONE};
+            bool local[2] = {false, false};
+            if (u < n) {
+                wf_run_visit(B, a->P, a->C, W, T, npass, lq[cur][u], tag, out, local, a->prev_key, a->prev_imp);
+                ++ran;
+            }
+            wf_release(W, r, nbase, out, local, lev + 1 < levels, lq[nxl], &lqn[nxl]);
+        }
+    }
+    // visits run: one atomic per wave
+    uint32_t wsum = ran;
+    for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off);
+    if (lane_id() == 0 && wsum) atomicAdd(W.done, wsum);
 }
 
 // ---------------------------------------------------------------------------
@@ -3703,6 +3781,7 @@ struct ne_gpu_ctx {
     uint32_t wf_tcap = 0, wf_ccap = 0, wf_ncap = 0;   // capacities: tasks, constraints, bodies
     uint32_t wf_vcap_pass = 0;                        // passes the visit arrays hold per task
     uint32_t wf_rounds_hint = 0;                      // rounds the last solve took
+    uint64_t wf_width_hint = 0;                       // visits per level in the last solve
     bool wf_expect = true;                            // expect large islands: read their task count back
     uint32_t wf_quiet = 0;                            // consecutive steps without large islands (checked ones)
     bool wf_seen = false;                             // this call had a step with large islands
@@ -4228,7 +4307,7 @@ void free_wf(ne_gpu_ctx* c) {
     if (c->h_wf_args) { (void)hipHostFree(c->h_wf_args); c->h_wf_args = nullptr; }
     WfDev& W = c->wf;
     dfree(W.flag); dfree(W.tid); dfree(W.task_k); dfree(W.next); dfree(W.pend0); dfree(W.pend); dfree(W.fr);
-    dfree(W.rcnt); dfree(W.roff); dfree(W.inc_cnt); dfree(W.inc_start); dfree(W.inc_list);
+    dfree(W.rcnt); dfree(W.roff); dfree(W.done); dfree(W.inc_cnt); dfree(W.inc_start); dfree(W.inc_list);
     dfree(W.task_n); dfree(W.task4); dfree(W.rec); dfree(W.bp);
     c->wf_tcap = c->wf_ccap = c->wf_ncap = c->wf_vcap_pass = 0;
 }
@@ -4502,6 +4581,7 @@ int ensure_wf(ne_gpu_ctx* c, uint32_t T, uint32_t npass) {
     if (!W.rcnt) {
         HIP_TRY(c, dalloc(W.rcnt, (size_t)WF_MAX_ROUNDS + 2));
         HIP_TRY(c, dalloc(W.roff, (size_t)WF_MAX_ROUNDS + 2));
+        HIP_TRY(c, dalloc(W.done, 4));
     }
     if (c->wf_ccap < c->ccap) {
         HIP_TRY(c, dalloc(W.flag, c->ccap));
@@ -4580,6 +4660,7 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     k_wf_link<<<gs_blocks(c->n), TPB, 0, s>>>(c->n, c->C, W);
     k_wf_init<<<gs_blocks(T), TPB, 0, s>>>(B, T, npass, c->C, W);
     k_wf_pack_bodies<<<blocks_for(c->n), TPB, 0, s>>>(B, W.bp);
+    k_fill_u32<<<1, 64, 0, s>>>(W.done, 0u, nullptr, 1u);
     const uint64_t total = (uint64_t)T * npass;
     if (!c->wf_args) HIP_TRY(c, hipMalloc((void**)&c->wf_args, sizeof(WfArgs)));
     if (!c->h_wf_args) HIP_TRY(c, hipHostMalloc((void**)&c->h_wf_args, sizeof(WfArgs), hipHostMallocDefault));
@@ -4590,6 +4671,8 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     const uint32_t hint = c->wf_rounds_hint ? c->wf_rounds_hint : 64u;
     uint64_t per_round = total / hint + 1;
     uint32_t grid = (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(8, (per_round * 3 / 2 + WF_THREADS - 1) / WF_THREADS));
+    // round width from the last solve's depth in single-level rounds
+    const int levels = (c->wf_width_hint > WF_WIDE_ROUND) ? WF_LEVELS : 1;
     uint32_t r = 0, batch = hint + 16;
     for (;;) {
         if (r + batch + 2 >= WF_MAX_ROUNDS) return set_err(c, NE_ERR_UNSUPPORTED, "large-island solve needs too many rounds");
@@ -4598,12 +4681,12 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
                                                                           r + batch + 2 - cleared);
             cleared = r + batch + 2;
         }
-        for (uint32_t q = 0; q < batch; ++q, ++r) k_wf_round<<<grid, WF_THREADS, 0, s>>>(c->wf_args, r);
-        HIP_TRY(c, hipMemcpyAsync(c->h_wf, W.roff + r, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        for (uint32_t q = 0; q < batch; ++q, ++r) k_wf_round<<<grid, WF_THREADS, 0, s>>>(c->wf_args, r, levels);
+        HIP_TRY(c, hipMemcpyAsync(c->h_wf, W.done, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(c, hipMemcpyAsync(c->h_wf + 1, W.rcnt + r, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(c, hipStreamSynchronize(s));
         if (c->h_wf[0] >= total) break;
-        if (c->h_wf[1] == 0)
+        if (c->h_wf[1] == 0)   // nothing queued for the next launch, and nothing else is running
             return set_err(c, NE_ERR_INVALID, "large-island solve stalled (a visit never became ready)");
         batch = 64;
     }
@@ -4614,6 +4697,8 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     uint32_t depth = r;
     while (depth > 0 && cnt[depth - 1] == 0) --depth;
     c->wf_rounds_hint = depth;
+    // visits per level: the launches' depth times the levels each ran
+    c->wf_width_hint = total / ((uint64_t)std::max<uint32_t>(depth, 1u) * (uint64_t)levels);
     return NE_OK;
 }
 
