@@ -120,6 +120,99 @@ def cpu_baseline(scene, steps: int, what: str):
     }
 
 
+# Kernels of each stage, by name (rocprofv3 names them with their namespace).
+STAGE_OF_KERNEL = (("narrow", ("k_narrow", "k_epa", "k_mesh", "k_snap_counter")),
+                   ("solve", ("k_wf_", "k_solve_", "k_copy_cache")))
+
+
+def _kernel_base(name: str) -> str:
+    """`(anonymous namespace)::k_epa_lds<8, 12, 8>(...)` -> `k_epa_lds<8, 12, 8>`."""
+    import re
+    name = re.sub(r"\(anonymous namespace\)::|nea::|^void ", "", name)
+    depth, out = 0, []
+    for ch in name:   # cut the argument list, keep template arguments
+        if ch == "(" and depth == 0:
+            break
+        depth += ch == "<"
+        depth -= ch == ">"
+        out.append(ch)
+    return "".join(out).strip()
+
+
+def _stage_of(kernel: str):
+    for stage, prefixes in STAGE_OF_KERNEL:
+        if kernel.startswith(prefixes):
+            return stage
+    return None
+
+
+def _counter_sums(path: str, counter: str):
+    """Per kernel: (sum of the counter over its dispatches, dispatches), and
+    the number of k_integrate dispatches (one per step)."""
+    import csv
+    sums, calls = {}, {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            if row.get("Counter_Name") != counter:
+                continue
+            k = _kernel_base(row["Kernel_Name"])
+            sums[k] = sums.get(k, 0.0) + float(row["Counter_Value"])
+            calls[k] = calls.get(k, 0) + 1
+    steps = sum(v for k, v in calls.items() if k.startswith("k_integrate"))
+    return sums, steps
+
+
+def pmc_summary(probe_f: str, probe_w: str, run_f: str, run_w: str) -> str:
+    """HBM bytes per step of each stage and kernel: FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch) summed over the run, each scaled so that the probe
+    kernel's reading equals the bytes it moves (reads 512 MiB, writes 256 MiB),
+    divided by the steps (k_integrate dispatches)."""
+    pf, _ = _counter_sums(probe_f, "FETCH_SIZE")
+    pw, _ = _counter_sums(probe_w, "WRITE_SIZE")
+    kf = (2 * 8 << 25) / (pf.get("k_fp_probe", 0.0) * 1024.0)
+    kw = (8 << 25) / (pw.get("k_fp_probe", 0.0) * 1024.0)
+    rf, steps = _counter_sums(run_f, "FETCH_SIZE")
+    rw, steps_w = _counter_sums(run_w, "WRITE_SIZE")
+    steps = max(steps, 1)
+    per = {}
+    for k in set(rf) | set(rw):
+        per[k] = (rf.get(k, 0.0) * 1024.0 * kf / steps, rw.get(k, 0.0) * 1024.0 * kw / max(steps_w, 1))
+    lines = [f"probe (`bench.py --traffic-probe`: k_fp_probe reads 512 MiB, writes 256 MiB): FETCH_SIZE scale {kf:.4f}, "
+             f"WRITE_SIZE scale {kw:.4f}", "", f"steps in the run (k_integrate dispatches): {steps}", ""]
+    stage_tot = {}
+    for k, (r, w) in per.items():
+        st = _stage_of(k)
+        if st:
+            a = stage_tot.setdefault(st, [0.0, 0.0])
+            a[0] += r
+            a[1] += w
+    for st, (r, w) in sorted(stage_tot.items()):
+        lines.append(f"- {st}: read {r / 1e9:.2f} GB, write {w / 1e9:.2f} GB, total {(r + w) / 1e9:.2f} GB per step")
+    lines += ["", "top kernels (GB per step, scaled):"]
+    for k, (r, w) in sorted(per.items(), key=lambda kv: -(kv[1][0] + kv[1][1]))[:16]:
+        lines.append(f"- `{k}`: read {r / 1e9:.2f}, write {w / 1e9:.2f}")
+    lines += ["", "traffic_bytes_per_launch: " + " ".join(f"{st}={int(r + w)}" for st, (r, w) in sorted(stage_tot.items()))]
+    return "\n".join(lines)
+
+
+def kernel_summary(path: str, top: int = 30) -> str:
+    """The per-kernel table of a rocprofv3 --stats kernel_stats.csv."""
+    import csv
+    rows = []
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            rows.append((_kernel_base(row["Name"]), int(row["Calls"]), float(row["TotalDurationNs"]),
+                         float(row["AverageNs"]), float(row["Percentage"])))
+    total = sum(r[2] for r in rows)
+    steps = sum(r[1] for r in rows if r[0].startswith("k_integrate"))
+    out = [f"total kernel time {total / 1e6:.1f} ms over {steps} steps (k_integrate dispatches): "
+           f"{total / 1e6 / max(steps, 1):.2f} ms per step", "",
+           "| kernel | calls | total ms | ms per step | avg us | % |", "|---|---|---|---|---|---|"]
+    for k, c, t, a, p in sorted(rows, key=lambda r: -r[2])[:top]:
+        out.append(f"| `{k}` | {c} | {t / 1e6:.1f} | {t / 1e6 / max(steps, 1):.3f} | {a / 1e3:.1f} | {p:.2f} |")
+    return "\n".join(out)
+
+
 def measured_traffic(stage: str, profile: str = "large_terrain_1M_traffic.md"):
     """HBM bytes per launch of a stage from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, each scaled by a kernel of known traffic), or
@@ -852,11 +945,22 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run one elementwise kernel of known traffic (2^25 doubles: reads 512 MiB, writes "
                          "256 MiB), the scale for rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("PROBE_FETCH", "PROBE_WRITE", "RUN_FETCH", "RUN_WRITE"),
+                    help="print the HBM traffic summary (markdown) of four rocprofv3 --pmc counter_collection.csv "
+                         "files: FETCH_SIZE and WRITE_SIZE of --traffic-probe, then of the profiled run")
+    ap.add_argument("--kernel-summary", metavar="KERNEL_STATS_CSV",
+                    help="print the per-kernel table (markdown) of a rocprofv3 --stats kernel_stats.csv")
     ap.add_argument("--cpu-sample", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-scene", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_sample:
         cpu_sample_main(args.cpu_sample, args.cpu_scene)
+        return
+    if args.pmc_summary:
+        print(pmc_summary(*args.pmc_summary))
+        return
+    if args.kernel_summary:
+        print(kernel_summary(args.kernel_summary))
         return
     if args.traffic_probe:
         import numpy as np
