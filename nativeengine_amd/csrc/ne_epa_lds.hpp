@@ -25,8 +25,9 @@
 // to the one-wave tier (k_epa_wave) and the one-block tier (k_narrow_big);
 // each reruns it from the start, so every tier gives the same bits.
 //
-// Included by ne_gpu.hip inside its anonymous namespace, after k_narrow
-// (load_geom, box_of, wave_add1, CNT_*).
+// Included by ne_gpu.hip inside its anonymous namespace, after k_narrow and
+// the mesh helpers (load_geom, box_of, wave_add1, CNT_*, MeshDev, mesh_pair,
+// mesh_prism, store_contact).
 #pragma once
 
 // [slot][lane] workspace of one wave: MV vertices, MF faces, MH horizon entries.
@@ -235,8 +236,20 @@ template <int MV, int MF>
 struct EpaGroupLds {
     double vx[MV], vy[MV], vz[MV];
     double fnx[MF], fny[MF], fnz[MF], fd[MF];
+    double prism[18];            // a mesh candidate's triangle prism (six vertices)
     uint32_t fabc[MF];           // a | b << 8 | c << 16
     uint16_t edge[3 * MF];       // u | w << 8: the visible faces' edges of one iteration
+};
+
+// Where the group tier's runs come from when they are mesh triangle
+// candidates (k_mesh_lds's leftovers) rather than pairs: the mesh, the
+// candidates' pairs and triangles, and their hit flags and contacts.
+struct MeshSrc {
+    MeshDev M;
+    const uint32_t* mq_pair;
+    const int32_t* mq_tri;
+    uint8_t* mhit;
+    double* mct;
 };
 
 // LDS written by some lanes of a wave and read by others after this point
@@ -268,14 +281,16 @@ __device__ __forceinline__ void eg_make_face(EpaGroupLds<MV, MF>& L, int i, int 
 
 // One-wave blocks; 64 / G runs per block at a time, grid-stride over the
 // queue `epa_q` (length in counters[count_slot]); runs past the workspace go
-// to `long_q` (counted in counters[long_slot]).
-template <int G, int MV, int MF>
+// to `long_q` (counted in counters[long_slot]).  MESH: the queue holds mesh
+// triangle candidates (detectContact(other, prism), contacts to src.mct and
+// src.mhit), else pairs (contacts to ct and ncont).
+template <int G, int MV, int MF, bool MESH>
 __global__ void __launch_bounds__(64) k_epa_group(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
                                                   const uint32_t* __restrict__ cand_i,
                                                   const uint32_t* __restrict__ cand_j, uint32_t* __restrict__ ncont,
                                                   double* __restrict__ ct, const uint32_t* __restrict__ epa_q,
                                                   int count_slot, uint32_t epa_cap, uint32_t* __restrict__ long_q,
-                                                  int long_slot) {
+                                                  int long_slot, MeshSrc src) {
     static_assert(G >= 4 && 64 % G == 0, "a group holds the initial tetrahedron's four faces");
     constexpr int NG = 64 / G;
     __shared__ EpaGroupLds<MV, MF> lds[NG];
@@ -288,16 +303,26 @@ __global__ void __launch_bounds__(64) k_epa_group(BodiesDev B, SimParams P, uint
     V3 dirs[MV];   // the group's first lane keeps the support directions
     for (uint32_t t0 = blockIdx.x * NG; t0 < nq; t0 += gridDim.x * NG) {
         const uint32_t t = t0 + g;
-        bool active = t < nq, defer = false, hit = false;
-        const uint32_t p = active ? epa_q[t] : 0u;
+        bool active = t < nq, defer = false, hit = false, flip = false;
+        const uint32_t p = active ? epa_q[t] : 0u;   // MESH: a triangle candidate
         Geom a{}, b{};
         V3 shift{0.0, 0.0, 0.0};
         const Box3* box = &P.box;
         if (active) {
-            const uint32_t i = cand_i[p], j = cand_j[p];
-            box = &box_of(B, P, i);
-            a = load_geom(B, i);
-            b = load_geom(B, j);
+            if constexpr (MESH) {
+                uint32_t o, m;
+                const uint32_t pr = src.mq_pair[p];
+                mesh_pair(B, cand_i[pr], cand_j[pr], o, m, flip);
+                box = &box_of(B, P, o);
+                a = load_geom(B, o);
+                mesh_prism(B, src.M, m, src.mq_tri[p], L.prism, b);   // non-degenerate: its EPA ran; every lane writes the same bits
+                wave_lds_sync();
+            } else {
+                const uint32_t i = cand_i[p], j = cand_j[p];
+                box = &box_of(B, P, i);
+                a = load_geom(B, i);
+                b = load_geom(B, j);
+            }
             shift = min_image_shift(*box, a.x, b.x);
             SupD tet[4];
             hit = gjk_tetra(a, b, shift, tet);   // k_narrow's GJK on the same inputs: always true here
@@ -418,10 +443,11 @@ __global__ void __launch_bounds__(64) k_epa_group(BodiesDev B, SimParams P, uint
                 wave_lds_sync();
             }
         }
-        // contacts: the group's first lane (narrow_pair's convex branch after EPA)
+        // contacts: the group's first lane (narrow_pair's convex branch after
+        // EPA; a mesh candidate's one contact, meshContacts in world.cpp:56-83)
         if (t < nq && gl == 0 && !defer) {
             uint32_t nc = 0;
-            double* out = ct + (size_t)p * 28;
+            Contact c = no_contact();
             if (hit) {
                 const int ia = (int)(babc & 0xffu), ib = (int)((babc >> 8) & 0xffu), ic = (int)((babc >> 16) & 0xffu);
                 SupV tri[3];
@@ -430,13 +456,21 @@ __global__ void __launch_bounds__(64) k_epa_group(BodiesDev B, SimParams P, uint
                 tri[2] = witness(a, b, shift, eg_vert(L, ic), dirs[ic]);
                 Face bf;
                 bf.n = bn; bf.dist = bdist; bf.a = 0; bf.b = 1; bf.c = 2;
-                const Contact c = epa_result(tri, bf);
-                nc = convex_pair_contacts(*box, a, b, P, c, out, fault);
+                c = epa_result(tri, bf);
+                if constexpr (!MESH) nc = convex_pair_contacts(*box, a, b, P, c, ct + (size_t)p * 28, fault);
             }
-            ncont[p] = nc;
+            if constexpr (MESH) {
+                if (c.hit) store_contact(src.mct + 7 * (size_t)p, c, flip);
+                src.mhit[p] = c.hit ? 1 : 0;
+            } else {
+                ncont[p] = nc;
+            }
         }
         const bool push = t < nq && gl == 0 && defer;
-        if (push) ncont[p] = 0u;   // the next tier writes it
+        if (push) {   // the next tier writes the result
+            if constexpr (MESH) src.mhit[p] = 0;
+            else ncont[p] = 0u;
+        }
         const uint32_t slot = wave_add1(counters, long_slot, push);
         if (push) long_q[slot] = p;   // at most one entry per queued pair: fits the queue's capacity
         wave_lds_sync();   // this round's LDS reads are done before the next runs' writes

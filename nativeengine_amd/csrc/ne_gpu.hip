@@ -132,8 +132,9 @@ enum CounterSlot : int {
     CNT_EPALONG2 = 22,   // convex pairs whose EPA ran past the second launch's budget (this step)
     CNT_HOSTCHK = 23,    // a host call's own check (snapshot loads); no step kernel uses it
     CNT_EV = 24,         // facade events pass: overlapping pairs of the post-step state
-    CNT_EPALONG3 = 25,   // convex pairs the second LDS EPA tier left for the per-thread scratch tier
-    CNT_N = 26
+    CNT_EPALONG3 = 25,   // convex pairs the second LDS EPA tier left for the group tier
+    CNT_MQLONG2 = 26,    // mesh triangle candidates the group tier left for the one-wave tier
+    CNT_N = 27
 };
 
 constexpr int TPB = 256;
@@ -1097,8 +1098,6 @@ __global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, uint32
     }
 }
 
-#include "ne_epa_lds.hpp"
-
 // EPA of one pair by one wave (src/epa.hpp:71-150, the same steps as
 // epa_contact), polytope and horizon list in LDS.  Each iteration the 64
 // lanes find the first closest face (a min/first-index reduction), test the
@@ -1657,6 +1656,8 @@ __device__ __forceinline__ void store_contact(double* out, const Contact& c, boo
 }
 
 constexpr int PV_STRIDE = 19;   // doubles per thread for a prism's vertices (18, padded against bank conflicts)
+
+#include "ne_epa_lds.hpp"
 
 // detectContact(other, prism) per triangle candidate: the plane's analytic
 // path, else GJK and the LDS EPA tier A (k_epa_lds's workspace, beside the
@@ -3750,7 +3751,8 @@ struct ne_gpu_ctx {
     // per step: triangle candidates of the mesh pairs
     uint32_t *mq_cnt = nullptr, *mq_off = nullptr, *mq_pair = nullptr;
     int32_t* mq_tri = nullptr;
-    uint32_t* mq_long = nullptr;                   // candidates the first mesh launch left for the second
+    uint32_t* mq_long = nullptr;                   // candidates the first mesh launch left for the group tier
+    uint32_t* mq_long2 = nullptr;                  // and the ones the group tier left for the one-wave tier
     uint8_t* mhit = nullptr;
     double* mct = nullptr;                         // 7 doubles per candidate: normal, point, depth
     uint32_t mqcap = 0, mq_pcap = 0, hint_mq = 4096;
@@ -4194,6 +4196,7 @@ int alloc_mesh_cands(ne_gpu_ctx* c, uint32_t cap) {
     HIP_TRY(c, dalloc(c->mq_pair, cap));
     HIP_TRY(c, dalloc(c->mq_tri, cap));
     HIP_TRY(c, dalloc(c->mq_long, cap));
+    HIP_TRY(c, dalloc(c->mq_long2, cap));
     HIP_TRY(c, dalloc(c->mhit, cap));
     HIP_TRY(c, dalloc(c->mct, (size_t)cap * 7));
     return NE_OK;
@@ -4207,7 +4210,7 @@ void free_meshes(ne_gpu_ctx* c) {
 }
 
 void free_mesh_work(ne_gpu_ctx* c) {
-    dfree(c->mq_cnt); dfree(c->mq_off); dfree(c->mq_pair); dfree(c->mq_tri); dfree(c->mq_long); dfree(c->mhit);
+    dfree(c->mq_cnt); dfree(c->mq_off); dfree(c->mq_pair); dfree(c->mq_tri); dfree(c->mq_long); dfree(c->mq_long2); dfree(c->mhit);
     dfree(c->mct);
     c->mqcap = c->mq_pcap = 0;
 }
@@ -4456,9 +4459,9 @@ void enqueue_detect(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool 
                                                                           c->ncont, c->ct, c->epa_long, CNT_EPALONG,
                                                                           c->pcap, c->epa_long3, CNT_EPALONG3);
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_EPALONG2, 0u, nullptr, 1u);
-            k_epa_group<8, 36, 64><<<wave_blocks(c->hint_c / 8), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j,
-                                                                           c->ncont, c->ct, c->epa_long3, CNT_EPALONG3,
-                                                                           c->pcap, c->epa_long2, CNT_EPALONG2);
+            k_epa_group<8, 36, 64, false><<<wave_blocks(c->hint_c / 8), 64, 0, s>>>(
+                B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_long3, CNT_EPALONG3, c->pcap,
+                c->epa_long2, CNT_EPALONG2, MeshSrc{});
             k_epa_wave<<<2048, 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_long2,
                                            CNT_EPALONG2, c->pcap, c->epa_big);
             k_snap_counter<<<1, 64, 0, s>>>(c->counters, CNT_EPABIG, CNT_EPABIG_PAIRS);
@@ -4484,8 +4487,12 @@ void enqueue_detect(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool 
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_MQLONG, 0u, nullptr, 1u);
             k_mesh_lds<<<wave_blocks(c->hint_mq), 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
                                                                c->mq_tri, c->mhit, c->mct, c->mq_long);
+            k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_MQLONG2, 0u, nullptr, 1u);
+            k_epa_group<8, 36, 64, true><<<wave_blocks(c->hint_mq / 16), 64, 0, s>>>(
+                B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->mq_long, CNT_MQLONG, c->mqcap,
+                c->mq_long2, CNT_MQLONG2, MeshSrc{M, c->mq_pair, c->mq_tri, c->mhit, c->mct});
             k_mesh_wave<<<2048, 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri,
-                                            c->mhit, c->mct, c->mq_long, CNT_MQLONG, c->epa_big);
+                                            c->mhit, c->mct, c->mq_long2, CNT_MQLONG2, c->epa_big);
             k_mesh_big<<<big_blocks(c), 64 * BIG_NW, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri, c->mhit,
                                          c->mct, c->epa_big);
             k_mesh_gather<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, c->counters, c->cand_i, c->cand_j, c->mq_off,
