@@ -176,14 +176,15 @@ __device__ __forceinline__ uint32_t convex_pair_contacts(const Box3& box, const 
 
 // One-wave blocks, one queued pair per lane, grid-stride over the queue
 // `epa_q` of overlapping convex pairs (its length in counters[count_slot]);
-// runs past the workspace go to `long_q` (counted in counters[long_slot]).
+// each starts from its GJK tetrahedron in `tets` (k_narrow's GJK).  Runs
+// past the workspace go to `long_q` (counted in counters[long_slot]).
 template <int MV, int MF, int MH>
 __global__ void __launch_bounds__(64) k_epa_lds(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
                                                 const uint32_t* __restrict__ cand_i,
                                                 const uint32_t* __restrict__ cand_j, uint32_t* __restrict__ ncont,
                                                 double* __restrict__ ct, const uint32_t* __restrict__ epa_q,
                                                 int count_slot, uint32_t epa_cap, uint32_t* __restrict__ long_q,
-                                                int long_slot) {
+                                                int long_slot, EpaTets tets) {
     __shared__ EpaLaneLds<MV, MF, MH> lds;
     EpaLane<MV, MF, MH> E{lds, (int)threadIdx.x};
     const uint32_t nq = min(counters[count_slot], epa_cap);
@@ -200,17 +201,16 @@ __global__ void __launch_bounds__(64) k_epa_lds(BodiesDev B, SimParams P, uint32
             const V3 shift = min_image_shift(box, a.x, b.x);
             uint32_t nc = 0;
             SupD tet[4];
-            if (gjk_tetra(a, b, shift, tet)) {   // k_narrow's GJK on the same inputs: always true here
-                Contact c;
-                if (epa_lane(E, a, b, shift, tet, dirs, c))
-                    nc = convex_pair_contacts(box, a, b, P, c, ct + (size_t)p * 28, fault);
-                else
-                    defer = true;
-            }
+            tets.get(p, tet);
+            Contact c;
+            if (epa_lane(E, a, b, shift, tet, dirs, c))
+                nc = convex_pair_contacts(box, a, b, P, c, ct + (size_t)p * 28, fault);
+            else
+                defer = true;
             ncont[p] = nc;   // 0 while deferred: the next tier writes it
         }
-        const uint32_t slot = wave_add1(counters, long_slot, defer);
-        if (defer) long_q[slot] = p;   // at most one entry per queued pair: fits the queue's capacity
+        const uint32_t at = wave_add1(counters, long_slot, defer);
+        if (defer) long_q[at] = p;   // at most one entry per queued pair: fits the queue's capacity
     }
 }
 
@@ -288,9 +288,9 @@ template <int G, int MV, int MF, bool MESH>
 __global__ void __launch_bounds__(64) k_epa_group(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
                                                   const uint32_t* __restrict__ cand_i,
                                                   const uint32_t* __restrict__ cand_j, uint32_t* __restrict__ ncont,
-                                                  double* __restrict__ ct, const uint32_t* __restrict__ epa_q,
+                                                  double* __restrict__ ct, const uint32_t* __restrict__ queue,
                                                   int count_slot, uint32_t epa_cap, uint32_t* __restrict__ long_q,
-                                                  int long_slot, MeshSrc src) {
+                                                  int long_slot, MeshSrc src, EpaTets tets) {
     static_assert(G >= 4 && 64 % G == 0, "a group holds the initial tetrahedron's four faces");
     constexpr int NG = 64 / G;
     __shared__ EpaGroupLds<MV, MF> lds[NG];
@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(64) k_epa_group(BodiesDev B, SimParams P, uint
     for (uint32_t t0 = blockIdx.x * NG; t0 < nq; t0 += gridDim.x * NG) {
         const uint32_t t = t0 + g;
         bool active = t < nq, defer = false, hit = false, flip = false;
-        const uint32_t p = active ? epa_q[t] : 0u;   // MESH: a triangle candidate
+        const uint32_t p = active ? queue[t] : 0u;     // a pair (MESH: a triangle candidate)
         Geom a{}, b{};
         V3 shift{0.0, 0.0, 0.0};
         const Box3* box = &P.box;
@@ -325,7 +325,12 @@ __global__ void __launch_bounds__(64) k_epa_group(BodiesDev B, SimParams P, uint
             }
             shift = min_image_shift(*box, a.x, b.x);
             SupD tet[4];
-            hit = gjk_tetra(a, b, shift, tet);   // k_narrow's GJK on the same inputs: always true here
+            if constexpr (MESH) {
+                hit = gjk_tetra(a, b, shift, tet);   // k_mesh_lds's GJK on the same inputs: always true here
+            } else {
+                tets.get(p, tet);                    // k_narrow's GJK
+                hit = true;
+            }
             active = hit;
             if (active && gl == 0) {
                 for (int k = 0; k < 4; ++k) {
@@ -471,8 +476,8 @@ __global__ void __launch_bounds__(64) k_epa_group(BodiesDev B, SimParams P, uint
             if constexpr (MESH) src.mhit[p] = 0;
             else ncont[p] = 0u;
         }
-        const uint32_t slot = wave_add1(counters, long_slot, push);
-        if (push) long_q[slot] = p;   // at most one entry per queued pair: fits the queue's capacity
+        const uint32_t at = wave_add1(counters, long_slot, push);
+        if (push) long_q[at] = p;   // at most one entry per queued run: fits the queue's capacity
         wave_lds_sync();   // this round's LDS reads are done before the next runs' writes
     }
 }

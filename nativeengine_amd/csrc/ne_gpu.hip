@@ -1019,25 +1019,11 @@ __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimPar
     } else if (!SPHERES_ONLY) {
         const Geom a = load_geom(B, i), b = load_geom(B, j);
         Contact c = detect_contact(box, a, b, ws);
-        if (c.hit == 2) return NARROW_DEFERRED;   // GJK overlap: EPA runs in the EPA tiers
-        if (ws.overflow) return 0;
-        if (c.hit) {
-            if (si == SH_BOX && sj == SH_BOX && !P.no_manifold) {
-                V3 pts[4];
-                double dep[4];
-                int m = box_box_manifold(box, a, b, c.normal, pts, dep, fault);
-                for (int k = 0; k < m; ++k) {
-                    double* o = out + 7 * k;
-                    o[0] = c.normal.x; o[1] = c.normal.y; o[2] = c.normal.z;
-                    o[3] = pts[k].x; o[4] = pts[k].y; o[5] = pts[k].z; o[6] = dep[k];
-                }
-                nc = (uint32_t)m;
-            }
-            if (nc == 0) {
-                out[0] = c.normal.x; out[1] = c.normal.y; out[2] = c.normal.z;
-                out[3] = c.point.x; out[4] = c.point.y; out[5] = c.point.z; out[6] = c.overlap;
-                nc = 1;
-            }
+        if (c.hit == 2) return NARROW_DEFERRED;   // GJK overlap: EPA (and a box pair's manifold) in the EPA tiers
+        if (c.hit) {   // an analytic contact: two boxes only ever reach GJK, so no manifold here
+            out[0] = c.normal.x; out[1] = c.normal.y; out[2] = c.normal.z;
+            out[3] = c.point.x; out[4] = c.point.y; out[5] = c.point.z; out[6] = c.overlap;
+            nc = 1;
         }
     }
     return nc;
@@ -1075,15 +1061,16 @@ template <bool SPHERES_ONLY>
 __global__ void __launch_bounds__(TPB) k_narrow(BodiesDev B, SimParams P, uint32_t* __restrict__ counters,
                                                 const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
                                                 uint32_t* __restrict__ ncont, double* __restrict__ ct,
-                                                uint32_t* __restrict__ epa_q, uint32_t epa_cap) {
+                                                uint32_t* __restrict__ epa_q, uint32_t epa_cap, EpaTets tets) {
     const uint32_t np = counters[CNT_P];
     uint32_t* fault = counters + CNT_FAULT;
     if (!SPHERES_ONLY && blockIdx.x == 0 && threadIdx.x == 0) counters[CNT_EPALONG] = 0u;   // the EPA tiers run after
     EpaWS ws{};
-    ws.defer = true;
+    ws.tets = tets;
     for (uint32_t p0 = blockIdx.x * blockDim.x; p0 < np; p0 += gridDim.x * blockDim.x) {
         const uint32_t p = p0 + threadIdx.x;
         uint32_t nc = 0;
+        ws.pair = p;
         if (p < np) nc = narrow_pair<SPHERES_ONLY>(B, P, cand_i[p], cand_j[p], ct + (size_t)p * 28, ws, fault);
         if (!SPHERES_ONLY) {
             const bool defer = p < np && nc == NARROW_DEFERRED;
@@ -1689,8 +1676,7 @@ __global__ void __launch_bounds__(64) k_mesh_lds(BodiesDev B, SimParams P, MeshD
                 const Box3& box = box_of(B, P, o);
                 Contact c = no_contact();
                 if (other.shape == SH_PLANE) {
-                    EpaWS ws{};   // the plane's path runs no EPA (and would stop after GJK)
-                    ws.defer = true;
+                    EpaWS ws{};   // the plane's path runs no GJK or EPA
                     c = detect_contact(box, other, tri, ws);
                 } else {
                     const V3 shift = min_image_shift(box, other.x, tri.x);   // convex_contact (epa.hpp:153-158)
@@ -3643,6 +3629,7 @@ struct ne_gpu_ctx {
     uint32_t *ex_flags = nullptr, *ex_off = nullptr;
     uint32_t *cand_i = nullptr, *cand_j = nullptr;
     uint32_t* epa_q = nullptr;        // convex pairs queued for EPA (pcap entries)
+    double* epa_tet = nullptr;        // GJK tetrahedra of overlapping convex pairs, by pair (24 doubles each, field-major, pcap stride)
     uint32_t* epa_long = nullptr;     // of those, the ones the first EPA launch left for the second (pcap entries)
     uint32_t* epa_long2 = nullptr;    // the ones the per-thread scratch tier left for the wave tier
     uint32_t* epa_long3 = nullptr;    // the ones the second LDS tier left for the scratch tier
@@ -4146,6 +4133,7 @@ int alloc_pairs(ne_gpu_ctx* c, uint32_t pcap) {
     HIP_TRY(c, dalloc(c->cand_i, pcap));
     HIP_TRY(c, dalloc(c->cand_j, pcap));
     HIP_TRY(c, dalloc(c->epa_q, pcap));
+    HIP_TRY(c, dalloc(c->epa_tet, 24 * (size_t)pcap));
     HIP_TRY(c, dalloc(c->epa_long, pcap));
     HIP_TRY(c, dalloc(c->epa_long2, pcap));
     HIP_TRY(c, dalloc(c->epa_long3, pcap));
@@ -4266,7 +4254,7 @@ void free_work(ne_gpu_ctx* c) {
     dfree(c->cell_of); dfree(c->cell_count); dfree(c->cell_start); dfree(c->sorted_idx); dfree(c->sorted);
     dfree(c->order_prev); dfree(c->xp);
     dfree(c->row_count); dfree(c->row_start); dfree(c->ex_flags); dfree(c->ex_off);
-    dfree(c->cand_i); dfree(c->cand_j); dfree(c->epa_q); dfree(c->epa_long); dfree(c->epa_long2); dfree(c->epa_long3); dfree(c->isp);
+    dfree(c->cand_i); dfree(c->cand_j); dfree(c->epa_q); dfree(c->epa_tet); dfree(c->epa_long); dfree(c->epa_long2); dfree(c->epa_long3); dfree(c->isp);
     dfree(c->wake_at); dfree(c->mint); dfree(c->wake_again); dfree(c->ncont); dfree(c->cons_off); dfree(c->ct);
     ConsDev& C = c->C;
     dfree(C.ci); dfree(C.cj); dfree(C.slot);
@@ -4447,21 +4435,23 @@ void enqueue_detect(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool 
         KTimer kt(c, NE_KERNEL_NARROW);
         if (c->spheres_only) {
             k_narrow<true><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                              c->epa_q, c->pcap);
+                                                              c->epa_q, c->pcap, EpaTets{c->epa_tet, c->pcap});
         } else {
+            const EpaTets tets{c->epa_tet, c->pcap};
             k_narrow<false><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont,
-                                                               c->ct, c->epa_q, c->pcap);
+                                                               c->ct, c->epa_q, c->pcap, tets);
+            // the EPA tiers start from k_narrow's GJK tetrahedra (tets, by pair)
             k_epa_lds<8, 12, 8><<<wave_blocks(c->hint_c), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j,
                                                                       c->ncont, c->ct, c->epa_q, CNT_EPAQ, c->pcap,
-                                                                      c->epa_long, CNT_EPALONG);
+                                                                      c->epa_long, CNT_EPALONG, tets);
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_EPALONG3, 0u, nullptr, 1u);
-            k_epa_lds<16, 24, 16><<<wave_blocks(c->hint_c / 4), 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j,
-                                                                          c->ncont, c->ct, c->epa_long, CNT_EPALONG,
-                                                                          c->pcap, c->epa_long3, CNT_EPALONG3);
+            k_epa_lds<16, 24, 16><<<wave_blocks(c->hint_c / 4), 64, 0, s>>>(
+                B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_long, CNT_EPALONG, c->pcap,
+                c->epa_long3, CNT_EPALONG3, tets);
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_EPALONG2, 0u, nullptr, 1u);
             k_epa_group<8, 36, 64, false><<<wave_blocks(c->hint_c / 8), 64, 0, s>>>(
                 B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_long3, CNT_EPALONG3, c->pcap,
-                c->epa_long2, CNT_EPALONG2, MeshSrc{});
+                c->epa_long2, CNT_EPALONG2, MeshSrc{}, tets);
             k_epa_wave<<<2048, 64, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_long2,
                                            CNT_EPALONG2, c->pcap, c->epa_big);
             k_snap_counter<<<1, 64, 0, s>>>(c->counters, CNT_EPABIG, CNT_EPABIG_PAIRS);
@@ -4490,7 +4480,7 @@ void enqueue_detect(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool 
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_MQLONG2, 0u, nullptr, 1u);
             k_epa_group<8, 36, 64, true><<<wave_blocks(c->hint_mq / 16), 64, 0, s>>>(
                 B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->mq_long, CNT_MQLONG, c->mqcap,
-                c->mq_long2, CNT_MQLONG2, MeshSrc{M, c->mq_pair, c->mq_tri, c->mhit, c->mct});
+                c->mq_long2, CNT_MQLONG2, MeshSrc{M, c->mq_pair, c->mq_tri, c->mhit, c->mct}, EpaTets{});
             k_mesh_wave<<<2048, 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri,
                                             c->mhit, c->mct, c->mq_long2, CNT_MQLONG2, c->epa_big);
             k_mesh_big<<<big_blocks(c), 64 * BIG_NW, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair, c->mq_tri, c->mhit,

@@ -254,11 +254,13 @@ __device__ inline Contact epa_result(const SupV* verts, const Face& best) {
     return c;
 }
 
+
 // src/epa.hpp:71-150.  Faces are compacted in place (kept faces in their
 // original order, then the new horizon faces), exactly the order the
 // reference's `keep` + push_back produces.  `verts` holds EPA_MAXV entries,
 // `faces` maxf; exceeding maxf or the horizon's capacity sets `overflow` and
-// stops (the result is then not used).
+// stops (the result is then not used).  (The overlap queries' per-thread
+// tier; the step's EPA tiers are in ne_epa_lds.hpp and ne_gpu.hip.)
 template <class Hz>
 __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, const SupV tet[4], SupV* verts,
                                       Face* faces, int maxf, int maxv, int max_iter, Hz& hz, bool& overflow,
@@ -316,31 +318,44 @@ __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, co
     return epa_result(verts, best);
 }
 
-// EPA workspace handed down the detect path: per-thread arrays with a linear
-// horizon.
-struct EpaWS {
-    SupV* verts;
-    Face* faces;
-    int maxf;
-    int maxv;
-    int max_iter;      // EPA iterations this launch runs; a run that needs more sets too_long
-    bool too_long;
-    HorizonList list;
-    bool overflow;
-    bool defer;      // stop after GJK: an overlapping convex pair reports hit == 2 (EPA runs later)
+// GJK tetrahedra of the overlapping convex pairs, by pair index: 24 doubles
+// each (v and d of four vertices), field-major so a wave's accesses coalesce.
+struct EpaTets {
+    double* d;
+    uint32_t stride;   // the pair capacity
+    __device__ void put(uint32_t slot, const SupD tet[4]) const {
+        for (int k = 0; k < 4; ++k) {
+            double* f = d + (size_t)(6 * k) * stride + slot;
+            f[0] = tet[k].v.x; f[stride] = tet[k].v.y; f[2 * (size_t)stride] = tet[k].v.z;
+            f[3 * (size_t)stride] = tet[k].d.x; f[4 * (size_t)stride] = tet[k].d.y; f[5 * (size_t)stride] = tet[k].d.z;
+        }
+    }
+    __device__ void get(uint32_t slot, SupD tet[4]) const {
+        for (int k = 0; k < 4; ++k) {
+            const double* f = d + (size_t)(6 * k) * stride + slot;
+            tet[k].v = V3{f[0], f[stride], f[2 * (size_t)stride]};
+            tet[k].d = V3{f[3 * (size_t)stride], f[4 * (size_t)stride], f[5 * (size_t)stride]};
+        }
+    }
 };
 
+// What the detect path needs for convex pairs: it stops after GJK (EPA runs
+// later, in the EPA tiers), and an overlapping pair reports hit == 2 and
+// leaves its tetrahedron in `tets` at index `pair` (when tets.d is set).
+struct EpaWS {
+    EpaTets tets;
+    uint32_t pair;
+};
+
+// src/epa.hpp:153-158 up to EPA: the minimum-image shift and GJK.
 __device__ inline Contact convex_contact(const Box3& box, const Geom& A, const Geom& B, EpaWS& ws) {
-    V3 shift = min_image_shift(box, A.x, B.x);
-    SupV tet[4];
+    const V3 shift = min_image_shift(box, A.x, B.x);
+    SupD tet[4];
     if (!gjk_tetra(A, B, shift, tet)) return no_contact();
-    if (ws.defer) {
-        Contact c = no_contact();
-        c.hit = 2;
-        return c;
-    }
-    return epa_contact(A, B, shift, tet, ws.verts, ws.faces, ws.maxf, ws.maxv, ws.max_iter, ws.list, ws.overflow,
-                       ws.too_long);
+    if (ws.tets.d) ws.tets.put(ws.pair, tet);
+    Contact c = no_contact();
+    c.hit = 2;
+    return c;
 }
 
 // src/narrowphase.hpp:34-81

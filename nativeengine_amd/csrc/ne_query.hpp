@@ -375,8 +375,7 @@ __device__ __forceinline__ Contact ovl_test(const BodiesDev& B, const Box3& box,
         const double reach = (Q.bR + rb) * (1.0 + 1e-5) + 1e-9;
         if (norm2(min_image(box, x - Q.g.x)) > reach * reach) return c;
     }
-    EpaWS ws{};
-    ws.defer = true;
+    EpaWS ws{};   // GJK only: overlapping convex pairs are queued for EPA
     return detect_contact(box, Q.g, load_geom(B, i), ws);
 }
 
