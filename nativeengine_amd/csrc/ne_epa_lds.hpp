@@ -18,12 +18,15 @@
 // are uniform, and only the final face's three are read back, to recompute
 // its witness points (the same support calls, so the same bits).
 //
-// Two such tiers run in turn: A (8 vertices, i.e. 4 iterations, 12 faces, 8
-// horizon entries; four waves a CU) over every queued pair, then B (16
-// vertices, 24 faces, 16 entries; two waves a CU) over the runs A could not
-// finish.  A run that outgrows B goes on to tier C (k_epa_group, below), then
-// to the one-wave tier (k_epa_wave) and the one-block tier (k_narrow_big);
-// each reruns it from the start, so every tier gives the same bits.
+// This is tier A (8 vertices, i.e. 4 iterations, 12 faces, 8 horizon
+// entries; four waves a CU), over every queued pair.  A run that outgrows it
+// goes on to the group tiers (k_epa_group, below: B with 4 lanes a run and
+// 20 iterations, C with 8 lanes and 32), then to the one-wave tier
+// (k_epa_wave) and the one-block tier (k_narrow_big); each restarts it from
+// its GJK tetrahedron, so every tier gives the same bits.  (Tier B as a
+// per-lane LDS tier of 16 vertices and 24 faces, two waves a CU, took 50.1
+// ms per 1M terrain step against 48.1 with groups of 4; tier A in groups of
+// 4 took 50.7.)
 //
 // Included by ne_gpu.hip inside its anonymous namespace, after k_narrow and
 // the mesh helpers (load_geom, box_of, wave_add1, CNT_*, MeshDev, mesh_pair,
@@ -38,10 +41,8 @@ struct EpaLaneLds {
     uint32_t fabc[MF][64];                       // a | b << 8 | c << 16
     uint16_t hor[MH][64];                        // u | w << 8
 };
-// Tier A: 4 iterations, 40 KiB a wave (four waves fill a CU's 160 KiB);
-// tier B: 12 iterations, 80 KiB a wave (two waves a CU).
+// Tier A: 4 iterations, 40 KiB a wave (four waves fill a CU's 160 KiB).
 static_assert(sizeof(EpaLaneLds<8, 12, 8>) == 40960, "tier A: four waves' workspaces fill a CU's LDS");
-static_assert(sizeof(EpaLaneLds<16, 24, 16>) == 81920, "tier B: two waves' workspaces fill a CU's LDS");
 
 template <int MV, int MF, int MH>
 struct EpaLane {
@@ -214,9 +215,9 @@ __global__ void __launch_bounds__(64) k_epa_lds(BodiesDev B, SimParams P, uint32
     }
 }
 
-// ---- Tier C: G lanes per run, for the long runs ----------------------------
+// ---- Tiers B and C: G lanes per run, for the long runs ---------------------
 //
-// Runs that outgrow tier B are few but long (curved against flat supports --
+// Runs that outgrow tier A are few but long (curved against flat supports --
 // sphere or capsule on a box -- refine their closest face for tens of
 // iterations over tens of faces).  A group of G lanes shares one run:
 //   - the first closest face is a min/first-index reduction over the group;
@@ -230,7 +231,7 @@ __global__ void __launch_bounds__(64) k_epa_lds(BodiesDev B, SimParams P, uint32
 //     entry, so the last P - N occurrences survive);
 //   - vertices, faces and the staged edges sit in the group's LDS; the
 //     support directions stay with the group's first lane.
-// Runs past MV - 4 iterations or MF faces go on to k_epa_wave.
+// Runs past MV - 4 iterations or MF faces go on to the next tier.
 
 template <int MV, int MF>
 struct EpaGroupLds {

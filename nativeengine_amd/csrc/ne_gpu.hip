@@ -4445,9 +4445,9 @@ void enqueue_detect(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool 
                                                                       c->ncont, c->ct, c->epa_q, CNT_EPAQ, c->pcap,
                                                                       c->epa_long, CNT_EPALONG, tets);
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_EPALONG3, 0u, nullptr, 1u);
-            k_epa_lds<16, 24, 16><<<wave_blocks(c->hint_c / 4), 64, 0, s>>>(
+            k_epa_group<4, 24, 40, false><<<wave_blocks(c->hint_c / 4), 64, 0, s>>>(
                 B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_long, CNT_EPALONG, c->pcap,
-                c->epa_long3, CNT_EPALONG3, tets);
+                c->epa_long3, CNT_EPALONG3, MeshSrc{}, tets);
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_EPALONG2, 0u, nullptr, 1u);
             k_epa_group<8, 36, 64, false><<<wave_blocks(c->hint_c / 8), 64, 0, s>>>(
                 B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->epa_long3, CNT_EPALONG3, c->pcap,
