@@ -6624,6 +6624,14 @@ int ne_gpu_epa_stats(ne_gpu_ctx* c, uint32_t* reruns, uint32_t* truncated) {
     return NE_OK;
 }
 
+int ne_gpu_epa_queues(ne_gpu_ctx* c, uint32_t out[8]) {
+    if (!c || !out) return NE_ERR_INVALID;
+    const uint32_t* h = c->h_counters;
+    const int slot[8] = {CNT_EPAQ, CNT_EPALONG, CNT_EPALONG3, CNT_EPALONG2, CNT_EPABIG, CNT_MQ, CNT_MQLONG, CNT_MQLONG2};
+    for (int k = 0; k < 8; ++k) out[k] = h[slot[k]];
+    return NE_OK;
+}
+
 int ne_gpu_use_graph(ne_gpu_ctx* c, int32_t enable) {
     if (!c) return NE_ERR_INVALID;
     c->use_graph = enable != 0;

@@ -34,7 +34,7 @@ EXPORTS = [
     "ne_gpu_rollout", "ne_gpu_rollout_backward", "ne_gpu_set_joints", "ne_gpu_set_meshes", "ne_gpu_mesh_bvh",
     "ne_gpu_get_sleep_store", "ne_gpu_save_state", "ne_gpu_load_state", "ne_gpu_save_state_noalloc",
     "ne_gpu_set_events", "ne_gpu_get_events", "ne_gpu_raycast", "ne_gpu_raycast_all", "ne_gpu_overlap",
-    "ne_gpu_set_general_joints", "ne_gpu_get_general_joints",
+    "ne_gpu_set_general_joints", "ne_gpu_get_general_joints", "ne_gpu_epa_queues",
 ]
 
 KERNELS = {"integrate": 1, "cell_count": 2, "cell_fill": 3, "cand_count": 4, "cand_write": 5, "narrow": 6,
@@ -112,6 +112,7 @@ def load_library(path: str = LIB_PATH):
     lib.ne_gpu_set_worlds.argtypes = [vp, C.c_uint32, u32p, dp, dp, dp]
     lib.ne_gpu_use_graph.argtypes = [vp, C.c_int32]
     lib.ne_gpu_epa_stats.argtypes = [vp, u32p, u32p]
+    lib.ne_gpu_epa_queues.argtypes = [vp, u32p]
     lib.ne_gpu_set_joints.argtypes = [vp, C.c_uint32, vp]
     lib.ne_gpu_rollout.argtypes = [vp, vp, dp, dp, C.c_int32, C.c_int32, vp]
     lib.ne_gpu_rollout_backward.argtypes = [vp, vp, vp, dp, dp]
@@ -238,6 +239,15 @@ class GpuWorld:
         a, b = C.c_uint32(), C.c_uint32()
         self._check(self.lib.ne_gpu_epa_stats(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    EPA_QUEUES = ("overlapping_pairs", "past_tier_a", "past_tier_b", "past_tier_c", "block_tier", "mesh_candidates",
+                  "mesh_past_tier_a", "mesh_past_group_tier")
+
+    def epa_queues(self):
+        """The last step's EPA tier queue lengths (ne_gpu_epa_queues), by name."""
+        out = np.zeros(8, np.uint32)
+        self._check(self.lib.ne_gpu_epa_queues(self.h, _abi.uptr(out)))
+        return dict(zip(self.EPA_QUEUES, map(int, out)))
 
     def rollout(self, v0=None, mu=None, e=None, nsteps: int = 120, every: int = 16):
         """Differentiable rollout from the current state (velocities v0, n x 3,

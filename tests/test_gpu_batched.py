@@ -92,6 +92,7 @@ def test_epa_blowup_pair_matches_through_the_large_workspace(gpu_available, orac
     g.step(1)
     reruns, trunc = g.epa_stats()
     assert reruns >= 1 and trunc == 0
+    assert g.epa_queues()["block_tier"] >= 1   # the run went all the way to the one-block tier
     cg, co = g.contacts(), o.contacts()
     for f in ("i", "j", "normal", "point", "depth", "impulse"):
         assert np.array_equal(cg[f], co[f]), f
