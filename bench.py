@@ -323,6 +323,7 @@ def scene_run(s, run: int, warmup: int, k: int, label: str, marks=(), after=None
     g.time_kernel(None)
     last = g.counters()
     reruns, trunc = g.epa_stats()
+    epa_queues = g.epa_queues()
     fl = g.body_aux()[0] if s.params.sleep_enabled else None
     after_out = after(g) if after else None
     g.close()
@@ -372,6 +373,7 @@ def scene_run(s, run: int, warmup: int, k: int, label: str, marks=(), after=None
                       "islands": last["islands"], "largest_island": last["max_island"]},
         "epa_large_workspace_reruns_last_step": reruns,
         "epa_truncated_last_step": trunc,
+        "epa_tier_queues_last_step": epa_queues,
         "steps_with_epa_at_face_cap": capped_steps,
         "step_ms_at": {t: round(step_ms[t - 1], 3) for t in marks},
     }
