@@ -710,7 +710,7 @@ def _loop_sample(g, b, sample, loop_steps):
 BATCH_CALL_STEPS = 10
 
 
-def batched_bench(worlds: int, loop_worlds: int, loop_steps: int):
+def batched_bench(worlds: int, loop_worlds: int, loop_steps: int, graph: bool = True):
     """Batched world-steps/s over the whole 300-step run of the batch (HIP
     graph replay; the autotuner runs after step 30, from a snapshot it
     restores, outside the timing), against looping the plain single-world
@@ -743,6 +743,8 @@ def batched_bench(worlds: int, loop_worlds: int, loop_steps: int):
         if tune is None:
             tune = g.autotune(steps=3)
             log(f"autotuner: {tune}")
+            if not graph:   # kernel by kernel (what a kernel trace profiles), whatever the autotuner chose
+                g.use_graph(False)
         loop_ms[chunk_end] = _loop_sample(g, b, sample, loop_steps)
         log(f"batched: step {done}, {timed_s * 1e3 / done:.2f} ms/step so far; loop {loop_ms[chunk_end]:.3f} ms "
             f"per world-step")
@@ -766,6 +768,7 @@ def batched_bench(worlds: int, loop_worlds: int, loop_steps: int):
                        f"at steps 30, 150 and 300, {loop_steps} steps each",
         "vs_loop": batched_wps / loop_wps,
         "autotune": tune,
+        "graph_replay": bool(graph and tune and tune.get("graph")),
         "calls_with_epa_at_face_cap": capped,
         "epa_large_workspace_reruns_last_call": reruns,
         "epa_truncated_last_call": trunc,
@@ -941,6 +944,9 @@ def main():
     ap.add_argument("--worlds", type=int, default=1024)
     ap.add_argument("--loop-worlds", type=int, default=128)
     ap.add_argument("--loop-steps", type=int, default=5)
+    ap.add_argument("--batched-no-graph", action="store_true",
+                    help="step the batch kernel by kernel after the autotuner instead of replaying HIP graphs "
+                         "(for rocprofv3 kernel traces)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=65536, help="bodies in the CPU's contact-phase sample")
     ap.add_argument("--cpu-timeout", type=float, default=90.0)
@@ -1008,7 +1014,7 @@ def main():
             f"(reference {c1['reference_energy_drift_rel']:.3e}), bitwise {c1['bitwise_equal_to_reference']}")
         out["gas4096"] = c1
     if not args.no_batched:
-        batched = batched_bench(args.worlds, args.loop_worlds, args.loop_steps)
+        batched = batched_bench(args.worlds, args.loop_worlds, args.loop_steps, graph=not args.batched_no_graph)
         log(f"batched: {batched['value']:.4g} world-steps/s, {batched['ms_per_step']:.3f} ms/step, "
             f"{batched['vs_loop']:.1f}x the loop")
         out["batched"] = batched

@@ -3629,7 +3629,7 @@ struct ne_gpu_ctx {
     uint32_t *ex_flags = nullptr, *ex_off = nullptr;
     uint32_t *cand_i = nullptr, *cand_j = nullptr;
     uint32_t* epa_q = nullptr;        // convex pairs queued for EPA (pcap entries)
-    double* epa_tet = nullptr;        // GJK tetrahedra of overlapping convex pairs, by pair (24 doubles each, field-major, pcap stride)
+    double* epa_tet = nullptr;        // GJK tetrahedra of overlapping convex pairs, by pair (24 doubles each)
     uint32_t* epa_long = nullptr;     // of those, the ones the first EPA launch left for the second (pcap entries)
     uint32_t* epa_long2 = nullptr;    // the ones the per-thread scratch tier left for the wave tier
     uint32_t* epa_long3 = nullptr;    // the ones the second LDS tier left for the scratch tier
@@ -4435,9 +4435,9 @@ void enqueue_detect(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool 
         KTimer kt(c, NE_KERNEL_NARROW);
         if (c->spheres_only) {
             k_narrow<true><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct,
-                                                              c->epa_q, c->pcap, EpaTets{c->epa_tet, c->pcap});
+                                                              c->epa_q, c->pcap, EpaTets{c->epa_tet});
         } else {
-            const EpaTets tets{c->epa_tet, c->pcap};
+            const EpaTets tets{c->epa_tet};
             k_narrow<false><<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont,
                                                                c->ct, c->epa_q, c->pcap, tets);
             // the EPA tiers start from k_narrow's GJK tetrahedra (tets, by pair)

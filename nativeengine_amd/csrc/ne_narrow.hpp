@@ -319,22 +319,25 @@ __device__ inline Contact epa_contact(const Geom& A, const Geom& B, V3 shift, co
 }
 
 // GJK tetrahedra of the overlapping convex pairs, by pair index: 24 doubles
-// each (v and d of four vertices), field-major so a wave's accesses coalesce.
+// each (v and d of four vertices), contiguous, so the EPA tiers -- which
+// take the pairs in queue order, scattered over the pair list -- read one
+// and a half cache lines a pair.
 struct EpaTets {
     double* d;
-    uint32_t stride;   // the pair capacity
-    __device__ void put(uint32_t slot, const SupD tet[4]) const {
+    __device__ void put(uint32_t pair, const SupD tet[4]) const {
+        double2* f = reinterpret_cast<double2*>(d + (size_t)pair * 24);
         for (int k = 0; k < 4; ++k) {
-            double* f = d + (size_t)(6 * k) * stride + slot;
-            f[0] = tet[k].v.x; f[stride] = tet[k].v.y; f[2 * (size_t)stride] = tet[k].v.z;
-            f[3 * (size_t)stride] = tet[k].d.x; f[4 * (size_t)stride] = tet[k].d.y; f[5 * (size_t)stride] = tet[k].d.z;
+            f[3 * k] = double2{tet[k].v.x, tet[k].v.y};
+            f[3 * k + 1] = double2{tet[k].v.z, tet[k].d.x};
+            f[3 * k + 2] = double2{tet[k].d.y, tet[k].d.z};
         }
     }
-    __device__ void get(uint32_t slot, SupD tet[4]) const {
+    __device__ void get(uint32_t pair, SupD tet[4]) const {
+        const double2* f = reinterpret_cast<const double2*>(d + (size_t)pair * 24);
         for (int k = 0; k < 4; ++k) {
-            const double* f = d + (size_t)(6 * k) * stride + slot;
-            tet[k].v = V3{f[0], f[stride], f[2 * (size_t)stride]};
-            tet[k].d = V3{f[3 * (size_t)stride], f[4 * (size_t)stride], f[5 * (size_t)stride]};
+            const double2 a = f[3 * k], b = f[3 * k + 1], c = f[3 * k + 2];
+            tet[k].v = V3{a.x, a.y, b.x};
+            tet[k].d = V3{b.y, c.x, c.y};
         }
     }
 };
