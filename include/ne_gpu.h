@@ -499,9 +499,9 @@ int ne_gpu_epa_stats(ne_gpu_ctx *ctx, uint32_t *reruns, uint32_t *truncated);
  * tier A, one lane per run); out[1]: runs past tier A (into tier B, 4 lanes
  * a run); out[2]: past B (into C, 8 lanes); out[3]: past C (into the
  * one-wave tier); out[4]: runs rerun by the one-block tier (pairs and mesh
- * candidates); out[5]: mesh triangle candidates (GJK, then tier A); out[6]:
- * mesh runs past tier A (into the group tier); out[7]: mesh runs past the
- * group tier (into the one-wave tier). */
+ * candidates); out[5]: mesh triangle candidates GJK found overlapping (each
+ * starts in the mesh tier A); out[6]: mesh runs past tier A (into the group
+ * tier); out[7]: mesh runs past the group tier (into the one-wave tier). */
 int ne_gpu_epa_queues(ne_gpu_ctx *ctx, uint32_t out[8]);
 
 /* Replays each step from a captured HIP graph (enable = 1) instead of

@@ -134,7 +134,8 @@ enum CounterSlot : int {
     CNT_EV = 24,         // facade events pass: overlapping pairs of the post-step state
     CNT_EPALONG3 = 25,   // convex pairs the second LDS EPA tier left for the group tier
     CNT_MQLONG2 = 26,    // mesh triangle candidates the group tier left for the one-wave tier
-    CNT_N = 27
+    CNT_MQEPA = 27,      // mesh triangle candidates GJK found overlapping (queued for EPA)
+    CNT_N = 28
 };
 
 constexpr int TPB = 256;
@@ -1646,24 +1647,22 @@ constexpr int PV_STRIDE = 19;   // doubles per thread for a prism's vertices (18
 
 #include "ne_epa_lds.hpp"
 
-// detectContact(other, prism) per triangle candidate: the plane's analytic
-// path, else GJK and the LDS EPA tier A (k_epa_lds's workspace, beside the
-// prism's six vertices: three waves a CU).  Runs that outgrow the workspace
-// go to `long_q` for k_mesh_wave, which reruns them from the start.
-__global__ void __launch_bounds__(64) k_mesh_lds(BodiesDev B, SimParams P, MeshDev M, uint32_t* __restrict__ counters,
+// detectContact(other, prism) per triangle candidate up to EPA: the
+// plane's analytic path, else GJK; candidates GJK finds overlapping are
+// queued for the EPA tiers (`epa_q`, counted in CNT_MQEPA).  Without the EPA
+// workspace this kernel runs several waves a SIMD over the many candidates
+// that do not touch (the prism's six vertices in LDS, 9.5 KiB a block).
+__global__ void __launch_bounds__(64) k_mesh_gjk(BodiesDev B, SimParams P, MeshDev M, uint32_t* __restrict__ counters,
                                                  const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
                                                  const uint32_t* __restrict__ mq_pair, const int32_t* __restrict__ mq_tri,
                                                  uint8_t* __restrict__ mhit, double* __restrict__ mct,
-                                                 uint32_t* __restrict__ long_q) {
-    __shared__ EpaLaneLds<8, 12, 8> lds;
+                                                 uint32_t* __restrict__ epa_q) {
     __shared__ double lds_pv[64 * PV_STRIDE];
-    EpaLane<8, 12, 8> E{lds, (int)threadIdx.x};
     double* pv = lds_pv + threadIdx.x * PV_STRIDE;
     const uint32_t nq = counters[CNT_MQ];
-    V3 dirs[8];
     for (uint32_t u0 = blockIdx.x * 64u; u0 < nq; u0 += gridDim.x * 64u) {
         const uint32_t q = u0 + threadIdx.x;
-        bool defer = false;
+        bool overlap = false;
         if (q < nq) {
             const uint32_t p = mq_pair[q];
             uint32_t o, m;
@@ -1673,22 +1672,55 @@ __global__ void __launch_bounds__(64) k_mesh_lds(BodiesDev B, SimParams P, MeshD
             uint8_t hit = 0;
             if (mesh_prism(B, M, m, mq_tri[q], pv, tri)) {
                 const Geom other = load_geom(B, o);
-                const Box3& box = box_of(B, P, o);
-                Contact c = no_contact();
-                if (other.shape == SH_PLANE) {
-                    EpaWS ws{};   // the plane's path runs no GJK or EPA
-                    c = detect_contact(box, other, tri, ws);
-                } else {
-                    const V3 shift = min_image_shift(box, other.x, tri.x);   // convex_contact (epa.hpp:153-158)
-                    SupD tet[4];
-                    if (gjk_tetra(other, tri, shift, tet) && !epa_lane(E, other, tri, shift, tet, dirs, c)) defer = true;
-                }
-                if (!defer && c.hit) {
+                EpaWS ws{};   // GJK only
+                const Contact c = detect_contact(box_of(B, P, o), other, tri, ws);
+                if (c.hit == 2) {
+                    overlap = true;
+                } else if (c.hit) {   // the plane's analytic contact
                     store_contact(mct + 7 * (size_t)q, c, flip);
                     hit = 1;
                 }
             }
-            mhit[q] = hit;
+            mhit[q] = hit;   // an overlapping candidate's EPA tier writes it
+        }
+        const uint32_t slot = wave_add1(counters, CNT_MQEPA, overlap);
+        if (overlap) epa_q[slot] = q;   // at most one entry per candidate: fits the candidate capacity
+    }
+}
+
+// EPA tier A for the queued mesh candidates (k_epa_lds's workspace beside
+// the prism's six vertices: three waves a CU), from GJK on the same inputs
+// again.  Runs that outgrow the workspace go to `long_q` for the group tier,
+// which reruns them from the start.
+__global__ void __launch_bounds__(64) k_mesh_lds(BodiesDev B, SimParams P, MeshDev M, uint32_t* __restrict__ counters,
+                                                 const uint32_t* __restrict__ cand_i, const uint32_t* __restrict__ cand_j,
+                                                 const uint32_t* __restrict__ mq_pair, const int32_t* __restrict__ mq_tri,
+                                                 uint8_t* __restrict__ mhit, double* __restrict__ mct,
+                                                 const uint32_t* __restrict__ epa_q, uint32_t* __restrict__ long_q) {
+    __shared__ EpaLaneLds<8, 12, 8> lds;
+    __shared__ double lds_pv[64 * PV_STRIDE];
+    EpaLane<8, 12, 8> E{lds, (int)threadIdx.x};
+    double* pv = lds_pv + threadIdx.x * PV_STRIDE;
+    const uint32_t nq = counters[CNT_MQEPA];
+    V3 dirs[8];
+    for (uint32_t u0 = blockIdx.x * 64u; u0 < nq; u0 += gridDim.x * 64u) {
+        const uint32_t u = u0 + threadIdx.x;
+        bool defer = false;
+        const uint32_t q = u < nq ? epa_q[u] : 0u;
+        if (u < nq) {
+            const uint32_t p = mq_pair[q];
+            uint32_t o, m;
+            bool flip;
+            mesh_pair(B, cand_i[p], cand_j[p], o, m, flip);
+            Geom tri;
+            mesh_prism(B, M, m, mq_tri[q], pv, tri);   // non-degenerate: it was queued
+            const Geom other = load_geom(B, o);
+            const V3 shift = min_image_shift(box_of(B, P, o), other.x, tri.x);   // convex_contact (epa.hpp:153-158)
+            SupD tet[4];
+            Contact c = no_contact();
+            if (gjk_tetra(other, tri, shift, tet) && !epa_lane(E, other, tri, shift, tet, dirs, c)) defer = true;
+            if (!defer && c.hit) store_contact(mct + 7 * (size_t)q, c, flip);
+            if (!defer) mhit[q] = c.hit ? 1 : 0;
         }
         const uint32_t slot = wave_add1(counters, CNT_MQLONG, defer);
         if (defer) long_q[slot] = q;   // at most one entry per candidate: fits the candidate capacity
@@ -3740,6 +3772,7 @@ struct ne_gpu_ctx {
     int32_t* mq_tri = nullptr;
     uint32_t* mq_long = nullptr;                   // candidates the first mesh launch left for the group tier
     uint32_t* mq_long2 = nullptr;                  // and the ones the group tier left for the one-wave tier
+    uint32_t* mq_epa = nullptr;                    // candidates GJK found overlapping, for EPA tier A
     uint8_t* mhit = nullptr;
     double* mct = nullptr;                         // 7 doubles per candidate: normal, point, depth
     uint32_t mqcap = 0, mq_pcap = 0, hint_mq = 4096;
@@ -4185,6 +4218,7 @@ int alloc_mesh_cands(ne_gpu_ctx* c, uint32_t cap) {
     HIP_TRY(c, dalloc(c->mq_tri, cap));
     HIP_TRY(c, dalloc(c->mq_long, cap));
     HIP_TRY(c, dalloc(c->mq_long2, cap));
+    HIP_TRY(c, dalloc(c->mq_epa, cap));
     HIP_TRY(c, dalloc(c->mhit, cap));
     HIP_TRY(c, dalloc(c->mct, (size_t)cap * 7));
     return NE_OK;
@@ -4198,7 +4232,7 @@ void free_meshes(ne_gpu_ctx* c) {
 }
 
 void free_mesh_work(ne_gpu_ctx* c) {
-    dfree(c->mq_cnt); dfree(c->mq_off); dfree(c->mq_pair); dfree(c->mq_tri); dfree(c->mq_long); dfree(c->mq_long2); dfree(c->mhit);
+    dfree(c->mq_cnt); dfree(c->mq_off); dfree(c->mq_pair); dfree(c->mq_tri); dfree(c->mq_long); dfree(c->mq_long2); dfree(c->mq_epa); dfree(c->mhit);
     dfree(c->mct);
     c->mqcap = c->mq_pcap = 0;
 }
@@ -4475,8 +4509,11 @@ void enqueue_detect(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool 
             k_mesh_emit<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_off,
                                                                c->mqcap, c->mq_pair, c->mq_tri);
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_MQLONG, 0u, nullptr, 1u);
-            k_mesh_lds<<<wave_blocks(c->hint_mq), 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
-                                                               c->mq_tri, c->mhit, c->mct, c->mq_long);
+            k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_MQEPA, 0u, nullptr, 1u);
+            k_mesh_gjk<<<wave_blocks(c->hint_mq), 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
+                                                               c->mq_tri, c->mhit, c->mct, c->mq_epa);
+            k_mesh_lds<<<wave_blocks(c->hint_mq / 4), 64, 0, s>>>(B, P, M, c->counters, c->cand_i, c->cand_j, c->mq_pair,
+                                                                   c->mq_tri, c->mhit, c->mct, c->mq_epa, c->mq_long);
             k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_MQLONG2, 0u, nullptr, 1u);
             k_epa_group<8, 36, 64, true><<<wave_blocks(c->hint_mq / 16), 64, 0, s>>>(
                 B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->ct, c->mq_long, CNT_MQLONG, c->mqcap,
@@ -6627,7 +6664,7 @@ int ne_gpu_epa_stats(ne_gpu_ctx* c, uint32_t* reruns, uint32_t* truncated) {
 int ne_gpu_epa_queues(ne_gpu_ctx* c, uint32_t out[8]) {
     if (!c || !out) return NE_ERR_INVALID;
     const uint32_t* h = c->h_counters;
-    const int slot[8] = {CNT_EPAQ, CNT_EPALONG, CNT_EPALONG3, CNT_EPALONG2, CNT_EPABIG, CNT_MQ, CNT_MQLONG, CNT_MQLONG2};
+    const int slot[8] = {CNT_EPAQ, CNT_EPALONG, CNT_EPALONG3, CNT_EPALONG2, CNT_EPABIG, CNT_MQEPA, CNT_MQLONG, CNT_MQLONG2};
     for (int k = 0; k < 8; ++k) out[k] = h[slot[k]];
     return NE_OK;
 }

@@ -240,7 +240,7 @@ class GpuWorld:
         self._check(self.lib.ne_gpu_epa_stats(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    EPA_QUEUES = ("overlapping_pairs", "past_tier_a", "past_tier_b", "past_tier_c", "block_tier", "mesh_candidates",
+    EPA_QUEUES = ("overlapping_pairs", "past_tier_a", "past_tier_b", "past_tier_c", "block_tier", "mesh_overlapping",
                   "mesh_past_tier_a", "mesh_past_group_tier")
 
     def epa_queues(self):

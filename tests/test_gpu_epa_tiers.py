@@ -65,5 +65,5 @@ def test_terrain_runs_reach_the_mesh_group_tier(gpu_available, oracle_lib):
     s = scene.terrain_pile(60, seed=7, shapes="mixed")
     _say(f"case: terrain pile, {s.n} bodies, 150 steps")
     seen = _step_both(s, 150, "terrain60")
-    assert seen["mesh_candidates"] > 0
+    assert seen["mesh_overlapping"] > 0
     assert seen["mesh_past_tier_a"] > 0, "no prism run reached the group tier"
