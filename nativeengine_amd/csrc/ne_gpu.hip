@@ -2523,7 +2523,6 @@ struct WfDev {
     uint32_t* flag;        // per constraint: 1 if it starts a task of a large island
     uint32_t* tid;         // per constraint: exclusive scan of flag (its task index)
     uint32_t* task_k;      // per task: first constraint
-    uint32_t* next;        // per task and side (0 = body i, 1 = body j): successor task, WF_WRAP | first, WF_NONE
     uint32_t* pend0;       // per task: predecessors in pass 0
     uint32_t* pend;        // per visit (pass * T + task): predecessors not yet run
     uint32_t* fr;          // visits in the order their rounds ran them
@@ -2534,7 +2533,9 @@ struct WfDev {
     uint32_t* inc_start;   // per body: start of its slice of inc_list
     uint32_t* inc_list;    // per movable body: its tasks
     uint32_t* task_n;      // per task: constraints in it
-    uint4* task4;          // per task: first constraint, constraints, body i, body j (one load per visit)
+    uint4* task4;          // per task, 32 bytes: (first constraint, constraints, body i, body j), then the
+                           // successor task per side (0 = body i, 1 = body j: a task, WF_WRAP | first, or
+                           // WF_NONE) -- one cache line a visit
     double* rec;           // per constraint of a task: n, rA, rB, restBias, Kn, mu, nImp, tImp, overlap (16 doubles)
     double* bp;            // per body: v, w, q, invI, invMass (16 doubles), the sweep's working copy
 };
@@ -2608,7 +2609,8 @@ __global__ void __launch_bounds__(TPB) k_wf_tasks(BodiesDev B, const uint32_t* _
         uint32_t k1 = k + 1;
         while (k1 < nc && C.ci[k1] == i && C.cj[k1] == j) ++k1;
         W.task_n[t] = k1 - k;
-        W.task4[t] = make_uint4(k, k1 - k, i, j);
+        W.task4[2 * (size_t)t] = make_uint4(k, k1 - k, i, j);
+        W.task4[2 * (size_t)t + 1] = make_uint4(WF_NONE, WF_NONE, 0u, 0u);
         for (uint32_t q = k; q < k1; ++q) {   // the sweep's record of constraint q
             double2* r = reinterpret_cast<double2*>(W.rec + (size_t)q * WF_REC);
             r[0] = double2{C.n0[q], C.n1[q]};
@@ -2645,7 +2647,7 @@ __global__ void __launch_bounds__(TPB) k_wf_link(uint32_t n, ConsDev C, WfDev W)
         for (uint32_t u = 0; u < m; ++u) {
             const uint32_t t = a[u];
             const uint32_t side = C.ci[W.task_k[t]] == x ? 0u : 1u;
-            W.next[2 * t + side] = (u + 1 < m) ? a[u + 1] : (WF_WRAP | a[0]);
+            reinterpret_cast<uint32_t*>(W.task4 + 2 * (size_t)t + 1)[side] = (u + 1 < m) ? a[u + 1] : (WF_WRAP | a[0]);
             if (u > 0) atomicAdd(&W.pend0[t], 1u);
         }
     }
@@ -2761,30 +2763,39 @@ constexpr uint64_t WF_WIDE_ROUND = 16384;
 
 // Run visit v, release its successors: out[side] = the successor made ready
 // (WF_NONE: none), local[side] = whether it may run in this block.
+//
+// The successors are released before the visit runs, so the atomics' round
+// trips overlap its loads and arithmetic: a successor made ready here runs
+// only at this block's next level (after a barrier, so after this visit's
+// writes) or in a later launch, never beside this visit.
 __device__ __forceinline__ void wf_run_visit(BodiesDev& B, const SimParams& P, const ConsDev& C, const WfDev& W,
                                              uint32_t T, uint32_t npass, uint32_t v, uint32_t tag, uint32_t out[2],
                                              bool local[2], uint64_t* __restrict__ prev_key,
                                              double* __restrict__ prev_imp) {
     const uint32_t pass = v / T, t = v - pass * T;
-    const uint4 tk = W.task4[t];
-    const uint2 nxt = reinterpret_cast<const uint2*>(W.next)[t];
-    wf_visit(B, P, C, W, pass, tk.z, tk.w, tk.x, tk.x + tk.y, prev_key, prev_imp);
+    const uint4 tk = W.task4[2 * (size_t)t], nxt = W.task4[2 * (size_t)t + 1];
+    uint32_t old[2] = {0u, 0u};
     for (int side = 0; side < 2; ++side) {
         out[side] = WF_NONE;
         local[side] = false;
         const uint32_t nx = side ? nxt.y : nxt.x;
         if (nx == WF_NONE) continue;
-        uint32_t w;
         if (nx & WF_WRAP) {
             if (pass + 1 >= npass) continue;
-            w = (pass + 1) * T + (nx & ~WF_WRAP);
+            out[side] = (pass + 1) * T + (nx & ~WF_WRAP);
         } else {
-            w = pass * T + nx;
+            out[side] = pass * T + nx;
         }
-        const uint32_t old = atomicAdd(&W.pend[w], (tag << 2) - 1u);
-        if ((old & 3u) != 1u) continue;   // another predecessor is still to run
-        const uint32_t other = old >> 2;
-        out[side] = w;
+        old[side] = atomicAdd(&W.pend[out[side]], (tag << 2) - 1u);
+    }
+    wf_visit(B, P, C, W, pass, tk.z, tk.w, tk.x, tk.x + tk.y, prev_key, prev_imp);
+    for (int side = 0; side < 2; ++side) {
+        if (out[side] == WF_NONE) continue;
+        if ((old[side] & 3u) != 1u) {   // another predecessor is still to run
+            out[side] = WF_NONE;
+            continue;
+        }
+        const uint32_t other = old[side] >> 2;
         local[side] = (other >> 12) != (tag >> 12) || other == tag;
     }
 }
@@ -4331,7 +4342,7 @@ void free_wf(ne_gpu_ctx* c) {
     dfree(c->wf_args);
     if (c->h_wf_args) { (void)hipHostFree(c->h_wf_args); c->h_wf_args = nullptr; }
     WfDev& W = c->wf;
-    dfree(W.flag); dfree(W.tid); dfree(W.task_k); dfree(W.next); dfree(W.pend0); dfree(W.pend); dfree(W.fr);
+    dfree(W.flag); dfree(W.tid); dfree(W.task_k); dfree(W.pend0); dfree(W.pend); dfree(W.fr);
     dfree(W.rcnt); dfree(W.roff); dfree(W.done); dfree(W.inc_cnt); dfree(W.inc_start); dfree(W.inc_list);
     dfree(W.task_n); dfree(W.task4); dfree(W.rec); dfree(W.bp);
     c->wf_tcap = c->wf_ccap = c->wf_ncap = c->wf_vcap_pass = 0;
@@ -4635,8 +4646,7 @@ int ensure_wf(ne_gpu_ctx* c, uint32_t T, uint32_t npass) {
         const uint32_t cap = T + T / 2 + 1024;
         HIP_TRY(c, dalloc(W.task_k, cap));
         HIP_TRY(c, dalloc(W.task_n, cap));
-        HIP_TRY(c, dalloc(W.task4, cap));
-        HIP_TRY(c, dalloc(W.next, 2 * (size_t)cap));
+        HIP_TRY(c, dalloc(W.task4, 2 * (size_t)cap));
         HIP_TRY(c, dalloc(W.pend0, cap));
         HIP_TRY(c, dalloc(W.inc_list, 2 * (size_t)cap));
         c->wf_tcap = cap;
@@ -4684,7 +4694,6 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     r0 = ensure_wf(c, T, npass);
     if (r0 != NE_OK) return r0;
     k_fill_u32<<<gs_blocks(c->n), TPB, 0, s>>>(W.inc_cnt, 0u, nullptr, c->n);
-    k_fill_u32<<<gs_blocks(2 * (size_t)T), TPB, 0, s>>>(W.next, WF_NONE, nullptr, 2 * T);
     k_fill_u32<<<gs_blocks(T), TPB, 0, s>>>(W.pend0, 0u, nullptr, T);
     uint32_t cleared = std::min<uint32_t>(WF_MAX_ROUNDS + 2, 2 * c->wf_rounds_hint + 256);
     k_fill_u32<<<gs_blocks(cleared), TPB, 0, s>>>(W.rcnt, 0u, nullptr, cleared);
