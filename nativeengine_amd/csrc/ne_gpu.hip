@@ -123,6 +123,7 @@ enum CounterSlot : int {
     CNT_EPATRUNC = 12,   // EPA runs stopped at the largest workspace (this call)
     CNT_EPARERUN = 13,   // large-workspace EPA reruns (this call)
     CNT_EPAQ = 14,       // convex pairs queued for EPA (this step)
+    CNT_WFC = 15,        // constraints of the large islands (this step)
     CNT_MQRAW = 16,      // mesh triangle candidates before clamping (this step)
     CNT_MQ = 17,         // mesh triangle candidates (clamped)
     CNT_WFT = 18,        // tasks (contact pairs) of the large islands (this step)
@@ -1870,7 +1871,7 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
                                                     const uint64_t* __restrict__ prev_key, const double* __restrict__ prev_imp,
                                                     const uint32_t* __restrict__ prev_n, uint32_t* __restrict__ parent,
                                                     uint32_t* __restrict__ claimed, const uint32_t* __restrict__ mq_off,
-                                                    const double* __restrict__ mct) {
+                                                    const double* __restrict__ mct, bool link) {
     const uint32_t np = counters_in[CNT_P];
     const uint32_t pn = *prev_n;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
@@ -1893,7 +1894,7 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
         const V3 wa = ld3(B.w0, B.w1, B.w2, i), wb = ld3(B.w0, B.w1, B.w2, j);
         const Qt qa = ldq(B, i), qb = ldq(B, j);
         const V3 iia = ld3(B.ii0, B.ii1, B.ii2, i), iib = ld3(B.ii0, B.ii1, B.ii2, j);
-        uf_union(B, parent, i, j);   // islands: join the two bodies when both can move
+        if (link) uf_union(B, parent, i, j);   // islands: join the two bodies when both can move
         const uint64_t key = ((uint64_t)i << 32) | (uint64_t)j;   // world.cpp:34-36
         // warm cache: previous step's constraints are sorted by (key, slot)
         uint32_t lo = 0;
@@ -2518,6 +2519,14 @@ constexpr uint32_t WF_MAX_ROUNDS = 1u << 16;
 constexpr int WF_THREADS = 64;
 constexpr uint32_t WF_PASS_MAX = 66;        // warm start + up to 64 iterations + correction
 constexpr uint32_t WF_ISLAND = STAGE_CONS;  // islands above this many constraints: the dataflow solve (eager) or k_solve_large (captured)
+// When the large islands hold at least WF_ALL_NUM / WF_ALL_DEN of the
+// constraints, the next eager steps put every constraint in the dataflow and
+// skip building islands (union-find, island ids and lists, the per-island
+// solvers); every WF_ALL_RECHECK-th step builds them again to check.  The
+// small islands' visits widen the dataflow's early rounds but add no depth
+// beyond the large islands'.
+constexpr uint32_t WF_ALL_NUM = 3, WF_ALL_DEN = 4;
+constexpr uint32_t WF_ALL_RECHECK = 32;
 
 struct WfDev {
     uint32_t* flag;        // per constraint: 1 if it starts a task of a large island
@@ -2584,15 +2593,31 @@ __global__ void __launch_bounds__(TPB) k_wf_unpack_bodies(BodiesDev B, const dou
     if (B.ii0[b] != 0.0 || B.ii1[b] != 0.0 || B.ii2[b] != 0.0) { B.w0[b] = r[3]; B.w1[b] = r[4]; B.w2[b] = r[5]; }
 }
 
-__global__ void __launch_bounds__(TPB) k_wf_mark(BodiesDev B, const uint32_t* __restrict__ counters, ConsDev C,
+// Flags the first constraint of each contact pair in an island of more than
+// `big_above` constraints (every pair when `cons_isl` is null: the whole
+// solve as one dataflow, no islands built) and counts those islands'
+// constraints.  The whole-solve plan also publishes the warm cache's length,
+// which k_solve_small does otherwise.
+__global__ void __launch_bounds__(TPB) k_wf_mark(BodiesDev B, uint32_t* __restrict__ counters, ConsDev C,
                                                  const uint32_t* __restrict__ cons_isl,
                                                  const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ flag,
-                                                 uint32_t big_above) {
+                                                 uint32_t big_above, uint32_t* __restrict__ prev_n) {
     const uint32_t nc = counters[CNT_C];
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
-        const bool start = k == 0 || C.ci[k] != C.ci[k - 1] || C.cj[k] != C.cj[k - 1];
-        const uint32_t id = cons_isl[k];
-        flag[k] = (start && isl_start[id + 1] - isl_start[id] > big_above) ? 1u : 0u;
+    if (prev_n && blockIdx.x == 0 && threadIdx.x == 0) *prev_n = nc;
+    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < nc; k0 += gridDim.x * blockDim.x) {
+        const uint32_t k = k0 + threadIdx.x;
+        bool big = false;
+        if (k < nc) {
+            const bool start = k == 0 || C.ci[k] != C.ci[k - 1] || C.cj[k] != C.cj[k - 1];
+            if (cons_isl) {
+                const uint32_t id = cons_isl[k];
+                big = isl_start[id + 1] - isl_start[id] > big_above;
+            } else {
+                big = true;
+            }
+            flag[k] = (start && big) ? 1u : 0u;
+        }
+        wave_add1(counters, CNT_WFC, big);
     }
 }
 
@@ -2604,13 +2629,16 @@ __global__ void __launch_bounds__(TPB) k_wf_tasks(BodiesDev B, const uint32_t* _
         const uint32_t t = W.tid[k];
         W.task_k[t] = k;
         const uint32_t i = C.ci[k], j = C.cj[k];
-        if (mutable_body(B, i)) atomicAdd(&W.inc_cnt[i], 1u);
-        if (mutable_body(B, j)) atomicAdd(&W.inc_cnt[j], 1u);
+        const bool mi = mutable_body(B, i), mj = mutable_body(B, j);
+        if (mi) atomicAdd(&W.inc_cnt[i], 1u);
+        if (mj) atomicAdd(&W.inc_cnt[j], 1u);
         uint32_t k1 = k + 1;
         while (k1 < nc && C.ci[k1] == i && C.cj[k1] == j) ++k1;
         W.task_n[t] = k1 - k;
         W.task4[2 * (size_t)t] = make_uint4(k, k1 - k, i, j);
-        W.task4[2 * (size_t)t + 1] = make_uint4(WF_NONE, WF_NONE, 0u, 0u);
+        // a pair of two bodies that cannot move (the pair filter drops them,
+        // world.cpp:513) would still order its own passes: it succeeds itself
+        W.task4[2 * (size_t)t + 1] = make_uint4((mi || mj) ? WF_NONE : (WF_WRAP | t), WF_NONE, 0u, 0u);
         for (uint32_t q = k; q < k1; ++q) {   // the sweep's record of constraint q
             double2* r = reinterpret_cast<double2*>(W.rec + (size_t)q * WF_REC);
             r[0] = double2{C.n0[q], C.n1[q]};
@@ -2662,7 +2690,7 @@ __global__ void __launch_bounds__(TPB) k_wf_init(BodiesDev B, uint32_t T, uint32
         bool root = false;
         if (t < T) {
             const uint32_t k = W.task_k[t];
-            const uint32_t sides = (mutable_body(B, C.ci[k]) ? 1u : 0u) + (mutable_body(B, C.cj[k]) ? 1u : 0u);
+            const uint32_t sides = max((mutable_body(B, C.ci[k]) ? 1u : 0u) + (mutable_body(B, C.cj[k]) ? 1u : 0u), 1u);
             const uint32_t p0 = W.pend0[t];
             W.pend[t] = p0;
             for (uint32_t p = 1; p < npass; ++p) W.pend[(size_t)p * T + t] = sides;
@@ -3818,6 +3846,11 @@ struct ne_gpu_ctx {
     bool wf_expect = true;                            // expect large islands: read their task count back
     uint32_t wf_quiet = 0;                            // consecutive steps without large islands (checked ones)
     bool wf_seen = false;                             // this call had a step with large islands
+    // The whole solve as one dataflow (no islands built, no per-island
+    // solvers) while the large islands hold most constraints; eager steps only.
+    bool solve_all = false;                           // this step
+    bool all_ok = false;                              // the last island step's verdict
+    uint32_t all_steps = 0;                           // steps since that island step
     struct WfArgs* wf_args = nullptr;                 // device copy of the round kernel's arguments
     struct WfArgs* h_wf_args = nullptr;               // its pinned host staging copy
     uint32_t* h_wf = nullptr;                         // pinned: values read back mid-step
@@ -4409,8 +4442,10 @@ int ensure_side(ne_gpu_ctx* c) {
     return NE_OK;
 }
 
-// The step up to the solve: integrate, grid, candidate rows, narrow phase,
-// constraint build and islands.
+// This step solves every constraint as one dataflow: islands are neither
+// linked nor built.  Captured steps keep the islands (k_solve_large).
+bool whole_dataflow(const ne_gpu_ctx* c) { return c->solve_all && !c->capturing; }
+
 // The grid, the candidate rows and the narrow phase of one step (or of the
 // facade events pass, with its own filter and detectContact's one contact
 // per pair), from the packed poses in c->xp and, with a static grid, the
@@ -4567,6 +4602,8 @@ void enqueue_ccd(ne_gpu_ctx* c, const BodiesDev& B, const SimParams& P, bool sta
                                 c->ccd_cap, c->xp, static_grid ? 1 : 0, c->sgrid, c->cell_count);
 }
 
+// The step up to the solve: integrate, grid, candidate rows, narrow phase,
+// constraint build and islands.
 void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
     hipStream_t s = c->stream;
     const SimParams P = make_sim_params(c->params);
@@ -4608,8 +4645,9 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
         k_cons_build<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->cons_off,
                                                         c->ct, c->C, c->ccap, c->prev_key, c->prev_imp, c->prev_n,
                                                         c->parent, c->claimed, c->nmeshes ? c->mq_off : nullptr,
-                                                        c->nmeshes ? c->mct : nullptr);
+                                                        c->nmeshes ? c->mct : nullptr, !whole_dataflow(c));
     }
+    if (whole_dataflow(c)) return;   // no islands: every constraint goes to the dataflow solve
     // --- islands (linked inside k_cons_build)
     k_isl_claim<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, c->parent, c->cons_root, c->claimed,
                                                    c->root_id, c->isl_count, c->isl_fill);
@@ -4668,7 +4706,8 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     hipStream_t s = c->stream;
     BodiesDev B = c->B;
     WfDev& W = c->wf;
-    if (!c->wf_expect) {
+    const bool all = whole_dataflow(c);
+    if (!c->wf_expect && !all) {
         // recent steps had no large islands: do not stop the stream to ask;
         // if this one has some, the call reruns from its start reading them back
         k_wf_expect_none<<<1, 64, 0, s>>>(c->counters);
@@ -4676,13 +4715,20 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     }
     int r0 = ensure_wf(c, 0, 1);
     if (r0 != NE_OK) return r0;
-    k_wf_mark<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, c->cons_isl, c->isl_start, W.flag,
-                                                     WF_ISLAND);
+    k_fill_u32<<<1, 64, 0, s>>>(c->counters + CNT_WFC, 0u, nullptr, 1u);
+    k_wf_mark<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, all ? nullptr : c->cons_isl, c->isl_start,
+                                                     W.flag, WF_ISLAND, (all && P.warm) ? c->prev_n : nullptr);
     scan_u32(W.flag, W.tid, c->counters + CNT_C, c->ccap, c->scan_part, s, false,
              ScanTotal{c->counters, CNT_WFT, CNT_WFT, CNT_FAULT, c->ccap, 0u, -1, -1});
     HIP_TRY(c, hipMemcpyAsync(c->h_wf, c->counters + CNT_WFT, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(c->h_wf + 2, c->counters + CNT_WFC, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(c->h_wf + 3, c->counters + CNT_C, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
     const uint32_t T = c->h_wf[0];
+    if (!all) {   // the next steps skip the islands while the large ones hold most constraints
+        c->all_ok = (uint64_t)c->h_wf[2] * WF_ALL_DEN >= (uint64_t)c->h_wf[3] * WF_ALL_NUM && T > 0;
+        c->all_steps = 0;
+    }
     if (T == 0) {
         ++c->wf_quiet;   // step_call stops asking after 8 such steps, between calls
         return NE_OK;
@@ -4752,6 +4798,7 @@ int enqueue_solve(ne_gpu_ctx* c) {
     const SimParams P = make_sim_params(c->params);
     BodiesDev B = c->B;
     KTimer kt(c, NE_KERNEL_SOLVE);
+    if (whole_dataflow(c)) return solve_large_islands(c, P);
     k_solve_single<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C,
                                                           c->prev_key, c->prev_imp);
     k_solve_small<<<std::min<uint32_t>(4096u, std::max<uint32_t>(64u, (c->hint_c * 2 + SOLVE_THREADS - 1) / SOLVE_THREADS)),
@@ -6079,7 +6126,9 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps) {
     HIP_TRY(c, hipMemsetAsync(c->counters, 0, sizeof(uint32_t) * CNT_N, c->stream));
     c->wf_seen = false;
     for (int32_t t = 0; t < nsteps; ++t) {
+        c->solve_all = c->all_ok && ++c->all_steps % WF_ALL_RECHECK != 0;
         int r = run_step(c, t > 0);
+        c->solve_all = false;
         if (r != NE_OK) return r;
         if (c->events_on) {
             r = enqueue_events(c);

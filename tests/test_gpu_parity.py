@@ -127,6 +127,40 @@ def test_mixed_pile_periodic_parity(gpu_available, oracle_lib):
     _run_parity(s, 60, "ppile150", check_every=3)
 
 
+def test_whole_dataflow_steps_parity(gpu_available, oracle_lib):
+    """Once islands of more than 16 constraints hold 3/4 of the constraints
+    (the pile below from about step 80), eager steps solve every constraint as
+    one dataflow without building islands, and every 32nd step builds them
+    again: both kinds of step, in one-step and ten-step calls, bit for bit."""
+    _say("case: config-size pile 300, 140 steps")
+    s = scene.mixed_pile(300, half=(5.0, 8.0, 5.0), seed=2, sizes="config")
+    g = _gpu()
+    o = engines.OracleWorld()
+    g.set_scene(s)
+    o.set_scene(s)
+    whole = island = 0
+    t = 0
+    while t < 140:
+        k = 1 if t < 90 else 10
+        g.step(k)
+        o.step(k)
+        t += k
+        c = g.counters()
+        if c["constraints"] > 0:
+            if c["islands"] == 0:
+                whole += 1
+            else:
+                island += 1
+        _assert_same_state(g.state(), o.state(), f"pile300 step {t}")
+        if t % 10 == 0:
+            _assert_same_contacts(g.contacts(), o.contacts(), f"pile300 step {t}")
+    g.close()
+    o.close()
+    _say(f"calls ending on a whole-dataflow step: {whole}, on an island step: {island}")
+    assert whole > 0, "no step solved every constraint as one dataflow"
+    assert island > 0
+
+
 def test_extension_noncubic_iterations(gpu_available, oracle_lib):
     """Per-axis box extents and 10 iterations: beyond the reference, pinned by
     the restatement."""
