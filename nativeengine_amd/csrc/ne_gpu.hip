@@ -2594,30 +2594,36 @@ __global__ void __launch_bounds__(TPB) k_wf_unpack_bodies(BodiesDev B, const dou
 }
 
 // Flags the first constraint of each contact pair in an island of more than
-// `big_above` constraints (every pair when `cons_isl` is null: the whole
-// solve as one dataflow, no islands built) and counts those islands'
-// constraints.  The whole-solve plan also publishes the warm cache's length,
+// `big_above` constraints and counts those islands' constraints (one atomic
+// per block), or, when `cons_isl` is null (the whole solve as one dataflow,
+// no islands built), flags every pair and publishes the warm cache's length,
 // which k_solve_small does otherwise.
 __global__ void __launch_bounds__(TPB) k_wf_mark(BodiesDev B, uint32_t* __restrict__ counters, ConsDev C,
                                                  const uint32_t* __restrict__ cons_isl,
                                                  const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ flag,
                                                  uint32_t big_above, uint32_t* __restrict__ prev_n) {
+    __shared__ uint32_t part[TPB / 64];
     const uint32_t nc = counters[CNT_C];
     if (prev_n && blockIdx.x == 0 && threadIdx.x == 0) *prev_n = nc;
-    for (uint32_t k0 = blockIdx.x * blockDim.x; k0 < nc; k0 += gridDim.x * blockDim.x) {
-        const uint32_t k = k0 + threadIdx.x;
-        bool big = false;
-        if (k < nc) {
-            const bool start = k == 0 || C.ci[k] != C.ci[k - 1] || C.cj[k] != C.cj[k - 1];
-            if (cons_isl) {
-                const uint32_t id = cons_isl[k];
-                big = isl_start[id + 1] - isl_start[id] > big_above;
-            } else {
-                big = true;
-            }
-            flag[k] = (start && big) ? 1u : 0u;
+    uint32_t nbig = 0;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        const bool start = k == 0 || C.ci[k] != C.ci[k - 1] || C.cj[k] != C.cj[k - 1];
+        bool big = true;
+        if (cons_isl) {
+            const uint32_t id = cons_isl[k];
+            big = isl_start[id + 1] - isl_start[id] > big_above;
+            nbig += big ? 1u : 0u;
         }
-        wave_add1(counters, CNT_WFC, big);
+        flag[k] = (start && big) ? 1u : 0u;
+    }
+    if (!cons_isl) return;
+    for (int off = 32; off > 0; off >>= 1) nbig += __shfl_xor(nbig, off);
+    if (lane_id() == 0) part[threadIdx.x >> 6] = nbig;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (int w = 0; w < TPB / 64; ++w) sum += part[w];
+        if (sum) atomicAdd(&counters[CNT_WFC], sum);
     }
 }
 
@@ -2663,26 +2669,108 @@ __global__ void __launch_bounds__(TPB) k_wf_fill(BodiesDev B, uint32_t T, ConsDe
     }
 }
 
+// One body's share of apply_impulse (solve: a.v += J m_a^-1, a.w += I_a^-1
+// (rA x J) on side a, the opposite on side b), in the same operations.
+__device__ __forceinline__ void apply_side(BodyView& x, bool side_a, V3 r, V3 J) {
+    if (side_a) {
+        if (x.mv) x.v = x.v + J * x.im;
+        if (x.mw) x.w = x.w + apply_inv_inertia(x.q, x.ii, cross(r, J));
+    } else {
+        if (x.mv) x.v = x.v - J * x.im;
+        if (x.mw) x.w = x.w - apply_inv_inertia(x.q, x.ii, cross(r, J));
+    }
+}
+
 // Per movable body: its tasks in order, chained; the last links to the first
-// of the next pass.  Leaves inc_cnt zero for the next step.
-__global__ void __launch_bounds__(TPB) k_wf_link(uint32_t n, ConsDev C, WfDev W) {
+// of the next iteration.  With warm starting, the body's share of each warm
+// impulse (world.cpp:559-568) is applied here, in constraint order: what
+// constraint k adds to a body depends only on k and on that body's
+// orientation, inverse mass and inertia, which the solve does not change, so
+// the warm start orders nothing across bodies.  Leaves inc_cnt zero for the
+// next step.
+__global__ void __launch_bounds__(TPB) k_wf_link(uint32_t n, SimParams P, WfDev W) {
     for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += gridDim.x * blockDim.x) {
         const uint32_t m = W.inc_cnt[x];
         if (m == 0) continue;
         W.inc_cnt[x] = 0u;
         uint32_t* a = W.inc_list + W.inc_start[x];
         sort_segment(a, m);
+        BodyView bx{};
+        if (P.warm) bx = load_packed(W.bp, x);
         for (uint32_t u = 0; u < m; ++u) {
             const uint32_t t = a[u];
-            const uint32_t side = C.ci[W.task_k[t]] == x ? 0u : 1u;
+            const uint4 tk = W.task4[2 * (size_t)t];
+            const uint32_t side = tk.z == x ? 0u : 1u;
             reinterpret_cast<uint32_t*>(W.task4 + 2 * (size_t)t + 1)[side] = (u + 1 < m) ? a[u + 1] : (WF_WRAP | a[0]);
             if (u > 0) atomicAdd(&W.pend0[t], 1u);
+            if (!P.warm) continue;
+            for (uint32_t k = tk.x; k < tk.x + tk.y; ++k) {
+                const double2* r = reinterpret_cast<const double2*>(W.rec + (size_t)k * WF_REC);
+                const double nimp = r[6].x;
+                if (nimp == 0.0) continue;   // warm_apply
+                const double2 r0 = r[0], r1 = r[1];
+                const V3 nrm{r0.x, r0.y, r1.x};
+                V3 arm;
+                if (side == 0) {
+                    const double2 r2 = r[2];
+                    arm = V3{r1.y, r2.x, r2.y};
+                } else {
+                    const double2 r3 = r[3], r4 = r[4];
+                    arm = V3{r3.x, r3.y, r4.x};
+                }
+                apply_side(bx, side == 0, arm, nrm * nimp);
+            }
+        }
+        if (P.warm) store_packed(W.bp, x, bx);
+    }
+}
+
+// After the iterations, per movable body: its positional correction
+// (world.cpp:615-616, contact.hpp:78-85) from its constraints in order.
+// What constraint k moves a body by depends only on k and the two inverse
+// masses, so, like the warm start, it orders nothing across bodies.
+__global__ void __launch_bounds__(TPB) k_wf_correct(BodiesDev B, SimParams P, WfDev W) {
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < B.n; x += gridDim.x * blockDim.x) {
+        const uint32_t s0 = W.inc_start[x], m = W.inc_start[x + 1] - s0;
+        if (m == 0) continue;
+        const double imx = B.im[x];
+        if (imx == 0.0) continue;   // its position never changes
+        V3 px = ld3(B.x0, B.x1, B.x2, x);
+        for (uint32_t u = 0; u < m; ++u) {
+            const uint4 tk = W.task4[2 * (size_t)W.inc_list[s0 + u]];
+            const bool side_a = tk.z == x;
+            const double invSum = B.im[tk.z] + B.im[tk.w];
+            if (invSum <= 0.0) continue;
+            for (uint32_t k = tk.x; k < tk.x + tk.y; ++k) {
+                const double2* r = reinterpret_cast<const double2*>(W.rec + (size_t)k * WF_REC);
+                const double ov = r[7].x;
+                if (ov <= 0.0) continue;
+                const double2 r0 = r[0], r1 = r[1];
+                const V3 corr = V3{r0.x, r0.y, r1.x} * (P.beta * ov / invSum);
+                px = side_a ? px + corr * imx : px - corr * imx;
+            }
+        }
+        st3(B.x0, B.x1, B.x2, x, px);
+    }
+}
+
+// The converged impulses back to the constraint list and, with warm
+// starting, into the next step's cache (world.cpp:619-626).
+__global__ void __launch_bounds__(TPB) k_wf_out(uint32_t T, SimParams P, ConsDev C, WfDev W,
+                                                uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+        const uint4 tk = W.task4[2 * (size_t)t];
+        for (uint32_t k = tk.x; k < tk.x + tk.y; ++k) {
+            const double2 r6 = reinterpret_cast<const double2*>(W.rec + (size_t)k * WF_REC)[6];
+            C.nimp[k] = r6.x;
+            C.timp[k] = r6.y;
+            if (P.warm) { prev_key[k] = C.key[k]; prev_imp[k] = r6.x; }
         }
     }
 }
 
-// Pending counts of every visit; the pass-0 visits with no predecessor are
-// round 0.
+// Pending counts of every visit (pass p = iteration p + 1); the first
+// iteration's visits with no predecessor are round 0.
 __global__ void __launch_bounds__(TPB) k_wf_init(BodiesDev B, uint32_t T, uint32_t npass, ConsDev C, WfDev W) {
     if (blockIdx.x == 0 && threadIdx.x == 0) W.roff[0] = 0u;
     for (uint32_t t0 = blockIdx.x * blockDim.x; t0 < T; t0 += gridDim.x * blockDim.x) {
@@ -2701,63 +2789,28 @@ __global__ void __launch_bounds__(TPB) k_wf_init(BodiesDev B, uint32_t T, uint32
     }
 }
 
-// One pass over one task's constraints [k0, k1) on the packed records: the
-// two bodies stay in registers across the run, as in the pair solver.
-__device__ __forceinline__ void wf_visit(BodiesDev& B, const SimParams& P, const ConsDev& C, const WfDev& W,
-                                         uint32_t pass, uint32_t i, uint32_t j, uint32_t k0, uint32_t k1,
-                                         uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
-    if (pass == 0) {
-        if (!P.warm) return;
-        BodyView a = load_packed(W.bp, i), b = load_packed(W.bp, j);
-        for (uint32_t k = k0; k < k1; ++k) {
-            const double2* r = reinterpret_cast<const double2*>(W.rec + (size_t)k * WF_REC);
-            const double2 r6 = r[6];
-            if (r6.x == 0.0) continue;
-            const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4];
-            const ConsGeom g{V3{r0.x, r0.y, r1.x}, V3{r1.y, r2.x, r2.y}, V3{r3.x, r3.y, r4.x}};
-            warm_apply(a, b, g, r6.x);
+// One iteration over one task's constraints [k0, k1) on the packed records
+// (world.cpp:572-612): the two bodies stay in registers across the run, as
+// in the pair solver.
+__device__ __forceinline__ void wf_visit(const WfDev& W, uint32_t i, uint32_t j, uint32_t k0, uint32_t k1) {
+    BodyView a = load_packed(W.bp, i), b = load_packed(W.bp, j);
+    // the next record is loaded while this one is solved (the records do not
+    // depend on the sweep's results; only nImp/tImp are written)
+    const double2* r = reinterpret_cast<const double2*>(W.rec + (size_t)k0 * WF_REC);
+    double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6];
+    for (uint32_t k = k0; k < k1; ++k) {
+        const ConsGeom g{V3{r0.x, r0.y, r1.x}, V3{r1.y, r2.x, r2.y}, V3{r3.x, r3.y, r4.x}};
+        const double rbias = r4.y, kn = r5.x, mu = r5.y;
+        double nimp = r6.x, timp = r6.y;
+        if (k + 1 < k1) {
+            const double2* rn = reinterpret_cast<const double2*>(W.rec + (size_t)(k + 1) * WF_REC);
+            r0 = rn[0]; r1 = rn[1]; r2 = rn[2]; r3 = rn[3]; r4 = rn[4]; r5 = rn[5]; r6 = rn[6];
         }
-        store_packed(W.bp, i, a);
-        store_packed(W.bp, j, b);
-    } else if (pass <= (uint32_t)P.iters) {
-        BodyView a = load_packed(W.bp, i), b = load_packed(W.bp, j);
-        // the next record is loaded while this one is solved (the records
-        // do not depend on the sweep's results; only nImp/tImp are written)
-        const double2* r = reinterpret_cast<const double2*>(W.rec + (size_t)k0 * WF_REC);
-        double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6];
-        for (uint32_t k = k0; k < k1; ++k) {
-            const ConsGeom g{V3{r0.x, r0.y, r1.x}, V3{r1.y, r2.x, r2.y}, V3{r3.x, r3.y, r4.x}};
-            const double rbias = r4.y, kn = r5.x, mu = r5.y;
-            double nimp = r6.x, timp = r6.y;
-            if (k + 1 < k1) {
-                const double2* rn = reinterpret_cast<const double2*>(W.rec + (size_t)(k + 1) * WF_REC);
-                r0 = rn[0]; r1 = rn[1]; r2 = rn[2]; r3 = rn[3]; r4 = rn[4]; r5 = rn[5]; r6 = rn[6];
-            }
-            gs_visit(a, b, g, rbias, kn, mu, nimp, timp);
-            reinterpret_cast<double2*>(W.rec + (size_t)k * WF_REC)[6] = double2{nimp, timp};
-        }
-        store_packed(W.bp, i, a);
-        store_packed(W.bp, j, b);
-    } else {
-        V3 xa = ld3(B.x0, B.x1, B.x2, i), xb = ld3(B.x0, B.x1, B.x2, j);
-        const double ima = B.im[i], imb = B.im[j];
-        const double invSum = ima + imb;
-        for (uint32_t k = k0; k < k1; ++k) {   // contact.hpp:78-85, as correct()
-            const double2* r = reinterpret_cast<const double2*>(W.rec + (size_t)k * WF_REC);
-            const double2 r0 = r[0], r1 = r[1], r6 = r[6], r7 = r[7];
-            C.nimp[k] = r6.x;
-            C.timp[k] = r6.y;
-            if (P.warm) { prev_key[k] = C.key[k]; prev_imp[k] = r6.x; }
-            const double ov = r7.x;
-            if (invSum <= 0.0 || ov <= 0.0) continue;
-            const V3 n{r0.x, r0.y, r1.x};
-            const V3 corr = n * (P.beta * ov / invSum);
-            if (ima != 0.0) xa = xa + corr * ima;
-            if (imb != 0.0) xb = xb - corr * imb;
-        }
-        if (ima != 0.0) st3(B.x0, B.x1, B.x2, i, xa);
-        if (imb != 0.0) st3(B.x0, B.x1, B.x2, j, xb);
+        gs_visit(a, b, g, rbias, kn, mu, nimp, timp);
+        reinterpret_cast<double2*>(W.rec + (size_t)k * WF_REC)[6] = double2{nimp, timp};
     }
+    store_packed(W.bp, i, a);
+    store_packed(W.bp, j, b);
 }
 
 // k_solve_small counted the islands above its limit (CNT_NBIG).
@@ -2796,10 +2849,8 @@ constexpr uint64_t WF_WIDE_ROUND = 16384;
 // trips overlap its loads and arithmetic: a successor made ready here runs
 // only at this block's next level (after a barrier, so after this visit's
 // writes) or in a later launch, never beside this visit.
-__device__ __forceinline__ void wf_run_visit(BodiesDev& B, const SimParams& P, const ConsDev& C, const WfDev& W,
-                                             uint32_t T, uint32_t npass, uint32_t v, uint32_t tag, uint32_t out[2],
-                                             bool local[2], uint64_t* __restrict__ prev_key,
-                                             double* __restrict__ prev_imp) {
+__device__ __forceinline__ void wf_run_visit(const WfDev& W, uint32_t T, uint32_t npass, uint32_t v, uint32_t tag,
+                                             uint32_t out[2], bool local[2]) {
     const uint32_t pass = v / T, t = v - pass * T;
     const uint4 tk = W.task4[2 * (size_t)t], nxt = W.task4[2 * (size_t)t + 1];
     uint32_t old[2] = {0u, 0u};
@@ -2816,7 +2867,7 @@ __device__ __forceinline__ void wf_run_visit(BodiesDev& B, const SimParams& P, c
         }
         old[side] = atomicAdd(&W.pend[out[side]], (tag << 2) - 1u);
     }
-    wf_visit(B, P, C, W, pass, tk.z, tk.w, tk.x, tk.x + tk.y, prev_key, prev_imp);
+    wf_visit(W, tk.z, tk.w, tk.x, tk.x + tk.y);
     for (int side = 0; side < 2; ++side) {
         if (out[side] == WF_NONE) continue;
         if ((old[side] & 3u) != 1u) {   // another predecessor is still to run
@@ -2857,13 +2908,8 @@ __device__ __forceinline__ void wf_release(const WfDev& W, uint32_t r, uint32_t 
 // The round kernel's arguments, copied to the device once per step: each of
 // the hundreds of launches per step then passes one pointer, not ~1 KB.
 struct WfArgs {
-    BodiesDev B;
-    SimParams P;
-    ConsDev C;
     WfDev W;
     uint32_t T, npass;
-    uint64_t* prev_key;
-    double* prev_imp;
 };
 
 namespace {
@@ -2871,7 +2917,6 @@ namespace {
 __global__ void __launch_bounds__(WF_THREADS) k_wf_round(const WfArgs* __restrict__ a, uint32_t r, int levels) {
     __shared__ uint32_t lq[2][WF_LCAP];
     __shared__ uint32_t lqn[2];
-    BodiesDev B = a->B;
     const WfDev W = a->W;
     const uint32_t T = a->T, npass = a->npass;
     const uint32_t base = W.roff[r], cnt = W.rcnt[r];
@@ -2888,7 +2933,7 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(const WfArgs* __restric
         uint32_t out[2] = {WF_NONE, WF_NONE};
         bool local[2] = {false, false};
         if (u < cnt) {
-            wf_run_visit(B, a->P, a->C, W, T, npass, W.fr[base + u], tag, out, local, a->prev_key, a->prev_imp);
+            wf_run_visit(W, T, npass, W.fr[base + u], tag, out, local);
             ++ran;
         }
         wf_release(W, r, nbase, out, local, levels > 1, lq[0], &lqn[0]);
@@ -2906,7 +2951,7 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(const WfArgs* __restric
             uint32_t out[2] = {WF_NONE, WF_NONE};
             bool local[2] = {false, false};
             if (u < n) {
-                wf_run_visit(B, a->P, a->C, W, T, npass, lq[cur][u], tag, out, local, a->prev_key, a->prev_imp);
+                wf_run_visit(W, T, npass, lq[cur][u], tag, out, local);
                 ++ran;
             }
             wf_release(W, r, nbase, out, local, lev + 1 < levels, lq[nxl], &lqn[nxl]);
@@ -4698,10 +4743,11 @@ int ensure_wf(ne_gpu_ctx* c, uint32_t T, uint32_t npass) {
     return NE_OK;
 }
 
-// Islands of more than WF_ISLAND constraints as a dataflow of (pass,
-// task) visits: one launch per round, the host reading back the number of
-// visits run after a batch of rounds (the schedule's depth is not known in
-// advance).  Two synchronisations per step.
+int solve_rounds(ne_gpu_ctx* c, uint32_t T, uint32_t npass, uint32_t cleared);
+
+// Islands of more than WF_ISLAND constraints (or every constraint): the warm
+// start and the positional correction per body, the iterations as a
+// dataflow of (iteration, task) visits.  Two synchronisations per step.
 int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     hipStream_t s = c->stream;
     BodiesDev B = c->B;
@@ -4735,7 +4781,7 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     }
     c->wf_quiet = 0;
     c->wf_seen = true;
-    const uint32_t npass = (uint32_t)P.iters + 2u;
+    const uint32_t npass = (uint32_t)std::max(P.iters, 0);   // the dataflow's passes: the iterations
     if (npass > WF_PASS_MAX) return set_err(c, NE_ERR_UNSUPPORTED, "too many solver iterations for the large-island solve");
     r0 = ensure_wf(c, T, npass);
     if (r0 != NE_OK) return r0;
@@ -4746,16 +4792,34 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     k_wf_tasks<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, W);
     scan_u32(W.inc_cnt, W.inc_start, nullptr, c->n, c->scan_part, s, true);
     k_wf_fill<<<gs_blocks(T), TPB, 0, s>>>(B, T, c->C, W);
-    k_wf_link<<<gs_blocks(c->n), TPB, 0, s>>>(c->n, c->C, W);
-    k_wf_init<<<gs_blocks(T), TPB, 0, s>>>(B, T, npass, c->C, W);
     k_wf_pack_bodies<<<blocks_for(c->n), TPB, 0, s>>>(B, W.bp);
+    k_wf_link<<<gs_blocks(c->n), TPB, 0, s>>>(c->n, P, W);   // and the warm start
+    const uint64_t total = (uint64_t)T * npass;
+    if (total > 0) {
+        r0 = solve_rounds(c, T, npass, cleared);
+        if (r0 != NE_OK) return r0;
+    }
+    k_wf_correct<<<gs_blocks(c->n), TPB, 0, s>>>(B, P, W);
+    k_wf_out<<<gs_blocks(T), TPB, 0, s>>>(T, P, c->C, W, c->prev_key, c->prev_imp);
+    k_wf_unpack_bodies<<<blocks_for(c->n), TPB, 0, s>>>(B, W.bp);
+    return NE_OK;
+}
+
+// The iterations: one launch per round, the host reading back the number of
+// visits run after a batch of rounds (the schedule's depth is not known in
+// advance).
+int solve_rounds(ne_gpu_ctx* c, uint32_t T, uint32_t npass, uint32_t cleared) {
+    hipStream_t s = c->stream;
+    BodiesDev B = c->B;
+    WfDev& W = c->wf;
+    k_wf_init<<<gs_blocks(T), TPB, 0, s>>>(B, T, npass, c->C, W);
     k_fill_u32<<<1, 64, 0, s>>>(W.done, 0u, nullptr, 1u);
     const uint64_t total = (uint64_t)T * npass;
     if (!c->wf_args) HIP_TRY(c, hipMalloc((void**)&c->wf_args, sizeof(WfArgs)));
     if (!c->h_wf_args) HIP_TRY(c, hipHostMalloc((void**)&c->h_wf_args, sizeof(WfArgs), hipHostMallocDefault));
     // (the last step's copy from the staging buffer finished before this
-    // step's read-back of T above)
-    *c->h_wf_args = WfArgs{B, P, c->C, W, T, npass, c->prev_key, c->prev_imp};
+    // step's read-back of T)
+    *c->h_wf_args = WfArgs{W, T, npass};
     HIP_TRY(c, hipMemcpyAsync(c->wf_args, c->h_wf_args, sizeof(WfArgs), hipMemcpyHostToDevice, s));
     const uint32_t hint = c->wf_rounds_hint ? c->wf_rounds_hint : 64u;
     uint64_t per_round = total / hint + 1;
@@ -4779,7 +4843,6 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
             return set_err(c, NE_ERR_INVALID, "large-island solve stalled (a visit never became ready)");
         batch = 64;
     }
-    k_wf_unpack_bodies<<<blocks_for(c->n), TPB, 0, s>>>(B, W.bp);
     // the depth: rounds up to the last one that ran a visit
     std::vector<uint32_t> cnt(r);
     HIP_TRY(c, hipMemcpy(cnt.data(), W.rcnt, sizeof(uint32_t) * r, hipMemcpyDeviceToHost));
