@@ -2517,7 +2517,7 @@ constexpr uint32_t WF_WRAP = 0x80000000u;   // successor is the body's first tas
 constexpr uint32_t WF_NONE = 0xFFFFFFFFu;   // no successor on this side (a static body)
 constexpr uint32_t WF_MAX_ROUNDS = 1u << 16;
 constexpr int WF_THREADS = 64;
-constexpr uint32_t WF_PASS_MAX = 66;        // warm start + up to 64 iterations + correction
+constexpr uint32_t WF_PASS_MAX = 64;        // iterations
 constexpr uint32_t WF_ISLAND = STAGE_CONS;  // islands above this many constraints: the dataflow solve (eager) or k_solve_large (captured)
 // When the large islands hold at least WF_ALL_NUM / WF_ALL_DEN of the
 // constraints, the next eager steps put every constraint in the dataflow and
@@ -2534,9 +2534,10 @@ struct WfDev {
     uint32_t* task_k;      // per task: first constraint
     uint32_t* pend0;       // per task: predecessors in pass 0
     uint32_t* pend;        // per visit (pass * T + task): predecessors not yet run
-    uint32_t* fr;          // visits in the order their rounds ran them
+    uint4* front;          // the visits queued for a launch: two halves of T entries (launch r reads half r & 1 and
+                           // fills the other; at most one visit per task is ready at a time), 32 bytes each: the
+                           // task's first line, then (successor side 0, successor side 1, task, pass)
     uint32_t* rcnt;        // per round: visits queued for it
-    uint32_t* roff;        // per round: index in fr of its first visit
     uint32_t* done;        // visits run so far this step (one counter)
     uint32_t* inc_cnt;     // per body: tasks touching it (zero between steps)
     uint32_t* inc_start;   // per body: start of its slice of inc_list
@@ -2772,7 +2773,6 @@ __global__ void __launch_bounds__(TPB) k_wf_out(uint32_t T, SimParams P, ConsDev
 // Pending counts of every visit (pass p = iteration p + 1); the first
 // iteration's visits with no predecessor are round 0.
 __global__ void __launch_bounds__(TPB) k_wf_init(BodiesDev B, uint32_t T, uint32_t npass, ConsDev C, WfDev W) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) W.roff[0] = 0u;
     for (uint32_t t0 = blockIdx.x * blockDim.x; t0 < T; t0 += gridDim.x * blockDim.x) {
         const uint32_t t = t0 + threadIdx.x;
         bool root = false;
@@ -2785,7 +2785,11 @@ __global__ void __launch_bounds__(TPB) k_wf_init(BodiesDev B, uint32_t T, uint32
             root = p0 == 0u;
         }
         const uint32_t pos = wave_add1(W.rcnt, 0u, root);
-        if (root) W.fr[pos] = t;
+        if (root) {
+            W.front[2 * (size_t)pos] = W.task4[2 * (size_t)t];
+            const uint4 nx = W.task4[2 * (size_t)t + 1];
+            W.front[2 * (size_t)pos + 1] = make_uint4(nx.x, nx.y, t, 0u);
+        }
     }
 }
 
@@ -2842,36 +2846,41 @@ constexpr int WF_LCAP = 512;     // a block's list of locally ready visits (per 
 // the launch, bounds a round.  The host picks one level below this width.
 constexpr uint64_t WF_WIDE_ROUND = 16384;
 
-// Run visit v, release its successors: out[side] = the successor made ready
-// (WF_NONE: none), local[side] = whether it may run in this block.
+// Run the visit whose entry is (ea, eb), release its successors: ready[side]
+// = the successor on that side became ready ((sa, sb)[side]: its entry),
+// local[side] = it may run in this block.
 //
-// The successors are released before the visit runs, so the atomics' round
-// trips overlap its loads and arithmetic: a successor made ready here runs
-// only at this block's next level (after a barrier, so after this visit's
-// writes) or in a later launch, never beside this visit.
-__device__ __forceinline__ void wf_run_visit(const WfDev& W, uint32_t T, uint32_t npass, uint32_t v, uint32_t tag,
-                                             uint32_t out[2], bool local[2]) {
-    const uint32_t pass = v / T, t = v - pass * T;
-    const uint4 tk = W.task4[2 * (size_t)t], nxt = W.task4[2 * (size_t)t + 1];
+// The successors are released, and their tasks' lines loaded, before the
+// visit runs, so those round trips overlap its loads and arithmetic: a
+// successor made ready here runs only at this block's next level (after a
+// barrier, so after this visit's writes) or in a later launch, never beside
+// this visit.
+__device__ __forceinline__ void wf_run_visit(const WfDev& W, uint32_t T, uint32_t npass, uint4 ea, uint4 eb,
+                                             uint32_t tag, uint4 sa[2], uint4 sb[2], bool ready[2], bool local[2]) {
+    const uint32_t pass = eb.w;
     uint32_t old[2] = {0u, 0u};
     for (int side = 0; side < 2; ++side) {
-        out[side] = WF_NONE;
+        ready[side] = false;
         local[side] = false;
-        const uint32_t nx = side ? nxt.y : nxt.x;
+        const uint32_t nx = side ? eb.y : eb.x;
         if (nx == WF_NONE) continue;
+        uint32_t st = nx, sp = pass;
         if (nx & WF_WRAP) {
             if (pass + 1 >= npass) continue;
-            out[side] = (pass + 1) * T + (nx & ~WF_WRAP);
-        } else {
-            out[side] = pass * T + nx;
+            st = nx & ~WF_WRAP;
+            sp = pass + 1;
         }
-        old[side] = atomicAdd(&W.pend[out[side]], (tag << 2) - 1u);
+        old[side] = atomicAdd(&W.pend[(size_t)sp * T + st], (tag << 2) - 1u);
+        sa[side] = W.task4[2 * (size_t)st];
+        const uint4 nxt = W.task4[2 * (size_t)st + 1];
+        sb[side] = make_uint4(nxt.x, nxt.y, st, sp);
+        ready[side] = true;
     }
-    wf_visit(W, tk.z, tk.w, tk.x, tk.x + tk.y);
+    wf_visit(W, ea.z, ea.w, ea.x, ea.x + ea.y);
     for (int side = 0; side < 2; ++side) {
-        if (out[side] == WF_NONE) continue;
+        if (!ready[side]) continue;
         if ((old[side] & 3u) != 1u) {   // another predecessor is still to run
-            out[side] = WF_NONE;
+            ready[side] = false;
             continue;
         }
         const uint32_t other = old[side] >> 2;
@@ -2879,17 +2888,20 @@ __device__ __forceinline__ void wf_run_visit(const WfDev& W, uint32_t T, uint32_
     }
 }
 
-// Queue the successors this wave's lanes made ready: local ones into the
-// block's list for the next level (when `may_local` and it has room), the
-// rest for launch r + 1 (one append per wave).
-__device__ __forceinline__ void wf_release(const WfDev& W, uint32_t r, uint32_t nbase, const uint32_t out[2],
+// Queue the successors this wave's lanes made ready: local ones (by visit
+// id) into the block's list for the next level (when `may_local` and it has
+// room), the rest, as entries, into `outq` for launch r + 1 (one append per
+// wave).  An index past T would break the one-ready-visit-per-task bound;
+// such a visit is dropped and the host's completion check reports the stall.
+__device__ __forceinline__ void wf_release(const WfDev& W, uint4* __restrict__ outq, uint32_t T, uint32_t r,
+                                           const uint4 sa[2], const uint4 sb[2], const bool ready[2],
                                            const bool local[2], bool may_local, uint32_t* lq, uint32_t* lqn) {
     bool glob[2];
     for (int side = 0; side < 2; ++side) {
-        glob[side] = out[side] != WF_NONE;
+        glob[side] = ready[side];
         if (glob[side] && local[side] && may_local) {
             const uint32_t at = atomicAdd(lqn, 1u);
-            if (at < (uint32_t)WF_LCAP) { lq[at] = out[side]; glob[side] = false; }
+            if (at < (uint32_t)WF_LCAP) { lq[at] = sb[side].w * T + sb[side].z; glob[side] = false; }
         }
     }
     const uint64_t b0 = __ballot(glob[0]), b1 = __ballot(glob[1]);
@@ -2899,44 +2911,50 @@ __device__ __forceinline__ void wf_release(const WfDev& W, uint32_t r, uint32_t 
     if (lane_id() == 0) slot0 = atomicAdd(&W.rcnt[r + 1], tot);
     slot0 = __shfl(slot0, 0);
     const uint64_t below = lane_id() ? (~0ull >> (64 - lane_id())) : 0ull;
-    if (glob[0]) W.fr[nbase + slot0 + (uint32_t)__popcll(b0 & below)] = out[0];
-    if (glob[1]) W.fr[nbase + slot0 + (uint32_t)__popcll(b0) + (uint32_t)__popcll(b1 & below)] = out[1];
+    const uint32_t at[2] = {slot0 + (uint32_t)__popcll(b0 & below),
+                            slot0 + (uint32_t)__popcll(b0) + (uint32_t)__popcll(b1 & below)};
+    for (int side = 0; side < 2; ++side) {
+        if (!glob[side] || at[side] >= T) continue;
+        outq[2 * (size_t)at[side]] = sa[side];
+        outq[2 * (size_t)at[side] + 1] = sb[side];
+    }
 }
 
-}  // namespace
-
-// The round kernel's arguments, copied to the device once per step: each of
-// the hundreds of launches per step then passes one pointer, not ~1 KB.
-struct WfArgs {
-    WfDev W;
-    uint32_t T, npass;
-};
-
-namespace {
-
-__global__ void __launch_bounds__(WF_THREADS) k_wf_round(const WfArgs* __restrict__ a, uint32_t r, int levels) {
+// Launch r: runs the visits queued for it, then (levels > 1) the successors
+// that became ready in this block.  The first entry is loaded before the
+// count it is checked against, so the two loads are in flight together (its
+// index is clamped into the half).
+__global__ void __launch_bounds__(WF_THREADS) k_wf_round(WfDev W, uint32_t T, uint32_t npass, uint32_t r, int levels) {
     __shared__ uint32_t lq[2][WF_LCAP];
     __shared__ uint32_t lqn[2];
-    const WfDev W = a->W;
-    const uint32_t T = a->T, npass = a->npass;
-    const uint32_t base = W.roff[r], cnt = W.rcnt[r];
-    if (blockIdx.x == 0 && threadIdx.x == 0) W.roff[r + 1] = base + cnt;
-    if (cnt == 0) return;
-    const uint32_t nbase = base + cnt;
+    const uint4* __restrict__ in = W.front + (size_t)(r & 1u) * 2u * T;
+    uint4* __restrict__ outq = W.front + (size_t)((r + 1u) & 1u) * 2u * T;
+    const uint32_t ub = blockIdx.x * blockDim.x;
+    const uint32_t us = min(ub + threadIdx.x, T - 1u);
+    uint4 ea = in[2 * (size_t)us], eb = in[2 * (size_t)us + 1];
+    const uint32_t cnt = W.rcnt[r];
+    // (an empty asm that takes the entry: the compiler would otherwise sink
+    // its loads below the count's test)
+    asm volatile("" : : "v"(ea.x), "v"(ea.y), "v"(ea.z), "v"(ea.w), "v"(eb.x), "v"(eb.y), "v"(eb.z), "v"(eb.w));
+    if (ub >= cnt) return;
     const uint32_t tag = ((r + 1u) << 12) | blockIdx.x;
     if (threadIdx.x == 0) { lqn[0] = 0u; lqn[1] = 0u; }
     __syncthreads();
     uint32_t ran = 0;
     // level 0: this block's share of the visits queued for launch r
-    for (uint32_t u0 = blockIdx.x * blockDim.x; u0 < cnt; u0 += gridDim.x * blockDim.x) {
+    for (uint32_t u0 = ub; u0 < cnt; u0 += gridDim.x * blockDim.x) {
         const uint32_t u = u0 + threadIdx.x;
-        uint32_t out[2] = {WF_NONE, WF_NONE};
-        bool local[2] = {false, false};
+        if (u0 != ub && u < cnt) {
+            ea = in[2 * (size_t)u];
+            eb = in[2 * (size_t)u + 1];
+        }
+        uint4 sa[2], sb[2];
+        bool ready[2] = {false, false}, local[2] = {false, false};
         if (u < cnt) {
-            wf_run_visit(W, T, npass, W.fr[base + u], tag, out, local);
+            wf_run_visit(W, T, npass, ea, eb, tag, sa, sb, ready, local);
             ++ran;
         }
-        wf_release(W, r, nbase, out, local, levels > 1, lq[0], &lqn[0]);
+        wf_release(W, outq, T, r, sa, sb, ready, local, levels > 1, lq[0], &lqn[0]);
     }
     // levels 1 ..: the visits this block made ready, from its own list
     for (int lev = 1; lev < levels; ++lev) {
@@ -2948,13 +2966,16 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(const WfArgs* __restric
         __syncthreads();
         for (uint32_t u0 = 0; u0 < n; u0 += blockDim.x) {
             const uint32_t u = u0 + threadIdx.x;
-            uint32_t out[2] = {WF_NONE, WF_NONE};
-            bool local[2] = {false, false};
+            uint4 sa[2], sb[2];
+            bool ready[2] = {false, false}, local[2] = {false, false};
             if (u < n) {
-                wf_run_visit(W, T, npass, lq[cur][u], tag, out, local);
+                const uint32_t v = lq[cur][u];
+                const uint32_t pass = v / T, t = v - pass * T;
+                const uint4 la = W.task4[2 * (size_t)t], nx = W.task4[2 * (size_t)t + 1];
+                wf_run_visit(W, T, npass, la, make_uint4(nx.x, nx.y, t, pass), tag, sa, sb, ready, local);
                 ++ran;
             }
-            wf_release(W, r, nbase, out, local, lev + 1 < levels, lq[nxl], &lqn[nxl]);
+            wf_release(W, outq, T, r, sa, sb, ready, local, lev + 1 < levels, lq[nxl], &lqn[nxl]);
         }
     }
     // visits run: one atomic per wave
@@ -3896,8 +3917,6 @@ struct ne_gpu_ctx {
     bool solve_all = false;                           // this step
     bool all_ok = false;                              // the last island step's verdict
     uint32_t all_steps = 0;                           // steps since that island step
-    struct WfArgs* wf_args = nullptr;                 // device copy of the round kernel's arguments
-    struct WfArgs* h_wf_args = nullptr;               // its pinned host staging copy
     uint32_t* h_wf = nullptr;                         // pinned: values read back mid-step
     bool capturing = false;                           // a step is being captured into a graph
     hipStream_t side = nullptr;                       // second stream (default priority) for work beside the main one
@@ -4417,11 +4436,9 @@ void free_adjoint(ne_gpu_ctx* c) {
 }
 
 void free_wf(ne_gpu_ctx* c) {
-    dfree(c->wf_args);
-    if (c->h_wf_args) { (void)hipHostFree(c->h_wf_args); c->h_wf_args = nullptr; }
     WfDev& W = c->wf;
-    dfree(W.flag); dfree(W.tid); dfree(W.task_k); dfree(W.pend0); dfree(W.pend); dfree(W.fr);
-    dfree(W.rcnt); dfree(W.roff); dfree(W.done); dfree(W.inc_cnt); dfree(W.inc_start); dfree(W.inc_list);
+    dfree(W.flag); dfree(W.tid); dfree(W.task_k); dfree(W.pend0); dfree(W.pend); dfree(W.front);
+    dfree(W.rcnt); dfree(W.done); dfree(W.inc_cnt); dfree(W.inc_start); dfree(W.inc_list);
     dfree(W.task_n); dfree(W.task4); dfree(W.rec); dfree(W.bp);
     c->wf_tcap = c->wf_ccap = c->wf_ncap = c->wf_vcap_pass = 0;
 }
@@ -4708,7 +4725,6 @@ int ensure_wf(ne_gpu_ctx* c, uint32_t T, uint32_t npass) {
     if (!c->h_wf) HIP_TRY(c, hipHostMalloc((void**)&c->h_wf, sizeof(uint32_t) * 4, hipHostMallocDefault));
     if (!W.rcnt) {
         HIP_TRY(c, dalloc(W.rcnt, (size_t)WF_MAX_ROUNDS + 2));
-        HIP_TRY(c, dalloc(W.roff, (size_t)WF_MAX_ROUNDS + 2));
         HIP_TRY(c, dalloc(W.done, 4));
     }
     if (c->wf_ccap < c->ccap) {
@@ -4732,12 +4748,12 @@ int ensure_wf(ne_gpu_ctx* c, uint32_t T, uint32_t npass) {
         HIP_TRY(c, dalloc(W.task4, 2 * (size_t)cap));
         HIP_TRY(c, dalloc(W.pend0, cap));
         HIP_TRY(c, dalloc(W.inc_list, 2 * (size_t)cap));
+        HIP_TRY(c, dalloc(W.front, 4 * (size_t)cap));
         c->wf_tcap = cap;
         c->wf_vcap_pass = 0;
     }
     if (c->wf_vcap_pass < npass) {
         HIP_TRY(c, dalloc(W.pend, (size_t)c->wf_tcap * npass));
-        HIP_TRY(c, dalloc(W.fr, (size_t)c->wf_tcap * npass));
         c->wf_vcap_pass = npass;
     }
     return NE_OK;
@@ -4815,12 +4831,6 @@ int solve_rounds(ne_gpu_ctx* c, uint32_t T, uint32_t npass, uint32_t cleared) {
     k_wf_init<<<gs_blocks(T), TPB, 0, s>>>(B, T, npass, c->C, W);
     k_fill_u32<<<1, 64, 0, s>>>(W.done, 0u, nullptr, 1u);
     const uint64_t total = (uint64_t)T * npass;
-    if (!c->wf_args) HIP_TRY(c, hipMalloc((void**)&c->wf_args, sizeof(WfArgs)));
-    if (!c->h_wf_args) HIP_TRY(c, hipHostMalloc((void**)&c->h_wf_args, sizeof(WfArgs), hipHostMallocDefault));
-    // (the last step's copy from the staging buffer finished before this
-    // step's read-back of T)
-    *c->h_wf_args = WfArgs{W, T, npass};
-    HIP_TRY(c, hipMemcpyAsync(c->wf_args, c->h_wf_args, sizeof(WfArgs), hipMemcpyHostToDevice, s));
     const uint32_t hint = c->wf_rounds_hint ? c->wf_rounds_hint : 64u;
     uint64_t per_round = total / hint + 1;
     uint32_t grid = (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(8, (per_round * 3 / 2 + WF_THREADS - 1) / WF_THREADS));
@@ -4834,7 +4844,7 @@ int solve_rounds(ne_gpu_ctx* c, uint32_t T, uint32_t npass, uint32_t cleared) {
                                                                           r + batch + 2 - cleared);
             cleared = r + batch + 2;
         }
-        for (uint32_t q = 0; q < batch; ++q, ++r) k_wf_round<<<grid, WF_THREADS, 0, s>>>(c->wf_args, r, levels);
+        for (uint32_t q = 0; q < batch; ++q, ++r) k_wf_round<<<grid, WF_THREADS, 0, s>>>(W, T, npass, r, levels);
         HIP_TRY(c, hipMemcpyAsync(c->h_wf, W.done, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(c, hipMemcpyAsync(c->h_wf + 1, W.rcnt + r, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(c, hipStreamSynchronize(s));
