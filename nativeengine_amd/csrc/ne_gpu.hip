@@ -2846,9 +2846,46 @@ constexpr int WF_LCAP = 512;     // a block's list of locally ready visits (per 
 // the launch, bounds a round.  The host picks one level below this width.
 constexpr uint64_t WF_WIDE_ROUND = 16384;
 
-// Run the visit whose entry is (ea, eb), release its successors: ready[side]
-// = the successor on that side became ready ((sa, sb)[side]: its entry),
-// local[side] = it may run in this block.
+// A released successor: its entry (the task's first line, then (successor
+// side 0, successor side 1, task, pass)), whether it became ready and may
+// run in this block.
+struct WfNext {
+    uint4 a = make_uint4(0u, 0u, 0u, 0u), b = make_uint4(0u, 0u, 0u, 0u);
+    bool ready = false, local = false;
+};
+
+// Release the successor `nx` of a visit in pass `pass`: the pending count's
+// old value, and the successor's entry loaded.
+__device__ __forceinline__ uint32_t wf_release_one(const WfDev& W, uint32_t T, uint32_t npass, uint32_t pass,
+                                                   uint32_t nx, uint32_t tag, WfNext& s) {
+    if (nx == WF_NONE) return 0u;
+    uint32_t st = nx, sp = pass;
+    if (nx & WF_WRAP) {
+        if (pass + 1 >= npass) return 0u;
+        st = nx & ~WF_WRAP;
+        sp = pass + 1;
+    }
+    const uint32_t old = atomicAdd(&W.pend[(size_t)sp * T + st], (tag << 2) - 1u);
+    s.a = W.task4[2 * (size_t)st];
+    const uint4 nxt = W.task4[2 * (size_t)st + 1];
+    s.b = make_uint4(nxt.x, nxt.y, st, sp);
+    s.ready = true;
+    return old;
+}
+
+// Whether the successor released with pending count `old` is ready, and
+// whether it may run in this block.
+__device__ __forceinline__ void wf_settle(uint32_t old, uint32_t tag, WfNext& s) {
+    if (!s.ready) return;
+    if ((old & 3u) != 1u) {   // another predecessor is still to run
+        s.ready = false;
+        return;
+    }
+    const uint32_t other = old >> 2;
+    s.local = (other >> 12) != (tag >> 12) || other == tag;
+}
+
+// Run the visit whose entry is (ea, eb) and release its two successors.
 //
 // The successors are released, and their tasks' lines loaded, before the
 // visit runs, so those round trips overlap its loads and arithmetic: a
@@ -2856,36 +2893,16 @@ constexpr uint64_t WF_WIDE_ROUND = 16384;
 // barrier, so after this visit's writes) or in a later launch, never beside
 // this visit.
 __device__ __forceinline__ void wf_run_visit(const WfDev& W, uint32_t T, uint32_t npass, uint4 ea, uint4 eb,
-                                             uint32_t tag, uint4 sa[2], uint4 sb[2], bool ready[2], bool local[2]) {
-    const uint32_t pass = eb.w;
-    uint32_t old[2] = {0u, 0u};
-    for (int side = 0; side < 2; ++side) {
-        ready[side] = false;
-        local[side] = false;
-        const uint32_t nx = side ? eb.y : eb.x;
-        if (nx == WF_NONE) continue;
-        uint32_t st = nx, sp = pass;
-        if (nx & WF_WRAP) {
-            if (pass + 1 >= npass) continue;
-            st = nx & ~WF_WRAP;
-            sp = pass + 1;
-        }
-        old[side] = atomicAdd(&W.pend[(size_t)sp * T + st], (tag << 2) - 1u);
-        sa[side] = W.task4[2 * (size_t)st];
-        const uint4 nxt = W.task4[2 * (size_t)st + 1];
-        sb[side] = make_uint4(nxt.x, nxt.y, st, sp);
-        ready[side] = true;
-    }
+                                             uint32_t tag, WfNext& s0, WfNext& s1) {
+    // the whole entry first (an empty asm that takes it): otherwise the
+    // compiler, which loads its fields separately, waits for every load in
+    // flight, the first successor's included, before the second release
+    asm volatile("" : : "v"(ea.x), "v"(ea.y), "v"(ea.z), "v"(ea.w), "v"(eb.x), "v"(eb.y), "v"(eb.z), "v"(eb.w));
+    const uint32_t o0 = wf_release_one(W, T, npass, eb.w, eb.x, tag, s0);
+    const uint32_t o1 = wf_release_one(W, T, npass, eb.w, eb.y, tag, s1);
     wf_visit(W, ea.z, ea.w, ea.x, ea.x + ea.y);
-    for (int side = 0; side < 2; ++side) {
-        if (!ready[side]) continue;
-        if ((old[side] & 3u) != 1u) {   // another predecessor is still to run
-            ready[side] = false;
-            continue;
-        }
-        const uint32_t other = old[side] >> 2;
-        local[side] = (other >> 12) != (tag >> 12) || other == tag;
-    }
+    wf_settle(o0, tag, s0);
+    wf_settle(o1, tag, s1);
 }
 
 // Queue the successors this wave's lanes made ready: local ones (by visit
@@ -2894,29 +2911,33 @@ __device__ __forceinline__ void wf_run_visit(const WfDev& W, uint32_t T, uint32_
 // wave).  An index past T would break the one-ready-visit-per-task bound;
 // such a visit is dropped and the host's completion check reports the stall.
 __device__ __forceinline__ void wf_release(const WfDev& W, uint4* __restrict__ outq, uint32_t T, uint32_t r,
-                                           const uint4 sa[2], const uint4 sb[2], const bool ready[2],
-                                           const bool local[2], bool may_local, uint32_t* lq, uint32_t* lqn) {
-    bool glob[2];
-    for (int side = 0; side < 2; ++side) {
-        glob[side] = ready[side];
-        if (glob[side] && local[side] && may_local) {
-            const uint32_t at = atomicAdd(lqn, 1u);
-            if (at < (uint32_t)WF_LCAP) { lq[at] = sb[side].w * T + sb[side].z; glob[side] = false; }
-        }
-    }
-    const uint64_t b0 = __ballot(glob[0]), b1 = __ballot(glob[1]);
+                                           const WfNext& s0, const WfNext& s1, bool may_local, uint32_t* lq,
+                                           uint32_t* lqn) {
+    auto keep_local = [&](const WfNext& sx) {
+        if (!(sx.ready && sx.local && may_local)) return false;
+        const uint32_t at = atomicAdd(lqn, 1u);
+        if (at >= (uint32_t)WF_LCAP) return false;
+        lq[at] = sx.b.w * T + sx.b.z;
+        return true;
+    };
+    const bool g0 = s0.ready && !keep_local(s0);
+    const bool g1 = s1.ready && !keep_local(s1);
+    const uint64_t b0 = __ballot(g0), b1 = __ballot(g1);
     const uint32_t tot = (uint32_t)(__popcll(b0) + __popcll(b1));
     if (tot == 0) return;
     uint32_t slot0 = 0;
     if (lane_id() == 0) slot0 = atomicAdd(&W.rcnt[r + 1], tot);
     slot0 = __shfl(slot0, 0);
     const uint64_t below = lane_id() ? (~0ull >> (64 - lane_id())) : 0ull;
-    const uint32_t at[2] = {slot0 + (uint32_t)__popcll(b0 & below),
-                            slot0 + (uint32_t)__popcll(b0) + (uint32_t)__popcll(b1 & below)};
-    for (int side = 0; side < 2; ++side) {
-        if (!glob[side] || at[side] >= T) continue;
-        outq[2 * (size_t)at[side]] = sa[side];
-        outq[2 * (size_t)at[side] + 1] = sb[side];
+    const uint32_t at0 = slot0 + (uint32_t)__popcll(b0 & below);
+    const uint32_t at1 = slot0 + (uint32_t)__popcll(b0) + (uint32_t)__popcll(b1 & below);
+    if (g0 && at0 < T) {
+        outq[2 * (size_t)at0] = s0.a;
+        outq[2 * (size_t)at0 + 1] = s0.b;
+    }
+    if (g1 && at1 < T) {
+        outq[2 * (size_t)at1] = s1.a;
+        outq[2 * (size_t)at1 + 1] = s1.b;
     }
 }
 
@@ -2948,13 +2969,12 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(WfDev W, uint32_t T, ui
             ea = in[2 * (size_t)u];
             eb = in[2 * (size_t)u + 1];
         }
-        uint4 sa[2], sb[2];
-        bool ready[2] = {false, false}, local[2] = {false, false};
+        WfNext s0, s1;
         if (u < cnt) {
-            wf_run_visit(W, T, npass, ea, eb, tag, sa, sb, ready, local);
+            wf_run_visit(W, T, npass, ea, eb, tag, s0, s1);
             ++ran;
         }
-        wf_release(W, outq, T, r, sa, sb, ready, local, levels > 1, lq[0], &lqn[0]);
+        wf_release(W, outq, T, r, s0, s1, levels > 1, lq[0], &lqn[0]);
     }
     // levels 1 ..: the visits this block made ready, from its own list
     for (int lev = 1; lev < levels; ++lev) {
@@ -2966,16 +2986,15 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(WfDev W, uint32_t T, ui
         __syncthreads();
         for (uint32_t u0 = 0; u0 < n; u0 += blockDim.x) {
             const uint32_t u = u0 + threadIdx.x;
-            uint4 sa[2], sb[2];
-            bool ready[2] = {false, false}, local[2] = {false, false};
+            WfNext s0, s1;
             if (u < n) {
                 const uint32_t v = lq[cur][u];
                 const uint32_t pass = v / T, t = v - pass * T;
                 const uint4 la = W.task4[2 * (size_t)t], nx = W.task4[2 * (size_t)t + 1];
-                wf_run_visit(W, T, npass, la, make_uint4(nx.x, nx.y, t, pass), tag, sa, sb, ready, local);
+                wf_run_visit(W, T, npass, la, make_uint4(nx.x, nx.y, t, pass), tag, s0, s1);
                 ++ran;
             }
-            wf_release(W, outq, T, r, sa, sb, ready, local, lev + 1 < levels, lq[nxl], &lqn[nxl]);
+            wf_release(W, outq, T, r, s0, s1, lev + 1 < levels, lq[nxl], &lqn[nxl]);
         }
     }
     // visits run: one atomic per wave
