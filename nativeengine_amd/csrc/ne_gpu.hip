@@ -2796,12 +2796,25 @@ __global__ void __launch_bounds__(TPB) k_wf_init(BodiesDev B, uint32_t T, uint32
 // One iteration over one task's constraints [k0, k1) on the packed records
 // (world.cpp:572-612): the two bodies stay in registers across the run, as
 // in the pair solver.
-__device__ __forceinline__ void wf_visit(const WfDev& W, uint32_t i, uint32_t j, uint32_t k0, uint32_t k1) {
-    BodyView a = load_packed(W.bp, i), b = load_packed(W.bp, j);
+// A visit's inputs: its two bodies and its first constraint's record.
+struct WfLoaded {
+    BodyView a, b;
+    double2 r0, r1, r2, r3, r4, r5, r6;
+};
+
+__device__ __forceinline__ void wf_visit_load(const WfDev& W, uint32_t i, uint32_t j, uint32_t k0, WfLoaded& L) {
+    L.a = load_packed(W.bp, i);
+    L.b = load_packed(W.bp, j);
+    const double2* r = reinterpret_cast<const double2*>(W.rec + (size_t)k0 * WF_REC);
+    L.r0 = r[0]; L.r1 = r[1]; L.r2 = r[2]; L.r3 = r[3]; L.r4 = r[4]; L.r5 = r[5]; L.r6 = r[6];
+}
+
+__device__ __forceinline__ void wf_visit(const WfDev& W, const WfLoaded& L, uint32_t i, uint32_t j, uint32_t k0,
+                                         uint32_t k1) {
+    BodyView a = L.a, b = L.b;
     // the next record is loaded while this one is solved (the records do not
     // depend on the sweep's results; only nImp/tImp are written)
-    const double2* r = reinterpret_cast<const double2*>(W.rec + (size_t)k0 * WF_REC);
-    double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6];
+    double2 r0 = L.r0, r1 = L.r1, r2 = L.r2, r3 = L.r3, r4 = L.r4, r5 = L.r5, r6 = L.r6;
     for (uint32_t k = k0; k < k1; ++k) {
         const ConsGeom g{V3{r0.x, r0.y, r1.x}, V3{r1.y, r2.x, r2.y}, V3{r3.x, r3.y, r4.x}};
         const double rbias = r4.y, kn = r5.x, mu = r5.y;
@@ -2885,34 +2898,36 @@ __device__ __forceinline__ void wf_settle(uint32_t old, uint32_t tag, WfNext& s)
     s.local = (other >> 12) != (tag >> 12) || other == tag;
 }
 
-// Run the visit whose entry is (ea, eb) and release its two successors.
-//
-// The successors are released, and their tasks' lines loaded, before the
-// visit runs, so those round trips overlap its loads and arithmetic: a
-// successor made ready here runs only at this block's next level (after a
-// barrier, so after this visit's writes) or in a later launch, never beside
-// this visit.
-__device__ __forceinline__ void wf_run_visit(const WfDev& W, uint32_t T, uint32_t npass, uint4 ea, uint4 eb,
-                                             uint32_t tag, WfNext& s0, WfNext& s1) {
+// Release the two successors of the visit whose entry is (ea, eb) and load
+// the visit's inputs: every round trip of the visit in flight at once.
+// (Releasing before the visit runs is safe: a successor made ready here runs
+// only at this block's next level, after a barrier, so after this visit's
+// writes, or in a later launch, never beside this visit.)
+__device__ __forceinline__ void wf_start_visit(const WfDev& W, uint32_t T, uint32_t npass, uint4 ea, uint4 eb,
+                                               uint32_t tag, WfNext& s0, WfNext& s1, uint32_t& o0, uint32_t& o1,
+                                               WfLoaded& L) {
     // the whole entry first (an empty asm that takes it): otherwise the
     // compiler, which loads its fields separately, waits for every load in
     // flight, the first successor's included, before the second release
     asm volatile("" : : "v"(ea.x), "v"(ea.y), "v"(ea.z), "v"(ea.w), "v"(eb.x), "v"(eb.y), "v"(eb.z), "v"(eb.w));
-    const uint32_t o0 = wf_release_one(W, T, npass, eb.w, eb.x, tag, s0);
-    const uint32_t o1 = wf_release_one(W, T, npass, eb.w, eb.y, tag, s1);
-    wf_visit(W, ea.z, ea.w, ea.x, ea.x + ea.y);
-    wf_settle(o0, tag, s0);
-    wf_settle(o1, tag, s1);
+    o0 = wf_release_one(W, T, npass, eb.w, eb.x, tag, s0);
+    o1 = wf_release_one(W, T, npass, eb.w, eb.y, tag, s1);
+    wf_visit_load(W, ea.z, ea.w, ea.x, L);
 }
 
-// Queue the successors this wave's lanes made ready: local ones (by visit
-// id) into the block's list for the next level (when `may_local` and it has
-// room), the rest, as entries, into `outq` for launch r + 1 (one append per
-// wave).  An index past T would break the one-ready-visit-per-task bound;
-// such a visit is dropped and the host's completion check reports the stall.
-__device__ __forceinline__ void wf_release(const WfDev& W, uint4* __restrict__ outq, uint32_t T, uint32_t r,
-                                           const WfNext& s0, const WfNext& s1, bool may_local, uint32_t* lq,
-                                           uint32_t* lqn) {
+// This wave's queue slots for its ready successors: local ones (by visit id)
+// go into the block's list for the next level (when `may_local` and it has
+// room), the rest are counted for launch r + 1 with one atomic per wave,
+// whose result wf_publish reads after the visit, so its round trip overlaps
+// the visit's arithmetic.
+struct WfSlots {
+    bool g0 = false, g1 = false;
+    uint64_t b0 = 0, b1 = 0;
+    uint32_t slot0;   // lane 0's only (no other lane assigns it: no copy joins the branches)
+};
+
+__device__ __forceinline__ WfSlots wf_reserve(const WfDev& W, uint32_t T, uint32_t r, const WfNext& s0,
+                                              const WfNext& s1, bool may_local, uint32_t* lq, uint32_t* lqn) {
     auto keep_local = [&](const WfNext& sx) {
         if (!(sx.ready && sx.local && may_local)) return false;
         const uint32_t at = atomicAdd(lqn, 1u);
@@ -2920,25 +2935,64 @@ __device__ __forceinline__ void wf_release(const WfDev& W, uint4* __restrict__ o
         lq[at] = sx.b.w * T + sx.b.z;
         return true;
     };
-    const bool g0 = s0.ready && !keep_local(s0);
-    const bool g1 = s1.ready && !keep_local(s1);
-    const uint64_t b0 = __ballot(g0), b1 = __ballot(g1);
-    const uint32_t tot = (uint32_t)(__popcll(b0) + __popcll(b1));
-    if (tot == 0) return;
-    uint32_t slot0 = 0;
-    if (lane_id() == 0) slot0 = atomicAdd(&W.rcnt[r + 1], tot);
-    slot0 = __shfl(slot0, 0);
+    WfSlots q;
+    q.g0 = s0.ready && !keep_local(s0);
+    q.g1 = s1.ready && !keep_local(s1);
+    q.b0 = __ballot(q.g0);
+    q.b1 = __ballot(q.g1);
+    const uint32_t tot = (uint32_t)(__popcll(q.b0) + __popcll(q.b1));
+    // The count's index through a register the compiler cannot see into:
+    // for a uniform address it would rewrite the atomic to broadcast its
+    // result at once, which waits for the round trip here.
+    uint32_t zero;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+    if (tot != 0 && lane_id() == 0) q.slot0 = atomicAdd(&W.rcnt[r + 1 + zero], tot);
+    return q;
+}
+
+// Write the entries of the reserved successors into `outq`.  An index past
+// T would break the one-ready-visit-per-task bound; such a visit is dropped
+// and the host's completion check reports the stall.
+__device__ __forceinline__ void wf_publish(uint4* __restrict__ outq, uint32_t T, const WfSlots& q, const WfNext& s0,
+                                           const WfNext& s1) {
+    if ((q.b0 | q.b1) == 0) return;
+    const uint32_t slot0 = __shfl(q.slot0, 0);
     const uint64_t below = lane_id() ? (~0ull >> (64 - lane_id())) : 0ull;
-    const uint32_t at0 = slot0 + (uint32_t)__popcll(b0 & below);
-    const uint32_t at1 = slot0 + (uint32_t)__popcll(b0) + (uint32_t)__popcll(b1 & below);
-    if (g0 && at0 < T) {
+    const uint32_t at0 = slot0 + (uint32_t)__popcll(q.b0 & below);
+    const uint32_t at1 = slot0 + (uint32_t)__popcll(q.b0) + (uint32_t)__popcll(q.b1 & below);
+    if (q.g0 && at0 < T) {
         outq[2 * (size_t)at0] = s0.a;
         outq[2 * (size_t)at0 + 1] = s0.b;
     }
-    if (g1 && at1 < T) {
+    if (q.g1 && at1 < T) {
         outq[2 * (size_t)at1] = s1.a;
         outq[2 * (size_t)at1 + 1] = s1.b;
     }
+}
+
+// One visit per lane (active lanes only) and its successors queued: all 64
+// lanes call this together.
+__device__ __forceinline__ void wf_step(const WfDev& W, uint4* __restrict__ outq, uint32_t T, uint32_t npass,
+                                        uint32_t r, uint32_t tag, bool act, uint4 ea, uint4 eb, bool may_local,
+                                        uint32_t* lq, uint32_t* lqn, uint32_t& ran) {
+    WfNext s0, s1;
+    WfLoaded L;
+    uint32_t o0 = 0, o1 = 0;
+    if (act) wf_start_visit(W, T, npass, ea, eb, tag, s0, s1, o0, o1, L);
+    wf_settle(o0, tag, s0);
+    wf_settle(o1, tag, s1);
+    // Every load of the visit has landed before the queue's atomic is issued
+    // (the visit needs them all next anyway).  Memory results return in
+    // issue order, so a wait for any load issued before the atomic would
+    // otherwise be compiled as a wait for the atomic too, and the visit would
+    // no longer overlap it.  (s_waitcnt vmcnt(0), other counters untouched.)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    const WfSlots q = wf_reserve(W, T, r, s0, s1, may_local, lq, lqn);
+    if (act) {
+        wf_visit(W, L, ea.z, ea.w, ea.x, ea.x + ea.y);
+        ++ran;
+    }
+    wf_publish(outq, T, q, s0, s1);
 }
 
 // Launch r: runs the visits queued for it, then (levels > 1) the successors
@@ -2969,12 +3023,7 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(WfDev W, uint32_t T, ui
             ea = in[2 * (size_t)u];
             eb = in[2 * (size_t)u + 1];
         }
-        WfNext s0, s1;
-        if (u < cnt) {
-            wf_run_visit(W, T, npass, ea, eb, tag, s0, s1);
-            ++ran;
-        }
-        wf_release(W, outq, T, r, s0, s1, levels > 1, lq[0], &lqn[0]);
+        wf_step(W, outq, T, npass, r, tag, u < cnt, ea, eb, levels > 1, lq[0], &lqn[0], ran);
     }
     // levels 1 ..: the visits this block made ready, from its own list
     for (int lev = 1; lev < levels; ++lev) {
@@ -2986,15 +3035,15 @@ __global__ void __launch_bounds__(WF_THREADS) k_wf_round(WfDev W, uint32_t T, ui
         __syncthreads();
         for (uint32_t u0 = 0; u0 < n; u0 += blockDim.x) {
             const uint32_t u = u0 + threadIdx.x;
-            WfNext s0, s1;
+            uint4 la = make_uint4(0u, 0u, 0u, 0u), lb = make_uint4(0u, 0u, 0u, 0u);
             if (u < n) {
                 const uint32_t v = lq[cur][u];
                 const uint32_t pass = v / T, t = v - pass * T;
-                const uint4 la = W.task4[2 * (size_t)t], nx = W.task4[2 * (size_t)t + 1];
-                wf_run_visit(W, T, npass, la, make_uint4(nx.x, nx.y, t, pass), tag, s0, s1);
-                ++ran;
+                la = W.task4[2 * (size_t)t];
+                const uint4 nx = W.task4[2 * (size_t)t + 1];
+                lb = make_uint4(nx.x, nx.y, t, pass);
             }
-            wf_release(W, outq, T, r, s0, s1, lev + 1 < levels, lq[nxl], &lqn[nxl]);
+            wf_step(W, outq, T, npass, r, tag, u < n, la, lb, lev + 1 < levels, lq[nxl], &lqn[nxl], ran);
         }
     }
     // visits run: one atomic per wave
