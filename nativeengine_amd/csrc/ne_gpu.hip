@@ -2339,8 +2339,12 @@ __global__ void __launch_bounds__(TPB) k_solve_single(BodiesDev B, SimParams P, 
 // Gauss-Seidel chain never waits on global memory.  Larger small islands take
 // the global-memory sweep.
 constexpr int SOLVE_THREADS = 32;
-constexpr int STAGE_BODIES = 17;   // 16 constraints join at most 17 bodies: every island of k_solve_small fits
-constexpr int STAGE_CONS = 16;
+// Measured on the 1024-world batch (captured steps): staging up to 4
+// constraints (40 KiB of LDS a block, four blocks a CU) and handing larger
+// islands to k_solve_large's waves took the step from 9.65 to 8.09 ms; 16
+// constraints (146 KiB, one block a CU) left most CUs' SIMDs idle.
+constexpr int STAGE_BODIES = 5;   // 4 constraints join at most 5 bodies
+constexpr int STAGE_CONS = 4;
 
 struct StagedBody {
     V3 v, w, x, ii;
@@ -2498,19 +2502,21 @@ __global__ void __launch_bounds__(64) k_solve_large(BodiesDev B, SimParams P, ui
 }
 
 // ---------------------------------------------------------------------------
-// Islands of more than WF_ISLAND constraints in eager steps: the sweep as a
-// dataflow of (pass, pair) visits, one round of independent visits per launch.
+// Islands of more than WF_ISLAND constraints in eager steps (or every
+// constraint): the sweep's iterations as a dataflow of (iteration, pair)
+// visits, one round of independent visits per launch.
 //
 // A task is one pair's run of constraints (consecutive in (i, j, slot)
-// order); a visit is one pass over one task: pass 0 the warm start
-// (world.cpp:559-568), passes 1..I the iterations (572-612), pass I+1 the
-// positional correction (615-616).  On each movable body the tasks of a pass
-// are chained in constraint order, and the body's first task of pass p+1
-// follows its last task of pass p.  A visit runs in the round after both of
-// its predecessors ran, so the visits of one round touch disjoint movable
-// bodies and every body receives exactly the sequential sweep's updates in
-// the sequential order: the result is the reference's bit for bit.  Static
-// bodies (nothing writes them) chain nothing.
+// order); a visit is one iteration (world.cpp:572-612) over one task.  On
+// each movable body the tasks of an iteration are chained in constraint
+// order, and the body's first task of iteration p+1 follows its last task of
+// iteration p.  A visit runs in the round after both of its predecessors
+// ran, so the visits of one round touch disjoint movable bodies and every
+// body receives exactly the sequential sweep's updates in the sequential
+// order: the result is the reference's bit for bit.  Static bodies (nothing
+// writes them) chain nothing.  The warm start (559-568) and the positional
+// correction (615-616) need no such order across bodies: each body applies
+// its own share of them, in constraint order, in k_wf_link and k_wf_correct.
 // ---------------------------------------------------------------------------
 
 constexpr uint32_t WF_WRAP = 0x80000000u;   // successor is the body's first task of the next pass
@@ -2518,7 +2524,8 @@ constexpr uint32_t WF_NONE = 0xFFFFFFFFu;   // no successor on this side (a stat
 constexpr uint32_t WF_MAX_ROUNDS = 1u << 16;
 constexpr int WF_THREADS = 64;
 constexpr uint32_t WF_PASS_MAX = 64;        // iterations
-constexpr uint32_t WF_ISLAND = STAGE_CONS;  // islands above this many constraints: the dataflow solve (eager) or k_solve_large (captured)
+constexpr uint32_t WF_ISLAND = 16;          // eager steps: islands above this many constraints take the dataflow solve
+constexpr uint32_t CAP_ISLAND = STAGE_CONS;  // captured steps: islands above this many constraints take k_solve_large
 // When the large islands hold at least WF_ALL_NUM / WF_ALL_DEN of the
 // constraints, the next eager steps put every constraint in the dataflow and
 // skip building islands (union-find, island ids and lists, the per-island
@@ -4944,9 +4951,9 @@ int enqueue_solve(ne_gpu_ctx* c) {
                                                           c->prev_key, c->prev_imp);
     k_solve_small<<<std::min<uint32_t>(4096u, std::max<uint32_t>(64u, (c->hint_c * 2 + SOLVE_THREADS - 1) / SOLVE_THREADS)),
                     SOLVE_THREADS, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list, c->prev_key,
-                                           c->prev_imp, c->prev_n, WF_ISLAND);
+                                           c->prev_imp, c->prev_n, c->capturing ? CAP_ISLAND : WF_ISLAND);
     if (c->capturing) {   // a graph cannot read back mid-step: one wave sweeps each large island
-        k_solve_large<<<std::min<uint32_t>(std::max<uint32_t>(c->hint_c / WF_ISLAND, 64u), 8192u), 64, 0, s>>>(
+        k_solve_large<<<std::min<uint32_t>(std::max<uint32_t>(c->hint_c / CAP_ISLAND, 64u), 8192u), 64, 0, s>>>(
             B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list, c->prev_key, c->prev_imp);
         return NE_OK;
     }

@@ -153,10 +153,10 @@ def test_batched_ground_worlds_with_excluded_ground(gpu_available, oracle_lib, g
 
 
 def test_graph_steps_sweep_large_islands_in_order(gpu_available, oracle_lib):
-    """Dense piles on a static ground, stepped from a HIP graph: islands above
-    16 constraints cannot take the dataflow solve there (no read-back), so
-    one wave sweeps each in ordered windows.  Every world equals the oracle
-    run alone, bit for bit, and the step did have such islands."""
+    """Dense piles on a static ground, stepped from a HIP graph: the dataflow
+    solve cannot run there (no read-back), so one wave sweeps each island
+    above 4 constraints in ordered windows.  Every world equals the oracle
+    run alone, bit for bit, and the step did have islands of more than 16."""
     batch = S.adjoint_worlds(2, n_spheres=100, n_boxes=100, side=8.0, seed=7)
     _say(f"dense ground worlds: {batch.nworlds} worlds, {batch.scene.n} bodies, graph")
     steps = 60
@@ -164,7 +164,7 @@ def test_graph_steps_sweep_large_islands_in_order(gpu_available, oracle_lib):
     g = _gpu_batch(batch, True)
     g.step(steps)
     big = g.counters()["max_island"]
-    _say(f"largest island above 16 constraints: {big}")
+    _say(f"largest island: {big} constraints")
     assert big > 16, big
     _check(batch, g, want, "dense ground worlds graph")
     g.close()
