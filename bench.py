@@ -616,6 +616,8 @@ def pile_bench(args):
     marks = [10, 150, 300, 450, 600]
     after = None if args.no_queries else (lambda g: query_bench(g, s, args.no_cpu))
     r, states, tmp = scene_run(s, 600, 0, args.pile_steps, "pile", marks=marks if not args.no_cpu else (), after=after)
+    r["traffic_profile"] = "profiles/mixed_pile_65536_traffic.md"
+    r["roofline"]["traffic"] = measured_traffic(r["roofline"]["kernel"], "mixed_pile_65536_traffic.md")
     if not args.no_cpu:
         cpu = phase_cpu(f"pile,{args.pile_n},0", s.n, states, args.cpu_timeout, "the same pile")
         r["cpu_baseline"] = cpu
