@@ -140,6 +140,7 @@ enum CounterSlot : int {
 };
 
 constexpr int TPB = 256;
+constexpr int WF_REC = 16;   // doubles per constraint record and per body record (one 128-byte line)
 
 // Wave-aggregated atomics.  When one island spans the scene, every lane of a
 // wave targets the same counter; lanes with equal keys combine first and one
@@ -1871,7 +1872,8 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
                                                     const uint64_t* __restrict__ prev_key, const double* __restrict__ prev_imp,
                                                     const uint32_t* __restrict__ prev_n, uint32_t* __restrict__ parent,
                                                     uint32_t* __restrict__ claimed, const uint32_t* __restrict__ mq_off,
-                                                    const double* __restrict__ mct, bool link) {
+                                                    const double* __restrict__ mct, bool link,
+                                                    double* __restrict__ rec) {
     const uint32_t np = counters_in[CNT_P];
     const uint32_t pn = *prev_n;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
@@ -1921,14 +1923,28 @@ __global__ void __launch_bounds__(TPB) k_cons_build(BodiesDev B, SimParams P, co
             C.ci[k] = i; C.cj[k] = j; C.slot[k] = s;
             C.n0[k] = n.x; C.n1[k] = n.y; C.n2[k] = n.z;
             C.p0[k] = pt.x; C.p1[k] = pt.y; C.p2[k] = pt.z;
-            C.ra0[k] = rA.x; C.ra1[k] = rA.y; C.ra2[k] = rA.z;
-            C.rb0[k] = rB.x; C.rb1[k] = rB.y; C.rb2[k] = rB.z;
             C.ov[k] = c[6];
-            C.rbias[k] = (vn < -0.5) ? -e * vn : 0.0;                     // world.cpp:40, 544
-            C.mu[k] = mu;
             C.nimp[k] = nimp;
             C.timp[k] = 0.0;
-            C.kn[k] = eff_mass(ima, qa, iia, imb, qb, iib, rA, rB, n);
+            const double rbias = (vn < -0.5) ? -e * vn : 0.0;                // world.cpp:40, 544
+            const double kn = eff_mass(ima, qa, iia, imb, qb, iib, rA, rB, n);
+            if (rec) {   // only the dataflow solve reads these: straight into its record
+                double2* r = reinterpret_cast<double2*>(rec + (size_t)k * WF_REC);
+                r[0] = double2{n.x, n.y};
+                r[1] = double2{n.z, rA.x};
+                r[2] = double2{rA.y, rA.z};
+                r[3] = double2{rB.x, rB.y};
+                r[4] = double2{rB.z, rbias};
+                r[5] = double2{kn, mu};
+                r[6] = double2{nimp, 0.0};
+                r[7] = double2{c[6], 0.0};
+            } else {
+                C.ra0[k] = rA.x; C.ra1[k] = rA.y; C.ra2[k] = rA.z;
+                C.rb0[k] = rB.x; C.rb1[k] = rB.y; C.rb2[k] = rB.z;
+                C.rbias[k] = rbias;
+                C.mu[k] = mu;
+                C.kn[k] = kn;
+            }
             C.key[k] = key;
             C.src[k] = (P.warm && lo + s < pn && prev_key[lo + s] == key) ? lo + s : 0xFFFFFFFFu;
         }
@@ -2557,7 +2573,6 @@ struct WfDev {
     double* bp;            // per body: v, w, q, invI, invMass (16 doubles), the sweep's working copy
 };
 
-constexpr int WF_REC = 16;   // doubles per constraint record and per body record (one 128-byte line)
 
 __device__ __forceinline__ BodyView load_packed(const double* __restrict__ bp, uint32_t b) {
     const double2* r = reinterpret_cast<const double2*>(bp + (size_t)b * WF_REC);
@@ -2635,8 +2650,10 @@ __global__ void __launch_bounds__(TPB) k_wf_mark(BodiesDev B, uint32_t* __restri
     }
 }
 
+// `packed`: k_cons_build wrote the constraints' records already (the whole
+// solve as one dataflow).
 __global__ void __launch_bounds__(TPB) k_wf_tasks(BodiesDev B, const uint32_t* __restrict__ counters, ConsDev C,
-                                                  WfDev W) {
+                                                  WfDev W, bool packed) {
     const uint32_t nc = counters[CNT_C];
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
         if (!W.flag[k]) continue;
@@ -2653,6 +2670,7 @@ __global__ void __launch_bounds__(TPB) k_wf_tasks(BodiesDev B, const uint32_t* _
         // a pair of two bodies that cannot move (the pair filter drops them,
         // world.cpp:513) would still order its own passes: it succeeds itself
         W.task4[2 * (size_t)t + 1] = make_uint4((mi || mj) ? WF_NONE : (WF_WRAP | t), WF_NONE, 0u, 0u);
+        if (packed) continue;
         for (uint32_t q = k; q < k1; ++q) {   // the sweep's record of constraint q
             double2* r = reinterpret_cast<double2*>(W.rec + (size_t)q * WF_REC);
             r[0] = double2{C.n0[q], C.n1[q]};
@@ -4782,7 +4800,8 @@ void enqueue_front(ne_gpu_ctx* c, bool wrap_first) {
         k_cons_build<<<hint_blocks(c->hint_p), TPB, 0, s>>>(B, P, c->counters, c->cand_i, c->cand_j, c->ncont, c->cons_off,
                                                         c->ct, c->C, c->ccap, c->prev_key, c->prev_imp, c->prev_n,
                                                         c->parent, c->claimed, c->nmeshes ? c->mq_off : nullptr,
-                                                        c->nmeshes ? c->mct : nullptr, !whole_dataflow(c));
+                                                        c->nmeshes ? c->mct : nullptr, !whole_dataflow(c),
+                                                        whole_dataflow(c) ? c->wf.rec : nullptr);
     }
     if (whole_dataflow(c)) return;   // no islands: every constraint goes to the dataflow solve
     // --- islands (linked inside k_cons_build)
@@ -4880,7 +4899,7 @@ int solve_large_islands(ne_gpu_ctx* c, const SimParams& P) {
     k_fill_u32<<<gs_blocks(T), TPB, 0, s>>>(W.pend0, 0u, nullptr, T);
     uint32_t cleared = std::min<uint32_t>(WF_MAX_ROUNDS + 2, 2 * c->wf_rounds_hint + 256);
     k_fill_u32<<<gs_blocks(cleared), TPB, 0, s>>>(W.rcnt, 0u, nullptr, cleared);
-    k_wf_tasks<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, W);
+    k_wf_tasks<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, c->counters, c->C, W, all);
     scan_u32(W.inc_cnt, W.inc_start, nullptr, c->n, c->scan_part, s, true);
     k_wf_fill<<<gs_blocks(T), TPB, 0, s>>>(B, T, c->C, W);
     k_wf_pack_bodies<<<blocks_for(c->n), TPB, 0, s>>>(B, W.bp);
@@ -6275,6 +6294,10 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps) {
     c->wf_seen = false;
     for (int32_t t = 0; t < nsteps; ++t) {
         c->solve_all = c->all_ok && ++c->all_steps % WF_ALL_RECHECK != 0;
+        if (c->solve_all) {   // the constraint build writes the solver's records
+            const int re = ensure_wf(c, 0, 1);
+            if (re != NE_OK) { c->solve_all = false; return re; }
+        }
         int r = run_step(c, t > 0);
         c->solve_all = false;
         if (r != NE_OK) return r;
