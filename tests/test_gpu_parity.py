@@ -161,6 +161,24 @@ def test_whole_dataflow_steps_parity(gpu_available, oracle_lib):
     assert island > 0
 
 
+@pytest.mark.parametrize("warm,iters,n,steps", [(False, 8, 300, 130), (True, 0, 200, 80), (True, 3, 300, 130)])
+def test_dataflow_warm_start_and_iteration_counts(gpu_available, oracle_lib, warm, iters, n, steps):
+    """The dataflow solve holds only the iterations; the warm start and the
+    positional correction run per body.  With warm starting off, with no
+    iterations at all (warm start and correction only, no rounds) and with
+    3, a settling pile whose large islands take the dataflow matches the
+    oracle bit for bit.  (Fewer iterations let bodies sink deeper; these
+    piles stay clear of EPA runs that outgrow the workspace.)"""
+    _say(f"case: pile {n}, warm {warm}, {iters} iterations, {steps} steps")
+    if n == 300:
+        s = scene.mixed_pile(300, half=(5.0, 8.0, 5.0), seed=2, sizes="config", iterations=iters)
+    else:
+        s = scene.mixed_pile(n, half=(6.0, 6.0, 6.0), seed=2, iterations=iters)
+    s.params.warm_start = warm
+    got = _run_parity(s, steps, f"pile{n} warm={warm} it={iters}", check_every=10)
+    assert got > 100
+
+
 def test_extension_noncubic_iterations(gpu_available, oracle_lib):
     """Per-axis box extents and 10 iterations: beyond the reference, pinned by
     the restatement."""
