@@ -3983,6 +3983,20 @@ struct ne_gpu_ctx {
         uint32_t cap = 0;
     };
     std::vector<Saved> ckpt, seg;
+    // The backward's copy of each recomputed step's front half (integrated
+    // body fields, constraints, islands, counters), restored instead of
+    // recomputed when the reverse sweep reaches that step.
+    struct FrontSave {
+        double* body = nullptr;     // 13 x ncap: the fields save_state keeps, after integrate
+        double* cd = nullptr;       // the constraint list's 18 double fields, ccap each
+        uint32_t* cu = nullptr;     // ci, cj, slot, src, ccap each
+        uint64_t* key = nullptr;    // ccap
+        uint32_t* isl = nullptr;    // isl_start (ccap + 1) then isl_list (ccap)
+        uint32_t* cnt = nullptr;    // the step's counters
+        uint32_t ccap = 0, ncap = 0;
+    };
+    std::vector<FrontSave> fronts;
+    FrontSave* front_save = nullptr;                  // set: enqueue_step copies its front half here
     int32_t roll_steps = 0, roll_every = 0;
     double* adj = nullptr;        // 13 x ncap body adjoints
     double* adj_pre = nullptr;    // 6 x ncap pre-solve v, w
@@ -4512,6 +4526,11 @@ void free_work(ne_gpu_ctx* c) {
     drop_graphs(c);
 }
 
+void free_fronts(std::vector<ne_gpu_ctx::FrontSave>& v) {
+    for (auto& F : v) { dfree(F.body); dfree(F.cd); dfree(F.cu); dfree(F.key); dfree(F.isl); dfree(F.cnt); }
+    v.clear();
+}
+
 void free_saved(std::vector<ne_gpu_ctx::Saved>& v) {
     for (auto& x : v) { dfree(x.state); dfree(x.key); dfree(x.imp); dfree(x.n); }
     v.clear();
@@ -4520,6 +4539,7 @@ void free_saved(std::vector<ne_gpu_ctx::Saved>& v) {
 void free_adjoint(ne_gpu_ctx* c) {
     free_saved(c->ckpt);
     free_saved(c->seg);
+    free_fronts(c->fronts);
     dfree(c->adj); dfree(c->adj_pre); dfree(c->adj_tape); dfree(c->adj_ca); dfree(c->adj_nfin);
     dfree(c->adj_nprev); dfree(c->adj_cw); dfree(c->adj_gw); dfree(c->stage);
     c->stage_n = 0;
@@ -5052,10 +5072,16 @@ int enqueue_events(ne_gpu_ctx* c) {
     return NE_OK;
 }
 
+int front_io(ne_gpu_ctx* c, ne_gpu_ctx::FrontSave& F, bool save);
+
 int enqueue_step(ne_gpu_ctx* c, bool wrap_first) {
     if (c->n == 0) return NE_OK;
     KTimer whole(c, NE_KERNEL_STEP);
     enqueue_front(c, wrap_first);
+    if (c->front_save) {
+        const int rf = front_io(c, *c->front_save, true);
+        if (rf != NE_OK) return rf;
+    }
     const int rs = enqueue_solve(c);
     if (rs != NE_OK) return rs;
     const SimParams P = make_sim_params(c->params);
@@ -5230,7 +5256,8 @@ int prepare_static_grid(ne_gpu_ctx* c) {
 // cell-order buffers in (the fill reads the previous order and writes the new
 // one), so one graph is kept per buffer parity and per wrap-first flag.
 int run_step(ne_gpu_ctx* c, bool wrap_first) {
-    const bool graphable = c->use_graph && !c->has_forces && c->timed_kernel == NE_KERNEL_NONE && !c->events_on;
+    const bool graphable = c->use_graph && !c->has_forces && c->timed_kernel == NE_KERNEL_NONE && !c->events_on &&
+                           !c->front_save;
     if (!graphable) return enqueue_step(c, wrap_first);
     // launch sizes follow the counts seen when captured: recapture when they drift
     auto drift = [](uint32_t now, uint32_t then) { return now > 2u * then + 1024u || 2u * now + 1024u < then; };
@@ -5296,6 +5323,41 @@ int load_state(ne_gpu_ctx* c, const ne_gpu_ctx::Saved& d) {
     HIP_TRY(c, hipMemcpyAsync(c->prev_key, d.key, sizeof(uint64_t) * m, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->prev_imp, d.imp, sizeof(double) * m, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->prev_n, d.n, sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    return NE_OK;
+}
+
+// Copy a step's front half into F (save) or back (restore): the body
+// fields k_integrate wrote, every constraint field, the island lists and the
+// counters.  F grows to the current capacities on a save.
+int front_io(ne_gpu_ctx* c, ne_gpu_ctx::FrontSave& F, bool save) {
+    hipStream_t s = c->stream;
+    if (save && (F.ccap < c->ccap || F.ncap < c->ncap)) {
+        HIP_TRY(c, dalloc(F.body, (size_t)13 * c->ncap));
+        HIP_TRY(c, dalloc(F.cd, (size_t)18 * c->ccap));
+        HIP_TRY(c, dalloc(F.cu, (size_t)4 * c->ccap));
+        HIP_TRY(c, dalloc(F.key, c->ccap));
+        HIP_TRY(c, dalloc(F.isl, (size_t)2 * c->ccap + 1));
+        if (!F.cnt) HIP_TRY(c, dalloc(F.cnt, CNT_N));
+        F.ccap = c->ccap;
+        F.ncap = c->ncap;
+    }
+    if (!save && (F.ccap > c->ccap || F.ncap != c->ncap)) return set_err(c, NE_ERR_INVALID, "adjoint: capacities changed");
+    const size_t cc = F.ccap;
+    auto cp = [&](void* live, void* kept, size_t bytes) -> hipError_t {
+        return save ? hipMemcpyAsync(kept, live, bytes, hipMemcpyDeviceToDevice, s)
+                    : hipMemcpyAsync(live, kept, bytes, hipMemcpyDeviceToDevice, s);
+    };
+    ConsDev& C = c->C;
+    double* dp[] = {C.n0, C.n1, C.n2, C.p0, C.p1, C.p2, C.ra0, C.ra1, C.ra2, C.rb0, C.rb1, C.rb2,
+                    C.ov, C.rbias, C.mu, C.nimp, C.timp, C.kn};
+    uint32_t* up[] = {C.ci, C.cj, C.slot, C.src};
+    HIP_TRY(c, cp(c->dfield, F.body, sizeof(double) * 13 * (size_t)F.ncap));
+    for (int f = 0; f < 18; ++f) HIP_TRY(c, cp(dp[f], F.cd + (size_t)f * cc, sizeof(double) * cc));
+    for (int f = 0; f < 4; ++f) HIP_TRY(c, cp(up[f], F.cu + (size_t)f * cc, sizeof(uint32_t) * cc));
+    HIP_TRY(c, cp(C.key, F.key, sizeof(uint64_t) * cc));
+    HIP_TRY(c, cp(c->isl_start, F.isl, sizeof(uint32_t) * (cc + 1)));
+    HIP_TRY(c, cp(c->isl_list, F.isl + cc + 1, sizeof(uint32_t) * cc));
+    HIP_TRY(c, cp(c->counters, F.cnt, sizeof(uint32_t) * CNT_N));
     return NE_OK;
 }
 
@@ -5430,6 +5492,10 @@ int ne_gpu_rollout_backward(ne_gpu_ctx* c, const double* gx, double* gv0, double
     const int32_t every = c->roll_every, nsteps = c->roll_steps;
     const int32_t nck = (nsteps + every - 1) / every;
     ensure_saved(c, c->seg, (size_t)every);
+    if (c->fronts.size() != (size_t)every) {
+        free_fronts(c->fronts);
+        c->fronts.resize((size_t)every);
+    }
     AdjBodies A{c->adj, ncap};
     for (int32_t k = nck - 1; k >= 0; --k) {
         const int32_t t0 = k * every, t1 = std::min(nsteps, t0 + every);
@@ -5439,19 +5505,26 @@ int ne_gpu_rollout_backward(ne_gpu_ctx* c, const double* gx, double* gv0, double
         for (int32_t t = t0; t < t1; ++t) {
             r = save_state(c, c->seg[t - t0]);
             if (r != NE_OK) return r;
-            if (t + 1 < t1) {
+            if (t + 1 < t1) {   // the step, keeping its front half for the reverse sweep
+                c->front_save = &c->fronts[t - t0];
                 r = ne_gpu_step(c, 1);
+                c->front_save = nullptr;
                 if (r == NE_WARN_EPA_CAPPED) r = NE_OK;   // the forward rollout reported it
                 if (r != NE_OK) return r;
             }
         }
         for (int32_t t = t1 - 1; t >= t0; --t) {
-            r = load_state(c, c->seg[t - t0]);
-            if (r != NE_OK) return r;
-            r = prepare_static_grid(c);
-            if (r != NE_OK) return r;
-            HIP_TRY(c, hipMemsetAsync(c->counters, 0, sizeof(uint32_t) * CNT_N, s));
-            enqueue_front(c, false);
+            if (t + 1 < t1) {   // the front half the recompute kept (the same inputs, the same bits)
+                r = front_io(c, c->fronts[t - t0], false);
+                if (r != NE_OK) return r;
+            } else {            // the segment's last step: its front half now
+                r = load_state(c, c->seg[t - t0]);
+                if (r != NE_OK) return r;
+                r = prepare_static_grid(c);
+                if (r != NE_OK) return r;
+                HIP_TRY(c, hipMemsetAsync(c->counters, 0, sizeof(uint32_t) * CNT_N, s));
+                enqueue_front(c, false);
+            }
             const SimParams P = make_sim_params(c->params);
             BodiesDev B = c->B;
             HIP_TRY(c, hipMemcpyAsync(c->adj_pre, B.v0, sizeof(double) * 3 * ncap, hipMemcpyDeviceToDevice, s));
@@ -6293,7 +6366,8 @@ int step_call(ne_gpu_ctx* c, int32_t nsteps) {
     HIP_TRY(c, hipMemsetAsync(c->counters, 0, sizeof(uint32_t) * CNT_N, c->stream));
     c->wf_seen = false;
     for (int32_t t = 0; t < nsteps; ++t) {
-        c->solve_all = c->all_ok && ++c->all_steps % WF_ALL_RECHECK != 0;
+        // (a step whose front half is kept for the adjoint keeps its islands)
+        c->solve_all = c->all_ok && !c->front_save && ++c->all_steps % WF_ALL_RECHECK != 0;
         if (c->solve_all) {   // the constraint build writes the solver's records
             const int re = ensure_wf(c, 0, 1);
             if (re != NE_OK) { c->solve_all = false; return re; }
