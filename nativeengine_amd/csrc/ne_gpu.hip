@@ -2160,6 +2160,25 @@ __device__ void wave_sort_list(uint32_t* list, uint32_t m, uint32_t* sk) {
     __syncthreads();
 }
 
+constexpr int TAPE_DOUBLES = 14;   // pre-visit va, wa, vb, wb, nimp, timp
+constexpr size_t ADJ_TAPE_STEP_MAX = size_t(2) << 30;   // bytes of one recomputed step's tape
+
+// The adjoint's tape of a step's solve, written by the solve itself when the
+// backward recomputes the step: per (iteration, constraint) the two bodies'
+// velocities and the impulses before the visit (d == nullptr: not recorded).
+struct Tape {
+    double* d = nullptr;
+    uint32_t stride = 0;   // constraints per iteration
+    __device__ __forceinline__ void put(uint32_t it, uint32_t k, const BodyView& a, const BodyView& b, double nimp,
+                                        double timp) const {
+        if (!d) return;
+        double* tp = d + ((size_t)it * stride + k) * TAPE_DOUBLES;
+        tp[0] = a.v.x; tp[1] = a.v.y; tp[2] = a.v.z; tp[3] = a.w.x; tp[4] = a.w.y; tp[5] = a.w.z;
+        tp[6] = b.v.x; tp[7] = b.v.y; tp[8] = b.v.z; tp[9] = b.w.x; tp[10] = b.w.y; tp[11] = b.w.z;
+        tp[12] = nimp; tp[13] = timp;
+    }
+};
+
 // Body-state store for the sequential island sweep in global memory.
 struct GlobalStore {
     BodiesDev& B;
@@ -2174,7 +2193,8 @@ __device__ __forceinline__ double C_im(const Store& S, uint32_t b);
 
 // Sequential sweep of one island in constraint order (world.cpp:559-616).
 template <class Store>
-__device__ void solve_island_seq(const Store& S, const SimParams& P, const ConsDev& C, const uint32_t* list, uint32_t m) {
+__device__ void solve_island_seq(const Store& S, const SimParams& P, const ConsDev& C, const uint32_t* list, uint32_t m,
+                                 const Tape& tape = Tape{}) {
     if (P.warm) {
         for (uint32_t t = 0; t < m; ++t) {
             const uint32_t k = list[t];
@@ -2193,6 +2213,7 @@ __device__ void solve_island_seq(const Store& S, const SimParams& P, const ConsD
             const uint32_t i = C.ci[k], j = C.cj[k];
             BodyView a = S.load(i), b = S.load(j);
             double nimp = C.nimp[k], timp = C.timp[k];
+            tape.put((uint32_t)it, k, a, b, nimp, timp);
             gs_visit(a, b, load_geom(C, k), C.rbias[k], C.kn[k], C.mu[k], nimp, timp);
             C.nimp[k] = nimp;
             C.timp[k] = timp;
@@ -2270,7 +2291,8 @@ __device__ void solve_island_wave(BodiesDev& B, const SimParams& P, const ConsDe
 // An island whose constraints all join the same two bodies (a sphere pair,
 // a box-box manifold: <= 4 contacts): both bodies and the impulses stay in
 // registers for the whole solve.
-__device__ void solve_pair_island(BodiesDev& B, const SimParams& P, const ConsDev& C, const uint32_t* list, uint32_t m) {
+__device__ void solve_pair_island(BodiesDev& B, const SimParams& P, const ConsDev& C, const uint32_t* list, uint32_t m,
+                                  const Tape& tape) {
     const uint32_t k0 = list[0];
     const uint32_t i = C.ci[k0], j = C.cj[k0];
     BodyView a = load_view(B, i), b = load_view(B, j);
@@ -2290,6 +2312,7 @@ __device__ void solve_pair_island(BodiesDev& B, const SimParams& P, const ConsDe
         for (int t = 0; t < 4; ++t) {
             if (t < (int)m) {
                 const uint32_t k = list[t];
+                tape.put((uint32_t)it, k, a, b, nimp[t], timp[t]);
                 gs_visit(a, b, load_geom(C, k), C.rbias[k], C.kn[k], C.mu[k], nimp[t], timp[t]);
             }
         }
@@ -2324,7 +2347,8 @@ __device__ __forceinline__ void publish_cache(const ConsDev& C, const uint32_t* 
 __global__ void __launch_bounds__(TPB) k_solve_single(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters,
                                                       const uint32_t* __restrict__ isl_start,
                                                       const uint32_t* __restrict__ isl_list, ConsDev C,
-                                                      uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp) {
+                                                      uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp,
+                                                      Tape tape) {
     const uint32_t nisl = counters[CNT_ISL];
     for (uint32_t isl = blockIdx.x * blockDim.x + threadIdx.x; isl < nisl; isl += gridDim.x * blockDim.x) {
         const uint32_t s = isl_start[isl];
@@ -2336,7 +2360,10 @@ __global__ void __launch_bounds__(TPB) k_solve_single(BodiesDev B, SimParams P, 
         const double rbias = C.rbias[k], kn = C.kn[k], mu = C.mu[k];
         double nimp = C.nimp[k], timp = 0.0;
         if (P.warm) warm_apply(a, b, g, nimp);
-        for (int it = 0; it < P.iters; ++it) gs_visit(a, b, g, rbias, kn, mu, nimp, timp);
+        for (int it = 0; it < P.iters; ++it) {
+            tape.put((uint32_t)it, k, a, b, nimp, timp);
+            gs_visit(a, b, g, rbias, kn, mu, nimp, timp);
+        }
         C.nimp[k] = nimp;
         C.timp[k] = timp;
         store_view(B, i, a);
@@ -2381,7 +2408,7 @@ __device__ __forceinline__ BodyView view_of(const StagedBody& s) {
 }
 
 __device__ void solve_island_staged(BodiesDev& B, const SimParams& P, ConsDev& C, const uint32_t* list, uint32_t m,
-                                    const uint32_t* ids, int nb, StagedBody* sb, StagedCons* sc) {
+                                    const uint32_t* ids, int nb, StagedBody* sb, StagedCons* sc, const Tape& tape) {
     for (int u = 0; u < nb; ++u) {
         const uint32_t b = ids[u];
         BodyView o = load_view(B, b);
@@ -2415,6 +2442,7 @@ __device__ void solve_island_staged(BodiesDev& B, const SimParams& P, ConsDev& C
             StagedCons& c = sc[t];
             BodyView a = view_of(sb[c.a]), b = view_of(sb[c.b]);
             double nimp = c.nimp, timp = c.timp;
+            tape.put((uint32_t)it, list[t], a, b, nimp, timp);
             gs_visit(a, b, ConsGeom{c.n, c.rA, c.rB}, c.rbias, c.kn, c.mu, nimp, timp);
             c.nimp = nimp; c.timp = timp;
             sb[c.a].v = a.v; sb[c.a].w = a.w; sb[c.b].v = b.v; sb[c.b].w = b.w;
@@ -2442,7 +2470,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve_small(BodiesDev B, SimP
                                                      const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
                                                      ConsDev C, uint32_t* __restrict__ big_list,
                                                      uint64_t* __restrict__ prev_key, double* __restrict__ prev_imp,
-                                                     uint32_t* __restrict__ prev_n, uint32_t big_above) {
+                                                     uint32_t* __restrict__ prev_n, uint32_t big_above, Tape tape) {
     __shared__ uint32_t lds_ids[SOLVE_THREADS * STAGE_BODIES];
     __shared__ StagedBody lds_b[SOLVE_THREADS * STAGE_BODIES];
     __shared__ StagedCons lds_c[SOLVE_THREADS * STAGE_CONS];
@@ -2464,7 +2492,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve_small(BodiesDev B, SimP
             for (uint32_t t = 1; t < m; ++t) pair = pair && C.ci[list[t]] == i0 && C.cj[list[t]] == j0;
         }
         if (pair) {
-            solve_pair_island(B, P, C, list, m);
+            solve_pair_island(B, P, C, list, m, tape);
         } else {
             uint32_t* ids = lds_ids + threadIdx.x * STAGE_BODIES;
             int nb = 0;
@@ -2483,9 +2511,9 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve_small(BodiesDev B, SimP
             }
             if (fits)
                 solve_island_staged(B, P, C, list, m, ids, nb, lds_b + threadIdx.x * STAGE_BODIES,
-                                    lds_c + threadIdx.x * STAGE_CONS);
+                                    lds_c + threadIdx.x * STAGE_CONS, tape);
             else
-                solve_island_seq(GlobalStore{B}, P, C, list, m);
+                solve_island_seq(GlobalStore{B}, P, C, list, m, tape);
         }
         if (P.warm) publish_cache(C, list, m, prev_key, prev_imp);
     }
@@ -2571,6 +2599,7 @@ struct WfDev {
                            // WF_NONE) -- one cache line a visit
     double* rec;           // per constraint of a task: n, rA, rB, restBias, Kn, mu, nImp, tImp, overlap (16 doubles)
     double* bp;            // per body: v, w, q, invI, invMass (16 doubles), the sweep's working copy
+    Tape tape;             // the adjoint's tape of this solve, when the backward recomputes the step
 };
 
 
@@ -2834,8 +2863,8 @@ __device__ __forceinline__ void wf_visit_load(const WfDev& W, uint32_t i, uint32
     L.r0 = r[0]; L.r1 = r[1]; L.r2 = r[2]; L.r3 = r[3]; L.r4 = r[4]; L.r5 = r[5]; L.r6 = r[6];
 }
 
-__device__ __forceinline__ void wf_visit(const WfDev& W, const WfLoaded& L, uint32_t i, uint32_t j, uint32_t k0,
-                                         uint32_t k1) {
+__device__ __forceinline__ void wf_visit(const WfDev& W, const WfLoaded& L, uint32_t pass, uint32_t i, uint32_t j,
+                                         uint32_t k0, uint32_t k1) {
     BodyView a = L.a, b = L.b;
     // the next record is loaded while this one is solved (the records do not
     // depend on the sweep's results; only nImp/tImp are written)
@@ -2848,6 +2877,7 @@ __device__ __forceinline__ void wf_visit(const WfDev& W, const WfLoaded& L, uint
             const double2* rn = reinterpret_cast<const double2*>(W.rec + (size_t)(k + 1) * WF_REC);
             r0 = rn[0]; r1 = rn[1]; r2 = rn[2]; r3 = rn[3]; r4 = rn[4]; r5 = rn[5]; r6 = rn[6];
         }
+        W.tape.put(pass, k, a, b, nimp, timp);
         gs_visit(a, b, g, rbias, kn, mu, nimp, timp);
         reinterpret_cast<double2*>(W.rec + (size_t)k * WF_REC)[6] = double2{nimp, timp};
     }
@@ -3014,7 +3044,7 @@ __device__ __forceinline__ void wf_step(const WfDev& W, uint4* __restrict__ outq
     __builtin_amdgcn_s_waitcnt(0x0F70);
     const WfSlots q = wf_reserve(W, T, r, s0, s1, may_local, lq, lqn);
     if (act) {
-        wf_visit(W, L, ea.z, ea.w, ea.x, ea.x + ea.y);
+        wf_visit(W, L, eb.w, ea.z, ea.w, ea.x, ea.x + ea.y);
         ++ran;
     }
     wf_publish(outq, T, q, s0, s1);
@@ -3439,7 +3469,6 @@ __global__ void __launch_bounds__(64) k_joint_rigid(BodiesDev B, SimParams P, Jo
 // w0..2), SoA over ncap.  Per-constraint accumulators (SoA over ccap):
 constexpr int CA_NINIT = 0, CA_NIMP = 1, CA_TIMP = 2, CA_RA = 3, CA_RB = 6, CA_RBIAS = 9, CA_KN = 10, CA_MU = 11,
               CA_COUNT = 12;
-constexpr int TAPE_DOUBLES = 14;   // pre-visit va, wa, vb, wb, nimp, timp
 
 struct AdjBodies {   // adjoints of one body table
     double* f;       // 13 fields x ncap
@@ -3470,6 +3499,7 @@ __device__ __forceinline__ Qt qadd(Qt a, Qt b) { return Qt{a.w + b.w, a.x + b.x,
 __global__ void __launch_bounds__(64) k_adj_island(BodiesDev B, SimParams P, const uint32_t* __restrict__ counters,
                                                    const uint32_t* __restrict__ isl_start, uint32_t* __restrict__ isl_list,
                                                    ConsDev C, uint32_t ccap, double* __restrict__ tape,
+                                                   uint32_t tstride, bool replay,
                                                    const double* __restrict__ pre, uint32_t ncap, AdjBodies A,
                                                    double* __restrict__ ca, const double* __restrict__ nfin,
                                                    double* __restrict__ nprev, double* __restrict__ cw) {
@@ -3499,7 +3529,10 @@ __global__ void __launch_bounds__(64) k_adj_island(BodiesDev B, SimParams P, con
             CA(CA_NIMP, k) = nfin[k];
         }
         __threadfence_block();
-        if (P.warm)
+        // (replay false: the recomputed step's solve wrote the tape, and the
+        // reverse passes below read only the tape and the bodies' masses and
+        // orientations)
+        if (replay && P.warm)
             wave_in_order(
                 m,
                 [&](uint32_t t, uint32_t& bi, uint32_t& bj) {
@@ -3513,13 +3546,14 @@ __global__ void __launch_bounds__(64) k_adj_island(BodiesDev B, SimParams P, con
                     store_view(B, C.ci[k], a);
                     store_view(B, C.cj[k], b);
                 });
+        if (replay)
         wave_in_order(
             iters * m, [&](uint32_t v, uint32_t& bi, uint32_t& bj) { cons_bodies(v % m, bi, bj); },
             [&](uint32_t v) {
                 const uint32_t it = v / m, k = list[v % m];
                 const uint32_t i = C.ci[k], j = C.cj[k];
                 BodyView a = load_view(B, i), b = load_view(B, j);
-                double* tp = tape + ((size_t)it * ccap + k) * TAPE_DOUBLES;
+                double* tp = tape + ((size_t)it * tstride + k) * TAPE_DOUBLES;
                 tp[0] = a.v.x; tp[1] = a.v.y; tp[2] = a.v.z; tp[3] = a.w.x; tp[4] = a.w.y; tp[5] = a.w.z;
                 tp[6] = b.v.x; tp[7] = b.v.y; tp[8] = b.v.z; tp[9] = b.w.x; tp[10] = b.w.y; tp[11] = b.w.z;
                 tp[12] = C.nimp[k]; tp[13] = C.timp[k];
@@ -3539,7 +3573,7 @@ __global__ void __launch_bounds__(64) k_adj_island(BodiesDev B, SimParams P, con
                 const uint32_t it = r / m, k = list[r % m];
                 const uint32_t i = C.ci[k], j = C.cj[k];
                 BodyView a = load_view(B, i), b = load_view(B, j);
-                const double* tp = tape + ((size_t)it * ccap + k) * TAPE_DOUBLES;
+                const double* tp = tape + ((size_t)it * tstride + k) * TAPE_DOUBLES;
                 a.v = V3{tp[0], tp[1], tp[2]}; a.w = V3{tp[3], tp[4], tp[5]};
                 b.v = V3{tp[6], tp[7], tp[8]}; b.w = V3{tp[9], tp[10], tp[11]};
                 const bool ma = a.mv || a.mw, mb = b.mv || b.mw;
@@ -3994,6 +4028,10 @@ struct ne_gpu_ctx {
         uint32_t* isl = nullptr;    // isl_start (ccap + 1) then isl_list (ccap)
         uint32_t* cnt = nullptr;    // the step's counters
         uint32_t ccap = 0, ncap = 0;
+        double* tape = nullptr;     // the solve's tape (iterations x tape_stride x TAPE_DOUBLES), or none
+        uint32_t tape_stride = 0;
+        size_t tape_n = 0;
+        bool taped = false;         // this step's solve wrote the tape
     };
     std::vector<FrontSave> fronts;
     FrontSave* front_save = nullptr;                  // set: enqueue_step copies its front half here
@@ -4527,7 +4565,9 @@ void free_work(ne_gpu_ctx* c) {
 }
 
 void free_fronts(std::vector<ne_gpu_ctx::FrontSave>& v) {
-    for (auto& F : v) { dfree(F.body); dfree(F.cd); dfree(F.cu); dfree(F.key); dfree(F.isl); dfree(F.cnt); }
+    for (auto& F : v) {
+        dfree(F.body); dfree(F.cd); dfree(F.cu); dfree(F.key); dfree(F.isl); dfree(F.cnt); dfree(F.tape);
+    }
     v.clear();
 }
 
@@ -4985,12 +5025,16 @@ int enqueue_solve(ne_gpu_ctx* c) {
     const SimParams P = make_sim_params(c->params);
     BodiesDev B = c->B;
     KTimer kt(c, NE_KERNEL_SOLVE);
+    // the adjoint's recompute of a step records the solve's tape
+    const ne_gpu_ctx::FrontSave* F = c->front_save;
+    const Tape tape = (F && F->taped) ? Tape{F->tape, F->tape_stride} : Tape{};
+    c->wf.tape = tape;
     if (whole_dataflow(c)) return solve_large_islands(c, P);
     k_solve_single<<<hint_blocks(c->hint_c), TPB, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C,
-                                                          c->prev_key, c->prev_imp);
+                                                          c->prev_key, c->prev_imp, tape);
     k_solve_small<<<std::min<uint32_t>(4096u, std::max<uint32_t>(64u, (c->hint_c * 2 + SOLVE_THREADS - 1) / SOLVE_THREADS)),
                     SOLVE_THREADS, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list, c->prev_key,
-                                           c->prev_imp, c->prev_n, c->capturing ? CAP_ISLAND : WF_ISLAND);
+                                           c->prev_imp, c->prev_n, c->capturing ? CAP_ISLAND : WF_ISLAND, tape);
     if (c->capturing) {   // a graph cannot read back mid-step: one wave sweeps each large island
         k_solve_large<<<std::min<uint32_t>(std::max<uint32_t>(c->hint_c / CAP_ISLAND, 64u), 8192u), 64, 0, s>>>(
             B, P, c->counters, c->isl_start, c->isl_list, c->C, c->big_list, c->prev_key, c->prev_imp);
@@ -5341,6 +5385,18 @@ int front_io(ne_gpu_ctx* c, ne_gpu_ctx::FrontSave& F, bool save) {
         F.ccap = c->ccap;
         F.ncap = c->ncap;
     }
+    if (save) {   // the tape, within ADJ_TAPE_STEP_MAX bytes a step (else the reverse replays the solve)
+        const size_t want = (size_t)std::max(c->params.solver_iterations, 0) * c->ccap * TAPE_DOUBLES;
+        F.taped = false;
+        if (want * sizeof(double) <= ADJ_TAPE_STEP_MAX) {
+            if (F.tape_n < want) {
+                HIP_TRY(c, dalloc(F.tape, want));
+                F.tape_n = want;
+            }
+            F.tape_stride = c->ccap;
+            F.taped = true;
+        }
+    }
     if (!save && (F.ccap > c->ccap || F.ncap != c->ncap)) return set_err(c, NE_ERR_INVALID, "adjoint: capacities changed");
     const size_t cc = F.ccap;
     auto cp = [&](void* live, void* kept, size_t bytes) -> hipError_t {
@@ -5531,9 +5587,12 @@ int ne_gpu_rollout_backward(ne_gpu_ctx* c, const double* gx, double* gv0, double
             HIP_TRY(c, hipMemcpyAsync(c->adj_pre + 3 * (size_t)ncap, B.w0, sizeof(double) * 3 * ncap,
                                       hipMemcpyDeviceToDevice, s));
             fill(c, c->adj_nprev, ccap, 0.0);
-            k_adj_island<<<std::min<uint32_t>(std::max<uint32_t>(c->hint_c, 64u), 16384u), 64, 0, s>>>(B, P, c->counters, c->isl_start, c->isl_list, c->C, ccap,
-                                                               c->adj_tape, c->adj_pre, ncap, A, c->adj_ca, c->adj_nfin,
-                                                               c->adj_nprev, c->adj_cw);
+            // the recompute's tape when its solve wrote one, else the island pass replays the solve
+            const ne_gpu_ctx::FrontSave* F = (t + 1 < t1 && c->fronts[t - t0].taped) ? &c->fronts[t - t0] : nullptr;
+            k_adj_island<<<std::min<uint32_t>(std::max<uint32_t>(c->hint_c, 64u), 16384u), 64, 0, s>>>(
+                B, P, c->counters, c->isl_start, c->isl_list, c->C, ccap, F ? F->tape : c->adj_tape,
+                F ? F->tape_stride : ccap, F == nullptr, c->adj_pre, ncap, A, c->adj_ca, c->adj_nfin, c->adj_nprev,
+                c->adj_cw);
             k_adj_integrate<<<blocks_for(n), TPB, 0, s>>>(B, P, c->seg[t - t0].state, ncap, A);
             k_adj_world_sum<<<nw, 256, 0, s>>>(B, c->counters, c->C, c->adj_cw, ccap, c->adj_gw, nw);
             std::swap(c->adj_nfin, c->adj_nprev);
