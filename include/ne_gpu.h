@@ -536,7 +536,9 @@ int ne_gpu_autotune(ne_gpu_ctx *ctx, int32_t steps, ne_tune_result *out);
  * adjoint of integrate, the constraint build, the warm start and the
  * fixed-iteration impulse solve, with each step's contact set (pairs,
  * normals, points, depths) held fixed; it recomputes each checkpointed
- * segment forward and leaves the world at the rollout's start.
+ * segment forward, keeping each recomputed step's constraint set and solve
+ * tape in device memory for the segment's reverse sweep (the tape up to
+ * 2 GiB a step), and leaves the world at the rollout's start.
  * v0, x_out, grad_x and grad_v0 are n x 3 doubles in host or device memory
  * (copied with hipMemcpyDefault); mu, e, grad_mu and grad_e are host arrays
  * with one entry per world (1 without ne_gpu_set_worlds).  Periodic worlds

@@ -4060,7 +4060,7 @@ struct ne_gpu_ctx {
     // The whole solve as one dataflow (no islands built, no per-island
     // solvers) while the large islands hold most constraints; eager steps only.
     bool solve_all = false;                           // this step
-    bool all_ok = false;                              // the last island step's verdict
+    bool all_ok = false;                              // the last island step's decision
     uint32_t all_steps = 0;                           // steps since that island step
     uint32_t* h_wf = nullptr;                         // pinned: values read back mid-step
     bool capturing = false;                           // a step is being captured into a graph
