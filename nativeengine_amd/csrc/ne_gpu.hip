@@ -1038,8 +1038,12 @@ __device__ __forceinline__ uint32_t narrow_pair(const BodiesDev& B, const SimPar
 // block, a workspace in global memory and the horizon built in parallel.
 // Both capacities grow when exceeded (the step is then run again).
 constexpr uint32_t EPA_BIG_PAIRS = 64;    // initial queue capacity
-constexpr int EPA_BIG_FACES = 4096;       // initial faces per queued pair
 constexpr int EPA_BIG_FACES_MAX = 1 << 17;   // default cap: beyond it a run stops at its closest face
+// Initial faces per queued pair: the default cap (about 6 MiB a slot, 64
+// slots).  Starting at 4,096 and growing x4 cost three reruns of the call at
+// the first run that grows past each size: the 1024-world batch's first 30
+// steps, whose overlapping start queues tens of such runs a step.
+constexpr int EPA_BIG_FACES = EPA_BIG_FACES_MAX;
 constexpr int EPA_KEYS = EPA_MAXV * EPA_MAXV;   // directed edges u * EPA_MAXV + w of one polytope
 constexpr int BIG_NW = 8;                       // waves per large-workspace run
 
@@ -7210,7 +7214,7 @@ int upload_bodies(ne_gpu_ctx* c, const ne_bodies* b) {
         if (r != NE_OK) return r;
         r = ensure_prev(c, ccap);
         if (r != NE_OK) return r;
-        r = alloc_epa_big(c, EPA_BIG_PAIRS, EPA_BIG_FACES);
+        r = alloc_epa_big(c, EPA_BIG_PAIRS, std::min(EPA_BIG_FACES, c->epa_faces_max));
         if (r != NE_OK) return r;
         // scan scratch: partial sums (<=1024) and radix histograms
         HIP_TRY(c, dalloc(c->scan_part, 1024));
