@@ -195,7 +195,10 @@ typedef struct ne_gpu_config {
                                    The reference's EPA grows its face list
                                    without bound; a run that needs more than
                                    the cap stops at its closest face and the
-                                   step returns NE_WARN_EPA_CAPPED */
+                                   step returns NE_WARN_EPA_CAPPED.  The
+                                   workspace is allocated at min(cap, 131072)
+                                   faces a run for 64 runs (about 0.4 GiB at
+                                   the default) and grows past that on use */
 } ne_gpu_config;
 
 /* Broadphase walk variants (same candidate rows; a tuning knob). */
