@@ -642,6 +642,7 @@ def config1_bench(args):
     s, steps, out = fixtures.load_scene("gas4096")
     g = GpuWorld()
     g.set_scene(s)
+    g.use_graph(True)       # each step replays its captured graph (0.149 -> 0.142 ms a step measured)
     p0, L0, e0 = g.diagnostics()
     g.step(1)
     g.set_scene(s)          # the first call loaded the kernels; start again from the initial state
@@ -671,7 +672,8 @@ def config1_bench(args):
         "bitwise_equal_to_reference": same,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(s.n * steps / (t1 - t0) / cpu["value"], 1) if cpu else None,
-        "note": "one synchronised call per step; 4096 bodies leave the GPU latency-bound, so no roofline is given",
+        "note": "one synchronised call per step, replayed from a HIP graph; 4096 bodies leave the GPU "
+                "latency-bound, so no roofline is given",
         "workload": "periodic elastic sphere gas: 4096 spheres r 0.5 m 1, side 40, rejection-sampled (seed 0), "
                     "v N(0,1) with the net momentum removed, e 1, mu 0, g 0, dt 1/60, 8 iterations, 1000 steps",
     }
